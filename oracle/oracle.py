@@ -1,0 +1,205 @@
+"""ctypes wrapper of the CPU oracle (test infrastructure only).
+
+Imported by ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+``bench.py``; the product never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+MAXLAYERS = 32
+NGLL = 5
+NGLL2 = 25
+
+
+class OrcParams(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int), ("nz", C.c_int), ("nlayers", C.c_int), ("nz_layer", C.c_int * MAXLAYERS),
+        ("xmin", C.c_double), ("xmax", C.c_double), ("zinterface", C.c_double * (MAXLAYERS + 1)),
+        ("absorb_bottom", C.c_int), ("absorb_right", C.c_int), ("absorb_top", C.c_int),
+        ("absorb_left", C.c_int), ("stacey", C.c_int), ("nstep", C.c_int), ("dt", C.c_double),
+        ("user_t0", C.c_double), ("anglerec_deg", C.c_double), ("material_mode", C.c_int),
+    ]
+
+
+class OrcSource(C.Structure):
+    _fields_ = [
+        ("x", C.c_double), ("z", C.c_double), ("f0", C.c_double), ("factor", C.c_double),
+        ("angle_deg", C.c_double), ("tshift", C.c_double), ("time_function_type", C.c_int),
+        ("pad", C.c_int),
+    ]
+
+
+_LIBS = {}
+
+
+def build():
+    """Compile the oracle shared libraries (gcc, no GPU)."""
+    import subprocess
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def _lib(precision: str):
+    if precision not in _LIBS:
+        path = os.path.join(_HERE, f"liboracle_{precision}.so")
+        if not os.path.exists(path):
+            build()
+        lib = C.CDLL(path)
+        P = C.c_void_p
+        dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+        ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+        lib.orc_new.restype = P
+        lib.orc_new.argtypes = [C.POINTER(OrcParams), dp, dp, dp, C.c_int, C.POINTER(OrcSource), C.c_int, dp]
+        lib.orc_free.argtypes = [P]
+        for f in ("orc_nspec", "orc_nglob", "orc_nabs"):
+            getattr(lib, f).argtypes = [P]
+            getattr(lib, f).restype = C.c_int
+        lib.orc_t0.argtypes = [P]
+        lib.orc_t0.restype = C.c_double
+        lib.orc_get_ibool.argtypes = [P, ip]
+        lib.orc_get_coord.argtypes = [P, dp]
+        lib.orc_get_elem_array.argtypes = [P, C.c_int, dp]
+        lib.orc_get_rmass_inverse.argtypes = [P, dp]
+        lib.orc_get_basis.argtypes = [P, dp, dp, dp, dp]
+        lib.orc_get_absorbing.argtypes = [P, ip, ip, ip, ip]
+        lib.orc_get_sources.argtypes = [P, ip, dp, dp, dp, dp]
+        lib.orc_get_receivers.argtypes = [P, ip, dp, dp, dp, dp]
+        lib.orc_forward.argtypes = [P, C.c_int, C.c_void_p, C.c_void_p]
+        lib.orc_get_fields.argtypes = [P, C.c_void_p, C.c_void_p, C.c_void_p]
+        lib.orc_set_fields.argtypes = [P, C.c_void_p, C.c_void_p, C.c_void_p]
+        lib.orc_apply_stiffness.argtypes = [P, dp, dp]
+        lib.orc_real_size.restype = C.c_int
+        lib.orc_set_ftz.argtypes = [P, C.c_int]
+        _LIBS[precision] = lib
+    return _LIBS[precision]
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class OracleSim:
+    """One simulation set up by the oracle from a :class:`SimulationConfig`."""
+
+    ELEM_ARRAYS = ("xix", "xiz", "gammax", "gammaz", "jacobian", "rho", "mu", "kappa", "rho_vp", "rho_vs")
+
+    def __init__(self, cfg, precision: str = "sp", ftz: bool = True):
+        self.cfg = cfg
+        self.lib = _lib(precision)
+        p = OrcParams()
+        p.nx, p.nz = cfg.nx, cfg.nz
+        p.nlayers = len(cfg.nz_layers)
+        for i, n in enumerate(cfg.nz_layers):
+            p.nz_layer[i] = int(n)
+        for i, z in enumerate(cfg.interfaces):
+            p.zinterface[i] = float(z)
+        p.xmin, p.xmax = cfg.xmin, cfg.xmax
+        p.absorb_bottom, p.absorb_right = int(cfg.absorbbottom), int(cfg.absorbright)
+        p.absorb_top, p.absorb_left = int(cfg.absorbtop), int(cfg.absorbleft)
+        p.stacey = int(cfg.stacey)
+        p.nstep, p.dt = cfg.nstep, cfg.dt
+        p.user_t0, p.anglerec_deg = cfg.user_t0, cfg.anglerec
+        p.material_mode = cfg.material_mode
+        rho, vp, vs = cfg.material_arrays()
+        srcs = (OrcSource * max(1, len(cfg.sources)))()
+        for k, s in enumerate(cfg.sources):
+            srcs[k].x, srcs[k].z, srcs[k].f0, srcs[k].factor = s.x, s.z, s.f0, s.factor
+            srcs[k].angle_deg, srcs[k].tshift, srcs[k].time_function_type = s.anglesource, s.tshift, s.time_function_type
+        rec = np.ascontiguousarray(np.asarray(cfg.receivers if cfg.nrec else np.zeros((1, 2)), dtype=np.float64))
+        self._keep = (p, rho, vp, vs, srcs, rec)
+        self.h = self.lib.orc_new(C.byref(p), rho, vp, vs, len(cfg.sources), srcs, cfg.nrec, rec)
+        self.nspec = self.lib.orc_nspec(self.h)
+        self.nglob = self.lib.orc_nglob(self.h)
+        self.nabs = self.lib.orc_nabs(self.h)
+        self.t0 = self.lib.orc_t0(self.h)
+        self.lib.orc_set_ftz(self.h, int(ftz))
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                self.lib.orc_free(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    # setup products --------------------------------------------------
+    def ibool(self):
+        out = np.zeros(self.nspec * NGLL2, dtype=np.int32)
+        self.lib.orc_get_ibool(self.h, out)
+        return out.reshape(self.nspec, NGLL2)
+
+    def coord(self):
+        out = np.zeros(2 * self.nglob)
+        self.lib.orc_get_coord(self.h, out)
+        return out.reshape(self.nglob, 2)
+
+    def elem_array(self, name):
+        out = np.zeros(self.nspec * NGLL2)
+        self.lib.orc_get_elem_array(self.h, self.ELEM_ARRAYS.index(name), out)
+        return out.reshape(self.nspec, NGLL2)
+
+    def rmass_inverse(self):
+        out = np.zeros(2 * self.nglob)
+        self.lib.orc_get_rmass_inverse(self.h, out)
+        return out.reshape(self.nglob, 2)
+
+    def basis(self):
+        xi, w, hp, hpw = np.zeros(5), np.zeros(5), np.zeros(25), np.zeros(25)
+        self.lib.orc_get_basis(self.h, xi, w, hp, hpw)
+        return xi, w, hp.reshape(5, 5), hpw.reshape(5, 5)
+
+    def absorbing(self):
+        n = max(1, self.nabs)
+        isp, code, corner, ib = (np.zeros(n, np.int32), np.zeros(n, np.int32), np.zeros(4 * n, np.int32),
+                                 np.zeros(n, np.int32))
+        self.lib.orc_get_absorbing(self.h, isp, code, corner, ib)
+        k = self.nabs
+        return isp[:k], code[:k], corner.reshape(n, 4)[:k], ib[:k]
+
+    def sources(self):
+        n = max(1, len(self.cfg.sources))
+        isp, xi, ga = np.zeros(n, np.int32), np.zeros(n), np.zeros(n)
+        sa = np.zeros(n * NGLL2 * 2)
+        stf = np.zeros(n * self.cfg.nstep)
+        self.lib.orc_get_sources(self.h, isp, xi, ga, sa, stf)
+        k = len(self.cfg.sources)
+        return isp[:k], xi[:k], ga[:k], sa.reshape(n, NGLL2, 2)[:k], stf.reshape(n, self.cfg.nstep)[:k]
+
+    def receivers(self):
+        n = max(1, self.cfg.nrec)
+        isp, xi, ga = np.zeros(n, np.int32), np.zeros(n), np.zeros(n)
+        xir, gar = np.zeros(5 * n), np.zeros(5 * n)
+        self.lib.orc_get_receivers(self.h, isp, xi, ga, xir, gar)
+        k = self.cfg.nrec
+        return isp[:k], xi[:k], ga[:k], xir.reshape(n, 5)[:k], gar.reshape(n, 5)[:k]
+
+    # time stepping ----------------------------------------------------
+    def forward(self, nstep=None, save_stacey=False):
+        """Run the forward loop; returns seismograms [nrec, 2, nstep] (and the Stacey
+        storage [nstep, nabs, 5, 2] when ``save_stacey``)."""
+        n = self.cfg.nstep if nstep is None else int(nstep)
+        seis = np.zeros((max(1, self.cfg.nrec), 2, n))
+        store = np.zeros((n, max(1, self.nabs), NGLL, 2)) if save_stacey else None
+        self.lib.orc_forward(self.h, n, _ptr(seis), _ptr(store))
+        seis = seis[: self.cfg.nrec]
+        return (seis, store) if save_stacey else seis
+
+    def fields(self):
+        d, v, a = (np.zeros(2 * self.nglob) for _ in range(3))
+        self.lib.orc_get_fields(self.h, _ptr(d), _ptr(v), _ptr(a))
+        return d.reshape(-1, 2), v.reshape(-1, 2), a.reshape(-1, 2)
+
+    def set_fields(self, displ=None, veloc=None, accel=None):
+        arrs = [None if x is None else np.ascontiguousarray(np.asarray(x, np.float64).reshape(-1))
+                for x in (displ, veloc, accel)]
+        self.lib.orc_set_fields(self.h, *[_ptr(x) for x in arrs])
+
+    def apply_stiffness(self, u):
+        u = np.ascontiguousarray(np.asarray(u, np.float64).reshape(-1))
+        out = np.zeros_like(u)
+        self.lib.orc_apply_stiffness(self.h, u, out)
+        return out.reshape(-1, 2)

@@ -1,0 +1,949 @@
+/*
+ * CPU oracle for the SPECFEM2D elastic time-stepping path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is a plain-C restatement of the
+ * reference Fortran solver (SPECFEM2D 8.1.0, CPU path) for an elastic,
+ * isotropic, P-SV, Newmark simulation on a structured internal mesh with
+ * Stacey absorbing edges.  It is the checker for the HIP implementation and
+ * the CPU baseline timed beside it; the product never links or calls it.
+ *
+ * It is compiled twice: -DREAL=float (CUSTOM_REAL=real, the reference
+ * default) and -DREAL=double (the reference --enable-double-precision
+ * build).  Operation order follows the Fortran so that the float build
+ * reproduces the reference's rounding; compile with -ffp-contract=off.
+ *
+ * Every routine cites the reference file it restates.  Parity status of
+ * each part (what pins it) is listed in the test suite's header
+ * (tests/test_oracle_golden.py) and in the design notes.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <xmmintrin.h>
+#include <pmmintrin.h>
+
+#ifndef REAL
+#define REAL float
+#endif
+
+#define NGLL 5
+#define NGLL2 25
+#define MAXLAYERS 32
+
+/* ------------------------------------------------------------------ */
+/* public parameter blocks (mirrored by tests/oracle.py via ctypes)    */
+/* ------------------------------------------------------------------ */
+typedef struct {
+  int nx, nz;                 /* elements along x and z (nxread, nzread) */
+  int nlayers;                /* flat layers, bottom to top */
+  int nz_layer[MAXLAYERS];    /* elements per layer */
+  double xmin, xmax;
+  double zinterface[MAXLAYERS + 1]; /* interface heights, bottom first */
+  int absorb_bottom, absorb_right, absorb_top, absorb_left;
+  int stacey;                 /* STACEY_ABSORBING_CONDITIONS */
+  int nstep;
+  double dt;
+  double user_t0;             /* USER_T0 (0 = automatic) */
+  double anglerec_deg;        /* anglerec */
+  int material_mode;          /* 0: Par_file material table, 1: MODEL=gll */
+} orc_params;
+
+typedef struct {
+  double x, z;
+  double f0;
+  double factor;
+  double angle_deg;           /* anglesource, degrees */
+  double tshift;
+  int time_function_type;     /* 1 Ricker, 2 first derivative of Gaussian */
+  int pad;
+} orc_source;
+
+typedef struct {
+  orc_params p;
+  int nspec, nglob, npgeo;
+  int *knods;                 /* [nspec][4] 0-based control nodes */
+  double *coorg;              /* [npgeo][2] */
+  int *ibool;                 /* [nspec][25] 1-based (i fastest, then j) */
+  double *coord;              /* [nglob][2] */
+  double xigll[NGLL];
+  REAL wxgll[NGLL];
+  REAL hprime[NGLL][NGLL];    /* hprime[i][k] = h'_k(xi_i)  (Fortran hprime_xx(i,k)) */
+  REAL hprimewgll[NGLL][NGLL];/* hprimewgll[k][i] = w_k h'_i(xi_k) (Fortran hprimewgll_xx(k,i)) */
+  REAL *xix, *xiz, *gammax, *gammaz, *jacobian; /* [nspec][25] */
+  REAL *rhostore, *mustore, *kappastore, *rho_vp, *rho_vs;
+  REAL *rmass_inv;            /* [nglob][2] */
+  /* absorbing edges */
+  int nabs;
+  int *abs_ispec;             /* 1-based */
+  int *abs_code;              /* 1 bottom, 2 right, 3 top, 4 left */
+  int *abs_corner;            /* [nabs][4] */
+  int *ib;                    /* index within its own side, 1-based */
+  int nspec_side[5];
+  /* sources */
+  int nsrc;
+  orc_source *src;
+  int *src_ispec;
+  double *src_xi, *src_gamma;
+  REAL *sourcearrays;         /* [nsrc][25][2]  (c fastest) */
+  REAL *stf;                  /* [nsrc][nstep] */
+  double t0;
+  /* receivers */
+  int nrec;
+  double *rec_xz;
+  int *rec_ispec;
+  double *rec_xi, *rec_gamma;
+  REAL *xir, *gammar;         /* [nrec][5] */
+  double cosrot, sinrot;
+  /* wavefields */
+  REAL *displ, *veloc, *accel;/* [nglob][2] */
+  int ftz;                    /* run the time loop with flush-to-zero */
+} orc_sim;
+
+/* ------------------------------------------------------------------ */
+/* GLL basis  (src/shared/gll_library.f90, lagrange_poly.f90,          */
+/*             src/specfem2D/define_derivation_matrices.f90:34-82)     */
+/* ------------------------------------------------------------------ */
+static double pnleg(double z, int n) {
+  /* Legendre polynomial by the three-term recurrence (lagrange_poly.f90 pnleg) */
+  double p1 = 1.0, p2 = z, p3 = p2;
+  if (n == 0) return 1.0;
+  for (int k = 1; k < n; k++) {
+    double fk = (double)k;
+    p3 = ((2.0 * fk + 1.0) * z * p2 - fk * p1) / (fk + 1.0);
+    p1 = p2;
+    p2 = p3;
+  }
+  return p3;
+}
+
+static double pndleg(double z, int n) {
+  /* derivative of the Legendre polynomial (lagrange_poly.f90 pndleg) */
+  double p1 = 1.0, p2 = z, p1d = 0.0, p2d = 1.0, p3d = 1.0;
+  if (n == 0) return 0.0;
+  for (int k = 1; k < n; k++) {
+    double fk = (double)k;
+    double p3 = ((2.0 * fk + 1.0) * z * p2 - fk * p1) / (fk + 1.0);
+    p3d = ((2.0 * fk + 1.0) * p2 + (2.0 * fk + 1.0) * z * p2d - fk * p1d) / (fk + 1.0);
+    p1 = p2; p2 = p3; p1d = p2d; p2d = p3d;
+  }
+  return p3d;
+}
+
+static double lagrange_deriv_gll(int i, int j, const double *z, int nz) {
+  /* h'_i(z_j)  (lagrange_poly.f90:102-136) */
+  int deg = nz - 1;
+  if (i == 0 && j == 0) return -(double)deg * ((double)deg + 1.0) * 0.25;
+  if (i == deg && j == deg) return (double)deg * ((double)deg + 1.0) * 0.25;
+  if (i == j) return 0.0;
+  return pnleg(z[j], deg) / (pnleg(z[i], deg) * (z[j] - z[i])) +
+         (1.0 - z[j] * z[j]) * pndleg(z[j], deg) /
+             ((double)deg * ((double)deg + 1.0) * pnleg(z[i], deg) * (z[j] - z[i]) * (z[j] - z[i]));
+}
+
+static void setup_gll(orc_sim *s) {
+  /* nodes and weights of 5-point Gauss-Lobatto-Legendre quadrature: roots of
+     (1-x^2) P'_4(x); weights 2/(N(N+1) P_N(x)^2) with N=4 */
+  const double r = sqrt(3.0 / 7.0);
+  double x[NGLL] = {-1.0, -r, 0.0, r, 1.0};
+  for (int i = 0; i < NGLL; i++) {
+    double pn = pnleg(x[i], NGLL - 1);
+    double w = 2.0 / ((double)(NGLL - 1) * (double)NGLL * pn * pn);
+    s->xigll[i] = x[i];
+    s->wxgll[i] = (REAL)w;
+  }
+  s->xigll[2] = 0.0; /* define_derivation_matrices.f90:57 */
+  for (int i1 = 0; i1 < NGLL; i1++)
+    for (int i2 = 0; i2 < NGLL; i2++) {
+      s->hprime[i2][i1] = (REAL)lagrange_deriv_gll(i1, i2, s->xigll, NGLL);
+      s->hprimewgll[i2][i1] = s->wxgll[i2] * s->hprime[i2][i1];
+    }
+}
+
+static void lagrange_any(double xi, const double *xigll, double *h) {
+  /* Lagrange interpolants at xi (lagrange_poly.f90:34-93) */
+  for (int d = 0; d < NGLL; d++) {
+    double prod1 = 1.0, prod2 = 1.0, x0 = xigll[d];
+    for (int i = 0; i < NGLL; i++)
+      if (i != d) {
+        prod1 *= (xi - xigll[i]);
+        prod2 *= (x0 - xigll[i]);
+      }
+    h[d] = prod1 * (1.0 / prod2);
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* mesher: structured grid with flat interfaces                         */
+/* (src/meshfem2D/meshfem2D.F90:133-141, read_mesh_files.F90:169-338)  */
+/* ------------------------------------------------------------------ */
+static void mesh_internal(orc_sim *s) {
+  const orc_params *p = &s->p;
+  int nx = p->nx, nz = p->nz;
+  s->nspec = nx * nz;
+  s->npgeo = (nx + 1) * (nz + 1);
+  s->coorg = (double *)calloc((size_t)2 * s->npgeo, sizeof(double));
+  s->knods = (int *)malloc(sizeof(int) * 4 * (size_t)s->nspec);
+  int ioff = 0;
+  for (int il = 0; il < p->nlayers; il++) {
+    double bot = p->zinterface[il], top = p->zinterface[il + 1];
+    for (int ix = 0; ix <= nx; ix++) {
+      double absx = p->xmin + (p->xmax - p->xmin) * (double)ix / (double)nx;
+      for (int iz = 0; iz <= p->nz_layer[il]; iz++) {
+        double g = (double)iz / (double)p->nz_layer[il];
+        int node = (iz + ioff) * (nx + 1) + ix;
+        s->coorg[2 * node] = absx;
+        s->coorg[2 * node + 1] = (1.0 - g) * bot + g * top;
+      }
+    }
+    ioff += p->nz_layer[il];
+  }
+  int e = 0;
+  for (int j = 1; j <= nz; j++)
+    for (int i = 1; i <= nx; i++) {
+      s->knods[4 * e + 0] = (j - 1) * (nx + 1) + (i - 1);
+      s->knods[4 * e + 1] = (j - 1) * (nx + 1) + (i - 1) + 1;
+      s->knods[4 * e + 2] = j * (nx + 1) + (i - 1) + 1;
+      s->knods[4 * e + 3] = j * (nx + 1) + (i - 1);
+      e++;
+    }
+}
+
+static void shape4(double xi, double ga, double *sh, double *dsh /* [2][4] */) {
+  /* bilinear shape functions (src/shared/define_shape_functions.f90, NGNOD=4) */
+  double sp = xi + 1.0, sm = xi - 1.0, tp = ga + 1.0, tm = ga - 1.0;
+  sh[0] = 0.25 * sm * tm;
+  sh[1] = -0.25 * sp * tm;
+  sh[2] = 0.25 * sp * tp;
+  sh[3] = -0.25 * sm * tp;
+  dsh[0] = 0.25 * tm;
+  dsh[1] = -0.25 * tm;
+  dsh[2] = 0.25 * tp;
+  dsh[3] = -0.25 * tp;
+  dsh[4] = 0.25 * sm;
+  dsh[5] = -0.25 * sp;
+  dsh[6] = 0.25 * sp;
+  dsh[7] = -0.25 * sm;
+}
+
+static void recompute_jacobian(const orc_sim *s, int ispec, double xi, double ga, double *x, double *z,
+                               double *xix, double *xiz, double *gax, double *gaz, double *jac) {
+  /* src/specfem2D/recompute_jacobian.f90:37-161 */
+  double sh[4], dsh[8];
+  shape4(xi, ga, sh, dsh);
+  double xx = 0, zz = 0, xxi = 0, zxi = 0, xga = 0, zga = 0;
+  for (int a = 0; a < 4; a++) {
+    int n = s->knods[4 * ispec + a];
+    double xe = s->coorg[2 * n], ze = s->coorg[2 * n + 1];
+    xx = xx + sh[a] * xe;
+    zz = zz + sh[a] * ze;
+    xxi = xxi + dsh[a] * xe;
+    zxi = zxi + dsh[a] * ze;
+    xga = xga + dsh[4 + a] * xe;
+    zga = zga + dsh[4 + a] * ze;
+  }
+  double j = xxi * zga - xga * zxi;
+  *x = xx;
+  *z = zz;
+  *jac = j;
+  *xix = zga / j;
+  *gax = -zxi / j;
+  *xiz = -xga / j;
+  *gaz = xxi / j;
+}
+
+/* ------------------------------------------------------------------ */
+/* global numbering: coordinate classes, then first-touch renumbering  */
+/* (src/specfem2D/createnum_fast.f90, get_global.f90:34-72)            */
+/* ------------------------------------------------------------------ */
+static const double *g_sortkey;
+static int cmp_by_x(const void *a, const void *b) {
+  double xa = g_sortkey[2 * *(const int *)a], xb = g_sortkey[2 * *(const int *)b];
+  return (xa > xb) - (xa < xb);
+}
+static int cmp_by_z(const void *a, const void *b) {
+  double za = g_sortkey[2 * *(const int *)a + 1], zb = g_sortkey[2 * *(const int *)b + 1];
+  return (za > zb) - (za < zb);
+}
+
+static void setup_numbering(orc_sim *s) {
+  int ntot = s->nspec * NGLL2;
+  double *xp = (double *)malloc(sizeof(double) * 2 * (size_t)ntot);
+  for (int e = 0; e < s->nspec; e++)
+    for (int j = 0; j < NGLL; j++)
+      for (int i = 0; i < NGLL; i++) {
+        double x, z, a, b, c, d, jac;
+        recompute_jacobian(s, e, s->xigll[i], s->xigll[j], &x, &z, &a, &b, &c, &d, &jac);
+        int l = e * NGLL2 + j * NGLL + i;
+        xp[2 * l] = x;
+        xp[2 * l + 1] = z;
+      }
+  /* tolerance relative to the mesh extent (createnum_fast: SMALLVALTOL * typical distance) */
+  double xmin = 1e300, xmax = -1e300, zmin = 1e300, zmax = -1e300;
+  for (int l = 0; l < ntot; l++) {
+    if (xp[2 * l] < xmin) xmin = xp[2 * l];
+    if (xp[2 * l] > xmax) xmax = xp[2 * l];
+    if (xp[2 * l + 1] < zmin) zmin = xp[2 * l + 1];
+    if (xp[2 * l + 1] > zmax) zmax = xp[2 * l + 1];
+  }
+  double ext = fmax(xmax - xmin, zmax - zmin);
+  double tol = 1e-10 * ext;
+  int *idx = (int *)malloc(sizeof(int) * (size_t)ntot);
+  int *cls = (int *)malloc(sizeof(int) * (size_t)ntot);
+  for (int l = 0; l < ntot; l++) idx[l] = l;
+  g_sortkey = xp;
+  qsort(idx, (size_t)ntot, sizeof(int), cmp_by_x);
+  int ncls = 0;
+  int a = 0;
+  while (a < ntot) {
+    int b = a + 1;
+    while (b < ntot && xp[2 * idx[b]] - xp[2 * idx[b - 1]] <= tol) b++;
+    qsort(idx + a, (size_t)(b - a), sizeof(int), cmp_by_z);
+    int c = a;
+    while (c < b) {
+      int d = c + 1;
+      while (d < b && xp[2 * idx[d] + 1] - xp[2 * idx[d - 1] + 1] <= tol) d++;
+      for (int k = c; k < d; k++) cls[idx[k]] = ncls;
+      ncls++;
+      c = d;
+    }
+    a = b;
+  }
+  /* first-touch renumbering, element-major, i fastest */
+  int *mask = (int *)malloc(sizeof(int) * (size_t)ncls);
+  for (int k = 0; k < ncls; k++) mask[k] = -1;
+  s->ibool = (int *)malloc(sizeof(int) * (size_t)ntot);
+  int inum = 0;
+  for (int l = 0; l < ntot; l++) {
+    int c = cls[l];
+    if (mask[c] == -1) {
+      inum++;
+      mask[c] = inum;
+    }
+    s->ibool[l] = mask[c];
+  }
+  s->nglob = inum;
+  s->coord = (double *)calloc((size_t)2 * s->nglob, sizeof(double));
+  for (int l = 0; l < ntot; l++) {
+    int g = s->ibool[l] - 1;
+    s->coord[2 * g] = xp[2 * l];
+    s->coord[2 * g + 1] = xp[2 * l + 1];
+  }
+  free(mask);
+  free(cls);
+  free(idx);
+  free(xp);
+}
+
+/* ------------------------------------------------------------------ */
+/* geometry and materials (setup_mesh.F90:195-236, 680-792;            */
+/* read_materials.f90:138-141)                                         */
+/* ------------------------------------------------------------------ */
+static void setup_geometry_materials(orc_sim *s, const double *rho, const double *vp, const double *vs) {
+  size_t n = (size_t)s->nspec * NGLL2;
+  s->xix = (REAL *)malloc(sizeof(REAL) * n);
+  s->xiz = (REAL *)malloc(sizeof(REAL) * n);
+  s->gammax = (REAL *)malloc(sizeof(REAL) * n);
+  s->gammaz = (REAL *)malloc(sizeof(REAL) * n);
+  s->jacobian = (REAL *)malloc(sizeof(REAL) * n);
+  s->rhostore = (REAL *)malloc(sizeof(REAL) * n);
+  s->mustore = (REAL *)malloc(sizeof(REAL) * n);
+  s->kappastore = (REAL *)malloc(sizeof(REAL) * n);
+  s->rho_vp = (REAL *)malloc(sizeof(REAL) * n);
+  s->rho_vs = (REAL *)malloc(sizeof(REAL) * n);
+  for (int e = 0; e < s->nspec; e++)
+    for (int j = 0; j < NGLL; j++)
+      for (int i = 0; i < NGLL; i++) {
+        size_t l = (size_t)e * NGLL2 + j * NGLL + i;
+        double x, z, a, b, c, d, jac;
+        recompute_jacobian(s, e, s->xigll[i], s->xigll[j], &x, &z, &a, &b, &c, &d, &jac);
+        s->xix[l] = (REAL)a;
+        s->xiz[l] = (REAL)b;
+        s->gammax[l] = (REAL)c;
+        s->gammaz[l] = (REAL)d;
+        s->jacobian[l] = (REAL)jac;
+        double rhol = rho[l], vpl = vp[l], vsl = vs[l], mul, kappal;
+        if (s->p.material_mode == 0) {
+          double lp2m = rhol * vpl * vpl;
+          mul = rhol * vsl * vsl;
+          double lambdal = lp2m - 2.0 * mul;
+          kappal = lambdal + mul;
+          vpl = sqrt((kappal + mul) / rhol);
+        } else {
+          mul = rhol * vsl * vsl;
+          kappal = rhol * vpl * vpl - mul;
+        }
+        s->rhostore[l] = (REAL)rhol;
+        s->mustore[l] = (REAL)mul;
+        s->kappastore[l] = (REAL)kappal;
+        double vsr = sqrt(mul / rhol);
+        s->rho_vp[l] = (REAL)(rhol * vpl);
+        s->rho_vs[l] = (REAL)(rhol * vsr);
+      }
+}
+
+/* ------------------------------------------------------------------ */
+/* absorbing edges (src/meshfem2D/determine_abs_surface.f90,           */
+/* src/specfem2D/read_mesh_databases.F90:957-1030)                     */
+/* ------------------------------------------------------------------ */
+static void setup_absorbing(orc_sim *s) {
+  const orc_params *p = &s->p;
+  int n = 0;
+  if (p->absorb_bottom) n += p->nx;
+  if (p->absorb_top) n += p->nx;
+  if (p->absorb_left) n += p->nz;
+  if (p->absorb_right) n += p->nz;
+  s->nabs = n;
+  s->abs_ispec = (int *)calloc((size_t)(n + 1), sizeof(int));
+  s->abs_code = (int *)calloc((size_t)(n + 1), sizeof(int));
+  s->abs_corner = (int *)calloc((size_t)4 * (n + 1), sizeof(int));
+  s->ib = (int *)calloc((size_t)(n + 1), sizeof(int));
+  n = 0;
+  for (int iz = 1; iz <= p->nz; iz++)
+    for (int ix = 1; ix <= p->nx; ix++) {
+      int el = (iz - 1) * p->nx + ix;
+      if (p->absorb_bottom && iz == 1) { s->abs_ispec[n] = el; s->abs_code[n] = 1; n++; }
+      if (p->absorb_right && ix == p->nx) { s->abs_ispec[n] = el; s->abs_code[n] = 2; n++; }
+      if (p->absorb_top && iz == p->nz) { s->abs_ispec[n] = el; s->abs_code[n] = 3; n++; }
+      if (p->absorb_left && ix == 1) { s->abs_ispec[n] = el; s->abs_code[n] = 4; n++; }
+    }
+  for (int a = 0; a < s->nabs; a++) {
+    if (s->abs_code[a] != 1 && s->abs_code[a] != 3) continue;
+    for (int b = 0; b < s->nabs; b++) {
+      if (a == b || s->abs_ispec[a] != s->abs_ispec[b]) continue;
+      int off = (s->abs_code[a] == 1) ? 0 : 2;
+      if (s->abs_code[b] == 4) s->abs_corner[4 * a + off + 0] = 1;
+      if (s->abs_code[b] == 2) s->abs_corner[4 * a + off + 1] = 1;
+    }
+  }
+  memset(s->nspec_side, 0, sizeof(s->nspec_side));
+  for (int a = 0; a < s->nabs; a++) s->ib[a] = ++s->nspec_side[s->abs_code[a]];
+}
+
+/* edge geometry at GLL point (i,j) of an absorbing element: outward normal
+   and 1D jacobian (compute_stacey_elastic.f90 / prepare_timerun.F90:1316-1460) */
+static void edge_normal(const orc_sim *s, int code, size_t l, REAL *nx, REAL *nz, REAL *jac1d) {
+  REAL xxi, zxi, xgamma, zgamma;
+  if (code == 4 || code == 2) {
+    xgamma = -s->xiz[l] * s->jacobian[l];
+    zgamma = +s->xix[l] * s->jacobian[l];
+    *jac1d = (REAL)sqrt(xgamma * xgamma + zgamma * zgamma);
+    if (code == 4) { *nx = -zgamma / *jac1d; *nz = +xgamma / *jac1d; }
+    else { *nx = +zgamma / *jac1d; *nz = -xgamma / *jac1d; }
+  } else {
+    xxi = +s->gammaz[l] * s->jacobian[l];
+    zxi = -s->gammax[l] * s->jacobian[l];
+    *jac1d = (REAL)sqrt(xxi * xxi + zxi * zxi);
+    if (code == 1) { *nx = +zxi / *jac1d; *nz = -xxi / *jac1d; }
+    else { *nx = -zxi / *jac1d; *nz = +xxi / *jac1d; }
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* mass matrix (invert_mass_matrix.F90:34-464, 470-792, 798-848)       */
+/* ------------------------------------------------------------------ */
+static void setup_mass(orc_sim *s) {
+  s->rmass_inv = (REAL *)calloc((size_t)2 * s->nglob, sizeof(REAL));
+  REAL *rm = s->rmass_inv;
+  for (int e = 0; e < s->nspec; e++)
+    for (int j = 0; j < NGLL; j++)
+      for (int i = 0; i < NGLL; i++) {
+        size_t l = (size_t)e * NGLL2 + j * NGLL + i;
+        int g = s->ibool[l] - 1;
+        double rhol = (double)s->rhostore[l];
+        REAL wxz = s->wxgll[i] * s->wxgll[j];
+        rm[2 * g] = (REAL)((double)rm[2 * g] + (double)wxz * rhol * (double)s->jacobian[l]);
+        rm[2 * g + 1] = rm[2 * g];
+      }
+  if (s->p.stacey) {
+    REAL deltatover2 = (REAL)(0.5 * (double)(REAL)s->p.dt);
+    for (int a = 0; a < s->nabs; a++) {
+      int e = s->abs_ispec[a] - 1, code = s->abs_code[a];
+      for (int k = 0; k < NGLL; k++) {
+        int i, j;
+        if (code == 4) { i = 0; j = k; }
+        else if (code == 2) { i = NGLL - 1; j = k; }
+        else if (code == 1) { i = k; j = 0; }
+        else { i = k; j = NGLL - 1; }
+        if (code == 1 && ((s->abs_corner[4 * a + 0] && i == 0) || (s->abs_corner[4 * a + 1] && i == NGLL - 1))) continue;
+        if (code == 3 && ((s->abs_corner[4 * a + 2] && i == 0) || (s->abs_corner[4 * a + 3] && i == NGLL - 1))) continue;
+        size_t l = (size_t)e * NGLL2 + j * NGLL + i;
+        int g = s->ibool[l] - 1;
+        REAL nx, nz, j1d;
+        edge_normal(s, code, l, &nx, &nz, &j1d);
+        REAL weight = j1d * ((code == 4 || code == 2) ? s->wxgll[j] : s->wxgll[i]);
+        REAL vx = deltatover2, vz = deltatover2;
+        REAL vn = nx * vx + nz * vz;
+        REAL tx = s->rho_vp[l] * vn * nx + s->rho_vs[l] * (vx - vn * nx);
+        REAL tz = s->rho_vp[l] * vn * nz + s->rho_vs[l] * (vz - vn * nz);
+        rm[2 * g] = rm[2 * g] + tx * weight;
+        rm[2 * g + 1] = rm[2 * g + 1] + tz * weight;
+      }
+    }
+  }
+  for (int g = 0; g < 2 * s->nglob; g++) {
+    if (rm[g] <= (REAL)0) rm[g] = (REAL)1;
+    rm[g] = (REAL)1 / rm[g];
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* point location (locate_receivers.F90:148-340, locate_source.F90)    */
+/* ------------------------------------------------------------------ */
+static void locate_point(const orc_sim *s, double px, double pz, int newton_iter, int *ispec_out, double *xi_out,
+                         double *gamma_out) {
+  double distmin = 1e30;
+  int isel = 0, ixg = 0, izg = 0;
+  for (int e = 0; e < s->nspec; e++)
+    for (int j = 0; j < NGLL; j++)
+      for (int i = 0; i < NGLL; i++) {
+        int g = s->ibool[e * NGLL2 + j * NGLL + i] - 1;
+        double dx = px - s->coord[2 * g], dz = pz - s->coord[2 * g + 1];
+        double d2 = dx * dx + dz * dz;
+        if (d2 < distmin) { distmin = d2; isel = e; ixg = i; izg = j; }
+      }
+  /* elements sharing a corner with the selected one, in element order */
+  int corners[4] = {s->ibool[isel * NGLL2 + 0], s->ibool[isel * NGLL2 + NGLL - 1],
+                    s->ibool[isel * NGLL2 + NGLL2 - 1], s->ibool[isel * NGLL2 + (NGLL - 1) * NGLL]};
+  int *cand = (int *)malloc(sizeof(int) * (size_t)(s->nspec + 1));
+  int nc = 0;
+  cand[nc++] = isel;
+  for (int e = 0; e < s->nspec; e++) {
+    if (e == isel) continue;
+    int hit = 0;
+    for (int j = 0; j < NGLL && !hit; j += NGLL - 1)
+      for (int i = 0; i < NGLL && !hit; i += NGLL - 1) {
+        int g = s->ibool[e * NGLL2 + j * NGLL + i];
+        for (int c = 0; c < 4; c++)
+          if (g == corners[c]) hit = 1;
+      }
+    if (hit) cand[nc++] = e;
+  }
+  double best = 1e30;
+  for (int c = 0; c < nc; c++) {
+    if (c > 0) { ixg = NGLL / 2 - 1; izg = NGLL / 2 - 1; } /* Fortran int(NGLLX/2.0)=2, 1-based */
+    int e = cand[c];
+    double xi = s->xigll[ixg], ga = s->xigll[izg];
+    double x, z, a, b, cc, d, jac;
+    for (int it = 0; it < newton_iter; it++) {
+      recompute_jacobian(s, e, xi, ga, &x, &z, &a, &b, &cc, &d, &jac);
+      double dx = -(x - px), dz = -(z - pz);
+      double dxi = a * dx + b * dz, dga = cc * dx + d * dz;
+      xi = xi + dxi;
+      ga = ga + dga;
+      if (xi > 1.01) xi = 1.01;
+      if (xi < -1.01) xi = -1.01;
+      if (ga > 1.01) ga = 1.01;
+      if (ga < -1.01) ga = -1.01;
+    }
+    recompute_jacobian(s, e, xi, ga, &x, &z, &a, &b, &cc, &d, &jac);
+    double dist = sqrt((px - x) * (px - x) + (pz - z) * (pz - z));
+    if (dist < best) {
+      best = dist;
+      *ispec_out = e + 1;
+      *xi_out = xi;
+      *gamma_out = ga;
+    }
+  }
+  free(cand);
+}
+
+static void setup_sources(orc_sim *s) {
+  const double PI = 3.141592653589793;
+  int n = s->nsrc;
+  s->src_ispec = (int *)calloc((size_t)n + 1, sizeof(int));
+  s->src_xi = (double *)calloc((size_t)n + 1, sizeof(double));
+  s->src_gamma = (double *)calloc((size_t)n + 1, sizeof(double));
+  s->sourcearrays = (REAL *)calloc((size_t)(n + 1) * NGLL2 * 2, sizeof(REAL));
+  s->stf = (REAL *)calloc((size_t)(n + 1) * s->p.nstep, sizeof(REAL));
+  /* t0 and time shifts (set_source_parameters.f90:116-250) */
+  double *t0s = (double *)calloc((size_t)n + 1, sizeof(double));
+  double *tsh = (double *)calloc((size_t)n + 1, sizeof(double));
+  for (int k = 0; k < n; k++) {
+    double hdur = 1.0 / s->src[k].f0;
+    tsh[k] = s->src[k].tshift;
+    t0s[k] = 1.20 * hdur + tsh[k];
+  }
+  double t0, min_tshift;
+  if (n == 1) {
+    t0 = t0s[0];
+    min_tshift = tsh[0];
+    tsh[0] = 0.0;
+  } else {
+    t0 = t0s[0];
+    min_tshift = tsh[0];
+    for (int k = 1; k < n; k++) {
+      if (t0s[k] < t0) t0 = t0s[k];
+      if (tsh[k] < min_tshift) min_tshift = tsh[k];
+    }
+    for (int k = 0; k < n; k++) tsh[k] = t0s[k] - t0;
+  }
+  if (s->p.user_t0 > 0.0 && t0 <= s->p.user_t0 + min_tshift) {
+    t0 = s->p.user_t0;
+    for (int k = 0; k < n; k++) tsh[k] = t0s[k] - 1.20 / s->src[k].f0;
+  }
+  s->t0 = t0;
+  for (int k = 0; k < n; k++) {
+    locate_point(s, s->src[k].x, s->src[k].z, 5, &s->src_ispec[k], &s->src_xi[k], &s->src_gamma[k]);
+    double hx[NGLL], hg[NGLL];
+    lagrange_any(s->src_xi[k], s->xigll, hx);
+    lagrange_any(s->src_gamma[k], s->xigll, hg);
+    double ang = s->src[k].angle_deg * PI / 180.0;
+    for (int j = 0; j < NGLL; j++)
+      for (int i = 0; i < NGLL; i++) {
+        double hl = hx[i] * hg[j];
+        s->sourcearrays[((size_t)k * NGLL2 + j * NGLL + i) * 2 + 0] = (REAL)(+sin(ang) * hl);
+        s->sourcearrays[((size_t)k * NGLL2 + j * NGLL + i) * 2 + 1] = (REAL)(-cos(ang) * hl);
+      }
+    /* source time function (prepare_source_time_function.f90, comp_source_time_function.f90) */
+    double f0 = s->src[k].f0, a = PI * PI * f0 * f0;
+    for (int it = 1; it <= s->p.nstep; it++) {
+      double timeval = (double)(it - 1) * s->p.dt;
+      double t = timeval - t0 - tsh[k];
+      double v;
+      if (s->src[k].time_function_type == 1)
+        v = -s->src[k].factor * ((1.0 - 2.0 * a * t * t) * exp(-a * t * t));
+      else
+        v = -s->src[k].factor * (t * exp(-a * t * t));
+      s->stf[(size_t)k * s->p.nstep + (it - 1)] = (REAL)v;
+    }
+  }
+  free(t0s);
+  free(tsh);
+}
+
+static void setup_receivers(orc_sim *s) {
+  const double PI = 3.141592653589793;
+  int n = s->nrec;
+  s->rec_ispec = (int *)calloc((size_t)n + 1, sizeof(int));
+  s->rec_xi = (double *)calloc((size_t)n + 1, sizeof(double));
+  s->rec_gamma = (double *)calloc((size_t)n + 1, sizeof(double));
+  s->xir = (REAL *)calloc((size_t)(n + 1) * NGLL, sizeof(REAL));
+  s->gammar = (REAL *)calloc((size_t)(n + 1) * NGLL, sizeof(REAL));
+  for (int r = 0; r < n; r++) {
+    locate_point(s, s->rec_xz[2 * r], s->rec_xz[2 * r + 1], 5, &s->rec_ispec[r], &s->rec_xi[r], &s->rec_gamma[r]);
+    double hx[NGLL], hg[NGLL];
+    lagrange_any(s->rec_xi[r], s->xigll, hx);
+    lagrange_any(s->rec_gamma[r], s->xigll, hg);
+    for (int i = 0; i < NGLL; i++) {
+      s->xir[r * NGLL + i] = (REAL)hx[i];
+      s->gammar[r * NGLL + i] = (REAL)hg[i];
+    }
+  }
+  double ang = s->p.anglerec_deg * PI / 180.0;
+  s->cosrot = cos(ang);
+  s->sinrot = sin(ang);
+}
+
+/* ------------------------------------------------------------------ */
+/* time step pieces                                                     */
+/* ------------------------------------------------------------------ */
+static void element_forces(orc_sim *s, const REAL *displ, REAL *accel) {
+  /* compute_forces_viscoelastic.F90:34-885, isotropic P-SV, no PML/AXISYM/attenuation */
+  const REAL (*H)[NGLL] = (const REAL(*)[NGLL])s->hprime;
+  const REAL (*HW)[NGLL] = (const REAL(*)[NGLL])s->hprimewgll;
+  REAL u[2][NGLL][NGLL]; /* u[c][j][i] */
+  REAL t1x[NGLL][NGLL], t1z[NGLL][NGLL], t2x[NGLL][NGLL], t2z[NGLL][NGLL];
+  for (int e = 0; e < s->nspec; e++) {
+    const int *ib = s->ibool + (size_t)e * NGLL2;
+    for (int j = 0; j < NGLL; j++)
+      for (int i = 0; i < NGLL; i++) {
+        int g = ib[j * NGLL + i] - 1;
+        u[0][j][i] = displ[2 * g];
+        u[1][j][i] = displ[2 * g + 1];
+      }
+    for (int j = 0; j < NGLL; j++)
+      for (int i = 0; i < NGLL; i++) {
+        size_t l = (size_t)e * NGLL2 + j * NGLL + i;
+        /* mxm_4comp_singleA: left-to-right sums of five products */
+        REAL dux_dxi = u[0][j][0] * H[i][0] + u[0][j][1] * H[i][1] + u[0][j][2] * H[i][2] + u[0][j][3] * H[i][3] +
+                       u[0][j][4] * H[i][4];
+        REAL duz_dxi = u[1][j][0] * H[i][0] + u[1][j][1] * H[i][1] + u[1][j][2] * H[i][2] + u[1][j][3] * H[i][3] +
+                       u[1][j][4] * H[i][4];
+        REAL dux_dga = u[0][0][i] * H[j][0] + u[0][1][i] * H[j][1] + u[0][2][i] * H[j][2] + u[0][3][i] * H[j][3] +
+                       u[0][4][i] * H[j][4];
+        REAL duz_dga = u[1][0][i] * H[j][0] + u[1][1][i] * H[j][1] + u[1][2][i] * H[j][2] + u[1][3][i] * H[j][3] +
+                       u[1][4][i] * H[j][4];
+        REAL xixl = s->xix[l], xizl = s->xiz[l], gaxl = s->gammax[l], gazl = s->gammaz[l], jacl = s->jacobian[l];
+        REAL dux_dxl = dux_dxi * xixl + dux_dga * gaxl;
+        REAL dux_dzl = dux_dxi * xizl + dux_dga * gazl;
+        REAL duz_dxl = duz_dxi * xixl + duz_dga * gaxl;
+        REAL duz_dzl = duz_dxi * xizl + duz_dga * gazl;
+        REAL mul = s->mustore[l], rhol = s->rhostore[l];
+        REAL cpl = s->rho_vp[l] / rhol;
+        REAL lambdal = rhol * cpl * cpl - (REAL)2 * mul;
+        REAL lp2m = lambdal + (REAL)2 * mul;
+        REAL sxx = lp2m * dux_dxl + lambdal * duz_dzl;
+        REAL szz = lp2m * duz_dzl + lambdal * dux_dxl;
+        REAL sxz = mul * (duz_dxl + dux_dzl);
+        REAL szx = sxz;
+        t1x[j][i] = jacl * (sxx * xixl + szx * xizl);
+        t1z[j][i] = jacl * (sxz * xixl + szz * xizl);
+        t2x[j][i] = jacl * (sxx * gaxl + szx * gazl);
+        t2z[j][i] = jacl * (sxz * gaxl + szz * gazl);
+      }
+    for (int j = 0; j < NGLL; j++)
+      for (int i = 0; i < NGLL; i++) {
+        REAL a1 = 0, a2 = 0, a3 = 0, a4 = 0;
+        for (int k = 0; k < NGLL; k++) {
+          a1 = a1 + t1x[j][k] * HW[k][i];
+          a2 = a2 + t2x[k][i] * HW[k][j];
+          a3 = a3 + t1z[j][k] * HW[k][i];
+          a4 = a4 + t2z[k][i] * HW[k][j];
+        }
+        int g = ib[j * NGLL + i] - 1;
+        accel[2 * g] = accel[2 * g] - (s->wxgll[j] * a1 + s->wxgll[i] * a2);
+        accel[2 * g + 1] = accel[2 * g + 1] - (s->wxgll[j] * a3 + s->wxgll[i] * a4);
+      }
+  }
+}
+
+static void stacey_forces(orc_sim *s, const REAL *veloc, REAL *accel, REAL *store /* may be NULL */, int it,
+                          int nstep_store) {
+  /* compute_stacey_elastic.f90:34-1001 (P-SV, no Bielak), SAVE_FORWARD storage */
+  for (int a = 0; a < s->nabs; a++) {
+    int e = s->abs_ispec[a] - 1, code = s->abs_code[a];
+    for (int k = 0; k < NGLL; k++) {
+      int i, j;
+      if (code == 4) { i = 0; j = k; }
+      else if (code == 2) { i = NGLL - 1; j = k; }
+      else if (code == 1) { i = k; j = 0; }
+      else { i = k; j = NGLL - 1; }
+      if (code == 1 && ((s->abs_corner[4 * a + 0] && i == 0) || (s->abs_corner[4 * a + 1] && i == NGLL - 1))) continue;
+      if (code == 3 && ((s->abs_corner[4 * a + 2] && i == 0) || (s->abs_corner[4 * a + 3] && i == NGLL - 1))) continue;
+      size_t l = (size_t)e * NGLL2 + j * NGLL + i;
+      int g = s->ibool[l] - 1;
+      REAL nx, nz, j1d;
+      edge_normal(s, code, l, &nx, &nz, &j1d);
+      REAL vx = veloc[2 * g], vz = veloc[2 * g + 1];
+      REAL vn = nx * vx + nz * vz;
+      REAL tx = s->rho_vp[l] * vn * nx + s->rho_vs[l] * (vx - vn * nx);
+      REAL tz = s->rho_vp[l] * vn * nz + s->rho_vs[l] * (vz - vn * nz);
+      REAL weight = j1d * ((code == 4 || code == 2) ? s->wxgll[j] : s->wxgll[i]);
+      /* the reference adds a double-precision traction offset, so the product and
+         the subtraction are evaluated in double before rounding */
+      double fx = (double)tx * (double)weight, fz = (double)tz * (double)weight;
+      accel[2 * g] = (REAL)((double)accel[2 * g] - fx);
+      accel[2 * g + 1] = (REAL)((double)accel[2 * g + 1] - fz);
+      if (store) {
+        (void)nstep_store;
+        store[(((size_t)it * s->nabs + a) * NGLL + k) * 2 + 0] = (REAL)fx;
+        store[(((size_t)it * s->nabs + a) * NGLL + k) * 2 + 1] = (REAL)fz;
+      }
+    }
+  }
+}
+
+static void add_sources(orc_sim *s, REAL *accel, int it /* 1-based */) {
+  /* compute_add_sources_viscoelastic.f90:34-100 */
+  for (int k = 0; k < s->nsrc; k++) {
+    int e = s->src_ispec[k] - 1;
+    REAL stf = s->stf[(size_t)k * s->p.nstep + (it - 1)];
+    for (int j = 0; j < NGLL; j++)
+      for (int i = 0; i < NGLL; i++) {
+        int g = s->ibool[e * NGLL2 + j * NGLL + i] - 1;
+        const REAL *sa = s->sourcearrays + ((size_t)k * NGLL2 + j * NGLL + i) * 2;
+        accel[2 * g] = accel[2 * g] + sa[0] * stf;
+        accel[2 * g + 1] = accel[2 * g + 1] + sa[1] * stf;
+      }
+  }
+}
+
+static void record_seismograms(orc_sim *s, const REAL *displ, double *seis, int it, int nstep) {
+  /* write_seismograms.F90:60-200, compute_interpolated_dva.f90 */
+  for (int r = 0; r < s->nrec; r++) {
+    int e = s->rec_ispec[r] - 1;
+    double vx = 0.0, vz = 0.0;
+    for (int j = 0; j < NGLL; j++)
+      for (int i = 0; i < NGLL; i++) {
+        int g = s->ibool[e * NGLL2 + j * NGLL + i] - 1;
+        double hl = (double)(s->xir[r * NGLL + i] * s->gammar[r * NGLL + j]);
+        vx = vx + (double)displ[2 * g] * hl;
+        vz = vz + (double)displ[2 * g + 1] * hl;
+      }
+    seis[((size_t)r * 2 + 0) * nstep + it] = s->cosrot * vx + s->sinrot * vz;
+    seis[((size_t)r * 2 + 1) * nstep + it] = -s->sinrot * vx + s->cosrot * vz;
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* public API                                                           */
+/* ------------------------------------------------------------------ */
+void *orc_new(const orc_params *p, const double *rho, const double *vp, const double *vs, int nsrc,
+              const orc_source *src, int nrec, const double *rec_xz) {
+  orc_sim *s = (orc_sim *)calloc(1, sizeof(orc_sim));
+  s->p = *p;
+  setup_gll(s);
+  mesh_internal(s);
+  setup_numbering(s);
+  setup_geometry_materials(s, rho, vp, vs);
+  if (p->stacey) setup_absorbing(s);
+  setup_mass(s);
+  s->nsrc = nsrc;
+  s->src = (orc_source *)calloc((size_t)nsrc + 1, sizeof(orc_source));
+  memcpy(s->src, src, sizeof(orc_source) * (size_t)nsrc);
+  setup_sources(s);
+  s->nrec = nrec;
+  s->rec_xz = (double *)calloc((size_t)2 * nrec + 2, sizeof(double));
+  memcpy(s->rec_xz, rec_xz, sizeof(double) * 2 * (size_t)nrec);
+  setup_receivers(s);
+  s->displ = (REAL *)calloc((size_t)2 * s->nglob, sizeof(REAL));
+  s->veloc = (REAL *)calloc((size_t)2 * s->nglob, sizeof(REAL));
+  s->accel = (REAL *)calloc((size_t)2 * s->nglob, sizeof(REAL));
+  return s;
+}
+
+void orc_free(void *h) {
+  orc_sim *s = (orc_sim *)h;
+  if (!s) return;
+  free(s->knods); free(s->coorg); free(s->ibool); free(s->coord);
+  free(s->xix); free(s->xiz); free(s->gammax); free(s->gammaz); free(s->jacobian);
+  free(s->rhostore); free(s->mustore); free(s->kappastore); free(s->rho_vp); free(s->rho_vs);
+  free(s->rmass_inv);
+  free(s->abs_ispec); free(s->abs_code); free(s->abs_corner); free(s->ib);
+  free(s->src); free(s->src_ispec); free(s->src_xi); free(s->src_gamma); free(s->sourcearrays); free(s->stf);
+  free(s->rec_xz); free(s->rec_ispec); free(s->rec_xi); free(s->rec_gamma); free(s->xir); free(s->gammar);
+  free(s->displ); free(s->veloc); free(s->accel);
+  free(s);
+}
+
+int orc_real_size(void) { return (int)sizeof(REAL); }
+int orc_nspec(void *h) { return ((orc_sim *)h)->nspec; }
+int orc_nglob(void *h) { return ((orc_sim *)h)->nglob; }
+int orc_nabs(void *h) { return ((orc_sim *)h)->nabs; }
+double orc_t0(void *h) { return ((orc_sim *)h)->t0; }
+
+void orc_get_ibool(void *h, int *out) {
+  orc_sim *s = (orc_sim *)h;
+  memcpy(out, s->ibool, sizeof(int) * (size_t)s->nspec * NGLL2);
+}
+void orc_get_coord(void *h, double *out) {
+  orc_sim *s = (orc_sim *)h;
+  memcpy(out, s->coord, sizeof(double) * 2 * (size_t)s->nglob);
+}
+/* which: 0 xix 1 xiz 2 gammax 3 gammaz 4 jacobian 5 rho 6 mu 7 kappa 8 rho_vp 9 rho_vs */
+void orc_get_elem_array(void *h, int which, double *out) {
+  orc_sim *s = (orc_sim *)h;
+  REAL *arr[10] = {s->xix, s->xiz, s->gammax, s->gammaz, s->jacobian,
+                   s->rhostore, s->mustore, s->kappastore, s->rho_vp, s->rho_vs};
+  size_t n = (size_t)s->nspec * NGLL2;
+  for (size_t l = 0; l < n; l++) out[l] = (double)arr[which][l];
+}
+void orc_get_rmass_inverse(void *h, double *out) {
+  orc_sim *s = (orc_sim *)h;
+  for (int g = 0; g < 2 * s->nglob; g++) out[g] = (double)s->rmass_inv[g];
+}
+void orc_get_basis(void *h, double *xigll, double *wxgll, double *hprime, double *hprimewgll) {
+  orc_sim *s = (orc_sim *)h;
+  for (int i = 0; i < NGLL; i++) {
+    xigll[i] = s->xigll[i];
+    wxgll[i] = (double)s->wxgll[i];
+    for (int k = 0; k < NGLL; k++) {
+      hprime[i * NGLL + k] = (double)s->hprime[i][k];
+      hprimewgll[i * NGLL + k] = (double)s->hprimewgll[i][k];
+    }
+  }
+}
+void orc_get_absorbing(void *h, int *ispec, int *code, int *corner, int *ib) {
+  orc_sim *s = (orc_sim *)h;
+  for (int a = 0; a < s->nabs; a++) {
+    ispec[a] = s->abs_ispec[a];
+    code[a] = s->abs_code[a];
+    ib[a] = s->ib[a];
+    for (int c = 0; c < 4; c++) corner[4 * a + c] = s->abs_corner[4 * a + c];
+  }
+}
+void orc_get_sources(void *h, int *ispec, double *xi, double *gamma, double *sourcearrays, double *stf) {
+  orc_sim *s = (orc_sim *)h;
+  for (int k = 0; k < s->nsrc; k++) {
+    ispec[k] = s->src_ispec[k];
+    xi[k] = s->src_xi[k];
+    gamma[k] = s->src_gamma[k];
+  }
+  for (size_t l = 0; l < (size_t)s->nsrc * NGLL2 * 2; l++) sourcearrays[l] = (double)s->sourcearrays[l];
+  for (size_t l = 0; l < (size_t)s->nsrc * s->p.nstep; l++) stf[l] = (double)s->stf[l];
+}
+void orc_get_receivers(void *h, int *ispec, double *xi, double *gamma, double *xir, double *gammar) {
+  orc_sim *s = (orc_sim *)h;
+  for (int r = 0; r < s->nrec; r++) {
+    ispec[r] = s->rec_ispec[r];
+    xi[r] = s->rec_xi[r];
+    gamma[r] = s->rec_gamma[r];
+    for (int i = 0; i < NGLL; i++) {
+      xir[r * NGLL + i] = (double)s->xir[r * NGLL + i];
+      gammar[r * NGLL + i] = (double)s->gammar[r * NGLL + i];
+    }
+  }
+}
+
+/* Forward time loop (iterate_time.F90:146-414 for SIMULATION_TYPE=1, Newmark).
+   Runs nstep_run steps (<= nstep) from the current state and records displacement
+   seismograms into seis[nrec][2][nstep_run].  If stacey_store is not NULL it
+   receives the Stacey traction*weight per step [nstep_run][nabs][5][2]. */
+void orc_forward(void *h, int nstep_run, double *seis, double *stacey_store_d) {
+  orc_sim *s = (orc_sim *)h;
+  unsigned int csr = _mm_getcsr();
+  if (s->ftz) _mm_setcsr(csr | 0x8040); /* flush-to-zero + denormals-are-zero */
+  REAL deltat = (REAL)s->p.dt;
+  REAL deltatover2 = (REAL)(0.5 * (double)deltat);
+  REAL deltatsqover2 = (REAL)(0.5 * (double)deltat * (double)deltat);
+  size_t n2 = (size_t)2 * s->nglob;
+  REAL *d = s->displ, *v = s->veloc, *a = s->accel;
+  REAL *store = NULL;
+  if (stacey_store_d && s->nabs > 0) store = (REAL *)malloc(sizeof(REAL) * (size_t)nstep_run * s->nabs * NGLL * 2);
+  for (int it = 1; it <= nstep_run; it++) {
+    /* update_displacement_newmark_elastic (update_displacement_Newmark.F90:408-454) */
+    for (size_t k = 0; k < n2; k++) {
+      d[k] = d[k] + deltat * v[k] + deltatsqover2 * a[k];
+      v[k] = v[k] + deltatover2 * a[k];
+      a[k] = (REAL)0;
+    }
+    /* compute_forces_viscoelastic_main: iphase 1 (Stacey, sources), iphase 2 (elements) */
+    if (s->p.stacey && s->nabs > 0) stacey_forces(s, v, a, store, it - 1, nstep_run);
+    add_sources(s, a, it);
+    element_forces(s, d, a);
+    for (size_t k = 0; k < n2; k++) a[k] = a[k] * s->rmass_inv[k];
+    /* update_veloc_elastic_Newmark */
+    for (size_t k = 0; k < n2; k++) v[k] = v[k] + deltatover2 * a[k];
+    if (seis && s->nrec > 0) record_seismograms(s, d, seis, it - 1, nstep_run);
+  }
+  if (store) {
+    size_t n = (size_t)nstep_run * s->nabs * NGLL * 2;
+    for (size_t k = 0; k < n; k++) stacey_store_d[k] = (double)store[k];
+    free(store);
+  }
+  _mm_setcsr(csr);
+}
+
+void orc_set_ftz(void *h, int on) { ((orc_sim *)h)->ftz = on; }
+
+void orc_get_fields(void *h, double *displ, double *veloc, double *accel) {
+  orc_sim *s = (orc_sim *)h;
+  for (int g = 0; g < 2 * s->nglob; g++) {
+    if (displ) displ[g] = (double)s->displ[g];
+    if (veloc) veloc[g] = (double)s->veloc[g];
+    if (accel) accel[g] = (double)s->accel[g];
+  }
+}
+
+void orc_set_fields(void *h, const double *displ, const double *veloc, const double *accel) {
+  orc_sim *s = (orc_sim *)h;
+  for (int g = 0; g < 2 * s->nglob; g++) {
+    s->displ[g] = displ ? (REAL)displ[g] : (REAL)0;
+    s->veloc[g] = veloc ? (REAL)veloc[g] : (REAL)0;
+    s->accel[g] = accel ? (REAL)accel[g] : (REAL)0;
+  }
+}
+
+/* One application of the assembled stiffness operator: out = -K u (element forces
+   only, no mass matrix), used to test the operator against brute force. */
+void orc_apply_stiffness(void *h, const double *u, double *out) {
+  orc_sim *s = (orc_sim *)h;
+  size_t n2 = (size_t)2 * s->nglob;
+  REAL *d = (REAL *)calloc(n2, sizeof(REAL)), *a = (REAL *)calloc(n2, sizeof(REAL));
+  for (size_t k = 0; k < n2; k++) d[k] = (REAL)u[k];
+  element_forces(s, d, a);
+  for (size_t k = 0; k < n2; k++) out[k] = (double)a[k];
+  free(d);
+  free(a);
+}

@@ -1,0 +1,205 @@
+"""Simulation description for the SPECFEM2D elastic time-stepping path.
+
+A :class:`SimulationConfig` carries the subset of the reference's ``DATA/Par_file``,
+``DATA/SOURCE`` and ``DATA/STATIONS`` that the hot path needs: a structured internal
+mesh with flat interfaces (``src/meshfem2D/read_mesh_files.F90:169-338``), per-GLL-point
+or table materials (``src/specfem2D/read_materials.f90:138-141``,
+``src/specfem2D/setup_mesh.F90:680-792``), Stacey absorbing edges, point force sources
+and receivers.  Parameter names follow the Par_file keys.
+"""
+from __future__ import annotations
+
+import dataclasses
+from dataclasses import dataclass, field
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+NGLL = 5
+NGLL2 = NGLL * NGLL
+
+# GLL nodes for 5 points (roots of (1-x^2) P'_4)
+_R37 = float(np.sqrt(3.0 / 7.0))
+XIGLL = np.array([-1.0, -_R37, 0.0, _R37, 1.0])
+
+
+@dataclass
+class Source:
+    """A collocated force source (``source_type = 1`` in ``DATA/SOURCE``)."""
+
+    x: float
+    z: float
+    f0: float
+    factor: float = 1.0e10
+    anglesource: float = 0.0  # degrees
+    tshift: float = 0.0
+    time_function_type: int = 1  # 1 Ricker, 2 first derivative of a Gaussian
+
+
+@dataclass
+class SimulationConfig:
+    nx: int
+    nz: int
+    xmin: float
+    xmax: float
+    interfaces: Sequence[float]  # bottom first
+    nz_layers: Sequence[int]
+    nstep: int
+    dt: float
+    rho: object = 2700.0  # scalar or array [nspec, 25] (i fastest)
+    vp: object = 3000.0
+    vs: object = 1732.0
+    material_mode: int = 0  # 0: Par_file material table, 1: MODEL=gll per-point values
+    stacey: bool = True
+    absorbbottom: bool = True
+    absorbright: bool = True
+    absorbtop: bool = False
+    absorbleft: bool = True
+    sources: List[Source] = field(default_factory=list)
+    receivers: Optional[np.ndarray] = None  # [nrec, 2] (x, z)
+    user_t0: float = 0.0
+    anglerec: float = 0.0
+    name: str = "custom"
+
+    # ------------------------------------------------------------------
+    @property
+    def nspec(self) -> int:
+        return self.nx * self.nz
+
+    @property
+    def nglob(self) -> int:
+        return (4 * self.nx + 1) * (4 * self.nz + 1)
+
+    @property
+    def nrec(self) -> int:
+        return 0 if self.receivers is None else int(len(self.receivers))
+
+    def replace(self, **kw) -> "SimulationConfig":
+        return dataclasses.replace(self, **kw)
+
+    def material_arrays(self):
+        """Per-point (rho, vp, vs) arrays of shape [nspec*25], float64."""
+        n = self.nspec * NGLL2
+        out = []
+        for v in (self.rho, self.vp, self.vs):
+            a = np.asarray(v, dtype=np.float64)
+            if a.ndim == 0:
+                a = np.full(n, float(a))
+            a = np.ascontiguousarray(a.reshape(n))
+            out.append(a)
+        return out
+
+    def gll_point_coords(self):
+        """Coordinates (x, z) of every element GLL point, shape [nspec, 25] each.
+
+        Valid for the flat-interface structured mesh this config describes."""
+        xs = self.xmin + (self.xmax - self.xmin) * np.arange(self.nx + 1) / self.nx
+        zrows = []
+        for il, nzl in enumerate(self.nz_layers):
+            bot, top = self.interfaces[il], self.interfaces[il + 1]
+            g = np.arange(nzl) / nzl
+            zrows.append((1.0 - g) * bot + g * top)
+        zrows.append(np.array([self.interfaces[-1]]))
+        zs = np.concatenate(zrows)
+        ex = np.tile(np.arange(self.nx), self.nz)
+        ez = np.repeat(np.arange(self.nz), self.nx)
+        x0, x1 = xs[ex], xs[ex + 1]
+        z0, z1 = zs[ez], zs[ez + 1]
+        xi = np.tile(XIGLL, NGLL)  # i fastest
+        ga = np.repeat(XIGLL, NGLL)
+        X = x0[:, None] + (x1 - x0)[:, None] * (xi[None, :] + 1.0) * 0.5
+        Z = z0[:, None] + (z1 - z0)[:, None] * (ga[None, :] + 1.0) * 0.5
+        return X, Z
+
+
+# ----------------------------------------------------------------------
+# seeded Gaussian random fields (correlation length, +-amplitude)
+# ----------------------------------------------------------------------
+def gaussian_random_field(cfg: SimulationConfig, corr_len: float, amplitude: float, seed: int,
+                          spacing: Optional[float] = None) -> np.ndarray:
+    """Zero-mean field in [-amplitude, amplitude] sampled at the GLL points of ``cfg``.
+
+    White noise on a regular grid is filtered with a Gaussian kernel of the given
+    correlation length (FFT convolution), scaled so that max |f| = amplitude, and read
+    at each GLL point by nearest grid node."""
+    zmin, zmax = float(cfg.interfaces[0]), float(cfg.interfaces[-1])
+    if spacing is None:
+        spacing = min((cfg.xmax - cfg.xmin) / cfg.nx, (zmax - zmin) / cfg.nz) / 2.0
+    nxg = int(np.ceil((cfg.xmax - cfg.xmin) / spacing)) + 1
+    nzg = int(np.ceil((zmax - zmin) / spacing)) + 1
+    rng = np.random.default_rng(seed)
+    noise = rng.standard_normal((nzg, nxg)).astype(np.float32)
+    kx = np.fft.rfftfreq(nxg, d=spacing)
+    kz = np.fft.fftfreq(nzg, d=spacing)
+    # Fourier transform of exp(-r^2/a^2) is proportional to exp(-pi^2 a^2 k^2)
+    filt = np.exp(-(np.pi * corr_len) ** 2 * (kz[:, None] ** 2 + kx[None, :] ** 2)).astype(np.float32)
+    f = np.fft.irfft2(np.fft.rfft2(noise) * filt, s=(nzg, nxg))
+    f *= amplitude / max(float(np.abs(f).max()), 1e-30)
+    X, Z = cfg.gll_point_coords()
+    ix = np.clip(np.rint((X - cfg.xmin) / spacing).astype(np.int64), 0, nxg - 1)
+    iz = np.clip(np.rint((Z - zmin) / spacing).astype(np.int64), 0, nzg - 1)
+    return f[iz, ix].reshape(-1).astype(np.float64)
+
+
+# ----------------------------------------------------------------------
+# presets
+# ----------------------------------------------------------------------
+def tromp2005_kernel(nstep: int = 3000) -> SimulationConfig:
+    """``EXAMPLES/Tromp2005_kernel/DATA/Par_file`` + ``SOURCE`` (forward run)."""
+    return SimulationConfig(
+        name="Tromp2005_kernel", nx=80, nz=32, xmin=0.0, xmax=200000.0, interfaces=[0.0, 80000.0],
+        nz_layers=[32], nstep=nstep, dt=2.0e-2, rho=2600.0, vp=5800.0, vs=3198.6, material_mode=0,
+        stacey=True, absorbbottom=True, absorbright=True, absorbtop=False, absorbleft=True,
+        sources=[Source(x=50000.0, z=40000.0, f0=0.42, factor=0.75e10, anglesource=270.0, time_function_type=2)],
+        receivers=np.array([[150000.0, 40000.0]]), user_t0=8.0)
+
+
+def plain_elastic_c1(nstep: int = 1600) -> SimulationConfig:
+    """Plain forward elastic half-space: 80x60 elements over 4000 m x 3000 m,
+    vp 3000, vs 1732, rho 2700, Stacey on three sides, Ricker 10 Hz at (2000, -2000),
+    dt 1.1e-3, 20 surface receivers at x = 100 + 200 k."""
+    rec = np.stack([100.0 + 200.0 * np.arange(20), np.zeros(20)], axis=1)
+    return SimulationConfig(
+        name="plain_elastic_80x60", nx=80, nz=60, xmin=0.0, xmax=4000.0, interfaces=[-3000.0, 0.0],
+        nz_layers=[60], nstep=nstep, dt=1.1e-3, rho=2700.0, vp=3000.0, vs=1732.0, material_mode=0,
+        stacey=True, absorbbottom=True, absorbright=True, absorbtop=False, absorbleft=True,
+        sources=[Source(x=2000.0, z=-2000.0, f0=10.0, factor=1.0e10, anglesource=0.0, time_function_type=1)],
+        receivers=rec, user_t0=0.0)
+
+
+def random_elastic(nx: int, nz: int, elem_size: float, nstep: int, dt: float, nrec: int, seed: int = 1,
+                   f0: float = 8.0, corr_len: float = 200.0, amplitude: float = 0.10,
+                   vp0: float = 3000.0, vs0: float = 1732.0, rho0: float = 2500.0,
+                   name: Optional[str] = None) -> SimulationConfig:
+    """Elastic half-space over a seeded Gaussian random field (+-amplitude around
+    vp0/vs0/rho0), Stacey on three sides, free surface on top, a Ricker source at
+    mid-width one third down, and ``nrec`` receivers evenly spread along the surface."""
+    xmax = nx * elem_size
+    depth = nz * elem_size
+    cfg = SimulationConfig(
+        name=name or f"random_elastic_{nx}x{nz}", nx=nx, nz=nz, xmin=0.0, xmax=xmax,
+        interfaces=[-depth, 0.0], nz_layers=[nz], nstep=nstep, dt=dt, material_mode=1,
+        stacey=True, absorbbottom=True, absorbright=True, absorbtop=False, absorbleft=True,
+        sources=[Source(x=xmax / 2, z=-depth / 3, f0=f0, factor=1.0e10, anglesource=0.0, time_function_type=1)],
+        receivers=np.stack([np.linspace(elem_size, xmax - elem_size, nrec), np.zeros(nrec)], axis=1))
+    pert = gaussian_random_field(cfg, corr_len, amplitude, seed)
+    return cfg.replace(rho=rho0 * (1.0 + pert), vp=vp0 * (1.0 + pert), vs=vs0 * (1.0 + pert))
+
+
+def small_test_mesh(nx: int = 8, nz: int = 6, nstep: int = 200, seed: int = 0, stacey: bool = True,
+                    gll: bool = False) -> SimulationConfig:
+    """A small mesh for fast parity tests: 100 m elements, Ricker 15 Hz inside."""
+    elem = 100.0
+    cfg = SimulationConfig(
+        name=f"small_{nx}x{nz}", nx=nx, nz=nz, xmin=0.0, xmax=nx * elem, interfaces=[-nz * elem, 0.0],
+        nz_layers=[nz], nstep=nstep, dt=2.0e-3, rho=2500.0, vp=3000.0, vs=1700.0, material_mode=0,
+        stacey=stacey,
+        sources=[Source(x=nx * elem * 0.37, z=-nz * elem * 0.45, f0=15.0, factor=1.0e9, anglesource=30.0)],
+        receivers=np.array([[nx * elem * 0.8, 0.0], [nx * elem * 0.25, -nz * elem * 0.6],
+                            [nx * elem * 0.5 + 13.0, -nz * elem * 0.3 - 7.0]]))
+    if gll:
+        rng = np.random.default_rng(seed)
+        n = cfg.nspec * NGLL2
+        p = 0.05 * rng.standard_normal(n)
+        cfg = cfg.replace(material_mode=1, rho=2500.0 * (1 + p), vp=3000.0 * (1 + p), vs=1700.0 * (1 - p))
+    return cfg
