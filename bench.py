@@ -1,0 +1,182 @@
+"""Benchmark of the SPECFEM2D elastic time-stepping hot path on one MI355X.
+
+A step is one Newmark step of the elastic wavefield, driven through the reference's
+Fortran-to-C GPU entry points (predictor, element forces, Stacey, sources, corrector,
+seismograms).  The metric is elastic GLL-point updates per second: nglob x steps / time.
+
+Default workload: the 2000 x 1000-element mesh of the large forward configuration
+(20 km x 10 km, 10 m elements, 32,012,001 GLL points) with a seeded Gaussian random-field
+elastic model (correlation length 200 m, +-10 % around vp 3000, vs 1732, rho 2500), Stacey
+on three sides and a free surface, Ricker 8 Hz source, 200 surface receivers.  The water
+layer, fluid-solid coupling and CPML of that configuration are not part of this path yet,
+so the whole mesh is elastic with Stacey edges.
+
+Prints one JSON line with the throughput, the roofline of the element-force kernel (HIP
+events on the library stream around every force launch inside the timed region) and a CPU
+baseline (the C oracle, one thread, on a bounded piece of the same workload).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def force_kernel_bytes(nspec: int, nglob: int) -> int:
+    """Algorithmic bytes of one element-force launch: per element 25 x (ibool + xix, xiz,
+    gammax, gammaz, kappa, mu) = 700 B; per unique point displ read (8 B) and accel
+    read-modify-write (16 B)."""
+    return 700 * nspec + 24 * nglob
+
+
+WORKLOADS = {
+    # name: (nx, nz, element size m, dt, receivers, f0)
+    "c2_mesh_elastic_2000x1000": (2000, 1000, 10.0, 1.0e-3, 200, 8.0),
+    "c3_elastic_800x400": (800, 400, 10.0, 1.0e-3, 200, 10.0),
+    "c4_elastic_600x300": (600, 300, 10.0, 1.0e-3, 64, 10.0),
+}
+
+
+def make_config(name: str, nstep: int):
+    from specfem2d_amd import config
+    nx, nz, h, dt, nrec, f0 = WORKLOADS[name]
+    return config.random_elastic(nx, nz, h, nstep, dt, nrec, seed=2024, f0=f0, corr_len=200.0, amplitude=0.10,
+                                 vp0=3000.0, vs0=1732.0, rho0=2500.0, name=name)
+
+
+def cpu_baseline(name: str, target_seconds: float = 15.0):
+    """Oracle (C restatement of the reference CPU path, FP32, 1 thread) on a piece of the workload:
+    a 250 x 125-element mesh with the same element size, model statistics and boundaries."""
+    from oracle.oracle import OracleSim
+    from specfem2d_amd import config
+    _, _, h, dt, _, f0 = WORKLOADS[name]
+    nx, nz = 250, 125
+    nstep = 2000
+    cfg = config.random_elastic(nx, nz, h, nstep, dt, 20, seed=7, f0=f0, name="cpu_sample")
+    orc = OracleSim(cfg, "sp")
+    orc.forward(nstep=2)  # warm caches
+    steps = 0
+    t0 = time.perf_counter()
+    chunk = 20
+    while time.perf_counter() - t0 < target_seconds and steps + chunk <= nstep - 2:
+        orc.forward(nstep=chunk)
+        steps += chunk
+    dt_s = time.perf_counter() - t0
+    return {
+        "value": orc.nglob * steps / dt_s,
+        "unit": "GLL-pt updates/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"{nx}x{nz}-element piece of the workload ({orc.nglob} GLL points), {steps} Newmark steps, "
+                  f"FP32 oracle (oracle/sem2d_oracle.c, gcc -O2), {dt_s:.1f} s",
+    }
+
+
+def read_traffic(name: str):
+    """HBM bytes per force launch from committed rocprofv3 PMC summaries, if present."""
+    path = os.path.join(ROOT, "profiles", f"elastic_forces_{name}_traffic.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        with open(path) as f:
+            return float(json.load(f)["bytes_per_launch"])
+    except Exception:
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="c2_mesh_elastic_2000x1000", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; the multi-GPU layout (shots per GPU, "
+                                   "strip decomposition with halo exchange) is a design, not implemented",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import torch
+    if not torch.cuda.is_available():
+        print(json.dumps({"error": "no HIP device"}))
+        sys.exit(1)
+    from specfem2d_amd.driver import Simulation
+
+    cfg = make_config(args.workload, args.warmup + args.steps)
+    t_setup = time.perf_counter()
+    sim = Simulation(cfg)
+    t_setup = time.perf_counter() - t_setup
+    # warmup
+    sim.forward(nstep=args.warmup)
+    torch.cuda.synchronize()
+    # timed region
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    seis, timing = sim.forward(nstep=args.steps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    if not np.all(np.isfinite(seis)):
+        print(json.dumps({"error": "non-finite seismograms"}))
+        sys.exit(1)
+    nglob, nspec = sim.nglob, sim.nspec
+    value = nglob * args.steps / wall
+    fk_ms = timing["forces_ms"] / args.steps
+    fk_bytes = force_kernel_bytes(nspec, nglob)
+    achieved = fk_bytes / (fk_ms * 1e-3) / 1e9
+    traffic = read_traffic(args.workload)
+    out = {
+        "metric": "elastic GLL-pt updates/s",
+        "value": value,
+        "unit": "GLL-pt updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded Gaussian random-field elastic model)",
+        "config": {
+            "workload": args.workload,
+            "nspec": nspec,
+            "nglob": nglob,
+            "receivers": cfg.nrec,
+            "dt": cfg.dt,
+            "boundaries": "Stacey bottom/left/right, free surface top",
+            "parallelism": "1 GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_elastic_forces",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "algorithmic_bytes_per_launch": fk_bytes,
+            "avg_launch_ms": fk_ms,
+        },
+        "gpu_loop_ms_events": timing["loop_ms"],
+        "setup_s": t_setup,
+    }
+    sim.close()
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(args.workload, args.cpu_seconds)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

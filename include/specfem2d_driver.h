@@ -1,0 +1,94 @@
+/*
+ * specfem2d_driver.h -- host side above the C-ABI of specfem2d_gpu.h.
+ *
+ * The driver plays the role of the reference's xspecfem2D on one GPU for
+ * structured internal meshes: it builds what the Fortran setup builds
+ * (src/specfem2D/setup_mesh.F90, createnum_fast.f90 + get_global.f90,
+ * define_derivation_matrices.f90, invert_mass_matrix.F90,
+ * prepare_timerun.F90, locate_source.F90, locate_receivers.F90,
+ * setup_sources_receivers.F90, prepare_source_time_function.f90), hands it to
+ * the GPU library through prepare_constants_device_ & co. exactly as
+ * src/specfem2D/prepare_gpu.f90:80-347 does, and then replays the GPU branch
+ * of the time loop (src/specfem2D/iterate_time.F90:146-414,
+ * compute_gpu_elastic.f90:36-311).  It never computes physics itself.
+ */
+#ifndef SPECFEM2D_DRIVER_H
+#define SPECFEM2D_DRIVER_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SF2D_MAXLAYERS 32
+
+/* Par_file subset for the internal mesher and the elastic solver */
+typedef struct {
+  int nx, nz;                          /* elements along x and z */
+  int nlayers;                         /* flat layers, bottom first */
+  int nz_layer[SF2D_MAXLAYERS];
+  double xmin, xmax;
+  double zinterface[SF2D_MAXLAYERS + 1];
+  int absorbbottom, absorbright, absorbtop, absorbleft;
+  int stacey;                          /* STACEY_ABSORBING_CONDITIONS */
+  int nstep;                           /* NSTEP */
+  double dt;                           /* DT */
+  double user_t0;                      /* USER_T0 */
+  double anglerec;                     /* degrees */
+  int material_mode;                   /* 0: material table (rho,vp,vs per point), 1: MODEL=gll */
+} sf2d_params;
+
+/* DATA/SOURCE subset: collocated force */
+typedef struct {
+  double x, z, f0, factor, anglesource, tshift;
+  int time_function_type;              /* 1 Ricker, 2 first derivative of a Gaussian */
+  int pad;
+} sf2d_source;
+
+/* Builds the host-side setup of a simulation (no device call).  rho/vp/vs are
+   per GLL point [nspec][25] (i fastest).  simulation_type 1 (forward) or 3 (adjoint);
+   save_forward stores the Stacey traction per step for a later adjoint run.
+   Returns NULL on invalid input (message on stderr). */
+void* sf2d_sim_create(const sf2d_params* p, const double* rho, const double* vp, const double* vs, int nsrc,
+                      const sf2d_source* src, int nrec, const double* rec_xz, int simulation_type,
+                      int save_forward);
+/* Hands the setup to the GPU library in the order of prepare_gpu.f90:80-347. Returns 0 on success. */
+int sf2d_sim_prepare_device(void* sim);
+void sf2d_sim_destroy(void* sim);
+
+int sf2d_sim_nspec(void* sim);
+int sf2d_sim_nglob(void* sim);
+int sf2d_sim_nabs(void* sim);
+double sf2d_sim_t0(void* sim);
+/* the opaque Fortran-style handle (value of Mesh_pointer) */
+long sf2d_sim_mesh_pointer(void* sim);
+
+/* setup products, for parity checks */
+void sf2d_sim_get_ibool(void* sim, int* out);                   /* [nspec][25], 1-based */
+void sf2d_sim_get_rmass_inverse(void* sim, float* out);         /* [nglob][2] */
+void sf2d_sim_get_elem_array(void* sim, int which, float* out); /* 0 xix 1 xiz 2 gammax 3 gammaz 4 jacobian
+                                                                   5 kappa 6 mu 7 rho_vp 8 rho_vs 9 rho */
+void sf2d_sim_get_sources(void* sim, int* ispec, double* xi, double* gamma, float* sourcearrays, float* stf);
+void sf2d_sim_get_receivers(void* sim, int* ispec, double* xi, double* gamma, float* xir, float* gammar);
+void sf2d_sim_get_stacey(void* sim, int* ispec, int* ij, float* normal, float* jacw, int* edge, int* ib);
+
+/* Forward time loop (SIMULATION_TYPE=1): runs the next nstep_run steps (continuing after the steps
+   already run, up to NSTEP), displacement seismograms of those steps into seis[nrec][2][nstep_run].
+   With save_forward the Stacey traction per step is copied to absorb[nstep_run][nabs][5][2] (may be
+   NULL).  If timing is not NULL, HIP events on the library stream give {loop ms, predictor ms,
+   forces ms, boundary+sources ms, corrector ms, seismogram ms}: the loop and the element-force
+   launches always, the other phases only when timing[0] < 0 on entry.  Returns 0 on success. */
+int sf2d_sim_forward(void* sim, int nstep_run, double* seis, float* absorb, double* timing);
+/* zero the wavefields and restart the step counter */
+void sf2d_sim_reset(void* sim);
+int sf2d_sim_steps_done(void* sim);
+
+/* device fields (interleaved x,z; 2*nglob floats); NULL pointers are skipped */
+void sf2d_sim_get_fields(void* sim, float* displ, float* veloc, float* accel);
+void sf2d_sim_set_fields(void* sim, const float* displ, const float* veloc, const float* accel);
+void sf2d_sim_get_b_fields(void* sim, float* displ, float* veloc, float* accel);
+void sf2d_sim_set_b_fields(void* sim, const float* displ, const float* veloc, const float* accel);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -890,7 +890,7 @@ void orc_forward(void *h, int nstep_run, double *seis, double *stacey_store_d) {
   size_t n2 = (size_t)2 * s->nglob;
   REAL *d = s->displ, *v = s->veloc, *a = s->accel;
   REAL *store = NULL;
-  if (stacey_store_d && s->nabs > 0) store = (REAL *)malloc(sizeof(REAL) * (size_t)nstep_run * s->nabs * NGLL * 2);
+  if (stacey_store_d && s->nabs > 0) store = (REAL *)calloc((size_t)nstep_run * s->nabs * NGLL * 2, sizeof(REAL));
   for (int it = 1; it <= nstep_run; it++) {
     /* update_displacement_newmark_elastic (update_displacement_Newmark.F90:408-454) */
     for (size_t k = 0; k < n2; k++) {

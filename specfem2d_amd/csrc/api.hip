@@ -1,0 +1,694 @@
+// Fortran-callable entry points of the elastic GPU path (the reference's src/gpu/*.cu host layer).
+//
+// Argument lists and meaning follow the reference's declarations (src/gpu/specfem2D_gpu_cuda_method_stubs.c);
+// each function cites the reference definition it replaces.  Host arrays are copied at prepare time; the
+// device layout is this library's own (unpadded [nspec][25], 0-based indices).
+#include <sys/stat.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "sf2d_mesh.h"
+#include "specfem2d_gpu.h"
+#include "specfem2d_gpu_ext.h"
+
+namespace sf2d {
+
+[[noreturn]] void fail(const char* fmt, ...) {
+  char msg[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(msg, sizeof(msg), fmt, ap);
+  va_end(ap);
+  fprintf(stderr, "\nspecfem2d_gpu error: %s\n", msg);
+  fflush(stderr);
+  struct stat st;
+  if (stat("OUTPUT_FILES", &st) == 0 && S_ISDIR(st.st_mode)) {
+    FILE* f = fopen("OUTPUT_FILES/error_message_000000.txt", "a");
+    if (f) {
+      fprintf(f, "specfem2d_gpu error: %s\n", msg);
+      fclose(f);
+    }
+  }
+  exit(EXIT_FAILURE);
+}
+
+void check_hip(hipError_t err, const char* what) {
+  if (err != hipSuccess) fail("%s: %s", what, hipGetErrorString(err));
+}
+
+namespace {
+
+std::vector<int> to_zero_based(const int* a, size_t n, int lo, int hi, const char* what) {
+  std::vector<int> out(n);
+  for (size_t k = 0; k < n; k++) {
+    int v = a[k];
+    if (v < lo || v > hi) fail("%s: index %d at position %zu outside [%d, %d]", what, v, k, lo, hi);
+    out[k] = v - 1;
+  }
+  return out;
+}
+
+int identity_prefix(const std::vector<int>& v) {
+  int n = 0;
+  while (n < (int)v.size() && v[n] == n) n++;
+  return n;
+}
+
+}  // namespace
+}  // namespace sf2d
+
+using namespace sf2d;
+
+extern "C" {
+
+// ---------------------------------------------------------------------------
+void initialize_cuda_device_(int* myrank_f, int* ncuda_devices) {  // initialize_cuda.cu:50
+  int count = 0;
+  check_hip(hipGetDeviceCount(&count), "hipGetDeviceCount");
+  if (count <= 0) fail("no HIP device found");
+  *ncuda_devices = count;
+  int device = *myrank_f % count;
+  check_hip(hipSetDevice(device), "hipSetDevice");
+  int got = -1;
+  check_hip(hipGetDevice(&got), "hipGetDevice");
+  if (got != device) fail("device id conflict: asked %d got %d", device, got);
+}
+
+void initialize_cuda_aware_mpi_(void) {}  // initialize_cuda.cu:264 (nothing to set up on one GPU)
+
+// ---------------------------------------------------------------------------
+void prepare_constants_device_(long* Mesh_pointer, int* h_NGLLX, int* NSPEC_AB, int* NGLOB_AB, realw* h_xix,
+                               realw* h_xiz, realw* h_gammax, realw* h_gammaz, realw* h_kappav, realw* h_muv,
+                               int* h_ibool, int* num_interfaces_ext_mesh, int* max_nibool_interfaces_ext_mesh,
+                               int* h_nibool_interfaces_ext_mesh, int* h_ibool_interfaces_ext_mesh,
+                               realw* h_hprime_xx, realw* h_hprimewgll_xx, realw* h_wxgll,
+                               int* STACEY_BOUNDARY_CONDITIONS, int* PML_BOUNDARY_CONDITIONS, int* h_ispec_is_inner,
+                               int* nsources_local_f, realw* h_sourcearrays, realw* h_source_time_function,
+                               int* NSTEP, int* h_ispec_selected_source, int* h_ispec_selected_rec_loc,
+                               int* nrec_local, realw* h_cosrot, realw* h_sinrot, int* SIMULATION_TYPE, int* P_SV,
+                               int* nspec_acoustic, int* nspec_elastic, int* ispec_is_acoustic,
+                               int* ispec_is_elastic, int* h_myrank, int* SAVE_FORWARD, realw* h_xir_store,
+                               realw* h_gammar_store, int* h_NSIGTYPE, int* h_seismotypeVec,
+                               int* nlength_seismogram) {  // prepare_mesh_constants_cuda.cu:83
+  (void)max_nibool_interfaces_ext_mesh;
+  (void)h_nibool_interfaces_ext_mesh;
+  (void)h_ibool_interfaces_ext_mesh;
+  (void)h_ispec_is_inner;
+  (void)ispec_is_acoustic;
+  if (*h_NGLLX != NGLL) fail("NGLLX = %d, this library is built for NGLLX = %d", *h_NGLLX, NGLL);
+  if (*num_interfaces_ext_mesh > 0)
+    fail("MPI interfaces (%d) passed: this library drives one GPU with one mesh partition", *num_interfaces_ext_mesh);
+  if (!*P_SV) fail("SH (membrane) waves are not supported by this library; use P_SV = .true.");
+  if (*PML_BOUNDARY_CONDITIONS) fail("PML boundary conditions are not supported by this library yet");
+
+  Mesh* mp = new Mesh();
+  *Mesh_pointer = (long)mp;
+  check_hip(hipGetDevice(&mp->device), "hipGetDevice");
+  check_hip(hipStreamCreate(&mp->stream), "hipStreamCreate");
+  mp->myrank = *h_myrank;
+  mp->nspec = *NSPEC_AB;
+  mp->nglob = *NGLOB_AB;
+  mp->simulation_type = *SIMULATION_TYPE;
+  mp->save_forward = *SAVE_FORWARD;
+  mp->p_sv = *P_SV;
+  mp->stacey = *STACEY_BOUNDARY_CONDITIONS;
+  mp->pml = *PML_BOUNDARY_CONDITIONS;
+  mp->nstep = *NSTEP;
+  if (mp->nspec <= 0 || mp->nglob <= 0) fail("empty mesh (nspec=%d, nglob=%d)", mp->nspec, mp->nglob);
+
+  // basis: Fortran hprime_xx(i,k) = h'_k(xi_i) at h[(k-1)*5+(i-1)], hprimewgll_xx(k,i) at h[(i-1)*5+(k-1)]
+  for (int i = 0; i < NGLL; i++) {
+    for (int k = 0; k < NGLL; k++) {
+      mp->basis.hp[i * NGLL + k] = h_hprime_xx[k * NGLL + i];
+      mp->basis.hpw[k * NGLL + i] = h_hprimewgll_xx[i * NGLL + k];
+    }
+    mp->basis.w[i] = h_wxgll[i];
+  }
+
+  const size_t ne = (size_t)mp->nspec * NGLL2;
+  mp->d_xix = device_copy(h_xix, ne);
+  mp->d_xiz = device_copy(h_xiz, ne);
+  mp->d_gammax = device_copy(h_gammax, ne);
+  mp->d_gammaz = device_copy(h_gammaz, ne);
+  mp->d_kappav = device_copy(h_kappav, ne);
+  mp->d_muv = device_copy(h_muv, ne);
+  {
+    std::vector<int> ib = to_zero_based(h_ibool, ne, 1, mp->nglob, "ibool");
+    mp->d_ibool = device_copy(ib.data(), ne);
+  }
+  std::vector<int> elastic(mp->nspec);
+  for (int e = 0; e < mp->nspec; e++) elastic[e] = ispec_is_elastic[e] ? 1 : 0;
+  mp->d_ispec_is_elastic = device_copy(elastic.data(), (size_t)mp->nspec);
+  mp->nspec_elastic = *nspec_elastic;
+  mp->nspec_acoustic = *nspec_acoustic;
+
+  // sources
+  mp->nsources_local = *nsources_local_f;
+  if (mp->nsources_local > 0) {
+    mp->d_sourcearrays = device_copy(h_sourcearrays, (size_t)mp->nsources_local * NGLL2 * 2);
+    mp->d_stf = device_copy(h_source_time_function, (size_t)mp->nsources_local * mp->nstep);
+    std::vector<int> is = to_zero_based(h_ispec_selected_source, mp->nsources_local, 1, mp->nspec, "ispec_selected_source");
+    mp->d_ispec_selected_source = device_copy(is.data(), is.size());
+  }
+
+  // receivers and seismogram buffers
+  mp->nrec_local = *nrec_local;
+  mp->nsigtype = *h_NSIGTYPE;
+  mp->h_seismotype = h_seismotypeVec;
+  mp->nlength_seismogram = *nlength_seismogram;
+  if (mp->nrec_local > 0) {
+    std::vector<int> ir = to_zero_based(h_ispec_selected_rec_loc, mp->nrec_local, 1, mp->nspec, "ispec_selected_rec_loc");
+    mp->d_ispec_selected_rec = device_copy(ir.data(), ir.size());
+    mp->d_cosrot = device_copy(h_cosrot, (size_t)mp->nrec_local);
+    mp->d_sinrot = device_copy(h_sinrot, (size_t)mp->nrec_local);
+    mp->d_xir = device_copy(h_xir_store, (size_t)mp->nrec_local * NGLL);
+    mp->d_gammar = device_copy(h_gammar_store, (size_t)mp->nrec_local * NGLL);
+    mp->d_seismo = new float*[mp->nsigtype]();
+    mp->h_seismo = new float*[mp->nsigtype]();
+    const size_t n = (size_t)2 * mp->nrec_local * mp->nlength_seismogram;
+    for (int s = 0; s < mp->nsigtype; s++) {
+      if (h_seismotypeVec[s] == 0) continue;
+      mp->d_seismo[s] = device_zeros<float>(n);
+      check_hip(hipHostMalloc((void**)&mp->h_seismo[s], n * sizeof(float), hipHostMallocDefault), "hipHostMalloc");
+    }
+  }
+  mp->d_scratch_u32 = device_zeros<unsigned int>(64);
+  check_hip(hipDeviceSynchronize(), "prepare_constants_device");
+}
+
+void prepare_fields_elastic_device_(long* Mesh_pointer, realw* rmassx, realw* rmassz, int* num_phase_ispec_elastic,
+                                    int* phase_ispec_inner_elastic, int* ispec_is_anisotropic, int* ANISOTROPY,
+                                    realw* c11store, realw* c12store, realw* c13store, realw* c15store,
+                                    realw* c23store, realw* c25store, realw* c33store, realw* c35store,
+                                    realw* c55store, int* h_ninterface_elastic, int* h_inum_interfaces_elastic,
+                                    int* ATTENUATION_VISCOELASTIC, realw* h_A_newmark_mu, realw* h_B_newmark_mu,
+                                    realw* h_A_newmark_kappa,
+                                    realw* h_B_newmark_kappa) {  // prepare_mesh_constants_cuda.cu:527
+  (void)ispec_is_anisotropic; (void)c11store; (void)c12store; (void)c13store; (void)c15store; (void)c23store;
+  (void)c25store; (void)c33store; (void)c35store; (void)c55store; (void)h_inum_interfaces_elastic;
+  (void)h_A_newmark_mu; (void)h_B_newmark_mu; (void)h_A_newmark_kappa; (void)h_B_newmark_kappa;
+  Mesh* mp = handle(Mesh_pointer);
+  if (*ANISOTROPY) fail("anisotropic elements are not supported by this library yet");
+  if (*ATTENUATION_VISCOELASTIC) fail("viscoelastic attenuation is not supported by this library yet");
+  if (*h_ninterface_elastic > 0) fail("MPI elastic interfaces passed: one GPU, one partition only");
+  const size_t n2 = (size_t)2 * mp->nglob;
+  mp->d_displ = device_zeros<float>(n2);
+  mp->d_veloc = device_zeros<float>(n2);
+  mp->d_accel = device_zeros<float>(n2);
+  mp->d_rmassx = device_copy(rmassx, (size_t)mp->nglob);
+  mp->d_rmassz = device_copy(rmassz, (size_t)mp->nglob);
+  const int np = *num_phase_ispec_elastic;
+  mp->num_phase_ispec_elastic = np;
+  for (int ph = 0; ph < 2; ph++) {
+    std::vector<int> lst = to_zero_based(phase_ispec_inner_elastic + (size_t)ph * np, (size_t)np, 0, mp->nspec,
+                                         "phase_ispec_inner_elastic");
+    // entries past the used prefix may be zero (unused); keep them in range
+    for (auto& v : lst) v = v < 0 ? 0 : v;
+    mp->phase_is_identity[ph] = identity_prefix(lst);
+    mp->d_phase_ispec[ph] = device_copy(lst.data(), lst.size());
+  }
+  check_hip(hipDeviceSynchronize(), "prepare_fields_elastic_device");
+}
+
+void prepare_fields_elastic_adj_dev_(long* Mesh_pointer, int* size_f, int* APPROXIMATE_HESS_KL,
+                                     int* ATTENUATION_VISCOELASTIC,
+                                     int* NO_BACKWARD_RECONSTRUCTION) {  // prepare_mesh_constants_cuda.cu:711
+  Mesh* mp = handle(Mesh_pointer);
+  if (*APPROXIMATE_HESS_KL) fail("approximate Hessian kernels are not supported on the GPU (as in the reference)");
+  if (*ATTENUATION_VISCOELASTIC) fail("viscoelastic attenuation is not supported by this library yet");
+  if (*NO_BACKWARD_RECONSTRUCTION) fail("NO_BACKWARD_RECONSTRUCTION is not supported by this library");
+  if (*size_f != 2 * mp->nglob) fail("prepare_fields_elastic_adj_dev: size %d != 2*nglob", *size_f);
+  const size_t n2 = (size_t)2 * mp->nglob;
+  mp->d_b_displ = device_zeros<float>(n2);
+  mp->d_b_veloc = device_zeros<float>(n2);
+  mp->d_b_accel = device_zeros<float>(n2);
+  const size_t ne = (size_t)mp->nspec * NGLL2;
+  mp->d_rho_kl = device_zeros<float>(ne);
+  mp->d_mu_kl = device_zeros<float>(ne);
+  mp->d_kappa_kl = device_zeros<float>(ne);
+}
+
+void prepare_stacey_device_(long* Mesh_pointer, int* ACOUSTIC_SIMULATION, int* ELASTIC_SIMULATION, realw* rho_vp,
+                            realw* rho_vs, int* h_nspec_bottom, int* h_nspec_left, int* h_nspec_right,
+                            int* h_nspec_top, int* h_abs_boundary_ispec, int* h_abs_boundary_ij,
+                            realw* h_abs_boundary_normal, realw* h_abs_boundary_jacobian1Dw,
+                            int* h_num_abs_boundary_faces, int* h_edge_abs, int* h_ib_bottom, int* h_ib_left,
+                            int* h_ib_right, int* h_ib_top) {  // prepare_mesh_constants_cuda.cu:926
+  Mesh* mp = handle(Mesh_pointer);
+  if (!mp->stacey) fail("Stacey flag inconsistent with prepare_stacey_device call");
+  if (*ACOUSTIC_SIMULATION) fail("acoustic Stacey boundaries are not supported by this library yet");
+  const int nf = *h_num_abs_boundary_faces;
+  mp->num_abs = nf;
+  mp->nspec_bottom = *h_nspec_bottom;
+  mp->nspec_left = *h_nspec_left;
+  mp->nspec_right = *h_nspec_right;
+  mp->nspec_top = *h_nspec_top;
+  if (nf <= 0) return;
+  std::vector<int> isp = to_zero_based(h_abs_boundary_ispec, nf, 1, mp->nspec, "abs_boundary_ispec");
+  std::vector<int> ij((size_t)nf * NGLL * 2);
+  for (size_t k = 0; k < ij.size(); k++) {
+    int v = h_abs_boundary_ij[k];
+    if (v < 1 || v > NGLL + 1) fail("abs_boundary_ij: value %d out of range", v);
+    ij[k] = (v == NGLL + 1) ? -1 : v - 1;  // NGLL+1 marks a point that the edge does not own
+  }
+  // after marking, a skipped point has -1 in one coordinate: make both negative
+  for (int k = 0; k < nf * NGLL; k++)
+    if (ij[2 * k] < 0 || ij[2 * k + 1] < 0) ij[2 * k] = ij[2 * k + 1] = -1;
+  std::vector<int> ib(nf);
+  const int counts[5] = {0, mp->nspec_bottom, mp->nspec_right, mp->nspec_top, mp->nspec_left};
+  for (int f = 0; f < nf; f++) {
+    int side = h_edge_abs[f];
+    int v = side == 1 ? h_ib_bottom[f] : side == 2 ? h_ib_right[f] : side == 3 ? h_ib_top[f] : side == 4 ? h_ib_left[f] : -1;
+    if (side < 1 || side > 4) fail("edge_abs(%d) = %d is not 1..4", f + 1, side);
+    if (v < 1 || v > counts[side]) fail("ib index %d of face %d outside 1..%d", v, f + 1, counts[side]);
+    ib[f] = v - 1;
+  }
+  mp->d_abs_ispec = device_copy(isp.data(), isp.size());
+  mp->d_abs_ij = device_copy(ij.data(), ij.size());
+  mp->d_abs_normal = device_copy(h_abs_boundary_normal, (size_t)nf * NGLL * 2);
+  mp->d_abs_jacw = device_copy(h_abs_boundary_jacobian1Dw, (size_t)nf * NGLL);
+  mp->d_edge_abs = device_copy(h_edge_abs, (size_t)nf);
+  mp->d_ib = device_copy(ib.data(), ib.size());
+  if (*ELASTIC_SIMULATION) {
+    mp->d_rho_vp = device_copy(rho_vp, (size_t)mp->nspec * NGLL2);
+    mp->d_rho_vs = device_copy(rho_vs, (size_t)mp->nspec * NGLL2);
+    if (mp->simulation_type == 3 || (mp->simulation_type == 1 && mp->save_forward)) {
+      for (int side = 1; side <= 4; side++)
+        if (counts[side] > 0) mp->d_b_absorb[side] = device_zeros<float>((size_t)counts[side] * NGLL * 2);
+    }
+  }
+}
+
+void prepare_sim2_or_3_const_device_(long* Mesh_pointer, int* nadj_rec_local, realw* h_source_adjoint,
+                                     int* NSTEP) {  // prepare_mesh_constants_cuda.cu:815
+  Mesh* mp = handle(Mesh_pointer);
+  mp->nadj_rec_local = *nadj_rec_local;
+  if (mp->nadj_rec_local > 0) {
+    if (mp->nadj_rec_local != mp->nrec_local) fail("nadj_rec_local %d != nrec_local %d", mp->nadj_rec_local, mp->nrec_local);
+    mp->d_source_adjoint = device_copy(h_source_adjoint, (size_t)2 * (*NSTEP) * mp->nadj_rec_local);
+  }
+}
+
+void prepare_cleanup_device_(long* Mesh_pointer, int* ACOUSTIC_SIMULATION, int* ELASTIC_SIMULATION, int* ANISOTROPY,
+                             int* APPROXIMATE_HESS_KL, int* ATTENUATION_VISCOACOUSTIC,
+                             int* ATTENUATION_VISCOELASTIC, int* NO_BACKWARD_RECONSTRUCTION,
+                             realw* h_no_backward_acoustic_buffer) {  // prepare_mesh_constants_cuda.cu:1139
+  (void)ACOUSTIC_SIMULATION; (void)ELASTIC_SIMULATION; (void)ANISOTROPY; (void)APPROXIMATE_HESS_KL;
+  (void)ATTENUATION_VISCOACOUSTIC; (void)ATTENUATION_VISCOELASTIC; (void)NO_BACKWARD_RECONSTRUCTION;
+  (void)h_no_backward_acoustic_buffer;
+  Mesh* mp = handle(Mesh_pointer);
+  (void)hipStreamSynchronize(mp->stream);
+  device_free(mp->d_ibool); device_free(mp->d_xix); device_free(mp->d_xiz); device_free(mp->d_gammax);
+  device_free(mp->d_gammaz); device_free(mp->d_kappav); device_free(mp->d_muv); device_free(mp->d_ispec_is_elastic);
+  device_free(mp->d_displ); device_free(mp->d_veloc); device_free(mp->d_accel);
+  device_free(mp->d_b_displ); device_free(mp->d_b_veloc); device_free(mp->d_b_accel);
+  device_free(mp->d_rmassx); device_free(mp->d_rmassz);
+  device_free(mp->d_phase_ispec[0]); device_free(mp->d_phase_ispec[1]);
+  device_free(mp->d_sourcearrays); device_free(mp->d_stf); device_free(mp->d_ispec_selected_source);
+  device_free(mp->d_xir); device_free(mp->d_gammar); device_free(mp->d_cosrot); device_free(mp->d_sinrot);
+  device_free(mp->d_ispec_selected_rec);
+  if (mp->d_seismo) {
+    for (int s = 0; s < mp->nsigtype; s++) {
+      device_free(mp->d_seismo[s]);
+      if (mp->h_seismo[s]) (void)hipHostFree(mp->h_seismo[s]);
+    }
+    delete[] mp->d_seismo;
+    delete[] mp->h_seismo;
+  }
+  device_free(mp->d_source_adjoint);
+  device_free(mp->d_abs_ispec); device_free(mp->d_abs_ij); device_free(mp->d_abs_normal); device_free(mp->d_abs_jacw);
+  device_free(mp->d_edge_abs); device_free(mp->d_ib); device_free(mp->d_rho_vp); device_free(mp->d_rho_vs);
+  for (int s = 0; s < 5; s++) device_free(mp->d_b_absorb[s]);
+  device_free(mp->d_rho_kl); device_free(mp->d_mu_kl); device_free(mp->d_kappa_kl);
+  device_free(mp->d_scratch_u32);
+  (void)hipStreamDestroy(mp->stream);
+  delete mp;
+  *Mesh_pointer = 0;
+}
+
+// ---------------------------------------------------------------------------
+// transfers (transfer_fields_cuda.cu)
+static void copy_field(float* dst_dev, const realw* src_host, int size, Mesh* mp, const char* what) {
+  if (size != 2 * mp->nglob) fail("%s: size %d != 2*nglob = %d", what, size, 2 * mp->nglob);
+  if (!dst_dev) fail("%s: field not allocated on the device", what);
+  check_hip(hipMemcpyAsync(dst_dev, src_host, sizeof(float) * size, hipMemcpyHostToDevice, mp->stream), what);
+  check_hip(hipStreamSynchronize(mp->stream), what);
+}
+static void fetch_field(realw* dst_host, const float* src_dev, int size, Mesh* mp, const char* what) {
+  if (size != 2 * mp->nglob) fail("%s: size %d != 2*nglob = %d", what, size, 2 * mp->nglob);
+  if (!src_dev) fail("%s: field not allocated on the device", what);
+  check_hip(hipMemcpyAsync(dst_host, src_dev, sizeof(float) * size, hipMemcpyDeviceToHost, mp->stream), what);
+  check_hip(hipStreamSynchronize(mp->stream), what);
+}
+
+void transfer_fields_el_to_device_(int* size, realw* displ, realw* veloc, realw* accel, long* Mesh_pointer) {
+  Mesh* mp = handle(Mesh_pointer);
+  copy_field(mp->d_displ, displ, *size, mp, "transfer_fields_el_to_device");
+  copy_field(mp->d_veloc, veloc, *size, mp, "transfer_fields_el_to_device");
+  copy_field(mp->d_accel, accel, *size, mp, "transfer_fields_el_to_device");
+}
+void transfer_fields_el_from_device_(int* size, realw* displ, realw* veloc, realw* accel, long* Mesh_pointer) {
+  Mesh* mp = handle(Mesh_pointer);
+  fetch_field(displ, mp->d_displ, *size, mp, "transfer_fields_el_from_device");
+  fetch_field(veloc, mp->d_veloc, *size, mp, "transfer_fields_el_from_device");
+  fetch_field(accel, mp->d_accel, *size, mp, "transfer_fields_el_from_device");
+}
+void transfer_b_fields_to_device_(int* size, realw* b_displ, realw* b_veloc, realw* b_accel, long* Mesh_pointer) {
+  Mesh* mp = handle(Mesh_pointer);
+  copy_field(mp->d_b_displ, b_displ, *size, mp, "transfer_b_fields_to_device");
+  copy_field(mp->d_b_veloc, b_veloc, *size, mp, "transfer_b_fields_to_device");
+  copy_field(mp->d_b_accel, b_accel, *size, mp, "transfer_b_fields_to_device");
+}
+void transfer_b_fields_from_device_(int* size, realw* b_displ, realw* b_veloc, realw* b_accel, long* Mesh_pointer) {
+  Mesh* mp = handle(Mesh_pointer);
+  fetch_field(b_displ, mp->d_b_displ, *size, mp, "transfer_b_fields_from_device");
+  fetch_field(b_veloc, mp->d_b_veloc, *size, mp, "transfer_b_fields_from_device");
+  fetch_field(b_accel, mp->d_b_accel, *size, mp, "transfer_b_fields_from_device");
+}
+void transfer_accel_to_device_(int* size, realw* accel, long* Mesh_pointer) {
+  Mesh* mp = handle(Mesh_pointer);
+  copy_field(mp->d_accel, accel, *size, mp, "transfer_accel_to_device");
+}
+void transfer_accel_from_device_(int* size, realw* accel, long* Mesh_pointer) {
+  Mesh* mp = handle(Mesh_pointer);
+  fetch_field(accel, mp->d_accel, *size, mp, "transfer_accel_from_device");
+}
+void transfer_b_accel_from_device_(int* size, realw* b_accel, long* Mesh_pointer) {
+  Mesh* mp = handle(Mesh_pointer);
+  fetch_field(b_accel, mp->d_b_accel, *size, mp, "transfer_b_accel_from_device");
+}
+void transfer_b_displ_from_device_(int* size, realw* displ, long* Mesh_pointer) {
+  Mesh* mp = handle(Mesh_pointer);
+  fetch_field(displ, mp->d_b_displ, *size, mp, "transfer_b_displ_from_device");
+}
+void transfer_displ_from_device_(int* size, realw* displ, long* Mesh_pointer) {
+  Mesh* mp = handle(Mesh_pointer);
+  fetch_field(displ, mp->d_displ, *size, mp, "transfer_displ_from_device");
+}
+void transfer_kernels_el_to_host_(long* Mesh_pointer, realw* h_rho_kl, realw* h_mu_kl, realw* h_kappa_kl,
+                                  int* NSPEC_AB) {  // transfer_fields_cuda.cu:194
+  Mesh* mp = handle(Mesh_pointer);
+  if (!mp->d_rho_kl) fail("transfer_kernels_el_to_host: kernels not allocated (SIMULATION_TYPE /= 3)");
+  const size_t n = (size_t)(*NSPEC_AB) * NGLL2;
+  if (*NSPEC_AB != mp->nspec) fail("transfer_kernels_el_to_host: NSPEC_AB mismatch");
+  check_hip(hipStreamSynchronize(mp->stream), "transfer_kernels_el_to_host");
+  check_hip(hipMemcpy(h_rho_kl, mp->d_rho_kl, n * sizeof(float), hipMemcpyDeviceToHost), "rho_kl D2H");
+  check_hip(hipMemcpy(h_mu_kl, mp->d_mu_kl, n * sizeof(float), hipMemcpyDeviceToHost), "mu_kl D2H");
+  check_hip(hipMemcpy(h_kappa_kl, mp->d_kappa_kl, n * sizeof(float), hipMemcpyDeviceToHost), "kappa_kl D2H");
+}
+void transfer_compute_kernel_answers_from_device_(long* Mesh_pointer, realw* rho_kl, int* size_rho, realw* mu_kl,
+                                                  int* size_mu, realw* kappa_kl,
+                                                  int* size_kappa) {  // transfer_fields_cuda.cu:598
+  Mesh* mp = handle(Mesh_pointer);
+  if (!mp->d_rho_kl) fail("transfer_compute_kernel_answers_from_device: kernels not allocated");
+  check_hip(hipStreamSynchronize(mp->stream), "transfer_compute_kernel_answers_from_device");
+  check_hip(hipMemcpy(rho_kl, mp->d_rho_kl, (size_t)*size_rho * sizeof(float), hipMemcpyDeviceToHost), "D2H");
+  check_hip(hipMemcpy(mu_kl, mp->d_mu_kl, (size_t)*size_mu * sizeof(float), hipMemcpyDeviceToHost), "D2H");
+  check_hip(hipMemcpy(kappa_kl, mp->d_kappa_kl, (size_t)*size_kappa * sizeof(float), hipMemcpyDeviceToHost), "D2H");
+}
+void transfer_compute_kernel_fields_from_device_(
+    long* Mesh_pointer, realw* accel, int* size_accel, realw* b_displ, int* size_b_displ, realw* epsilondev_xx,
+    realw* epsilondev_yy, realw* epsilondev_xy, realw* epsilondev_xz, realw* epsilondev_yz, int* size_epsilondev,
+    realw* b_epsilondev_xx, realw* b_epsilondev_yy, realw* b_epsilondev_xy, realw* b_epsilondev_xz,
+    realw* b_epsilondev_yz, int* size_b_epsilondev, realw* rho_kl, int* size_rho, realw* mu_kl, int* size_mu,
+    realw* kappa_kl, int* size_kappa, realw* epsilon_trace_over_3, realw* b_epsilon_trace_over_3,
+    int* size_epsilon_trace_over_3) {  // transfer_fields_cuda.cu:617 (3D leftover; strain arrays are not kept)
+  (void)epsilondev_xx; (void)epsilondev_yy; (void)epsilondev_xy; (void)epsilondev_xz; (void)epsilondev_yz;
+  (void)size_epsilondev; (void)b_epsilondev_xx; (void)b_epsilondev_yy; (void)b_epsilondev_xy;
+  (void)b_epsilondev_xz; (void)b_epsilondev_yz; (void)size_b_epsilondev; (void)epsilon_trace_over_3;
+  (void)b_epsilon_trace_over_3; (void)size_epsilon_trace_over_3;
+  Mesh* mp = handle(Mesh_pointer);
+  fetch_field(accel, mp->d_accel, *size_accel, mp, "transfer_compute_kernel_fields_from_device");
+  fetch_field(b_displ, mp->d_b_displ, *size_b_displ, mp, "transfer_compute_kernel_fields_from_device");
+  transfer_compute_kernel_answers_from_device_(Mesh_pointer, rho_kl, size_rho, mu_kl, size_mu, kappa_kl, size_kappa);
+}
+
+// ---------------------------------------------------------------------------
+// time step
+void update_displacement_cuda_(long* Mesh_pointer, realw* deltat_F, realw* deltatsqover2_F, realw* deltatover2_F,
+                               realw* b_deltat_F, realw* b_deltatsqover2_F,
+                               realw* b_deltatover2_F) {  // update_displacement_cuda.cu:52
+  Mesh* mp = handle(Mesh_pointer);
+  launch_newmark_predictor(mp->d_displ, mp->d_veloc, mp->d_accel, mp->nglob, *deltat_F, *deltatsqover2_F,
+                           *deltatover2_F, mp->stream);
+  if (mp->simulation_type == 3)
+    launch_newmark_predictor(mp->d_b_displ, mp->d_b_veloc, mp->d_b_accel, mp->nglob, *b_deltat_F, *b_deltatsqover2_F,
+                             *b_deltatover2_F, mp->stream);
+}
+
+void compute_forces_viscoelastic_cuda_(long* Mesh_pointer, int* iphase, realw* deltat, int* nspec_outer_elastic,
+                                       int* nspec_inner_elastic, int* ANISOTROPY, int* ATTENUATION_VISCOELASTIC,
+                                       int* compute_wavefield_1,
+                                       int* compute_wavefield_2) {  // compute_forces_viscoelastic_cuda.cu:327
+  (void)deltat;
+  Mesh* mp = handle(Mesh_pointer);
+  if (*ANISOTROPY) fail("anisotropic elements are not supported by this library yet");
+  if (*ATTENUATION_VISCOELASTIC) fail("viscoelastic attenuation is not supported by this library yet");
+  const int ph = (*iphase == 1) ? 0 : 1;
+  const int nelem = (*iphase == 1) ? *nspec_outer_elastic : *nspec_inner_elastic;
+  if (nelem <= 0) return;
+  if (nelem > mp->num_phase_ispec_elastic) fail("compute_forces: %d elements > phase list size %d", nelem, mp->num_phase_ispec_elastic);
+  const int* elist = (nelem <= mp->phase_is_identity[ph]) ? nullptr : mp->d_phase_ispec[ph];
+  if (*compute_wavefield_1) launch_elastic_forces(mp, elist, nelem, mp->d_displ, mp->d_accel);
+  if (*compute_wavefield_2) {
+    if (!mp->d_b_displ) fail("compute_forces: backward wavefield not allocated");
+    launch_elastic_forces(mp, elist, nelem, mp->d_b_displ, mp->d_b_accel);
+  }
+}
+
+void compute_stacey_viscoelastic_cuda_(long* Mesh_pointer, int* iphasef, realw* h_b_absorb_elastic_left,
+                                       realw* h_b_absorb_elastic_right, realw* h_b_absorb_elastic_top,
+                                       realw* h_b_absorb_elastic_bottom, int* compute_wavefield_1,
+                                       int* compute_wavefield_2,
+                                       int* UNDO_ATTENUATION_AND_OR_PML) {  // compute_stacey_viscoelastic_cuda.cu:40
+  Mesh* mp = handle(Mesh_pointer);
+  if (mp->num_abs == 0 || *iphasef != 1) return;
+  const int read_abs = (mp->simulation_type == 3 && !*UNDO_ATTENUATION_AND_OR_PML) ? 1 : 0;
+  const int write_abs = (mp->simulation_type == 1 && mp->save_forward && !*UNDO_ATTENUATION_AND_OR_PML) ? 1 : 0;
+  realw* host[5] = {nullptr, h_b_absorb_elastic_bottom, h_b_absorb_elastic_right, h_b_absorb_elastic_top,
+                    h_b_absorb_elastic_left};
+  const int counts[5] = {0, mp->nspec_bottom, mp->nspec_right, mp->nspec_top, mp->nspec_left};
+  if (read_abs) {
+    for (int s = 1; s <= 4; s++)
+      if (counts[s] > 0)
+        check_hip(hipMemcpyAsync(mp->d_b_absorb[s], host[s], sizeof(float) * 2 * NGLL * counts[s],
+                                 hipMemcpyHostToDevice, mp->stream), "b_absorb H2D");
+  }
+  if (*compute_wavefield_1) launch_stacey(mp, mp->d_veloc, mp->d_accel, write_abs);
+  if (*compute_wavefield_2) launch_stacey_backward(mp, mp->d_b_accel);
+  if (write_abs) {
+    for (int s = 1; s <= 4; s++)
+      if (counts[s] > 0)
+        check_hip(hipMemcpyAsync(host[s], mp->d_b_absorb[s], sizeof(float) * 2 * NGLL * counts[s],
+                                 hipMemcpyDeviceToHost, mp->stream), "b_absorb D2H");
+    check_hip(hipStreamSynchronize(mp->stream), "compute_stacey_viscoelastic_cuda");
+  } else if (read_abs) {
+    check_hip(hipStreamSynchronize(mp->stream), "compute_stacey_viscoelastic_cuda");  // host slice is reused
+  }
+}
+
+void compute_add_sources_el_cuda_(long* Mesh_pointer, int* iphasef,
+                                  int* itf) {  // compute_add_sources_viscoelastic_cuda.cu:46
+  Mesh* mp = handle(Mesh_pointer);
+  if (mp->nsources_local == 0 || *iphasef != 1) return;
+  if (*itf < 1 || *itf > mp->nstep) fail("compute_add_sources_el_cuda: it=%d outside 1..%d", *itf, mp->nstep);
+  launch_sources(mp, mp->d_accel, *itf - 1);
+}
+
+void compute_add_sources_el_s3_cuda_(long* Mesh_pointer, int* iphasef,
+                                     int* itf) {  // compute_add_sources_viscoelastic_cuda.cu:86
+  Mesh* mp = handle(Mesh_pointer);
+  if (mp->nsources_local == 0 || *iphasef != 1) return;
+  if (*itf < 1 || *itf > mp->nstep) fail("compute_add_sources_el_s3_cuda: it=%d outside 1..%d", *itf, mp->nstep);
+  if (!mp->d_b_accel) fail("compute_add_sources_el_s3_cuda: backward wavefield not allocated");
+  launch_sources(mp, mp->d_b_accel, *itf - 1);
+}
+
+void add_sources_el_sim_type_2_or_3_(long* Mesh_pointer, int* iphasef, int* itf, int* nadj_rec_local,
+                                     int* NSTEP) {  // compute_add_sources_viscoelastic_cuda.cu:131
+  Mesh* mp = handle(Mesh_pointer);
+  if (*nadj_rec_local != mp->nadj_rec_local) fail("add_sources_el_sim_type_2_or_3: nadj_rec_local not equal");
+  if (mp->nadj_rec_local == 0 || *iphasef != 1) return;
+  if (*NSTEP != mp->nstep) fail("add_sources_el_sim_type_2_or_3: NSTEP mismatch");
+  if (*itf < 1 || *itf > mp->nstep) fail("add_sources_el_sim_type_2_or_3: it=%d outside 1..%d", *itf, mp->nstep);
+  launch_adjoint_sources(mp, mp->d_accel, *itf - 1);
+}
+
+void kernel_3_a_cuda_(long* Mesh_pointer, realw* deltatover2_F, realw* b_deltatover2_F, int* compute_wavefield_1,
+                      int* compute_wavefield_2) {  // update_displacement_cuda.cu:221
+  Mesh* mp = handle(Mesh_pointer);
+  if (*compute_wavefield_1)
+    launch_newmark_corrector(mp->d_veloc, mp->d_accel, mp->d_rmassx, mp->d_rmassz, mp->nglob, *deltatover2_F, 1,
+                             mp->stream);
+  if (*compute_wavefield_2)
+    launch_newmark_corrector(mp->d_b_veloc, mp->d_b_accel, mp->d_rmassx, mp->d_rmassz, mp->nglob, *b_deltatover2_F, 1,
+                             mp->stream);
+}
+
+void kernel_3_b_cuda_(long* Mesh_pointer, realw* deltatover2_F,
+                      realw* b_deltatover2_F) {  // update_displacement_cuda.cu:273 (velocity-only corrector)
+  Mesh* mp = handle(Mesh_pointer);
+  launch_newmark_corrector(mp->d_veloc, mp->d_accel, nullptr, nullptr, mp->nglob, *deltatover2_F, 0, mp->stream);
+  if (mp->simulation_type == 3)
+    launch_newmark_corrector(mp->d_b_veloc, mp->d_b_accel, nullptr, nullptr, mp->nglob, *b_deltatover2_F, 0,
+                             mp->stream);
+}
+
+void compute_seismograms_cuda_(long* Mesh_pointer_f, int* i_sigf, double* sisux, double* sisuz,
+                               int* seismo_currentf, int* nlength_seismogramf, int* ELASTIC_SIMULATION,
+                               int* ACOUSTIC_SIMULATION, int* USE_TRICK_FOR_BETTER_PRESSURE,
+                               int* ATTENUATION_VISCOELASTIC, int* itf,
+                               int* it_endf) {  // write_seismograms_cuda.cu:41
+  (void)ELASTIC_SIMULATION; (void)ACOUSTIC_SIMULATION; (void)USE_TRICK_FOR_BETTER_PRESSURE;
+  (void)ATTENUATION_VISCOELASTIC;
+  Mesh* mp = handle(Mesh_pointer_f);
+  if (mp->nrec_local == 0) return;
+  const int i_sig = *i_sigf - 1;
+  const int cur = *seismo_currentf - 1;
+  const int nlength = *nlength_seismogramf;
+  if (i_sig < 0 || i_sig >= mp->nsigtype) fail("compute_seismograms_cuda: i_sig %d out of range", i_sig + 1);
+  if (nlength != mp->nlength_seismogram) fail("compute_seismograms_cuda: nlength_seismogram mismatch");
+  if (cur < 0 || cur >= nlength) fail("compute_seismograms_cuda: seismo_current %d outside 1..%d", cur + 1, nlength);
+  const int stype = mp->h_seismotype[i_sig];
+  const float* field = nullptr;
+  switch (stype) {
+    case 1: field = mp->d_displ; break;
+    case 2: field = mp->d_veloc; break;
+    case 3: field = mp->d_accel; break;
+    default: fail("seismotype %d (pressure/curl) is not supported by this library yet", stype);
+  }
+  float* d_dst = mp->d_seismo[i_sig];
+  launch_seismograms(mp, field, d_dst, cur);
+  if (cur == nlength - 1 || *itf == *it_endf) {
+    const size_t size = (size_t)mp->nrec_local * nlength;
+    float* h = mp->h_seismo[i_sig];
+    check_hip(hipMemcpyAsync(h, d_dst, sizeof(float) * 2 * size, hipMemcpyDeviceToHost, mp->stream), "seismo D2H");
+    check_hip(hipStreamSynchronize(mp->stream), "compute_seismograms_cuda");
+    for (size_t k = 0; k < size; k++) {
+      sisux[k] = (double)h[k];
+      sisuz[k] = (double)h[k + size];
+    }
+  }
+}
+
+void compute_kernels_elastic_cuda_(long* Mesh_pointer, realw* deltat) {  // compute_kernels_cuda.cu:47
+  (void)deltat;
+  Mesh* mp = handle(Mesh_pointer);
+  if (!mp->d_rho_kl || !mp->d_b_displ) fail("compute_kernels_elastic_cuda: adjoint arrays not prepared");
+  launch_kernels(mp);
+}
+
+// ---------------------------------------------------------------------------
+// housekeeping
+void get_norm_elastic_from_device_(realw* norm, long* Mesh_pointer,
+                                   const int* FORWARD_OR_ADJOINT) {  // check_fields_cuda.cu:358
+  Mesh* mp = handle(Mesh_pointer);
+  const float* f = (*FORWARD_OR_ADJOINT == 3) ? mp->d_b_displ : mp->d_displ;
+  if (!f) fail("get_norm_elastic_from_device: field not allocated");
+  check_hip(hipMemsetAsync(mp->d_scratch_u32, 0, sizeof(unsigned int), mp->stream), "hipMemsetAsync");
+  launch_max_norm(mp, f, mp->d_scratch_u32);
+  unsigned int bits = 0;
+  check_hip(hipMemcpyAsync(&bits, mp->d_scratch_u32, sizeof(bits), hipMemcpyDeviceToHost, mp->stream), "norm D2H");
+  check_hip(hipStreamSynchronize(mp->stream), "get_norm_elastic_from_device");
+  float m;
+  memcpy(&m, &bits, sizeof(m));
+  *norm = std::sqrt(m);
+}
+
+void get_free_device_memory_(realw* free_mb, realw* used_mb, realw* total_mb) {  // check_fields_cuda.cu:165
+  size_t fr = 0, tot = 0;
+  check_hip(hipMemGetInfo(&fr, &tot), "hipMemGetInfo");
+  *free_mb = (realw)((double)fr / 1024.0 / 1024.0);
+  *used_mb = (realw)((double)(tot - fr) / 1024.0 / 1024.0);
+  *total_mb = (realw)((double)tot / 1024.0 / 1024.0);
+}
+
+void output_free_device_memory_(int* myrank_f) {  // check_fields_cuda.cu:149
+  realw f, u, t;
+  get_free_device_memory_(&f, &u, &t);
+  printf("%d: GPU memory usage: used = %f MB, free = %f MB, total = %f MB\n", *myrank_f, u, f, t);
+}
+
+void get_max_accel_(int* itf, int* sizef, long* Mesh_pointer) {  // check_fields_cuda.cu:484
+  Mesh* mp = handle(Mesh_pointer);
+  std::vector<float> a((size_t)*sizef);
+  check_hip(hipStreamSynchronize(mp->stream), "get_max_accel");
+  check_hip(hipMemcpy(a.data(), mp->d_accel, a.size() * sizeof(float), hipMemcpyDeviceToHost), "accel D2H");
+  float m = 0.f;
+  for (float v : a) m = std::fmax(m, std::fabs(v));
+  printf("%d/%d: max=%e\n", *itf, *sizef, m);
+}
+
+static void print_max(const char* what, const float* v, int n, int id) {
+  float m = 0.f;
+  int at = 0;
+  for (int k = 0; k < n; k++)
+    if (std::fabs(v[k]) > m) {
+      m = std::fabs(v[k]);
+      at = k;
+    }
+  printf("%d: %s max = %e at %d\n", id, what, m, at);
+}
+static void print_device_max(Mesh* mp, const float* d, int n, int id, const char* what) {
+  std::vector<float> h((size_t)n);
+  check_hip(hipStreamSynchronize(mp->stream), what);
+  check_hip(hipMemcpy(h.data(), d, h.size() * sizeof(float), hipMemcpyDeviceToHost), what);
+  print_max(what, h.data(), n, id);
+}
+void check_max_norm_displ_gpu_(int* size, realw* displ, long* Mesh_pointer, int* announceID) {
+  (void)displ;
+  Mesh* mp = handle(Mesh_pointer);
+  print_device_max(mp, mp->d_displ, *size, *announceID, "displ (gpu)");
+}
+void check_max_norm_vector_(int* size, realw* vector1, int* announceID) { print_max("vector", vector1, *size, *announceID); }
+void check_max_norm_displ_(int* size, realw* displ, int* announceID) { print_max("displ", displ, *size, *announceID); }
+void check_max_norm_b_displ_gpu_(int* size, realw* b_displ, long* Mesh_pointer, int* announceID) {
+  (void)b_displ;
+  Mesh* mp = handle(Mesh_pointer);
+  print_device_max(mp, mp->d_b_displ, *size, *announceID, "b_displ (gpu)");
+}
+void check_max_norm_b_accel_gpu_(int* size, realw* b_accel, long* Mesh_pointer, int* announceID) {
+  (void)b_accel;
+  Mesh* mp = handle(Mesh_pointer);
+  print_device_max(mp, mp->d_b_accel, *size, *announceID, "b_accel (gpu)");
+}
+void check_max_norm_b_veloc_gpu_(int* size, realw* b_veloc, long* Mesh_pointer, int* announceID) {
+  (void)b_veloc;
+  Mesh* mp = handle(Mesh_pointer);
+  print_device_max(mp, mp->d_b_veloc, *size, *announceID, "b_veloc (gpu)");
+}
+void check_max_norm_b_displ_(int* size, realw* b_displ, int* announceID) { print_max("b_displ", b_displ, *size, *announceID); }
+void check_max_norm_b_accel_(int* size, realw* b_accel, int* announceID) { print_max("b_accel", b_accel, *size, *announceID); }
+void check_error_vectors_(int* sizef, realw* vector1, realw* vector2) {  // check_fields_cuda.cu:879
+  double maxerr = 0.0;
+  for (int k = 0; k < *sizef; k++) maxerr = std::fmax(maxerr, std::fabs((double)vector1[k] - (double)vector2[k]));
+  printf("maximum error between vectors = %e\n", maxerr);
+}
+void pause_for_debug_(void) {}  // helper_functions.cu:281
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// extensions without a reference counterpart (specfem2d_gpu_ext.h)
+extern "C" void* sf2d_mesh_stream(long* Mesh_pointer) { return (void*)handle(Mesh_pointer)->stream; }
+
+extern "C" void* sf2d_mesh_field(long* Mesh_pointer, int which) {
+  Mesh* mp = handle(Mesh_pointer);
+  switch (which) {
+    case 0: return mp->d_displ;
+    case 1: return mp->d_veloc;
+    case 2: return mp->d_accel;
+    case 3: return mp->d_b_displ;
+    case 4: return mp->d_b_veloc;
+    case 5: return mp->d_b_accel;
+    case 6: return mp->d_rho_kl;
+    case 7: return mp->d_mu_kl;
+    case 8: return mp->d_kappa_kl;
+    default: return nullptr;
+  }
+}

@@ -1,0 +1,838 @@
+// Host driver: builds the solver's setup for a structured internal mesh and replays the GPU branch of the
+// SPECFEM2D time loop through the Fortran-style C-ABI (specfem2d_gpu.h).  See specfem2d_driver.h.
+//
+// Structured meshes let several setup steps use direct formulas where the reference searches:
+//   * global numbering: GLL point (i,j) of element (ex,ez) sits on node (4ex+i, 4ez+j) of the global GLL lattice,
+//     then the first-touch renumbering of src/specfem2D/get_global.f90:34-72 gives the reference's ibool;
+//   * point location: the closest-GLL-point search of locate_receivers.F90:148-200 only visits the lattice
+//     neighbourhood of the target, with the reference's tie-break (first element, then j, then i).
+// Everything handed to the GPU has the reference's types, layouts and 1-based indices.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <limits>
+#include <vector>
+
+#include "specfem2d_driver.h"
+#include "specfem2d_gpu.h"
+#include "specfem2d_gpu_ext.h"
+
+namespace {
+
+constexpr int NGLL = 5;
+constexpr int NGLL2 = 25;
+constexpr double PI = 3.141592653589793;
+
+void hc(hipError_t e) {
+  if (e != hipSuccess) {
+    fprintf(stderr, "specfem2d driver: HIP error %s\n", hipGetErrorString(e));
+    exit(EXIT_FAILURE);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// GLL basis (src/specfem2D/define_derivation_matrices.f90:34-82, src/shared/gll_library.f90)
+double legendre(int n, double x) {
+  double p0 = 1.0, p1 = x;
+  if (n == 0) return p0;
+  for (int k = 1; k < n; k++) {
+    double p2 = ((2.0 * k + 1.0) * x * p1 - k * p0) / (k + 1.0);
+    p0 = p1;
+    p1 = p2;
+  }
+  return p1;
+}
+
+struct Basis {
+  double xi[NGLL];
+  float w[NGLL];
+  float hprime_f[NGLL * NGLL];     // Fortran hprime_xx(i,k) at [(k)*5+(i)] = h'_k(xi_i)
+  float hprimewgll_f[NGLL * NGLL]; // Fortran hprimewgll_xx(k,i) at [(i)*5+(k)] = w_k h'_i(xi_k)
+};
+
+Basis make_basis() {
+  Basis b;
+  const int N = NGLL - 1;
+  const double r = std::sqrt(3.0 / 7.0);
+  const double x[NGLL] = {-1.0, -r, 0.0, r, 1.0};
+  for (int i = 0; i < NGLL; i++) {
+    b.xi[i] = x[i];
+    double pn = legendre(N, x[i]);
+    b.w[i] = (float)(2.0 / (N * (N + 1.0) * pn * pn));
+  }
+  // GLL differentiation matrix D(i,k) = h'_k(xi_i): P_N(x_i)/(P_N(x_k)(x_i-x_k)) off the diagonal,
+  // -/+ N(N+1)/4 at the two end points, 0 elsewhere on the diagonal
+  for (int i = 0; i < NGLL; i++)
+    for (int k = 0; k < NGLL; k++) {
+      double d;
+      if (i == k)
+        d = (i == 0) ? -N * (N + 1.0) / 4.0 : (i == N ? N * (N + 1.0) / 4.0 : 0.0);
+      else
+        d = legendre(N, x[i]) / (legendre(N, x[k]) * (x[i] - x[k]));
+      b.hprime_f[k * NGLL + i] = (float)d;
+    }
+  for (int i = 0; i < NGLL; i++)
+    for (int k = 0; k < NGLL; k++) b.hprimewgll_f[i * NGLL + k] = b.w[k] * b.hprime_f[i * NGLL + k];
+  return b;
+}
+
+void lagrange(double xi, const double* xg, double* h) {
+  for (int d = 0; d < NGLL; d++) {
+    double num = 1.0, den = 1.0;
+    for (int i = 0; i < NGLL; i++)
+      if (i != d) {
+        num *= (xi - xg[i]);
+        den *= (xg[d] - xg[i]);
+      }
+    h[d] = num / den;
+  }
+}
+
+// ---------------------------------------------------------------------------
+struct Sim {
+  sf2d_params p{};
+  int nspec = 0, nglob = 0;
+  int simulation_type = 1, save_forward = 0;
+  Basis basis{};
+  // control nodes of the structured grid
+  std::vector<double> gx, gz;  // [(nz+1)][(nx+1)]
+  std::vector<int> ibool;      // 1-based [nspec][25]
+  std::vector<double> coord;   // [nglob][2]
+  std::vector<float> xix, xiz, gammax, gammaz, jacobian, kappa, mu, rho_vp, rho_vs, rho;
+  std::vector<float> rmass_inverse;  // [nglob][2]
+  std::vector<float> rmassx, rmassz;
+  // Stacey (prepare_timerun.F90:1270-1460)
+  int nabs = 0;
+  int nspec_side[5] = {0, 0, 0, 0, 0};
+  std::vector<int> abs_ispec, abs_ij, edge_abs, ib_bottom, ib_left, ib_right, ib_top, ib0;
+  std::vector<float> abs_normal, abs_jacw;
+  // sources
+  int nsrc = 0;
+  std::vector<sf2d_source> src;
+  std::vector<int> src_ispec;
+  std::vector<double> src_xi, src_gamma;
+  std::vector<float> sourcearrays, stf;  // [nsrc][5][5][2], [nstep][nsrc]
+  double t0 = 0.0;
+  // receivers
+  int nrec = 0;
+  std::vector<int> rec_ispec;
+  std::vector<double> rec_xi, rec_gamma;
+  std::vector<float> xir, gammar;  // [5][nrec] receiver fastest
+  std::vector<float> cosrot, sinrot;
+  int seismotype[1] = {1};
+  // phase lists
+  std::vector<int> phase_ispec;  // [2][nspec]
+  long mesh = 0;
+  int it_done = 0;                 // time steps already run
+  std::vector<double> sisux, sisuz;  // host seismograms (nlength = NSTEP), as the Fortran caller keeps them
+};
+
+// bilinear map of element e (src/shared/define_shape_functions.f90 NGNOD=4, recompute_jacobian.f90)
+void map_point(const Sim& s, int e, double xi, double ga, double* x, double* z, double* dxdxi, double* dzdxi,
+               double* dxdga, double* dzdga) {
+  const int nx = s.p.nx;
+  const int ex = e % nx, ez = e / nx;
+  const int n0 = ez * (nx + 1) + ex, n1 = n0 + 1, n2 = n0 + (nx + 1) + 1, n3 = n0 + (nx + 1);
+  const int nodes[4] = {n0, n1, n2, n3};
+  const double sp = xi + 1.0, sm = xi - 1.0, tp = ga + 1.0, tm = ga - 1.0;
+  const double sh[4] = {0.25 * sm * tm, -0.25 * sp * tm, 0.25 * sp * tp, -0.25 * sm * tp};
+  const double dxi[4] = {0.25 * tm, -0.25 * tm, 0.25 * tp, -0.25 * tp};
+  const double dga[4] = {0.25 * sm, -0.25 * sp, 0.25 * sp, -0.25 * sm};
+  double X = 0, Z = 0, a = 0, b = 0, c = 0, d = 0;
+  for (int k = 0; k < 4; k++) {
+    const double xe = s.gx[nodes[k]], ze = s.gz[nodes[k]];
+    X += sh[k] * xe;
+    Z += sh[k] * ze;
+    a += dxi[k] * xe;
+    b += dxi[k] * ze;
+    c += dga[k] * xe;
+    d += dga[k] * ze;
+  }
+  *x = X;
+  *z = Z;
+  *dxdxi = a;
+  *dzdxi = b;
+  *dxdga = c;
+  *dzdga = d;
+}
+
+bool build_mesh(Sim& s) {
+  const sf2d_params& p = s.p;
+  if (p.nx < 1 || p.nz < 1 || p.nlayers < 1 || p.nlayers > SF2D_MAXLAYERS) return false;
+  int nzsum = 0;
+  for (int l = 0; l < p.nlayers; l++) nzsum += p.nz_layer[l];
+  if (nzsum != p.nz) return false;
+  s.nspec = p.nx * p.nz;
+  s.gx.assign((size_t)(p.nx + 1) * (p.nz + 1), 0.0);
+  s.gz.assign(s.gx.size(), 0.0);
+  // node grid (src/meshfem2D/read_mesh_files.F90:169-338, flat interfaces)
+  int row0 = 0;
+  for (int l = 0; l < p.nlayers; l++) {
+    const double bot = p.zinterface[l], top = p.zinterface[l + 1];
+    for (int iz = 0; iz <= p.nz_layer[l]; iz++) {
+      const double g = (double)iz / (double)p.nz_layer[l];
+      for (int ix = 0; ix <= p.nx; ix++) {
+        const size_t n = (size_t)(row0 + iz) * (p.nx + 1) + ix;
+        s.gx[n] = p.xmin + (p.xmax - p.xmin) * (double)ix / (double)p.nx;
+        s.gz[n] = (1.0 - g) * bot + g * top;
+      }
+    }
+    row0 += p.nz_layer[l];
+  }
+  // numbering: lattice index, then first-touch renumbering in element order
+  const long lx = 4L * p.nx + 1, lz = 4L * p.nz + 1;
+  std::vector<int> mask((size_t)(lx * lz), 0);
+  s.ibool.resize((size_t)s.nspec * NGLL2);
+  int next = 0;
+  for (int e = 0; e < s.nspec; e++) {
+    const int ex = e % p.nx, ez = e / p.nx;
+    for (int j = 0; j < NGLL; j++)
+      for (int i = 0; i < NGLL; i++) {
+        const size_t L = (size_t)(4L * ez + j) * lx + (4L * ex + i);
+        if (mask[L] == 0) mask[L] = ++next;
+        s.ibool[(size_t)e * NGLL2 + j * NGLL + i] = mask[L];
+      }
+  }
+  s.nglob = next;
+  // geometry at the GLL points (setup_mesh.F90:195-236)
+  const size_t ne = (size_t)s.nspec * NGLL2;
+  s.coord.assign((size_t)2 * s.nglob, 0.0);
+  s.xix.resize(ne); s.xiz.resize(ne); s.gammax.resize(ne); s.gammaz.resize(ne); s.jacobian.resize(ne);
+  for (int e = 0; e < s.nspec; e++)
+    for (int j = 0; j < NGLL; j++)
+      for (int i = 0; i < NGLL; i++) {
+        double x, z, xxi, zxi, xga, zga;
+        map_point(s, e, s.basis.xi[i], s.basis.xi[j], &x, &z, &xxi, &zxi, &xga, &zga);
+        const double jac = xxi * zga - xga * zxi;
+        if (!(jac > 0.0)) return false;
+        const size_t l = (size_t)e * NGLL2 + j * NGLL + i;
+        s.xix[l] = (float)(zga / jac);
+        s.gammax[l] = (float)(-zxi / jac);
+        s.xiz[l] = (float)(-xga / jac);
+        s.gammaz[l] = (float)(xxi / jac);
+        s.jacobian[l] = (float)jac;
+        const int g = s.ibool[l] - 1;
+        s.coord[2 * g] = x;
+        s.coord[2 * g + 1] = z;
+      }
+  return true;
+}
+
+// materials (read_materials.f90:138-141, setup_mesh.F90:680-792)
+void build_materials(Sim& s, const double* rho, const double* vp, const double* vs) {
+  const size_t ne = (size_t)s.nspec * NGLL2;
+  s.kappa.resize(ne); s.mu.resize(ne); s.rho_vp.resize(ne); s.rho_vs.resize(ne); s.rho.resize(ne);
+  for (size_t l = 0; l < ne; l++) {
+    const double r = rho[l];
+    double v = vp[l];
+    const double mu = r * vs[l] * vs[l];
+    double kap;
+    if (s.p.material_mode == 0) {
+      const double lambda = r * v * v - 2.0 * mu;
+      kap = lambda + mu;
+      v = std::sqrt((kap + mu) / r);
+    } else {
+      kap = r * v * v - mu;
+    }
+    s.rho[l] = (float)r;
+    s.mu[l] = (float)mu;
+    s.kappa[l] = (float)kap;
+    s.rho_vp[l] = (float)(r * v);
+    s.rho_vs[l] = (float)(r * std::sqrt(mu / r));
+  }
+}
+
+// absorbing edges in the mesher's order (determine_abs_surface.f90) and their GPU arrays
+void build_stacey(Sim& s) {
+  const sf2d_params& p = s.p;
+  s.nabs = 0;
+  if (!p.stacey) return;
+  std::vector<int> el, code;
+  for (int iz = 1; iz <= p.nz; iz++)
+    for (int ix = 1; ix <= p.nx; ix++) {
+      const int e = (iz - 1) * p.nx + ix;
+      if (p.absorbbottom && iz == 1) { el.push_back(e); code.push_back(1); }
+      if (p.absorbright && ix == p.nx) { el.push_back(e); code.push_back(2); }
+      if (p.absorbtop && iz == p.nz) { el.push_back(e); code.push_back(3); }
+      if (p.absorbleft && ix == 1) { el.push_back(e); code.push_back(4); }
+    }
+  const int nf = (int)el.size();
+  s.nabs = nf;
+  s.abs_ispec = el;
+  s.edge_abs = code;
+  s.abs_ij.assign((size_t)nf * NGLL * 2, 0);
+  s.abs_normal.assign((size_t)nf * NGLL * 2, 0.f);
+  s.abs_jacw.assign((size_t)nf * NGLL, 0.f);
+  s.ib_bottom.assign(nf, 0); s.ib_left.assign(nf, 0); s.ib_right.assign(nf, 0); s.ib_top.assign(nf, 0);
+  s.ib0.assign(nf, 0);
+  for (int a = 0; a < 5; a++) s.nspec_side[a] = 0;
+  for (int f = 0; f < nf; f++) {
+    const int c = code[f];
+    const int k = ++s.nspec_side[c];
+    (c == 1 ? s.ib_bottom : c == 2 ? s.ib_right : c == 3 ? s.ib_top : s.ib_left)[f] = k;
+    s.ib0[f] = k;
+    // corner points shared with a left/right edge of the same element (read_mesh_databases.F90:957-1030)
+    bool cl = false, cr = false;
+    if (c == 1 || c == 3)
+      for (int g = 0; g < nf; g++)
+        if (g != f && el[g] == el[f]) {
+          if (code[g] == 4) cl = true;
+          if (code[g] == 2) cr = true;
+        }
+    const int e = el[f] - 1;
+    for (int k2 = 0; k2 < NGLL; k2++) {
+      int i, j;
+      if (c == 4) { i = 0; j = k2; }
+      else if (c == 2) { i = NGLL - 1; j = k2; }
+      else if (c == 1) { i = k2; j = 0; }
+      else { i = k2; j = NGLL - 1; }
+      const size_t l = (size_t)e * NGLL2 + j * NGLL + i;
+      float nx, nz, j1;
+      if (c == 4 || c == 2) {
+        const float xg = -s.xiz[l] * s.jacobian[l];
+        const float zg = +s.xix[l] * s.jacobian[l];
+        j1 = std::sqrt(xg * xg + zg * zg);
+        nx = (c == 4 ? -zg : zg) / j1;
+        nz = (c == 4 ? xg : -xg) / j1;
+        s.abs_jacw[(size_t)f * NGLL + k2] = j1 * s.basis.w[j];
+      } else {
+        const float xx = +s.gammaz[l] * s.jacobian[l];
+        const float zx = -s.gammax[l] * s.jacobian[l];
+        j1 = std::sqrt(xx * xx + zx * zx);
+        nx = (c == 1 ? zx : -zx) / j1;
+        nz = (c == 1 ? -xx : xx) / j1;
+        s.abs_jacw[(size_t)f * NGLL + k2] = j1 * s.basis.w[i];
+      }
+      s.abs_normal[((size_t)f * NGLL + k2) * 2] = nx;
+      s.abs_normal[((size_t)f * NGLL + k2) * 2 + 1] = nz;
+      int ii = i + 1, jj = j + 1;
+      if ((c == 1 || c == 3) && ((cl && i == 0) || (cr && i == NGLL - 1))) ii = NGLL + 1;
+      s.abs_ij[((size_t)f * NGLL + k2) * 2] = ii;
+      s.abs_ij[((size_t)f * NGLL + k2) * 2 + 1] = jj;
+    }
+  }
+}
+
+// mass matrix with the Newmark Stacey term (invert_mass_matrix.F90:34-464, 470-792, 798-848)
+void build_mass(Sim& s) {
+  std::vector<float>& rm = s.rmass_inverse;
+  rm.assign((size_t)2 * s.nglob, 0.f);
+  for (int e = 0; e < s.nspec; e++)
+    for (int j = 0; j < NGLL; j++)
+      for (int i = 0; i < NGLL; i++) {
+        const size_t l = (size_t)e * NGLL2 + j * NGLL + i;
+        const int g = s.ibool[l] - 1;
+        const float wxz = s.basis.w[i] * s.basis.w[j];
+        rm[2 * g] = (float)((double)rm[2 * g] + (double)wxz * (double)s.rho[l] * (double)s.jacobian[l]);
+        rm[2 * g + 1] = rm[2 * g];
+      }
+  if (s.p.stacey) {
+    const float dto2 = (float)(0.5 * (double)(float)s.p.dt);
+    for (int f = 0; f < s.nabs; f++) {
+      const int e = s.abs_ispec[f] - 1;
+      for (int k = 0; k < NGLL; k++) {
+        const int ii = s.abs_ij[((size_t)f * NGLL + k) * 2], jj = s.abs_ij[((size_t)f * NGLL + k) * 2 + 1];
+        if (ii > NGLL || jj > NGLL) continue;
+        const size_t l = (size_t)e * NGLL2 + (jj - 1) * NGLL + (ii - 1);
+        const int g = s.ibool[l] - 1;
+        const float nx = s.abs_normal[((size_t)f * NGLL + k) * 2], nz = s.abs_normal[((size_t)f * NGLL + k) * 2 + 1];
+        const float w = s.abs_jacw[(size_t)f * NGLL + k];
+        const float vn = nx * dto2 + nz * dto2;
+        const float tx = s.rho_vp[l] * vn * nx + s.rho_vs[l] * (dto2 - vn * nx);
+        const float tz = s.rho_vp[l] * vn * nz + s.rho_vs[l] * (dto2 - vn * nz);
+        rm[2 * g] = rm[2 * g] + tx * w;
+        rm[2 * g + 1] = rm[2 * g + 1] + tz * w;
+      }
+    }
+  }
+  for (auto& v : rm) {
+    if (v <= 0.f) v = 1.f;
+    v = 1.f / v;
+  }
+  s.rmassx.resize(s.nglob);
+  s.rmassz.resize(s.nglob);
+  for (int g = 0; g < s.nglob; g++) {
+    s.rmassx[g] = rm[2 * g];
+    s.rmassz[g] = rm[2 * g + 1];
+  }
+}
+
+// point location: closest GLL point (strict <, scan order ispec,j,i), then Newton from the selected element and
+// every element sharing one of its corners (locate_receivers.F90:148-340, locate_source.F90)
+void locate(const Sim& s, double px, double pz, int* ispec_out, double* xi_out, double* ga_out) {
+  const sf2d_params& p = s.p;
+  // lattice lines of the GLL points: x along columns, z along rows
+  const long lx = 4L * p.nx + 1, lz = 4L * p.nz + 1;
+  auto lattice_x = [&](long c) {
+    const long ex = std::min<long>(c / 4, p.nx - 1);
+    const int i = (int)(c - 4 * ex);
+    const int e = (int)ex;
+    return s.coord[2 * (size_t)(s.ibool[(size_t)e * NGLL2 + i] - 1)];
+  };
+  auto lattice_z = [&](long r) {
+    const long ez = std::min<long>(r / 4, p.nz - 1);
+    const int j = (int)(r - 4 * ez);
+    const int e = (int)(ez * p.nx);
+    return s.coord[2 * (size_t)(s.ibool[(size_t)e * NGLL2 + j * NGLL] - 1) + 1];
+  };
+  // nearest lattice column/row by bisection, then a +-2 neighbourhood
+  auto nearest = [](long n, double t, auto f) {
+    long lo = 0, hi = n - 1;
+    while (hi - lo > 1) {
+      long mid = (lo + hi) / 2;
+      if (f(mid) <= t) lo = mid; else hi = mid;
+    }
+    return lo;
+  };
+  const long c0 = nearest(lx, px, lattice_x), r0 = nearest(lz, pz, lattice_z);
+  double dmin = std::numeric_limits<double>::infinity();
+  long best_key = std::numeric_limits<long>::max();
+  int isel = 0, ixg = 0, izg = 0;
+  for (long r = std::max(0L, r0 - 2); r <= std::min(lz - 1, r0 + 3); r++)
+    for (long c = std::max(0L, c0 - 2); c <= std::min(lx - 1, c0 + 3); c++) {
+      // every element holding lattice point (c, r), first in scan order wins ties
+      for (long ez = std::max(0L, (r - 1) / 4); ez <= std::min<long>(p.nz - 1, r / 4); ez++)
+        for (long ex = std::max(0L, (c - 1) / 4); ex <= std::min<long>(p.nx - 1, c / 4); ex++) {
+          const int i = (int)(c - 4 * ex), j = (int)(r - 4 * ez);
+          if (i < 0 || i >= NGLL || j < 0 || j >= NGLL) continue;
+          const int e = (int)(ez * p.nx + ex);
+          const int g = s.ibool[(size_t)e * NGLL2 + j * NGLL + i] - 1;
+          const double dx = px - s.coord[2 * (size_t)g], dz = pz - s.coord[2 * (size_t)g + 1];
+          const double d2 = dx * dx + dz * dz;
+          const long key = ((long)e * NGLL + j) * NGLL + i;
+          if (d2 < dmin || (d2 == dmin && key < best_key)) {
+            dmin = d2;
+            best_key = key;
+            isel = e;
+            ixg = i;
+            izg = j;
+          }
+        }
+    }
+  // candidates: the selected element first, then corner neighbours in element order
+  std::vector<int> cand{isel};
+  {
+    const int ex0 = isel % p.nx, ez0 = isel / p.nx;
+    for (int ez = std::max(0, ez0 - 1); ez <= std::min(p.nz - 1, ez0 + 1); ez++)
+      for (int ex = std::max(0, ex0 - 1); ex <= std::min(p.nx - 1, ex0 + 1); ex++) {
+        const int e = ez * p.nx + ex;
+        if (e != isel) cand.push_back(e);
+      }
+    std::sort(cand.begin() + 1, cand.end());
+  }
+  double best = std::numeric_limits<double>::infinity();
+  for (size_t c = 0; c < cand.size(); c++) {
+    if (c > 0) { ixg = 1; izg = 1; }  // int(NGLLX/2.0) = 2 in the reference's 1-based indexing
+    const int e = cand[c];
+    double xi = s.basis.xi[ixg], ga = s.basis.xi[izg];
+    double x, z, a, b, cc, d;
+    for (int it = 0; it < 5; it++) {
+      map_point(s, e, xi, ga, &x, &z, &a, &b, &cc, &d);
+      const double jac = a * d - cc * b;
+      const double xixl = d / jac, gaxl = -b / jac, xizl = -cc / jac, gazl = a / jac;
+      const double dx = -(x - px), dz = -(z - pz);
+      xi = xi + (xixl * dx + xizl * dz);
+      ga = ga + (gaxl * dx + gazl * dz);
+      xi = std::min(1.01, std::max(-1.01, xi));
+      ga = std::min(1.01, std::max(-1.01, ga));
+    }
+    map_point(s, e, xi, ga, &x, &z, &a, &b, &cc, &d);
+    const double dist = std::sqrt((px - x) * (px - x) + (pz - z) * (pz - z));
+    if (dist < best) {
+      best = dist;
+      *ispec_out = e + 1;
+      *xi_out = xi;
+      *ga_out = ga;
+    }
+  }
+}
+
+// sources (setup_sources_receivers.F90:979-1040, set_source_parameters.f90:116-250,
+// prepare_source_time_function.f90, comp_source_time_function.f90)
+void build_sources(Sim& s) {
+  const int n = s.nsrc;
+  s.src_ispec.assign(n, 0);
+  s.src_xi.assign(n, 0.0);
+  s.src_gamma.assign(n, 0.0);
+  s.sourcearrays.assign((size_t)n * NGLL2 * 2, 0.f);
+  s.stf.assign((size_t)n * s.p.nstep, 0.f);
+  std::vector<double> t0s(n), tsh(n);
+  for (int k = 0; k < n; k++) {
+    tsh[k] = s.src[k].tshift;
+    t0s[k] = 1.20 / s.src[k].f0 + tsh[k];
+  }
+  double t0 = 0.0, min_tshift = 0.0;
+  if (n == 1) {
+    t0 = t0s[0];
+    min_tshift = tsh[0];
+    tsh[0] = 0.0;
+  } else if (n > 1) {
+    t0 = *std::min_element(t0s.begin(), t0s.end());
+    min_tshift = *std::min_element(tsh.begin(), tsh.end());
+    for (int k = 0; k < n; k++) tsh[k] = t0s[k] - t0;
+  }
+  if (s.p.user_t0 > 0.0 && t0 <= s.p.user_t0 + min_tshift) {
+    t0 = s.p.user_t0;
+    for (int k = 0; k < n; k++) tsh[k] = t0s[k] - 1.20 / s.src[k].f0;
+  }
+  s.t0 = t0;
+  for (int k = 0; k < n; k++) {
+    locate(s, s.src[k].x, s.src[k].z, &s.src_ispec[k], &s.src_xi[k], &s.src_gamma[k]);
+    double hx[NGLL], hg[NGLL];
+    lagrange(s.src_xi[k], s.basis.xi, hx);
+    lagrange(s.src_gamma[k], s.basis.xi, hg);
+    const double ang = s.src[k].anglesource * PI / 180.0;
+    for (int j = 0; j < NGLL; j++)
+      for (int i = 0; i < NGLL; i++) {
+        const double h = hx[i] * hg[j];
+        s.sourcearrays[(((size_t)k * NGLL + j) * NGLL + i) * 2] = (float)(std::sin(ang) * h);
+        s.sourcearrays[(((size_t)k * NGLL + j) * NGLL + i) * 2 + 1] = (float)(-std::cos(ang) * h);
+      }
+    const double a = PI * PI * s.src[k].f0 * s.src[k].f0;
+    for (int it = 0; it < s.p.nstep; it++) {
+      const double t = (double)it * s.p.dt - t0 - tsh[k];
+      const double g = std::exp(-a * t * t);
+      const double v = (s.src[k].time_function_type == 1) ? (1.0 - 2.0 * a * t * t) * g : t * g;
+      s.stf[(size_t)it * n + k] = (float)(-s.src[k].factor * v);
+    }
+  }
+}
+
+void build_receivers(Sim& s, const double* rec_xz) {
+  const int n = s.nrec;
+  s.rec_ispec.assign(n, 0);
+  s.rec_xi.assign(n, 0.0);
+  s.rec_gamma.assign(n, 0.0);
+  s.xir.assign((size_t)NGLL * std::max(n, 1), 0.f);
+  s.gammar.assign((size_t)NGLL * std::max(n, 1), 0.f);
+  s.cosrot.assign(std::max(n, 1), 1.f);
+  s.sinrot.assign(std::max(n, 1), 0.f);
+  const double ang = s.p.anglerec * PI / 180.0;
+  for (int r = 0; r < n; r++) {
+    locate(s, rec_xz[2 * r], rec_xz[2 * r + 1], &s.rec_ispec[r], &s.rec_xi[r], &s.rec_gamma[r]);
+    double hx[NGLL], hg[NGLL];
+    lagrange(s.rec_xi[r], s.basis.xi, hx);
+    lagrange(s.rec_gamma[r], s.basis.xi, hg);
+    for (int i = 0; i < NGLL; i++) {
+      s.xir[(size_t)i * n + r] = (float)hx[i];
+      s.gammar[(size_t)i * n + r] = (float)hg[i];
+    }
+    s.cosrot[r] = (float)std::cos(ang);
+    s.sinrot[r] = (float)std::sin(ang);
+  }
+}
+
+// device preparation, in the order of prepare_gpu.f90:80-347
+void prepare_device(Sim& s) {
+  int myrank = 0, ndev = 0;
+  initialize_cuda_device_(&myrank, &ndev);
+  int ngllx = NGLL, nspec = s.nspec, nglob = s.nglob, zero = 0, one = 1;
+  int stacey = s.p.stacey ? 1 : 0, pml = 0;
+  std::vector<int> inner(s.nspec, 1), is_elastic(s.nspec, 1), is_acoustic(s.nspec, 0);
+  int nsrc = s.nsrc, nstep = s.p.nstep, nrec = s.nrec;
+  int simtype = s.simulation_type, p_sv = 1, nspec_ac = 0, nspec_el = s.nspec, save_fwd = s.save_forward;
+  int nsig = 1, nlength = s.p.nstep;
+  int dummy_i = 0;
+  std::vector<int> src_ispec = s.src_ispec, rec_ispec = s.rec_ispec;
+  if (src_ispec.empty()) src_ispec.push_back(1);
+  if (rec_ispec.empty()) rec_ispec.push_back(1);
+  std::vector<float> sa = s.sourcearrays, stf = s.stf;
+  if (sa.empty()) sa.assign(2 * NGLL2, 0.f);
+  if (stf.empty()) stf.assign(1, 0.f);
+  prepare_constants_device_(&s.mesh, &ngllx, &nspec, &nglob, s.xix.data(), s.xiz.data(), s.gammax.data(),
+                            s.gammaz.data(), s.kappa.data(), s.mu.data(), s.ibool.data(), &zero, &zero, &dummy_i,
+                            &dummy_i, s.basis.hprime_f, s.basis.hprimewgll_f, s.basis.w, &stacey, &pml, inner.data(),
+                            &nsrc, sa.data(), stf.data(), &nstep, src_ispec.data(), rec_ispec.data(), &nrec,
+                            s.cosrot.data(), s.sinrot.data(), &simtype, &p_sv, &nspec_ac, &nspec_el,
+                            is_acoustic.data(), is_elastic.data(), &myrank, &save_fwd, s.xir.data(), s.gammar.data(),
+                            &nsig, s.seismotype, &nlength);
+  // single partition: every element is "inner" (iphase 2); the outer list is empty
+  int num_phase = s.nspec;
+  s.phase_ispec.assign((size_t)2 * s.nspec, 0);
+  for (int e = 0; e < s.nspec; e++) s.phase_ispec[(size_t)s.nspec + e] = e + 1;
+  std::vector<int> aniso(s.nspec, 0);
+  float dummy_f = 0.f;
+  prepare_fields_elastic_device_(&s.mesh, s.rmassx.data(), s.rmassz.data(), &num_phase, s.phase_ispec.data(),
+                                 aniso.data(), &zero, &dummy_f, &dummy_f, &dummy_f, &dummy_f, &dummy_f, &dummy_f,
+                                 &dummy_f, &dummy_f, &dummy_f, &zero, &dummy_i, &zero, &dummy_f, &dummy_f, &dummy_f,
+                                 &dummy_f);
+  if (s.simulation_type == 3) {
+    int size = 2 * s.nglob;
+    prepare_fields_elastic_adj_dev_(&s.mesh, &size, &zero, &zero, &zero);
+  }
+  if (s.p.stacey) {
+    int nf = s.nabs;
+    prepare_stacey_device_(&s.mesh, &zero, &one, s.rho_vp.data(), s.rho_vs.data(), &s.nspec_side[1], &s.nspec_side[4],
+                           &s.nspec_side[2], &s.nspec_side[3], s.abs_ispec.data(), s.abs_ij.data(),
+                           s.abs_normal.data(), s.abs_jacw.data(), &nf, s.edge_abs.data(), s.ib_bottom.data(),
+                           s.ib_left.data(), s.ib_right.data(), s.ib_top.data());
+  }
+  std::vector<float> zeros((size_t)2 * s.nglob, 0.f);
+  int size = 2 * s.nglob;
+  transfer_fields_el_to_device_(&size, zeros.data(), zeros.data(), zeros.data(), &s.mesh);
+  if (s.simulation_type == 3) transfer_b_fields_to_device_(&size, zeros.data(), zeros.data(), zeros.data(), &s.mesh);
+}
+
+}  // namespace
+
+extern "C" {
+
+void* sf2d_sim_create(const sf2d_params* p, const double* rho, const double* vp, const double* vs, int nsrc,
+                      const sf2d_source* src, int nrec, const double* rec_xz, int simulation_type,
+                      int save_forward) {
+  Sim* s = new Sim();
+  s->p = *p;
+  s->simulation_type = simulation_type;
+  s->save_forward = save_forward;
+  s->basis = make_basis();
+  if (!build_mesh(*s)) {
+    fprintf(stderr, "sf2d_sim_create: invalid mesh parameters\n");
+    delete s;
+    return nullptr;
+  }
+  build_materials(*s, rho, vp, vs);
+  build_stacey(*s);
+  build_mass(*s);
+  s->nsrc = nsrc;
+  s->src.assign(src, src + nsrc);
+  build_sources(*s);
+  s->nrec = nrec;
+  build_receivers(*s, rec_xz);
+  return s;
+}
+
+int sf2d_sim_prepare_device(void* sim) {
+  Sim* s = (Sim*)sim;
+  if (!s) return 1;
+  if (s->mesh) return 0;
+  prepare_device(*s);
+  return 0;
+}
+
+void sf2d_sim_destroy(void* sim) {
+  Sim* s = (Sim*)sim;
+  if (!s) return;
+  if (s->mesh) {
+    int one = 1, zero = 0;
+    float dummy = 0.f;
+    prepare_cleanup_device_(&s->mesh, &zero, &one, &zero, &zero, &zero, &zero, &zero, &dummy);
+  }
+  delete s;
+}
+
+int sf2d_sim_nspec(void* sim) { return ((Sim*)sim)->nspec; }
+int sf2d_sim_nglob(void* sim) { return ((Sim*)sim)->nglob; }
+int sf2d_sim_nabs(void* sim) { return ((Sim*)sim)->nabs; }
+double sf2d_sim_t0(void* sim) { return ((Sim*)sim)->t0; }
+long sf2d_sim_mesh_pointer(void* sim) { return ((Sim*)sim)->mesh; }
+
+void sf2d_sim_get_ibool(void* sim, int* out) {
+  Sim* s = (Sim*)sim;
+  std::copy(s->ibool.begin(), s->ibool.end(), out);
+}
+void sf2d_sim_get_rmass_inverse(void* sim, float* out) {
+  Sim* s = (Sim*)sim;
+  std::copy(s->rmass_inverse.begin(), s->rmass_inverse.end(), out);
+}
+void sf2d_sim_get_elem_array(void* sim, int which, float* out) {
+  Sim* s = (Sim*)sim;
+  const std::vector<float>* a[10] = {&s->xix, &s->xiz, &s->gammax, &s->gammaz, &s->jacobian,
+                                     &s->kappa, &s->mu, &s->rho_vp, &s->rho_vs, &s->rho};
+  if (which < 0 || which > 9) return;
+  std::copy(a[which]->begin(), a[which]->end(), out);
+}
+void sf2d_sim_get_sources(void* sim, int* ispec, double* xi, double* gamma, float* sourcearrays, float* stf) {
+  Sim* s = (Sim*)sim;
+  for (int k = 0; k < s->nsrc; k++) {
+    ispec[k] = s->src_ispec[k];
+    xi[k] = s->src_xi[k];
+    gamma[k] = s->src_gamma[k];
+  }
+  std::copy(s->sourcearrays.begin(), s->sourcearrays.end(), sourcearrays);
+  std::copy(s->stf.begin(), s->stf.end(), stf);
+}
+void sf2d_sim_get_receivers(void* sim, int* ispec, double* xi, double* gamma, float* xir, float* gammar) {
+  Sim* s = (Sim*)sim;
+  for (int r = 0; r < s->nrec; r++) {
+    ispec[r] = s->rec_ispec[r];
+    xi[r] = s->rec_xi[r];
+    gamma[r] = s->rec_gamma[r];
+    for (int i = 0; i < NGLL; i++) {
+      xir[r * NGLL + i] = s->xir[(size_t)i * s->nrec + r];
+      gammar[r * NGLL + i] = s->gammar[(size_t)i * s->nrec + r];
+    }
+  }
+}
+void sf2d_sim_get_stacey(void* sim, int* ispec, int* ij, float* normal, float* jacw, int* edge, int* ib) {
+  Sim* s = (Sim*)sim;
+  std::copy(s->abs_ispec.begin(), s->abs_ispec.end(), ispec);
+  std::copy(s->abs_ij.begin(), s->abs_ij.end(), ij);
+  std::copy(s->abs_normal.begin(), s->abs_normal.end(), normal);
+  std::copy(s->abs_jacw.begin(), s->abs_jacw.end(), jacw);
+  std::copy(s->edge_abs.begin(), s->edge_abs.end(), edge);
+  std::copy(s->ib0.begin(), s->ib0.end(), ib);
+}
+
+int sf2d_sim_forward(void* sim, int nstep_run, double* seis, float* absorb, double* timing) {
+  Sim* s = (Sim*)sim;
+  if (!s || !s->mesh || nstep_run < 1 || s->it_done + nstep_run > s->p.nstep) return 1;
+  // Newmark constants (prepare_timerun.F90:262-264)
+  float deltat = (float)s->p.dt;
+  float deltatover2 = (float)(0.5 * (double)deltat);
+  float deltatsqover2 = (float)(0.5 * (double)deltat * (double)deltat);
+  float b_deltat = 0.f, b_deltatover2 = 0.f, b_deltatsqover2 = 0.f;
+  int nspec_outer = 0, nspec_inner = s->nspec, zero = 0, one = 1;
+  const int it_begin = s->it_done + 1;
+  int i_sig = 1, nlength = s->p.nstep, it_end = s->it_done + nstep_run;
+  int elastic_sim = 1, acoustic_sim = 0;
+  if (s->sisux.empty()) {
+    s->sisux.assign((size_t)std::max(1, s->nrec) * nlength, 0.0);
+    s->sisuz.assign(s->sisux.size(), 0.0);
+  }
+  const int counts[5] = {0, s->nspec_side[1], s->nspec_side[2], s->nspec_side[3], s->nspec_side[4]};
+  std::vector<float> slice[5];
+  for (int c = 1; c <= 4; c++) slice[c].assign((size_t)2 * NGLL * std::max(1, counts[c]), 0.f);
+  // timing: events are recorded on the library's stream and read once after the loop
+  const bool timed = timing != nullptr;
+  const bool phases = timed && timing[0] < 0.0;
+  hipStream_t stream = (hipStream_t)sf2d_mesh_stream(&s->mesh);
+  const int nev = phases ? 10 : 2;  // per step: forces (2) [+ predictor, boundary, corrector, seismograms]
+  std::vector<hipEvent_t> ev(timed ? (size_t)nev * nstep_run : 0);
+  for (auto& e : ev) hc(hipEventCreate(&e));
+  hipEvent_t loop_start, loop_end;
+  if (timed) {
+    hc(hipEventCreate(&loop_start));
+    hc(hipEventCreate(&loop_end));
+    hc(hipEventRecord(loop_start, stream));
+  }
+  auto mark = [&](int step, int k) {
+    if (timed && k < nev) hc(hipEventRecord(ev[(size_t)step * nev + k], stream));
+  };
+  for (int it = it_begin; it <= it_end; it++) {
+    const int step = it - it_begin;
+    mark(step, 2);
+    update_displacement_cuda_(&s->mesh, &deltat, &deltatsqover2, &deltatover2, &b_deltat, &b_deltatsqover2,
+                              &b_deltatover2);
+    mark(step, 3);
+    for (int iphase = 1; iphase <= 2; iphase++) {
+      if (iphase == 2) mark(step, 0);
+      compute_forces_viscoelastic_cuda_(&s->mesh, &iphase, &deltat, &nspec_outer, &nspec_inner, &zero, &zero, &one,
+                                        &zero);
+      if (iphase == 2) mark(step, 1);
+      if (iphase == 1) {
+        mark(step, 4);
+        if (s->p.stacey)
+          compute_stacey_viscoelastic_cuda_(&s->mesh, &iphase, slice[4].data(), slice[2].data(), slice[3].data(),
+                                            slice[1].data(), &one, &zero, &zero);
+        compute_add_sources_el_cuda_(&s->mesh, &iphase, &it);
+        mark(step, 5);
+      }
+    }
+    if (absorb && s->save_forward && s->nabs > 0) {
+      // the Fortran caller keeps the per-step slices (compute_gpu_elastic.f90:267-273)
+      for (int f = 0; f < s->nabs; f++) {
+        const int c = s->edge_abs[f];
+        const int ib = s->ib0[f] - 1;
+        for (int k = 0; k < NGLL; k++)
+          for (int d = 0; d < 2; d++)
+            absorb[(((size_t)step * s->nabs + f) * NGLL + k) * 2 + d] = slice[c][((size_t)ib * NGLL + k) * 2 + d];
+      }
+    }
+    mark(step, 6);
+    kernel_3_a_cuda_(&s->mesh, &deltatover2, &b_deltatover2, &one, &zero);
+    mark(step, 7);
+    mark(step, 8);
+    if (s->nrec > 0) {
+      int cur = it;
+      compute_seismograms_cuda_(&s->mesh, &i_sig, s->sisux.data(), s->sisuz.data(), &cur, &nlength, &elastic_sim,
+                                &acoustic_sim, &zero, &zero, &it, &it_end);
+    }
+    mark(step, 9);
+  }
+  s->it_done = it_end;
+  if (timed) {
+    hc(hipEventRecord(loop_end, stream));
+    hc(hipEventSynchronize(loop_end));
+    float ms = 0.f;
+    hc(hipEventElapsedTime(&ms, loop_start, loop_end));
+    double acc[6] = {ms, 0, 0, 0, 0, 0};
+    const int pairs[5][2] = {{2, 3}, {0, 1}, {4, 5}, {6, 7}, {8, 9}};  // predictor, forces, boundary, corrector, seismo
+    for (int step = 0; step < nstep_run; step++)
+      for (int q = 0; q < 5; q++) {
+        if (pairs[q][1] >= nev) continue;
+        float t = 0.f;
+        hc(hipEventElapsedTime(&t, ev[(size_t)step * nev + pairs[q][0]], ev[(size_t)step * nev + pairs[q][1]]));
+        acc[q + 1] += t;
+      }
+    for (int k = 0; k < 6; k++) timing[k] = acc[k];
+    for (auto& e : ev) hc(hipEventDestroy(e));
+    hc(hipEventDestroy(loop_start));
+    hc(hipEventDestroy(loop_end));
+  } else {
+    hc(hipStreamSynchronize(stream));
+  }
+  if (seis)
+    for (int r = 0; r < s->nrec; r++)
+      for (int k = 0; k < nstep_run; k++) {
+        seis[((size_t)r * 2 + 0) * nstep_run + k] = s->sisux[(size_t)r * nlength + it_begin - 1 + k];
+        seis[((size_t)r * 2 + 1) * nstep_run + k] = s->sisuz[(size_t)r * nlength + it_begin - 1 + k];
+      }
+  return 0;
+}
+
+void sf2d_sim_reset(void* sim) {
+  Sim* s = (Sim*)sim;
+  if (!s || !s->mesh) return;
+  std::vector<float> zeros((size_t)2 * s->nglob, 0.f);
+  int size = 2 * s->nglob;
+  transfer_fields_el_to_device_(&size, zeros.data(), zeros.data(), zeros.data(), &s->mesh);
+  s->it_done = 0;
+  std::fill(s->sisux.begin(), s->sisux.end(), 0.0);
+  std::fill(s->sisuz.begin(), s->sisuz.end(), 0.0);
+}
+
+int sf2d_sim_steps_done(void* sim) { return ((Sim*)sim)->it_done; }
+
+void sf2d_sim_get_fields(void* sim, float* displ, float* veloc, float* accel) {
+  Sim* s = (Sim*)sim;
+  int size = 2 * s->nglob;
+  std::vector<float> tmp((size_t)size);
+  if (displ) transfer_displ_from_device_(&size, displ, &s->mesh);
+  if (accel) transfer_accel_from_device_(&size, accel, &s->mesh);
+  if (veloc) {
+    transfer_fields_el_from_device_(&size, tmp.data(), veloc, tmp.data(), &s->mesh);
+  }
+}
+void sf2d_sim_set_fields(void* sim, const float* displ, const float* veloc, const float* accel) {
+  Sim* s = (Sim*)sim;
+  int size = 2 * s->nglob;
+  std::vector<float> d((size_t)size, 0.f), v((size_t)size, 0.f), a((size_t)size, 0.f);
+  if (displ) std::copy(displ, displ + size, d.begin());
+  if (veloc) std::copy(veloc, veloc + size, v.begin());
+  if (accel) std::copy(accel, accel + size, a.begin());
+  transfer_fields_el_to_device_(&size, d.data(), v.data(), a.data(), &s->mesh);
+}
+void sf2d_sim_get_b_fields(void* sim, float* displ, float* veloc, float* accel) {
+  Sim* s = (Sim*)sim;
+  int size = 2 * s->nglob;
+  std::vector<float> d((size_t)size), v((size_t)size), a((size_t)size);
+  transfer_b_fields_from_device_(&size, d.data(), v.data(), a.data(), &s->mesh);
+  if (displ) std::copy(d.begin(), d.end(), displ);
+  if (veloc) std::copy(v.begin(), v.end(), veloc);
+  if (accel) std::copy(a.begin(), a.end(), accel);
+}
+void sf2d_sim_set_b_fields(void* sim, const float* displ, const float* veloc, const float* accel) {
+  Sim* s = (Sim*)sim;
+  int size = 2 * s->nglob;
+  std::vector<float> d((size_t)size, 0.f), v((size_t)size, 0.f), a((size_t)size, 0.f);
+  if (displ) std::copy(displ, displ + size, d.begin());
+  if (veloc) std::copy(veloc, veloc + size, v.begin());
+  if (accel) std::copy(accel, accel + size, a.begin());
+  transfer_b_fields_to_device_(&size, d.data(), v.data(), a.data(), &s->mesh);
+}
+
+}  // extern "C"

@@ -1,0 +1,474 @@
+// Elastic time-stepping kernels for gfx950 (MI355X).
+//
+// The element force kernel is the hot spot (reference: src/gpu/kernels/Kernel_2_viscoelastic_impl.cu:294-611,
+// CPU: src/specfem2D/compute_forces_viscoelastic.F90:34-885).  It is HBM-bound (~2.9 flop/B), so the design is
+// about bytes and latency hiding, not math:
+//   * flat GLL-point mapping: a 256-thread block owns EPB=10 consecutive elements (250 active lanes), so every
+//     per-element array is read with contiguous, fully coalesced loads (no 25->32 padding traffic);
+//   * the element's displacement and the two stress-flux fields live in LDS between the two 5x5 contractions;
+//     the derivative matrices are staged in LDS once per block;
+//   * assembly into accel uses hardware FP32 atomics (global_atomic_add_f32, no return);
+//   * blocks are remapped so that each XCD walks a contiguous range of elements: neighbouring elements share
+//     nodes, so their gathers and atomics hit the same L2.
+// All kernels run on the handle's stream.  Every launch bounds its indices against the array sizes it was given.
+#include "sf2d_mesh.h"
+
+namespace sf2d {
+
+namespace {
+
+constexpr int FORCE_THREADS = 256;
+constexpr int FORCE_EPB = 10;  // elements per block: 250 of 256 lanes active
+constexpr int NXCD = 8;        // gfx950: 8 accelerator complex dies, workgroups dispatched round-robin
+
+__device__ __forceinline__ void atomic_add_f32(float* p, float v) { unsafeAtomicAdd(p, v); }
+
+// ---------------------------------------------------------------------------
+// Newmark predictor  (src/gpu/kernels/UpdateDispVeloc_kernel.cu:37-66; update_displacement_Newmark.F90:408-454)
+__global__ __launch_bounds__(256) void k_predictor(float2* __restrict__ d, float2* __restrict__ v,
+                                                   float2* __restrict__ a, int n, float dt, float dt2o2, float dto2) {
+  int stride = gridDim.x * blockDim.x;
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
+    float2 dd = d[k], vv = v[k], aa = a[k];
+    dd.x = dd.x + dt * vv.x + dt2o2 * aa.x;
+    dd.y = dd.y + dt * vv.y + dt2o2 * aa.y;
+    vv.x = vv.x + dto2 * aa.x;
+    vv.y = vv.y + dto2 * aa.y;
+    d[k] = dd;
+    v[k] = vv;
+    a[k] = make_float2(0.f, 0.f);
+  }
+}
+
+// Newmark corrector with the inverse mass matrix (src/gpu/kernels/kernel_3_cuda_device.cu:37-65)
+__global__ __launch_bounds__(256) void k_corrector(float2* __restrict__ v, float2* __restrict__ a,
+                                                   const float* __restrict__ rmx, const float* __restrict__ rmz,
+                                                   int n, float dto2, int apply_mass) {
+  int stride = gridDim.x * blockDim.x;
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
+    float2 aa = a[k], vv = v[k];
+    if (apply_mass) {
+      aa.x = aa.x * rmx[k];
+      aa.y = aa.y * rmz[k];
+      a[k] = aa;
+    }
+    vv.x = vv.x + dto2 * aa.x;
+    vv.y = vv.y + dto2 * aa.y;
+    v[k] = vv;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// isotropic elastic element forces, P-SV
+__global__ __launch_bounds__(FORCE_THREADS) void k_elastic_forces(
+    int nelem, int nbatch, int per_xcd, const int* __restrict__ elist, const int* __restrict__ ibool,
+    const float* __restrict__ xix, const float* __restrict__ xiz, const float* __restrict__ gammax,
+    const float* __restrict__ gammaz, const float* __restrict__ kappav, const float* __restrict__ muv,
+    const float2* __restrict__ displ, float* __restrict__ accel, const Basis basis) {
+  // XCD-aware remap: blocks b, b+8, b+16, ... (one XCD) take consecutive batches
+  const int bid = blockIdx.x;
+  const int batch = (bid % NXCD) * per_xcd + bid / NXCD;
+  if (batch >= nbatch) return;
+
+  __shared__ float s_hp[NGLL2], s_hpw[NGLL2], s_w[NGLL];
+  __shared__ float s_ux[FORCE_THREADS], s_uz[FORCE_THREADS];
+  __shared__ float s_t1x[FORCE_THREADS], s_t1z[FORCE_THREADS], s_t2x[FORCE_THREADS], s_t2z[FORCE_THREADS];
+
+  const int tid = threadIdx.x;
+  if (tid < NGLL2) {
+    s_hp[tid] = basis.hp[tid];
+    s_hpw[tid] = basis.hpw[tid];
+  }
+  if (tid < NGLL) s_w[tid] = basis.w[tid];
+
+  const int le = tid / NGLL2;
+  const int p = tid - le * NGLL2;
+  const int i = p % NGLL;
+  const int j = p / NGLL;
+  const int e = batch * FORCE_EPB + le;
+  const bool active = (tid < FORCE_EPB * NGLL2) && (e < nelem);
+
+  int g = 0;
+  float xixl = 0.f, xizl = 0.f, gaxl = 0.f, gazl = 0.f, kap = 0.f, mu = 0.f;
+  if (active) {
+    const int ispec = elist ? elist[e] : e;
+    const size_t l = (size_t)ispec * NGLL2 + p;
+    g = ibool[l];
+    const float2 u = displ[g];
+    s_ux[tid] = u.x;
+    s_uz[tid] = u.y;
+    xixl = xix[l];
+    xizl = xiz[l];
+    gaxl = gammax[l];
+    gazl = gammaz[l];
+    kap = kappav[l];
+    mu = muv[l];
+  }
+  __syncthreads();
+
+  const int base = le * NGLL2;
+  if (active) {
+    float dux_dxi = 0.f, duz_dxi = 0.f, dux_dga = 0.f, duz_dga = 0.f;
+#pragma unroll
+    for (int k = 0; k < NGLL; k++) {
+      const float hi = s_hp[i * NGLL + k];
+      const float hj = s_hp[j * NGLL + k];
+      dux_dxi += s_ux[base + j * NGLL + k] * hi;
+      duz_dxi += s_uz[base + j * NGLL + k] * hi;
+      dux_dga += s_ux[base + k * NGLL + i] * hj;
+      duz_dga += s_uz[base + k * NGLL + i] * hj;
+    }
+    const float dux_dxl = dux_dxi * xixl + dux_dga * gaxl;
+    const float dux_dzl = dux_dxi * xizl + dux_dga * gazl;
+    const float duz_dxl = duz_dxi * xixl + duz_dga * gaxl;
+    const float duz_dzl = duz_dxi * xizl + duz_dga * gazl;
+    const float jac = 1.f / (xixl * gazl - gaxl * xizl);
+    const float lambdal = kap - mu;
+    const float lp2m = kap + mu;
+    const float sxx = lp2m * dux_dxl + lambdal * duz_dzl;
+    const float szz = lp2m * duz_dzl + lambdal * dux_dxl;
+    const float sxz = mu * (duz_dxl + dux_dzl);
+    s_t1x[tid] = jac * (sxx * xixl + sxz * xizl);
+    s_t1z[tid] = jac * (sxz * xixl + szz * xizl);
+    s_t2x[tid] = jac * (sxx * gaxl + sxz * gazl);
+    s_t2z[tid] = jac * (sxz * gaxl + szz * gazl);
+  }
+  __syncthreads();
+
+  if (active) {
+    float t1x = 0.f, t1z = 0.f, t2x = 0.f, t2z = 0.f;
+#pragma unroll
+    for (int k = 0; k < NGLL; k++) {
+      const float wi = s_hpw[k * NGLL + i];
+      const float wj = s_hpw[k * NGLL + j];
+      t1x += s_t1x[base + j * NGLL + k] * wi;
+      t1z += s_t1z[base + j * NGLL + k] * wi;
+      t2x += s_t2x[base + k * NGLL + i] * wj;
+      t2z += s_t2z[base + k * NGLL + i] * wj;
+    }
+    const float fx = -(s_w[j] * t1x + s_w[i] * t2x);
+    const float fz = -(s_w[j] * t1z + s_w[i] * t2z);
+    atomic_add_f32(&accel[2 * (size_t)g], fx);
+    atomic_add_f32(&accel[2 * (size_t)g + 1], fz);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Stacey absorbing traction (src/gpu/kernels/compute_stacey_elastic_kernel.cu:37-150,
+// src/specfem2D/compute_stacey_elastic.f90:34-1001)
+__global__ __launch_bounds__(64) void k_stacey(int nabs, const int* __restrict__ abs_ispec,
+                                               const int* __restrict__ abs_ij, const float* __restrict__ normal,
+                                               const float* __restrict__ jacw, const int* __restrict__ edge_abs,
+                                               const int* __restrict__ ib, const int* __restrict__ ibool,
+                                               const int* __restrict__ is_elastic, const float* __restrict__ rho_vp,
+                                               const float* __restrict__ rho_vs, const float2* __restrict__ veloc,
+                                               float* __restrict__ accel, int write_store, float* store_bottom,
+                                               float* store_right, float* store_top, float* store_left) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  const int f = idx / NGLL;
+  const int k = idx - f * NGLL;
+  if (f >= nabs) return;
+  const int i = abs_ij[(f * NGLL + k) * 2];
+  const int j = abs_ij[(f * NGLL + k) * 2 + 1];
+  if (i < 0 || j < 0) return;  // duplicated corner point, handled by the neighbouring edge
+  const int ispec = abs_ispec[f];
+  if (!is_elastic[ispec]) return;
+  const size_t l = (size_t)ispec * NGLL2 + j * NGLL + i;
+  const int g = ibool[l];
+  const float2 v = veloc[g];
+  const float nx = normal[(f * NGLL + k) * 2];
+  const float nz = normal[(f * NGLL + k) * 2 + 1];
+  const float vn = nx * v.x + nz * v.y;
+  const float rvp = rho_vp[l], rvs = rho_vs[l];
+  const float tx = rvp * vn * nx + rvs * (v.x - vn * nx);
+  const float tz = rvp * vn * nz + rvs * (v.y - vn * nz);
+  const float w = jacw[f * NGLL + k];
+  const float fx = tx * w, fz = tz * w;
+  atomic_add_f32(&accel[2 * (size_t)g], -fx);
+  atomic_add_f32(&accel[2 * (size_t)g + 1], -fz);
+  if (write_store) {
+    const int side = edge_abs[f];
+    float* st = side == 1 ? store_bottom : side == 2 ? store_right : side == 3 ? store_top : store_left;
+    if (st) {
+      const int o = (ib[f] * NGLL + k) * 2;
+      st[o] = fx;
+      st[o + 1] = fz;
+    }
+  }
+}
+
+// backward wavefield: subtract the stored forward traction (compute_stacey_elastic_kernel.cu:155-218)
+__global__ __launch_bounds__(64) void k_stacey_backward(int nabs, const int* __restrict__ abs_ispec,
+                                                        const int* __restrict__ abs_ij,
+                                                        const int* __restrict__ edge_abs, const int* __restrict__ ib,
+                                                        const int* __restrict__ ibool,
+                                                        const int* __restrict__ is_elastic, float* __restrict__ b_accel,
+                                                        const float* sb, const float* sr, const float* stp,
+                                                        const float* sl) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  const int f = idx / NGLL;
+  const int k = idx - f * NGLL;
+  if (f >= nabs) return;
+  const int i = abs_ij[(f * NGLL + k) * 2];
+  const int j = abs_ij[(f * NGLL + k) * 2 + 1];
+  if (i < 0 || j < 0) return;
+  const int ispec = abs_ispec[f];
+  if (!is_elastic[ispec]) return;
+  const int g = ibool[(size_t)ispec * NGLL2 + j * NGLL + i];
+  const int side = edge_abs[f];
+  const float* st = side == 1 ? sb : side == 2 ? sr : side == 3 ? stp : sl;
+  if (!st) return;
+  const int o = (ib[f] * NGLL + k) * 2;
+  atomic_add_f32(&b_accel[2 * (size_t)g], -st[o]);
+  atomic_add_f32(&b_accel[2 * (size_t)g + 1], -st[o + 1]);
+}
+
+// ---------------------------------------------------------------------------
+// collocated force sources (src/gpu/kernels/compute_add_sources_kernel.cu:37-70)
+__global__ __launch_bounds__(64) void k_sources(int nsrc, int it, const int* __restrict__ ispec_src,
+                                                const int* __restrict__ is_elastic, const int* __restrict__ ibool,
+                                                const float* __restrict__ sourcearrays, const float* __restrict__ stf,
+                                                float* __restrict__ accel) {
+  const int s = blockIdx.x;
+  const int p = threadIdx.x;
+  if (s >= nsrc || p >= NGLL2) return;
+  const int ispec = ispec_src[s];
+  if (!is_elastic[ispec]) return;
+  const int g = ibool[(size_t)ispec * NGLL2 + p];
+  const float f = stf[(size_t)it * nsrc + s];
+  atomic_add_f32(&accel[2 * (size_t)g], sourcearrays[(s * NGLL2 + p) * 2] * f);
+  atomic_add_f32(&accel[2 * (size_t)g + 1], sourcearrays[(s * NGLL2 + p) * 2 + 1] * f);
+}
+
+// adjoint sources at the receivers (src/gpu/kernels/add_sources_el_SIM_TYPE_2_OR_3_kernel.cu:37-74)
+__global__ __launch_bounds__(64) void k_adjoint_sources(int nadj, int it, int nstep, const int* __restrict__ ispec_rec,
+                                                        const int* __restrict__ is_elastic,
+                                                        const int* __restrict__ ibool, const float* __restrict__ xir,
+                                                        const float* __restrict__ gammar,
+                                                        const float* __restrict__ adj, float* __restrict__ accel) {
+  const int r = blockIdx.x;
+  const int p = threadIdx.x;
+  if (r >= nadj || p >= NGLL2) return;
+  const int ispec = ispec_rec[r];
+  if (!is_elastic[ispec]) return;
+  const int i = p % NGLL, j = p / NGLL;
+  const int g = ibool[(size_t)ispec * NGLL2 + p];
+  const float h = xir[i * nadj + r] * gammar[j * nadj + r];
+  const float ax = adj[((size_t)0 * nstep + it) * nadj + r];
+  const float az = adj[((size_t)1 * nstep + it) * nadj + r];
+  atomic_add_f32(&accel[2 * (size_t)g], ax * h);
+  atomic_add_f32(&accel[2 * (size_t)g + 1], az * h);
+}
+
+// ---------------------------------------------------------------------------
+// seismograms: Lagrange interpolation at the receivers, FP64 accumulation, rotation
+// (src/gpu/kernels/compute_elastic_seismogram_kernel.cu:69-192; compute_interpolated_dva.f90)
+__global__ __launch_bounds__(64) void k_seismograms(int nrec, int nlength, int cur, const int* __restrict__ ispec_rec,
+                                                    const int* __restrict__ ibool, const float* __restrict__ xir,
+                                                    const float* __restrict__ gammar, const float* __restrict__ cosrot,
+                                                    const float* __restrict__ sinrot, const float2* __restrict__ field,
+                                                    float* __restrict__ dst) {
+  const int r = blockIdx.x;
+  const int p = threadIdx.x;
+  if (r >= nrec) return;
+  double vx = 0.0, vz = 0.0;
+  if (p < NGLL2) {
+    const int i = p % NGLL, j = p / NGLL;
+    const int g = ibool[(size_t)ispec_rec[r] * NGLL2 + p];
+    const double h = (double)(xir[i * nrec + r] * gammar[j * nrec + r]);
+    const float2 u = field[g];
+    vx = (double)u.x * h;
+    vz = (double)u.y * h;
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    vx += __shfl_xor(vx, off, 64);
+    vz += __shfl_xor(vz, off, 64);
+  }
+  if (p == 0) {
+    const double c = cosrot[r], s = sinrot[r];
+    dst[(size_t)r * nlength + cur] = (float)(c * vx + s * vz);
+    dst[((size_t)nrec + r) * nlength + cur] = (float)(-s * vx + c * vz);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// adjoint sensitivity kernels rho/mu/kappa (src/gpu/kernels/compute_kernels_cuda_kernel.cu:37-200,
+// src/specfem2D/compute_kernels.f90)
+__global__ __launch_bounds__(FORCE_THREADS) void k_kernels(int nspec, const int* __restrict__ ibool,
+                                                           const int* __restrict__ is_elastic,
+                                                           const float* __restrict__ xix,
+                                                           const float* __restrict__ xiz,
+                                                           const float* __restrict__ gammax,
+                                                           const float* __restrict__ gammaz,
+                                                           const float2* __restrict__ displ,
+                                                           const float2* __restrict__ accel,
+                                                           const float2* __restrict__ b_displ,
+                                                           float* __restrict__ rho_kl, float* __restrict__ mu_kl,
+                                                           float* __restrict__ kappa_kl, const Basis basis) {
+  __shared__ float s_hp[NGLL2];
+  __shared__ float s_ux[FORCE_THREADS], s_uz[FORCE_THREADS], s_bx[FORCE_THREADS], s_bz[FORCE_THREADS];
+  const int tid = threadIdx.x;
+  if (tid < NGLL2) s_hp[tid] = basis.hp[tid];
+  const int le = tid / NGLL2;
+  const int p = tid - le * NGLL2;
+  const int i = p % NGLL, j = p / NGLL;
+  const int ispec = blockIdx.x * FORCE_EPB + le;
+  bool active = (tid < FORCE_EPB * NGLL2) && (ispec < nspec);
+  if (active) active = is_elastic[ispec] != 0;
+  int g = 0;
+  size_t l = 0;
+  if (active) {
+    l = (size_t)ispec * NGLL2 + p;
+    g = ibool[l];
+    const float2 u = displ[g], b = b_displ[g];
+    s_ux[tid] = u.x;
+    s_uz[tid] = u.y;
+    s_bx[tid] = b.x;
+    s_bz[tid] = b.y;
+  }
+  __syncthreads();
+  if (!active) return;
+  const int base = le * NGLL2;
+  float t1x = 0.f, t1z = 0.f, t3x = 0.f, t3z = 0.f, b1x = 0.f, b1z = 0.f, b3x = 0.f, b3z = 0.f;
+#pragma unroll
+  for (int k = 0; k < NGLL; k++) {
+    const float hi = s_hp[i * NGLL + k], hj = s_hp[j * NGLL + k];
+    const int o1 = base + j * NGLL + k, o3 = base + k * NGLL + i;
+    t1x += s_ux[o1] * hi;
+    t1z += s_uz[o1] * hi;
+    b1x += s_bx[o1] * hi;
+    b1z += s_bz[o1] * hi;
+    t3x += s_ux[o3] * hj;
+    t3z += s_uz[o3] * hj;
+    b3x += s_bx[o3] * hj;
+    b3z += s_bz[o3] * hj;
+  }
+  const float xl = xix[l], xzl = xiz[l], gxl = gammax[l], gzl = gammaz[l];
+  const float duxdxl = xl * t1x + gxl * t3x, duxdzl = xzl * t1x + gzl * t3x;
+  const float duzdxl = xl * t1z + gxl * t3z, duzdzl = xzl * t1z + gzl * t3z;
+  const float bduxdxl = xl * b1x + gxl * b3x, bduxdzl = xzl * b1x + gzl * b3x;
+  const float bduzdxl = xl * b1z + gxl * b3z, bduzdzl = xzl * b1z + gzl * b3z;
+  const float dsxx = duxdxl, dszz = duzdzl, dsxz = 0.5f * (duzdxl + duxdzl);
+  const float bdsxx = bduxdxl, bdszz = bduzdzl, bdsxz = 0.5f * (bduzdxl + bduxdzl);
+  const float2 a = accel[g], b = b_displ[g];
+  rho_kl[l] += a.x * b.x + a.y * b.y;
+  const float prod = (dsxx + dszz) * (bdsxx + bdszz);
+  mu_kl[l] += dsxx * bdsxx + dszz * bdszz + 2.0f * dsxz * bdsxz - prod / 3.0f;
+  kappa_kl[l] += prod;
+}
+
+// ---------------------------------------------------------------------------
+// max |u|^2 over the points (src/gpu/kernels/get_maximum_vector_kernel.cu:37)
+__global__ __launch_bounds__(256) void k_max_norm(const float2* __restrict__ f, int n, unsigned int* __restrict__ out) {
+  float m = 0.f;
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    const float2 u = f[k];
+    const float v = u.x * u.x + u.y * u.y;
+    m = (v > m || v != v) ? v : m;  // a NaN propagates
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const float o = __shfl_xor(m, off, 64);
+    m = (o > m || o != o) ? o : m;
+  }
+  if ((threadIdx.x & 63) == 0) atomicMax(out, __float_as_uint(m));  // non-negative floats order as uints
+}
+
+inline int blocks_for(long n, int threads, int cap = 1 << 20) {
+  long b = (n + threads - 1) / threads;
+  if (b < 1) b = 1;
+  if (b > cap) b = cap;
+  return (int)b;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// launchers
+void launch_newmark_predictor(float* d, float* v, float* a, int nglob, float dt, float dt2o2, float dto2,
+                              hipStream_t s) {
+  if (nglob <= 0) return;
+  hipLaunchKernelGGL(k_predictor, dim3(blocks_for(nglob, 256, 65536)), dim3(256), 0, s, (float2*)d, (float2*)v,
+                     (float2*)a, nglob, dt, dt2o2, dto2);
+  check_hip(hipGetLastError(), "k_predictor launch");
+}
+
+void launch_newmark_corrector(float* v, float* a, const float* rmx, const float* rmz, int nglob, float dto2,
+                              int apply_mass, hipStream_t s) {
+  if (nglob <= 0) return;
+  hipLaunchKernelGGL(k_corrector, dim3(blocks_for(nglob, 256, 65536)), dim3(256), 0, s, (float2*)v, (float2*)a, rmx,
+                     rmz, nglob, dto2, apply_mass);
+  check_hip(hipGetLastError(), "k_corrector launch");
+}
+
+void launch_elastic_forces(const Mesh* mp, const int* elist, int nelem, const float* displ, float* accel) {
+  if (nelem <= 0) return;
+  const int nbatch = (nelem + FORCE_EPB - 1) / FORCE_EPB;
+  const int per_xcd = (nbatch + NXCD - 1) / NXCD;
+  const int nblocks = per_xcd * NXCD;
+  hipLaunchKernelGGL(k_elastic_forces, dim3(nblocks), dim3(FORCE_THREADS), 0, mp->stream, nelem, nbatch, per_xcd,
+                     elist, mp->d_ibool, mp->d_xix, mp->d_xiz, mp->d_gammax, mp->d_gammaz, mp->d_kappav, mp->d_muv,
+                     (const float2*)displ, accel, mp->basis);
+  check_hip(hipGetLastError(), "k_elastic_forces launch");
+}
+
+void launch_stacey(const Mesh* mp, const float* veloc, float* accel, int write_store) {
+  if (mp->num_abs <= 0) return;
+  hipLaunchKernelGGL(k_stacey, dim3(blocks_for((long)mp->num_abs * NGLL, 64)), dim3(64), 0, mp->stream, mp->num_abs,
+                     mp->d_abs_ispec, mp->d_abs_ij, mp->d_abs_normal, mp->d_abs_jacw, mp->d_edge_abs, mp->d_ib,
+                     mp->d_ibool, mp->d_ispec_is_elastic, mp->d_rho_vp, mp->d_rho_vs, (const float2*)veloc, accel,
+                     write_store, mp->d_b_absorb[1], mp->d_b_absorb[2], mp->d_b_absorb[3], mp->d_b_absorb[4]);
+  check_hip(hipGetLastError(), "k_stacey launch");
+}
+
+void launch_stacey_backward(const Mesh* mp, float* b_accel) {
+  if (mp->num_abs <= 0) return;
+  hipLaunchKernelGGL(k_stacey_backward, dim3(blocks_for((long)mp->num_abs * NGLL, 64)), dim3(64), 0, mp->stream,
+                     mp->num_abs, mp->d_abs_ispec, mp->d_abs_ij, mp->d_edge_abs, mp->d_ib, mp->d_ibool,
+                     mp->d_ispec_is_elastic, b_accel, mp->d_b_absorb[1], mp->d_b_absorb[2], mp->d_b_absorb[3],
+                     mp->d_b_absorb[4]);
+  check_hip(hipGetLastError(), "k_stacey_backward launch");
+}
+
+void launch_sources(const Mesh* mp, float* accel, int it) {
+  if (mp->nsources_local <= 0) return;
+  hipLaunchKernelGGL(k_sources, dim3(mp->nsources_local), dim3(64), 0, mp->stream, mp->nsources_local, it,
+                     mp->d_ispec_selected_source, mp->d_ispec_is_elastic, mp->d_ibool, mp->d_sourcearrays, mp->d_stf,
+                     accel);
+  check_hip(hipGetLastError(), "k_sources launch");
+}
+
+void launch_adjoint_sources(const Mesh* mp, float* accel, int it) {
+  if (mp->nadj_rec_local <= 0) return;
+  hipLaunchKernelGGL(k_adjoint_sources, dim3(mp->nadj_rec_local), dim3(64), 0, mp->stream, mp->nadj_rec_local, it,
+                     mp->nstep, mp->d_ispec_selected_rec, mp->d_ispec_is_elastic, mp->d_ibool, mp->d_xir,
+                     mp->d_gammar, mp->d_source_adjoint, accel);
+  check_hip(hipGetLastError(), "k_adjoint_sources launch");
+}
+
+void launch_seismograms(const Mesh* mp, const float* field, float* dst, int seismo_current) {
+  if (mp->nrec_local <= 0) return;
+  hipLaunchKernelGGL(k_seismograms, dim3(mp->nrec_local), dim3(64), 0, mp->stream, mp->nrec_local,
+                     mp->nlength_seismogram, seismo_current, mp->d_ispec_selected_rec, mp->d_ibool, mp->d_xir,
+                     mp->d_gammar, mp->d_cosrot, mp->d_sinrot, (const float2*)field, dst);
+  check_hip(hipGetLastError(), "k_seismograms launch");
+}
+
+void launch_kernels(const Mesh* mp) {
+  if (mp->nspec <= 0) return;
+  const int nblocks = (mp->nspec + FORCE_EPB - 1) / FORCE_EPB;
+  hipLaunchKernelGGL(k_kernels, dim3(nblocks), dim3(FORCE_THREADS), 0, mp->stream, mp->nspec, mp->d_ibool,
+                     mp->d_ispec_is_elastic, mp->d_xix, mp->d_xiz, mp->d_gammax, mp->d_gammaz,
+                     (const float2*)mp->d_displ, (const float2*)mp->d_accel, (const float2*)mp->d_b_displ,
+                     mp->d_rho_kl, mp->d_mu_kl, mp->d_kappa_kl, mp->basis);
+  check_hip(hipGetLastError(), "k_kernels launch");
+}
+
+void launch_max_norm(const Mesh* mp, const float* field, unsigned int* result) {
+  hipLaunchKernelGGL(k_max_norm, dim3(blocks_for(mp->nglob, 256, 4096)), dim3(256), 0, mp->stream,
+                     (const float2*)field, mp->nglob, result);
+  check_hip(hipGetLastError(), "k_max_norm launch");
+}
+
+}  // namespace sf2d

@@ -1,0 +1,261 @@
+"""Python handle on the host driver and the HIP library (ctypes, no torch types).
+
+``Simulation(cfg)`` builds the solver setup in the C++ driver (``libspecfem2d_host.so``) and,
+on a GPU, hands it to the Fortran-style C-ABI of ``libspecfem2d_gpu.so``.  There is no CPU
+fallback: if the libraries are missing or no HIP device is present the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+
+from .config import NGLL2, SimulationConfig
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+GPU_LIB = os.path.join(_HERE, "libspecfem2d_gpu.so")
+HOST_LIB = os.path.join(_HERE, "libspecfem2d_host.so")
+MAXLAYERS = 32
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int), ("nz", C.c_int), ("nlayers", C.c_int), ("nz_layer", C.c_int * MAXLAYERS),
+        ("xmin", C.c_double), ("xmax", C.c_double), ("zinterface", C.c_double * (MAXLAYERS + 1)),
+        ("absorbbottom", C.c_int), ("absorbright", C.c_int), ("absorbtop", C.c_int), ("absorbleft", C.c_int),
+        ("stacey", C.c_int), ("nstep", C.c_int), ("dt", C.c_double), ("user_t0", C.c_double),
+        ("anglerec", C.c_double), ("material_mode", C.c_int),
+    ]
+
+
+class SourceC(C.Structure):
+    _fields_ = [
+        ("x", C.c_double), ("z", C.c_double), ("f0", C.c_double), ("factor", C.c_double),
+        ("anglesource", C.c_double), ("tshift", C.c_double), ("time_function_type", C.c_int), ("pad", C.c_int),
+    ]
+
+
+_lib_cache = {}
+
+
+def load_libraries():
+    """Load the HIP C-ABI library and the host driver; raise if they are not built."""
+    if "host" in _lib_cache:
+        return _lib_cache["gpu"], _lib_cache["host"]
+    for path in (GPU_LIB, HOST_LIB):
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`"
+                               " (make -C specfem2d_amd)")
+    gpu = C.CDLL(GPU_LIB, mode=C.RTLD_GLOBAL)
+    host = C.CDLL(HOST_LIB)
+    P = C.c_void_p
+    dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+    host.sf2d_sim_create.restype = P
+    host.sf2d_sim_create.argtypes = [C.POINTER(Params), dp, dp, dp, C.c_int, C.POINTER(SourceC), C.c_int, dp,
+                                     C.c_int, C.c_int]
+    host.sf2d_sim_prepare_device.argtypes = [P]
+    host.sf2d_sim_prepare_device.restype = C.c_int
+    host.sf2d_sim_destroy.argtypes = [P]
+    for f in ("sf2d_sim_nspec", "sf2d_sim_nglob", "sf2d_sim_nabs"):
+        getattr(host, f).argtypes = [P]
+        getattr(host, f).restype = C.c_int
+    host.sf2d_sim_t0.argtypes = [P]
+    host.sf2d_sim_t0.restype = C.c_double
+    host.sf2d_sim_mesh_pointer.argtypes = [P]
+    host.sf2d_sim_mesh_pointer.restype = C.c_long
+    for f in ("sf2d_sim_get_ibool", "sf2d_sim_get_rmass_inverse"):
+        getattr(host, f).argtypes = [P, P]
+    host.sf2d_sim_get_elem_array.argtypes = [P, C.c_int, P]
+    host.sf2d_sim_get_sources.argtypes = [P, P, P, P, P, P]
+    host.sf2d_sim_get_receivers.argtypes = [P, P, P, P, P, P]
+    host.sf2d_sim_get_stacey.argtypes = [P, P, P, P, P, P, P]
+    host.sf2d_sim_forward.argtypes = [P, C.c_int, P, P, P]
+    host.sf2d_sim_forward.restype = C.c_int
+    host.sf2d_sim_reset.argtypes = [P]
+    host.sf2d_sim_steps_done.argtypes = [P]
+    host.sf2d_sim_steps_done.restype = C.c_int
+    for f in ("sf2d_sim_get_fields", "sf2d_sim_set_fields", "sf2d_sim_get_b_fields", "sf2d_sim_set_b_fields"):
+        getattr(host, f).argtypes = [P, P, P, P]
+    gpu.sf2d_mesh_stream.argtypes = [C.POINTER(C.c_long)]
+    gpu.sf2d_mesh_stream.restype = P
+    gpu.sf2d_mesh_field.argtypes = [C.POINTER(C.c_long), C.c_int]
+    gpu.sf2d_mesh_field.restype = P
+    _lib_cache["gpu"], _lib_cache["host"] = gpu, host
+    return gpu, host
+
+
+def _ptr(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def params_from_config(cfg: SimulationConfig) -> Params:
+    p = Params()
+    p.nx, p.nz = cfg.nx, cfg.nz
+    p.nlayers = len(cfg.nz_layers)
+    if p.nlayers > MAXLAYERS:
+        raise ValueError("too many layers")
+    for i, n in enumerate(cfg.nz_layers):
+        p.nz_layer[i] = int(n)
+    for i, z in enumerate(cfg.interfaces):
+        p.zinterface[i] = float(z)
+    p.xmin, p.xmax = float(cfg.xmin), float(cfg.xmax)
+    p.absorbbottom, p.absorbright = int(cfg.absorbbottom), int(cfg.absorbright)
+    p.absorbtop, p.absorbleft = int(cfg.absorbtop), int(cfg.absorbleft)
+    p.stacey = int(cfg.stacey)
+    p.nstep, p.dt = int(cfg.nstep), float(cfg.dt)
+    p.user_t0, p.anglerec = float(cfg.user_t0), float(cfg.anglerec)
+    p.material_mode = int(cfg.material_mode)
+    return p
+
+
+class Simulation:
+    """An elastic simulation driven through the GPU C-ABI.
+
+    ``device=False`` builds only the host setup (usable without a GPU, for parity checks of
+    the setup); ``device=True`` also prepares the GPU and enables the time loop."""
+
+    ELEM = {"xix": 0, "xiz": 1, "gammax": 2, "gammaz": 3, "jacobian": 4, "kappa": 5, "mu": 6,
+            "rho_vp": 7, "rho_vs": 8, "rho": 9}
+
+    def __init__(self, cfg: SimulationConfig, simulation_type: int = 1, save_forward: bool = False,
+                 device: bool = True):
+        self.cfg = cfg
+        self.gpu, self.host = load_libraries()
+        rho, vp, vs = cfg.material_arrays()
+        srcs = (SourceC * max(1, len(cfg.sources)))()
+        for k, s in enumerate(cfg.sources):
+            srcs[k].x, srcs[k].z, srcs[k].f0, srcs[k].factor = s.x, s.z, s.f0, s.factor
+            srcs[k].anglesource, srcs[k].tshift, srcs[k].time_function_type = s.anglesource, s.tshift, s.time_function_type
+        rec = np.ascontiguousarray(np.asarray(cfg.receivers if cfg.nrec else np.zeros((1, 2)), dtype=np.float64))
+        p = params_from_config(cfg)
+        self._keep = (p, rho, vp, vs, srcs, rec)
+        self.h = self.host.sf2d_sim_create(C.byref(p), rho, vp, vs, len(cfg.sources), srcs, cfg.nrec, rec,
+                                           int(simulation_type), int(save_forward))
+        if not self.h:
+            raise ValueError("invalid simulation parameters")
+        self.simulation_type = simulation_type
+        self.save_forward = save_forward
+        self.nspec = self.host.sf2d_sim_nspec(self.h)
+        self.nglob = self.host.sf2d_sim_nglob(self.h)
+        self.nabs = self.host.sf2d_sim_nabs(self.h)
+        self.t0 = self.host.sf2d_sim_t0(self.h)
+        self.on_device = False
+        if device:
+            self.prepare_device()
+
+    def prepare_device(self):
+        if self.host.sf2d_sim_prepare_device(self.h) != 0:
+            raise RuntimeError("device preparation failed")
+        self.on_device = True
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.host.sf2d_sim_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------------
+    @property
+    def mesh_pointer(self) -> int:
+        return self.host.sf2d_sim_mesh_pointer(self.h)
+
+    def stream(self) -> int:
+        mp = C.c_long(self.mesh_pointer)
+        return self.gpu.sf2d_mesh_stream(C.byref(mp))
+
+    def device_field(self, which: int) -> int:
+        mp = C.c_long(self.mesh_pointer)
+        return self.gpu.sf2d_mesh_field(C.byref(mp), int(which)) or 0
+
+    # setup products -----------------------------------------------------
+    def ibool(self):
+        out = np.zeros(self.nspec * NGLL2, np.int32)
+        self.host.sf2d_sim_get_ibool(self.h, _ptr(out))
+        return out.reshape(self.nspec, NGLL2)
+
+    def rmass_inverse(self):
+        out = np.zeros(2 * self.nglob, np.float32)
+        self.host.sf2d_sim_get_rmass_inverse(self.h, _ptr(out))
+        return out.reshape(self.nglob, 2)
+
+    def elem_array(self, name):
+        out = np.zeros(self.nspec * NGLL2, np.float32)
+        self.host.sf2d_sim_get_elem_array(self.h, self.ELEM[name], _ptr(out))
+        return out.reshape(self.nspec, NGLL2)
+
+    def sources(self):
+        n = max(1, len(self.cfg.sources))
+        isp, xi, ga = np.zeros(n, np.int32), np.zeros(n), np.zeros(n)
+        sa = np.zeros(n * NGLL2 * 2, np.float32)
+        stf = np.zeros(self.cfg.nstep * n, np.float32)
+        self.host.sf2d_sim_get_sources(self.h, _ptr(isp), _ptr(xi), _ptr(ga), _ptr(sa), _ptr(stf))
+        k = len(self.cfg.sources)
+        return isp[:k], xi[:k], ga[:k], sa.reshape(n, NGLL2, 2)[:k], stf.reshape(self.cfg.nstep, n)[:, :k].T
+
+    def receivers(self):
+        n = max(1, self.cfg.nrec)
+        isp, xi, ga = np.zeros(n, np.int32), np.zeros(n), np.zeros(n)
+        xir, gar = np.zeros(5 * n, np.float32), np.zeros(5 * n, np.float32)
+        self.host.sf2d_sim_get_receivers(self.h, _ptr(isp), _ptr(xi), _ptr(ga), _ptr(xir), _ptr(gar))
+        k = self.cfg.nrec
+        return isp[:k], xi[:k], ga[:k], xir.reshape(n, 5)[:k], gar.reshape(n, 5)[:k]
+
+    def stacey(self):
+        n = max(1, self.nabs)
+        isp, ij = np.zeros(n, np.int32), np.zeros(n * 10, np.int32)
+        nrm, jw = np.zeros(n * 10, np.float32), np.zeros(n * 5, np.float32)
+        edge, ib = np.zeros(n, np.int32), np.zeros(n, np.int32)
+        self.host.sf2d_sim_get_stacey(self.h, _ptr(isp), _ptr(ij), _ptr(nrm), _ptr(jw), _ptr(edge), _ptr(ib))
+        k = self.nabs
+        return isp[:k], ij.reshape(n, 5, 2)[:k], nrm.reshape(n, 5, 2)[:k], jw.reshape(n, 5)[:k], edge[:k], ib[:k]
+
+    # time loop ----------------------------------------------------------
+    def _need_device(self):
+        if not self.on_device:
+            raise RuntimeError("simulation is not prepared on a device")
+
+    def reset(self):
+        """Zero the wavefields and restart at step 1."""
+        self._need_device()
+        self.host.sf2d_sim_reset(self.h)
+
+    @property
+    def steps_done(self) -> int:
+        return self.host.sf2d_sim_steps_done(self.h)
+
+    def forward(self, nstep: Optional[int] = None, save_stacey: bool = False, phase_timing: bool = False):
+        """Runs the next ``nstep`` steps of the GPU time loop (default: the remaining ones);
+        returns (seismograms [nrec,2,nstep], timing dict[, stacey storage])."""
+        self._need_device()
+        n = (self.cfg.nstep - self.steps_done) if nstep is None else int(nstep)
+        seis = np.zeros((max(1, self.cfg.nrec), 2, n))
+        store = np.zeros((n, max(1, self.nabs), 5, 2), np.float32) if save_stacey else None
+        timing = np.zeros(6)
+        if phase_timing:
+            timing[0] = -1.0
+        rc = self.host.sf2d_sim_forward(self.h, n, _ptr(seis), _ptr(store), _ptr(timing))
+        if rc != 0:
+            raise RuntimeError("sf2d_sim_forward failed")
+        t = dict(zip(("loop_ms", "predictor_ms", "forces_ms", "boundary_sources_ms", "corrector_ms",
+                      "seismograms_ms"), timing.tolist()))
+        seis = seis[: self.cfg.nrec]
+        return (seis, t, store) if save_stacey else (seis, t)
+
+    def fields(self):
+        self._need_device()
+        d, v, a = (np.zeros(2 * self.nglob, np.float32) for _ in range(3))
+        self.host.sf2d_sim_get_fields(self.h, _ptr(d), _ptr(v), _ptr(a))
+        return d.reshape(-1, 2), v.reshape(-1, 2), a.reshape(-1, 2)
+
+    def set_fields(self, displ=None, veloc=None, accel=None):
+        self._need_device()
+        arrs = [None if x is None else np.ascontiguousarray(np.asarray(x, np.float32).reshape(-1))
+                for x in (displ, veloc, accel)]
+        self.host.sf2d_sim_set_fields(self.h, *[_ptr(x) for x in arrs])

@@ -1,0 +1,105 @@
+"""Parity of the HIP path (called through the Fortran-style C-ABI by the host driver) with the CPU
+oracle and with the reference's committed traces.
+
+Tolerance: FP32 seismograms within rel-L2 1e-5 of the FP32 oracle (the reference's own spread across
+precision, MPI partitioning and OpenMP is <= 2.6e-6); GPU atomics and FMA contraction change the
+rounding, not the scheme.  The error against the FP64 oracle is reported alongside.
+"""
+import numpy as np
+import pytest
+
+from oracle.oracle import OracleSim
+from specfem2d_amd import config
+from specfem2d_amd.driver import Simulation
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-5
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+
+
+def test_tromp2005_against_reference_traces(golden_dir):
+    cfg = config.tromp2005_kernel()
+    sim = Simulation(cfg)
+    seis, _ = sim.forward()
+    g = np.load(f"{golden_dir}/tromp2005_kernel_ref_seis.npz")
+    for c, comp in enumerate(("BXX", "BXZ")):
+        r = rel(seis[0, c], g[f"AA.S0001.{comp}"])
+        print(f"Tromp2005 {comp}: rel-L2 vs REF_SEIS = {r:.3e}")
+        assert r < TOL
+    dp = OracleSim(cfg, "dp").forward()
+    print(f"Tromp2005: rel-L2 vs FP64 oracle = {rel(seis, dp):.3e}")
+    sim.close()
+
+
+def test_plain_elastic_80x60_against_oracle():
+    cfg = config.plain_elastic_c1()
+    sim = Simulation(cfg)
+    seis, t = sim.forward()
+    ref = OracleSim(cfg, "sp").forward()
+    r = rel(seis, ref)
+    print(f"80x60 plain elastic, 1600 steps, 20 receivers: rel-L2 vs FP32 oracle = {r:.3e}, loop {t['loop_ms']:.1f} ms")
+    assert r < TOL
+    for k in range(cfg.nrec):
+        assert rel(seis[k], ref[k]) < 5 * TOL
+    sim.close()
+
+
+@pytest.mark.parametrize("case", ["gll_random", "layers_all_absorbing", "free_boundaries"])
+def test_small_meshes_against_oracle(case):
+    if case == "gll_random":
+        cfg = config.small_test_mesh(nx=14, nz=11, nstep=600, gll=True, seed=5)
+    elif case == "layers_all_absorbing":
+        cfg = config.small_test_mesh(nx=13, nz=10, nstep=600).replace(
+            interfaces=[-1000.0, -620.0, -330.0, 0.0], nz_layers=[3, 4, 3], xmax=1390.0, absorbtop=True)
+    else:
+        cfg = config.small_test_mesh(nx=10, nz=8, nstep=500, stacey=False)
+    sim = Simulation(cfg)
+    seis, _ = sim.forward()
+    orc = OracleSim(cfg, "sp")
+    ref = orc.forward()
+    r = rel(seis, ref)
+    print(f"{case}: rel-L2 vs FP32 oracle = {r:.3e}")
+    assert r < TOL
+    d, v, a = sim.fields()
+    od, ov, oa = orc.fields()
+    assert rel(d, od) < TOL and rel(v, ov) < TOL
+    sim.close()
+
+
+def test_save_forward_stacey_storage():
+    cfg = config.small_test_mesh(nx=9, nz=7, nstep=300)
+    sim = Simulation(cfg, save_forward=True)
+    seis, _, store = sim.forward(save_stacey=True)
+    _, ref_store = OracleSim(cfg, "sp").forward(save_stacey=True)
+    assert store.shape == ref_store.shape
+    assert rel(store, ref_store) < TOL
+    sim.close()
+
+
+def test_partial_run_then_continue_matches_full_run():
+    cfg = config.small_test_mesh(nx=8, nz=6, nstep=200)
+    a = Simulation(cfg)
+    full, _ = a.forward()
+    d, v, acc = a.fields()
+    b = Simulation(cfg.replace(nstep=200))
+    part, _ = b.forward(nstep=120)
+    np.testing.assert_array_equal(part, full[:, :, :120])
+    a.close()
+    b.close()
+
+
+def test_max_norm_entry_point():
+    import ctypes
+    cfg = config.small_test_mesh(nx=8, nz=6, nstep=150)
+    sim = Simulation(cfg)
+    sim.forward()
+    d, _, _ = sim.fields()
+    norm = ctypes.c_float(0)
+    mp = ctypes.c_long(sim.mesh_pointer)
+    fwd = ctypes.c_int(1)
+    sim.gpu.get_norm_elastic_from_device_(ctypes.byref(norm), ctypes.byref(mp), ctypes.byref(fwd))
+    assert norm.value == pytest.approx(float(np.sqrt((d.astype(np.float64) ** 2).sum(axis=1)).max()), rel=1e-6)
+    sim.close()

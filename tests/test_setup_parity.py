@@ -1,0 +1,82 @@
+"""The host driver's setup (what xspecfem2D hands to prepare_constants_device_ & co.) against the
+oracle: integer/index data bit-exact, FP32 arrays to a few ulp.  Runs on the CPU (no device call)."""
+import numpy as np
+import pytest
+
+from oracle.oracle import OracleSim
+from specfem2d_amd import config
+from specfem2d_amd.driver import Simulation
+
+CASES = {
+    "tromp2005": lambda: config.tromp2005_kernel(nstep=50),
+    "c1_80x60": lambda: config.plain_elastic_c1(nstep=50),
+    "small_gll": lambda: config.small_test_mesh(nx=9, nz=7, nstep=50, gll=True),
+    "layers": lambda: config.small_test_mesh(nx=6, nz=5, nstep=50).replace(
+        interfaces=[-500.0, -320.0, -90.0, 0.0], nz_layers=[2, 2, 1], xmax=730.0, absorbtop=True),
+}
+
+
+@pytest.fixture(scope="module", params=sorted(CASES))
+def pair(request):
+    cfg = CASES[request.param]()
+    return cfg, Simulation(cfg, device=False), OracleSim(cfg, "sp")
+
+
+def test_ibool_bit_exact(pair):
+    cfg, sim, orc = pair
+    np.testing.assert_array_equal(sim.ibool(), orc.ibool())
+    assert sim.nglob == orc.nglob == cfg.nglob
+
+
+def test_geometry_and_materials(pair):
+    _, sim, orc = pair
+    for name in ("xix", "xiz", "gammax", "gammaz", "jacobian", "mu", "kappa", "rho_vp", "rho_vs"):
+        a, b = sim.elem_array(name), orc.elem_array(name).astype(np.float32)
+        scale = np.abs(b).max()
+        np.testing.assert_allclose(a, b, rtol=4e-7, atol=1e-6 * scale, err_msg=name)
+
+
+def test_mass_matrix(pair):
+    _, sim, orc = pair
+    np.testing.assert_allclose(sim.rmass_inverse(), orc.rmass_inverse(), rtol=2e-6)
+
+
+def test_sources_and_receivers_bit_exact_indices(pair):
+    cfg, sim, orc = pair
+    si, sx, sg, sa, stf = sim.sources()
+    oi, ox, og, osa, ostf = orc.sources()
+    np.testing.assert_array_equal(si, oi)
+    np.testing.assert_allclose(sx, ox, atol=1e-12)
+    np.testing.assert_allclose(sg, og, atol=1e-12)
+    np.testing.assert_allclose(sa, osa, rtol=1e-6, atol=1e-7)
+    np.testing.assert_allclose(stf, ostf, rtol=1e-6, atol=1e-6 * np.abs(ostf).max())
+    assert sim.t0 == orc.t0
+    ri, rx, rg, rxi, rga = sim.receivers()
+    qi, qx, qg, qxi, qga = orc.receivers()
+    np.testing.assert_array_equal(ri, qi)
+    np.testing.assert_allclose(rx, qx, atol=1e-12)
+    np.testing.assert_allclose(rg, qg, atol=1e-12)
+    np.testing.assert_allclose(rxi, qxi, atol=1e-7)
+    np.testing.assert_allclose(rga, qga, atol=1e-7)
+
+
+def test_stacey_arrays(pair):
+    cfg, sim, orc = pair
+    isp, ij, nrm, jw, edge, ib = sim.stacey()
+    oisp, ocode, ocorner, oib = orc.absorbing()
+    np.testing.assert_array_equal(isp, oisp)
+    np.testing.assert_array_equal(edge, ocode)
+    np.testing.assert_array_equal(ib, oib)
+    # skipped corner points carry NGLL+1, exactly where the oracle's corner flags say
+    for f in range(len(isp)):
+        skip = ij[f, :, 0] == 6
+        exp = np.zeros(5, bool)
+        if edge[f] == 1:
+            exp[0], exp[4] = ocorner[f, 0], ocorner[f, 1]
+        if edge[f] == 3:
+            exp[0], exp[4] = ocorner[f, 2], ocorner[f, 3]
+        np.testing.assert_array_equal(skip, exp)
+    # axis-aligned edges: outward unit normals
+    expect = {1: (0, -1), 2: (1, 0), 3: (0, 1), 4: (-1, 0)}
+    for f in range(len(isp)):
+        np.testing.assert_allclose(nrm[f], np.tile(expect[edge[f]], (5, 1)), atol=1e-6)
