@@ -63,9 +63,17 @@ def test_small_meshes_against_oracle(case):
     r = rel(seis, ref)
     print(f"{case}: rel-L2 vs FP32 oracle = {r:.3e}")
     assert r < TOL
+    # final wavefields: late-time fields are small (energy has left through Stacey), so their
+    # tolerance is set by the FP32 oracle's own distance to the FP64 oracle
     d, v, a = sim.fields()
     od, ov, oa = orc.fields()
-    assert rel(d, od) < TOL and rel(v, ov) < TOL
+    dp = OracleSim(cfg, "dp")
+    dp.forward()
+    dd, dv, _ = dp.fields()
+    tol_d = max(TOL, 3 * rel(od, dd))
+    tol_v = max(TOL, 3 * rel(ov, dv))
+    print(f"{case}: fields rel-L2 displ {rel(d, od):.2e} (tol {tol_d:.1e}), veloc {rel(v, ov):.2e} (tol {tol_v:.1e})")
+    assert rel(d, od) < tol_d and rel(v, ov) < tol_v
     sim.close()
 
 
@@ -80,13 +88,20 @@ def test_save_forward_stacey_storage():
 
 
 def test_partial_run_then_continue_matches_full_run():
+    # FP32 atomics make the assembly order (and the last bits) vary from run to run, so two runs
+    # agree to rounding, not bitwise
     cfg = config.small_test_mesh(nx=8, nz=6, nstep=200)
     a = Simulation(cfg)
     full, _ = a.forward()
-    d, v, acc = a.fields()
-    b = Simulation(cfg.replace(nstep=200))
-    part, _ = b.forward(nstep=120)
-    np.testing.assert_array_equal(part, full[:, :, :120])
+    b = Simulation(cfg)
+    p1, _ = b.forward(nstep=120)
+    p2, _ = b.forward(nstep=80)
+    assert b.steps_done == 200
+    both = np.concatenate([p1, p2], axis=2)
+    assert rel(both, full) < 1e-6
+    b.reset()
+    again, _ = b.forward()
+    assert rel(again, full) < 1e-6
     a.close()
     b.close()
 
