@@ -74,6 +74,8 @@ def _lib(precision: str):
         lib.orc_apply_stiffness.argtypes = [P, dp, dp]
         lib.orc_real_size.restype = C.c_int
         lib.orc_set_ftz.argtypes = [P, C.c_int]
+        lib.orc_adjoint.argtypes = [P, dp, dp, dp, dp, dp, C.c_void_p, dp, C.c_int, C.c_void_p, C.c_void_p]
+        lib.orc_kernel_increment.argtypes = [P, dp, dp, dp, dp]
         _LIBS[precision] = lib
     return _LIBS[precision]
 
@@ -203,3 +205,33 @@ class OracleSim:
         out = np.zeros_like(u)
         self.lib.orc_apply_stiffness(self.h, u, out)
         return out.reshape(-1, 2)
+
+    def adjoint(self, adj, stacey_store, lastframe, dump_steps=()):
+        """SIMULATION_TYPE=3 run with backward reconstruction.  ``adj`` [nrec,2,nstep] adjoint
+        source traces, ``stacey_store`` from a SAVE_FORWARD run, ``lastframe`` = (displ, veloc, accel)
+        at the end of that run.  Returns (adjoint seismograms [nrec,2,nstep], kernels dict of
+        [nspec,25] arrays rho/mu/kappa, scaled as save_adjoint_kernels.f90 writes them)."""
+        n = self.cfg.nstep
+        adj = np.ascontiguousarray(np.asarray(adj, np.float64).reshape(-1))
+        st = np.ascontiguousarray(np.asarray(stacey_store, np.float64).reshape(-1))
+        if st.size == 0:
+            st = np.zeros(1)
+        d, v, a = (np.ascontiguousarray(np.asarray(x, np.float64).reshape(-1)) for x in lastframe)
+        seis = np.zeros((max(1, self.cfg.nrec), 2, n))
+        ker = np.zeros(3 * self.nspec * NGLL2)
+        steps = np.ascontiguousarray(np.asarray(dump_steps, np.int32))
+        dumps = np.zeros((max(1, len(steps)), 2 * self.nglob))
+        self.lib.orc_adjoint(self.h, adj, st, d, v, a, _ptr(seis), ker, len(steps), _ptr(steps), _ptr(dumps))
+        ker = ker.reshape(3, self.nspec, NGLL2)
+        out = (seis[: self.cfg.nrec], {"rho": ker[0], "mu": ker[1], "kappa": ker[2]})
+        if len(steps):
+            return out + (dumps[: len(steps)].reshape(len(steps), -1, 2),)
+        return out
+
+    def kernel_increment(self, displ, accel, b_displ):
+        """One unscaled kernel accumulation (compute_kernels_el) for given fields."""
+        args = [np.ascontiguousarray(np.asarray(x, np.float64).reshape(-1)) for x in (displ, accel, b_displ)]
+        out = np.zeros(3 * self.nspec * NGLL2)
+        self.lib.orc_kernel_increment(self.h, *args, out)
+        out = out.reshape(3, self.nspec, NGLL2)
+        return {"rho": out[0], "mu": out[1], "kappa": out[2]}

@@ -98,6 +98,7 @@ typedef struct {
   /* wavefields */
   REAL *displ, *veloc, *accel;/* [nglob][2] */
   int ftz;                    /* run the time loop with flush-to-zero */
+  int it_done;                /* forward steps already run */
 } orc_sim;
 
 /* ------------------------------------------------------------------ */
@@ -877,11 +878,12 @@ void orc_get_receivers(void *h, int *ispec, double *xi, double *gamma, double *x
 }
 
 /* Forward time loop (iterate_time.F90:146-414 for SIMULATION_TYPE=1, Newmark).
-   Runs nstep_run steps (<= nstep) from the current state and records displacement
+   Runs the next nstep_run steps (continuing after the steps already run, up to NSTEP) and records displacement
    seismograms into seis[nrec][2][nstep_run].  If stacey_store is not NULL it
    receives the Stacey traction*weight per step [nstep_run][nabs][5][2]. */
 void orc_forward(void *h, int nstep_run, double *seis, double *stacey_store_d) {
   orc_sim *s = (orc_sim *)h;
+  if (nstep_run < 1 || s->it_done + nstep_run > s->p.nstep) return;
   unsigned int csr = _mm_getcsr();
   if (s->ftz) _mm_setcsr(csr | 0x8040); /* flush-to-zero + denormals-are-zero */
   REAL deltat = (REAL)s->p.dt;
@@ -891,7 +893,8 @@ void orc_forward(void *h, int nstep_run, double *seis, double *stacey_store_d) {
   REAL *d = s->displ, *v = s->veloc, *a = s->accel;
   REAL *store = NULL;
   if (stacey_store_d && s->nabs > 0) store = (REAL *)calloc((size_t)nstep_run * s->nabs * NGLL * 2, sizeof(REAL));
-  for (int it = 1; it <= nstep_run; it++) {
+  for (int step = 0; step < nstep_run; step++) {
+    const int it = s->it_done + step + 1;
     /* update_displacement_newmark_elastic (update_displacement_Newmark.F90:408-454) */
     for (size_t k = 0; k < n2; k++) {
       d[k] = d[k] + deltat * v[k] + deltatsqover2 * a[k];
@@ -899,14 +902,15 @@ void orc_forward(void *h, int nstep_run, double *seis, double *stacey_store_d) {
       a[k] = (REAL)0;
     }
     /* compute_forces_viscoelastic_main: iphase 1 (Stacey, sources), iphase 2 (elements) */
-    if (s->p.stacey && s->nabs > 0) stacey_forces(s, v, a, store, it - 1, nstep_run);
+    if (s->p.stacey && s->nabs > 0) stacey_forces(s, v, a, store, step, nstep_run);
     add_sources(s, a, it);
     element_forces(s, d, a);
     for (size_t k = 0; k < n2; k++) a[k] = a[k] * s->rmass_inv[k];
     /* update_veloc_elastic_Newmark */
     for (size_t k = 0; k < n2; k++) v[k] = v[k] + deltatover2 * a[k];
-    if (seis && s->nrec > 0) record_seismograms(s, d, seis, it - 1, nstep_run);
+    if (seis && s->nrec > 0) record_seismograms(s, d, seis, step, nstep_run);
   }
+  s->it_done += nstep_run;
   if (store) {
     size_t n = (size_t)nstep_run * s->nabs * NGLL * 2;
     for (size_t k = 0; k < n; k++) stacey_store_d[k] = (double)store[k];
@@ -946,4 +950,162 @@ void orc_apply_stiffness(void *h, const double *u, double *out) {
   for (size_t k = 0; k < n2; k++) out[k] = (double)a[k];
   free(d);
   free(a);
+}
+
+
+/* ------------------------------------------------------------------ */
+/* adjoint simulation, SIMULATION_TYPE = 3 with backward reconstruction */
+/* ------------------------------------------------------------------ */
+static void kernels_accumulate(orc_sim *s, const REAL *displ, const REAL *accel, const REAL *b_displ, REAL *rho_kl,
+                               REAL *mu_kl, REAL *kappa_kl) {
+  /* compute_kernels.f90 compute_kernels_el (P-SV, isotropic); the jacobian terms are double there */
+  const REAL (*H)[NGLL] = (const REAL(*)[NGLL])s->hprime;
+  for (int e = 0; e < s->nspec; e++) {
+    const int *ib = s->ibool + (size_t)e * NGLL2;
+    for (int j = 0; j < NGLL; j++)
+      for (int i = 0; i < NGLL; i++) {
+        REAL dux_dxi = 0, duz_dxi = 0, dux_dga = 0, duz_dga = 0;
+        REAL bdux_dxi = 0, bduz_dxi = 0, bdux_dga = 0, bduz_dga = 0;
+        for (int k = 0; k < NGLL; k++) {
+          int g1 = ib[j * NGLL + k] - 1, g2 = ib[k * NGLL + i] - 1;
+          dux_dxi = dux_dxi + displ[2 * g1] * H[i][k];
+          duz_dxi = duz_dxi + displ[2 * g1 + 1] * H[i][k];
+          dux_dga = dux_dga + displ[2 * g2] * H[j][k];
+          duz_dga = duz_dga + displ[2 * g2 + 1] * H[j][k];
+          bdux_dxi = bdux_dxi + b_displ[2 * g1] * H[i][k];
+          bduz_dxi = bduz_dxi + b_displ[2 * g1 + 1] * H[i][k];
+          bdux_dga = bdux_dga + b_displ[2 * g2] * H[j][k];
+          bduz_dga = bduz_dga + b_displ[2 * g2 + 1] * H[j][k];
+        }
+        size_t l = (size_t)e * NGLL2 + j * NGLL + i;
+        double xixl = s->xix[l], xizl = s->xiz[l], gaxl = s->gammax[l], gazl = s->gammaz[l];
+        REAL dux_dxl = (REAL)(dux_dxi * xixl + dux_dga * gaxl);
+        REAL dux_dzl = (REAL)(dux_dxi * xizl + dux_dga * gazl);
+        REAL duz_dxl = (REAL)(duz_dxi * xixl + duz_dga * gaxl);
+        REAL duz_dzl = (REAL)(duz_dxi * xizl + duz_dga * gazl);
+        REAL bdux_dxl = (REAL)(bdux_dxi * xixl + bdux_dga * gaxl);
+        REAL bdux_dzl = (REAL)(bdux_dxi * xizl + bdux_dga * gazl);
+        REAL bduz_dxl = (REAL)(bduz_dxi * xixl + bduz_dga * gaxl);
+        REAL bduz_dzl = (REAL)(bduz_dxi * xizl + bduz_dga * gazl);
+        REAL dsxx = dux_dxl, dsxz = (REAL)0.5 * (duz_dxl + dux_dzl), dszz = duz_dzl;
+        REAL bdsxx = bdux_dxl, bdsxz = (REAL)0.5 * (bduz_dxl + bdux_dzl), bdszz = bduz_dzl;
+        REAL kap = (dsxx + dszz) * (bdsxx + bdszz);
+        REAL mu = dsxx * bdsxx + dszz * bdszz + (REAL)2 * dsxz * bdsxz - (REAL)1 / (REAL)3 * kap;
+        int g = ib[j * NGLL + i] - 1;
+        REAL rho = accel[2 * g] * b_displ[2 * g] + accel[2 * g + 1] * b_displ[2 * g + 1];
+        rho_kl[l] = rho_kl[l] + rho;
+        mu_kl[l] = mu_kl[l] + mu;
+        kappa_kl[l] = kappa_kl[l] + kap;
+      }
+  }
+}
+
+/* Adjoint run (iterate_time.F90:146-414 with SIMULATION_TYPE=3, Newmark, backward reconstruction):
+   adj[nrec][2][nstep] are the adjoint source traces (SEM/ .adj files, component x then z);
+   stacey_store[nstep][nabs][5][2] and the last forward frame (lastframe_elastic) come from the
+   SAVE_FORWARD run.  Records the adjoint displacement at the receivers into seis[nrec][2][nstep] and
+   returns the scaled kernels (save_adjoint_kernels.f90:100-140) in kernels[3][nspec*25] (rho, mu, kappa). */
+void orc_adjoint(void *h, const double *adj, const double *stacey_store, const double *last_d, const double *last_v,
+                 const double *last_a, double *seis, double *kernels, int ndump, const int *dump_steps,
+                 double *b_displ_dump) {
+  orc_sim *s = (orc_sim *)h;
+  unsigned int csr = _mm_getcsr();
+  if (s->ftz) _mm_setcsr(csr | 0x8040);
+  const int nstep = s->p.nstep;
+  REAL deltat = (REAL)s->p.dt;
+  REAL deltatover2 = (REAL)(0.5 * (double)deltat);
+  REAL deltatsqover2 = (REAL)(0.5 * (double)deltat * (double)deltat);
+  REAL b_deltat = -deltat;
+  REAL b_deltatover2 = (REAL)(0.5 * (double)b_deltat);
+  REAL b_deltatsqover2 = (REAL)(0.5 * (double)b_deltat * (double)b_deltat);
+  size_t n2 = (size_t)2 * s->nglob, ne = (size_t)s->nspec * NGLL2;
+  REAL *d = s->displ, *v = s->veloc, *a = s->accel;
+  memset(d, 0, sizeof(REAL) * n2); memset(v, 0, sizeof(REAL) * n2); memset(a, 0, sizeof(REAL) * n2);
+  REAL *bd = (REAL *)calloc(n2, sizeof(REAL)), *bv = (REAL *)calloc(n2, sizeof(REAL)), *ba = (REAL *)calloc(n2, sizeof(REAL));
+  REAL *rho_kl = (REAL *)calloc(ne, sizeof(REAL)), *mu_kl = (REAL *)calloc(ne, sizeof(REAL)), *kappa_kl = (REAL *)calloc(ne, sizeof(REAL));
+  REAL *sadj = (REAL *)calloc((size_t)(s->nrec + 1) * 2 * nstep, sizeof(REAL));
+  for (size_t k = 0; k < (size_t)s->nrec * 2 * nstep; k++) sadj[k] = (REAL)adj[k];
+  for (int it = 1; it <= nstep; it++) {
+    int it_tmp = nstep - it + 1;
+    for (size_t k = 0; k < n2; k++) {
+      d[k] = d[k] + deltat * v[k] + deltatsqover2 * a[k];
+      v[k] = v[k] + deltatover2 * a[k];
+      a[k] = (REAL)0;
+    }
+    for (size_t k = 0; k < n2; k++) {
+      bd[k] = bd[k] + b_deltat * bv[k] + b_deltatsqover2 * ba[k];
+      bv[k] = bv[k] + b_deltatover2 * ba[k];
+      ba[k] = (REAL)0;
+    }
+    /* adjoint wavefield: Stacey, adjoint sources, elements, mass, corrector */
+    if (s->p.stacey && s->nabs > 0) stacey_forces(s, v, a, NULL, 0, 0);
+    for (int r = 0; r < s->nrec; r++) {
+      int e = s->rec_ispec[r] - 1;
+      for (int j = 0; j < NGLL; j++)
+        for (int i = 0; i < NGLL; i++) {
+          int g = s->ibool[e * NGLL2 + j * NGLL + i] - 1;
+          REAL hl = s->xir[r * NGLL + i] * s->gammar[r * NGLL + j];
+          REAL stfx = hl * sadj[((size_t)r * 2 + 0) * nstep + (it_tmp - 1)];
+          REAL stfz = hl * sadj[((size_t)r * 2 + 1) * nstep + (it_tmp - 1)];
+          a[2 * g] = a[2 * g] + stfx;
+          a[2 * g + 1] = a[2 * g + 1] + stfz;
+        }
+    }
+    element_forces(s, d, a);
+    for (size_t k = 0; k < n2; k++) a[k] = a[k] * s->rmass_inv[k];
+    for (size_t k = 0; k < n2; k++) v[k] = v[k] + deltatover2 * a[k];
+    /* backward (reconstructed) wavefield: stored Stacey, source at NSTEP-it+1, elements, mass, corrector */
+    if (s->p.stacey && s->nabs > 0) {
+      for (int f = 0; f < s->nabs; f++) {
+        int e = s->abs_ispec[f] - 1, code = s->abs_code[f];
+        for (int k = 0; k < NGLL; k++) {
+          int i, j;
+          if (code == 4) { i = 0; j = k; }
+          else if (code == 2) { i = NGLL - 1; j = k; }
+          else if (code == 1) { i = k; j = 0; }
+          else { i = k; j = NGLL - 1; }
+          int g = s->ibool[e * NGLL2 + j * NGLL + i] - 1;
+          const double *st = stacey_store + (((size_t)(it_tmp - 1) * s->nabs + f) * NGLL + k) * 2;
+          ba[2 * g] = ba[2 * g] - (REAL)st[0];
+          ba[2 * g + 1] = ba[2 * g + 1] - (REAL)st[1];
+        }
+      }
+    }
+    add_sources(s, ba, it_tmp);
+    element_forces(s, bd, ba);
+    for (size_t k = 0; k < n2; k++) ba[k] = ba[k] * s->rmass_inv[k];
+    for (size_t k = 0; k < n2; k++) bv[k] = bv[k] + b_deltatover2 * ba[k];
+    if (it == 1) {
+      for (size_t k = 0; k < n2; k++) {
+        bd[k] = (REAL)last_d[k];
+        bv[k] = (REAL)last_v[k];
+        ba[k] = (REAL)last_a[k];
+      }
+    }
+    if (seis && s->nrec > 0) record_seismograms(s, d, seis, it - 1, nstep);
+    kernels_accumulate(s, d, a, bd, rho_kl, mu_kl, kappa_kl);
+    for (int q = 0; q < ndump; q++)
+      if (dump_steps[q] == it)
+        for (size_t k = 0; k < n2; k++) b_displ_dump[(size_t)q * n2 + k] = (double)bd[k];
+  }
+  for (size_t l = 0; l < ne; l++) {
+    REAL rhol = s->rhostore[l], mul = s->mustore[l], kappal = s->kappastore[l];
+    kernels[l] = (double)(-rhol * (deltat * (REAL)1) * rho_kl[l]);
+    kernels[ne + l] = (double)(REAL)(-2.0 * (double)mul * (double)(deltat * (REAL)1) * (double)mu_kl[l]);
+    kernels[2 * ne + l] = (double)(-kappal * (deltat * (REAL)1) * kappa_kl[l]);
+  }
+  free(bd); free(bv); free(ba); free(rho_kl); free(mu_kl); free(kappa_kl); free(sadj);
+  _mm_setcsr(csr);
+}
+
+/* One accumulation step of the kernels (unscaled) for given fields; out[3][nspec*25]. */
+void orc_kernel_increment(void *h, const double *displ, const double *accel, const double *b_displ, double *out) {
+  orc_sim *s = (orc_sim *)h;
+  size_t n2 = (size_t)2 * s->nglob, ne = (size_t)s->nspec * NGLL2;
+  REAL *d = (REAL *)malloc(sizeof(REAL) * n2), *a = (REAL *)malloc(sizeof(REAL) * n2), *b = (REAL *)malloc(sizeof(REAL) * n2);
+  for (size_t k = 0; k < n2; k++) { d[k] = (REAL)displ[k]; a[k] = (REAL)accel[k]; b[k] = (REAL)b_displ[k]; }
+  REAL *r = (REAL *)calloc(ne, sizeof(REAL)), *m = (REAL *)calloc(ne, sizeof(REAL)), *kk = (REAL *)calloc(ne, sizeof(REAL));
+  kernels_accumulate(s, d, a, b, r, m, kk);
+  for (size_t l = 0; l < ne; l++) { out[l] = r[l]; out[ne + l] = m[l]; out[2 * ne + l] = kk[l]; }
+  free(d); free(a); free(b); free(r); free(m); free(kk);
 }
