@@ -157,6 +157,7 @@ def main():
             "dt": cfg.dt,
             "boundaries": "Stacey bottom/left/right, free surface top",
             "parallelism": "1 GPU",
+            "assembly": os.environ.get("SF2D_ASSEMBLY", "patch"),
         },
         "roofline": {
             "bound": "hbm",

@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <algorithm>
 #include <vector>
 
 #include "sf2d_mesh.h"
@@ -41,6 +42,16 @@ void check_hip(hipError_t err, const char* what) {
   if (err != hipSuccess) fail("%s: %s", what, hipGetErrorString(err));
 }
 
+void flush_correctors(Mesh* mp) {
+  if (mp->pending_corrector[0])
+    launch_newmark_corrector(mp->d_veloc, mp->d_accel, mp->d_rmassx, mp->d_rmassz, mp->nglob, mp->pending_dto2[0], 1,
+                             mp->stream);
+  if (mp->pending_corrector[1])
+    launch_newmark_corrector(mp->d_b_veloc, mp->d_b_accel, mp->d_rmassx, mp->d_rmassz, mp->nglob,
+                             mp->pending_dto2[1], 1, mp->stream);
+  mp->pending_corrector[0] = mp->pending_corrector[1] = 0;
+}
+
 namespace {
 
 std::vector<int> to_zero_based(const int* a, size_t n, int lo, int hi, const char* what) {
@@ -53,10 +64,243 @@ std::vector<int> to_zero_based(const int* a, size_t n, int lo, int hi, const cha
   return out;
 }
 
-int identity_prefix(const std::vector<int>& v) {
-  int n = 0;
-  while (n < (int)v.size() && v[n] == n) n++;
-  return n;
+
+
+// Greedy colouring of an element list: each element takes the lowest colour not yet used by any element
+// sharing one of its GLL points (per-point colour bit masks).  Structured quad meshes get 4 colours.
+int colour_elements(const std::vector<int>& ibool, const std::vector<int>& elems, int nglob, std::vector<int>& colour) {
+  std::vector<uint32_t> used((size_t)nglob, 0u);
+  colour.assign(elems.size(), 0);
+  int ncol = 0;
+  for (size_t k = 0; k < elems.size(); k++) {
+    const int* ib = ibool.data() + (size_t)elems[k] * NGLL2;
+    uint32_t forbidden = 0;
+    for (int p = 0; p < NGLL2; p++) forbidden |= used[ib[p]];
+    if (forbidden == 0xffffffffu) fail("mesh colouring needs more than %d colours", Mesh::MAX_COLOURS);
+    int c = 0;
+    while (forbidden & (1u << c)) c++;
+    colour[k] = c;
+    for (int p = 0; p < NGLL2; p++) used[ib[p]] |= (1u << c);
+    if (c + 1 > ncol) ncol = c + 1;
+  }
+  return ncol;
+}
+
+// Groups a phase's element list into patches: runs of consecutive list entries, at most PATCH_ELEMS long,
+// in which every element shares a point with the patch built so far.  For the internal mesher's row-major
+// element order these are row segments of PATCH_ELEMS elements.
+std::vector<int> make_patches(const std::vector<int>& ibool, const std::vector<int>& elems, int nglob) {
+  std::vector<int> start;  // index in elems of each patch's first element, plus elems.size()
+  std::vector<int> mark((size_t)nglob, -1);
+  int cur = -1, size = 0;
+  for (size_t k = 0; k < elems.size(); k++) {
+    const int* ib = ibool.data() + (size_t)elems[k] * NGLL2;
+    bool touches = false;
+    if (cur >= 0)
+      for (int p = 0; p < NGLL2 && !touches; p++) touches = (mark[ib[p]] == cur);
+    if (cur < 0 || size == Mesh::PATCH_ELEMS || !touches) {
+      cur = (int)start.size();
+      start.push_back((int)k);
+      size = 0;
+    }
+    for (int p = 0; p < NGLL2; p++) mark[ib[p]] = cur;
+    size++;
+  }
+  start.push_back((int)elems.size());
+  return start;
+}
+
+// Device element order and assembly schedule of both phase lists (host only, no device call):
+//   patch  (mode 0): patches coloured greedily (patches of one colour share no point); order (phase, colour, patch)
+//   colour (mode 1): elements coloured greedily; order (phase, colour)
+//   atomic (mode 2): the phase list order, one launch per phase with FP32 atomics
+struct Plan {
+  std::vector<int> order;  // ispec of each device position
+  int phase_count[2] = {0, 0};
+  int ncolour[2] = {0, 0};
+  int colour_start[2][Mesh::MAX_COLOURS] = {};
+  int colour_count[2][Mesh::MAX_COLOURS] = {};
+  std::vector<int> patch_elem_off, patch_node_off, patch_nodes;  // patch mode, with closing entries
+  std::vector<unsigned char> slot;                                // patch mode, per device position
+};
+
+Plan build_plan(const std::vector<int>& ibool, int nspec, int nglob, const int* phase_lists, int np, int mode) {
+  Plan P;
+  std::vector<int>& order = P.order;
+  order.reserve(nspec);
+  std::vector<char> placed(nspec, 0);
+  if (mode == 0) P.slot.assign((size_t)nspec * NGLL2, 0);
+  for (int ph = 0; ph < 2; ph++) {
+    std::vector<int> elems;
+    for (int k = 0; k < np; k++) {
+      const int v = phase_lists[(size_t)ph * np + k];
+      if (v < 1) break;  // lists are padded with zeros after their used prefix
+      if (v > nspec) fail("phase_ispec_inner_elastic: element %d outside 1..%d", v, nspec);
+      if (placed[v - 1]) fail("phase_ispec_inner_elastic: element %d listed twice", v);
+      placed[v - 1] = 1;
+      elems.push_back(v - 1);
+    }
+    P.phase_count[ph] = (int)elems.size();
+    if (mode == 2) {  // atomics: one range in list order
+      P.ncolour[ph] = elems.empty() ? 0 : 1;
+      P.colour_start[ph][0] = (int)order.size();
+      P.colour_count[ph][0] = (int)elems.size();
+      order.insert(order.end(), elems.begin(), elems.end());
+      continue;
+    }
+    if (mode == 1) {  // coloured elements
+      std::vector<int> colour;
+      const int ncol = colour_elements(ibool, elems, nglob, colour);
+      P.ncolour[ph] = ncol;
+      for (int c = 0; c < ncol; c++) {
+        P.colour_start[ph][c] = (int)order.size();
+        for (size_t k = 0; k < elems.size(); k++)
+          if (colour[k] == c) order.push_back(elems[k]);
+        P.colour_count[ph][c] = (int)order.size() - P.colour_start[ph][c];
+      }
+      continue;
+    }
+    // coloured patches
+    const std::vector<int> pstart = make_patches(ibool, elems, nglob);
+    const int np_ph = (int)pstart.size() - 1;
+    std::vector<std::vector<int>> pnodes(np_ph);
+    for (int q = 0; q < np_ph; q++) {
+      std::vector<int>& nodes = pnodes[q];
+      for (int k = pstart[q]; k < pstart[q + 1]; k++) {
+        const int* ib = ibool.data() + (size_t)elems[k] * NGLL2;
+        nodes.insert(nodes.end(), ib, ib + NGLL2);
+      }
+      std::sort(nodes.begin(), nodes.end());
+      nodes.erase(std::unique(nodes.begin(), nodes.end()), nodes.end());
+      if (nodes.size() > 256) fail("patch with %zu points exceeds the 256-point workgroup", nodes.size());
+    }
+    std::vector<uint32_t> used((size_t)nglob, 0u);
+    std::vector<int> pcol(np_ph, 0);
+    int ncol = 0;
+    for (int q = 0; q < np_ph; q++) {
+      uint32_t forbidden = 0;
+      for (int g : pnodes[q]) forbidden |= used[g];
+      if (forbidden == 0xffffffffu) fail("patch colouring needs more than %d colours", Mesh::MAX_COLOURS);
+      int c = 0;
+      while (forbidden & (1u << c)) c++;
+      pcol[q] = c;
+      for (int g : pnodes[q]) used[g] |= (1u << c);
+      ncol = std::max(ncol, c + 1);
+    }
+    P.ncolour[ph] = ncol;
+    for (int c = 0; c < ncol; c++) {
+      P.colour_start[ph][c] = (int)P.patch_elem_off.size();
+      for (int q = 0; q < np_ph; q++) {
+        if (pcol[q] != c) continue;
+        P.patch_elem_off.push_back((int)order.size());
+        P.patch_node_off.push_back((int)P.patch_nodes.size());
+        const std::vector<int>& nodes = pnodes[q];
+        for (int k = pstart[q]; k < pstart[q + 1]; k++) {
+          const int e = elems[k];
+          const int pos = (int)order.size();
+          order.push_back(e);
+          const int* ib = ibool.data() + (size_t)e * NGLL2;
+          for (int p = 0; p < NGLL2; p++)
+            P.slot[(size_t)pos * NGLL2 + p] =
+                (unsigned char)(std::lower_bound(nodes.begin(), nodes.end(), ib[p]) - nodes.begin());
+        }
+        P.patch_nodes.insert(P.patch_nodes.end(), nodes.begin(), nodes.end());
+      }
+      P.colour_count[ph][c] = (int)P.patch_elem_off.size() - P.colour_start[ph][c];
+    }
+  }
+  for (int e = 0; e < nspec; e++)
+    if (!placed[e]) order.push_back(e);  // elements outside the elastic phase lists keep their relative order
+  if (mode == 0) {
+    P.patch_elem_off.push_back(P.phase_count[0] + P.phase_count[1]);
+    P.patch_node_off.push_back((int)P.patch_nodes.size());
+  }
+  return P;
+}
+
+// Applies the plan: schedule into the handle, per-element device arrays moved into the plan's order.
+void sort_elements_by_colour(Mesh* mp, const int* phase_lists, int np) {
+  const int nspec = mp->nspec;
+  Plan P = build_plan(mp->h_ibool, nspec, mp->nglob, phase_lists, np, mp->assembly);
+  std::vector<int>& order = P.order;
+  for (int ph = 0; ph < 2; ph++) {
+    mp->phase_count[ph] = P.phase_count[ph];
+    mp->ncolour[ph] = P.ncolour[ph];
+    for (int c = 0; c < Mesh::MAX_COLOURS; c++) {
+      mp->colour_start[ph][c] = P.colour_start[ph][c];
+      mp->colour_count[ph][c] = P.colour_count[ph][c];
+    }
+  }
+  mp->ispec_of_pos = order;
+  mp->pos_of_ispec.assign(nspec, 0);
+  for (int pos = 0; pos < nspec; pos++) mp->pos_of_ispec[order[pos]] = pos;
+  if (mp->assembly == 0) {
+    mp->npatch = (int)P.patch_elem_off.size() - 1;
+    mp->d_patch_elem_off = device_copy(P.patch_elem_off.data(), P.patch_elem_off.size());
+    mp->d_patch_node_off = device_copy(P.patch_node_off.data(), P.patch_node_off.size());
+    mp->d_patch_nodes = device_copy(P.patch_nodes.data(), P.patch_nodes.size());
+    mp->d_elem_slot = device_copy(P.slot.data(), P.slot.size());
+  }
+
+  // move the per-element device arrays
+  int* d_map = device_copy(order.data(), order.size());
+  const size_t ne = (size_t)nspec * NGLL2;
+  float** farrs[6] = {&mp->d_xix, &mp->d_xiz, &mp->d_gammax, &mp->d_gammaz, &mp->d_kappav, &mp->d_muv};
+  for (float** a : farrs) {
+    float* dst = nullptr;
+    check_hip(hipMalloc(&dst, ne * sizeof(float)), "hipMalloc");
+    launch_permute_elements(*a, dst, d_map, nspec, mp->stream);
+    check_hip(hipStreamSynchronize(mp->stream), "permute");
+    device_free(*a);
+    *a = dst;
+  }
+  {
+    int* dst = nullptr;
+    check_hip(hipMalloc(&dst, ne * sizeof(int)), "hipMalloc");
+    launch_permute_elements_int(mp->d_ibool, dst, d_map, nspec, mp->stream);
+    check_hip(hipStreamSynchronize(mp->stream), "permute");
+    device_free(mp->d_ibool);
+    mp->d_ibool = dst;
+  }
+  {
+    std::vector<int> el(nspec);
+    std::vector<int> h(nspec);
+    check_hip(hipMemcpy(h.data(), mp->d_ispec_is_elastic, nspec * sizeof(int), hipMemcpyDeviceToHost), "D2H");
+    for (int pos = 0; pos < nspec; pos++) el[pos] = h[order[pos]];
+    check_hip(hipMemcpy(mp->d_ispec_is_elastic, el.data(), nspec * sizeof(int), hipMemcpyHostToDevice), "H2D");
+  }
+  device_free(d_map);
+  // element indices of sources and receivers follow the new order
+  if (mp->nsources_local > 0) {
+    std::vector<int> v(mp->h_src_ispec.size());
+    for (size_t k = 0; k < v.size(); k++) v[k] = mp->pos_of_ispec[mp->h_src_ispec[k]];
+    check_hip(hipMemcpy(mp->d_ispec_selected_source, v.data(), v.size() * sizeof(int), hipMemcpyHostToDevice), "H2D");
+  }
+  if (mp->nrec_local > 0) {
+    std::vector<int> v(mp->h_rec_ispec.size());
+    for (size_t k = 0; k < v.size(); k++) v[k] = mp->pos_of_ispec[mp->h_rec_ispec[k]];
+    check_hip(hipMemcpy(mp->d_ispec_selected_rec, v.data(), v.size() * sizeof(int), hipMemcpyHostToDevice), "H2D");
+  }
+  mp->sorted = 1;
+  std::vector<int>().swap(mp->h_ibool);
+}
+
+// per-element host array in ispec order -> device order (identity before sorting)
+std::vector<float> to_device_order(const Mesh* mp, const float* a) {
+  const size_t ne = (size_t)mp->nspec * NGLL2;
+  std::vector<float> out(a, a + ne);
+  if (mp->sorted)
+    for (int pos = 0; pos < mp->nspec; pos++)
+      memcpy(out.data() + (size_t)pos * NGLL2, a + (size_t)mp->ispec_of_pos[pos] * NGLL2, NGLL2 * sizeof(float));
+  return out;
+}
+
+// device-order per-element array -> caller's ispec order
+void to_ispec_order(const Mesh* mp, const float* dev_order, float* out) {
+  for (int e = 0; e < mp->nspec; e++) {
+    const int pos = mp->sorted ? mp->pos_of_ispec[e] : e;
+    memcpy(out + (size_t)e * NGLL2, dev_order + (size_t)pos * NGLL2, NGLL2 * sizeof(float));
+  }
 }
 
 }  // namespace
@@ -137,9 +381,14 @@ void prepare_constants_device_(long* Mesh_pointer, int* h_NGLLX, int* NSPEC_AB, 
   mp->d_gammaz = device_copy(h_gammaz, ne);
   mp->d_kappav = device_copy(h_kappav, ne);
   mp->d_muv = device_copy(h_muv, ne);
+  mp->h_ibool = to_zero_based(h_ibool, ne, 1, mp->nglob, "ibool");
+  mp->d_ibool = device_copy(mp->h_ibool.data(), ne);
   {
-    std::vector<int> ib = to_zero_based(h_ibool, ne, 1, mp->nglob, "ibool");
-    mp->d_ibool = device_copy(ib.data(), ne);
+    const char* mode = getenv("SF2D_ASSEMBLY");
+    mp->assembly = 0;
+    if (mode && strcmp(mode, "colour") == 0) mp->assembly = 1;
+    if (mode && strcmp(mode, "atomic") == 0) mp->assembly = 2;
+    if (mode && mp->assembly == 0 && strcmp(mode, "patch") != 0) fail("SF2D_ASSEMBLY=%s: use patch, colour or atomic", mode);
   }
   std::vector<int> elastic(mp->nspec);
   for (int e = 0; e < mp->nspec; e++) elastic[e] = ispec_is_elastic[e] ? 1 : 0;
@@ -152,8 +401,8 @@ void prepare_constants_device_(long* Mesh_pointer, int* h_NGLLX, int* NSPEC_AB, 
   if (mp->nsources_local > 0) {
     mp->d_sourcearrays = device_copy(h_sourcearrays, (size_t)mp->nsources_local * NGLL2 * 2);
     mp->d_stf = device_copy(h_source_time_function, (size_t)mp->nsources_local * mp->nstep);
-    std::vector<int> is = to_zero_based(h_ispec_selected_source, mp->nsources_local, 1, mp->nspec, "ispec_selected_source");
-    mp->d_ispec_selected_source = device_copy(is.data(), is.size());
+    mp->h_src_ispec = to_zero_based(h_ispec_selected_source, mp->nsources_local, 1, mp->nspec, "ispec_selected_source");
+    mp->d_ispec_selected_source = device_copy(mp->h_src_ispec.data(), mp->h_src_ispec.size());
   }
 
   // receivers and seismogram buffers
@@ -162,8 +411,8 @@ void prepare_constants_device_(long* Mesh_pointer, int* h_NGLLX, int* NSPEC_AB, 
   mp->h_seismotype = h_seismotypeVec;
   mp->nlength_seismogram = *nlength_seismogram;
   if (mp->nrec_local > 0) {
-    std::vector<int> ir = to_zero_based(h_ispec_selected_rec_loc, mp->nrec_local, 1, mp->nspec, "ispec_selected_rec_loc");
-    mp->d_ispec_selected_rec = device_copy(ir.data(), ir.size());
+    mp->h_rec_ispec = to_zero_based(h_ispec_selected_rec_loc, mp->nrec_local, 1, mp->nspec, "ispec_selected_rec_loc");
+    mp->d_ispec_selected_rec = device_copy(mp->h_rec_ispec.data(), mp->h_rec_ispec.size());
     mp->d_cosrot = device_copy(h_cosrot, (size_t)mp->nrec_local);
     mp->d_sinrot = device_copy(h_sinrot, (size_t)mp->nrec_local);
     mp->d_xir = device_copy(h_xir_store, (size_t)mp->nrec_local * NGLL);
@@ -204,14 +453,7 @@ void prepare_fields_elastic_device_(long* Mesh_pointer, realw* rmassx, realw* rm
   mp->d_rmassz = device_copy(rmassz, (size_t)mp->nglob);
   const int np = *num_phase_ispec_elastic;
   mp->num_phase_ispec_elastic = np;
-  for (int ph = 0; ph < 2; ph++) {
-    std::vector<int> lst = to_zero_based(phase_ispec_inner_elastic + (size_t)ph * np, (size_t)np, 0, mp->nspec,
-                                         "phase_ispec_inner_elastic");
-    // entries past the used prefix may be zero (unused); keep them in range
-    for (auto& v : lst) v = v < 0 ? 0 : v;
-    mp->phase_is_identity[ph] = identity_prefix(lst);
-    mp->d_phase_ispec[ph] = device_copy(lst.data(), lst.size());
-  }
+  sort_elements_by_colour(mp, phase_ispec_inner_elastic, np);
   check_hip(hipDeviceSynchronize(), "prepare_fields_elastic_device");
 }
 
@@ -250,6 +492,8 @@ void prepare_stacey_device_(long* Mesh_pointer, int* ACOUSTIC_SIMULATION, int* E
   mp->nspec_top = *h_nspec_top;
   if (nf <= 0) return;
   std::vector<int> isp = to_zero_based(h_abs_boundary_ispec, nf, 1, mp->nspec, "abs_boundary_ispec");
+  if (mp->sorted)
+    for (auto& e : isp) e = mp->pos_of_ispec[e];
   std::vector<int> ij((size_t)nf * NGLL * 2);
   for (size_t k = 0; k < ij.size(); k++) {
     int v = h_abs_boundary_ij[k];
@@ -275,8 +519,9 @@ void prepare_stacey_device_(long* Mesh_pointer, int* ACOUSTIC_SIMULATION, int* E
   mp->d_edge_abs = device_copy(h_edge_abs, (size_t)nf);
   mp->d_ib = device_copy(ib.data(), ib.size());
   if (*ELASTIC_SIMULATION) {
-    mp->d_rho_vp = device_copy(rho_vp, (size_t)mp->nspec * NGLL2);
-    mp->d_rho_vs = device_copy(rho_vs, (size_t)mp->nspec * NGLL2);
+    std::vector<float> rvp = to_device_order(mp, rho_vp), rvs = to_device_order(mp, rho_vs);
+    mp->d_rho_vp = device_copy(rvp.data(), rvp.size());
+    mp->d_rho_vs = device_copy(rvs.data(), rvs.size());
     if (mp->simulation_type == 3 || (mp->simulation_type == 1 && mp->save_forward)) {
       for (int side = 1; side <= 4; side++)
         if (counts[side] > 0) mp->d_b_absorb[side] = device_zeros<float>((size_t)counts[side] * NGLL * 2);
@@ -308,7 +553,6 @@ void prepare_cleanup_device_(long* Mesh_pointer, int* ACOUSTIC_SIMULATION, int* 
   device_free(mp->d_displ); device_free(mp->d_veloc); device_free(mp->d_accel);
   device_free(mp->d_b_displ); device_free(mp->d_b_veloc); device_free(mp->d_b_accel);
   device_free(mp->d_rmassx); device_free(mp->d_rmassz);
-  device_free(mp->d_phase_ispec[0]); device_free(mp->d_phase_ispec[1]);
   device_free(mp->d_sourcearrays); device_free(mp->d_stf); device_free(mp->d_ispec_selected_source);
   device_free(mp->d_xir); device_free(mp->d_gammar); device_free(mp->d_cosrot); device_free(mp->d_sinrot);
   device_free(mp->d_ispec_selected_rec);
@@ -326,6 +570,8 @@ void prepare_cleanup_device_(long* Mesh_pointer, int* ACOUSTIC_SIMULATION, int* 
   for (int s = 0; s < 5; s++) device_free(mp->d_b_absorb[s]);
   device_free(mp->d_rho_kl); device_free(mp->d_mu_kl); device_free(mp->d_kappa_kl);
   device_free(mp->d_scratch_u32);
+  device_free(mp->d_patch_elem_off); device_free(mp->d_patch_node_off); device_free(mp->d_patch_nodes);
+  device_free(mp->d_elem_slot);
   (void)hipStreamDestroy(mp->stream);
   delete mp;
   *Mesh_pointer = 0;
@@ -348,38 +594,45 @@ static void fetch_field(realw* dst_host, const float* src_dev, int size, Mesh* m
 
 void transfer_fields_el_to_device_(int* size, realw* displ, realw* veloc, realw* accel, long* Mesh_pointer) {
   Mesh* mp = handle(Mesh_pointer);
+  flush_correctors(mp);
   copy_field(mp->d_displ, displ, *size, mp, "transfer_fields_el_to_device");
   copy_field(mp->d_veloc, veloc, *size, mp, "transfer_fields_el_to_device");
   copy_field(mp->d_accel, accel, *size, mp, "transfer_fields_el_to_device");
 }
 void transfer_fields_el_from_device_(int* size, realw* displ, realw* veloc, realw* accel, long* Mesh_pointer) {
   Mesh* mp = handle(Mesh_pointer);
+  flush_correctors(mp);
   fetch_field(displ, mp->d_displ, *size, mp, "transfer_fields_el_from_device");
   fetch_field(veloc, mp->d_veloc, *size, mp, "transfer_fields_el_from_device");
   fetch_field(accel, mp->d_accel, *size, mp, "transfer_fields_el_from_device");
 }
 void transfer_b_fields_to_device_(int* size, realw* b_displ, realw* b_veloc, realw* b_accel, long* Mesh_pointer) {
   Mesh* mp = handle(Mesh_pointer);
+  flush_correctors(mp);
   copy_field(mp->d_b_displ, b_displ, *size, mp, "transfer_b_fields_to_device");
   copy_field(mp->d_b_veloc, b_veloc, *size, mp, "transfer_b_fields_to_device");
   copy_field(mp->d_b_accel, b_accel, *size, mp, "transfer_b_fields_to_device");
 }
 void transfer_b_fields_from_device_(int* size, realw* b_displ, realw* b_veloc, realw* b_accel, long* Mesh_pointer) {
   Mesh* mp = handle(Mesh_pointer);
+  flush_correctors(mp);
   fetch_field(b_displ, mp->d_b_displ, *size, mp, "transfer_b_fields_from_device");
   fetch_field(b_veloc, mp->d_b_veloc, *size, mp, "transfer_b_fields_from_device");
   fetch_field(b_accel, mp->d_b_accel, *size, mp, "transfer_b_fields_from_device");
 }
 void transfer_accel_to_device_(int* size, realw* accel, long* Mesh_pointer) {
   Mesh* mp = handle(Mesh_pointer);
+  flush_correctors(mp);
   copy_field(mp->d_accel, accel, *size, mp, "transfer_accel_to_device");
 }
 void transfer_accel_from_device_(int* size, realw* accel, long* Mesh_pointer) {
   Mesh* mp = handle(Mesh_pointer);
+  flush_correctors(mp);
   fetch_field(accel, mp->d_accel, *size, mp, "transfer_accel_from_device");
 }
 void transfer_b_accel_from_device_(int* size, realw* b_accel, long* Mesh_pointer) {
   Mesh* mp = handle(Mesh_pointer);
+  flush_correctors(mp);
   fetch_field(b_accel, mp->d_b_accel, *size, mp, "transfer_b_accel_from_device");
 }
 void transfer_b_displ_from_device_(int* size, realw* displ, long* Mesh_pointer) {
@@ -397,19 +650,24 @@ void transfer_kernels_el_to_host_(long* Mesh_pointer, realw* h_rho_kl, realw* h_
   const size_t n = (size_t)(*NSPEC_AB) * NGLL2;
   if (*NSPEC_AB != mp->nspec) fail("transfer_kernels_el_to_host: NSPEC_AB mismatch");
   check_hip(hipStreamSynchronize(mp->stream), "transfer_kernels_el_to_host");
-  check_hip(hipMemcpy(h_rho_kl, mp->d_rho_kl, n * sizeof(float), hipMemcpyDeviceToHost), "rho_kl D2H");
-  check_hip(hipMemcpy(h_mu_kl, mp->d_mu_kl, n * sizeof(float), hipMemcpyDeviceToHost), "mu_kl D2H");
-  check_hip(hipMemcpy(h_kappa_kl, mp->d_kappa_kl, n * sizeof(float), hipMemcpyDeviceToHost), "kappa_kl D2H");
+  std::vector<float> tmp(n);
+  float* outs[3] = {h_rho_kl, h_mu_kl, h_kappa_kl};
+  const float* devs[3] = {mp->d_rho_kl, mp->d_mu_kl, mp->d_kappa_kl};
+  for (int q = 0; q < 3; q++) {
+    check_hip(hipMemcpy(tmp.data(), devs[q], n * sizeof(float), hipMemcpyDeviceToHost), "kernels D2H");
+    to_ispec_order(mp, tmp.data(), outs[q]);
+  }
 }
 void transfer_compute_kernel_answers_from_device_(long* Mesh_pointer, realw* rho_kl, int* size_rho, realw* mu_kl,
                                                   int* size_mu, realw* kappa_kl,
                                                   int* size_kappa) {  // transfer_fields_cuda.cu:598
   Mesh* mp = handle(Mesh_pointer);
   if (!mp->d_rho_kl) fail("transfer_compute_kernel_answers_from_device: kernels not allocated");
-  check_hip(hipStreamSynchronize(mp->stream), "transfer_compute_kernel_answers_from_device");
-  check_hip(hipMemcpy(rho_kl, mp->d_rho_kl, (size_t)*size_rho * sizeof(float), hipMemcpyDeviceToHost), "D2H");
-  check_hip(hipMemcpy(mu_kl, mp->d_mu_kl, (size_t)*size_mu * sizeof(float), hipMemcpyDeviceToHost), "D2H");
-  check_hip(hipMemcpy(kappa_kl, mp->d_kappa_kl, (size_t)*size_kappa * sizeof(float), hipMemcpyDeviceToHost), "D2H");
+  const size_t n = (size_t)mp->nspec * NGLL2;
+  if ((size_t)*size_rho != n || (size_t)*size_mu != n || (size_t)*size_kappa != n)
+    fail("transfer_compute_kernel_answers_from_device: sizes must be NGLL2*NSPEC_AB = %zu", n);
+  int nspec = mp->nspec;
+  transfer_kernels_el_to_host_(Mesh_pointer, rho_kl, mu_kl, kappa_kl, &nspec);
 }
 void transfer_compute_kernel_fields_from_device_(
     long* Mesh_pointer, realw* accel, int* size_accel, realw* b_displ, int* size_b_displ, realw* epsilondev_xx,
@@ -423,6 +681,7 @@ void transfer_compute_kernel_fields_from_device_(
   (void)b_epsilondev_xz; (void)b_epsilondev_yz; (void)size_b_epsilondev; (void)epsilon_trace_over_3;
   (void)b_epsilon_trace_over_3; (void)size_epsilon_trace_over_3;
   Mesh* mp = handle(Mesh_pointer);
+  flush_correctors(mp);
   fetch_field(accel, mp->d_accel, *size_accel, mp, "transfer_compute_kernel_fields_from_device");
   fetch_field(b_displ, mp->d_b_displ, *size_b_displ, mp, "transfer_compute_kernel_fields_from_device");
   transfer_compute_kernel_answers_from_device_(Mesh_pointer, rho_kl, size_rho, mu_kl, size_mu, kappa_kl, size_kappa);
@@ -434,11 +693,24 @@ void update_displacement_cuda_(long* Mesh_pointer, realw* deltat_F, realw* delta
                                realw* b_deltat_F, realw* b_deltatsqover2_F,
                                realw* b_deltatover2_F) {  // update_displacement_cuda.cu:52
   Mesh* mp = handle(Mesh_pointer);
-  launch_newmark_predictor(mp->d_displ, mp->d_veloc, mp->d_accel, mp->nglob, *deltat_F, *deltatsqover2_F,
-                           *deltatover2_F, mp->stream);
-  if (mp->simulation_type == 3)
-    launch_newmark_predictor(mp->d_b_displ, mp->d_b_veloc, mp->d_b_accel, mp->nglob, *b_deltat_F, *b_deltatsqover2_F,
-                             *b_deltatover2_F, mp->stream);
+  if (mp->pending_corrector[0])
+    launch_newmark_corrector_predictor(mp->d_displ, mp->d_veloc, mp->d_accel, mp->d_rmassx, mp->d_rmassz, mp->nglob,
+                                       mp->pending_dto2[0], *deltat_F, *deltatsqover2_F, *deltatover2_F, mp->stream);
+  else
+    launch_newmark_predictor(mp->d_displ, mp->d_veloc, mp->d_accel, mp->nglob, *deltat_F, *deltatsqover2_F,
+                             *deltatover2_F, mp->stream);
+  if (mp->simulation_type == 3) {
+    if (mp->pending_corrector[1])
+      launch_newmark_corrector_predictor(mp->d_b_displ, mp->d_b_veloc, mp->d_b_accel, mp->d_rmassx, mp->d_rmassz,
+                                         mp->nglob, mp->pending_dto2[1], *b_deltat_F, *b_deltatsqover2_F,
+                                         *b_deltatover2_F, mp->stream);
+    else
+      launch_newmark_predictor(mp->d_b_displ, mp->d_b_veloc, mp->d_b_accel, mp->nglob, *b_deltat_F, *b_deltatsqover2_F,
+                               *b_deltatover2_F, mp->stream);
+  } else if (mp->pending_corrector[1]) {
+    flush_correctors(mp);
+  }
+  mp->pending_corrector[0] = mp->pending_corrector[1] = 0;
 }
 
 void compute_forces_viscoelastic_cuda_(long* Mesh_pointer, int* iphase, realw* deltat, int* nspec_outer_elastic,
@@ -447,17 +719,32 @@ void compute_forces_viscoelastic_cuda_(long* Mesh_pointer, int* iphase, realw* d
                                        int* compute_wavefield_2) {  // compute_forces_viscoelastic_cuda.cu:327
   (void)deltat;
   Mesh* mp = handle(Mesh_pointer);
+  flush_correctors(mp);
   if (*ANISOTROPY) fail("anisotropic elements are not supported by this library yet");
   if (*ATTENUATION_VISCOELASTIC) fail("viscoelastic attenuation is not supported by this library yet");
   const int ph = (*iphase == 1) ? 0 : 1;
   const int nelem = (*iphase == 1) ? *nspec_outer_elastic : *nspec_inner_elastic;
   if (nelem <= 0) return;
-  if (nelem > mp->num_phase_ispec_elastic) fail("compute_forces: %d elements > phase list size %d", nelem, mp->num_phase_ispec_elastic);
-  const int* elist = (nelem <= mp->phase_is_identity[ph]) ? nullptr : mp->d_phase_ispec[ph];
-  if (*compute_wavefield_1) launch_elastic_forces(mp, elist, nelem, mp->d_displ, mp->d_accel);
-  if (*compute_wavefield_2) {
-    if (!mp->d_b_displ) fail("compute_forces: backward wavefield not allocated");
-    launch_elastic_forces(mp, elist, nelem, mp->d_b_displ, mp->d_b_accel);
+  if (!mp->sorted) fail("compute_forces: prepare_fields_elastic_device was not called");
+  if (nelem != mp->phase_count[ph])
+    fail("compute_forces: iphase %d has %d elements, the prepared phase list has %d", *iphase, nelem, mp->phase_count[ph]);
+  const bool w1 = *compute_wavefield_1 != 0, w2 = *compute_wavefield_2 != 0;
+  if (w2 && !mp->d_b_displ) fail("compute_forces: backward wavefield not allocated");
+  if (mp->assembly == 2) {
+    const int first = mp->colour_start[ph][0];
+    if (w1) launch_elastic_forces(mp, first, nelem, 1, mp->d_displ, mp->d_accel);
+    if (w2) launch_elastic_forces(mp, first, nelem, 1, mp->d_b_displ, mp->d_b_accel);
+    return;
+  }
+  for (int c = 0; c < mp->ncolour[ph]; c++) {
+    const int first = mp->colour_start[ph][c], count = mp->colour_count[ph][c];
+    if (mp->assembly == 0) {
+      if (w1) launch_elastic_forces_patch(mp, first, count, mp->d_displ, mp->d_accel);
+      if (w2) launch_elastic_forces_patch(mp, first, count, mp->d_b_displ, mp->d_b_accel);
+    } else {
+      if (w1) launch_elastic_forces(mp, first, count, 0, mp->d_displ, mp->d_accel);
+      if (w2) launch_elastic_forces(mp, first, count, 0, mp->d_b_displ, mp->d_b_accel);
+    }
   }
 }
 
@@ -467,6 +754,7 @@ void compute_stacey_viscoelastic_cuda_(long* Mesh_pointer, int* iphasef, realw* 
                                        int* compute_wavefield_2,
                                        int* UNDO_ATTENUATION_AND_OR_PML) {  // compute_stacey_viscoelastic_cuda.cu:40
   Mesh* mp = handle(Mesh_pointer);
+  flush_correctors(mp);
   if (mp->num_abs == 0 || *iphasef != 1) return;
   const int read_abs = (mp->simulation_type == 3 && !*UNDO_ATTENUATION_AND_OR_PML) ? 1 : 0;
   const int write_abs = (mp->simulation_type == 1 && mp->save_forward && !*UNDO_ATTENUATION_AND_OR_PML) ? 1 : 0;
@@ -495,6 +783,7 @@ void compute_stacey_viscoelastic_cuda_(long* Mesh_pointer, int* iphasef, realw* 
 void compute_add_sources_el_cuda_(long* Mesh_pointer, int* iphasef,
                                   int* itf) {  // compute_add_sources_viscoelastic_cuda.cu:46
   Mesh* mp = handle(Mesh_pointer);
+  flush_correctors(mp);
   if (mp->nsources_local == 0 || *iphasef != 1) return;
   if (*itf < 1 || *itf > mp->nstep) fail("compute_add_sources_el_cuda: it=%d outside 1..%d", *itf, mp->nstep);
   launch_sources(mp, mp->d_accel, *itf - 1);
@@ -503,6 +792,7 @@ void compute_add_sources_el_cuda_(long* Mesh_pointer, int* iphasef,
 void compute_add_sources_el_s3_cuda_(long* Mesh_pointer, int* iphasef,
                                      int* itf) {  // compute_add_sources_viscoelastic_cuda.cu:86
   Mesh* mp = handle(Mesh_pointer);
+  flush_correctors(mp);
   if (mp->nsources_local == 0 || *iphasef != 1) return;
   if (*itf < 1 || *itf > mp->nstep) fail("compute_add_sources_el_s3_cuda: it=%d outside 1..%d", *itf, mp->nstep);
   if (!mp->d_b_accel) fail("compute_add_sources_el_s3_cuda: backward wavefield not allocated");
@@ -512,6 +802,7 @@ void compute_add_sources_el_s3_cuda_(long* Mesh_pointer, int* iphasef,
 void add_sources_el_sim_type_2_or_3_(long* Mesh_pointer, int* iphasef, int* itf, int* nadj_rec_local,
                                      int* NSTEP) {  // compute_add_sources_viscoelastic_cuda.cu:131
   Mesh* mp = handle(Mesh_pointer);
+  flush_correctors(mp);
   if (*nadj_rec_local != mp->nadj_rec_local) fail("add_sources_el_sim_type_2_or_3: nadj_rec_local not equal");
   if (mp->nadj_rec_local == 0 || *iphasef != 1) return;
   if (*NSTEP != mp->nstep) fail("add_sources_el_sim_type_2_or_3: NSTEP mismatch");
@@ -522,17 +813,23 @@ void add_sources_el_sim_type_2_or_3_(long* Mesh_pointer, int* iphasef, int* itf,
 void kernel_3_a_cuda_(long* Mesh_pointer, realw* deltatover2_F, realw* b_deltatover2_F, int* compute_wavefield_1,
                       int* compute_wavefield_2) {  // update_displacement_cuda.cu:221
   Mesh* mp = handle(Mesh_pointer);
-  if (*compute_wavefield_1)
-    launch_newmark_corrector(mp->d_veloc, mp->d_accel, mp->d_rmassx, mp->d_rmassz, mp->nglob, *deltatover2_F, 1,
-                             mp->stream);
-  if (*compute_wavefield_2)
-    launch_newmark_corrector(mp->d_b_veloc, mp->d_b_accel, mp->d_rmassx, mp->d_rmassz, mp->nglob, *b_deltatover2_F, 1,
-                             mp->stream);
+  flush_correctors(mp);
+  // recorded here, applied by the next predictor (or by the first entry point that reads veloc/accel)
+  if (*compute_wavefield_1) {
+    mp->pending_corrector[0] = 1;
+    mp->pending_dto2[0] = *deltatover2_F;
+  }
+  if (*compute_wavefield_2) {
+    if (!mp->d_b_veloc) fail("kernel_3_a_cuda: backward wavefield not allocated");
+    mp->pending_corrector[1] = 1;
+    mp->pending_dto2[1] = *b_deltatover2_F;
+  }
 }
 
 void kernel_3_b_cuda_(long* Mesh_pointer, realw* deltatover2_F,
                       realw* b_deltatover2_F) {  // update_displacement_cuda.cu:273 (velocity-only corrector)
   Mesh* mp = handle(Mesh_pointer);
+  flush_correctors(mp);
   launch_newmark_corrector(mp->d_veloc, mp->d_accel, nullptr, nullptr, mp->nglob, *deltatover2_F, 0, mp->stream);
   if (mp->simulation_type == 3)
     launch_newmark_corrector(mp->d_b_veloc, mp->d_b_accel, nullptr, nullptr, mp->nglob, *b_deltatover2_F, 0,
@@ -556,6 +853,7 @@ void compute_seismograms_cuda_(long* Mesh_pointer_f, int* i_sigf, double* sisux,
   if (cur < 0 || cur >= nlength) fail("compute_seismograms_cuda: seismo_current %d outside 1..%d", cur + 1, nlength);
   const int stype = mp->h_seismotype[i_sig];
   const float* field = nullptr;
+  if (stype == 2 || stype == 3) flush_correctors(mp);
   switch (stype) {
     case 1: field = mp->d_displ; break;
     case 2: field = mp->d_veloc; break;
@@ -579,6 +877,7 @@ void compute_seismograms_cuda_(long* Mesh_pointer_f, int* i_sigf, double* sisux,
 void compute_kernels_elastic_cuda_(long* Mesh_pointer, realw* deltat) {  // compute_kernels_cuda.cu:47
   (void)deltat;
   Mesh* mp = handle(Mesh_pointer);
+  flush_correctors(mp);
   if (!mp->d_rho_kl || !mp->d_b_displ) fail("compute_kernels_elastic_cuda: adjoint arrays not prepared");
   launch_kernels(mp);
 }
@@ -616,6 +915,7 @@ void output_free_device_memory_(int* myrank_f) {  // check_fields_cuda.cu:149
 
 void get_max_accel_(int* itf, int* sizef, long* Mesh_pointer) {  // check_fields_cuda.cu:484
   Mesh* mp = handle(Mesh_pointer);
+  flush_correctors(mp);
   std::vector<float> a((size_t)*sizef);
   check_hip(hipStreamSynchronize(mp->stream), "get_max_accel");
   check_hip(hipMemcpy(a.data(), mp->d_accel, a.size() * sizeof(float), hipMemcpyDeviceToHost), "accel D2H");
@@ -655,11 +955,13 @@ void check_max_norm_b_displ_gpu_(int* size, realw* b_displ, long* Mesh_pointer, 
 void check_max_norm_b_accel_gpu_(int* size, realw* b_accel, long* Mesh_pointer, int* announceID) {
   (void)b_accel;
   Mesh* mp = handle(Mesh_pointer);
+  flush_correctors(mp);
   print_device_max(mp, mp->d_b_accel, *size, *announceID, "b_accel (gpu)");
 }
 void check_max_norm_b_veloc_gpu_(int* size, realw* b_veloc, long* Mesh_pointer, int* announceID) {
   (void)b_veloc;
   Mesh* mp = handle(Mesh_pointer);
+  flush_correctors(mp);
   print_device_max(mp, mp->d_b_veloc, *size, *announceID, "b_veloc (gpu)");
 }
 void check_max_norm_b_displ_(int* size, realw* b_displ, int* announceID) { print_max("b_displ", b_displ, *size, *announceID); }
@@ -676,6 +978,69 @@ void pause_for_debug_(void) {}  // helper_functions.cu:281
 // ---------------------------------------------------------------------------
 // extensions without a reference counterpart (specfem2d_gpu_ext.h)
 extern "C" void* sf2d_mesh_stream(long* Mesh_pointer) { return (void*)handle(Mesh_pointer)->stream; }
+
+// Builds the assembly plan for a mesh on the host and checks its invariants (no device involved):
+// every listed element in exactly one unit, no two units of one colour share a point, patch slots address
+// their patch's points.  stats = {colours phase 1, colours phase 2, patches, max points per patch}.
+// Returns 0 when every check holds.
+extern "C" int sf2d_plan_validate(int nspec, int nglob, const int* ibool_1based, int np, const int* phase_lists,
+                                  int mode, int* stats) {
+  std::vector<int> ib = to_zero_based(ibool_1based, (size_t)nspec * NGLL2, 1, nglob, "ibool");
+  Plan P = build_plan(ib, nspec, nglob, phase_lists, np, mode);
+  std::vector<int> seen(nspec, 0);
+  for (int e : P.order) {
+    if (e < 0 || e >= nspec) return 1;
+    seen[e]++;
+  }
+  for (int e = 0; e < nspec; e++)
+    if (seen[e] != 1) return 2;
+  std::vector<int> owner(nglob, -1);
+  int maxpts = 0;
+  for (int ph = 0; ph < 2; ph++)
+    for (int c = 0; c < P.ncolour[ph]; c++) {
+      std::fill(owner.begin(), owner.end(), -1);
+      for (int u = P.colour_start[ph][c]; u < P.colour_start[ph][c] + P.colour_count[ph][c]; u++) {
+        int e0 = u, e1 = u + 1;  // element range of the unit
+        if (mode == 0) {
+          e0 = P.patch_elem_off[u];
+          e1 = P.patch_elem_off[u + 1];
+          const int n0 = P.patch_node_off[u], n1 = P.patch_node_off[u + 1];
+          maxpts = std::max(maxpts, n1 - n0);
+          if (e1 - e0 < 1 || e1 - e0 > Mesh::PATCH_ELEMS || n1 - n0 > 256) return 3;
+          for (int pos = e0; pos < e1; pos++)
+            for (int p = 0; p < NGLL2; p++) {
+              const int sl = P.slot[(size_t)pos * NGLL2 + p];
+              if (sl >= n1 - n0 || P.patch_nodes[n0 + sl] != ib[(size_t)P.order[pos] * NGLL2 + p]) return 4;
+            }
+        }
+        if (mode == 2) break;  // atomics may share points
+        for (int pos = e0; pos < e1; pos++)
+          for (int p = 0; p < NGLL2; p++) {
+            const int g = ib[(size_t)P.order[pos] * NGLL2 + p];
+            if (owner[g] != -1 && owner[g] != u) return 5;  // two units of one colour share a point
+            owner[g] = u;
+          }
+      }
+    }
+  stats[0] = P.ncolour[0];
+  stats[1] = P.ncolour[1];
+  stats[2] = mode == 0 ? (int)P.patch_elem_off.size() - 1 : 0;
+  stats[3] = maxpts;
+  return 0;
+}
+
+extern "C" void sf2d_mesh_synchronize(long* Mesh_pointer) {
+  Mesh* mp = handle(Mesh_pointer);
+  flush_correctors(mp);
+  check_hip(hipStreamSynchronize(mp->stream), "sf2d_mesh_synchronize");
+}
+
+extern "C" void sf2d_mesh_flush(long* Mesh_pointer) { flush_correctors(handle(Mesh_pointer)); }
+
+extern "C" int sf2d_mesh_colours(long* Mesh_pointer, int iphase) {
+  Mesh* mp = handle(Mesh_pointer);
+  return (iphase == 1 || iphase == 2) ? mp->ncolour[iphase - 1] : -1;
+}
 
 extern "C" void* sf2d_mesh_field(long* Mesh_pointer, int which) {
   Mesh* mp = handle(Mesh_pointer);

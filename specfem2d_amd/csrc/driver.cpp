@@ -754,6 +754,8 @@ int sf2d_sim_forward(void* sim, int nstep_run, double* seis, float* absorb, doub
     mark(step, 9);
   }
   s->it_done = it_end;
+  // the last corrector is queued by kernel_3_a_cuda_; run it inside the measured loop
+  sf2d_mesh_flush(&s->mesh);
   if (timed) {
     hc(hipEventRecord(loop_end, stream));
     hc(hipEventSynchronize(loop_end));

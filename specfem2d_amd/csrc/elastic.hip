@@ -21,6 +21,9 @@ constexpr int FORCE_THREADS = 256;
 constexpr int FORCE_EPB = 10;  // elements per block: 250 of 256 lanes active
 constexpr int NXCD = 8;        // gfx950: 8 accelerator complex dies, workgroups dispatched round-robin
 
+static_assert(Mesh::PATCH_ELEMS == FORCE_EPB, "one workgroup holds one patch");
+static_assert(FORCE_EPB * NGLL2 <= FORCE_THREADS && FORCE_THREADS <= 256, "patch points indexed by 8-bit slots");
+
 __device__ __forceinline__ void atomic_add_f32(float* p, float v) { unsafeAtomicAdd(p, v); }
 
 // ---------------------------------------------------------------------------
@@ -58,10 +61,36 @@ __global__ __launch_bounds__(256) void k_corrector(float2* __restrict__ v, float
   }
 }
 
+// Corrector of the previous step fused with the predictor of this one: the same per-point operations as
+// k_corrector followed by k_predictor, in one pass (56 B per point instead of 88).
+__global__ __launch_bounds__(256) void k_corrector_predictor(float2* __restrict__ d, float2* __restrict__ v,
+                                                             float2* __restrict__ a, const float* __restrict__ rmx,
+                                                             const float* __restrict__ rmz, int n, float dto2_prev,
+                                                             float dt, float dt2o2, float dto2) {
+  int stride = gridDim.x * blockDim.x;
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
+    float2 dd = d[k], vv = v[k], aa = a[k];
+    aa.x = aa.x * rmx[k];
+    aa.y = aa.y * rmz[k];
+    vv.x = vv.x + dto2_prev * aa.x;
+    vv.y = vv.y + dto2_prev * aa.y;
+    dd.x = dd.x + dt * vv.x + dt2o2 * aa.x;
+    dd.y = dd.y + dt * vv.y + dt2o2 * aa.y;
+    vv.x = vv.x + dto2 * aa.x;
+    vv.y = vv.y + dto2 * aa.y;
+    d[k] = dd;
+    v[k] = vv;
+    a[k] = make_float2(0.f, 0.f);
+  }
+}
+
 // ---------------------------------------------------------------------------
-// isotropic elastic element forces, P-SV
+// isotropic elastic element forces, P-SV, for the elements at positions [first, first + nelem).
+// ATOMIC = false: the range is one colour (no two elements share a point), so the assembly is a plain
+// read-modify-write; ATOMIC = true: FP32 atomics, valid for any range.
+template <bool ATOMIC>
 __global__ __launch_bounds__(FORCE_THREADS) void k_elastic_forces(
-    int nelem, int nbatch, int per_xcd, const int* __restrict__ elist, const int* __restrict__ ibool,
+    int first, int nelem, int nbatch, int per_xcd, const int* __restrict__ ibool,
     const float* __restrict__ xix, const float* __restrict__ xiz, const float* __restrict__ gammax,
     const float* __restrict__ gammaz, const float* __restrict__ kappav, const float* __restrict__ muv,
     const float2* __restrict__ displ, float* __restrict__ accel, const Basis basis) {
@@ -91,8 +120,7 @@ __global__ __launch_bounds__(FORCE_THREADS) void k_elastic_forces(
   int g = 0;
   float xixl = 0.f, xizl = 0.f, gaxl = 0.f, gazl = 0.f, kap = 0.f, mu = 0.f;
   if (active) {
-    const int ispec = elist ? elist[e] : e;
-    const size_t l = (size_t)ispec * NGLL2 + p;
+    const size_t l = (size_t)(first + e) * NGLL2 + p;
     g = ibool[l];
     const float2 u = displ[g];
     s_ux[tid] = u.x;
@@ -148,8 +176,141 @@ __global__ __launch_bounds__(FORCE_THREADS) void k_elastic_forces(
     }
     const float fx = -(s_w[j] * t1x + s_w[i] * t2x);
     const float fz = -(s_w[j] * t1z + s_w[i] * t2z);
-    atomic_add_f32(&accel[2 * (size_t)g], fx);
-    atomic_add_f32(&accel[2 * (size_t)g + 1], fz);
+    if (ATOMIC) {
+      atomic_add_f32(&accel[2 * (size_t)g], fx);
+      atomic_add_f32(&accel[2 * (size_t)g + 1], fz);
+    } else {
+      float2* a2 = reinterpret_cast<float2*>(accel) + g;
+      float2 a = *a2;
+      a.x += fx;
+      a.y += fz;
+      *a2 = a;
+    }
+  }
+}
+
+// Coloured-patch variant: one workgroup per patch of <= FORCE_EPB connected elements.  The patch's unique points
+// are gathered once into LDS, the element contributions are summed there (LDS atomics), and every unique point
+// is written back with one read-modify-write: patches of one colour share no point.
+__global__ __launch_bounds__(FORCE_THREADS) void k_elastic_forces_patch(
+    int first_patch, int npatch, int per_xcd, const int* __restrict__ patch_elem_off,
+    const int* __restrict__ patch_node_off, const int* __restrict__ patch_nodes,
+    const unsigned char* __restrict__ elem_slot, const float* __restrict__ xix, const float* __restrict__ xiz,
+    const float* __restrict__ gammax, const float* __restrict__ gammaz, const float* __restrict__ kappav,
+    const float* __restrict__ muv, const float2* __restrict__ displ, float2* __restrict__ accel, const Basis basis) {
+  const int bid = blockIdx.x;
+  const int q = (bid % NXCD) * per_xcd + bid / NXCD;
+  if (q >= npatch) return;
+  const int patch = first_patch + q;
+
+  __shared__ float s_hp[NGLL2], s_hpw[NGLL2], s_w[NGLL];
+  __shared__ float2 s_u[FORCE_THREADS];    // displacement of the patch's unique points
+  __shared__ float2 s_acc[FORCE_THREADS];  // force accumulator of the patch's unique points
+  __shared__ float s_ux[FORCE_THREADS], s_uz[FORCE_THREADS];
+  __shared__ float s_t1x[FORCE_THREADS], s_t1z[FORCE_THREADS], s_t2x[FORCE_THREADS], s_t2z[FORCE_THREADS];
+
+  const int tid = threadIdx.x;
+  if (tid < NGLL2) {
+    s_hp[tid] = basis.hp[tid];
+    s_hpw[tid] = basis.hpw[tid];
+  }
+  if (tid < NGLL) s_w[tid] = basis.w[tid];
+  const int e0 = patch_elem_off[patch], ne = patch_elem_off[patch + 1] - e0;
+  const int n0 = patch_node_off[patch], nn = patch_node_off[patch + 1] - n0;
+  // each of the first nn lanes owns one point of the patch: its displacement goes to LDS, its current
+  // acceleration is fetched now so that the final read-modify-write does not wait on a load
+  int gown = 0;
+  float2 aown = make_float2(0.f, 0.f);
+  if (tid < nn) {
+    gown = patch_nodes[n0 + tid];
+    s_u[tid] = displ[gown];
+    aown = accel[gown];
+    s_acc[tid] = make_float2(0.f, 0.f);
+  }
+  const int le = tid / NGLL2;
+  const int p = tid - le * NGLL2;
+  const int i = p % NGLL;
+  const int j = p / NGLL;
+  const bool active = (tid < FORCE_EPB * NGLL2) && (le < ne);
+  int slot = 0;
+  float xixl = 0.f, xizl = 0.f, gaxl = 0.f, gazl = 0.f, kap = 0.f, mu = 0.f;
+  if (active) {
+    const size_t l = (size_t)(e0 + le) * NGLL2 + p;
+    slot = elem_slot[l];
+    xixl = xix[l];
+    xizl = xiz[l];
+    gaxl = gammax[l];
+    gazl = gammaz[l];
+    kap = kappav[l];
+    mu = muv[l];
+  }
+  __syncthreads();
+  if (active) {
+    const float2 u = s_u[slot];
+    s_ux[tid] = u.x;
+    s_uz[tid] = u.y;
+  }
+  __syncthreads();
+
+  const int base = le * NGLL2;
+  if (active) {
+    float dux_dxi = 0.f, duz_dxi = 0.f, dux_dga = 0.f, duz_dga = 0.f;
+#pragma unroll
+    for (int k = 0; k < NGLL; k++) {
+      const float hi = s_hp[i * NGLL + k];
+      const float hj = s_hp[j * NGLL + k];
+      dux_dxi += s_ux[base + j * NGLL + k] * hi;
+      duz_dxi += s_uz[base + j * NGLL + k] * hi;
+      dux_dga += s_ux[base + k * NGLL + i] * hj;
+      duz_dga += s_uz[base + k * NGLL + i] * hj;
+    }
+    const float dux_dxl = dux_dxi * xixl + dux_dga * gaxl;
+    const float dux_dzl = dux_dxi * xizl + dux_dga * gazl;
+    const float duz_dxl = duz_dxi * xixl + duz_dga * gaxl;
+    const float duz_dzl = duz_dxi * xizl + duz_dga * gazl;
+    const float jac = 1.f / (xixl * gazl - gaxl * xizl);
+    const float lambdal = kap - mu;
+    const float lp2m = kap + mu;
+    const float sxx = lp2m * dux_dxl + lambdal * duz_dzl;
+    const float szz = lp2m * duz_dzl + lambdal * dux_dxl;
+    const float sxz = mu * (duz_dxl + dux_dzl);
+    s_t1x[tid] = jac * (sxx * xixl + sxz * xizl);
+    s_t1z[tid] = jac * (sxz * xixl + szz * xizl);
+    s_t2x[tid] = jac * (sxx * gaxl + sxz * gazl);
+    s_t2z[tid] = jac * (sxz * gaxl + szz * gazl);
+  }
+  __syncthreads();
+  if (active) {
+    float t1x = 0.f, t1z = 0.f, t2x = 0.f, t2z = 0.f;
+#pragma unroll
+    for (int k = 0; k < NGLL; k++) {
+      const float wi = s_hpw[k * NGLL + i];
+      const float wj = s_hpw[k * NGLL + j];
+      t1x += s_t1x[base + j * NGLL + k] * wi;
+      t1z += s_t1z[base + j * NGLL + k] * wi;
+      t2x += s_t2x[base + k * NGLL + i] * wj;
+      t2z += s_t2z[base + k * NGLL + i] * wj;
+    }
+    atomicAdd(&s_acc[slot].x, -(s_w[j] * t1x + s_w[i] * t2x));
+    atomicAdd(&s_acc[slot].y, -(s_w[j] * t1z + s_w[i] * t2z));
+  }
+  __syncthreads();
+  if (tid < nn) {
+    const float2 f = s_acc[tid];
+    aown.x += f.x;
+    aown.y += f.y;
+    accel[gown] = aown;
+  }
+}
+
+// gather per-element arrays into the device element order: dst[pos] = src[ispec_of_pos[pos]]
+template <typename T>
+__global__ __launch_bounds__(256) void k_permute(const T* __restrict__ src, T* __restrict__ dst,
+                                                 const int* __restrict__ ispec_of_pos, int nspec) {
+  const size_t n = (size_t)nspec * NGLL2;
+  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x) {
+    const size_t pos = k / NGLL2, p = k - pos * NGLL2;
+    dst[k] = src[(size_t)ispec_of_pos[pos] * NGLL2 + p];
   }
 }
 
@@ -402,15 +563,53 @@ void launch_newmark_corrector(float* v, float* a, const float* rmx, const float*
   check_hip(hipGetLastError(), "k_corrector launch");
 }
 
-void launch_elastic_forces(const Mesh* mp, const int* elist, int nelem, const float* displ, float* accel) {
+void launch_newmark_corrector_predictor(float* d, float* v, float* a, const float* rmx, const float* rmz, int nglob,
+                                       float dto2_prev, float dt, float dt2o2, float dto2, hipStream_t s) {
+  if (nglob <= 0) return;
+  hipLaunchKernelGGL(k_corrector_predictor, dim3(blocks_for(nglob, 256, 65536)), dim3(256), 0, s, (float2*)d,
+                     (float2*)v, (float2*)a, rmx, rmz, nglob, dto2_prev, dt, dt2o2, dto2);
+  check_hip(hipGetLastError(), "k_corrector_predictor launch");
+}
+
+void launch_elastic_forces(const Mesh* mp, int first, int nelem, int atomic, const float* displ, float* accel) {
   if (nelem <= 0) return;
+  if (first < 0 || first + nelem > mp->nspec) fail("element range [%d, %d) outside the mesh", first, first + nelem);
   const int nbatch = (nelem + FORCE_EPB - 1) / FORCE_EPB;
   const int per_xcd = (nbatch + NXCD - 1) / NXCD;
   const int nblocks = per_xcd * NXCD;
-  hipLaunchKernelGGL(k_elastic_forces, dim3(nblocks), dim3(FORCE_THREADS), 0, mp->stream, nelem, nbatch, per_xcd,
-                     elist, mp->d_ibool, mp->d_xix, mp->d_xiz, mp->d_gammax, mp->d_gammaz, mp->d_kappav, mp->d_muv,
-                     (const float2*)displ, accel, mp->basis);
+  if (atomic)
+    hipLaunchKernelGGL(k_elastic_forces<true>, dim3(nblocks), dim3(FORCE_THREADS), 0, mp->stream, first, nelem, nbatch,
+                       per_xcd, mp->d_ibool, mp->d_xix, mp->d_xiz, mp->d_gammax, mp->d_gammaz, mp->d_kappav,
+                       mp->d_muv, (const float2*)displ, accel, mp->basis);
+  else
+    hipLaunchKernelGGL(k_elastic_forces<false>, dim3(nblocks), dim3(FORCE_THREADS), 0, mp->stream, first, nelem,
+                       nbatch, per_xcd, mp->d_ibool, mp->d_xix, mp->d_xiz, mp->d_gammax, mp->d_gammaz, mp->d_kappav,
+                       mp->d_muv, (const float2*)displ, accel, mp->basis);
   check_hip(hipGetLastError(), "k_elastic_forces launch");
+}
+
+void launch_elastic_forces_patch(const Mesh* mp, int first_patch, int npatch, const float* displ, float* accel) {
+  if (npatch <= 0) return;
+  if (first_patch < 0 || first_patch + npatch > mp->npatch)
+    fail("patch range [%d, %d) outside the %d patches", first_patch, first_patch + npatch, mp->npatch);
+  const int per_xcd = (npatch + NXCD - 1) / NXCD;
+  hipLaunchKernelGGL(k_elastic_forces_patch, dim3(per_xcd * NXCD), dim3(FORCE_THREADS), 0, mp->stream, first_patch,
+                     npatch, per_xcd, mp->d_patch_elem_off, mp->d_patch_node_off, mp->d_patch_nodes, mp->d_elem_slot,
+                     mp->d_xix, mp->d_xiz, mp->d_gammax, mp->d_gammaz, mp->d_kappav, mp->d_muv, (const float2*)displ,
+                     (float2*)accel, mp->basis);
+  check_hip(hipGetLastError(), "k_elastic_forces_patch launch");
+}
+
+void launch_permute_elements(const float* src, float* dst, const int* ispec_of_pos, int nspec, hipStream_t s) {
+  hipLaunchKernelGGL(k_permute<float>, dim3(blocks_for((long)nspec * NGLL2, 256, 65536)), dim3(256), 0, s, src, dst,
+                     ispec_of_pos, nspec);
+  check_hip(hipGetLastError(), "k_permute launch");
+}
+
+void launch_permute_elements_int(const int* src, int* dst, const int* ispec_of_pos, int nspec, hipStream_t s) {
+  hipLaunchKernelGGL(k_permute<int>, dim3(blocks_for((long)nspec * NGLL2, 256, 65536)), dim3(256), 0, s, src, dst,
+                     ispec_of_pos, nspec);
+  check_hip(hipGetLastError(), "k_permute launch");
 }
 
 void launch_stacey(const Mesh* mp, const float* veloc, float* accel, int write_store) {

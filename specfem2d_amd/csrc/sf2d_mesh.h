@@ -14,6 +14,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <vector>
 
 namespace sf2d {
 
@@ -49,10 +50,31 @@ struct Mesh {
   float *d_b_displ = nullptr, *d_b_veloc = nullptr, *d_b_accel = nullptr;
   float *d_rmassx = nullptr, *d_rmassz = nullptr;
 
-  // element phase lists (0-based), [2][num_phase]; identity lists are not stored
+  // Element order on the device.  prepare_fields_elastic_device_ sorts the elements by (phase, colour):
+  // a greedy colouring of each phase list in which no two elements of one colour share a GLL point, so
+  // the force kernel assembles with plain read-modify-writes, one launch per colour.  All per-element
+  // device arrays are stored in this order; pos_of_ispec / ispec_of_pos translate.
+  static constexpr int MAX_COLOURS = 32;
   int num_phase_ispec_elastic = 0;
-  int* d_phase_ispec[2] = {nullptr, nullptr};
-  int phase_is_identity[2] = {0, 0};
+  int phase_count[2] = {0, 0};
+  int ncolour[2] = {0, 0};
+  int colour_start[2][MAX_COLOURS] = {};
+  int colour_count[2][MAX_COLOURS] = {};
+  // assembly: 0 coloured patches (default), 1 coloured elements, 2 FP32 atomics (SF2D_ASSEMBLY=patch|colour|atomic)
+  int assembly = 0;
+  // coloured patches: connected groups of <= PATCH_ELEMS consecutive elements of a phase list; one workgroup per
+  // patch gathers the patch's unique points into LDS and writes each back once.  In patch mode the colour
+  // ranges above count patches, and the element order is (phase, colour, patch, element in patch).
+  static constexpr int PATCH_ELEMS = 10;
+  int npatch = 0;
+  int* d_patch_elem_off = nullptr;   // [npatch+1] first element (device order) of each patch
+  int* d_patch_node_off = nullptr;   // [npatch+1] offsets into d_patch_nodes
+  int* d_patch_nodes = nullptr;      // unique global points of each patch, ascending
+  unsigned char* d_elem_slot = nullptr;  // [nspec][25] index of each element point in its patch's point list
+  std::vector<int> h_ibool;          // 0-based, ispec order, kept until the elements are sorted
+  std::vector<int> pos_of_ispec, ispec_of_pos;
+  std::vector<int> h_src_ispec, h_rec_ispec;  // 0-based ispec, as passed
+  int sorted = 0;
 
   // sources
   int nsources_local = 0;
@@ -91,9 +113,18 @@ struct Mesh {
   // kernels
   float *d_rho_kl = nullptr, *d_mu_kl = nullptr, *d_kappa_kl = nullptr;
 
+  // Deferred Newmark corrector: kernel_3_a_cuda_ records it and the next update_displacement_cuda_
+  // applies corrector and predictor in one pass over the points; any entry point that reads veloc or
+  // accel (or writes the fields) applies it first, so callers observe the reference's sequence.
+  int pending_corrector[2] = {0, 0};  // forward/adjoint wavefield, backward wavefield
+  float pending_dto2[2] = {0.f, 0.f};
+
   // scratch
   unsigned int* d_scratch_u32 = nullptr;  // reductions
 };
+
+// applies a deferred corrector now (no-op when none is pending)
+void flush_correctors(Mesh* mp);
 
 // Errors follow the reference's convention: message, error file, exit(EXIT_FAILURE).
 [[noreturn]] void fail(const char* fmt, ...);
@@ -138,7 +169,12 @@ void launch_newmark_predictor(float* d, float* v, float* a, int nglob, float dt,
                               hipStream_t s);
 void launch_newmark_corrector(float* v, float* a, const float* rmx, const float* rmz, int nglob, float dto2,
                               int apply_mass, hipStream_t s);
-void launch_elastic_forces(const Mesh* mp, const int* elist, int nelem, const float* displ, float* accel);
+void launch_newmark_corrector_predictor(float* d, float* v, float* a, const float* rmx, const float* rmz, int nglob,
+                                       float dto2_prev, float dt, float dt2o2, float dto2, hipStream_t s);
+void launch_elastic_forces(const Mesh* mp, int first, int nelem, int atomic, const float* displ, float* accel);
+void launch_elastic_forces_patch(const Mesh* mp, int first_patch, int npatch, const float* displ, float* accel);
+void launch_permute_elements(const float* src, float* dst, const int* ispec_of_pos, int nspec, hipStream_t s);
+void launch_permute_elements_int(const int* src, int* dst, const int* ispec_of_pos, int nspec, hipStream_t s);
 void launch_stacey(const Mesh* mp, const float* veloc, float* accel, int write_store);
 void launch_stacey_backward(const Mesh* mp, float* b_accel);
 void launch_sources(const Mesh* mp, float* accel, int it);

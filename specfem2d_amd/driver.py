@@ -80,6 +80,10 @@ def load_libraries():
         getattr(host, f).argtypes = [P, P, P, P]
     gpu.sf2d_mesh_stream.argtypes = [C.POINTER(C.c_long)]
     gpu.sf2d_mesh_stream.restype = P
+    gpu.sf2d_mesh_colours.argtypes = [C.POINTER(C.c_long), C.c_int]
+    gpu.sf2d_mesh_colours.restype = C.c_int
+    gpu.sf2d_mesh_synchronize.argtypes = [C.POINTER(C.c_long)]
+    gpu.sf2d_mesh_flush.argtypes = [C.POINTER(C.c_long)]
     gpu.sf2d_mesh_field.argtypes = [C.POINTER(C.c_long), C.c_int]
     gpu.sf2d_mesh_field.restype = P
     _lib_cache["gpu"], _lib_cache["host"] = gpu, host
@@ -169,6 +173,10 @@ class Simulation:
     def stream(self) -> int:
         mp = C.c_long(self.mesh_pointer)
         return self.gpu.sf2d_mesh_stream(C.byref(mp))
+
+    def colours(self, iphase: int = 2) -> int:
+        mp = C.c_long(self.mesh_pointer)
+        return self.gpu.sf2d_mesh_colours(C.byref(mp), int(iphase))
 
     def device_field(self, which: int) -> int:
         mp = C.c_long(self.mesh_pointer)

@@ -1,0 +1,71 @@
+"""Host-side checks of the element assembly plan (no GPU): coloured patches, coloured elements, atomics.
+
+Invariants (sf2d_plan_validate): every listed element appears once in the device order; no two units
+(patches or elements) of one colour share a GLL point; patch slots address their patch's point list;
+patches hold at most 10 elements and 256 points."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from oracle.oracle import OracleSim
+from specfem2d_amd import config, driver
+
+
+def validate(ibool, nglob, lists, mode):
+    gpu, _ = driver.load_libraries()
+    nspec = ibool.shape[0]
+    np_ = lists.shape[1]
+    ib = np.ascontiguousarray(ibool.reshape(-1), dtype=np.int32)
+    ph = np.ascontiguousarray(lists.reshape(-1), dtype=np.int32)
+    stats = np.zeros(4, np.int32)
+    f = gpu.sf2d_plan_validate
+    f.restype = C.c_int
+    rc = f(C.c_int(nspec), C.c_int(nglob), ib.ctypes.data_as(C.c_void_p), C.c_int(np_),
+           ph.ctypes.data_as(C.c_void_p), C.c_int(mode), stats.ctypes.data_as(C.c_void_p))
+    return rc, stats
+
+
+def inner_only(nspec):
+    lists = np.zeros((2, nspec), np.int32)
+    lists[1] = np.arange(1, nspec + 1)
+    return lists
+
+
+MESHES = {
+    "tromp": lambda: config.tromp2005_kernel(nstep=2),
+    "c1": lambda: config.plain_elastic_c1(nstep=2),
+    "layers": lambda: config.small_test_mesh(nx=13, nz=10, nstep=2).replace(
+        interfaces=[-1000.0, -620.0, -330.0, 0.0], nz_layers=[3, 4, 3], xmax=1390.0),
+}
+
+
+@pytest.mark.parametrize("name", sorted(MESHES))
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_plan_invariants_structured(name, mode):
+    cfg = MESHES[name]()
+    o = OracleSim(cfg, "sp")
+    rc, stats = validate(o.ibool(), o.nglob, inner_only(o.nspec), mode)
+    assert rc == 0
+    if mode in (0, 1):
+        assert stats[0] == 0 and stats[1] == 4  # 2 x 2 checkerboard of elements or of row patches
+    if mode == 0:
+        assert stats[2] == sum((cfg.nx + 9) // 10 for _ in range(cfg.nz))  # row segments of <= 10 elements
+        assert stats[3] <= 41 * 5
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_plan_invariants_shuffled_and_split_lists(mode):
+    # an arbitrary element order and an outer/inner split, as a partitioned mesh would pass
+    cfg = config.small_test_mesh(nx=17, nz=11, nstep=2)
+    o = OracleSim(cfg, "sp")
+    rng = np.random.default_rng(4)
+    perm = rng.permutation(o.nspec) + 1
+    nout = 37
+    lists = np.zeros((2, o.nspec - nout), np.int32)
+    lists[0, :nout] = perm[:nout]
+    lists[1, :] = perm[nout:]
+    rc, stats = validate(o.ibool(), o.nglob, lists, mode)
+    assert rc == 0
+    if mode != 2:
+        assert stats[0] >= 1 and stats[1] >= 4

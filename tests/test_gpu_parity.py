@@ -118,3 +118,21 @@ def test_max_norm_entry_point():
     sim.gpu.get_norm_elastic_from_device_(ctypes.byref(norm), ctypes.byref(mp), ctypes.byref(fwd))
     assert norm.value == pytest.approx(float(np.sqrt((d.astype(np.float64) ** 2).sum(axis=1)).max()), rel=1e-6)
     sim.close()
+
+
+@pytest.mark.parametrize("mode", ["patch", "colour", "atomic"])
+def test_assembly_modes_agree_with_oracle(mode, monkeypatch):
+    # patch: coloured patches of <= 10 elements assembled in LDS (default);
+    # colour: coloured elements, plain read-modify-write per colour;
+    # atomic: FP32 atomics, as the reference GPU code assembles
+    monkeypatch.setenv("SF2D_ASSEMBLY", mode)
+    cfg = config.small_test_mesh(nx=23, nz=9, nstep=500, gll=True, seed=9)
+    sim = Simulation(cfg)
+    if mode in ("patch", "colour"):
+        assert sim.colours(2) == 4  # structured quads: 2 x 2 checkerboard of elements or row patches
+    seis, _ = sim.forward()
+    ref = OracleSim(cfg, "sp").forward()
+    r = rel(seis, ref)
+    print(f"assembly {mode}: rel-L2 vs FP32 oracle = {r:.3e}")
+    assert r < TOL
+    sim.close()
