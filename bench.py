@@ -12,7 +12,8 @@ layer, fluid-solid coupling and CPML of that configuration are not part of this 
 so the whole mesh is elastic with Stacey edges.
 
 Prints one JSON line with the throughput, the roofline of the element-force kernel (HIP
-events on the library stream around every force launch inside the timed region) and a CPU
+events on the library stream around the force launches of every step inside the timed region;
+one launch per element colour, so a launch covers 1/ncolour of the elements) and a CPU
 baseline (the C oracle, one thread, on a bounded piece of the same workload).
 """
 from __future__ import annotations
@@ -20,6 +21,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -82,15 +84,28 @@ def cpu_baseline(name: str, target_seconds: float = 15.0):
 
 
 def read_traffic(name: str):
-    """HBM bytes per force launch from committed rocprofv3 PMC summaries, if present."""
-    path = os.path.join(ROOT, "profiles", f"elastic_forces_{name}_traffic.json")
+    """HBM bytes per force launch (FETCH_SIZE + WRITE_SIZE, calibrated against a plain copy) from the
+    committed rocprofv3 PMC summary of this workload, if present."""
+    path = os.path.join(ROOT, "profiles", f"elastic_forces_{name}_traffic.md")
     if not os.path.exists(path):
         return None
-    try:
-        with open(path) as f:
-            return float(json.load(f)["bytes_per_launch"])
-    except Exception:
-        return None
+    with open(path) as f:
+        m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", f.read(), re.M)
+    return float(m.group(1)) if m else None
+
+
+def copy_calibration(mib: int, reps: int = 5):
+    """A plain device-to-device copy of known size, for scaling the HBM byte counters: each launch reads
+    and writes exactly ``mib`` MiB."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.arange(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_read": n * 4, "copy_calibration_bytes_written": n * 4,
+                      "launches": reps}), file=sys.stderr)
 
 
 def main():
@@ -101,6 +116,8 @@ def main():
     ap.add_argument("--workload", default="c2_mesh_elastic_2000x1000", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="after the timed region, run a plain copy of this size (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; the multi-GPU layout (shots per GPU, "
@@ -132,8 +149,10 @@ def main():
         sys.exit(1)
     nglob, nspec = sim.nglob, sim.nspec
     value = nglob * args.steps / wall
-    fk_ms = timing["forces_ms"] / args.steps
-    fk_bytes = force_kernel_bytes(nspec, nglob)
+    # one force launch per colour; the events bracket all launches of a step
+    launches = sim.colours(2)
+    fk_ms = timing["forces_ms"] / args.steps / launches
+    fk_bytes = force_kernel_bytes(nspec, nglob) / launches
     achieved = fk_bytes / (fk_ms * 1e-3) / 1e9
     traffic = read_traffic(args.workload)
     out = {
@@ -161,7 +180,9 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_elastic_forces",
+            "kernel": "k_elastic_forces_patch" if os.environ.get("SF2D_ASSEMBLY", "patch") == "patch"
+                      else "k_elastic_forces",
+            "launches_per_step": launches,
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
@@ -174,6 +195,8 @@ def main():
         "setup_s": t_setup,
     }
     sim.close()
+    if args.copy_calibration_mib:
+        copy_calibration(args.copy_calibration_mib)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.workload, args.cpu_seconds)
     print(json.dumps(out))

@@ -78,6 +78,14 @@ void sf2d_sim_get_stacey(void* sim, int* ispec, int* ij, float* normal, float* j
    forces ms, boundary+sources ms, corrector ms, seismogram ms}: the loop and the element-force
    launches always, the other phases only when timing[0] < 0 on entry.  Returns 0 on success. */
 int sf2d_sim_forward(void* sim, int nstep_run, double* seis, float* absorb, double* timing);
+/* Adjoint run (SIMULATION_TYPE=3, simulation created with simulation_type 3): adj[nrec][2][NSTEP]
+   adjoint source traces (x then z), absorb[NSTEP][nabs][5][2] and the last frame from a save_forward
+   run of the same model.  Replays the GPU branch of iterate_time.F90 for SIMULATION_TYPE=3 with both
+   wavefields; records the adjoint displacement at the receivers into seis[nrec][2][NSTEP] and the
+   kernels K_rho, K_mu, K_kappa (scaled as save_adjoint_kernels.f90 writes them) into
+   kernels[3][nspec][25] in ispec order.  Returns 0 on success. */
+int sf2d_sim_adjoint(void* sim, const float* adj, const float* absorb, const float* last_displ,
+                     const float* last_veloc, const float* last_accel, double* seis, float* kernels);
 /* zero the wavefields and restart the step counter */
 void sf2d_sim_reset(void* sim);
 int sf2d_sim_steps_done(void* sim);

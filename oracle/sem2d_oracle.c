@@ -14,7 +14,7 @@
  *
  * Every routine cites the reference file it restates.  Parity status of
  * each part (what pins it) is listed in the test suite's header
- * (tests/test_oracle_golden.py) and in the design notes.
+ * (tests/test_oracle_golden.py, tests/test_oracle_adjoint.py).
  */
 #include <math.h>
 #include <stdint.h>
@@ -32,7 +32,7 @@
 #define MAXLAYERS 32
 
 /* ------------------------------------------------------------------ */
-/* public parameter blocks (mirrored by tests/oracle.py via ctypes)    */
+/* public parameter blocks (mirrored by oracle/oracle.py via ctypes)    */
 /* ------------------------------------------------------------------ */
 typedef struct {
   int nx, nz;                 /* elements along x and z (nxread, nzread) */

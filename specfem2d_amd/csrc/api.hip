@@ -532,6 +532,7 @@ void prepare_stacey_device_(long* Mesh_pointer, int* ACOUSTIC_SIMULATION, int* E
 void prepare_sim2_or_3_const_device_(long* Mesh_pointer, int* nadj_rec_local, realw* h_source_adjoint,
                                      int* NSTEP) {  // prepare_mesh_constants_cuda.cu:815
   Mesh* mp = handle(Mesh_pointer);
+  device_free(mp->d_source_adjoint);  // a new adjoint run may replace the sources
   mp->nadj_rec_local = *nadj_rec_local;
   if (mp->nadj_rec_local > 0) {
     if (mp->nadj_rec_local != mp->nrec_local) fail("nadj_rec_local %d != nrec_local %d", mp->nadj_rec_local, mp->nrec_local);

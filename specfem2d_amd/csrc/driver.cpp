@@ -786,6 +786,98 @@ int sf2d_sim_forward(void* sim, int nstep_run, double* seis, float* absorb, doub
   return 0;
 }
 
+int sf2d_sim_adjoint(void* sim, const float* adj, const float* absorb, const float* last_displ,
+                     const float* last_veloc, const float* last_accel, double* seis, float* kernels) {
+  Sim* s = (Sim*)sim;
+  if (!s || !s->mesh || s->simulation_type != 3) return 1;
+  const int nstep = s->p.nstep;
+  // adjoint sources: Fortran source_adjoint(nrecloc, NSTEP, 2) (setup_sources_receivers.F90:578)
+  {
+    std::vector<float> sa((size_t)2 * nstep * std::max(1, s->nrec), 0.f);
+    for (int r = 0; r < s->nrec; r++)
+      for (int c = 0; c < 2; c++)
+        for (int k = 0; k < nstep; k++) sa[((size_t)c * nstep + k) * s->nrec + r] = adj[((size_t)r * 2 + c) * nstep + k];
+    int nadj = s->nrec, ns = nstep;
+    prepare_sim2_or_3_const_device_(&s->mesh, &nadj, sa.data(), &ns);
+  }
+  // Newmark constants; the reconstructed field runs backwards (prepare_timerun.F90:262-277)
+  float deltat = (float)s->p.dt;
+  float deltatover2 = (float)(0.5 * (double)deltat);
+  float deltatsqover2 = (float)(0.5 * (double)deltat * (double)deltat);
+  float b_deltat = -deltat;
+  float b_deltatover2 = (float)(0.5 * (double)b_deltat);
+  float b_deltatsqover2 = (float)(0.5 * (double)b_deltat * (double)b_deltat);
+  int nspec_outer = 0, nspec_inner = s->nspec, zero = 0, one = 1;
+  int i_sig = 1, nlength = nstep, it_end = nstep, nadj = s->nrec, ns = nstep;
+  int elastic_sim = 1, acoustic_sim = 0;
+  s->sisux.assign((size_t)std::max(1, s->nrec) * nlength, 0.0);
+  s->sisuz.assign(s->sisux.size(), 0.0);
+  const int counts[5] = {0, s->nspec_side[1], s->nspec_side[2], s->nspec_side[3], s->nspec_side[4]};
+  std::vector<float> slice[5];
+  for (int c = 1; c <= 4; c++) slice[c].assign((size_t)2 * NGLL * std::max(1, counts[c]), 0.f);
+  int size = 2 * s->nglob;
+  for (int it = 1; it <= nstep; it++) {
+    update_displacement_cuda_(&s->mesh, &deltat, &deltatsqover2, &deltatover2, &b_deltat, &b_deltatsqover2,
+                              &b_deltatover2);
+    const int it_tmp = nstep - it + 1;
+    for (int iphase = 1; iphase <= 2; iphase++) {
+      compute_forces_viscoelastic_cuda_(&s->mesh, &iphase, &deltat, &nspec_outer, &nspec_inner, &zero, &zero, &one,
+                                        &one);
+      if (iphase == 1) {
+        if (s->p.stacey && s->nabs > 0) {
+          // the Fortran caller hands the slices of step NSTEP-it+1 (compute_gpu_elastic.f90:252-258)
+          for (int f = 0; f < s->nabs; f++) {
+            const int c = s->edge_abs[f];
+            const int ib = s->ib0[f] - 1;
+            for (int k = 0; k < NGLL; k++)
+              for (int d = 0; d < 2; d++)
+                slice[c][((size_t)ib * NGLL + k) * 2 + d] = absorb[(((size_t)(it_tmp - 1) * s->nabs + f) * NGLL + k) * 2 + d];
+          }
+          compute_stacey_viscoelastic_cuda_(&s->mesh, &iphase, slice[4].data(), slice[2].data(), slice[3].data(),
+                                            slice[1].data(), &one, &one, &zero);
+        }
+        int itt = it_tmp;
+        if (s->nrec > 0) add_sources_el_sim_type_2_or_3_(&s->mesh, &iphase, &itt, &nadj, &ns);
+        compute_add_sources_el_s3_cuda_(&s->mesh, &iphase, &itt);
+      }
+    }
+    kernel_3_a_cuda_(&s->mesh, &deltatover2, &b_deltatover2, &one, &one);
+    if (it == 1) {
+      // read_forward_arrays.f90:77-98: the last forward frame replaces the reconstructed field
+      std::vector<float> d(last_displ, last_displ + size), v(last_veloc, last_veloc + size),
+          a(last_accel, last_accel + size);
+      transfer_b_fields_to_device_(&size, d.data(), v.data(), a.data(), &s->mesh);
+    }
+    if (s->nrec > 0) {
+      int cur = it;
+      compute_seismograms_cuda_(&s->mesh, &i_sig, s->sisux.data(), s->sisuz.data(), &cur, &nlength, &elastic_sim,
+                                &acoustic_sim, &zero, &zero, &it, &it_end);
+    }
+    compute_kernels_elastic_cuda_(&s->mesh, &deltat);
+  }
+  s->it_done = nstep;
+  sf2d_mesh_synchronize(&s->mesh);
+  if (seis)
+    for (int r = 0; r < s->nrec; r++)
+      for (int k = 0; k < nstep; k++) {
+        seis[((size_t)r * 2 + 0) * nstep + k] = s->sisux[(size_t)r * nlength + k];
+        seis[((size_t)r * 2 + 1) * nstep + k] = s->sisuz[(size_t)r * nlength + k];
+      }
+  if (kernels) {
+    // transfer_kernels_el_to_host + save_adjoint_kernels.f90:100-140 scaling
+    const size_t ne = (size_t)s->nspec * NGLL2;
+    std::vector<float> rk(ne), mk(ne), kk(ne);
+    int nsp = s->nspec;
+    transfer_kernels_el_to_host_(&s->mesh, rk.data(), mk.data(), kk.data(), &nsp);
+    for (size_t l = 0; l < ne; l++) {
+      kernels[l] = -s->rho[l] * deltat * rk[l];
+      kernels[ne + l] = (float)(-2.0 * (double)s->mu[l] * (double)deltat * (double)mk[l]);
+      kernels[2 * ne + l] = -s->kappa[l] * deltat * kk[l];
+    }
+  }
+  return 0;
+}
+
 void sf2d_sim_reset(void* sim) {
   Sim* s = (Sim*)sim;
   if (!s || !s->mesh) return;

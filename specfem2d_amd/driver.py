@@ -73,6 +73,8 @@ def load_libraries():
     host.sf2d_sim_get_stacey.argtypes = [P, P, P, P, P, P, P]
     host.sf2d_sim_forward.argtypes = [P, C.c_int, P, P, P]
     host.sf2d_sim_forward.restype = C.c_int
+    host.sf2d_sim_adjoint.argtypes = [P, P, P, P, P, P, P, P]
+    host.sf2d_sim_adjoint.restype = C.c_int
     host.sf2d_sim_reset.argtypes = [P]
     host.sf2d_sim_steps_done.argtypes = [P]
     host.sf2d_sim_steps_done.restype = C.c_int
@@ -255,6 +257,26 @@ class Simulation:
                       "seismograms_ms"), timing.tolist()))
         seis = seis[: self.cfg.nrec]
         return (seis, t, store) if save_stacey else (seis, t)
+
+    def adjoint(self, adj, absorb, lastframe):
+        """SIMULATION_TYPE=3 run on the GPU (the simulation must be created with simulation_type=3).
+        ``adj`` [nrec,2,nstep] adjoint source traces, ``absorb`` the Stacey storage and ``lastframe``
+        (displ, veloc, accel) of a save_forward run.  Returns (adjoint seismograms [nrec,2,nstep],
+        kernels dict rho/mu/kappa of [nspec,25] arrays)."""
+        self._need_device()
+        if self.simulation_type != 3:
+            raise RuntimeError("create the simulation with simulation_type=3 for an adjoint run")
+        n = self.cfg.nstep
+        a = np.ascontiguousarray(np.asarray(adj, np.float32).reshape(-1))
+        st = np.ascontiguousarray(np.asarray(absorb, np.float32).reshape(-1)) if self.nabs else np.zeros(1, np.float32)
+        d, v, ac = (np.ascontiguousarray(np.asarray(x, np.float32).reshape(-1)) for x in lastframe)
+        seis = np.zeros((max(1, self.cfg.nrec), 2, n))
+        ker = np.zeros(3 * self.nspec * NGLL2, np.float32)
+        rc = self.host.sf2d_sim_adjoint(self.h, _ptr(a), _ptr(st), _ptr(d), _ptr(v), _ptr(ac), _ptr(seis), _ptr(ker))
+        if rc != 0:
+            raise RuntimeError("sf2d_sim_adjoint failed")
+        ker = ker.reshape(3, self.nspec, NGLL2)
+        return seis[: self.cfg.nrec], {"rho": ker[0], "mu": ker[1], "kappa": ker[2]}
 
     def fields(self):
         self._need_device()

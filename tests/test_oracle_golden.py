@@ -208,3 +208,20 @@ def test_ricker_stf_properties():
     tz = 1 / (np.sqrt(2) * np.pi * f0)
     s = np.interp([tz, -tz], t, stf[0])
     assert np.all(np.abs(s) < 0.02 * factor)
+
+
+def test_gll_model_mode_matches_the_pinned_table_mode():
+    """MODEL=gll materials (setup_mesh.F90:680-687: mu = rho vs^2, kappa = rho vp^2 - mu) against the
+    Par_file table path (lambda = rho vp^2 - 2 mu, kappa = lambda + mu, vp recomputed; pinned bit-exactly by
+    REF_SEIS): a homogeneous per-point model equal to the table gives the same moduli to FP64 rounding, so
+    the same seismograms; a mistaken 3D bulk modulus (rho vp^2 - 4/3 mu) would move kappa by 1/3 mu."""
+    base = config.small_test_mesh(nx=8, nz=6, nstep=300)
+    n = base.nspec * 25
+    gll = base.replace(material_mode=1, rho=np.full(n, base.rho), vp=np.full(n, base.vp), vs=np.full(n, base.vs))
+    a, b = OracleSim(base, "dp"), OracleSim(gll, "dp")
+    for name in ("kappa", "mu", "rho_vp", "rho_vs"):
+        np.testing.assert_allclose(b.elem_array(name), a.elem_array(name), rtol=1e-14)
+    mu = base.rho * base.vs ** 2
+    np.testing.assert_allclose(b.elem_array("kappa"), base.rho * base.vp ** 2 - mu, rtol=1e-14)
+    sa, sb = a.forward(), b.forward()
+    assert np.linalg.norm(sa - sb) <= 1e-12 * np.linalg.norm(sa)
