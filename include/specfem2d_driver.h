@@ -86,6 +86,17 @@ int sf2d_sim_forward(void* sim, int nstep_run, double* seis, float* absorb, doub
    kernels[3][nspec][25] in ispec order.  Returns 0 on success. */
 int sf2d_sim_adjoint(void* sim, const float* adj, const float* absorb, const float* last_displ,
                      const float* last_veloc, const float* last_accel, double* seis, float* kernels);
+/* replaces the model (MODEL=gll parameters rho, vp, vs per GLL point, [nspec][25]) of a simulation: moduli,
+   Stacey impedances and mass matrix are recomputed and uploaded.  Returns 0 on success. */
+int sf2d_sim_set_model(void* sim, const double* rho, const double* vp, const double* vs);
+/* Exact discrete gradient of chi = 1/2 dt sum_n sum_r |s_n - o_n|^2 (obs[nrec][2][NSTEP], displacement)
+   over a full run from rest.  The forward call runs all NSTEP steps keeping a checkpoint every `chunk` steps in
+   device memory and returns chi (and the seismograms into seis if not NULL); the backward call recomputes
+   each segment from its checkpoint and runs the reverse sweep, writing d chi / d (rho, vp, vs) into
+   grad[3][nspec][25] and, if parts is not NULL, d chi / d (kappa, mu, rho through the mass, rho vp, rho vs)
+   into parts[5][nspec][25].  Return 0 on success. */
+int sf2d_sim_gradient_forward(void* sim, const double* obs, int chunk, double* chi, double* seis);
+int sf2d_sim_gradient_backward(void* sim, double* grad, double* parts);
 /* zero the wavefields and restart the step counter */
 void sf2d_sim_reset(void* sim);
 int sf2d_sim_steps_done(void* sim);

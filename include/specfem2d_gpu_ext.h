@@ -22,13 +22,45 @@ void sf2d_mesh_synchronize(long* Mesh_pointer);
 int sf2d_mesh_colours(long* Mesh_pointer, int iphase);
 /* host-only check of the element assembly plan built for ibool (1-based, [nspec][25]) and the phase
    lists (Fortran phase_ispec_inner_elastic(np,2), zero-padded); mode 0 coloured patches, 1 coloured
-   elements, 2 atomics.  stats = {colours phase 1, colours phase 2, patches, max points per patch}.
-   Returns 0 when the plan is valid. */
+   elements, 2 atomics.  stats[6] = {colours phase 1, colours phase 2, patches, max points per patch,
+   halo entries of all patches, points owned by patches}.  Returns 0 when the plan is valid; a positive
+   code names the violated invariant (1-2 element order, 3 patch size, 4 slots, 5 colour conflict,
+   6 point numbering not a permutation, 7 owned ranges overlap, 8 halo list, 9 unused patch point). */
 int sf2d_plan_validate(int nspec, int nglob, const int* ibool_1based, int np, const int* phase_lists, int mode,
                        int* stats);
 /* device pointer of a field: 0 displ 1 veloc 2 accel 3 b_displ 4 b_veloc 5 b_accel
-   6 rho_kl 7 mu_kl 8 kappa_kl (NULL when not allocated) */
+   6 rho_kl 7 mu_kl 8 kappa_kl (NULL when not allocated).  On the device, point arrays are in the
+   library's internal point numbering and kernels in its element order; the transfer entry points
+   return the caller's numbering. */
 void* sf2d_mesh_field(long* Mesh_pointer, int which);
+
+/* ---------------------------------------------------------------------------
+ * Exact discrete gradient of the L2 waveform misfit (reverse mode of the elastic time loop with
+ * checkpointed recomputation).  No counterpart in the reference, whose SIMULATION_TYPE=3 path reconstructs
+ * the forward field backwards instead; the host driver (specfem2d_driver.h) sequences these calls. */
+/* allocates checkpoints every `chunk` steps, one segment of `chunk` step states and the accumulators;
+   returns the number of checkpoints */
+int sf2d_grad_prepare(long* Mesh_pointer, int chunk, int nrec);
+void sf2d_grad_release(long* Mesh_pointer);
+/* copies the fields after a forward step (d_n, predicted v, unscaled force r_n; corrector pending) into
+   checkpoint (kind 0) or segment (kind 1) slot */
+void sf2d_grad_save_state(long* Mesh_pointer, int kind, int slot);
+/* restores a checkpoint into the fields and re-arms the corrector of that step */
+void sf2d_grad_restore_checkpoint(long* Mesh_pointer, int slot, float dto2);
+/* residuals dt (s - o) [nstep][nrec][2]; zeroes the adjoint state and the accumulators */
+void sf2d_grad_begin_reverse(long* Mesh_pointer, const float* res);
+/* reverse step n (1-based) from segment slot; first = 1 for n = NSTEP */
+void sf2d_grad_reverse_step(long* Mesh_pointer, int n, int slot, int first, float dt);
+/* adds the FP32 accumulators of the segment into the FP64 totals */
+void sf2d_grad_flush(long* Mesh_pointer);
+/* d chi / d kappa, d mu [nspec][25] (ispec order), d chi / d M [nglob][2] (caller numbering), Stacey traction
+   parts d chi / d (rho vp), d (rho vs) per absorbing face point [nabs][5] */
+void sf2d_grad_result(long* Mesh_pointer, double* g_kappa, double* g_mu, double* g_mass, double* g_rho_vp,
+                      double* g_rho_vs);
+/* replaces kappa, mu, rho vp, rho vs ([nspec][25], ispec order) and rmassx, rmassz ([nglob], caller numbering);
+   NULL leaves an array unchanged */
+void sf2d_mesh_set_materials(long* Mesh_pointer, const float* kappav, const float* muv, const float* rho_vp,
+                             const float* rho_vs, const float* rmassx, const float* rmassz);
 
 #ifdef __cplusplus
 }

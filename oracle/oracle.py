@@ -76,6 +76,8 @@ def _lib(precision: str):
         lib.orc_set_ftz.argtypes = [P, C.c_int]
         lib.orc_adjoint.argtypes = [P, dp, dp, dp, dp, dp, C.c_void_p, dp, C.c_int, C.c_void_p, C.c_void_p]
         lib.orc_kernel_increment.argtypes = [P, dp, dp, dp, dp]
+        lib.orc_gradient.argtypes = [P, dp, dp, dp]
+        lib.orc_gradient.restype = C.c_double
         _LIBS[precision] = lib
     return _LIBS[precision]
 
@@ -227,6 +229,21 @@ class OracleSim:
         if len(steps):
             return out + (dumps[: len(steps)].reshape(len(steps), -1, 2),)
         return out
+
+    def gradient(self, obs):
+        """Exact gradient of chi = 1/2 dt sum |s - obs|^2 over a full forward run from rest (seismotype
+        displacement).  Returns (chi, grad dict rho/vp/vs, parts dict kappa/mu/rho_mass/rho_vp/rho_vs), all
+        [nspec,25] in the MODEL=gll per-point parameterisation."""
+        o = np.ascontiguousarray(np.asarray(obs, np.float64).reshape(-1))
+        if o.size != max(1, self.cfg.nrec) * 2 * self.cfg.nstep and self.cfg.nrec > 0:
+            raise ValueError("obs must be [nrec, 2, nstep]")
+        parts = np.zeros(5 * self.nspec * NGLL2)
+        grad = np.zeros(3 * self.nspec * NGLL2)
+        chi = self.lib.orc_gradient(self.h, o, parts, grad)
+        parts = parts.reshape(5, self.nspec, NGLL2)
+        grad = grad.reshape(3, self.nspec, NGLL2)
+        return chi, {"rho": grad[0], "vp": grad[1], "vs": grad[2]}, dict(
+            zip(("kappa", "mu", "rho_mass", "rho_vp", "rho_vs"), parts))
 
     def kernel_increment(self, displ, accel, b_displ):
         """One unscaled kernel accumulation (compute_kernels_el) for given fields."""

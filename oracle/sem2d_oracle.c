@@ -1109,3 +1109,195 @@ void orc_kernel_increment(void *h, const double *displ, const double *accel, con
   for (size_t l = 0; l < ne; l++) { out[l] = r[l]; out[ne + l] = m[l]; out[2 * ne + l] = kk[l]; }
   free(d); free(a); free(b); free(r); free(m); free(kk);
 }
+
+/* ------------------------------------------------------------------ */
+/* Exact discrete gradient of the L2 waveform misfit (reverse mode of  */
+/* the forward time loop above).  No counterpart in the reference: the */
+/* reference's SIMULATION_TYPE=3 kernels approximate the same gradient */
+/* by a continuous adjoint with backward reconstruction.               */
+/* ------------------------------------------------------------------ */
+/* field gradient (d/dx, d/dz of ux and uz) at the element points of element e, as the force kernel forms it */
+static void point_gradients(const orc_sim *s, const REAL *u, int e, REAL (*gr)[4] /* [25][4]: ux_x ux_z uz_x uz_z */) {
+  const REAL (*H)[NGLL] = (const REAL(*)[NGLL])s->hprime;
+  const int *ib = s->ibool + (size_t)e * NGLL2;
+  for (int j = 0; j < NGLL; j++)
+    for (int i = 0; i < NGLL; i++) {
+      REAL dux_dxi = 0, duz_dxi = 0, dux_dga = 0, duz_dga = 0;
+      for (int k = 0; k < NGLL; k++) {
+        int g1 = ib[j * NGLL + k] - 1, g2 = ib[k * NGLL + i] - 1;
+        dux_dxi = dux_dxi + u[2 * g1] * H[i][k];
+        duz_dxi = duz_dxi + u[2 * g1 + 1] * H[i][k];
+        dux_dga = dux_dga + u[2 * g2] * H[j][k];
+        duz_dga = duz_dga + u[2 * g2 + 1] * H[j][k];
+      }
+      size_t l = (size_t)e * NGLL2 + j * NGLL + i;
+      REAL xixl = s->xix[l], xizl = s->xiz[l], gaxl = s->gammax[l], gazl = s->gammaz[l];
+      gr[j * NGLL + i][0] = dux_dxi * xixl + dux_dga * gaxl;
+      gr[j * NGLL + i][1] = dux_dxi * xizl + dux_dga * gazl;
+      gr[j * NGLL + i][2] = duz_dxi * xixl + duz_dga * gaxl;
+      gr[j * NGLL + i][3] = duz_dxi * xizl + duz_dga * gazl;
+    }
+}
+
+/* Misfit chi = 1/2 dt sum_n sum_r |s_n - o_n|^2 of the forward run of orc_forward (from rest, all NSTEP steps;
+   seismotype displacement), with obs[nrec][2][nstep].  Returns chi and the exact gradient of the discrete
+   forward map with respect to the per-GLL-point parameters, parts[5][nspec*25] = d chi / d (kappa, mu, rho
+   through the mass matrix, rho*vp and rho*vs through the Stacey traction and its mass term), and with respect
+   to the MODEL=gll parameters grad[3][nspec*25] = d chi / d (rho, vp, vs) (mu = rho vs^2,
+   kappa = rho vp^2 - mu).  The forward states are kept in memory (small meshes only). */
+double orc_gradient(void *h, const double *obs, double *parts, double *grad) {
+  orc_sim *s = (orc_sim *)h;
+  const int N = s->p.nstep;
+  const size_t n2 = (size_t)2 * s->nglob, ne = (size_t)s->nspec * NGLL2;
+  unsigned int csr = _mm_getcsr();
+  if (s->ftz) _mm_setcsr(csr | 0x8040);
+  REAL deltat = (REAL)s->p.dt;
+  REAL deltatover2 = (REAL)(0.5 * (double)deltat);
+  REAL deltatsqover2 = (REAL)(0.5 * (double)deltat * (double)deltat);
+  /* forward: d_n, predicted veloc v'_n and the unscaled force r_n for n = 1..N */
+  REAL *Dn = (REAL *)calloc(n2 * N, sizeof(REAL)), *Vp = (REAL *)calloc(n2 * N, sizeof(REAL));
+  REAL *Rn = (REAL *)calloc(n2 * N, sizeof(REAL));
+  double *seis = (double *)calloc((size_t)(s->nrec > 0 ? s->nrec : 1) * 2 * N, sizeof(double));
+  REAL *d = (REAL *)calloc(n2, sizeof(REAL)), *v = (REAL *)calloc(n2, sizeof(REAL)), *a = (REAL *)calloc(n2, sizeof(REAL));
+  for (int n = 1; n <= N; n++) {
+    for (size_t k = 0; k < n2; k++) {
+      d[k] = d[k] + deltat * v[k] + deltatsqover2 * a[k];
+      v[k] = v[k] + deltatover2 * a[k];
+      a[k] = (REAL)0;
+    }
+    if (s->p.stacey && s->nabs > 0) stacey_forces(s, v, a, NULL, 0, 0);
+    add_sources(s, a, n);
+    element_forces(s, d, a);
+    memcpy(Dn + n2 * (n - 1), d, n2 * sizeof(REAL));
+    memcpy(Vp + n2 * (n - 1), v, n2 * sizeof(REAL));
+    memcpy(Rn + n2 * (n - 1), a, n2 * sizeof(REAL));
+    for (size_t k = 0; k < n2; k++) a[k] = a[k] * s->rmass_inv[k];
+    for (size_t k = 0; k < n2; k++) v[k] = v[k] + deltatover2 * a[k];
+    if (s->nrec > 0) record_seismograms(s, d, seis, n - 1, N);
+  }
+  double chi = 0.0;
+  for (int r = 0; r < s->nrec; r++)
+    for (int c = 0; c < 2; c++)
+      for (int n = 0; n < N; n++) {
+        double q = seis[((size_t)r * 2 + c) * N + n] - obs[((size_t)r * 2 + c) * N + n];
+        chi += 0.5 * (double)deltat * q * q;
+      }
+  /* reverse: (D, V, A) are the cotangents of (d_n, v_n, a_n) */
+  REAL *D = (REAL *)calloc(n2, sizeof(REAL)), *V = (REAL *)calloc(n2, sizeof(REAL)), *A = (REAL *)calloc(n2, sizeof(REAL));
+  REAL *Rb = (REAL *)calloc(n2, sizeof(REAL)), *Vb = (REAL *)calloc(n2, sizeof(REAL));
+  double *gk = parts, *gm = parts + ne, *gr = parts + 2 * ne, *gvp = parts + 3 * ne, *gvs = parts + 4 * ne;
+  memset(parts, 0, sizeof(double) * 5 * ne);
+  double *pm = (double *)calloc(n2, sizeof(double)); /* d chi / d M_c(g) summed over steps */
+  REAL (*gd)[4] = (REAL(*)[4])malloc(sizeof(REAL) * 4 * NGLL2), (*gb)[4] = (REAL(*)[4])malloc(sizeof(REAL) * 4 * NGLL2);
+  for (int n = N; n >= 1; n--) {
+    const REAL *dn = Dn + n2 * (n - 1), *vpn = Vp + n2 * (n - 1), *rn = Rn + n2 * (n - 1);
+    /* seismogram s_n = R d_n: D += R^T dt (s_n - o_n) */
+    for (int r = 0; r < s->nrec; r++) {
+      double px = (double)deltat * (seis[((size_t)r * 2 + 0) * N + n - 1] - obs[((size_t)r * 2 + 0) * N + n - 1]);
+      double pz = (double)deltat * (seis[((size_t)r * 2 + 1) * N + n - 1] - obs[((size_t)r * 2 + 1) * N + n - 1]);
+      double cx = s->cosrot * px - s->sinrot * pz, cz = s->sinrot * px + s->cosrot * pz;
+      int e = s->rec_ispec[r] - 1;
+      for (int j = 0; j < NGLL; j++)
+        for (int i = 0; i < NGLL; i++) {
+          int g = s->ibool[e * NGLL2 + j * NGLL + i] - 1;
+          double hl = (double)(s->xir[r * NGLL + i] * s->gammar[r * NGLL + j]);
+          D[2 * g] = (REAL)((double)D[2 * g] + hl * cx);
+          D[2 * g + 1] = (REAL)((double)D[2 * g + 1] + hl * cz);
+        }
+    }
+    /* v_n = v' + dt/2 a_n */
+    for (size_t k = 0; k < n2; k++) {
+      Vb[k] = V[k];
+      A[k] = A[k] + deltatover2 * V[k];
+    }
+    /* a_n = W r_n (W = rmass_inverse): r-bar = W A; d chi / d M = -A r W^2 */
+    for (size_t k = 0; k < n2; k++) {
+      Rb[k] = s->rmass_inv[k] * A[k];
+      pm[k] -= (double)A[k] * (double)rn[k] * (double)s->rmass_inv[k] * (double)s->rmass_inv[k];
+    }
+    /* r_n = -K d_n - B v'_n + f_n */
+    element_forces(s, Rb, D);
+    if (s->p.stacey && s->nabs > 0) stacey_forces(s, Rb, Vb, NULL, 0, 0);
+    for (int e = 0; e < s->nspec; e++) {
+      point_gradients(s, dn, e, gd);
+      point_gradients(s, Rb, e, gb);
+      for (int p = 0; p < NGLL2; p++) {
+        size_t l = (size_t)e * NGLL2 + p;
+        int i = p % NGLL, j = p / NGLL;
+        double wj = (double)s->wxgll[i] * (double)s->wxgll[j] * (double)s->jacobian[l];
+        double ux = gd[p][0], uxz = gd[p][1], uzx = gd[p][2], uz = gd[p][3];
+        double wx = gb[p][0], wxz = gb[p][1], wzx = gb[p][2], wz = gb[p][3];
+        gk[l] -= wj * (ux + uz) * (wx + wz);
+        gm[l] -= wj * (ux * wx + uz * wz - ux * wz - uz * wx + (uxz + uzx) * (wxz + wzx));
+      }
+    }
+    for (int ab = 0; ab < s->nabs; ab++) {
+      int e = s->abs_ispec[ab] - 1, code = s->abs_code[ab];
+      for (int k = 0; k < NGLL; k++) {
+        int i, j;
+        if (code == 4) { i = 0; j = k; }
+        else if (code == 2) { i = NGLL - 1; j = k; }
+        else if (code == 1) { i = k; j = 0; }
+        else { i = k; j = NGLL - 1; }
+        if (code == 1 && ((s->abs_corner[4 * ab + 0] && i == 0) || (s->abs_corner[4 * ab + 1] && i == NGLL - 1))) continue;
+        if (code == 3 && ((s->abs_corner[4 * ab + 2] && i == 0) || (s->abs_corner[4 * ab + 3] && i == NGLL - 1))) continue;
+        size_t l = (size_t)e * NGLL2 + j * NGLL + i;
+        int g = s->ibool[l] - 1;
+        REAL nx, nz, j1d;
+        edge_normal(s, code, l, &nx, &nz, &j1d);
+        double weight = (double)j1d * (double)((code == 4 || code == 2) ? s->wxgll[j] : s->wxgll[i]);
+        double vx = vpn[2 * g], vz = vpn[2 * g + 1], bx = Rb[2 * g], bz = Rb[2 * g + 1];
+        double vn = nx * vx + nz * vz, bn = nx * bx + nz * bz;
+        gvp[l] -= weight * vn * bn;
+        gvs[l] -= weight * (vx * bx + vz * bz - vn * bn);
+      }
+    }
+    /* d_n = d + dt v + dt^2/2 a,  v' = v + dt/2 a */
+    for (size_t k = 0; k < n2; k++) {
+      REAL vb = Vb[k], db = D[k];
+      V[k] = vb + deltat * db;
+      A[k] = deltatover2 * vb + deltatsqover2 * db;
+    }
+  }
+  /* mass matrix M_c(g) = sum w_i w_j J rho + Stacey term dt/2 weight T_c(rho vp, rho vs) (setup_mass above) */
+  for (int e = 0; e < s->nspec; e++)
+    for (int p = 0; p < NGLL2; p++) {
+      size_t l = (size_t)e * NGLL2 + p;
+      int i = p % NGLL, j = p / NGLL, g = s->ibool[l] - 1;
+      double wj = (double)s->wxgll[i] * (double)s->wxgll[j] * (double)s->jacobian[l];
+      gr[l] += wj * (pm[2 * g] + pm[2 * g + 1]);
+    }
+  if (s->p.stacey)
+    for (int ab = 0; ab < s->nabs; ab++) {
+      int e = s->abs_ispec[ab] - 1, code = s->abs_code[ab];
+      for (int k = 0; k < NGLL; k++) {
+        int i, j;
+        if (code == 4) { i = 0; j = k; }
+        else if (code == 2) { i = NGLL - 1; j = k; }
+        else if (code == 1) { i = k; j = 0; }
+        else { i = k; j = NGLL - 1; }
+        if (code == 1 && ((s->abs_corner[4 * ab + 0] && i == 0) || (s->abs_corner[4 * ab + 1] && i == NGLL - 1))) continue;
+        if (code == 3 && ((s->abs_corner[4 * ab + 2] && i == 0) || (s->abs_corner[4 * ab + 3] && i == NGLL - 1))) continue;
+        size_t l = (size_t)e * NGLL2 + j * NGLL + i;
+        int g = s->ibool[l] - 1;
+        REAL nx, nz, j1d;
+        edge_normal(s, code, l, &nx, &nz, &j1d);
+        double weight = (double)j1d * (double)((code == 4 || code == 2) ? s->wxgll[j] : s->wxgll[i]);
+        double h2 = 0.5 * (double)deltat, sn = (double)nx + (double)nz;
+        /* T_x = dt/2 [rho_vp (nx+nz) nx + rho_vs (1 - (nx+nz) nx)], T_z likewise with nz */
+        gvp[l] += weight * h2 * (pm[2 * g] * sn * nx + pm[2 * g + 1] * sn * nz);
+        gvs[l] += weight * h2 * (pm[2 * g] * (1.0 - sn * nx) + pm[2 * g + 1] * (1.0 - sn * nz));
+      }
+    }
+  /* chain to (rho, vp, vs): mu = rho vs^2, kappa = rho vp^2 - mu, rho_vp = rho vp, rho_vs = rho vs */
+  for (size_t l = 0; l < ne; l++) {
+    double rho = s->rhostore[l], vp = (double)s->rho_vp[l] / rho, vs = (double)s->rho_vs[l] / rho;
+    grad[l] = gk[l] * (vp * vp - vs * vs) + gm[l] * vs * vs + gr[l] + gvp[l] * vp + gvs[l] * vs;
+    grad[ne + l] = gk[l] * 2.0 * rho * vp + gvp[l] * rho;
+    grad[2 * ne + l] = -gk[l] * 2.0 * rho * vs + gm[l] * 2.0 * rho * vs + gvs[l] * rho;
+  }
+  free(Dn); free(Vp); free(Rn); free(seis); free(d); free(v); free(a);
+  free(D); free(V); free(A); free(Rb); free(Vb); free(pm); free(gd); free(gb);
+  _mm_setcsr(csr);
+  return chi;
+}

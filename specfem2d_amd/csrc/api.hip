@@ -111,27 +111,39 @@ std::vector<int> make_patches(const std::vector<int>& ibool, const std::vector<i
 }
 
 // Device element order and assembly schedule of both phase lists (host only, no device call):
-//   patch  (mode 0): patches coloured greedily (patches of one colour share no point); order (phase, colour, patch)
+//   patch  (mode 0): patches coloured greedily (patches of one colour share no point); order (phase, colour, patch).
+//                    The points are renumbered patch-major (see below).
 //   colour (mode 1): elements coloured greedily; order (phase, colour)
 //   atomic (mode 2): the phase list order, one launch per phase with FP32 atomics
+//
+// Point renumbering of the patch mode.  Walking the patches in phase-list order, each patch owns the points
+// that no earlier patch touched and numbers them consecutively, row by row (GLL row j outermost, then its
+// elements, then i).  A patch therefore reads its owned points as one contiguous range, and only the points
+// owned by earlier patches (its halo, sorted) through an index list.  On a strip of elements the halo is the
+// strip's bottom row, which is the top row of the strip below and hence itself contiguous.
 struct Plan {
   std::vector<int> order;  // ispec of each device position
   int phase_count[2] = {0, 0};
   int ncolour[2] = {0, 0};
   int colour_start[2][Mesh::MAX_COLOURS] = {};
   int colour_count[2][Mesh::MAX_COLOURS] = {};
-  std::vector<int> patch_elem_off, patch_node_off, patch_nodes;  // patch mode, with closing entries
-  std::vector<unsigned char> slot;                                // patch mode, per device position
+  // patch mode
+  std::vector<int> newid;             // internal id of every point (a permutation of 0..nglob-1)
+  std::vector<int> desc;              // [npatch][4] = {first element position, first owned id, halo offset, counts}
+  std::vector<int> halo;              // internal ids of the halo points of all patches, one sorted list per patch
+  std::vector<unsigned char> slot;    // per device position and GLL point: index in its patch's point list
 };
+
+int desc_counts(int ne, int nown, int nhalo) { return ne | (nown << 8) | (nhalo << 20); }
 
 Plan build_plan(const std::vector<int>& ibool, int nspec, int nglob, const int* phase_lists, int np, int mode) {
   Plan P;
   std::vector<int>& order = P.order;
   order.reserve(nspec);
   std::vector<char> placed(nspec, 0);
-  if (mode == 0) P.slot.assign((size_t)nspec * NGLL2, 0);
+  std::vector<std::vector<int>> phase_elems(2);
   for (int ph = 0; ph < 2; ph++) {
-    std::vector<int> elems;
+    std::vector<int>& elems = phase_elems[ph];
     for (int k = 0; k < np; k++) {
       const int v = phase_lists[(size_t)ph * np + k];
       if (v < 1) break;  // lists are padded with zeros after their used prefix
@@ -141,6 +153,14 @@ Plan build_plan(const std::vector<int>& ibool, int nspec, int nglob, const int* 
       elems.push_back(v - 1);
     }
     P.phase_count[ph] = (int)elems.size();
+  }
+  if (mode == 0) {
+    P.slot.assign((size_t)nspec * NGLL2, 0);
+    P.newid.assign((size_t)nglob, -1);
+  }
+  int next_id = 0;
+  for (int ph = 0; ph < 2; ph++) {
+    const std::vector<int>& elems = phase_elems[ph];
     if (mode == 2) {  // atomics: one range in list order
       P.ncolour[ph] = elems.empty() ? 0 : 1;
       P.colour_start[ph][0] = (int)order.size();
@@ -160,19 +180,28 @@ Plan build_plan(const std::vector<int>& ibool, int nspec, int nglob, const int* 
       }
       continue;
     }
-    // coloured patches
+    // coloured patches; ownership and numbering in phase-list order
     const std::vector<int> pstart = make_patches(ibool, elems, nglob);
     const int np_ph = (int)pstart.size() - 1;
-    std::vector<std::vector<int>> pnodes(np_ph);
+    std::vector<std::vector<int>> pnodes(np_ph), phalo(np_ph);
+    std::vector<int> own0(np_ph), nown(np_ph);
     for (int q = 0; q < np_ph; q++) {
       std::vector<int>& nodes = pnodes[q];
-      for (int k = pstart[q]; k < pstart[q + 1]; k++) {
-        const int* ib = ibool.data() + (size_t)elems[k] * NGLL2;
-        nodes.insert(nodes.end(), ib, ib + NGLL2);
-      }
+      own0[q] = next_id;
+      for (int j = 0; j < NGLL; j++)
+        for (int k = pstart[q]; k < pstart[q + 1]; k++)
+          for (int i = 0; i < NGLL; i++) {
+            const int g = ibool[(size_t)elems[k] * NGLL2 + j * NGLL + i];
+            nodes.push_back(g);
+            if (P.newid[g] < 0) P.newid[g] = next_id++;
+          }
+      nown[q] = next_id - own0[q];
       std::sort(nodes.begin(), nodes.end());
       nodes.erase(std::unique(nodes.begin(), nodes.end()), nodes.end());
       if (nodes.size() > 256) fail("patch with %zu points exceeds the 256-point workgroup", nodes.size());
+      for (int g : nodes)
+        if (P.newid[g] < own0[q]) phalo[q].push_back(P.newid[g]);
+      std::sort(phalo[q].begin(), phalo[q].end());
     }
     std::vector<uint32_t> used((size_t)nglob, 0u);
     std::vector<int> pcol(np_ph, 0);
@@ -189,36 +218,42 @@ Plan build_plan(const std::vector<int>& ibool, int nspec, int nglob, const int* 
     }
     P.ncolour[ph] = ncol;
     for (int c = 0; c < ncol; c++) {
-      P.colour_start[ph][c] = (int)P.patch_elem_off.size();
+      P.colour_start[ph][c] = (int)P.desc.size() / 4;
       for (int q = 0; q < np_ph; q++) {
         if (pcol[q] != c) continue;
-        P.patch_elem_off.push_back((int)order.size());
-        P.patch_node_off.push_back((int)P.patch_nodes.size());
-        const std::vector<int>& nodes = pnodes[q];
+        const int ne = pstart[q + 1] - pstart[q];
+        const std::vector<int>& hl = phalo[q];
+        P.desc.insert(P.desc.end(), {(int)order.size(), own0[q], (int)P.halo.size(),
+                                     desc_counts(ne, nown[q], (int)hl.size())});
         for (int k = pstart[q]; k < pstart[q + 1]; k++) {
           const int e = elems[k];
           const int pos = (int)order.size();
           order.push_back(e);
           const int* ib = ibool.data() + (size_t)e * NGLL2;
-          for (int p = 0; p < NGLL2; p++)
-            P.slot[(size_t)pos * NGLL2 + p] =
-                (unsigned char)(std::lower_bound(nodes.begin(), nodes.end(), ib[p]) - nodes.begin());
+          for (int p = 0; p < NGLL2; p++) {
+            const int id = P.newid[ib[p]];
+            const int sl = id >= own0[q] ? id - own0[q]
+                                         : nown[q] + (int)(std::lower_bound(hl.begin(), hl.end(), id) - hl.begin());
+            P.slot[(size_t)pos * NGLL2 + p] = (unsigned char)sl;
+          }
         }
-        P.patch_nodes.insert(P.patch_nodes.end(), nodes.begin(), nodes.end());
+        P.halo.insert(P.halo.end(), hl.begin(), hl.end());
       }
-      P.colour_count[ph][c] = (int)P.patch_elem_off.size() - P.colour_start[ph][c];
+      P.colour_count[ph][c] = (int)P.desc.size() / 4 - P.colour_start[ph][c];
     }
   }
   for (int e = 0; e < nspec; e++)
     if (!placed[e]) order.push_back(e);  // elements outside the elastic phase lists keep their relative order
   if (mode == 0) {
-    P.patch_elem_off.push_back(P.phase_count[0] + P.phase_count[1]);
-    P.patch_node_off.push_back((int)P.patch_nodes.size());
+    for (int g = 0; g < nglob; g++)
+      if (P.newid[g] < 0) P.newid[g] = next_id++;  // points of no elastic element
+    P.halo.push_back(0);  // one spare entry: lanes without a halo point read a valid address
   }
   return P;
 }
 
-// Applies the plan: schedule into the handle, per-element device arrays moved into the plan's order.
+// Applies the plan: schedule into the handle, per-element device arrays moved into the plan's order, ibool
+// rewritten in the plan's point numbering (patch mode).
 void sort_elements_by_colour(Mesh* mp, const int* phase_lists, int np) {
   const int nspec = mp->nspec;
   Plan P = build_plan(mp->h_ibool, nspec, mp->nglob, phase_lists, np, mp->assembly);
@@ -235,11 +270,11 @@ void sort_elements_by_colour(Mesh* mp, const int* phase_lists, int np) {
   mp->pos_of_ispec.assign(nspec, 0);
   for (int pos = 0; pos < nspec; pos++) mp->pos_of_ispec[order[pos]] = pos;
   if (mp->assembly == 0) {
-    mp->npatch = (int)P.patch_elem_off.size() - 1;
-    mp->d_patch_elem_off = device_copy(P.patch_elem_off.data(), P.patch_elem_off.size());
-    mp->d_patch_node_off = device_copy(P.patch_node_off.data(), P.patch_node_off.size());
-    mp->d_patch_nodes = device_copy(P.patch_nodes.data(), P.patch_nodes.size());
+    mp->npatch = (int)P.desc.size() / 4;
+    mp->d_patch_desc = reinterpret_cast<int4*>(device_copy(P.desc.data(), P.desc.size()));
+    mp->d_halo_nodes = device_copy(P.halo.data(), P.halo.size());
     mp->d_elem_slot = device_copy(P.slot.data(), P.slot.size());
+    mp->point_id = std::move(P.newid);
   }
 
   // move the per-element device arrays
@@ -254,13 +289,16 @@ void sort_elements_by_colour(Mesh* mp, const int* phase_lists, int np) {
     device_free(*a);
     *a = dst;
   }
+  device_free(d_map);
   {
-    int* dst = nullptr;
-    check_hip(hipMalloc(&dst, ne * sizeof(int)), "hipMalloc");
-    launch_permute_elements_int(mp->d_ibool, dst, d_map, nspec, mp->stream);
-    check_hip(hipStreamSynchronize(mp->stream), "permute");
-    device_free(mp->d_ibool);
-    mp->d_ibool = dst;
+    // ibool in device element order and, in patch mode, internal point numbering
+    std::vector<int> ib(ne);
+    for (int pos = 0; pos < nspec; pos++) {
+      const int* src = mp->h_ibool.data() + (size_t)order[pos] * NGLL2;
+      int* dst = ib.data() + (size_t)pos * NGLL2;
+      for (int p = 0; p < NGLL2; p++) dst[p] = mp->point_id.empty() ? src[p] : mp->point_id[src[p]];
+    }
+    check_hip(hipMemcpy(mp->d_ibool, ib.data(), ne * sizeof(int), hipMemcpyHostToDevice), "ibool H2D");
   }
   {
     std::vector<int> el(nspec);
@@ -269,7 +307,6 @@ void sort_elements_by_colour(Mesh* mp, const int* phase_lists, int np) {
     for (int pos = 0; pos < nspec; pos++) el[pos] = h[order[pos]];
     check_hip(hipMemcpy(mp->d_ispec_is_elastic, el.data(), nspec * sizeof(int), hipMemcpyHostToDevice), "H2D");
   }
-  device_free(d_map);
   // element indices of sources and receivers follow the new order
   if (mp->nsources_local > 0) {
     std::vector<int> v(mp->h_src_ispec.size());
@@ -283,6 +320,37 @@ void sort_elements_by_colour(Mesh* mp, const int* phase_lists, int np) {
   }
   mp->sorted = 1;
   std::vector<int>().swap(mp->h_ibool);
+}
+
+}  // namespace
+
+// Point arrays cross the C-ABI in the caller's numbering; on the device they are stored in the plan's
+// numbering (identity unless the patch plan renumbered them).  n2 = values per point (1 or 2).
+void points_to_device(const Mesh* mp, float* dev, const float* host, int n2, const char* what) {
+  const size_t n = (size_t)mp->nglob * n2;
+  if (mp->point_id.empty()) {
+    check_hip(hipMemcpyAsync(dev, host, n * sizeof(float), hipMemcpyHostToDevice, mp->stream), what);
+    check_hip(hipStreamSynchronize(mp->stream), what);
+    return;
+  }
+  std::vector<float> tmp(n);
+  for (int g = 0; g < mp->nglob; g++)
+    for (int c = 0; c < n2; c++) tmp[(size_t)mp->point_id[g] * n2 + c] = host[(size_t)g * n2 + c];
+  check_hip(hipMemcpyAsync(dev, tmp.data(), n * sizeof(float), hipMemcpyHostToDevice, mp->stream), what);
+  check_hip(hipStreamSynchronize(mp->stream), what);
+}
+void points_from_device(const Mesh* mp, float* host, const float* dev, int n2, const char* what) {
+  const size_t n = (size_t)mp->nglob * n2;
+  if (mp->point_id.empty()) {
+    check_hip(hipMemcpyAsync(host, dev, n * sizeof(float), hipMemcpyDeviceToHost, mp->stream), what);
+    check_hip(hipStreamSynchronize(mp->stream), what);
+    return;
+  }
+  std::vector<float> tmp(n);
+  check_hip(hipMemcpyAsync(tmp.data(), dev, n * sizeof(float), hipMemcpyDeviceToHost, mp->stream), what);
+  check_hip(hipStreamSynchronize(mp->stream), what);
+  for (int g = 0; g < mp->nglob; g++)
+    for (int c = 0; c < n2; c++) host[(size_t)g * n2 + c] = tmp[(size_t)mp->point_id[g] * n2 + c];
 }
 
 // per-element host array in ispec order -> device order (identity before sorting)
@@ -303,7 +371,22 @@ void to_ispec_order(const Mesh* mp, const float* dev_order, float* out) {
   }
 }
 
-}  // namespace
+// acc -= K u over the elements of phase ph (0 outer, 1 inner), with the handle's assembly scheme
+void apply_element_forces(const Mesh* mp, int ph, const float* u, float* acc) {
+  if (mp->phase_count[ph] <= 0) return;
+  if (mp->assembly == 2) {
+    launch_elastic_forces(mp, mp->colour_start[ph][0], mp->phase_count[ph], 1, u, acc);
+    return;
+  }
+  for (int c = 0; c < mp->ncolour[ph]; c++) {
+    const int first = mp->colour_start[ph][c], count = mp->colour_count[ph][c];
+    if (mp->assembly == 0)
+      launch_elastic_forces_patch(mp, first, count, u, acc);
+    else
+      launch_elastic_forces(mp, first, count, 0, u, acc);
+  }
+}
+
 }  // namespace sf2d
 
 using namespace sf2d;
@@ -364,6 +447,10 @@ void prepare_constants_device_(long* Mesh_pointer, int* h_NGLLX, int* NSPEC_AB, 
   mp->pml = *PML_BOUNDARY_CONDITIONS;
   mp->nstep = *NSTEP;
   if (mp->nspec <= 0 || mp->nglob <= 0) fail("empty mesh (nspec=%d, nglob=%d)", mp->nspec, mp->nglob);
+  // the force kernels address element arrays and fields with 32-bit byte offsets
+  if ((long long)mp->nspec * NGLL2 * 4 >= (1LL << 32) || (long long)mp->nglob * 8 >= (1LL << 32))
+    fail("mesh too large for one handle (nspec=%d, nglob=%d): element arrays and fields must stay below 4 GiB",
+         mp->nspec, mp->nglob);
 
   // basis: Fortran hprime_xx(i,k) = h'_k(xi_i) at h[(k-1)*5+(i-1)], hprimewgll_xx(k,i) at h[(i-1)*5+(k-1)]
   for (int i = 0; i < NGLL; i++) {
@@ -449,11 +536,13 @@ void prepare_fields_elastic_device_(long* Mesh_pointer, realw* rmassx, realw* rm
   mp->d_displ = device_zeros<float>(n2);
   mp->d_veloc = device_zeros<float>(n2);
   mp->d_accel = device_zeros<float>(n2);
-  mp->d_rmassx = device_copy(rmassx, (size_t)mp->nglob);
-  mp->d_rmassz = device_copy(rmassz, (size_t)mp->nglob);
   const int np = *num_phase_ispec_elastic;
   mp->num_phase_ispec_elastic = np;
   sort_elements_by_colour(mp, phase_ispec_inner_elastic, np);
+  mp->d_rmassx = device_zeros<float>((size_t)mp->nglob);
+  mp->d_rmassz = device_zeros<float>((size_t)mp->nglob);
+  points_to_device(mp, mp->d_rmassx, rmassx, 1, "rmassx H2D");
+  points_to_device(mp, mp->d_rmassz, rmassz, 1, "rmassz H2D");
   check_hip(hipDeviceSynchronize(), "prepare_fields_elastic_device");
 }
 
@@ -571,8 +660,9 @@ void prepare_cleanup_device_(long* Mesh_pointer, int* ACOUSTIC_SIMULATION, int* 
   for (int s = 0; s < 5; s++) device_free(mp->d_b_absorb[s]);
   device_free(mp->d_rho_kl); device_free(mp->d_mu_kl); device_free(mp->d_kappa_kl);
   device_free(mp->d_scratch_u32);
-  device_free(mp->d_patch_elem_off); device_free(mp->d_patch_node_off); device_free(mp->d_patch_nodes);
+  device_free(mp->d_patch_desc); device_free(mp->d_halo_nodes);
   device_free(mp->d_elem_slot);
+  if (mp->grad) sf2d_grad_release(Mesh_pointer);
   (void)hipStreamDestroy(mp->stream);
   delete mp;
   *Mesh_pointer = 0;
@@ -583,14 +673,12 @@ void prepare_cleanup_device_(long* Mesh_pointer, int* ACOUSTIC_SIMULATION, int* 
 static void copy_field(float* dst_dev, const realw* src_host, int size, Mesh* mp, const char* what) {
   if (size != 2 * mp->nglob) fail("%s: size %d != 2*nglob = %d", what, size, 2 * mp->nglob);
   if (!dst_dev) fail("%s: field not allocated on the device", what);
-  check_hip(hipMemcpyAsync(dst_dev, src_host, sizeof(float) * size, hipMemcpyHostToDevice, mp->stream), what);
-  check_hip(hipStreamSynchronize(mp->stream), what);
+  points_to_device(mp, dst_dev, src_host, 2, what);
 }
 static void fetch_field(realw* dst_host, const float* src_dev, int size, Mesh* mp, const char* what) {
   if (size != 2 * mp->nglob) fail("%s: size %d != 2*nglob = %d", what, size, 2 * mp->nglob);
   if (!src_dev) fail("%s: field not allocated on the device", what);
-  check_hip(hipMemcpyAsync(dst_host, src_dev, sizeof(float) * size, hipMemcpyDeviceToHost, mp->stream), what);
-  check_hip(hipStreamSynchronize(mp->stream), what);
+  points_from_device(mp, dst_host, src_dev, 2, what);
 }
 
 void transfer_fields_el_to_device_(int* size, realw* displ, realw* veloc, realw* accel, long* Mesh_pointer) {
@@ -731,22 +819,8 @@ void compute_forces_viscoelastic_cuda_(long* Mesh_pointer, int* iphase, realw* d
     fail("compute_forces: iphase %d has %d elements, the prepared phase list has %d", *iphase, nelem, mp->phase_count[ph]);
   const bool w1 = *compute_wavefield_1 != 0, w2 = *compute_wavefield_2 != 0;
   if (w2 && !mp->d_b_displ) fail("compute_forces: backward wavefield not allocated");
-  if (mp->assembly == 2) {
-    const int first = mp->colour_start[ph][0];
-    if (w1) launch_elastic_forces(mp, first, nelem, 1, mp->d_displ, mp->d_accel);
-    if (w2) launch_elastic_forces(mp, first, nelem, 1, mp->d_b_displ, mp->d_b_accel);
-    return;
-  }
-  for (int c = 0; c < mp->ncolour[ph]; c++) {
-    const int first = mp->colour_start[ph][c], count = mp->colour_count[ph][c];
-    if (mp->assembly == 0) {
-      if (w1) launch_elastic_forces_patch(mp, first, count, mp->d_displ, mp->d_accel);
-      if (w2) launch_elastic_forces_patch(mp, first, count, mp->d_b_displ, mp->d_b_accel);
-    } else {
-      if (w1) launch_elastic_forces(mp, first, count, 0, mp->d_displ, mp->d_accel);
-      if (w2) launch_elastic_forces(mp, first, count, 0, mp->d_b_displ, mp->d_b_accel);
-    }
-  }
+  if (w1) apply_element_forces(mp, ph, mp->d_displ, mp->d_accel);
+  if (w2) apply_element_forces(mp, ph, mp->d_b_displ, mp->d_b_accel);
 }
 
 void compute_stacey_viscoelastic_cuda_(long* Mesh_pointer, int* iphasef, realw* h_b_absorb_elastic_left,
@@ -936,9 +1010,10 @@ static void print_max(const char* what, const float* v, int n, int id) {
   printf("%d: %s max = %e at %d\n", id, what, m, at);
 }
 static void print_device_max(Mesh* mp, const float* d, int n, int id, const char* what) {
+  if (n != 2 * mp->nglob) fail("%s: size %d != 2*nglob = %d", what, n, 2 * mp->nglob);
   std::vector<float> h((size_t)n);
   check_hip(hipStreamSynchronize(mp->stream), what);
-  check_hip(hipMemcpy(h.data(), d, h.size() * sizeof(float), hipMemcpyDeviceToHost), what);
+  points_from_device(mp, h.data(), d, 2, what);
   print_max(what, h.data(), n, id);
 }
 void check_max_norm_displ_gpu_(int* size, realw* displ, long* Mesh_pointer, int* announceID) {
@@ -995,6 +1070,24 @@ extern "C" int sf2d_plan_validate(int nspec, int nglob, const int* ibool_1based,
   }
   for (int e = 0; e < nspec; e++)
     if (seen[e] != 1) return 2;
+  // point numbering of the patch mode: a permutation; owned ranges disjoint and consecutive
+  std::vector<int> id_owner;
+  if (mode == 0) {
+    std::vector<char> hit(nglob, 0);
+    for (int g = 0; g < nglob; g++) {
+      const int id = P.newid[g];
+      if (id < 0 || id >= nglob || hit[id]) return 6;
+      hit[id] = 1;
+    }
+    id_owner.assign(nglob, -1);
+    for (size_t u = 0; u < P.desc.size() / 4; u++) {
+      const int o0 = P.desc[4 * u + 1], no = (P.desc[4 * u + 3] >> 8) & 0xfff;
+      for (int id = o0; id < o0 + no; id++) {
+        if (id < 0 || id >= nglob || id_owner[id] != -1) return 7;
+        id_owner[id] = (int)u;
+      }
+    }
+  }
   std::vector<int> owner(nglob, -1);
   int maxpts = 0;
   for (int ph = 0; ph < 2; ph++)
@@ -1003,16 +1096,28 @@ extern "C" int sf2d_plan_validate(int nspec, int nglob, const int* ibool_1based,
       for (int u = P.colour_start[ph][c]; u < P.colour_start[ph][c] + P.colour_count[ph][c]; u++) {
         int e0 = u, e1 = u + 1;  // element range of the unit
         if (mode == 0) {
-          e0 = P.patch_elem_off[u];
-          e1 = P.patch_elem_off[u + 1];
-          const int n0 = P.patch_node_off[u], n1 = P.patch_node_off[u + 1];
-          maxpts = std::max(maxpts, n1 - n0);
-          if (e1 - e0 < 1 || e1 - e0 > Mesh::PATCH_ELEMS || n1 - n0 > 256) return 3;
+          const int* d = P.desc.data() + 4 * (size_t)u;
+          const int ne = d[3] & 0xff, no = (d[3] >> 8) & 0xfff, nh = (d[3] >> 20) & 0xfff;
+          e0 = d[0];
+          e1 = d[0] + ne;
+          maxpts = std::max(maxpts, no + nh);
+          if (ne < 1 || ne > Mesh::PATCH_ELEMS || no + nh > 256 || no < 1) return 3;
+          for (int h = 0; h < nh; h++) {  // halo points: sorted, owned by another patch
+            const int id = P.halo[(size_t)d[2] + h];
+            if (id < 0 || id >= nglob || id_owner[id] == u || id_owner[id] < 0) return 8;
+            if (h > 0 && P.halo[(size_t)d[2] + h - 1] >= id) return 8;
+          }
+          std::vector<char> used_slot(no + nh, 0);
           for (int pos = e0; pos < e1; pos++)
             for (int p = 0; p < NGLL2; p++) {
               const int sl = P.slot[(size_t)pos * NGLL2 + p];
-              if (sl >= n1 - n0 || P.patch_nodes[n0 + sl] != ib[(size_t)P.order[pos] * NGLL2 + p]) return 4;
+              if (sl >= no + nh) return 4;
+              const int id = sl < no ? d[1] + sl : P.halo[(size_t)d[2] + sl - no];
+              if (id != P.newid[ib[(size_t)P.order[pos] * NGLL2 + p]]) return 4;
+              used_slot[sl] = 1;
             }
+          for (char x : used_slot)
+            if (!x) return 9;  // every listed point belongs to an element of the patch
         }
         if (mode == 2) break;  // atomics may share points
         for (int pos = e0; pos < e1; pos++)
@@ -1025,8 +1130,12 @@ extern "C" int sf2d_plan_validate(int nspec, int nglob, const int* ibool_1based,
     }
   stats[0] = P.ncolour[0];
   stats[1] = P.ncolour[1];
-  stats[2] = mode == 0 ? (int)P.patch_elem_off.size() - 1 : 0;
+  stats[2] = mode == 0 ? (int)P.desc.size() / 4 : 0;
   stats[3] = maxpts;
+  stats[4] = mode == 0 ? (int)P.halo.size() - 1 : 0;  // halo entries of all patches (spare entry excluded)
+  long long owned = 0;
+  for (size_t u = 0; mode == 0 && u < P.desc.size() / 4; u++) owned += (P.desc[4 * u + 3] >> 8) & 0xfff;
+  stats[5] = (int)owned;
   return 0;
 }
 

@@ -128,6 +128,12 @@ struct Sim {
   std::vector<int> phase_ispec;  // [2][nspec]
   long mesh = 0;
   int it_done = 0;                 // time steps already run
+  // model in the MODEL=gll parameterisation, double (gradient chain rule)
+  std::vector<double> model_rho, model_vp, model_vs;
+  // gradient: state saves inside the forward loop (0 none, 1 checkpoints every grad_chunk steps, 2 every step
+  // of the segment that starts after step grad_seg_base)
+  int grad_save = 0, grad_chunk = 0, grad_seg_base = 0, grad_ready = 0;
+  std::vector<float> grad_res;     // [nstep][nrec][2] dt (s - o)
   std::vector<double> sisux, sisuz;  // host seismograms (nlength = NSTEP), as the Fortran caller keeps them
 };
 
@@ -595,6 +601,10 @@ void* sf2d_sim_create(const sf2d_params* p, const double* rho, const double* vp,
     return nullptr;
   }
   build_materials(*s, rho, vp, vs);
+  const size_t ne = (size_t)s->nspec * NGLL2;
+  s->model_rho.assign(rho, rho + ne);
+  s->model_vp.assign(vp, vp + ne);
+  s->model_vs.assign(vs, vs + ne);
   build_stacey(*s);
   build_mass(*s);
   s->nsrc = nsrc;
@@ -745,6 +755,11 @@ int sf2d_sim_forward(void* sim, int nstep_run, double* seis, float* absorb, doub
     mark(step, 6);
     kernel_3_a_cuda_(&s->mesh, &deltatover2, &b_deltatover2, &one, &zero);
     mark(step, 7);
+    // the fields now hold (d_n, v', r_n) with the corrector pending: the state the reverse sweep needs
+    if (s->grad_save == 1 && it % s->grad_chunk == 0 && it < s->p.nstep)
+      sf2d_grad_save_state(&s->mesh, 0, it / s->grad_chunk - 1);
+    else if (s->grad_save == 2)
+      sf2d_grad_save_state(&s->mesh, 1, it - s->grad_seg_base - 1);
     mark(step, 8);
     if (s->nrec > 0) {
       int cur = it;
@@ -873,6 +888,119 @@ int sf2d_sim_adjoint(void* sim, const float* adj, const float* absorb, const flo
       kernels[l] = -s->rho[l] * deltat * rk[l];
       kernels[ne + l] = (float)(-2.0 * (double)s->mu[l] * (double)deltat * (double)mk[l]);
       kernels[2 * ne + l] = -s->kappa[l] * deltat * kk[l];
+    }
+  }
+  return 0;
+}
+
+int sf2d_sim_set_model(void* sim, const double* rho, const double* vp, const double* vs) {
+  Sim* s = (Sim*)sim;
+  if (!s) return 1;
+  const size_t ne = (size_t)s->nspec * NGLL2;
+  for (size_t l = 0; l < ne; l++)
+    if (!(rho[l] > 0.0) || !(vp[l] > 0.0) || !(vs[l] >= 0.0)) return 2;
+  build_materials(*s, rho, vp, vs);
+  s->model_rho.assign(rho, rho + ne);
+  s->model_vp.assign(vp, vp + ne);
+  s->model_vs.assign(vs, vs + ne);
+  build_mass(*s);
+  if (s->mesh)
+    sf2d_mesh_set_materials(&s->mesh, s->kappa.data(), s->mu.data(), s->rho_vp.data(), s->rho_vs.data(),
+                            s->rmassx.data(), s->rmassz.data());
+  return 0;
+}
+
+int sf2d_sim_gradient_forward(void* sim, const double* obs, int chunk, double* chi, double* seis) {
+  Sim* s = (Sim*)sim;
+  if (!s || !s->mesh || s->simulation_type != 1 || chunk < 1) return 1;
+  const int N = s->p.nstep;
+  sf2d_grad_prepare(&s->mesh, chunk, s->nrec);
+  s->grad_chunk = std::min(chunk, N);
+  sf2d_sim_reset(sim);
+  std::vector<double> sm((size_t)std::max(1, s->nrec) * 2 * N, 0.0);
+  s->grad_save = 1;
+  const int rc = sf2d_sim_forward(sim, N, sm.data(), nullptr, nullptr);
+  s->grad_save = 0;
+  if (rc != 0) return rc;
+  // chi = 1/2 dt sum |s - o|^2 in double; residuals dt (s - o) for the reverse sweep
+  const double dt = (double)(float)s->p.dt;
+  double acc = 0.0;
+  s->grad_res.assign((size_t)std::max(1, s->nrec) * 2 * N, 0.f);
+  for (int r = 0; r < s->nrec; r++)
+    for (int c = 0; c < 2; c++)
+      for (int n = 0; n < N; n++) {
+        const double q = sm[((size_t)r * 2 + c) * N + n] - obs[((size_t)r * 2 + c) * N + n];
+        acc += 0.5 * dt * q * q;
+        s->grad_res[((size_t)n * s->nrec + r) * 2 + c] = (float)(dt * q);
+      }
+  *chi = acc;
+  if (seis) std::copy(sm.begin(), sm.begin() + (size_t)s->nrec * 2 * N, seis);
+  s->grad_ready = 1;
+  return 0;
+}
+
+int sf2d_sim_gradient_backward(void* sim, double* grad, double* parts) {
+  Sim* s = (Sim*)sim;
+  if (!s || !s->mesh || !s->grad_ready) return 1;
+  const int N = s->p.nstep, C = s->grad_chunk;
+  const float dt = (float)s->p.dt, dto2 = (float)(0.5 * (double)dt);
+  sf2d_grad_begin_reverse(&s->mesh, s->grad_res.data());
+  const int nseg = (N + C - 1) / C;
+  for (int k = nseg - 1; k >= 0; k--) {
+    const int n0 = k * C, n1 = std::min(N, n0 + C);
+    if (k == 0) {
+      sf2d_sim_reset(sim);
+    } else {
+      sf2d_grad_restore_checkpoint(&s->mesh, k - 1, dto2);
+      s->it_done = n0;
+    }
+    s->grad_save = 2;
+    s->grad_seg_base = n0;
+    const int rc = sf2d_sim_forward(sim, n1 - n0, nullptr, nullptr, nullptr);
+    s->grad_save = 0;
+    if (rc != 0) return rc;
+    for (int n = n1; n > n0; n--) sf2d_grad_reverse_step(&s->mesh, n, n - n0 - 1, n == N ? 1 : 0, dt);
+    sf2d_grad_flush(&s->mesh);
+  }
+  const size_t ne = (size_t)s->nspec * NGLL2, nf = (size_t)s->nabs * NGLL;
+  std::vector<double> gk(ne), gm(ne), pm((size_t)2 * s->nglob), fvp(std::max<size_t>(1, nf)), fvs(std::max<size_t>(1, nf));
+  sf2d_grad_result(&s->mesh, gk.data(), gm.data(), pm.data(), fvp.data(), fvs.data());
+  // mass term: M_c(g) = sum w_i w_j J rho (+ the Stacey term below)
+  std::vector<double> gr(ne, 0.0), gvp(ne, 0.0), gvs(ne, 0.0);
+  for (int e = 0; e < s->nspec; e++)
+    for (int p = 0; p < NGLL2; p++) {
+      const size_t l = (size_t)e * NGLL2 + p;
+      const int g = s->ibool[l] - 1;
+      const double w = (double)s->basis.w[p % NGLL] * (double)s->basis.w[p / NGLL] * (double)s->jacobian[l];
+      gr[l] = w * (pm[2 * (size_t)g] + pm[2 * (size_t)g + 1]);
+    }
+  // Stacey traction and its Newmark mass term T_c = dt/2 w [rho_vp (nx+nz) n_c + rho_vs (1 - (nx+nz) n_c)]
+  for (int f = 0; f < s->nabs; f++) {
+    const int e = s->abs_ispec[f] - 1;
+    for (int k = 0; k < NGLL; k++) {
+      const size_t q = (size_t)f * NGLL + k;
+      const int ii = s->abs_ij[2 * q], jj = s->abs_ij[2 * q + 1];
+      if (ii > NGLL || jj > NGLL) continue;
+      const size_t l = (size_t)e * NGLL2 + (jj - 1) * NGLL + (ii - 1);
+      const int g = s->ibool[l] - 1;
+      const double nx = s->abs_normal[2 * q], nz = s->abs_normal[2 * q + 1], w = s->abs_jacw[q];
+      const double sn = nx + nz, px = pm[2 * (size_t)g], pz = pm[2 * (size_t)g + 1];
+      gvp[l] += fvp[q] + w * (double)dto2 * (px * sn * nx + pz * sn * nz);
+      gvs[l] += fvs[q] + w * (double)dto2 * (px * (1.0 - sn * nx) + pz * (1.0 - sn * nz));
+    }
+  }
+  // chain to (rho, vp, vs): mu = rho vs^2, kappa = rho vp^2 - mu, rho_vp = rho vp, rho_vs = rho vs
+  for (size_t l = 0; l < ne; l++) {
+    const double rho = s->model_rho[l], vp = s->model_vp[l], vs = s->model_vs[l];
+    grad[l] = gk[l] * (vp * vp - vs * vs) + gm[l] * vs * vs + gr[l] + gvp[l] * vp + gvs[l] * vs;
+    grad[ne + l] = gk[l] * 2.0 * rho * vp + gvp[l] * rho;
+    grad[2 * ne + l] = -gk[l] * 2.0 * rho * vs + gm[l] * 2.0 * rho * vs + gvs[l] * rho;
+    if (parts) {
+      parts[l] = gk[l];
+      parts[ne + l] = gm[l];
+      parts[2 * ne + l] = gr[l];
+      parts[3 * ne + l] = gvp[l];
+      parts[4 * ne + l] = gvs[l];
     }
   }
   return 0;

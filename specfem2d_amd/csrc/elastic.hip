@@ -3,13 +3,15 @@
 // The element force kernel is the hot spot (reference: src/gpu/kernels/Kernel_2_viscoelastic_impl.cu:294-611,
 // CPU: src/specfem2D/compute_forces_viscoelastic.F90:34-885).  It is HBM-bound (~2.9 flop/B), so the design is
 // about bytes and latency hiding, not math:
-//   * flat GLL-point mapping: a 256-thread block owns EPB=10 consecutive elements (250 active lanes), so every
-//     per-element array is read with contiguous, fully coalesced loads (no 25->32 padding traffic);
-//   * the element's displacement and the two stress-flux fields live in LDS between the two 5x5 contractions;
-//     the derivative matrices are staged in LDS once per block;
-//   * assembly into accel uses hardware FP32 atomics (global_atomic_add_f32, no return);
-//   * blocks are remapped so that each XCD walks a contiguous range of elements: neighbouring elements share
-//     nodes, so their gathers and atomics hit the same L2.
+//   * flat GLL-point mapping: a 256-thread workgroup holds 10 elements (250 active lanes), so every per-element
+//     array is read with contiguous, fully coalesced loads (no 25->32 padding traffic);
+//   * default assembly by coloured patches: a workgroup gathers a patch's points into LDS once, sums the element
+//     contributions there and writes each point back once; patches of one launch share no point, so no global
+//     atomics.  Points are renumbered so that a patch's own points form one contiguous range;
+//   * the patch kernel is persistent and software-pipelined: the loads of the next patch are in flight while
+//     the current one is computed;
+//   * workgroups are remapped so that each XCD walks a contiguous range: neighbouring patches share an L2.
+// Coloured-element (plain read-modify-write per colour) and FP32-atomic variants remain selectable.
 // All kernels run on the handle's stream.  Every launch bounds its indices against the array sizes it was given.
 #include "sf2d_mesh.h"
 
@@ -189,25 +191,47 @@ __global__ __launch_bounds__(FORCE_THREADS) void k_elastic_forces(
   }
 }
 
-// Coloured-patch variant: one workgroup per patch of <= FORCE_EPB connected elements.  The patch's unique points
-// are gathered once into LDS, the element contributions are summed there (LDS atomics), and every unique point
-// is written back with one read-modify-write: patches of one colour share no point.
-__global__ __launch_bounds__(FORCE_THREADS) void k_elastic_forces_patch(
-    int first_patch, int npatch, int per_xcd, const int* __restrict__ patch_elem_off,
-    const int* __restrict__ patch_node_off, const int* __restrict__ patch_nodes,
+// Coloured-patch variant, persistent and software-pipelined.  A patch is <= FORCE_EPB connected elements
+// whose points (nown owned, a contiguous id range, then nhalo halo ids) are gathered once into LDS; the
+// element contributions are summed there (LDS atomics) and every patch point is written back with one
+// read-modify-write (patches of one launch share no point).  Each workgroup walks a contiguous range of
+// patches; while it computes patch q, the loads of patch q+1 and the halo ids of patch q+2 are in flight.
+struct PatchRegs {
+  float xix, xiz, gax, gaz, kap, mu;
+  int slot;  // this lane's element point -> LDS slot
+  int g;     // point id loaded by this lane (lanes < nown + nhalo)
+  float2 u, a;
+};
+
+// load through a 32-bit byte offset from a uniform base (global_load with saddr + voffset); every array indexed
+// this way is smaller than 4 GiB (checked when the handle is prepared)
+template <typename T>
+__device__ __forceinline__ T ld_off(const T* base, unsigned byte_off) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + byte_off);
+}
+
+__device__ __forceinline__ int patch_ne(int c) { return c & 0xff; }
+__device__ __forceinline__ int patch_nown(int c) { return (c >> 8) & 0xfff; }
+__device__ __forceinline__ int patch_nhalo(int c) { return (c >> 20) & 0xfff; }
+
+__global__ __launch_bounds__(FORCE_THREADS) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_elastic_forces_patch(
+    int first_patch, int npatch, int chunk, const int4* __restrict__ desc, const int* __restrict__ halo,
     const unsigned char* __restrict__ elem_slot, const float* __restrict__ xix, const float* __restrict__ xiz,
     const float* __restrict__ gammax, const float* __restrict__ gammaz, const float* __restrict__ kappav,
     const float* __restrict__ muv, const float2* __restrict__ displ, float2* __restrict__ accel, const Basis basis) {
+  // workgroups of one XCD (bid % NXCD) take consecutive ranges; gridDim.x is a multiple of NXCD
   const int bid = blockIdx.x;
-  const int q = (bid % NXCD) * per_xcd + bid / NXCD;
-  if (q >= npatch) return;
-  const int patch = first_patch + q;
+  const int w = (bid % NXCD) * (gridDim.x / NXCD) + bid / NXCD;
+  const int qbeg = w * chunk;
+  const int qend = min(npatch, qbeg + chunk);
+  if (qbeg >= qend) return;
 
   __shared__ float s_hp[NGLL2], s_hpw[NGLL2], s_w[NGLL];
-  __shared__ float2 s_u[FORCE_THREADS];    // displacement of the patch's unique points
-  __shared__ float2 s_acc[FORCE_THREADS];  // force accumulator of the patch's unique points
-  __shared__ float s_ux[FORCE_THREADS], s_uz[FORCE_THREADS];
-  __shared__ float s_t1x[FORCE_THREADS], s_t1z[FORCE_THREADS], s_t2x[FORCE_THREADS], s_t2z[FORCE_THREADS];
+  __shared__ float2 s_u[FORCE_THREADS];    // displacement of the patch's points
+  __shared__ float2 s_acc[FORCE_THREADS];  // force accumulator of the patch's points
+  __shared__ float2 s_ue[FORCE_THREADS];  // displacement at each element point
+  __shared__ float2 s_t1[FORCE_THREADS];  // J (sigma . grad xi) at each element point
+  __shared__ float2 s_t2[FORCE_THREADS];  // J (sigma . grad gamma)
 
   const int tid = threadIdx.x;
   if (tid < NGLL2) {
@@ -215,91 +239,113 @@ __global__ __launch_bounds__(FORCE_THREADS) void k_elastic_forces_patch(
     s_hpw[tid] = basis.hpw[tid];
   }
   if (tid < NGLL) s_w[tid] = basis.w[tid];
-  const int e0 = patch_elem_off[patch], ne = patch_elem_off[patch + 1] - e0;
-  const int n0 = patch_node_off[patch], nn = patch_node_off[patch + 1] - n0;
-  // each of the first nn lanes owns one point of the patch: its displacement goes to LDS, its current
-  // acceleration is fetched now so that the final read-modify-write does not wait on a load
-  int gown = 0;
-  float2 aown = make_float2(0.f, 0.f);
-  if (tid < nn) {
-    gown = patch_nodes[n0 + tid];
-    s_u[tid] = displ[gown];
-    aown = accel[gown];
-    s_acc[tid] = make_float2(0.f, 0.f);
-  }
   const int le = tid / NGLL2;
   const int p = tid - le * NGLL2;
   const int i = p % NGLL;
   const int j = p / NGLL;
-  const bool active = (tid < FORCE_EPB * NGLL2) && (le < ne);
-  int slot = 0;
-  float xixl = 0.f, xizl = 0.f, gaxl = 0.f, gazl = 0.f, kap = 0.f, mu = 0.f;
-  if (active) {
-    const size_t l = (size_t)(e0 + le) * NGLL2 + p;
-    slot = elem_slot[l];
-    xixl = xix[l];
-    xizl = xiz[l];
-    gaxl = gammax[l];
-    gazl = gammaz[l];
-    kap = kappav[l];
-    mu = muv[l];
-  }
-  __syncthreads();
-  if (active) {
-    const float2 u = s_u[slot];
-    s_ux[tid] = u.x;
-    s_uz[tid] = u.y;
-  }
-  __syncthreads();
-
   const int base = le * NGLL2;
-  if (active) {
-    float dux_dxi = 0.f, duz_dxi = 0.f, dux_dga = 0.f, duz_dga = 0.f;
-#pragma unroll
-    for (int k = 0; k < NGLL; k++) {
-      const float hi = s_hp[i * NGLL + k];
-      const float hj = s_hp[j * NGLL + k];
-      dux_dxi += s_ux[base + j * NGLL + k] * hi;
-      duz_dxi += s_uz[base + j * NGLL + k] * hi;
-      dux_dga += s_ux[base + k * NGLL + i] * hj;
-      duz_dga += s_uz[base + k * NGLL + i] * hj;
+
+  // Loads are unconditional with clamped indices (lanes past the patch re-read a valid entry, the patch after
+  // the last one is the last one again), so nothing waits on a branch; only stores are predicated.
+  auto load_elements = [&](const int4 d, PatchRegs& r) {
+    const unsigned l = (unsigned)(d.x + min(le, patch_ne(d.w) - 1)) * NGLL2 + p;
+    r.slot = ld_off(elem_slot, l);
+    r.xix = ld_off(xix, 4u * l);
+    r.xiz = ld_off(xiz, 4u * l);
+    r.gax = ld_off(gammax, 4u * l);
+    r.gaz = ld_off(gammaz, 4u * l);
+    r.kap = ld_off(kappav, 4u * l);
+    r.mu = ld_off(muv, 4u * l);
+  };
+  auto load_halo_id = [&](const int4 d) -> int {
+    const int h = min(max(tid - patch_nown(d.w), 0), max(patch_nhalo(d.w) - 1, 0));
+    return ld_off(halo, 4u * (unsigned)(d.z + h));
+  };
+  auto load_points = [&](const int4 d, int halo_id, PatchRegs& r) {
+    r.g = tid < patch_nown(d.w) ? d.y + tid : halo_id;
+    r.u = ld_off(displ, 8u * (unsigned)r.g);
+    r.a = ld_off((const float2*)accel, 8u * (unsigned)r.g);
+  };
+
+  int q = qbeg;
+  int4 dc = desc[first_patch + q];
+  PatchRegs cur, nxt;
+  load_elements(dc, cur);
+  load_points(dc, load_halo_id(dc), cur);
+  int4 dn = desc[first_patch + min(q + 1, qend - 1)];
+  int hn = load_halo_id(dn);
+
+  for (; q < qend; q++) {
+    // in flight during this patch: the next patch, and the halo ids of the one after
+    load_elements(dn, nxt);
+    load_points(dn, hn, nxt);
+    const int4 dnn = desc[first_patch + min(q + 2, qend - 1)];
+    const int hnn = load_halo_id(dnn);
+
+    const int npts = patch_nown(dc.w) + patch_nhalo(dc.w);
+    const bool active = le < patch_ne(dc.w);
+    if (tid < npts) {
+      s_u[tid] = cur.u;
+      s_acc[tid] = make_float2(0.f, 0.f);
     }
-    const float dux_dxl = dux_dxi * xixl + dux_dga * gaxl;
-    const float dux_dzl = dux_dxi * xizl + dux_dga * gazl;
-    const float duz_dxl = duz_dxi * xixl + duz_dga * gaxl;
-    const float duz_dzl = duz_dxi * xizl + duz_dga * gazl;
-    const float jac = 1.f / (xixl * gazl - gaxl * xizl);
-    const float lambdal = kap - mu;
-    const float lp2m = kap + mu;
-    const float sxx = lp2m * dux_dxl + lambdal * duz_dzl;
-    const float szz = lp2m * duz_dzl + lambdal * dux_dxl;
-    const float sxz = mu * (duz_dxl + dux_dzl);
-    s_t1x[tid] = jac * (sxx * xixl + sxz * xizl);
-    s_t1z[tid] = jac * (sxz * xixl + szz * xizl);
-    s_t2x[tid] = jac * (sxx * gaxl + sxz * gazl);
-    s_t2z[tid] = jac * (sxz * gaxl + szz * gazl);
-  }
-  __syncthreads();
-  if (active) {
-    float t1x = 0.f, t1z = 0.f, t2x = 0.f, t2z = 0.f;
+    __syncthreads();
+    if (active) s_ue[tid] = s_u[cur.slot];
+    __syncthreads();
+    if (active) {
+      float dux_dxi = 0.f, duz_dxi = 0.f, dux_dga = 0.f, duz_dga = 0.f;
 #pragma unroll
-    for (int k = 0; k < NGLL; k++) {
-      const float wi = s_hpw[k * NGLL + i];
-      const float wj = s_hpw[k * NGLL + j];
-      t1x += s_t1x[base + j * NGLL + k] * wi;
-      t1z += s_t1z[base + j * NGLL + k] * wi;
-      t2x += s_t2x[base + k * NGLL + i] * wj;
-      t2z += s_t2z[base + k * NGLL + i] * wj;
+      for (int k = 0; k < NGLL; k++) {
+        const float hi = s_hp[i * NGLL + k];
+        const float hj = s_hp[j * NGLL + k];
+        const float2 ur = s_ue[base + j * NGLL + k];
+        const float2 uc = s_ue[base + k * NGLL + i];
+        dux_dxi += ur.x * hi;
+        duz_dxi += ur.y * hi;
+        dux_dga += uc.x * hj;
+        duz_dga += uc.y * hj;
+      }
+      const float dux_dxl = dux_dxi * cur.xix + dux_dga * cur.gax;
+      const float dux_dzl = dux_dxi * cur.xiz + dux_dga * cur.gaz;
+      const float duz_dxl = duz_dxi * cur.xix + duz_dga * cur.gax;
+      const float duz_dzl = duz_dxi * cur.xiz + duz_dga * cur.gaz;
+      const float jac = 1.f / (cur.xix * cur.gaz - cur.gax * cur.xiz);
+      const float lambdal = cur.kap - cur.mu;
+      const float lp2m = cur.kap + cur.mu;
+      const float sxx = lp2m * dux_dxl + lambdal * duz_dzl;
+      const float szz = lp2m * duz_dzl + lambdal * dux_dxl;
+      const float sxz = cur.mu * (duz_dxl + dux_dzl);
+      s_t1[tid] = make_float2(jac * (sxx * cur.xix + sxz * cur.xiz), jac * (sxz * cur.xix + szz * cur.xiz));
+      s_t2[tid] = make_float2(jac * (sxx * cur.gax + sxz * cur.gaz), jac * (sxz * cur.gax + szz * cur.gaz));
     }
-    atomicAdd(&s_acc[slot].x, -(s_w[j] * t1x + s_w[i] * t2x));
-    atomicAdd(&s_acc[slot].y, -(s_w[j] * t1z + s_w[i] * t2z));
-  }
-  __syncthreads();
-  if (tid < nn) {
-    const float2 f = s_acc[tid];
-    aown.x += f.x;
-    aown.y += f.y;
-    accel[gown] = aown;
+    __syncthreads();
+    if (active) {
+      float t1x = 0.f, t1z = 0.f, t2x = 0.f, t2z = 0.f;
+#pragma unroll
+      for (int k = 0; k < NGLL; k++) {
+        const float wi = s_hpw[k * NGLL + i];
+        const float wj = s_hpw[k * NGLL + j];
+        const float2 a1 = s_t1[base + j * NGLL + k];
+        const float2 a2 = s_t2[base + k * NGLL + i];
+        t1x += a1.x * wi;
+        t1z += a1.y * wi;
+        t2x += a2.x * wj;
+        t2z += a2.y * wj;
+      }
+      atomicAdd(&s_acc[cur.slot].x, -(s_w[j] * t1x + s_w[i] * t2x));
+      atomicAdd(&s_acc[cur.slot].y, -(s_w[j] * t1z + s_w[i] * t2z));
+    }
+    __syncthreads();
+    if (tid < npts) {
+      const float2 f = s_acc[tid];
+      float2 a = cur.a;
+      a.x += f.x;
+      a.y += f.y;
+      *reinterpret_cast<float2*>(reinterpret_cast<char*>(accel) + 8u * (unsigned)cur.g) = a;
+    }
+    cur = nxt;
+    dc = dn;
+    dn = dnn;
+    hn = hnn;
   }
 }
 
@@ -592,11 +638,27 @@ void launch_elastic_forces_patch(const Mesh* mp, int first_patch, int npatch, co
   if (npatch <= 0) return;
   if (first_patch < 0 || first_patch + npatch > mp->npatch)
     fail("patch range [%d, %d) outside the %d patches", first_patch, first_patch + npatch, mp->npatch);
-  const int per_xcd = (npatch + NXCD - 1) / NXCD;
-  hipLaunchKernelGGL(k_elastic_forces_patch, dim3(per_xcd * NXCD), dim3(FORCE_THREADS), 0, mp->stream, first_patch,
-                     npatch, per_xcd, mp->d_patch_elem_off, mp->d_patch_node_off, mp->d_patch_nodes, mp->d_elem_slot,
-                     mp->d_xix, mp->d_xiz, mp->d_gammax, mp->d_gammaz, mp->d_kappav, mp->d_muv, (const float2*)displ,
-                     (float2*)accel, mp->basis);
+  // persistent grid: as many workgroups as stay resident on the device (a multiple of the XCD count);
+  // SF2D_PATCH_WGS_PER_CU overrides the workgroups per CU (tuning), 0 gives one workgroup per patch
+  static int resident = -1;
+  if (resident < 0) {
+    int dev = 0, cus = 0, per_cu = 0;
+    check_hip(hipGetDevice(&dev), "hipGetDevice");
+    check_hip(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "CU count");
+    check_hip(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_elastic_forces_patch, FORCE_THREADS, 0),
+              "occupancy");
+    if (const char* env = getenv("SF2D_PATCH_WGS_PER_CU")) per_cu = atoi(env);
+    resident = per_cu <= 0 ? 0 : std::max(NXCD, (cus * per_cu) / NXCD * NXCD);
+    if (getenv("SF2D_VERBOSE")) fprintf(stderr, "k_elastic_forces_patch: %d CUs, %d workgroups resident\n", cus, resident);
+  }
+  const int all = (npatch + NXCD - 1) / NXCD * NXCD;
+  int nwg = resident > 0 ? std::min(resident, all) : all;
+  const int chunk = (npatch + nwg - 1) / nwg;
+  nwg = ((npatch + chunk - 1) / chunk + NXCD - 1) / NXCD * NXCD;
+  hipLaunchKernelGGL(k_elastic_forces_patch, dim3(nwg), dim3(FORCE_THREADS), 0, mp->stream, first_patch, npatch, chunk,
+                     (const int4*)mp->d_patch_desc, mp->d_halo_nodes, mp->d_elem_slot, mp->d_xix, mp->d_xiz,
+                     mp->d_gammax, mp->d_gammaz, mp->d_kappav, mp->d_muv, (const float2*)displ, (float2*)accel,
+                     mp->basis);
   check_hip(hipGetLastError(), "k_elastic_forces_patch launch");
 }
 

@@ -1,0 +1,348 @@
+// Exact discrete gradient of the L2 waveform misfit on the GPU: reverse mode of the elastic Newmark time loop
+// with checkpointed forward recomputation.  No counterpart in the reference, whose SIMULATION_TYPE=3 path
+// reconstructs the forward field backwards in time instead (src/specfem2D/compute_kernels.f90).
+//
+// One forward step n, as the reference's GPU sequence runs it (iterate_time.F90, compute_gpu_elastic.f90):
+//     d_n = d + dt v + dt^2/2 a          v' = v + dt/2 a                      (predictor)
+//     r_n = -K d_n - B v' + f_n                                               (forces, Stacey, sources)
+//     a_n = W r_n                        v_n = v' + dt/2 a_n                  (W = rmass_inverse; corrector)
+//     s_n = R d_n                                                             (seismograms)
+// With the corrector deferred to the next step, the device fields after step n hold exactly (d_n, v', r_n): a
+// checkpoint is a copy of them, and restoring one re-arms the pending corrector.
+//
+// Reverse step n, with (D, V, A) the cotangents of (d_n, v_n, a_n) and chi = 1/2 dt sum |s - o|^2:
+//     D += R^T dt (s_n - o_n)
+//     V' = V;  A += dt/2 V;  R~ = W A;   d chi/d M += -A r_n W^2           (k_adj_pointwise)
+//     D += -K R~;  V' += -B R~                                              (the forward force and Stacey kernels)
+//     d chi/d(kappa, mu) += -d a(d_n, R~)/d(kappa, mu);  d chi/d(rho vp, rho vs) += -R~ . dB v'   (k_grad_*)
+//     V = V' + dt D;  A = dt/2 V' + dt^2/2 D                                 (folded into the next k_adj_pointwise)
+// K and B are symmetric, so their transposes are the forward kernels themselves.  Gradients accumulate in FP32
+// over one checkpoint segment and are added into FP64 totals at its end.
+#include <algorithm>
+
+#include "sf2d_mesh.h"
+#include "specfem2d_gpu_ext.h"
+
+namespace sf2d {
+
+struct GradState {
+  int nstep = 0, chunk = 0, nckpt = 0, nrec = 0;
+  size_t n2 = 0, ne = 0, nf = 0;
+  float* ckpt = nullptr;  // [nckpt][3][n2]: (d, v', r) after steps chunk, 2 chunk, ...
+  float* seg = nullptr;   // [chunk][3][n2]: states of the segment being reversed
+  float *D = nullptr, *Vb = nullptr, *Rb = nullptr;
+  float *gk32 = nullptr, *gm32 = nullptr, *pm32 = nullptr, *gvp32 = nullptr, *gvs32 = nullptr;
+  double *gk = nullptr, *gm = nullptr, *pm = nullptr, *gvp = nullptr, *gvs = nullptr;
+  float* res = nullptr;   // [nstep][nrec][2] = dt (s - o)
+};
+
+namespace {
+
+constexpr int GT = 256;   // threads per workgroup
+constexpr int GEPB = 10;  // elements per workgroup in k_grad_elastic
+
+__global__ __launch_bounds__(GT) void k_adj_pointwise(int first, int n, float dt, float dto2, float h,
+                                                      const float* __restrict__ D, float* __restrict__ Vb,
+                                                      float* __restrict__ Rb, const float* __restrict__ rmx,
+                                                      const float* __restrict__ rmz, const float* __restrict__ r,
+                                                      float* __restrict__ pm) {
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    const int g = k >> 1;
+    const float w = (k & 1) ? rmz[g] : rmx[g];
+    float v = 0.f, a = 0.f;
+    if (!first) {  // state cotangents of the step after this one
+      const float vb = Vb[k], db = D[k];
+      v = vb + dt * db;
+      a = dto2 * vb + h * db;
+    }
+    a = a + dto2 * v;
+    Vb[k] = v;
+    Rb[k] = w * a;
+    pm[k] -= a * r[k] * w * w;
+  }
+}
+
+__global__ __launch_bounds__(64) void k_adj_receivers(int nrec, const float* __restrict__ res, const int* __restrict__ ispec_rec,
+                                                      const int* __restrict__ ibool, const float* __restrict__ xir,
+                                                      const float* __restrict__ gammar, const float* __restrict__ cosrot,
+                                                      const float* __restrict__ sinrot, float* __restrict__ D) {
+  const int r = blockIdx.x;
+  const int p = threadIdx.x;
+  if (r >= nrec || p >= NGLL2) return;
+  const int i = p % NGLL, j = p / NGLL;
+  const float px = res[2 * r], pz = res[2 * r + 1];
+  const float c = cosrot[r], sn = sinrot[r];
+  const float cx = c * px - sn * pz, cz = sn * px + c * pz;
+  const float hl = xir[i * nrec + r] * gammar[j * nrec + r];
+  const int g = ibool[(size_t)ispec_rec[r] * NGLL2 + p];
+  atomicAdd(&D[2 * g], hl * cx);
+  atomicAdd(&D[2 * g + 1], hl * cz);
+}
+
+// strain products of the forward displacement d_n and the adjoint force R~ at every element point
+__global__ __launch_bounds__(GT) void k_grad_elastic(int nelem, const int* __restrict__ ibool,
+                                                     const float* __restrict__ xix, const float* __restrict__ xiz,
+                                                     const float* __restrict__ gammax, const float* __restrict__ gammaz,
+                                                     const float2* __restrict__ dn, const float2* __restrict__ rb,
+                                                     float* __restrict__ gk, float* __restrict__ gm, const Basis basis) {
+  __shared__ float s_hp[NGLL2];
+  __shared__ float2 s_d[GT], s_b[GT];
+  const int tid = threadIdx.x;
+  if (tid < NGLL2) s_hp[tid] = basis.hp[tid];
+  const int le = tid / NGLL2, p = tid - le * NGLL2, i = p % NGLL, j = p / NGLL;
+  const int e = blockIdx.x * GEPB + le;
+  const bool active = le < GEPB && e < nelem;
+  const size_t l = (size_t)e * NGLL2 + p;
+  if (active) {
+    const int g = ibool[l];
+    s_d[tid] = dn[g];
+    s_b[tid] = rb[g];
+  }
+  __syncthreads();
+  if (!active) return;
+  const int base = le * NGLL2;
+  float a[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int k = 0; k < NGLL; k++) {
+    const float hi = s_hp[i * NGLL + k], hj = s_hp[j * NGLL + k];
+    const float2 dr = s_d[base + j * NGLL + k], dc = s_d[base + k * NGLL + i];
+    const float2 br = s_b[base + j * NGLL + k], bc = s_b[base + k * NGLL + i];
+    a[0] += dr.x * hi; a[1] += dr.y * hi; a[2] += dc.x * hj; a[3] += dc.y * hj;
+    a[4] += br.x * hi; a[5] += br.y * hi; a[6] += bc.x * hj; a[7] += bc.y * hj;
+  }
+  const float X = xix[l], Z = xiz[l], G = gammax[l], H = gammaz[l];
+  const float ux = a[0] * X + a[2] * G, uxz = a[0] * Z + a[2] * H, uzx = a[1] * X + a[3] * G, uz = a[1] * Z + a[3] * H;
+  const float wx = a[4] * X + a[6] * G, wxz = a[4] * Z + a[6] * H, wzx = a[5] * X + a[7] * G, wz = a[5] * Z + a[7] * H;
+  const float wj = basis.w[i] * basis.w[j] / (X * H - G * Z);
+  gk[l] -= wj * (ux + uz) * (wx + wz);
+  gm[l] -= wj * (ux * wx + uz * wz - ux * wz - uz * wx + (uxz + uzx) * (wxz + wzx));
+}
+
+// Stacey traction's dependence on rho vp and rho vs, per absorbing face point (same order as abs_ij)
+__global__ __launch_bounds__(64) void k_grad_stacey(int nabs, const int* __restrict__ abs_ispec, const int* __restrict__ abs_ij,
+                                                    const float* __restrict__ normal, const float* __restrict__ jacw,
+                                                    const int* __restrict__ ibool, const float2* __restrict__ vpn,
+                                                    const float2* __restrict__ rb, float* __restrict__ gvp,
+                                                    float* __restrict__ gvs) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nabs * NGLL) return;
+  const int f = q / NGLL;
+  const int ii = abs_ij[2 * q], jj = abs_ij[2 * q + 1];
+  if (ii < 0 || jj < 0) return;  // corner point owned by the other edge
+  const int g = ibool[(size_t)abs_ispec[f] * NGLL2 + jj * NGLL + ii];
+  const float nx = normal[2 * q], nz = normal[2 * q + 1], w = jacw[q];
+  const float2 v = vpn[g], b = rb[g];
+  const float vn = nx * v.x + nz * v.y, bn = nx * b.x + nz * b.y;
+  gvp[q] -= w * vn * bn;
+  gvs[q] -= w * (v.x * b.x + v.y * b.y - vn * bn);
+}
+
+__global__ __launch_bounds__(GT) void k_flush64(size_t n, float* __restrict__ a32, double* __restrict__ a64) {
+  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x) {
+    a64[k] += (double)a32[k];
+    a32[k] = 0.f;
+  }
+}
+
+int grid_for(size_t n) { return (int)std::min<size_t>((n + GT - 1) / GT, 65536); }
+
+GradState* need(Mesh* mp) {
+  if (!mp->grad) fail("gradient buffers not prepared (sf2d_grad_prepare)");
+  return mp->grad;
+}
+
+void copy_d2d(float* dst, const float* src, size_t n, hipStream_t s) {
+  check_hip(hipMemcpyAsync(dst, src, n * sizeof(float), hipMemcpyDeviceToDevice, s), "gradient state copy");
+}
+
+}  // namespace
+}  // namespace sf2d
+
+using namespace sf2d;
+
+extern "C" {
+
+int sf2d_grad_prepare(long* Mesh_pointer, int chunk, int nrec) {
+  Mesh* mp = handle(Mesh_pointer);
+  if (!mp->sorted) fail("sf2d_grad_prepare: prepare_fields_elastic_device was not called");
+  if (chunk < 1) fail("sf2d_grad_prepare: chunk %d < 1", chunk);
+  if (nrec != mp->nrec_local) fail("sf2d_grad_prepare: %d receivers, the handle has %d", nrec, mp->nrec_local);
+  if (mp->grad) sf2d_grad_release(Mesh_pointer);
+  GradState* G = new GradState();
+  G->nstep = mp->nstep;
+  G->chunk = std::min(chunk, mp->nstep);
+  G->nckpt = (mp->nstep - 1) / G->chunk;
+  G->nrec = nrec;
+  G->n2 = (size_t)2 * mp->nglob;
+  G->ne = (size_t)mp->nspec * NGLL2;
+  G->nf = (size_t)mp->num_abs * NGLL;
+  G->ckpt = device_zeros<float>(3 * G->n2 * std::max(1, G->nckpt));
+  G->seg = device_zeros<float>(3 * G->n2 * G->chunk);
+  G->D = device_zeros<float>(G->n2);
+  G->Vb = device_zeros<float>(G->n2);
+  G->Rb = device_zeros<float>(G->n2);
+  G->gk32 = device_zeros<float>(G->ne);
+  G->gm32 = device_zeros<float>(G->ne);
+  G->pm32 = device_zeros<float>(G->n2);
+  G->gvp32 = device_zeros<float>(std::max<size_t>(1, G->nf));
+  G->gvs32 = device_zeros<float>(std::max<size_t>(1, G->nf));
+  G->gk = device_zeros<double>(G->ne);
+  G->gm = device_zeros<double>(G->ne);
+  G->pm = device_zeros<double>(G->n2);
+  G->gvp = device_zeros<double>(std::max<size_t>(1, G->nf));
+  G->gvs = device_zeros<double>(std::max<size_t>(1, G->nf));
+  G->res = device_zeros<float>((size_t)std::max(1, nrec) * 2 * mp->nstep);
+  mp->grad = G;
+  return G->nckpt;
+}
+
+void sf2d_grad_release(long* Mesh_pointer) {
+  Mesh* mp = handle(Mesh_pointer);
+  GradState* G = mp->grad;
+  if (!G) return;
+  check_hip(hipStreamSynchronize(mp->stream), "sf2d_grad_release");
+  for (float** p : {&G->ckpt, &G->seg, &G->D, &G->Vb, &G->Rb, &G->gk32, &G->gm32, &G->pm32, &G->gvp32, &G->gvs32,
+                    &G->res})
+    device_free(*p);
+  for (double** p : {&G->gk, &G->gm, &G->pm, &G->gvp, &G->gvs}) device_free(*p);
+  delete G;
+  mp->grad = nullptr;
+}
+
+// copies the device fields (d_n, v', r_n) after a forward step into checkpoint (kind 0) or segment (kind 1) slot
+void sf2d_grad_save_state(long* Mesh_pointer, int kind, int slot) {
+  Mesh* mp = handle(Mesh_pointer);
+  GradState* G = need(mp);
+  const int nslot = kind == 0 ? G->nckpt : G->chunk;
+  if (slot < 0 || slot >= nslot) fail("sf2d_grad_save_state: slot %d outside 0..%d", slot, nslot - 1);
+  float* dst = (kind == 0 ? G->ckpt : G->seg) + (size_t)slot * 3 * G->n2;
+  copy_d2d(dst, mp->d_displ, G->n2, mp->stream);
+  copy_d2d(dst + G->n2, mp->d_veloc, G->n2, mp->stream);
+  copy_d2d(dst + 2 * G->n2, mp->d_accel, G->n2, mp->stream);
+}
+
+// restores checkpoint slot into the device fields, with the corrector of that step pending (dto2 = dt/2)
+void sf2d_grad_restore_checkpoint(long* Mesh_pointer, int slot, float dto2) {
+  Mesh* mp = handle(Mesh_pointer);
+  GradState* G = need(mp);
+  if (slot < 0 || slot >= G->nckpt) fail("sf2d_grad_restore_checkpoint: slot %d outside 0..%d", slot, G->nckpt - 1);
+  const float* src = G->ckpt + (size_t)slot * 3 * G->n2;
+  copy_d2d(mp->d_displ, src, G->n2, mp->stream);
+  copy_d2d(mp->d_veloc, src + G->n2, G->n2, mp->stream);
+  copy_d2d(mp->d_accel, src + 2 * G->n2, G->n2, mp->stream);
+  mp->pending_corrector[0] = 1;
+  mp->pending_dto2[0] = dto2;
+}
+
+// residuals dt (s - o) for all steps, [nstep][nrec][2]; zeroes the adjoint state and the gradient accumulators
+void sf2d_grad_begin_reverse(long* Mesh_pointer, const float* res) {
+  Mesh* mp = handle(Mesh_pointer);
+  GradState* G = need(mp);
+  if (G->nrec > 0)
+    check_hip(hipMemcpyAsync(G->res, res, sizeof(float) * 2 * G->nrec * G->nstep, hipMemcpyHostToDevice, mp->stream),
+              "residual H2D");
+  for (float* p : {G->D, G->Vb, G->Rb, G->pm32}) check_hip(hipMemsetAsync(p, 0, G->n2 * sizeof(float), mp->stream), "zero");
+  for (float* p : {G->gk32, G->gm32}) check_hip(hipMemsetAsync(p, 0, G->ne * sizeof(float), mp->stream), "zero");
+  for (double* p : {G->gk, G->gm}) check_hip(hipMemsetAsync(p, 0, G->ne * sizeof(double), mp->stream), "zero");
+  check_hip(hipMemsetAsync(G->pm, 0, G->n2 * sizeof(double), mp->stream), "zero");
+  const size_t nf = std::max<size_t>(1, G->nf);
+  for (float* p : {G->gvp32, G->gvs32}) check_hip(hipMemsetAsync(p, 0, nf * sizeof(float), mp->stream), "zero");
+  for (double* p : {G->gvp, G->gvs}) check_hip(hipMemsetAsync(p, 0, nf * sizeof(double), mp->stream), "zero");
+  check_hip(hipStreamSynchronize(mp->stream), "sf2d_grad_begin_reverse");
+}
+
+// reverse step n (1-based) from the states in segment slot (d_n, v', r_n); first = 1 for n = NSTEP
+void sf2d_grad_reverse_step(long* Mesh_pointer, int n, int slot, int first, float dt) {
+  Mesh* mp = handle(Mesh_pointer);
+  GradState* G = need(mp);
+  if (n < 1 || n > G->nstep) fail("sf2d_grad_reverse_step: step %d outside 1..%d", n, G->nstep);
+  if (slot < 0 || slot >= G->chunk) fail("sf2d_grad_reverse_step: slot %d outside 0..%d", slot, G->chunk - 1);
+  const float* dn = G->seg + (size_t)slot * 3 * G->n2;
+  const float* vpn = dn + G->n2;
+  const float* rn = dn + 2 * G->n2;
+  const float dto2 = (float)(0.5 * (double)dt), h = (float)(0.5 * (double)dt * (double)dt);
+  hipLaunchKernelGGL(k_adj_pointwise, dim3(grid_for(G->n2)), dim3(GT), 0, mp->stream, first, (int)G->n2, dt, dto2, h,
+                     G->D, G->Vb, G->Rb, mp->d_rmassx, mp->d_rmassz, rn, G->pm32);
+  if (G->nrec > 0)
+    hipLaunchKernelGGL(k_adj_receivers, dim3(G->nrec), dim3(64), 0, mp->stream, G->nrec,
+                       G->res + (size_t)(n - 1) * 2 * G->nrec, mp->d_ispec_selected_rec, mp->d_ibool, mp->d_xir,
+                       mp->d_gammar, mp->d_cosrot, mp->d_sinrot, G->D);
+  apply_element_forces(mp, 0, G->Rb, G->D);
+  apply_element_forces(mp, 1, G->Rb, G->D);
+  if (mp->stacey && mp->num_abs > 0) {
+    launch_stacey(mp, G->Rb, G->Vb, 0);
+    hipLaunchKernelGGL(k_grad_stacey, dim3((mp->num_abs * NGLL + 63) / 64), dim3(64), 0, mp->stream, mp->num_abs,
+                       mp->d_abs_ispec, mp->d_abs_ij, mp->d_abs_normal, mp->d_abs_jacw, mp->d_ibool,
+                       (const float2*)vpn, (const float2*)G->Rb, G->gvp32, G->gvs32);
+  }
+  hipLaunchKernelGGL(k_grad_elastic, dim3((mp->nspec + GEPB - 1) / GEPB), dim3(GT), 0, mp->stream, mp->nspec,
+                     mp->d_ibool, mp->d_xix, mp->d_xiz, mp->d_gammax, mp->d_gammaz, (const float2*)dn,
+                     (const float2*)G->Rb, G->gk32, G->gm32, mp->basis);
+  check_hip(hipGetLastError(), "sf2d_grad_reverse_step launch");
+}
+
+// adds the FP32 segment accumulators into the FP64 totals
+void sf2d_grad_flush(long* Mesh_pointer) {
+  Mesh* mp = handle(Mesh_pointer);
+  GradState* G = need(mp);
+  hipLaunchKernelGGL(k_flush64, dim3(grid_for(G->ne)), dim3(GT), 0, mp->stream, G->ne, G->gk32, G->gk);
+  hipLaunchKernelGGL(k_flush64, dim3(grid_for(G->ne)), dim3(GT), 0, mp->stream, G->ne, G->gm32, G->gm);
+  hipLaunchKernelGGL(k_flush64, dim3(grid_for(G->n2)), dim3(GT), 0, mp->stream, G->n2, G->pm32, G->pm);
+  if (G->nf) {
+    hipLaunchKernelGGL(k_flush64, dim3(grid_for(G->nf)), dim3(GT), 0, mp->stream, G->nf, G->gvp32, G->gvp);
+    hipLaunchKernelGGL(k_flush64, dim3(grid_for(G->nf)), dim3(GT), 0, mp->stream, G->nf, G->gvs32, G->gvs);
+  }
+  check_hip(hipGetLastError(), "sf2d_grad_flush launch");
+}
+
+// FP64 totals in the caller's numbering: g_kappa, g_mu [nspec][25] (ispec order), d chi/d M [nglob][2],
+// Stacey traction parts [nabs][5] (rho vp, rho vs)
+void sf2d_grad_result(long* Mesh_pointer, double* gk, double* gm, double* pm, double* gvp, double* gvs) {
+  Mesh* mp = handle(Mesh_pointer);
+  GradState* G = need(mp);
+  check_hip(hipStreamSynchronize(mp->stream), "sf2d_grad_result");
+  std::vector<double> t(std::max(G->ne, G->n2));
+  auto elem = [&](const double* dev, double* out) {
+    check_hip(hipMemcpy(t.data(), dev, G->ne * sizeof(double), hipMemcpyDeviceToHost), "gradient D2H");
+    for (int e = 0; e < mp->nspec; e++) {
+      const int pos = mp->pos_of_ispec[e];
+      for (int p = 0; p < NGLL2; p++) out[(size_t)e * NGLL2 + p] = t[(size_t)pos * NGLL2 + p];
+    }
+  };
+  elem(G->gk, gk);
+  elem(G->gm, gm);
+  check_hip(hipMemcpy(t.data(), G->pm, G->n2 * sizeof(double), hipMemcpyDeviceToHost), "gradient D2H");
+  for (int g = 0; g < mp->nglob; g++) {
+    const size_t id = mp->point_id.empty() ? (size_t)g : (size_t)mp->point_id[g];
+    pm[2 * g] = t[2 * id];
+    pm[2 * g + 1] = t[2 * id + 1];
+  }
+  if (G->nf) {
+    check_hip(hipMemcpy(gvp, G->gvp, G->nf * sizeof(double), hipMemcpyDeviceToHost), "gradient D2H");
+    check_hip(hipMemcpy(gvs, G->gvs, G->nf * sizeof(double), hipMemcpyDeviceToHost), "gradient D2H");
+  }
+}
+
+// replaces the materials of a prepared handle: kappa, mu, rho vp, rho vs [nspec][25] in ispec order and the
+// inverse mass rmassx, rmassz [nglob] in the caller's numbering
+void sf2d_mesh_set_materials(long* Mesh_pointer, const float* kappav, const float* muv, const float* rho_vp,
+                             const float* rho_vs, const float* rmassx, const float* rmassz) {
+  Mesh* mp = handle(Mesh_pointer);
+  if (!mp->sorted) fail("sf2d_mesh_set_materials: prepare_fields_elastic_device was not called");
+  flush_correctors(mp);
+  check_hip(hipStreamSynchronize(mp->stream), "sf2d_mesh_set_materials");
+  const size_t ne = (size_t)mp->nspec * NGLL2;
+  auto put = [&](float* dev, const float* host) {
+    if (!dev || !host) return;
+    std::vector<float> v = to_device_order(mp, host);
+    check_hip(hipMemcpy(dev, v.data(), ne * sizeof(float), hipMemcpyHostToDevice), "materials H2D");
+  };
+  put(mp->d_kappav, kappav);
+  put(mp->d_muv, muv);
+  put(mp->d_rho_vp, rho_vp);
+  put(mp->d_rho_vs, rho_vs);
+  if (rmassx) points_to_device(mp, mp->d_rmassx, rmassx, 1, "rmassx H2D");
+  if (rmassz) points_to_device(mp, mp->d_rmassz, rmassz, 1, "rmassz H2D");
+}
+
+}  // extern "C"

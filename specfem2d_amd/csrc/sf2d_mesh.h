@@ -62,15 +62,17 @@ struct Mesh {
   int colour_count[2][MAX_COLOURS] = {};
   // assembly: 0 coloured patches (default), 1 coloured elements, 2 FP32 atomics (SF2D_ASSEMBLY=patch|colour|atomic)
   int assembly = 0;
-  // coloured patches: connected groups of <= PATCH_ELEMS consecutive elements of a phase list; one workgroup per
-  // patch gathers the patch's unique points into LDS and writes each back once.  In patch mode the colour
-  // ranges above count patches, and the element order is (phase, colour, patch, element in patch).
+  // coloured patches: connected groups of <= PATCH_ELEMS consecutive elements of a phase list; one workgroup
+  // gathers a patch's unique points into LDS and writes each back once.  In patch mode the colour ranges above
+  // count patches, the element order is (phase, colour, patch, element in patch), and the points are
+  // renumbered patch-major: each patch owns a contiguous range of point ids and lists the few points it
+  // shares with earlier patches (its halo).  Every point array on the device uses that numbering.
   static constexpr int PATCH_ELEMS = 10;
   int npatch = 0;
-  int* d_patch_elem_off = nullptr;   // [npatch+1] first element (device order) of each patch
-  int* d_patch_node_off = nullptr;   // [npatch+1] offsets into d_patch_nodes
-  int* d_patch_nodes = nullptr;      // unique global points of each patch, ascending
+  int4* d_patch_desc = nullptr;      // [npatch] {first element, first owned id, halo offset, ne | nown<<8 | nhalo<<20}
+  int* d_halo_nodes = nullptr;       // halo point ids of each patch (sorted), plus one spare entry
   unsigned char* d_elem_slot = nullptr;  // [nspec][25] index of each element point in its patch's point list
+  std::vector<int> point_id;         // device id of each caller point (empty: identity)
   std::vector<int> h_ibool;          // 0-based, ispec order, kept until the elements are sorted
   std::vector<int> pos_of_ispec, ispec_of_pos;
   std::vector<int> h_src_ispec, h_rec_ispec;  // 0-based ispec, as passed
@@ -121,10 +123,21 @@ struct Mesh {
 
   // scratch
   unsigned int* d_scratch_u32 = nullptr;  // reductions
+
+  // exact discrete adjoint (gradient.hip), allocated by sf2d_grad_prepare
+  struct GradState* grad = nullptr;
 };
 
 // applies a deferred corrector now (no-op when none is pending)
 void flush_correctors(Mesh* mp);
+// acc -= K u over the elements of phase ph (0 outer, 1 inner), with the handle's assembly scheme
+void apply_element_forces(const Mesh* mp, int ph, const float* u, float* acc);
+// point arrays in the caller's numbering <-> the device numbering (n2 values per point)
+void points_to_device(const Mesh* mp, float* dev, const float* host, int n2, const char* what);
+void points_from_device(const Mesh* mp, float* host, const float* dev, int n2, const char* what);
+// per-element arrays: caller's ispec order <-> device element order
+std::vector<float> to_device_order(const Mesh* mp, const float* a);
+void to_ispec_order(const Mesh* mp, const float* dev_order, float* out);
 
 // Errors follow the reference's convention: message, error file, exit(EXIT_FAILURE).
 [[noreturn]] void fail(const char* fmt, ...);

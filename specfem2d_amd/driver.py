@@ -75,6 +75,12 @@ def load_libraries():
     host.sf2d_sim_forward.restype = C.c_int
     host.sf2d_sim_adjoint.argtypes = [P, P, P, P, P, P, P, P]
     host.sf2d_sim_adjoint.restype = C.c_int
+    host.sf2d_sim_set_model.argtypes = [P, P, P, P]
+    host.sf2d_sim_set_model.restype = C.c_int
+    host.sf2d_sim_gradient_forward.argtypes = [P, P, C.c_int, P, P]
+    host.sf2d_sim_gradient_forward.restype = C.c_int
+    host.sf2d_sim_gradient_backward.argtypes = [P, P, P]
+    host.sf2d_sim_gradient_backward.restype = C.c_int
     host.sf2d_sim_reset.argtypes = [P]
     host.sf2d_sim_steps_done.argtypes = [P]
     host.sf2d_sim_steps_done.restype = C.c_int
@@ -277,6 +283,50 @@ class Simulation:
             raise RuntimeError("sf2d_sim_adjoint failed")
         ker = ker.reshape(3, self.nspec, NGLL2)
         return seis[: self.cfg.nrec], {"rho": ker[0], "mu": ker[1], "kappa": ker[2]}
+
+    # model and gradient ------------------------------------------------
+    def set_model(self, rho, vp, vs):
+        """Replace the MODEL=gll parameters (per GLL point, [nspec,25] or flat); moduli, impedances and the
+        mass matrix are recomputed and, on a prepared device, uploaded."""
+        arrs = [np.ascontiguousarray(np.asarray(x, np.float64).reshape(-1)) for x in (rho, vp, vs)]
+        if any(a.size != self.nspec * NGLL2 for a in arrs):
+            raise ValueError("model arrays must hold nspec*25 values")
+        rc = self.host.sf2d_sim_set_model(self.h, *[_ptr(a) for a in arrs])
+        if rc != 0:
+            raise ValueError("invalid model (rho, vp must be > 0, vs >= 0)")
+        self._keep_model = arrs
+
+    def gradient_forward(self, obs, chunk: int = 100):
+        """Forward run from rest over all NSTEP steps with a checkpoint every ``chunk`` steps kept on the
+        device; returns (chi, seismograms) with chi = 1/2 dt sum |s - obs|^2 (obs [nrec,2,nstep])."""
+        self._need_device()
+        n = self.cfg.nstep
+        o = np.ascontiguousarray(np.asarray(obs, np.float64).reshape(-1))
+        if o.size != self.cfg.nrec * 2 * n:
+            raise ValueError("obs must be [nrec, 2, nstep]")
+        chi = C.c_double(0.0)
+        seis = np.zeros((max(1, self.cfg.nrec), 2, n))
+        rc = self.host.sf2d_sim_gradient_forward(self.h, _ptr(o), int(chunk), C.byref(chi), _ptr(seis))
+        if rc != 0:
+            raise RuntimeError("sf2d_sim_gradient_forward failed")
+        return chi.value, seis[: self.cfg.nrec]
+
+    def gradient_backward(self, parts: bool = False):
+        """Reverse sweep after gradient_forward: d chi / d (rho, vp, vs) as [nspec,25] arrays (dict), and with
+        ``parts`` also d chi / d (kappa, mu, rho via mass, rho vp, rho vs)."""
+        self._need_device()
+        ne = self.nspec * NGLL2
+        g = np.zeros(3 * ne)
+        pp = np.zeros(5 * ne) if parts else None
+        rc = self.host.sf2d_sim_gradient_backward(self.h, _ptr(g), _ptr(pp))
+        if rc != 0:
+            raise RuntimeError("sf2d_sim_gradient_backward failed (call gradient_forward first)")
+        g = g.reshape(3, self.nspec, NGLL2)
+        out = {"rho": g[0], "vp": g[1], "vs": g[2]}
+        if parts:
+            pp = pp.reshape(5, self.nspec, NGLL2)
+            return out, dict(zip(("kappa", "mu", "rho_mass", "rho_vp", "rho_vs"), pp))
+        return out
 
     def fields(self):
         self._need_device()

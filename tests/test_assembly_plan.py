@@ -2,7 +2,10 @@
 
 Invariants (sf2d_plan_validate): every listed element appears once in the device order; no two units
 (patches or elements) of one colour share a GLL point; patch slots address their patch's point list;
-patches hold at most 10 elements and 256 points."""
+patches hold at most 10 elements and 256 points; the patch mode's internal point numbering is a
+permutation in which every patch owns one contiguous id range (ranges disjoint) and lists, sorted, only
+points owned by other patches as its halo.  On structured meshes a 10-element strip owns its upper four
+GLL rows (160 points) and reads 45 halo points (the row below and the column to its left)."""
 import ctypes as C
 
 import numpy as np
@@ -18,7 +21,7 @@ def validate(ibool, nglob, lists, mode):
     np_ = lists.shape[1]
     ib = np.ascontiguousarray(ibool.reshape(-1), dtype=np.int32)
     ph = np.ascontiguousarray(lists.reshape(-1), dtype=np.int32)
-    stats = np.zeros(4, np.int32)
+    stats = np.zeros(6, np.int32)
     f = gpu.sf2d_plan_validate
     f.restype = C.c_int
     rc = f(C.c_int(nspec), C.c_int(nglob), ib.ctypes.data_as(C.c_void_p), C.c_int(np_),
@@ -50,8 +53,20 @@ def test_plan_invariants_structured(name, mode):
     if mode in (0, 1):
         assert stats[0] == 0 and stats[1] == 4  # 2 x 2 checkerboard of elements or of row patches
     if mode == 0:
-        assert stats[2] == sum((cfg.nx + 9) // 10 for _ in range(cfg.nz))  # row segments of <= 10 elements
+        npatch = sum((cfg.nx + 9) // 10 for _ in range(cfg.nz))  # row segments of <= 10 elements
+        assert stats[2] == npatch
         assert stats[3] <= 41 * 5
+        assert stats[5] == o.nglob  # every point owned by exactly one patch
+        # halo of each strip: the GLL row below it and the column to its left, where those exist
+        expect = 0
+        for ez in range(cfg.nz):
+            for x0 in range(0, cfg.nx, 10):
+                w = min(10, cfg.nx - x0)
+                if ez > 0:
+                    expect += 4 * w + 1 + (4 if x0 > 0 else 0)  # row below (with its corner), left column
+                elif x0 > 0:
+                    expect += 5  # bottom strip: the whole left column
+        assert stats[4] == expect
 
 
 @pytest.mark.parametrize("mode", [0, 1, 2])
