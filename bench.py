@@ -7,9 +7,15 @@ seismograms).  The metric is elastic GLL-point updates per second: nglob x steps
 Default workload: the 2000 x 1000-element mesh of the large forward configuration
 (20 km x 10 km, 10 m elements, 32,012,001 GLL points) with a seeded Gaussian random-field
 elastic model (correlation length 200 m, +-10 % around vp 3000, vs 1732, rho 2500), Stacey
-on three sides and a free surface, Ricker 8 Hz source, 200 surface receivers.  The water
+on three sides and a free surface, Ricker 8 Hz source, 200 surface receivers, dt 2e-4 s
+(Courant number 0.38 over the closest GLL points at the model's highest vp).  The water
 layer, fluid-solid coupling and CPML of that configuration are not part of this path yet,
 so the whole mesh is elastic with Stacey edges.
+
+The same line carries the second headline figure, the FWI gradient time per shot: the exact
+discrete gradient of an L2 waveform misfit (forward run with a checkpoint every 100 steps,
+segment recomputation and reverse sweep, all on the GPU) on the single-shot 800 x 400-element,
+2000-step adjoint-gradient workload, observed data from a +5 % vs anomaly.
 
 Prints one JSON line with the throughput, the roofline of the element-force kernel (HIP
 events on the library stream around the force launches of every step inside the timed region;
@@ -41,10 +47,10 @@ def force_kernel_bytes(nspec: int, nglob: int) -> int:
 
 
 WORKLOADS = {
-    # name: (nx, nz, element size m, dt, receivers, f0)
-    "c2_mesh_elastic_2000x1000": (2000, 1000, 10.0, 1.0e-3, 200, 8.0),
-    "c3_elastic_800x400": (800, 400, 10.0, 1.0e-3, 200, 10.0),
-    "c4_elastic_600x300": (600, 300, 10.0, 1.0e-3, 64, 10.0),
+    # name: (nx, nz, element size m, dt, receivers, f0); dt keeps the Courant number below 0.45
+    "c2_mesh_elastic_2000x1000": (2000, 1000, 10.0, 2.0e-4, 200, 8.0),
+    "c3_elastic_800x400": (800, 400, 20.0, 5.0e-4, 200, 10.0),
+    "c4_elastic_600x300": (600, 300, 20.0, 5.0e-4, 64, 10.0),
 }
 
 
@@ -83,6 +89,33 @@ def cpu_baseline(name: str, target_seconds: float = 15.0):
     }
 
 
+def fwi_gradient(repeats: int = 2):
+    """Seconds per shot of the exact FWI gradient on the 800 x 400-element adjoint-gradient workload."""
+    from specfem2d_amd import config
+    from specfem2d_amd.driver import Simulation
+    true_cfg = config.adjoint_gradient_800x400(anomaly=True)
+    cfg = config.adjoint_gradient_800x400(anomaly=False)
+    sim = Simulation(cfg)
+    # observed data: the same simulation run in the anomaly model
+    sim.set_model(true_cfg.rho, true_cfg.vp, true_cfg.vs)
+    sim.reset()
+    obs, _ = sim.forward()
+    sim.set_model(cfg.rho, cfg.vp, cfg.vs)
+    times = []
+    for _ in range(repeats):  # the first run includes the allocation of the checkpoint buffers
+        t0 = time.perf_counter()
+        chi, _ = sim.gradient_forward(obs, chunk=100)
+        g = sim.gradient_backward()
+        times.append(time.perf_counter() - t0)
+    gnorm = float(np.linalg.norm(g["vs"]))
+    sim.close()
+    if not (np.isfinite(chi) and chi > 0 and np.isfinite(gnorm) and gnorm > 0):
+        raise RuntimeError("degenerate gradient")
+    return {"workload": "adjoint_gradient_800x400", "s_per_shot": min(times), "nstep": cfg.nstep,
+            "checkpoint_every": 100, "nglob": sim.nglob, "misfit": chi, "grad_vs_l2": gnorm,
+            "method": "exact discrete adjoint, checkpointed recompute, FP64 gradient accumulation", "shots": 1}
+
+
 def read_traffic(name: str):
     """HBM bytes per force launch (FETCH_SIZE + WRITE_SIZE, calibrated against a plain copy) from the
     committed rocprofv3 PMC summary of this workload, if present."""
@@ -116,6 +149,7 @@ def main():
     ap.add_argument("--workload", default="c2_mesh_elastic_2000x1000", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-gradient", action="store_true", help="skip the FWI gradient time per shot")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="after the timed region, run a plain copy of this size (counter calibration)")
     args = ap.parse_args()
@@ -197,6 +231,8 @@ def main():
     sim.close()
     if args.copy_calibration_mib:
         copy_calibration(args.copy_calibration_mib)
+    if not args.no_gradient:
+        out["fwi_gradient"] = fwi_gradient()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.workload, args.cpu_seconds)
     print(json.dumps(out))

@@ -186,6 +186,29 @@ def random_elastic(nx: int, nz: int, elem_size: float, nstep: int, dt: float, nr
     return cfg.replace(rho=rho0 * (1.0 + pert), vp=vp0 * (1.0 + pert), vs=vs0 * (1.0 + pert))
 
 
+def adjoint_gradient_800x400(nstep: int = 2000, anomaly: bool = False) -> SimulationConfig:
+    """Single-shot adjoint-gradient workload: 800 x 400 elements of 20 m (16 km x 8 km), dt 5e-4 s (Courant
+    number 0.43 at vp 3000 over the closest GLL points), 2000 steps, homogeneous background vp 3000, vs 1732,
+    rho 2500 (MODEL=gll arrays), Stacey on three sides and a free surface, a vertical Ricker force of 10 Hz at
+    1200 m depth under the centre, 200 surface receivers.  With ``anomaly`` the model carries a +5 % Gaussian vs
+    anomaly of radius 500 m half-way between the source and the surface (the 'observed' model)."""
+    nx, nz, h = 800, 400, 20.0
+    xmax, depth = nx * h, nz * h
+    cfg = SimulationConfig(
+        name="adjoint_gradient_800x400", nx=nx, nz=nz, xmin=0.0, xmax=xmax, interfaces=[-depth, 0.0],
+        nz_layers=[nz], nstep=nstep, dt=5.0e-4, material_mode=1, stacey=True, absorbbottom=True,
+        absorbright=True, absorbtop=False, absorbleft=True,
+        sources=[Source(x=xmax / 2, z=-1200.0, f0=10.0, factor=1.0e10, anglesource=0.0, time_function_type=1)],
+        receivers=np.stack([np.linspace(h, xmax - h, 200), np.zeros(200)], axis=1))
+    n = cfg.nspec * NGLL2
+    rho, vp, vs = np.full(n, 2500.0), np.full(n, 3000.0), np.full(n, 1732.0)
+    if anomaly:
+        X, Z = cfg.gll_point_coords()
+        r2 = (X.reshape(-1) - xmax / 2) ** 2 + (Z.reshape(-1) + 600.0) ** 2
+        vs = vs * (1.0 + 0.05 * np.exp(-r2 / 500.0 ** 2))
+    return cfg.replace(rho=rho, vp=vp, vs=vs)
+
+
 def small_test_mesh(nx: int = 8, nz: int = 6, nstep: int = 200, seed: int = 0, stacey: bool = True,
                     gll: bool = False) -> SimulationConfig:
     """A small mesh for fast parity tests: 100 m elements, Ricker 15 Hz inside."""

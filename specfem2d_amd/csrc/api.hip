@@ -131,7 +131,9 @@ struct Plan {
   std::vector<int> newid;             // internal id of every point (a permutation of 0..nglob-1)
   std::vector<int> desc;              // [npatch][4] = {first element position, first owned id, halo offset, counts}
   std::vector<int> halo;              // internal ids of the halo points of all patches, one sorted list per patch
-  std::vector<unsigned char> slot;    // per device position and GLL point: index in its patch's point list
+  // per device position and GLL point: index of its point in the patch's point list (low 8 bits) and its
+  // rank among the element points of the patch that share that point (bits 8-9)
+  std::vector<uint16_t> slot;
 };
 
 int desc_counts(int ne, int nown, int nhalo) { return ne | (nown << 8) | (nhalo << 20); }
@@ -225,6 +227,7 @@ Plan build_plan(const std::vector<int>& ibool, int nspec, int nglob, const int* 
         const std::vector<int>& hl = phalo[q];
         P.desc.insert(P.desc.end(), {(int)order.size(), own0[q], (int)P.halo.size(),
                                      desc_counts(ne, nown[q], (int)hl.size())});
+        std::vector<int> mult(nown[q] + hl.size(), 0);
         for (int k = pstart[q]; k < pstart[q + 1]; k++) {
           const int e = elems[k];
           const int pos = (int)order.size();
@@ -234,7 +237,10 @@ Plan build_plan(const std::vector<int>& ibool, int nspec, int nglob, const int* 
             const int id = P.newid[ib[p]];
             const int sl = id >= own0[q] ? id - own0[q]
                                          : nown[q] + (int)(std::lower_bound(hl.begin(), hl.end(), id) - hl.begin());
-            P.slot[(size_t)pos * NGLL2 + p] = (unsigned char)sl;
+            const int rank = mult[sl]++;
+            if (rank >= Mesh::PATCH_SHARE)
+              fail("a point is shared by more than %d element points of one patch", Mesh::PATCH_SHARE);
+            P.slot[(size_t)pos * NGLL2 + p] = (uint16_t)(sl | (rank << 8));
           }
         }
         P.halo.insert(P.halo.end(), hl.begin(), hl.end());
@@ -290,6 +296,12 @@ void sort_elements_by_colour(Mesh* mp, const int* phase_lists, int np) {
     *a = dst;
   }
   device_free(d_map);
+  if (mp->assembly == 0) {
+    mp->d_geom = device_zeros<float4>(ne);
+    mp->d_moduli = device_zeros<float2>(ne);
+    launch_pack_elements(mp);
+    check_hip(hipStreamSynchronize(mp->stream), "pack elements");
+  }
   {
     // ibool in device element order and, in patch mode, internal point numbering
     std::vector<int> ib(ne);
@@ -662,6 +674,8 @@ void prepare_cleanup_device_(long* Mesh_pointer, int* ACOUSTIC_SIMULATION, int* 
   device_free(mp->d_scratch_u32);
   device_free(mp->d_patch_desc); device_free(mp->d_halo_nodes);
   device_free(mp->d_elem_slot);
+  device_free(mp->d_geom);
+  device_free(mp->d_moduli);
   if (mp->grad) sf2d_grad_release(Mesh_pointer);
   (void)hipStreamDestroy(mp->stream);
   delete mp;
@@ -1108,10 +1122,14 @@ extern "C" int sf2d_plan_validate(int nspec, int nglob, const int* ibool_1based,
             if (h > 0 && P.halo[(size_t)d[2] + h - 1] >= id) return 8;
           }
           std::vector<char> used_slot(no + nh, 0);
+          std::vector<int> rank_used(no + nh, 0);
           for (int pos = e0; pos < e1; pos++)
             for (int p = 0; p < NGLL2; p++) {
-              const int sl = P.slot[(size_t)pos * NGLL2 + p];
-              if (sl >= no + nh) return 4;
+              const int sl = P.slot[(size_t)pos * NGLL2 + p] & 0xff;
+              const int rank = P.slot[(size_t)pos * NGLL2 + p] >> 8;
+              if (sl >= no + nh || rank >= Mesh::PATCH_SHARE) return 4;
+              if (rank_used[sl] & (1 << rank)) return 10;  // two element points with one (point, rank) cell
+              rank_used[sl] |= 1 << rank;
               const int id = sl < no ? d[1] + sl : P.halo[(size_t)d[2] + sl - no];
               if (id != P.newid[ib[(size_t)P.order[pos] * NGLL2 + p]]) return 4;
               used_slot[sl] = 1;

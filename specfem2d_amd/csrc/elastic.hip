@@ -25,6 +25,7 @@ constexpr int NXCD = 8;        // gfx950: 8 accelerator complex dies, workgroups
 
 static_assert(Mesh::PATCH_ELEMS == FORCE_EPB, "one workgroup holds one patch");
 static_assert(FORCE_EPB * NGLL2 <= FORCE_THREADS && FORCE_THREADS <= 256, "patch points indexed by 8-bit slots");
+static_assert(Mesh::PATCH_SHARE == 4, "force cells are summed as two float4 per point");
 
 __device__ __forceinline__ void atomic_add_f32(float* p, float v) { unsafeAtomicAdd(p, v); }
 
@@ -192,14 +193,17 @@ __global__ __launch_bounds__(FORCE_THREADS) void k_elastic_forces(
 }
 
 // Coloured-patch variant, persistent and software-pipelined.  A patch is <= FORCE_EPB connected elements
-// whose points (nown owned, a contiguous id range, then nhalo halo ids) are gathered once into LDS; the
-// element contributions are summed there (LDS atomics) and every patch point is written back with one
-// read-modify-write (patches of one launch share no point).  Each workgroup walks a contiguous range of
-// patches; while it computes patch q, the loads of patch q+1 and the halo ids of patch q+2 are in flight.
+// whose points (nown owned, a contiguous id range, then nhalo halo ids) are gathered once into LDS.  Every
+// element point writes its force into its own LDS cell (point slot, rank among the element points sharing
+// that point), and each patch point sums its cells and is written back with one read-modify-write (patches of
+// one launch share no point): no atomics, and the sum is deterministic.  Each workgroup walks every W-th
+// patch; while it computes one, the loads of the next two patches are in flight (three register sets rotate,
+// so no copy forces an early wait).
 struct PatchRegs {
-  float xix, xiz, gax, gaz, kap, mu;
-  int slot;  // this lane's element point -> LDS slot
-  int g;     // point id loaded by this lane (lanes < nown + nhalo)
+  float4 geo;   // xix, xiz, gammax, gammaz of this lane's element point
+  float2 mod;   // kappa, mu
+  int slot;     // point slot | rank << 8
+  int g;        // point id loaded by this lane (lanes < nown + nhalo)
   float2 u, a;
 };
 
@@ -214,24 +218,28 @@ __device__ __forceinline__ int patch_ne(int c) { return c & 0xff; }
 __device__ __forceinline__ int patch_nown(int c) { return (c >> 8) & 0xfff; }
 __device__ __forceinline__ int patch_nhalo(int c) { return (c >> 20) & 0xfff; }
 
-__global__ __launch_bounds__(FORCE_THREADS) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_elastic_forces_patch(
-    int first_patch, int npatch, int chunk, const int4* __restrict__ desc, const int* __restrict__ halo,
-    const unsigned char* __restrict__ elem_slot, const float* __restrict__ xix, const float* __restrict__ xiz,
-    const float* __restrict__ gammax, const float* __restrict__ gammaz, const float* __restrict__ kappav,
-    const float* __restrict__ muv, const float2* __restrict__ displ, float2* __restrict__ accel, const Basis basis) {
-  // workgroups of one XCD (bid % NXCD) take consecutive ranges; gridDim.x is a multiple of NXCD
+// element-local LDS rows: 5 float2 padded to 6 (48 B), so that a row or column is read with two 16-byte and
+// one 8-byte access and consecutive rows fall on different banks
+constexpr int ROWF2 = 6;
+
+__global__ __launch_bounds__(FORCE_THREADS) void k_elastic_forces_patch(
+    int first_patch, int npatch, const int4* __restrict__ desc, const int* __restrict__ halo,
+    const uint16_t* __restrict__ elem_slot, const float4* __restrict__ geom, const float2* __restrict__ moduli,
+    const float2* __restrict__ displ, float2* __restrict__ accel, const Basis basis) {
+  // Workgroup w takes patches w, w + W, w + 2W, ... (W = gridDim.x): at any time the grid works on one window of
+  // consecutive patches, so the element arrays stream through DRAM in order.  Consecutive w run on one XCD
+  // (blocks are dispatched round-robin over the XCDs; gridDim.x is a multiple of NXCD).
   const int bid = blockIdx.x;
-  const int w = (bid % NXCD) * (gridDim.x / NXCD) + bid / NXCD;
-  const int qbeg = w * chunk;
-  const int qend = min(npatch, qbeg + chunk);
-  if (qbeg >= qend) return;
+  const int nw = gridDim.x;
+  const int w = (bid % NXCD) * (nw / NXCD) + bid / NXCD;
+  if (w >= npatch) return;
+  const int qlast = w + (npatch - 1 - w) / nw * nw;  // last patch of this workgroup
 
   __shared__ float s_hp[NGLL2], s_hpw[NGLL2], s_w[NGLL];
-  __shared__ float2 s_u[FORCE_THREADS];    // displacement of the patch's points
-  __shared__ float2 s_acc[FORCE_THREADS];  // force accumulator of the patch's points
-  __shared__ float2 s_ue[FORCE_THREADS];  // displacement at each element point
-  __shared__ float2 s_t1[FORCE_THREADS];  // J (sigma . grad xi) at each element point
-  __shared__ float2 s_t2[FORCE_THREADS];  // J (sigma . grad gamma)
+  __shared__ float2 s_u[FORCE_THREADS];                          // displacement of the patch's points
+  __shared__ float4 s_f[FORCE_THREADS * Mesh::PATCH_SHARE / 2];  // per point: PATCH_SHARE force cells (float2)
+  __shared__ float4 s_ur[FORCE_EPB * NGLL * ROWF2 / 2], s_uc[FORCE_EPB * NGLL * ROWF2 / 2];   // u by rows, columns
+  __shared__ float4 s_t1r[FORCE_EPB * NGLL * ROWF2 / 2], s_t2c[FORCE_EPB * NGLL * ROWF2 / 2]; // fluxes
 
   const int tid = threadIdx.x;
   if (tid < NGLL2) {
@@ -243,19 +251,15 @@ __global__ __launch_bounds__(FORCE_THREADS) __attribute__((amdgpu_waves_per_eu(8
   const int p = tid - le * NGLL2;
   const int i = p % NGLL;
   const int j = p / NGLL;
-  const int base = le * NGLL2;
+  const int erow = min(le, FORCE_EPB - 1) * NGLL;  // first row (and column) of this lane's element
 
   // Loads are unconditional with clamped indices (lanes past the patch re-read a valid entry, the patch after
   // the last one is the last one again), so nothing waits on a branch; only stores are predicated.
   auto load_elements = [&](const int4 d, PatchRegs& r) {
     const unsigned l = (unsigned)(d.x + min(le, patch_ne(d.w) - 1)) * NGLL2 + p;
-    r.slot = ld_off(elem_slot, l);
-    r.xix = ld_off(xix, 4u * l);
-    r.xiz = ld_off(xiz, 4u * l);
-    r.gax = ld_off(gammax, 4u * l);
-    r.gaz = ld_off(gammaz, 4u * l);
-    r.kap = ld_off(kappav, 4u * l);
-    r.mu = ld_off(muv, 4u * l);
+    r.slot = ld_off(elem_slot, 2u * l);
+    r.geo = ld_off(geom, 16u * l);
+    r.mod = ld_off(moduli, 8u * l);
   };
   auto load_halo_id = [&](const int4 d) -> int {
     const int h = min(max(tid - patch_nown(d.w), 0), max(patch_nhalo(d.w) - 1, 0));
@@ -267,85 +271,124 @@ __global__ __launch_bounds__(FORCE_THREADS) __attribute__((amdgpu_waves_per_eu(8
     r.a = ld_off((const float2*)accel, 8u * (unsigned)r.g);
   };
 
-  int q = qbeg;
-  int4 dc = desc[first_patch + q];
-  PatchRegs cur, nxt;
-  load_elements(dc, cur);
-  load_points(dc, load_halo_id(dc), cur);
-  int4 dn = desc[first_patch + min(q + 1, qend - 1)];
-  int hn = load_halo_id(dn);
+  int q = w;
+  // three register sets rotate through (compute, loaded, loading): no copies, so each compute step waits only
+  // for its own loads, issued two steps earlier
+  PatchRegs ra, rb, rc;
+  int4 da = desc[first_patch + q];
+  load_elements(da, ra);
+  load_points(da, load_halo_id(da), ra);
+  int4 db = desc[first_patch + min(q + nw, qlast)];
+  load_elements(db, rb);
+  load_points(db, load_halo_id(db), rb);
+  int4 dc = desc[first_patch + min(q + 2 * nw, qlast)];
+  int hc = load_halo_id(dc);
+  int ha = 0, hb = 0;
+  __syncthreads();
+  // the basis stays in LDS (broadcast reads): in registers it would cost 22 VGPRs and a wave per SIMD
+#define hpi(k) s_hp[i * NGLL + (k)]
+#define hpj(k) s_hp[j * NGLL + (k)]
+#define hwi(k) s_hpw[(k) * NGLL + i]
+#define hwj(k) s_hpw[(k) * NGLL + j]
+  float2* const ur2 = reinterpret_cast<float2*>(s_ur);
+  float2* const uc2 = reinterpret_cast<float2*>(s_uc);
+  float2* const t1r2 = reinterpret_cast<float2*>(s_t1r);
+  float2* const t2c2 = reinterpret_cast<float2*>(s_t2c);
+  float2* const f2 = reinterpret_cast<float2*>(s_f);
+  const int rrow = (erow + j) * ROWF2, rcol = (erow + i) * ROWF2;  // float2 index of this lane's row, column
 
-  for (; q < qend; q++) {
-    // in flight during this patch: the next patch, and the halo ids of the one after
-    load_elements(dn, nxt);
-    load_points(dn, hn, nxt);
-    const int4 dnn = desc[first_patch + min(q + 2, qend - 1)];
-    const int hnn = load_halo_id(dnn);
-
-    const int npts = patch_nown(dc.w) + patch_nhalo(dc.w);
-    const bool active = le < patch_ne(dc.w);
+  auto compute = [&](const PatchRegs& C, const int4 DC) {
+    const int npts = patch_nown(DC.w) + patch_nhalo(DC.w);
+    const bool active = le < patch_ne(DC.w);
     if (tid < npts) {
-      s_u[tid] = cur.u;
-      s_acc[tid] = make_float2(0.f, 0.f);
+      s_u[tid] = C.u;
+      s_f[2 * tid] = make_float4(0.f, 0.f, 0.f, 0.f);
+      s_f[2 * tid + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
     __syncthreads();
-    if (active) s_ue[tid] = s_u[cur.slot];
+    if (active) {
+      const float2 u = s_u[C.slot & 0xff];
+      ur2[rrow + i] = u;
+      uc2[rcol + j] = u;
+    }
     __syncthreads();
     if (active) {
-      float dux_dxi = 0.f, duz_dxi = 0.f, dux_dga = 0.f, duz_dga = 0.f;
-#pragma unroll
-      for (int k = 0; k < NGLL; k++) {
-        const float hi = s_hp[i * NGLL + k];
-        const float hj = s_hp[j * NGLL + k];
-        const float2 ur = s_ue[base + j * NGLL + k];
-        const float2 uc = s_ue[base + k * NGLL + i];
-        dux_dxi += ur.x * hi;
-        duz_dxi += ur.y * hi;
-        dux_dga += uc.x * hj;
-        duz_dga += uc.y * hj;
-      }
-      const float dux_dxl = dux_dxi * cur.xix + dux_dga * cur.gax;
-      const float dux_dzl = dux_dxi * cur.xiz + dux_dga * cur.gaz;
-      const float duz_dxl = duz_dxi * cur.xix + duz_dga * cur.gax;
-      const float duz_dzl = duz_dxi * cur.xiz + duz_dga * cur.gaz;
-      const float jac = 1.f / (cur.xix * cur.gaz - cur.gax * cur.xiz);
-      const float lambdal = cur.kap - cur.mu;
-      const float lp2m = cur.kap + cur.mu;
+      const float4 r01 = s_ur[rrow / 2], r23 = s_ur[rrow / 2 + 1];
+      const float2 r4 = ur2[rrow + 4];
+      const float4 c01 = s_uc[rcol / 2], c23 = s_uc[rcol / 2 + 1];
+      const float2 c4 = uc2[rcol + 4];
+      const float dux_dxi = r01.x * hpi(0) + r01.z * hpi(1) + r23.x * hpi(2) + r23.z * hpi(3) + r4.x * hpi(4);
+      const float duz_dxi = r01.y * hpi(0) + r01.w * hpi(1) + r23.y * hpi(2) + r23.w * hpi(3) + r4.y * hpi(4);
+      const float dux_dga = c01.x * hpj(0) + c01.z * hpj(1) + c23.x * hpj(2) + c23.z * hpj(3) + c4.x * hpj(4);
+      const float duz_dga = c01.y * hpj(0) + c01.w * hpj(1) + c23.y * hpj(2) + c23.w * hpj(3) + c4.y * hpj(4);
+      const float xixl = C.geo.x, xizl = C.geo.y, gaxl = C.geo.z, gazl = C.geo.w;
+      const float dux_dxl = dux_dxi * xixl + dux_dga * gaxl;
+      const float dux_dzl = dux_dxi * xizl + dux_dga * gazl;
+      const float duz_dxl = duz_dxi * xixl + duz_dga * gaxl;
+      const float duz_dzl = duz_dxi * xizl + duz_dga * gazl;
+      const float jac = 1.f / (xixl * gazl - gaxl * xizl);
+      const float lambdal = C.mod.x - C.mod.y;
+      const float lp2m = C.mod.x + C.mod.y;
       const float sxx = lp2m * dux_dxl + lambdal * duz_dzl;
       const float szz = lp2m * duz_dzl + lambdal * dux_dxl;
-      const float sxz = cur.mu * (duz_dxl + dux_dzl);
-      s_t1[tid] = make_float2(jac * (sxx * cur.xix + sxz * cur.xiz), jac * (sxz * cur.xix + szz * cur.xiz));
-      s_t2[tid] = make_float2(jac * (sxx * cur.gax + sxz * cur.gaz), jac * (sxz * cur.gax + szz * cur.gaz));
+      const float sxz = C.mod.y * (duz_dxl + dux_dzl);
+      t1r2[rrow + i] = make_float2(jac * (sxx * xixl + sxz * xizl), jac * (sxz * xixl + szz * xizl));
+      t2c2[rcol + j] = make_float2(jac * (sxx * gaxl + sxz * gazl), jac * (sxz * gaxl + szz * gazl));
     }
     __syncthreads();
     if (active) {
-      float t1x = 0.f, t1z = 0.f, t2x = 0.f, t2z = 0.f;
-#pragma unroll
-      for (int k = 0; k < NGLL; k++) {
-        const float wi = s_hpw[k * NGLL + i];
-        const float wj = s_hpw[k * NGLL + j];
-        const float2 a1 = s_t1[base + j * NGLL + k];
-        const float2 a2 = s_t2[base + k * NGLL + i];
-        t1x += a1.x * wi;
-        t1z += a1.y * wi;
-        t2x += a2.x * wj;
-        t2z += a2.y * wj;
-      }
-      atomicAdd(&s_acc[cur.slot].x, -(s_w[j] * t1x + s_w[i] * t2x));
-      atomicAdd(&s_acc[cur.slot].y, -(s_w[j] * t1z + s_w[i] * t2z));
+      const float4 a01 = s_t1r[rrow / 2], a23 = s_t1r[rrow / 2 + 1];
+      const float2 a4 = t1r2[rrow + 4];
+      const float4 b01 = s_t2c[rcol / 2], b23 = s_t2c[rcol / 2 + 1];
+      const float2 b4 = t2c2[rcol + 4];
+      const float t1x = a01.x * hwi(0) + a01.z * hwi(1) + a23.x * hwi(2) + a23.z * hwi(3) + a4.x * hwi(4);
+      const float t1z = a01.y * hwi(0) + a01.w * hwi(1) + a23.y * hwi(2) + a23.w * hwi(3) + a4.y * hwi(4);
+      const float t2x = b01.x * hwj(0) + b01.z * hwj(1) + b23.x * hwj(2) + b23.z * hwj(3) + b4.x * hwj(4);
+      const float t2z = b01.y * hwj(0) + b01.w * hwj(1) + b23.y * hwj(2) + b23.w * hwj(3) + b4.y * hwj(4);
+      f2[(C.slot & 0xff) * Mesh::PATCH_SHARE + (C.slot >> 8)] =
+          make_float2(-(s_w[j] * t1x + s_w[i] * t2x), -(s_w[j] * t1z + s_w[i] * t2z));
     }
     __syncthreads();
     if (tid < npts) {
-      const float2 f = s_acc[tid];
-      float2 a = cur.a;
-      a.x += f.x;
-      a.y += f.y;
-      *reinterpret_cast<float2*>(reinterpret_cast<char*>(accel) + 8u * (unsigned)cur.g) = a;
+      const float4 f01 = s_f[2 * tid], f23 = s_f[2 * tid + 1];
+      float2 a = C.a;
+      a.x += ((f01.x + f01.z) + f23.x) + f23.z;
+      a.y += ((f01.y + f01.w) + f23.y) + f23.w;
+      *reinterpret_cast<float2*>(reinterpret_cast<char*>(accel) + 8u * (unsigned)C.g) = a;
     }
-    cur = nxt;
-    dc = dn;
-    dn = dnn;
-    hn = hnn;
+  };
+#define SF2D_PATCH_STEP(RC, DCUR, RL, DL, HL, DNEXT, HNEXT)                 \
+  {                                                                         \
+    const int4 dn_ = desc[first_patch + min(q + 3 * nw, qlast)];            \
+    const int hn_ = load_halo_id(dn_);                                      \
+    load_elements(DL, RL);                                                  \
+    load_points(DL, HL, RL);                                                \
+    compute(RC, DCUR);                                                      \
+    DNEXT = dn_;                                                            \
+    HNEXT = hn_;                                                            \
+    q += nw;                                                                \
+    if (q > qlast) break;                                                   \
+  }
+  for (;;) {
+    SF2D_PATCH_STEP(ra, da, rc, dc, hc, da, ha)
+    SF2D_PATCH_STEP(rb, db, ra, da, ha, db, hb)
+    SF2D_PATCH_STEP(rc, dc, rb, db, hb, dc, hc)
+  }
+#undef SF2D_PATCH_STEP
+#undef hpi
+#undef hpj
+#undef hwi
+#undef hwj
+}
+
+// packs the patch kernel's per-point element data: geom = {xix, xiz, gammax, gammaz}, moduli = {kappa, mu}
+__global__ __launch_bounds__(256) void k_pack_elements(size_t n, const float* __restrict__ xix, const float* __restrict__ xiz,
+                                                       const float* __restrict__ gammax, const float* __restrict__ gammaz,
+                                                       const float* __restrict__ kappav, const float* __restrict__ muv,
+                                                       float4* __restrict__ geom, float2* __restrict__ moduli) {
+  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x) {
+    geom[k] = make_float4(xix[k], xiz[k], gammax[k], gammaz[k]);
+    moduli[k] = make_float2(kappav[k], muv[k]);
   }
 }
 
@@ -652,14 +695,18 @@ void launch_elastic_forces_patch(const Mesh* mp, int first_patch, int npatch, co
     if (getenv("SF2D_VERBOSE")) fprintf(stderr, "k_elastic_forces_patch: %d CUs, %d workgroups resident\n", cus, resident);
   }
   const int all = (npatch + NXCD - 1) / NXCD * NXCD;
-  int nwg = resident > 0 ? std::min(resident, all) : all;
-  const int chunk = (npatch + nwg - 1) / nwg;
-  nwg = ((npatch + chunk - 1) / chunk + NXCD - 1) / NXCD * NXCD;
-  hipLaunchKernelGGL(k_elastic_forces_patch, dim3(nwg), dim3(FORCE_THREADS), 0, mp->stream, first_patch, npatch, chunk,
-                     (const int4*)mp->d_patch_desc, mp->d_halo_nodes, mp->d_elem_slot, mp->d_xix, mp->d_xiz,
-                     mp->d_gammax, mp->d_gammaz, mp->d_kappav, mp->d_muv, (const float2*)displ, (float2*)accel,
-                     mp->basis);
+  const int nwg = resident > 0 ? std::min(resident, all) : all;
+  hipLaunchKernelGGL(k_elastic_forces_patch, dim3(nwg), dim3(FORCE_THREADS), 0, mp->stream, first_patch, npatch,
+                     (const int4*)mp->d_patch_desc, mp->d_halo_nodes, mp->d_elem_slot, mp->d_geom, mp->d_moduli,
+                     (const float2*)displ, (float2*)accel, mp->basis);
   check_hip(hipGetLastError(), "k_elastic_forces_patch launch");
+}
+
+void launch_pack_elements(const Mesh* mp) {
+  const size_t n = (size_t)mp->nspec * NGLL2;
+  hipLaunchKernelGGL(k_pack_elements, dim3(blocks_for((long)n, 256, 65536)), dim3(256), 0, mp->stream, n, mp->d_xix,
+                     mp->d_xiz, mp->d_gammax, mp->d_gammaz, mp->d_kappav, mp->d_muv, mp->d_geom, mp->d_moduli);
+  check_hip(hipGetLastError(), "k_pack_elements launch");
 }
 
 void launch_permute_elements(const float* src, float* dst, const int* ispec_of_pos, int nspec, hipStream_t s) {

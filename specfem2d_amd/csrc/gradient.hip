@@ -339,6 +339,10 @@ void sf2d_mesh_set_materials(long* Mesh_pointer, const float* kappav, const floa
   };
   put(mp->d_kappav, kappav);
   put(mp->d_muv, muv);
+  if (mp->d_moduli && (kappav || muv)) {
+    launch_pack_elements(mp);
+    check_hip(hipStreamSynchronize(mp->stream), "pack elements");
+  }
   put(mp->d_rho_vp, rho_vp);
   put(mp->d_rho_vs, rho_vs);
   if (rmassx) points_to_device(mp, mp->d_rmassx, rmassx, 1, "rmassx H2D");

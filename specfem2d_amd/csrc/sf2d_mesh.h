@@ -68,10 +68,13 @@ struct Mesh {
   // renumbered patch-major: each patch owns a contiguous range of point ids and lists the few points it
   // shares with earlier patches (its halo).  Every point array on the device uses that numbering.
   static constexpr int PATCH_ELEMS = 10;
+  static constexpr int PATCH_SHARE = 4;  // element points of one patch that may share a point
   int npatch = 0;
   int4* d_patch_desc = nullptr;      // [npatch] {first element, first owned id, halo offset, ne | nown<<8 | nhalo<<20}
   int* d_halo_nodes = nullptr;       // halo point ids of each patch (sorted), plus one spare entry
-  unsigned char* d_elem_slot = nullptr;  // [nspec][25] index of each element point in its patch's point list
+  uint16_t* d_elem_slot = nullptr;   // [nspec][25] slot of each element point in its patch's point list | rank << 8
+  float4* d_geom = nullptr;          // [nspec][25] {xix, xiz, gammax, gammaz}, device element order (patch mode)
+  float2* d_moduli = nullptr;        // [nspec][25] {kappa, mu}, device element order (patch mode)
   std::vector<int> point_id;         // device id of each caller point (empty: identity)
   std::vector<int> h_ibool;          // 0-based, ispec order, kept until the elements are sorted
   std::vector<int> pos_of_ispec, ispec_of_pos;
@@ -187,6 +190,7 @@ void launch_newmark_corrector_predictor(float* d, float* v, float* a, const floa
 void launch_elastic_forces(const Mesh* mp, int first, int nelem, int atomic, const float* displ, float* accel);
 void launch_elastic_forces_patch(const Mesh* mp, int first_patch, int npatch, const float* displ, float* accel);
 void launch_permute_elements(const float* src, float* dst, const int* ispec_of_pos, int nspec, hipStream_t s);
+void launch_pack_elements(const Mesh* mp);  // geom / moduli from the per-element arrays (patch mode)
 void launch_permute_elements_int(const int* src, int* dst, const int* ispec_of_pos, int nspec, hipStream_t s);
 void launch_stacey(const Mesh* mp, const float* veloc, float* accel, int write_store);
 void launch_stacey_backward(const Mesh* mp, float* b_accel);

@@ -80,3 +80,17 @@ def test_stacey_arrays(pair):
     expect = {1: (0, -1), 2: (1, 0), 3: (0, 1), 4: (-1, 0)}
     for f in range(len(isp)):
         np.testing.assert_allclose(nrm[f], np.tile(expect[edge[f]], (5, 1)), atol=1e-6)
+
+
+def test_set_model_equals_a_fresh_setup():
+    """sf2d_sim_set_model recomputes moduli, impedances and the mass matrix exactly as a new simulation would."""
+    base = config.small_test_mesh(nx=9, nz=7, nstep=50, gll=True, seed=1).replace(absorbtop=True)
+    other = config.small_test_mesh(nx=9, nz=7, nstep=50, gll=True, seed=2).replace(absorbtop=True)
+    sim = Simulation(base, device=False)
+    sim.set_model(other.rho, other.vp, other.vs)
+    fresh = Simulation(other, device=False)
+    for name in ("mu", "kappa", "rho_vp", "rho_vs", "rho"):
+        np.testing.assert_array_equal(sim.elem_array(name), fresh.elem_array(name), err_msg=name)
+    np.testing.assert_array_equal(sim.rmass_inverse(), fresh.rmass_inverse())
+    with pytest.raises(ValueError):
+        sim.set_model(-np.asarray(other.rho), other.vp, other.vs)
