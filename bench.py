@@ -39,6 +39,13 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
 
+def kernel_min_bytes(nspec: int, nglob: int, n_affine: int) -> int:
+    """Minimal bytes of the patch kernel's representation per step: per element 25 x (kappa, mu) = 200 B, the
+    25 slot/rank words (50 B), the geometry (16 B for an affine element, 400 B otherwise); per unique point
+    displ read and accel read-modify-write (24 B)."""
+    return 250 * nspec + 16 * n_affine + 400 * (nspec - n_affine) + 24 * nglob
+
+
 def force_kernel_bytes(nspec: int, nglob: int) -> int:
     """Algorithmic bytes of one element-force launch: per element 25 x (ibool + xix, xiz,
     gammax, gammaz, kappa, mu) = 700 B; per unique point displ read (8 B) and accel
@@ -185,6 +192,7 @@ def main():
     value = nglob * args.steps / wall
     # one force launch per colour; the events bracket all launches of a step
     launches = sim.colours(2)
+    n_affine = sim.affine_elements() if os.environ.get("SF2D_ASSEMBLY", "patch") == "patch" else 0
     fk_ms = timing["forces_ms"] / args.steps / launches
     fk_bytes = force_kernel_bytes(nspec, nglob) / launches
     achieved = fk_bytes / (fk_ms * 1e-3) / 1e9
@@ -211,6 +219,7 @@ def main():
             "boundaries": "Stacey bottom/left/right, free surface top",
             "parallelism": "1 GPU",
             "assembly": os.environ.get("SF2D_ASSEMBLY", "patch"),
+            "affine_elements": n_affine,
         },
         "roofline": {
             "bound": "hbm",
@@ -224,6 +233,9 @@ def main():
             "traffic": traffic,
             "algorithmic_bytes_per_launch": fk_bytes,
             "avg_launch_ms": fk_ms,
+            # the bytes this kernel's own representation needs at minimum: affine elements carry one 16-byte
+            # geometry record instead of 25 x 16 bytes (what it reads, not the formula above)
+            "kernel_min_bytes_per_launch": kernel_min_bytes(nspec, nglob, n_affine) / launches,
         },
         "gpu_loop_ms_events": timing["loop_ms"],
         "setup_s": t_setup,

@@ -299,8 +299,10 @@ void sort_elements_by_colour(Mesh* mp, const int* phase_lists, int np) {
   if (mp->assembly == 0) {
     mp->d_geom = device_zeros<float4>(ne);
     mp->d_moduli = device_zeros<float2>(ne);
+    mp->d_geom_elem = device_zeros<float4>((size_t)nspec);
     launch_pack_elements(mp);
-    check_hip(hipStreamSynchronize(mp->stream), "pack elements");
+    mp->no_affine = getenv("SF2D_NO_AFFINE") != nullptr;
+    mp->n_affine = launch_flag_affine(mp);
   }
   {
     // ibool in device element order and, in patch mode, internal point numbering
@@ -676,6 +678,7 @@ void prepare_cleanup_device_(long* Mesh_pointer, int* ACOUSTIC_SIMULATION, int* 
   device_free(mp->d_elem_slot);
   device_free(mp->d_geom);
   device_free(mp->d_moduli);
+  device_free(mp->d_geom_elem);
   if (mp->grad) sf2d_grad_release(Mesh_pointer);
   (void)hipStreamDestroy(mp->stream);
   delete mp;
@@ -1111,7 +1114,7 @@ extern "C" int sf2d_plan_validate(int nspec, int nglob, const int* ibool_1based,
         int e0 = u, e1 = u + 1;  // element range of the unit
         if (mode == 0) {
           const int* d = P.desc.data() + 4 * (size_t)u;
-          const int ne = d[3] & 0xff, no = (d[3] >> 8) & 0xfff, nh = (d[3] >> 20) & 0xfff;
+          const int ne = d[3] & 0xf, no = (d[3] >> 8) & 0xfff, nh = (d[3] >> 20) & 0xfff;
           e0 = d[0];
           e1 = d[0] + ne;
           maxpts = std::max(maxpts, no + nh);
@@ -1164,6 +1167,11 @@ extern "C" void sf2d_mesh_synchronize(long* Mesh_pointer) {
 }
 
 extern "C" void sf2d_mesh_flush(long* Mesh_pointer) { flush_correctors(handle(Mesh_pointer)); }
+
+extern "C" int sf2d_mesh_affine_elements(long* Mesh_pointer) {
+  Mesh* mp = handle(Mesh_pointer);
+  return mp->no_affine ? 0 : mp->n_affine;
+}
 
 extern "C" int sf2d_mesh_colours(long* Mesh_pointer, int iphase) {
   Mesh* mp = handle(Mesh_pointer);

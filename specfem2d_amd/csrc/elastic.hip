@@ -26,6 +26,7 @@ constexpr int NXCD = 8;        // gfx950: 8 accelerator complex dies, workgroups
 static_assert(Mesh::PATCH_ELEMS == FORCE_EPB, "one workgroup holds one patch");
 static_assert(FORCE_EPB * NGLL2 <= FORCE_THREADS && FORCE_THREADS <= 256, "patch points indexed by 8-bit slots");
 static_assert(Mesh::PATCH_SHARE == 4, "force cells are summed as two float4 per point");
+static_assert(Mesh::PATCH_ELEMS <= 15, "the element count of a patch descriptor has 4 bits");
 
 __device__ __forceinline__ void atomic_add_f32(float* p, float v) { unsafeAtomicAdd(p, v); }
 
@@ -214,7 +215,9 @@ __device__ __forceinline__ T ld_off(const T* base, unsigned byte_off) {
   return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + byte_off);
 }
 
-__device__ __forceinline__ int patch_ne(int c) { return c & 0xff; }
+// counts word of a patch descriptor: ne (bits 0-3), affine flag (bit 7), nown (bits 8-19), nhalo (bits 20-31)
+__device__ __forceinline__ int patch_ne(int c) { return c & 0xf; }
+__device__ __forceinline__ bool patch_affine(int c) { return (c & 0x80) != 0; }
 __device__ __forceinline__ int patch_nown(int c) { return (c >> 8) & 0xfff; }
 __device__ __forceinline__ int patch_nhalo(int c) { return (c >> 20) & 0xfff; }
 
@@ -224,8 +227,9 @@ constexpr int ROWF2 = 6;
 
 __global__ __launch_bounds__(FORCE_THREADS) void k_elastic_forces_patch(
     int first_patch, int npatch, const int4* __restrict__ desc, const int* __restrict__ halo,
-    const uint16_t* __restrict__ elem_slot, const float4* __restrict__ geom, const float2* __restrict__ moduli,
-    const float2* __restrict__ displ, float2* __restrict__ accel, const Basis basis) {
+    const uint16_t* __restrict__ elem_slot, const float4* __restrict__ geom, const float4* __restrict__ geom_elem,
+    const float2* __restrict__ moduli, const float2* __restrict__ displ, float2* __restrict__ accel,
+    const Basis basis) {
   // Workgroup w takes patches w, w + W, w + 2W, ... (W = gridDim.x): at any time the grid works on one window of
   // consecutive patches, so the element arrays stream through DRAM in order.  Consecutive w run on one XCD
   // (blocks are dispatched round-robin over the XCDs; gridDim.x is a multiple of NXCD).
@@ -255,10 +259,12 @@ __global__ __launch_bounds__(FORCE_THREADS) void k_elastic_forces_patch(
 
   // Loads are unconditional with clamped indices (lanes past the patch re-read a valid entry, the patch after
   // the last one is the last one again), so nothing waits on a branch; only stores are predicated.
+  // affine patches (every element's geometry constant over its 25 points) read one float4 per element
   auto load_elements = [&](const int4 d, PatchRegs& r) {
-    const unsigned l = (unsigned)(d.x + min(le, patch_ne(d.w) - 1)) * NGLL2 + p;
+    const unsigned e = (unsigned)(d.x + min(le, patch_ne(d.w) - 1));
+    const unsigned l = e * NGLL2 + p;
     r.slot = ld_off(elem_slot, 2u * l);
-    r.geo = ld_off(geom, 16u * l);
+    r.geo = patch_affine(d.w) ? ld_off(geom_elem, 16u * e) : ld_off(geom, 16u * l);
     r.mod = ld_off(moduli, 8u * l);
   };
   auto load_halo_id = [&](const int4 d) -> int {
@@ -379,6 +385,44 @@ __global__ __launch_bounds__(FORCE_THREADS) void k_elastic_forces_patch(
 #undef hpj
 #undef hwi
 #undef hwj
+}
+
+// per element: geometry of point 0, and whether it stands for all 25 points: every value within 2^-22 of the
+// element's largest geometry entry of the value at point 0.  Meshers leave cancellation residues (1e-18 against
+// 2e-2) where a parallelogram's entries are constant, so exact equality would almost never hold; within this
+// bound the substitution changes the element forces at the level of FP32 rounding.
+__global__ __launch_bounds__(256) void k_affine_elements(int nspec, const float4* __restrict__ geom,
+                                                         float4* __restrict__ geom_elem, int* __restrict__ affine) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nspec) return;
+  const float4 g0 = geom[(size_t)e * NGLL2];
+  float m = 0.f;
+  for (int p = 0; p < NGLL2; p++) {
+    const float4 g = geom[(size_t)e * NGLL2 + p];
+    m = fmaxf(m, fmaxf(fmaxf(fabsf(g.x), fabsf(g.y)), fmaxf(fabsf(g.z), fabsf(g.w))));
+  }
+  const float tol = m * 0x1p-22f;
+  int same = m > 0.f;  // false for NaN too
+  for (int p = 0; p < NGLL2; p++) {
+    const float4 g = geom[(size_t)e * NGLL2 + p];
+    same &= (fabsf(g.x - g0.x) <= tol) & (fabsf(g.y - g0.y) <= tol) & (fabsf(g.z - g0.z) <= tol) &
+            (fabsf(g.w - g0.w) <= tol);
+  }
+  geom_elem[e] = g0;
+  affine[e] = same;
+}
+
+// sets the affine flag (bit 7 of the counts word) of every patch whose elements are all affine
+__global__ __launch_bounds__(256) void k_flag_affine_patches(int npatch, int4* __restrict__ desc,
+                                                             const int* __restrict__ affine) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= npatch) return;
+  int4 d = desc[q];
+  const int ne = d.w & 0xf;
+  int all = 1;
+  for (int k = 0; k < ne; k++) all &= affine[d.x + k];
+  d.w = all ? (d.w | 0x80) : (d.w & ~0x80);
+  desc[q] = d;
 }
 
 // packs the patch kernel's per-point element data: geom = {xix, xiz, gammax, gammaz}, moduli = {kappa, mu}
@@ -697,8 +741,8 @@ void launch_elastic_forces_patch(const Mesh* mp, int first_patch, int npatch, co
   const int all = (npatch + NXCD - 1) / NXCD * NXCD;
   const int nwg = resident > 0 ? std::min(resident, all) : all;
   hipLaunchKernelGGL(k_elastic_forces_patch, dim3(nwg), dim3(FORCE_THREADS), 0, mp->stream, first_patch, npatch,
-                     (const int4*)mp->d_patch_desc, mp->d_halo_nodes, mp->d_elem_slot, mp->d_geom, mp->d_moduli,
-                     (const float2*)displ, (float2*)accel, mp->basis);
+                     (const int4*)mp->d_patch_desc, mp->d_halo_nodes, mp->d_elem_slot, mp->d_geom,
+                     mp->d_geom_elem, mp->d_moduli, (const float2*)displ, (float2*)accel, mp->basis);
   check_hip(hipGetLastError(), "k_elastic_forces_patch launch");
 }
 
@@ -707,6 +751,23 @@ void launch_pack_elements(const Mesh* mp) {
   hipLaunchKernelGGL(k_pack_elements, dim3(blocks_for((long)n, 256, 65536)), dim3(256), 0, mp->stream, n, mp->d_xix,
                      mp->d_xiz, mp->d_gammax, mp->d_gammaz, mp->d_kappav, mp->d_muv, mp->d_geom, mp->d_moduli);
   check_hip(hipGetLastError(), "k_pack_elements launch");
+}
+
+int launch_flag_affine(Mesh* mp) {
+  int* d_aff = device_zeros<int>((size_t)mp->nspec);
+  hipLaunchKernelGGL(k_affine_elements, dim3((mp->nspec + 255) / 256), dim3(256), 0, mp->stream, mp->nspec, mp->d_geom,
+                     mp->d_geom_elem, d_aff);
+  if (mp->npatch > 0 && !mp->no_affine)
+    hipLaunchKernelGGL(k_flag_affine_patches, dim3((mp->npatch + 255) / 256), dim3(256), 0, mp->stream, mp->npatch,
+                       mp->d_patch_desc, d_aff);
+  check_hip(hipGetLastError(), "affine flags launch");
+  std::vector<int> h((size_t)mp->nspec);
+  check_hip(hipMemcpyAsync(h.data(), d_aff, h.size() * sizeof(int), hipMemcpyDeviceToHost, mp->stream), "affine D2H");
+  check_hip(hipStreamSynchronize(mp->stream), "affine flags");
+  device_free(d_aff);
+  int n = 0;
+  for (int v : h) n += v;
+  return n;
 }
 
 void launch_permute_elements(const float* src, float* dst, const int* ispec_of_pos, int nspec, hipStream_t s) {

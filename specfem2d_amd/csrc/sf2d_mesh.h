@@ -75,6 +75,9 @@ struct Mesh {
   uint16_t* d_elem_slot = nullptr;   // [nspec][25] slot of each element point in its patch's point list | rank << 8
   float4* d_geom = nullptr;          // [nspec][25] {xix, xiz, gammax, gammaz}, device element order (patch mode)
   float2* d_moduli = nullptr;        // [nspec][25] {kappa, mu}, device element order (patch mode)
+  float4* d_geom_elem = nullptr;     // [nspec] geometry of point 0; the whole element's when it is affine
+  int n_affine = 0;                  // elements whose 25 points carry bitwise the same geometry
+  int no_affine = 0;                 // SF2D_NO_AFFINE: always read the per-point geometry
   std::vector<int> point_id;         // device id of each caller point (empty: identity)
   std::vector<int> h_ibool;          // 0-based, ispec order, kept until the elements are sorted
   std::vector<int> pos_of_ispec, ispec_of_pos;
@@ -191,6 +194,8 @@ void launch_elastic_forces(const Mesh* mp, int first, int nelem, int atomic, con
 void launch_elastic_forces_patch(const Mesh* mp, int first_patch, int npatch, const float* displ, float* accel);
 void launch_permute_elements(const float* src, float* dst, const int* ispec_of_pos, int nspec, hipStream_t s);
 void launch_pack_elements(const Mesh* mp);  // geom / moduli from the per-element arrays (patch mode)
+// per-element geometry and the affine flags of the patches; returns the number of affine elements
+int launch_flag_affine(Mesh* mp);
 void launch_permute_elements_int(const int* src, int* dst, const int* ispec_of_pos, int nspec, hipStream_t s);
 void launch_stacey(const Mesh* mp, const float* veloc, float* accel, int write_store);
 void launch_stacey_backward(const Mesh* mp, float* b_accel);

@@ -88,6 +88,8 @@ def load_libraries():
         getattr(host, f).argtypes = [P, P, P, P]
     gpu.sf2d_mesh_stream.argtypes = [C.POINTER(C.c_long)]
     gpu.sf2d_mesh_stream.restype = P
+    gpu.sf2d_mesh_affine_elements.argtypes = [C.POINTER(C.c_long)]
+    gpu.sf2d_mesh_affine_elements.restype = C.c_int
     gpu.sf2d_mesh_colours.argtypes = [C.POINTER(C.c_long), C.c_int]
     gpu.sf2d_mesh_colours.restype = C.c_int
     gpu.sf2d_mesh_synchronize.argtypes = [C.POINTER(C.c_long)]
@@ -181,6 +183,12 @@ class Simulation:
     def stream(self) -> int:
         mp = C.c_long(self.mesh_pointer)
         return self.gpu.sf2d_mesh_stream(C.byref(mp))
+
+    def affine_elements(self) -> int:
+        """Elements whose geometry is read as one record (patch assembly)."""
+        self._need_device()
+        mp = C.c_long(self.mesh_pointer)
+        return self.gpu.sf2d_mesh_affine_elements(C.byref(mp))
 
     def colours(self, iphase: int = 2) -> int:
         mp = C.c_long(self.mesh_pointer)

@@ -136,3 +136,31 @@ def test_assembly_modes_agree_with_oracle(mode, monkeypatch):
     print(f"assembly {mode}: rel-L2 vs FP32 oracle = {r:.3e}")
     assert r < TOL
     sim.close()
+
+
+def test_affine_geometry_records_are_exact(monkeypatch):
+    """Patches of elements whose geometry is constant over the 25 points (to 2^-22 of the largest entry) read
+    one record per element; the per-point path (SF2D_NO_AFFINE) must give the same wavefield to FP32 rounding,
+    and the oracle's traces to the usual tolerance."""
+    from specfem2d_amd.driver import Simulation
+    cfg = config.small_test_mesh(nx=14, nz=9, nstep=300, gll=True, seed=3, stacey=False)
+    monkeypatch.delenv("SF2D_NO_AFFINE", raising=False)
+    monkeypatch.setenv("SF2D_ASSEMBLY", "patch")
+    a = Simulation(cfg)
+    assert a.affine_elements() == cfg.nspec  # rectangular elements
+    sa, _ = a.forward()
+    fa = a.fields()
+    a.close()
+    monkeypatch.setenv("SF2D_NO_AFFINE", "1")
+    b = Simulation(cfg)
+    assert b.affine_elements() == 0
+    sb, _ = b.forward()
+    fb = b.fields()
+    b.close()
+    err = np.linalg.norm(sa - sb) / np.linalg.norm(sb)
+    print(f"affine records vs per-point geometry: traces rel-L2 {err:.2e}")
+    assert err < 1e-6
+    for x, y in zip(fa, fb):
+        assert np.linalg.norm(x - y) <= 1e-6 * np.linalg.norm(y)
+    ref = OracleSim(cfg, "sp").forward()
+    assert np.linalg.norm(sa - ref) / np.linalg.norm(ref) < 1e-5
