@@ -51,6 +51,19 @@ typedef struct {
 void* sf2d_sim_create(const sf2d_params* p, const double* rho, const double* vp, const double* vs, int nsrc,
                       const sf2d_source* src, int nrec, const double* rec_xz, int simulation_type,
                       int save_forward);
+/* A shot batch (no counterpart in the reference): nshot forward simulations of one mesh, model and receiver
+   set, shot k driven by the next shot_nsrc[k] sources of src (timed as that shot's own simulation: its t0 and
+   time shifts are those of a run with only its sources).  On the device all shots advance together
+   (sf2d_batch_* of specfem2d_gpu_ext.h); the gradient calls below then take obs[nshot][nrec][2][NSTEP],
+   return the misfit summed over the shots and the gradient of that sum. */
+void* sf2d_sim_create_shots(const sf2d_params* p, const double* rho, const double* vp, const double* vs, int nshot,
+                            const int* shot_nsrc, const sf2d_source* src, int nrec, const double* rec_xz);
+/* all NSTEP steps of every shot from rest; seis[nshot][nrec][2][NSTEP] displacement; timing[0] = device ms */
+int sf2d_sim_batch_forward(void* sim, double* seis, double* timing);
+int sf2d_sim_nshot(void* sim);                      /* 0 for a single simulation */
+void sf2d_sim_shot_t0(void* sim, double* t0);       /* [nshot] */
+void sf2d_sim_shot_misfits(void* sim, double* chi); /* [nshot], from the last gradient forward */
+void sf2d_sim_batch_get_fields(void* sim, int shot, float* displ, float* veloc, float* accel);
 /* Hands the setup to the GPU library in the order of prepare_gpu.f90:80-347. Returns 0 on success. */
 int sf2d_sim_prepare_device(void* sim);
 void sf2d_sim_destroy(void* sim);

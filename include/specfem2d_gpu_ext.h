@@ -60,6 +60,35 @@ void sf2d_grad_flush(long* Mesh_pointer);
    parts d chi / d (rho vp), d (rho vs) per absorbing face point [nabs][5] */
 void sf2d_grad_result(long* Mesh_pointer, double* g_kappa, double* g_mu, double* g_mass, double* g_rho_vp,
                       double* g_rho_vs);
+/* gradient buffers over the shot batch (sf2d_batch_prepare): the calls above then checkpoint and reverse all
+   shots together, residuals are [nstep][nshot][nrec][2] and the results are sums over the shots */
+int sf2d_batch_grad_prepare(long* Mesh_pointer, int chunk);
+
+/* ---------------------------------------------------------------------------
+ * Shot batches: nshot wavefields of one mesh and model, each driven by its own sources and recorded at the
+ * handle's receivers, advanced with one launch per kernel for the whole batch.  No counterpart in the
+ * reference, which runs one source set per simulation (iterate_time.F90); a step applies the sequence of
+ * update_displacement_cuda, compute_forces_viscoelastic_cuda, compute_stacey_viscoelastic_cuda,
+ * compute_add_sources_el_cuda and kernel_3_a_cuda to every shot.  The handle is prepared as usual with the
+ * sources of all shots (prepare_constants_device). */
+/* src_shot[k] = shot (0-based) of source k; allocates the fields at rest */
+int sf2d_batch_prepare(long* Mesh_pointer, int nshot, const int* src_shot);
+void sf2d_batch_release(long* Mesh_pointer);
+/* all fields and seismograms to zero (a run from rest) */
+void sf2d_batch_reset(long* Mesh_pointer);
+/* time step it (1-based) of every shot, with the reference's Newmark constants */
+void sf2d_batch_step(long* Mesh_pointer, int it, float deltat, float deltatsqover2, float deltatover2);
+/* records the displacement seismograms of step it (1-based) */
+void sf2d_batch_record(long* Mesh_pointer, int it);
+/* seismograms [nshot][2][nrec][nstep] */
+void sf2d_batch_seismograms(long* Mesh_pointer, float* out);
+/* applies the pending Newmark corrector */
+void sf2d_batch_flush(long* Mesh_pointer);
+/* fields of one shot, [nglob][2] each in the caller's numbering (NULL skips one) */
+void sf2d_batch_get_fields(long* Mesh_pointer, int shot, float* displ, float* veloc, float* accel);
+/* shots of the batch (0 without one) */
+int sf2d_batch_shots(long* Mesh_pointer);
+
 /* replaces kappa, mu, rho vp, rho vs ([nspec][25], ispec order) and rmassx, rmassz ([nglob], caller numbering);
    NULL leaves an array unchanged */
 void sf2d_mesh_set_materials(long* Mesh_pointer, const float* kappav, const float* muv, const float* rho_vp,

@@ -386,18 +386,22 @@ void to_ispec_order(const Mesh* mp, const float* dev_order, float* out) {
 }
 
 // acc -= K u over the elements of phase ph (0 outer, 1 inner), with the handle's assembly scheme
-void apply_element_forces(const Mesh* mp, int ph, const float* u, float* acc) {
+void apply_element_forces(const Mesh* mp, int ph, const float* u, float* acc, int nshot, long shot_stride) {
   if (mp->phase_count[ph] <= 0) return;
-  if (mp->assembly == 2) {
-    launch_elastic_forces(mp, mp->colour_start[ph][0], mp->phase_count[ph], 1, u, acc);
+  if (mp->assembly == 0) {  // patches: every shot of a colour in one launch
+    for (int c = 0; c < mp->ncolour[ph]; c++)
+      launch_elastic_forces_patch(mp, mp->colour_start[ph][c], mp->colour_count[ph][c], u, acc, nshot, shot_stride);
     return;
   }
-  for (int c = 0; c < mp->ncolour[ph]; c++) {
-    const int first = mp->colour_start[ph][c], count = mp->colour_count[ph][c];
-    if (mp->assembly == 0)
-      launch_elastic_forces_patch(mp, first, count, u, acc);
-    else
-      launch_elastic_forces(mp, first, count, 0, u, acc);
+  for (int s = 0; s < nshot; s++) {
+    const float* us = u + 2 * s * shot_stride;
+    float* as = acc + 2 * s * shot_stride;
+    if (mp->assembly == 2) {
+      launch_elastic_forces(mp, mp->colour_start[ph][0], mp->phase_count[ph], 1, us, as);
+      continue;
+    }
+    for (int c = 0; c < mp->ncolour[ph]; c++)
+      launch_elastic_forces(mp, mp->colour_start[ph][c], mp->colour_count[ph][c], 0, us, as);
   }
 }
 
@@ -680,6 +684,7 @@ void prepare_cleanup_device_(long* Mesh_pointer, int* ACOUSTIC_SIMULATION, int* 
   device_free(mp->d_moduli);
   device_free(mp->d_geom_elem);
   if (mp->grad) sf2d_grad_release(Mesh_pointer);
+  if (mp->batch) sf2d_batch_release(Mesh_pointer);
   (void)hipStreamDestroy(mp->stream);
   delete mp;
   *Mesh_pointer = 0;

@@ -135,6 +135,11 @@ struct Sim {
   int grad_save = 0, grad_chunk = 0, grad_seg_base = 0, grad_ready = 0;
   std::vector<float> grad_res;     // [nstep][nrec][2] dt (s - o)
   std::vector<double> sisux, sisuz;  // host seismograms (nlength = NSTEP), as the Fortran caller keeps them
+  // shot batch: nshot > 0 runs every shot's sources in its own wavefield (src_shot[k] = shot of source k)
+  int nshot = 0;
+  std::vector<int> src_shot, shot_nsrc;
+  std::vector<double> shot_t0;
+  std::vector<double> shot_chi;
 };
 
 // bilinear map of element e (src/shared/define_shape_functions.f90 NGNOD=4, recompute_jacobian.f90)
@@ -459,17 +464,13 @@ void locate(const Sim& s, double px, double pz, int* ispec_out, double* xi_out, 
 
 // sources (setup_sources_receivers.F90:979-1040, set_source_parameters.f90:116-250,
 // prepare_source_time_function.f90, comp_source_time_function.f90)
-void build_sources(Sim& s) {
-  const int n = s.nsrc;
-  s.src_ispec.assign(n, 0);
-  s.src_xi.assign(n, 0.0);
-  s.src_gamma.assign(n, 0.0);
-  s.sourcearrays.assign((size_t)n * NGLL2 * 2, 0.f);
-  s.stf.assign((size_t)n * s.p.nstep, 0.f);
+// sources k0 .. k0+n-1 as one simulation's source set: their common t0 and time shifts (setup_sources_receivers.F90,
+// compute_source_time_function), source arrays and time functions into the handle's [nsrc] layouts
+double build_source_set(Sim& s, int k0, int n) {
   std::vector<double> t0s(n), tsh(n);
   for (int k = 0; k < n; k++) {
-    tsh[k] = s.src[k].tshift;
-    t0s[k] = 1.20 / s.src[k].f0 + tsh[k];
+    tsh[k] = s.src[k0 + k].tshift;
+    t0s[k] = 1.20 / s.src[k0 + k].f0 + tsh[k];
   }
   double t0 = 0.0, min_tshift = 0.0;
   if (n == 1) {
@@ -483,10 +484,11 @@ void build_sources(Sim& s) {
   }
   if (s.p.user_t0 > 0.0 && t0 <= s.p.user_t0 + min_tshift) {
     t0 = s.p.user_t0;
-    for (int k = 0; k < n; k++) tsh[k] = t0s[k] - 1.20 / s.src[k].f0;
+    for (int k = 0; k < n; k++) tsh[k] = t0s[k] - 1.20 / s.src[k0 + k].f0;
   }
-  s.t0 = t0;
-  for (int k = 0; k < n; k++) {
+  const int ns = s.nsrc;
+  for (int q = 0; q < n; q++) {
+    const int k = k0 + q;
     locate(s, s.src[k].x, s.src[k].z, &s.src_ispec[k], &s.src_xi[k], &s.src_gamma[k]);
     double hx[NGLL], hg[NGLL];
     lagrange(s.src_xi[k], s.basis.xi, hx);
@@ -500,12 +502,34 @@ void build_sources(Sim& s) {
       }
     const double a = PI * PI * s.src[k].f0 * s.src[k].f0;
     for (int it = 0; it < s.p.nstep; it++) {
-      const double t = (double)it * s.p.dt - t0 - tsh[k];
+      const double t = (double)it * s.p.dt - t0 - tsh[q];
       const double g = std::exp(-a * t * t);
       const double v = (s.src[k].time_function_type == 1) ? (1.0 - 2.0 * a * t * t) * g : t * g;
-      s.stf[(size_t)it * n + k] = (float)(-s.src[k].factor * v);
+      s.stf[(size_t)it * ns + k] = (float)(-s.src[k].factor * v);
     }
   }
+  return t0;
+}
+
+// one source set for the simulation, or one per shot of a batch (each shot timed as its own simulation)
+void build_sources(Sim& s) {
+  const int n = s.nsrc;
+  s.src_ispec.assign(n, 0);
+  s.src_xi.assign(n, 0.0);
+  s.src_gamma.assign(n, 0.0);
+  s.sourcearrays.assign((size_t)n * NGLL2 * 2, 0.f);
+  s.stf.assign((size_t)n * s.p.nstep, 0.f);
+  if (s.nshot == 0) {
+    s.t0 = build_source_set(s, 0, n);
+    return;
+  }
+  s.shot_t0.assign(s.nshot, 0.0);
+  s.src_shot.assign(n, 0);
+  for (int sh = 0, k0 = 0; sh < s.nshot; k0 += s.shot_nsrc[sh], sh++) {
+    s.shot_t0[sh] = build_source_set(s, k0, s.shot_nsrc[sh]);
+    for (int q = 0; q < s.shot_nsrc[sh]; q++) s.src_shot[k0 + q] = sh;
+  }
+  s.t0 = s.shot_t0[0];
 }
 
 void build_receivers(Sim& s, const double* rec_xz) {
@@ -581,15 +605,12 @@ void prepare_device(Sim& s) {
   int size = 2 * s.nglob;
   transfer_fields_el_to_device_(&size, zeros.data(), zeros.data(), zeros.data(), &s.mesh);
   if (s.simulation_type == 3) transfer_b_fields_to_device_(&size, zeros.data(), zeros.data(), zeros.data(), &s.mesh);
+  if (s.nshot > 0) sf2d_batch_prepare(&s.mesh, s.nshot, s.src_shot.data());
 }
 
-}  // namespace
-
-extern "C" {
-
-void* sf2d_sim_create(const sf2d_params* p, const double* rho, const double* vp, const double* vs, int nsrc,
-                      const sf2d_source* src, int nrec, const double* rec_xz, int simulation_type,
-                      int save_forward) {
+void* create(const sf2d_params* p, const double* rho, const double* vp, const double* vs, int nsrc,
+             const sf2d_source* src, int nrec, const double* rec_xz, int simulation_type, int save_forward, int nshot,
+             const int* shot_nsrc) {
   Sim* s = new Sim();
   s->p = *p;
   s->simulation_type = simulation_type;
@@ -609,10 +630,61 @@ void* sf2d_sim_create(const sf2d_params* p, const double* rho, const double* vp,
   build_mass(*s);
   s->nsrc = nsrc;
   s->src.assign(src, src + nsrc);
+  s->nshot = nshot;
+  if (nshot > 0) s->shot_nsrc.assign(shot_nsrc, shot_nsrc + nshot);
   build_sources(*s);
   s->nrec = nrec;
   build_receivers(*s, rec_xz);
   return s;
+}
+
+// steps it_begin..it_end of every shot of a batch, with the gradient's state saves; seismograms when record
+void batch_steps(Sim* s, int it_begin, int it_end, bool record) {
+  const float deltat = (float)s->p.dt;
+  const float deltatover2 = (float)(0.5 * (double)deltat);
+  const float deltatsqover2 = (float)(0.5 * (double)deltat * (double)deltat);
+  for (int it = it_begin; it <= it_end; it++) {
+    sf2d_batch_step(&s->mesh, it, deltat, deltatsqover2, deltatover2);
+    if (s->grad_save == 1 && it % s->grad_chunk == 0 && it < s->p.nstep)
+      sf2d_grad_save_state(&s->mesh, 0, it / s->grad_chunk - 1);
+    else if (s->grad_save == 2)
+      sf2d_grad_save_state(&s->mesh, 1, it - s->grad_seg_base - 1);
+    if (record && s->nrec > 0) sf2d_batch_record(&s->mesh, it);
+  }
+  s->it_done = it_end;
+}
+
+// device seismograms [nshot][2][nrec][nstep] -> [nshot][nrec][2][nstep] doubles
+void batch_seismograms(Sim* s, double* out) {
+  const int N = s->p.nstep, R = s->nrec;
+  std::vector<float> dev((size_t)s->nshot * 2 * std::max(1, R) * N);
+  if (R > 0) sf2d_batch_seismograms(&s->mesh, dev.data());
+  for (int sh = 0; sh < s->nshot; sh++)
+    for (int r = 0; r < R; r++)
+      for (int c = 0; c < 2; c++)
+        for (int n = 0; n < N; n++)
+          out[(((size_t)sh * R + r) * 2 + c) * N + n] = dev[(((size_t)sh * 2 + c) * R + r) * N + n];
+}
+
+}  // namespace
+
+extern "C" {
+
+void* sf2d_sim_create(const sf2d_params* p, const double* rho, const double* vp, const double* vs, int nsrc,
+                      const sf2d_source* src, int nrec, const double* rec_xz, int simulation_type,
+                      int save_forward) {
+  return create(p, rho, vp, vs, nsrc, src, nrec, rec_xz, simulation_type, save_forward, 0, nullptr);
+}
+
+void* sf2d_sim_create_shots(const sf2d_params* p, const double* rho, const double* vp, const double* vs, int nshot,
+                            const int* shot_nsrc, const sf2d_source* src, int nrec, const double* rec_xz) {
+  if (nshot < 1 || !shot_nsrc) return nullptr;
+  int nsrc = 0;
+  for (int k = 0; k < nshot; k++) {
+    if (shot_nsrc[k] < 1) return nullptr;
+    nsrc += shot_nsrc[k];
+  }
+  return create(p, rho, vp, vs, nsrc, src, nrec, rec_xz, 1, 0, nshot, shot_nsrc);
 }
 
 int sf2d_sim_prepare_device(void* sim) {
@@ -801,6 +873,49 @@ int sf2d_sim_forward(void* sim, int nstep_run, double* seis, float* absorb, doub
   return 0;
 }
 
+int sf2d_sim_batch_forward(void* sim, double* seis, double* timing) {
+  Sim* s = (Sim*)sim;
+  if (!s || !s->mesh || s->nshot < 1) return 1;
+  hipStream_t stream = (hipStream_t)sf2d_mesh_stream(&s->mesh);
+  sf2d_batch_reset(&s->mesh);
+  hipEvent_t e0, e1;
+  if (timing) {
+    hc(hipEventCreate(&e0));
+    hc(hipEventCreate(&e1));
+    hc(hipEventRecord(e0, stream));
+  }
+  batch_steps(s, 1, s->p.nstep, true);
+  sf2d_batch_flush(&s->mesh);
+  if (timing) {
+    hc(hipEventRecord(e1, stream));
+    hc(hipEventSynchronize(e1));
+    float ms = 0.f;
+    hc(hipEventElapsedTime(&ms, e0, e1));
+    timing[0] = ms;
+    hc(hipEventDestroy(e0));
+    hc(hipEventDestroy(e1));
+  }
+  if (seis) batch_seismograms(s, seis);
+  return 0;
+}
+
+int sf2d_sim_nshot(void* sim) { return ((Sim*)sim)->nshot; }
+
+void sf2d_sim_shot_t0(void* sim, double* t0) {
+  Sim* s = (Sim*)sim;
+  for (int k = 0; k < s->nshot; k++) t0[k] = s->shot_t0[k];
+}
+
+void sf2d_sim_shot_misfits(void* sim, double* chi) {
+  Sim* s = (Sim*)sim;
+  for (int k = 0; k < s->nshot && k < (int)s->shot_chi.size(); k++) chi[k] = s->shot_chi[k];
+}
+
+void sf2d_sim_batch_get_fields(void* sim, int shot, float* displ, float* veloc, float* accel) {
+  Sim* s = (Sim*)sim;
+  sf2d_batch_get_fields(&s->mesh, shot, displ, veloc, accel);
+}
+
 int sf2d_sim_adjoint(void* sim, const float* adj, const float* absorb, const float* last_displ,
                      const float* last_veloc, const float* last_accel, double* seis, float* kernels) {
   Sim* s = (Sim*)sim;
@@ -913,28 +1028,43 @@ int sf2d_sim_set_model(void* sim, const double* rho, const double* vp, const dou
 int sf2d_sim_gradient_forward(void* sim, const double* obs, int chunk, double* chi, double* seis) {
   Sim* s = (Sim*)sim;
   if (!s || !s->mesh || s->simulation_type != 1 || chunk < 1) return 1;
-  const int N = s->p.nstep;
-  sf2d_grad_prepare(&s->mesh, chunk, s->nrec);
+  const int N = s->p.nstep, R = s->nrec, S = std::max(1, s->nshot);
   s->grad_chunk = std::min(chunk, N);
-  sf2d_sim_reset(sim);
-  std::vector<double> sm((size_t)std::max(1, s->nrec) * 2 * N, 0.0);
+  std::vector<double> sm((size_t)S * std::max(1, R) * 2 * N, 0.0);
   s->grad_save = 1;
-  const int rc = sf2d_sim_forward(sim, N, sm.data(), nullptr, nullptr);
-  s->grad_save = 0;
-  if (rc != 0) return rc;
-  // chi = 1/2 dt sum |s - o|^2 in double; residuals dt (s - o) for the reverse sweep
+  if (s->nshot > 0) {
+    sf2d_batch_grad_prepare(&s->mesh, chunk);
+    sf2d_batch_reset(&s->mesh);
+    batch_steps(s, 1, N, true);
+    s->grad_save = 0;
+    batch_seismograms(s, sm.data());
+  } else {
+    sf2d_grad_prepare(&s->mesh, chunk, R);
+    sf2d_sim_reset(sim);
+    const int rc = sf2d_sim_forward(sim, N, sm.data(), nullptr, nullptr);
+    s->grad_save = 0;
+    if (rc != 0) return rc;
+  }
+  // chi = 1/2 dt sum |s - o|^2 in double (per shot, then summed); residuals dt (s - o), [N][S][R][2]
   const double dt = (double)(float)s->p.dt;
-  double acc = 0.0;
-  s->grad_res.assign((size_t)std::max(1, s->nrec) * 2 * N, 0.f);
-  for (int r = 0; r < s->nrec; r++)
-    for (int c = 0; c < 2; c++)
-      for (int n = 0; n < N; n++) {
-        const double q = sm[((size_t)r * 2 + c) * N + n] - obs[((size_t)r * 2 + c) * N + n];
-        acc += 0.5 * dt * q * q;
-        s->grad_res[((size_t)n * s->nrec + r) * 2 + c] = (float)(dt * q);
-      }
-  *chi = acc;
-  if (seis) std::copy(sm.begin(), sm.begin() + (size_t)s->nrec * 2 * N, seis);
+  s->grad_res.assign((size_t)S * std::max(1, R) * 2 * N, 0.f);
+  s->shot_chi.assign(S, 0.0);
+  double total = 0.0;
+  for (int sh = 0; sh < S; sh++) {
+    double acc = 0.0;
+    for (int r = 0; r < R; r++)
+      for (int c = 0; c < 2; c++)
+        for (int n = 0; n < N; n++) {
+          const size_t at = (((size_t)sh * R + r) * 2 + c) * N + n;
+          const double q = sm[at] - obs[at];
+          acc += 0.5 * dt * q * q;
+          s->grad_res[(((size_t)n * S + sh) * R + r) * 2 + c] = (float)(dt * q);
+        }
+    s->shot_chi[sh] = acc;
+    total += acc;
+  }
+  *chi = total;
+  if (seis) std::copy(sm.begin(), sm.begin() + (size_t)S * R * 2 * N, seis);
   s->grad_ready = 1;
   return 0;
 }
@@ -949,16 +1079,24 @@ int sf2d_sim_gradient_backward(void* sim, double* grad, double* parts) {
   for (int k = nseg - 1; k >= 0; k--) {
     const int n0 = k * C, n1 = std::min(N, n0 + C);
     if (k == 0) {
-      sf2d_sim_reset(sim);
+      if (s->nshot > 0)
+        sf2d_batch_reset(&s->mesh);
+      else
+        sf2d_sim_reset(sim);
     } else {
       sf2d_grad_restore_checkpoint(&s->mesh, k - 1, dto2);
-      s->it_done = n0;
     }
+    s->it_done = n0;
     s->grad_save = 2;
     s->grad_seg_base = n0;
-    const int rc = sf2d_sim_forward(sim, n1 - n0, nullptr, nullptr, nullptr);
-    s->grad_save = 0;
-    if (rc != 0) return rc;
+    if (s->nshot > 0) {
+      batch_steps(s, n0 + 1, n1, false);
+      s->grad_save = 0;
+    } else {
+      const int rc = sf2d_sim_forward(sim, n1 - n0, nullptr, nullptr, nullptr);
+      s->grad_save = 0;
+      if (rc != 0) return rc;
+    }
     for (int n = n1; n > n0; n--) sf2d_grad_reverse_step(&s->mesh, n, n - n0 - 1, n == N ? 1 : 0, dt);
     sf2d_grad_flush(&s->mesh);
   }

@@ -33,7 +33,11 @@ __device__ __forceinline__ void atomic_add_f32(float* p, float v) { unsafeAtomic
 // ---------------------------------------------------------------------------
 // Newmark predictor  (src/gpu/kernels/UpdateDispVeloc_kernel.cu:37-66; update_displacement_Newmark.F90:408-454)
 __global__ __launch_bounds__(256) void k_predictor(float2* __restrict__ d, float2* __restrict__ v,
-                                                   float2* __restrict__ a, int n, float dt, float dt2o2, float dto2) {
+                                                   float2* __restrict__ a, int n, float dt, float dt2o2, float dto2,
+                                                   long shot_stride) {
+  d += blockIdx.y * shot_stride;
+  v += blockIdx.y * shot_stride;
+  a += blockIdx.y * shot_stride;
   int stride = gridDim.x * blockDim.x;
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
     float2 dd = d[k], vv = v[k], aa = a[k];
@@ -50,7 +54,9 @@ __global__ __launch_bounds__(256) void k_predictor(float2* __restrict__ d, float
 // Newmark corrector with the inverse mass matrix (src/gpu/kernels/kernel_3_cuda_device.cu:37-65)
 __global__ __launch_bounds__(256) void k_corrector(float2* __restrict__ v, float2* __restrict__ a,
                                                    const float* __restrict__ rmx, const float* __restrict__ rmz,
-                                                   int n, float dto2, int apply_mass) {
+                                                   int n, float dto2, int apply_mass, long shot_stride) {
+  v += blockIdx.y * shot_stride;
+  a += blockIdx.y * shot_stride;
   int stride = gridDim.x * blockDim.x;
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
     float2 aa = a[k], vv = v[k];
@@ -70,7 +76,10 @@ __global__ __launch_bounds__(256) void k_corrector(float2* __restrict__ v, float
 __global__ __launch_bounds__(256) void k_corrector_predictor(float2* __restrict__ d, float2* __restrict__ v,
                                                              float2* __restrict__ a, const float* __restrict__ rmx,
                                                              const float* __restrict__ rmz, int n, float dto2_prev,
-                                                             float dt, float dt2o2, float dto2) {
+                                                             float dt, float dt2o2, float dto2, long shot_stride) {
+  d += blockIdx.y * shot_stride;
+  v += blockIdx.y * shot_stride;
+  a += blockIdx.y * shot_stride;
   int stride = gridDim.x * blockDim.x;
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
     float2 dd = d[k], vv = v[k], aa = a[k];
@@ -226,18 +235,22 @@ __device__ __forceinline__ int patch_nhalo(int c) { return (c >> 20) & 0xfff; }
 constexpr int ROWF2 = 6;
 
 __global__ __launch_bounds__(FORCE_THREADS) void k_elastic_forces_patch(
-    int first_patch, int npatch, const int4* __restrict__ desc, const int* __restrict__ halo,
-    const uint16_t* __restrict__ elem_slot, const float4* __restrict__ geom, const float4* __restrict__ geom_elem,
-    const float2* __restrict__ moduli, const float2* __restrict__ displ, float2* __restrict__ accel,
-    const Basis basis) {
-  // Workgroup w takes patches w, w + W, w + 2W, ... (W = gridDim.x): at any time the grid works on one window of
-  // consecutive patches, so the element arrays stream through DRAM in order.  Consecutive w run on one XCD
-  // (blocks are dispatched round-robin over the XCDs; gridDim.x is a multiple of NXCD).
+    int first_patch, int npatch, int nshot, long shot_stride, const int4* __restrict__ desc,
+    const int* __restrict__ halo, const uint16_t* __restrict__ elem_slot, const float4* __restrict__ geom,
+    const float4* __restrict__ geom_elem, const float2* __restrict__ moduli, const float2* __restrict__ displ,
+    float2* __restrict__ accel, const Basis basis) {
+  // Work items are (patch, shot) pairs, item = patch * nshot + shot: the shots of one patch are consecutive
+  // items, so their element data is fetched from HBM once and then hit in L2.  Shot s's fields start
+  // s * shot_stride points into displ and accel.  Workgroup w takes items w, w + W, w + 2W, ... (W = gridDim.x):
+  // at any time the grid works on one window of consecutive items, so the element arrays stream through DRAM
+  // in order.  Consecutive w run on one XCD (blocks are dispatched round-robin over the XCDs; gridDim.x is a
+  // multiple of NXCD).
   const int bid = blockIdx.x;
   const int nw = gridDim.x;
   const int w = (bid % NXCD) * (nw / NXCD) + bid / NXCD;
-  if (w >= npatch) return;
-  const int qlast = w + (npatch - 1 - w) / nw * nw;  // last patch of this workgroup
+  const int nitem = npatch * nshot;
+  if (w >= nitem) return;
+  const int qlast = w + (nitem - 1 - w) / nw * nw;  // last item of this workgroup
 
   __shared__ float s_hp[NGLL2], s_hpw[NGLL2], s_w[NGLL];
   __shared__ float2 s_u[FORCE_THREADS];                          // displacement of the patch's points
@@ -271,23 +284,25 @@ __global__ __launch_bounds__(FORCE_THREADS) void k_elastic_forces_patch(
     const int h = min(max(tid - patch_nown(d.w), 0), max(patch_nhalo(d.w) - 1, 0));
     return ld_off(halo, 4u * (unsigned)(d.z + h));
   };
-  auto load_points = [&](const int4 d, int halo_id, PatchRegs& r) {
+  auto item_desc = [&](int item) { return desc[first_patch + item / nshot]; };
+  auto load_points = [&](const int4 d, int halo_id, PatchRegs& r, int item) {
+    const long base = (long)(item % nshot) * shot_stride;  // uniform
     r.g = tid < patch_nown(d.w) ? d.y + tid : halo_id;
-    r.u = ld_off(displ, 8u * (unsigned)r.g);
-    r.a = ld_off((const float2*)accel, 8u * (unsigned)r.g);
+    r.u = ld_off(displ + base, 8u * (unsigned)r.g);
+    r.a = ld_off((const float2*)accel + base, 8u * (unsigned)r.g);
   };
 
   int q = w;
   // three register sets rotate through (compute, loaded, loading): no copies, so each compute step waits only
   // for its own loads, issued two steps earlier
   PatchRegs ra, rb, rc;
-  int4 da = desc[first_patch + q];
+  int4 da = item_desc(q);
   load_elements(da, ra);
-  load_points(da, load_halo_id(da), ra);
-  int4 db = desc[first_patch + min(q + nw, qlast)];
+  load_points(da, load_halo_id(da), ra, q);
+  int4 db = item_desc(min(q + nw, qlast));
   load_elements(db, rb);
-  load_points(db, load_halo_id(db), rb);
-  int4 dc = desc[first_patch + min(q + 2 * nw, qlast)];
+  load_points(db, load_halo_id(db), rb, min(q + nw, qlast));
+  int4 dc = item_desc(min(q + 2 * nw, qlast));
   int hc = load_halo_id(dc);
   int ha = 0, hb = 0;
   __syncthreads();
@@ -360,15 +375,16 @@ __global__ __launch_bounds__(FORCE_THREADS) void k_elastic_forces_patch(
       float2 a = C.a;
       a.x += ((f01.x + f01.z) + f23.x) + f23.z;
       a.y += ((f01.y + f01.w) + f23.y) + f23.w;
-      *reinterpret_cast<float2*>(reinterpret_cast<char*>(accel) + 8u * (unsigned)C.g) = a;
+      *reinterpret_cast<float2*>(reinterpret_cast<char*>(accel + (long)(q % nshot) * shot_stride) +
+                                 8u * (unsigned)C.g) = a;
     }
   };
 #define SF2D_PATCH_STEP(RC, DCUR, RL, DL, HL, DNEXT, HNEXT)                 \
   {                                                                         \
-    const int4 dn_ = desc[first_patch + min(q + 3 * nw, qlast)];            \
+    const int4 dn_ = item_desc(min(q + 3 * nw, qlast));                     \
     const int hn_ = load_halo_id(dn_);                                      \
     load_elements(DL, RL);                                                  \
-    load_points(DL, HL, RL);                                                \
+    load_points(DL, HL, RL, min(q + 2 * nw, qlast));                        \
     compute(RC, DCUR);                                                      \
     DNEXT = dn_;                                                            \
     HNEXT = hn_;                                                            \
@@ -457,7 +473,10 @@ __global__ __launch_bounds__(64) void k_stacey(int nabs, const int* __restrict__
                                                const int* __restrict__ is_elastic, const float* __restrict__ rho_vp,
                                                const float* __restrict__ rho_vs, const float2* __restrict__ veloc,
                                                float* __restrict__ accel, int write_store, float* store_bottom,
-                                               float* store_right, float* store_top, float* store_left) {
+                                               float* store_right, float* store_top, float* store_left,
+                                               long shot_stride) {
+  veloc += blockIdx.y * shot_stride;
+  accel += 2 * blockIdx.y * shot_stride;
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   const int f = idx / NGLL;
   const int k = idx - f * NGLL;
@@ -522,10 +541,12 @@ __global__ __launch_bounds__(64) void k_stacey_backward(int nabs, const int* __r
 __global__ __launch_bounds__(64) void k_sources(int nsrc, int it, const int* __restrict__ ispec_src,
                                                 const int* __restrict__ is_elastic, const int* __restrict__ ibool,
                                                 const float* __restrict__ sourcearrays, const float* __restrict__ stf,
-                                                float* __restrict__ accel) {
+                                                float* __restrict__ accel, const int* __restrict__ src_shot,
+                                                long shot_stride) {
   const int s = blockIdx.x;
   const int p = threadIdx.x;
   if (s >= nsrc || p >= NGLL2) return;
+  if (src_shot) accel += 2 * (long)src_shot[s] * shot_stride;  // each source drives its own shot's field
   const int ispec = ispec_src[s];
   if (!is_elastic[ispec]) return;
   const int g = ibool[(size_t)ispec * NGLL2 + p];
@@ -561,7 +582,9 @@ __global__ __launch_bounds__(64) void k_seismograms(int nrec, int nlength, int c
                                                     const int* __restrict__ ibool, const float* __restrict__ xir,
                                                     const float* __restrict__ gammar, const float* __restrict__ cosrot,
                                                     const float* __restrict__ sinrot, const float2* __restrict__ field,
-                                                    float* __restrict__ dst) {
+                                                    float* __restrict__ dst, long shot_stride, long dst_stride) {
+  field += blockIdx.y * shot_stride;
+  dst += blockIdx.y * dst_stride;
   const int r = blockIdx.x;
   const int p = threadIdx.x;
   if (r >= nrec) return;
@@ -680,27 +703,35 @@ inline int blocks_for(long n, int threads, int cap = 1 << 20) {
 
 // ---------------------------------------------------------------------------
 // launchers
+// Newmark, Stacey and seismogram launchers take nshot wavefields of shot_stride points each (shot-major),
+// with the shot in blockIdx.y; nshot = 1 is the single-wavefield case of the reference entry points.
+static dim3 shot_grid(long n, int threads, int cap, int nshot) {
+  if (nshot < 1 || nshot > 65535) fail("shot count %d outside [1, 65535]", nshot);
+  return dim3(blocks_for(n, threads, cap), nshot);
+}
+
 void launch_newmark_predictor(float* d, float* v, float* a, int nglob, float dt, float dt2o2, float dto2,
-                              hipStream_t s) {
+                              hipStream_t s, int nshot, long shot_stride) {
   if (nglob <= 0) return;
-  hipLaunchKernelGGL(k_predictor, dim3(blocks_for(nglob, 256, 65536)), dim3(256), 0, s, (float2*)d, (float2*)v,
-                     (float2*)a, nglob, dt, dt2o2, dto2);
+  hipLaunchKernelGGL(k_predictor, shot_grid(nglob, 256, 65536, nshot), dim3(256), 0, s, (float2*)d, (float2*)v,
+                     (float2*)a, nglob, dt, dt2o2, dto2, shot_stride);
   check_hip(hipGetLastError(), "k_predictor launch");
 }
 
 void launch_newmark_corrector(float* v, float* a, const float* rmx, const float* rmz, int nglob, float dto2,
-                              int apply_mass, hipStream_t s) {
+                              int apply_mass, hipStream_t s, int nshot, long shot_stride) {
   if (nglob <= 0) return;
-  hipLaunchKernelGGL(k_corrector, dim3(blocks_for(nglob, 256, 65536)), dim3(256), 0, s, (float2*)v, (float2*)a, rmx,
-                     rmz, nglob, dto2, apply_mass);
+  hipLaunchKernelGGL(k_corrector, shot_grid(nglob, 256, 65536, nshot), dim3(256), 0, s, (float2*)v, (float2*)a, rmx,
+                     rmz, nglob, dto2, apply_mass, shot_stride);
   check_hip(hipGetLastError(), "k_corrector launch");
 }
 
 void launch_newmark_corrector_predictor(float* d, float* v, float* a, const float* rmx, const float* rmz, int nglob,
-                                       float dto2_prev, float dt, float dt2o2, float dto2, hipStream_t s) {
+                                       float dto2_prev, float dt, float dt2o2, float dto2, hipStream_t s,
+                                       int nshot, long shot_stride) {
   if (nglob <= 0) return;
-  hipLaunchKernelGGL(k_corrector_predictor, dim3(blocks_for(nglob, 256, 65536)), dim3(256), 0, s, (float2*)d,
-                     (float2*)v, (float2*)a, rmx, rmz, nglob, dto2_prev, dt, dt2o2, dto2);
+  hipLaunchKernelGGL(k_corrector_predictor, shot_grid(nglob, 256, 65536, nshot), dim3(256), 0, s, (float2*)d,
+                     (float2*)v, (float2*)a, rmx, rmz, nglob, dto2_prev, dt, dt2o2, dto2, shot_stride);
   check_hip(hipGetLastError(), "k_corrector_predictor launch");
 }
 
@@ -721,8 +752,11 @@ void launch_elastic_forces(const Mesh* mp, int first, int nelem, int atomic, con
   check_hip(hipGetLastError(), "k_elastic_forces launch");
 }
 
-void launch_elastic_forces_patch(const Mesh* mp, int first_patch, int npatch, const float* displ, float* accel) {
+void launch_elastic_forces_patch(const Mesh* mp, int first_patch, int npatch, const float* displ, float* accel,
+                                 int nshot, long shot_stride) {
   if (npatch <= 0) return;
+  if (nshot < 1 || (nshot > 1 && shot_stride < mp->nglob) || (long)npatch * nshot > (1L << 30))
+    fail("batched force launch: %d shots of stride %ld over %d patches", nshot, shot_stride, npatch);
   if (first_patch < 0 || first_patch + npatch > mp->npatch)
     fail("patch range [%d, %d) outside the %d patches", first_patch, first_patch + npatch, mp->npatch);
   // persistent grid: as many workgroups as stay resident on the device (a multiple of the XCD count);
@@ -738,10 +772,10 @@ void launch_elastic_forces_patch(const Mesh* mp, int first_patch, int npatch, co
     resident = per_cu <= 0 ? 0 : std::max(NXCD, (cus * per_cu) / NXCD * NXCD);
     if (getenv("SF2D_VERBOSE")) fprintf(stderr, "k_elastic_forces_patch: %d CUs, %d workgroups resident\n", cus, resident);
   }
-  const int all = (npatch + NXCD - 1) / NXCD * NXCD;
+  const int all = (npatch * nshot + NXCD - 1) / NXCD * NXCD;
   const int nwg = resident > 0 ? std::min(resident, all) : all;
   hipLaunchKernelGGL(k_elastic_forces_patch, dim3(nwg), dim3(FORCE_THREADS), 0, mp->stream, first_patch, npatch,
-                     (const int4*)mp->d_patch_desc, mp->d_halo_nodes, mp->d_elem_slot, mp->d_geom,
+                     nshot, shot_stride, (const int4*)mp->d_patch_desc, mp->d_halo_nodes, mp->d_elem_slot, mp->d_geom,
                      mp->d_geom_elem, mp->d_moduli, (const float2*)displ, (float2*)accel, mp->basis);
   check_hip(hipGetLastError(), "k_elastic_forces_patch launch");
 }
@@ -782,12 +816,14 @@ void launch_permute_elements_int(const int* src, int* dst, const int* ispec_of_p
   check_hip(hipGetLastError(), "k_permute launch");
 }
 
-void launch_stacey(const Mesh* mp, const float* veloc, float* accel, int write_store) {
+void launch_stacey(const Mesh* mp, const float* veloc, float* accel, int write_store, int nshot, long shot_stride) {
   if (mp->num_abs <= 0) return;
-  hipLaunchKernelGGL(k_stacey, dim3(blocks_for((long)mp->num_abs * NGLL, 64)), dim3(64), 0, mp->stream, mp->num_abs,
-                     mp->d_abs_ispec, mp->d_abs_ij, mp->d_abs_normal, mp->d_abs_jacw, mp->d_edge_abs, mp->d_ib,
-                     mp->d_ibool, mp->d_ispec_is_elastic, mp->d_rho_vp, mp->d_rho_vs, (const float2*)veloc, accel,
-                     write_store, mp->d_b_absorb[1], mp->d_b_absorb[2], mp->d_b_absorb[3], mp->d_b_absorb[4]);
+  if (nshot > 1 && write_store) fail("the Stacey boundary store holds one wavefield");
+  hipLaunchKernelGGL(k_stacey, shot_grid((long)mp->num_abs * NGLL, 64, 1 << 20, nshot), dim3(64), 0, mp->stream,
+                     mp->num_abs, mp->d_abs_ispec, mp->d_abs_ij, mp->d_abs_normal, mp->d_abs_jacw, mp->d_edge_abs,
+                     mp->d_ib, mp->d_ibool, mp->d_ispec_is_elastic, mp->d_rho_vp, mp->d_rho_vs, (const float2*)veloc,
+                     accel, write_store, mp->d_b_absorb[1], mp->d_b_absorb[2], mp->d_b_absorb[3], mp->d_b_absorb[4],
+                     shot_stride);
   check_hip(hipGetLastError(), "k_stacey launch");
 }
 
@@ -800,11 +836,11 @@ void launch_stacey_backward(const Mesh* mp, float* b_accel) {
   check_hip(hipGetLastError(), "k_stacey_backward launch");
 }
 
-void launch_sources(const Mesh* mp, float* accel, int it) {
+void launch_sources(const Mesh* mp, float* accel, int it, const int* src_shot, long shot_stride) {
   if (mp->nsources_local <= 0) return;
   hipLaunchKernelGGL(k_sources, dim3(mp->nsources_local), dim3(64), 0, mp->stream, mp->nsources_local, it,
                      mp->d_ispec_selected_source, mp->d_ispec_is_elastic, mp->d_ibool, mp->d_sourcearrays, mp->d_stf,
-                     accel);
+                     accel, src_shot, shot_stride);
   check_hip(hipGetLastError(), "k_sources launch");
 }
 
@@ -816,11 +852,12 @@ void launch_adjoint_sources(const Mesh* mp, float* accel, int it) {
   check_hip(hipGetLastError(), "k_adjoint_sources launch");
 }
 
-void launch_seismograms(const Mesh* mp, const float* field, float* dst, int seismo_current) {
+void launch_seismograms(const Mesh* mp, const float* field, float* dst, int seismo_current, int nshot,
+                        long shot_stride, long dst_stride) {
   if (mp->nrec_local <= 0) return;
-  hipLaunchKernelGGL(k_seismograms, dim3(mp->nrec_local), dim3(64), 0, mp->stream, mp->nrec_local,
+  hipLaunchKernelGGL(k_seismograms, dim3(mp->nrec_local, nshot), dim3(64), 0, mp->stream, mp->nrec_local,
                      mp->nlength_seismogram, seismo_current, mp->d_ispec_selected_rec, mp->d_ibool, mp->d_xir,
-                     mp->d_gammar, mp->d_cosrot, mp->d_sinrot, (const float2*)field, dst);
+                     mp->d_gammar, mp->d_cosrot, mp->d_sinrot, (const float2*)field, dst, shot_stride, dst_stride);
   check_hip(hipGetLastError(), "k_seismograms launch");
 }
 

@@ -18,6 +18,10 @@
 //     V = V' + dt D;  A = dt/2 V' + dt^2/2 D                                 (folded into the next k_adj_pointwise)
 // K and B are symmetric, so their transposes are the forward kernels themselves.  Gradients accumulate in FP32
 // over one checkpoint segment and are added into FP64 totals at its end.
+//
+// A shot batch (batch.hip) runs the same sweep for all its shots at once: states, adjoint fields and residuals
+// carry a shot dimension, while the gradient accumulators hold the sum over shots; the kernels that touch the
+// accumulators loop over the shots, so each accumulator is read and written once per step for the batch.
 #include <algorithm>
 
 #include "sf2d_mesh.h"
@@ -26,14 +30,18 @@
 namespace sf2d {
 
 struct GradState {
-  int nstep = 0, chunk = 0, nckpt = 0, nrec = 0;
+  int nstep = 0, chunk = 0, nckpt = 0, nrec = 0, nshot = 1, batch = 0;
   size_t n2 = 0, ne = 0, nf = 0;
-  float* ckpt = nullptr;  // [nckpt][3][n2]: (d, v', r) after steps chunk, 2 chunk, ...
-  float* seg = nullptr;   // [chunk][3][n2]: states of the segment being reversed
-  float *D = nullptr, *Vb = nullptr, *Rb = nullptr;
+  long stride = 0;        // points between shots
+  size_t sstr = 0;        // floats between shots (2 stride)
+  size_t nfield = 0;      // floats of one field over all shots (nshot sstr)
+  float *fd = nullptr, *fv = nullptr, *fa = nullptr;  // the forward fields that states are copied from
+  float* ckpt = nullptr;  // [nckpt][3][nfield]: (d, v', r) after steps chunk, 2 chunk, ...
+  float* seg = nullptr;   // [chunk][3][nfield]: states of the segment being reversed
+  float *D = nullptr, *Vb = nullptr, *Rb = nullptr;  // [nfield]
   float *gk32 = nullptr, *gm32 = nullptr, *pm32 = nullptr, *gvp32 = nullptr, *gvs32 = nullptr;
   double *gk = nullptr, *gm = nullptr, *pm = nullptr, *gvp = nullptr, *gvs = nullptr;
-  float* res = nullptr;   // [nstep][nrec][2] = dt (s - o)
+  float* res = nullptr;   // [nstep][nshot][nrec][2] = dt (s - o)
 };
 
 namespace {
@@ -41,34 +49,43 @@ namespace {
 constexpr int GT = 256;   // threads per workgroup
 constexpr int GEPB = 10;  // elements per workgroup in k_grad_elastic
 
-__global__ __launch_bounds__(GT) void k_adj_pointwise(int first, int n, float dt, float dto2, float h,
-                                                      const float* __restrict__ D, float* __restrict__ Vb,
+__global__ __launch_bounds__(GT) void k_adj_pointwise(int first, int n, int nshot, size_t sstr, float dt, float dto2,
+                                                      float h, const float* __restrict__ D, float* __restrict__ Vb,
                                                       float* __restrict__ Rb, const float* __restrict__ rmx,
                                                       const float* __restrict__ rmz, const float* __restrict__ r,
                                                       float* __restrict__ pm) {
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
     const int g = k >> 1;
     const float w = (k & 1) ? rmz[g] : rmx[g];
-    float v = 0.f, a = 0.f;
-    if (!first) {  // state cotangents of the step after this one
-      const float vb = Vb[k], db = D[k];
-      v = vb + dt * db;
-      a = dto2 * vb + h * db;
+    float dm = 0.f;
+    for (int s = 0; s < nshot; s++) {
+      const size_t q = s * sstr + k;
+      float v = 0.f, a = 0.f;
+      if (!first) {  // state cotangents of the step after this one
+        const float vb = Vb[q], db = D[q];
+        v = vb + dt * db;
+        a = dto2 * vb + h * db;
+      }
+      a = a + dto2 * v;
+      Vb[q] = v;
+      Rb[q] = w * a;
+      dm += a * r[q] * w * w;
     }
-    a = a + dto2 * v;
-    Vb[k] = v;
-    Rb[k] = w * a;
-    pm[k] -= a * r[k] * w * w;
+    pm[k] -= dm;
   }
 }
 
+// residual injection; shot blockIdx.y (residuals [nshot][nrec][2] of this step, D [nshot][sstr])
 __global__ __launch_bounds__(64) void k_adj_receivers(int nrec, const float* __restrict__ res, const int* __restrict__ ispec_rec,
                                                       const int* __restrict__ ibool, const float* __restrict__ xir,
                                                       const float* __restrict__ gammar, const float* __restrict__ cosrot,
-                                                      const float* __restrict__ sinrot, float* __restrict__ D) {
+                                                      const float* __restrict__ sinrot, float* __restrict__ D,
+                                                      size_t sstr) {
   const int r = blockIdx.x;
   const int p = threadIdx.x;
   if (r >= nrec || p >= NGLL2) return;
+  res += (size_t)blockIdx.y * 2 * nrec;
+  D += blockIdx.y * sstr;
   const int i = p % NGLL, j = p / NGLL;
   const float px = res[2 * r], pz = res[2 * r + 1];
   const float c = cosrot[r], sn = sinrot[r];
@@ -79,8 +96,9 @@ __global__ __launch_bounds__(64) void k_adj_receivers(int nrec, const float* __r
   atomicAdd(&D[2 * g + 1], hl * cz);
 }
 
-// strain products of the forward displacement d_n and the adjoint force R~ at every element point
-__global__ __launch_bounds__(GT) void k_grad_elastic(int nelem, const int* __restrict__ ibool,
+// strain products of the forward displacement d_n and the adjoint force R~ at every element point, summed over
+// the shots (fields sstr floats apart)
+__global__ __launch_bounds__(GT) void k_grad_elastic(int nelem, int nshot, size_t sstr, const int* __restrict__ ibool,
                                                      const float* __restrict__ xix, const float* __restrict__ xiz,
                                                      const float* __restrict__ gammax, const float* __restrict__ gammaz,
                                                      const float2* __restrict__ dn, const float2* __restrict__ rb,
@@ -93,33 +111,50 @@ __global__ __launch_bounds__(GT) void k_grad_elastic(int nelem, const int* __res
   const int e = blockIdx.x * GEPB + le;
   const bool active = le < GEPB && e < nelem;
   const size_t l = (size_t)e * NGLL2 + p;
+  const int g = active ? ibool[l] : 0;
+  float X = 0.f, Z = 0.f, G = 0.f, H = 0.f;
   if (active) {
-    const int g = ibool[l];
-    s_d[tid] = dn[g];
-    s_b[tid] = rb[g];
+    X = xix[l];
+    Z = xiz[l];
+    G = gammax[l];
+    H = gammaz[l];
   }
-  __syncthreads();
-  if (!active) return;
   const int base = le * NGLL2;
-  float a[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  const size_t pstr = sstr / 2;  // points between shots
+  float sk = 0.f, sm = 0.f;
+  for (int s = 0; s < nshot; s++) {
+    if (s > 0) __syncthreads();  // the previous shot's LDS reads are done
+    if (active) {
+      s_d[tid] = dn[s * pstr + g];
+      s_b[tid] = rb[s * pstr + g];
+    }
+    __syncthreads();
+    if (active) {
+      float a[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-  for (int k = 0; k < NGLL; k++) {
-    const float hi = s_hp[i * NGLL + k], hj = s_hp[j * NGLL + k];
-    const float2 dr = s_d[base + j * NGLL + k], dc = s_d[base + k * NGLL + i];
-    const float2 br = s_b[base + j * NGLL + k], bc = s_b[base + k * NGLL + i];
-    a[0] += dr.x * hi; a[1] += dr.y * hi; a[2] += dc.x * hj; a[3] += dc.y * hj;
-    a[4] += br.x * hi; a[5] += br.y * hi; a[6] += bc.x * hj; a[7] += bc.y * hj;
+      for (int k = 0; k < NGLL; k++) {
+        const float hi = s_hp[i * NGLL + k], hj = s_hp[j * NGLL + k];
+        const float2 dr = s_d[base + j * NGLL + k], dc = s_d[base + k * NGLL + i];
+        const float2 br = s_b[base + j * NGLL + k], bc = s_b[base + k * NGLL + i];
+        a[0] += dr.x * hi; a[1] += dr.y * hi; a[2] += dc.x * hj; a[3] += dc.y * hj;
+        a[4] += br.x * hi; a[5] += br.y * hi; a[6] += bc.x * hj; a[7] += bc.y * hj;
+      }
+      const float ux = a[0] * X + a[2] * G, uxz = a[0] * Z + a[2] * H, uzx = a[1] * X + a[3] * G, uz = a[1] * Z + a[3] * H;
+      const float wx = a[4] * X + a[6] * G, wxz = a[4] * Z + a[6] * H, wzx = a[5] * X + a[7] * G, wz = a[5] * Z + a[7] * H;
+      const float wj = basis.w[i] * basis.w[j] / (X * H - G * Z);
+      sk += wj * (ux + uz) * (wx + wz);
+      sm += wj * (ux * wx + uz * wz - ux * wz - uz * wx + (uxz + uzx) * (wxz + wzx));
+    }
   }
-  const float X = xix[l], Z = xiz[l], G = gammax[l], H = gammaz[l];
-  const float ux = a[0] * X + a[2] * G, uxz = a[0] * Z + a[2] * H, uzx = a[1] * X + a[3] * G, uz = a[1] * Z + a[3] * H;
-  const float wx = a[4] * X + a[6] * G, wxz = a[4] * Z + a[6] * H, wzx = a[5] * X + a[7] * G, wz = a[5] * Z + a[7] * H;
-  const float wj = basis.w[i] * basis.w[j] / (X * H - G * Z);
-  gk[l] -= wj * (ux + uz) * (wx + wz);
-  gm[l] -= wj * (ux * wx + uz * wz - ux * wz - uz * wx + (uxz + uzx) * (wxz + wzx));
+  if (!active) return;
+  gk[l] -= sk;
+  gm[l] -= sm;
 }
 
-// Stacey traction's dependence on rho vp and rho vs, per absorbing face point (same order as abs_ij)
-__global__ __launch_bounds__(64) void k_grad_stacey(int nabs, const int* __restrict__ abs_ispec, const int* __restrict__ abs_ij,
+// Stacey traction's dependence on rho vp and rho vs, per absorbing face point (same order as abs_ij), summed over
+// the shots (fields pstr points apart)
+__global__ __launch_bounds__(64) void k_grad_stacey(int nabs, int nshot, size_t pstr, const int* __restrict__ abs_ispec,
+                                                    const int* __restrict__ abs_ij,
                                                     const float* __restrict__ normal, const float* __restrict__ jacw,
                                                     const int* __restrict__ ibool, const float2* __restrict__ vpn,
                                                     const float2* __restrict__ rb, float* __restrict__ gvp,
@@ -131,10 +166,15 @@ __global__ __launch_bounds__(64) void k_grad_stacey(int nabs, const int* __restr
   if (ii < 0 || jj < 0) return;  // corner point owned by the other edge
   const int g = ibool[(size_t)abs_ispec[f] * NGLL2 + jj * NGLL + ii];
   const float nx = normal[2 * q], nz = normal[2 * q + 1], w = jacw[q];
-  const float2 v = vpn[g], b = rb[g];
-  const float vn = nx * v.x + nz * v.y, bn = nx * b.x + nz * b.y;
-  gvp[q] -= w * vn * bn;
-  gvs[q] -= w * (v.x * b.x + v.y * b.y - vn * bn);
+  float sp = 0.f, ss = 0.f;
+  for (int s = 0; s < nshot; s++) {
+    const float2 v = vpn[s * pstr + g], b = rb[s * pstr + g];
+    const float vn = nx * v.x + nz * v.y, bn = nx * b.x + nz * b.y;
+    sp += w * vn * bn;
+    ss += w * (v.x * b.x + v.y * b.y - vn * bn);
+  }
+  gvp[q] -= sp;
+  gvs[q] -= ss;
 }
 
 __global__ __launch_bounds__(GT) void k_flush64(size_t n, float* __restrict__ a32, double* __restrict__ a64) {
@@ -155,18 +195,11 @@ void copy_d2d(float* dst, const float* src, size_t n, hipStream_t s) {
   check_hip(hipMemcpyAsync(dst, src, n * sizeof(float), hipMemcpyDeviceToDevice, s), "gradient state copy");
 }
 
-}  // namespace
-}  // namespace sf2d
-
-using namespace sf2d;
-
-extern "C" {
-
-int sf2d_grad_prepare(long* Mesh_pointer, int chunk, int nrec) {
-  Mesh* mp = handle(Mesh_pointer);
-  if (!mp->sorted) fail("sf2d_grad_prepare: prepare_fields_elastic_device was not called");
-  if (chunk < 1) fail("sf2d_grad_prepare: chunk %d < 1", chunk);
-  if (nrec != mp->nrec_local) fail("sf2d_grad_prepare: %d receivers, the handle has %d", nrec, mp->nrec_local);
+// buffers for checkpoints every `chunk` steps, one segment of states and the accumulators, over the handle's
+// own forward fields (batch = 0) or over its shot batch (batch = 1); returns the number of checkpoints
+int grad_prepare(Mesh* mp, long* Mesh_pointer, int chunk, int nrec, int batch) {
+  if (chunk < 1) fail("gradient: chunk %d < 1", chunk);
+  if (nrec != mp->nrec_local) fail("gradient: %d receivers, the handle has %d", nrec, mp->nrec_local);
   if (mp->grad) sf2d_grad_release(Mesh_pointer);
   GradState* G = new GradState();
   G->nstep = mp->nstep;
@@ -176,11 +209,26 @@ int sf2d_grad_prepare(long* Mesh_pointer, int chunk, int nrec) {
   G->n2 = (size_t)2 * mp->nglob;
   G->ne = (size_t)mp->nspec * NGLL2;
   G->nf = (size_t)mp->num_abs * NGLL;
-  G->ckpt = device_zeros<float>(3 * G->n2 * std::max(1, G->nckpt));
-  G->seg = device_zeros<float>(3 * G->n2 * G->chunk);
-  G->D = device_zeros<float>(G->n2);
-  G->Vb = device_zeros<float>(G->n2);
-  G->Rb = device_zeros<float>(G->n2);
+  G->batch = batch;
+  if (batch) {
+    batch_fields(mp, &G->fd, &G->fv, &G->fa, &G->nshot, &G->stride);
+  } else {
+    G->fd = mp->d_displ;
+    G->fv = mp->d_veloc;
+    G->fa = mp->d_accel;
+    G->nshot = 1;
+    G->stride = mp->nglob;
+  }
+  G->sstr = (size_t)2 * G->stride;
+  G->nfield = G->sstr * G->nshot;
+  const size_t states = (size_t)std::max(1, G->nckpt) + G->chunk;
+  if (states * 3 * G->nfield * sizeof(float) > ((size_t)240 << 30))
+    fail("gradient: %zu states of %d shots exceed 240 GiB; use a longer chunk or fewer shots", states, G->nshot);
+  G->ckpt = device_zeros<float>(3 * G->nfield * std::max(1, G->nckpt));
+  G->seg = device_zeros<float>(3 * G->nfield * G->chunk);
+  G->D = device_zeros<float>(G->nfield);
+  G->Vb = device_zeros<float>(G->nfield);
+  G->Rb = device_zeros<float>(G->nfield);
   G->gk32 = device_zeros<float>(G->ne);
   G->gm32 = device_zeros<float>(G->ne);
   G->pm32 = device_zeros<float>(G->n2);
@@ -191,9 +239,29 @@ int sf2d_grad_prepare(long* Mesh_pointer, int chunk, int nrec) {
   G->pm = device_zeros<double>(G->n2);
   G->gvp = device_zeros<double>(std::max<size_t>(1, G->nf));
   G->gvs = device_zeros<double>(std::max<size_t>(1, G->nf));
-  G->res = device_zeros<float>((size_t)std::max(1, nrec) * 2 * mp->nstep);
+  G->res = device_zeros<float>((size_t)std::max(1, nrec) * 2 * G->nshot * mp->nstep);
   mp->grad = G;
+  mp->grad_batch = batch;
   return G->nckpt;
+}
+
+}  // namespace
+}  // namespace sf2d
+
+using namespace sf2d;
+
+extern "C" {
+
+int sf2d_grad_prepare(long* Mesh_pointer, int chunk, int nrec) {
+  Mesh* mp = handle(Mesh_pointer);
+  if (!mp->sorted) fail("sf2d_grad_prepare: prepare_fields_elastic_device was not called");
+  return grad_prepare(mp, Mesh_pointer, chunk, nrec, 0);
+}
+
+int sf2d_batch_grad_prepare(long* Mesh_pointer, int chunk) {
+  Mesh* mp = handle(Mesh_pointer);
+  if (!mp->batch) fail("sf2d_batch_grad_prepare: shot batch not prepared (sf2d_batch_prepare)");
+  return grad_prepare(mp, Mesh_pointer, chunk, mp->nrec_local, 1);
 }
 
 void sf2d_grad_release(long* Mesh_pointer) {
@@ -207,41 +275,48 @@ void sf2d_grad_release(long* Mesh_pointer) {
   for (double** p : {&G->gk, &G->gm, &G->pm, &G->gvp, &G->gvs}) device_free(*p);
   delete G;
   mp->grad = nullptr;
+  mp->grad_batch = 0;
 }
 
-// copies the device fields (d_n, v', r_n) after a forward step into checkpoint (kind 0) or segment (kind 1) slot
+// copies the forward fields (d_n, v', r_n) after a step into checkpoint (kind 0) or segment (kind 1) slot
 void sf2d_grad_save_state(long* Mesh_pointer, int kind, int slot) {
   Mesh* mp = handle(Mesh_pointer);
   GradState* G = need(mp);
   const int nslot = kind == 0 ? G->nckpt : G->chunk;
   if (slot < 0 || slot >= nslot) fail("sf2d_grad_save_state: slot %d outside 0..%d", slot, nslot - 1);
-  float* dst = (kind == 0 ? G->ckpt : G->seg) + (size_t)slot * 3 * G->n2;
-  copy_d2d(dst, mp->d_displ, G->n2, mp->stream);
-  copy_d2d(dst + G->n2, mp->d_veloc, G->n2, mp->stream);
-  copy_d2d(dst + 2 * G->n2, mp->d_accel, G->n2, mp->stream);
+  float* dst = (kind == 0 ? G->ckpt : G->seg) + (size_t)slot * 3 * G->nfield;
+  copy_d2d(dst, G->fd, G->nfield, mp->stream);
+  copy_d2d(dst + G->nfield, G->fv, G->nfield, mp->stream);
+  copy_d2d(dst + 2 * G->nfield, G->fa, G->nfield, mp->stream);
 }
 
-// restores checkpoint slot into the device fields, with the corrector of that step pending (dto2 = dt/2)
+// restores checkpoint slot into the forward fields, with the corrector of that step pending (dto2 = dt/2)
 void sf2d_grad_restore_checkpoint(long* Mesh_pointer, int slot, float dto2) {
   Mesh* mp = handle(Mesh_pointer);
   GradState* G = need(mp);
   if (slot < 0 || slot >= G->nckpt) fail("sf2d_grad_restore_checkpoint: slot %d outside 0..%d", slot, G->nckpt - 1);
-  const float* src = G->ckpt + (size_t)slot * 3 * G->n2;
-  copy_d2d(mp->d_displ, src, G->n2, mp->stream);
-  copy_d2d(mp->d_veloc, src + G->n2, G->n2, mp->stream);
-  copy_d2d(mp->d_accel, src + 2 * G->n2, G->n2, mp->stream);
-  mp->pending_corrector[0] = 1;
-  mp->pending_dto2[0] = dto2;
+  const float* src = G->ckpt + (size_t)slot * 3 * G->nfield;
+  copy_d2d(G->fd, src, G->nfield, mp->stream);
+  copy_d2d(G->fv, src + G->nfield, G->nfield, mp->stream);
+  copy_d2d(G->fa, src + 2 * G->nfield, G->nfield, mp->stream);
+  if (G->batch) {
+    batch_set_pending(mp, dto2);
+  } else {
+    mp->pending_corrector[0] = 1;
+    mp->pending_dto2[0] = dto2;
+  }
 }
 
-// residuals dt (s - o) for all steps, [nstep][nrec][2]; zeroes the adjoint state and the gradient accumulators
+// residuals dt (s - o) for all steps, [nstep][nshot][nrec][2]; zeroes the adjoint state and the accumulators
 void sf2d_grad_begin_reverse(long* Mesh_pointer, const float* res) {
   Mesh* mp = handle(Mesh_pointer);
   GradState* G = need(mp);
   if (G->nrec > 0)
-    check_hip(hipMemcpyAsync(G->res, res, sizeof(float) * 2 * G->nrec * G->nstep, hipMemcpyHostToDevice, mp->stream),
+    check_hip(hipMemcpyAsync(G->res, res, sizeof(float) * 2 * G->nrec * G->nshot * G->nstep, hipMemcpyHostToDevice,
+                             mp->stream),
               "residual H2D");
-  for (float* p : {G->D, G->Vb, G->Rb, G->pm32}) check_hip(hipMemsetAsync(p, 0, G->n2 * sizeof(float), mp->stream), "zero");
+  for (float* p : {G->D, G->Vb, G->Rb}) check_hip(hipMemsetAsync(p, 0, G->nfield * sizeof(float), mp->stream), "zero");
+  check_hip(hipMemsetAsync(G->pm32, 0, G->n2 * sizeof(float), mp->stream), "zero");
   for (float* p : {G->gk32, G->gm32}) check_hip(hipMemsetAsync(p, 0, G->ne * sizeof(float), mp->stream), "zero");
   for (double* p : {G->gk, G->gm}) check_hip(hipMemsetAsync(p, 0, G->ne * sizeof(double), mp->stream), "zero");
   check_hip(hipMemsetAsync(G->pm, 0, G->n2 * sizeof(double), mp->stream), "zero");
@@ -257,27 +332,27 @@ void sf2d_grad_reverse_step(long* Mesh_pointer, int n, int slot, int first, floa
   GradState* G = need(mp);
   if (n < 1 || n > G->nstep) fail("sf2d_grad_reverse_step: step %d outside 1..%d", n, G->nstep);
   if (slot < 0 || slot >= G->chunk) fail("sf2d_grad_reverse_step: slot %d outside 0..%d", slot, G->chunk - 1);
-  const float* dn = G->seg + (size_t)slot * 3 * G->n2;
-  const float* vpn = dn + G->n2;
-  const float* rn = dn + 2 * G->n2;
+  const float* dn = G->seg + (size_t)slot * 3 * G->nfield;
+  const float* vpn = dn + G->nfield;
+  const float* rn = dn + 2 * G->nfield;
   const float dto2 = (float)(0.5 * (double)dt), h = (float)(0.5 * (double)dt * (double)dt);
-  hipLaunchKernelGGL(k_adj_pointwise, dim3(grid_for(G->n2)), dim3(GT), 0, mp->stream, first, (int)G->n2, dt, dto2, h,
-                     G->D, G->Vb, G->Rb, mp->d_rmassx, mp->d_rmassz, rn, G->pm32);
+  hipLaunchKernelGGL(k_adj_pointwise, dim3(grid_for(G->n2)), dim3(GT), 0, mp->stream, first, (int)G->n2, G->nshot,
+                     G->sstr, dt, dto2, h, G->D, G->Vb, G->Rb, mp->d_rmassx, mp->d_rmassz, rn, G->pm32);
   if (G->nrec > 0)
-    hipLaunchKernelGGL(k_adj_receivers, dim3(G->nrec), dim3(64), 0, mp->stream, G->nrec,
-                       G->res + (size_t)(n - 1) * 2 * G->nrec, mp->d_ispec_selected_rec, mp->d_ibool, mp->d_xir,
-                       mp->d_gammar, mp->d_cosrot, mp->d_sinrot, G->D);
-  apply_element_forces(mp, 0, G->Rb, G->D);
-  apply_element_forces(mp, 1, G->Rb, G->D);
+    hipLaunchKernelGGL(k_adj_receivers, dim3(G->nrec, G->nshot), dim3(64), 0, mp->stream, G->nrec,
+                       G->res + (size_t)(n - 1) * 2 * G->nrec * G->nshot, mp->d_ispec_selected_rec, mp->d_ibool,
+                       mp->d_xir, mp->d_gammar, mp->d_cosrot, mp->d_sinrot, G->D, G->sstr);
+  apply_element_forces(mp, 0, G->Rb, G->D, G->nshot, G->stride);
+  apply_element_forces(mp, 1, G->Rb, G->D, G->nshot, G->stride);
   if (mp->stacey && mp->num_abs > 0) {
-    launch_stacey(mp, G->Rb, G->Vb, 0);
+    launch_stacey(mp, G->Rb, G->Vb, 0, G->nshot, G->stride);
     hipLaunchKernelGGL(k_grad_stacey, dim3((mp->num_abs * NGLL + 63) / 64), dim3(64), 0, mp->stream, mp->num_abs,
-                       mp->d_abs_ispec, mp->d_abs_ij, mp->d_abs_normal, mp->d_abs_jacw, mp->d_ibool,
-                       (const float2*)vpn, (const float2*)G->Rb, G->gvp32, G->gvs32);
+                       G->nshot, (size_t)G->stride, mp->d_abs_ispec, mp->d_abs_ij, mp->d_abs_normal, mp->d_abs_jacw,
+                       mp->d_ibool, (const float2*)vpn, (const float2*)G->Rb, G->gvp32, G->gvs32);
   }
   hipLaunchKernelGGL(k_grad_elastic, dim3((mp->nspec + GEPB - 1) / GEPB), dim3(GT), 0, mp->stream, mp->nspec,
-                     mp->d_ibool, mp->d_xix, mp->d_xiz, mp->d_gammax, mp->d_gammaz, (const float2*)dn,
-                     (const float2*)G->Rb, G->gk32, G->gm32, mp->basis);
+                     G->nshot, G->sstr, mp->d_ibool, mp->d_xix, mp->d_xiz, mp->d_gammax, mp->d_gammaz,
+                     (const float2*)dn, (const float2*)G->Rb, G->gk32, G->gm32, mp->basis);
   check_hip(hipGetLastError(), "sf2d_grad_reverse_step launch");
 }
 

@@ -130,14 +130,35 @@ struct Mesh {
   // scratch
   unsigned int* d_scratch_u32 = nullptr;  // reductions
 
-  // exact discrete adjoint (gradient.hip), allocated by sf2d_grad_prepare
+  // exact discrete adjoint (gradient.hip), allocated by sf2d_grad_prepare / sf2d_batch_grad_prepare
   struct GradState* grad = nullptr;
+  int grad_batch = 0;  // the gradient buffers follow the shot batch
+  // shot batch (batch.hip), allocated by sf2d_batch_prepare
+  struct BatchState* batch = nullptr;
 };
+
+// nshot wavefields of one mesh and model, stored shot-major: field k of shot s at (s * stride + g) float2
+// points; every source of the handle belongs to one shot (src_shot).  The Newmark corrector is deferred
+// as for the handle's own fields.
+struct BatchState {
+  int nshot = 0;
+  long stride = 0;                   // points between shots (>= nglob, a multiple of 64)
+  float *d = nullptr, *v = nullptr, *a = nullptr;
+  int* src_shot = nullptr;           // [nsources_local]
+  float* seismo = nullptr;           // [nshot][2][nrec][nstep] displacement seismograms
+  int pending = 0;
+  float pending_dto2 = 0.f;
+};
+
+// shot batch fields and the re-arming of its deferred corrector (batch.hip)
+void batch_fields(Mesh* mp, float** d, float** v, float** a, int* nshot, long* stride);
+void batch_set_pending(Mesh* mp, float dto2);
 
 // applies a deferred corrector now (no-op when none is pending)
 void flush_correctors(Mesh* mp);
-// acc -= K u over the elements of phase ph (0 outer, 1 inner), with the handle's assembly scheme
-void apply_element_forces(const Mesh* mp, int ph, const float* u, float* acc);
+// acc -= K u over the elements of phase ph (0 outer, 1 inner), with the handle's assembly scheme, for nshot
+// wavefields shot_stride points apart
+void apply_element_forces(const Mesh* mp, int ph, const float* u, float* acc, int nshot = 1, long shot_stride = 0);
 // point arrays in the caller's numbering <-> the device numbering (n2 values per point)
 void points_to_device(const Mesh* mp, float* dev, const float* host, int n2, const char* what);
 void points_from_device(const Mesh* mp, float* host, const float* dev, int n2, const char* what);
@@ -185,23 +206,29 @@ void device_free(T*& p) {
 // ---------------------------------------------------------------------------
 // kernel launchers (defined in elastic.hip), all asynchronous on mp->stream
 void launch_newmark_predictor(float* d, float* v, float* a, int nglob, float dt, float dt2o2, float dto2,
-                              hipStream_t s);
+                              hipStream_t s, int nshot = 1, long shot_stride = 0);
 void launch_newmark_corrector(float* v, float* a, const float* rmx, const float* rmz, int nglob, float dto2,
-                              int apply_mass, hipStream_t s);
+                              int apply_mass, hipStream_t s, int nshot = 1, long shot_stride = 0);
 void launch_newmark_corrector_predictor(float* d, float* v, float* a, const float* rmx, const float* rmz, int nglob,
-                                       float dto2_prev, float dt, float dt2o2, float dto2, hipStream_t s);
+                                       float dto2_prev, float dt, float dt2o2, float dto2, hipStream_t s,
+                                       int nshot = 1, long shot_stride = 0);
 void launch_elastic_forces(const Mesh* mp, int first, int nelem, int atomic, const float* displ, float* accel);
-void launch_elastic_forces_patch(const Mesh* mp, int first_patch, int npatch, const float* displ, float* accel);
+// nshot wavefields at a stride of shot_stride points (float2) in displ and accel
+void launch_elastic_forces_patch(const Mesh* mp, int first_patch, int npatch, const float* displ, float* accel,
+                                 int nshot = 1, long shot_stride = 0);
 void launch_permute_elements(const float* src, float* dst, const int* ispec_of_pos, int nspec, hipStream_t s);
 void launch_pack_elements(const Mesh* mp);  // geom / moduli from the per-element arrays (patch mode)
 // per-element geometry and the affine flags of the patches; returns the number of affine elements
 int launch_flag_affine(Mesh* mp);
 void launch_permute_elements_int(const int* src, int* dst, const int* ispec_of_pos, int nspec, hipStream_t s);
-void launch_stacey(const Mesh* mp, const float* veloc, float* accel, int write_store);
+void launch_stacey(const Mesh* mp, const float* veloc, float* accel, int write_store, int nshot = 1,
+                   long shot_stride = 0);
 void launch_stacey_backward(const Mesh* mp, float* b_accel);
-void launch_sources(const Mesh* mp, float* accel, int it);
+// src_shot (device, one entry per source) sends each source to its own wavefield; null: all into accel
+void launch_sources(const Mesh* mp, float* accel, int it, const int* src_shot = nullptr, long shot_stride = 0);
 void launch_adjoint_sources(const Mesh* mp, float* accel, int it);
-void launch_seismograms(const Mesh* mp, const float* field, float* dst, int seismo_current);
+void launch_seismograms(const Mesh* mp, const float* field, float* dst, int seismo_current, int nshot = 1,
+                        long shot_stride = 0, long dst_stride = 0);
 void launch_kernels(const Mesh* mp);
 void launch_max_norm(const Mesh* mp, const float* field, unsigned int* result);
 

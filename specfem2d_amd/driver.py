@@ -81,6 +81,16 @@ def load_libraries():
     host.sf2d_sim_gradient_forward.restype = C.c_int
     host.sf2d_sim_gradient_backward.argtypes = [P, P, P]
     host.sf2d_sim_gradient_backward.restype = C.c_int
+    host.sf2d_sim_create_shots.restype = P
+    host.sf2d_sim_create_shots.argtypes = [C.POINTER(Params), dp, dp, dp, C.c_int, P, C.POINTER(SourceC), C.c_int,
+                                           dp]
+    host.sf2d_sim_batch_forward.argtypes = [P, P, P]
+    host.sf2d_sim_batch_forward.restype = C.c_int
+    host.sf2d_sim_nshot.argtypes = [P]
+    host.sf2d_sim_nshot.restype = C.c_int
+    host.sf2d_sim_shot_t0.argtypes = [P, P]
+    host.sf2d_sim_shot_misfits.argtypes = [P, P]
+    host.sf2d_sim_batch_get_fields.argtypes = [P, C.c_int, P, P, P]
     host.sf2d_sim_reset.argtypes = [P]
     host.sf2d_sim_steps_done.argtypes = [P]
     host.sf2d_sim_steps_done.restype = C.c_int
@@ -124,6 +134,14 @@ def params_from_config(cfg: SimulationConfig) -> Params:
     return p
 
 
+def _sources_c(sources):
+    srcs = (SourceC * max(1, len(sources)))()
+    for k, s in enumerate(sources):
+        srcs[k].x, srcs[k].z, srcs[k].f0, srcs[k].factor = s.x, s.z, s.f0, s.factor
+        srcs[k].anglesource, srcs[k].tshift, srcs[k].time_function_type = s.anglesource, s.tshift, s.time_function_type
+    return srcs
+
+
 class Simulation:
     """An elastic simulation driven through the GPU C-ABI.
 
@@ -138,15 +156,9 @@ class Simulation:
         self.cfg = cfg
         self.gpu, self.host = load_libraries()
         rho, vp, vs = cfg.material_arrays()
-        srcs = (SourceC * max(1, len(cfg.sources)))()
-        for k, s in enumerate(cfg.sources):
-            srcs[k].x, srcs[k].z, srcs[k].f0, srcs[k].factor = s.x, s.z, s.f0, s.factor
-            srcs[k].anglesource, srcs[k].tshift, srcs[k].time_function_type = s.anglesource, s.tshift, s.time_function_type
         rec = np.ascontiguousarray(np.asarray(cfg.receivers if cfg.nrec else np.zeros((1, 2)), dtype=np.float64))
         p = params_from_config(cfg)
-        self._keep = (p, rho, vp, vs, srcs, rec)
-        self.h = self.host.sf2d_sim_create(C.byref(p), rho, vp, vs, len(cfg.sources), srcs, cfg.nrec, rec,
-                                           int(simulation_type), int(save_forward))
+        self.h = self._create(p, rho, vp, vs, rec, int(simulation_type), int(save_forward))
         if not self.h:
             raise ValueError("invalid simulation parameters")
         self.simulation_type = simulation_type
@@ -158,6 +170,12 @@ class Simulation:
         self.on_device = False
         if device:
             self.prepare_device()
+
+    def _create(self, p, rho, vp, vs, rec, simulation_type, save_forward):
+        srcs = _sources_c(self.cfg.sources)
+        self._keep = (p, rho, vp, vs, srcs, rec)
+        return self.host.sf2d_sim_create(C.byref(p), rho, vp, vs, len(self.cfg.sources), srcs, self.cfg.nrec, rec,
+                                         simulation_type, save_forward)
 
     def prepare_device(self):
         if self.host.sf2d_sim_prepare_device(self.h) != 0:
@@ -214,13 +232,17 @@ class Simulation:
         self.host.sf2d_sim_get_elem_array(self.h, self.ELEM[name], _ptr(out))
         return out.reshape(self.nspec, NGLL2)
 
+    def _source_list(self):
+        return self.cfg.sources
+
     def sources(self):
-        n = max(1, len(self.cfg.sources))
+        """Located sources: (ispec 1-based, xi, gamma, source arrays [nsrc,25,2], time functions [nsrc,nstep])."""
+        n = max(1, len(self._source_list()))
         isp, xi, ga = np.zeros(n, np.int32), np.zeros(n), np.zeros(n)
         sa = np.zeros(n * NGLL2 * 2, np.float32)
         stf = np.zeros(self.cfg.nstep * n, np.float32)
         self.host.sf2d_sim_get_sources(self.h, _ptr(isp), _ptr(xi), _ptr(ga), _ptr(sa), _ptr(stf))
-        k = len(self.cfg.sources)
+        k = len(self._source_list())
         return isp[:k], xi[:k], ga[:k], sa.reshape(n, NGLL2, 2)[:k], stf.reshape(self.cfg.nstep, n)[:, :k].T
 
     def receivers(self):
@@ -308,16 +330,19 @@ class Simulation:
         """Forward run from rest over all NSTEP steps with a checkpoint every ``chunk`` steps kept on the
         device; returns (chi, seismograms) with chi = 1/2 dt sum |s - obs|^2 (obs [nrec,2,nstep])."""
         self._need_device()
-        n = self.cfg.nstep
+        n, shape = self.cfg.nstep, self._seis_shape()
         o = np.ascontiguousarray(np.asarray(obs, np.float64).reshape(-1))
-        if o.size != self.cfg.nrec * 2 * n:
-            raise ValueError("obs must be [nrec, 2, nstep]")
+        if o.size != int(np.prod(shape)):
+            raise ValueError(f"obs must be {list(shape)} ([nshot,] nrec, 2, nstep)")
         chi = C.c_double(0.0)
-        seis = np.zeros((max(1, self.cfg.nrec), 2, n))
+        seis = np.zeros(int(np.prod(shape)) + 2 * n)
         rc = self.host.sf2d_sim_gradient_forward(self.h, _ptr(o), int(chunk), C.byref(chi), _ptr(seis))
         if rc != 0:
             raise RuntimeError("sf2d_sim_gradient_forward failed")
-        return chi.value, seis[: self.cfg.nrec]
+        return chi.value, seis[: int(np.prod(shape))].reshape(shape)
+
+    def _seis_shape(self):
+        return (self.cfg.nrec, 2, self.cfg.nstep)
 
     def gradient_backward(self, parts: bool = False):
         """Reverse sweep after gradient_forward: d chi / d (rho, vp, vs) as [nspec,25] arrays (dict), and with
@@ -347,3 +372,63 @@ class Simulation:
         arrs = [None if x is None else np.ascontiguousarray(np.asarray(x, np.float32).reshape(-1))
                 for x in (displ, veloc, accel)]
         self.host.sf2d_sim_set_fields(self.h, *[_ptr(x) for x in arrs])
+
+
+class ShotBatch(Simulation):
+    """``len(shots)`` forward simulations of one mesh, model and receiver set advanced together on the GPU:
+    shot k is driven by the sources ``shots[k]`` (a list of config.Source), timed as a simulation of its own
+    (``cfg.sources`` is ignored).  ``forward()`` returns seismograms [nshot, nrec, 2, nstep]; the gradient
+    calls take observations of that shape and return the misfit summed over the shots and its gradient."""
+
+    def __init__(self, cfg: SimulationConfig, shots, device: bool = True):
+        self.shots = [list(s) for s in shots]
+        if not self.shots or any(len(s) == 0 for s in self.shots):
+            raise ValueError("every shot needs at least one source")
+        super().__init__(cfg, simulation_type=1, save_forward=False, device=device)
+        self.nshot = len(self.shots)
+
+    def _create(self, p, rho, vp, vs, rec, simulation_type, save_forward):
+        flat = [s for shot in self.shots for s in shot]
+        srcs = _sources_c(flat)
+        counts = np.asarray([len(s) for s in self.shots], np.int32)
+        self._keep = (p, rho, vp, vs, srcs, rec, counts)
+        return self.host.sf2d_sim_create_shots(C.byref(p), rho, vp, vs, len(self.shots), _ptr(counts), srcs,
+                                               self.cfg.nrec, rec)
+
+    def _seis_shape(self):
+        return (self.nshot, self.cfg.nrec, 2, self.cfg.nstep)
+
+    def _source_list(self):
+        return [s for shot in self.shots for s in shot]
+
+    def shot_t0(self):
+        out = np.zeros(self.nshot)
+        self.host.sf2d_sim_shot_t0(self.h, _ptr(out))
+        return out
+
+    def forward(self, timing: bool = False):
+        """All NSTEP steps of every shot from rest: seismograms [nshot, nrec, 2, nstep] (and device ms)."""
+        self._need_device()
+        seis = np.zeros(int(np.prod(self._seis_shape())) + 1)
+        t = np.zeros(1)
+        if self.host.sf2d_sim_batch_forward(self.h, _ptr(seis), _ptr(t) if timing else None) != 0:
+            raise RuntimeError("sf2d_sim_batch_forward failed")
+        seis = seis[:-1].reshape(self._seis_shape())
+        return (seis, float(t[0])) if timing else seis
+
+    def shot_misfits(self):
+        """Per-shot misfits of the last gradient_forward."""
+        out = np.zeros(self.nshot)
+        self.host.sf2d_sim_shot_misfits(self.h, _ptr(out))
+        return out
+
+    def fields(self, shot: int = 0):
+        self._need_device()
+        if not 0 <= shot < self.nshot:
+            raise IndexError(shot)
+        d, v, a = (np.zeros(2 * self.nglob, np.float32) for _ in range(3))
+        self.host.sf2d_sim_batch_get_fields(self.h, int(shot), _ptr(d), _ptr(v), _ptr(a))
+        return d.reshape(-1, 2), v.reshape(-1, 2), a.reshape(-1, 2)
+
+    def adjoint(self, *a, **k):
+        raise RuntimeError("SIMULATION_TYPE=3 runs take one simulation; use Simulation(simulation_type=3)")

@@ -1,0 +1,170 @@
+"""GPU parity of shot batches (all shots of one mesh and model advanced together, one launch per kernel).
+
+* forward: every shot's seismograms and final fields equal those of a single simulation with only that
+  shot's sources (rel L2 1e-6: the runs differ only in the order of FP32 atomic additions at source and
+  corner points), and the FP32 oracle's traces of that simulation to the parity tolerance 1e-5;
+* gradient: the batch misfit is the sum of the single misfits (per shot too), the batch gradient the sum of
+  the single gradients (rel L2 1e-5), and the sum of the FP64 oracle's gradients to the tolerance of the
+  single-shot gradient test; checkpoint length does not change it; the torch.autograd misfit runs on a batch.
+"""
+import numpy as np
+import pytest
+
+from oracle.oracle import OracleSim
+from specfem2d_amd import config
+from tests.test_batch_setup import shots_for
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-300))
+
+
+@pytest.fixture(scope="module")
+def case():
+    cfg = config.small_test_mesh(nx=12, nz=9, nstep=300, gll=True, seed=5).replace(
+        absorbtop=True, absorbbottom=True, absorbleft=True, absorbright=True)
+    return cfg, shots_for(cfg)
+
+
+@pytest.fixture(scope="module")
+def singles(case):
+    from specfem2d_amd.driver import Simulation
+    cfg, shots = case
+    out = []
+    for shot in shots:
+        sim = Simulation(cfg.replace(sources=shot))
+        seis, _ = sim.forward()
+        out.append((seis, sim.fields()))
+        sim.close()
+    return out
+
+
+def test_batch_forward_matches_single_runs(case, singles):
+    from specfem2d_amd.driver import ShotBatch
+    cfg, shots = case
+    b = ShotBatch(cfg, shots)
+    seis, ms = b.forward(timing=True)
+    assert seis.shape == (len(shots), cfg.nrec, 2, cfg.nstep)
+    for k, (ref, fields) in enumerate(singles):
+        err = _rel(seis[k], ref)
+        print(f"shot {k}: batch vs single seismograms rel L2 {err:.2e}")
+        assert err < 1e-6, (k, err)
+        for name, x, y in zip(("displ", "veloc", "accel"), b.fields(k), fields):
+            e = _rel(x, y)
+            assert e < 1e-6, (k, name, e)
+    print(f"batch of {len(shots)} shots, {cfg.nstep} steps: {ms:.1f} ms")
+    b.close()
+
+
+def test_batch_forward_matches_the_oracle(case):
+    from specfem2d_amd.driver import ShotBatch
+    cfg, shots = case
+    b = ShotBatch(cfg, shots)
+    seis = b.forward()
+    b.close()
+    for k, shot in enumerate(shots):
+        ref = OracleSim(cfg.replace(sources=shot), "sp").forward()
+        err = _rel(seis[k], ref)
+        print(f"shot {k}: batch vs FP32 oracle rel L2 {err:.2e}")
+        assert err < 1e-5, (k, err)
+
+
+@pytest.fixture(scope="module")
+def observed(case):
+    cfg, shots = case
+    rng = np.random.default_rng(1)
+    vs0 = np.asarray(cfg.vs, float)
+    true = cfg.replace(vs=vs0 * (1 + 0.02 * rng.standard_normal(vs0.size)))
+    return np.stack([OracleSim(true.replace(sources=shot), "dp").forward() for shot in shots])
+
+
+@pytest.fixture(scope="module")
+def batch_grad(case, observed):
+    from specfem2d_amd.driver import ShotBatch
+    cfg, shots = case
+    b = ShotBatch(cfg, shots)
+    chi, seis = b.gradient_forward(observed, chunk=64)
+    per_shot = b.shot_misfits()
+    g = b.gradient_backward()
+    b.close()
+    return chi, per_shot, seis, g
+
+
+def test_batch_gradient_is_the_sum_of_single_gradients(case, observed, batch_grad):
+    from specfem2d_amd.driver import Simulation
+    cfg, shots = case
+    chi, per_shot, seis, g = batch_grad
+    total = {k: 0.0 for k in ("rho", "vp", "vs")}
+    chis = []
+    for k, shot in enumerate(shots):
+        sim = Simulation(cfg.replace(sources=shot))
+        c, s = sim.gradient_forward(observed[k], chunk=64)
+        gk = sim.gradient_backward()
+        sim.close()
+        chis.append(c)
+        assert _rel(seis[k], s) < 1e-6
+        for name in total:
+            total[name] = total[name] + gk[name]
+    print(f"chi batch {chi:.6e}, sum of singles {sum(chis):.6e}, per shot {per_shot} vs {chis}")
+    assert abs(chi - sum(chis)) < 1e-6 * sum(chis)
+    np.testing.assert_allclose(per_shot, chis, rtol=1e-6)
+    for name in total:
+        err = _rel(g[name], total[name])
+        print(f"batch gradient {name}: rel L2 vs sum of single gradients {err:.2e}")
+        assert err < 1e-5, (name, err)
+
+
+def test_batch_gradient_against_oracle(case, observed, batch_grad):
+    cfg, shots = case
+    chi, _, _, g = batch_grad
+    gd = {k: 0.0 for k in ("rho", "vp", "vs")}
+    gs = {k: 0.0 for k in ("rho", "vp", "vs")}
+    cd = 0.0
+    for k, shot in enumerate(shots):
+        c, d, _ = OracleSim(cfg.replace(sources=shot), "dp").gradient(observed[k])
+        _, s, _ = OracleSim(cfg.replace(sources=shot), "sp").gradient(observed[k])
+        cd += c
+        for name in gd:
+            gd[name] = gd[name] + d[name]
+            gs[name] = gs[name] + s[name]
+    assert abs(chi - cd) < 1e-4 * cd
+    for name in gd:
+        tol = max(1e-5, 3 * _rel(gs[name], gd[name]))
+        err = _rel(g[name], gd[name])
+        print(f"batch gradient {name}: rel L2 vs FP64 oracle {err:.2e} (tol {tol:.1e})")
+        assert err < tol, (name, err, tol)
+
+
+def test_batch_checkpoint_length_does_not_matter(case, observed, batch_grad):
+    from specfem2d_amd.driver import ShotBatch
+    cfg, shots = case
+    _, _, _, g64 = batch_grad
+    b = ShotBatch(cfg, shots)
+    for chunk in (1, 37, cfg.nstep):
+        b.gradient_forward(observed, chunk=chunk)
+        g = b.gradient_backward()
+        for name in g64:
+            err = _rel(g[name], g64[name])
+            assert err < 1e-5, (chunk, name, err)
+    b.close()
+
+
+def test_autograd_misfit_on_a_batch(case, observed, batch_grad):
+    import torch
+    from specfem2d_amd.autograd import waveform_misfit
+    from specfem2d_amd.driver import ShotBatch
+    cfg, shots = case
+    chi_ref, _, _, g = batch_grad
+    b = ShotBatch(cfg, shots)
+    dev = torch.device("cuda:0")
+    params = [torch.tensor(np.asarray(getattr(cfg, k), np.float64).reshape(cfg.nspec, 25), device=dev,
+                           dtype=torch.float32, requires_grad=True) for k in ("rho", "vp", "vs")]
+    chi = waveform_misfit(*params, b, observed, chunk=50)
+    chi.backward()
+    assert abs(chi.item() - chi_ref) < 1e-4 * chi_ref
+    for t, k in zip(params, ("rho", "vp", "vs")):
+        err = _rel(t.grad.double().cpu().numpy(), g[k])
+        assert err < 1e-4, (k, err)
+    b.close()
