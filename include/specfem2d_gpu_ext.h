@@ -63,6 +63,9 @@ void sf2d_grad_result(long* Mesh_pointer, double* g_kappa, double* g_mu, double*
 /* gradient buffers over the shot batch (sf2d_batch_prepare): the calls above then checkpoint and reverse all
    shots together, residuals are [nstep][nshot][nrec][2] and the results are sums over the shots */
 int sf2d_batch_grad_prepare(long* Mesh_pointer, int chunk);
+/* shot batch: recomputes steps n0+1..n1 directly into segment slots 0..n1-n0-1, starting from checkpoint slot
+   ckpt (the state after step n0) or from rest (ckpt = -1 with n0 = 0); replaces restore + forward + saves */
+void sf2d_grad_recompute_segment(long* Mesh_pointer, int ckpt, int n0, int n1, float dt);
 
 /* ---------------------------------------------------------------------------
  * Shot batches: nshot wavefields of one mesh and model, each driven by its own sources and recorded at the

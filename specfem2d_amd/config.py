@@ -209,6 +209,37 @@ def adjoint_gradient_800x400(nstep: int = 2000, anomaly: bool = False) -> Simula
     return cfg.replace(rho=rho, vp=vp, vs=vs)
 
 
+def batched_fwi_600x300(nstep: int = 1500, true_model: bool = False, nrec: int = 150) -> SimulationConfig:
+    """Multi-shot FWI workload: 600 x 300 elements of 20 m (12 km x 6 km), dt 4e-4 s (Courant number 0.38 at
+    the true model's highest vp over the closest GLL points), 1500 steps, Stacey on three sides and a free
+    surface, ``nrec`` receivers along the surface.  The synthetic model is homogeneous (vp 3000, vs 1732,
+    rho 2500, MODEL=gll arrays); ``true_model`` gives the 'observed' model, a seeded Gaussian random field
+    (correlation length 200 m, +-10 %) around it.  The shots come from ``surface_shots``."""
+    nx, nz, h = 600, 300, 20.0
+    xmax, depth = nx * h, nz * h
+    cfg = SimulationConfig(
+        name="batched_fwi_600x300", nx=nx, nz=nz, xmin=0.0, xmax=xmax, interfaces=[-depth, 0.0],
+        nz_layers=[nz], nstep=nstep, dt=4.0e-4, material_mode=1, stacey=True, absorbbottom=True,
+        absorbright=True, absorbtop=False, absorbleft=True,
+        sources=[Source(x=xmax / 2, z=-10.0, f0=10.0, factor=1.0e10)],
+        receivers=np.stack([np.linspace(h, xmax - h, nrec), np.zeros(nrec)], axis=1))
+    n = cfg.nspec * NGLL2
+    if true_model:
+        pert = gaussian_random_field(cfg, 200.0, 0.10, 4242)
+    else:
+        pert = np.zeros(n)
+    return cfg.replace(rho=2500.0 * (1.0 + pert), vp=3000.0 * (1.0 + pert), vs=1732.0 * (1.0 + pert))
+
+
+def surface_shots(cfg: SimulationConfig, nshot: int, f0: float = 10.0, depth: float = 10.0) -> List[List[Source]]:
+    """``nshot`` vertical Ricker force sources of frequency f0 evenly spaced along the free surface (from 2 % to
+    98 % of the width, ``depth`` metres down), one source per shot."""
+    w = cfg.xmax - cfg.xmin
+    xs = cfg.xmin + w * (0.02 + 0.96 * (np.arange(nshot) + 0.5) / nshot)
+    top = cfg.interfaces[-1]
+    return [[Source(x=float(x), z=top - depth, f0=f0, factor=1.0e10)] for x in xs]
+
+
 def small_test_mesh(nx: int = 8, nz: int = 6, nstep: int = 200, seed: int = 0, stacey: bool = True,
                     gll: bool = False) -> SimulationConfig:
     """A small mesh for fast parity tests: 100 m elements, Ricker 15 Hz inside."""

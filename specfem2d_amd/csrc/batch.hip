@@ -38,6 +38,27 @@ void batch_fields(Mesh* mp, float** d, float** v, float** a, int* nshot, long* s
   *stride = B->stride;
 }
 
+void batch_step_into(Mesh* mp, int it, const float* d, const float* v, const float* a, float dto2_prev, float* d_out,
+                     float* v_out, float* a_out, float deltat, float deltatsqover2, float deltatover2) {
+  BatchState* B = need_batch(mp);
+  if (it < 1 || it > mp->nstep) fail("batch step %d outside 1..%d", it, mp->nstep);
+  const size_t n = (size_t)2 * B->stride * B->nshot;
+  if (!d) {  // the predictor of a state at rest is the state at rest
+    for (float* p : {d_out, v_out, a_out}) check_hip(hipMemsetAsync(p, 0, n * sizeof(float), mp->stream), "zero");
+  } else {
+    launch_newmark_corrector_predictor_to(d, v, a, d_out, v_out, a_out, mp->d_rmassx, mp->d_rmassz, mp->nglob,
+                                          dto2_prev, deltat, deltatsqover2, deltatover2, mp->stream, B->nshot,
+                                          B->stride);
+  }
+  for (int ph = 0; ph < 2; ph++) {
+    apply_element_forces(mp, ph, d_out, a_out, B->nshot, B->stride);
+    if (ph == 0) {
+      if (mp->stacey) launch_stacey(mp, v_out, a_out, 0, B->nshot, B->stride);
+      launch_sources(mp, a_out, it - 1, B->src_shot, B->stride);
+    }
+  }
+}
+
 void batch_set_pending(Mesh* mp, float dto2) {
   BatchState* B = need_batch(mp);
   B->pending = 1;

@@ -1078,21 +1078,17 @@ int sf2d_sim_gradient_backward(void* sim, double* grad, double* parts) {
   const int nseg = (N + C - 1) / C;
   for (int k = nseg - 1; k >= 0; k--) {
     const int n0 = k * C, n1 = std::min(N, n0 + C);
-    if (k == 0) {
-      if (s->nshot > 0)
-        sf2d_batch_reset(&s->mesh);
-      else
-        sf2d_sim_reset(sim);
-    } else {
-      sf2d_grad_restore_checkpoint(&s->mesh, k - 1, dto2);
-    }
-    s->it_done = n0;
-    s->grad_save = 2;
-    s->grad_seg_base = n0;
     if (s->nshot > 0) {
-      batch_steps(s, n0 + 1, n1, false);
-      s->grad_save = 0;
+      // a batch recomputes the segment straight into its state slots
+      sf2d_grad_recompute_segment(&s->mesh, k - 1, n0, n1, dt);
     } else {
+      if (k == 0)
+        sf2d_sim_reset(sim);
+      else
+        sf2d_grad_restore_checkpoint(&s->mesh, k - 1, dto2);
+      s->it_done = n0;
+      s->grad_save = 2;
+      s->grad_seg_base = n0;
       const int rc = sf2d_sim_forward(sim, n1 - n0, nullptr, nullptr, nullptr);
       s->grad_save = 0;
       if (rc != 0) return rc;

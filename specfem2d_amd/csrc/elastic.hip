@@ -97,6 +97,37 @@ __global__ __launch_bounds__(256) void k_corrector_predictor(float2* __restrict_
   }
 }
 
+// The same update out of place: state (d, v, a) with the corrector pending -> predicted state (dd, dv, 0), for
+// recomputations that keep every step's state.
+__global__ __launch_bounds__(256) void k_corrector_predictor_to(
+    const float2* __restrict__ d, const float2* __restrict__ v, const float2* __restrict__ a,
+    float2* __restrict__ dd_out, float2* __restrict__ dv_out, float2* __restrict__ da_out,
+    const float* __restrict__ rmx, const float* __restrict__ rmz, int n, float dto2_prev, float dt, float dt2o2,
+    float dto2, long shot_stride) {
+  const long off = blockIdx.y * shot_stride;
+  d += off;
+  v += off;
+  a += off;
+  dd_out += off;
+  dv_out += off;
+  da_out += off;
+  int stride = gridDim.x * blockDim.x;
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
+    float2 dd = d[k], vv = v[k], aa = a[k];
+    aa.x = aa.x * rmx[k];
+    aa.y = aa.y * rmz[k];
+    vv.x = vv.x + dto2_prev * aa.x;
+    vv.y = vv.y + dto2_prev * aa.y;
+    dd.x = dd.x + dt * vv.x + dt2o2 * aa.x;
+    dd.y = dd.y + dt * vv.y + dt2o2 * aa.y;
+    vv.x = vv.x + dto2 * aa.x;
+    vv.y = vv.y + dto2 * aa.y;
+    dd_out[k] = dd;
+    dv_out[k] = vv;
+    da_out[k] = make_float2(0.f, 0.f);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // isotropic elastic element forces, P-SV, for the elements at positions [first, first + nelem).
 // ATOMIC = false: the range is one colour (no two elements share a point), so the assembly is a plain
@@ -733,6 +764,17 @@ void launch_newmark_corrector_predictor(float* d, float* v, float* a, const floa
   hipLaunchKernelGGL(k_corrector_predictor, shot_grid(nglob, 256, 65536, nshot), dim3(256), 0, s, (float2*)d,
                      (float2*)v, (float2*)a, rmx, rmz, nglob, dto2_prev, dt, dt2o2, dto2, shot_stride);
   check_hip(hipGetLastError(), "k_corrector_predictor launch");
+}
+
+void launch_newmark_corrector_predictor_to(const float* d, const float* v, const float* a, float* d_out, float* v_out,
+                                          float* a_out, const float* rmx, const float* rmz, int nglob,
+                                          float dto2_prev, float dt, float dt2o2, float dto2, hipStream_t s,
+                                          int nshot, long shot_stride) {
+  if (nglob <= 0) return;
+  hipLaunchKernelGGL(k_corrector_predictor_to, shot_grid(nglob, 256, 65536, nshot), dim3(256), 0, s,
+                     (const float2*)d, (const float2*)v, (const float2*)a, (float2*)d_out, (float2*)v_out,
+                     (float2*)a_out, rmx, rmz, nglob, dto2_prev, dt, dt2o2, dto2, shot_stride);
+  check_hip(hipGetLastError(), "k_corrector_predictor_to launch");
 }
 
 void launch_elastic_forces(const Mesh* mp, int first, int nelem, int atomic, const float* displ, float* accel) {

@@ -200,7 +200,19 @@ void copy_d2d(float* dst, const float* src, size_t n, hipStream_t s) {
 int grad_prepare(Mesh* mp, long* Mesh_pointer, int chunk, int nrec, int batch) {
   if (chunk < 1) fail("gradient: chunk %d < 1", chunk);
   if (nrec != mp->nrec_local) fail("gradient: %d receivers, the handle has %d", nrec, mp->nrec_local);
-  if (mp->grad) sf2d_grad_release(Mesh_pointer);
+  if (GradState* old = mp->grad) {  // same layout as the buffers in place: keep them (repeated gradients)
+    bool same = old->batch == batch && old->chunk == std::min(chunk, mp->nstep) && old->nstep == mp->nstep &&
+                old->nrec == nrec;
+    if (same && batch) {
+      float *d, *v, *a;
+      int ns;
+      long st;
+      batch_fields(mp, &d, &v, &a, &ns, &st);
+      same = old->fd == d && old->nshot == ns && old->stride == st;
+    }
+    if (same) return old->nckpt;
+    sf2d_grad_release(Mesh_pointer);
+  }
   GradState* G = new GradState();
   G->nstep = mp->nstep;
   G->chunk = std::min(chunk, mp->nstep);
@@ -224,8 +236,8 @@ int grad_prepare(Mesh* mp, long* Mesh_pointer, int chunk, int nrec, int batch) {
   const size_t states = (size_t)std::max(1, G->nckpt) + G->chunk;
   if (states * 3 * G->nfield * sizeof(float) > ((size_t)240 << 30))
     fail("gradient: %zu states of %d shots exceed 240 GiB; use a longer chunk or fewer shots", states, G->nshot);
-  G->ckpt = device_zeros<float>(3 * G->nfield * std::max(1, G->nckpt));
-  G->seg = device_zeros<float>(3 * G->nfield * G->chunk);
+  G->ckpt = device_alloc<float>(3 * G->nfield * std::max(1, G->nckpt));  // written before they are read
+  G->seg = device_alloc<float>(3 * G->nfield * G->chunk);
   G->D = device_zeros<float>(G->nfield);
   G->Vb = device_zeros<float>(G->nfield);
   G->Rb = device_zeros<float>(G->nfield);
@@ -354,6 +366,26 @@ void sf2d_grad_reverse_step(long* Mesh_pointer, int n, int slot, int first, floa
                      G->nshot, G->sstr, mp->d_ibool, mp->d_xix, mp->d_xiz, mp->d_gammax, mp->d_gammaz,
                      (const float2*)dn, (const float2*)G->Rb, G->gk32, G->gm32, mp->basis);
   check_hip(hipGetLastError(), "sf2d_grad_reverse_step launch");
+}
+
+// shot batch: recomputes steps n0+1 .. n1 straight into the segment slots, from checkpoint slot ckpt (the state
+// after step n0) or from rest (ckpt = -1, n0 = 0); no state is copied
+void sf2d_grad_recompute_segment(long* Mesh_pointer, int ckpt, int n0, int n1, float dt) {
+  Mesh* mp = handle(Mesh_pointer);
+  GradState* G = need(mp);
+  if (!G->batch) fail("sf2d_grad_recompute_segment: the gradient buffers do not follow a shot batch");
+  if (n0 < 0 || n1 <= n0 || n1 > G->nstep || n1 - n0 > G->chunk)
+    fail("sf2d_grad_recompute_segment: steps %d..%d (segments of at most %d)", n0 + 1, n1, G->chunk);
+  if (ckpt < -1 || ckpt >= G->nckpt || (ckpt == -1) != (n0 == 0))
+    fail("sf2d_grad_recompute_segment: checkpoint %d for a segment after step %d", ckpt, n0);
+  const float dto2 = (float)(0.5 * (double)dt), h = (float)(0.5 * (double)dt * (double)dt);
+  const float* src = ckpt < 0 ? nullptr : G->ckpt + (size_t)ckpt * 3 * G->nfield;
+  for (int n = n0 + 1; n <= n1; n++) {
+    float* dst = G->seg + (size_t)(n - n0 - 1) * 3 * G->nfield;
+    batch_step_into(mp, n, src, src ? src + G->nfield : nullptr, src ? src + 2 * G->nfield : nullptr, dto2, dst,
+                    dst + G->nfield, dst + 2 * G->nfield, dt, h, dto2);
+    src = dst;
+  }
 }
 
 // adds the FP32 segment accumulators into the FP64 totals

@@ -153,6 +153,10 @@ struct BatchState {
 // shot batch fields and the re-arming of its deferred corrector (batch.hip)
 void batch_fields(Mesh* mp, float** d, float** v, float** a, int* nshot, long* stride);
 void batch_set_pending(Mesh* mp, float dto2);
+// time step it of every shot from state (d, v, a) -- corrector pending with dto2_prev, or at rest when d is
+// null -- into state (d_out, v_out, a_out), each [nshot][stride][2]
+void batch_step_into(Mesh* mp, int it, const float* d, const float* v, const float* a, float dto2_prev, float* d_out,
+                     float* v_out, float* a_out, float deltat, float deltatsqover2, float deltatover2);
 
 // applies a deferred corrector now (no-op when none is pending)
 void flush_correctors(Mesh* mp);
@@ -187,6 +191,13 @@ T* device_copy(const T* host, size_t n, hipStream_t s = nullptr) {
 }
 
 template <typename T>
+T* device_alloc(size_t n) {  // uninitialised
+  T* d = nullptr;
+  check_hip(hipMalloc(&d, (n ? n : 1) * sizeof(T)), "hipMalloc");
+  return d;
+}
+
+template <typename T>
 T* device_zeros(size_t n) {
   T* d = nullptr;
   size_t bytes = (n ? n : 1) * sizeof(T);
@@ -212,6 +223,11 @@ void launch_newmark_corrector(float* v, float* a, const float* rmx, const float*
 void launch_newmark_corrector_predictor(float* d, float* v, float* a, const float* rmx, const float* rmz, int nglob,
                                        float dto2_prev, float dt, float dt2o2, float dto2, hipStream_t s,
                                        int nshot = 1, long shot_stride = 0);
+// out of place: (d, v, a) with the corrector pending -> (d_out, v_out, 0); the two states must not overlap
+void launch_newmark_corrector_predictor_to(const float* d, const float* v, const float* a, float* d_out, float* v_out,
+                                          float* a_out, const float* rmx, const float* rmz, int nglob,
+                                          float dto2_prev, float dt, float dt2o2, float dto2, hipStream_t s,
+                                          int nshot = 1, long shot_stride = 0);
 void launch_elastic_forces(const Mesh* mp, int first, int nelem, int atomic, const float* displ, float* accel);
 // nshot wavefields at a stride of shot_stride points (float2) in displ and accel
 void launch_elastic_forces_patch(const Mesh* mp, int first_patch, int npatch, const float* displ, float* accel,
