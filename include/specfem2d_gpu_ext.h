@@ -31,6 +31,13 @@ int sf2d_mesh_colours(long* Mesh_pointer, int iphase);
    6 point numbering not a permutation, 7 owned ranges overlap, 8 halo list, 9 unused patch point). */
 int sf2d_plan_validate(int nspec, int nglob, const int* ibool_1based, int np, const int* phase_lists, int mode,
                        int* stats);
+/* 1 when every patch of the coloured-patch plan for this mesh is a chain (element k's right edge is element
+   k+1's left edge, no other point shared inside the patch): such meshes use the row-lane force kernel.
+   Host only, same arguments as sf2d_plan_validate. */
+int sf2d_plan_chains(int nspec, int nglob, const int* ibool_1based, int np, const int* phase_lists);
+/* 1 when the handle's element forces run on the row-lane kernel for chain patches (0 with SF2D_NO_CHAIN set
+   or for other meshes) */
+int sf2d_mesh_chain(long* Mesh_pointer);
 /* device pointer of a field: 0 displ 1 veloc 2 accel 3 b_displ 4 b_veloc 5 b_accel
    6 rho_kl 7 mu_kl 8 kappa_kl (NULL when not allocated).  On the device, point arrays are in the
    library's internal point numbering and kernels in its element order; the transfer entry points

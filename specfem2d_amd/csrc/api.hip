@@ -134,7 +134,27 @@ struct Plan {
   // per device position and GLL point: index of its point in the patch's point list (low 8 bits) and its
   // rank among the element points of the patch that share that point (bits 8-9)
   std::vector<uint16_t> slot;
+  // every patch is a chain: element k's right edge (i = 4) is element k+1's left edge (i = 0), and no other
+  // point is shared inside the patch
+  bool chains = false;
 };
+
+// whether elements elems[k0..k1) form a chain (see Plan::chains)
+bool is_chain(const std::vector<int>& ibool, const std::vector<int>& elems, int k0, int k1) {
+  std::vector<int> pts;
+  for (int k = k0; k < k1; k++) {
+    const int* a = ibool.data() + (size_t)elems[k] * NGLL2;
+    if (k + 1 < k1) {
+      const int* b = ibool.data() + (size_t)elems[k + 1] * NGLL2;
+      for (int j = 0; j < NGLL; j++)
+        if (a[j * NGLL + NGLL - 1] != b[j * NGLL]) return false;
+    }
+    pts.insert(pts.end(), a, a + NGLL2);
+  }
+  std::sort(pts.begin(), pts.end());
+  const size_t distinct = std::unique(pts.begin(), pts.end()) - pts.begin();
+  return distinct == (size_t)(NGLL * (4 * (k1 - k0) + 1));
+}
 
 int desc_counts(int ne, int nown, int nhalo) { return ne | (nown << 8) | (nhalo << 20); }
 
@@ -159,6 +179,7 @@ Plan build_plan(const std::vector<int>& ibool, int nspec, int nglob, const int* 
   if (mode == 0) {
     P.slot.assign((size_t)nspec * NGLL2, 0);
     P.newid.assign((size_t)nglob, -1);
+    P.chains = true;
   }
   int next_id = 0;
   for (int ph = 0; ph < 2; ph++) {
@@ -185,6 +206,7 @@ Plan build_plan(const std::vector<int>& ibool, int nspec, int nglob, const int* 
     // coloured patches; ownership and numbering in phase-list order
     const std::vector<int> pstart = make_patches(ibool, elems, nglob);
     const int np_ph = (int)pstart.size() - 1;
+    for (int q = 0; q < np_ph && P.chains; q++) P.chains = is_chain(ibool, elems, pstart[q], pstart[q + 1]);
     std::vector<std::vector<int>> pnodes(np_ph), phalo(np_ph);
     std::vector<int> own0(np_ph), nown(np_ph);
     for (int q = 0; q < np_ph; q++) {
@@ -281,6 +303,7 @@ void sort_elements_by_colour(Mesh* mp, const int* phase_lists, int np) {
     mp->d_halo_nodes = device_copy(P.halo.data(), P.halo.size());
     mp->d_elem_slot = device_copy(P.slot.data(), P.slot.size());
     mp->point_id = std::move(P.newid);
+    mp->chain = P.chains && !getenv("SF2D_NO_CHAIN");
   }
 
   // move the per-element device arrays
@@ -1177,6 +1200,14 @@ extern "C" int sf2d_mesh_affine_elements(long* Mesh_pointer) {
   Mesh* mp = handle(Mesh_pointer);
   return mp->no_affine ? 0 : mp->n_affine;
 }
+
+// whether every patch of the plan for this mesh is a chain of elements (host only)
+extern "C" int sf2d_plan_chains(int nspec, int nglob, const int* ibool_1based, int np, const int* phase_lists) {
+  std::vector<int> ib = to_zero_based(ibool_1based, (size_t)nspec * NGLL2, 1, nglob, "ibool");
+  return build_plan(ib, nspec, nglob, phase_lists, np, 0).chains ? 1 : 0;
+}
+
+extern "C" int sf2d_mesh_chain(long* Mesh_pointer) { return handle(Mesh_pointer)->chain; }
 
 extern "C" int sf2d_mesh_colours(long* Mesh_pointer, int iphase) {
   Mesh* mp = handle(Mesh_pointer);

@@ -434,6 +434,179 @@ __global__ __launch_bounds__(FORCE_THREADS) void k_elastic_forces_patch(
 #undef hwj
 }
 
+// Chain patches: the patch's elements form a row in which element k's right edge (i = 4) is element k+1's left
+// edge (i = 0) and no other point is shared inside the patch (the structured meshes of the internal mesher,
+// checked on the host when the plan is built).  One wave per (patch, shot) item; lane 5 k + j holds row j of
+// element k (its five points (i, j), i = 0..4) in registers:
+//   * xi-derivatives and the xi-flux contraction run along the lane's own row, in registers;
+//   * the gamma direction goes through two LDS transposes per operand: the lane writes its row, reads column j,
+//     contracts along it and writes the result back by columns, then reads its row again;
+//   * the force at the shared edge point (4, j) moves one element to the right with a cross-lane shuffle and is
+//     added at (0, j), so every point of the patch is updated by exactly one lane: no force cells, no atomics.
+// The arithmetic per point is the element operator of the other kernels (Kernel_2_noatt_iso_impl:
+// compute_forces_viscoelastic_cuda.cu:163-215, :501-565).
+constexpr int CHAIN_LANES = 64;
+__global__ __launch_bounds__(CHAIN_LANES) void k_elastic_forces_chain(
+    int first_patch, int npatch, int nshot, long shot_stride, const int4* __restrict__ desc,
+    const int* __restrict__ ibool, const float4* __restrict__ geom, const float4* __restrict__ geom_elem,
+    const float2* __restrict__ moduli, const float2* __restrict__ displ, float2* __restrict__ accel,
+    const Basis basis) {
+  // [element][row][column], rows padded to 6; element FORCE_EPB is scratch for lanes past the patch
+  __shared__ float2 s_t[FORCE_EPB + 1][NGLL][NGLL + 1];
+  __shared__ float2 s_v[FORCE_EPB + 1][NGLL][NGLL + 1];
+  __shared__ float s_hp[NGLL2], s_hpw[NGLL2];
+  const int lane = threadIdx.x;
+  if (lane < NGLL2) {
+    s_hp[lane] = basis.hp[lane];
+    s_hpw[lane] = basis.hpw[lane];
+  }
+  // items in XCD-contiguous blocks: the shots of a patch run on one XCD and share its L2
+  const int nw = gridDim.x;
+  const int item = (blockIdx.x % NXCD) * (nw / NXCD) + blockIdx.x / NXCD;
+  const int nitem = npatch * nshot;
+  if (item >= nitem) return;  // uniform over the wave
+  const int4 d = desc[first_patch + item / nshot];
+  const long base = (long)(item % nshot) * shot_stride;
+  const int ne = patch_ne(d.w);
+  const int k = min(lane / NGLL, FORCE_EPB);  // element of the patch (FORCE_EPB: lanes past the patch)
+  const int j = lane % NGLL;
+  const bool active = lane / NGLL < ne;
+  const int e = d.x + min(k, ne - 1);
+  const unsigned l0 = (unsigned)e * NGLL2 + j * NGLL;  // first point of this lane's row
+  int g[NGLL];
+  float2 u[NGLL], mod[NGLL];
+  float4 geo[NGLL];
+#pragma unroll
+  for (int i = 0; i < NGLL; i++) {
+    g[i] = ld_off(ibool, 4u * (l0 + i));
+    mod[i] = ld_off(moduli, 8u * (l0 + i));
+  }
+  if (patch_affine(d.w)) {
+    const float4 ge = ld_off(geom_elem, 16u * (unsigned)e);
+#pragma unroll
+    for (int i = 0; i < NGLL; i++) geo[i] = ge;
+  } else {
+#pragma unroll
+    for (int i = 0; i < NGLL; i++) geo[i] = ld_off(geom, 16u * (l0 + i));
+  }
+  const float2* ds = displ + base;
+#pragma unroll
+  for (int i = 0; i < NGLL; i++) u[i] = ld_off(ds, 8u * (unsigned)g[i]);
+  float2* as = accel + base;
+  const bool last = k == ne - 1;
+  float2 a[NGLL];
+#pragma unroll
+  for (int i = 0; i < NGLL - 1; i++) a[i] = ld_off((const float2*)as, 8u * (unsigned)g[i]);
+  a[NGLL - 1] = last ? ld_off((const float2*)as, 8u * (unsigned)g[NGLL - 1]) : make_float2(0.f, 0.f);
+
+  // row j of u into the tile, gamma-derivatives by columns
+#pragma unroll
+  for (int i = 0; i < NGLL; i++) s_t[k][j][i] = u[i];
+  __syncthreads();
+  {
+    const int c = j;  // this lane's column
+    float2 col[NGLL];
+#pragma unroll
+    for (int m = 0; m < NGLL; m++) col[m] = s_t[k][m][c];
+#pragma unroll
+    for (int r = 0; r < NGLL; r++) {
+      float2 dg = make_float2(0.f, 0.f);
+#pragma unroll
+      for (int m = 0; m < NGLL; m++) {
+        const float h = s_hp[r * NGLL + m];
+        dg.x += col[m].x * h;
+        dg.y += col[m].y * h;
+      }
+      s_v[k][r][c] = dg;  // d u / d gamma at point (c, r)
+    }
+  }
+  __syncthreads();
+  // stress and fluxes at the row's points
+  float2 t1[NGLL], t2[NGLL];
+#pragma unroll
+  for (int i = 0; i < NGLL; i++) {
+    float dux_dxi = 0.f, duz_dxi = 0.f;
+#pragma unroll
+    for (int m = 0; m < NGLL; m++) {
+      const float h = s_hp[i * NGLL + m];
+      dux_dxi += u[m].x * h;
+      duz_dxi += u[m].y * h;
+    }
+    const float2 dg = s_v[k][j][i];
+    const float xixl = geo[i].x, xizl = geo[i].y, gaxl = geo[i].z, gazl = geo[i].w;
+    const float dux_dxl = dux_dxi * xixl + dg.x * gaxl;
+    const float dux_dzl = dux_dxi * xizl + dg.x * gazl;
+    const float duz_dxl = duz_dxi * xixl + dg.y * gaxl;
+    const float duz_dzl = duz_dxi * xizl + dg.y * gazl;
+    const float jac = 1.f / (xixl * gazl - gaxl * xizl);
+    const float kap = mod[i].x, mu = mod[i].y;
+    const float lambdal = kap - mu;
+    const float lp2m = kap + mu;
+    const float sxx = lp2m * dux_dxl + lambdal * duz_dzl;
+    const float szz = lp2m * duz_dzl + lambdal * dux_dxl;
+    const float sxz = mu * (duz_dxl + dux_dzl);
+    t1[i] = make_float2(jac * (sxx * xixl + sxz * xizl), jac * (sxz * xixl + szz * xizl));
+    t2[i] = make_float2(jac * (sxx * gaxl + sxz * gazl), jac * (sxz * gaxl + szz * gazl));
+  }
+  __syncthreads();  // every lane has read its derivatives and the tile's u
+#pragma unroll
+  for (int i = 0; i < NGLL; i++) s_t[k][j][i] = t2[i];
+  __syncthreads();
+  {
+    const int c = j;
+    float2 col[NGLL];
+#pragma unroll
+    for (int m = 0; m < NGLL; m++) col[m] = s_t[k][m][c];
+#pragma unroll
+    for (int r = 0; r < NGLL; r++) {
+      float2 sg = make_float2(0.f, 0.f);
+#pragma unroll
+      for (int m = 0; m < NGLL; m++) {
+        const float h = s_hpw[m * NGLL + r];
+        sg.x += col[m].x * h;
+        sg.y += col[m].y * h;
+      }
+      s_v[k][r][c] = sg;  // gamma-flux contraction at point (c, r)
+    }
+  }
+  __syncthreads();
+  float2 f[NGLL];
+  const float wj = basis.w[j];
+#pragma unroll
+  for (int i = 0; i < NGLL; i++) {
+    float sx = 0.f, sz = 0.f;
+#pragma unroll
+    for (int m = 0; m < NGLL; m++) {
+      const float h = s_hpw[m * NGLL + i];
+      sx += t1[m].x * h;
+      sz += t1[m].y * h;
+    }
+    const float2 sg = s_v[k][j][i];
+    const float wi = basis.w[i];
+    f[i] = make_float2(-(wj * sx + wi * sg.x), -(wj * sz + wi * sg.y));
+  }
+  // the right edge's force moves to the next element's left edge
+  const float fx4 = __shfl_up(f[NGLL - 1].x, NGLL, CHAIN_LANES);
+  const float fz4 = __shfl_up(f[NGLL - 1].y, NGLL, CHAIN_LANES);
+  if (k > 0) {
+    f[0].x += fx4;
+    f[0].y += fz4;
+  }
+  if (active) {
+#pragma unroll
+    for (int i = 0; i < NGLL - 1; i++) {
+      a[i].x += f[i].x;
+      a[i].y += f[i].y;
+      *reinterpret_cast<float2*>(reinterpret_cast<char*>(as) + 8u * (unsigned)g[i]) = a[i];
+    }
+    if (last) {
+      a[NGLL - 1].x += f[NGLL - 1].x;
+      a[NGLL - 1].y += f[NGLL - 1].y;
+      *reinterpret_cast<float2*>(reinterpret_cast<char*>(as) + 8u * (unsigned)g[NGLL - 1]) = a[NGLL - 1];
+    }
+  }
+}
+
 // per element: geometry of point 0, and whether it stands for all 25 points: every value within 2^-22 of the
 // element's largest geometry entry of the value at point 0.  Meshers leave cancellation residues (1e-18 against
 // 2e-2) where a parallelogram's entries are constant, so exact equality would almost never hold; within this
@@ -815,6 +988,13 @@ void launch_elastic_forces_patch(const Mesh* mp, int first_patch, int npatch, co
     if (getenv("SF2D_VERBOSE")) fprintf(stderr, "k_elastic_forces_patch: %d CUs, %d workgroups resident\n", cus, resident);
   }
   const int all = (npatch * nshot + NXCD - 1) / NXCD * NXCD;
+  if (mp->chain) {  // one wave per item
+    hipLaunchKernelGGL(k_elastic_forces_chain, dim3(all), dim3(CHAIN_LANES), 0, mp->stream, first_patch, npatch,
+                       nshot, shot_stride, (const int4*)mp->d_patch_desc, mp->d_ibool, mp->d_geom, mp->d_geom_elem,
+                       mp->d_moduli, (const float2*)displ, (float2*)accel, mp->basis);
+    check_hip(hipGetLastError(), "k_elastic_forces_chain launch");
+    return;
+  }
   const int nwg = resident > 0 ? std::min(resident, all) : all;
   hipLaunchKernelGGL(k_elastic_forces_patch, dim3(nwg), dim3(FORCE_THREADS), 0, mp->stream, first_patch, npatch,
                      nshot, shot_stride, (const int4*)mp->d_patch_desc, mp->d_halo_nodes, mp->d_elem_slot, mp->d_geom,

@@ -78,6 +78,8 @@ struct Mesh {
   float4* d_geom_elem = nullptr;     // [nspec] geometry of point 0; the whole element's when it is affine
   int n_affine = 0;                  // elements whose 25 points carry bitwise the same geometry
   int no_affine = 0;                 // SF2D_NO_AFFINE: always read the per-point geometry
+  int chain = 0;                     // every patch is a chain of elements: the row-lane force kernel (SF2D_NO_CHAIN
+                                     // keeps the general patch kernel)
   std::vector<int> point_id;         // device id of each caller point (empty: identity)
   std::vector<int> h_ibool;          // 0-based, ispec order, kept until the elements are sorted
   std::vector<int> pos_of_ispec, ispec_of_pos;

@@ -100,6 +100,8 @@ def load_libraries():
     gpu.sf2d_mesh_stream.restype = P
     gpu.sf2d_mesh_affine_elements.argtypes = [C.POINTER(C.c_long)]
     gpu.sf2d_mesh_affine_elements.restype = C.c_int
+    gpu.sf2d_mesh_chain.argtypes = [C.POINTER(C.c_long)]
+    gpu.sf2d_mesh_chain.restype = C.c_int
     gpu.sf2d_mesh_colours.argtypes = [C.POINTER(C.c_long), C.c_int]
     gpu.sf2d_mesh_colours.restype = C.c_int
     gpu.sf2d_mesh_synchronize.argtypes = [C.POINTER(C.c_long)]
@@ -207,6 +209,12 @@ class Simulation:
         self._need_device()
         mp = C.c_long(self.mesh_pointer)
         return self.gpu.sf2d_mesh_affine_elements(C.byref(mp))
+
+    def chain_kernel(self) -> bool:
+        """Whether the element forces run on the row-lane kernel for chain patches."""
+        self._need_device()
+        mp = C.c_long(self.mesh_pointer)
+        return bool(self.gpu.sf2d_mesh_chain(C.byref(mp)))
 
     def colours(self, iphase: int = 2) -> int:
         mp = C.c_long(self.mesh_pointer)

@@ -84,3 +84,35 @@ def test_plan_invariants_shuffled_and_split_lists(mode):
     assert rc == 0
     if mode != 2:
         assert stats[0] >= 1 and stats[1] >= 4
+
+
+def chains(ibool, nglob, lists):
+    gpu, _ = driver.load_libraries()
+    ib = np.ascontiguousarray(ibool.reshape(-1), dtype=np.int32)
+    ph = np.ascontiguousarray(lists.reshape(-1), dtype=np.int32)
+    f = gpu.sf2d_plan_chains
+    f.restype = C.c_int
+    return f(C.c_int(ibool.shape[0]), C.c_int(nglob), ib.ctypes.data_as(C.c_void_p), C.c_int(lists.shape[1]),
+             ph.ctypes.data_as(C.c_void_p))
+
+
+@pytest.mark.parametrize("nx,expect", [(80, 1), (12, 1), (13, 1), (10, 1), (8, 0), (1, 0)])
+def test_chain_patches_on_structured_meshes(nx, expect):
+    """Rows of at least ten elements split into strips that are chains (the row-lane force kernel applies);
+    narrower meshes make patches that wrap into the next row (a 2-D block) and fall back to the general
+    patch kernel.  The chain property itself is re-derived here from ibool: along a strip of consecutive
+    elements of one row, element k's points (4, j) are element k+1's points (0, j)."""
+    cfg = config.small_test_mesh(nx=nx, nz=4, nstep=2)
+    o = OracleSim(cfg, "sp")
+    ib = o.ibool()
+    assert chains(ib, o.nglob, inner_only(o.nspec)) == expect
+    if expect:
+        e = np.arange(o.nspec).reshape(cfg.nz, cfg.nx)
+        left, right = e[:, 1:].reshape(-1), e[:, :-1].reshape(-1)
+        b = ib.reshape(-1, 5, 5)
+        np.testing.assert_array_equal(b[right][:, :, 4], b[left][:, :, 0])
+    # a shuffled element list breaks the strips
+    rng = np.random.default_rng(3)
+    lists = inner_only(o.nspec)
+    lists[1] = rng.permutation(lists[1])
+    assert chains(ib, o.nglob, lists) == 0
