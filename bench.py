@@ -15,7 +15,8 @@ so the whole mesh is elastic with Stacey edges.
 The same line carries the second headline figure, the FWI gradient time per shot: the exact
 discrete gradient of an L2 waveform misfit (forward run with a checkpoint every 100 steps,
 segment recomputation and reverse sweep, all on the GPU) on the single-shot 800 x 400-element,
-2000-step adjoint-gradient workload, observed data from a +5 % vs anomaly.
+2000-step adjoint-gradient workload, observed data from a +5 % vs anomaly; and batched against
+plain on the multi-shot 600 x 300-element, 1500-step workload (shots along the free surface).
 
 Prints one JSON line with the throughput, the roofline of the element-force kernel (HIP
 events on the library stream around the force launches of every step inside the timed region;
@@ -39,11 +40,13 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
 
-def kernel_min_bytes(nspec: int, nglob: int, n_affine: int) -> int:
-    """Minimal bytes of the patch kernel's representation per step: per element 25 x (kappa, mu) = 200 B, the
-    25 slot/rank words (50 B), the geometry (16 B for an affine element, 400 B otherwise); per unique point
-    displ read and accel read-modify-write (24 B)."""
-    return 250 * nspec + 16 * n_affine + 400 * (nspec - n_affine) + 24 * nglob
+def kernel_min_bytes(nspec: int, nglob: int, n_affine: int, chain: bool) -> int:
+    """Minimal bytes of the force kernel's representation per step: per element 25 x (kappa, mu) = 200 B, the
+    25 point indices (4 B each for the row-lane chain kernel, 2 B slot/rank words for the general patch kernel),
+    the geometry (16 B for an affine element, 400 B otherwise); per unique point displ read and accel
+    read-modify-write (24 B)."""
+    per_elem = 300 if chain else 250
+    return per_elem * nspec + 16 * n_affine + 400 * (nspec - n_affine) + 24 * nglob
 
 
 def force_kernel_bytes(nspec: int, nglob: int) -> int:
@@ -123,6 +126,46 @@ def fwi_gradient(repeats: int = 2):
             "method": "exact discrete adjoint, checkpointed recompute, FP64 gradient accumulation", "shots": 1}
 
 
+def fwi_gradient_batched(nshot: int = 32, repeats: int = 2):
+    """Seconds per shot of the exact FWI gradient on the multi-shot 600 x 300-element, 1500-step workload,
+    plain (one shot through the reference's entry points, checkpoint every 100 steps) against batched (nshot
+    shots advanced together, checkpoints every 39 steps, segments recomputed into their state slots).
+    Observed data: the shots run in the random-field model; synthetics: the homogeneous background."""
+    from specfem2d_amd import config
+    from specfem2d_amd.driver import ShotBatch, Simulation
+    true_cfg = config.batched_fwi_600x300(true_model=True)
+    cfg = config.batched_fwi_600x300()
+    shots = config.surface_shots(cfg, nshot)
+    b = ShotBatch(true_cfg, shots)
+    obs = b.forward()
+    b.close()
+    sim = Simulation(cfg.replace(sources=shots[0]))
+    plain = []
+    for _ in range(repeats):  # the first run includes the allocation of the checkpoint buffers
+        t0 = time.perf_counter()
+        sim.gradient_forward(obs[0], chunk=100)
+        sim.gradient_backward()
+        plain.append(time.perf_counter() - t0)
+    sim.close()
+    chunk = int(round(np.sqrt(cfg.nstep)))
+    b = ShotBatch(cfg, shots)
+    batched = []
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        chi, _ = b.gradient_forward(obs, chunk=chunk)
+        g = b.gradient_backward()
+        batched.append(time.perf_counter() - t0)
+    b.close()
+    gnorm = float(np.linalg.norm(g["vs"]))
+    if not (np.isfinite(chi) and chi > 0 and np.isfinite(gnorm) and gnorm > 0):
+        raise RuntimeError("degenerate batched gradient")
+    sp, sb = min(plain), min(batched) / nshot
+    return {"workload": "batched_fwi_600x300", "nstep": cfg.nstep, "nglob": cfg.nglob, "shots_per_batch": nshot,
+            "plain_s_per_shot": sp, "batched_s_per_shot": sb, "speedup_per_shot": sp / sb,
+            "checkpoint_every": {"plain": 100, "batched": chunk}, "misfit_sum": chi, "grad_vs_l2": gnorm,
+            "note": "64 shots = 2 batches of 32 (checkpoint memory)"}
+
+
 def read_traffic(name: str):
     """HBM bytes per force launch (FETCH_SIZE + WRITE_SIZE, calibrated against a plain copy) from the
     committed rocprofv3 PMC summary of this workload, if present."""
@@ -157,6 +200,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-gradient", action="store_true", help="skip the FWI gradient time per shot")
+    ap.add_argument("--batch-shots", type=int, default=32,
+                    help="shots per batch of the multi-shot FWI gradient (32 fill about 170 GB of checkpoints)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="after the timed region, run a plain copy of this size (counter calibration)")
     args = ap.parse_args()
@@ -192,7 +237,9 @@ def main():
     value = nglob * args.steps / wall
     # one force launch per colour; the events bracket all launches of a step
     launches = sim.colours(2)
-    n_affine = sim.affine_elements() if os.environ.get("SF2D_ASSEMBLY", "patch") == "patch" else 0
+    patch_mode = os.environ.get("SF2D_ASSEMBLY", "patch") == "patch"
+    n_affine = sim.affine_elements() if patch_mode else 0
+    chain = patch_mode and sim.chain_kernel()
     fk_ms = timing["forces_ms"] / args.steps / launches
     fk_bytes = force_kernel_bytes(nspec, nglob) / launches
     achieved = fk_bytes / (fk_ms * 1e-3) / 1e9
@@ -223,7 +270,7 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_elastic_forces_patch" if os.environ.get("SF2D_ASSEMBLY", "patch") == "patch"
+            "kernel": ("k_elastic_forces_chain" if chain else "k_elastic_forces_patch") if patch_mode
                       else "k_elastic_forces",
             "launches_per_step": launches,
             "achieved": achieved,
@@ -235,7 +282,7 @@ def main():
             "avg_launch_ms": fk_ms,
             # the bytes this kernel's own representation needs at minimum: affine elements carry one 16-byte
             # geometry record instead of 25 x 16 bytes (what it reads, not the formula above)
-            "kernel_min_bytes_per_launch": kernel_min_bytes(nspec, nglob, n_affine) / launches,
+            "kernel_min_bytes_per_launch": kernel_min_bytes(nspec, nglob, n_affine, chain) / launches,
         },
         "gpu_loop_ms_events": timing["loop_ms"],
         "setup_s": t_setup,
@@ -245,6 +292,7 @@ def main():
         copy_calibration(args.copy_calibration_mib)
     if not args.no_gradient:
         out["fwi_gradient"] = fwi_gradient()
+        out["fwi_gradient_batched"] = fwi_gradient_batched(args.batch_shots)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.workload, args.cpu_seconds)
     print(json.dumps(out))

@@ -33,8 +33,11 @@ int sf2d_plan_validate(int nspec, int nglob, const int* ibool_1based, int np, co
                        int* stats);
 /* 1 when every patch of the coloured-patch plan for this mesh is a chain (element k's right edge is element
    k+1's left edge, no other point shared inside the patch): such meshes use the row-lane force kernel.
-   Host only, same arguments as sf2d_plan_validate. */
-int sf2d_plan_chains(int nspec, int nglob, const int* ibool_1based, int np, const int* phase_lists);
+   Host only, same arguments as sf2d_plan_validate; first_writes (if not NULL) receives the number of element
+   points whose update is the first of its point in a step (first-touch assembly: nglob when every point has
+   exactly one, 0 when the plan cannot use it). */
+int sf2d_plan_chains(int nspec, int nglob, const int* ibool_1based, int np, const int* phase_lists,
+                     int* first_writes);
 /* 1 when the handle's element forces run on the row-lane kernel for chain patches (0 with SF2D_NO_CHAIN set
    or for other meshes) */
 int sf2d_mesh_chain(long* Mesh_pointer);

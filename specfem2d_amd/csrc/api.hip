@@ -137,6 +137,9 @@ struct Plan {
   // every patch is a chain: element k's right edge (i = 4) is element k+1's left edge (i = 0), and no other
   // point is shared inside the patch
   bool chains = false;
+  // chains: per device position and GLL point, whether the chain kernel's update of that point is the first of
+  // the step (the earliest colour, in launch order, of the patches that contain the point)
+  std::vector<uint8_t> first;
 };
 
 // whether elements elems[k0..k1) form a chain (see Plan::chains)
@@ -270,6 +273,37 @@ Plan build_plan(const std::vector<int>& ibool, int nspec, int nglob, const int* 
       P.colour_count[ph][c] = (int)P.desc.size() / 4 - P.colour_start[ph][c];
     }
   }
+  if (mode == 0 && P.chains) {
+    // launch order of the colours: phase 1's, then phase 2's; a lane updates points i = 0..3 of its row, and
+    // i = 4 in the patch's last element
+    P.first.assign((size_t)nspec * NGLL2, 0);
+    std::vector<int> earliest((size_t)nglob, 1 << 30);
+    const int npatch = (int)P.desc.size() / 4;
+    std::vector<int> launch(npatch, 0);
+    for (int ph = 0, base = 0; ph < 2; base += P.ncolour[ph], ph++)
+      for (int c = 0; c < P.ncolour[ph]; c++)
+        for (int q = P.colour_start[ph][c]; q < P.colour_start[ph][c] + P.colour_count[ph][c]; q++) launch[q] = base + c;
+    for (int pass = 0; pass < 2; pass++)
+      for (int q = 0; q < npatch; q++) {
+        const int pos0 = P.desc[4 * q], ne = P.desc[4 * q + 3] & 0xf;
+        for (int k = 0; k < ne; k++)
+          for (int j = 0; j < NGLL; j++)
+            for (int i = 0; i < (k == ne - 1 ? NGLL : NGLL - 1); i++) {
+              const size_t l = (size_t)(pos0 + k) * NGLL2 + j * NGLL + i;
+              const int g = ibool[(size_t)order[pos0 + k] * NGLL2 + j * NGLL + i];
+              if (pass == 0)
+                earliest[g] = std::min(earliest[g], launch[q]);
+              else
+                P.first[l] = earliest[g] == launch[q];
+            }
+      }
+    // first-touch assembly needs every point written by some force launch of the step
+    for (int g = 0; g < nglob; g++)
+      if (earliest[g] == (1 << 30)) {
+        P.first.clear();
+        break;
+      }
+  }
   for (int e = 0; e < nspec; e++)
     if (!placed[e]) order.push_back(e);  // elements outside the elastic phase lists keep their relative order
   if (mode == 0) {
@@ -282,6 +316,33 @@ Plan build_plan(const std::vector<int>& ibool, int nspec, int nglob, const int* 
 
 // Applies the plan: schedule into the handle, per-element device arrays moved into the plan's order, ibool
 // rewritten in the plan's point numbering (patch mode).
+// Chain-kernel point ids with the first-touch flags, and the special points of first-touch assembly (points that
+// the Stacey edges and the sources also update: the predictor clears them and every force update adds to them).
+// Called when the plan is built and again when the Stacey edges arrive.
+void upload_chain_ids(Mesh* mp) {
+  if (!mp->chain) return;
+  const size_t ne = (size_t)mp->nspec * NGLL2;
+  std::vector<char> special((size_t)mp->nglob, 0);
+  for (int e : mp->h_src_ispec) {
+    const int pos = mp->pos_of_ispec[e];
+    for (int p = 0; p < NGLL2; p++) special[mp->h_chain_ids[(size_t)pos * NGLL2 + p]] = 1;
+  }
+  for (int g : mp->h_stacey_points) special[g] = 1;
+  std::vector<int> ids(mp->h_chain_ids);
+  std::vector<int> list;
+  if (mp->first_touch) {
+    for (size_t l = 0; l < ne; l++)
+      if (mp->h_first[l] && !special[ids[l]]) ids[l] |= Mesh::FIRST_TOUCH;
+    for (int g = 0; g < mp->nglob; g++)
+      if (special[g]) list.push_back(g);
+  }
+  if (!mp->d_chain_ids) check_hip(hipMalloc(&mp->d_chain_ids, ne * sizeof(int)), "hipMalloc");
+  check_hip(hipMemcpy(mp->d_chain_ids, ids.data(), ne * sizeof(int), hipMemcpyHostToDevice), "chain ids H2D");
+  device_free(mp->d_special);
+  mp->nspecial = (int)list.size();
+  if (!list.empty()) mp->d_special = device_copy(list.data(), list.size());
+}
+
 void sort_elements_by_colour(Mesh* mp, const int* phase_lists, int np) {
   const int nspec = mp->nspec;
   Plan P = build_plan(mp->h_ibool, nspec, mp->nglob, phase_lists, np, mp->assembly);
@@ -336,6 +397,12 @@ void sort_elements_by_colour(Mesh* mp, const int* phase_lists, int np) {
       for (int p = 0; p < NGLL2; p++) dst[p] = mp->point_id.empty() ? src[p] : mp->point_id[src[p]];
     }
     check_hip(hipMemcpy(mp->d_ibool, ib.data(), ne * sizeof(int), hipMemcpyHostToDevice), "ibool H2D");
+    if (mp->chain) {
+      mp->h_chain_ids = std::move(ib);
+      mp->h_first = std::move(P.first);
+      mp->first_touch = mp->simulation_type == 1 && !mp->h_first.empty() && !getenv("SF2D_NO_FIRST_TOUCH");
+      upload_chain_ids(mp);
+    }
   }
   {
     std::vector<int> el(nspec);
@@ -409,13 +476,16 @@ void to_ispec_order(const Mesh* mp, const float* dev_order, float* out) {
 }
 
 // acc -= K u over the elements of phase ph (0 outer, 1 inner), with the handle's assembly scheme
-void apply_element_forces(const Mesh* mp, int ph, const float* u, float* acc, int nshot, long shot_stride) {
+void apply_element_forces(const Mesh* mp, int ph, const float* u, float* acc, int nshot, long shot_stride,
+                          int first_touch) {
   if (mp->phase_count[ph] <= 0) return;
   if (mp->assembly == 0) {  // patches: every shot of a colour in one launch
     for (int c = 0; c < mp->ncolour[ph]; c++)
-      launch_elastic_forces_patch(mp, mp->colour_start[ph][c], mp->colour_count[ph][c], u, acc, nshot, shot_stride);
+      launch_elastic_forces_patch(mp, mp->colour_start[ph][c], mp->colour_count[ph][c], u, acc, nshot, shot_stride,
+                                  first_touch);
     return;
   }
+  if (first_touch) fail("first-touch assembly needs the chain kernel");
   for (int s = 0; s < nshot; s++) {
     const float* us = u + 2 * s * shot_stride;
     float* as = acc + 2 * s * shot_stride;
@@ -642,6 +712,14 @@ void prepare_stacey_device_(long* Mesh_pointer, int* ACOUSTIC_SIMULATION, int* E
     if (v < 1 || v > counts[side]) fail("ib index %d of face %d outside 1..%d", v, f + 1, counts[side]);
     ib[f] = v - 1;
   }
+  if (mp->chain) {  // points of the absorbing edges, for first-touch assembly
+    for (int f = 0; f < nf; f++)
+      for (int k = 0; k < NGLL; k++) {
+        const int ii = ij[((size_t)f * NGLL + k) * 2], jj = ij[((size_t)f * NGLL + k) * 2 + 1];
+        if (ii >= 0 && jj >= 0) mp->h_stacey_points.push_back(mp->h_chain_ids[(size_t)isp[f] * NGLL2 + jj * NGLL + ii]);
+      }
+    upload_chain_ids(mp);
+  }
   mp->d_abs_ispec = device_copy(isp.data(), isp.size());
   mp->d_abs_ij = device_copy(ij.data(), ij.size());
   mp->d_abs_normal = device_copy(h_abs_boundary_normal, (size_t)nf * NGLL * 2);
@@ -706,6 +784,8 @@ void prepare_cleanup_device_(long* Mesh_pointer, int* ACOUSTIC_SIMULATION, int* 
   device_free(mp->d_geom);
   device_free(mp->d_moduli);
   device_free(mp->d_geom_elem);
+  device_free(mp->d_chain_ids);
+  device_free(mp->d_special);
   if (mp->grad) sf2d_grad_release(Mesh_pointer);
   if (mp->batch) sf2d_batch_release(Mesh_pointer);
   (void)hipStreamDestroy(mp->stream);
@@ -715,6 +795,14 @@ void prepare_cleanup_device_(long* Mesh_pointer, int* ACOUSTIC_SIMULATION, int* 
 
 // ---------------------------------------------------------------------------
 // transfers (transfer_fields_cuda.cu)
+// First-touch assembly leaves accel incomplete from update_displacement_cuda until the last force launch of the
+// step; the reference's sequence never reads it there.
+static void check_accel_complete(const Mesh* mp, const char* what) {
+  if (mp->accel_partial)
+    fail("%s: accel is read between update_displacement_cuda and compute_forces_viscoelastic_cuda (iphase 2); "
+         "set SF2D_NO_FIRST_TOUCH for this call sequence", what);
+}
+
 static void copy_field(float* dst_dev, const realw* src_host, int size, Mesh* mp, const char* what) {
   if (size != 2 * mp->nglob) fail("%s: size %d != 2*nglob = %d", what, size, 2 * mp->nglob);
   if (!dst_dev) fail("%s: field not allocated on the device", what);
@@ -729,6 +817,7 @@ static void fetch_field(realw* dst_host, const float* src_dev, int size, Mesh* m
 void transfer_fields_el_to_device_(int* size, realw* displ, realw* veloc, realw* accel, long* Mesh_pointer) {
   Mesh* mp = handle(Mesh_pointer);
   flush_correctors(mp);
+  mp->accel_partial = 0;
   copy_field(mp->d_displ, displ, *size, mp, "transfer_fields_el_to_device");
   copy_field(mp->d_veloc, veloc, *size, mp, "transfer_fields_el_to_device");
   copy_field(mp->d_accel, accel, *size, mp, "transfer_fields_el_to_device");
@@ -736,6 +825,7 @@ void transfer_fields_el_to_device_(int* size, realw* displ, realw* veloc, realw*
 void transfer_fields_el_from_device_(int* size, realw* displ, realw* veloc, realw* accel, long* Mesh_pointer) {
   Mesh* mp = handle(Mesh_pointer);
   flush_correctors(mp);
+  check_accel_complete(mp, "transfer_fields_el_from_device");
   fetch_field(displ, mp->d_displ, *size, mp, "transfer_fields_el_from_device");
   fetch_field(veloc, mp->d_veloc, *size, mp, "transfer_fields_el_from_device");
   fetch_field(accel, mp->d_accel, *size, mp, "transfer_fields_el_from_device");
@@ -757,11 +847,13 @@ void transfer_b_fields_from_device_(int* size, realw* b_displ, realw* b_veloc, r
 void transfer_accel_to_device_(int* size, realw* accel, long* Mesh_pointer) {
   Mesh* mp = handle(Mesh_pointer);
   flush_correctors(mp);
+  mp->accel_partial = 0;
   copy_field(mp->d_accel, accel, *size, mp, "transfer_accel_to_device");
 }
 void transfer_accel_from_device_(int* size, realw* accel, long* Mesh_pointer) {
   Mesh* mp = handle(Mesh_pointer);
   flush_correctors(mp);
+  check_accel_complete(mp, "transfer_accel_from_device");
   fetch_field(accel, mp->d_accel, *size, mp, "transfer_accel_from_device");
 }
 void transfer_b_accel_from_device_(int* size, realw* b_accel, long* Mesh_pointer) {
@@ -827,12 +919,18 @@ void update_displacement_cuda_(long* Mesh_pointer, realw* deltat_F, realw* delta
                                realw* b_deltat_F, realw* b_deltatsqover2_F,
                                realw* b_deltatover2_F) {  // update_displacement_cuda.cu:52
   Mesh* mp = handle(Mesh_pointer);
+  const int keep = mp->first_touch;  // first-touch assembly: the force kernels define accel
   if (mp->pending_corrector[0])
     launch_newmark_corrector_predictor(mp->d_displ, mp->d_veloc, mp->d_accel, mp->d_rmassx, mp->d_rmassz, mp->nglob,
-                                       mp->pending_dto2[0], *deltat_F, *deltatsqover2_F, *deltatover2_F, mp->stream);
+                                       mp->pending_dto2[0], *deltat_F, *deltatsqover2_F, *deltatover2_F, mp->stream,
+                                       1, 0, !keep);
   else
     launch_newmark_predictor(mp->d_displ, mp->d_veloc, mp->d_accel, mp->nglob, *deltat_F, *deltatsqover2_F,
-                             *deltatover2_F, mp->stream);
+                             *deltatover2_F, mp->stream, 1, 0, !keep);
+  if (keep) {
+    launch_zero_points(mp, mp->d_accel);
+    mp->accel_partial = 1;
+  }
   if (mp->simulation_type == 3) {
     if (mp->pending_corrector[1])
       launch_newmark_corrector_predictor(mp->d_b_displ, mp->d_b_veloc, mp->d_b_accel, mp->d_rmassx, mp->d_rmassz,
@@ -858,13 +956,15 @@ void compute_forces_viscoelastic_cuda_(long* Mesh_pointer, int* iphase, realw* d
   if (*ATTENUATION_VISCOELASTIC) fail("viscoelastic attenuation is not supported by this library yet");
   const int ph = (*iphase == 1) ? 0 : 1;
   const int nelem = (*iphase == 1) ? *nspec_outer_elastic : *nspec_inner_elastic;
+  const int first_touch = mp->accel_partial && *compute_wavefield_1;
+  if (ph == 1) mp->accel_partial = 0;  // the last force launches of the step complete accel
   if (nelem <= 0) return;
   if (!mp->sorted) fail("compute_forces: prepare_fields_elastic_device was not called");
   if (nelem != mp->phase_count[ph])
     fail("compute_forces: iphase %d has %d elements, the prepared phase list has %d", *iphase, nelem, mp->phase_count[ph]);
   const bool w1 = *compute_wavefield_1 != 0, w2 = *compute_wavefield_2 != 0;
   if (w2 && !mp->d_b_displ) fail("compute_forces: backward wavefield not allocated");
-  if (w1) apply_element_forces(mp, ph, mp->d_displ, mp->d_accel);
+  if (w1) apply_element_forces(mp, ph, mp->d_displ, mp->d_accel, 1, 0, first_touch);
   if (w2) apply_element_forces(mp, ph, mp->d_b_displ, mp->d_b_accel);
 }
 
@@ -934,6 +1034,7 @@ void kernel_3_a_cuda_(long* Mesh_pointer, realw* deltatover2_F, realw* b_deltato
                       int* compute_wavefield_2) {  // update_displacement_cuda.cu:221
   Mesh* mp = handle(Mesh_pointer);
   flush_correctors(mp);
+  check_accel_complete(mp, "kernel_3_a_cuda");
   // recorded here, applied by the next predictor (or by the first entry point that reads veloc/accel)
   if (*compute_wavefield_1) {
     mp->pending_corrector[0] = 1;
@@ -977,7 +1078,10 @@ void compute_seismograms_cuda_(long* Mesh_pointer_f, int* i_sigf, double* sisux,
   switch (stype) {
     case 1: field = mp->d_displ; break;
     case 2: field = mp->d_veloc; break;
-    case 3: field = mp->d_accel; break;
+    case 3:
+      check_accel_complete(mp, "compute_seismograms_cuda");
+      field = mp->d_accel;
+      break;
     default: fail("seismotype %d (pressure/curl) is not supported by this library yet", stype);
   }
   float* d_dst = mp->d_seismo[i_sig];
@@ -1036,6 +1140,7 @@ void output_free_device_memory_(int* myrank_f) {  // check_fields_cuda.cu:149
 void get_max_accel_(int* itf, int* sizef, long* Mesh_pointer) {  // check_fields_cuda.cu:484
   Mesh* mp = handle(Mesh_pointer);
   flush_correctors(mp);
+  check_accel_complete(mp, "get_max_accel");
   std::vector<float> a((size_t)*sizef);
   check_hip(hipStreamSynchronize(mp->stream), "get_max_accel");
   check_hip(hipMemcpy(a.data(), mp->d_accel, a.size() * sizeof(float), hipMemcpyDeviceToHost), "accel D2H");
@@ -1202,9 +1307,15 @@ extern "C" int sf2d_mesh_affine_elements(long* Mesh_pointer) {
 }
 
 // whether every patch of the plan for this mesh is a chain of elements (host only)
-extern "C" int sf2d_plan_chains(int nspec, int nglob, const int* ibool_1based, int np, const int* phase_lists) {
+extern "C" int sf2d_plan_chains(int nspec, int nglob, const int* ibool_1based, int np, const int* phase_lists,
+                                int* first_writes) {
   std::vector<int> ib = to_zero_based(ibool_1based, (size_t)nspec * NGLL2, 1, nglob, "ibool");
-  return build_plan(ib, nspec, nglob, phase_lists, np, 0).chains ? 1 : 0;
+  Plan P = build_plan(ib, nspec, nglob, phase_lists, np, 0);
+  if (first_writes) {
+    *first_writes = 0;
+    for (uint8_t f : P.first) *first_writes += f;
+  }
+  return P.chains ? 1 : 0;
 }
 
 extern "C" int sf2d_mesh_chain(long* Mesh_pointer) { return handle(Mesh_pointer)->chain; }

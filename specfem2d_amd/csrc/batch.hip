@@ -43,15 +43,17 @@ void batch_step_into(Mesh* mp, int it, const float* d, const float* v, const flo
   BatchState* B = need_batch(mp);
   if (it < 1 || it > mp->nstep) fail("batch step %d outside 1..%d", it, mp->nstep);
   const size_t n = (size_t)2 * B->stride * B->nshot;
+  const int keep = mp->first_touch;  // first-touch assembly: the force kernels define accel
   if (!d) {  // the predictor of a state at rest is the state at rest
     for (float* p : {d_out, v_out, a_out}) check_hip(hipMemsetAsync(p, 0, n * sizeof(float), mp->stream), "zero");
   } else {
     launch_newmark_corrector_predictor_to(d, v, a, d_out, v_out, a_out, mp->d_rmassx, mp->d_rmassz, mp->nglob,
                                           dto2_prev, deltat, deltatsqover2, deltatover2, mp->stream, B->nshot,
-                                          B->stride);
+                                          B->stride, !keep);
+    if (keep) launch_zero_points(mp, a_out, B->nshot, B->stride);
   }
   for (int ph = 0; ph < 2; ph++) {
-    apply_element_forces(mp, ph, d_out, a_out, B->nshot, B->stride);
+    apply_element_forces(mp, ph, d_out, a_out, B->nshot, B->stride, keep);
     if (ph == 0) {
       if (mp->stacey) launch_stacey(mp, v_out, a_out, 0, B->nshot, B->stride);
       launch_sources(mp, a_out, it - 1, B->src_shot, B->stride);
@@ -130,15 +132,17 @@ void sf2d_batch_step(long* Mesh_pointer, int it, float deltat, float deltatsqove
   Mesh* mp = handle(Mesh_pointer);
   BatchState* B = need_batch(mp);
   if (it < 1 || it > mp->nstep) fail("sf2d_batch_step: step %d outside 1..%d", it, mp->nstep);
+  const int keep = mp->first_touch;  // first-touch assembly: the force kernels define accel
   if (B->pending)
     launch_newmark_corrector_predictor(B->d, B->v, B->a, mp->d_rmassx, mp->d_rmassz, mp->nglob, B->pending_dto2,
-                                       deltat, deltatsqover2, deltatover2, mp->stream, B->nshot, B->stride);
+                                       deltat, deltatsqover2, deltatover2, mp->stream, B->nshot, B->stride, !keep);
   else
     launch_newmark_predictor(B->d, B->v, B->a, mp->nglob, deltat, deltatsqover2, deltatover2, mp->stream, B->nshot,
-                             B->stride);
+                             B->stride, !keep);
+  if (keep) launch_zero_points(mp, B->a, B->nshot, B->stride);
   B->pending = 0;
   for (int ph = 0; ph < 2; ph++) {
-    apply_element_forces(mp, ph, B->d, B->a, B->nshot, B->stride);
+    apply_element_forces(mp, ph, B->d, B->a, B->nshot, B->stride, keep);
     if (ph == 0) {
       if (mp->stacey) launch_stacey(mp, B->v, B->a, 0, B->nshot, B->stride);
       launch_sources(mp, B->a, it - 1, B->src_shot, B->stride);

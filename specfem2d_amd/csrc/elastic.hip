@@ -34,7 +34,7 @@ __device__ __forceinline__ void atomic_add_f32(float* p, float v) { unsafeAtomic
 // Newmark predictor  (src/gpu/kernels/UpdateDispVeloc_kernel.cu:37-66; update_displacement_Newmark.F90:408-454)
 __global__ __launch_bounds__(256) void k_predictor(float2* __restrict__ d, float2* __restrict__ v,
                                                    float2* __restrict__ a, int n, float dt, float dt2o2, float dto2,
-                                                   long shot_stride) {
+                                                   long shot_stride, int zero_accel) {
   d += blockIdx.y * shot_stride;
   v += blockIdx.y * shot_stride;
   a += blockIdx.y * shot_stride;
@@ -47,7 +47,7 @@ __global__ __launch_bounds__(256) void k_predictor(float2* __restrict__ d, float
     vv.y = vv.y + dto2 * aa.y;
     d[k] = dd;
     v[k] = vv;
-    a[k] = make_float2(0.f, 0.f);
+    if (zero_accel) a[k] = make_float2(0.f, 0.f);
   }
 }
 
@@ -76,7 +76,8 @@ __global__ __launch_bounds__(256) void k_corrector(float2* __restrict__ v, float
 __global__ __launch_bounds__(256) void k_corrector_predictor(float2* __restrict__ d, float2* __restrict__ v,
                                                              float2* __restrict__ a, const float* __restrict__ rmx,
                                                              const float* __restrict__ rmz, int n, float dto2_prev,
-                                                             float dt, float dt2o2, float dto2, long shot_stride) {
+                                                             float dt, float dt2o2, float dto2, long shot_stride,
+                                                             int zero_accel) {
   d += blockIdx.y * shot_stride;
   v += blockIdx.y * shot_stride;
   a += blockIdx.y * shot_stride;
@@ -93,7 +94,7 @@ __global__ __launch_bounds__(256) void k_corrector_predictor(float2* __restrict_
     vv.y = vv.y + dto2 * aa.y;
     d[k] = dd;
     v[k] = vv;
-    a[k] = make_float2(0.f, 0.f);
+    if (zero_accel) a[k] = make_float2(0.f, 0.f);
   }
 }
 
@@ -103,7 +104,7 @@ __global__ __launch_bounds__(256) void k_corrector_predictor_to(
     const float2* __restrict__ d, const float2* __restrict__ v, const float2* __restrict__ a,
     float2* __restrict__ dd_out, float2* __restrict__ dv_out, float2* __restrict__ da_out,
     const float* __restrict__ rmx, const float* __restrict__ rmz, int n, float dto2_prev, float dt, float dt2o2,
-    float dto2, long shot_stride) {
+    float dto2, long shot_stride, int zero_accel) {
   const long off = blockIdx.y * shot_stride;
   d += off;
   v += off;
@@ -124,8 +125,16 @@ __global__ __launch_bounds__(256) void k_corrector_predictor_to(
     vv.y = vv.y + dto2 * aa.y;
     dd_out[k] = dd;
     dv_out[k] = vv;
-    da_out[k] = make_float2(0.f, 0.f);
+    if (zero_accel) da_out[k] = make_float2(0.f, 0.f);
   }
+}
+
+// accel = 0 at a list of points (first-touch assembly: the points that boundary terms and sources also update)
+__global__ __launch_bounds__(256) void k_zero_points(float2* __restrict__ a, const int* __restrict__ pts, int n,
+                                                     long shot_stride) {
+  a += blockIdx.y * shot_stride;
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x)
+    a[pts[k]] = make_float2(0.f, 0.f);
 }
 
 // ---------------------------------------------------------------------------
@@ -445,10 +454,15 @@ __global__ __launch_bounds__(FORCE_THREADS) void k_elastic_forces_patch(
 //     added at (0, j), so every point of the patch is updated by exactly one lane: no force cells, no atomics.
 // The arithmetic per point is the element operator of the other kernels (Kernel_2_noatt_iso_impl:
 // compute_forces_viscoelastic_cuda.cu:163-215, :501-565).
+//
+// FIRST (first-touch assembly): a point id carrying Mesh::FIRST_TOUCH belongs to the first colour that updates the
+// point in the step, and that colour writes the force instead of adding it to accel, so the predictor does not
+// have to clear accel and the first update reads nothing.
 constexpr int CHAIN_LANES = 64;
+template <bool FIRST>
 __global__ __launch_bounds__(CHAIN_LANES) void k_elastic_forces_chain(
     int first_patch, int npatch, int nshot, long shot_stride, const int4* __restrict__ desc,
-    const int* __restrict__ ibool, const float4* __restrict__ geom, const float4* __restrict__ geom_elem,
+    const int* __restrict__ ids, const float4* __restrict__ geom, const float4* __restrict__ geom_elem,
     const float2* __restrict__ moduli, const float2* __restrict__ displ, float2* __restrict__ accel,
     const Basis basis) {
   // [element][row][column], rows padded to 6; element FORCE_EPB is scratch for lanes past the patch
@@ -474,11 +488,14 @@ __global__ __launch_bounds__(CHAIN_LANES) void k_elastic_forces_chain(
   const int e = d.x + min(k, ne - 1);
   const unsigned l0 = (unsigned)e * NGLL2 + j * NGLL;  // first point of this lane's row
   int g[NGLL];
+  bool first[NGLL];
   float2 u[NGLL], mod[NGLL];
   float4 geo[NGLL];
 #pragma unroll
   for (int i = 0; i < NGLL; i++) {
-    g[i] = ld_off(ibool, 4u * (l0 + i));
+    const int id = ld_off(ids, 4u * (l0 + i));
+    g[i] = id & ~Mesh::FIRST_TOUCH;
+    first[i] = FIRST && (id & Mesh::FIRST_TOUCH);
     mod[i] = ld_off(moduli, 8u * (l0 + i));
   }
   if (patch_affine(d.w)) {
@@ -496,8 +513,10 @@ __global__ __launch_bounds__(CHAIN_LANES) void k_elastic_forces_chain(
   const bool last = k == ne - 1;
   float2 a[NGLL];
 #pragma unroll
-  for (int i = 0; i < NGLL - 1; i++) a[i] = ld_off((const float2*)as, 8u * (unsigned)g[i]);
-  a[NGLL - 1] = last ? ld_off((const float2*)as, 8u * (unsigned)g[NGLL - 1]) : make_float2(0.f, 0.f);
+  for (int i = 0; i < NGLL - 1; i++)
+    a[i] = first[i] ? make_float2(0.f, 0.f) : ld_off((const float2*)as, 8u * (unsigned)g[i]);
+  a[NGLL - 1] = (last && !first[NGLL - 1]) ? ld_off((const float2*)as, 8u * (unsigned)g[NGLL - 1])
+                                          : make_float2(0.f, 0.f);
 
   // row j of u into the tile, gamma-derivatives by columns
 #pragma unroll
@@ -915,10 +934,10 @@ static dim3 shot_grid(long n, int threads, int cap, int nshot) {
 }
 
 void launch_newmark_predictor(float* d, float* v, float* a, int nglob, float dt, float dt2o2, float dto2,
-                              hipStream_t s, int nshot, long shot_stride) {
+                              hipStream_t s, int nshot, long shot_stride, int zero_accel) {
   if (nglob <= 0) return;
   hipLaunchKernelGGL(k_predictor, shot_grid(nglob, 256, 65536, nshot), dim3(256), 0, s, (float2*)d, (float2*)v,
-                     (float2*)a, nglob, dt, dt2o2, dto2, shot_stride);
+                     (float2*)a, nglob, dt, dt2o2, dto2, shot_stride, zero_accel);
   check_hip(hipGetLastError(), "k_predictor launch");
 }
 
@@ -932,22 +951,29 @@ void launch_newmark_corrector(float* v, float* a, const float* rmx, const float*
 
 void launch_newmark_corrector_predictor(float* d, float* v, float* a, const float* rmx, const float* rmz, int nglob,
                                        float dto2_prev, float dt, float dt2o2, float dto2, hipStream_t s,
-                                       int nshot, long shot_stride) {
+                                       int nshot, long shot_stride, int zero_accel) {
   if (nglob <= 0) return;
   hipLaunchKernelGGL(k_corrector_predictor, shot_grid(nglob, 256, 65536, nshot), dim3(256), 0, s, (float2*)d,
-                     (float2*)v, (float2*)a, rmx, rmz, nglob, dto2_prev, dt, dt2o2, dto2, shot_stride);
+                     (float2*)v, (float2*)a, rmx, rmz, nglob, dto2_prev, dt, dt2o2, dto2, shot_stride, zero_accel);
   check_hip(hipGetLastError(), "k_corrector_predictor launch");
 }
 
 void launch_newmark_corrector_predictor_to(const float* d, const float* v, const float* a, float* d_out, float* v_out,
                                           float* a_out, const float* rmx, const float* rmz, int nglob,
                                           float dto2_prev, float dt, float dt2o2, float dto2, hipStream_t s,
-                                          int nshot, long shot_stride) {
+                                          int nshot, long shot_stride, int zero_accel) {
   if (nglob <= 0) return;
   hipLaunchKernelGGL(k_corrector_predictor_to, shot_grid(nglob, 256, 65536, nshot), dim3(256), 0, s,
                      (const float2*)d, (const float2*)v, (const float2*)a, (float2*)d_out, (float2*)v_out,
-                     (float2*)a_out, rmx, rmz, nglob, dto2_prev, dt, dt2o2, dto2, shot_stride);
+                     (float2*)a_out, rmx, rmz, nglob, dto2_prev, dt, dt2o2, dto2, shot_stride, zero_accel);
   check_hip(hipGetLastError(), "k_corrector_predictor_to launch");
+}
+
+void launch_zero_points(const Mesh* mp, float* accel, int nshot, long shot_stride) {
+  if (mp->nspecial <= 0) return;
+  hipLaunchKernelGGL(k_zero_points, shot_grid(mp->nspecial, 256, 4096, nshot), dim3(256), 0, mp->stream,
+                     (float2*)accel, mp->d_special, mp->nspecial, shot_stride);
+  check_hip(hipGetLastError(), "k_zero_points launch");
 }
 
 void launch_elastic_forces(const Mesh* mp, int first, int nelem, int atomic, const float* displ, float* accel) {
@@ -968,7 +994,7 @@ void launch_elastic_forces(const Mesh* mp, int first, int nelem, int atomic, con
 }
 
 void launch_elastic_forces_patch(const Mesh* mp, int first_patch, int npatch, const float* displ, float* accel,
-                                 int nshot, long shot_stride) {
+                                 int nshot, long shot_stride, int first_touch) {
   if (npatch <= 0) return;
   if (nshot < 1 || (nshot > 1 && shot_stride < mp->nglob) || (long)npatch * nshot > (1L << 30))
     fail("batched force launch: %d shots of stride %ld over %d patches", nshot, shot_stride, npatch);
@@ -989,12 +1015,19 @@ void launch_elastic_forces_patch(const Mesh* mp, int first_patch, int npatch, co
   }
   const int all = (npatch * nshot + NXCD - 1) / NXCD * NXCD;
   if (mp->chain) {  // one wave per item
-    hipLaunchKernelGGL(k_elastic_forces_chain, dim3(all), dim3(CHAIN_LANES), 0, mp->stream, first_patch, npatch,
-                       nshot, shot_stride, (const int4*)mp->d_patch_desc, mp->d_ibool, mp->d_geom, mp->d_geom_elem,
-                       mp->d_moduli, (const float2*)displ, (float2*)accel, mp->basis);
+    if (first_touch && !mp->first_touch) fail("first-touch assembly is not set up for this handle");
+    if (first_touch)
+      hipLaunchKernelGGL(k_elastic_forces_chain<true>, dim3(all), dim3(CHAIN_LANES), 0, mp->stream, first_patch,
+                         npatch, nshot, shot_stride, (const int4*)mp->d_patch_desc, mp->d_chain_ids, mp->d_geom,
+                         mp->d_geom_elem, mp->d_moduli, (const float2*)displ, (float2*)accel, mp->basis);
+    else
+      hipLaunchKernelGGL(k_elastic_forces_chain<false>, dim3(all), dim3(CHAIN_LANES), 0, mp->stream, first_patch,
+                         npatch, nshot, shot_stride, (const int4*)mp->d_patch_desc, mp->d_chain_ids, mp->d_geom,
+                         mp->d_geom_elem, mp->d_moduli, (const float2*)displ, (float2*)accel, mp->basis);
     check_hip(hipGetLastError(), "k_elastic_forces_chain launch");
     return;
   }
+  if (first_touch) fail("first-touch assembly needs the chain kernel");
   const int nwg = resident > 0 ? std::min(resident, all) : all;
   hipLaunchKernelGGL(k_elastic_forces_patch, dim3(nwg), dim3(FORCE_THREADS), 0, mp->stream, first_patch, npatch,
                      nshot, shot_stride, (const int4*)mp->d_patch_desc, mp->d_halo_nodes, mp->d_elem_slot, mp->d_geom,

@@ -316,6 +316,7 @@ void sf2d_grad_restore_checkpoint(long* Mesh_pointer, int slot, float dto2) {
   } else {
     mp->pending_corrector[0] = 1;
     mp->pending_dto2[0] = dto2;
+    mp->accel_partial = 0;
   }
 }
 

@@ -80,6 +80,19 @@ struct Mesh {
   int no_affine = 0;                 // SF2D_NO_AFFINE: always read the per-point geometry
   int chain = 0;                     // every patch is a chain of elements: the row-lane force kernel (SF2D_NO_CHAIN
                                      // keeps the general patch kernel)
+  int* d_chain_ids = nullptr;        // chain kernel: [nspec][25] device order, internal point id | FIRST_TOUCH
+  std::vector<int> h_chain_ids;      // the same ids without flags, and the first-touch flags of the plan
+  std::vector<uint8_t> h_first;
+  // First-touch assembly (forward wavefield of SIMULATION_TYPE=1 and shot batches; SF2D_NO_FIRST_TOUCH turns it
+  // off): the predictor leaves accel as it is except at the special points (Stacey edges, source elements),
+  // which it clears; the first colour that updates a point in a step writes its force there instead of adding.
+  // Between the predictor and the last force launch of the step accel is therefore incomplete (accel_partial).
+  static constexpr int FIRST_TOUCH = 1 << 30;
+  int first_touch = 0;
+  std::vector<int> h_stacey_points;  // internal ids of the points of the absorbing edges
+  int* d_special = nullptr;
+  int nspecial = 0;
+  int accel_partial = 0;
   std::vector<int> point_id;         // device id of each caller point (empty: identity)
   std::vector<int> h_ibool;          // 0-based, ispec order, kept until the elements are sorted
   std::vector<int> pos_of_ispec, ispec_of_pos;
@@ -164,7 +177,9 @@ void batch_step_into(Mesh* mp, int it, const float* d, const float* v, const flo
 void flush_correctors(Mesh* mp);
 // acc -= K u over the elements of phase ph (0 outer, 1 inner), with the handle's assembly scheme, for nshot
 // wavefields shot_stride points apart
-void apply_element_forces(const Mesh* mp, int ph, const float* u, float* acc, int nshot = 1, long shot_stride = 0);
+// first_touch: the step's accel was left by a predictor with zero_accel = 0 (first-touch assembly)
+void apply_element_forces(const Mesh* mp, int ph, const float* u, float* acc, int nshot = 1, long shot_stride = 0,
+                          int first_touch = 0);
 // point arrays in the caller's numbering <-> the device numbering (n2 values per point)
 void points_to_device(const Mesh* mp, float* dev, const float* host, int n2, const char* what);
 void points_from_device(const Mesh* mp, float* host, const float* dev, int n2, const char* what);
@@ -218,22 +233,25 @@ void device_free(T*& p) {
 
 // ---------------------------------------------------------------------------
 // kernel launchers (defined in elastic.hip), all asynchronous on mp->stream
+// zero_accel = 0 leaves accel as it is (first-touch assembly)
 void launch_newmark_predictor(float* d, float* v, float* a, int nglob, float dt, float dt2o2, float dto2,
-                              hipStream_t s, int nshot = 1, long shot_stride = 0);
+                              hipStream_t s, int nshot = 1, long shot_stride = 0, int zero_accel = 1);
 void launch_newmark_corrector(float* v, float* a, const float* rmx, const float* rmz, int nglob, float dto2,
                               int apply_mass, hipStream_t s, int nshot = 1, long shot_stride = 0);
 void launch_newmark_corrector_predictor(float* d, float* v, float* a, const float* rmx, const float* rmz, int nglob,
                                        float dto2_prev, float dt, float dt2o2, float dto2, hipStream_t s,
-                                       int nshot = 1, long shot_stride = 0);
+                                       int nshot = 1, long shot_stride = 0, int zero_accel = 1);
 // out of place: (d, v, a) with the corrector pending -> (d_out, v_out, 0); the two states must not overlap
 void launch_newmark_corrector_predictor_to(const float* d, const float* v, const float* a, float* d_out, float* v_out,
                                           float* a_out, const float* rmx, const float* rmz, int nglob,
                                           float dto2_prev, float dt, float dt2o2, float dto2, hipStream_t s,
-                                          int nshot = 1, long shot_stride = 0);
+                                          int nshot = 1, long shot_stride = 0, int zero_accel = 1);
+// accel = 0 at the special points of first-touch assembly
+void launch_zero_points(const Mesh* mp, float* accel, int nshot = 1, long shot_stride = 0);
 void launch_elastic_forces(const Mesh* mp, int first, int nelem, int atomic, const float* displ, float* accel);
 // nshot wavefields at a stride of shot_stride points (float2) in displ and accel
 void launch_elastic_forces_patch(const Mesh* mp, int first_patch, int npatch, const float* displ, float* accel,
-                                 int nshot = 1, long shot_stride = 0);
+                                 int nshot = 1, long shot_stride = 0, int first_touch = 0);
 void launch_permute_elements(const float* src, float* dst, const int* ispec_of_pos, int nspec, hipStream_t s);
 void launch_pack_elements(const Mesh* mp);  // geom / moduli from the per-element arrays (patch mode)
 // per-element geometry and the affine flags of the patches; returns the number of affine elements

@@ -86,14 +86,18 @@ def test_plan_invariants_shuffled_and_split_lists(mode):
         assert stats[0] >= 1 and stats[1] >= 4
 
 
-def chains(ibool, nglob, lists):
+def chains(ibool, nglob, lists, first=None):
     gpu, _ = driver.load_libraries()
     ib = np.ascontiguousarray(ibool.reshape(-1), dtype=np.int32)
     ph = np.ascontiguousarray(lists.reshape(-1), dtype=np.int32)
     f = gpu.sf2d_plan_chains
     f.restype = C.c_int
-    return f(C.c_int(ibool.shape[0]), C.c_int(nglob), ib.ctypes.data_as(C.c_void_p), C.c_int(lists.shape[1]),
-             ph.ctypes.data_as(C.c_void_p))
+    nf = C.c_int(-1)
+    rc = f(C.c_int(ibool.shape[0]), C.c_int(nglob), ib.ctypes.data_as(C.c_void_p), C.c_int(lists.shape[1]),
+           ph.ctypes.data_as(C.c_void_p), C.byref(nf))
+    if first is not None:
+        first.append(nf.value)
+    return rc
 
 
 @pytest.mark.parametrize("nx,expect", [(80, 1), (12, 1), (13, 1), (10, 1), (8, 0), (1, 0)])
@@ -105,7 +109,10 @@ def test_chain_patches_on_structured_meshes(nx, expect):
     cfg = config.small_test_mesh(nx=nx, nz=4, nstep=2)
     o = OracleSim(cfg, "sp")
     ib = o.ibool()
-    assert chains(ib, o.nglob, inner_only(o.nspec)) == expect
+    first = []
+    assert chains(ib, o.nglob, inner_only(o.nspec), first) == expect
+    # first-touch assembly: exactly one update per point is the step's first
+    assert first[0] == (o.nglob if expect else 0)
     if expect:
         e = np.arange(o.nspec).reshape(cfg.nz, cfg.nx)
         left, right = e[:, 1:].reshape(-1), e[:, :-1].reshape(-1)

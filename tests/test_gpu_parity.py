@@ -200,3 +200,32 @@ def test_chain_kernel_matches_the_general_patch_kernel(monkeypatch, no_affine):
     # the acceleration is the last step's force itself, not smoothed by the time integration
     assert err < 5e-6 and ferr[0] < 5e-6 and ferr[1] < 5e-6 and ferr[2] < 5e-5
     assert oerr < TOL
+
+
+def test_first_touch_assembly_matches_read_modify_write(monkeypatch):
+    """First-touch assembly (the predictor leaves accel alone except at the Stacey and source points; the first
+    force update of a point in a step writes instead of adding) against plain read-modify-write with the same
+    chain kernel (SF2D_NO_FIRST_TOUCH): the same wavefield to FP32 rounding, the oracle's traces to the usual
+    tolerance."""
+    from specfem2d_amd.driver import Simulation
+    cfg = config.small_test_mesh(nx=23, nz=9, nstep=300, gll=True, seed=4).replace(
+        absorbtop=True, absorbbottom=True, absorbleft=True, absorbright=True)
+    monkeypatch.setenv("SF2D_ASSEMBLY", "patch")
+    monkeypatch.delenv("SF2D_NO_CHAIN", raising=False)
+    monkeypatch.delenv("SF2D_NO_FIRST_TOUCH", raising=False)
+    a = Simulation(cfg)
+    sa, _ = a.forward()
+    fa = a.fields()
+    a.close()
+    monkeypatch.setenv("SF2D_NO_FIRST_TOUCH", "1")
+    b = Simulation(cfg)
+    sb, _ = b.forward()
+    fb = b.fields()
+    b.close()
+    err = rel(sa, sb)
+    ferr = [rel(x, y) for x, y in zip(fa, fb)]
+    oerr = rel(sa, OracleSim(cfg, "sp").forward())
+    print(f"first-touch vs read-modify-write: traces {err:.2e}, displ/veloc/accel {ferr[0]:.2e} {ferr[1]:.2e} "
+          f"{ferr[2]:.2e}; vs FP32 oracle {oerr:.2e}")
+    assert err < 5e-6 and ferr[0] < 5e-6 and ferr[1] < 5e-6 and ferr[2] < 5e-5
+    assert oerr < TOL
