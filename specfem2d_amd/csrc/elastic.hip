@@ -459,9 +459,13 @@ __global__ __launch_bounds__(FORCE_THREADS) void k_elastic_forces_patch(
 // point in the step, and that colour writes the force instead of adding it to accel, so the predictor does not
 // have to clear accel and the first update reads nothing.
 constexpr int CHAIN_LANES = 64;
-template <bool FIRST>
+constexpr int CHAIN_SHOTS_PER_WAVE = 4;  // shots per wave in batched launches (SF2D_SHOTS_PER_WAVE overrides)
+//
+// Shots: an item is a patch and a group of up to `group` consecutive shots; the wave loads the patch's element
+// data once and runs the shots of its group one after the other.
+template <bool FIRST, bool MULTI>
 __global__ __launch_bounds__(CHAIN_LANES) void k_elastic_forces_chain(
-    int first_patch, int npatch, int nshot, long shot_stride, const int4* __restrict__ desc,
+    int first_patch, int npatch, int nshot, int group, long shot_stride, const int4* __restrict__ desc,
     const int* __restrict__ ids, const float4* __restrict__ geom, const float4* __restrict__ geom_elem,
     const float2* __restrict__ moduli, const float2* __restrict__ displ, float2* __restrict__ accel,
     const Basis basis) {
@@ -477,10 +481,12 @@ __global__ __launch_bounds__(CHAIN_LANES) void k_elastic_forces_chain(
   // items in XCD-contiguous blocks: the shots of a patch run on one XCD and share its L2
   const int nw = gridDim.x;
   const int item = (blockIdx.x % NXCD) * (nw / NXCD) + blockIdx.x / NXCD;
-  const int nitem = npatch * nshot;
+  if (!MULTI) group = 1;  // one shot per wave: the element data need not stay live across shots
+  const int ngroup = (nshot + group - 1) / group;
+  const int nitem = npatch * ngroup;
   if (item >= nitem) return;  // uniform over the wave
-  const int4 d = desc[first_patch + item / nshot];
-  const long base = (long)(item % nshot) * shot_stride;
+  const int4 d = desc[first_patch + item / ngroup];
+  const int s0 = (item % ngroup) * group, s1 = min(nshot, s0 + group);
   const int ne = patch_ne(d.w);
   const int k = min(lane / NGLL, FORCE_EPB);  // element of the patch (FORCE_EPB: lanes past the patch)
   const int j = lane % NGLL;
@@ -506,11 +512,13 @@ __global__ __launch_bounds__(CHAIN_LANES) void k_elastic_forces_chain(
 #pragma unroll
     for (int i = 0; i < NGLL; i++) geo[i] = ld_off(geom, 16u * (l0 + i));
   }
+  const bool last = k == ne - 1;
+  for (int shot = s0; shot < s1; shot++) {
+  const long base = (long)shot * shot_stride;
   const float2* ds = displ + base;
 #pragma unroll
   for (int i = 0; i < NGLL; i++) u[i] = ld_off(ds, 8u * (unsigned)g[i]);
   float2* as = accel + base;
-  const bool last = k == ne - 1;
   float2 a[NGLL];
 #pragma unroll
   for (int i = 0; i < NGLL - 1; i++)
@@ -624,6 +632,7 @@ __global__ __launch_bounds__(CHAIN_LANES) void k_elastic_forces_chain(
       *reinterpret_cast<float2*>(reinterpret_cast<char*>(as) + 8u * (unsigned)g[NGLL - 1]) = a[NGLL - 1];
     }
   }
+  }  // shots
 }
 
 // per element: geometry of point 0, and whether it stands for all 25 points: every value within 2^-22 of the
@@ -1014,16 +1023,21 @@ void launch_elastic_forces_patch(const Mesh* mp, int first_patch, int npatch, co
     if (getenv("SF2D_VERBOSE")) fprintf(stderr, "k_elastic_forces_patch: %d CUs, %d workgroups resident\n", cus, resident);
   }
   const int all = (npatch * nshot + NXCD - 1) / NXCD * NXCD;
-  if (mp->chain) {  // one wave per item
+  if (mp->chain) {  // one wave per (patch, group of shots)
     if (first_touch && !mp->first_touch) fail("first-touch assembly is not set up for this handle");
-    if (first_touch)
-      hipLaunchKernelGGL(k_elastic_forces_chain<true>, dim3(all), dim3(CHAIN_LANES), 0, mp->stream, first_patch,
-                         npatch, nshot, shot_stride, (const int4*)mp->d_patch_desc, mp->d_chain_ids, mp->d_geom,
-                         mp->d_geom_elem, mp->d_moduli, (const float2*)displ, (float2*)accel, mp->basis);
-    else
-      hipLaunchKernelGGL(k_elastic_forces_chain<false>, dim3(all), dim3(CHAIN_LANES), 0, mp->stream, first_patch,
-                         npatch, nshot, shot_stride, (const int4*)mp->d_patch_desc, mp->d_chain_ids, mp->d_geom,
-                         mp->d_geom_elem, mp->d_moduli, (const float2*)displ, (float2*)accel, mp->basis);
+    static int group_env = -1;
+    if (group_env < 0) {
+      const char* env = getenv("SF2D_SHOTS_PER_WAVE");
+      group_env = env ? std::max(1, atoi(env)) : 0;
+    }
+    const int group = std::min(nshot, group_env > 0 ? group_env : CHAIN_SHOTS_PER_WAVE);
+    const int items = npatch * ((nshot + group - 1) / group);
+    const int grid = (items + NXCD - 1) / NXCD * NXCD;
+    auto kern = group > 1 ? (first_touch ? k_elastic_forces_chain<true, true> : k_elastic_forces_chain<false, true>)
+                          : (first_touch ? k_elastic_forces_chain<true, false> : k_elastic_forces_chain<false, false>);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(CHAIN_LANES), 0, mp->stream, first_patch, npatch, nshot, group,
+                       shot_stride, (const int4*)mp->d_patch_desc, mp->d_chain_ids, mp->d_geom, mp->d_geom_elem,
+                       mp->d_moduli, (const float2*)displ, (float2*)accel, mp->basis);
     check_hip(hipGetLastError(), "k_elastic_forces_chain launch");
     return;
   }
