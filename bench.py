@@ -126,15 +126,15 @@ def fwi_gradient(repeats: int = 2):
             "method": "exact discrete adjoint, checkpointed recompute, FP64 gradient accumulation", "shots": 1}
 
 
-def fwi_gradient_batched(nshot: int = 32, repeats: int = 2):
+def fwi_gradient_batched(nshot: int = 32, repeats: int = 2, nstep: int = 1500):
     """Seconds per shot of the exact FWI gradient on the multi-shot 600 x 300-element, 1500-step workload,
     plain (one shot through the reference's entry points, checkpoint every 100 steps) against batched (nshot
     shots advanced together, checkpoints every 39 steps, segments recomputed into their state slots).
     Observed data: the shots run in the random-field model; synthetics: the homogeneous background."""
     from specfem2d_amd import config
     from specfem2d_amd.driver import ShotBatch, Simulation
-    true_cfg = config.batched_fwi_600x300(true_model=True)
-    cfg = config.batched_fwi_600x300()
+    true_cfg = config.batched_fwi_600x300(nstep=nstep, true_model=True)
+    cfg = config.batched_fwi_600x300(nstep=nstep)
     shots = config.surface_shots(cfg, nshot)
     b = ShotBatch(true_cfg, shots)
     obs = b.forward()
@@ -200,6 +200,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-gradient", action="store_true", help="skip the FWI gradient time per shot")
+    ap.add_argument("--fwi-only", action="store_true",
+                    help="only the batched-vs-plain FWI gradient (profiling); prints its JSON object")
+    ap.add_argument("--fwi-steps", type=int, default=1500, help="time steps of the batched FWI workload")
     ap.add_argument("--batch-shots", type=int, default=32,
                     help="shots per batch of the multi-shot FWI gradient (32 fill about 170 GB of checkpoints)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -217,6 +220,9 @@ def main():
         sys.exit(1)
     from specfem2d_amd.driver import Simulation
 
+    if args.fwi_only:
+        print(json.dumps(fwi_gradient_batched(args.batch_shots, nstep=args.fwi_steps)))
+        return
     cfg = make_config(args.workload, args.warmup + args.steps)
     t_setup = time.perf_counter()
     sim = Simulation(cfg)
@@ -292,7 +298,7 @@ def main():
         copy_calibration(args.copy_calibration_mib)
     if not args.no_gradient:
         out["fwi_gradient"] = fwi_gradient()
-        out["fwi_gradient_batched"] = fwi_gradient_batched(args.batch_shots)
+        out["fwi_gradient_batched"] = fwi_gradient_batched(args.batch_shots, nstep=args.fwi_steps)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.workload, args.cpu_seconds)
     print(json.dumps(out))

@@ -101,6 +101,10 @@ void sf2d_batch_flush(long* Mesh_pointer);
 void sf2d_batch_get_fields(long* Mesh_pointer, int shot, float* displ, float* veloc, float* accel);
 /* shots of the batch (0 without one) */
 int sf2d_batch_shots(long* Mesh_pointer);
+/* 1 when the batch fields are shot-minor (point-major, nshot values per point: the lane-per-shot kernels, on
+   request with SF2D_SHOT_MINOR for chain meshes and 2, 4, ..., 64 shots), 0 when shot-major (the default), -1
+   without a batch */
+int sf2d_batch_shot_minor(long* Mesh_pointer);
 
 /* replaces kappa, mu, rho vp, rho vs ([nspec][25], ispec order) and rmassx, rmassz ([nglob], caller numbering);
    NULL leaves an array unchanged */

@@ -477,8 +477,14 @@ void to_ispec_order(const Mesh* mp, const float* dev_order, float* out) {
 
 // acc -= K u over the elements of phase ph (0 outer, 1 inner), with the handle's assembly scheme
 void apply_element_forces(const Mesh* mp, int ph, const float* u, float* acc, int nshot, long shot_stride,
-                          int first_touch) {
+                          int first_touch, int ps) {
   if (mp->phase_count[ph] <= 0) return;
+  if (ps > 1) {  // shot-minor batch: lanes are shots
+    if (mp->assembly != 0 || !mp->chain) fail("shot-minor batches need chain patches");
+    for (int c = 0; c < mp->ncolour[ph]; c++)
+      launch_elastic_forces_walk(mp, mp->colour_start[ph][c], mp->colour_count[ph][c], u, acc, ps, first_touch);
+    return;
+  }
   if (mp->assembly == 0) {  // patches: every shot of a colour in one launch
     for (int c = 0; c < mp->ncolour[ph]; c++)
       launch_elastic_forces_patch(mp, mp->colour_start[ph][c], mp->colour_count[ph][c], u, acc, nshot, shot_stride,

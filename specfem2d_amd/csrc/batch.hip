@@ -3,8 +3,10 @@
 // simulation (src/specfem2D/iterate_time.F90); a batch step applies the reference's GPU sequence
 // (update_displacement_cuda, compute_forces_viscoelastic_cuda, compute_stacey_viscoelastic_cuda,
 // compute_add_sources_el_cuda, kernel_3_a_cuda, compute_seismograms_cuda) to every shot in one launch per
-// kernel.  The patch force kernel walks (patch, shot) items with the shots of a patch adjacent, so a patch's
-// geometry and moduli are fetched from HBM once per step for the whole batch and re-read from L2.
+// kernel.  Two layouts: shot-major (shot s at s * stride, the default) runs the patch kernels over (patch, shot
+// group) items, a patch's element data fetched from HBM once per step and re-read from L2; shot-minor (point g of
+// shot s at g * nshot + s, SF2D_SHOT_MINOR) runs the lane-per-shot kernels, whose element data is shared by the
+// lanes of an item and whose point accesses are nshot consecutive values.
 #include <algorithm>
 
 #include "sf2d_mesh.h"
@@ -22,43 +24,53 @@ void flush_batch(Mesh* mp) {
   BatchState* B = mp->batch;
   if (!B || !B->pending) return;
   launch_newmark_corrector(B->v, B->a, mp->d_rmassx, mp->d_rmassz, mp->nglob, B->pending_dto2, 1, mp->stream,
-                           B->nshot, B->stride);
+                           B->nshot, B->stride, B->ps);
   B->pending = 0;
+}
+
+size_t field_floats(const Mesh* mp, const BatchState* B) {
+  return B->ps > 1 ? (size_t)2 * mp->nglob * B->nshot : (size_t)2 * B->stride * B->nshot;
+}
+
+// forces, Stacey and sources of step it on state (d, v, a) whose accel holds the predictor's output
+void batch_forces(Mesh* mp, const BatchState* B, int it, const float* d, const float* v, float* a, int keep) {
+  for (int ph = 0; ph < 2; ph++) {
+    apply_element_forces(mp, ph, d, a, B->nshot, B->stride, keep, B->ps);
+    if (ph == 0) {
+      if (mp->stacey) launch_stacey(mp, v, a, 0, B->nshot, B->stride, B->ps);
+      launch_sources(mp, a, it - 1, B->src_shot, B->stride, B->ps);
+    }
+  }
 }
 
 }  // namespace
 
 // device addresses of the batch fields (gradient.hip checkpoints them)
-void batch_fields(Mesh* mp, float** d, float** v, float** a, int* nshot, long* stride) {
+void batch_fields(Mesh* mp, float** d, float** v, float** a, int* nshot, long* stride, int* ps) {
   BatchState* B = need_batch(mp);
   *d = B->d;
   *v = B->v;
   *a = B->a;
   *nshot = B->nshot;
   *stride = B->stride;
+  *ps = B->ps;
 }
 
 void batch_step_into(Mesh* mp, int it, const float* d, const float* v, const float* a, float dto2_prev, float* d_out,
                      float* v_out, float* a_out, float deltat, float deltatsqover2, float deltatover2) {
   BatchState* B = need_batch(mp);
   if (it < 1 || it > mp->nstep) fail("batch step %d outside 1..%d", it, mp->nstep);
-  const size_t n = (size_t)2 * B->stride * B->nshot;
+  const size_t n = field_floats(mp, B);
   const int keep = mp->first_touch;  // first-touch assembly: the force kernels define accel
   if (!d) {  // the predictor of a state at rest is the state at rest
     for (float* p : {d_out, v_out, a_out}) check_hip(hipMemsetAsync(p, 0, n * sizeof(float), mp->stream), "zero");
   } else {
     launch_newmark_corrector_predictor_to(d, v, a, d_out, v_out, a_out, mp->d_rmassx, mp->d_rmassz, mp->nglob,
                                           dto2_prev, deltat, deltatsqover2, deltatover2, mp->stream, B->nshot,
-                                          B->stride, !keep);
-    if (keep) launch_zero_points(mp, a_out, B->nshot, B->stride);
+                                          B->stride, !keep, B->ps);
+    if (keep) launch_zero_points(mp, a_out, B->nshot, B->stride, B->ps);
   }
-  for (int ph = 0; ph < 2; ph++) {
-    apply_element_forces(mp, ph, d_out, a_out, B->nshot, B->stride, keep);
-    if (ph == 0) {
-      if (mp->stacey) launch_stacey(mp, v_out, a_out, 0, B->nshot, B->stride);
-      launch_sources(mp, a_out, it - 1, B->src_shot, B->stride);
-    }
-  }
+  batch_forces(mp, B, it, d_out, v_out, a_out, keep);
 }
 
 void batch_set_pending(Mesh* mp, float dto2) {
@@ -84,13 +96,21 @@ int sf2d_batch_prepare(long* Mesh_pointer, int nshot, const int* src_shot) {
            nshot - 1);
     owner[k] = src_shot[k];
   }
-  const long stride = ((long)mp->nglob + 63) / 64 * 64;
-  const size_t n = (size_t)2 * stride * nshot;
+  // shot-minor on request (SF2D_SHOT_MINOR) when the lane-per-shot kernels apply: chain patches, 32-bit offsets,
+  // a power of two of at most 64 shots (the reverse sweep sums the shots of a point across lanes).  Shot-major is
+  // the default: with one wave walking a whole patch, the lane-per-shot force kernel has too few waves per SIMD
+  // to hide its load latency and runs slower per shot than the patch kernels.
+  const bool minor = getenv("SF2D_SHOT_MINOR") && !getenv("SF2D_SHOT_MAJOR") && nshot > 1 && nshot <= 64 &&
+                     (nshot & (nshot - 1)) == 0 && mp->chain && mp->assembly == 0 &&
+                     (size_t)mp->nglob * nshot * 8 < ((size_t)1 << 32);
+  const long stride = minor ? 1 : ((long)mp->nglob + 63) / 64 * 64;
+  const size_t n = minor ? (size_t)2 * mp->nglob * nshot : (size_t)2 * stride * nshot;
   if (n * 3 * sizeof(float) > ((size_t)1 << 38)) fail("sf2d_batch_prepare: %d shots exceed 256 GiB of fields", nshot);
   if (mp->batch) sf2d_batch_release(Mesh_pointer);
   BatchState* B = new BatchState();
   B->nshot = nshot;
   B->stride = stride;
+  B->ps = minor ? nshot : 1;
   B->d = device_zeros<float>(n);
   B->v = device_zeros<float>(n);
   B->a = device_zeros<float>(n);
@@ -119,7 +139,7 @@ void sf2d_batch_release(long* Mesh_pointer) {
 void sf2d_batch_reset(long* Mesh_pointer) {
   Mesh* mp = handle(Mesh_pointer);
   BatchState* B = need_batch(mp);
-  const size_t n = (size_t)2 * B->stride * B->nshot;
+  const size_t n = field_floats(mp, B);
   for (float* p : {B->d, B->v, B->a}) check_hip(hipMemsetAsync(p, 0, n * sizeof(float), mp->stream), "zero");
   check_hip(hipMemsetAsync(B->seismo, 0, sizeof(float) * B->nshot * 2 * std::max(1, mp->nrec_local) * mp->nstep,
                            mp->stream),
@@ -135,19 +155,14 @@ void sf2d_batch_step(long* Mesh_pointer, int it, float deltat, float deltatsqove
   const int keep = mp->first_touch;  // first-touch assembly: the force kernels define accel
   if (B->pending)
     launch_newmark_corrector_predictor(B->d, B->v, B->a, mp->d_rmassx, mp->d_rmassz, mp->nglob, B->pending_dto2,
-                                       deltat, deltatsqover2, deltatover2, mp->stream, B->nshot, B->stride, !keep);
+                                       deltat, deltatsqover2, deltatover2, mp->stream, B->nshot, B->stride, !keep,
+                                       B->ps);
   else
     launch_newmark_predictor(B->d, B->v, B->a, mp->nglob, deltat, deltatsqover2, deltatover2, mp->stream, B->nshot,
-                             B->stride, !keep);
-  if (keep) launch_zero_points(mp, B->a, B->nshot, B->stride);
+                             B->stride, !keep, B->ps);
+  if (keep) launch_zero_points(mp, B->a, B->nshot, B->stride, B->ps);
   B->pending = 0;
-  for (int ph = 0; ph < 2; ph++) {
-    apply_element_forces(mp, ph, B->d, B->a, B->nshot, B->stride, keep);
-    if (ph == 0) {
-      if (mp->stacey) launch_stacey(mp, B->v, B->a, 0, B->nshot, B->stride);
-      launch_sources(mp, B->a, it - 1, B->src_shot, B->stride);
-    }
-  }
+  batch_forces(mp, B, it, B->d, B->v, B->a, keep);
   B->pending = 1;
   B->pending_dto2 = deltatover2;
 }
@@ -157,7 +172,7 @@ void sf2d_batch_record(long* Mesh_pointer, int it) {
   Mesh* mp = handle(Mesh_pointer);
   BatchState* B = need_batch(mp);
   if (it < 1 || it > mp->nstep) fail("sf2d_batch_record: step %d outside 1..%d", it, mp->nstep);
-  launch_seismograms(mp, B->d, B->seismo, it - 1, B->nshot, B->stride, (long)2 * mp->nrec_local * mp->nstep);
+  launch_seismograms(mp, B->d, B->seismo, it - 1, B->nshot, B->stride, (long)2 * mp->nrec_local * mp->nstep, B->ps);
 }
 
 // seismograms [nshot][2][nrec][nstep] (x then z component, receiver-major)
@@ -184,15 +199,38 @@ void sf2d_batch_get_fields(long* Mesh_pointer, int shot, float* displ, float* ve
   if (shot < 0 || shot >= B->nshot) fail("sf2d_batch_get_fields: shot %d outside 0..%d", shot, B->nshot - 1);
   flush_batch(mp);
   check_hip(hipStreamSynchronize(mp->stream), "sf2d_batch_get_fields");
-  const size_t off = (size_t)2 * B->stride * shot;
-  if (displ) points_from_device(mp, displ, B->d + off, 2, "batch displ D2H");
-  if (veloc) points_from_device(mp, veloc, B->v + off, 2, "batch veloc D2H");
-  if (accel) points_from_device(mp, accel, B->a + off, 2, "batch accel D2H");
+  if (B->ps == 1) {
+    const size_t off = (size_t)2 * B->stride * shot;
+    if (displ) points_from_device(mp, displ, B->d + off, 2, "batch displ D2H");
+    if (veloc) points_from_device(mp, veloc, B->v + off, 2, "batch veloc D2H");
+    if (accel) points_from_device(mp, accel, B->a + off, 2, "batch accel D2H");
+    return;
+  }
+  // shot-minor: the shot's values are every nshot-th float2
+  std::vector<float> all(field_floats(mp, B));
+  const float* dev[3] = {B->d, B->v, B->a};
+  float* out[3] = {displ, veloc, accel};
+  for (int f = 0; f < 3; f++) {
+    if (!out[f]) continue;
+    check_hip(hipMemcpy(all.data(), dev[f], all.size() * sizeof(float), hipMemcpyDeviceToHost), "batch field D2H");
+    for (int g = 0; g < mp->nglob; g++) {
+      const size_t id = mp->point_id.empty() ? (size_t)g : (size_t)mp->point_id[g];
+      const size_t q = (id * B->nshot + shot) * 2;
+      out[f][2 * (size_t)g] = all[q];
+      out[f][2 * (size_t)g + 1] = all[q + 1];
+    }
+  }
 }
 
 int sf2d_batch_shots(long* Mesh_pointer) {
   Mesh* mp = handle(Mesh_pointer);
   return mp->batch ? mp->batch->nshot : 0;
+}
+
+// 1 when the batch fields are shot-minor (the lane-per-shot force kernel), 0 shot-major, -1 without a batch
+int sf2d_batch_shot_minor(long* Mesh_pointer) {
+  Mesh* mp = handle(Mesh_pointer);
+  return mp->batch ? (mp->batch->ps > 1 ? 1 : 0) : -1;
 }
 
 }  // extern "C"

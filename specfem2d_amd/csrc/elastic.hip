@@ -54,15 +54,16 @@ __global__ __launch_bounds__(256) void k_predictor(float2* __restrict__ d, float
 // Newmark corrector with the inverse mass matrix (src/gpu/kernels/kernel_3_cuda_device.cu:37-65)
 __global__ __launch_bounds__(256) void k_corrector(float2* __restrict__ v, float2* __restrict__ a,
                                                    const float* __restrict__ rmx, const float* __restrict__ rmz,
-                                                   int n, float dto2, int apply_mass, long shot_stride) {
+                                                   int n, float dto2, int apply_mass, long shot_stride, int ps) {
   v += blockIdx.y * shot_stride;
   a += blockIdx.y * shot_stride;
   int stride = gridDim.x * blockDim.x;
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
     float2 aa = a[k], vv = v[k];
     if (apply_mass) {
-      aa.x = aa.x * rmx[k];
-      aa.y = aa.y * rmz[k];
+      const int g = ps == 1 ? k : k / ps;
+      aa.x = aa.x * rmx[g];
+      aa.y = aa.y * rmz[g];
       a[k] = aa;
     }
     vv.x = vv.x + dto2 * aa.x;
@@ -77,15 +78,16 @@ __global__ __launch_bounds__(256) void k_corrector_predictor(float2* __restrict_
                                                              float2* __restrict__ a, const float* __restrict__ rmx,
                                                              const float* __restrict__ rmz, int n, float dto2_prev,
                                                              float dt, float dt2o2, float dto2, long shot_stride,
-                                                             int zero_accel) {
+                                                             int zero_accel, int ps) {
   d += blockIdx.y * shot_stride;
   v += blockIdx.y * shot_stride;
   a += blockIdx.y * shot_stride;
   int stride = gridDim.x * blockDim.x;
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
     float2 dd = d[k], vv = v[k], aa = a[k];
-    aa.x = aa.x * rmx[k];
-    aa.y = aa.y * rmz[k];
+    const int g = ps == 1 ? k : k / ps;
+    aa.x = aa.x * rmx[g];
+    aa.y = aa.y * rmz[g];
     vv.x = vv.x + dto2_prev * aa.x;
     vv.y = vv.y + dto2_prev * aa.y;
     dd.x = dd.x + dt * vv.x + dt2o2 * aa.x;
@@ -104,7 +106,7 @@ __global__ __launch_bounds__(256) void k_corrector_predictor_to(
     const float2* __restrict__ d, const float2* __restrict__ v, const float2* __restrict__ a,
     float2* __restrict__ dd_out, float2* __restrict__ dv_out, float2* __restrict__ da_out,
     const float* __restrict__ rmx, const float* __restrict__ rmz, int n, float dto2_prev, float dt, float dt2o2,
-    float dto2, long shot_stride, int zero_accel) {
+    float dto2, long shot_stride, int zero_accel, int ps) {
   const long off = blockIdx.y * shot_stride;
   d += off;
   v += off;
@@ -115,8 +117,9 @@ __global__ __launch_bounds__(256) void k_corrector_predictor_to(
   int stride = gridDim.x * blockDim.x;
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
     float2 dd = d[k], vv = v[k], aa = a[k];
-    aa.x = aa.x * rmx[k];
-    aa.y = aa.y * rmz[k];
+    const int g = ps == 1 ? k : k / ps;
+    aa.x = aa.x * rmx[g];
+    aa.y = aa.y * rmz[g];
     vv.x = vv.x + dto2_prev * aa.x;
     vv.y = vv.y + dto2_prev * aa.y;
     dd.x = dd.x + dt * vv.x + dt2o2 * aa.x;
@@ -130,11 +133,12 @@ __global__ __launch_bounds__(256) void k_corrector_predictor_to(
 }
 
 // accel = 0 at a list of points (first-touch assembly: the points that boundary terms and sources also update)
+// (shot-minor batches: n = points x ps entries, point k / ps, shot k % ps)
 __global__ __launch_bounds__(256) void k_zero_points(float2* __restrict__ a, const int* __restrict__ pts, int n,
-                                                     long shot_stride) {
+                                                     long shot_stride, int ps) {
   a += blockIdx.y * shot_stride;
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x)
-    a[pts[k]] = make_float2(0.f, 0.f);
+    a[ps == 1 ? (long)pts[k] : (long)pts[k / ps] * ps + k % ps] = make_float2(0.f, 0.f);
 }
 
 // ---------------------------------------------------------------------------
@@ -635,6 +639,154 @@ __global__ __launch_bounds__(CHAIN_LANES) void k_elastic_forces_chain(
   }  // shots
 }
 
+// Shot-minor batches (fields [point][shot]: ps = nshot shots per point): the lanes of a wave are shots.  A wave holds
+// 64 / L (patch, shot group) items, L = nshot rounded up to a power of two and at most 64 (lanes past the batch
+// idle), and each lane walks its item's chain element by element with the element's 25 points of its shot in
+// registers: the derivatives along both directions need no exchange between lanes (no LDS, no barriers), the
+// element data is the same for the L lanes of an item (one cache line serves them all), and each field access
+// of a point is L consecutive float2.  The left column of element k+1 is the right column of element k, so its
+// displacement and force stay in registers from one element to the next.  The weights are folded into the fluxes:
+// f(i, j) = -(sum_m [w_j t1(m, j)] hpw(m, i) + sum_m [w_i t2(i, m)] hpw(m, j)).
+template <bool FIRST, bool AFFINE>
+__device__ __forceinline__ void walk_body(
+    int first_patch, int npatch, int nshot, int lshift, const int4* __restrict__ desc, const int* __restrict__ ids,
+    const float4* __restrict__ geom, const float4* __restrict__ geom_elem, const float2* __restrict__ moduli,
+    const float2* __restrict__ displ, float2* __restrict__ accel, const Basis& basis, float2 (*s_u)[64]) {
+  const int lane = threadIdx.x;
+  const int ngroup = (nshot + (1 << lshift) - 1) >> lshift;
+  const int nitem = npatch * ngroup;
+  const int nw = gridDim.x;
+  const int wave = (blockIdx.x % NXCD) * (nw / NXCD) + blockIdx.x / NXCD;
+  const int item = wave * (64 >> lshift) + (lane >> lshift);
+  if (wave * (64 >> lshift) >= nitem) return;  // uniform: the wave has no item
+  const int itc = min(item, nitem - 1);
+  const int shot = (itc % ngroup) * (1 << lshift) + (lane & ((1 << lshift) - 1));
+  const bool live = item < nitem && shot < nshot;
+  // 32-bit element offsets (the launcher checks that points x shots x 8 B stays below 4 GiB)
+  const unsigned sh = (unsigned)min(shot, nshot - 1);
+  const unsigned ps = (unsigned)nshot;
+  const int4 d = desc[first_patch + itc / ngroup];
+  const int ne = patch_ne(d.w);
+#define U(j, i) s_u[(j) * NGLL + (i)][lane]
+  float2 fc[NGLL];  // force carried along the shared column
+#pragma unroll 1
+  for (int k = 0; k < ne; k++) {
+    const unsigned l0 = (unsigned)(d.x + k) * NGLL2;
+#pragma unroll
+    for (int j = 0; j < NGLL; j++) {
+      if (k > 0) U(j, 0) = U(j, NGLL - 1);  // the shared column
+#pragma unroll
+      for (int i = (k > 0 ? 1 : 0); i < NGLL; i++) {
+        const unsigned g = (unsigned)(ld_off(ids, 4u * (l0 + j * NGLL + i)) & ~Mesh::FIRST_TOUCH);
+        U(j, i) = ld_off(displ, 8u * (g * ps + sh));
+      }
+    }
+    const float4 ge = AFFINE ? ld_off(geom_elem, 16u * (l0 / NGLL2)) : make_float4(0.f, 0.f, 0.f, 0.f);
+    float2 f[NGLL][NGLL];
+#pragma unroll
+    for (int j = 0; j < NGLL; j++)
+#pragma unroll
+      for (int i = 0; i < NGLL; i++) f[j][i] = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int j = 0; j < NGLL; j++) {
+      __builtin_amdgcn_sched_barrier(0);  // one row at a time: bounds the loads in flight, and the registers
+      float2 row[NGLL], t1[NGLL];
+#pragma unroll
+      for (int m = 0; m < NGLL; m++) row[m] = U(j, m);
+#pragma unroll
+      for (int i = 0; i < NGLL; i++) {
+        float dux_dxi = 0.f, duz_dxi = 0.f, dux_dga = 0.f, duz_dga = 0.f;
+#pragma unroll
+        for (int m = 0; m < NGLL; m++) {
+          const float hi = basis.hp[i * NGLL + m], hj = basis.hp[j * NGLL + m];
+          const float2 c = U(m, i);
+          dux_dxi += row[m].x * hi;
+          duz_dxi += row[m].y * hi;
+          dux_dga += c.x * hj;
+          duz_dga += c.y * hj;
+        }
+        const unsigned l = l0 + j * NGLL + i;
+        const float4 gp = AFFINE ? ge : ld_off(geom, 16u * l);
+        const float2 md = ld_off(moduli, 8u * l);
+        const float xixl = gp.x, xizl = gp.y, gaxl = gp.z, gazl = gp.w;
+        const float dux_dxl = dux_dxi * xixl + dux_dga * gaxl;
+        const float dux_dzl = dux_dxi * xizl + dux_dga * gazl;
+        const float duz_dxl = duz_dxi * xixl + duz_dga * gaxl;
+        const float duz_dzl = duz_dxi * xizl + duz_dga * gazl;
+        const float jac = 1.f / (xixl * gazl - gaxl * xizl);
+        const float lambdal = md.x - md.y;
+        const float lp2m = md.x + md.y;
+        const float sxx = lp2m * dux_dxl + lambdal * duz_dzl;
+        const float szz = lp2m * duz_dzl + lambdal * dux_dxl;
+        const float sxz = md.y * (duz_dxl + dux_dzl);
+        const float wj = basis.w[j] * jac, wi = basis.w[i] * jac;
+        t1[i] = make_float2(wj * (sxx * xixl + sxz * xizl), wj * (sxz * xixl + szz * xizl));
+        const float2 t2 = make_float2(wi * (sxx * gaxl + sxz * gazl), wi * (sxz * gaxl + szz * gazl));
+#pragma unroll
+        for (int jp = 0; jp < NGLL; jp++) {  // gamma-flux contraction, column i
+          const float h = basis.hpw[j * NGLL + jp];
+          f[jp][i].x += t2.x * h;
+          f[jp][i].y += t2.y * h;
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < NGLL; i++)  // xi-flux contraction, row j
+#pragma unroll
+        for (int m = 0; m < NGLL; m++) {
+          const float h = basis.hpw[m * NGLL + i];
+          f[j][i].x += t1[m].x * h;
+          f[j][i].y += t1[m].y * h;
+        }
+    }
+    if (k > 0)
+#pragma unroll
+      for (int j = 0; j < NGLL; j++) {
+        f[j][0].x += fc[j].x;
+        f[j][0].y += fc[j].y;
+      }
+    const bool last = k == ne - 1;
+#pragma unroll
+    for (int j = 0; j < NGLL; j++) {
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int i = 0; i < NGLL; i++) {
+        if (i == NGLL - 1 && !last) continue;
+        const int raw = ld_off(ids, 4u * (l0 + j * NGLL + i));
+        const unsigned off = 8u * ((unsigned)(raw & ~Mesh::FIRST_TOUCH) * ps + sh);
+        float2 a = (FIRST && (raw & Mesh::FIRST_TOUCH)) ? make_float2(0.f, 0.f) : ld_off((const float2*)accel, off);
+        a.x -= f[j][i].x;
+        a.y -= f[j][i].y;
+        if (live) *reinterpret_cast<float2*>(reinterpret_cast<char*>(accel) + off) = a;
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < NGLL; j++) fc[j] = f[j][NGLL - 1];
+  }
+#undef U
+}
+
+// every element affine (one geometry record per element): a register budget of three waves per SIMD
+template <bool FIRST>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_elastic_forces_walk_affine(
+    int first_patch, int npatch, int nshot, int lshift, const int4* __restrict__ desc, const int* __restrict__ ids,
+    const float4* __restrict__ geom, const float4* __restrict__ geom_elem, const float2* __restrict__ moduli,
+    const float2* __restrict__ displ, float2* __restrict__ accel, const Basis basis) {
+  __shared__ float2 s_u[NGLL2][64];  // each lane's element displacement: point p of lane l at s_u[p][l]
+  walk_body<FIRST, true>(first_patch, npatch, nshot, lshift, desc, ids, geom, geom_elem, moduli, displ, accel, basis,
+                         s_u);
+}
+
+// per-point geometry
+template <bool FIRST>
+__global__ __launch_bounds__(64) void k_elastic_forces_walk(
+    int first_patch, int npatch, int nshot, int lshift, const int4* __restrict__ desc, const int* __restrict__ ids,
+    const float4* __restrict__ geom, const float4* __restrict__ geom_elem, const float2* __restrict__ moduli,
+    const float2* __restrict__ displ, float2* __restrict__ accel, const Basis basis) {
+  __shared__ float2 s_u[NGLL2][64];
+  walk_body<FIRST, false>(first_patch, npatch, nshot, lshift, desc, ids, geom, geom_elem, moduli, displ, accel, basis,
+                          s_u);
+}
+
 // per element: geometry of point 0, and whether it stands for all 25 points: every value within 2^-22 of the
 // element's largest geometry entry of the value at point 0.  Meshers leave cancellation residues (1e-18 against
 // 2e-2) where a parallelogram's entries are constant, so exact equality would almost never hold; within this
@@ -706,7 +858,7 @@ __global__ __launch_bounds__(64) void k_stacey(int nabs, const int* __restrict__
                                                const float* __restrict__ rho_vs, const float2* __restrict__ veloc,
                                                float* __restrict__ accel, int write_store, float* store_bottom,
                                                float* store_right, float* store_top, float* store_left,
-                                               long shot_stride) {
+                                               long shot_stride, int ps) {
   veloc += blockIdx.y * shot_stride;
   accel += 2 * blockIdx.y * shot_stride;
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
@@ -719,7 +871,7 @@ __global__ __launch_bounds__(64) void k_stacey(int nabs, const int* __restrict__
   const int ispec = abs_ispec[f];
   if (!is_elastic[ispec]) return;
   const size_t l = (size_t)ispec * NGLL2 + j * NGLL + i;
-  const int g = ibool[l];
+  const long g = (long)ibool[l] * ps;
   const float2 v = veloc[g];
   const float nx = normal[(f * NGLL + k) * 2];
   const float nz = normal[(f * NGLL + k) * 2 + 1];
@@ -774,14 +926,14 @@ __global__ __launch_bounds__(64) void k_sources(int nsrc, int it, const int* __r
                                                 const int* __restrict__ is_elastic, const int* __restrict__ ibool,
                                                 const float* __restrict__ sourcearrays, const float* __restrict__ stf,
                                                 float* __restrict__ accel, const int* __restrict__ src_shot,
-                                                long shot_stride) {
+                                                long shot_stride, int ps) {
   const int s = blockIdx.x;
   const int p = threadIdx.x;
   if (s >= nsrc || p >= NGLL2) return;
   if (src_shot) accel += 2 * (long)src_shot[s] * shot_stride;  // each source drives its own shot's field
   const int ispec = ispec_src[s];
   if (!is_elastic[ispec]) return;
-  const int g = ibool[(size_t)ispec * NGLL2 + p];
+  const long g = (long)ibool[(size_t)ispec * NGLL2 + p] * ps;
   const float f = stf[(size_t)it * nsrc + s];
   atomic_add_f32(&accel[2 * (size_t)g], sourcearrays[(s * NGLL2 + p) * 2] * f);
   atomic_add_f32(&accel[2 * (size_t)g + 1], sourcearrays[(s * NGLL2 + p) * 2 + 1] * f);
@@ -814,7 +966,8 @@ __global__ __launch_bounds__(64) void k_seismograms(int nrec, int nlength, int c
                                                     const int* __restrict__ ibool, const float* __restrict__ xir,
                                                     const float* __restrict__ gammar, const float* __restrict__ cosrot,
                                                     const float* __restrict__ sinrot, const float2* __restrict__ field,
-                                                    float* __restrict__ dst, long shot_stride, long dst_stride) {
+                                                    float* __restrict__ dst, long shot_stride, long dst_stride,
+                                                    int ps) {
   field += blockIdx.y * shot_stride;
   dst += blockIdx.y * dst_stride;
   const int r = blockIdx.x;
@@ -823,7 +976,7 @@ __global__ __launch_bounds__(64) void k_seismograms(int nrec, int nlength, int c
   double vx = 0.0, vz = 0.0;
   if (p < NGLL2) {
     const int i = p % NGLL, j = p / NGLL;
-    const int g = ibool[(size_t)ispec_rec[r] * NGLL2 + p];
+    const long g = (long)ibool[(size_t)ispec_rec[r] * NGLL2 + p] * ps;
     const double h = (double)(xir[i * nrec + r] * gammar[j * nrec + r]);
     const float2 u = field[g];
     vx = (double)u.x * h;
@@ -942,46 +1095,58 @@ static dim3 shot_grid(long n, int threads, int cap, int nshot) {
   return dim3(blocks_for(n, threads, cap), nshot);
 }
 
+// shot-minor batches (ps = shots per point > 1): one flat pass over the points x shots entries
 void launch_newmark_predictor(float* d, float* v, float* a, int nglob, float dt, float dt2o2, float dto2,
-                              hipStream_t s, int nshot, long shot_stride, int zero_accel) {
+                              hipStream_t s, int nshot, long shot_stride, int zero_accel, int ps) {
   if (nglob <= 0) return;
-  hipLaunchKernelGGL(k_predictor, shot_grid(nglob, 256, 65536, nshot), dim3(256), 0, s, (float2*)d, (float2*)v,
-                     (float2*)a, nglob, dt, dt2o2, dto2, shot_stride, zero_accel);
+  const long n = (long)nglob * ps;
+  if (ps > 1) nshot = 1;
+  hipLaunchKernelGGL(k_predictor, shot_grid(n, 256, 65536, nshot), dim3(256), 0, s, (float2*)d, (float2*)v,
+                     (float2*)a, (int)n, dt, dt2o2, dto2, shot_stride, zero_accel);
   check_hip(hipGetLastError(), "k_predictor launch");
 }
 
 void launch_newmark_corrector(float* v, float* a, const float* rmx, const float* rmz, int nglob, float dto2,
-                              int apply_mass, hipStream_t s, int nshot, long shot_stride) {
+                              int apply_mass, hipStream_t s, int nshot, long shot_stride, int ps) {
   if (nglob <= 0) return;
-  hipLaunchKernelGGL(k_corrector, shot_grid(nglob, 256, 65536, nshot), dim3(256), 0, s, (float2*)v, (float2*)a, rmx,
-                     rmz, nglob, dto2, apply_mass, shot_stride);
+  const long n = (long)nglob * ps;
+  if (ps > 1) nshot = 1;
+  hipLaunchKernelGGL(k_corrector, shot_grid(n, 256, 65536, nshot), dim3(256), 0, s, (float2*)v, (float2*)a, rmx,
+                     rmz, (int)n, dto2, apply_mass, shot_stride, ps);
   check_hip(hipGetLastError(), "k_corrector launch");
 }
 
 void launch_newmark_corrector_predictor(float* d, float* v, float* a, const float* rmx, const float* rmz, int nglob,
                                        float dto2_prev, float dt, float dt2o2, float dto2, hipStream_t s,
-                                       int nshot, long shot_stride, int zero_accel) {
+                                       int nshot, long shot_stride, int zero_accel, int ps) {
   if (nglob <= 0) return;
-  hipLaunchKernelGGL(k_corrector_predictor, shot_grid(nglob, 256, 65536, nshot), dim3(256), 0, s, (float2*)d,
-                     (float2*)v, (float2*)a, rmx, rmz, nglob, dto2_prev, dt, dt2o2, dto2, shot_stride, zero_accel);
+  const long n = (long)nglob * ps;
+  if (ps > 1) nshot = 1;
+  hipLaunchKernelGGL(k_corrector_predictor, shot_grid(n, 256, 65536, nshot), dim3(256), 0, s, (float2*)d,
+                     (float2*)v, (float2*)a, rmx, rmz, (int)n, dto2_prev, dt, dt2o2, dto2, shot_stride, zero_accel,
+                     ps);
   check_hip(hipGetLastError(), "k_corrector_predictor launch");
 }
 
 void launch_newmark_corrector_predictor_to(const float* d, const float* v, const float* a, float* d_out, float* v_out,
                                           float* a_out, const float* rmx, const float* rmz, int nglob,
                                           float dto2_prev, float dt, float dt2o2, float dto2, hipStream_t s,
-                                          int nshot, long shot_stride, int zero_accel) {
+                                          int nshot, long shot_stride, int zero_accel, int ps) {
   if (nglob <= 0) return;
-  hipLaunchKernelGGL(k_corrector_predictor_to, shot_grid(nglob, 256, 65536, nshot), dim3(256), 0, s,
+  const long n = (long)nglob * ps;
+  if (ps > 1) nshot = 1;
+  hipLaunchKernelGGL(k_corrector_predictor_to, shot_grid(n, 256, 65536, nshot), dim3(256), 0, s,
                      (const float2*)d, (const float2*)v, (const float2*)a, (float2*)d_out, (float2*)v_out,
-                     (float2*)a_out, rmx, rmz, nglob, dto2_prev, dt, dt2o2, dto2, shot_stride, zero_accel);
+                     (float2*)a_out, rmx, rmz, (int)n, dto2_prev, dt, dt2o2, dto2, shot_stride, zero_accel, ps);
   check_hip(hipGetLastError(), "k_corrector_predictor_to launch");
 }
 
-void launch_zero_points(const Mesh* mp, float* accel, int nshot, long shot_stride) {
+void launch_zero_points(const Mesh* mp, float* accel, int nshot, long shot_stride, int ps) {
   if (mp->nspecial <= 0) return;
-  hipLaunchKernelGGL(k_zero_points, shot_grid(mp->nspecial, 256, 4096, nshot), dim3(256), 0, mp->stream,
-                     (float2*)accel, mp->d_special, mp->nspecial, shot_stride);
+  const long n = (long)mp->nspecial * ps;
+  if (ps > 1) nshot = 1;
+  hipLaunchKernelGGL(k_zero_points, shot_grid(n, 256, 4096, nshot), dim3(256), 0, mp->stream, (float2*)accel,
+                     mp->d_special, (int)n, shot_stride, ps);
   check_hip(hipGetLastError(), "k_zero_points launch");
 }
 
@@ -1049,6 +1214,30 @@ void launch_elastic_forces_patch(const Mesh* mp, int first_patch, int npatch, co
   check_hip(hipGetLastError(), "k_elastic_forces_patch launch");
 }
 
+void launch_elastic_forces_walk(const Mesh* mp, int first_patch, int npatch, const float* displ, float* accel,
+                                int nshot, int first_touch) {
+  if (npatch <= 0) return;
+  if (!mp->chain) fail("shot-minor batches need chain patches");
+  if ((size_t)mp->nglob * nshot * sizeof(float2) >= ((size_t)1 << 32))
+    fail("shot-minor batch fields of %d points x %d shots exceed the 4 GiB of 32-bit offsets", mp->nglob, nshot);
+  if (first_touch && !mp->first_touch) fail("first-touch assembly is not set up for this handle");
+  if (first_patch < 0 || first_patch + npatch > mp->npatch)
+    fail("patch range [%d, %d) outside the %d patches", first_patch, first_patch + npatch, mp->npatch);
+  int lshift = 0;
+  while ((1 << lshift) < nshot && lshift < 6) lshift++;
+  const long items = (long)npatch * ((nshot + (1 << lshift) - 1) >> lshift);
+  const long waves = (items + (64 >> lshift) - 1) / (64 >> lshift);
+  if (waves > (1L << 30)) fail("shot-minor force launch too large");
+  const int grid = (int)((waves + NXCD - 1) / NXCD * NXCD);
+  const bool affine = !mp->no_affine && mp->n_affine == mp->nspec;
+  auto kern = affine ? (first_touch ? k_elastic_forces_walk_affine<true> : k_elastic_forces_walk_affine<false>)
+                     : (first_touch ? k_elastic_forces_walk<true> : k_elastic_forces_walk<false>);
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(64), 0, mp->stream, first_patch, npatch, nshot, lshift,
+                     (const int4*)mp->d_patch_desc, mp->d_chain_ids, mp->d_geom, mp->d_geom_elem, mp->d_moduli,
+                     (const float2*)displ, (float2*)accel, mp->basis);
+  check_hip(hipGetLastError(), "k_elastic_forces_walk launch");
+}
+
 void launch_pack_elements(const Mesh* mp) {
   const size_t n = (size_t)mp->nspec * NGLL2;
   hipLaunchKernelGGL(k_pack_elements, dim3(blocks_for((long)n, 256, 65536)), dim3(256), 0, mp->stream, n, mp->d_xix,
@@ -1085,14 +1274,15 @@ void launch_permute_elements_int(const int* src, int* dst, const int* ispec_of_p
   check_hip(hipGetLastError(), "k_permute launch");
 }
 
-void launch_stacey(const Mesh* mp, const float* veloc, float* accel, int write_store, int nshot, long shot_stride) {
+void launch_stacey(const Mesh* mp, const float* veloc, float* accel, int write_store, int nshot, long shot_stride,
+                   int ps) {
   if (mp->num_abs <= 0) return;
   if (nshot > 1 && write_store) fail("the Stacey boundary store holds one wavefield");
   hipLaunchKernelGGL(k_stacey, shot_grid((long)mp->num_abs * NGLL, 64, 1 << 20, nshot), dim3(64), 0, mp->stream,
                      mp->num_abs, mp->d_abs_ispec, mp->d_abs_ij, mp->d_abs_normal, mp->d_abs_jacw, mp->d_edge_abs,
                      mp->d_ib, mp->d_ibool, mp->d_ispec_is_elastic, mp->d_rho_vp, mp->d_rho_vs, (const float2*)veloc,
                      accel, write_store, mp->d_b_absorb[1], mp->d_b_absorb[2], mp->d_b_absorb[3], mp->d_b_absorb[4],
-                     shot_stride);
+                     shot_stride, ps);
   check_hip(hipGetLastError(), "k_stacey launch");
 }
 
@@ -1105,11 +1295,11 @@ void launch_stacey_backward(const Mesh* mp, float* b_accel) {
   check_hip(hipGetLastError(), "k_stacey_backward launch");
 }
 
-void launch_sources(const Mesh* mp, float* accel, int it, const int* src_shot, long shot_stride) {
+void launch_sources(const Mesh* mp, float* accel, int it, const int* src_shot, long shot_stride, int ps) {
   if (mp->nsources_local <= 0) return;
   hipLaunchKernelGGL(k_sources, dim3(mp->nsources_local), dim3(64), 0, mp->stream, mp->nsources_local, it,
                      mp->d_ispec_selected_source, mp->d_ispec_is_elastic, mp->d_ibool, mp->d_sourcearrays, mp->d_stf,
-                     accel, src_shot, shot_stride);
+                     accel, src_shot, shot_stride, ps);
   check_hip(hipGetLastError(), "k_sources launch");
 }
 
@@ -1122,11 +1312,12 @@ void launch_adjoint_sources(const Mesh* mp, float* accel, int it) {
 }
 
 void launch_seismograms(const Mesh* mp, const float* field, float* dst, int seismo_current, int nshot,
-                        long shot_stride, long dst_stride) {
+                        long shot_stride, long dst_stride, int ps) {
   if (mp->nrec_local <= 0) return;
   hipLaunchKernelGGL(k_seismograms, dim3(mp->nrec_local, nshot), dim3(64), 0, mp->stream, mp->nrec_local,
                      mp->nlength_seismogram, seismo_current, mp->d_ispec_selected_rec, mp->d_ibool, mp->d_xir,
-                     mp->d_gammar, mp->d_cosrot, mp->d_sinrot, (const float2*)field, dst, shot_stride, dst_stride);
+                     mp->d_gammar, mp->d_cosrot, mp->d_sinrot, (const float2*)field, dst, shot_stride, dst_stride,
+                     ps);
   check_hip(hipGetLastError(), "k_seismograms launch");
 }
 

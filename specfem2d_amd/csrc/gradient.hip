@@ -32,9 +32,9 @@ namespace sf2d {
 struct GradState {
   int nstep = 0, chunk = 0, nckpt = 0, nrec = 0, nshot = 1, batch = 0;
   size_t n2 = 0, ne = 0, nf = 0;
-  long stride = 0;        // points between shots
-  size_t sstr = 0;        // floats between shots (2 stride)
-  size_t nfield = 0;      // floats of one field over all shots (nshot sstr)
+  long stride = 0;        // float2 between shots (shot-major: >= nglob; shot-minor: 1)
+  int ps = 1;             // float2 between points (shot-minor: nshot)
+  size_t nfield = 0;      // floats of one field over all shots
   float *fd = nullptr, *fv = nullptr, *fa = nullptr;  // the forward fields that states are copied from
   float* ckpt = nullptr;  // [nckpt][3][nfield]: (d, v', r) after steps chunk, 2 chunk, ...
   float* seg = nullptr;   // [chunk][3][nfield]: states of the segment being reversed
@@ -47,9 +47,15 @@ struct GradState {
 namespace {
 
 constexpr int GT = 256;   // threads per workgroup
+
+template <typename T>
+__device__ __forceinline__ T ld_off(const T* base, unsigned byte_off) {  // 32-bit byte offset from a base
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + byte_off);
+}
 constexpr int GEPB = 10;  // elements per workgroup in k_grad_elastic
 
-__global__ __launch_bounds__(GT) void k_adj_pointwise(int first, int n, int nshot, size_t sstr, float dt, float dto2,
+// fields of shot s hold point g, component c at float 2 (g ps + s ss) + c
+__global__ __launch_bounds__(GT) void k_adj_pointwise(int first, int n, int nshot, int ps, long ss, float dt, float dto2,
                                                       float h, const float* __restrict__ D, float* __restrict__ Vb,
                                                       float* __restrict__ Rb, const float* __restrict__ rmx,
                                                       const float* __restrict__ rmz, const float* __restrict__ r,
@@ -58,8 +64,9 @@ __global__ __launch_bounds__(GT) void k_adj_pointwise(int first, int n, int nsho
     const int g = k >> 1;
     const float w = (k & 1) ? rmz[g] : rmx[g];
     float dm = 0.f;
+    const size_t q0 = (size_t)2 * ((size_t)g * ps) + (k & 1);
     for (int s = 0; s < nshot; s++) {
-      const size_t q = s * sstr + k;
+      const size_t q = q0 + (size_t)2 * s * ss;
       float v = 0.f, a = 0.f;
       if (!first) {  // state cotangents of the step after this one
         const float vb = Vb[q], db = D[q];
@@ -75,30 +82,31 @@ __global__ __launch_bounds__(GT) void k_adj_pointwise(int first, int n, int nsho
   }
 }
 
-// residual injection; shot blockIdx.y (residuals [nshot][nrec][2] of this step, D [nshot][sstr])
+// residual injection; shot blockIdx.y (residuals [nshot][nrec][2] of this step; D point g of shot s at float2
+// g ps + s ss)
 __global__ __launch_bounds__(64) void k_adj_receivers(int nrec, const float* __restrict__ res, const int* __restrict__ ispec_rec,
                                                       const int* __restrict__ ibool, const float* __restrict__ xir,
                                                       const float* __restrict__ gammar, const float* __restrict__ cosrot,
                                                       const float* __restrict__ sinrot, float* __restrict__ D,
-                                                      size_t sstr) {
+                                                      int ps, long ss) {
   const int r = blockIdx.x;
   const int p = threadIdx.x;
   if (r >= nrec || p >= NGLL2) return;
   res += (size_t)blockIdx.y * 2 * nrec;
-  D += blockIdx.y * sstr;
+  D += (size_t)2 * blockIdx.y * ss;
   const int i = p % NGLL, j = p / NGLL;
   const float px = res[2 * r], pz = res[2 * r + 1];
   const float c = cosrot[r], sn = sinrot[r];
   const float cx = c * px - sn * pz, cz = sn * px + c * pz;
   const float hl = xir[i * nrec + r] * gammar[j * nrec + r];
-  const int g = ibool[(size_t)ispec_rec[r] * NGLL2 + p];
+  const size_t g = (size_t)ibool[(size_t)ispec_rec[r] * NGLL2 + p] * ps;
   atomicAdd(&D[2 * g], hl * cx);
   atomicAdd(&D[2 * g + 1], hl * cz);
 }
 
 // strain products of the forward displacement d_n and the adjoint force R~ at every element point, summed over
-// the shots (fields sstr floats apart)
-__global__ __launch_bounds__(GT) void k_grad_elastic(int nelem, int nshot, size_t sstr, const int* __restrict__ ibool,
+// the shots (point g of shot s at float2 g ps + s ss)
+__global__ __launch_bounds__(GT) void k_grad_elastic(int nelem, int nshot, int ps, long ss, const int* __restrict__ ibool,
                                                      const float* __restrict__ xix, const float* __restrict__ xiz,
                                                      const float* __restrict__ gammax, const float* __restrict__ gammaz,
                                                      const float2* __restrict__ dn, const float2* __restrict__ rb,
@@ -120,13 +128,13 @@ __global__ __launch_bounds__(GT) void k_grad_elastic(int nelem, int nshot, size_
     H = gammaz[l];
   }
   const int base = le * NGLL2;
-  const size_t pstr = sstr / 2;  // points between shots
   float sk = 0.f, sm = 0.f;
   for (int s = 0; s < nshot; s++) {
     if (s > 0) __syncthreads();  // the previous shot's LDS reads are done
     if (active) {
-      s_d[tid] = dn[s * pstr + g];
-      s_b[tid] = rb[s * pstr + g];
+      const size_t q = (size_t)g * ps + (size_t)s * ss;
+      s_d[tid] = dn[q];
+      s_b[tid] = rb[q];
     }
     __syncthreads();
     if (active) {
@@ -152,8 +160,8 @@ __global__ __launch_bounds__(GT) void k_grad_elastic(int nelem, int nshot, size_
 }
 
 // Stacey traction's dependence on rho vp and rho vs, per absorbing face point (same order as abs_ij), summed over
-// the shots (fields pstr points apart)
-__global__ __launch_bounds__(64) void k_grad_stacey(int nabs, int nshot, size_t pstr, const int* __restrict__ abs_ispec,
+// the shots (point g of shot s at float2 g ps + s ss)
+__global__ __launch_bounds__(64) void k_grad_stacey(int nabs, int nshot, int ps, long ss, const int* __restrict__ abs_ispec,
                                                     const int* __restrict__ abs_ij,
                                                     const float* __restrict__ normal, const float* __restrict__ jacw,
                                                     const int* __restrict__ ibool, const float2* __restrict__ vpn,
@@ -166,15 +174,105 @@ __global__ __launch_bounds__(64) void k_grad_stacey(int nabs, int nshot, size_t 
   if (ii < 0 || jj < 0) return;  // corner point owned by the other edge
   const int g = ibool[(size_t)abs_ispec[f] * NGLL2 + jj * NGLL + ii];
   const float nx = normal[2 * q], nz = normal[2 * q + 1], w = jacw[q];
-  float sp = 0.f, ss = 0.f;
+  float sp = 0.f, sv = 0.f;
   for (int s = 0; s < nshot; s++) {
-    const float2 v = vpn[s * pstr + g], b = rb[s * pstr + g];
+    const size_t o = (size_t)g * ps + (size_t)s * ss;
+    const float2 v = vpn[o], b = rb[o];
     const float vn = nx * v.x + nz * v.y, bn = nx * b.x + nz * b.y;
     sp += w * vn * bn;
-    ss += w * (v.x * b.x + v.y * b.y - vn * bn);
+    sv += w * (v.x * b.x + v.y * b.y - vn * bn);
   }
   gvp[q] -= sp;
-  gvs[q] -= ss;
+  gvs[q] -= sv;
+}
+
+// Shot-minor batches (nshot = 2^lshift values per point, at most 64): the same two reverse-sweep kernels with the
+// shots in the lanes, so that every access is to consecutive values and the sum over the shots is a reduction
+// across lanes.
+// k_adj_pointwise over the flat points x shots x components entries; lanes of one (point, component) are 2 apart
+__global__ __launch_bounds__(GT) void k_adj_pointwise_minor(int first, long n, int lshift, float dt, float dto2,
+                                                            float h, const float* __restrict__ D,
+                                                            float* __restrict__ Vb, float* __restrict__ Rb,
+                                                            const float* __restrict__ rmx,
+                                                            const float* __restrict__ rmz,
+                                                            const float* __restrict__ r, float* __restrict__ pm) {
+  const long stride = (long)gridDim.x * blockDim.x;  // a multiple of 64: groups of 2 nshot lanes stay together
+  for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
+    const long g = k >> (lshift + 1);
+    const int c = (int)(k & 1);
+    const float w = c ? rmz[g] : rmx[g];
+    float v = 0.f, a = 0.f;
+    if (!first) {
+      const float vb = Vb[k], db = D[k];
+      v = vb + dt * db;
+      a = dto2 * vb + h * db;
+    }
+    a = a + dto2 * v;
+    Vb[k] = v;
+    Rb[k] = w * a;
+    float dm = a * r[k] * w * w;
+    for (int o = 2; o < (2 << lshift); o <<= 1) dm += __shfl_xor(dm, o, 64);
+    if (((k >> 1) & ((1 << lshift) - 1)) == 0) pm[2 * g + c] -= dm;
+  }
+}
+
+// k_grad_elastic with lane = shot: a wave takes 64 / 2^lshift elements; each lane stages its element's two fields in
+// lane-private LDS and the strain products of the 2^lshift lanes of an element are summed by shuffles
+__global__ __launch_bounds__(64) void k_grad_elastic_minor(int nelem, int nshot, int lshift,
+                                                           const int* __restrict__ ibool,
+                                                           const float4* __restrict__ geom,
+                                                           const float2* __restrict__ dn,
+                                                           const float2* __restrict__ rb, float* __restrict__ gk,
+                                                           float* __restrict__ gm, const Basis basis) {
+  __shared__ float2 s_d[NGLL2][64], s_b[NGLL2][64];
+  const int lane = threadIdx.x;
+  const int per_wave = 64 >> lshift;
+  const int wave = blockIdx.x;
+  if (wave * per_wave >= nelem) return;  // uniform
+  const int e = wave * per_wave + (lane >> lshift);
+  const int s = lane & ((1 << lshift) - 1);
+  const bool live = e < nelem && s < nshot;
+  const unsigned ec = (unsigned)min(e, nelem - 1);
+  const unsigned sh = (unsigned)min(s, nshot - 1);
+  const unsigned ps = 1u << lshift;
+#pragma unroll
+  for (int p = 0; p < NGLL2; p++) {
+    const unsigned g = (unsigned)ld_off(ibool, 4u * (ec * NGLL2 + p));
+    s_d[p][lane] = ld_off(dn, 8u * (g * ps + sh));
+    s_b[p][lane] = ld_off(rb, 8u * (g * ps + sh));
+  }
+#pragma unroll
+  for (int j = 0; j < NGLL; j++) {
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int i = 0; i < NGLL; i++) {
+      float a[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int k = 0; k < NGLL; k++) {
+        const float hi = basis.hp[i * NGLL + k], hj = basis.hp[j * NGLL + k];
+        const float2 dr = s_d[j * NGLL + k][lane], dc = s_d[k * NGLL + i][lane];
+        const float2 br = s_b[j * NGLL + k][lane], bc = s_b[k * NGLL + i][lane];
+        a[0] += dr.x * hi; a[1] += dr.y * hi; a[2] += dc.x * hj; a[3] += dc.y * hj;
+        a[4] += br.x * hi; a[5] += br.y * hi; a[6] += bc.x * hj; a[7] += bc.y * hj;
+      }
+      const unsigned l = ec * NGLL2 + j * NGLL + i;
+      const float4 gp = ld_off(geom, 16u * l);
+      const float X = gp.x, Z = gp.y, G = gp.z, H = gp.w;
+      const float ux = a[0] * X + a[2] * G, uxz = a[0] * Z + a[2] * H, uzx = a[1] * X + a[3] * G, uz = a[1] * Z + a[3] * H;
+      const float wx = a[4] * X + a[6] * G, wxz = a[4] * Z + a[6] * H, wzx = a[5] * X + a[7] * G, wz = a[5] * Z + a[7] * H;
+      const float wj = basis.w[i] * basis.w[j] / (X * H - G * Z);
+      float sk = live ? wj * (ux + uz) * (wx + wz) : 0.f;
+      float sm = live ? wj * (ux * wx + uz * wz - ux * wz - uz * wx + (uxz + uzx) * (wxz + wzx)) : 0.f;
+      for (int o = 1; o < (1 << lshift); o <<= 1) {
+        sk += __shfl_xor(sk, o, 64);
+        sm += __shfl_xor(sm, o, 64);
+      }
+      if (s == 0 && e < nelem) {
+        gk[l] -= sk;
+        gm[l] -= sm;
+      }
+    }
+  }
 }
 
 __global__ __launch_bounds__(GT) void k_flush64(size_t n, float* __restrict__ a32, double* __restrict__ a64) {
@@ -205,10 +303,10 @@ int grad_prepare(Mesh* mp, long* Mesh_pointer, int chunk, int nrec, int batch) {
                 old->nrec == nrec;
     if (same && batch) {
       float *d, *v, *a;
-      int ns;
+      int ns, ps;
       long st;
-      batch_fields(mp, &d, &v, &a, &ns, &st);
-      same = old->fd == d && old->nshot == ns && old->stride == st;
+      batch_fields(mp, &d, &v, &a, &ns, &st, &ps);
+      same = old->fd == d && old->nshot == ns && old->stride == st && old->ps == ps;
     }
     if (same) return old->nckpt;
     sf2d_grad_release(Mesh_pointer);
@@ -223,16 +321,18 @@ int grad_prepare(Mesh* mp, long* Mesh_pointer, int chunk, int nrec, int batch) {
   G->nf = (size_t)mp->num_abs * NGLL;
   G->batch = batch;
   if (batch) {
-    batch_fields(mp, &G->fd, &G->fv, &G->fa, &G->nshot, &G->stride);
+    batch_fields(mp, &G->fd, &G->fv, &G->fa, &G->nshot, &G->stride, &G->ps);
   } else {
     G->fd = mp->d_displ;
     G->fv = mp->d_veloc;
     G->fa = mp->d_accel;
     G->nshot = 1;
     G->stride = mp->nglob;
+    G->ps = 1;
   }
-  G->sstr = (size_t)2 * G->stride;
-  G->nfield = G->sstr * G->nshot;
+  G->nfield = G->ps > 1 ? (size_t)2 * mp->nglob * G->nshot : (size_t)2 * G->stride * G->nshot;
+  if (G->ps > 1 && (G->ps > 64 || (G->ps & (G->ps - 1)) || !mp->d_geom))
+    fail("gradient: shot-minor batches need a power of two of at most 64 shots and the patch assembly");
   const size_t states = (size_t)std::max(1, G->nckpt) + G->chunk;
   if (states * 3 * G->nfield * sizeof(float) > ((size_t)240 << 30))
     fail("gradient: %zu states of %d shots exceed 240 GiB; use a longer chunk or fewer shots", states, G->nshot);
@@ -349,23 +449,35 @@ void sf2d_grad_reverse_step(long* Mesh_pointer, int n, int slot, int first, floa
   const float* vpn = dn + G->nfield;
   const float* rn = dn + 2 * G->nfield;
   const float dto2 = (float)(0.5 * (double)dt), h = (float)(0.5 * (double)dt * (double)dt);
-  hipLaunchKernelGGL(k_adj_pointwise, dim3(grid_for(G->n2)), dim3(GT), 0, mp->stream, first, (int)G->n2, G->nshot,
-                     G->sstr, dt, dto2, h, G->D, G->Vb, G->Rb, mp->d_rmassx, mp->d_rmassz, rn, G->pm32);
+  int lshift = 0;
+  while ((1 << lshift) < G->ps) lshift++;
+  if (G->ps > 1)
+    hipLaunchKernelGGL(k_adj_pointwise_minor, dim3(grid_for(G->nfield)), dim3(GT), 0, mp->stream, first,
+                       (long)G->nfield, lshift, dt, dto2, h, G->D, G->Vb, G->Rb, mp->d_rmassx, mp->d_rmassz, rn,
+                       G->pm32);
+  else
+    hipLaunchKernelGGL(k_adj_pointwise, dim3(grid_for(G->n2)), dim3(GT), 0, mp->stream, first, (int)G->n2, G->nshot,
+                       G->ps, G->stride, dt, dto2, h, G->D, G->Vb, G->Rb, mp->d_rmassx, mp->d_rmassz, rn, G->pm32);
   if (G->nrec > 0)
     hipLaunchKernelGGL(k_adj_receivers, dim3(G->nrec, G->nshot), dim3(64), 0, mp->stream, G->nrec,
                        G->res + (size_t)(n - 1) * 2 * G->nrec * G->nshot, mp->d_ispec_selected_rec, mp->d_ibool,
-                       mp->d_xir, mp->d_gammar, mp->d_cosrot, mp->d_sinrot, G->D, G->sstr);
-  apply_element_forces(mp, 0, G->Rb, G->D, G->nshot, G->stride);
-  apply_element_forces(mp, 1, G->Rb, G->D, G->nshot, G->stride);
+                       mp->d_xir, mp->d_gammar, mp->d_cosrot, mp->d_sinrot, G->D, G->ps, G->stride);
+  apply_element_forces(mp, 0, G->Rb, G->D, G->nshot, G->stride, 0, G->ps);
+  apply_element_forces(mp, 1, G->Rb, G->D, G->nshot, G->stride, 0, G->ps);
   if (mp->stacey && mp->num_abs > 0) {
-    launch_stacey(mp, G->Rb, G->Vb, 0, G->nshot, G->stride);
+    launch_stacey(mp, G->Rb, G->Vb, 0, G->nshot, G->stride, G->ps);
     hipLaunchKernelGGL(k_grad_stacey, dim3((mp->num_abs * NGLL + 63) / 64), dim3(64), 0, mp->stream, mp->num_abs,
-                       G->nshot, (size_t)G->stride, mp->d_abs_ispec, mp->d_abs_ij, mp->d_abs_normal, mp->d_abs_jacw,
+                       G->nshot, G->ps, G->stride, mp->d_abs_ispec, mp->d_abs_ij, mp->d_abs_normal, mp->d_abs_jacw,
                        mp->d_ibool, (const float2*)vpn, (const float2*)G->Rb, G->gvp32, G->gvs32);
   }
-  hipLaunchKernelGGL(k_grad_elastic, dim3((mp->nspec + GEPB - 1) / GEPB), dim3(GT), 0, mp->stream, mp->nspec,
-                     G->nshot, G->sstr, mp->d_ibool, mp->d_xix, mp->d_xiz, mp->d_gammax, mp->d_gammaz,
-                     (const float2*)dn, (const float2*)G->Rb, G->gk32, G->gm32, mp->basis);
+  if (G->ps > 1)
+    hipLaunchKernelGGL(k_grad_elastic_minor, dim3((mp->nspec + (64 >> lshift) - 1) / (64 >> lshift)), dim3(64), 0,
+                       mp->stream, mp->nspec, G->nshot, lshift, mp->d_ibool, mp->d_geom, (const float2*)dn,
+                       (const float2*)G->Rb, G->gk32, G->gm32, mp->basis);
+  else
+    hipLaunchKernelGGL(k_grad_elastic, dim3((mp->nspec + GEPB - 1) / GEPB), dim3(GT), 0, mp->stream, mp->nspec,
+                       G->nshot, G->ps, G->stride, mp->d_ibool, mp->d_xix, mp->d_xiz, mp->d_gammax, mp->d_gammaz,
+                       (const float2*)dn, (const float2*)G->Rb, G->gk32, G->gm32, mp->basis);
   check_hip(hipGetLastError(), "sf2d_grad_reverse_step launch");
 }
 

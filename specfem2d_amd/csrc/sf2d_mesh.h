@@ -152,12 +152,15 @@ struct Mesh {
   struct BatchState* batch = nullptr;
 };
 
-// nshot wavefields of one mesh and model, stored shot-major: field k of shot s at (s * stride + g) float2
-// points; every source of the handle belongs to one shot (src_shot).  The Newmark corrector is deferred
-// as for the handle's own fields.
+// nshot wavefields of one mesh and model, point g of shot s at float2 g * ps + s * stride (shot-major: ps = 1;
+// shot-minor: ps = nshot, stride = 1); every source of the handle belongs to one shot (src_shot).  The Newmark
+// corrector is deferred as for the handle's own fields.
 struct BatchState {
   int nshot = 0;
-  long stride = 0;                   // points between shots (>= nglob, a multiple of 64)
+  // layout: shot-major (ps = 1; shot s at s * stride points, stride >= nglob, a multiple of 64) or shot-minor
+  // (ps = nshot; point g of shot s at g * nshot + s, stride = 1), chosen by sf2d_batch_prepare
+  long stride = 0;
+  int ps = 1;
   float *d = nullptr, *v = nullptr, *a = nullptr;
   int* src_shot = nullptr;           // [nsources_local]
   float* seismo = nullptr;           // [nshot][2][nrec][nstep] displacement seismograms
@@ -166,7 +169,7 @@ struct BatchState {
 };
 
 // shot batch fields and the re-arming of its deferred corrector (batch.hip)
-void batch_fields(Mesh* mp, float** d, float** v, float** a, int* nshot, long* stride);
+void batch_fields(Mesh* mp, float** d, float** v, float** a, int* nshot, long* stride, int* ps);
 void batch_set_pending(Mesh* mp, float dto2);
 // time step it of every shot from state (d, v, a) -- corrector pending with dto2_prev, or at rest when d is
 // null -- into state (d_out, v_out, a_out), each [nshot][stride][2]
@@ -178,8 +181,9 @@ void flush_correctors(Mesh* mp);
 // acc -= K u over the elements of phase ph (0 outer, 1 inner), with the handle's assembly scheme, for nshot
 // wavefields shot_stride points apart
 // first_touch: the step's accel was left by a predictor with zero_accel = 0 (first-touch assembly)
+// ps > 1: shot-minor fields of ps shots per point (nshot and shot_stride unused)
 void apply_element_forces(const Mesh* mp, int ph, const float* u, float* acc, int nshot = 1, long shot_stride = 0,
-                          int first_touch = 0);
+                          int first_touch = 0, int ps = 1);
 // point arrays in the caller's numbering <-> the device numbering (n2 values per point)
 void points_to_device(const Mesh* mp, float* dev, const float* host, int n2, const char* what);
 void points_from_device(const Mesh* mp, float* host, const float* dev, int n2, const char* what);
@@ -235,36 +239,41 @@ void device_free(T*& p) {
 // kernel launchers (defined in elastic.hip), all asynchronous on mp->stream
 // zero_accel = 0 leaves accel as it is (first-touch assembly)
 void launch_newmark_predictor(float* d, float* v, float* a, int nglob, float dt, float dt2o2, float dto2,
-                              hipStream_t s, int nshot = 1, long shot_stride = 0, int zero_accel = 1);
+                              hipStream_t s, int nshot = 1, long shot_stride = 0, int zero_accel = 1, int ps = 1);
 void launch_newmark_corrector(float* v, float* a, const float* rmx, const float* rmz, int nglob, float dto2,
-                              int apply_mass, hipStream_t s, int nshot = 1, long shot_stride = 0);
+                              int apply_mass, hipStream_t s, int nshot = 1, long shot_stride = 0, int ps = 1);
 void launch_newmark_corrector_predictor(float* d, float* v, float* a, const float* rmx, const float* rmz, int nglob,
                                        float dto2_prev, float dt, float dt2o2, float dto2, hipStream_t s,
-                                       int nshot = 1, long shot_stride = 0, int zero_accel = 1);
+                                       int nshot = 1, long shot_stride = 0, int zero_accel = 1, int ps = 1);
 // out of place: (d, v, a) with the corrector pending -> (d_out, v_out, 0); the two states must not overlap
 void launch_newmark_corrector_predictor_to(const float* d, const float* v, const float* a, float* d_out, float* v_out,
                                           float* a_out, const float* rmx, const float* rmz, int nglob,
                                           float dto2_prev, float dt, float dt2o2, float dto2, hipStream_t s,
-                                          int nshot = 1, long shot_stride = 0, int zero_accel = 1);
+                                          int nshot = 1, long shot_stride = 0, int zero_accel = 1, int ps = 1);
 // accel = 0 at the special points of first-touch assembly
-void launch_zero_points(const Mesh* mp, float* accel, int nshot = 1, long shot_stride = 0);
+void launch_zero_points(const Mesh* mp, float* accel, int nshot = 1, long shot_stride = 0, int ps = 1);
 void launch_elastic_forces(const Mesh* mp, int first, int nelem, int atomic, const float* displ, float* accel);
 // nshot wavefields at a stride of shot_stride points (float2) in displ and accel
 void launch_elastic_forces_patch(const Mesh* mp, int first_patch, int npatch, const float* displ, float* accel,
                                  int nshot = 1, long shot_stride = 0, int first_touch = 0);
+// shot-minor batches: fields [point][nshot] (chain patches only)
+void launch_elastic_forces_walk(const Mesh* mp, int first_patch, int npatch, const float* displ, float* accel,
+                                int nshot, int first_touch);
 void launch_permute_elements(const float* src, float* dst, const int* ispec_of_pos, int nspec, hipStream_t s);
 void launch_pack_elements(const Mesh* mp);  // geom / moduli from the per-element arrays (patch mode)
 // per-element geometry and the affine flags of the patches; returns the number of affine elements
 int launch_flag_affine(Mesh* mp);
 void launch_permute_elements_int(const int* src, int* dst, const int* ispec_of_pos, int nspec, hipStream_t s);
+// ps: point stride (float2) of the fields -- the shots per point of a shot-minor batch, 1 otherwise
 void launch_stacey(const Mesh* mp, const float* veloc, float* accel, int write_store, int nshot = 1,
-                   long shot_stride = 0);
+                   long shot_stride = 0, int ps = 1);
 void launch_stacey_backward(const Mesh* mp, float* b_accel);
 // src_shot (device, one entry per source) sends each source to its own wavefield; null: all into accel
-void launch_sources(const Mesh* mp, float* accel, int it, const int* src_shot = nullptr, long shot_stride = 0);
+void launch_sources(const Mesh* mp, float* accel, int it, const int* src_shot = nullptr, long shot_stride = 0,
+                    int ps = 1);
 void launch_adjoint_sources(const Mesh* mp, float* accel, int it);
 void launch_seismograms(const Mesh* mp, const float* field, float* dst, int seismo_current, int nshot = 1,
-                        long shot_stride = 0, long dst_stride = 0);
+                        long shot_stride = 0, long dst_stride = 0, int ps = 1);
 void launch_kernels(const Mesh* mp);
 void launch_max_norm(const Mesh* mp, const float* field, unsigned int* result);
 

@@ -100,6 +100,8 @@ def load_libraries():
     gpu.sf2d_mesh_stream.restype = P
     gpu.sf2d_mesh_affine_elements.argtypes = [C.POINTER(C.c_long)]
     gpu.sf2d_mesh_affine_elements.restype = C.c_int
+    gpu.sf2d_batch_shot_minor.argtypes = [C.POINTER(C.c_long)]
+    gpu.sf2d_batch_shot_minor.restype = C.c_int
     gpu.sf2d_mesh_chain.argtypes = [C.POINTER(C.c_long)]
     gpu.sf2d_mesh_chain.restype = C.c_int
     gpu.sf2d_mesh_colours.argtypes = [C.POINTER(C.c_long), C.c_int]
@@ -408,6 +410,12 @@ class ShotBatch(Simulation):
 
     def _source_list(self):
         return [s for shot in self.shots for s in shot]
+
+    def shot_minor(self) -> bool:
+        """Whether the batch fields are shot-minor (the lane-per-shot force kernel)."""
+        self._need_device()
+        mp = C.c_long(self.mesh_pointer)
+        return self.gpu.sf2d_batch_shot_minor(C.byref(mp)) == 1
 
     def shot_t0(self):
         out = np.zeros(self.nshot)

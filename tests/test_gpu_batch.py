@@ -1,8 +1,8 @@
 """GPU parity of shot batches (all shots of one mesh and model advanced together, one launch per kernel).
 
 * forward: every shot's seismograms and final fields equal those of a single simulation with only that
-  shot's sources (rel L2 1e-6: the runs differ only in the order of FP32 atomic additions at source and
-  corner points), and the FP32 oracle's traces of that simulation to the parity tolerance 1e-5;
+  shot's sources (rel L2 2e-6 on the traces: FP32 summation order), and the FP32 oracle's traces of that
+  simulation to the parity tolerance 1e-5; shot-minor and shot-major batch layouts agree;
 * gradient: the batch misfit is the sum of the single misfits (per shot too), the batch gradient the sum of
   the single gradients (rel L2 1e-5), and the sum of the FP64 oracle's gradients to the tolerance of the
   single-shot gradient test; checkpoint length does not change it; the torch.autograd misfit runs on a batch.
@@ -50,12 +50,42 @@ def test_batch_forward_matches_single_runs(case, singles):
     for k, (ref, fields) in enumerate(singles):
         err = _rel(seis[k], ref)
         print(f"shot {k}: batch vs single seismograms rel L2 {err:.2e}")
-        assert err < 1e-6, (k, err)
+        assert err < 2e-6, (k, err)
         for name, x, y in zip(("displ", "veloc", "accel"), b.fields(k), fields):
             e = _rel(x, y)
-            assert e < 1e-6, (k, name, e)
+            print(f"  {name} rel L2 {e:.2e}")
+            assert e < {"displ": 5e-6, "veloc": 2e-5, "accel": 5e-5}[name], (k, name, e)
     print(f"batch of {len(shots)} shots, {cfg.nstep} steps: {ms:.1f} ms")
     b.close()
+
+
+def test_shot_minor_and_shot_major_layouts_agree(case, observed, monkeypatch):
+    """The lane-per-shot kernels on shot-minor fields (SF2D_SHOT_MINOR) and the patch kernels on shot-major
+    fields (the default) give the same seismograms and the same summed gradient to FP32 rounding."""
+    from specfem2d_amd.driver import ShotBatch
+    cfg, shots = case
+    out = {}
+    monkeypatch.delenv("SF2D_SHOT_MAJOR", raising=False)
+    for major in (False, True):
+        if major:
+            monkeypatch.delenv("SF2D_SHOT_MINOR", raising=False)
+        else:
+            monkeypatch.setenv("SF2D_SHOT_MINOR", "1")
+        b = ShotBatch(cfg, shots)
+        assert b.shot_minor() == (not major)
+        seis = b.forward()
+        b.gradient_forward(observed, chunk=64)
+        out[major] = (seis, b.gradient_backward(), b.fields(2))
+        b.close()
+    err = _rel(out[False][0], out[True][0])
+    print(f"shot-minor vs shot-major: seismograms rel L2 {err:.2e}")
+    assert err < 2e-6
+    for name in ("rho", "vp", "vs"):
+        e = _rel(out[False][1][name], out[True][1][name])
+        print(f"  gradient {name}: {e:.2e}")
+        assert e < 1e-5
+    for x, y, tol in zip(out[False][2][:2], out[True][2][:2], (5e-6, 2e-5)):
+        assert _rel(x, y) < tol
 
 
 def test_batch_forward_matches_the_oracle(case):
@@ -104,12 +134,14 @@ def test_batch_gradient_is_the_sum_of_single_gradients(case, observed, batch_gra
         gk = sim.gradient_backward()
         sim.close()
         chis.append(c)
-        assert _rel(seis[k], s) < 1e-6
+        assert _rel(seis[k], s) < 2e-6
         for name in total:
             total[name] = total[name] + gk[name]
     print(f"chi batch {chi:.6e}, sum of singles {sum(chis):.6e}, per shot {per_shot} vs {chis}")
     assert abs(chi - sum(chis)) < 1e-6 * sum(chis)
-    np.testing.assert_allclose(per_shot, chis, rtol=1e-6)
+    # per shot against the total: a shot whose synthetics nearly equal its observations has a misfit made of
+    # rounding-sensitive residuals
+    np.testing.assert_allclose(per_shot, chis, rtol=0, atol=1e-6 * sum(chis))
     for name in total:
         err = _rel(g[name], total[name])
         print(f"batch gradient {name}: rel L2 vs sum of single gradients {err:.2e}")
