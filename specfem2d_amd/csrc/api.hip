@@ -89,7 +89,8 @@ int colour_elements(const std::vector<int>& ibool, const std::vector<int>& elems
 // Groups a phase's element list into patches: runs of consecutive list entries, at most PATCH_ELEMS long,
 // in which every element shares a point with the patch built so far.  For the internal mesher's row-major
 // element order these are row segments of PATCH_ELEMS elements.
-std::vector<int> make_patches(const std::vector<int>& ibool, const std::vector<int>& elems, int nglob) {
+std::vector<int> make_patches(const std::vector<int>& ibool, const std::vector<int>& elems, int nglob,
+                              int max_elems) {
   std::vector<int> start;  // index in elems of each patch's first element, plus elems.size()
   std::vector<int> mark((size_t)nglob, -1);
   int cur = -1, size = 0;
@@ -98,7 +99,7 @@ std::vector<int> make_patches(const std::vector<int>& ibool, const std::vector<i
     bool touches = false;
     if (cur >= 0)
       for (int p = 0; p < NGLL2 && !touches; p++) touches = (mark[ib[p]] == cur);
-    if (cur < 0 || size == Mesh::PATCH_ELEMS || !touches) {
+    if (cur < 0 || size == max_elems || !touches) {
       cur = (int)start.size();
       start.push_back((int)k);
       size = 0;
@@ -161,7 +162,8 @@ bool is_chain(const std::vector<int>& ibool, const std::vector<int>& elems, int 
 
 int desc_counts(int ne, int nown, int nhalo) { return ne | (nown << 8) | (nhalo << 20); }
 
-Plan build_plan(const std::vector<int>& ibool, int nspec, int nglob, const int* phase_lists, int np, int mode) {
+Plan build_plan(const std::vector<int>& ibool, int nspec, int nglob, const int* phase_lists, int np, int mode,
+                int max_elems = Mesh::PATCH_ELEMS) {
   Plan P;
   std::vector<int>& order = P.order;
   order.reserve(nspec);
@@ -207,7 +209,7 @@ Plan build_plan(const std::vector<int>& ibool, int nspec, int nglob, const int* 
       continue;
     }
     // coloured patches; ownership and numbering in phase-list order
-    const std::vector<int> pstart = make_patches(ibool, elems, nglob);
+    const std::vector<int> pstart = make_patches(ibool, elems, nglob, max_elems);
     const int np_ph = (int)pstart.size() - 1;
     for (int q = 0; q < np_ph && P.chains; q++) P.chains = is_chain(ibool, elems, pstart[q], pstart[q + 1]);
     std::vector<std::vector<int>> pnodes(np_ph), phalo(np_ph);
@@ -345,7 +347,12 @@ void upload_chain_ids(Mesh* mp) {
 
 void sort_elements_by_colour(Mesh* mp, const int* phase_lists, int np) {
   const int nspec = mp->nspec;
-  Plan P = build_plan(mp->h_ibool, nspec, mp->nglob, phase_lists, np, mp->assembly);
+  // patch mode: chains of up to CHAIN_ELEMS elements when every patch is one (the chain kernel), otherwise
+  // patches of up to PATCH_ELEMS for the general patch kernel
+  const bool try_chain = mp->assembly == 0 && !getenv("SF2D_NO_CHAIN");
+  Plan P = build_plan(mp->h_ibool, nspec, mp->nglob, phase_lists, np, mp->assembly,
+                      try_chain ? Mesh::CHAIN_ELEMS : Mesh::PATCH_ELEMS);
+  if (try_chain && !P.chains) P = build_plan(mp->h_ibool, nspec, mp->nglob, phase_lists, np, mp->assembly);
   std::vector<int>& order = P.order;
   for (int ph = 0; ph < 2; ph++) {
     mp->phase_count[ph] = P.phase_count[ph];
@@ -364,7 +371,7 @@ void sort_elements_by_colour(Mesh* mp, const int* phase_lists, int np) {
     mp->d_halo_nodes = device_copy(P.halo.data(), P.halo.size());
     mp->d_elem_slot = device_copy(P.slot.data(), P.slot.size());
     mp->point_id = std::move(P.newid);
-    mp->chain = P.chains && !getenv("SF2D_NO_CHAIN");
+    mp->chain = P.chains && try_chain;
   }
 
   // move the per-element device arrays
@@ -1316,7 +1323,8 @@ extern "C" int sf2d_mesh_affine_elements(long* Mesh_pointer) {
 extern "C" int sf2d_plan_chains(int nspec, int nglob, const int* ibool_1based, int np, const int* phase_lists,
                                 int* first_writes) {
   std::vector<int> ib = to_zero_based(ibool_1based, (size_t)nspec * NGLL2, 1, nglob, "ibool");
-  Plan P = build_plan(ib, nspec, nglob, phase_lists, np, 0);
+  Plan P = build_plan(ib, nspec, nglob, phase_lists, np, 0, Mesh::CHAIN_ELEMS);  // as the device plan is chosen
+  if (!P.chains) P = build_plan(ib, nspec, nglob, phase_lists, np, 0);
   if (first_writes) {
     *first_writes = 0;
     for (uint8_t f : P.first) *first_writes += f;

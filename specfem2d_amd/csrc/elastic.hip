@@ -26,7 +26,9 @@ constexpr int NXCD = 8;        // gfx950: 8 accelerator complex dies, workgroups
 static_assert(Mesh::PATCH_ELEMS == FORCE_EPB, "one workgroup holds one patch");
 static_assert(FORCE_EPB * NGLL2 <= FORCE_THREADS && FORCE_THREADS <= 256, "patch points indexed by 8-bit slots");
 static_assert(Mesh::PATCH_SHARE == 4, "force cells are summed as two float4 per point");
-static_assert(Mesh::PATCH_ELEMS <= 15, "the element count of a patch descriptor has 4 bits");
+static_assert(Mesh::PATCH_ELEMS <= 15 && Mesh::CHAIN_ELEMS <= 15, "the element count of a patch descriptor has 4 bits");
+static_assert(Mesh::CHAIN_ELEMS * NGLL <= 64 && (4 * Mesh::CHAIN_ELEMS + 1) * NGLL <= 256,
+              "a chain patch fills one wave by rows and has 8-bit point slots");
 
 __device__ __forceinline__ void atomic_add_f32(float* p, float v) { unsafeAtomicAdd(p, v); }
 
@@ -473,9 +475,9 @@ __global__ __launch_bounds__(CHAIN_LANES) void k_elastic_forces_chain(
     const int* __restrict__ ids, const float4* __restrict__ geom, const float4* __restrict__ geom_elem,
     const float2* __restrict__ moduli, const float2* __restrict__ displ, float2* __restrict__ accel,
     const Basis basis) {
-  // [element][row][column], rows padded to 6; element FORCE_EPB is scratch for lanes past the patch
-  __shared__ float2 s_t[FORCE_EPB + 1][NGLL][NGLL + 1];
-  __shared__ float2 s_v[FORCE_EPB + 1][NGLL][NGLL + 1];
+  // [element][row][column], rows padded to 6; element CHAIN_ELEMS is scratch for lanes past the patch
+  __shared__ float2 s_t[Mesh::CHAIN_ELEMS + 1][NGLL][NGLL + 1];
+  __shared__ float2 s_v[Mesh::CHAIN_ELEMS + 1][NGLL][NGLL + 1];
   __shared__ float s_hp[NGLL2], s_hpw[NGLL2];
   const int lane = threadIdx.x;
   if (lane < NGLL2) {
@@ -492,7 +494,7 @@ __global__ __launch_bounds__(CHAIN_LANES) void k_elastic_forces_chain(
   const int4 d = desc[first_patch + item / ngroup];
   const int s0 = (item % ngroup) * group, s1 = min(nshot, s0 + group);
   const int ne = patch_ne(d.w);
-  const int k = min(lane / NGLL, FORCE_EPB);  // element of the patch (FORCE_EPB: lanes past the patch)
+  const int k = min(lane / NGLL, Mesh::CHAIN_ELEMS);  // element of the patch (CHAIN_ELEMS: lanes past it)
   const int j = lane % NGLL;
   const bool active = lane / NGLL < ne;
   const int e = d.x + min(k, ne - 1);

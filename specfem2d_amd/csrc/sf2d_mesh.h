@@ -68,6 +68,7 @@ struct Mesh {
   // renumbered patch-major: each patch owns a contiguous range of point ids and lists the few points it
   // shares with earlier patches (its halo).  Every point array on the device uses that numbering.
   static constexpr int PATCH_ELEMS = 10;
+  static constexpr int CHAIN_ELEMS = 12;  // patches of meshes whose patches are all chains (the chain kernel)
   static constexpr int PATCH_SHARE = 4;  // element points of one patch that may share a point
   int npatch = 0;
   int4* d_patch_desc = nullptr;      // [npatch] {first element, first owned id, halo offset, ne | nown<<8 | nhalo<<20}
