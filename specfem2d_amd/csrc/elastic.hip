@@ -465,7 +465,7 @@ __global__ __launch_bounds__(FORCE_THREADS) void k_elastic_forces_patch(
 // point in the step, and that colour writes the force instead of adding it to accel, so the predictor does not
 // have to clear accel and the first update reads nothing.
 constexpr int CHAIN_LANES = 64;
-constexpr int CHAIN_SHOTS_PER_WAVE = 4;  // shots per wave in batched launches (SF2D_SHOTS_PER_WAVE overrides)
+constexpr int CHAIN_SHOTS_PER_WAVE = 8;  // shots per wave in batched launches (SF2D_SHOTS_PER_WAVE overrides)
 //
 // Shots: an item is a patch and a group of up to `group` consecutive shots; the wave loads the patch's element
 // data once and runs the shots of its group one after the other.
@@ -519,11 +519,18 @@ __global__ __launch_bounds__(CHAIN_LANES) void k_elastic_forces_chain(
     for (int i = 0; i < NGLL; i++) geo[i] = ld_off(geom, 16u * (l0 + i));
   }
   const bool last = k == ne - 1;
+  float2 un[NGLL];  // the displacement of the group's next shot, loaded while the current one is computed
+#pragma unroll
+  for (int i = 0; i < NGLL; i++) un[i] = ld_off(displ + (long)s0 * shot_stride, 8u * (unsigned)g[i]);
   for (int shot = s0; shot < s1; shot++) {
   const long base = (long)shot * shot_stride;
-  const float2* ds = displ + base;
 #pragma unroll
-  for (int i = 0; i < NGLL; i++) u[i] = ld_off(ds, 8u * (unsigned)g[i]);
+  for (int i = 0; i < NGLL; i++) u[i] = un[i];
+  if (MULTI && shot + 1 < s1) {
+    const float2* dn = displ + base + shot_stride;
+#pragma unroll
+    for (int i = 0; i < NGLL; i++) un[i] = ld_off(dn, 8u * (unsigned)g[i]);
+  }
   float2* as = accel + base;
   float2 a[NGLL];
 #pragma unroll
