@@ -41,6 +41,9 @@ int sf2d_plan_chains(int nspec, int nglob, const int* ibool_1based, int np, cons
 /* 1 when the handle's element forces run on the row-lane kernel for chain patches (0 with SF2D_NO_CHAIN set
    or for other meshes) */
 int sf2d_mesh_chain(long* Mesh_pointer);
+/* 1 when every element is affine and axis-aligned (xi_z = gamma_x = 0 to 2^-22 of the element's largest geometry
+   entry): the chain kernel then uses its rectangle form (SF2D_NO_RECT keeps the general form) */
+int sf2d_mesh_rect(long* Mesh_pointer);
 /* device pointer of a field: 0 displ 1 veloc 2 accel 3 b_displ 4 b_veloc 5 b_accel
    6 rho_kl 7 mu_kl 8 kappa_kl (NULL when not allocated).  On the device, point arrays are in the
    library's internal point numbering and kernels in its element order; the transfer entry points

@@ -1334,6 +1334,8 @@ extern "C" int sf2d_plan_chains(int nspec, int nglob, const int* ibool_1based, i
 
 extern "C" int sf2d_mesh_chain(long* Mesh_pointer) { return handle(Mesh_pointer)->chain; }
 
+extern "C" int sf2d_mesh_rect(long* Mesh_pointer) { return handle(Mesh_pointer)->rect; }
+
 extern "C" int sf2d_mesh_colours(long* Mesh_pointer, int iphase) {
   Mesh* mp = handle(Mesh_pointer);
   return (iphase == 1 || iphase == 2) ? mp->ncolour[iphase - 1] : -1;

@@ -469,7 +469,7 @@ constexpr int CHAIN_SHOTS_PER_WAVE = 8;  // shots per wave in batched launches (
 //
 // Shots: an item is a patch and a group of up to `group` consecutive shots; the wave loads the patch's element
 // data once and runs the shots of its group one after the other.
-template <bool FIRST, bool MULTI>
+template <bool FIRST, bool MULTI, bool RECT>
 __global__ __launch_bounds__(CHAIN_LANES) void k_elastic_forces_chain(
     int first_patch, int npatch, int nshot, int group, long shot_stride, const int4* __restrict__ desc,
     const int* __restrict__ ids, const float4* __restrict__ geom, const float4* __restrict__ geom_elem,
@@ -510,7 +510,7 @@ __global__ __launch_bounds__(CHAIN_LANES) void k_elastic_forces_chain(
     first[i] = FIRST && (id & Mesh::FIRST_TOUCH);
     mod[i] = ld_off(moduli, 8u * (l0 + i));
   }
-  if (patch_affine(d.w)) {
+  if (RECT || patch_affine(d.w)) {
     const float4 ge = ld_off(geom_elem, 16u * (unsigned)e);
 #pragma unroll
     for (int i = 0; i < NGLL; i++) geo[i] = ge;
@@ -573,20 +573,32 @@ __global__ __launch_bounds__(CHAIN_LANES) void k_elastic_forces_chain(
       duz_dxi += u[m].y * h;
     }
     const float2 dg = s_v[k][j][i];
-    const float xixl = geo[i].x, xizl = geo[i].y, gaxl = geo[i].z, gazl = geo[i].w;
-    const float dux_dxl = dux_dxi * xixl + dg.x * gaxl;
-    const float dux_dzl = dux_dxi * xizl + dg.x * gazl;
-    const float duz_dxl = duz_dxi * xixl + dg.y * gaxl;
-    const float duz_dzl = duz_dxi * xizl + dg.y * gazl;
-    const float jac = 1.f / (xixl * gazl - gaxl * xizl);
     const float kap = mod[i].x, mu = mod[i].y;
     const float lambdal = kap - mu;
     const float lp2m = kap + mu;
-    const float sxx = lp2m * dux_dxl + lambdal * duz_dzl;
-    const float szz = lp2m * duz_dzl + lambdal * dux_dxl;
-    const float sxz = mu * (duz_dxl + dux_dzl);
-    t1[i] = make_float2(jac * (sxx * xixl + sxz * xizl), jac * (sxz * xixl + szz * xizl));
-    t2[i] = make_float2(jac * (sxx * gaxl + sxz * gazl), jac * (sxz * gaxl + szz * gazl));
+    if (RECT) {  // xi_z = gamma_x = 0: J xi_x = 1 / gamma_z, J gamma_z = 1 / xi_x
+      const float xixl = geo[i].x, gazl = geo[i].w;
+      const float dux_dxl = dux_dxi * xixl, dux_dzl = dg.x * gazl;
+      const float duz_dxl = duz_dxi * xixl, duz_dzl = dg.y * gazl;
+      const float sxx = lp2m * dux_dxl + lambdal * duz_dzl;
+      const float szz = lp2m * duz_dzl + lambdal * dux_dxl;
+      const float sxz = mu * (duz_dxl + dux_dzl);
+      const float cx = 1.f / gazl, cz = 1.f / xixl;
+      t1[i] = make_float2(cx * sxx, cx * sxz);
+      t2[i] = make_float2(cz * sxz, cz * szz);
+    } else {
+      const float xixl = geo[i].x, xizl = geo[i].y, gaxl = geo[i].z, gazl = geo[i].w;
+      const float dux_dxl = dux_dxi * xixl + dg.x * gaxl;
+      const float dux_dzl = dux_dxi * xizl + dg.x * gazl;
+      const float duz_dxl = duz_dxi * xixl + dg.y * gaxl;
+      const float duz_dzl = duz_dxi * xizl + dg.y * gazl;
+      const float jac = 1.f / (xixl * gazl - gaxl * xizl);
+      const float sxx = lp2m * dux_dxl + lambdal * duz_dzl;
+      const float szz = lp2m * duz_dzl + lambdal * dux_dxl;
+      const float sxz = mu * (duz_dxl + dux_dzl);
+      t1[i] = make_float2(jac * (sxx * xixl + sxz * xizl), jac * (sxz * xixl + szz * xizl));
+      t2[i] = make_float2(jac * (sxx * gaxl + sxz * gazl), jac * (sxz * gaxl + szz * gazl));
+    }
   }
   __syncthreads();  // every lane has read its derivatives and the tile's u
 #pragma unroll
@@ -817,8 +829,10 @@ __global__ __launch_bounds__(256) void k_affine_elements(int nspec, const float4
     same &= (fabsf(g.x - g0.x) <= tol) & (fabsf(g.y - g0.y) <= tol) & (fabsf(g.z - g0.z) <= tol) &
             (fabsf(g.w - g0.w) <= tol);
   }
+  // 2: affine and axis-aligned (xi_z and gamma_x zero to the same bound), 1: affine, 0: neither
+  const int rect = same && fabsf(g0.y) <= tol && fabsf(g0.z) <= tol;
   geom_elem[e] = g0;
-  affine[e] = same;
+  affine[e] = same ? 1 + rect : 0;
 }
 
 // sets the affine flag (bit 7 of the counts word) of every patch whose elements are all affine
@@ -829,7 +843,7 @@ __global__ __launch_bounds__(256) void k_flag_affine_patches(int npatch, int4* _
   int4 d = desc[q];
   const int ne = d.w & 0xf;
   int all = 1;
-  for (int k = 0; k < ne; k++) all &= affine[d.x + k];
+  for (int k = 0; k < ne; k++) all &= affine[d.x + k] != 0;
   d.w = all ? (d.w | 0x80) : (d.w & ~0x80);
   desc[q] = d;
 }
@@ -1207,8 +1221,13 @@ void launch_elastic_forces_patch(const Mesh* mp, int first_patch, int npatch, co
     const int group = std::min(nshot, group_env > 0 ? group_env : CHAIN_SHOTS_PER_WAVE);
     const int items = npatch * ((nshot + group - 1) / group);
     const int grid = (items + NXCD - 1) / NXCD * NXCD;
-    auto kern = group > 1 ? (first_touch ? k_elastic_forces_chain<true, true> : k_elastic_forces_chain<false, true>)
-                          : (first_touch ? k_elastic_forces_chain<true, false> : k_elastic_forces_chain<false, false>);
+    using K = decltype(&k_elastic_forces_chain<true, true, true>);
+    const K kerns[2][2][2] = {
+        {{k_elastic_forces_chain<false, false, false>, k_elastic_forces_chain<false, false, true>},
+         {k_elastic_forces_chain<false, true, false>, k_elastic_forces_chain<false, true, true>}},
+        {{k_elastic_forces_chain<true, false, false>, k_elastic_forces_chain<true, false, true>},
+         {k_elastic_forces_chain<true, true, false>, k_elastic_forces_chain<true, true, true>}}};
+    const K kern = kerns[first_touch ? 1 : 0][group > 1 ? 1 : 0][mp->rect ? 1 : 0];
     hipLaunchKernelGGL(kern, dim3(grid), dim3(CHAIN_LANES), 0, mp->stream, first_patch, npatch, nshot, group,
                        shot_stride, (const int4*)mp->d_patch_desc, mp->d_chain_ids, mp->d_geom, mp->d_geom_elem,
                        mp->d_moduli, (const float2*)displ, (float2*)accel, mp->basis);
@@ -1266,8 +1285,13 @@ int launch_flag_affine(Mesh* mp) {
   check_hip(hipMemcpyAsync(h.data(), d_aff, h.size() * sizeof(int), hipMemcpyDeviceToHost, mp->stream), "affine D2H");
   check_hip(hipStreamSynchronize(mp->stream), "affine flags");
   device_free(d_aff);
-  int n = 0;
-  for (int v : h) n += v;
+  int n = 0, nrect = 0;
+  for (int v : h) {
+    n += v != 0;
+    nrect += v == 2;
+  }
+  // every element affine and axis-aligned: the chain kernel's rectangle form (SF2D_NO_RECT keeps the general one)
+  mp->rect = !mp->no_affine && nrect == mp->nspec && !getenv("SF2D_NO_RECT");
   return n;
 }
 

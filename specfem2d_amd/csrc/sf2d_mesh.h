@@ -79,6 +79,7 @@ struct Mesh {
   float4* d_geom_elem = nullptr;     // [nspec] geometry of point 0; the whole element's when it is affine
   int n_affine = 0;                  // elements whose 25 points carry bitwise the same geometry
   int no_affine = 0;                 // SF2D_NO_AFFINE: always read the per-point geometry
+  int rect = 0;                      // every element affine and axis-aligned (chain kernel's rectangle form)
   int chain = 0;                     // every patch is a chain of elements: the row-lane force kernel (SF2D_NO_CHAIN
                                      // keeps the general patch kernel)
   int* d_chain_ids = nullptr;        // chain kernel: [nspec][25] device order, internal point id | FIRST_TOUCH

@@ -102,6 +102,8 @@ def load_libraries():
     gpu.sf2d_mesh_affine_elements.restype = C.c_int
     gpu.sf2d_batch_shot_minor.argtypes = [C.POINTER(C.c_long)]
     gpu.sf2d_batch_shot_minor.restype = C.c_int
+    gpu.sf2d_mesh_rect.argtypes = [C.POINTER(C.c_long)]
+    gpu.sf2d_mesh_rect.restype = C.c_int
     gpu.sf2d_mesh_chain.argtypes = [C.POINTER(C.c_long)]
     gpu.sf2d_mesh_chain.restype = C.c_int
     gpu.sf2d_mesh_colours.argtypes = [C.POINTER(C.c_long), C.c_int]
@@ -217,6 +219,12 @@ class Simulation:
         self._need_device()
         mp = C.c_long(self.mesh_pointer)
         return bool(self.gpu.sf2d_mesh_chain(C.byref(mp)))
+
+    def rect_kernel(self) -> bool:
+        """Whether the chain kernel uses its rectangle form (every element affine and axis-aligned)."""
+        self._need_device()
+        mp = C.c_long(self.mesh_pointer)
+        return bool(self.gpu.sf2d_mesh_rect(C.byref(mp)))
 
     def colours(self, iphase: int = 2) -> int:
         mp = C.c_long(self.mesh_pointer)
