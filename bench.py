@@ -133,12 +133,18 @@ def fwi_gradient_batched(nshot: int = 32, repeats: int = 2, nstep: int = 1500):
     Observed data: the shots run in the random-field model; synthetics: the homogeneous background."""
     from specfem2d_amd import config
     from specfem2d_amd.driver import ShotBatch, Simulation
+    from specfem2d_amd import autotune
     true_cfg = config.batched_fwi_600x300(nstep=nstep, true_model=True)
     cfg = config.batched_fwi_600x300(nstep=nstep)
     shots = config.surface_shots(cfg, nshot)
     b = ShotBatch(true_cfg, shots)
     obs = b.forward()
     b.close()
+    saved = {k: os.environ.get(k) for k in ("SF2D_CHAIN_ELEMS", "SF2D_SHOTS_PER_WAVE")}
+    # launch shapes tuned on this workload's mesh for each path (short forwards from rest)
+    tuned_plain = autotune.tune(cfg.replace(sources=shots[0]), None, steps=20)
+    tuned_batch = autotune.tune(cfg, shots, steps=20)
+    autotune.apply(tuned_plain)
     sim = Simulation(cfg.replace(sources=shots[0]))
     plain = []
     for _ in range(repeats):  # the first run includes the allocation of the checkpoint buffers
@@ -147,6 +153,9 @@ def fwi_gradient_batched(nshot: int = 32, repeats: int = 2, nstep: int = 1500):
         sim.gradient_backward()
         plain.append(time.perf_counter() - t0)
     sim.close()
+    for k, v in saved.items():
+        os.environ.pop(k, None) if v is None else os.environ.__setitem__(k, v)
+    autotune.apply(tuned_batch)
     chunk = int(round(np.sqrt(cfg.nstep)))
     b = ShotBatch(cfg, shots)
     batched = []
@@ -156,6 +165,8 @@ def fwi_gradient_batched(nshot: int = 32, repeats: int = 2, nstep: int = 1500):
         g = b.gradient_backward()
         batched.append(time.perf_counter() - t0)
     b.close()
+    for k, v in saved.items():
+        os.environ.pop(k, None) if v is None else os.environ.__setitem__(k, v)
     gnorm = float(np.linalg.norm(g["vs"]))
     if not (np.isfinite(chi) and chi > 0 and np.isfinite(gnorm) and gnorm > 0):
         raise RuntimeError("degenerate batched gradient")
@@ -163,6 +174,8 @@ def fwi_gradient_batched(nshot: int = 32, repeats: int = 2, nstep: int = 1500):
     return {"workload": "batched_fwi_600x300", "nstep": cfg.nstep, "nglob": cfg.nglob, "shots_per_batch": nshot,
             "plain_s_per_shot": sp, "batched_s_per_shot": sb, "speedup_per_shot": sp / sb,
             "checkpoint_every": {"plain": 100, "batched": chunk}, "misfit_sum": chi, "grad_vs_l2": gnorm,
+            "autotuned": {"plain": {k: tuned_plain[k] for k in ("chain_elems", "shots_per_wave")},
+                          "batched": {k: tuned_batch[k] for k in ("chain_elems", "shots_per_wave")}},
             "note": "64 shots = 2 batches of 32 (checkpoint memory)"}
 
 

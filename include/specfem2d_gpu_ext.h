@@ -44,6 +44,11 @@ int sf2d_mesh_chain(long* Mesh_pointer);
 /* 1 when every element is affine and axis-aligned (xi_z = gamma_x = 0 to 2^-22 of the element's largest geometry
    entry): the chain kernel then uses its rectangle form (SF2D_NO_RECT keeps the general form) */
 int sf2d_mesh_rect(long* Mesh_pointer);
+/* tuning knobs (specfem2d_amd/autotune.py picks them by timing): elements per chain patch, chosen at set-up
+   from SF2D_CHAIN_ELEMS (1..12, default 12), and shots per wave of the batched chain kernel (0 = default 8;
+   SF2D_SHOTS_PER_WAVE at set-up) */
+int sf2d_mesh_chain_elems(long* Mesh_pointer);
+void sf2d_mesh_set_shots_per_wave(long* Mesh_pointer, int shots);
 /* device pointer of a field: 0 displ 1 veloc 2 accel 3 b_displ 4 b_veloc 5 b_accel
    6 rho_kl 7 mu_kl 8 kappa_kl (NULL when not allocated).  On the device, point arrays are in the
    library's internal point numbering and kernels in its element order; the transfer entry points

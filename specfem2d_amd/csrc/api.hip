@@ -350,8 +350,15 @@ void sort_elements_by_colour(Mesh* mp, const int* phase_lists, int np) {
   // patch mode: chains of up to CHAIN_ELEMS elements when every patch is one (the chain kernel), otherwise
   // patches of up to PATCH_ELEMS for the general patch kernel
   const bool try_chain = mp->assembly == 0 && !getenv("SF2D_NO_CHAIN");
+  mp->chain_elems = Mesh::CHAIN_ELEMS;
+  if (const char* env = getenv("SF2D_CHAIN_ELEMS")) {
+    const int n = atoi(env);
+    if (n < 1 || n > Mesh::CHAIN_ELEMS) fail("SF2D_CHAIN_ELEMS=%s outside 1..%d", env, Mesh::CHAIN_ELEMS);
+    mp->chain_elems = n;
+  }
+  if (const char* env = getenv("SF2D_SHOTS_PER_WAVE")) mp->shots_per_wave = std::max(1, atoi(env));
   Plan P = build_plan(mp->h_ibool, nspec, mp->nglob, phase_lists, np, mp->assembly,
-                      try_chain ? Mesh::CHAIN_ELEMS : Mesh::PATCH_ELEMS);
+                      try_chain ? mp->chain_elems : Mesh::PATCH_ELEMS);
   if (try_chain && !P.chains) P = build_plan(mp->h_ibool, nspec, mp->nglob, phase_lists, np, mp->assembly);
   std::vector<int>& order = P.order;
   for (int ph = 0; ph < 2; ph++) {
@@ -1335,6 +1342,15 @@ extern "C" int sf2d_plan_chains(int nspec, int nglob, const int* ibool_1based, i
 extern "C" int sf2d_mesh_chain(long* Mesh_pointer) { return handle(Mesh_pointer)->chain; }
 
 extern "C" int sf2d_mesh_rect(long* Mesh_pointer) { return handle(Mesh_pointer)->rect; }
+
+// tuning knobs of a handle: elements per chain patch (fixed at set-up) and shots per wave of batched launches
+extern "C" int sf2d_mesh_chain_elems(long* Mesh_pointer) { return handle(Mesh_pointer)->chain_elems; }
+
+extern "C" void sf2d_mesh_set_shots_per_wave(long* Mesh_pointer, int shots) {
+  Mesh* mp = handle(Mesh_pointer);
+  if (shots < 0 || shots > 64) fail("sf2d_mesh_set_shots_per_wave: %d outside 0..64", shots);
+  mp->shots_per_wave = shots;
+}
 
 extern "C" int sf2d_mesh_colours(long* Mesh_pointer, int iphase) {
   Mesh* mp = handle(Mesh_pointer);

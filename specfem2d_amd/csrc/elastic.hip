@@ -465,7 +465,7 @@ __global__ __launch_bounds__(FORCE_THREADS) void k_elastic_forces_patch(
 // point in the step, and that colour writes the force instead of adding it to accel, so the predictor does not
 // have to clear accel and the first update reads nothing.
 constexpr int CHAIN_LANES = 64;
-constexpr int CHAIN_SHOTS_PER_WAVE = 8;  // shots per wave in batched launches (SF2D_SHOTS_PER_WAVE overrides)
+constexpr int CHAIN_SHOTS_PER_WAVE = 8;  // shots per wave in batched launches unless the handle is tuned
 //
 // Shots: an item is a patch and a group of up to `group` consecutive shots; the wave loads the patch's element
 // data once and runs the shots of its group one after the other.
@@ -1213,12 +1213,7 @@ void launch_elastic_forces_patch(const Mesh* mp, int first_patch, int npatch, co
   const int all = (npatch * nshot + NXCD - 1) / NXCD * NXCD;
   if (mp->chain) {  // one wave per (patch, group of shots)
     if (first_touch && !mp->first_touch) fail("first-touch assembly is not set up for this handle");
-    static int group_env = -1;
-    if (group_env < 0) {
-      const char* env = getenv("SF2D_SHOTS_PER_WAVE");
-      group_env = env ? std::max(1, atoi(env)) : 0;
-    }
-    const int group = std::min(nshot, group_env > 0 ? group_env : CHAIN_SHOTS_PER_WAVE);
+    const int group = std::min(nshot, mp->shots_per_wave > 0 ? mp->shots_per_wave : CHAIN_SHOTS_PER_WAVE);
     const int items = npatch * ((nshot + group - 1) / group);
     const int grid = (items + NXCD - 1) / NXCD * NXCD;
     using K = decltype(&k_elastic_forces_chain<true, true, true>);

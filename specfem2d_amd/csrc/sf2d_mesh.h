@@ -80,6 +80,10 @@ struct Mesh {
   int n_affine = 0;                  // elements whose 25 points carry bitwise the same geometry
   int no_affine = 0;                 // SF2D_NO_AFFINE: always read the per-point geometry
   int rect = 0;                      // every element affine and axis-aligned (chain kernel's rectangle form)
+  // tuning: elements per chain patch (SF2D_CHAIN_ELEMS at set-up, 1..CHAIN_ELEMS), shots per wave of the
+  // batched chain kernel (SF2D_SHOTS_PER_WAVE or sf2d_mesh_set_shots_per_wave; 0 = built-in default)
+  int chain_elems = 0;
+  int shots_per_wave = 0;
   int chain = 0;                     // every patch is a chain of elements: the row-lane force kernel (SF2D_NO_CHAIN
                                      // keeps the general patch kernel)
   int* d_chain_ids = nullptr;        // chain kernel: [nspec][25] device order, internal point id | FIRST_TOUCH

@@ -102,6 +102,9 @@ def load_libraries():
     gpu.sf2d_mesh_affine_elements.restype = C.c_int
     gpu.sf2d_batch_shot_minor.argtypes = [C.POINTER(C.c_long)]
     gpu.sf2d_batch_shot_minor.restype = C.c_int
+    gpu.sf2d_mesh_chain_elems.argtypes = [C.POINTER(C.c_long)]
+    gpu.sf2d_mesh_chain_elems.restype = C.c_int
+    gpu.sf2d_mesh_set_shots_per_wave.argtypes = [C.POINTER(C.c_long), C.c_int]
     gpu.sf2d_mesh_rect.argtypes = [C.POINTER(C.c_long)]
     gpu.sf2d_mesh_rect.restype = C.c_int
     gpu.sf2d_mesh_chain.argtypes = [C.POINTER(C.c_long)]
@@ -219,6 +222,18 @@ class Simulation:
         self._need_device()
         mp = C.c_long(self.mesh_pointer)
         return bool(self.gpu.sf2d_mesh_chain(C.byref(mp)))
+
+    def chain_elems(self) -> int:
+        """Elements per chain patch of this handle's plan."""
+        self._need_device()
+        mp = C.c_long(self.mesh_pointer)
+        return self.gpu.sf2d_mesh_chain_elems(C.byref(mp))
+
+    def set_shots_per_wave(self, shots: int):
+        """Shots per wave of the batched chain kernel (0: the built-in default)."""
+        self._need_device()
+        mp = C.c_long(self.mesh_pointer)
+        self.gpu.sf2d_mesh_set_shots_per_wave(C.byref(mp), int(shots))
 
     def rect_kernel(self) -> bool:
         """Whether the chain kernel uses its rectangle form (every element affine and axis-aligned)."""

@@ -200,3 +200,26 @@ def test_autograd_misfit_on_a_batch(case, observed, batch_grad):
         err = _rel(t.grad.double().cpu().numpy(), g[k])
         assert err < 1e-4, (k, err)
     b.close()
+
+
+def test_autotuner_picks_a_candidate_and_keeps_the_results(case, monkeypatch):
+    """The autotuner times every (elements per chain patch, shots per wave) candidate on the workload and
+    returns the fastest; any candidate gives the same seismograms (FP32 rounding)."""
+    from specfem2d_amd import autotune
+    from specfem2d_amd.driver import ShotBatch
+    cfg, shots = case
+    for k in ("SF2D_CHAIN_ELEMS", "SF2D_SHOTS_PER_WAVE"):
+        monkeypatch.delenv(k, raising=False)
+    choice = autotune.tune(cfg, shots, steps=10, chain_elems=(4, 12), shots_per_wave=(1, 4))
+    print("autotuner:", choice)
+    assert {(r["chain_elems"], r["shots_per_wave"]) for r in choice["table"]} == {(4, 1), (4, 4), (12, 1), (12, 4)}
+    assert choice["ms_per_step"] == min(r["ms_per_step"] for r in choice["table"])
+    out = []
+    for n in (4, 12):
+        monkeypatch.setenv("SF2D_CHAIN_ELEMS", str(n))
+        b = ShotBatch(cfg, shots)
+        assert b.chain_elems() == n
+        b.set_shots_per_wave(1 if n == 4 else 4)
+        out.append(b.forward())
+        b.close()
+    assert _rel(out[0], out[1]) < 2e-6
