@@ -99,6 +99,12 @@ void sf2d_batch_release(long* Mesh_pointer);
 void sf2d_batch_reset(long* Mesh_pointer);
 /* time step it (1-based) of every shot, with the reference's Newmark constants */
 void sf2d_batch_step(long* Mesh_pointer, int it, float deltat, float deltatsqover2, float deltatover2);
+/* steps it_begin..it_end (1-based) of every shot, with the displacement seismograms when record != 0 and a copy
+   of the fields into gradient checkpoint it / ckpt_every - 1 after every ckpt_every-th step (ckpt_every > 0;
+   none after NSTEP).  With SF2D_BATCH_GRAPH set, each step after the first of a run replays one captured HIP
+   graph of the step (its index read from device memory) instead of launching its kernels. */
+void sf2d_batch_run(long* Mesh_pointer, int it_begin, int it_end, float deltat, float deltatsqover2,
+                    float deltatover2, int record, int ckpt_every);
 /* records the displacement seismograms of step it (1-based) */
 void sf2d_batch_record(long* Mesh_pointer, int it);
 /* seismograms [nshot][2][nrec][nstep] */

@@ -32,15 +32,65 @@ size_t field_floats(const Mesh* mp, const BatchState* B) {
   return B->ps > 1 ? (size_t)2 * mp->nglob * B->nshot : (size_t)2 * B->stride * B->nshot;
 }
 
-// forces, Stacey and sources of step it on state (d, v, a) whose accel holds the predictor's output
-void batch_forces(Mesh* mp, const BatchState* B, int it, const float* d, const float* v, float* a, int keep) {
+// forces, Stacey and sources of step it (or *step on the device) on state (d, v, a) whose accel holds the
+// predictor's output
+void batch_forces(Mesh* mp, const BatchState* B, int it, const float* d, const float* v, float* a, int keep,
+                  const int* step = nullptr) {
   for (int ph = 0; ph < 2; ph++) {
     apply_element_forces(mp, ph, d, a, B->nshot, B->stride, keep, B->ps);
     if (ph == 0) {
       if (mp->stacey) launch_stacey(mp, v, a, 0, B->nshot, B->stride, B->ps);
-      launch_sources(mp, a, it - 1, B->src_shot, B->stride, B->ps);
+      launch_sources(mp, a, it - 1, B->src_shot, B->stride, B->ps, step);
     }
   }
+}
+
+// one step of every shot on the batch fields: predictor (with the pending corrector), forces, Stacey, sources
+void batch_step_fields(Mesh* mp, BatchState* B, int it, float deltat, float deltatsqover2, float deltatover2,
+                       const int* step) {
+  const int keep = mp->first_touch;  // first-touch assembly: the force kernels define accel
+  if (B->pending)
+    launch_newmark_corrector_predictor(B->d, B->v, B->a, mp->d_rmassx, mp->d_rmassz, mp->nglob, B->pending_dto2,
+                                       deltat, deltatsqover2, deltatover2, mp->stream, B->nshot, B->stride, !keep,
+                                       B->ps);
+  else
+    launch_newmark_predictor(B->d, B->v, B->a, mp->nglob, deltat, deltatsqover2, deltatover2, mp->stream, B->nshot,
+                             B->stride, !keep, B->ps);
+  if (keep) launch_zero_points(mp, B->a, B->nshot, B->stride, B->ps);
+  batch_forces(mp, B, it, B->d, B->v, B->a, keep, step);
+  B->pending = 1;
+  B->pending_dto2 = deltatover2;
+}
+
+void drop_graph(BatchState* B) {
+  if (B->graph_exec) (void)hipGraphExecDestroy(B->graph_exec);
+  if (B->graph) (void)hipGraphDestroy(B->graph);
+  B->graph_exec = nullptr;
+  B->graph = nullptr;
+}
+
+// captures one steady-state step (corrector pending with dto2 = deltatover2) reading its index from d_step
+void capture_step(Mesh* mp, BatchState* B, float deltat, float deltatsqover2, float deltatover2, int record) {
+  drop_graph(B);
+  if (!B->d_step) B->d_step = device_zeros<int>(1);
+  const int pending = B->pending;
+  const float pdto2 = B->pending_dto2;
+  check_hip(hipStreamBeginCapture(mp->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+  B->pending = 1;
+  B->pending_dto2 = deltatover2;
+  batch_step_fields(mp, B, 1, deltat, deltatsqover2, deltatover2, B->d_step);
+  if (record && mp->nrec_local > 0)
+    launch_seismograms(mp, B->d, B->seismo, 0, B->nshot, B->stride, (long)2 * mp->nrec_local * mp->nstep, B->ps,
+                       B->d_step);
+  launch_next_step(B->d_step, mp->stream);
+  check_hip(hipStreamEndCapture(mp->stream, &B->graph), "hipStreamEndCapture");
+  check_hip(hipGraphInstantiate(&B->graph_exec, B->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+  B->pending = pending;
+  B->pending_dto2 = pdto2;
+  B->graph_dt[0] = deltat;
+  B->graph_dt[1] = deltatsqover2;
+  B->graph_dt[2] = deltatover2;
+  B->graph_record = record;
 }
 
 }  // namespace
@@ -131,6 +181,8 @@ void sf2d_batch_release(long* Mesh_pointer) {
   device_free(B->a);
   device_free(B->src_shot);
   device_free(B->seismo);
+  drop_graph(B);
+  device_free(B->d_step);
   delete B;
   mp->batch = nullptr;
 }
@@ -152,19 +204,41 @@ void sf2d_batch_step(long* Mesh_pointer, int it, float deltat, float deltatsqove
   Mesh* mp = handle(Mesh_pointer);
   BatchState* B = need_batch(mp);
   if (it < 1 || it > mp->nstep) fail("sf2d_batch_step: step %d outside 1..%d", it, mp->nstep);
-  const int keep = mp->first_touch;  // first-touch assembly: the force kernels define accel
-  if (B->pending)
-    launch_newmark_corrector_predictor(B->d, B->v, B->a, mp->d_rmassx, mp->d_rmassz, mp->nglob, B->pending_dto2,
-                                       deltat, deltatsqover2, deltatover2, mp->stream, B->nshot, B->stride, !keep,
-                                       B->ps);
-  else
-    launch_newmark_predictor(B->d, B->v, B->a, mp->nglob, deltat, deltatsqover2, deltatover2, mp->stream, B->nshot,
-                             B->stride, !keep, B->ps);
-  if (keep) launch_zero_points(mp, B->a, B->nshot, B->stride, B->ps);
-  B->pending = 0;
-  batch_forces(mp, B, it, B->d, B->v, B->a, keep);
-  B->pending = 1;
-  B->pending_dto2 = deltatover2;
+  batch_step_fields(mp, B, it, deltat, deltatsqover2, deltatover2, nullptr);
+}
+
+// steps it_begin..it_end of every shot, recording the seismograms when record is set and copying the fields into
+// gradient checkpoint it / ckpt_every - 1 after every ckpt_every-th step (ckpt_every > 0; not after NSTEP).
+// With SF2D_BATCH_GRAPH set, each step after the first of a run is one replay of a captured graph.
+void sf2d_batch_run(long* Mesh_pointer, int it_begin, int it_end, float deltat, float deltatsqover2,
+                    float deltatover2, int record, int ckpt_every) {
+  Mesh* mp = handle(Mesh_pointer);
+  BatchState* B = need_batch(mp);
+  if (it_begin < 1 || it_end > mp->nstep || it_end < it_begin)
+    fail("sf2d_batch_run: steps %d..%d outside 1..%d", it_begin, it_end, mp->nstep);
+  // graph replays on request: at the sizes measured (80x60 elements and up, 8 shots and more) a batch step is
+  // device-bound and a replay per step costs more than the launches it replaces
+  const bool use_graph = getenv("SF2D_BATCH_GRAPH") != nullptr;
+  bool counting = false;  // d_step holds the index of the next replay
+  for (int it = it_begin; it <= it_end; it++) {
+    if (!use_graph || !B->pending || B->pending_dto2 != deltatover2) {
+      batch_step_fields(mp, B, it, deltat, deltatsqover2, deltatover2, nullptr);
+      if (record && mp->nrec_local > 0)
+        launch_seismograms(mp, B->d, B->seismo, it - 1, B->nshot, B->stride, (long)2 * mp->nrec_local * mp->nstep,
+                           B->ps);
+      counting = false;
+    } else {
+      if (!B->graph_exec || B->graph_record != record || B->graph_dt[0] != deltat ||
+          B->graph_dt[1] != deltatsqover2 || B->graph_dt[2] != deltatover2)
+        capture_step(mp, B, deltat, deltatsqover2, deltatover2, record);
+      if (!counting) launch_set_step(B->d_step, it, mp->stream);
+      counting = true;
+      check_hip(hipGraphLaunch(B->graph_exec, mp->stream), "hipGraphLaunch");
+      B->pending = 1;
+      B->pending_dto2 = deltatover2;
+    }
+    if (ckpt_every > 0 && it % ckpt_every == 0 && it < mp->nstep) sf2d_grad_save_state(Mesh_pointer, 0, it / ckpt_every - 1);
+  }
 }
 
 // displacement seismograms of step it (1-based) for every shot

@@ -643,14 +643,8 @@ void batch_steps(Sim* s, int it_begin, int it_end, bool record) {
   const float deltat = (float)s->p.dt;
   const float deltatover2 = (float)(0.5 * (double)deltat);
   const float deltatsqover2 = (float)(0.5 * (double)deltat * (double)deltat);
-  for (int it = it_begin; it <= it_end; it++) {
-    sf2d_batch_step(&s->mesh, it, deltat, deltatsqover2, deltatover2);
-    if (s->grad_save == 1 && it % s->grad_chunk == 0 && it < s->p.nstep)
-      sf2d_grad_save_state(&s->mesh, 0, it / s->grad_chunk - 1);
-    else if (s->grad_save == 2)
-      sf2d_grad_save_state(&s->mesh, 1, it - s->grad_seg_base - 1);
-    if (record && s->nrec > 0) sf2d_batch_record(&s->mesh, it);
-  }
+  sf2d_batch_run(&s->mesh, it_begin, it_end, deltat, deltatsqover2, deltatover2, record && s->nrec > 0,
+                 s->grad_save == 1 ? s->grad_chunk : 0);
   s->it_done = it_end;
 }
 

@@ -949,10 +949,11 @@ __global__ __launch_bounds__(64) void k_sources(int nsrc, int it, const int* __r
                                                 const int* __restrict__ is_elastic, const int* __restrict__ ibool,
                                                 const float* __restrict__ sourcearrays, const float* __restrict__ stf,
                                                 float* __restrict__ accel, const int* __restrict__ src_shot,
-                                                long shot_stride, int ps) {
+                                                long shot_stride, int ps, const int* __restrict__ step) {
   const int s = blockIdx.x;
   const int p = threadIdx.x;
   if (s >= nsrc || p >= NGLL2) return;
+  if (step) it = *step - 1;  // graph replays: the 1-based step from device memory
   if (src_shot) accel += 2 * (long)src_shot[s] * shot_stride;  // each source drives its own shot's field
   const int ispec = ispec_src[s];
   if (!is_elastic[ispec]) return;
@@ -990,7 +991,8 @@ __global__ __launch_bounds__(64) void k_seismograms(int nrec, int nlength, int c
                                                     const float* __restrict__ gammar, const float* __restrict__ cosrot,
                                                     const float* __restrict__ sinrot, const float2* __restrict__ field,
                                                     float* __restrict__ dst, long shot_stride, long dst_stride,
-                                                    int ps) {
+                                                    int ps, const int* __restrict__ step) {
+  if (step) cur = *step - 1;  // graph replays: the 1-based step from device memory
   field += blockIdx.y * shot_stride;
   dst += blockIdx.y * dst_stride;
   const int r = blockIdx.x;
@@ -1108,6 +1110,10 @@ inline int blocks_for(long n, int threads, int cap = 1 << 20) {
 }
 
 }  // namespace
+
+// step counter of graph replays (one thread)
+__global__ void k_set_step(int* step, int value) { *step = value; }
+__global__ void k_next_step(int* step) { *step += 1; }
 
 // ---------------------------------------------------------------------------
 // launchers
@@ -1261,6 +1267,15 @@ void launch_elastic_forces_walk(const Mesh* mp, int first_patch, int npatch, con
   check_hip(hipGetLastError(), "k_elastic_forces_walk launch");
 }
 
+void launch_set_step(int* step, int value, hipStream_t s) {
+  hipLaunchKernelGGL(k_set_step, dim3(1), dim3(1), 0, s, step, value);
+  check_hip(hipGetLastError(), "k_set_step launch");
+}
+void launch_next_step(int* step, hipStream_t s) {
+  hipLaunchKernelGGL(k_next_step, dim3(1), dim3(1), 0, s, step);
+  check_hip(hipGetLastError(), "k_next_step launch");
+}
+
 void launch_pack_elements(const Mesh* mp) {
   const size_t n = (size_t)mp->nspec * NGLL2;
   hipLaunchKernelGGL(k_pack_elements, dim3(blocks_for((long)n, 256, 65536)), dim3(256), 0, mp->stream, n, mp->d_xix,
@@ -1323,11 +1338,12 @@ void launch_stacey_backward(const Mesh* mp, float* b_accel) {
   check_hip(hipGetLastError(), "k_stacey_backward launch");
 }
 
-void launch_sources(const Mesh* mp, float* accel, int it, const int* src_shot, long shot_stride, int ps) {
+void launch_sources(const Mesh* mp, float* accel, int it, const int* src_shot, long shot_stride, int ps,
+                    const int* step) {
   if (mp->nsources_local <= 0) return;
   hipLaunchKernelGGL(k_sources, dim3(mp->nsources_local), dim3(64), 0, mp->stream, mp->nsources_local, it,
                      mp->d_ispec_selected_source, mp->d_ispec_is_elastic, mp->d_ibool, mp->d_sourcearrays, mp->d_stf,
-                     accel, src_shot, shot_stride, ps);
+                     accel, src_shot, shot_stride, ps, step);
   check_hip(hipGetLastError(), "k_sources launch");
 }
 
@@ -1340,12 +1356,12 @@ void launch_adjoint_sources(const Mesh* mp, float* accel, int it) {
 }
 
 void launch_seismograms(const Mesh* mp, const float* field, float* dst, int seismo_current, int nshot,
-                        long shot_stride, long dst_stride, int ps) {
+                        long shot_stride, long dst_stride, int ps, const int* step) {
   if (mp->nrec_local <= 0) return;
   hipLaunchKernelGGL(k_seismograms, dim3(mp->nrec_local, nshot), dim3(64), 0, mp->stream, mp->nrec_local,
                      mp->nlength_seismogram, seismo_current, mp->d_ispec_selected_rec, mp->d_ibool, mp->d_xir,
                      mp->d_gammar, mp->d_cosrot, mp->d_sinrot, (const float2*)field, dst, shot_stride, dst_stride,
-                     ps);
+                     ps, step);
   check_hip(hipGetLastError(), "k_seismograms launch");
 }
 

@@ -172,6 +172,13 @@ struct BatchState {
   float* seismo = nullptr;           // [nshot][2][nrec][nstep] displacement seismograms
   int pending = 0;
   float pending_dto2 = 0.f;
+  // one captured step (corrector-predictor, forces, Stacey, sources, seismograms, step counter), replayed for
+  // the steps after the first of a run; d_step is its 1-based step index in device memory
+  int* d_step = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  float graph_dt[3] = {0.f, 0.f, 0.f};
+  int graph_record = -1;
 };
 
 // shot batch fields and the re-arming of its deferred corrector (batch.hip)
@@ -275,11 +282,14 @@ void launch_stacey(const Mesh* mp, const float* veloc, float* accel, int write_s
                    long shot_stride = 0, int ps = 1);
 void launch_stacey_backward(const Mesh* mp, float* b_accel);
 // src_shot (device, one entry per source) sends each source to its own wavefield; null: all into accel
+// step (device, 1-based) replaces it when not null (graph replays)
 void launch_sources(const Mesh* mp, float* accel, int it, const int* src_shot = nullptr, long shot_stride = 0,
-                    int ps = 1);
+                    int ps = 1, const int* step = nullptr);
 void launch_adjoint_sources(const Mesh* mp, float* accel, int it);
 void launch_seismograms(const Mesh* mp, const float* field, float* dst, int seismo_current, int nshot = 1,
-                        long shot_stride = 0, long dst_stride = 0, int ps = 1);
+                        long shot_stride = 0, long dst_stride = 0, int ps = 1, const int* step = nullptr);
+void launch_set_step(int* step, int value, hipStream_t s);
+void launch_next_step(int* step, hipStream_t s);
 void launch_kernels(const Mesh* mp);
 void launch_max_norm(const Mesh* mp, const float* field, unsigned int* result);
 

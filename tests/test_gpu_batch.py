@@ -223,3 +223,27 @@ def test_autotuner_picks_a_candidate_and_keeps_the_results(case, monkeypatch):
         out.append(b.forward())
         b.close()
     assert _rel(out[0], out[1]) < 2e-6
+
+
+def test_graph_replayed_steps_match_launched_steps(case, observed, monkeypatch):
+    """With SF2D_BATCH_GRAPH, every batch step after the first of a run is one replay of a captured HIP graph
+    (the step index read from device memory); it gives the same seismograms as launching every kernel, and so
+    do repeated runs that replay the same graph with eager resets and checkpoint copies in between."""
+    from specfem2d_amd.driver import ShotBatch
+    cfg, shots = case
+    monkeypatch.delenv("SF2D_BATCH_GRAPH", raising=False)
+    b = ShotBatch(cfg, shots)
+    ref = b.forward()
+    b.gradient_forward(observed, chunk=37)
+    g_ref = b.gradient_backward()
+    monkeypatch.setenv("SF2D_BATCH_GRAPH", "1")
+    runs = [b.forward() for _ in range(3)]
+    b.gradient_forward(observed, chunk=37)  # replays interleaved with checkpoint copies
+    g = b.gradient_backward()
+    b.close()
+    for k, r in enumerate(runs):
+        err = _rel(r, ref)
+        print(f"graph run {k}: seismograms rel L2 vs launched steps {err:.2e}")
+        assert err < 1e-6
+    for name in g_ref:
+        assert _rel(g[name], g_ref[name]) < 1e-6, name
