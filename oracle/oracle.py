@@ -23,6 +23,7 @@ class OrcParams(C.Structure):
         ("absorb_bottom", C.c_int), ("absorb_right", C.c_int), ("absorb_top", C.c_int),
         ("absorb_left", C.c_int), ("stacey", C.c_int), ("nstep", C.c_int), ("dt", C.c_double),
         ("user_t0", C.c_double), ("anglerec_deg", C.c_double), ("material_mode", C.c_int),
+        ("seismotype", C.c_int), ("use_trick", C.c_int),
     ]
 
 
@@ -78,6 +79,17 @@ def _lib(precision: str):
         lib.orc_kernel_increment.argtypes = [P, dp, dp, dp, dp]
         lib.orc_gradient.argtypes = [P, dp, dp, dp]
         lib.orc_gradient.restype = C.c_double
+        for f in ("orc_nspec_acoustic", "orc_ncoupling", "orc_nfree"):
+            getattr(lib, f).argtypes = [P]
+            getattr(lib, f).restype = C.c_int
+        lib.orc_get_is_acoustic.argtypes = [P, ip]
+        lib.orc_get_rmass_acoustic.argtypes = [P, dp]
+        lib.orc_get_coupling.argtypes = [P, ip, ip, ip, ip]
+        lib.orc_get_free_surface.argtypes = [P, ip, ip]
+        lib.orc_get_acoustic_stacey_range.argtypes = [P, ip, ip]
+        lib.orc_get_acoustic_fields.argtypes = [P, C.c_void_p, C.c_void_p, C.c_void_p]
+        lib.orc_set_acoustic_fields.argtypes = [P, C.c_void_p, C.c_void_p, C.c_void_p]
+        lib.orc_apply_acoustic_stiffness.argtypes = [P, dp, dp]
         _LIBS[precision] = lib
     return _LIBS[precision]
 
@@ -108,6 +120,7 @@ class OracleSim:
         p.nstep, p.dt = cfg.nstep, cfg.dt
         p.user_t0, p.anglerec_deg = cfg.user_t0, cfg.anglerec
         p.material_mode = cfg.material_mode
+        p.seismotype, p.use_trick = int(cfg.seismotype), int(cfg.use_trick)
         rho, vp, vs = cfg.material_arrays()
         srcs = (OrcSource * max(1, len(cfg.sources)))()
         for k, s in enumerate(cfg.sources):
@@ -120,6 +133,8 @@ class OracleSim:
         self.nglob = self.lib.orc_nglob(self.h)
         self.nabs = self.lib.orc_nabs(self.h)
         self.t0 = self.lib.orc_t0(self.h)
+        self.nspec_acoustic = self.lib.orc_nspec_acoustic(self.h)
+        self.ncoupling = self.lib.orc_ncoupling(self.h)
         self.lib.orc_set_ftz(self.h, int(ftz))
 
     def __del__(self):
@@ -208,11 +223,64 @@ class OracleSim:
         self.lib.orc_apply_stiffness(self.h, u, out)
         return out.reshape(-1, 2)
 
+    # acoustic domain ----------------------------------------------------
+    def is_acoustic(self):
+        out = np.zeros(self.nspec, np.int32)
+        self.lib.orc_get_is_acoustic(self.h, out)
+        return out.astype(bool)
+
+    def rmass_acoustic(self):
+        out = np.zeros(self.nglob)
+        self.lib.orc_get_rmass_acoustic(self.h, out)
+        return out
+
+    def coupling(self):
+        """Fluid-solid edges: (acoustic element, side, elastic element, side), 0-based elements, sides 1 bottom,
+        2 right, 3 top, 4 left."""
+        n = max(1, self.ncoupling)
+        arrs = [np.zeros(n, np.int32) for _ in range(4)]
+        self.lib.orc_get_coupling(self.h, *arrs)
+        return np.stack(arrs, axis=1)[: self.ncoupling]
+
+    def free_surface(self):
+        n = max(1, self.lib.orc_nfree(self.h))
+        isp, code = np.zeros(n, np.int32), np.zeros(n, np.int32)
+        self.lib.orc_get_free_surface(self.h, isp, code)
+        k = self.lib.orc_nfree(self.h)
+        return isp[:k], code[:k]
+
+    def acoustic_stacey_range(self):
+        n = max(1, self.nabs)
+        b, e = np.zeros(n, np.int32), np.zeros(n, np.int32)
+        self.lib.orc_get_acoustic_stacey_range(self.h, b, e)
+        return b[: self.nabs], e[: self.nabs]
+
+    def acoustic_fields(self):
+        p, v, a = (np.zeros(self.nglob) for _ in range(3))
+        self.lib.orc_get_acoustic_fields(self.h, _ptr(p), _ptr(v), _ptr(a))
+        return p, v, a
+
+    def set_acoustic_fields(self, pot=None, pot_dot=None, pot_dd=None):
+        arrs = [None if x is None else np.ascontiguousarray(np.asarray(x, np.float64).reshape(-1))
+                for x in (pot, pot_dot, pot_dd)]
+        self.lib.orc_set_acoustic_fields(self.h, *[_ptr(x) for x in arrs])
+
+    def apply_acoustic_stiffness(self, pot):
+        pot = np.ascontiguousarray(np.asarray(pot, np.float64).reshape(-1))
+        out = np.zeros_like(pot)
+        self.lib.orc_apply_acoustic_stiffness(self.h, pot, out)
+        return out
+
+    def _elastic_only(self, what):
+        if self.nspec_acoustic:
+            raise NotImplementedError(f"{what}: the oracle's adjoint path covers elastic meshes only")
+
     def adjoint(self, adj, stacey_store, lastframe, dump_steps=()):
         """SIMULATION_TYPE=3 run with backward reconstruction.  ``adj`` [nrec,2,nstep] adjoint
         source traces, ``stacey_store`` from a SAVE_FORWARD run, ``lastframe`` = (displ, veloc, accel)
         at the end of that run.  Returns (adjoint seismograms [nrec,2,nstep], kernels dict of
         [nspec,25] arrays rho/mu/kappa, scaled as save_adjoint_kernels.f90 writes them)."""
+        self._elastic_only("adjoint")
         n = self.cfg.nstep
         adj = np.ascontiguousarray(np.asarray(adj, np.float64).reshape(-1))
         st = np.ascontiguousarray(np.asarray(stacey_store, np.float64).reshape(-1))
@@ -234,6 +302,7 @@ class OracleSim:
         """Exact gradient of chi = 1/2 dt sum |s - obs|^2 over a full forward run from rest (seismotype
         displacement).  Returns (chi, grad dict rho/vp/vs, parts dict kappa/mu/rho_mass/rho_vp/rho_vs), all
         [nspec,25] in the MODEL=gll per-point parameterisation."""
+        self._elastic_only("gradient")
         o = np.ascontiguousarray(np.asarray(obs, np.float64).reshape(-1))
         if o.size != max(1, self.cfg.nrec) * 2 * self.cfg.nstep and self.cfg.nrec > 0:
             raise ValueError("obs must be [nrec, 2, nstep]")

@@ -4,7 +4,9 @@
  * TEST INFRASTRUCTURE ONLY.  This file is a plain-C restatement of the
  * reference Fortran solver (SPECFEM2D 8.1.0, CPU path) for an elastic,
  * isotropic, P-SV, Newmark simulation on a structured internal mesh with
- * Stacey absorbing edges.  It is the checker for the HIP implementation and
+ * Stacey absorbing edges, extended to acoustic (fluid) elements: the
+ * potential formulation, the acoustic free surface, acoustic Stacey edges,
+ * pressure sources, pressure seismograms and fluid-solid coupling.  It is the checker for the HIP implementation and
  * the CPU baseline timed beside it; the product never links or calls it.
  *
  * It is compiled twice: -DREAL=float (CUSTOM_REAL=real, the reference
@@ -47,6 +49,8 @@ typedef struct {
   double user_t0;             /* USER_T0 (0 = automatic) */
   double anglerec_deg;        /* anglerec */
   int material_mode;          /* 0: Par_file material table, 1: MODEL=gll */
+  int seismotype;             /* 1 displacement (elastic receivers), 4 pressure (acoustic receivers) */
+  int use_trick;              /* USE_TRICK_FOR_BETTER_PRESSURE */
 } orc_params;
 
 typedef struct {
@@ -55,7 +59,7 @@ typedef struct {
   double factor;
   double angle_deg;           /* anglesource, degrees */
   double tshift;
-  int time_function_type;     /* 1 Ricker, 2 first derivative of Gaussian */
+  int time_function_type;     /* 1 Ricker, 2 first derivative of Gaussian, 3 Gaussian */
   int pad;
 } orc_source;
 
@@ -95,8 +99,18 @@ typedef struct {
   double *rec_xi, *rec_gamma;
   REAL *xir, *gammar;         /* [nrec][5] */
   double cosrot, sinrot;
+  /* domains: an element is acoustic when vs = 0 at all its GLL points */
+  int *is_acoustic;           /* [nspec] */
+  int nspec_acoustic, any_acoustic, any_elastic;
+  REAL *rmass_ac_inv;         /* [nglob] acoustic inverse mass */
+  int nfree;                  /* acoustic free-surface edges: [nfree] element, side code */
+  int *free_ispec, *free_code;
+  int ncoup;                  /* fluid-solid edges: acoustic element/side, elastic element/side */
+  int *coup_ac_ispec, *coup_ac_code, *coup_el_ispec, *coup_el_code;
+  int *abs_jbegin, *abs_jend; /* acoustic Stacey point range along each absorbing edge, 0-based inclusive */
   /* wavefields */
   REAL *displ, *veloc, *accel;/* [nglob][2] */
+  REAL *pot, *pot_dot, *pot_dd;/* [nglob] acoustic potential and its time derivatives */
   int ftz;                    /* run the time loop with flush-to-zero */
   int it_done;                /* forward steps already run */
 } orc_sim;
@@ -421,6 +435,108 @@ static void setup_absorbing(orc_sim *s) {
   for (int a = 0; a < s->nabs; a++) s->ib[a] = ++s->nspec_side[s->abs_code[a]];
 }
 
+/* ------------------------------------------------------------------ */
+/* domains, acoustic free surface, fluid-solid edges                    */
+/* (src/meshfem2D/determine_acoustic_surface.f90:34-160,              */
+/*  repartition_coupling.f90:38-106, src/specfem2D/get_coupling_edges.f90) */
+/* ------------------------------------------------------------------ */
+static void setup_domains(orc_sim *s) {
+  const orc_params *p = &s->p;
+  const int nx = p->nx, nz = p->nz;
+  s->is_acoustic = (int *)calloc((size_t)s->nspec + 1, sizeof(int));
+  s->nspec_acoustic = 0;
+  for (int e = 0; e < s->nspec; e++) {
+    int nzero = 0;
+    for (int l = 0; l < NGLL2; l++) nzero += s->mustore[(size_t)e * NGLL2 + l] == (REAL)0;
+    s->is_acoustic[e] = nzero == NGLL2;
+    s->nspec_acoustic += s->is_acoustic[e];
+  }
+  s->any_acoustic = s->nspec_acoustic > 0;
+  s->any_elastic = s->nspec_acoustic < s->nspec;
+  /* free surface: the non-absorbing outer sides of acoustic elements, in the mesher's order (top, bottom, left,
+     right) */
+  s->free_ispec = (int *)calloc((size_t)2 * (nx + nz) + 1, sizeof(int));
+  s->free_code = (int *)calloc((size_t)2 * (nx + nz) + 1, sizeof(int));
+  s->nfree = 0;
+  if (!p->absorb_top)
+    for (int i = 0; i < nx; i++) {
+      int e = (nz - 1) * nx + i;
+      if (s->is_acoustic[e]) { s->free_ispec[s->nfree] = e; s->free_code[s->nfree++] = 3; }
+    }
+  if (!p->absorb_bottom)
+    for (int i = 0; i < nx; i++) {
+      int e = i;
+      if (s->is_acoustic[e]) { s->free_ispec[s->nfree] = e; s->free_code[s->nfree++] = 1; }
+    }
+  if (!p->absorb_left)
+    for (int j = 0; j < nz; j++) {
+      int e = j * nx;
+      if (s->is_acoustic[e]) { s->free_ispec[s->nfree] = e; s->free_code[s->nfree++] = 4; }
+    }
+  if (!p->absorb_right)
+    for (int j = 0; j < nz; j++) {
+      int e = j * nx + nx - 1;
+      if (s->is_acoustic[e]) { s->free_ispec[s->nfree] = e; s->free_code[s->nfree++] = 2; }
+    }
+  /* fluid-solid edges: every acoustic element with an elastic edge neighbour, acoustic elements in order;
+     side codes 1 bottom, 2 right, 3 top, 4 left */
+  s->coup_ac_ispec = (int *)calloc((size_t)4 * s->nspec + 1, sizeof(int));
+  s->coup_ac_code = (int *)calloc((size_t)4 * s->nspec + 1, sizeof(int));
+  s->coup_el_ispec = (int *)calloc((size_t)4 * s->nspec + 1, sizeof(int));
+  s->coup_el_code = (int *)calloc((size_t)4 * s->nspec + 1, sizeof(int));
+  s->ncoup = 0;
+  if (s->any_acoustic && s->any_elastic)
+    for (int e = 0; e < s->nspec; e++) {
+      if (!s->is_acoustic[e]) continue;
+      const int ix = e % nx, iz = e / nx;
+      const int nb[4] = {iz > 0 ? e - nx : -1, ix > 0 ? e - 1 : -1, ix < nx - 1 ? e + 1 : -1,
+                         iz < nz - 1 ? e + nx : -1};
+      const int ac_code[4] = {1, 4, 2, 3}, el_code[4] = {3, 2, 4, 1};
+      for (int k = 0; k < 4; k++) {
+        if (nb[k] < 0 || s->is_acoustic[nb[k]]) continue;
+        s->coup_ac_ispec[s->ncoup] = e;
+        s->coup_ac_code[s->ncoup] = ac_code[k];
+        s->coup_el_ispec[s->ncoup] = nb[k];
+        s->coup_el_code[s->ncoup] = el_code[k];
+        s->ncoup++;
+      }
+    }
+  /* acoustic Stacey point ranges: an absorbing edge of an acoustic element leaves out the point it shares with
+     a fluid-solid edge of the same element (get_coupling_edges.f90:528-563) and, on bottom/top edges, the
+     corner points of codeabs_corner (compute_stacey_acoustic.f90) */
+  s->abs_jbegin = (int *)calloc((size_t)s->nabs + 1, sizeof(int));
+  s->abs_jend = (int *)calloc((size_t)s->nabs + 1, sizeof(int));
+  for (int a = 0; a < s->nabs; a++) {
+    const int e = s->abs_ispec[a] - 1, code = s->abs_code[a];
+    int b = 0, en = NGLL - 1;
+    for (int c = 0; c < s->ncoup; c++) {
+      if (s->coup_ac_ispec[c] != e) continue;
+      const int ce = s->coup_ac_code[c];
+      if (code == 4 || code == 2) {
+        if (ce == 1) b = 1;
+        if (ce == 3) en = NGLL - 2;
+      } else {
+        if (ce == 4) b = 1;
+        if (ce == 2) en = NGLL - 2;
+      }
+    }
+    if (code == 1 && s->abs_corner[4 * a + 0]) b = 1;
+    if (code == 1 && s->abs_corner[4 * a + 1]) en = NGLL - 2;
+    if (code == 3 && s->abs_corner[4 * a + 2]) b = 1;
+    if (code == 3 && s->abs_corner[4 * a + 3]) en = NGLL - 2;
+    s->abs_jbegin[a] = b;
+    s->abs_jend[a] = en;
+  }
+}
+
+/* (i, j) of point k of side code of an element, in increasing i or j */
+static void side_point(int code, int k, int *i, int *j) {
+  if (code == 4) { *i = 0; *j = k; }
+  else if (code == 2) { *i = NGLL - 1; *j = k; }
+  else if (code == 1) { *i = k; *j = 0; }
+  else { *i = k; *j = NGLL - 1; }
+}
+
 /* edge geometry at GLL point (i,j) of an absorbing element: outward normal
    and 1D jacobian (compute_stacey_elastic.f90 / prepare_timerun.F90:1316-1460) */
 static void edge_normal(const orc_sim *s, int code, size_t l, REAL *nx, REAL *nz, REAL *jac1d) {
@@ -445,14 +561,21 @@ static void edge_normal(const orc_sim *s, int code, size_t l, REAL *nx, REAL *nz
 /* ------------------------------------------------------------------ */
 static void setup_mass(orc_sim *s) {
   s->rmass_inv = (REAL *)calloc((size_t)2 * s->nglob, sizeof(REAL));
-  REAL *rm = s->rmass_inv;
+  s->rmass_ac_inv = (REAL *)calloc((size_t)s->nglob, sizeof(REAL));
+  REAL *rm = s->rmass_inv, *rma = s->rmass_ac_inv;
   for (int e = 0; e < s->nspec; e++)
     for (int j = 0; j < NGLL; j++)
       for (int i = 0; i < NGLL; i++) {
         size_t l = (size_t)e * NGLL2 + j * NGLL + i;
         int g = s->ibool[l] - 1;
-        double rhol = (double)s->rhostore[l];
         REAL wxz = s->wxgll[i] * s->wxgll[j];
+        if (s->is_acoustic[e]) {
+          /* w_i w_j J (single precision) divided by the double kappa (invert_mass_matrix.F90:424-427) */
+          double kappal = (double)s->kappastore[l];
+          rma[g] = (REAL)((double)rma[g] + (double)(wxz * s->jacobian[l]) / kappal);
+          continue;
+        }
+        double rhol = (double)s->rhostore[l];
         rm[2 * g] = (REAL)((double)rm[2 * g] + (double)wxz * rhol * (double)s->jacobian[l]);
         rm[2 * g + 1] = rm[2 * g];
       }
@@ -460,6 +583,20 @@ static void setup_mass(orc_sim *s) {
     REAL deltatover2 = (REAL)(0.5 * (double)(REAL)s->p.dt);
     for (int a = 0; a < s->nabs; a++) {
       int e = s->abs_ispec[a] - 1, code = s->abs_code[a];
+      if (s->is_acoustic[e]) {
+        /* acoustic Newmark Stacey term dt/2 w / rho_vp (invert_mass_matrix.F90:709-790) */
+        for (int k = s->abs_jbegin[a]; k <= s->abs_jend[a]; k++) {
+          int i, j;
+          side_point(code, k, &i, &j);
+          size_t l = (size_t)e * NGLL2 + j * NGLL + i;
+          int g = s->ibool[l] - 1;
+          REAL nx, nz, j1d;
+          edge_normal(s, code, l, &nx, &nz, &j1d);
+          REAL weight = j1d * ((code == 4 || code == 2) ? s->wxgll[j] : s->wxgll[i]);
+          rma[g] = rma[g] + deltatover2 * weight / s->rho_vp[l];
+        }
+        continue;
+      }
       for (int k = 0; k < NGLL; k++) {
         int i, j;
         if (code == 4) { i = 0; j = k; }
@@ -485,6 +622,10 @@ static void setup_mass(orc_sim *s) {
   for (int g = 0; g < 2 * s->nglob; g++) {
     if (rm[g] <= (REAL)0) rm[g] = (REAL)1;
     rm[g] = (REAL)1 / rm[g];
+  }
+  for (int g = 0; g < s->nglob; g++) {
+    if (rma[g] <= (REAL)0) rma[g] = (REAL)1;
+    rma[g] = (REAL)1 / rma[g];
   }
 }
 
@@ -590,23 +731,35 @@ static void setup_sources(orc_sim *s) {
     lagrange_any(s->src_xi[k], s->xigll, hx);
     lagrange_any(s->src_gamma[k], s->xigll, hg);
     double ang = s->src[k].angle_deg * PI / 180.0;
+    const int acoustic = s->is_acoustic[s->src_ispec[k] - 1];
     for (int j = 0; j < NGLL; j++)
       for (int i = 0; i < NGLL; i++) {
         double hl = hx[i] * hg[j];
-        s->sourcearrays[((size_t)k * NGLL2 + j * NGLL + i) * 2 + 0] = (REAL)(+sin(ang) * hl);
-        s->sourcearrays[((size_t)k * NGLL2 + j * NGLL + i) * 2 + 1] = (REAL)(-cos(ang) * hl);
+        /* setup_sources_receivers.F90:994-1010: a pressure source in a fluid element, a force in a solid one */
+        s->sourcearrays[((size_t)k * NGLL2 + j * NGLL + i) * 2 + 0] = (REAL)(acoustic ? hl : +sin(ang) * hl);
+        s->sourcearrays[((size_t)k * NGLL2 + j * NGLL + i) * 2 + 1] = (REAL)(acoustic ? hl : -cos(ang) * hl);
       }
-    /* source time function (prepare_source_time_function.f90, comp_source_time_function.f90) */
-    double f0 = s->src[k].f0, a = PI * PI * f0 * f0;
+    /* source time function (prepare_source_time_function.f90:200-262, comp_source_time_function.f90:52-200);
+       with USE_TRICK_FOR_BETTER_PRESSURE every type is replaced by its second time derivative */
+    double f0 = s->src[k].f0, a = (PI * PI) * (f0 * f0); /* PI**2 * f0**2 */
+    const int type = s->src[k].time_function_type, trick = s->p.use_trick;
     for (int it = 1; it <= s->p.nstep; it++) {
       double timeval = (double)(it - 1) * s->p.dt;
       double t = timeval - t0 - tsh[k];
-      double v;
-      if (s->src[k].time_function_type == 1)
-        v = -s->src[k].factor * ((1.0 - 2.0 * a * t * t) * exp(-a * t * t));
-      else
-        v = -s->src[k].factor * (t * exp(-a * t * t));
-      s->stf[(size_t)k * s->p.nstep + (it - 1)] = (REAL)v;
+      double v, t2 = t * t; /* t**2, in the Fortran's evaluation order below */
+      if (type == 1 && !trick) /* Ricker = second derivative of the Gaussian */
+        v = (1.0 - 2.0 * a * t2) * exp(-a * t2);
+      else if (type == 1) /* fourth derivative of the Gaussian */
+        v = -(2.0 * a * (3.0 - 12.0 * a * t * t + 4.0 * (a * a) * t * t * t * t) * exp(-a * t * t));
+      else if (type == 2 && !trick) /* first derivative of the Gaussian */
+        v = t * exp(-a * t2);
+      else if (type == 2) /* third derivative */
+        v = 2.0 * a * t * exp(-a * t2) * (-3.0 + 2.0 * a * t2);
+      else if (!trick) /* Gaussian */
+        v = -exp(-a * t2) / (2.0 * a);
+      else /* second derivative of the Gaussian */
+        v = (1.0 - 2.0 * a * t2) * exp(-a * t2);
+      s->stf[(size_t)k * s->p.nstep + (it - 1)] = (REAL)(-s->src[k].factor * v);
     }
   }
   free(t0s);
@@ -646,6 +799,7 @@ static void element_forces(orc_sim *s, const REAL *displ, REAL *accel) {
   REAL u[2][NGLL][NGLL]; /* u[c][j][i] */
   REAL t1x[NGLL][NGLL], t1z[NGLL][NGLL], t2x[NGLL][NGLL], t2z[NGLL][NGLL];
   for (int e = 0; e < s->nspec; e++) {
+    if (s->is_acoustic[e]) continue;
     const int *ib = s->ibool + (size_t)e * NGLL2;
     for (int j = 0; j < NGLL; j++)
       for (int i = 0; i < NGLL; i++) {
@@ -704,6 +858,7 @@ static void stacey_forces(orc_sim *s, const REAL *veloc, REAL *accel, REAL *stor
   /* compute_stacey_elastic.f90:34-1001 (P-SV, no Bielak), SAVE_FORWARD storage */
   for (int a = 0; a < s->nabs; a++) {
     int e = s->abs_ispec[a] - 1, code = s->abs_code[a];
+    if (s->is_acoustic[e]) continue;
     for (int k = 0; k < NGLL; k++) {
       int i, j;
       if (code == 4) { i = 0; j = k; }
@@ -739,6 +894,7 @@ static void add_sources(orc_sim *s, REAL *accel, int it /* 1-based */) {
   /* compute_add_sources_viscoelastic.f90:34-100 */
   for (int k = 0; k < s->nsrc; k++) {
     int e = s->src_ispec[k] - 1;
+    if (s->is_acoustic[e]) continue;
     REAL stf = s->stf[(size_t)k * s->p.nstep + (it - 1)];
     for (int j = 0; j < NGLL; j++)
       for (int i = 0; i < NGLL; i++) {
@@ -750,8 +906,169 @@ static void add_sources(orc_sim *s, REAL *accel, int it /* 1-based */) {
   }
 }
 
+/* acoustic element forces: compute_forces_viscoacoustic.f90:34-330 (no PML, attenuation or AXISYM) */
+static void acoustic_forces(orc_sim *s, const REAL *pot, REAL *pot_dd) {
+  const REAL (*H)[NGLL] = (const REAL(*)[NGLL])s->hprime;
+  const REAL (*HW)[NGLL] = (const REAL(*)[NGLL])s->hprimewgll;
+  REAL u[NGLL][NGLL], t1[NGLL][NGLL], t2[NGLL][NGLL]; /* [j][i] */
+  for (int e = 0; e < s->nspec; e++) {
+    if (!s->is_acoustic[e]) continue;
+    const int *ib = s->ibool + (size_t)e * NGLL2;
+    for (int j = 0; j < NGLL; j++)
+      for (int i = 0; i < NGLL; i++) u[j][i] = pot[ib[j * NGLL + i] - 1];
+    for (int j = 0; j < NGLL; j++)
+      for (int i = 0; i < NGLL; i++) {
+        size_t l = (size_t)e * NGLL2 + j * NGLL + i;
+        /* mxm_2comp_singleA (compute_forces_viscoacoustic.f90:418-477) */
+        REAL dxi = u[j][0] * H[i][0] + u[j][1] * H[i][1] + u[j][2] * H[i][2] + u[j][3] * H[i][3] + u[j][4] * H[i][4];
+        REAL dga = u[0][i] * H[j][0] + u[1][i] * H[j][1] + u[2][i] * H[j][2] + u[3][i] * H[j][3] + u[4][i] * H[j][4];
+        REAL xixl = s->xix[l], xizl = s->xiz[l], gaxl = s->gammax[l], gazl = s->gammaz[l];
+        REAL fac = s->jacobian[l] / s->rhostore[l];
+        REAL dxl = dxi * xixl + dga * gaxl;
+        REAL dzl = dxi * xizl + dga * gazl;
+        t1[j][i] = fac * (xixl * dxl + xizl * dzl);
+        t2[j][i] = fac * (gaxl * dxl + gazl * dzl);
+      }
+    for (int j = 0; j < NGLL; j++)
+      for (int i = 0; i < NGLL; i++) {
+        REAL a1 = 0, a2 = 0;
+        for (int k = 0; k < NGLL; k++) {
+          a1 = a1 + t1[j][k] * HW[k][i];
+          a2 = a2 + t2[k][i] * HW[k][j];
+        }
+        int g = ib[j * NGLL + i] - 1;
+        pot_dd[g] = pot_dd[g] - (s->wxgll[j] * a1 + s->wxgll[i] * a2);
+      }
+  }
+}
+
+/* compute_stacey_acoustic.f90:34-160 */
+static void acoustic_stacey(orc_sim *s, const REAL *pot_dot, REAL *pot_dd) {
+  for (int a = 0; a < s->nabs; a++) {
+    int e = s->abs_ispec[a] - 1, code = s->abs_code[a];
+    if (!s->is_acoustic[e]) continue;
+    for (int k = s->abs_jbegin[a]; k <= s->abs_jend[a]; k++) {
+      int i, j;
+      side_point(code, k, &i, &j);
+      size_t l = (size_t)e * NGLL2 + j * NGLL + i;
+      int g = s->ibool[l] - 1;
+      REAL nx, nz, j1d;
+      edge_normal(s, code, l, &nx, &nz, &j1d);
+      REAL weight = j1d * ((code == 4 || code == 2) ? s->wxgll[j] : s->wxgll[i]);
+      pot_dd[g] = pot_dd[g] - pot_dot[g] * weight / s->rho_vp[l];
+    }
+  }
+}
+
+/* compute_add_sources_acoustic.f90:34-75 (pressure sources, source_type 1) */
+static void add_sources_acoustic(orc_sim *s, REAL *pot_dd, int it /* 1-based */) {
+  for (int k = 0; k < s->nsrc; k++) {
+    int e = s->src_ispec[k] - 1;
+    if (!s->is_acoustic[e]) continue;
+    REAL stf = s->stf[(size_t)k * s->p.nstep + (it - 1)];
+    for (int j = 0; j < NGLL; j++)
+      for (int i = 0; i < NGLL; i++) {
+        size_t l = (size_t)e * NGLL2 + j * NGLL + i;
+        int g = s->ibool[l] - 1;
+        pot_dd[g] = pot_dd[g] + s->sourcearrays[((size_t)k * NGLL2 + j * NGLL + i) * 2] * stf / s->kappastore[l];
+      }
+  }
+}
+
+/* point k (ipoin1D - 1) of a fluid-solid edge: the reference walks the acoustic side with ivalue_inverse and the
+   elastic side with ivalue, which meet at the same global point (get_coupling_edges.f90:93-111) */
+static void coupling_point(int code, int k, int inverse, int *i, int *j) {
+  switch (code) {
+    case 1: *i = inverse ? NGLL - 1 - k : k; *j = 0; break;              /* bottom */
+    case 2: *i = NGLL - 1; *j = inverse ? NGLL - 1 - k : k; break;      /* right */
+    case 3: *i = inverse ? k : NGLL - 1 - k; *j = NGLL - 1; break;      /* top */
+    default: *i = 0; *j = inverse ? k : NGLL - 1 - k; break;            /* left */
+  }
+}
+
+/* normal (out of the fluid) and weight at point (i,j) of side code of an acoustic element
+   (compute_coupling_acoustic_el.f90:80-140) */
+static void coupling_normal(const orc_sim *s, int e, int code, int i, int j, REAL *nx, REAL *nz, REAL *weight) {
+  size_t l = (size_t)e * NGLL2 + j * NGLL + i;
+  REAL j1d;
+  edge_normal(s, code, l, nx, nz, &j1d);
+  *weight = j1d * ((code == 4 || code == 2) ? s->wxgll[j] : s->wxgll[i]);
+}
+
+/* fluid side: pot_dd += w (u . n) with the solid displacement (compute_coupling_acoustic_el.f90:34-150) */
+static void coupling_acoustic_el(orc_sim *s, const REAL *displ, REAL *pot_dd) {
+  for (int c = 0; c < s->ncoup; c++) {
+    const int ea = s->coup_ac_ispec[c], ca = s->coup_ac_code[c];
+    const int ee = s->coup_el_ispec[c], ce = s->coup_el_code[c];
+    for (int k = 0; k < NGLL; k++) {
+      int i, j;
+      coupling_point(ce, k, 1, &i, &j);
+      int g = s->ibool[(size_t)ee * NGLL2 + j * NGLL + i] - 1;
+      REAL dx = displ[2 * g], dz = displ[2 * g + 1];
+      coupling_point(ca, k, 0, &i, &j);
+      g = s->ibool[(size_t)ea * NGLL2 + j * NGLL + i] - 1;
+      REAL nx, nz, w;
+      coupling_normal(s, ea, ca, i, j, &nx, &nz, &w);
+      REAL dn = dx * nx + dz * nz;
+      pot_dd[g] = pot_dd[g] + w * dn;
+    }
+  }
+}
+
+/* solid side: accel += w n p with the fluid pressure p = -pot_dd (compute_coupling_viscoelastic_ac.f90:34-130) */
+static void coupling_elastic_ac(orc_sim *s, const REAL *pot_dd, REAL *accel) {
+  for (int c = 0; c < s->ncoup; c++) {
+    const int ea = s->coup_ac_ispec[c], ca = s->coup_ac_code[c];
+    const int ee = s->coup_el_ispec[c], ce = s->coup_el_code[c];
+    for (int k = 0; k < NGLL; k++) {
+      int i, j, ii, jj;
+      coupling_point(ca, k, 1, &i, &j);
+      int g = s->ibool[(size_t)ea * NGLL2 + j * NGLL + i] - 1;
+      REAL pressure = -pot_dd[g];
+      coupling_point(ce, k, 0, &ii, &jj);
+      g = s->ibool[(size_t)ee * NGLL2 + jj * NGLL + ii] - 1;
+      REAL nx, nz, w;
+      coupling_normal(s, ea, ca, i, j, &nx, &nz, &w);
+      accel[2 * g] = accel[2 * g] + w * nx * pressure;
+      accel[2 * g + 1] = accel[2 * g + 1] + w * nz * pressure;
+    }
+  }
+}
+
+/* enforce_acoustic_free_surface.f90:34-80 */
+static void enforce_free_surface(orc_sim *s) {
+  for (int f = 0; f < s->nfree; f++)
+    for (int k = 0; k < NGLL; k++) {
+      int i, j;
+      side_point(s->free_code[f], k, &i, &j);
+      int g = s->ibool[(size_t)s->free_ispec[f] * NGLL2 + j * NGLL + i] - 1;
+      s->pot[g] = (REAL)0;
+      s->pot_dot[g] = (REAL)0;
+      s->pot_dd[g] = (REAL)0;
+    }
+}
+
 static void record_seismograms(orc_sim *s, const REAL *displ, double *seis, int it, int nstep) {
   /* write_seismograms.F90:60-200, compute_interpolated_dva.f90 */
+  if (s->p.seismotype == 4) {
+    /* pressure in fluid elements: -pot with USE_TRICK_FOR_BETTER_PRESSURE, else -pot_dd
+       (compute_pressure.f90:460-490); solid receivers record zero here */
+    const REAL *f = s->p.use_trick ? s->pot : s->pot_dd;
+    for (int r = 0; r < s->nrec; r++) {
+      int e = s->rec_ispec[r] - 1;
+      double vx = 0.0;
+      if (s->is_acoustic[e])
+        for (int j = 0; j < NGLL; j++)
+          for (int i = 0; i < NGLL; i++) {
+            int g = s->ibool[e * NGLL2 + j * NGLL + i] - 1;
+            double hl = (double)(s->xir[r * NGLL + i] * s->gammar[r * NGLL + j]);
+            vx = vx + (double)(-f[g]) * hl;
+          }
+      seis[((size_t)r * 2 + 0) * nstep + it] = vx;
+      seis[((size_t)r * 2 + 1) * nstep + it] = 0.0;
+    }
+    return;
+  }
   for (int r = 0; r < s->nrec; r++) {
     int e = s->rec_ispec[r] - 1;
     double vx = 0.0, vz = 0.0;
@@ -779,6 +1096,7 @@ void *orc_new(const orc_params *p, const double *rho, const double *vp, const do
   setup_numbering(s);
   setup_geometry_materials(s, rho, vp, vs);
   if (p->stacey) setup_absorbing(s);
+  setup_domains(s);
   setup_mass(s);
   s->nsrc = nsrc;
   s->src = (orc_source *)calloc((size_t)nsrc + 1, sizeof(orc_source));
@@ -791,6 +1109,9 @@ void *orc_new(const orc_params *p, const double *rho, const double *vp, const do
   s->displ = (REAL *)calloc((size_t)2 * s->nglob, sizeof(REAL));
   s->veloc = (REAL *)calloc((size_t)2 * s->nglob, sizeof(REAL));
   s->accel = (REAL *)calloc((size_t)2 * s->nglob, sizeof(REAL));
+  s->pot = (REAL *)calloc((size_t)s->nglob, sizeof(REAL));
+  s->pot_dot = (REAL *)calloc((size_t)s->nglob, sizeof(REAL));
+  s->pot_dd = (REAL *)calloc((size_t)s->nglob, sizeof(REAL));
   return s;
 }
 
@@ -805,6 +1126,10 @@ void orc_free(void *h) {
   free(s->src); free(s->src_ispec); free(s->src_xi); free(s->src_gamma); free(s->sourcearrays); free(s->stf);
   free(s->rec_xz); free(s->rec_ispec); free(s->rec_xi); free(s->rec_gamma); free(s->xir); free(s->gammar);
   free(s->displ); free(s->veloc); free(s->accel);
+  free(s->is_acoustic); free(s->rmass_ac_inv); free(s->free_ispec); free(s->free_code);
+  free(s->coup_ac_ispec); free(s->coup_ac_code); free(s->coup_el_ispec); free(s->coup_el_code);
+  free(s->abs_jbegin); free(s->abs_jend);
+  free(s->pot); free(s->pot_dot); free(s->pot_dd);
   free(s);
 }
 
@@ -893,21 +1218,47 @@ void orc_forward(void *h, int nstep_run, double *seis, double *stacey_store_d) {
   REAL *d = s->displ, *v = s->veloc, *a = s->accel;
   REAL *store = NULL;
   if (stacey_store_d && s->nabs > 0) store = (REAL *)calloc((size_t)nstep_run * s->nabs * NGLL * 2, sizeof(REAL));
+  REAL *pa = s->pot, *pv = s->pot_dot, *pdd = s->pot_dd;
+  const size_t n1 = (size_t)s->nglob;
   for (int step = 0; step < nstep_run; step++) {
     const int it = s->it_done + step + 1;
-    /* update_displacement_newmark_elastic (update_displacement_Newmark.F90:408-454) */
-    for (size_t k = 0; k < n2; k++) {
-      d[k] = d[k] + deltat * v[k] + deltatsqover2 * a[k];
-      v[k] = v[k] + deltatover2 * a[k];
-      a[k] = (REAL)0;
+    /* update_displacement_newmark_acoustic / _elastic (update_displacement_Newmark.F90:190-454) */
+    if (s->any_acoustic)
+      for (size_t k = 0; k < n1; k++) {
+        pa[k] = pa[k] + deltat * pv[k] + deltatsqover2 * pdd[k];
+        pv[k] = pv[k] + deltatover2 * pdd[k];
+        pdd[k] = (REAL)0;
+      }
+    if (s->any_elastic)
+      for (size_t k = 0; k < n2; k++) {
+        d[k] = d[k] + deltat * v[k] + deltatsqover2 * a[k];
+        v[k] = v[k] + deltatover2 * a[k];
+        a[k] = (REAL)0;
+      }
+    /* compute_forces_viscoacoustic_main (compute_forces_viscoacoustic_calling_routine.F90:34-230): free
+       surface; iphase 1 (Stacey, coupling with the predicted solid displacement, sources); iphase 2
+       (elements); mass; corrector; free surface */
+    if (s->any_acoustic) {
+      enforce_free_surface(s);
+      if (s->p.stacey && s->nabs > 0) acoustic_stacey(s, pv, pdd);
+      if (s->ncoup > 0) coupling_acoustic_el(s, d, pdd);
+      add_sources_acoustic(s, pdd, it);
+      acoustic_forces(s, pa, pdd);
+      for (size_t k = 0; k < n1; k++) pdd[k] = pdd[k] * s->rmass_ac_inv[k];
+      for (size_t k = 0; k < n1; k++) pv[k] = pv[k] + deltatover2 * pdd[k];
+      enforce_free_surface(s);
     }
-    /* compute_forces_viscoelastic_main: iphase 1 (Stacey, sources), iphase 2 (elements) */
-    if (s->p.stacey && s->nabs > 0) stacey_forces(s, v, a, store, step, nstep_run);
-    add_sources(s, a, it);
-    element_forces(s, d, a);
-    for (size_t k = 0; k < n2; k++) a[k] = a[k] * s->rmass_inv[k];
-    /* update_veloc_elastic_Newmark */
-    for (size_t k = 0; k < n2; k++) v[k] = v[k] + deltatover2 * a[k];
+    /* compute_forces_viscoelastic_main: iphase 1 (Stacey, coupling with the new fluid pressure, sources),
+       iphase 2 (elements) */
+    if (s->any_elastic) {
+      if (s->p.stacey && s->nabs > 0) stacey_forces(s, v, a, store, step, nstep_run);
+      if (s->ncoup > 0) coupling_elastic_ac(s, pdd, a);
+      add_sources(s, a, it);
+      element_forces(s, d, a);
+      for (size_t k = 0; k < n2; k++) a[k] = a[k] * s->rmass_inv[k];
+      /* update_veloc_elastic_Newmark */
+      for (size_t k = 0; k < n2; k++) v[k] = v[k] + deltatover2 * a[k];
+    }
     if (seis && s->nrec > 0) record_seismograms(s, d, seis, step, nstep_run);
   }
   s->it_done += nstep_run;
@@ -920,6 +1271,69 @@ void orc_forward(void *h, int nstep_run, double *seis, double *stacey_store_d) {
 }
 
 void orc_set_ftz(void *h, int on) { ((orc_sim *)h)->ftz = on; }
+
+/* acoustic domain (test accessors) */
+int orc_nspec_acoustic(void *h) { return ((orc_sim *)h)->nspec_acoustic; }
+int orc_ncoupling(void *h) { return ((orc_sim *)h)->ncoup; }
+int orc_nfree(void *h) { return ((orc_sim *)h)->nfree; }
+void orc_get_is_acoustic(void *h, int *out) {
+  orc_sim *s = (orc_sim *)h;
+  memcpy(out, s->is_acoustic, sizeof(int) * (size_t)s->nspec);
+}
+void orc_get_rmass_acoustic(void *h, double *out) {
+  orc_sim *s = (orc_sim *)h;
+  for (int g = 0; g < s->nglob; g++) out[g] = (double)s->rmass_ac_inv[g];
+}
+/* fluid-solid edges: 0-based elements and side codes (1 bottom, 2 right, 3 top, 4 left) */
+void orc_get_coupling(void *h, int *ac_ispec, int *ac_code, int *el_ispec, int *el_code) {
+  orc_sim *s = (orc_sim *)h;
+  for (int c = 0; c < s->ncoup; c++) {
+    ac_ispec[c] = s->coup_ac_ispec[c];
+    ac_code[c] = s->coup_ac_code[c];
+    el_ispec[c] = s->coup_el_ispec[c];
+    el_code[c] = s->coup_el_code[c];
+  }
+}
+void orc_get_free_surface(void *h, int *ispec, int *code) {
+  orc_sim *s = (orc_sim *)h;
+  for (int f = 0; f < s->nfree; f++) {
+    ispec[f] = s->free_ispec[f];
+    code[f] = s->free_code[f];
+  }
+}
+void orc_get_acoustic_stacey_range(void *h, int *jbegin, int *jend) {
+  orc_sim *s = (orc_sim *)h;
+  for (int a = 0; a < s->nabs; a++) {
+    jbegin[a] = s->abs_jbegin[a];
+    jend[a] = s->abs_jend[a];
+  }
+}
+void orc_get_acoustic_fields(void *h, double *pot, double *pot_dot, double *pot_dd) {
+  orc_sim *s = (orc_sim *)h;
+  for (int g = 0; g < s->nglob; g++) {
+    if (pot) pot[g] = (double)s->pot[g];
+    if (pot_dot) pot_dot[g] = (double)s->pot_dot[g];
+    if (pot_dd) pot_dd[g] = (double)s->pot_dd[g];
+  }
+}
+void orc_set_acoustic_fields(void *h, const double *pot, const double *pot_dot, const double *pot_dd) {
+  orc_sim *s = (orc_sim *)h;
+  for (int g = 0; g < s->nglob; g++) {
+    s->pot[g] = pot ? (REAL)pot[g] : (REAL)0;
+    s->pot_dot[g] = pot_dot ? (REAL)pot_dot[g] : (REAL)0;
+    s->pot_dd[g] = pot_dd ? (REAL)pot_dd[g] : (REAL)0;
+  }
+}
+/* one application of the assembled acoustic operator: out = -K pot (element forces only) */
+void orc_apply_acoustic_stiffness(void *h, const double *pot, double *out) {
+  orc_sim *s = (orc_sim *)h;
+  REAL *u = (REAL *)calloc((size_t)s->nglob, sizeof(REAL)), *a = (REAL *)calloc((size_t)s->nglob, sizeof(REAL));
+  for (int g = 0; g < s->nglob; g++) u[g] = (REAL)pot[g];
+  acoustic_forces(s, u, a);
+  for (int g = 0; g < s->nglob; g++) out[g] = (double)a[g];
+  free(u);
+  free(a);
+}
 
 void orc_get_fields(void *h, double *displ, double *veloc, double *accel) {
   orc_sim *s = (orc_sim *)h;

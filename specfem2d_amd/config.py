@@ -33,7 +33,7 @@ class Source:
     factor: float = 1.0e10
     anglesource: float = 0.0  # degrees
     tshift: float = 0.0
-    time_function_type: int = 1  # 1 Ricker, 2 first derivative of a Gaussian
+    time_function_type: int = 1  # 1 Ricker, 2 first derivative of a Gaussian, 3 Gaussian
 
 
 @dataclass
@@ -59,6 +59,8 @@ class SimulationConfig:
     receivers: Optional[np.ndarray] = None  # [nrec, 2] (x, z)
     user_t0: float = 0.0
     anglerec: float = 0.0
+    seismotype: int = 1  # 1 displacement (receivers in solid elements), 4 pressure (receivers in fluid elements)
+    use_trick: bool = False  # USE_TRICK_FOR_BETTER_PRESSURE
     name: str = "custom"
 
     # ------------------------------------------------------------------
@@ -257,3 +259,73 @@ def small_test_mesh(nx: int = 8, nz: int = 6, nstep: int = 200, seed: int = 0, s
         p = 0.05 * rng.standard_normal(n)
         cfg = cfg.replace(material_mode=1, rho=2500.0 * (1 + p), vp=3000.0 * (1 + p), vs=1700.0 * (1 - p))
     return cfg
+
+
+def pressure_source_acoustic_44x44(nstep: int = 1400) -> SimulationConfig:
+    """``EXAMPLES/check_absolute_amplitude_of_pressure_source_seismograms_acoustic/REF_SEIS/Par_file`` +
+    ``SOURCE``: a homogeneous fluid (rho 2000, vp 3297.849, vs 0) of 44 x 44 elements over 2000 m x 2000 m, a
+    Gaussian pressure source of 18 Hz at (1000, 1000), one pressure receiver at (1500, 1500), dt 4e-4, 1400
+    steps, USE_TRICK_FOR_BETTER_PRESSURE.  The reference absorbs with a 3-element CPML on every side; here the
+    sides are Stacey edges, which changes the trace only after the waves reach that layer (about 0.35 s)."""
+    return SimulationConfig(
+        name="pressure_source_acoustic_44x44", nx=44, nz=44, xmin=0.0, xmax=2000.0, interfaces=[0.0, 2000.0],
+        nz_layers=[44], nstep=nstep, dt=4.0e-4, rho=2000.0, vp=3297.849, vs=0.0, material_mode=0, stacey=True,
+        absorbbottom=True, absorbright=True, absorbtop=True, absorbleft=True,
+        sources=[Source(x=1000.0, z=1000.0, f0=18.0, factor=1.0, time_function_type=3)],
+        receivers=np.array([[1500.0, 1500.0]]), seismotype=4, use_trick=True)
+
+
+def layered_fluid_solid(nx: int, nz: int, nz_fluid: int, elem_size: float, nstep: int, dt: float,
+                        rho_f: float = 1020.0, vp_f: float = 1500.0, rho_s: float = 2500.0, vp_s: float = 3000.0,
+                        vs_s: float = 1732.0, absorbtop: bool = False, pert=None, name: Optional[str] = None,
+                        sources: Optional[List[Source]] = None, receivers=None,
+                        seismotype: int = 1) -> SimulationConfig:
+    """A fluid layer of ``nz_fluid`` elements (vs = 0) over a solid: ``nx`` x ``nz`` elements of ``elem_size``,
+    the free surface (or, with ``absorbtop``, a Stacey edge) on top of the fluid and Stacey edges elsewhere,
+    MODEL=gll arrays; ``pert`` (an array over the GLL points) scales the solid's rho, vp and vs by 1 + pert."""
+    xmax, depth = nx * elem_size, nz * elem_size
+    cfg = SimulationConfig(
+        name=name or f"fluid_solid_{nx}x{nz}", nx=nx, nz=nz, xmin=0.0, xmax=xmax, interfaces=[-depth, 0.0],
+        nz_layers=[nz], nstep=nstep, dt=dt, material_mode=1, stacey=True, absorbbottom=True, absorbright=True,
+        absorbtop=absorbtop, absorbleft=True, sources=sources or [], receivers=receivers, seismotype=seismotype)
+    ez = np.repeat(np.arange(nz), nx)
+    fluid = np.repeat((ez >= nz - nz_fluid)[:, None], NGLL2, axis=1).reshape(-1)
+    p = np.zeros(fluid.size) if pert is None else np.asarray(pert, np.float64).reshape(-1)
+    rho = np.where(fluid, rho_f, rho_s * (1.0 + p))
+    vp = np.where(fluid, vp_f, vp_s * (1.0 + p))
+    vs = np.where(fluid, 0.0, vs_s * (1.0 + p))
+    return cfg.replace(rho=rho, vp=vp, vs=vs)
+
+
+def fluid_solid_plane_wave(nstep: int = 1700, solid: bool = True, absorbtop: bool = True) -> SimulationConfig:
+    """Plane-wave reflection at a flat fluid-solid interface: 120 x 80 elements of 50 m (6 km x 4 km), the top
+    40 rows fluid (rho 1020, vp 1500), the bottom 40 solid (rho 2500, vp 3000, vs 1732; fluid too when not
+    ``solid``), a line of 480 Ricker pressure sources of 5 Hz 1000 m above the interface, pressure receivers
+    500 m above it (and one 500 m above the source line), dt 1e-3, USE_TRICK_FOR_BETTER_PRESSURE."""
+    depth = 4000.0
+    srcs = [Source(x=float(x), z=-1000.0, f0=5.0, factor=1.0, time_function_type=1)
+            for x in np.arange(6.25, 6000.0, 12.5)]
+    rec = np.array([[2500.0, -1500.0], [3000.0, -1500.0], [3500.0, -1500.0], [3000.0, -500.0]])
+    cfg = layered_fluid_solid(120, 80, 40 if solid else 80, 50.0, nstep, 1.0e-3, absorbtop=absorbtop,
+                              name="fluid_solid_plane_wave", sources=srcs, receivers=rec, seismotype=4)
+    assert cfg.interfaces[0] == -depth
+    return cfg.replace(use_trick=True)
+
+
+def c2_mesh_coupled(nstep: int = 3000, nx: int = 2000, nz: int = 1000, elem_size: float = 10.0,
+                    nrec: int = 200, seed: int = 2) -> SimulationConfig:
+    """The large coupled forward workload on one GPU: 2000 x 1000 elements of 10 m (20 km x 10 km, about 32 M
+    GLL points), the top 1 km an acoustic water layer (vp 1500, rho 1020) under a free surface, the solid below a
+    seeded Gaussian random field (correlation length 200 m, +-10 %) around vp 3000, vs 1732, rho 2500; a Ricker
+    force of 8 Hz at a third of the depth, ``nrec`` displacement receivers 20 m below the sea floor; Stacey
+    edges on the other three sides (the reference configuration's CPML is not part of this path), dt 2e-4."""
+    xmax, depth = nx * elem_size, nz * elem_size
+    nzf = nz // 10
+    sea_floor = -nzf * elem_size
+    rec = np.stack([np.linspace(elem_size, xmax - elem_size, nrec), np.full(nrec, sea_floor - 20.0)], axis=1)
+    src = [Source(x=xmax / 2, z=-depth / 3, f0=8.0, factor=1.0e10)]
+    base = SimulationConfig(name="c2", nx=nx, nz=nz, xmin=0.0, xmax=xmax, interfaces=[-depth, 0.0],
+                            nz_layers=[nz], nstep=nstep, dt=2.0e-4)
+    pert = gaussian_random_field(base, 200.0, 0.10, seed)
+    return layered_fluid_solid(nx, nz, nzf, elem_size, nstep, 2.0e-4, pert=pert,
+                               name=f"c2_mesh_coupled_{nx}x{nz}", sources=src, receivers=rec)

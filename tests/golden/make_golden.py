@@ -30,6 +30,14 @@ def pack(example, sub, stations, out):
     print("wrote", out, {k: a.shape for k, a in arrays.items()})
 
 
+def pack_pressure(example, sub, station, out):
+    t, v = read_trace(os.path.join(REF, "EXAMPLES", example, sub, f"{station}.PRE.semp"))
+    np.savez_compressed(os.path.join(HERE, out), time=t, **{f"{station}.PRE": v})
+    print("wrote", out, t.shape)
+
+
 if __name__ == "__main__":
     pack("Tromp2005_kernel", "REF_SEIS", ["AA.S0001"], "tromp2005_kernel_ref_seis.npz")
+    pack_pressure("check_absolute_amplitude_of_pressure_source_seismograms_acoustic", "REF_SEIS", "AA.S0001",
+                  "pressure_source_acoustic_ref_seis.npz")
     pack("Tromp2005_kernel", "REF_KERNEL", ["AA.S0001"], "tromp2005_kernel_ref_kernel_traces.npz")

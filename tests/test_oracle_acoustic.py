@@ -1,0 +1,156 @@
+"""Pins the oracle's acoustic (fluid) and fluid-solid coupling path before it is trusted as the checker.
+
+What pins each part (oracle/sem2d_oracle.c, acoustic routines):
+
+* acoustic element forces, acoustic mass matrix, pressure source arrays (pressure source in a fluid
+  element), Gaussian time function with USE_TRICK_FOR_BETTER_PRESSURE, Newmark on the potential, pressure
+  seismograms (-potential) -- the reference's committed trace
+  ``EXAMPLES/check_absolute_amplitude_of_pressure_source_seismograms_acoustic/REF_SEIS/AA.S0001.PRE.semp``,
+  to 2e-6 of its peak before the waves reach the reference's absorbing layer (a CPML there, Stacey here) and
+  to 1e-6 relative at the peak (a few float ulps);
+* pressure without the trick (-potential_dot_dot with the Gaussian itself): the same trace, since the two
+  recordings are the same continuous quantity (mathematical equivalence, discretisation-level tolerance);
+* fluid-solid coupling, both directions: the normal-incidence plane-wave reflection coefficient
+  (Z2 - Z1) / (Z2 + Z1) of a flat fluid-solid interface (a one-way or sign-flipped coupling gives +1 or
+  worse);
+* acoustic free surface: reflection coefficient -1 for pressure;
+* acoustic Stacey edges: reflection below 1e-3 at normal incidence;
+* acoustic operator: symmetric negative semi-definite, and zero on a constant potential (brute force on a
+  small mesh);
+* domain detection, free-surface edges, fluid-solid edges and the acoustic Stacey point ranges -- the mesher's
+  rules (determine_acoustic_surface.f90, get_coupling_edges.f90) checked on a layered mesh.
+"""
+import numpy as np
+import pytest
+
+from oracle.oracle import OracleSim
+from specfem2d_amd import config
+
+
+def _ref_trace(golden_dir):
+    g = np.load(f"{golden_dir}/pressure_source_acoustic_ref_seis.npz")
+    return g["time"], g["AA.S0001.PRE"]
+
+
+@pytest.fixture(scope="module")
+def pressure_run():
+    o = OracleSim(config.pressure_source_acoustic_44x44(), "sp", ftz=False)
+    return o, o.forward()[0, 0]
+
+
+def test_pressure_source_ref_seis(pressure_run, golden_dir):
+    t, ref = _ref_trace(golden_dir)
+    o, p = pressure_run
+    assert o.nspec_acoustic == 44 * 44 and o.ncoupling == 0
+    assert t[0] == pytest.approx(-o.t0, rel=1e-15)
+    peak = np.abs(ref).max()
+    early = t < 0.35
+    assert np.abs(p[early] - ref[early]).max() < 2e-6 * peak
+    k = int(np.argmax(np.abs(ref)))
+    assert p[k] == pytest.approx(ref[k], rel=1e-6)  # a few float ulps of the pressure at the peak
+    # later the reference's CPML and the Stacey edges here differ
+    assert np.abs(p[t < 0.40] - ref[t < 0.40]).max() < 2e-4 * peak
+
+
+def test_pressure_without_trick_matches(pressure_run):
+    _, p_trick = pressure_run
+    cfg = config.pressure_source_acoustic_44x44().replace(use_trick=False)
+    o = OracleSim(cfg, "sp", ftz=False)
+    p = o.forward()[0, 0]
+    t = np.arange(cfg.nstep) * cfg.dt - o.t0
+    w = t < 0.35
+    peak = np.abs(p_trick).max()
+    assert np.abs(p[w] - p_trick[w]).max() < 2e-2 * peak
+    assert np.corrcoef(p[w], p_trick[w])[0, 1] > 0.999
+
+
+def _direct_and_later(p, t, t_direct_end, window):
+    d = p[t < t_direct_end]
+    i = int(np.argmax(np.abs(d)))
+    m = (t > window[0]) & (t < window[1])
+    j = int(np.argmax(np.abs(p[m])))
+    return d[i], p[m][j]
+
+
+@pytest.fixture(scope="module")
+def plane_wave():
+    cfg = config.fluid_solid_plane_wave()
+    o = OracleSim(cfg, "sp")
+    return cfg, o, o.forward()
+
+
+def test_fluid_solid_reflection_coefficient(plane_wave):
+    cfg, o, s = plane_wave
+    assert o.ncoupling == cfg.nx
+    t = np.arange(cfg.nstep) * cfg.dt - o.t0
+    z1, z2 = 1020.0 * 1500.0, 2500.0 * 3000.0
+    r_exact = (z2 - z1) / (z2 + z1)
+    for r in range(3):
+        direct, refl = _direct_and_later(s[r, 0], t, 0.7, (0.8, 1.3))
+        assert refl / direct == pytest.approx(r_exact, rel=2e-3)
+
+
+def test_acoustic_stacey_absorbs(plane_wave):
+    cfg, o, s = plane_wave
+    t = np.arange(cfg.nstep) * cfg.dt - o.t0
+    direct, refl = _direct_and_later(s[3, 0], t, 0.7, (0.8, 1.3))  # the top edge is 500 m above this receiver
+    assert abs(refl / direct) < 1e-3
+
+
+def test_acoustic_free_surface_reflection():
+    cfg = config.fluid_solid_plane_wave(solid=False, absorbtop=False)
+    o = OracleSim(cfg, "sp")
+    s = o.forward()
+    t = np.arange(cfg.nstep) * cfg.dt - o.t0
+    assert o.ncoupling == 0
+    direct, refl = _direct_and_later(s[3, 0], t, 0.7, (0.8, 1.3))
+    assert refl / direct == pytest.approx(-1.0, rel=2e-3)
+    # the free surface holds the potential at zero
+    pot, pot_dot, pot_dd = o.acoustic_fields()
+    isp, code = o.free_surface()
+    ib = o.ibool()
+    top = ib[isp][:, 20:25].reshape(-1) - 1
+    assert np.all(code == 3) and len(isp) == cfg.nx
+    assert np.all(pot[top] == 0) and np.all(pot_dot[top] == 0) and np.all(pot_dd[top] == 0)
+
+
+def test_acoustic_operator_symmetric_negative_semidefinite():
+    cfg = config.layered_fluid_solid(5, 4, 4, 100.0, 10, 1e-3)
+    rng = np.random.default_rng(3)
+    n = cfg.nspec * 25
+    cfg = cfg.replace(rho=1000.0 * (1 + 0.2 * rng.random(n)), vp=1500.0 * (1 + 0.2 * rng.random(n)))
+    o = OracleSim(cfg, "dp")
+    assert o.nspec_acoustic == cfg.nspec
+    K = np.stack([o.apply_acoustic_stiffness(e) for e in np.eye(o.nglob)], axis=1)  # -K
+    assert np.abs(K - K.T).max() < 1e-12 * np.abs(K).max()
+    ev = np.linalg.eigvalsh(0.5 * (K + K.T))
+    assert ev.max() < 1e-10 * np.abs(ev).max()
+    assert np.abs(o.apply_acoustic_stiffness(np.ones(o.nglob))).max() < 1e-12 * np.abs(K).max()
+
+
+def test_domains_edges_and_stacey_ranges():
+    cfg = config.layered_fluid_solid(6, 5, 2, 100.0, 10, 1e-3)
+    o = OracleSim(cfg, "sp")
+    isac = o.is_acoustic()
+    assert isac.sum() == 12 and np.all(isac[18:]) and not np.any(isac[:18])
+    cp = o.coupling()
+    # every bottom-row fluid element over the solid element below it
+    assert cp.tolist() == [[18 + i, 1, 12 + i, 3] for i in range(6)]
+    isp, code = o.free_surface()
+    assert isp.tolist() == list(range(24, 30)) and np.all(code == 3)
+    # a fluid element's left/right absorbing edge leaves out the point on the fluid-solid edge
+    aisp, acode, corner, _ = o.absorbing()
+    jb, je = o.acoustic_stacey_range()
+    for a in range(len(aisp)):
+        e = aisp[a] - 1
+        if e in (18, 23) and acode[a] in (2, 4):
+            assert (jb[a], je[a]) == (1, 4)
+        elif isac[e]:
+            assert (jb[a], je[a]) == (0, 4)
+    # masses: the solid's at fluid-only points and the fluid's at solid-only points are unused (set to 1)
+    rma = o.rmass_acoustic()
+    rme = o.rmass_inverse()
+    ib = o.ibool() - 1
+    solid_only = np.setdiff1d(ib[:12].reshape(-1), ib[18:].reshape(-1))
+    fluid_only = np.setdiff1d(ib[18:].reshape(-1), ib[:18].reshape(-1))
+    assert np.all(rma[solid_only] == 1) and np.all(rme[fluid_only] == 1)
