@@ -35,12 +35,14 @@ typedef struct {
   double user_t0;                      /* USER_T0 */
   double anglerec;                     /* degrees */
   int material_mode;                   /* 0: material table (rho,vp,vs per point), 1: MODEL=gll */
+  int seismotype;                      /* 1 displacement (receivers in solid elements), 4 pressure (in fluid) */
+  int use_trick;                       /* USE_TRICK_FOR_BETTER_PRESSURE */
 } sf2d_params;
 
-/* DATA/SOURCE subset: collocated force */
+/* DATA/SOURCE subset: collocated force in a solid element, pressure source in a fluid element (vs = 0) */
 typedef struct {
   double x, z, f0, factor, anglesource, tshift;
-  int time_function_type;              /* 1 Ricker, 2 first derivative of a Gaussian */
+  int time_function_type;              /* 1 Ricker, 2 first derivative of a Gaussian, 3 Gaussian */
   int pad;
 } sf2d_source;
 
@@ -115,6 +117,14 @@ void sf2d_sim_reset(void* sim);
 int sf2d_sim_steps_done(void* sim);
 
 /* device fields (interleaved x,z; 2*nglob floats); NULL pointers are skipped */
+/* fluid elements (vs = 0): their count, and the potential, its first and second time derivatives [nglob] */
+int sf2d_sim_nspec_acoustic(void* sim);
+int sf2d_sim_ncoupling(void* sim);                       /* fluid-solid edges */
+void sf2d_sim_get_rmass_acoustic(void* sim, float* out); /* [nglob] inverse fluid mass (1 off the fluid) */
+/* fluid-solid edges: fluid element (1-based) and side (1 bottom, 2 right, 3 top, 4 left), solid element and side */
+void sf2d_sim_get_coupling(void* sim, int* ac_ispec, int* ac_code, int* el_ispec, int* el_code);
+void sf2d_sim_get_acoustic_fields(void* sim, float* pot, float* pot_dot, float* pot_dot_dot);
+void sf2d_sim_set_acoustic_fields(void* sim, const float* pot, const float* pot_dot, const float* pot_dot_dot);
 void sf2d_sim_get_fields(void* sim, float* displ, float* veloc, float* accel);
 void sf2d_sim_set_fields(void* sim, const float* displ, const float* veloc, const float* accel);
 void sf2d_sim_get_b_fields(void* sim, float* displ, float* veloc, float* accel);

@@ -13,10 +13,13 @@
  * Errors: like the reference (src/gpu/helper_functions.cu:95-121), a failing
  * call prints a message, appends it to OUTPUT_FILES/error_message_*.txt when
  * that directory exists, and terminates the process with EXIT_FAILURE.
- * Entry points of physics this library does not implement (acoustic
- * potentials, fluid-solid coupling, PML, attenuation, MPI halo exchange,
- * kernel smoothing, Hessians, moving sources) are exported so that the
- * reference binary links, and fail loudly when called.
+ * Forward fluid (acoustic) elements are implemented: potential, free
+ * surface, Stacey edges, pressure sources and seismograms (seismotype 4)
+ * and fluid-solid coupling.  Entry points of physics this library does not
+ * implement (backward/adjoint fluid wavefields and acoustic kernels, PML,
+ * attenuation, MPI halo exchange, kernel smoothing, Hessians, moving
+ * sources) are exported so that the reference binary links, and fail loudly
+ * when called.
  *
  * Each declaration cites the reference definition it replaces.
  */

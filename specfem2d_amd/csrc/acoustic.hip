@@ -1,0 +1,700 @@
+// Acoustic (fluid) elements on MI355X: the potential formulation of SPECFEM2D's fluid, its free surface, its
+// Stacey edges, pressure sources, pressure seismograms and the fluid-solid coupling terms, behind the reference's
+// Fortran-callable entry points (src/gpu/compute_forces_acoustic_cuda.cu, compute_stacey_acoustic_cuda.cu,
+// enforce_acoustic_free_surface_cuda.cu, compute_add_sources_viscoacoustic_cuda.cu, compute_coupling_cuda.cu,
+// update_displacement_cuda.cu, transfer_fields_cuda.cu, prepare_mesh_constants_cuda.cu:319-436).
+//
+// Layout.  The fluid's state lives beside the solid's in the same point numbering (the device numbering of the
+// handle, see points_to_device): potential, its first and second time derivatives and the inverse fluid mass,
+// one float per point.  The fluid elements are coloured on their own (greedy, no two elements of a colour share
+// a point) and their per-point operands are packed once in colour order -- {xix, xiz, gammax, gammaz}, J/rho and
+// the point id -- so one colour launch streams them with unit stride and assembles with plain read-modify-writes.
+// Boundary terms (free surface, Stacey, fluid-solid edges, sources, receivers) are flattened at set-up into
+// point lists with their weights, so each is one small launch that reads only what it updates.
+//
+// Set-up is two-phase: the entry points of the preparation record the caller's arrays; the first call of the
+// time loop resolves them into device ids (after the solid's preparation has fixed the numbering) and uploads
+// them.  Unsupported cases (backward or adjoint fluid wavefields, attenuation, MPI interfaces, SAVE_FORWARD
+// storage of the fluid's Stacey terms) stop the run with a message, as the reference does for its own limits.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "sf2d_mesh.h"
+#include "specfem2d_gpu.h"
+
+namespace sf2d {
+
+namespace {
+constexpr int AC_EPB = 10;  // fluid elements per 256-thread workgroup (250 active lanes)
+
+__device__ __forceinline__ void atomic_add(float* p, float v) { unsafeAtomicAdd(p, v); }
+
+// One colour of fluid elements: pdd -= K pot over elements first .. first+n-1 of the packed colour order
+// (compute_forces_viscoacoustic.f90:107-330; Kernel_2_acoustic_impl in src/gpu/kernels).
+__global__ __launch_bounds__(256) void k_acoustic_forces(int first, int n, const int* __restrict__ ids,
+                                                         const float4* __restrict__ geom,
+                                                         const float* __restrict__ fac, Basis b,
+                                                         const float* __restrict__ pot, float* __restrict__ pdd) {
+  __shared__ float s_u[AC_EPB][NGLL2];
+  __shared__ float s_t1[AC_EPB][NGLL2];
+  __shared__ float s_t2[AC_EPB][NGLL2];
+  const int t = threadIdx.x;
+  const int le = t / NGLL2;
+  const int p = t - le * NGLL2;
+  const int e = first + blockIdx.x * AC_EPB + le;
+  const bool active = le < AC_EPB && e < first + n;
+  const int i = p % NGLL, j = p / NGLL;
+  int g = 0;
+  float4 gm = make_float4(0.f, 0.f, 0.f, 0.f);
+  float f = 0.f;
+  if (active) {
+    const size_t l = (size_t)e * NGLL2 + p;
+    g = ids[l];
+    gm = geom[l];
+    f = fac[l];
+    s_u[le][p] = pot[g];
+  }
+  __syncthreads();
+  if (active) {
+    const float* u = s_u[le];
+    const float dxi = u[j * NGLL + 0] * b.hp[i * NGLL + 0] + u[j * NGLL + 1] * b.hp[i * NGLL + 1] +
+                      u[j * NGLL + 2] * b.hp[i * NGLL + 2] + u[j * NGLL + 3] * b.hp[i * NGLL + 3] +
+                      u[j * NGLL + 4] * b.hp[i * NGLL + 4];
+    const float dga = u[0 * NGLL + i] * b.hp[j * NGLL + 0] + u[1 * NGLL + i] * b.hp[j * NGLL + 1] +
+                      u[2 * NGLL + i] * b.hp[j * NGLL + 2] + u[3 * NGLL + i] * b.hp[j * NGLL + 3] +
+                      u[4 * NGLL + i] * b.hp[j * NGLL + 4];
+    const float dxl = dxi * gm.x + dga * gm.z;  // d/dx: xix, gammax
+    const float dzl = dxi * gm.y + dga * gm.w;  // d/dz: xiz, gammaz
+    s_t1[le][p] = f * (gm.x * dxl + gm.y * dzl);
+    s_t2[le][p] = f * (gm.z * dxl + gm.w * dzl);
+  }
+  __syncthreads();
+  if (active) {
+    float a1 = 0.f, a2 = 0.f;
+#pragma unroll
+    for (int k = 0; k < NGLL; k++) {
+      a1 = a1 + s_t1[le][j * NGLL + k] * b.hpw[k * NGLL + i];
+      a2 = a2 + s_t2[le][k * NGLL + i] * b.hpw[k * NGLL + j];
+    }
+    pdd[g] = pdd[g] - (b.w[j] * a1 + b.w[i] * a2);
+  }
+}
+
+// Newmark predictor on the potential (update_displacement_Newmark.F90:190-240)
+__global__ __launch_bounds__(256) void k_ac_predictor(float* __restrict__ pot, float* __restrict__ pd,
+                                                      float* __restrict__ pdd, int n, float dt, float dt2o2,
+                                                      float dto2) {
+  const int stride = gridDim.x * blockDim.x;
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
+    const float a = pdd[k], v = pd[k];
+    pot[k] = pot[k] + dt * v + dt2o2 * a;
+    pd[k] = v + dto2 * a;
+    pdd[k] = 0.f;
+  }
+}
+
+// mass and corrector (compute_forces_viscoacoustic_calling_routine.F90:183-205, update_veloc_acoustic_Newmark)
+__global__ __launch_bounds__(256) void k_ac_corrector(float* __restrict__ pd, float* __restrict__ pdd,
+                                                      const float* __restrict__ rm, int n, float dto2) {
+  const int stride = gridDim.x * blockDim.x;
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
+    const float a = pdd[k] * rm[k];
+    pdd[k] = a;
+    pd[k] = pd[k] + dto2 * a;
+  }
+}
+
+// enforce_acoustic_free_surface.f90:34-80
+__global__ __launch_bounds__(256) void k_ac_free_surface(const int* __restrict__ pts, int n, float* __restrict__ pot,
+                                                         float* __restrict__ pd, float* __restrict__ pdd) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int g = pts[k];
+  pot[g] = 0.f;
+  pd[g] = 0.f;
+  pdd[g] = 0.f;
+}
+
+// compute_stacey_acoustic.f90:34-160: pdd -= pd w / (rho vp) along the fluid's absorbing edges
+__global__ __launch_bounds__(256) void k_ac_stacey(int n, const int* __restrict__ pts, const float* __restrict__ w,
+                                                   const float* __restrict__ rvp, const float* __restrict__ pd,
+                                                   float* __restrict__ pdd) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int g = pts[k];
+  atomic_add(&pdd[g], -(pd[g] * w[k] / rvp[k]));
+}
+
+// fluid side of a fluid-solid edge: pdd += w (u . n) (compute_coupling_acoustic_el.f90:34-150)
+__global__ __launch_bounds__(256) void k_coupling_ac_el(int n, const int* __restrict__ pts,
+                                                        const float2* __restrict__ nrm, const float* __restrict__ w,
+                                                        const float2* __restrict__ displ, float* __restrict__ pdd) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int g = pts[k];
+  const float2 u = displ[g], nv = nrm[k];
+  const float dn = u.x * nv.x + u.y * nv.y;
+  atomic_add(&pdd[g], w[k] * dn);
+}
+
+// solid side: accel += w n p with the fluid pressure p = -pdd (compute_coupling_viscoelastic_ac.f90:34-130)
+__global__ __launch_bounds__(256) void k_coupling_el_ac(int n, const int* __restrict__ pts,
+                                                        const float2* __restrict__ nrm, const float* __restrict__ w,
+                                                        const float* __restrict__ pdd, float* __restrict__ accel) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int g = pts[k];
+  const float pr = -pdd[g];
+  const float2 nv = nrm[k];
+  const float wk = w[k];
+  atomic_add(&accel[2 * (size_t)g], wk * nv.x * pr);
+  atomic_add(&accel[2 * (size_t)g + 1], wk * nv.y * pr);
+}
+
+// pressure sources in fluid elements: pdd += h stf / kappa (compute_add_sources_acoustic.f90:34-75)
+__global__ __launch_bounds__(64) void k_ac_sources(int nas, int it, int nsrc, const int* __restrict__ src,
+                                                   const int* __restrict__ pts, const float* __restrict__ sa,
+                                                   const float* __restrict__ kap, const float* __restrict__ stf,
+                                                   float* __restrict__ pdd) {
+  const int s = blockIdx.x, p = threadIdx.x;
+  if (s >= nas || p >= NGLL2) return;
+  const size_t l = (size_t)s * NGLL2 + p;
+  const float f = stf[(size_t)it * nsrc + src[s]];
+  atomic_add(&pdd[pts[l]], sa[l] * f / kap[l]);
+}
+
+// pressure seismograms of the receivers in fluid elements: -field interpolated with the receivers' Lagrange
+// weights, accumulated in double (write_seismograms.F90:124-135, compute_pressure.f90:460-490,
+// compute_interpolated_dva.f90); receivers in solid elements (pts = -1) record 0
+__global__ __launch_bounds__(64) void k_ac_seismograms(int nrec, int nlength, int cur, const int* __restrict__ pts,
+                                                       const float* __restrict__ xir, const float* __restrict__ gammar,
+                                                       const float* __restrict__ field, float* __restrict__ dst) {
+  const int r = blockIdx.x, p = threadIdx.x;
+  if (r >= nrec) return;
+  double v = 0.0;
+  if (p < NGLL2) {
+    const int g = pts[(size_t)r * NGLL2 + p];
+    if (g >= 0) {
+      const int i = p % NGLL, j = p / NGLL;
+      const double h = (double)(xir[i * nrec + r] * gammar[j * nrec + r]);
+      v = (double)(-field[g]) * h;
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  if (p == 0) {
+    dst[(size_t)r * nlength + cur] = (float)v;
+    dst[((size_t)nrec + r) * nlength + cur] = 0.f;
+  }
+}
+
+// max |field| as the bit pattern of a non-negative float (ordered like the value)
+__global__ __launch_bounds__(256) void k_ac_max_abs(const float* __restrict__ f, int n, unsigned int* out) {
+  float m = 0.f;
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) m = fmaxf(m, fabsf(f[k]));
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+  if ((threadIdx.x & 63) == 0) atomicMax(out, __float_as_uint(m));
+}
+
+int grid_for(long n, int block, int cap) { return (int)std::max<long>(1, std::min<long>((n + block - 1) / block, cap)); }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// state
+struct AcousticState {
+  int ready = 0;
+  // recorded at preparation (caller numbering, 0-based elements and (i, j))
+  std::vector<int> phase_elems[2];        // fluid elements of each phase list
+  std::vector<float> rho_rows, kappa_rows;  // [fluid element][25], fluid elements in ispec order
+  std::vector<int> row_of_ispec;          // fluid row of each element, -1 for solid elements
+  std::vector<float> h_rm;                // [nglob] inverse fluid mass, caller numbering
+  std::vector<int> free_e, free_ij;       // free-surface points: element, i + 5 j
+  std::vector<int> coup_e, coup_ij;       // fluid-solid points: fluid element, i + 5 j
+  std::vector<float> coup_n, coup_w;      // [2] normal out of the fluid, weight
+  std::vector<int> st_e, st_ij;           // fluid Stacey points
+  std::vector<float> st_w, st_rvp;
+  // device
+  float *d_pot = nullptr, *d_pd = nullptr, *d_pdd = nullptr, *d_rm = nullptr;
+  int ncolour[2] = {0, 0};
+  int colour_start[2][Mesh::MAX_COLOURS] = {};
+  int colour_count[2][Mesh::MAX_COLOURS] = {};
+  int nelem = 0;
+  int* d_ids = nullptr;      // [nelem][25] point ids, colour order
+  float4* d_geom = nullptr;  // [nelem][25]
+  float* d_fac = nullptr;    // [nelem][25] J / rho
+  int nfree = 0;
+  int* d_free = nullptr;
+  int ncoup = 0;
+  int* d_coup = nullptr;
+  float2* d_coup_n = nullptr;
+  float* d_coup_w = nullptr;
+  int nst = 0;
+  int* d_st = nullptr;
+  float *d_st_w = nullptr, *d_st_rvp = nullptr;
+  int nas = 0;               // sources in fluid elements
+  int *d_src = nullptr, *d_src_pts = nullptr;
+  float *d_src_sa = nullptr, *d_src_kap = nullptr;
+  int nrec_fluid = 0;
+  int* d_rec_pts = nullptr;  // [nrec][25], -1 rows for receivers in solid elements
+};
+
+void acoustic_release(Mesh* mp) {
+  AcousticState* ac = mp->ac;
+  if (!ac) return;
+  device_free(ac->d_pot); device_free(ac->d_pd); device_free(ac->d_pdd); device_free(ac->d_rm);
+  device_free(ac->d_ids); device_free(ac->d_geom); device_free(ac->d_fac); device_free(ac->d_free);
+  device_free(ac->d_coup); device_free(ac->d_coup_n); device_free(ac->d_coup_w);
+  device_free(ac->d_st); device_free(ac->d_st_w); device_free(ac->d_st_rvp);
+  device_free(ac->d_src); device_free(ac->d_src_pts); device_free(ac->d_src_sa); device_free(ac->d_src_kap);
+  device_free(ac->d_rec_pts);
+  delete ac;
+  mp->ac = nullptr;
+}
+
+// the fluid's Stacey points, recorded by prepare_stacey_device_ (ispec 0-based caller order; ij 0-based with -1 for
+// points the edge does not own; rho_vp caller order [nspec][25])
+void acoustic_note_stacey(Mesh* mp, int nf, const int* ispec, const int* ij, const float* jacw, const float* rho_vp) {
+  AcousticState* ac = mp->ac;
+  if (!ac) fail("prepare_stacey_device: fluid elements without prepare_fields_acoustic_device");
+  for (int f = 0; f < nf; f++) {
+    const int e = ispec[f];
+    if (ac->row_of_ispec[e] < 0) continue;
+    for (int k = 0; k < NGLL; k++) {
+      const int i = ij[((size_t)f * NGLL + k) * 2], j = ij[((size_t)f * NGLL + k) * 2 + 1];
+      if (i < 0 || j < 0) continue;
+      ac->st_e.push_back(e);
+      ac->st_ij.push_back(i + NGLL * j);
+      ac->st_w.push_back(jacw[(size_t)f * NGLL + k]);
+      ac->st_rvp.push_back(rho_vp[(size_t)e * NGLL2 + j * NGLL + i]);
+    }
+  }
+}
+
+namespace {
+
+// device point ids of (element, i + 5 j) pairs, from the handle's current ibool
+struct IboolView {
+  std::vector<int> ib;  // device element order, device ids
+  const Mesh* mp;
+  explicit IboolView(const Mesh* m) : mp(m) {
+    ib.resize((size_t)m->nspec * NGLL2);
+    check_hip(hipMemcpy(ib.data(), m->d_ibool, ib.size() * sizeof(int), hipMemcpyDeviceToHost), "ibool D2H");
+  }
+  int row(int e) const { return mp->sorted ? mp->pos_of_ispec[e] : e; }
+  int id(int e, int p) const { return ib[(size_t)row(e) * NGLL2 + p]; }
+};
+
+std::vector<float> download_rows(const Mesh* mp, const float* d, const std::vector<int>& elems) {
+  std::vector<float> all((size_t)mp->nspec * NGLL2), out(elems.size() * NGLL2);
+  check_hip(hipMemcpy(all.data(), d, all.size() * sizeof(float), hipMemcpyDeviceToHost), "geometry D2H");
+  for (size_t q = 0; q < elems.size(); q++) {
+    const int r = mp->sorted ? mp->pos_of_ispec[elems[q]] : elems[q];
+    std::copy(all.begin() + (size_t)r * NGLL2, all.begin() + (size_t)(r + 1) * NGLL2, out.begin() + q * NGLL2);
+  }
+  return out;
+}
+
+// resolves the recorded set-up into device arrays (first call of the time loop)
+AcousticState* ac_ready(Mesh* mp, const char* who) {
+  AcousticState* ac = mp->ac;
+  if (!ac) fail("%s: prepare_fields_acoustic_device was not called", who);
+  if (ac->ready) return ac;
+  if (mp->nspec_elastic > 0 && !mp->sorted)
+    fail("%s: prepare_fields_elastic_device must run before the first acoustic step of a coupled mesh", who);
+  IboolView iv(mp);
+  // colour the fluid elements of each phase: greedy, first free colour over the points of the element
+  std::vector<int> order;
+  std::vector<unsigned int> used((size_t)mp->nglob, 0u);
+  for (int ph = 0; ph < 2; ph++) {
+    const std::vector<int>& el = ac->phase_elems[ph];
+    std::vector<int> col(el.size(), 0);
+    int ncol = 0;
+    std::fill(used.begin(), used.end(), 0u);
+    for (size_t q = 0; q < el.size(); q++) {
+      unsigned int forbidden = 0;
+      for (int p = 0; p < NGLL2; p++) forbidden |= used[iv.id(el[q], p)];
+      if (forbidden == 0xffffffffu) fail("%s: fluid elements need more than 32 colours", who);
+      int c = 0;
+      while (forbidden & (1u << c)) c++;
+      col[q] = c;
+      ncol = std::max(ncol, c + 1);
+      for (int p = 0; p < NGLL2; p++) used[iv.id(el[q], p)] |= 1u << c;
+    }
+    ac->ncolour[ph] = ncol;
+    for (int c = 0; c < ncol; c++) {
+      ac->colour_start[ph][c] = (int)order.size();
+      for (size_t q = 0; q < el.size(); q++)
+        if (col[q] == c) order.push_back(el[q]);
+      ac->colour_count[ph][c] = (int)order.size() - ac->colour_start[ph][c];
+    }
+  }
+  ac->nelem = (int)order.size();
+  {
+    const size_t n = order.size() * NGLL2;
+    std::vector<float> xix = download_rows(mp, mp->d_xix, order), xiz = download_rows(mp, mp->d_xiz, order);
+    std::vector<float> gax = download_rows(mp, mp->d_gammax, order), gaz = download_rows(mp, mp->d_gammaz, order);
+    std::vector<float4> geom(n);
+    std::vector<float> fac(n);
+    std::vector<int> ids(n);
+    for (size_t q = 0; q < order.size(); q++) {
+      const int e = order[q];
+      const float* rho = ac->rho_rows.data() + (size_t)ac->row_of_ispec[e] * NGLL2;
+      for (int p = 0; p < NGLL2; p++) {
+        const size_t l = q * NGLL2 + p;
+        geom[l] = make_float4(xix[l], xiz[l], gax[l], gaz[l]);
+        // the jacobian from the inverse map, as the device solver forms it (no jacobian crosses the C-ABI)
+        const float jac = 1.f / (xix[l] * gaz[l] - gax[l] * xiz[l]);
+        fac[l] = jac / rho[p];
+        ids[l] = iv.id(e, p);
+      }
+    }
+    ac->d_ids = device_copy(ids.data(), n);
+    ac->d_geom = device_copy(geom.data(), n);
+    ac->d_fac = device_copy(fac.data(), n);
+  }
+  ac->d_rm = device_zeros<float>((size_t)mp->nglob);
+  points_to_device(mp, ac->d_rm, ac->h_rm.data(), 1, "rmass_acoustic H2D");
+  {
+    std::vector<int> pts;
+    for (size_t k = 0; k < ac->free_e.size(); k++) pts.push_back(iv.id(ac->free_e[k], ac->free_ij[k]));
+    std::sort(pts.begin(), pts.end());
+    pts.erase(std::unique(pts.begin(), pts.end()), pts.end());
+    ac->nfree = (int)pts.size();
+    if (ac->nfree) ac->d_free = device_copy(pts.data(), pts.size());
+  }
+  {
+    std::vector<int> pts;
+    for (size_t k = 0; k < ac->coup_e.size(); k++) pts.push_back(iv.id(ac->coup_e[k], ac->coup_ij[k]));
+    ac->ncoup = (int)pts.size();
+    if (ac->ncoup) {
+      ac->d_coup = device_copy(pts.data(), pts.size());
+      ac->d_coup_n = reinterpret_cast<float2*>(device_copy(ac->coup_n.data(), ac->coup_n.size()));
+      ac->d_coup_w = device_copy(ac->coup_w.data(), ac->coup_w.size());
+    }
+  }
+  {
+    std::vector<int> pts;
+    for (size_t k = 0; k < ac->st_e.size(); k++) pts.push_back(iv.id(ac->st_e[k], ac->st_ij[k]));
+    ac->nst = (int)pts.size();
+    if (ac->nst) {
+      ac->d_st = device_copy(pts.data(), pts.size());
+      ac->d_st_w = device_copy(ac->st_w.data(), ac->st_w.size());
+      ac->d_st_rvp = device_copy(ac->st_rvp.data(), ac->st_rvp.size());
+    }
+  }
+  if (mp->nsources_local > 0) {
+    std::vector<float> sa((size_t)mp->nsources_local * NGLL2 * 2);
+    check_hip(hipMemcpy(sa.data(), mp->d_sourcearrays, sa.size() * sizeof(float), hipMemcpyDeviceToHost), "D2H");
+    std::vector<int> src, pts;
+    std::vector<float> a, kap;
+    for (int s = 0; s < mp->nsources_local; s++) {
+      const int e = mp->h_src_ispec[s];
+      const int row = ac->row_of_ispec[e];
+      if (row < 0) continue;
+      src.push_back(s);
+      for (int p = 0; p < NGLL2; p++) {
+        pts.push_back(iv.id(e, p));
+        a.push_back(sa[((size_t)s * NGLL2 + p) * 2]);
+        kap.push_back(ac->kappa_rows[(size_t)row * NGLL2 + p]);
+      }
+    }
+    ac->nas = (int)src.size();
+    if (ac->nas) {
+      ac->d_src = device_copy(src.data(), src.size());
+      ac->d_src_pts = device_copy(pts.data(), pts.size());
+      ac->d_src_sa = device_copy(a.data(), a.size());
+      ac->d_src_kap = device_copy(kap.data(), kap.size());
+    }
+  }
+  if (mp->nrec_local > 0) {
+    std::vector<int> pts((size_t)mp->nrec_local * NGLL2, -1);
+    for (int r = 0; r < mp->nrec_local; r++) {
+      const int e = mp->h_rec_ispec[r];
+      if (ac->row_of_ispec[e] < 0) continue;
+      ac->nrec_fluid++;
+      for (int p = 0; p < NGLL2; p++) pts[(size_t)r * NGLL2 + p] = iv.id(e, p);
+    }
+    ac->d_rec_pts = device_copy(pts.data(), pts.size());
+  }
+  ac->ready = 1;
+  return ac;
+}
+
+void require_forward(int cw2, int fwd_or_adj, const char* who) {
+  if (cw2) fail("%s: the backward fluid wavefield is not supported by this library", who);
+  if (fwd_or_adj != 1) fail("%s: adjoint fluid wavefields are not supported by this library", who);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// launchers used by other translation units
+void launch_acoustic_pressure_seismograms(Mesh* mp, float* dst, int cur, int use_trick) {
+  AcousticState* ac = ac_ready(mp, "compute_seismograms_cuda");
+  if (mp->nrec_local == 0) return;
+  if (ac->nrec_fluid != mp->nrec_local)
+    fail("compute_seismograms_cuda: pressure of receivers in solid elements is not supported by this library");
+  hipLaunchKernelGGL(k_ac_seismograms, dim3(mp->nrec_local), dim3(64), 0, mp->stream, mp->nrec_local,
+                     mp->nlength_seismogram, cur, ac->d_rec_pts, mp->d_xir, mp->d_gammar,
+                     use_trick ? ac->d_pot : ac->d_pdd, dst);
+  check_hip(hipGetLastError(), "k_ac_seismograms launch");
+}
+
+// receivers in fluid elements record pressure only
+int acoustic_receivers(Mesh* mp) { return mp->ac ? ac_ready(mp, "compute_seismograms_cuda")->nrec_fluid : 0; }
+
+}  // namespace sf2d
+
+using namespace sf2d;
+
+extern "C" {
+
+void prepare_fields_acoustic_device_(long* Mesh_pointer, realw* rmass_acoustic, realw* rhostore, realw* kappastore,
+                                     int* num_phase_ispec_acoustic, int* phase_ispec_inner_acoustic,
+                                     int* num_free_surface_faces, int* free_surface_ispec, int* free_surface_ijk,
+                                     int* ELASTIC_SIMULATION, int* num_coupling_ac_el_faces, int* coupling_ac_el_ispec,
+                                     int* coupling_ac_el_ijk, realw* coupling_ac_el_normal,
+                                     realw* coupling_ac_el_jacobian2Dw, int* h_ninterface_acoustic,
+                                     int* h_inum_interfaces_acoustic, int* ATTENUATION_VISCOACOUSTIC,
+                                     realw* h_A_newmark, realw* h_B_newmark, int* NO_BACKWARD_RECONSTRUCTION,
+                                     realw* h_no_backward_acoustic_buffer) {  // prepare_mesh_constants_cuda.cu:319
+  (void)h_inum_interfaces_acoustic; (void)h_A_newmark; (void)h_B_newmark; (void)NO_BACKWARD_RECONSTRUCTION;
+  (void)h_no_backward_acoustic_buffer;
+  Mesh* mp = handle(Mesh_pointer);
+  if (*ATTENUATION_VISCOACOUSTIC) fail("viscoacoustic attenuation is not supported by this library yet");
+  if (*h_ninterface_acoustic > 0) fail("MPI acoustic interfaces passed: one GPU, one partition only");
+  if (mp->ac) fail("prepare_fields_acoustic_device called twice");
+  if (mp->h_is_acoustic.empty()) fail("prepare_fields_acoustic_device: no fluid element (ispec_is_acoustic)");
+  AcousticState* ac = new AcousticState();
+  mp->ac = ac;
+  const int nglob = mp->nglob, nspec = mp->nspec;
+  ac->d_pot = device_zeros<float>((size_t)nglob);
+  ac->d_pd = device_zeros<float>((size_t)nglob);
+  ac->d_pdd = device_zeros<float>((size_t)nglob);
+  ac->h_rm.assign(rmass_acoustic, rmass_acoustic + nglob);
+  ac->row_of_ispec.assign(nspec, -1);
+  int nrows = 0;
+  for (int e = 0; e < nspec; e++)
+    if (mp->h_is_acoustic[e]) ac->row_of_ispec[e] = nrows++;
+  ac->rho_rows.resize((size_t)nrows * NGLL2);
+  ac->kappa_rows.resize((size_t)nrows * NGLL2);
+  for (int e = 0; e < nspec; e++) {
+    const int r = ac->row_of_ispec[e];
+    if (r < 0) continue;
+    std::memcpy(&ac->rho_rows[(size_t)r * NGLL2], rhostore + (size_t)e * NGLL2, NGLL2 * sizeof(float));
+    std::memcpy(&ac->kappa_rows[(size_t)r * NGLL2], kappastore + (size_t)e * NGLL2, NGLL2 * sizeof(float));
+  }
+  // phase lists (Fortran (num_phase, 2)): outer then inner
+  const int np = *num_phase_ispec_acoustic;
+  for (int ph = 0; ph < 2; ph++)
+    for (int q = 0; q < np; q++) {
+      const int e1 = phase_ispec_inner_acoustic[(size_t)ph * np + q];
+      if (e1 <= 0) continue;
+      if (e1 > nspec) fail("phase_ispec_inner_acoustic: element %d outside 1..%d", e1, nspec);
+      if (ac->row_of_ispec[e1 - 1] < 0) fail("phase_ispec_inner_acoustic: element %d is not acoustic", e1);
+      ac->phase_elems[ph].push_back(e1 - 1);
+    }
+  if (!ac->phase_elems[0].empty()) fail("outer acoustic elements passed: one GPU, one partition only");
+  auto ij_of = [&](const int* ijk, size_t k, const char* what) {
+    const int i = ijk[2 * k], j = ijk[2 * k + 1];
+    if (i < 1 || i > NGLL || j < 1 || j > NGLL) fail("%s: GLL index (%d, %d) out of range", what, i, j);
+    return (i - 1) + NGLL * (j - 1);
+  };
+  for (int f = 0; f < *num_free_surface_faces; f++) {
+    const int e1 = free_surface_ispec[f];
+    if (e1 < 1 || e1 > nspec) fail("free_surface_ispec: element %d outside 1..%d", e1, nspec);
+    for (int k = 0; k < NGLL; k++) {
+      ac->free_e.push_back(e1 - 1);
+      ac->free_ij.push_back(ij_of(free_surface_ijk, (size_t)f * NGLL + k, "free_surface_ij"));
+    }
+  }
+  if (*ELASTIC_SIMULATION)
+    for (int f = 0; f < *num_coupling_ac_el_faces; f++) {
+      const int e1 = coupling_ac_el_ispec[f];
+      if (e1 < 1 || e1 > nspec || ac->row_of_ispec[e1 - 1] < 0)
+        fail("coupling_ac_el_ispec: element %d is not a fluid element", e1);
+      for (int k = 0; k < NGLL; k++) {
+        const size_t q = (size_t)f * NGLL + k;
+        ac->coup_e.push_back(e1 - 1);
+        ac->coup_ij.push_back(ij_of(coupling_ac_el_ijk, q, "coupling_ac_el_ij"));
+        ac->coup_n.push_back(coupling_ac_el_normal[2 * q]);
+        ac->coup_n.push_back(coupling_ac_el_normal[2 * q + 1]);
+        ac->coup_w.push_back(coupling_ac_el_jacobian2Dw[q]);
+      }
+    }
+  check_hip(hipDeviceSynchronize(), "prepare_fields_acoustic_device");
+}
+
+void prepare_fields_acoustic_adj_dev_(long* Mesh_pointer, int* APPROXIMATE_HESS_KL, int* ATTENUATION_VISCOACOUSTIC,
+                                      int* NO_BACKWARD_RECONSTRUCTION) {  // prepare_mesh_constants_cuda.cu:442
+  (void)APPROXIMATE_HESS_KL; (void)ATTENUATION_VISCOACOUSTIC; (void)NO_BACKWARD_RECONSTRUCTION;
+  Mesh* mp = handle(Mesh_pointer);
+  if (mp->simulation_type == 3) fail("adjoint simulations of fluid elements are not supported by this library");
+}
+
+void update_displacement_ac_cuda_(long* Mesh_pointer, realw* deltat_F, realw* deltatsqover2_F, realw* deltatover2_F,
+                                  realw* b_deltat_F, realw* b_deltatsqover2_F, realw* b_deltatover2_F,
+                                  int* compute_b_wavefield, int* UNDO_ATTENUATION_AND_OR_PML) {  // update_displacement_cuda.cu:118
+  (void)b_deltat_F; (void)b_deltatsqover2_F; (void)b_deltatover2_F; (void)UNDO_ATTENUATION_AND_OR_PML;
+  Mesh* mp = handle(Mesh_pointer);
+  if (*compute_b_wavefield) fail("update_displacement_ac_cuda: the backward fluid wavefield is not supported");
+  AcousticState* ac = ac_ready(mp, "update_displacement_ac_cuda");
+  const int n = mp->nglob;
+  hipLaunchKernelGGL(k_ac_predictor, dim3(grid_for(n, 256, 65536)), dim3(256), 0, mp->stream, ac->d_pot, ac->d_pd,
+                     ac->d_pdd, n, *deltat_F, *deltatsqover2_F, *deltatover2_F);
+  check_hip(hipGetLastError(), "k_ac_predictor launch");
+}
+
+void compute_forces_acoustic_cuda_(long* Mesh_pointer, int* iphase, int* nspec_outer_acoustic,
+                                   int* nspec_inner_acoustic, int* ATTENUATION_VISCOACOUSTIC, int* compute_wavefield_1,
+                                   int* compute_wavefield_2) {  // compute_forces_acoustic_cuda.cu:248
+  (void)nspec_outer_acoustic; (void)nspec_inner_acoustic;
+  Mesh* mp = handle(Mesh_pointer);
+  if (*ATTENUATION_VISCOACOUSTIC) fail("viscoacoustic attenuation is not supported by this library yet");
+  require_forward(*compute_wavefield_2, 1, "compute_forces_acoustic_cuda");
+  if (!*compute_wavefield_1) return;
+  AcousticState* ac = ac_ready(mp, "compute_forces_acoustic_cuda");
+  const int ph = *iphase - 1;
+  if (ph < 0 || ph > 1) fail("compute_forces_acoustic_cuda: iphase %d", *iphase);
+  for (int c = 0; c < ac->ncolour[ph]; c++) {
+    const int n = ac->colour_count[ph][c];
+    if (n == 0) continue;
+    hipLaunchKernelGGL(k_acoustic_forces, dim3((n + AC_EPB - 1) / AC_EPB), dim3(256), 0, mp->stream,
+                       ac->colour_start[ph][c], n, ac->d_ids, ac->d_geom, ac->d_fac, mp->basis, ac->d_pot, ac->d_pdd);
+    check_hip(hipGetLastError(), "k_acoustic_forces launch");
+  }
+}
+
+void compute_stacey_acoustic_cuda_(long* Mesh_pointer, int* iphasef, realw* h_b_absorb_potential_left,
+                                   realw* h_b_absorb_potential_right, realw* h_b_absorb_potential_top,
+                                   realw* h_b_absorb_potential_bottom, int* compute_wavefield_1,
+                                   int* compute_wavefield_2, int* UNDO_ATTENUATION_AND_OR_PML) {  // compute_stacey_acoustic_cuda.cu:40
+  (void)h_b_absorb_potential_left; (void)h_b_absorb_potential_right; (void)h_b_absorb_potential_top;
+  (void)h_b_absorb_potential_bottom; (void)UNDO_ATTENUATION_AND_OR_PML;
+  Mesh* mp = handle(Mesh_pointer);
+  if (*iphasef != 1) return;
+  require_forward(*compute_wavefield_2, 1, "compute_stacey_acoustic_cuda");
+  if (mp->save_forward || mp->simulation_type != 1)
+    fail("compute_stacey_acoustic_cuda: storage of the fluid's Stacey terms (SAVE_FORWARD) is not supported");
+  if (!*compute_wavefield_1) return;
+  AcousticState* ac = ac_ready(mp, "compute_stacey_acoustic_cuda");
+  if (ac->nst == 0) return;
+  hipLaunchKernelGGL(k_ac_stacey, dim3((ac->nst + 255) / 256), dim3(256), 0, mp->stream, ac->nst, ac->d_st,
+                     ac->d_st_w, ac->d_st_rvp, ac->d_pd, ac->d_pdd);
+  check_hip(hipGetLastError(), "k_ac_stacey launch");
+}
+
+void acoustic_enforce_free_surf_cuda_(long* Mesh_pointer, int* compute_wavefield_1,
+                                      int* compute_wavefield_2) {  // enforce_acoustic_free_surface_cuda.cu:46
+  Mesh* mp = handle(Mesh_pointer);
+  require_forward(*compute_wavefield_2, 1, "acoustic_enforce_free_surf_cuda");
+  if (!*compute_wavefield_1) return;
+  AcousticState* ac = ac_ready(mp, "acoustic_enforce_free_surf_cuda");
+  if (ac->nfree == 0) return;
+  hipLaunchKernelGGL(k_ac_free_surface, dim3((ac->nfree + 255) / 256), dim3(256), 0, mp->stream, ac->d_free,
+                     ac->nfree, ac->d_pot, ac->d_pd, ac->d_pdd);
+  check_hip(hipGetLastError(), "k_ac_free_surface launch");
+}
+
+void compute_add_sources_ac_cuda_(long* Mesh_pointer, int* iphasef, int* itf) {  // compute_add_sources_viscoacoustic_cuda.cu:46
+  Mesh* mp = handle(Mesh_pointer);
+  if (*iphasef != 1) return;
+  AcousticState* ac = ac_ready(mp, "compute_add_sources_ac_cuda");
+  if (ac->nas == 0) return;
+  const int it = *itf - 1;
+  if (it < 0 || it >= mp->nstep) fail("compute_add_sources_ac_cuda: it %d outside 1..%d", *itf, mp->nstep);
+  hipLaunchKernelGGL(k_ac_sources, dim3(ac->nas), dim3(64), 0, mp->stream, ac->nas, it, mp->nsources_local,
+                     ac->d_src, ac->d_src_pts, ac->d_src_sa, ac->d_src_kap, mp->d_stf, ac->d_pdd);
+  check_hip(hipGetLastError(), "k_ac_sources launch");
+}
+
+void compute_coupling_ac_el_cuda_(long* Mesh_pointer, int* iphasef, int* num_coupling_ac_el_facesf,
+                                  int* FORWARD_OR_ADJOINT) {  // compute_coupling_cuda.cu:47
+  Mesh* mp = handle(Mesh_pointer);
+  if (*iphasef != 1) return;
+  require_forward(0, *FORWARD_OR_ADJOINT, "compute_coupling_ac_el_cuda");
+  AcousticState* ac = ac_ready(mp, "compute_coupling_ac_el_cuda");
+  if (*num_coupling_ac_el_facesf * NGLL != ac->ncoup)
+    fail("compute_coupling_ac_el_cuda: %d faces, %d prepared points", *num_coupling_ac_el_facesf, ac->ncoup);
+  if (ac->ncoup == 0) return;
+  if (!mp->d_displ) fail("compute_coupling_ac_el_cuda: the solid fields are not prepared");
+  hipLaunchKernelGGL(k_coupling_ac_el, dim3((ac->ncoup + 255) / 256), dim3(256), 0, mp->stream, ac->ncoup,
+                     ac->d_coup, ac->d_coup_n, ac->d_coup_w, reinterpret_cast<const float2*>(mp->d_displ), ac->d_pdd);
+  check_hip(hipGetLastError(), "k_coupling_ac_el launch");
+}
+
+void compute_coupling_el_ac_cuda_(long* Mesh_pointer, int* iphasef, int* num_coupling_ac_el_facesf,
+                                  int* FORWARD_OR_ADJOINT) {  // compute_coupling_cuda.cu:125
+  Mesh* mp = handle(Mesh_pointer);
+  if (*iphasef != 1) return;
+  require_forward(0, *FORWARD_OR_ADJOINT, "compute_coupling_el_ac_cuda");
+  AcousticState* ac = ac_ready(mp, "compute_coupling_el_ac_cuda");
+  if (*num_coupling_ac_el_facesf * NGLL != ac->ncoup)
+    fail("compute_coupling_el_ac_cuda: %d faces, %d prepared points", *num_coupling_ac_el_facesf, ac->ncoup);
+  if (ac->ncoup == 0) return;
+  if (mp->first_touch) fail("compute_coupling_el_ac_cuda: first-touch assembly is on for a coupled mesh");
+  hipLaunchKernelGGL(k_coupling_el_ac, dim3((ac->ncoup + 255) / 256), dim3(256), 0, mp->stream, ac->ncoup,
+                     ac->d_coup, ac->d_coup_n, ac->d_coup_w, ac->d_pdd, mp->d_accel);
+  check_hip(hipGetLastError(), "k_coupling_el_ac launch");
+}
+
+void kernel_3_acoustic_cuda_(long* Mesh_pointer, realw* deltatover2, realw* b_deltatover2, int* compute_wavefield_1,
+                             int* compute_wavefield_2) {  // update_displacement_cuda.cu:243
+  (void)b_deltatover2;
+  Mesh* mp = handle(Mesh_pointer);
+  require_forward(*compute_wavefield_2, 1, "kernel_3_acoustic_cuda");
+  if (!*compute_wavefield_1) return;
+  AcousticState* ac = ac_ready(mp, "kernel_3_acoustic_cuda");
+  const int n = mp->nglob;
+  hipLaunchKernelGGL(k_ac_corrector, dim3(grid_for(n, 256, 65536)), dim3(256), 0, mp->stream, ac->d_pd, ac->d_pdd,
+                     ac->d_rm, n, *deltatover2);
+  check_hip(hipGetLastError(), "k_ac_corrector launch");
+}
+
+void transfer_fields_ac_to_device_(int* size, realw* potential_acoustic, realw* potential_dot_acoustic,
+                                   realw* potential_dot_dot_acoustic, long* Mesh_pointer) {  // transfer_fields_cuda.cu:229
+  Mesh* mp = handle(Mesh_pointer);
+  if (*size != mp->nglob) fail("transfer_fields_ac_to_device: size %d != nglob %d", *size, mp->nglob);
+  AcousticState* ac = ac_ready(mp, "transfer_fields_ac_to_device");
+  points_to_device(mp, ac->d_pot, potential_acoustic, 1, "potential H2D");
+  points_to_device(mp, ac->d_pd, potential_dot_acoustic, 1, "potential_dot H2D");
+  points_to_device(mp, ac->d_pdd, potential_dot_dot_acoustic, 1, "potential_dot_dot H2D");
+}
+
+void transfer_fields_ac_from_device_(int* size, realw* potential_acoustic, realw* potential_dot_acoustic,
+                                     realw* potential_dot_dot_acoustic, long* Mesh_pointer) {  // transfer_fields_cuda.cu:281
+  Mesh* mp = handle(Mesh_pointer);
+  if (*size != mp->nglob) fail("transfer_fields_ac_from_device: size %d != nglob %d", *size, mp->nglob);
+  AcousticState* ac = ac_ready(mp, "transfer_fields_ac_from_device");
+  points_from_device(mp, potential_acoustic, ac->d_pot, 1, "potential D2H");
+  points_from_device(mp, potential_dot_acoustic, ac->d_pd, 1, "potential_dot D2H");
+  points_from_device(mp, potential_dot_dot_acoustic, ac->d_pdd, 1, "potential_dot_dot D2H");
+}
+
+void transfer_dot_dot_from_device_(int* size, realw* potential_dot_dot_acoustic, long* Mesh_pointer) {  // transfer_fields_cuda.cu:357
+  Mesh* mp = handle(Mesh_pointer);
+  if (*size != mp->nglob) fail("transfer_dot_dot_from_device: size %d != nglob %d", *size, mp->nglob);
+  AcousticState* ac = ac_ready(mp, "transfer_dot_dot_from_device");
+  points_from_device(mp, potential_dot_dot_acoustic, ac->d_pdd, 1, "potential_dot_dot D2H");
+}
+
+void get_norm_acoustic_from_device_(realw* norm, long* Mesh_pointer, const int* FORWARD_OR_ADJOINT) {  // check_fields_cuda.cu:240
+  Mesh* mp = handle(Mesh_pointer);
+  require_forward(0, *FORWARD_OR_ADJOINT, "get_norm_acoustic_from_device");
+  AcousticState* ac = ac_ready(mp, "get_norm_acoustic_from_device");
+  check_hip(hipMemsetAsync(mp->d_scratch_u32, 0, sizeof(unsigned int), mp->stream), "memset");
+  hipLaunchKernelGGL(k_ac_max_abs, dim3(grid_for(mp->nglob, 256, 2048)), dim3(256), 0, mp->stream, ac->d_pot,
+                     mp->nglob, mp->d_scratch_u32);
+  check_hip(hipGetLastError(), "k_ac_max_abs launch");
+  unsigned int bits = 0;
+  check_hip(hipMemcpyAsync(&bits, mp->d_scratch_u32, sizeof(bits), hipMemcpyDeviceToHost, mp->stream), "D2H");
+  check_hip(hipStreamSynchronize(mp->stream), "get_norm_acoustic_from_device");
+  float v;
+  std::memcpy(&v, &bits, sizeof(v));
+  *norm = v;
+}
+
+}  // extern "C"

@@ -347,6 +347,7 @@ void upload_chain_ids(Mesh* mp) {
 
 void sort_elements_by_colour(Mesh* mp, const int* phase_lists, int np) {
   const int nspec = mp->nspec;
+  if (mp->sorted) fail("prepare_fields_elastic_device called twice");
   // patch mode: chains of up to CHAIN_ELEMS elements when every patch is one (the chain kernel), otherwise
   // patches of up to PATCH_ELEMS for the general patch kernel
   const bool try_chain = mp->assembly == 0 && !getenv("SF2D_NO_CHAIN");
@@ -414,7 +415,9 @@ void sort_elements_by_colour(Mesh* mp, const int* phase_lists, int np) {
     if (mp->chain) {
       mp->h_chain_ids = std::move(ib);
       mp->h_first = std::move(P.first);
-      mp->first_touch = mp->simulation_type == 1 && !mp->h_first.empty() && !getenv("SF2D_NO_FIRST_TOUCH");
+      // coupled meshes keep the accumulating scheme: the fluid-solid terms add to the solid's accel
+      mp->first_touch = mp->simulation_type == 1 && !mp->h_first.empty() && mp->nspec_acoustic == 0 &&
+                        !getenv("SF2D_NO_FIRST_TOUCH");
       upload_chain_ids(mp);
     }
   }
@@ -613,6 +616,16 @@ void prepare_constants_device_(long* Mesh_pointer, int* h_NGLLX, int* NSPEC_AB, 
   mp->d_ispec_is_elastic = device_copy(elastic.data(), (size_t)mp->nspec);
   mp->nspec_elastic = *nspec_elastic;
   mp->nspec_acoustic = *nspec_acoustic;
+  if (mp->nspec_acoustic > 0) {
+    mp->h_is_acoustic.assign(mp->nspec, 0);
+    int n = 0;
+    for (int e = 0; e < mp->nspec; e++) {
+      mp->h_is_acoustic[e] = ispec_is_acoustic[e] ? 1 : 0;
+      n += mp->h_is_acoustic[e];
+      if (ispec_is_acoustic[e] && ispec_is_elastic[e]) fail("element %d is both acoustic and elastic", e + 1);
+    }
+    if (n != mp->nspec_acoustic) fail("nspec_acoustic %d != %d flagged elements", mp->nspec_acoustic, n);
+  }
 
   // sources
   mp->nsources_local = *nsources_local_f;
@@ -703,7 +716,6 @@ void prepare_stacey_device_(long* Mesh_pointer, int* ACOUSTIC_SIMULATION, int* E
                             int* h_ib_right, int* h_ib_top) {  // prepare_mesh_constants_cuda.cu:926
   Mesh* mp = handle(Mesh_pointer);
   if (!mp->stacey) fail("Stacey flag inconsistent with prepare_stacey_device call");
-  if (*ACOUSTIC_SIMULATION) fail("acoustic Stacey boundaries are not supported by this library yet");
   const int nf = *h_num_abs_boundary_faces;
   mp->num_abs = nf;
   mp->nspec_bottom = *h_nspec_bottom;
@@ -723,6 +735,11 @@ void prepare_stacey_device_(long* Mesh_pointer, int* ACOUSTIC_SIMULATION, int* E
   // after marking, a skipped point has -1 in one coordinate: make both negative
   for (int k = 0; k < nf * NGLL; k++)
     if (ij[2 * k] < 0 || ij[2 * k + 1] < 0) ij[2 * k] = ij[2 * k + 1] = -1;
+  if (*ACOUSTIC_SIMULATION && mp->nspec_acoustic > 0) {
+    // the fluid's points keep the caller's element numbering until the first acoustic step resolves them
+    std::vector<int> isp0 = to_zero_based(h_abs_boundary_ispec, nf, 1, mp->nspec, "abs_boundary_ispec");
+    acoustic_note_stacey(mp, nf, isp0.data(), ij.data(), h_abs_boundary_jacobian1Dw, rho_vp);
+  }
   std::vector<int> ib(nf);
   const int counts[5] = {0, mp->nspec_bottom, mp->nspec_right, mp->nspec_top, mp->nspec_left};
   for (int f = 0; f < nf; f++) {
@@ -777,6 +794,7 @@ void prepare_cleanup_device_(long* Mesh_pointer, int* ACOUSTIC_SIMULATION, int* 
   (void)h_no_backward_acoustic_buffer;
   Mesh* mp = handle(Mesh_pointer);
   (void)hipStreamSynchronize(mp->stream);
+  acoustic_release(mp);
   device_free(mp->d_ibool); device_free(mp->d_xix); device_free(mp->d_xiz); device_free(mp->d_gammax);
   device_free(mp->d_gammaz); device_free(mp->d_kappav); device_free(mp->d_muv); device_free(mp->d_ispec_is_elastic);
   device_free(mp->d_displ); device_free(mp->d_veloc); device_free(mp->d_accel);
@@ -1082,8 +1100,7 @@ void compute_seismograms_cuda_(long* Mesh_pointer_f, int* i_sigf, double* sisux,
                                int* ACOUSTIC_SIMULATION, int* USE_TRICK_FOR_BETTER_PRESSURE,
                                int* ATTENUATION_VISCOELASTIC, int* itf,
                                int* it_endf) {  // write_seismograms_cuda.cu:41
-  (void)ELASTIC_SIMULATION; (void)ACOUSTIC_SIMULATION; (void)USE_TRICK_FOR_BETTER_PRESSURE;
-  (void)ATTENUATION_VISCOELASTIC;
+  (void)ELASTIC_SIMULATION; (void)ACOUSTIC_SIMULATION; (void)ATTENUATION_VISCOELASTIC;
   Mesh* mp = handle(Mesh_pointer_f);
   if (mp->nrec_local == 0) return;
   const int i_sig = *i_sigf - 1;
@@ -1102,10 +1119,17 @@ void compute_seismograms_cuda_(long* Mesh_pointer_f, int* i_sigf, double* sisux,
       check_accel_complete(mp, "compute_seismograms_cuda");
       field = mp->d_accel;
       break;
-    default: fail("seismotype %d (pressure/curl) is not supported by this library yet", stype);
+    case 4: break;
+    default: fail("seismotype %d (curl/potential) is not supported by this library yet", stype);
   }
   float* d_dst = mp->d_seismo[i_sig];
-  launch_seismograms(mp, field, d_dst, cur);
+  if (stype == 4)
+    launch_acoustic_pressure_seismograms(mp, d_dst, cur, *USE_TRICK_FOR_BETTER_PRESSURE);
+  else if (acoustic_receivers(mp) > 0)
+    fail("compute_seismograms_cuda: displacement, velocity or acceleration of receivers in fluid elements is not "
+         "supported by this library (use seismotype 4)");
+  else
+    launch_seismograms(mp, field, d_dst, cur);
   if (cur == nlength - 1 || *itf == *it_endf) {
     const size_t size = (size_t)mp->nrec_local * nlength;
     float* h = mp->h_seismo[i_sig];

@@ -138,6 +138,7 @@ extern "C" {
 int sf2d_batch_prepare(long* Mesh_pointer, int nshot, const int* src_shot) {
   Mesh* mp = handle(Mesh_pointer);
   if (!mp->sorted) fail("sf2d_batch_prepare: prepare_fields_elastic_device was not called");
+  if (mp->nspec_acoustic > 0) fail("sf2d_batch_prepare: shot batches of meshes with fluid elements are not supported");
   if (nshot < 1 || nshot > 65535) fail("sf2d_batch_prepare: %d shots outside [1, 65535]", nshot);
   std::vector<int> owner((size_t)std::max(1, mp->nsources_local), 0);
   for (int k = 0; k < mp->nsources_local; k++) {

@@ -105,6 +105,15 @@ struct Sim {
   std::vector<float> xix, xiz, gammax, gammaz, jacobian, kappa, mu, rho_vp, rho_vs, rho;
   std::vector<float> rmass_inverse;  // [nglob][2]
   std::vector<float> rmassx, rmassz;
+  // fluid elements (vs = 0 at all 25 points) and their boundaries
+  std::vector<int> is_acoustic;      // [nspec]
+  int nspec_ac = 0;
+  std::vector<float> rmass_ac;       // [nglob] inverse fluid mass
+  std::vector<int> free_ispec, free_ij;            // acoustic free-surface edges: 1-based element, [nf][5][2] (i, j)
+  std::vector<int> coup_ac_ispec, coup_ac_code;    // fluid-solid edges: fluid element (1-based) and its side
+  std::vector<int> coup_el_ispec, coup_el_code;
+  std::vector<int> coup_ij;                        // [n][5][2] 1-based (i, j) in the fluid element
+  std::vector<float> coup_normal, coup_jacw;       // [n][5][2], [n][5]
   // Stacey (prepare_timerun.F90:1270-1460)
   int nabs = 0;
   int nspec_side[5] = {0, 0, 0, 0, 0};
@@ -257,6 +266,89 @@ void build_materials(Sim& s, const double* rho, const double* vp, const double* 
   }
 }
 
+// fluid elements, the acoustic free surface (every non-absorbing outer side of a fluid element,
+// src/meshfem2D/determine_acoustic_surface.f90:34-160) and the fluid-solid edges (fluid elements in order, then their
+// solid neighbours below, left, right, above: src/meshfem2D/repartition_coupling.f90:38-106), with the points,
+// normals and weights of src/specfem2D/prepare_gpu.f90:545-660 (ivalue order: get_coupling_edges.f90:93-111)
+void build_domains(Sim& s) {
+  const int nx = s.p.nx, nz = s.p.nz;
+  s.is_acoustic.assign(s.nspec, 0);
+  s.nspec_ac = 0;
+  for (int e = 0; e < s.nspec; e++) {
+    int nzero = 0;
+    for (int p = 0; p < NGLL2; p++) nzero += s.mu[(size_t)e * NGLL2 + p] == 0.f;
+    s.is_acoustic[e] = nzero == NGLL2;
+    s.nspec_ac += s.is_acoustic[e];
+  }
+  s.free_ispec.clear(); s.free_ij.clear();
+  s.coup_ac_ispec.clear(); s.coup_ac_code.clear(); s.coup_el_ispec.clear(); s.coup_el_code.clear();
+  s.coup_ij.clear(); s.coup_normal.clear(); s.coup_jacw.clear();
+  if (s.nspec_ac == 0) return;
+  // (i, j) of point k along side code in ivalue order (1 bottom, 2 right, 3 top, 4 left)
+  auto ivalue = [](int code, int k, int* i, int* j) {
+    switch (code) {
+      case 1: *i = k; *j = 0; break;
+      case 2: *i = NGLL - 1; *j = k; break;
+      case 3: *i = NGLL - 1 - k; *j = NGLL - 1; break;
+      default: *i = 0; *j = NGLL - 1 - k; break;
+    }
+  };
+  auto add_free = [&](int e, int code) {
+    if (!s.is_acoustic[e]) return;
+    s.free_ispec.push_back(e + 1);
+    for (int k = 0; k < NGLL; k++) {
+      int i, j;
+      ivalue(code, k, &i, &j);
+      s.free_ij.push_back(i + 1);
+      s.free_ij.push_back(j + 1);
+    }
+  };
+  if (!s.p.absorbtop) for (int i = 0; i < nx; i++) add_free((nz - 1) * nx + i, 3);
+  if (!s.p.absorbbottom) for (int i = 0; i < nx; i++) add_free(i, 1);
+  if (!s.p.absorbleft) for (int j = 0; j < nz; j++) add_free(j * nx, 4);
+  if (!s.p.absorbright) for (int j = 0; j < nz; j++) add_free(j * nx + nx - 1, 2);
+  for (int e = 0; e < s.nspec; e++) {
+    if (!s.is_acoustic[e]) continue;
+    const int ix = e % nx, iz = e / nx;
+    const int nb[4] = {iz > 0 ? e - nx : -1, ix > 0 ? e - 1 : -1, ix < nx - 1 ? e + 1 : -1, iz < nz - 1 ? e + nx : -1};
+    const int ac_code[4] = {1, 4, 2, 3}, el_code[4] = {3, 2, 4, 1};
+    for (int q = 0; q < 4; q++) {
+      if (nb[q] < 0 || s.is_acoustic[nb[q]]) continue;
+      const int code = ac_code[q];
+      s.coup_ac_ispec.push_back(e + 1);
+      s.coup_ac_code.push_back(code);
+      s.coup_el_ispec.push_back(nb[q] + 1);
+      s.coup_el_code.push_back(el_code[q]);
+      for (int k = 0; k < NGLL; k++) {
+        int i, j;
+        ivalue(code, k, &i, &j);
+        const size_t l = (size_t)e * NGLL2 + j * NGLL + i;
+        float nxv, nzv, j1, w;
+        if (code == 1 || code == 3) {
+          const float xx = +s.gammaz[l] * s.jacobian[l];
+          const float zx = -s.gammax[l] * s.jacobian[l];
+          j1 = std::sqrt(xx * xx + zx * zx);
+          nxv = (code == 3 ? -zx : zx) / j1;
+          nzv = (code == 3 ? xx : -xx) / j1;
+          w = j1 * s.basis.w[i];
+        } else {
+          const float xg = -s.xiz[l] * s.jacobian[l];
+          const float zg = +s.xix[l] * s.jacobian[l];
+          j1 = std::sqrt(xg * xg + zg * zg);
+          nxv = (code == 4 ? -zg : zg) / j1;
+          nzv = (code == 4 ? xg : -xg) / j1;
+          w = j1 * s.basis.w[j];
+        }
+        s.coup_ij.push_back(i + 1);
+        s.coup_ij.push_back(j + 1);
+        s.coup_normal.push_back(nxv);
+        s.coup_normal.push_back(nzv);
+        s.coup_jacw.push_back(w);
+      }
+    }
+  }
+}
+
 // absorbing edges in the mesher's order (determine_abs_surface.f90) and their GPU arrays
 void build_stacey(Sim& s) {
   const sf2d_params& p = s.p;
@@ -322,6 +414,14 @@ void build_stacey(Sim& s) {
       s.abs_normal[((size_t)f * NGLL + k2) * 2 + 1] = nz;
       int ii = i + 1, jj = j + 1;
       if ((c == 1 || c == 3) && ((cl && i == 0) || (cr && i == NGLL - 1))) ii = NGLL + 1;
+      // a fluid element's absorbing edge leaves out the point it shares with one of its fluid-solid edges
+      // (get_coupling_edges.f90:528-563, prepare_timerun.F90:1360-1451)
+      for (size_t q = 0; q < s.coup_ac_ispec.size(); q++) {
+        if (s.coup_ac_ispec[q] != el[f]) continue;
+        const int ce = s.coup_ac_code[q];
+        if ((c == 4 || c == 2) && ((ce == 1 && j == 0) || (ce == 3 && j == NGLL - 1))) jj = NGLL + 1;
+        if ((c == 1 || c == 3) && ((ce == 4 && i == 0) || (ce == 2 && i == NGLL - 1))) ii = NGLL + 1;
+      }
       s.abs_ij[((size_t)f * NGLL + k2) * 2] = ii;
       s.abs_ij[((size_t)f * NGLL + k2) * 2 + 1] = jj;
     }
@@ -331,13 +431,20 @@ void build_stacey(Sim& s) {
 // mass matrix with the Newmark Stacey term (invert_mass_matrix.F90:34-464, 470-792, 798-848)
 void build_mass(Sim& s) {
   std::vector<float>& rm = s.rmass_inverse;
+  std::vector<float>& rma = s.rmass_ac;
   rm.assign((size_t)2 * s.nglob, 0.f);
+  rma.assign(s.nspec_ac > 0 ? (size_t)s.nglob : 0, 0.f);
   for (int e = 0; e < s.nspec; e++)
     for (int j = 0; j < NGLL; j++)
       for (int i = 0; i < NGLL; i++) {
         const size_t l = (size_t)e * NGLL2 + j * NGLL + i;
         const int g = s.ibool[l] - 1;
         const float wxz = s.basis.w[i] * s.basis.w[j];
+        if (s.nspec_ac > 0 && s.is_acoustic[e]) {
+          // w_i w_j J in single precision over the double kappa (invert_mass_matrix.F90:424-427)
+          rma[g] = (float)((double)rma[g] + (double)(wxz * s.jacobian[l]) / (double)s.kappa[l]);
+          continue;
+        }
         rm[2 * g] = (float)((double)rm[2 * g] + (double)wxz * (double)s.rho[l] * (double)s.jacobian[l]);
         rm[2 * g + 1] = rm[2 * g];
       }
@@ -345,11 +452,16 @@ void build_mass(Sim& s) {
     const float dto2 = (float)(0.5 * (double)(float)s.p.dt);
     for (int f = 0; f < s.nabs; f++) {
       const int e = s.abs_ispec[f] - 1;
+      const bool fluid = s.nspec_ac > 0 && s.is_acoustic[e];
       for (int k = 0; k < NGLL; k++) {
         const int ii = s.abs_ij[((size_t)f * NGLL + k) * 2], jj = s.abs_ij[((size_t)f * NGLL + k) * 2 + 1];
         if (ii > NGLL || jj > NGLL) continue;
         const size_t l = (size_t)e * NGLL2 + (jj - 1) * NGLL + (ii - 1);
         const int g = s.ibool[l] - 1;
+        if (fluid) {  // dt/2 w / (rho vp) (invert_mass_matrix.F90:709-790)
+          rma[g] = rma[g] + dto2 * s.abs_jacw[(size_t)f * NGLL + k] / s.rho_vp[l];
+          continue;
+        }
         const float nx = s.abs_normal[((size_t)f * NGLL + k) * 2], nz = s.abs_normal[((size_t)f * NGLL + k) * 2 + 1];
         const float w = s.abs_jacw[(size_t)f * NGLL + k];
         const float vn = nx * dto2 + nz * dto2;
@@ -361,6 +473,10 @@ void build_mass(Sim& s) {
     }
   }
   for (auto& v : rm) {
+    if (v <= 0.f) v = 1.f;
+    v = 1.f / v;
+  }
+  for (auto& v : rma) {
     if (v <= 0.f) v = 1.f;
     v = 1.f / v;
   }
@@ -470,7 +586,8 @@ double build_source_set(Sim& s, int k0, int n) {
   std::vector<double> t0s(n), tsh(n);
   for (int k = 0; k < n; k++) {
     tsh[k] = s.src[k0 + k].tshift;
-    t0s[k] = 1.20 / s.src[k0 + k].f0 + tsh[k];
+    const double hdur = 1.0 / s.src[k0 + k].f0;
+    t0s[k] = 1.20 * hdur + tsh[k];
   }
   double t0 = 0.0, min_tshift = 0.0;
   if (n == 1) {
@@ -494,17 +611,35 @@ double build_source_set(Sim& s, int k0, int n) {
     lagrange(s.src_xi[k], s.basis.xi, hx);
     lagrange(s.src_gamma[k], s.basis.xi, hg);
     const double ang = s.src[k].anglesource * PI / 180.0;
+    const bool fluid = s.nspec_ac > 0 && s.is_acoustic[s.src_ispec[k] - 1];
     for (int j = 0; j < NGLL; j++)
       for (int i = 0; i < NGLL; i++) {
+        // a pressure source in a fluid element, a force in a solid one (setup_sources_receivers.F90:994-1010)
         const double h = hx[i] * hg[j];
-        s.sourcearrays[(((size_t)k * NGLL + j) * NGLL + i) * 2] = (float)(std::sin(ang) * h);
-        s.sourcearrays[(((size_t)k * NGLL + j) * NGLL + i) * 2 + 1] = (float)(-std::cos(ang) * h);
+        s.sourcearrays[(((size_t)k * NGLL + j) * NGLL + i) * 2] = (float)(fluid ? h : std::sin(ang) * h);
+        s.sourcearrays[(((size_t)k * NGLL + j) * NGLL + i) * 2 + 1] = (float)(fluid ? h : -std::cos(ang) * h);
       }
-    const double a = PI * PI * s.src[k].f0 * s.src[k].f0;
+    // time functions (comp_source_time_function.f90:52-200); USE_TRICK_FOR_BETTER_PRESSURE replaces each by its
+    // second time derivative (prepare_source_time_function.f90:200-262)
+    const double a = (PI * PI) * (s.src[k].f0 * s.src[k].f0);
+    const int type = s.src[k].time_function_type;
+    const bool trick = s.p.use_trick != 0;
     for (int it = 0; it < s.p.nstep; it++) {
       const double t = (double)it * s.p.dt - t0 - tsh[q];
-      const double g = std::exp(-a * t * t);
-      const double v = (s.src[k].time_function_type == 1) ? (1.0 - 2.0 * a * t * t) * g : t * g;
+      const double t2 = t * t;
+      double v;
+      if (type == 1 && !trick)
+        v = (1.0 - 2.0 * a * t2) * std::exp(-a * t2);
+      else if (type == 1)
+        v = -(2.0 * a * (3.0 - 12.0 * a * t * t + 4.0 * (a * a) * t * t * t * t) * std::exp(-a * t * t));
+      else if (type == 2 && !trick)
+        v = t * std::exp(-a * t2);
+      else if (type == 2)
+        v = 2.0 * a * t * std::exp(-a * t2) * (-3.0 + 2.0 * a * t2);
+      else if (!trick)
+        v = -std::exp(-a * t2) / (2.0 * a);
+      else
+        v = (1.0 - 2.0 * a * t2) * std::exp(-a * t2);
       s.stf[(size_t)it * ns + k] = (float)(-s.src[k].factor * v);
     }
   }
@@ -563,8 +698,14 @@ void prepare_device(Sim& s) {
   int ngllx = NGLL, nspec = s.nspec, nglob = s.nglob, zero = 0, one = 1;
   int stacey = s.p.stacey ? 1 : 0, pml = 0;
   std::vector<int> inner(s.nspec, 1), is_elastic(s.nspec, 1), is_acoustic(s.nspec, 0);
+  for (int e = 0; e < s.nspec && s.nspec_ac > 0; e++) {
+    is_acoustic[e] = s.is_acoustic[e];
+    is_elastic[e] = !s.is_acoustic[e];
+  }
   int nsrc = s.nsrc, nstep = s.p.nstep, nrec = s.nrec;
-  int simtype = s.simulation_type, p_sv = 1, nspec_ac = 0, nspec_el = s.nspec, save_fwd = s.save_forward;
+  int simtype = s.simulation_type, p_sv = 1, nspec_ac = s.nspec_ac, nspec_el = s.nspec - s.nspec_ac;
+  int save_fwd = s.save_forward;
+  s.seismotype[0] = s.p.seismotype == 4 ? 4 : 1;
   int nsig = 1, nlength = s.p.nstep;
   int dummy_i = 0;
   std::vector<int> src_ispec = s.src_ispec, rec_ispec = s.rec_ispec;
@@ -580,31 +721,55 @@ void prepare_device(Sim& s) {
                             s.cosrot.data(), s.sinrot.data(), &simtype, &p_sv, &nspec_ac, &nspec_el,
                             is_acoustic.data(), is_elastic.data(), &myrank, &save_fwd, s.xir.data(), s.gammar.data(),
                             &nsig, s.seismotype, &nlength);
-  // single partition: every element is "inner" (iphase 2); the outer list is empty
-  int num_phase = s.nspec;
-  s.phase_ispec.assign((size_t)2 * s.nspec, 0);
-  for (int e = 0; e < s.nspec; e++) s.phase_ispec[(size_t)s.nspec + e] = e + 1;
-  std::vector<int> aniso(s.nspec, 0);
+  // single partition: every element is "inner" (iphase 2); the outer lists are empty
   float dummy_f = 0.f;
-  prepare_fields_elastic_device_(&s.mesh, s.rmassx.data(), s.rmassz.data(), &num_phase, s.phase_ispec.data(),
-                                 aniso.data(), &zero, &dummy_f, &dummy_f, &dummy_f, &dummy_f, &dummy_f, &dummy_f,
-                                 &dummy_f, &dummy_f, &dummy_f, &zero, &dummy_i, &zero, &dummy_f, &dummy_f, &dummy_f,
-                                 &dummy_f);
+  if (s.nspec_ac > 0) {
+    int num_phase = s.nspec_ac, nfree = (int)s.free_ispec.size(), ncoup = (int)s.coup_ac_ispec.size();
+    int any_el = nspec_el > 0 ? 1 : 0;
+    std::vector<int> phase((size_t)2 * num_phase, 0);
+    for (int e = 0, q = 0; e < s.nspec; e++)
+      if (s.is_acoustic[e]) phase[(size_t)num_phase + q++] = e + 1;
+    std::vector<int> fi = s.free_ispec, fij = s.free_ij, ci = s.coup_ac_ispec, cij = s.coup_ij;
+    std::vector<float> cn = s.coup_normal, cw = s.coup_jacw;
+    if (fi.empty()) { fi.push_back(1); fij.assign(2 * NGLL, 1); }
+    if (ci.empty()) { ci.push_back(1); cij.assign(2 * NGLL, 1); cn.assign(2 * NGLL, 0.f); cw.assign(NGLL, 0.f); }
+    prepare_fields_acoustic_device_(&s.mesh, s.rmass_ac.data(), s.rho.data(), s.kappa.data(), &num_phase,
+                                    phase.data(), &nfree, fi.data(), fij.data(), &any_el, &ncoup, ci.data(),
+                                    cij.data(), cn.data(), cw.data(), &zero, &dummy_i, &zero, &dummy_f, &dummy_f,
+                                    &zero, &dummy_f);
+  }
+  if (nspec_el > 0) {
+    int num_phase = nspec_el;
+    s.phase_ispec.assign((size_t)2 * num_phase, 0);
+    for (int e = 0, q = 0; e < s.nspec; e++)
+      if (is_elastic[e]) s.phase_ispec[(size_t)num_phase + q++] = e + 1;
+    std::vector<int> aniso(s.nspec, 0);
+    prepare_fields_elastic_device_(&s.mesh, s.rmassx.data(), s.rmassz.data(), &num_phase, s.phase_ispec.data(),
+                                   aniso.data(), &zero, &dummy_f, &dummy_f, &dummy_f, &dummy_f, &dummy_f, &dummy_f,
+                                   &dummy_f, &dummy_f, &dummy_f, &zero, &dummy_i, &zero, &dummy_f, &dummy_f, &dummy_f,
+                                   &dummy_f);
+  }
   if (s.simulation_type == 3) {
     int size = 2 * s.nglob;
     prepare_fields_elastic_adj_dev_(&s.mesh, &size, &zero, &zero, &zero);
   }
   if (s.p.stacey) {
-    int nf = s.nabs;
-    prepare_stacey_device_(&s.mesh, &zero, &one, s.rho_vp.data(), s.rho_vs.data(), &s.nspec_side[1], &s.nspec_side[4],
+    int nf = s.nabs, ac_sim = s.nspec_ac > 0 ? 1 : 0, el_sim = nspec_el > 0 ? 1 : 0;
+    prepare_stacey_device_(&s.mesh, &ac_sim, &el_sim, s.rho_vp.data(), s.rho_vs.data(), &s.nspec_side[1], &s.nspec_side[4],
                            &s.nspec_side[2], &s.nspec_side[3], s.abs_ispec.data(), s.abs_ij.data(),
                            s.abs_normal.data(), s.abs_jacw.data(), &nf, s.edge_abs.data(), s.ib_bottom.data(),
                            s.ib_left.data(), s.ib_right.data(), s.ib_top.data());
   }
   std::vector<float> zeros((size_t)2 * s.nglob, 0.f);
   int size = 2 * s.nglob;
-  transfer_fields_el_to_device_(&size, zeros.data(), zeros.data(), zeros.data(), &s.mesh);
-  if (s.simulation_type == 3) transfer_b_fields_to_device_(&size, zeros.data(), zeros.data(), zeros.data(), &s.mesh);
+  if (s.nspec_ac > 0) {
+    int nac = s.nglob;
+    transfer_fields_ac_to_device_(&nac, zeros.data(), zeros.data(), zeros.data(), &s.mesh);
+  }
+  if (nspec_el > 0) {
+    transfer_fields_el_to_device_(&size, zeros.data(), zeros.data(), zeros.data(), &s.mesh);
+    if (s.simulation_type == 3) transfer_b_fields_to_device_(&size, zeros.data(), zeros.data(), zeros.data(), &s.mesh);
+  }
   if (s.nshot > 0) sf2d_batch_prepare(&s.mesh, s.nshot, s.src_shot.data());
 }
 
@@ -622,6 +787,7 @@ void* create(const sf2d_params* p, const double* rho, const double* vp, const do
     return nullptr;
   }
   build_materials(*s, rho, vp, vs);
+  build_domains(*s);
   const size_t ne = (size_t)s->nspec * NGLL2;
   s->model_rho.assign(rho, rho + ne);
   s->model_vp.assign(vp, vp + ne);
@@ -764,7 +930,11 @@ int sf2d_sim_forward(void* sim, int nstep_run, double* seis, float* absorb, doub
   int nspec_outer = 0, nspec_inner = s->nspec, zero = 0, one = 1;
   const int it_begin = s->it_done + 1;
   int i_sig = 1, nlength = s->p.nstep, it_end = s->it_done + nstep_run;
-  int elastic_sim = 1, acoustic_sim = 0;
+  const bool any_ac = s->nspec_ac > 0, any_el = s->nspec_ac < s->nspec;
+  int elastic_sim = any_el ? 1 : 0, acoustic_sim = any_ac ? 1 : 0, trick = s->p.use_trick ? 1 : 0;
+  int nspec_inner_ac = s->nspec_ac, nfluid_solid = (int)s->coup_ac_ispec.size();
+  nspec_inner = s->nspec - s->nspec_ac;
+  if (any_ac && (s->save_forward || s->grad_save)) return 1;  // no fluid Stacey storage or fluid gradients
   if (s->sisux.empty()) {
     s->sisux.assign((size_t)std::max(1, s->nrec) * nlength, 0.0);
     s->sisuz.assign(s->sisux.size(), 0.0);
@@ -791,10 +961,38 @@ int sf2d_sim_forward(void* sim, int nstep_run, double* seis, float* absorb, doub
   for (int it = it_begin; it <= it_end; it++) {
     const int step = it - it_begin;
     mark(step, 2);
-    update_displacement_cuda_(&s->mesh, &deltat, &deltatsqover2, &deltatover2, &b_deltat, &b_deltatsqover2,
-                              &b_deltatover2);
+    // update_displ_Newmark (update_displacement_Newmark.F90:130-300): fluid, then solid
+    if (any_ac)
+      update_displacement_ac_cuda_(&s->mesh, &deltat, &deltatsqover2, &deltatover2, &b_deltat, &b_deltatsqover2,
+                                   &b_deltatover2, &zero, &zero);
+    if (any_el)
+      update_displacement_cuda_(&s->mesh, &deltat, &deltatsqover2, &deltatover2, &b_deltat, &b_deltatsqover2,
+                                &b_deltatover2);
     mark(step, 3);
-    for (int iphase = 1; iphase <= 2; iphase++) {
+    // the fluid first, coupled to the predicted solid displacement (compute_gpu_acoustic.f90:34-220)
+    if (any_ac && !any_el) mark(step, 0);
+    if (any_ac) {
+      acoustic_enforce_free_surf_cuda_(&s->mesh, &one, &zero);
+      for (int iphase = 1; iphase <= 2; iphase++) {
+        compute_forces_acoustic_cuda_(&s->mesh, &iphase, &zero, &nspec_inner_ac, &zero, &one, &zero);
+        if (iphase == 1) {
+          if (s->p.stacey)
+            compute_stacey_acoustic_cuda_(&s->mesh, &iphase, slice[4].data(), slice[2].data(), slice[3].data(),
+                                          slice[1].data(), &one, &zero, &zero);
+          if (any_el && nfluid_solid > 0) compute_coupling_ac_el_cuda_(&s->mesh, &iphase, &nfluid_solid, &one);
+          compute_add_sources_ac_cuda_(&s->mesh, &iphase, &it);
+        }
+      }
+      kernel_3_acoustic_cuda_(&s->mesh, &deltatover2, &b_deltatover2, &one, &zero);
+      acoustic_enforce_free_surf_cuda_(&s->mesh, &one, &zero);
+    }
+    if (any_ac && !any_el) {
+      mark(step, 1);
+      mark(step, 4);
+      mark(step, 5);
+    }
+    // then the solid, coupled to the new fluid pressure (compute_gpu_elastic.f90:34-215)
+    for (int iphase = 1; iphase <= 2 && any_el; iphase++) {
       if (iphase == 2) mark(step, 0);
       compute_forces_viscoelastic_cuda_(&s->mesh, &iphase, &deltat, &nspec_outer, &nspec_inner, &zero, &zero, &one,
                                         &zero);
@@ -804,6 +1002,7 @@ int sf2d_sim_forward(void* sim, int nstep_run, double* seis, float* absorb, doub
         if (s->p.stacey)
           compute_stacey_viscoelastic_cuda_(&s->mesh, &iphase, slice[4].data(), slice[2].data(), slice[3].data(),
                                             slice[1].data(), &one, &zero, &zero);
+        if (any_ac && nfluid_solid > 0) compute_coupling_el_ac_cuda_(&s->mesh, &iphase, &nfluid_solid, &one);
         compute_add_sources_el_cuda_(&s->mesh, &iphase, &it);
         mark(step, 5);
       }
@@ -819,7 +1018,7 @@ int sf2d_sim_forward(void* sim, int nstep_run, double* seis, float* absorb, doub
       }
     }
     mark(step, 6);
-    kernel_3_a_cuda_(&s->mesh, &deltatover2, &b_deltatover2, &one, &zero);
+    if (any_el) kernel_3_a_cuda_(&s->mesh, &deltatover2, &b_deltatover2, &one, &zero);
     mark(step, 7);
     // the fields now hold (d_n, v', r_n) with the corrector pending: the state the reverse sweep needs
     if (s->grad_save == 1 && it % s->grad_chunk == 0 && it < s->p.nstep)
@@ -830,13 +1029,13 @@ int sf2d_sim_forward(void* sim, int nstep_run, double* seis, float* absorb, doub
     if (s->nrec > 0) {
       int cur = it;
       compute_seismograms_cuda_(&s->mesh, &i_sig, s->sisux.data(), s->sisuz.data(), &cur, &nlength, &elastic_sim,
-                                &acoustic_sim, &zero, &zero, &it, &it_end);
+                                &acoustic_sim, &trick, &zero, &it, &it_end);
     }
     mark(step, 9);
   }
   s->it_done = it_end;
   // the last corrector is queued by kernel_3_a_cuda_; run it inside the measured loop
-  sf2d_mesh_flush(&s->mesh);
+  if (any_el) sf2d_mesh_flush(&s->mesh);
   if (timed) {
     hc(hipEventRecord(loop_end, stream));
     hc(hipEventSynchronize(loop_end));
@@ -913,7 +1112,7 @@ void sf2d_sim_batch_get_fields(void* sim, int shot, float* displ, float* veloc, 
 int sf2d_sim_adjoint(void* sim, const float* adj, const float* absorb, const float* last_displ,
                      const float* last_veloc, const float* last_accel, double* seis, float* kernels) {
   Sim* s = (Sim*)sim;
-  if (!s || !s->mesh || s->simulation_type != 3) return 1;
+  if (!s || !s->mesh || s->simulation_type != 3 || s->nspec_ac > 0) return 1;
   const int nstep = s->p.nstep;
   // adjoint sources: Fortran source_adjoint(nrecloc, NSTEP, 2) (setup_sources_receivers.F90:578)
   {
@@ -1021,7 +1220,7 @@ int sf2d_sim_set_model(void* sim, const double* rho, const double* vp, const dou
 
 int sf2d_sim_gradient_forward(void* sim, const double* obs, int chunk, double* chi, double* seis) {
   Sim* s = (Sim*)sim;
-  if (!s || !s->mesh || s->simulation_type != 1 || chunk < 1) return 1;
+  if (!s || !s->mesh || s->simulation_type != 1 || chunk < 1 || s->nspec_ac > 0) return 1;
   const int N = s->p.nstep, R = s->nrec, S = std::max(1, s->nshot);
   s->grad_chunk = std::min(chunk, N);
   std::vector<double> sm((size_t)S * std::max(1, R) * 2 * N, 0.0);
@@ -1138,14 +1337,53 @@ void sf2d_sim_reset(void* sim) {
   Sim* s = (Sim*)sim;
   if (!s || !s->mesh) return;
   std::vector<float> zeros((size_t)2 * s->nglob, 0.f);
-  int size = 2 * s->nglob;
-  transfer_fields_el_to_device_(&size, zeros.data(), zeros.data(), zeros.data(), &s->mesh);
+  int size = 2 * s->nglob, nac = s->nglob;
+  if (s->nspec_ac > 0) transfer_fields_ac_to_device_(&nac, zeros.data(), zeros.data(), zeros.data(), &s->mesh);
+  if (s->nspec_ac < s->nspec) transfer_fields_el_to_device_(&size, zeros.data(), zeros.data(), zeros.data(), &s->mesh);
   s->it_done = 0;
   std::fill(s->sisux.begin(), s->sisux.end(), 0.0);
   std::fill(s->sisuz.begin(), s->sisuz.end(), 0.0);
 }
 
 int sf2d_sim_steps_done(void* sim) { return ((Sim*)sim)->it_done; }
+
+int sf2d_sim_nspec_acoustic(void* sim) { return ((Sim*)sim)->nspec_ac; }
+int sf2d_sim_ncoupling(void* sim) { return (int)((Sim*)sim)->coup_ac_ispec.size(); }
+
+void sf2d_sim_get_rmass_acoustic(void* sim, float* out) {
+  Sim* s = (Sim*)sim;
+  for (int g = 0; g < s->nglob; g++) out[g] = s->nspec_ac > 0 ? s->rmass_ac[g] : 1.f;
+}
+
+void sf2d_sim_get_coupling(void* sim, int* ac_ispec, int* ac_code, int* el_ispec, int* el_code) {
+  Sim* s = (Sim*)sim;
+  for (size_t q = 0; q < s->coup_ac_ispec.size(); q++) {
+    ac_ispec[q] = s->coup_ac_ispec[q];
+    ac_code[q] = s->coup_ac_code[q];
+    el_ispec[q] = s->coup_el_ispec[q];
+    el_code[q] = s->coup_el_code[q];
+  }
+}
+
+void sf2d_sim_get_acoustic_fields(void* sim, float* pot, float* pot_dot, float* pot_dot_dot) {
+  Sim* s = (Sim*)sim;
+  int size = s->nglob;
+  std::vector<float> p((size_t)size), v((size_t)size), a((size_t)size);
+  transfer_fields_ac_from_device_(&size, p.data(), v.data(), a.data(), &s->mesh);
+  if (pot) std::copy(p.begin(), p.end(), pot);
+  if (pot_dot) std::copy(v.begin(), v.end(), pot_dot);
+  if (pot_dot_dot) std::copy(a.begin(), a.end(), pot_dot_dot);
+}
+
+void sf2d_sim_set_acoustic_fields(void* sim, const float* pot, const float* pot_dot, const float* pot_dot_dot) {
+  Sim* s = (Sim*)sim;
+  int size = s->nglob;
+  std::vector<float> p((size_t)size, 0.f), v((size_t)size, 0.f), a((size_t)size, 0.f);
+  if (pot) std::copy(pot, pot + size, p.begin());
+  if (pot_dot) std::copy(pot_dot, pot_dot + size, v.begin());
+  if (pot_dot_dot) std::copy(pot_dot_dot, pot_dot_dot + size, a.begin());
+  transfer_fields_ac_to_device_(&size, p.data(), v.data(), a.data(), &s->mesh);
+}
 
 void sf2d_sim_get_fields(void* sim, float* displ, float* veloc, float* accel) {
   Sim* s = (Sim*)sim;

@@ -367,6 +367,7 @@ extern "C" {
 int sf2d_grad_prepare(long* Mesh_pointer, int chunk, int nrec) {
   Mesh* mp = handle(Mesh_pointer);
   if (!mp->sorted) fail("sf2d_grad_prepare: prepare_fields_elastic_device was not called");
+  if (mp->nspec_acoustic > 0) fail("sf2d_grad_prepare: gradients of meshes with fluid elements are not supported");
   return grad_prepare(mp, Mesh_pointer, chunk, nrec, 0);
 }
 

@@ -27,6 +27,8 @@ struct Basis {
   float w[NGLL];     // GLL weights
 };
 
+struct AcousticState;  // fluid elements (acoustic.hip)
+
 struct Mesh {
   int myrank = 0;
   int device = 0;
@@ -44,6 +46,8 @@ struct Mesh {
   int* d_ispec_is_elastic = nullptr;
   int nspec_elastic = 0, nspec_acoustic = 0;
   int all_elastic = 1;
+  std::vector<char> h_is_acoustic;  // [nspec] caller order (empty: no fluid element)
+  AcousticState* ac = nullptr;      // fluid potential and its set-up, from prepare_fields_acoustic_device_
 
   // elastic wavefields [nglob][2] and mass matrix
   float *d_displ = nullptr, *d_veloc = nullptr, *d_accel = nullptr;
@@ -197,6 +201,14 @@ void flush_correctors(Mesh* mp);
 // ps > 1: shot-minor fields of ps shots per point (nshot and shot_stride unused)
 void apply_element_forces(const Mesh* mp, int ph, const float* u, float* acc, int nshot = 1, long shot_stride = 0,
                           int first_touch = 0, int ps = 1);
+// fluid elements (acoustic.hip): release, the fluid's Stacey points (recorded by prepare_stacey_device_: ispec
+// 0-based caller order, ij 0-based with -1 for points an edge does not own, rho_vp [nspec][25] caller order),
+// pressure seismograms (-potential with USE_TRICK_FOR_BETTER_PRESSURE, else -potential_dot_dot) and the number of
+// receivers in fluid elements
+void acoustic_release(Mesh* mp);
+void acoustic_note_stacey(Mesh* mp, int nf, const int* ispec, const int* ij, const float* jacw, const float* rho_vp);
+void launch_acoustic_pressure_seismograms(Mesh* mp, float* dst, int cur, int use_trick);
+int acoustic_receivers(Mesh* mp);
 // point arrays in the caller's numbering <-> the device numbering (n2 values per point)
 void points_to_device(const Mesh* mp, float* dev, const float* host, int n2, const char* what);
 void points_from_device(const Mesh* mp, float* host, const float* dev, int n2, const char* what);

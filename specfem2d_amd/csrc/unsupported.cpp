@@ -2,8 +2,8 @@
 //
 // They exist so that a SPECFEM2D binary built with GPU_MODE links against this library unchanged; each
 // terminates the run with a clear message (the reference's error convention) instead of silently doing
-// nothing.  Acoustic potentials, fluid-solid coupling, PML, attenuation transfers, MPI halo exchange,
-// kernel smoothing, Hessians and moving sources fall in this group.
+// nothing.  Backward and adjoint fluid wavefields, acoustic kernels, PML, attenuation transfers, MPI halo
+// exchange, kernel smoothing, Hessians and moving sources fall in this group.
 #include "specfem2d_gpu.h"
 
 namespace sf2d {
@@ -22,33 +22,21 @@ void prepare_boundary_on_device_(long*) { SF2D_UNSUPPORTED(prepare_boundary_on_d
 void transfer_boundary_to_device_a_(long*, realw*, const int*) { SF2D_UNSUPPORTED(transfer_boundary_to_device_a, "MPI halo exchange"); }
 void transfer_asmbl_accel_to_device_(long*, realw*, const int*, const int*, const int*, const int*) { SF2D_UNSUPPORTED(transfer_asmbl_accel_to_device, "MPI halo exchange"); }
 void sync_copy_from_device_(long*, int*, realw*) { SF2D_UNSUPPORTED(sync_copy_from_device, "MPI halo exchange"); }
-void get_norm_acoustic_from_device_(realw*, long*, const int*) { SF2D_UNSUPPORTED(get_norm_acoustic_from_device, "the acoustic potential"); }
-void compute_add_sources_ac_cuda_(long*, int*, int*) { SF2D_UNSUPPORTED(compute_add_sources_ac_cuda, "acoustic sources"); }
 void compute_add_sources_ac_s3_cuda_(long*, int*, int*) { SF2D_UNSUPPORTED(compute_add_sources_ac_s3_cuda, "acoustic sources"); }
 void compute_add_moving_sources_ac_cuda_(long*, int*, int*, int*, int*, int*) { SF2D_UNSUPPORTED(compute_add_moving_sources_ac_cuda, "moving sources"); }
 void add_sources_ac_sim_2_or_3_cuda_(long*, int*, int*, int*, int*) { SF2D_UNSUPPORTED(add_sources_ac_sim_2_or_3_cuda, "acoustic adjoint sources"); }
-void compute_coupling_ac_el_cuda_(long*, int*, int*, int*) { SF2D_UNSUPPORTED(compute_coupling_ac_el_cuda, "fluid-solid coupling"); }
-void compute_coupling_el_ac_cuda_(long*, int*, int*, int*) { SF2D_UNSUPPORTED(compute_coupling_el_ac_cuda, "fluid-solid coupling"); }
-void compute_forces_acoustic_cuda_(long*, int*, int*, int*, int*, int*, int*) { SF2D_UNSUPPORTED(compute_forces_acoustic_cuda, "the acoustic potential"); }
 void compute_kernels_acoustic_cuda_(long*, realw*) { SF2D_UNSUPPORTED(compute_kernels_acoustic_cuda, "acoustic kernels"); }
 void compute_kernels_hess_cuda_(long*, int*, int*) { SF2D_UNSUPPORTED(compute_kernels_hess_cuda, "approximate Hessian kernels"); }
-void compute_stacey_acoustic_cuda_(long*, int*, realw*, realw*, realw*, realw*, int*, int*, int*) { SF2D_UNSUPPORTED(compute_stacey_acoustic_cuda, "acoustic Stacey boundaries"); }
-void acoustic_enforce_free_surf_cuda_(long*, int*, int*) { SF2D_UNSUPPORTED(acoustic_enforce_free_surf_cuda, "the acoustic free surface"); }
 void pml_boundary_acoustic_cuda_(long*, int*, int*) { SF2D_UNSUPPORTED(pml_boundary_acoustic_cuda, "PML"); }
-void prepare_fields_acoustic_device_(long*, realw*, realw*, realw*, int*, int*, int*, int*, int*, int*, int*, int*, int*, realw*, realw*, int*, int*, int*, realw*, realw*, int*, realw*) { SF2D_UNSUPPORTED(prepare_fields_acoustic_device, "the acoustic potential"); }
-void prepare_fields_acoustic_adj_dev_(long*, int*, int*, int*) { SF2D_UNSUPPORTED(prepare_fields_acoustic_adj_dev, "the acoustic potential"); }
 void prepare_pml_device_(long*, int*, int*, int*, int*, int*, realw*, realw*, realw*, realw*, realw*, realw*, realw*, realw*, int*, int*) { SF2D_UNSUPPORTED(prepare_pml_device, "PML"); }
 void prepare_moving_sources_cuda_(long*, int*, int*, realw*, int*, int*, realw*) { SF2D_UNSUPPORTED(prepare_moving_sources_cuda, "moving sources"); }
 void prepare_arrays_gpu_(long*, realw*, realw*, realw*, realw*, realw*, realw*, int*, int*, realw*) { SF2D_UNSUPPORTED(prepare_arrays_gpu, "kernel smoothing"); }
 void compute_smooth_(long*, realw*, realw*, realw*, realw*, const int*) { SF2D_UNSUPPORTED(compute_smooth, "kernel smoothing"); }
 void get_smooth_(long*, realw*) { SF2D_UNSUPPORTED(get_smooth, "kernel smoothing"); }
-void transfer_fields_ac_to_device_(int*, realw*, realw*, realw*, long*) { SF2D_UNSUPPORTED(transfer_fields_ac_to_device, "the acoustic potential"); }
 void transfer_b_fields_ac_to_device_(int*, realw*, realw*, realw*, long*) { SF2D_UNSUPPORTED(transfer_b_fields_ac_to_device, "the acoustic potential"); }
-void transfer_fields_ac_from_device_(int*, realw*, realw*, realw*, long*) { SF2D_UNSUPPORTED(transfer_fields_ac_from_device, "the acoustic potential"); }
 void transfer_b_fields_ac_from_device_(int*, realw*, realw*, realw*, long*) { SF2D_UNSUPPORTED(transfer_b_fields_ac_from_device, "the acoustic potential"); }
 void transfer_b_potential_ac_from_device_(int*, realw*, long*) { SF2D_UNSUPPORTED(transfer_b_potential_ac_from_device, "the acoustic potential"); }
 void transfer_b_potential_ac_to_device_(int*, realw*, long*) { SF2D_UNSUPPORTED(transfer_b_potential_ac_to_device, "the acoustic potential"); }
-void transfer_dot_dot_from_device_(int*, realw*, long*) { SF2D_UNSUPPORTED(transfer_dot_dot_from_device, "the acoustic potential"); }
 void transfer_b_dot_dot_from_device_(int*, realw*, long*) { SF2D_UNSUPPORTED(transfer_b_dot_dot_from_device, "the acoustic potential"); }
 void transfer_kernels_ac_to_host_(long*, realw*, realw*, int*) { SF2D_UNSUPPORTED(transfer_kernels_ac_to_host, "acoustic kernels"); }
 void transfer_kernels_hess_el_tohost_(long*, realw*, int*) { SF2D_UNSUPPORTED(transfer_kernels_hess_el_tohost, "approximate Hessian kernels"); }
@@ -59,7 +47,5 @@ void transfer_viscoelastic_b_var_to_device_(int*, realw*, realw*, realw*, realw*
 void transfer_viscoelastic_var_from_device_(int*, realw*, realw*, realw*, realw*, realw*, realw*, long*) { SF2D_UNSUPPORTED(transfer_viscoelastic_var_from_device, "viscoelastic attenuation"); }
 void transfer_async_pot_ac_from_device_(realw*, long*) { SF2D_UNSUPPORTED(transfer_async_pot_ac_from_device, "the acoustic potential"); }
 void transfer_async_pot_ac_to_device_(realw*, long*) { SF2D_UNSUPPORTED(transfer_async_pot_ac_to_device, "the acoustic potential"); }
-void update_displacement_ac_cuda_(long*, realw*, realw*, realw*, realw*, realw*, realw*, int*, int*) { SF2D_UNSUPPORTED(update_displacement_ac_cuda, "the acoustic potential"); }
-void kernel_3_acoustic_cuda_(long*, realw*, realw*, int*, int*) { SF2D_UNSUPPORTED(kernel_3_acoustic_cuda, "the acoustic potential"); }
 
 }  // extern "C"

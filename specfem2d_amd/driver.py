@@ -26,7 +26,7 @@ class Params(C.Structure):
         ("xmin", C.c_double), ("xmax", C.c_double), ("zinterface", C.c_double * (MAXLAYERS + 1)),
         ("absorbbottom", C.c_int), ("absorbright", C.c_int), ("absorbtop", C.c_int), ("absorbleft", C.c_int),
         ("stacey", C.c_int), ("nstep", C.c_int), ("dt", C.c_double), ("user_t0", C.c_double),
-        ("anglerec", C.c_double), ("material_mode", C.c_int),
+        ("anglerec", C.c_double), ("material_mode", C.c_int), ("seismotype", C.c_int), ("use_trick", C.c_int),
     ]
 
 
@@ -58,7 +58,7 @@ def load_libraries():
     host.sf2d_sim_prepare_device.argtypes = [P]
     host.sf2d_sim_prepare_device.restype = C.c_int
     host.sf2d_sim_destroy.argtypes = [P]
-    for f in ("sf2d_sim_nspec", "sf2d_sim_nglob", "sf2d_sim_nabs"):
+    for f in ("sf2d_sim_nspec", "sf2d_sim_nglob", "sf2d_sim_nabs", "sf2d_sim_nspec_acoustic", "sf2d_sim_ncoupling"):
         getattr(host, f).argtypes = [P]
         getattr(host, f).restype = C.c_int
     host.sf2d_sim_t0.argtypes = [P]
@@ -94,8 +94,11 @@ def load_libraries():
     host.sf2d_sim_reset.argtypes = [P]
     host.sf2d_sim_steps_done.argtypes = [P]
     host.sf2d_sim_steps_done.restype = C.c_int
-    for f in ("sf2d_sim_get_fields", "sf2d_sim_set_fields", "sf2d_sim_get_b_fields", "sf2d_sim_set_b_fields"):
+    for f in ("sf2d_sim_get_fields", "sf2d_sim_set_fields", "sf2d_sim_get_b_fields", "sf2d_sim_set_b_fields",
+              "sf2d_sim_get_acoustic_fields", "sf2d_sim_set_acoustic_fields"):
         getattr(host, f).argtypes = [P, P, P, P]
+    host.sf2d_sim_get_rmass_acoustic.argtypes = [P, P]
+    host.sf2d_sim_get_coupling.argtypes = [P, P, P, P, P]
     gpu.sf2d_mesh_stream.argtypes = [C.POINTER(C.c_long)]
     gpu.sf2d_mesh_stream.restype = P
     gpu.sf2d_mesh_affine_elements.argtypes = [C.POINTER(C.c_long)]
@@ -140,6 +143,7 @@ def params_from_config(cfg: SimulationConfig) -> Params:
     p.nstep, p.dt = int(cfg.nstep), float(cfg.dt)
     p.user_t0, p.anglerec = float(cfg.user_t0), float(cfg.anglerec)
     p.material_mode = int(cfg.material_mode)
+    p.seismotype, p.use_trick = int(cfg.seismotype), int(cfg.use_trick)
     return p
 
 
@@ -176,6 +180,8 @@ class Simulation:
         self.nglob = self.host.sf2d_sim_nglob(self.h)
         self.nabs = self.host.sf2d_sim_nabs(self.h)
         self.t0 = self.host.sf2d_sim_t0(self.h)
+        self.nspec_acoustic = self.host.sf2d_sim_nspec_acoustic(self.h)
+        self.ncoupling = self.host.sf2d_sim_ncoupling(self.h)
         self.on_device = False
         if device:
             self.prepare_device()
@@ -333,6 +339,7 @@ class Simulation:
         (displ, veloc, accel) of a save_forward run.  Returns (adjoint seismograms [nrec,2,nstep],
         kernels dict rho/mu/kappa of [nspec,25] arrays)."""
         self._need_device()
+        self._solid_only("adjoint")
         if self.simulation_type != 3:
             raise RuntimeError("create the simulation with simulation_type=3 for an adjoint run")
         n = self.cfg.nstep
@@ -346,6 +353,42 @@ class Simulation:
             raise RuntimeError("sf2d_sim_adjoint failed")
         ker = ker.reshape(3, self.nspec, NGLL2)
         return seis[: self.cfg.nrec], {"rho": ker[0], "mu": ker[1], "kappa": ker[2]}
+
+    # fluid elements ------------------------------------------------------
+    def rmass_acoustic(self):
+        out = np.zeros(self.nglob, np.float32)
+        self.host.sf2d_sim_get_rmass_acoustic(self.h, _ptr(out))
+        return out
+
+    def coupling(self):
+        """Fluid-solid edges: (fluid element, side, solid element, side), 0-based elements, sides 1 bottom,
+        2 right, 3 top, 4 left."""
+        n = max(1, self.ncoupling)
+        arrs = [np.zeros(n, np.int32) for _ in range(4)]
+        self.host.sf2d_sim_get_coupling(self.h, *[_ptr(a) for a in arrs])
+        out = np.stack(arrs, axis=1)[: self.ncoupling]
+        out[:, 0] -= 1
+        out[:, 2] -= 1
+        return out
+
+    def acoustic_fields(self):
+        """(potential, its first and second time derivatives) [nglob] of the fluid elements."""
+        self._need_device()
+        if not self.nspec_acoustic:
+            raise RuntimeError("no fluid element in this mesh")
+        p, v, a = (np.zeros(self.nglob, np.float32) for _ in range(3))
+        self.host.sf2d_sim_get_acoustic_fields(self.h, _ptr(p), _ptr(v), _ptr(a))
+        return p, v, a
+
+    def set_acoustic_fields(self, pot=None, pot_dot=None, pot_dot_dot=None):
+        self._need_device()
+        arrs = [None if x is None else np.ascontiguousarray(np.asarray(x, np.float32).reshape(-1))
+                for x in (pot, pot_dot, pot_dot_dot)]
+        self.host.sf2d_sim_set_acoustic_fields(self.h, *[_ptr(x) for x in arrs])
+
+    def _solid_only(self, what):
+        if self.nspec_acoustic:
+            raise NotImplementedError(f"{what}: meshes with fluid elements support forward runs only")
 
     # model and gradient ------------------------------------------------
     def set_model(self, rho, vp, vs):
@@ -363,6 +406,7 @@ class Simulation:
         """Forward run from rest over all NSTEP steps with a checkpoint every ``chunk`` steps kept on the
         device; returns (chi, seismograms) with chi = 1/2 dt sum |s - obs|^2 (obs [nrec,2,nstep])."""
         self._need_device()
+        self._solid_only("gradient")
         n, shape = self.cfg.nstep, self._seis_shape()
         o = np.ascontiguousarray(np.asarray(obs, np.float64).reshape(-1))
         if o.size != int(np.prod(shape)):
@@ -417,6 +461,8 @@ class ShotBatch(Simulation):
         self.shots = [list(s) for s in shots]
         if not self.shots or any(len(s) == 0 for s in self.shots):
             raise ValueError("every shot needs at least one source")
+        if np.any(np.asarray(cfg.material_arrays()[2]) == 0.0):
+            raise NotImplementedError("shot batches of meshes with fluid elements (vs = 0) are not supported")
         super().__init__(cfg, simulation_type=1, save_forward=False, device=device)
         self.nshot = len(self.shots)
 

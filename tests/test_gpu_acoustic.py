@@ -1,0 +1,133 @@
+"""Fluid elements on the GPU (acoustic potential, free surface, acoustic Stacey edges, pressure sources and
+seismograms, fluid-solid coupling), called through the Fortran-style C-ABI by the host driver, against the CPU
+oracle (itself pinned by the reference's pressure-source trace and by plane-wave reflection coefficients) and
+against the reference's committed trace.
+
+Tolerance: FP32 seismograms within rel-L2 1e-5 of the FP32 oracle, as for the solid path (atomics and FMA
+contraction change the rounding, not the scheme).
+"""
+import numpy as np
+import pytest
+
+from oracle.oracle import OracleSim
+from specfem2d_amd import config
+from specfem2d_amd.config import Source
+from specfem2d_amd.driver import Simulation
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-5
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+
+
+def test_pressure_source_against_oracle_and_reference(golden_dir):
+    cfg = config.pressure_source_acoustic_44x44()
+    sim = Simulation(cfg)
+    assert sim.nspec_acoustic == cfg.nspec
+    seis, t = sim.forward()
+    o = OracleSim(cfg, "sp")
+    ref = o.forward()
+    r = rel(seis, ref)
+    g = np.load(f"{golden_dir}/pressure_source_acoustic_ref_seis.npz")
+    tr, pr = g["time"], g["AA.S0001.PRE"]
+    early = tr < 0.35
+    e_ref = np.abs(seis[0, 0][early] - pr[early]).max() / np.abs(pr).max()
+    print(f"pressure source 44x44 fluid, 1400 steps: rel-L2 vs FP32 oracle {r:.3e}; max error vs REF_SEIS before "
+          f"0.35 s {e_ref:.2e} of the peak; loop {t['loop_ms']:.1f} ms")
+    assert r < TOL
+    assert e_ref < 1e-5
+    assert np.all(seis[0, 1] == 0.0)
+    # the final fields are the residue left after the Stacey edges absorbed the pulse, where FP32 rounding
+    # dominates (the FP32 oracle is 1.6e-5 / 1.2e-4 from the FP64 one): no further from FP64 than the FP32 oracle
+    dp = OracleSim(cfg, "dp")
+    dp.forward()
+    for gpu_f, sp_f, dp_f in zip(sim.acoustic_fields()[:2], o.acoustic_fields()[:2], dp.acoustic_fields()[:2]):
+        assert rel(gpu_f, dp_f) < 2 * rel(sp_f, dp_f) + TOL
+    sim.close()
+
+
+def test_pressure_without_trick_against_oracle():
+    # -potential_dot_dot carries the FP32 rounding noise that USE_TRICK_FOR_BETTER_PRESSURE exists to avoid (the
+    # FP32 oracle is 6e-4 from the FP64 one here): the GPU must be no further from FP64 than the FP32 oracle
+    cfg = config.pressure_source_acoustic_44x44(nstep=900).replace(use_trick=False)
+    sim = Simulation(cfg)
+    seis, _ = sim.forward()
+    sp, dp = OracleSim(cfg, "sp").forward(), OracleSim(cfg, "dp").forward()
+    e_gpu, e_sp = rel(seis, dp), rel(sp, dp)
+    print(f"pressure (-potential_dot_dot, Gaussian source), 900 steps: rel-L2 vs FP64 oracle {e_gpu:.3e} "
+          f"(FP32 oracle {e_sp:.3e}), vs FP32 oracle {rel(seis, sp):.3e}")
+    assert e_gpu < 2 * e_sp + TOL
+    sim.close()
+
+
+def _coupled_case(seismotype):
+    """24 x 16 elements of 50 m, the top 6 rows fluid under a free surface, a random solid below; a pressure
+    source in the fluid and a tilted force in the solid; receivers in the fluid (pressure, with
+    USE_TRICK_FOR_BETTER_PRESSURE) or the solid (displacement)."""
+    rng = np.random.default_rng(11)
+    pert = 0.05 * rng.standard_normal(24 * 16 * 25)
+    if seismotype == 4:
+        rec = np.array([[310.0, -90.0], [640.0, -210.0], [1010.0, -260.0], [575.0, -150.0]])
+    else:
+        rec = np.array([[320.0, -320.0], [650.0, -510.0], [990.0, -700.0], [600.0, -301.0]])
+    src = [Source(x=430.0, z=-140.0, f0=9.0, factor=1.0e4), Source(x=800.0, z=-520.0, f0=7.0, factor=1.0e9,
+                                                                  anglesource=35.0, tshift=0.02)]
+    cfg = config.layered_fluid_solid(24, 16, 6, 50.0, 900, 1.0e-3, pert=pert, sources=src, receivers=rec,
+                                     seismotype=seismotype, name=f"coupled_24x16_s{seismotype}")
+    return cfg.replace(use_trick=seismotype == 4)
+
+
+@pytest.mark.parametrize("seismotype", [4, 1])
+def test_coupled_fluid_solid_against_oracle(seismotype):
+    cfg = _coupled_case(seismotype)
+    sim = Simulation(cfg)
+    assert sim.nspec_acoustic == 24 * 6 and sim.ncoupling == 24
+    seis, t = sim.forward()
+    o = OracleSim(cfg, "sp")
+    ref = o.forward()
+    r = rel(seis, ref)
+    print(f"coupled 24x16 (6 fluid rows), 900 steps, seismotype {seismotype}: rel-L2 vs FP32 oracle {r:.3e}, "
+          f"loop {t['loop_ms']:.1f} ms")
+    assert r < TOL
+    for k in range(cfg.nrec):
+        assert rel(seis[k], ref[k]) < 5 * TOL
+    d, v, _ = sim.fields()
+    od, ov, _ = o.fields()
+    pot, pd, _ = sim.acoustic_fields()
+    opot, opd, _ = o.acoustic_fields()
+    assert rel(d, od) < 5 * TOL and rel(pot, opot) < 5 * TOL
+    # the free surface holds the potential at zero
+    isp, code = o.free_surface()
+    top = o.ibool()[isp][:, 20:25].reshape(-1) - 1
+    assert np.all(code == 3) and np.all(pot[top] == 0) and np.all(pd[top] == 0)
+    sim.close()
+
+
+def test_plane_wave_reflection_coefficient():
+    cfg = config.fluid_solid_plane_wave()
+    sim = Simulation(cfg)
+    seis, t = sim.forward()
+    ref = OracleSim(cfg, "sp").forward()
+    tt = np.arange(cfg.nstep) * cfg.dt - sim.t0
+    z1, z2 = 1020.0 * 1500.0, 2500.0 * 3000.0
+    r_exact = (z2 - z1) / (z2 + z1)
+    p = seis[1, 0]
+    d = p[tt < 0.7]
+    direct = d[np.argmax(np.abs(d))]
+    m = (tt > 0.8) & (tt < 1.3)
+    refl = p[m][np.argmax(np.abs(p[m]))]
+    print(f"plane wave at a fluid-solid interface: R = {refl / direct:.5f} (exact {r_exact:.5f}); rel-L2 vs FP32 "
+          f"oracle {rel(seis, ref):.3e}; loop {t['loop_ms']:.1f} ms")
+    assert refl / direct == pytest.approx(r_exact, rel=2e-3)
+    assert rel(seis, ref) < TOL
+    sim.close()
+
+
+def test_solid_only_paths_refuse_fluid_meshes():
+    cfg = _coupled_case(1).replace(nstep=10)
+    sim = Simulation(cfg)
+    with pytest.raises(NotImplementedError):
+        sim.gradient_forward(np.zeros((cfg.nrec, 2, cfg.nstep)))
+    sim.close()
