@@ -204,6 +204,32 @@ def copy_calibration(mib: int, reps: int = 5):
                       "launches": reps}), file=sys.stderr)
 
 
+def c2_coupled(steps: int = 100, warmup: int = 10):
+    """Forward of the coupled C2 mesh (2000 x 1000 elements of 10 m, the top 100 rows a water layer under a free
+    surface, a random-field solid below, fluid-solid coupling along the sea floor): GLL-point updates per second
+    over all points (fluid and solid), wall clock around the timed steps."""
+    import torch
+    from specfem2d_amd import config
+    from specfem2d_amd.driver import Simulation
+    cfg = config.c2_mesh_coupled(nstep=warmup + steps)
+    t = time.perf_counter()
+    sim = Simulation(cfg)
+    setup = time.perf_counter() - t
+    sim.forward(nstep=warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    seis, timing = sim.forward(nstep=steps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    out = {"workload": "c2_mesh_coupled_2000x1000", "nspec": sim.nspec, "nspec_fluid": sim.nspec_acoustic,
+           "nglob": sim.nglob, "fluid_solid_edges": sim.ncoupling, "steps": steps, "warmup": warmup,
+           "ms_per_step": wall * 1e3 / steps, "value": sim.nglob * steps / wall, "unit": "GLL-pt updates/s",
+           "gpu_loop_ms_events": timing["loop_ms"], "finite": bool(np.all(np.isfinite(seis))), "setup_s": setup,
+           "boundaries": "free surface on the water, Stacey bottom/left/right (no CPML)"}
+    sim.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -218,6 +244,9 @@ def main():
     ap.add_argument("--fwi-steps", type=int, default=1500, help="time steps of the batched FWI workload")
     ap.add_argument("--batch-shots", type=int, default=32,
                     help="shots per batch of the multi-shot FWI gradient (32 fill about 170 GB of checkpoints)")
+    ap.add_argument("--no-coupled", action="store_true", help="skip the coupled fluid-solid C2 forward")
+    ap.add_argument("--coupled-only", action="store_true",
+                    help="only the coupled fluid-solid C2 forward (profiling); prints its JSON object")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="after the timed region, run a plain copy of this size (counter calibration)")
     args = ap.parse_args()
@@ -235,6 +264,9 @@ def main():
 
     if args.fwi_only:
         print(json.dumps(fwi_gradient_batched(args.batch_shots, nstep=args.fwi_steps)))
+        return
+    if args.coupled_only:
+        print(json.dumps(c2_coupled(min(args.steps, 100), min(args.warmup, 10))))
         return
     cfg = make_config(args.workload, args.warmup + args.steps)
     t_setup = time.perf_counter()
@@ -309,6 +341,8 @@ def main():
     sim.close()
     if args.copy_calibration_mib:
         copy_calibration(args.copy_calibration_mib)
+    if not args.no_coupled:
+        out["c2_coupled"] = c2_coupled()
     if not args.no_gradient:
         out["fwi_gradient"] = fwi_gradient()
         out["fwi_gradient_batched"] = fwi_gradient_batched(args.batch_shots, nstep=args.fwi_steps)

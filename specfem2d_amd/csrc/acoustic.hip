@@ -10,7 +10,9 @@
 // a point) and their per-point operands are packed once in colour order -- {xix, xiz, gammax, gammaz}, J/rho and
 // the point id -- so one colour launch streams them with unit stride and assembles with plain read-modify-writes.
 // Boundary terms (free surface, Stacey, fluid-solid edges, sources, receivers) are flattened at set-up into
-// point lists with their weights, so each is one small launch that reads only what it updates.
+// point lists with their weights, so each is one small launch that reads only what it updates.  The Newmark
+// updates visit only the points of fluid elements (an id range plus a short list, see ac_ready): elsewhere the
+// potential has no source of change and keeps the value last transferred (zero in any physical state).
 //
 // Set-up is two-phase: the entry points of the preparation record the caller's arrays; the first call of the
 // time loop resolves them into device ids (after the solid's preparation has fixed the numbering) and uploads
@@ -81,12 +83,15 @@ __global__ __launch_bounds__(256) void k_acoustic_forces(int first, int n, const
   }
 }
 
-// Newmark predictor on the potential (update_displacement_Newmark.F90:190-240)
+// Newmark predictor on the potential (update_displacement_Newmark.F90:190-240) over the fluid's points: the id
+// range [r0, r0 + n) and the listed ids below it (list[0 .. nlist))
 __global__ __launch_bounds__(256) void k_ac_predictor(float* __restrict__ pot, float* __restrict__ pd,
-                                                      float* __restrict__ pdd, int n, float dt, float dt2o2,
+                                                      float* __restrict__ pdd, int r0, int n,
+                                                      const int* __restrict__ list, int nlist, float dt, float dt2o2,
                                                       float dto2) {
   const int stride = gridDim.x * blockDim.x;
-  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n + nlist; q += stride) {
+    const int k = q < n ? r0 + q : list[q - n];
     const float a = pdd[k], v = pd[k];
     pot[k] = pot[k] + dt * v + dt2o2 * a;
     pd[k] = v + dto2 * a;
@@ -94,11 +99,14 @@ __global__ __launch_bounds__(256) void k_ac_predictor(float* __restrict__ pot, f
   }
 }
 
-// mass and corrector (compute_forces_viscoacoustic_calling_routine.F90:183-205, update_veloc_acoustic_Newmark)
+// mass and corrector (compute_forces_viscoacoustic_calling_routine.F90:183-205, update_veloc_acoustic_Newmark),
+// same point set
 __global__ __launch_bounds__(256) void k_ac_corrector(float* __restrict__ pd, float* __restrict__ pdd,
-                                                      const float* __restrict__ rm, int n, float dto2) {
+                                                      const float* __restrict__ rm, int r0, int n,
+                                                      const int* __restrict__ list, int nlist, float dto2) {
   const int stride = gridDim.x * blockDim.x;
-  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n + nlist; q += stride) {
+    const int k = q < n ? r0 + q : list[q - n];
     const float a = pdd[k] * rm[k];
     pdd[k] = a;
     pd[k] = pd[k] + dto2 * a;
@@ -222,6 +230,9 @@ struct AcousticState {
   int colour_start[2][Mesh::MAX_COLOURS] = {};
   int colour_count[2][Mesh::MAX_COLOURS] = {};
   int nelem = 0;
+  // the fluid's points for the Newmark updates: the id range [r0, r0 + nrange) and the ids below it
+  int r0 = 0, nrange = 0, nlist = 0;
+  int* d_list = nullptr;
   int* d_ids = nullptr;      // [nelem][25] point ids, colour order
   float4* d_geom = nullptr;  // [nelem][25]
   float* d_fac = nullptr;    // [nelem][25] J / rho
@@ -245,7 +256,7 @@ void acoustic_release(Mesh* mp) {
   AcousticState* ac = mp->ac;
   if (!ac) return;
   device_free(ac->d_pot); device_free(ac->d_pd); device_free(ac->d_pdd); device_free(ac->d_rm);
-  device_free(ac->d_ids); device_free(ac->d_geom); device_free(ac->d_fac); device_free(ac->d_free);
+  device_free(ac->d_list); device_free(ac->d_ids); device_free(ac->d_geom); device_free(ac->d_fac); device_free(ac->d_free);
   device_free(ac->d_coup); device_free(ac->d_coup_n); device_free(ac->d_coup_w);
   device_free(ac->d_st); device_free(ac->d_st_w); device_free(ac->d_st_rvp);
   device_free(ac->d_src); device_free(ac->d_src_pts); device_free(ac->d_src_sa); device_free(ac->d_src_kap);
@@ -354,6 +365,27 @@ AcousticState* ac_ready(Mesh* mp, const char* who) {
     ac->d_ids = device_copy(ids.data(), n);
     ac->d_geom = device_copy(geom.data(), n);
     ac->d_fac = device_copy(fac.data(), n);
+    // the Newmark updates visit the fluid's points only (the potential is zero elsewhere): the solid's numbering
+    // puts the points of fluid elements alone after all others, so one id range holds them and a short list the
+    // ones on fluid-solid edges; the split minimises range length + 3 x list length
+    std::vector<int> pts(ids);
+    std::sort(pts.begin(), pts.end());
+    pts.erase(std::unique(pts.begin(), pts.end()), pts.end());
+    size_t best = 0;
+    long best_cost = -1;
+    for (size_t k = 0; k < pts.size(); k++) {
+      const long cost = (long)(pts.back() - pts[k] + 1) + 3L * (long)k;
+      if (best_cost < 0 || cost < best_cost) {
+        best_cost = cost;
+        best = k;
+      }
+    }
+    if (!pts.empty()) {
+      ac->r0 = pts[best];
+      ac->nrange = pts.back() - pts[best] + 1;
+      ac->nlist = (int)best;
+      if (best) ac->d_list = device_copy(pts.data(), best);
+    }
   }
   ac->d_rm = device_zeros<float>((size_t)mp->nglob);
   points_to_device(mp, ac->d_rm, ac->h_rm.data(), 1, "rmass_acoustic H2D");
@@ -525,6 +557,12 @@ void prepare_fields_acoustic_device_(long* Mesh_pointer, realw* rmass_acoustic, 
         ac->coup_w.push_back(coupling_ac_el_jacobian2Dw[q]);
       }
     }
+  if (!ac->coup_e.empty()) {
+    // first-touch assembly of the solid treats these points as special (cleared by the predictor, added to)
+    if (mp->sorted) fail("prepare_fields_acoustic_device must run before prepare_fields_elastic_device");
+    for (size_t k = 0; k < ac->coup_e.size(); k++)
+      mp->h_coupling_points.push_back(mp->h_ibool[(size_t)ac->coup_e[k] * NGLL2 + ac->coup_ij[k]]);
+  }
   check_hip(hipDeviceSynchronize(), "prepare_fields_acoustic_device");
 }
 
@@ -542,9 +580,11 @@ void update_displacement_ac_cuda_(long* Mesh_pointer, realw* deltat_F, realw* de
   Mesh* mp = handle(Mesh_pointer);
   if (*compute_b_wavefield) fail("update_displacement_ac_cuda: the backward fluid wavefield is not supported");
   AcousticState* ac = ac_ready(mp, "update_displacement_ac_cuda");
-  const int n = mp->nglob;
-  hipLaunchKernelGGL(k_ac_predictor, dim3(grid_for(n, 256, 65536)), dim3(256), 0, mp->stream, ac->d_pot, ac->d_pd,
-                     ac->d_pdd, n, *deltat_F, *deltatsqover2_F, *deltatover2_F);
+  const long m = (long)ac->nrange + ac->nlist;
+  if (m == 0) return;
+  hipLaunchKernelGGL(k_ac_predictor, dim3(grid_for(m, 256, 65536)), dim3(256), 0, mp->stream, ac->d_pot, ac->d_pd,
+                     ac->d_pdd, ac->r0, ac->nrange, ac->d_list, ac->nlist, *deltat_F, *deltatsqover2_F,
+                     *deltatover2_F);
   check_hip(hipGetLastError(), "k_ac_predictor launch");
 }
 
@@ -635,7 +675,6 @@ void compute_coupling_el_ac_cuda_(long* Mesh_pointer, int* iphasef, int* num_cou
   if (*num_coupling_ac_el_facesf * NGLL != ac->ncoup)
     fail("compute_coupling_el_ac_cuda: %d faces, %d prepared points", *num_coupling_ac_el_facesf, ac->ncoup);
   if (ac->ncoup == 0) return;
-  if (mp->first_touch) fail("compute_coupling_el_ac_cuda: first-touch assembly is on for a coupled mesh");
   hipLaunchKernelGGL(k_coupling_el_ac, dim3((ac->ncoup + 255) / 256), dim3(256), 0, mp->stream, ac->ncoup,
                      ac->d_coup, ac->d_coup_n, ac->d_coup_w, ac->d_pdd, mp->d_accel);
   check_hip(hipGetLastError(), "k_coupling_el_ac launch");
@@ -648,9 +687,10 @@ void kernel_3_acoustic_cuda_(long* Mesh_pointer, realw* deltatover2, realw* b_de
   require_forward(*compute_wavefield_2, 1, "kernel_3_acoustic_cuda");
   if (!*compute_wavefield_1) return;
   AcousticState* ac = ac_ready(mp, "kernel_3_acoustic_cuda");
-  const int n = mp->nglob;
-  hipLaunchKernelGGL(k_ac_corrector, dim3(grid_for(n, 256, 65536)), dim3(256), 0, mp->stream, ac->d_pd, ac->d_pdd,
-                     ac->d_rm, n, *deltatover2);
+  const long m = (long)ac->nrange + ac->nlist;
+  if (m == 0) return;
+  hipLaunchKernelGGL(k_ac_corrector, dim3(grid_for(m, 256, 65536)), dim3(256), 0, mp->stream, ac->d_pd, ac->d_pdd,
+                     ac->d_rm, ac->r0, ac->nrange, ac->d_list, ac->nlist, *deltatover2);
   check_hip(hipGetLastError(), "k_ac_corrector launch");
 }
 

@@ -138,6 +138,7 @@ struct Plan {
   // every patch is a chain: element k's right edge (i = 4) is element k+1's left edge (i = 0), and no other
   // point is shared inside the patch
   bool chains = false;
+  std::vector<int> uncovered;         // chains: points (caller numbering) of no element of the phase lists
   // chains: per device position and GLL point, whether the chain kernel's update of that point is the first of
   // the step (the earliest colour, in launch order, of the patches that contain the point)
   std::vector<uint8_t> first;
@@ -299,12 +300,10 @@ Plan build_plan(const std::vector<int>& ibool, int nspec, int nglob, const int* 
                 P.first[l] = earliest[g] == launch[q];
             }
       }
-    // first-touch assembly needs every point written by some force launch of the step
+    // points that no force launch writes (those of fluid elements only) join the special points, which the
+    // predictor clears
     for (int g = 0; g < nglob; g++)
-      if (earliest[g] == (1 << 30)) {
-        P.first.clear();
-        break;
-      }
+      if (earliest[g] == (1 << 30)) P.uncovered.push_back(g);
   }
   for (int e = 0; e < nspec; e++)
     if (!placed[e]) order.push_back(e);  // elements outside the elastic phase lists keep their relative order
@@ -330,6 +329,8 @@ void upload_chain_ids(Mesh* mp) {
     for (int p = 0; p < NGLL2; p++) special[mp->h_chain_ids[(size_t)pos * NGLL2 + p]] = 1;
   }
   for (int g : mp->h_stacey_points) special[g] = 1;
+  for (int g : mp->h_uncovered_points) special[g] = 1;
+  for (int g : mp->h_coupling_points) special[mp->point_id.empty() ? g : mp->point_id[g]] = 1;
   std::vector<int> ids(mp->h_chain_ids);
   std::vector<int> list;
   if (mp->first_touch) {
@@ -415,9 +416,8 @@ void sort_elements_by_colour(Mesh* mp, const int* phase_lists, int np) {
     if (mp->chain) {
       mp->h_chain_ids = std::move(ib);
       mp->h_first = std::move(P.first);
-      // coupled meshes keep the accumulating scheme: the fluid-solid terms add to the solid's accel
-      mp->first_touch = mp->simulation_type == 1 && !mp->h_first.empty() && mp->nspec_acoustic == 0 &&
-                        !getenv("SF2D_NO_FIRST_TOUCH");
+      for (int g : P.uncovered) mp->h_uncovered_points.push_back(mp->point_id.empty() ? g : mp->point_id[g]);
+      mp->first_touch = mp->simulation_type == 1 && !mp->h_first.empty() && !getenv("SF2D_NO_FIRST_TOUCH");
       upload_chain_ids(mp);
     }
   }

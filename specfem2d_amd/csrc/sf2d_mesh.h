@@ -100,6 +100,8 @@ struct Mesh {
   static constexpr int FIRST_TOUCH = 1 << 30;
   int first_touch = 0;
   std::vector<int> h_stacey_points;  // internal ids of the points of the absorbing edges
+  std::vector<int> h_uncovered_points;  // internal ids of the points of no solid element (fluid-only points)
+  std::vector<int> h_coupling_points;   // caller ids of the fluid-solid edge points (the fluid adds to accel there)
   int* d_special = nullptr;
   int nspecial = 0;
   int accel_partial = 0;
