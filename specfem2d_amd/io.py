@@ -1,0 +1,358 @@
+"""The reference's on-disk formats for this path: ``DATA/Par_file``, ``DATA/SOURCE``, the interfaces file and
+``DATA/STATIONS`` in; seismograms out in the reference's ASCII and binary layouts.
+
+Readers (``src/shared/param_reader.c``, ``read_parameter_file.F90``, ``read_material_table.f90``,
+``read_regions.f90``, ``src/meshfem2D/read_interfaces_file.f90``, ``read_source_file.f90``,
+``save_stations_file.f90``) build a :class:`~specfem2d_amd.config.SimulationConfig` for what this library runs:
+the internal mesher with flat interfaces (NGNOD = 4), isotropic material tables (solid, or fluid when vs = 0),
+Stacey or free boundaries, collocated force / pressure sources with time functions 1-3, Newmark, P-SV.
+Anything else in the files is refused with a message naming the parameter.
+
+Writers follow ``write_seismograms.F90:252-584``: ASCII traces ``NET.STA.CMP.semX`` with the gfortran
+list-directed layout of ``write(11,*) time, ' ', value`` (byte-compatible), and the binary
+``U{x,z,p}_file_{single,double}_X.bin`` streams ([receiver][sample], native endianness).
+"""
+from __future__ import annotations
+
+import math
+import os
+import re
+from dataclasses import dataclass
+from typing import Dict, List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from .config import NGLL2, SimulationConfig, Source
+
+
+# ----------------------------------------------------------------------------------------------------------
+# Par_file
+# ----------------------------------------------------------------------------------------------------------
+@dataclass
+class ParFile:
+    """``name = value`` pairs in file order (names repeat for receiver sets), plus the material table and the
+    regions, which follow ``nbmodels`` and ``nbregions`` as bare lines."""
+
+    entries: List[Tuple[str, str]]
+    materials: Dict[int, List[float]]  # model number -> [icodemat, val0 ... val12]
+    regions: List[Tuple[int, int, int, int, int]]  # ix_start, ix_end, iz_start, iz_end, model (1-based)
+
+    def get(self, name: str, default: Optional[str] = None) -> str:
+        for k, v in self.entries:
+            if k == name:
+                return v
+        if default is None:
+            raise KeyError(f"Par_file: parameter {name} is missing")
+        return default
+
+    def all(self, name: str) -> List[str]:
+        return [v for k, v in self.entries if k == name]
+
+
+def _strip_comment(line: str) -> str:
+    return line.split("#", 1)[0].strip()
+
+
+def fortran_float(s: str) -> float:
+    """A Fortran real literal: 1.d0, 2.5D-3, 4e-4, 3000."""
+    return float(s.strip().replace("d", "e").replace("D", "e"))
+
+
+def fortran_bool(s: str) -> bool:
+    t = s.strip().lower()
+    if t in (".true.", "true", "t", ".t."):
+        return True
+    if t in (".false.", "false", "f", ".f."):
+        return False
+    raise ValueError(f"not a Fortran logical: {s!r}")
+
+
+def read_par_file(path: str) -> ParFile:
+    entries: List[Tuple[str, str]] = []
+    materials: Dict[int, List[float]] = {}
+    regions: List[Tuple[int, int, int, int, int]] = []
+    pending_models = pending_regions = 0
+    with open(path) as f:
+        for raw in f:
+            line = _strip_comment(raw)
+            if not line:
+                continue
+            if "=" in line:
+                k, v = (x.strip() for x in line.split("=", 1))
+                entries.append((k, v))
+                if k == "nbmodels":
+                    pending_models = int(v)
+                elif k == "nbregions":
+                    pending_regions = int(v)
+                continue
+            toks = line.split()
+            if pending_models > 0:
+                vals = [fortran_float(t) for t in toks]
+                materials[int(vals[0])] = vals[1:]
+                pending_models -= 1
+            elif pending_regions > 0:
+                ix0, ix1, iz0, iz1, m = (int(t) for t in toks[:5])
+                regions.append((ix0, ix1, iz0, iz1, m))
+                pending_regions -= 1
+            else:
+                raise ValueError(f"{path}: unexpected line {raw.rstrip()!r}")
+    if pending_models or pending_regions:
+        raise ValueError(f"{path}: material table or regions incomplete")
+    return ParFile(entries, materials, regions)
+
+
+def read_interfaces(path: str) -> Tuple[List[float], List[int], List[Tuple[np.ndarray, np.ndarray]]]:
+    """Interfaces file of the internal mesher: returns (interface heights bottom first, elements per layer,
+    the (x, z) points of each interface).  Only flat interfaces are supported by this library's mesher."""
+    toks: List[str] = []
+    with open(path) as f:
+        for raw in f:
+            line = _strip_comment(raw)
+            if line:
+                toks.extend(line.split())
+    pos = 0
+
+    def nxt():
+        nonlocal pos
+        pos += 1
+        return toks[pos - 1]
+
+    nint = int(nxt())
+    curves = []
+    for _ in range(nint):
+        npts = int(nxt())
+        xs = np.array([fortran_float(nxt()) for _ in range(2 * npts)]).reshape(npts, 2)
+        curves.append((xs[:, 0], xs[:, 1]))
+    nz_layers = [int(nxt()) for _ in range(nint - 1)]
+    heights = []
+    for k, (x, z) in enumerate(curves):
+        if np.ptp(z) != 0.0:
+            raise NotImplementedError(f"{path}: interface {k + 1} is not flat (topography is not supported)")
+        heights.append(float(z[0]))
+    return heights, nz_layers, curves
+
+
+def read_source_file(path: str, nsources: int) -> List[Source]:
+    """``DATA/SOURCE``: ``nsources`` blocks of ``name = value`` lines."""
+    blocks: List[Dict[str, str]] = []
+    cur: Dict[str, str] = {}
+    with open(path) as f:
+        for raw in f:
+            line = _strip_comment(raw)
+            if not line or "=" not in line:
+                continue
+            k, v = (x.strip() for x in line.split("=", 1))
+            if k == "source_surf" and cur:
+                blocks.append(cur)
+                cur = {}
+            cur[k] = v
+    if cur:
+        blocks.append(cur)
+    if len(blocks) < nsources:
+        raise ValueError(f"{path}: {len(blocks)} sources for NSOURCES = {nsources}")
+    out = []
+    for b in blocks[:nsources]:
+        if fortran_bool(b.get("source_surf", ".false.")):
+            raise NotImplementedError("SOURCE: source_surf = .true. is not supported")
+        st = int(b["source_type"])
+        if st != 1:
+            raise NotImplementedError(f"SOURCE: source_type {st} (only collocated force / pressure sources)")
+        tf = int(b["time_function_type"])
+        if tf not in (1, 2, 3):
+            raise NotImplementedError(f"SOURCE: time_function_type {tf} (only 1, 2, 3)")
+        out.append(Source(x=fortran_float(b["xs"]), z=fortran_float(b["zs"]), f0=fortran_float(b["f0"]),
+                          factor=fortran_float(b["factor"]), anglesource=fortran_float(b.get("anglesource", "0")),
+                          tshift=fortran_float(b.get("tshift", "0")), time_function_type=tf))
+    return out
+
+
+@dataclass
+class Station:
+    name: str
+    network: str
+    x: float
+    z: float
+
+
+def stations_from_receiver_sets(par: ParFile) -> List[Station]:
+    """The receiver lines of the Par_file as ``xmeshfem2D`` writes them to ``DATA/STATIONS``
+    (save_stations_file.f90:93-110): evenly spaced along each line, positions rounded to 7 decimals (f20.7)."""
+    nsets = int(par.get("nreceiversets"))
+    nrecs, xd, zd, xf, zf = (par.all(k) for k in ("nrec", "xdeb", "zdeb", "xfin", "zfin"))
+    surf = par.all("record_at_surface_same_vertical")
+    if min(len(nrecs), len(xd), len(zd), len(xf), len(zf)) < nsets:
+        raise ValueError("Par_file: receiver sets incomplete")
+    out: List[Station] = []
+    k = 0
+    for s in range(nsets):
+        if s < len(surf) and fortran_bool(surf[s]):
+            raise NotImplementedError("record_at_surface_same_vertical = .true. is not supported")
+        n = int(nrecs[s])
+        x0, z0, x1, z1 = (fortran_float(v[s]) for v in (xd, zd, xf, zf))
+        for i in range(1, n + 1):
+            if n > 1:
+                x = x0 + float(i - 1) * (x1 - x0) / float(n - 1)
+                z = z0 + float(i - 1) * (z1 - z0) / float(n - 1)
+            else:
+                x, z = x0, z0
+            k += 1
+            out.append(Station(f"S{k:04d}", "AA", float(f"{x:20.7f}"), float(f"{z:20.7f}")))
+    return out
+
+
+def read_stations(path: str) -> List[Station]:
+    """``DATA/STATIONS``: ``station network x z elevation burial`` per line."""
+    out = []
+    with open(path) as f:
+        for raw in f:
+            t = raw.split()
+            if len(t) >= 4:
+                out.append(Station(t[0], t[1], fortran_float(t[2]), fortran_float(t[3])))
+    return out
+
+
+def config_from_par_file(data_dir: str, cpml: str = "error", nstep: Optional[int] = None) -> Tuple[SimulationConfig, List[Station]]:
+    """Build the configuration of ``data_dir/Par_file`` (+ its SOURCE, interfaces file and STATIONS).
+
+    ``cpml``: this path has no CPML; ``"error"`` refuses a Par_file that asks for it, ``"stacey"`` puts Stacey
+    edges on the absorbing sides instead (same results until the waves reach the absorbing layer)."""
+    par = read_par_file(os.path.join(data_dir, "Par_file"))
+    g = par.get
+
+    def refuse(cond, what):
+        if cond:
+            raise NotImplementedError(f"Par_file: {what} is not supported by this library")
+
+    refuse(int(g("NGNOD")) != 4, "NGNOD = 9")
+    refuse(not fortran_bool(g("P_SV")), "P_SV = .false. (SH waves)")
+    refuse(fortran_bool(g("AXISYM", ".false.")), "AXISYM")
+    refuse(int(g("time_stepping_scheme", "1")) != 1, "time_stepping_scheme other than Newmark")
+    refuse(g("MODEL", "default").strip().lower() != "default", f"MODEL = {g('MODEL')}")
+    refuse(fortran_bool(g("read_external_mesh", ".false.")), "read_external_mesh (external meshes)")
+    refuse(fortran_bool(g("ATTENUATION_VISCOELASTIC", ".false.")), "ATTENUATION_VISCOELASTIC")
+    refuse(fortran_bool(g("ATTENUATION_VISCOACOUSTIC", ".false.")), "ATTENUATION_VISCOACOUSTIC")
+    refuse(fortran_bool(g("initialfield", ".false.")), "initialfield (plane-wave initial fields)")
+    refuse(fortran_bool(g("ADD_PERIODIC_CONDITIONS", ".false.")), "ADD_PERIODIC_CONDITIONS")
+    refuse(fortran_bool(g("ACOUSTIC_FORCING", ".false.")), "ACOUSTIC_FORCING")
+    refuse(int(g("NTSTEP_BETWEEN_OUTPUT_SAMPLE", "1")) != 1, "NTSTEP_BETWEEN_OUTPUT_SAMPLE > 1")
+    pml = fortran_bool(g("PML_BOUNDARY_CONDITIONS", ".false."))
+    stacey = fortran_bool(g("STACEY_ABSORBING_CONDITIONS", ".false."))
+    if pml:
+        if cpml != "stacey":
+            raise NotImplementedError("Par_file: PML_BOUNDARY_CONDITIONS (CPML) is not on this path; pass "
+                                      "cpml='stacey' to use Stacey edges on the absorbing sides instead")
+        stacey = True
+    seismotypes = [int(t) for t in re.split(r"[ ,]+", g("seismotype").strip()) if t]
+    refuse(len(seismotypes) != 1 or seismotypes[0] not in (1, 2, 3, 4), f"seismotype = {g('seismotype')}")
+    heights, nz_layers, _ = read_interfaces(os.path.join(data_dir, os.path.basename(g("interfacesfile"))))
+    nx = int(g("nx"))
+    nz = sum(nz_layers)
+    xmin, xmax = fortran_float(g("xmin")), fortran_float(g("xmax"))
+    # materials of the elements (read_regions.f90: later regions overwrite earlier ones), per GLL point
+    mat = np.zeros(nx * nz, np.int64)
+    for ix0, ix1, iz0, iz1, m in par.regions:
+        if m not in par.materials:
+            raise ValueError(f"Par_file: region uses undefined material {m}")
+        for iz in range(iz0, iz1 + 1):
+            mat[(iz - 1) * nx + ix0 - 1:(iz - 1) * nx + ix1] = m
+    if np.any(mat == 0):
+        raise ValueError("Par_file: regions leave elements without a material")
+    rho = np.zeros(nx * nz)
+    vp = np.zeros(nx * nz)
+    vs = np.zeros(nx * nz)
+    for m, vals in par.materials.items():
+        icode = int(vals[0])
+        refuse(icode != 1, f"material {m} with icodemat {icode} (only isotropic tables)")
+        sel = mat == m
+        rho[sel], vp[sel], vs[sel] = vals[1], vals[2], vals[3]
+    rep = lambda a: np.repeat(a, NGLL2)  # noqa: E731
+    nsrc = int(g("NSOURCES"))
+    sources = read_source_file(os.path.join(data_dir, "SOURCE"), nsrc)
+    if fortran_bool(g("use_existing_STATIONS", ".false.")):
+        stations = read_stations(os.path.join(data_dir, "STATIONS"))
+    else:
+        stations = stations_from_receiver_sets(par)
+    cfg = SimulationConfig(
+        name=g("title", "Par_file").strip(), nx=nx, nz=nz, xmin=xmin, xmax=xmax, interfaces=heights,
+        nz_layers=nz_layers, nstep=nstep or int(g("NSTEP")), dt=fortran_float(g("DT")), rho=rep(rho), vp=rep(vp),
+        vs=rep(vs), material_mode=0, stacey=stacey, absorbbottom=fortran_bool(g("absorbbottom")),
+        absorbright=fortran_bool(g("absorbright")), absorbtop=fortran_bool(g("absorbtop")),
+        absorbleft=fortran_bool(g("absorbleft")), sources=sources,
+        receivers=np.array([[s.x, s.z] for s in stations]) if stations else None,
+        user_t0=fortran_float(g("USER_T0", "0")), anglerec=fortran_float(g("anglerec", "0")),
+        seismotype=seismotypes[0], use_trick=fortran_bool(g("USE_TRICK_FOR_BETTER_PRESSURE", ".false.")))
+    return cfg, stations
+
+
+# ----------------------------------------------------------------------------------------------------------
+# seismograms out
+# ----------------------------------------------------------------------------------------------------------
+def gfortran_real8(x: float) -> str:
+    """One REAL(8) item of gfortran list-directed output: 17 significant digits in a 25-character field, F form
+    for 0.1 <= |x| < 1e17 (the exponent's 5 columns left blank), E form with a 3-digit exponent otherwise."""
+    if x != x or math.isinf(x):
+        body = "NaN" if x != x else "Infinity"
+        return f"{('-' if x < 0 else '') + body:>25}"
+    sign = "-" if (x < 0 or (x == 0.0 and math.copysign(1.0, x) < 0)) else " "
+    a = abs(x)
+    if a == 0.0:
+        body = "0." + "0" * 16 + " " * 5
+    else:
+        e = f"{a:.16E}"  # mantissa with 17 significant digits, rounded
+        mant, exp = e.split("E")
+        ex = int(exp)
+        if -1 <= ex < 17:  # 0.1 <= |x| (after rounding) < 1e17: F form with 17 significant digits
+            digits = 16 - ex  # decimals
+            body = f"{a:.{digits}f}" + " " * 5
+        else:
+            body = f"{mant}E{'-' if ex < 0 else '+'}{abs(ex):03d}"
+    return f"{sign + body:>25}"
+
+
+def ascii_trace_lines(times: Sequence[float], values: Sequence[float]) -> str:
+    """The text of one ASCII trace as ``write(11,*) time_t,' ',value`` writes it (write_seismograms.F90:521)."""
+    return "".join(" " + gfortran_real8(float(t)) + "   " + gfortran_real8(float(v)) + "\n"
+                   for t, v in zip(times, values))
+
+
+_CHANNEL_SUFFIX = {1: "d", 2: "v", 3: "a", 4: "p"}
+
+
+def write_ascii_seismograms(out_dir: str, seis: np.ndarray, stations: Sequence[Station], dt: float, t0: float,
+                            seismotype: int = 1) -> List[str]:
+    """``seis`` [nrec, 2, nstep] (component 0 only for pressure); writes ``NET.STA.BX{X,Z}.semX`` (or
+    ``.PRE.semp``) with time = (it - 1) dt - t0; returns the paths."""
+    os.makedirs(out_dir, exist_ok=True)
+    nstep = seis.shape[-1]
+    times = [float(i) * dt - t0 for i in range(nstep)]
+    comp = _CHANNEL_SUFFIX[seismotype]
+    channels = ["PRE"] if seismotype == 4 else ["BXX", "BXZ"]
+    paths = []
+    for r, st in enumerate(stations):
+        for c, ch in enumerate(channels):
+            p = os.path.join(out_dir, f"{st.network}.{st.name}.{ch}.sem{comp}")
+            with open(p, "w") as f:
+                f.write(ascii_trace_lines(times, seis[r, c]))
+            paths.append(p)
+    return paths
+
+
+def write_binary_seismograms(out_dir: str, seis: np.ndarray, seismotype: int = 1, double: bool = False) -> List[str]:
+    """``U{x,z}_file_{single,double}_X.bin`` (``Up_`` for pressure): [receiver][sample] streams
+    (write_seismograms.F90:526-560)."""
+    os.makedirs(out_dir, exist_ok=True)
+    comp = _CHANNEL_SUFFIX[seismotype]
+    kind = "double" if double else "single"
+    dtype = np.float64 if double else np.float32
+    names = [("Up", 0)] if seismotype == 4 else [("Ux", 0), ("Uz", 1)]
+    paths = []
+    for pre, c in names:
+        p = os.path.join(out_dir, f"{pre}_file_{kind}_{comp}.bin")
+        np.ascontiguousarray(seis[:, c, :], dtype=dtype).tofile(p)
+        paths.append(p)
+    return paths
+
+
+def read_ascii_trace(path: str) -> Tuple[np.ndarray, np.ndarray]:
+    t, v = np.loadtxt(path, unpack=True)
+    return t, v

@@ -36,8 +36,50 @@ def pack_pressure(example, sub, station, out):
     print("wrote", out, t.shape)
 
 
+def copy_inputs(example, files, out):
+    """The example's input files (Par_file, SOURCE, interfaces: plain-text data) for the on-disk format tests."""
+    import shutil
+    dst = os.path.join(HERE, "examples", out)
+    os.makedirs(dst, exist_ok=True)
+    for rel in files:
+        shutil.copyfile(os.path.join(REF, "EXAMPLES", example, rel), os.path.join(dst, os.path.basename(rel)))
+    print("copied", files, "->", dst)
+
+
+def trace_digests(out):
+    """sha256 of one ASCII trace per example and recording kind the reference keeps under EXAMPLES/*/REF_SEIS,
+    with its parsed values, so the writer's byte compatibility is checked without copying the files."""
+    import glob
+    import hashlib
+    arrays, names = {}, []
+    seen = set()
+    for path in sorted(glob.glob(os.path.join(REF, "EXAMPLES", "*", "REF_SEIS", "*.sem?"))):
+        kind = (path.split(os.sep)[-3], path[-1])  # one trace per example and recording kind
+        if kind in seen:
+            continue
+        seen.add(kind)
+        try:
+            t, v = read_trace(path)
+        except ValueError:
+            continue
+        if np.ndim(t) == 0:
+            continue
+        k = len(names)
+        names.append(os.path.relpath(path, os.path.join(REF, "EXAMPLES")))
+        arrays[f"t{k}"], arrays[f"v{k}"] = t, v
+        arrays[f"sha{k}"] = np.frombuffer(hashlib.sha256(open(path, "rb").read()).digest(), np.uint8)
+    arrays["names"] = np.array(names)
+    np.savez_compressed(os.path.join(HERE, out), **arrays)
+    print("wrote", out, len(names), "traces")
+
+
 if __name__ == "__main__":
     pack("Tromp2005_kernel", "REF_SEIS", ["AA.S0001"], "tromp2005_kernel_ref_seis.npz")
     pack_pressure("check_absolute_amplitude_of_pressure_source_seismograms_acoustic", "REF_SEIS", "AA.S0001",
                   "pressure_source_acoustic_ref_seis.npz")
+    copy_inputs("Tromp2005_kernel", ["DATA/Par_file", "DATA/SOURCE", "DATA/interfaces_Tromp2005.dat"], "Tromp2005_kernel")
+    copy_inputs("check_absolute_amplitude_of_pressure_source_seismograms_acoustic",
+                ["REF_SEIS/Par_file", "REF_SEIS/SOURCE", "DATA/interfaces_attenuation_analytic.dat"],
+                "pressure_source_acoustic")
+    trace_digests("ref_seis_ascii_digests.npz")
     pack("Tromp2005_kernel", "REF_KERNEL", ["AA.S0001"], "tromp2005_kernel_ref_kernel_traces.npz")
