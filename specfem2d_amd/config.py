@@ -297,19 +297,40 @@ def layered_fluid_solid(nx: int, nz: int, nz_fluid: int, elem_size: float, nstep
     return cfg.replace(rho=rho, vp=vp, vs=vs)
 
 
-def fluid_solid_plane_wave(nstep: int = 1700, solid: bool = True, absorbtop: bool = True) -> SimulationConfig:
-    """Plane-wave reflection at a flat fluid-solid interface: 120 x 80 elements of 50 m (6 km x 4 km), the top
-    40 rows fluid (rho 1020, vp 1500), the bottom 40 solid (rho 2500, vp 3000, vs 1732; fluid too when not
-    ``solid``), a line of 480 Ricker pressure sources of 5 Hz 1000 m above the interface, pressure receivers
-    500 m above it (and one 500 m above the source line), dt 1e-3, USE_TRICK_FOR_BETTER_PRESSURE."""
-    depth = 4000.0
-    srcs = [Source(x=float(x), z=-1000.0, f0=5.0, factor=1.0, time_function_type=1)
-            for x in np.arange(6.25, 6000.0, 12.5)]
-    rec = np.array([[2500.0, -1500.0], [3000.0, -1500.0], [3500.0, -1500.0], [3000.0, -500.0]])
-    cfg = layered_fluid_solid(120, 80, 40 if solid else 80, 50.0, nstep, 1.0e-3, absorbtop=absorbtop,
-                              name="fluid_solid_plane_wave", sources=srcs, receivers=rec, seismotype=4)
-    assert cfg.interfaces[0] == -depth
-    return cfg.replace(use_trick=True)
+def fluid_solid_plane_wave(nstep: int = 1700, solid: bool = True, absorbtop: bool = True,
+                           orientation: str = "above") -> SimulationConfig:
+    """Plane-wave reflection at a flat fluid-solid interface, 50 m elements, fluid rho 1020, vp 1500 and solid
+    rho 2500, vp 3000, vs 1732 (fluid too when not ``solid``), Ricker pressure sources of 5 Hz every 12.5 m along a
+    line 1000 m from the interface in the fluid, pressure receivers 500 m from the interface (three) and 500 m
+    beyond the source line (one), dt 1e-3, USE_TRICK_FOR_BETTER_PRESSURE.
+
+    ``orientation``: ``"above"`` -- 120 x 80 elements (6 km x 4 km), the fluid the top 40 rows (the free surface
+    on top unless ``absorbtop``); ``"below"`` -- the fluid the bottom 40 rows under the solid; ``"beside"`` --
+    80 x 120 elements, the fluid the right 40 columns beside the solid.  Stacey edges elsewhere."""
+    h = 50.0
+    if orientation == "beside":
+        nx, nz = 80, 120
+        fluid = np.tile(np.arange(nx), nz) >= nx // 2
+        depth = nz * h
+        srcs = [Source(x=3000.0, z=float(z), f0=5.0, factor=1.0) for z in np.arange(-depth + 6.25, 0.0, 12.5)]
+        rec = np.array([[2500.0, -3000.0], [2500.0, -2500.0], [2500.0, -3500.0], [3500.0, -3000.0]])
+    else:
+        nx, nz = 120, 80
+        ez = np.repeat(np.arange(nz), nx)
+        fluid = ez >= nz // 2 if orientation == "above" else ez < nz // 2
+        z_src, z_rec, z_far = (-1000.0, -1500.0, -500.0) if orientation == "above" else (-3000.0, -2500.0, -3500.0)
+        srcs = [Source(x=float(x), z=z_src, f0=5.0, factor=1.0) for x in np.arange(6.25, nx * h, 12.5)]
+        rec = np.array([[2500.0, z_rec], [3000.0, z_rec], [3500.0, z_rec], [3000.0, z_far]])
+    if not solid:
+        fluid = np.ones_like(fluid)
+    f25 = np.repeat(fluid, NGLL2)
+    cfg = SimulationConfig(
+        name=f"fluid_solid_plane_wave_{orientation}", nx=nx, nz=nz, xmin=0.0, xmax=nx * h,
+        interfaces=[-nz * h, 0.0], nz_layers=[nz], nstep=nstep, dt=1.0e-3, material_mode=1, stacey=True,
+        absorbbottom=True, absorbright=True, absorbtop=absorbtop or orientation != "above", absorbleft=True,
+        sources=srcs, receivers=rec, seismotype=4, use_trick=True,
+        rho=np.where(f25, 1020.0, 2500.0), vp=np.where(f25, 1500.0, 3000.0), vs=np.where(f25, 0.0, 1732.0))
+    return cfg
 
 
 def c2_mesh_coupled(nstep: int = 3000, nx: int = 2000, nz: int = 1000, elem_size: float = 10.0,

@@ -105,8 +105,9 @@ def test_coupled_fluid_solid_against_oracle(seismotype):
     sim.close()
 
 
-def test_plane_wave_reflection_coefficient():
-    cfg = config.fluid_solid_plane_wave()
+@pytest.mark.parametrize("orientation", ["above", "below", "beside"])
+def test_plane_wave_reflection_coefficient(orientation):
+    cfg = config.fluid_solid_plane_wave(orientation=orientation)
     sim = Simulation(cfg)
     seis, t = sim.forward()
     ref = OracleSim(cfg, "sp").forward()
@@ -118,7 +119,7 @@ def test_plane_wave_reflection_coefficient():
     direct = d[np.argmax(np.abs(d))]
     m = (tt > 0.8) & (tt < 1.3)
     refl = p[m][np.argmax(np.abs(p[m]))]
-    print(f"plane wave at a fluid-solid interface: R = {refl / direct:.5f} (exact {r_exact:.5f}); rel-L2 vs FP32 "
+    print(f"plane wave, fluid {orientation} the solid: R = {refl / direct:.5f} (exact {r_exact:.5f}); rel-L2 vs FP32 "
           f"oracle {rel(seis, ref):.3e}; loop {t['loop_ms']:.1f} ms")
     assert refl / direct == pytest.approx(r_exact, rel=2e-3)
     assert rel(seis, ref) < TOL

@@ -11,8 +11,8 @@ What pins each part (oracle/sem2d_oracle.c, acoustic routines):
 * pressure without the trick (-potential_dot_dot with the Gaussian itself): the same trace, since the two
   recordings are the same continuous quantity (mathematical equivalence, discretisation-level tolerance);
 * fluid-solid coupling, both directions: the normal-incidence plane-wave reflection coefficient
-  (Z2 - Z1) / (Z2 + Z1) of a flat fluid-solid interface (a one-way or sign-flipped coupling gives +1 or
-  worse);
+  (Z2 - Z1) / (Z2 + Z1) of a flat fluid-solid interface with the fluid above, below and beside the solid (a
+  one-way or sign-flipped coupling gives +1 or worse);
 * acoustic free surface: reflection coefficient -1 for pressure;
 * acoustic Stacey edges: reflection below 1e-3 at normal incidence;
 * acoustic operator: symmetric negative semi-definite, and zero on a constant potential (brute force on a
@@ -79,15 +79,29 @@ def plane_wave():
     return cfg, o, o.forward()
 
 
-def test_fluid_solid_reflection_coefficient(plane_wave):
-    cfg, o, s = plane_wave
-    assert o.ncoupling == cfg.nx
+def _check_reflection(cfg, o, s):
     t = np.arange(cfg.nstep) * cfg.dt - o.t0
     z1, z2 = 1020.0 * 1500.0, 2500.0 * 3000.0
     r_exact = (z2 - z1) / (z2 + z1)
     for r in range(3):
         direct, refl = _direct_and_later(s[r, 0], t, 0.7, (0.8, 1.3))
         assert refl / direct == pytest.approx(r_exact, rel=2e-3)
+
+
+def test_fluid_solid_reflection_coefficient(plane_wave):
+    cfg, o, s = plane_wave
+    assert o.ncoupling == cfg.nx
+    _check_reflection(cfg, o, s)
+
+
+@pytest.mark.parametrize("orientation", ["beside", "below"])
+def test_fluid_solid_reflection_other_orientations(orientation):
+    """Fluid-solid edges on the fluid elements' left and top sides (the solid to the left of or above the fluid)."""
+    cfg = config.fluid_solid_plane_wave(orientation=orientation)
+    o = OracleSim(cfg, "sp")
+    cp = o.coupling()
+    assert len(cp) == 120 and set(cp[:, 1]) == ({4} if orientation == "beside" else {3})
+    _check_reflection(cfg, o, o.forward())
 
 
 def test_acoustic_stacey_absorbs(plane_wave):
