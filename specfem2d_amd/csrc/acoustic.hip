@@ -19,6 +19,8 @@
 // them.  Unsupported cases (backward or adjoint fluid wavefields, attenuation, MPI interfaces, SAVE_FORWARD
 // storage of the fluid's Stacey terms) stop the run with a message, as the reference does for its own limits.
 #include <algorithm>
+#include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -33,9 +35,13 @@ constexpr int AC_EPB = 10;  // fluid elements per 256-thread workgroup (250 acti
 __device__ __forceinline__ void atomic_add(float* p, float v) { unsafeAtomicAdd(p, v); }
 
 // One colour of fluid elements: pdd -= K pot over elements first .. first+n-1 of the packed colour order
-// (compute_forces_viscoacoustic.f90:107-330; Kernel_2_acoustic_impl in src/gpu/kernels).
+// (compute_forces_viscoacoustic.f90:107-330; Kernel_2_acoustic_impl in src/gpu/kernels).  RECT: every element is
+// an axis-aligned rectangle (xi_z = gamma_x = 0, xi_x and gamma_z constant over the element), whose geometry is one
+// float2 {xi_x, gamma_z} per element instead of a float4 per point.
+template <bool RECT>
 __global__ __launch_bounds__(256) void k_acoustic_forces(int first, int n, const int* __restrict__ ids,
                                                          const float4* __restrict__ geom,
+                                                         const float2* __restrict__ geom_rect,
                                                          const float* __restrict__ fac, Basis b,
                                                          const float* __restrict__ pot, float* __restrict__ pdd) {
   __shared__ float s_u[AC_EPB][NGLL2];
@@ -53,7 +59,12 @@ __global__ __launch_bounds__(256) void k_acoustic_forces(int first, int n, const
   if (active) {
     const size_t l = (size_t)e * NGLL2 + p;
     g = ids[l];
-    gm = geom[l];
+    if (RECT) {
+      const float2 r = geom_rect[e];
+      gm = make_float4(r.x, 0.f, 0.f, r.y);
+    } else {
+      gm = geom[l];
+    }
     f = fac[l];
     s_u[le][p] = pot[g];
   }
@@ -66,10 +77,15 @@ __global__ __launch_bounds__(256) void k_acoustic_forces(int first, int n, const
     const float dga = u[0 * NGLL + i] * b.hp[j * NGLL + 0] + u[1 * NGLL + i] * b.hp[j * NGLL + 1] +
                       u[2 * NGLL + i] * b.hp[j * NGLL + 2] + u[3 * NGLL + i] * b.hp[j * NGLL + 3] +
                       u[4 * NGLL + i] * b.hp[j * NGLL + 4];
-    const float dxl = dxi * gm.x + dga * gm.z;  // d/dx: xix, gammax
-    const float dzl = dxi * gm.y + dga * gm.w;  // d/dz: xiz, gammaz
-    s_t1[le][p] = f * (gm.x * dxl + gm.y * dzl);
-    s_t2[le][p] = f * (gm.z * dxl + gm.w * dzl);
+    if (RECT) {
+      s_t1[le][p] = f * (gm.x * (dxi * gm.x));
+      s_t2[le][p] = f * (gm.w * (dga * gm.w));
+    } else {
+      const float dxl = dxi * gm.x + dga * gm.z;  // d/dx: xix, gammax
+      const float dzl = dxi * gm.y + dga * gm.w;  // d/dz: xiz, gammaz
+      s_t1[le][p] = f * (gm.x * dxl + gm.y * dzl);
+      s_t2[le][p] = f * (gm.z * dxl + gm.w * dzl);
+    }
   }
   __syncthreads();
   if (active) {
@@ -235,6 +251,8 @@ struct AcousticState {
   int* d_list = nullptr;
   int* d_ids = nullptr;      // [nelem][25] point ids, colour order
   float4* d_geom = nullptr;  // [nelem][25]
+  float2* d_geom_rect = nullptr;  // [nelem] {xi_x, gamma_z} when every fluid element is an axis-aligned rectangle
+  int rect = 0;
   float* d_fac = nullptr;    // [nelem][25] J / rho
   int nfree = 0;
   int* d_free = nullptr;
@@ -256,7 +274,7 @@ void acoustic_release(Mesh* mp) {
   AcousticState* ac = mp->ac;
   if (!ac) return;
   device_free(ac->d_pot); device_free(ac->d_pd); device_free(ac->d_pdd); device_free(ac->d_rm);
-  device_free(ac->d_list); device_free(ac->d_ids); device_free(ac->d_geom); device_free(ac->d_fac); device_free(ac->d_free);
+  device_free(ac->d_list); device_free(ac->d_ids); device_free(ac->d_geom); device_free(ac->d_geom_rect); device_free(ac->d_fac); device_free(ac->d_free);
   device_free(ac->d_coup); device_free(ac->d_coup_n); device_free(ac->d_coup_w);
   device_free(ac->d_st); device_free(ac->d_st_w); device_free(ac->d_st_rvp);
   device_free(ac->d_src); device_free(ac->d_src_pts); device_free(ac->d_src_sa); device_free(ac->d_src_kap);
@@ -365,6 +383,23 @@ AcousticState* ac_ready(Mesh* mp, const char* who) {
     ac->d_ids = device_copy(ids.data(), n);
     ac->d_geom = device_copy(geom.data(), n);
     ac->d_fac = device_copy(fac.data(), n);
+    // rectangles: xi_z, gamma_x zero and xi_x, gamma_z constant over each element, to 2^-22 of the largest entry
+    // (meshers leave cancellation residues) -- SF2D_NO_RECT keeps the per-point geometry
+    bool rect = !getenv("SF2D_NO_RECT") && !order.empty();
+    std::vector<float2> gr(order.size());
+    for (size_t q = 0; q < order.size() && rect; q++) {
+      const float4* g4 = geom.data() + q * NGLL2;
+      float m = 0.f;
+      for (int p = 0; p < NGLL2; p++)
+        m = std::max({m, std::fabs(g4[p].x), std::fabs(g4[p].y), std::fabs(g4[p].z), std::fabs(g4[p].w)});
+      const float tol = m * 0x1p-22f;
+      for (int p = 0; p < NGLL2 && rect; p++)
+        rect = std::fabs(g4[p].y) <= tol && std::fabs(g4[p].z) <= tol && std::fabs(g4[p].x - g4[0].x) <= tol &&
+               std::fabs(g4[p].w - g4[0].w) <= tol;
+      gr[q] = make_float2(g4[0].x, g4[0].w);
+    }
+    ac->rect = rect;
+    if (rect) ac->d_geom_rect = device_copy(gr.data(), gr.size());
     // the Newmark updates visit the fluid's points only (the potential is zero elsewhere): the solid's numbering
     // puts the points of fluid elements alone after all others, so one id range holds them and a short list the
     // ones on fluid-solid edges; the split minimises range length + 3 x list length
@@ -602,8 +637,9 @@ void compute_forces_acoustic_cuda_(long* Mesh_pointer, int* iphase, int* nspec_o
   for (int c = 0; c < ac->ncolour[ph]; c++) {
     const int n = ac->colour_count[ph][c];
     if (n == 0) continue;
-    hipLaunchKernelGGL(k_acoustic_forces, dim3((n + AC_EPB - 1) / AC_EPB), dim3(256), 0, mp->stream,
-                       ac->colour_start[ph][c], n, ac->d_ids, ac->d_geom, ac->d_fac, mp->basis, ac->d_pot, ac->d_pdd);
+    hipLaunchKernelGGL(ac->rect ? k_acoustic_forces<true> : k_acoustic_forces<false>,
+                       dim3((n + AC_EPB - 1) / AC_EPB), dim3(256), 0, mp->stream, ac->colour_start[ph][c], n,
+                       ac->d_ids, ac->d_geom, ac->d_geom_rect, ac->d_fac, mp->basis, ac->d_pot, ac->d_pdd);
     check_hip(hipGetLastError(), "k_acoustic_forces launch");
   }
 }

@@ -154,3 +154,19 @@ def test_pressure_example_from_its_input_files(golden_dir, tmp_path):
     print(f"pressure example from its input files: max error vs REF_SEIS before 0.35 s {err:.2e} of the peak")
     assert err < 1e-5
     sim.close()
+
+
+def test_fluid_rectangle_form_matches_per_point_geometry(monkeypatch):
+    """Axis-aligned rectangular fluid elements read one {xi_x, gamma_z} record per element; the per-point form
+    (SF2D_NO_RECT) gives the same traces to FP32 rounding."""
+    cfg = _coupled_case(4).replace(nstep=500)
+    sim = Simulation(cfg)
+    seis, _ = sim.forward()
+    sim.close()
+    monkeypatch.setenv("SF2D_NO_RECT", "1")
+    sim2 = Simulation(cfg)
+    seis2, _ = sim2.forward()
+    sim2.close()
+    r = rel(seis, seis2)
+    print(f"fluid rectangle form vs per-point geometry: rel-L2 {r:.2e}")
+    assert r < TOL
