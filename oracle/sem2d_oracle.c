@@ -49,7 +49,7 @@ typedef struct {
   double user_t0;             /* USER_T0 (0 = automatic) */
   double anglerec_deg;        /* anglerec */
   int material_mode;          /* 0: Par_file material table, 1: MODEL=gll */
-  int seismotype;             /* 1 displacement (elastic receivers), 4 pressure (acoustic receivers) */
+  int seismotype;             /* 1 displacement (elastic receivers), 4 pressure, 6 potential (acoustic receivers) */
   int use_trick;              /* USE_TRICK_FOR_BETTER_PRESSURE */
 } orc_params;
 
@@ -1050,10 +1050,12 @@ static void enforce_free_surface(orc_sim *s) {
 
 static void record_seismograms(orc_sim *s, const REAL *displ, double *seis, int it, int nstep) {
   /* write_seismograms.F90:60-200, compute_interpolated_dva.f90 */
-  if (s->p.seismotype == 4) {
+  if (s->p.seismotype == 4 || s->p.seismotype == 6) {
     /* pressure in fluid elements: -pot with USE_TRICK_FOR_BETTER_PRESSURE, else -pot_dd
-       (compute_pressure.f90:460-490); solid receivers record zero here */
-    const REAL *f = s->p.use_trick ? s->pot : s->pot_dd;
+       (compute_pressure.f90:460-490); seismotype 6, the potential itself (write_seismograms.F90:147-170);
+       solid receivers record zero here */
+    const int potential = s->p.seismotype == 6;
+    const REAL *f = potential ? s->pot : s->p.use_trick ? s->pot : s->pot_dd;
     for (int r = 0; r < s->nrec; r++) {
       int e = s->rec_ispec[r] - 1;
       double vx = 0.0;
@@ -1062,7 +1064,7 @@ static void record_seismograms(orc_sim *s, const REAL *displ, double *seis, int 
           for (int i = 0; i < NGLL; i++) {
             int g = s->ibool[e * NGLL2 + j * NGLL + i] - 1;
             double hl = (double)(s->xir[r * NGLL + i] * s->gammar[r * NGLL + j]);
-            vx = vx + (double)(-f[g]) * hl;
+            vx = vx + (double)(potential ? f[g] : -f[g]) * hl;
           }
       seis[((size_t)r * 2 + 0) * nstep + it] = vx;
       seis[((size_t)r * 2 + 1) * nstep + it] = 0.0;

@@ -59,7 +59,7 @@ class SimulationConfig:
     receivers: Optional[np.ndarray] = None  # [nrec, 2] (x, z)
     user_t0: float = 0.0
     anglerec: float = 0.0
-    seismotype: int = 1  # 1 displacement (receivers in solid elements), 4 pressure (receivers in fluid elements)
+    seismotype: int = 1  # 1 displacement (receivers in solid elements), 4 pressure, 6 potential (in fluid elements)
     use_trick: bool = False  # USE_TRICK_FOR_BETTER_PRESSURE
     name: str = "custom"
 
@@ -350,3 +350,21 @@ def c2_mesh_coupled(nstep: int = 3000, nx: int = 2000, nz: int = 1000, elem_size
     pert = gaussian_random_field(base, 200.0, 0.10, seed)
     return layered_fluid_solid(nx, nz, nzf, elem_size, nstep, 2.0e-4, pert=pert,
                                name=f"c2_mesh_coupled_{nx}x{nz}", sources=src, receivers=rec)
+
+
+def luoyang_fluid_solid(nstep: int = 1600) -> SimulationConfig:
+    """``EXAMPLES/LuoYang_fluid_solid_kernel/DATA/Par_file`` + ``SOURCE`` (forward run): 100 x 60 elements of 50 m
+    (5 km x 3 km, three layers of 20 rows), fluid (rho 2500, vp 2700) in columns 1-40 and 61-100 around a solid
+    strip (rho 2700, vp 3000, vs 1732.051) in columns 41-60, Stacey edges on all sides, a Ricker pressure source of
+    10 Hz (factor 1e10) at (4000, 1500) in the fluid, one potential receiver (seismotype 6) at (1000, 1500),
+    dt 1.1e-3.  The reference meshes it with 9-node elements; on these rectangles the geometry is the same."""
+    nx, nz = 100, 60
+    ex = np.tile(np.arange(nx), nz)
+    solid = np.repeat((ex >= 40) & (ex < 60), NGLL2)
+    return SimulationConfig(
+        name="LuoYang_fluid_solid_kernel", nx=nx, nz=nz, xmin=0.0, xmax=5000.0, interfaces=[0.0, 1000.0, 2000.0, 3000.0],
+        nz_layers=[20, 20, 20], nstep=nstep, dt=1.1e-3, rho=np.where(solid, 2700.0, 2500.0),
+        vp=np.where(solid, 3000.0, 2700.0), vs=np.where(solid, 1732.051, 0.0), material_mode=0, stacey=True,
+        absorbbottom=True, absorbright=True, absorbtop=True, absorbleft=True,
+        sources=[Source(x=4000.0, z=1500.0, f0=10.0, factor=1.0e10, time_function_type=1)],
+        receivers=np.array([[1000.0, 1500.0]]), seismotype=6)

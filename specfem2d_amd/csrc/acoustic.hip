@@ -188,12 +188,14 @@ __global__ __launch_bounds__(64) void k_ac_sources(int nas, int it, int nsrc, co
   atomic_add(&pdd[pts[l]], sa[l] * f / kap[l]);
 }
 
-// pressure seismograms of the receivers in fluid elements: -field interpolated with the receivers' Lagrange
-// weights, accumulated in double (write_seismograms.F90:124-135, compute_pressure.f90:460-490,
-// compute_interpolated_dva.f90); receivers in solid elements (pts = -1) record 0
+// pressure (sign = -1: -potential or -potential_dot_dot) or potential (sign = +1) seismograms of the receivers in
+// fluid elements, interpolated with the receivers' Lagrange weights and accumulated in double
+// (write_seismograms.F90:124-170, compute_pressure.f90:460-490, compute_interpolated_dva.f90); receivers in solid
+// elements (pts = -1) record 0
 __global__ __launch_bounds__(64) void k_ac_seismograms(int nrec, int nlength, int cur, const int* __restrict__ pts,
                                                        const float* __restrict__ xir, const float* __restrict__ gammar,
-                                                       const float* __restrict__ field, float* __restrict__ dst) {
+                                                       const float* __restrict__ field, float sign,
+                                                       float* __restrict__ dst) {
   const int r = blockIdx.x, p = threadIdx.x;
   if (r >= nrec) return;
   double v = 0.0;
@@ -202,7 +204,7 @@ __global__ __launch_bounds__(64) void k_ac_seismograms(int nrec, int nlength, in
     if (g >= 0) {
       const int i = p % NGLL, j = p / NGLL;
       const double h = (double)(xir[i * nrec + r] * gammar[j * nrec + r]);
-      v = (double)(-field[g]) * h;
+      v = (double)(sign * field[g]) * h;
     }
   }
 #pragma unroll
@@ -499,14 +501,15 @@ void require_forward(int cw2, int fwd_or_adj, const char* who) {
 
 // ---------------------------------------------------------------------------
 // launchers used by other translation units
-void launch_acoustic_pressure_seismograms(Mesh* mp, float* dst, int cur, int use_trick) {
+void launch_acoustic_pressure_seismograms(Mesh* mp, float* dst, int cur, int use_trick, int potential) {
   AcousticState* ac = ac_ready(mp, "compute_seismograms_cuda");
   if (mp->nrec_local == 0) return;
   if (ac->nrec_fluid != mp->nrec_local)
-    fail("compute_seismograms_cuda: pressure of receivers in solid elements is not supported by this library");
+    fail("compute_seismograms_cuda: pressure or potential of receivers in solid elements is not supported by this "
+         "library");
   hipLaunchKernelGGL(k_ac_seismograms, dim3(mp->nrec_local), dim3(64), 0, mp->stream, mp->nrec_local,
                      mp->nlength_seismogram, cur, ac->d_rec_pts, mp->d_xir, mp->d_gammar,
-                     use_trick ? ac->d_pot : ac->d_pdd, dst);
+                     (potential || use_trick) ? ac->d_pot : ac->d_pdd, potential ? 1.f : -1.f, dst);
   check_hip(hipGetLastError(), "k_ac_seismograms launch");
 }
 

@@ -1119,12 +1119,12 @@ void compute_seismograms_cuda_(long* Mesh_pointer_f, int* i_sigf, double* sisux,
       check_accel_complete(mp, "compute_seismograms_cuda");
       field = mp->d_accel;
       break;
-    case 4: break;
-    default: fail("seismotype %d (curl/potential) is not supported by this library yet", stype);
+    case 4: case 6: break;
+    default: fail("seismotype %d (curl) is not supported by this library yet", stype);
   }
   float* d_dst = mp->d_seismo[i_sig];
-  if (stype == 4)
-    launch_acoustic_pressure_seismograms(mp, d_dst, cur, *USE_TRICK_FOR_BETTER_PRESSURE);
+  if (stype == 4 || stype == 6)
+    launch_acoustic_pressure_seismograms(mp, d_dst, cur, *USE_TRICK_FOR_BETTER_PRESSURE, stype == 6);
   else if (acoustic_receivers(mp) > 0)
     fail("compute_seismograms_cuda: displacement, velocity or acceleration of receivers in fluid elements is not "
          "supported by this library (use seismotype 4)");

@@ -705,7 +705,7 @@ void prepare_device(Sim& s) {
   int nsrc = s.nsrc, nstep = s.p.nstep, nrec = s.nrec;
   int simtype = s.simulation_type, p_sv = 1, nspec_ac = s.nspec_ac, nspec_el = s.nspec - s.nspec_ac;
   int save_fwd = s.save_forward;
-  s.seismotype[0] = s.p.seismotype == 4 ? 4 : 1;
+  s.seismotype[0] = (s.p.seismotype == 4 || s.p.seismotype == 6) ? s.p.seismotype : 1;
   int nsig = 1, nlength = s.p.nstep;
   int dummy_i = 0;
   std::vector<int> src_ispec = s.src_ispec, rec_ispec = s.rec_ispec;

@@ -205,11 +205,11 @@ void apply_element_forces(const Mesh* mp, int ph, const float* u, float* acc, in
                           int first_touch = 0, int ps = 1);
 // fluid elements (acoustic.hip): release, the fluid's Stacey points (recorded by prepare_stacey_device_: ispec
 // 0-based caller order, ij 0-based with -1 for points an edge does not own, rho_vp [nspec][25] caller order),
-// pressure seismograms (-potential with USE_TRICK_FOR_BETTER_PRESSURE, else -potential_dot_dot) and the number of
-// receivers in fluid elements
+// pressure seismograms (-potential with USE_TRICK_FOR_BETTER_PRESSURE, else -potential_dot_dot) or potential
+// seismograms (potential != 0) and the number of receivers in fluid elements
 void acoustic_release(Mesh* mp);
 void acoustic_note_stacey(Mesh* mp, int nf, const int* ispec, const int* ij, const float* jacw, const float* rho_vp);
-void launch_acoustic_pressure_seismograms(Mesh* mp, float* dst, int cur, int use_trick);
+void launch_acoustic_pressure_seismograms(Mesh* mp, float* dst, int cur, int use_trick, int potential);
 int acoustic_receivers(Mesh* mp);
 // point arrays in the caller's numbering <-> the device numbering (n2 values per point)
 void points_to_device(const Mesh* mp, float* dev, const float* host, int n2, const char* what);

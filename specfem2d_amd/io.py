@@ -4,7 +4,8 @@
 Readers (``src/shared/param_reader.c``, ``read_parameter_file.F90``, ``read_material_table.f90``,
 ``read_regions.f90``, ``src/meshfem2D/read_interfaces_file.f90``, ``read_source_file.f90``,
 ``save_stations_file.f90``) build a :class:`~specfem2d_amd.config.SimulationConfig` for what this library runs:
-the internal mesher with flat interfaces (NGNOD = 4), isotropic material tables (solid, or fluid when vs = 0),
+the internal mesher with flat interfaces (NGNOD = 4, or 9, which on flat layers gives the same rectangles),
+isotropic material tables (solid, or fluid when vs = 0), seismotypes 1-4 and 6,
 Stacey or free boundaries, collocated force / pressure sources with time functions 1-3, Newmark, P-SV.
 Anything else in the files is refused with a message naming the parameter.
 
@@ -223,7 +224,8 @@ def config_from_par_file(data_dir: str, cpml: str = "error", nstep: Optional[int
         if cond:
             raise NotImplementedError(f"Par_file: {what} is not supported by this library")
 
-    refuse(int(g("NGNOD")) != 4, "NGNOD = 9")
+    # 9-node elements on the internal mesher's flat layers are rectangles: the same geometry as 4-node ones
+    refuse(int(g("NGNOD")) not in (4, 9), f"NGNOD = {g('NGNOD')}")
     refuse(not fortran_bool(g("P_SV")), "P_SV = .false. (SH waves)")
     refuse(fortran_bool(g("AXISYM", ".false.")), "AXISYM")
     refuse(int(g("time_stepping_scheme", "1")) != 1, "time_stepping_scheme other than Newmark")
@@ -243,7 +245,7 @@ def config_from_par_file(data_dir: str, cpml: str = "error", nstep: Optional[int
                                       "cpml='stacey' to use Stacey edges on the absorbing sides instead")
         stacey = True
     seismotypes = [int(t) for t in re.split(r"[ ,]+", g("seismotype").strip()) if t]
-    refuse(len(seismotypes) != 1 or seismotypes[0] not in (1, 2, 3, 4), f"seismotype = {g('seismotype')}")
+    refuse(len(seismotypes) != 1 or seismotypes[0] not in (1, 2, 3, 4, 6), f"seismotype = {g('seismotype')}")
     heights, nz_layers, _ = read_interfaces(os.path.join(data_dir, os.path.basename(g("interfacesfile"))))
     nx = int(g("nx"))
     nz = sum(nz_layers)
@@ -315,18 +317,18 @@ def ascii_trace_lines(times: Sequence[float], values: Sequence[float]) -> str:
                    for t, v in zip(times, values))
 
 
-_CHANNEL_SUFFIX = {1: "d", 2: "v", 3: "a", 4: "p"}
+_CHANNEL_SUFFIX = {1: "d", 2: "v", 3: "a", 4: "p", 6: "x"}
 
 
 def write_ascii_seismograms(out_dir: str, seis: np.ndarray, stations: Sequence[Station], dt: float, t0: float,
                             seismotype: int = 1) -> List[str]:
-    """``seis`` [nrec, 2, nstep] (component 0 only for pressure); writes ``NET.STA.BX{X,Z}.semX`` (or
-    ``.PRE.semp``) with time = (it - 1) dt - t0; returns the paths."""
+    """``seis`` [nrec, 2, nstep] (component 0 only for pressure and potential); writes ``NET.STA.BX{X,Z}.semX``
+    (``.PRE.semp`` for pressure, ``.POT.semx`` for the potential) with time = (it - 1) dt - t0; returns the paths."""
     os.makedirs(out_dir, exist_ok=True)
     nstep = seis.shape[-1]
     times = [float(i) * dt - t0 for i in range(nstep)]
     comp = _CHANNEL_SUFFIX[seismotype]
-    channels = ["PRE"] if seismotype == 4 else ["BXX", "BXZ"]
+    channels = {4: ["PRE"], 6: ["POT"]}.get(seismotype, ["BXX", "BXZ"])
     paths = []
     for r, st in enumerate(stations):
         for c, ch in enumerate(channels):
@@ -344,7 +346,7 @@ def write_binary_seismograms(out_dir: str, seis: np.ndarray, seismotype: int = 1
     comp = _CHANNEL_SUFFIX[seismotype]
     kind = "double" if double else "single"
     dtype = np.float64 if double else np.float32
-    names = [("Up", 0)] if seismotype == 4 else [("Ux", 0), ("Uz", 1)]
+    names = [("Up", 0)] if seismotype in (4, 6) else [("Ux", 0), ("Uz", 1)]
     paths = []
     for pre, c in names:
         p = os.path.join(out_dir, f"{pre}_file_{kind}_{comp}.bin")

@@ -36,6 +36,15 @@ def pack_pressure(example, sub, station, out):
     print("wrote", out, t.shape)
 
 
+def pack_forward_potential(out):
+    """The forward potential trace of EXAMPLES/LuoYang_fluid_solid_kernel: its adjoint-source tool
+    (adj_source.f90) copies the forward run's AA.S0001.POT.semx into REF_KERNEL/AA.S0001.POT.adj unchanged up to
+    t = 1.274 s (zero after)."""
+    t, v = read_trace(os.path.join(REF, "EXAMPLES", "LuoYang_fluid_solid_kernel", "REF_KERNEL", "AA.S0001.POT.adj"))
+    np.savez_compressed(os.path.join(HERE, out), time=t, **{"AA.S0001.POT": v}, window_end=np.array(1.274))
+    print("wrote", out, t.shape)
+
+
 def copy_inputs(example, files, out):
     """The example's input files (Par_file, SOURCE, interfaces: plain-text data) for the on-disk format tests."""
     import shutil
@@ -82,4 +91,7 @@ if __name__ == "__main__":
                 ["REF_SEIS/Par_file", "REF_SEIS/SOURCE", "DATA/interfaces_attenuation_analytic.dat"],
                 "pressure_source_acoustic")
     trace_digests("ref_seis_ascii_digests.npz")
+    pack_forward_potential("luoyang_fluid_solid_forward_potential.npz")
+    copy_inputs("LuoYang_fluid_solid_kernel", ["DATA/Par_file", "DATA/SOURCE", "DATA/interfaces.dat"],
+                "LuoYang_fluid_solid_kernel")
     pack("Tromp2005_kernel", "REF_KERNEL", ["AA.S0001"], "tromp2005_kernel_ref_kernel_traces.npz")

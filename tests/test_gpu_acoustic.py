@@ -170,3 +170,27 @@ def test_fluid_rectangle_form_matches_per_point_geometry(monkeypatch):
     r = rel(seis, seis2)
     print(f"fluid rectangle form vs per-point geometry: rel-L2 {r:.2e}")
     assert r < TOL
+
+
+def test_luoyang_fluid_solid_against_reference_trace(golden_dir, tmp_path):
+    """The LuoYang fluid-solid example (a solid strip between two fluids, fluid-solid edges on vertical sides,
+    fluid Stacey edges next to them), run from its own input files, against the reference's forward potential
+    trace (bit-exact for the oracle) and written as the reference writes a potential trace."""
+    import os
+
+    from specfem2d_amd import io
+    cfg, stations = io.config_from_par_file(os.path.join(golden_dir, "examples", "LuoYang_fluid_solid_kernel"))
+    sim = Simulation(cfg)
+    assert sim.nspec_acoustic == 4800 and sim.ncoupling == 120
+    seis, t = sim.forward()
+    g = np.load(f"{golden_dir}/luoyang_fluid_solid_forward_potential.npz")
+    w = g["time"] <= float(g["window_end"])
+    r = rel(seis[0, 0][w], g["AA.S0001.POT"][w])
+    print(f"LuoYang fluid-solid example, 1600 steps: potential rel-L2 vs the reference's trace {r:.3e}, "
+          f"loop {t['loop_ms']:.1f} ms")
+    assert r < TOL
+    (path,) = io.write_ascii_seismograms(str(tmp_path), seis, stations, cfg.dt, sim.t0, seismotype=6)
+    assert os.path.basename(path) == "AA.S0001.POT.semx"
+    tt, _ = io.read_ascii_trace(path)
+    np.testing.assert_array_equal(tt, g["time"])
+    sim.close()

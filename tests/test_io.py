@@ -3,8 +3,9 @@ ASCII and binary seismograms out.
 
 * ASCII writer: byte-for-byte equal to the reference's own committed traces (sha256 of 21 files from
   ``EXAMPLES/*/REF_SEIS``, one per example and recording kind, regenerated from their parsed values);
-* readers: the reference's Tromp2005 and pressure-source example inputs give the configurations written by hand
-  from their Par_files, and the oracle run from the parsed Tromp2005 files reproduces REF_SEIS bit-exactly;
+* readers: the reference's Tromp2005, pressure-source and LuoYang fluid-solid example inputs give the
+  configurations written by hand from their Par_files, and the oracle run from the parsed Tromp2005 files
+  reproduces REF_SEIS bit-exactly;
 * receiver lines: the STATIONS rounding (f20.7) of save_stations_file.f90.
 """
 import hashlib
@@ -103,3 +104,14 @@ def test_binary_writer_layout(tmp_path):
     (pp,) = io.write_binary_seismograms(str(tmp_path), seis, seismotype=4, double=True)
     assert os.path.basename(pp) == "Up_file_double_p.bin"
     np.testing.assert_array_equal(np.fromfile(pp, np.float64).reshape(3, 5), seis[:, 0, :])
+
+
+def test_luoyang_inputs_give_the_configuration(golden_dir):
+    """A fluid-solid example with 9-node elements, three regions and a potential receiver (seismotype 6)."""
+    cfg, stations = io.config_from_par_file(os.path.join(golden_dir, "examples", "LuoYang_fluid_solid_kernel"))
+    ref = config.luoyang_fluid_solid()
+    _same(cfg, ref, KEYS)
+    assert cfg.sources == ref.sources
+    np.testing.assert_array_equal(cfg.receivers, ref.receivers)
+    for a, b in zip(cfg.material_arrays(), ref.material_arrays()):
+        np.testing.assert_array_equal(a, b)

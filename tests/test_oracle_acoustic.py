@@ -8,6 +8,10 @@ What pins each part (oracle/sem2d_oracle.c, acoustic routines):
   ``EXAMPLES/check_absolute_amplitude_of_pressure_source_seismograms_acoustic/REF_SEIS/AA.S0001.PRE.semp``,
   to 2e-6 of its peak before the waves reach the reference's absorbing layer (a CPML there, Stacey here) and
   to 1e-6 relative at the peak (a few float ulps);
+* fluid-solid coupling across vertical interfaces, fluid Stacey edges next to fluid-solid edges, and potential
+  seismograms (seismotype 6) -- **bit-exact** against the reference's forward potential trace of
+  ``EXAMPLES/LuoYang_fluid_solid_kernel`` (its adjoint-source tool keeps it unchanged up to t = 1.274 s in
+  ``REF_KERNEL/AA.S0001.POT.adj``), 1268 samples;
 * pressure without the trick (-potential_dot_dot with the Gaussian itself): the same trace, since the two
   recordings are the same continuous quantity (mathematical equivalence, discretisation-level tolerance);
 * fluid-solid coupling, both directions: the normal-incidence plane-wave reflection coefficient
@@ -168,3 +172,15 @@ def test_domains_edges_and_stacey_ranges():
     solid_only = np.setdiff1d(ib[:12].reshape(-1), ib[18:].reshape(-1))
     fluid_only = np.setdiff1d(ib[18:].reshape(-1), ib[:18].reshape(-1))
     assert np.all(rma[solid_only] == 1) and np.all(rme[fluid_only] == 1)
+
+
+def test_luoyang_fluid_solid_forward_bit_exact(golden_dir):
+    cfg = config.luoyang_fluid_solid()
+    o = OracleSim(cfg, "sp", ftz=True)
+    assert o.nspec_acoustic == 4800 and o.ncoupling == 120
+    pot = o.forward()[0, 0]
+    g = np.load(f"{golden_dir}/luoyang_fluid_solid_forward_potential.npz")
+    t, ref = g["time"], g["AA.S0001.POT"]
+    w = t <= float(g["window_end"])
+    assert t[0] == pytest.approx(-o.t0, rel=1e-15) and w.sum() == 1268
+    np.testing.assert_array_equal(pot[w], ref[w])
