@@ -39,11 +39,13 @@ typedef struct {
   int use_trick;                       /* USE_TRICK_FOR_BETTER_PRESSURE */
 } sf2d_params;
 
-/* DATA/SOURCE subset: collocated force in a solid element, pressure source in a fluid element (vs = 0) */
+/* DATA/SOURCE subset: collocated force in a solid element, pressure source in a fluid element (vs = 0), or a
+   moment tensor in a solid element */
 typedef struct {
   double x, z, f0, factor, anglesource, tshift;
   int time_function_type;              /* 1 Ricker, 2 first derivative of a Gaussian, 3 Gaussian */
-  int pad;
+  int source_type;                     /* 1 (or 0) force / pressure, 2 moment tensor */
+  double Mxx, Mzz, Mxz;                /* moment tensor (source_type 2) */
 } sf2d_source;
 
 /* Builds the host-side setup of a simulation (no device call).  rho/vp/vs are

@@ -31,7 +31,7 @@ class OrcSource(C.Structure):
     _fields_ = [
         ("x", C.c_double), ("z", C.c_double), ("f0", C.c_double), ("factor", C.c_double),
         ("angle_deg", C.c_double), ("tshift", C.c_double), ("time_function_type", C.c_int),
-        ("pad", C.c_int),
+        ("source_type", C.c_int), ("Mxx", C.c_double), ("Mzz", C.c_double), ("Mxz", C.c_double),
     ]
 
 
@@ -126,6 +126,7 @@ class OracleSim:
         for k, s in enumerate(cfg.sources):
             srcs[k].x, srcs[k].z, srcs[k].f0, srcs[k].factor = s.x, s.z, s.f0, s.factor
             srcs[k].angle_deg, srcs[k].tshift, srcs[k].time_function_type = s.anglesource, s.tshift, s.time_function_type
+            srcs[k].source_type, srcs[k].Mxx, srcs[k].Mzz, srcs[k].Mxz = s.source_type, s.Mxx, s.Mzz, s.Mxz
         rec = np.ascontiguousarray(np.asarray(cfg.receivers if cfg.nrec else np.zeros((1, 2)), dtype=np.float64))
         self._keep = (p, rho, vp, vs, srcs, rec)
         self.h = self.lib.orc_new(C.byref(p), rho, vp, vs, len(cfg.sources), srcs, cfg.nrec, rec)

@@ -60,7 +60,8 @@ typedef struct {
   double angle_deg;           /* anglesource, degrees */
   double tshift;
   int time_function_type;     /* 1 Ricker, 2 first derivative of Gaussian, 3 Gaussian */
-  int pad;
+  int source_type;            /* 1 (or 0) collocated force / pressure, 2 moment tensor */
+  double Mxx, Mzz, Mxz;       /* moment tensor (source_type 2) */
 } orc_source;
 
 typedef struct {
@@ -173,6 +174,29 @@ static void setup_gll(orc_sim *s) {
       s->hprime[i2][i1] = (REAL)lagrange_deriv_gll(i1, i2, s->xigll, NGLL);
       s->hprimewgll[i2][i1] = s->wxgll[i2] * s->hprime[i2][i1];
     }
+}
+
+/* Lagrange interpolants and their derivatives at xi (lagrange_poly.f90:34-93) */
+static void lagrange_any_deriv(double xi, const double *xigll, double *h, double *hp) {
+  for (int d = 0; d < NGLL; d++) {
+    double prod1 = 1.0, prod2 = 1.0, x0 = xigll[d];
+    for (int i = 0; i < NGLL; i++)
+      if (i != d) {
+        prod1 = prod1 * (xi - xigll[i]);
+        prod2 = prod2 * (x0 - xigll[i]);
+      }
+    double prod2_inv = 1.0 / prod2;
+    h[d] = prod1 * prod2_inv;
+    double sum = 0.0;
+    for (int i = 0; i < NGLL; i++) {
+      if (i == d) continue;
+      double prod3 = 1.0;
+      for (int j = 0; j < NGLL; j++)
+        if (j != d && j != i) prod3 = prod3 * (xi - xigll[j]);
+      sum = sum + prod3;
+    }
+    hp[d] = sum * prod2_inv;
+  }
 }
 
 static void lagrange_any(double xi, const double *xigll, double *h) {
@@ -732,6 +756,32 @@ static void setup_sources(orc_sim *s) {
     lagrange_any(s->src_gamma[k], s->xigll, hg);
     double ang = s->src[k].angle_deg * PI / 180.0;
     const int acoustic = s->is_acoustic[s->src_ispec[k] - 1];
+    if (s->src[k].source_type == 2) {
+      /* moment tensor (compute_arrays_source.f90:34-160): the gradient of the Lagrange weights at the source,
+         mapped by the element's inverse jacobian interpolated there */
+      double hpx[NGLL], hpg[NGLL];
+      lagrange_any_deriv(s->src_xi[k], s->xigll, hx, hpx);
+      lagrange_any_deriv(s->src_gamma[k], s->xigll, hg, hpg);
+      const int e = s->src_ispec[k] - 1;
+      double dxis_dx = 0.0, dxis_dz = 0.0, dgam_dx = 0.0, dgam_dz = 0.0;
+      for (int m = 0; m < NGLL; m++)
+        for (int kk = 0; kk < NGLL; kk++) {
+          size_t l = (size_t)e * NGLL2 + m * NGLL + kk;
+          double hl = hx[kk] * hg[m];
+          dxis_dx = dxis_dx + hl * (double)s->xix[l];
+          dxis_dz = dxis_dz + hl * (double)s->xiz[l];
+          dgam_dx = dgam_dx + hl * (double)s->gammax[l];
+          dgam_dz = dgam_dz + hl * (double)s->gammaz[l];
+        }
+      const double Mxx = s->src[k].Mxx, Mzz = s->src[k].Mzz, Mxz = s->src[k].Mxz;
+      for (int m = 0; m < NGLL; m++)
+        for (int kk = 0; kk < NGLL; kk++) {
+          double dsx = (hpx[kk] * dxis_dx) * hg[m] + hx[kk] * (hpg[m] * dgam_dx);
+          double dsz = (hpx[kk] * dxis_dz) * hg[m] + hx[kk] * (hpg[m] * dgam_dz);
+          s->sourcearrays[((size_t)k * NGLL2 + m * NGLL + kk) * 2 + 0] = (REAL)(Mxx * dsx + Mxz * dsz);
+          s->sourcearrays[((size_t)k * NGLL2 + m * NGLL + kk) * 2 + 1] = (REAL)(Mxz * dsx + Mzz * dsz);
+        }
+    } else
     for (int j = 0; j < NGLL; j++)
       for (int i = 0; i < NGLL; i++) {
         double hl = hx[i] * hg[j];

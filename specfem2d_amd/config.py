@@ -25,7 +25,8 @@ XIGLL = np.array([-1.0, -_R37, 0.0, _R37, 1.0])
 
 @dataclass
 class Source:
-    """A collocated force source (``source_type = 1`` in ``DATA/SOURCE``)."""
+    """A source of ``DATA/SOURCE``: a collocated force (a pressure source in a fluid element), ``source_type = 1``,
+    or a moment tensor (Mxx, Mzz, Mxz), ``source_type = 2``."""
 
     x: float
     z: float
@@ -34,6 +35,10 @@ class Source:
     anglesource: float = 0.0  # degrees
     tshift: float = 0.0
     time_function_type: int = 1  # 1 Ricker, 2 first derivative of a Gaussian, 3 Gaussian
+    source_type: int = 1
+    Mxx: float = 0.0
+    Mzz: float = 0.0
+    Mxz: float = 0.0
 
 
 @dataclass

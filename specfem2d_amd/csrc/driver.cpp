@@ -80,7 +80,8 @@ Basis make_basis() {
   return b;
 }
 
-void lagrange(double xi, const double* xg, double* h) {
+// Lagrange interpolants at xi, and their derivatives when hp is given (src/shared/lagrange_poly.f90:34-93)
+void lagrange(double xi, const double* xg, double* h, double* hp = nullptr) {
   for (int d = 0; d < NGLL; d++) {
     double num = 1.0, den = 1.0;
     for (int i = 0; i < NGLL; i++)
@@ -88,7 +89,18 @@ void lagrange(double xi, const double* xg, double* h) {
         num *= (xi - xg[i]);
         den *= (xg[d] - xg[i]);
       }
-    h[d] = num / den;
+    const double inv = 1.0 / den;
+    h[d] = num * inv;
+    if (!hp) continue;
+    double sum = 0.0;
+    for (int i = 0; i < NGLL; i++) {
+      if (i == d) continue;
+      double prod = 1.0;
+      for (int j = 0; j < NGLL; j++)
+        if (j != d && j != i) prod *= (xi - xg[j]);
+      sum += prod;
+    }
+    hp[d] = sum * inv;
   }
 }
 
@@ -149,6 +161,7 @@ struct Sim {
   std::vector<int> src_shot, shot_nsrc;
   std::vector<double> shot_t0;
   std::vector<double> shot_chi;
+  bool bad_source = false;
 };
 
 // bilinear map of element e (src/shared/define_shape_functions.f90 NGNOD=4, recompute_jacobian.f90)
@@ -612,6 +625,35 @@ double build_source_set(Sim& s, int k0, int n) {
     lagrange(s.src_gamma[k], s.basis.xi, hg);
     const double ang = s.src[k].anglesource * PI / 180.0;
     const bool fluid = s.nspec_ac > 0 && s.is_acoustic[s.src_ispec[k] - 1];
+    if (s.src[k].source_type == 2 && fluid) {
+      fprintf(stderr, "sf2d_sim_create: source %d: a moment-tensor source cannot sit in a fluid element\n", k + 1);
+      s.bad_source = true;
+    } else if (s.src[k].source_type == 2) {
+      // moment tensor (compute_arrays_source.f90:34-160): gradient of the Lagrange weights at the source through
+      // the element's inverse jacobian interpolated there
+      double hpx[NGLL], hpg[NGLL];
+      lagrange(s.src_xi[k], s.basis.xi, hx, hpx);
+      lagrange(s.src_gamma[k], s.basis.xi, hg, hpg);
+      const size_t e0 = (size_t)(s.src_ispec[k] - 1) * NGLL2;
+      double dxi_dx = 0.0, dxi_dz = 0.0, dga_dx = 0.0, dga_dz = 0.0;
+      for (int m = 0; m < NGLL; m++)
+        for (int i = 0; i < NGLL; i++) {
+          const size_t l = e0 + m * NGLL + i;
+          const double hl = hx[i] * hg[m];
+          dxi_dx += hl * (double)s.xix[l];
+          dxi_dz += hl * (double)s.xiz[l];
+          dga_dx += hl * (double)s.gammax[l];
+          dga_dz += hl * (double)s.gammaz[l];
+        }
+      const double mxx = s.src[k].Mxx, mzz = s.src[k].Mzz, mxz = s.src[k].Mxz;
+      for (int m = 0; m < NGLL; m++)
+        for (int i = 0; i < NGLL; i++) {
+          const double dsx = (hpx[i] * dxi_dx) * hg[m] + hx[i] * (hpg[m] * dga_dx);
+          const double dsz = (hpx[i] * dxi_dz) * hg[m] + hx[i] * (hpg[m] * dga_dz);
+          s.sourcearrays[(((size_t)k * NGLL + m) * NGLL + i) * 2] = (float)(mxx * dsx + mxz * dsz);
+          s.sourcearrays[(((size_t)k * NGLL + m) * NGLL + i) * 2 + 1] = (float)(mxz * dsx + mzz * dsz);
+        }
+    } else
     for (int j = 0; j < NGLL; j++)
       for (int i = 0; i < NGLL; i++) {
         // a pressure source in a fluid element, a force in a solid one (setup_sources_receivers.F90:994-1010)
@@ -799,6 +841,10 @@ void* create(const sf2d_params* p, const double* rho, const double* vp, const do
   s->nshot = nshot;
   if (nshot > 0) s->shot_nsrc.assign(shot_nsrc, shot_nsrc + nshot);
   build_sources(*s);
+  if (s->bad_source) {
+    delete s;
+    return nullptr;
+  }
   s->nrec = nrec;
   build_receivers(*s, rec_xz);
   return s;

@@ -33,7 +33,8 @@ class Params(C.Structure):
 class SourceC(C.Structure):
     _fields_ = [
         ("x", C.c_double), ("z", C.c_double), ("f0", C.c_double), ("factor", C.c_double),
-        ("anglesource", C.c_double), ("tshift", C.c_double), ("time_function_type", C.c_int), ("pad", C.c_int),
+        ("anglesource", C.c_double), ("tshift", C.c_double), ("time_function_type", C.c_int),
+        ("source_type", C.c_int), ("Mxx", C.c_double), ("Mzz", C.c_double), ("Mxz", C.c_double),
     ]
 
 
@@ -152,6 +153,7 @@ def _sources_c(sources):
     for k, s in enumerate(sources):
         srcs[k].x, srcs[k].z, srcs[k].f0, srcs[k].factor = s.x, s.z, s.f0, s.factor
         srcs[k].anglesource, srcs[k].tshift, srcs[k].time_function_type = s.anglesource, s.tshift, s.time_function_type
+        srcs[k].source_type, srcs[k].Mxx, srcs[k].Mzz, srcs[k].Mxz = s.source_type, s.Mxx, s.Mzz, s.Mxz
     return srcs
 
 

@@ -156,14 +156,17 @@ def read_source_file(path: str, nsources: int) -> List[Source]:
         if fortran_bool(b.get("source_surf", ".false.")):
             raise NotImplementedError("SOURCE: source_surf = .true. is not supported")
         st = int(b["source_type"])
-        if st != 1:
-            raise NotImplementedError(f"SOURCE: source_type {st} (only collocated force / pressure sources)")
+        if st not in (1, 2):
+            raise NotImplementedError(f"SOURCE: source_type {st} (only force / pressure and moment-tensor sources)")
         tf = int(b["time_function_type"])
         if tf not in (1, 2, 3):
             raise NotImplementedError(f"SOURCE: time_function_type {tf} (only 1, 2, 3)")
         out.append(Source(x=fortran_float(b["xs"]), z=fortran_float(b["zs"]), f0=fortran_float(b["f0"]),
                           factor=fortran_float(b["factor"]), anglesource=fortran_float(b.get("anglesource", "0")),
-                          tshift=fortran_float(b.get("tshift", "0")), time_function_type=tf))
+                          tshift=fortran_float(b.get("tshift", "0")), time_function_type=tf, source_type=st,
+                          Mxx=fortran_float(b.get("Mxx", "0")) if st == 2 else 0.0,
+                          Mzz=fortran_float(b.get("Mzz", "0")) if st == 2 else 0.0,
+                          Mxz=fortran_float(b.get("Mxz", "0")) if st == 2 else 0.0))
     return out
 
 

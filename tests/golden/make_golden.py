@@ -92,6 +92,10 @@ if __name__ == "__main__":
                 "pressure_source_acoustic")
     trace_digests("ref_seis_ascii_digests.npz")
     pack_forward_potential("luoyang_fluid_solid_forward_potential.npz")
+    pack("infinite_homogeneous_moment_tensor_vertical_dip_slip", "REF_SEIS", ["AA.S0001"],
+         "moment_tensor_ref_seis.npz")
+    copy_inputs("infinite_homogeneous_moment_tensor_vertical_dip_slip",
+                ["DATA/Par_file", "DATA/SOURCE", "DATA/interfaces_elastic_analytic.dat"], "moment_tensor")
     copy_inputs("LuoYang_fluid_solid_kernel", ["DATA/Par_file", "DATA/SOURCE", "DATA/interfaces.dat"],
                 "LuoYang_fluid_solid_kernel")
     pack("Tromp2005_kernel", "REF_KERNEL", ["AA.S0001"], "tromp2005_kernel_ref_kernel_traces.npz")

@@ -1,0 +1,57 @@
+"""Moment-tensor sources (``source_type = 2``, compute_arrays_source.f90:34-160) in the oracle and the host driver.
+
+Pinned by the reference's ``EXAMPLES/infinite_homogeneous_moment_tensor_vertical_dip_slip/REF_SEIS`` traces: the
+oracle, set up from the example's own input files, follows them to 1e-6 of their peak until the waves reach the
+reference's CPML layer (Stacey edges here), a window that holds the peak and more than half of the energy.  The
+host driver's moment-tensor source arrays equal the oracle's bit for bit."""
+import os
+
+import numpy as np
+import pytest
+
+from oracle.oracle import OracleSim
+from specfem2d_amd import io
+from specfem2d_amd.driver import Simulation
+
+WINDOW_END = 0.22  # s: the waves reach the absorbing layer afterwards
+
+
+def _cfg(golden_dir):
+    return io.config_from_par_file(os.path.join(golden_dir, "examples", "moment_tensor"), cpml="stacey")
+
+
+def test_moment_tensor_example_early_window(golden_dir):
+    cfg, stations = _cfg(golden_dir)
+    assert cfg.sources[0].source_type == 2 and (cfg.sources[0].Mxz, cfg.sources[0].Mxx) == (1.0, 0.0)
+    o = OracleSim(cfg, "sp")
+    seis = o.forward()
+    g = np.load(f"{golden_dir}/moment_tensor_ref_seis.npz")
+    t = g["time"]
+    w = t < WINDOW_END
+    for c, ch in enumerate(("BXX", "BXZ")):
+        ref = g[f"AA.S0001.{ch}"]
+        peak = np.abs(ref).max()
+        assert np.abs(ref[w]).max() == peak  # the peak is inside the window
+        assert np.abs(seis[0, c][w] - ref[w]).max() < 1e-6 * peak
+
+
+def test_moment_tensor_source_arrays_match_the_driver(golden_dir):
+    cfg, _ = _cfg(golden_dir)
+    sim, o = Simulation(cfg, device=False), OracleSim(cfg, "sp")
+    si, sx, sg, sa, st = sim.sources()
+    oi, ox, og, osa, ost = o.sources()
+    np.testing.assert_array_equal(si, oi)
+    np.testing.assert_array_equal(sa.astype(np.float64), osa)
+    np.testing.assert_array_equal(st.astype(np.float64), ost)
+    # a moment tensor acts through the gradient of the Lagrange weights: its arrays sum to zero (no net force)
+    assert abs(osa[0, :, 0].sum()) < 1e-6 * np.abs(osa).max() and abs(osa[0, :, 1].sum()) < 1e-6 * np.abs(osa).max()
+    sim.close()
+
+
+def test_moment_tensor_in_a_fluid_element_is_refused():
+    from specfem2d_amd import config
+    from specfem2d_amd.config import Source
+    cfg = config.layered_fluid_solid(8, 6, 2, 100.0, 10, 1e-3, sources=[
+        Source(x=420.0, z=-50.0, f0=10.0, factor=1.0, source_type=2, Mxx=1.0, Mzz=1.0)])
+    with pytest.raises(ValueError):
+        Simulation(cfg, device=False)
