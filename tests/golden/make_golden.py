@@ -94,6 +94,14 @@ if __name__ == "__main__":
     pack_forward_potential("luoyang_fluid_solid_forward_potential.npz")
     pack("infinite_homogeneous_moment_tensor_vertical_dip_slip", "REF_SEIS", ["AA.S0001"],
          "moment_tensor_ref_seis.npz")
+    pack("semi_infinite_homogeneous", "REF_SEIS", ["AA.S0001"], "semi_infinite_homogeneous_ref_seis.npz")
+    copy_inputs("semi_infinite_homogeneous", ["DATA/Par_file", "DATA/SOURCE", "DATA/interfaces_elastic_analytic.dat"],
+                "semi_infinite_homogeneous")
+    pack("check_absolute_amplitude_of_force_source_seismograms_elastic", "REF_SEIS", ["AA.S0001"],
+         "force_source_elastic_ref_seis.npz")
+    copy_inputs("check_absolute_amplitude_of_force_source_seismograms_elastic",
+                ["REF_SEIS/Par_file", "REF_SEIS/SOURCE", "DATA/interfaces_attenuation_analytic.dat"],
+                "force_source_elastic")
     copy_inputs("infinite_homogeneous_moment_tensor_vertical_dip_slip",
                 ["DATA/Par_file", "DATA/SOURCE", "DATA/interfaces_elastic_analytic.dat"], "moment_tensor")
     copy_inputs("LuoYang_fluid_solid_kernel", ["DATA/Par_file", "DATA/SOURCE", "DATA/interfaces.dat"],

@@ -1,4 +1,5 @@
-"""Moment-tensor sources (``source_type = 2``, compute_arrays_source.f90:34-160) in the oracle and the host driver.
+"""Moment-tensor sources (``source_type = 2``, compute_arrays_source.f90:34-160) in the oracle and the host driver,
+and more of the reference's examples set up from their own input files.
 
 Pinned by the reference's ``EXAMPLES/infinite_homogeneous_moment_tensor_vertical_dip_slip/REF_SEIS`` traces: the
 oracle, set up from the example's own input files, follows them to 1e-6 of their peak until the waves reach the
@@ -55,3 +56,18 @@ def test_moment_tensor_in_a_fluid_element_is_refused():
         Source(x=420.0, z=-50.0, f0=10.0, factor=1.0, source_type=2, Mxx=1.0, Mzz=1.0)])
     with pytest.raises(ValueError):
         Simulation(cfg, device=False)
+
+
+@pytest.mark.parametrize("example,window_end", [("semi_infinite_homogeneous", 0.23), ("force_source_elastic", 0.26)])
+def test_force_source_examples_early_window(golden_dir, example, window_end):
+    """Two more reference examples with a force source (a half-space with a free surface; a whole space), set up
+    from their own input files: within 1e-6 of their traces' peak until the waves reach the reference's CPML."""
+    cfg, _ = io.config_from_par_file(os.path.join(golden_dir, "examples", example), cpml="stacey")
+    o = OracleSim(cfg, "sp")
+    seis = o.forward()
+    g = np.load(f"{golden_dir}/{example}_ref_seis.npz")
+    w = g["time"] < window_end
+    assert g["time"][0] == pytest.approx(-o.t0, rel=1e-15)
+    for c, ch in enumerate(("BXX", "BXZ")):
+        ref = g[f"AA.S0001.{ch}"]
+        assert np.abs(seis[0, c][w] - ref[w]).max() < 1e-6 * np.abs(ref).max()
