@@ -35,7 +35,10 @@ typedef struct {
   double user_t0;                      /* USER_T0 */
   double anglerec;                     /* degrees */
   int material_mode;                   /* 0: material table (rho,vp,vs per point), 1: MODEL=gll */
-  int seismotype;                      /* 1 displacement (receivers in solid elements), 4 pressure, 6 potential (in fluid) */
+  int seismotype;                      /* 1 displacement, 2 velocity, 3 acceleration (in fluid elements the gradient of
+                                          the potential or of its time derivatives over the density), 4 pressure, 6
+                                          potential (receivers in fluid elements); shot batches, adjoint runs and
+                                          gradients record displacement only */
   int use_trick;                       /* USE_TRICK_FOR_BETTER_PRESSURE */
 } sf2d_params;
 

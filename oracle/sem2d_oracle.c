@@ -49,7 +49,7 @@ typedef struct {
   double user_t0;             /* USER_T0 (0 = automatic) */
   double anglerec_deg;        /* anglerec */
   int material_mode;          /* 0: Par_file material table, 1: MODEL=gll */
-  int seismotype;             /* 1 displacement (elastic receivers), 4 pressure, 6 potential (acoustic receivers) */
+  int seismotype;             /* 1 displacement, 2 velocity, 3 acceleration, 4 pressure, 6 potential */
   int use_trick;              /* USE_TRICK_FOR_BETTER_PRESSURE */
 } orc_params;
 
@@ -1098,8 +1098,14 @@ static void enforce_free_surface(orc_sim *s) {
     }
 }
 
-static void record_seismograms(orc_sim *s, const REAL *displ, double *seis, int it, int nstep) {
-  /* write_seismograms.F90:60-200, compute_interpolated_dva.f90 */
+static void record_seismograms(orc_sim *s, const REAL *d, const REAL *v, const REAL *a, double *seis, int it,
+                               int nstep) {
+  /* write_seismograms.F90:60-200, compute_interpolated_dva.f90: seismotype 1, 2, 3 interpolate the displacement,
+     velocity or acceleration (in fluid elements the gradient over the density of the potential, its first or its
+     second time derivative), 4 the pressure, 6 the potential */
+  const int st = s->p.seismotype;
+  const REAL *displ = st == 2 ? v : st == 3 ? a : d;
+  const REAL *chi = st == 2 ? s->pot_dot : st == 3 ? s->pot_dd : s->pot;
   if (s->p.seismotype == 4 || s->p.seismotype == 6) {
     /* pressure in fluid elements: -pot with USE_TRICK_FOR_BETTER_PRESSURE, else -pot_dd
        (compute_pressure.f90:460-490); seismotype 6, the potential itself (write_seismograms.F90:147-170);
@@ -1124,6 +1130,25 @@ static void record_seismograms(orc_sim *s, const REAL *displ, double *seis, int 
   for (int r = 0; r < s->nrec; r++) {
     int e = s->rec_ispec[r] - 1;
     double vx = 0.0, vz = 0.0;
+    if (s->is_acoustic[e]) {
+      /* vector in a fluid element: grad(chi) / rho, per GLL point in double, stored in single precision, then
+         interpolated (compute_vector_field.f90:143-210) */
+      const REAL (*H)[NGLL] = (const REAL(*)[NGLL])s->hprime;
+      for (int j = 0; j < NGLL; j++)
+        for (int i = 0; i < NGLL; i++) {
+          size_t l = (size_t)e * NGLL2 + j * NGLL + i;
+          double t1 = 0.0, t2 = 0.0;
+          for (int k = 0; k < NGLL; k++) t1 = t1 + (double)chi[s->ibool[e * NGLL2 + j * NGLL + k] - 1] * (double)H[i][k];
+          for (int k = 0; k < NGLL; k++) t2 = t2 + (double)chi[s->ibool[e * NGLL2 + k * NGLL + i] - 1] * (double)H[j][k];
+          double dchi_dx = t1 * (double)s->xix[l] + t2 * (double)s->gammax[l];
+          double dchi_dz = t1 * (double)s->xiz[l] + t2 * (double)s->gammaz[l];
+          double rhol = (double)s->rhostore[l];
+          REAL fx = (REAL)(dchi_dx / rhol), fz = (REAL)(dchi_dz / rhol);
+          double hl = (double)(s->xir[r * NGLL + i] * s->gammar[r * NGLL + j]);
+          vx = vx + (double)fx * hl;
+          vz = vz + (double)fz * hl;
+        }
+    } else
     for (int j = 0; j < NGLL; j++)
       for (int i = 0; i < NGLL; i++) {
         int g = s->ibool[e * NGLL2 + j * NGLL + i] - 1;
@@ -1311,7 +1336,7 @@ void orc_forward(void *h, int nstep_run, double *seis, double *stacey_store_d) {
       /* update_veloc_elastic_Newmark */
       for (size_t k = 0; k < n2; k++) v[k] = v[k] + deltatover2 * a[k];
     }
-    if (seis && s->nrec > 0) record_seismograms(s, d, seis, step, nstep_run);
+    if (seis && s->nrec > 0) record_seismograms(s, d, v, a, seis, step, nstep_run);
   }
   s->it_done += nstep_run;
   if (store) {
@@ -1548,7 +1573,7 @@ void orc_adjoint(void *h, const double *adj, const double *stacey_store, const d
         ba[k] = (REAL)last_a[k];
       }
     }
-    if (seis && s->nrec > 0) record_seismograms(s, d, seis, it - 1, nstep);
+    if (seis && s->nrec > 0) record_seismograms(s, d, v, a, seis, it - 1, nstep);
     kernels_accumulate(s, d, a, bd, rho_kl, mu_kl, kappa_kl);
     for (int q = 0; q < ndump; q++)
       if (dump_steps[q] == it)
@@ -1639,7 +1664,7 @@ double orc_gradient(void *h, const double *obs, double *parts, double *grad) {
     memcpy(Rn + n2 * (n - 1), a, n2 * sizeof(REAL));
     for (size_t k = 0; k < n2; k++) a[k] = a[k] * s->rmass_inv[k];
     for (size_t k = 0; k < n2; k++) v[k] = v[k] + deltatover2 * a[k];
-    if (s->nrec > 0) record_seismograms(s, d, seis, n - 1, N);
+    if (s->nrec > 0) record_seismograms(s, d, d, d, seis, n - 1, N); /* the misfit is on the displacement */
   }
   double chi = 0.0;
   for (int r = 0; r < s->nrec; r++)

@@ -64,7 +64,7 @@ class SimulationConfig:
     receivers: Optional[np.ndarray] = None  # [nrec, 2] (x, z)
     user_t0: float = 0.0
     anglerec: float = 0.0
-    seismotype: int = 1  # 1 displacement (receivers in solid elements), 4 pressure, 6 potential (in fluid elements)
+    seismotype: int = 1  # 1 displacement, 2 velocity, 3 acceleration, 4 pressure, 6 potential (in fluid elements)
     use_trick: bool = False  # USE_TRICK_FOR_BETTER_PRESSURE
     name: str = "custom"
 

@@ -192,6 +192,54 @@ __global__ __launch_bounds__(64) void k_ac_sources(int nas, int it, int nsrc, co
 // fluid elements, interpolated with the receivers' Lagrange weights and accumulated in double
 // (write_seismograms.F90:124-170, compute_pressure.f90:460-490, compute_interpolated_dva.f90); receivers in solid
 // elements (pts = -1) record 0
+// displacement, velocity or acceleration seismograms (seismotypes 1, 2, 3) of the receivers in fluid elements: the
+// gradient over the density of the potential or of its first or second time derivative at each point of the element, interpolated with the receivers' Lagrange weights and rotated
+// (compute_elastic_seismogram_kernel.cu:120-175, compute_vector_field.f90:143-210); one block per receiver, blocks
+// of receivers in solid elements (pts = -1) leave the displacement recorded there untouched
+__global__ __launch_bounds__(64) void k_ac_vector_seismograms(int nrec, int nlength, int cur,
+                                                              const int* __restrict__ pts,
+                                                              const float4* __restrict__ geom,
+                                                              const float* __restrict__ rho, Basis b,
+                                                              const float* __restrict__ xir,
+                                                              const float* __restrict__ gammar,
+                                                              const float* __restrict__ cosrot,
+                                                              const float* __restrict__ sinrot,
+                                                              const float* __restrict__ field,
+                                                              float* __restrict__ dst) {
+  __shared__ float s[NGLL2];
+  const int r = blockIdx.x, p = threadIdx.x;
+  if (r >= nrec || pts[(size_t)r * NGLL2] < 0) return;  // uniform over the block
+  const size_t l = (size_t)r * NGLL2 + p;
+  if (p < NGLL2) s[p] = field[pts[l]];
+  __syncthreads();
+  double vx = 0.0, vz = 0.0;
+  if (p < NGLL2) {
+    const int i = p % NGLL, j = p / NGLL;
+    float t1 = 0.f, t2 = 0.f;
+#pragma unroll
+    for (int k = 0; k < NGLL; k++) {
+      t1 += s[j * NGLL + k] * b.hp[i * NGLL + k];
+      t2 += s[k * NGLL + i] * b.hp[j * NGLL + k];
+    }
+    const float4 g = geom[l];
+    const float fx = (t1 * g.x + t2 * g.z) / rho[l];
+    const float fz = (t1 * g.y + t2 * g.w) / rho[l];
+    const double h = (double)(xir[i * nrec + r] * gammar[j * nrec + r]);
+    vx = (double)fx * h;
+    vz = (double)fz * h;
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    vx += __shfl_xor(vx, off, 64);
+    vz += __shfl_xor(vz, off, 64);
+  }
+  if (p == 0) {
+    const double c = cosrot[r], sn = sinrot[r];
+    dst[(size_t)r * nlength + cur] = (float)(c * vx + sn * vz);
+    dst[((size_t)nrec + r) * nlength + cur] = (float)(-sn * vx + c * vz);
+  }
+}
+
 __global__ __launch_bounds__(64) void k_ac_seismograms(int nrec, int nlength, int cur, const int* __restrict__ pts,
                                                        const float* __restrict__ xir, const float* __restrict__ gammar,
                                                        const float* __restrict__ field, float sign,
@@ -270,6 +318,8 @@ struct AcousticState {
   float *d_src_sa = nullptr, *d_src_kap = nullptr;
   int nrec_fluid = 0;
   int* d_rec_pts = nullptr;  // [nrec][25], -1 rows for receivers in solid elements
+  float4* d_rec_geom = nullptr;  // [nrec][25] {xi_x, xi_z, gamma_x, gamma_z} of the receiver's element (fluid rows)
+  float* d_rec_rho = nullptr;    // [nrec][25] density (fluid rows)
 };
 
 void acoustic_release(Mesh* mp) {
@@ -280,7 +330,7 @@ void acoustic_release(Mesh* mp) {
   device_free(ac->d_coup); device_free(ac->d_coup_n); device_free(ac->d_coup_w);
   device_free(ac->d_st); device_free(ac->d_st_w); device_free(ac->d_st_rvp);
   device_free(ac->d_src); device_free(ac->d_src_pts); device_free(ac->d_src_sa); device_free(ac->d_src_kap);
-  device_free(ac->d_rec_pts);
+  device_free(ac->d_rec_pts); device_free(ac->d_rec_geom); device_free(ac->d_rec_rho);
   delete ac;
   mp->ac = nullptr;
 }
@@ -479,14 +529,32 @@ AcousticState* ac_ready(Mesh* mp, const char* who) {
     }
   }
   if (mp->nrec_local > 0) {
-    std::vector<int> pts((size_t)mp->nrec_local * NGLL2, -1);
+    std::vector<int> pts((size_t)mp->nrec_local * NGLL2, -1), rec_el;
     for (int r = 0; r < mp->nrec_local; r++) {
       const int e = mp->h_rec_ispec[r];
+      rec_el.push_back(e);
       if (ac->row_of_ispec[e] < 0) continue;
       ac->nrec_fluid++;
       for (int p = 0; p < NGLL2; p++) pts[(size_t)r * NGLL2 + p] = iv.id(e, p);
     }
     ac->d_rec_pts = device_copy(pts.data(), pts.size());
+    if (ac->nrec_fluid > 0) {
+      std::vector<float> xix = download_rows(mp, mp->d_xix, rec_el), xiz = download_rows(mp, mp->d_xiz, rec_el);
+      std::vector<float> gax = download_rows(mp, mp->d_gammax, rec_el), gaz = download_rows(mp, mp->d_gammaz, rec_el);
+      std::vector<float4> geom(pts.size(), make_float4(0.f, 0.f, 0.f, 0.f));
+      std::vector<float> rho(pts.size(), 1.f);
+      for (int r = 0; r < mp->nrec_local; r++) {
+        const int row = ac->row_of_ispec[rec_el[r]];
+        if (row < 0) continue;
+        for (int p = 0; p < NGLL2; p++) {
+          const size_t l = (size_t)r * NGLL2 + p;
+          geom[l] = make_float4(xix[l], xiz[l], gax[l], gaz[l]);
+          rho[l] = ac->rho_rows[(size_t)row * NGLL2 + p];
+        }
+      }
+      ac->d_rec_geom = device_copy(geom.data(), geom.size());
+      ac->d_rec_rho = device_copy(rho.data(), rho.size());
+    }
   }
   ac->ready = 1;
   return ac;
@@ -513,7 +581,17 @@ void launch_acoustic_pressure_seismograms(Mesh* mp, float* dst, int cur, int use
   check_hip(hipGetLastError(), "k_ac_seismograms launch");
 }
 
-// receivers in fluid elements record pressure only
+void launch_acoustic_vector_seismograms(Mesh* mp, float* dst, int cur, int seismotype) {
+  AcousticState* ac = ac_ready(mp, "compute_seismograms_cuda");
+  if (ac->nrec_fluid == 0) return;
+  const float* chi = seismotype == 2 ? ac->d_pd : seismotype == 3 ? ac->d_pdd : ac->d_pot;
+  hipLaunchKernelGGL(k_ac_vector_seismograms, dim3(mp->nrec_local), dim3(64), 0, mp->stream, mp->nrec_local,
+                     mp->nlength_seismogram, cur, ac->d_rec_pts, ac->d_rec_geom, ac->d_rec_rho, mp->basis,
+                     mp->d_xir, mp->d_gammar, mp->d_cosrot, mp->d_sinrot, chi, dst);
+  check_hip(hipGetLastError(), "k_ac_vector_seismograms launch");
+}
+
+// number of receivers in fluid elements
 int acoustic_receivers(Mesh* mp) { return mp->ac ? ac_ready(mp, "compute_seismograms_cuda")->nrec_fluid : 0; }
 
 }  // namespace sf2d

@@ -1125,11 +1125,12 @@ void compute_seismograms_cuda_(long* Mesh_pointer_f, int* i_sigf, double* sisux,
   float* d_dst = mp->d_seismo[i_sig];
   if (stype == 4 || stype == 6)
     launch_acoustic_pressure_seismograms(mp, d_dst, cur, *USE_TRICK_FOR_BETTER_PRESSURE, stype == 6);
-  else if (acoustic_receivers(mp) > 0)
-    fail("compute_seismograms_cuda: displacement, velocity or acceleration of receivers in fluid elements is not "
-         "supported by this library (use seismotype 4)");
-  else
-    launch_seismograms(mp, field, d_dst, cur);
+  else {
+    // receivers in solid elements interpolate the vector field; those in fluid elements then overwrite theirs with
+    // the gradient of the matching scalar field over the density
+    if (mp->nspec_elastic > 0) launch_seismograms(mp, field, d_dst, cur);
+    if (acoustic_receivers(mp) > 0) launch_acoustic_vector_seismograms(mp, d_dst, cur, stype);
+  }
   if (cur == nlength - 1 || *itf == *it_endf) {
     const size_t size = (size_t)mp->nrec_local * nlength;
     float* h = mp->h_seismo[i_sig];

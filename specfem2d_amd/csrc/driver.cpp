@@ -747,7 +747,7 @@ void prepare_device(Sim& s) {
   int nsrc = s.nsrc, nstep = s.p.nstep, nrec = s.nrec;
   int simtype = s.simulation_type, p_sv = 1, nspec_ac = s.nspec_ac, nspec_el = s.nspec - s.nspec_ac;
   int save_fwd = s.save_forward;
-  s.seismotype[0] = (s.p.seismotype == 4 || s.p.seismotype == 6) ? s.p.seismotype : 1;
+  s.seismotype[0] = s.p.seismotype;
   int nsig = 1, nlength = s.p.nstep;
   int dummy_i = 0;
   std::vector<int> src_ispec = s.src_ispec, rec_ispec = s.rec_ispec;
@@ -818,6 +818,15 @@ void prepare_device(Sim& s) {
 void* create(const sf2d_params* p, const double* rho, const double* vp, const double* vs, int nsrc,
              const sf2d_source* src, int nrec, const double* rec_xz, int simulation_type, int save_forward, int nshot,
              const int* shot_nsrc) {
+  const int st = p->seismotype;
+  if (st != 1 && st != 2 && st != 3 && st != 4 && st != 6) {
+    fprintf(stderr, "sf2d_sim_create: seismotype %d is not supported (1, 2, 3, 4 or 6)\n", st);
+    return nullptr;
+  }
+  if (st != 1 && (nshot > 0 || simulation_type != 1)) {
+    fprintf(stderr, "sf2d_sim_create: shot batches and adjoint runs record displacement (seismotype 1)\n");
+    return nullptr;
+  }
   Sim* s = new Sim();
   s->p = *p;
   s->simulation_type = simulation_type;
@@ -1266,7 +1275,7 @@ int sf2d_sim_set_model(void* sim, const double* rho, const double* vp, const dou
 
 int sf2d_sim_gradient_forward(void* sim, const double* obs, int chunk, double* chi, double* seis) {
   Sim* s = (Sim*)sim;
-  if (!s || !s->mesh || s->simulation_type != 1 || chunk < 1 || s->nspec_ac > 0) return 1;
+  if (!s || !s->mesh || s->simulation_type != 1 || chunk < 1 || s->nspec_ac > 0 || s->p.seismotype != 1) return 1;
   const int N = s->p.nstep, R = s->nrec, S = std::max(1, s->nshot);
   s->grad_chunk = std::min(chunk, N);
   std::vector<double> sm((size_t)S * std::max(1, R) * 2 * N, 0.0);

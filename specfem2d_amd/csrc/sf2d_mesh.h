@@ -210,6 +210,7 @@ void apply_element_forces(const Mesh* mp, int ph, const float* u, float* acc, in
 void acoustic_release(Mesh* mp);
 void acoustic_note_stacey(Mesh* mp, int nf, const int* ispec, const int* ij, const float* jacw, const float* rho_vp);
 void launch_acoustic_pressure_seismograms(Mesh* mp, float* dst, int cur, int use_trick, int potential);
+void launch_acoustic_vector_seismograms(Mesh* mp, float* dst, int cur, int seismotype);
 int acoustic_receivers(Mesh* mp);
 // point arrays in the caller's numbering <-> the device numbering (n2 values per point)
 void points_to_device(const Mesh* mp, float* dev, const float* host, int n2, const char* what);

@@ -194,3 +194,34 @@ def test_luoyang_fluid_solid_against_reference_trace(golden_dir, tmp_path):
     tt, _ = io.read_ascii_trace(path)
     np.testing.assert_array_equal(tt, g["time"])
     sim.close()
+
+
+@pytest.mark.parametrize("seismotype", [1, 2, 3])
+def test_vector_seismograms_in_fluid_and_solid(seismotype):
+    """Displacement, velocity, acceleration (seismotypes 1-3) of receivers in fluid elements (gradient of the
+    potential or its time derivatives over the density) next to receivers in solid elements, rotated by anglerec."""
+    cfg = _coupled_case(1).replace(nstep=600, anglerec=25.0, seismotype=seismotype, receivers=np.array(
+        [[310.0, -90.0], [640.0, -210.0], [1010.0, -299.0], [320.0, -320.0], [650.0, -510.0], [990.0, -301.0]]))
+    sim = Simulation(cfg)
+    seis, t = sim.forward()
+    sim.close()
+    sp, dp = OracleSim(cfg, "sp").forward(), OracleSim(cfg, "dp").forward()
+    r, e_gpu, e_sp = rel(seis, sp), rel(seis, dp), rel(sp, dp)
+    print(f"coupled 24x16, seismotype {seismotype}, 3 fluid + 3 solid receivers: rel-L2 vs FP32 oracle {r:.3e}; vs "
+          f"FP64 oracle {e_gpu:.3e} (FP32 oracle {e_sp:.3e}); loop {t['loop_ms']:.1f} ms")
+    assert e_gpu < 2 * e_sp + TOL
+    for k in range(cfg.nrec):
+        assert rel(seis[k], dp[k]) < 2 * rel(sp[k], dp[k]) + 5 * TOL
+
+
+def test_displacement_in_a_pure_fluid():
+    cfg = config.pressure_source_acoustic_44x44(nstep=600).replace(
+        seismotype=1, use_trick=False, receivers=np.array([[1500.0, 1500.0], [1210.0, 830.0], [900.0, 1600.0]]))
+    sim = Simulation(cfg)
+    seis, _ = sim.forward()
+    sim.close()
+    sp, dp = OracleSim(cfg, "sp").forward(), OracleSim(cfg, "dp").forward()
+    e_gpu, e_sp = rel(seis, dp), rel(sp, dp)
+    print(f"pure fluid 44x44, displacement of 3 receivers, 600 steps: rel-L2 vs FP32 oracle {rel(seis, sp):.3e}, vs "
+          f"FP64 oracle {e_gpu:.3e} (FP32 oracle {e_sp:.3e})")
+    assert e_gpu < 2 * e_sp + TOL

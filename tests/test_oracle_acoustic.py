@@ -17,6 +17,14 @@ What pins each part (oracle/sem2d_oracle.c, acoustic routines):
 * fluid-solid coupling, both directions: the normal-incidence plane-wave reflection coefficient
   (Z2 - Z1) / (Z2 + Z1) of a flat fluid-solid interface with the fluid above, below and beside the solid (a
   one-way or sign-flipped coupling gives +1 or worse);
+* displacement of receivers in fluid elements (grad(potential) / rho): a downgoing plane wave has
+  u_z = +potential_dot / (rho c), an upgoing one -potential_dot / (rho c), and u_x = 0; next to the fluid-solid
+  interface the fluid receivers follow the incident plus reflected wave and the solid ones the transmitted wave
+  (velocity coefficients (Z1 - Z2) / (Z1 + Z2) and 2 Z1 / (Z1 + Z2)); a missing density, a transposed derivative
+  matrix or swapped metric terms fail these by orders of magnitude;
+* velocity and acceleration seismograms (seismotypes 2, 3) in fluid and solid elements: the central-difference
+  Newmark scheme makes them exactly (d[n+1] - d[n-1]) / 2 dt and (d[n+1] - 2 d[n] + d[n-1]) / dt^2 of the
+  displacement seismogram (FP64 oracle, to rounding);
 * acoustic free surface: reflection coefficient -1 for pressure;
 * acoustic Stacey edges: reflection below 1e-3 at normal incidence;
 * acoustic operator: symmetric negative semi-definite, and zero on a constant potential (brute force on a
@@ -184,3 +192,60 @@ def test_luoyang_fluid_solid_forward_bit_exact(golden_dir):
     w = t <= float(g["window_end"])
     assert t[0] == pytest.approx(-o.t0, rel=1e-15) and w.sum() == 1268
     np.testing.assert_array_equal(pot[w], ref[w])
+
+
+def test_fluid_displacement_plane_wave():
+    """Displacement (seismotype 1) of receivers in fluid elements against plane-wave relations of the potential."""
+    cfg0 = config.fluid_solid_plane_wave(nstep=1100)
+    zs = [-1500.0, -1960.0, -1990.0, -1999.0, -2001.0, -2025.0]  # fluid above z = -2000, solid below
+    rec = np.array([[3000.0, z] for z in zs])
+    o6 = OracleSim(cfg0.replace(receivers=rec[:1], seismotype=6), "sp")
+    chi = o6.forward()[0, 0]
+    o1 = OracleSim(cfg0.replace(receivers=rec, seismotype=1), "sp")
+    u = o1.forward()
+    t = np.arange(cfg0.nstep) * cfg0.dt - o1.t0
+    rho1, c1, z2 = 1020.0, 1500.0, 2500.0 * 3000.0
+    z1 = rho1 * c1
+    pred = np.gradient(chi, cfg0.dt) / z1
+    direct, later = t < 0.55, (t > 0.75) & (t < 1.15)
+    uz = u[0, 1]
+
+    def err(a, b, m):
+        return float(np.linalg.norm(a[m] - b[m]) / np.linalg.norm(b[m]))
+
+    e_down, e_up = err(uz, pred, direct), err(uz, -pred, later)
+    ux = float(np.abs(u[:, 0]).max() / np.abs(u[:, 1]).max())
+    # the direct wave at 500 m from the source line, shifted to each receiver near the interface
+    d = np.where(direct, uz, 0.0)
+    rv, tv = (z1 - z2) / (z1 + z2), 2 * z1 / (z1 + z2)
+    window = (t > 0.2) & (t < 0.95)
+    errs = []
+    for k, z in enumerate(zs[1:], 1):
+        if z > -2000.0:
+            p = np.interp(t - (-1500.0 - z) / c1, t, d) + rv * np.interp(t - (2500.0 + z) / c1, t, d)
+        else:
+            p = tv * np.interp(t - 500.0 / c1 - (-2000.0 - z) / 3000.0, t, d)
+        errs.append(err(u[k, 1], p, window))
+    print(f"fluid displacement: direct {e_down:.4f}, reflected {e_up:.4f}, |u_x|/|u_z| {ux:.1e}, near the "
+          f"interface {np.round(errs, 4)}")
+    assert e_down < 0.01 and e_up < 0.01 and ux < 1e-3
+    assert max(errs[:3]) < 0.02 and max(errs[3:]) < 0.01
+
+
+def test_velocity_and_acceleration_seismograms_are_differences_of_displacement():
+    rng = np.random.default_rng(5)
+    rec = np.array([[310.0, -90.0], [640.0, -210.0], [1010.0, -299.0], [320.0, -320.0], [650.0, -510.0]])
+    src = [config.Source(x=430.0, z=-140.0, f0=9.0, factor=1.0e4),
+           config.Source(x=800.0, z=-520.0, f0=7.0, factor=1.0e9, anglesource=35.0, tshift=0.02)]
+    cfg = config.layered_fluid_solid(24, 16, 6, 50.0, 400, 1.0e-3, pert=0.05 * rng.standard_normal(24 * 16 * 25),
+                                     sources=src, receivers=rec).replace(anglerec=25.0)
+    d, v, a = (OracleSim(cfg.replace(seismotype=k), "dp").forward() for k in (1, 2, 3))
+    dt = cfg.dt
+    vd = (d[..., 2:] - d[..., :-2]) / (2 * dt)
+    ad = (d[..., 2:] - 2 * d[..., 1:-1] + d[..., :-2]) / dt ** 2
+    ev = np.abs(v[..., 1:-1] - vd).max() / np.abs(v).max()
+    ea = np.abs(a[..., 1:-1] - ad).max() / np.abs(a).max()
+    print(f"velocity vs central difference {ev:.2e}, acceleration vs second difference {ea:.2e}")
+    assert ev < 1e-8 and ea < 1e-6
+    for k in range(3):  # fluid receivers record a vector too
+        assert np.abs(d[k]).max() > 0 and np.abs(d[k, 0]).max() > 0
