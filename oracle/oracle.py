@@ -79,6 +79,8 @@ def _lib(precision: str):
         lib.orc_kernel_increment.argtypes = [P, dp, dp, dp, dp]
         lib.orc_gradient.argtypes = [P, dp, dp, dp]
         lib.orc_gradient.restype = C.c_double
+        lib.orc_set_attenuation.argtypes = [P, dp, dp, C.c_int, C.c_double]
+        lib.orc_set_attenuation.restype = C.c_int
         for f in ("orc_nspec_acoustic", "orc_ncoupling", "orc_nfree"):
             getattr(lib, f).argtypes = [P]
             getattr(lib, f).restype = C.c_int
@@ -137,6 +139,11 @@ class OracleSim:
         self.nspec_acoustic = self.lib.orc_nspec_acoustic(self.h)
         self.ncoupling = self.lib.orc_ncoupling(self.h)
         self.lib.orc_set_ftz(self.h, int(ftz))
+        if cfg.attenuation:
+            qk, qm = cfg.attenuation_arrays()
+            self._keep_q = (qk, qm)
+            if self.lib.orc_set_attenuation(self.h, qk, qm, int(cfg.n_sls), cfg.attenuation_f0()) != 0:
+                raise ValueError("oracle: invalid attenuation set-up")
 
     def __del__(self):
         try:
@@ -273,8 +280,8 @@ class OracleSim:
         return out
 
     def _elastic_only(self, what):
-        if self.nspec_acoustic:
-            raise NotImplementedError(f"{what}: the oracle's adjoint path covers elastic meshes only")
+        if self.nspec_acoustic or self.cfg.attenuation:
+            raise NotImplementedError(f"{what}: the oracle's adjoint path covers elastic meshes without attenuation only")
 
     def adjoint(self, adj, stacey_store, lastframe, dump_steps=()):
         """SIMULATION_TYPE=3 run with backward reconstruction.  ``adj`` [nrec,2,nstep] adjoint

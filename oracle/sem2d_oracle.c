@@ -113,6 +113,14 @@ typedef struct {
   REAL *displ, *veloc, *accel;/* [nglob][2] */
   REAL *pot, *pot_dot, *pot_dd;/* [nglob] acoustic potential and its time derivatives */
   int ftz;                    /* run the time loop with flush-to-zero */
+  /* viscoelastic attenuation (ATTENUATION_VISCOELASTIC, Newmark): nsls standard linear solids per point, the
+     convolution coefficients of prepare_attenuation.f90:368-397 for the bulk (1) and shear (2) modes, memory
+     variables and the previous step's strains (compute_attenuation_viscoelastic.f90) */
+  int att, att_live, nsls;
+  char *att_on;               /* [nspec*25] Qkappa or Qmu below 9999 */
+  REAL *att_A1, *att_B1, *att_A2, *att_B2; /* [nspec*25][nsls] */
+  REAL *e1, *e11, *e13;       /* [nspec*25][nsls] */
+  REAL *old_xx, *old_zz, *old_xz; /* [nspec*25] */
   int it_done;                /* forward steps already run */
 } orc_sim;
 
@@ -843,7 +851,8 @@ static void setup_receivers(orc_sim *s) {
 /* time step pieces                                                     */
 /* ------------------------------------------------------------------ */
 static void element_forces(orc_sim *s, const REAL *displ, REAL *accel) {
-  /* compute_forces_viscoelastic.F90:34-885, isotropic P-SV, no PML/AXISYM/attenuation */
+  /* compute_forces_viscoelastic.F90:34-885, isotropic P-SV, no PML/AXISYM; viscoelastic attenuation in forward
+     runs (att_live) */
   const REAL (*H)[NGLL] = (const REAL(*)[NGLL])s->hprime;
   const REAL (*HW)[NGLL] = (const REAL(*)[NGLL])s->hprimewgll;
   REAL u[2][NGLL][NGLL]; /* u[c][j][i] */
@@ -881,6 +890,34 @@ static void element_forces(orc_sim *s, const REAL *displ, REAL *accel) {
         REAL sxx = lp2m * dux_dxl + lambdal * duz_dzl;
         REAL szz = lp2m * duz_dzl + lambdal * dux_dxl;
         REAL sxz = mul * (duz_dxl + dux_dzl);
+        if (s->att_live && s->att_on[l]) {
+          /* compute_attenuation_viscoelastic.f90:93-121 (Newmark, convolution memory variables), then the
+             unrelaxed-moduli stress correction of compute_forces_viscoelastic.F90:481-496 */
+          const int N = s->nsls;
+          REAL th = dux_dxl + duz_dzl, th_old = s->old_xx[l] + s->old_zz[l];
+          REAL e1s = 0, e11s = 0, e13s = 0;
+          for (int k = 0; k < N; k++) {
+            size_t q = l * N + k;
+            REAL an = s->att_A1[q];
+            s->e1[q] = an * an * s->e1[q] + s->att_B1[q] * (th + an * th_old);
+            an = s->att_A2[q];
+            s->e11[q] = an * an * s->e11[q] +
+                        s->att_B2[q] * (dux_dxl - th / (REAL)2 + an * (s->old_xx[l] - th_old / (REAL)2));
+            s->e13[q] = an * an * s->e13[q] + s->att_B2[q] * (dux_dzl + duz_dxl + an * s->old_xz[l]);
+          }
+          for (int k = 0; k < N; k++) {
+            e1s = e1s + s->e1[l * N + k];
+            e11s = e11s + s->e11[l * N + k];
+            e13s = e13s + s->e13[l * N + k];
+          }
+          s->old_xx[l] = dux_dxl;
+          s->old_zz[l] = duz_dzl;
+          s->old_xz[l] = dux_dzl + duz_dxl;
+          REAL lpm = lambdal + mul;
+          sxx = sxx + lpm * e1s + (REAL)2 * mul * e11s;
+          sxz = sxz + mul * e13s;
+          szz = szz + lpm * e1s - (REAL)2 * mul * e11s;
+        }
         REAL szx = sxz;
         t1x[j][i] = jacl * (sxx * xixl + szx * xizl);
         t1z[j][i] = jacl * (sxz * xixl + szz * xizl);
@@ -1207,7 +1244,105 @@ void orc_free(void *h) {
   free(s->coup_ac_ispec); free(s->coup_ac_code); free(s->coup_el_ispec); free(s->coup_el_code);
   free(s->abs_jbegin); free(s->abs_jend);
   free(s->pot); free(s->pot_dot); free(s->pot_dd);
+  free(s->att_on); free(s->att_A1); free(s->att_B1); free(s->att_A2); free(s->att_B2);
+  free(s->e1); free(s->e11); free(s->e13); free(s->old_xx); free(s->old_zz); free(s->old_xz);
   free(s);
+}
+
+/* Relaxation times of N standard linear solids whose 1/Q fits 1/Qref over [fmin, fmax] (Hz): the linear least
+   squares step of attenuation_model.f90:443-486 / 762-799 (USE_SOLVOPT false): relaxation frequencies
+   p_l = 2 pi fmin (fmax/fmin)^((l-1)/(N-1)) (exponent in single precision, as remplit_point computes it), 2N-1
+   sample frequencies spaced the same way, model 1/Q(w) = sum_l y_l w (p_l - w/Qref) / (p_l^2 + w^2), normal
+   equations solved by Gaussian elimination with partial pivoting; tau_sigma = 1/p_l,
+   tau_epsilon = tau_sigma (1 + N y_l). */
+static void zener_fit(int N, double Qref, double fmin, double fmax, double *tau_eps, double *tau_sig) {
+  const double two_pi = 6.283185307179586476925286766559005768;
+  double p[16], y[16], A[16][17];
+  if (N == 1) p[0] = sqrt(fmin * fmax);
+  else
+    for (int l = 0; l < N; l++) p[l] = two_pi * pow(fmax / fmin, (double)((float)l / (float)(N - 1))) * fmin;
+  const int m = 2 * N - 1;
+  for (int a = 0; a < N; a++)
+    for (int b = 0; b <= N; b++) A[a][b] = 0.0;
+  for (int k = 0; k < m; k++) {
+    double w = pow(fmax / fmin, (double)((float)k / (float)(m - 1)));
+    w = two_pi * fmin * w;
+    double f[16];
+    for (int l = 0; l < N; l++) f[l] = w * (p[l] - w / Qref) / (p[l] * p[l] + w * w);
+    for (int a = 0; a < N; a++) {
+      for (int b = 0; b < N; b++) A[a][b] += f[a] * f[b];
+      A[a][N] += (1.0 / Qref) * f[a];
+    }
+  }
+  for (int c = 0; c < N; c++) {
+    int piv = c;
+    for (int r = c + 1; r < N; r++)
+      if (fabs(A[r][c]) > fabs(A[piv][c])) piv = r;
+    if (piv != c)
+      for (int b = 0; b <= N; b++) {
+        double t = A[c][b];
+        A[c][b] = A[piv][b];
+        A[piv][b] = t;
+      }
+    for (int r = c + 1; r < N; r++) {
+      double f = A[r][c] / A[c][c];
+      for (int b = c; b <= N; b++) A[r][b] -= f * A[c][b];
+    }
+  }
+  for (int r = N - 1; r >= 0; r--) {
+    double t = A[r][N];
+    for (int b = r + 1; b < N; b++) t -= A[r][b] * y[b];
+    y[r] = t / A[r][r];
+  }
+  for (int l = 0; l < N; l++) {
+    tau_sig[l] = 1.0 / p[l];
+    tau_eps[l] = tau_sig[l] * (1.0 + N * y[l]);
+  }
+}
+
+/* ATTENUATION_VISCOELASTIC with Qkappa, Qmu per GLL point ([nspec][25]) and nsls solids, reference frequency f0
+   (the first source's f0 unless it is a Dirac or Heaviside, prepare_attenuation.f90:319-322); fit band
+   [f0/10, 10 f0] (attenuation_model.f90:95-104).  Points with both Q above 9998.999 do not attenuate.  Returns 0,
+   or 1 for invalid input (nsls outside 1..16, a fluid element, negative relaxation times). */
+int orc_set_attenuation(void *h, const double *qkappa, const double *qmu, int nsls, double f0) {
+  orc_sim *s = (orc_sim *)h;
+  if (nsls < 1 || nsls > 16 || !(f0 > 0.0)) return 1;
+  const size_t ne = (size_t)s->nspec * NGLL2;
+  s->nsls = nsls;
+  s->att_on = (char *)calloc(ne, 1);
+  REAL **arr[] = {&s->att_A1, &s->att_B1, &s->att_A2, &s->att_B2, &s->e1, &s->e11, &s->e13};
+  for (int k = 0; k < 7; k++) *arr[k] = (REAL *)calloc(ne * nsls, sizeof(REAL));
+  s->old_xx = (REAL *)calloc(ne, sizeof(REAL));
+  s->old_zz = (REAL *)calloc(ne, sizeof(REAL));
+  s->old_xz = (REAL *)calloc(ne, sizeof(REAL));
+  const double fmin = f0 / 10.0, fmax = f0 * 10.0;
+  double te[2][16], ts[2][16];
+  for (size_t l = 0; l < ne; l++) {
+    if (qkappa[l] > 9998.999 && qmu[l] > 9998.999) continue;
+    if (s->is_acoustic[l / NGLL2]) return 1;
+    s->att_on[l] = 1;
+    zener_fit(nsls, qkappa[l], fmin, fmax, te[0], ts[0]);
+    zener_fit(nsls, qmu[l], fmin, fmax, te[1], ts[1]);
+    for (int mode = 0; mode < 2; mode++) {
+      double ratio = 0.0;
+      for (int k = 0; k < nsls; k++) {
+        if (te[mode][k] < 0.0 || ts[mode][k] < 0.0) return 1;
+        ratio += te[mode][k] / ts[mode][k];
+      }
+      REAL *A = mode ? s->att_A2 : s->att_A1, *B = mode ? s->att_B2 : s->att_B1;
+      for (int k = 0; k < nsls; k++) {
+        /* attenuation_model.f90:152-159, then prepare_attenuation.f90:385-396 in single precision */
+        REAL inv_tau = (REAL)(1.0 / ts[mode][k]);
+        REAL phi = (REAL)((1.0 - te[mode][k] / ts[mode][k]) / ts[mode][k] / ratio);
+        REAL temp = (REAL)exp(-0.5 * (double)inv_tau * s->p.dt);
+        REAL coef = (REAL)((1.0 - (double)temp) / (double)inv_tau);
+        A[l * nsls + k] = temp;
+        B[l * nsls + k] = phi * coef;
+      }
+    }
+  }
+  s->att = 1;
+  return 0;
 }
 
 int orc_real_size(void) { return (int)sizeof(REAL); }
@@ -1294,6 +1429,7 @@ void orc_forward(void *h, int nstep_run, double *seis, double *stacey_store_d) {
   size_t n2 = (size_t)2 * s->nglob;
   REAL *d = s->displ, *v = s->veloc, *a = s->accel;
   REAL *store = NULL;
+  s->att_live = s->att;
   if (stacey_store_d && s->nabs > 0) store = (REAL *)calloc((size_t)nstep_run * s->nabs * NGLL * 2, sizeof(REAL));
   REAL *pa = s->pot, *pv = s->pot_dot, *pdd = s->pot_dd;
   const size_t n1 = (size_t)s->nglob;
@@ -1344,6 +1480,7 @@ void orc_forward(void *h, int nstep_run, double *seis, double *stacey_store_d) {
     for (size_t k = 0; k < n; k++) stacey_store_d[k] = (double)store[k];
     free(store);
   }
+  s->att_live = 0;
   _mm_setcsr(csr);
 }
 

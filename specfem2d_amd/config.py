@@ -66,6 +66,12 @@ class SimulationConfig:
     anglerec: float = 0.0
     seismotype: int = 1  # 1 displacement, 2 velocity, 3 acceleration, 4 pressure, 6 potential (in fluid elements)
     use_trick: bool = False  # USE_TRICK_FOR_BETTER_PRESSURE
+    # ATTENUATION_VISCOELASTIC: Qkappa, Qmu (scalars or per GLL point; 9999 = no attenuation) and N_SLS; the
+    # reference frequency is the first source's f0
+    attenuation: bool = False
+    qkappa: object = 9999.0
+    qmu: object = 9999.0
+    n_sls: int = 3
     name: str = "custom"
 
     # ------------------------------------------------------------------
@@ -95,6 +101,19 @@ class SimulationConfig:
             a = np.ascontiguousarray(a.reshape(n))
             out.append(a)
         return out
+
+    def attenuation_arrays(self):
+        """Per-point (Qkappa, Qmu) arrays of shape [nspec*25], float64."""
+        n = self.nspec * NGLL2
+        return [np.ascontiguousarray(np.broadcast_to(np.asarray(v, dtype=np.float64).reshape(-1)
+                                                     if np.ndim(v) else float(v), (n,)))
+                for v in (self.qkappa, self.qmu)]
+
+    def attenuation_f0(self) -> float:
+        """ATTENUATION_f0_REFERENCE: the f0 of the first source (prepare_attenuation.f90:319-322)."""
+        if not self.sources:
+            raise ValueError("attenuation needs a source (its f0 is the reference frequency)")
+        return float(self.sources[0].f0)
 
     def gll_point_coords(self):
         """Coordinates (x, z) of every element GLL point, shape [nspec, 25] each.

@@ -234,7 +234,10 @@ def config_from_par_file(data_dir: str, cpml: str = "error", nstep: Optional[int
     refuse(int(g("time_stepping_scheme", "1")) != 1, "time_stepping_scheme other than Newmark")
     refuse(g("MODEL", "default").strip().lower() != "default", f"MODEL = {g('MODEL')}")
     refuse(fortran_bool(g("read_external_mesh", ".false.")), "read_external_mesh (external meshes)")
-    refuse(fortran_bool(g("ATTENUATION_VISCOELASTIC", ".false.")), "ATTENUATION_VISCOELASTIC")
+    attenuation = fortran_bool(g("ATTENUATION_VISCOELASTIC", ".false."))
+    if attenuation:
+        refuse(fortran_bool(g("USE_SOLVOPT", ".false.")), "USE_SOLVOPT (nonlinear fit of the relaxation times)")
+        refuse(fortran_bool(g("READ_VELOCITIES_AT_f0", ".false.")), "READ_VELOCITIES_AT_f0")
     refuse(fortran_bool(g("ATTENUATION_VISCOACOUSTIC", ".false.")), "ATTENUATION_VISCOACOUSTIC")
     refuse(fortran_bool(g("initialfield", ".false.")), "initialfield (plane-wave initial fields)")
     refuse(fortran_bool(g("ADD_PERIODIC_CONDITIONS", ".false.")), "ADD_PERIODIC_CONDITIONS")
@@ -265,11 +268,15 @@ def config_from_par_file(data_dir: str, cpml: str = "error", nstep: Optional[int
     rho = np.zeros(nx * nz)
     vp = np.zeros(nx * nz)
     vs = np.zeros(nx * nz)
+    qk = np.full(nx * nz, 9999.0)
+    qm = np.full(nx * nz, 9999.0)
     for m, vals in par.materials.items():
         icode = int(vals[0])
         refuse(icode != 1, f"material {m} with icodemat {icode} (only isotropic tables)")
         sel = mat == m
         rho[sel], vp[sel], vs[sel] = vals[1], vals[2], vals[3]
+        if attenuation and len(vals) > 7:  # read_material_table.f90: QKappa, Qmu are the 7th and 8th values
+            qk[sel], qm[sel] = vals[6], vals[7]
     rep = lambda a: np.repeat(a, NGLL2)  # noqa: E731
     nsrc = int(g("NSOURCES"))
     sources = read_source_file(os.path.join(data_dir, "SOURCE"), nsrc)
@@ -285,7 +292,8 @@ def config_from_par_file(data_dir: str, cpml: str = "error", nstep: Optional[int
         absorbleft=fortran_bool(g("absorbleft")), sources=sources,
         receivers=np.array([[s.x, s.z] for s in stations]) if stations else None,
         user_t0=fortran_float(g("USER_T0", "0")), anglerec=fortran_float(g("anglerec", "0")),
-        seismotype=seismotypes[0], use_trick=fortran_bool(g("USE_TRICK_FOR_BETTER_PRESSURE", ".false.")))
+        seismotype=seismotypes[0], use_trick=fortran_bool(g("USE_TRICK_FOR_BETTER_PRESSURE", ".false.")),
+        attenuation=attenuation, qkappa=rep(qk), qmu=rep(qm), n_sls=int(g("N_SLS", "3")))
     return cfg, stations
 
 

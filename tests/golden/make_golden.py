@@ -107,3 +107,8 @@ if __name__ == "__main__":
     copy_inputs("LuoYang_fluid_solid_kernel", ["DATA/Par_file", "DATA/SOURCE", "DATA/interfaces.dat"],
                 "LuoYang_fluid_solid_kernel")
     pack("Tromp2005_kernel", "REF_KERNEL", ["AA.S0001"], "tromp2005_kernel_ref_kernel_traces.npz")
+    pack("check_absolute_amplitude_of_force_source_seismograms_viscoelastic", "REF_SEIS", ["AA.S0001"],
+         "force_source_viscoelastic_ref_seis.npz")
+    copy_inputs("check_absolute_amplitude_of_force_source_seismograms_viscoelastic",
+                ["REF_SEIS/Par_file", "REF_SEIS/SOURCE", "DATA/interfaces_attenuation_analytic.dat"],
+                "force_source_viscoelastic")
