@@ -230,6 +230,34 @@ def c2_coupled(steps: int = 100, warmup: int = 10):
     return out
 
 
+def c2_attenuation(steps: int = 50, warmup: int = 5):
+    """Forward of the C2 elastic mesh (2000 x 1000 elements of 10 m, the random-field model) with viscoelastic
+    attenuation everywhere (Qkappa 60, Qmu 30, 3 standard linear solids): GLL-point updates per second, and the
+    element-force launches' share of the step with their algorithmic traffic, 4300 B per element (25 points x
+    (ibool 4 B, geometry 16 B, moduli 8 B, attenuation 144 B: 12 coefficients read, 9 memory variables and 3
+    previous strains read and written)) plus 24 B per point (displacement read, acceleration read-modify-write)."""
+    import torch
+    from specfem2d_amd.driver import Simulation
+    cfg = make_config("c2_mesh_elastic_2000x1000", warmup + steps).replace(attenuation=True, qkappa=60.0, qmu=30.0)
+    sim = Simulation(cfg)
+    sim.forward(nstep=warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    seis, timing = sim.forward(nstep=steps, phase_timing=True)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    traffic = 4300 * sim.nspec + 24 * sim.nglob
+    forces_s = timing["forces_ms"] * 1e-3 / steps
+    out = {"workload": "c2_mesh_elastic_2000x1000 + attenuation (Qkappa 60, Qmu 30, N_SLS 3)", "nspec": sim.nspec,
+           "nglob": sim.nglob, "steps": steps, "ms_per_step": wall * 1e3 / steps, "value": sim.nglob * steps / wall,
+           "unit": "GLL-pt updates/s", "forces_ms_per_step": forces_s * 1e3,
+           "forces_roofline": {"bound": "hbm", "achieved": traffic / forces_s / 1e9, "peak": 8000.0, "unit": "GB/s",
+                               "frac": traffic / forces_s / 8e12, "traffic": None},
+           "finite": bool(np.all(np.isfinite(seis)))}
+    sim.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -245,6 +273,9 @@ def main():
     ap.add_argument("--batch-shots", type=int, default=32,
                     help="shots per batch of the multi-shot FWI gradient (32 fill about 170 GB of checkpoints)")
     ap.add_argument("--no-coupled", action="store_true", help="skip the coupled fluid-solid C2 forward")
+    ap.add_argument("--no-attenuation", action="store_true", help="skip the C2 forward with attenuation")
+    ap.add_argument("--attenuation-only", action="store_true",
+                    help="only the C2 forward with viscoelastic attenuation (profiling); prints its JSON object")
     ap.add_argument("--coupled-only", action="store_true",
                     help="only the coupled fluid-solid C2 forward (profiling); prints its JSON object")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -264,6 +295,9 @@ def main():
 
     if args.fwi_only:
         print(json.dumps(fwi_gradient_batched(args.batch_shots, nstep=args.fwi_steps)))
+        return
+    if args.attenuation_only:
+        print(json.dumps(c2_attenuation(min(args.steps, 50), min(args.warmup, 5))))
         return
     if args.coupled_only:
         print(json.dumps(c2_coupled(min(args.steps, 100), min(args.warmup, 10))))
@@ -343,6 +377,8 @@ def main():
         copy_calibration(args.copy_calibration_mib)
     if not args.no_coupled:
         out["c2_coupled"] = c2_coupled()
+    if not args.no_attenuation:
+        out["c2_attenuation"] = c2_attenuation()
     if not args.no_gradient:
         out["fwi_gradient"] = fwi_gradient()
         out["fwi_gradient_batched"] = fwi_gradient_batched(args.batch_shots, nstep=args.fwi_steps)

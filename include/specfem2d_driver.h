@@ -71,6 +71,12 @@ int sf2d_sim_nshot(void* sim);                      /* 0 for a single simulation
 void sf2d_sim_shot_t0(void* sim, double* t0);       /* [nshot] */
 void sf2d_sim_shot_misfits(void* sim, double* chi); /* [nshot], from the last gradient forward */
 void sf2d_sim_batch_get_fields(void* sim, int shot, float* displ, float* veloc, float* accel);
+/* ATTENUATION_VISCOELASTIC (before sf2d_sim_prepare_device): quality factors Qkappa, Qmu per GLL point
+   ([nspec][25]; both above 9998.999 = no attenuation at that point), n_sls standard linear solids (3, as the
+   reference's GPU code requires), reference frequency the first source's f0.  The relaxation times come from the
+   least-squares Zener fit of attenuation_model.f90 (USE_SOLVOPT false) and the coefficients from
+   prepare_attenuation.f90; forward runs of solid meshes only (as prepare_gpu.f90:69-72).  Returns 0 on success. */
+int sf2d_sim_set_attenuation(void* sim, const double* qkappa, const double* qmu, int n_sls);
 /* Hands the setup to the GPU library in the order of prepare_gpu.f90:80-347. Returns 0 on success. */
 int sf2d_sim_prepare_device(void* sim);
 void sf2d_sim_destroy(void* sim);

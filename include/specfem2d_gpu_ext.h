@@ -18,6 +18,9 @@ void* sf2d_mesh_stream(long* Mesh_pointer);
 void sf2d_mesh_flush(long* Mesh_pointer);
 /* sf2d_mesh_flush, then waits for the handle's stream */
 void sf2d_mesh_synchronize(long* Mesh_pointer);
+/* zeroes the memory variables and previous strains of viscoelastic attenuation (a run restarted from rest);
+   no effect without attenuation */
+void sf2d_attenuation_reset(long* Mesh_pointer);
 /* elements whose geometry (xix, xiz, gammax, gammaz) has the same values at all 25 points; in patch assembly,
    patches made only of such elements read one geometry record per element (SF2D_NO_AFFINE disables it) */
 int sf2d_mesh_affine_elements(long* Mesh_pointer);

@@ -509,15 +509,18 @@ void apply_element_forces(const Mesh* mp, int ph, const float* u, float* acc, in
     return;
   }
   if (first_touch) fail("first-touch assembly needs the chain kernel");
+  // the memory variables of viscoelastic attenuation follow the forward wavefield only
+  const int att = mp->attenuation && acc == mp->d_accel;
+  if (att && nshot != 1) fail("viscoelastic attenuation: one wavefield per call");
   for (int s = 0; s < nshot; s++) {
     const float* us = u + 2 * s * shot_stride;
     float* as = acc + 2 * s * shot_stride;
     if (mp->assembly == 2) {
-      launch_elastic_forces(mp, mp->colour_start[ph][0], mp->phase_count[ph], 1, us, as);
+      launch_elastic_forces(mp, mp->colour_start[ph][0], mp->phase_count[ph], 1, us, as, att);
       continue;
     }
     for (int c = 0; c < mp->ncolour[ph]; c++)
-      launch_elastic_forces(mp, mp->colour_start[ph][c], mp->colour_count[ph][c], 0, us, as);
+      launch_elastic_forces(mp, mp->colour_start[ph][c], mp->colour_count[ph][c], 0, us, as, att);
   }
 }
 
@@ -671,11 +674,18 @@ void prepare_fields_elastic_device_(long* Mesh_pointer, realw* rmassx, realw* rm
                                     realw* h_B_newmark_kappa) {  // prepare_mesh_constants_cuda.cu:527
   (void)ispec_is_anisotropic; (void)c11store; (void)c12store; (void)c13store; (void)c15store; (void)c23store;
   (void)c25store; (void)c33store; (void)c35store; (void)c55store; (void)h_inum_interfaces_elastic;
-  (void)h_A_newmark_mu; (void)h_B_newmark_mu; (void)h_A_newmark_kappa; (void)h_B_newmark_kappa;
   Mesh* mp = handle(Mesh_pointer);
   if (*ANISOTROPY) fail("anisotropic elements are not supported by this library yet");
-  if (*ATTENUATION_VISCOELASTIC) fail("viscoelastic attenuation is not supported by this library yet");
   if (*h_ninterface_elastic > 0) fail("MPI elastic interfaces passed: one GPU, one partition only");
+  if (*ATTENUATION_VISCOELASTIC) {
+    // as the reference's GPU code (prepare_gpu.f90:69-72): forward runs of solid meshes, 3 solids
+    if (mp->simulation_type != 1) fail("viscoelastic attenuation: forward simulations (SIMULATION_TYPE=1) only");
+    if (mp->nspec_acoustic > 0) fail("viscoelastic attenuation: coupled fluid-solid meshes are not supported");
+    if (!h_A_newmark_mu || !h_B_newmark_mu || !h_A_newmark_kappa || !h_B_newmark_kappa)
+      fail("viscoelastic attenuation: coefficient arrays missing");
+    mp->attenuation = 1;
+    if (mp->assembly == 0) mp->assembly = 1;  // coloured elements: the element kernel carries the memory variables
+  }
   const size_t n2 = (size_t)2 * mp->nglob;
   mp->d_displ = device_zeros<float>(n2);
   mp->d_veloc = device_zeros<float>(n2);
@@ -683,6 +693,22 @@ void prepare_fields_elastic_device_(long* Mesh_pointer, realw* rmassx, realw* rm
   const int np = *num_phase_ispec_elastic;
   mp->num_phase_ispec_elastic = np;
   sort_elements_by_colour(mp, phase_ispec_inner_elastic, np);
+  if (mp->attenuation) {
+    // host arrays A(N_SLS, NGLLX, NGLLZ, nspec) (solid fastest, ispec order) -> [plane][ne] in device order
+    const size_t ne = (size_t)mp->nspec * NGLL2;
+    std::vector<float> coef(12 * ne);
+    const realw* src[4] = {h_A_newmark_kappa, h_B_newmark_kappa, h_A_newmark_mu, h_B_newmark_mu};
+    for (int e = 0; e < mp->nspec; e++) {
+      const size_t row = (size_t)mp->pos_of_ispec[e] * NGLL2;
+      for (int p = 0; p < NGLL2; p++)
+        for (int k = 0; k < 3; k++)
+          for (int w = 0; w < 4; w++)
+            coef[(size_t)(4 * k + w) * ne + row + p] = src[w][((size_t)e * NGLL2 + p) * 3 + k];
+    }
+    mp->d_att_coef = device_copy(coef.data(), coef.size());
+    mp->d_att_e = device_zeros<float>(9 * ne);
+    mp->d_att_old = device_zeros<float>(3 * ne);
+  }
   mp->d_rmassx = device_zeros<float>((size_t)mp->nglob);
   mp->d_rmassz = device_zeros<float>((size_t)mp->nglob);
   points_to_device(mp, mp->d_rmassx, rmassx, 1, "rmassx H2D");
@@ -695,7 +721,8 @@ void prepare_fields_elastic_adj_dev_(long* Mesh_pointer, int* size_f, int* APPRO
                                      int* NO_BACKWARD_RECONSTRUCTION) {  // prepare_mesh_constants_cuda.cu:711
   Mesh* mp = handle(Mesh_pointer);
   if (*APPROXIMATE_HESS_KL) fail("approximate Hessian kernels are not supported on the GPU (as in the reference)");
-  if (*ATTENUATION_VISCOELASTIC) fail("viscoelastic attenuation is not supported by this library yet");
+  if (*ATTENUATION_VISCOELASTIC)
+    fail("viscoelastic attenuation with adjoint simulations is not supported (as in the reference's GPU code)");
   if (*NO_BACKWARD_RECONSTRUCTION) fail("NO_BACKWARD_RECONSTRUCTION is not supported by this library");
   if (*size_f != 2 * mp->nglob) fail("prepare_fields_elastic_adj_dev: size %d != 2*nglob", *size_f);
   const size_t n2 = (size_t)2 * mp->nglob;
@@ -796,7 +823,7 @@ void prepare_cleanup_device_(long* Mesh_pointer, int* ACOUSTIC_SIMULATION, int* 
   (void)hipStreamSynchronize(mp->stream);
   acoustic_release(mp);
   device_free(mp->d_ibool); device_free(mp->d_xix); device_free(mp->d_xiz); device_free(mp->d_gammax);
-  device_free(mp->d_gammaz); device_free(mp->d_kappav); device_free(mp->d_muv); device_free(mp->d_ispec_is_elastic);
+  device_free(mp->d_gammaz); device_free(mp->d_kappav); device_free(mp->d_muv); device_free(mp->d_att_coef); device_free(mp->d_att_e); device_free(mp->d_att_old); device_free(mp->d_ispec_is_elastic);
   device_free(mp->d_displ); device_free(mp->d_veloc); device_free(mp->d_accel);
   device_free(mp->d_b_displ); device_free(mp->d_b_veloc); device_free(mp->d_b_accel);
   device_free(mp->d_rmassx); device_free(mp->d_rmassz);
@@ -991,7 +1018,8 @@ void compute_forces_viscoelastic_cuda_(long* Mesh_pointer, int* iphase, realw* d
   Mesh* mp = handle(Mesh_pointer);
   flush_correctors(mp);
   if (*ANISOTROPY) fail("anisotropic elements are not supported by this library yet");
-  if (*ATTENUATION_VISCOELASTIC) fail("viscoelastic attenuation is not supported by this library yet");
+  if ((*ATTENUATION_VISCOELASTIC != 0) != (mp->attenuation != 0))
+    fail("compute_forces_viscoelastic_cuda: ATTENUATION_VISCOELASTIC differs from prepare_fields_elastic_device");
   const int ph = (*iphase == 1) ? 0 : 1;
   const int nelem = (*iphase == 1) ? *nspec_outer_elastic : *nspec_inner_elastic;
   const int first_touch = mp->accel_partial && *compute_wavefield_1;
@@ -1345,6 +1373,14 @@ extern "C" void sf2d_mesh_synchronize(long* Mesh_pointer) {
 }
 
 extern "C" void sf2d_mesh_flush(long* Mesh_pointer) { flush_correctors(handle(Mesh_pointer)); }
+
+extern "C" void sf2d_attenuation_reset(long* Mesh_pointer) {
+  Mesh* mp = handle(Mesh_pointer);
+  if (!mp->attenuation) return;
+  const size_t ne = (size_t)mp->nspec * NGLL2;
+  check_hip(hipMemsetAsync(mp->d_att_e, 0, 9 * ne * sizeof(float), mp->stream), "attenuation reset");
+  check_hip(hipMemsetAsync(mp->d_att_old, 0, 3 * ne * sizeof(float), mp->stream), "attenuation reset");
+}
 
 extern "C" int sf2d_mesh_affine_elements(long* Mesh_pointer) {
   Mesh* mp = handle(Mesh_pointer);

@@ -10,11 +10,13 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <limits>
+#include <map>
 #include <vector>
 
 #include "specfem2d_driver.h"
@@ -108,6 +110,9 @@ void lagrange(double xi, const double* xg, double* h, double* hp = nullptr) {
 struct Sim {
   sf2d_params p{};
   int nspec = 0, nglob = 0;
+  // viscoelastic attenuation: convolution coefficients per point and solid, [nspec][25][3] (solid fastest)
+  int attenuation = 0;
+  std::vector<float> att_A_kappa, att_B_kappa, att_A_mu, att_B_mu;
   int simulation_type = 1, save_forward = 0;
   Basis basis{};
   // control nodes of the structured grid
@@ -786,10 +791,12 @@ void prepare_device(Sim& s) {
     for (int e = 0, q = 0; e < s.nspec; e++)
       if (is_elastic[e]) s.phase_ispec[(size_t)num_phase + q++] = e + 1;
     std::vector<int> aniso(s.nspec, 0);
+    int att = s.attenuation;
     prepare_fields_elastic_device_(&s.mesh, s.rmassx.data(), s.rmassz.data(), &num_phase, s.phase_ispec.data(),
                                    aniso.data(), &zero, &dummy_f, &dummy_f, &dummy_f, &dummy_f, &dummy_f, &dummy_f,
-                                   &dummy_f, &dummy_f, &dummy_f, &zero, &dummy_i, &zero, &dummy_f, &dummy_f, &dummy_f,
-                                   &dummy_f);
+                                   &dummy_f, &dummy_f, &dummy_f, &zero, &dummy_i, &att,
+                                   att ? s.att_A_mu.data() : &dummy_f, att ? s.att_B_mu.data() : &dummy_f,
+                                   att ? s.att_A_kappa.data() : &dummy_f, att ? s.att_B_kappa.data() : &dummy_f);
   }
   if (s.simulation_type == 3) {
     int size = 2 * s.nglob;
@@ -813,6 +820,74 @@ void prepare_device(Sim& s) {
     if (s.simulation_type == 3) transfer_b_fields_to_device_(&size, zeros.data(), zeros.data(), zeros.data(), &s.mesh);
   }
   if (s.nshot > 0) sf2d_batch_prepare(&s.mesh, s.nshot, s.src_shot.data());
+}
+
+// Relaxation times of n standard linear solids fitting 1/Q over [fmin, fmax] Hz by linear least squares, as
+// attenuation_model.f90:443-486, 636-799 does with USE_SOLVOPT false: relaxation angular frequencies and 2n-1
+// sample frequencies log-spaced over the band (exponents formed in single precision, as remplit_point and
+// classical_linear_least_squares form them), basis w (p - w/Q) / (p^2 + w^2) per solid, the normal equations
+// solved here by a Cholesky factorisation.  Returns false when the fit is not positive definite.
+bool zener_relaxation(int n, double q, double fmin, double fmax, double* tau_eps, double* tau_sig) {
+  const double two_pi = 2.0 * 3.141592653589793;
+  double p[3], m[3][3] = {}, r[3] = {};
+  for (int l = 0; l < n; l++) p[l] = two_pi * std::pow(fmax / fmin, (double)((float)l / (float)(n - 1))) * fmin;
+  const int nk = 2 * n - 1;
+  for (int k = 0; k < nk; k++) {
+    const double w = two_pi * fmin * std::pow(fmax / fmin, (double)((float)k / (float)(nk - 1)));
+    double b[3];
+    for (int l = 0; l < n; l++) b[l] = w * (p[l] - w / q) / (p[l] * p[l] + w * w);
+    for (int a = 0; a < n; a++) {
+      r[a] += b[a] / q;
+      for (int c = 0; c <= a; c++) m[a][c] += b[a] * b[c];
+    }
+  }
+  double L[3][3] = {};
+  for (int a = 0; a < n; a++)
+    for (int c = 0; c <= a; c++) {
+      double t = m[a][c];
+      for (int k = 0; k < c; k++) t -= L[a][k] * L[c][k];
+      if (a == c) {
+        if (!(t > 0.0)) return false;
+        L[a][a] = std::sqrt(t);
+      } else {
+        L[a][c] = t / L[c][c];
+      }
+    }
+  double z[3], y[3];
+  for (int a = 0; a < n; a++) {
+    double t = r[a];
+    for (int k = 0; k < a; k++) t -= L[a][k] * z[k];
+    z[a] = t / L[a][a];
+  }
+  for (int a = n - 1; a >= 0; a--) {
+    double t = z[a];
+    for (int k = a + 1; k < n; k++) t -= L[k][a] * y[k];
+    y[a] = t / L[a][a];
+  }
+  for (int l = 0; l < n; l++) {
+    tau_sig[l] = 1.0 / p[l];
+    tau_eps[l] = tau_sig[l] * (1.0 + n * y[l]);
+    if (tau_sig[l] < 0.0 || tau_eps[l] < 0.0) return false;
+  }
+  return true;
+}
+
+// the single-precision recursive-convolution coefficients of one mode (prepare_attenuation.f90:368-397 after
+// attenuation_model.f90:152-159): A = exp(-dt / (2 tau_sigma)), B = phi (1 - A) tau_sigma
+bool newmark_coefficients(double q, double f0, double dt, float* A, float* B) {
+  double te[3], ts[3];
+  if (!zener_relaxation(3, q, f0 / 10.0, f0 * 10.0, te, ts)) return false;
+  double ratio = 0.0;
+  for (int k = 0; k < 3; k++) ratio += te[k] / ts[k];
+  for (int k = 0; k < 3; k++) {
+    const float inv_tau = (float)(1.0 / ts[k]);
+    const float phi = (float)((1.0 - te[k] / ts[k]) / ts[k] / ratio);
+    const float a = (float)std::exp(-0.5 * (double)inv_tau * dt);
+    const float c = (float)((1.0 - (double)a) / (double)inv_tau);
+    A[k] = a;
+    B[k] = phi * c;
+  }
+  return true;
 }
 
 void* create(const sf2d_params* p, const double* rho, const double* vp, const double* vs, int nsrc,
@@ -982,7 +1057,7 @@ int sf2d_sim_forward(void* sim, int nstep_run, double* seis, float* absorb, doub
   float deltatover2 = (float)(0.5 * (double)deltat);
   float deltatsqover2 = (float)(0.5 * (double)deltat * (double)deltat);
   float b_deltat = 0.f, b_deltatover2 = 0.f, b_deltatsqover2 = 0.f;
-  int nspec_outer = 0, nspec_inner = s->nspec, zero = 0, one = 1;
+  int nspec_outer = 0, nspec_inner = s->nspec, zero = 0, one = 1, att = s->attenuation;
   const int it_begin = s->it_done + 1;
   int i_sig = 1, nlength = s->p.nstep, it_end = s->it_done + nstep_run;
   const bool any_ac = s->nspec_ac > 0, any_el = s->nspec_ac < s->nspec;
@@ -1049,7 +1124,7 @@ int sf2d_sim_forward(void* sim, int nstep_run, double* seis, float* absorb, doub
     // then the solid, coupled to the new fluid pressure (compute_gpu_elastic.f90:34-215)
     for (int iphase = 1; iphase <= 2 && any_el; iphase++) {
       if (iphase == 2) mark(step, 0);
-      compute_forces_viscoelastic_cuda_(&s->mesh, &iphase, &deltat, &nspec_outer, &nspec_inner, &zero, &zero, &one,
+      compute_forces_viscoelastic_cuda_(&s->mesh, &iphase, &deltat, &nspec_outer, &nspec_inner, &zero, &att, &one,
                                         &zero);
       if (iphase == 2) mark(step, 1);
       if (iphase == 1) {
@@ -1185,7 +1260,7 @@ int sf2d_sim_adjoint(void* sim, const float* adj, const float* absorb, const flo
   float b_deltat = -deltat;
   float b_deltatover2 = (float)(0.5 * (double)b_deltat);
   float b_deltatsqover2 = (float)(0.5 * (double)b_deltat * (double)b_deltat);
-  int nspec_outer = 0, nspec_inner = s->nspec, zero = 0, one = 1;
+  int nspec_outer = 0, nspec_inner = s->nspec, zero = 0, one = 1, att = s->attenuation;
   int i_sig = 1, nlength = nstep, it_end = nstep, nadj = s->nrec, ns = nstep;
   int elastic_sim = 1, acoustic_sim = 0;
   s->sisux.assign((size_t)std::max(1, s->nrec) * nlength, 0.0);
@@ -1199,7 +1274,7 @@ int sf2d_sim_adjoint(void* sim, const float* adj, const float* absorb, const flo
                               &b_deltatover2);
     const int it_tmp = nstep - it + 1;
     for (int iphase = 1; iphase <= 2; iphase++) {
-      compute_forces_viscoelastic_cuda_(&s->mesh, &iphase, &deltat, &nspec_outer, &nspec_inner, &zero, &zero, &one,
+      compute_forces_viscoelastic_cuda_(&s->mesh, &iphase, &deltat, &nspec_outer, &nspec_inner, &zero, &att, &one,
                                         &one);
       if (iphase == 1) {
         if (s->p.stacey && s->nabs > 0) {
@@ -1275,7 +1350,9 @@ int sf2d_sim_set_model(void* sim, const double* rho, const double* vp, const dou
 
 int sf2d_sim_gradient_forward(void* sim, const double* obs, int chunk, double* chi, double* seis) {
   Sim* s = (Sim*)sim;
-  if (!s || !s->mesh || s->simulation_type != 1 || chunk < 1 || s->nspec_ac > 0 || s->p.seismotype != 1) return 1;
+  if (!s || !s->mesh || s->simulation_type != 1 || chunk < 1 || s->nspec_ac > 0 || s->p.seismotype != 1 ||
+      s->attenuation)
+    return 1;
   const int N = s->p.nstep, R = s->nrec, S = std::max(1, s->nshot);
   s->grad_chunk = std::min(chunk, N);
   std::vector<double> sm((size_t)S * std::max(1, R) * 2 * N, 0.0);
@@ -1388,9 +1465,46 @@ int sf2d_sim_gradient_backward(void* sim, double* grad, double* parts) {
   return 0;
 }
 
+int sf2d_sim_set_attenuation(void* sim, const double* qkappa, const double* qmu, int n_sls) {
+  Sim* s = (Sim*)sim;
+  if (!s || s->mesh || !qkappa || !qmu) return 1;
+  if (n_sls != 3 || s->nspec_ac > 0 || s->simulation_type != 1 || s->nshot > 0 || s->src.empty()) {
+    fprintf(stderr, "sf2d_sim_set_attenuation: 3 solids, forward runs of solid meshes with a source only\n");
+    return 1;
+  }
+  const double f0 = s->src[0].f0;  // ATTENUATION_f0_REFERENCE (prepare_attenuation.f90:319-322)
+  const size_t ne = (size_t)s->nspec * NGLL2;
+  for (auto* v : {&s->att_A_kappa, &s->att_B_kappa, &s->att_A_mu, &s->att_B_mu}) v->assign(3 * ne, 0.f);
+  std::map<double, std::array<float, 6>> cache;  // Q -> A[3], B[3]
+  auto last = cache.end();
+  auto coefficients = [&](double q, float* A, float* B) {
+    auto it = (last != cache.end() && last->first == q) ? last : cache.find(q);
+    if (it == cache.end()) {
+      std::array<float, 6> c{};
+      if (!newmark_coefficients(q, f0, s->p.dt, c.data(), c.data() + 3)) return false;
+      it = cache.emplace(q, c).first;
+    }
+    last = it;
+    std::copy(it->second.begin(), it->second.begin() + 3, A);
+    std::copy(it->second.begin() + 3, it->second.end(), B);
+    return true;
+  };
+  for (size_t l = 0; l < ne; l++) {
+    if (qkappa[l] > 9998.999 && qmu[l] > 9998.999) continue;  // no attenuation at this point: A = B = 0
+    if (!(qkappa[l] > 0.0 && qmu[l] > 0.0) || !coefficients(qkappa[l], &s->att_A_kappa[3 * l], &s->att_B_kappa[3 * l]) ||
+        !coefficients(qmu[l], &s->att_A_mu[3 * l], &s->att_B_mu[3 * l])) {
+      fprintf(stderr, "sf2d_sim_set_attenuation: invalid quality factors at point %zu\n", l);
+      return 1;
+    }
+  }
+  s->attenuation = 1;
+  return 0;
+}
+
 void sf2d_sim_reset(void* sim) {
   Sim* s = (Sim*)sim;
   if (!s || !s->mesh) return;
+  sf2d_attenuation_reset(&s->mesh);
   std::vector<float> zeros((size_t)2 * s->nglob, 0.f);
   int size = 2 * s->nglob, nac = s->nglob;
   if (s->nspec_ac > 0) transfer_fields_ac_to_device_(&nac, zeros.data(), zeros.data(), zeros.data(), &s->mesh);

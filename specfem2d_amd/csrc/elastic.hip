@@ -147,12 +147,18 @@ __global__ __launch_bounds__(256) void k_zero_points(float2* __restrict__ a, con
 // isotropic elastic element forces, P-SV, for the elements at positions [first, first + nelem).
 // ATOMIC = false: the range is one colour (no two elements share a point), so the assembly is a plain
 // read-modify-write; ATOMIC = true: FP32 atomics, valid for any range.
-template <bool ATOMIC>
+// ATT: viscoelastic attenuation with 3 standard linear solids -- the memory variables advance with the
+// recursive convolution of compute_attenuation_viscoelastic.f90:93-121 (Kernel_2_viscoelastic_impl.cu:1019-1065
+// on the reference's GPU) and correct the stress with the unrelaxed moduli; coef, e and old are [plane][ne]
+// arrays in device element order (planes: A_kappa, B_kappa, A_mu, B_mu per solid; e1, e11, e13 per solid;
+// previous dux_dxl, duz_dzl, dux_dzl + duz_dxl)
+template <bool ATOMIC, bool ATT>
 __global__ __launch_bounds__(FORCE_THREADS) void k_elastic_forces(
     int first, int nelem, int nbatch, int per_xcd, const int* __restrict__ ibool,
     const float* __restrict__ xix, const float* __restrict__ xiz, const float* __restrict__ gammax,
     const float* __restrict__ gammaz, const float* __restrict__ kappav, const float* __restrict__ muv,
-    const float2* __restrict__ displ, float* __restrict__ accel, const Basis basis) {
+    const float2* __restrict__ displ, float* __restrict__ accel, const Basis basis,
+    const float* __restrict__ att_coef, float* __restrict__ att_e, float* __restrict__ att_old, long ne) {
   // XCD-aware remap: blocks b, b+8, b+16, ... (one XCD) take consecutive batches
   const int bid = blockIdx.x;
   const int batch = (bid % NXCD) * per_xcd + bid / NXCD;
@@ -212,9 +218,39 @@ __global__ __launch_bounds__(FORCE_THREADS) void k_elastic_forces(
     const float jac = 1.f / (xixl * gazl - gaxl * xizl);
     const float lambdal = kap - mu;
     const float lp2m = kap + mu;
-    const float sxx = lp2m * dux_dxl + lambdal * duz_dzl;
-    const float szz = lp2m * duz_dzl + lambdal * dux_dxl;
-    const float sxz = mu * (duz_dxl + dux_dzl);
+    float sxx = lp2m * dux_dxl + lambdal * duz_dzl;
+    float szz = lp2m * duz_dzl + lambdal * dux_dxl;
+    float sxz = mu * (duz_dxl + dux_dzl);
+    if (ATT) {
+      const size_t l = (size_t)(first + e) * NGLL2 + p;
+      const float th = dux_dxl + duz_dzl;
+      const float oxx = att_old[l], ozz = att_old[ne + l], oxz = att_old[2 * ne + l];
+      const float th_old = oxx + ozz;
+      float e1s = 0.f, e11s = 0.f, e13s = 0.f;
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        const float a1 = att_coef[(size_t)(4 * k) * ne + l], b1 = att_coef[(size_t)(4 * k + 1) * ne + l];
+        const float a2 = att_coef[(size_t)(4 * k + 2) * ne + l], b2 = att_coef[(size_t)(4 * k + 3) * ne + l];
+        float* e1p = att_e + (size_t)(3 * k) * ne + l;
+        float* e11p = att_e + (size_t)(3 * k + 1) * ne + l;
+        float* e13p = att_e + (size_t)(3 * k + 2) * ne + l;
+        const float e1 = a1 * a1 * *e1p + b1 * (th + a1 * th_old);
+        const float e11 = a2 * a2 * *e11p + b2 * (dux_dxl - 0.5f * th + a2 * (oxx - 0.5f * th_old));
+        const float e13 = a2 * a2 * *e13p + b2 * (dux_dzl + duz_dxl + a2 * oxz);
+        *e1p = e1;
+        *e11p = e11;
+        *e13p = e13;
+        e1s += e1;
+        e11s += e11;
+        e13s += e13;
+      }
+      att_old[l] = dux_dxl;
+      att_old[ne + l] = duz_dzl;
+      att_old[2 * ne + l] = dux_dzl + duz_dxl;
+      sxx += kap * e1s + 2.f * mu * e11s;
+      sxz += mu * e13s;
+      szz += kap * e1s - 2.f * mu * e11s;
+    }
     s_t1x[tid] = jac * (sxx * xixl + sxz * xizl);
     s_t1z[tid] = jac * (sxz * xixl + szz * xizl);
     s_t2x[tid] = jac * (sxx * gaxl + sxz * gazl);
@@ -1179,20 +1215,21 @@ void launch_zero_points(const Mesh* mp, float* accel, int nshot, long shot_strid
   check_hip(hipGetLastError(), "k_zero_points launch");
 }
 
-void launch_elastic_forces(const Mesh* mp, int first, int nelem, int atomic, const float* displ, float* accel) {
+void launch_elastic_forces(const Mesh* mp, int first, int nelem, int atomic, const float* displ, float* accel,
+                           int att) {
   if (nelem <= 0) return;
   if (first < 0 || first + nelem > mp->nspec) fail("element range [%d, %d) outside the mesh", first, first + nelem);
+  if (att && (!mp->d_att_coef || !mp->d_att_e || !mp->d_att_old)) fail("attenuation arrays not prepared");
   const int nbatch = (nelem + FORCE_EPB - 1) / FORCE_EPB;
   const int per_xcd = (nbatch + NXCD - 1) / NXCD;
   const int nblocks = per_xcd * NXCD;
-  if (atomic)
-    hipLaunchKernelGGL(k_elastic_forces<true>, dim3(nblocks), dim3(FORCE_THREADS), 0, mp->stream, first, nelem, nbatch,
-                       per_xcd, mp->d_ibool, mp->d_xix, mp->d_xiz, mp->d_gammax, mp->d_gammaz, mp->d_kappav,
-                       mp->d_muv, (const float2*)displ, accel, mp->basis);
-  else
-    hipLaunchKernelGGL(k_elastic_forces<false>, dim3(nblocks), dim3(FORCE_THREADS), 0, mp->stream, first, nelem,
-                       nbatch, per_xcd, mp->d_ibool, mp->d_xix, mp->d_xiz, mp->d_gammax, mp->d_gammaz, mp->d_kappav,
-                       mp->d_muv, (const float2*)displ, accel, mp->basis);
+  using K = decltype(&k_elastic_forces<true, true>);
+  const K kernels[2][2] = {{k_elastic_forces<false, false>, k_elastic_forces<false, true>},
+                           {k_elastic_forces<true, false>, k_elastic_forces<true, true>}};
+  hipLaunchKernelGGL(kernels[atomic ? 1 : 0][att ? 1 : 0], dim3(nblocks), dim3(FORCE_THREADS), 0, mp->stream, first,
+                     nelem, nbatch, per_xcd, mp->d_ibool, mp->d_xix, mp->d_xiz, mp->d_gammax, mp->d_gammaz,
+                     mp->d_kappav, mp->d_muv, (const float2*)displ, accel, mp->basis, mp->d_att_coef, mp->d_att_e,
+                     mp->d_att_old, (long)mp->nspec * NGLL2);
   check_hip(hipGetLastError(), "k_elastic_forces launch");
 }
 

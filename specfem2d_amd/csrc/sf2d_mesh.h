@@ -43,6 +43,11 @@ struct Mesh {
   int* d_ibool = nullptr;
   float *d_xix = nullptr, *d_xiz = nullptr, *d_gammax = nullptr, *d_gammaz = nullptr;
   float *d_kappav = nullptr, *d_muv = nullptr;
+  // viscoelastic attenuation (3 standard linear solids), device element order, [ne] planes with
+  // ne = nspec * 25: coefficients A_kappa, B_kappa, A_mu, B_mu x 3 solids (12 planes), memory variables
+  // e1, e11, e13 x 3 solids (9 planes), previous strains dux_dxl, duz_dzl, dux_dzl + duz_dxl (3 planes)
+  int attenuation = 0;
+  float *d_att_coef = nullptr, *d_att_e = nullptr, *d_att_old = nullptr;
   int* d_ispec_is_elastic = nullptr;
   int nspec_elastic = 0, nspec_acoustic = 0;
   int all_elastic = 1;
@@ -280,7 +285,8 @@ void launch_newmark_corrector_predictor_to(const float* d, const float* v, const
                                           int nshot = 1, long shot_stride = 0, int zero_accel = 1, int ps = 1);
 // accel = 0 at the special points of first-touch assembly
 void launch_zero_points(const Mesh* mp, float* accel, int nshot = 1, long shot_stride = 0, int ps = 1);
-void launch_elastic_forces(const Mesh* mp, int first, int nelem, int atomic, const float* displ, float* accel);
+void launch_elastic_forces(const Mesh* mp, int first, int nelem, int atomic, const float* displ, float* accel,
+                           int att = 0);
 // nshot wavefields at a stride of shot_stride points (float2) in displ and accel
 void launch_elastic_forces_patch(const Mesh* mp, int first_patch, int npatch, const float* displ, float* accel,
                                  int nshot = 1, long shot_stride = 0, int first_touch = 0);

@@ -57,6 +57,8 @@ def load_libraries():
     host.sf2d_sim_create.argtypes = [C.POINTER(Params), dp, dp, dp, C.c_int, C.POINTER(SourceC), C.c_int, dp,
                                      C.c_int, C.c_int]
     host.sf2d_sim_prepare_device.argtypes = [P]
+    host.sf2d_sim_set_attenuation.argtypes = [P, dp, dp, C.c_int]
+    host.sf2d_sim_set_attenuation.restype = C.c_int
     host.sf2d_sim_prepare_device.restype = C.c_int
     host.sf2d_sim_destroy.argtypes = [P]
     for f in ("sf2d_sim_nspec", "sf2d_sim_nglob", "sf2d_sim_nabs", "sf2d_sim_nspec_acoustic", "sf2d_sim_ncoupling"):
@@ -184,6 +186,10 @@ class Simulation:
         self.t0 = self.host.sf2d_sim_t0(self.h)
         self.nspec_acoustic = self.host.sf2d_sim_nspec_acoustic(self.h)
         self.ncoupling = self.host.sf2d_sim_ncoupling(self.h)
+        if cfg.attenuation:
+            qk, qm = cfg.attenuation_arrays()
+            if self.host.sf2d_sim_set_attenuation(self.h, qk, qm, int(cfg.n_sls)) != 0:
+                raise NotImplementedError("viscoelastic attenuation: 3 solids, forward runs of solid meshes only")
         self.on_device = False
         if device:
             self.prepare_device()

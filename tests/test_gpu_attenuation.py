@@ -1,0 +1,80 @@
+"""Viscoelastic attenuation on the GPU (3 standard linear solids, Newmark memory variables carried by the
+element kernel), called through the Fortran-style C-ABI by the host driver, against the CPU oracle (itself pinned
+by the reference's viscoelastic force-source trace) and against that committed trace.
+
+Tolerance: FP32 seismograms within rel-L2 1e-5 of the FP32 oracle, as for the other paths.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from oracle.oracle import OracleSim
+from specfem2d_amd import config, io
+from specfem2d_amd.config import Source
+from specfem2d_amd.driver import Simulation
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-5
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+
+
+def test_viscoelastic_example_from_its_input_files(golden_dir):
+    cfg, _ = io.config_from_par_file(os.path.join(golden_dir, "examples", "force_source_viscoelastic"),
+                                     cpml="stacey")
+    sim = Simulation(cfg)
+    seis, t = sim.forward()
+    sim.reset()
+    seis2, _ = sim.forward()  # a second run from rest: the memory variables restart at zero
+    sim.close()
+    ref = OracleSim(cfg, "sp").forward()
+    g = np.load(f"{golden_dir}/force_source_viscoelastic_ref_seis.npz")
+    w = g["time"] < 0.26
+    errs = [np.abs(seis[0, c][w] - g[f"AA.S0001.{ch}"][w]).max() / np.abs(g[f"AA.S0001.{ch}"]).max()
+            for c, ch in enumerate(("BXX", "BXZ"))]
+    r = rel(seis, ref)
+    print(f"viscoelastic example (Qkappa 22.3, Qmu 20), 1400 steps: rel-L2 vs FP32 oracle {r:.3e}; max error vs "
+          f"REF_SEIS before 0.26 s {max(errs):.2e} of the peak; loop {t['loop_ms']:.1f} ms")
+    assert r < TOL and max(errs) < 1e-5
+    np.testing.assert_array_equal(seis, seis2)
+
+
+def test_heterogeneous_quality_factors_against_oracle():
+    """Per-point Qkappa, Qmu over a random model, some elements without attenuation (Q = 9999), Stacey edges, a
+    tilted force and a moment tensor, receivers rotated by anglerec."""
+    nx, nz = 40, 30
+    rng = np.random.default_rng(21)
+    n = nx * nz * 25
+    qk = np.repeat(rng.uniform(15.0, 80.0, nx * nz), 25)
+    qm = np.repeat(rng.uniform(10.0, 60.0, nx * nz), 25)
+    off = np.repeat(rng.random(nx * nz) < 0.2, 25)
+    qk[off], qm[off] = 9999.0, 9999.0
+    pert = 0.05 * rng.standard_normal(n)
+    src = [Source(x=800.0, z=-700.0, f0=12.0, factor=1e9, anglesource=30.0),
+           Source(x=1300.0, z=-900.0, f0=12.0, factor=1e9, source_type=2, Mxx=1.0, Mzz=-0.5, Mxz=0.3, tshift=0.01)]
+    rec = np.array([[300.0, -100.0], [1000.0, -400.0], [1700.0, -1200.0], [500.0, -1300.0]])
+    cfg = config.SimulationConfig(
+        name="att_40x30", nx=nx, nz=nz, xmin=0.0, xmax=2000.0, interfaces=[-1500.0, 0.0], nz_layers=[nz], nstep=900,
+        dt=5e-4, rho=2500.0 * (1 + pert), vp=3000.0 * (1 + pert), vs=1700.0 * (1 + pert), material_mode=1,
+        stacey=True, absorbtop=False, sources=src, receivers=rec, anglerec=20.0, attenuation=True, qkappa=qk, qmu=qm)
+    sim = Simulation(cfg)
+    seis, t = sim.forward()
+    d, _, _ = sim.fields()
+    sim.close()
+    o = OracleSim(cfg, "sp")
+    ref = o.forward()
+    od, _, _ = o.fields()
+    el = OracleSim(cfg.replace(attenuation=False), "sp").forward()
+    r = rel(seis, ref)
+    print(f"heterogeneous Q 40x30, 900 steps: rel-L2 vs FP32 oracle {r:.3e} (elastic run {rel(el, ref):.2f} away), "
+          f"final displacement {rel(d, od):.2e}; loop {t['loop_ms']:.1f} ms")
+    assert r < TOL and rel(d, od) < 5 * TOL and rel(el, ref) > 0.05
+
+
+def test_attenuation_refuses_fluids():
+    cfg = config.layered_fluid_solid(8, 6, 2, 100.0, 10, 1e-3, sources=[Source(x=420.0, z=-450.0, f0=10.0)])
+    with pytest.raises(NotImplementedError):
+        Simulation(cfg.replace(attenuation=True, qkappa=30.0, qmu=20.0))
