@@ -233,9 +233,10 @@ def c2_coupled(steps: int = 100, warmup: int = 10):
 def c2_attenuation(steps: int = 50, warmup: int = 5):
     """Forward of the C2 elastic mesh (2000 x 1000 elements of 10 m, the random-field model) with viscoelastic
     attenuation everywhere (Qkappa 60, Qmu 30, 3 standard linear solids): GLL-point updates per second, and the
-    element-force launches' share of the step with their algorithmic traffic, 4300 B per element (25 points x
-    (ibool 4 B, geometry 16 B, moduli 8 B, attenuation 144 B: 12 coefficients read, 9 memory variables and 3
-    previous strains read and written)) plus 24 B per point (displacement read, acceleration read-modify-write)."""
+    element-force launches' share of the step with their algorithmic traffic, 3300 B per element (25 points x
+    (ibool 4 B, geometry 16 B, moduli 8 B, attenuation 104 B: a coefficient-set index read, 9 memory variables
+    and 3 previous strains read and written)) plus 24 B per point (displacement read, acceleration
+    read-modify-write)."""
     import torch
     from specfem2d_amd.driver import Simulation
     cfg = make_config("c2_mesh_elastic_2000x1000", warmup + steps).replace(attenuation=True, qkappa=60.0, qmu=30.0)
@@ -246,7 +247,7 @@ def c2_attenuation(steps: int = 50, warmup: int = 5):
     seis, timing = sim.forward(nstep=steps, phase_timing=True)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    traffic = 4300 * sim.nspec + 24 * sim.nglob
+    traffic = 3300 * sim.nspec + 24 * sim.nglob
     forces_s = timing["forces_ms"] * 1e-3 / steps
     out = {"workload": "c2_mesh_elastic_2000x1000 + attenuation (Qkappa 60, Qmu 30, N_SLS 3)", "nspec": sim.nspec,
            "nglob": sim.nglob, "steps": steps, "ms_per_step": wall * 1e3 / steps, "value": sim.nglob * steps / wall,

@@ -11,6 +11,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <algorithm>
+#include <array>
+#include <map>
 #include <vector>
 
 #include "sf2d_mesh.h"
@@ -694,18 +696,39 @@ void prepare_fields_elastic_device_(long* Mesh_pointer, realw* rmassx, realw* rm
   mp->num_phase_ispec_elastic = np;
   sort_elements_by_colour(mp, phase_ispec_inner_elastic, np);
   if (mp->attenuation) {
-    // host arrays A(N_SLS, NGLLX, NGLLZ, nspec) (solid fastest, ispec order) -> [plane][ne] in device order
+    // host arrays A(N_SLS, NGLLX, NGLLZ, nspec) (solid fastest, ispec order): the distinct coefficient sets of
+    // the points go to a table (a few per model: one per distinct pair of quality factors), each point keeps a
+    // 4-byte index in device element order -- 44 B less per point and step than per-point coefficients
     const size_t ne = (size_t)mp->nspec * NGLL2;
-    std::vector<float> coef(12 * ne);
     const realw* src[4] = {h_A_newmark_kappa, h_B_newmark_kappa, h_A_newmark_mu, h_B_newmark_mu};
+    std::map<std::array<float, 12>, unsigned int> ids;
+    std::vector<float4> table;
+    std::vector<unsigned int> idx(ne);
+    std::array<float, 12> last_c{};
+    unsigned int last_id = 0;
     for (int e = 0; e < mp->nspec; e++) {
       const size_t row = (size_t)mp->pos_of_ispec[e] * NGLL2;
-      for (int p = 0; p < NGLL2; p++)
+      for (int p = 0; p < NGLL2; p++) {
+        std::array<float, 12> c;
         for (int k = 0; k < 3; k++)
-          for (int w = 0; w < 4; w++)
-            coef[(size_t)(4 * k + w) * ne + row + p] = src[w][((size_t)e * NGLL2 + p) * 3 + k];
+          for (int w = 0; w < 4; w++) c[4 * k + w] = src[w][((size_t)e * NGLL2 + p) * 3 + k];
+        if (!table.empty() && c == last_c) {  // neighbouring points mostly share their set
+          idx[row + p] = last_id;
+          continue;
+        }
+        auto it = ids.find(c);
+        if (it == ids.end()) {
+          it = ids.emplace(c, (unsigned int)(table.size() / 3)).first;
+          for (int k = 0; k < 3; k++) table.push_back(make_float4(c[4 * k], c[4 * k + 1], c[4 * k + 2], c[4 * k + 3]));
+        }
+        idx[row + p] = it->second;
+        last_c = c;
+        last_id = it->second;
+      }
     }
-    mp->d_att_coef = device_copy(coef.data(), coef.size());
+    mp->att_ntable = (int)(table.size() / 3);
+    mp->d_att_table = device_copy(table.data(), table.size());
+    mp->d_att_idx = device_copy(idx.data(), ne);
     mp->d_att_e = device_zeros<float>(9 * ne);
     mp->d_att_old = device_zeros<float>(3 * ne);
   }
@@ -823,7 +846,7 @@ void prepare_cleanup_device_(long* Mesh_pointer, int* ACOUSTIC_SIMULATION, int* 
   (void)hipStreamSynchronize(mp->stream);
   acoustic_release(mp);
   device_free(mp->d_ibool); device_free(mp->d_xix); device_free(mp->d_xiz); device_free(mp->d_gammax);
-  device_free(mp->d_gammaz); device_free(mp->d_kappav); device_free(mp->d_muv); device_free(mp->d_att_coef); device_free(mp->d_att_e); device_free(mp->d_att_old); device_free(mp->d_ispec_is_elastic);
+  device_free(mp->d_gammaz); device_free(mp->d_kappav); device_free(mp->d_muv); device_free(mp->d_att_idx); device_free(mp->d_att_table); device_free(mp->d_att_e); device_free(mp->d_att_old); device_free(mp->d_ispec_is_elastic);
   device_free(mp->d_displ); device_free(mp->d_veloc); device_free(mp->d_accel);
   device_free(mp->d_b_displ); device_free(mp->d_b_veloc); device_free(mp->d_b_accel);
   device_free(mp->d_rmassx); device_free(mp->d_rmassz);

@@ -149,16 +149,18 @@ __global__ __launch_bounds__(256) void k_zero_points(float2* __restrict__ a, con
 // read-modify-write; ATOMIC = true: FP32 atomics, valid for any range.
 // ATT: viscoelastic attenuation with 3 standard linear solids -- the memory variables advance with the
 // recursive convolution of compute_attenuation_viscoelastic.f90:93-121 (Kernel_2_viscoelastic_impl.cu:1019-1065
-// on the reference's GPU) and correct the stress with the unrelaxed moduli; coef, e and old are [plane][ne]
-// arrays in device element order (planes: A_kappa, B_kappa, A_mu, B_mu per solid; e1, e11, e13 per solid;
-// previous dux_dxl, duz_dzl, dux_dzl + duz_dxl)
+// on the reference's GPU) and correct the stress with the unrelaxed moduli; each point reads its coefficient set
+// {A_kappa, B_kappa, A_mu, B_mu} x 3 solids from a small table through a per-point index; e and old are
+// [plane][ne] arrays in device element order (planes e1, e11, e13 per solid; previous dux_dxl, duz_dzl,
+// dux_dzl + duz_dxl)
 template <bool ATOMIC, bool ATT>
 __global__ __launch_bounds__(FORCE_THREADS) void k_elastic_forces(
     int first, int nelem, int nbatch, int per_xcd, const int* __restrict__ ibool,
     const float* __restrict__ xix, const float* __restrict__ xiz, const float* __restrict__ gammax,
     const float* __restrict__ gammaz, const float* __restrict__ kappav, const float* __restrict__ muv,
     const float2* __restrict__ displ, float* __restrict__ accel, const Basis basis,
-    const float* __restrict__ att_coef, float* __restrict__ att_e, float* __restrict__ att_old, long ne) {
+    const unsigned int* __restrict__ att_idx, const float4* __restrict__ att_table, float* __restrict__ att_e,
+    float* __restrict__ att_old, long ne) {
   // XCD-aware remap: blocks b, b+8, b+16, ... (one XCD) take consecutive batches
   const int bid = blockIdx.x;
   const int batch = (bid % NXCD) * per_xcd + bid / NXCD;
@@ -184,8 +186,20 @@ __global__ __launch_bounds__(FORCE_THREADS) void k_elastic_forces(
 
   int g = 0;
   float xixl = 0.f, xizl = 0.f, gaxl = 0.f, gazl = 0.f, kap = 0.f, mu = 0.f;
+  // attenuation state of this point, loaded with the element's other inputs so the loads overlap
+  float4 cf[3];
+  float ev[9], ov[3];
   if (active) {
     const size_t l = (size_t)(first + e) * NGLL2 + p;
+    if (ATT) {
+      const float4* coef = att_table + 3 * (size_t)att_idx[l];
+#pragma unroll
+      for (int k = 0; k < 3; k++) cf[k] = coef[k];
+#pragma unroll
+      for (int q = 0; q < 9; q++) ev[q] = att_e[(size_t)q * ne + l];
+#pragma unroll
+      for (int q = 0; q < 3; q++) ov[q] = att_old[(size_t)q * ne + l];
+    }
     g = ibool[l];
     const float2 u = displ[g];
     s_ux[tid] = u.x;
@@ -224,22 +238,18 @@ __global__ __launch_bounds__(FORCE_THREADS) void k_elastic_forces(
     if (ATT) {
       const size_t l = (size_t)(first + e) * NGLL2 + p;
       const float th = dux_dxl + duz_dzl;
-      const float oxx = att_old[l], ozz = att_old[ne + l], oxz = att_old[2 * ne + l];
+      const float oxx = ov[0], ozz = ov[1], oxz = ov[2];
       const float th_old = oxx + ozz;
       float e1s = 0.f, e11s = 0.f, e13s = 0.f;
 #pragma unroll
       for (int k = 0; k < 3; k++) {
-        const float a1 = att_coef[(size_t)(4 * k) * ne + l], b1 = att_coef[(size_t)(4 * k + 1) * ne + l];
-        const float a2 = att_coef[(size_t)(4 * k + 2) * ne + l], b2 = att_coef[(size_t)(4 * k + 3) * ne + l];
-        float* e1p = att_e + (size_t)(3 * k) * ne + l;
-        float* e11p = att_e + (size_t)(3 * k + 1) * ne + l;
-        float* e13p = att_e + (size_t)(3 * k + 2) * ne + l;
-        const float e1 = a1 * a1 * *e1p + b1 * (th + a1 * th_old);
-        const float e11 = a2 * a2 * *e11p + b2 * (dux_dxl - 0.5f * th + a2 * (oxx - 0.5f * th_old));
-        const float e13 = a2 * a2 * *e13p + b2 * (dux_dzl + duz_dxl + a2 * oxz);
-        *e1p = e1;
-        *e11p = e11;
-        *e13p = e13;
+        const float a1 = cf[k].x, b1 = cf[k].y, a2 = cf[k].z, b2 = cf[k].w;
+        const float e1 = a1 * a1 * ev[3 * k] + b1 * (th + a1 * th_old);
+        const float e11 = a2 * a2 * ev[3 * k + 1] + b2 * (dux_dxl - 0.5f * th + a2 * (oxx - 0.5f * th_old));
+        const float e13 = a2 * a2 * ev[3 * k + 2] + b2 * (dux_dzl + duz_dxl + a2 * oxz);
+        att_e[(size_t)(3 * k) * ne + l] = e1;
+        att_e[(size_t)(3 * k + 1) * ne + l] = e11;
+        att_e[(size_t)(3 * k + 2) * ne + l] = e13;
         e1s += e1;
         e11s += e11;
         e13s += e13;
@@ -1219,7 +1229,8 @@ void launch_elastic_forces(const Mesh* mp, int first, int nelem, int atomic, con
                            int att) {
   if (nelem <= 0) return;
   if (first < 0 || first + nelem > mp->nspec) fail("element range [%d, %d) outside the mesh", first, first + nelem);
-  if (att && (!mp->d_att_coef || !mp->d_att_e || !mp->d_att_old)) fail("attenuation arrays not prepared");
+  if (att && (!mp->d_att_idx || !mp->d_att_table || !mp->d_att_e || !mp->d_att_old))
+    fail("attenuation arrays not prepared");
   const int nbatch = (nelem + FORCE_EPB - 1) / FORCE_EPB;
   const int per_xcd = (nbatch + NXCD - 1) / NXCD;
   const int nblocks = per_xcd * NXCD;
@@ -1228,8 +1239,8 @@ void launch_elastic_forces(const Mesh* mp, int first, int nelem, int atomic, con
                            {k_elastic_forces<true, false>, k_elastic_forces<true, true>}};
   hipLaunchKernelGGL(kernels[atomic ? 1 : 0][att ? 1 : 0], dim3(nblocks), dim3(FORCE_THREADS), 0, mp->stream, first,
                      nelem, nbatch, per_xcd, mp->d_ibool, mp->d_xix, mp->d_xiz, mp->d_gammax, mp->d_gammaz,
-                     mp->d_kappav, mp->d_muv, (const float2*)displ, accel, mp->basis, mp->d_att_coef, mp->d_att_e,
-                     mp->d_att_old, (long)mp->nspec * NGLL2);
+                     mp->d_kappav, mp->d_muv, (const float2*)displ, accel, mp->basis, mp->d_att_idx,
+                     mp->d_att_table, mp->d_att_e, mp->d_att_old, (long)mp->nspec * NGLL2);
   check_hip(hipGetLastError(), "k_elastic_forces launch");
 }
 

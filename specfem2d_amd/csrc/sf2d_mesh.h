@@ -43,11 +43,13 @@ struct Mesh {
   int* d_ibool = nullptr;
   float *d_xix = nullptr, *d_xiz = nullptr, *d_gammax = nullptr, *d_gammaz = nullptr;
   float *d_kappav = nullptr, *d_muv = nullptr;
-  // viscoelastic attenuation (3 standard linear solids), device element order, [ne] planes with
-  // ne = nspec * 25: coefficients A_kappa, B_kappa, A_mu, B_mu x 3 solids (12 planes), memory variables
-  // e1, e11, e13 x 3 solids (9 planes), previous strains dux_dxl, duz_dzl, dux_dzl + duz_dxl (3 planes)
-  int attenuation = 0;
-  float *d_att_coef = nullptr, *d_att_e = nullptr, *d_att_old = nullptr;
+  // viscoelastic attenuation (3 standard linear solids), device element order, ne = nspec * 25: per point an
+  // index into a table of distinct coefficient sets (3 x {A_kappa, B_kappa, A_mu, B_mu}), memory variables
+  // e1, e11, e13 x 3 solids (9 [ne] planes), previous strains dux_dxl, duz_dzl, dux_dzl + duz_dxl (3 planes)
+  int attenuation = 0, att_ntable = 0;
+  unsigned int* d_att_idx = nullptr;
+  float4* d_att_table = nullptr;
+  float *d_att_e = nullptr, *d_att_old = nullptr;
   int* d_ispec_is_elastic = nullptr;
   int nspec_elastic = 0, nspec_acoustic = 0;
   int all_elastic = 1;
