@@ -28,7 +28,9 @@ def test_viscoelastic_example_from_its_input_files(golden_dir):
     sim = Simulation(cfg)
     seis, t = sim.forward()
     sim.reset()
-    seis2, _ = sim.forward()  # a second run from rest: the memory variables restart at zero
+    sim.forward(nstep=400)  # stopped while the waves (and the memory variables) are strong
+    sim.reset()
+    seis2, _ = sim.forward()  # a run from rest again: the memory variables restart at zero
     sim.close()
     ref = OracleSim(cfg, "sp").forward()
     g = np.load(f"{golden_dir}/force_source_viscoelastic_ref_seis.npz")
@@ -39,12 +41,16 @@ def test_viscoelastic_example_from_its_input_files(golden_dir):
     print(f"viscoelastic example (Qkappa 22.3, Qmu 20), 1400 steps: rel-L2 vs FP32 oracle {r:.3e}; max error vs "
           f"REF_SEIS before 0.26 s {max(errs):.2e} of the peak; loop {t['loop_ms']:.1f} ms")
     assert r < TOL and max(errs) < 1e-5
-    np.testing.assert_array_equal(seis, seis2)
+    assert rel(seis2, seis) < TOL  # atomic assembly may reorder FP32 sums between runs
 
 
-def test_heterogeneous_quality_factors_against_oracle():
+@pytest.mark.parametrize("assembly", ["atomic", "colour"])
+def test_heterogeneous_quality_factors_against_oracle(assembly, monkeypatch):
     """Per-point Qkappa, Qmu over a random model, some elements without attenuation (Q = 9999), Stacey edges, a
-    tilted force and a moment tensor, receivers rotated by anglerec."""
+    tilted force and a moment tensor, receivers rotated by anglerec; elements assembled with FP32 atomics in one
+    launch (the default with attenuation) or by colours."""
+    if assembly == "colour":
+        monkeypatch.setenv("SF2D_ASSEMBLY", "colour")
     nx, nz = 40, 30
     rng = np.random.default_rng(21)
     n = nx * nz * 25
@@ -69,7 +75,7 @@ def test_heterogeneous_quality_factors_against_oracle():
     od, _, _ = o.fields()
     el = OracleSim(cfg.replace(attenuation=False), "sp").forward()
     r = rel(seis, ref)
-    print(f"heterogeneous Q 40x30, 900 steps: rel-L2 vs FP32 oracle {r:.3e} (elastic run {rel(el, ref):.2f} away), "
+    print(f"heterogeneous Q 40x30 ({assembly}), 900 steps: rel-L2 vs FP32 oracle {r:.3e} (elastic run {rel(el, ref):.2f} away), "
           f"final displacement {rel(d, od):.2e}; loop {t['loop_ms']:.1f} ms")
     assert r < TOL and rel(d, od) < 5 * TOL and rel(el, ref) > 0.05
 
