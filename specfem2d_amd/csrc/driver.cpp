@@ -742,7 +742,7 @@ void build_receivers(Sim& s, const double* rec_xz) {
 void prepare_device(Sim& s) {
   int myrank = 0, ndev = 0;
   initialize_cuda_device_(&myrank, &ndev);
-  int ngllx = NGLL, nspec = s.nspec, nglob = s.nglob, zero = 0, one = 1;
+  int ngllx = NGLL, nspec = s.nspec, nglob = s.nglob, zero = 0;
   int stacey = s.p.stacey ? 1 : 0, pml = 0;
   std::vector<int> inner(s.nspec, 1), is_elastic(s.nspec, 1), is_acoustic(s.nspec, 0);
   for (int e = 0; e < s.nspec && s.nspec_ac > 0; e++) {

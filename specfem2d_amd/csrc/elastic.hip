@@ -872,8 +872,8 @@ __global__ __launch_bounds__(256) void k_affine_elements(int nspec, const float4
   int same = m > 0.f;  // false for NaN too
   for (int p = 0; p < NGLL2; p++) {
     const float4 g = geom[(size_t)e * NGLL2 + p];
-    same &= (fabsf(g.x - g0.x) <= tol) & (fabsf(g.y - g0.y) <= tol) & (fabsf(g.z - g0.z) <= tol) &
-            (fabsf(g.w - g0.w) <= tol);
+    same &= (int)(fabsf(g.x - g0.x) <= tol) & (int)(fabsf(g.y - g0.y) <= tol) & (int)(fabsf(g.z - g0.z) <= tol) &
+            (int)(fabsf(g.w - g0.w) <= tol);
   }
   // 2: affine and axis-aligned (xi_z and gamma_x zero to the same bound), 1: affine, 0: neither
   const int rect = same && fabsf(g0.y) <= tol && fabsf(g0.z) <= tol;
