@@ -77,6 +77,8 @@ void sf2d_sim_batch_get_fields(void* sim, int shot, float* displ, float* veloc, 
    least-squares Zener fit of attenuation_model.f90 (USE_SOLVOPT false) and the coefficients from
    prepare_attenuation.f90; forward runs of solid meshes only (as prepare_gpu.f90:69-72).  Returns 0 on success. */
 int sf2d_sim_set_attenuation(void* sim, const double* qkappa, const double* qmu, int n_sls);
+/* the attenuation coefficients handed to prepare_fields_elastic_device_, [nspec][25][3] each (for parity checks) */
+void sf2d_sim_get_attenuation(void* sim, float* A_kappa, float* B_kappa, float* A_mu, float* B_mu);
 /* Hands the setup to the GPU library in the order of prepare_gpu.f90:80-347. Returns 0 on success. */
 int sf2d_sim_prepare_device(void* sim);
 void sf2d_sim_destroy(void* sim);

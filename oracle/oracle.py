@@ -81,6 +81,7 @@ def _lib(precision: str):
         lib.orc_gradient.restype = C.c_double
         lib.orc_set_attenuation.argtypes = [P, dp, dp, C.c_int, C.c_double]
         lib.orc_set_attenuation.restype = C.c_int
+        lib.orc_get_attenuation.argtypes = [P, dp, dp, dp, dp]
         for f in ("orc_nspec_acoustic", "orc_ncoupling", "orc_nfree"):
             getattr(lib, f).argtypes = [P]
             getattr(lib, f).restype = C.c_int
@@ -278,6 +279,13 @@ class OracleSim:
         out = np.zeros_like(pot)
         self.lib.orc_apply_acoustic_stiffness(self.h, pot, out)
         return out
+
+    def attenuation_coefficients(self):
+        """(A_kappa, B_kappa, A_mu, B_mu), each [nspec, 25, n_sls]."""
+        n = self.nspec * 25 * self.cfg.n_sls
+        out = [np.zeros(n) for _ in range(4)]
+        self.lib.orc_get_attenuation(self.h, *out)
+        return tuple(o.reshape(self.nspec, 25, self.cfg.n_sls) for o in out)
 
     def _elastic_only(self, what):
         if self.nspec_acoustic or self.cfg.attenuation:

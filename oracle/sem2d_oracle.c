@@ -1300,6 +1300,18 @@ static void zener_fit(int N, double Qref, double fmin, double fmax, double *tau_
   }
 }
 
+/* convolution coefficients [nspec*25][nsls] of the bulk (1) and shear (2) modes */
+void orc_get_attenuation(void *h, double *A1, double *B1, double *A2, double *B2) {
+  orc_sim *s = (orc_sim *)h;
+  const size_t n = (size_t)s->nspec * NGLL2 * s->nsls;
+  for (size_t k = 0; k < n && s->att; k++) {
+    A1[k] = (double)s->att_A1[k];
+    B1[k] = (double)s->att_B1[k];
+    A2[k] = (double)s->att_A2[k];
+    B2[k] = (double)s->att_B2[k];
+  }
+}
+
 /* ATTENUATION_VISCOELASTIC with Qkappa, Qmu per GLL point ([nspec][25]) and nsls solids, reference frequency f0
    (the first source's f0 unless it is a Dirac or Heaviside, prepare_attenuation.f90:319-322); fit band
    [f0/10, 10 f0] (attenuation_model.f90:95-104).  Points with both Q above 9998.999 do not attenuate.  Returns 0,

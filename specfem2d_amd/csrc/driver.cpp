@@ -1501,6 +1501,15 @@ int sf2d_sim_set_attenuation(void* sim, const double* qkappa, const double* qmu,
   return 0;
 }
 
+void sf2d_sim_get_attenuation(void* sim, float* A_kappa, float* B_kappa, float* A_mu, float* B_mu) {
+  Sim* s = (Sim*)sim;
+  if (!s || !s->attenuation) return;
+  std::copy(s->att_A_kappa.begin(), s->att_A_kappa.end(), A_kappa);
+  std::copy(s->att_B_kappa.begin(), s->att_B_kappa.end(), B_kappa);
+  std::copy(s->att_A_mu.begin(), s->att_A_mu.end(), A_mu);
+  std::copy(s->att_B_mu.begin(), s->att_B_mu.end(), B_mu);
+}
+
 void sf2d_sim_reset(void* sim) {
   Sim* s = (Sim*)sim;
   if (!s || !s->mesh) return;

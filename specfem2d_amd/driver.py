@@ -59,6 +59,7 @@ def load_libraries():
     host.sf2d_sim_prepare_device.argtypes = [P]
     host.sf2d_sim_set_attenuation.argtypes = [P, dp, dp, C.c_int]
     host.sf2d_sim_set_attenuation.restype = C.c_int
+    host.sf2d_sim_get_attenuation.argtypes = [P, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     host.sf2d_sim_prepare_device.restype = C.c_int
     host.sf2d_sim_destroy.argtypes = [P]
     for f in ("sf2d_sim_nspec", "sf2d_sim_nglob", "sf2d_sim_nabs", "sf2d_sim_nspec_acoustic", "sf2d_sim_ncoupling"):
@@ -308,6 +309,12 @@ class Simulation:
         self.host.sf2d_sim_get_stacey(self.h, _ptr(isp), _ptr(ij), _ptr(nrm), _ptr(jw), _ptr(edge), _ptr(ib))
         k = self.nabs
         return isp[:k], ij.reshape(n, 5, 2)[:k], nrm.reshape(n, 5, 2)[:k], jw.reshape(n, 5)[:k], edge[:k], ib[:k]
+
+    def attenuation_coefficients(self):
+        """(A_kappa, B_kappa, A_mu, B_mu), each [nspec, 25, 3] float32, as handed to the device."""
+        out = [np.zeros((self.nspec, 25, 3), np.float32) for _ in range(4)]
+        self.host.sf2d_sim_get_attenuation(self.h, *[o.ctypes.data for o in out])
+        return tuple(out)
 
     # time loop ----------------------------------------------------------
     def _need_device(self):

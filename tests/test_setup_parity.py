@@ -124,3 +124,23 @@ def test_fluid_setup_bit_exact(case):
     np.testing.assert_array_equal(sa.astype(np.float64), osa)
     np.testing.assert_array_equal(st.astype(np.float64), ost)
     sim.close()
+
+
+def test_attenuation_coefficients_match_oracle():
+    """The host driver's Zener fit (Cholesky on the normal equations) and single-precision convolution
+    coefficients against the oracle's (Gaussian elimination), per point, on a heterogeneous-Q mesh."""
+    rng = np.random.default_rng(3)
+    cfg = config.small_test_mesh(nx=8, nz=6, nstep=10)
+    n = cfg.nspec * 25
+    qk = np.repeat(rng.uniform(10.0, 200.0, cfg.nspec), 25)
+    qm = np.repeat(rng.uniform(8.0, 150.0, cfg.nspec), 25)
+    qk[:50], qm[:50] = 9999.0, 9999.0
+    cfg = cfg.replace(attenuation=True, qkappa=qk, qmu=qm)
+    sim = Simulation(cfg, device=False)
+    mine = sim.attenuation_coefficients()
+    sim.close()
+    ref = OracleSim(cfg, "sp").attenuation_coefficients()
+    for a, b in zip(mine, ref):
+        b = b.astype(np.float32)
+        assert np.all(np.abs(a - b) <= 2 * np.spacing(np.abs(b).astype(np.float32)))
+    assert np.all(mine[0].reshape(n, 3)[:50] == 0)  # Q = 9999: no attenuation
