@@ -23,7 +23,9 @@ class OrcParams(C.Structure):
         ("absorb_bottom", C.c_int), ("absorb_right", C.c_int), ("absorb_top", C.c_int),
         ("absorb_left", C.c_int), ("stacey", C.c_int), ("nstep", C.c_int), ("dt", C.c_double),
         ("user_t0", C.c_double), ("anglerec_deg", C.c_double), ("material_mode", C.c_int),
-        ("seismotype", C.c_int), ("use_trick", C.c_int),
+        ("seismotype", C.c_int), ("use_trick", C.c_int), ("pml", C.c_int), ("nelem_pml", C.c_int),
+        ("k_min_pml", C.c_double), ("k_max_pml", C.c_double), ("damping_ac", C.c_double),
+        ("damping_el", C.c_double),
     ]
 
 
@@ -93,8 +95,29 @@ def _lib(precision: str):
         lib.orc_get_acoustic_fields.argtypes = [P, C.c_void_p, C.c_void_p, C.c_void_p]
         lib.orc_set_acoustic_fields.argtypes = [P, C.c_void_p, C.c_void_p, C.c_void_p]
         lib.orc_apply_acoustic_stiffness.argtypes = [P, dp, dp]
+        lib.orc_nspec_pml.argtypes = [P]
+        lib.orc_nspec_pml.restype = C.c_int
+        lib.orc_npml_points.argtypes = [P, C.c_int]
+        lib.orc_npml_points.restype = C.c_int
+        lib.orc_get_pml.argtypes = [P, ip, ip, dp, ip, ip]
+        lib.orc_set_pml_mistake.argtypes = [C.c_int]
+        lib.orc_pml_coefficients.argtypes = [C.c_double] * 7 + [C.c_int, dp]
         _LIBS[precision] = lib
     return _LIBS[precision]
+
+
+def pml_coefficients(dt, kx, dx, ax, kz, dz, az, region, precision="dp"):
+    """The oracle's CPML coefficients of one point: dict with lik31 / lik13 (A0..A2, (c0, c1, c2) for bb_1 and bb_2)
+    and l (A0..A4, (c0, c1, c2) for alpha_x and alpha_z)."""
+    out = np.zeros(29)
+    _lib(precision).orc_pml_coefficients(dt, kx, dx, ax, kz, dz, az, int(region), out)
+    return {"lik31": (out[0:3], out[3:6], out[6:9]), "lik13": (out[9:12], out[12:15], out[15:18]),
+            "l": (out[18:23], out[23:26], out[26:29])}
+
+
+def set_pml_mistake(which, precision="sp"):
+    """Plant a mistake in the oracle's CPML terms (test infrastructure; 0 removes it)."""
+    _lib(precision).orc_set_pml_mistake(int(which))
 
 
 def _ptr(a):
@@ -124,6 +147,9 @@ class OracleSim:
         p.user_t0, p.anglerec_deg = cfg.user_t0, cfg.anglerec
         p.material_mode = cfg.material_mode
         p.seismotype, p.use_trick = int(cfg.seismotype), int(cfg.use_trick)
+        p.pml, p.nelem_pml = int(cfg.pml), int(cfg.nelem_pml)
+        p.k_min_pml, p.k_max_pml = float(cfg.pml_k_min), float(cfg.pml_k_max)
+        p.damping_ac, p.damping_el = float(cfg.pml_damping_acoustic), float(cfg.pml_damping_elastic)
         rho, vp, vs = cfg.material_arrays()
         srcs = (OrcSource * max(1, len(cfg.sources)))()
         for k, s in enumerate(cfg.sources):
@@ -278,6 +304,21 @@ class OracleSim:
         pot = np.ascontiguousarray(np.asarray(pot, np.float64).reshape(-1))
         out = np.zeros_like(pot)
         self.lib.orc_apply_acoustic_stiffness(self.h, pot, out)
+        return out
+
+    def pml(self):
+        """CPML set-up: dict with is_pml, region [nspec]; kx, kz, dx, dz, ax, az [nspec_pml, 25] (double); the
+        outer-boundary points of elastic and acoustic PML elements (0-based, increasing)."""
+        npml = self.lib.orc_nspec_pml(self.h)
+        nel, nac = self.lib.orc_npml_points(self.h, 1), self.lib.orc_npml_points(self.h, 0)
+        isp, reg = np.zeros(self.nspec, np.int32), np.zeros(self.nspec, np.int32)
+        prof = np.zeros(6 * max(1, npml) * NGLL2)
+        pe, pa = np.zeros(max(1, nel), np.int32), np.zeros(max(1, nac), np.int32)
+        self.lib.orc_get_pml(self.h, isp, reg, prof, pe, pa)
+        prof = prof[: 6 * npml * NGLL2].reshape(6, npml, NGLL2)
+        out = {"is_pml": isp.astype(bool), "region": reg, "points_elastic": pe[:nel] - 1,
+               "points_acoustic": pa[:nac] - 1}
+        out.update(zip(("kx", "kz", "dx", "dz", "ax", "az"), prof))
         return out
 
     def attenuation_coefficients(self):

@@ -6,7 +6,8 @@
  * isotropic, P-SV, Newmark simulation on a structured internal mesh with
  * Stacey absorbing edges, extended to acoustic (fluid) elements: the
  * potential formulation, the acoustic free surface, acoustic Stacey edges,
- * pressure sources, pressure seismograms and fluid-solid coupling.  It is the checker for the HIP implementation and
+ * pressure sources, pressure seismograms and fluid-solid coupling, and to CPML absorbing layers (elastic and
+ * acoustic, Newmark, convolutional memory variables).  It is the checker for the HIP implementation and
  * the CPU baseline timed beside it; the product never links or calls it.
  *
  * It is compiled twice: -DREAL=float (CUSTOM_REAL=real, the reference
@@ -51,6 +52,10 @@ typedef struct {
   int material_mode;          /* 0: Par_file material table, 1: MODEL=gll */
   int seismotype;             /* 1 displacement, 2 velocity, 3 acceleration, 4 pressure, 6 potential */
   int use_trick;              /* USE_TRICK_FOR_BETTER_PRESSURE */
+  int pml;                    /* PML_BOUNDARY_CONDITIONS (CPML on the absorbing sides) */
+  int nelem_pml;              /* NELEM_PML_THICKNESS */
+  double k_min_pml, k_max_pml;/* K_MIN_PML, K_MAX_PML */
+  double damping_ac, damping_el; /* damping_change_factor_acoustic, damping_change_factor_elastic */
 } orc_params;
 
 typedef struct {
@@ -122,6 +127,19 @@ typedef struct {
   REAL *e1, *e11, *e13;       /* [nspec*25][nsls] */
   REAL *old_xx, *old_zz, *old_xz; /* [nspec*25] */
   int it_done;                /* forward steps already run */
+  /* CPML (pml_init.F90, prepare_pml.f90): PML elements, their region (1 X only, 2 Z only, 3 XZ), per-point
+     damping profiles in double precision (K, d, alpha along x and z), the outer-boundary points that are held
+     at zero, the memory variables of the recursive convolutions and the previous-step fields */
+  int pml_on, nspec_pml;
+  int *is_pml, *region, *spec_to_pml; /* [nspec]; spec_to_pml 1-based, 0 outside */
+  double *pml_kx, *pml_kz, *pml_dx, *pml_dz, *pml_ax, *pml_az; /* [nspec_pml][25] */
+  int npml_el, npml_ac;       /* outer-boundary points of elastic / acoustic PML elements */
+  int *pml_pts_el, *pml_pts_ac; /* 1-based iglob, increasing */
+  REAL *rm_dux_dx, *rm_duz_dx, *rm_dux_dz, *rm_duz_dz; /* [nspec_pml*25][2] */
+  REAL *rm_displ;             /* [nspec_pml*25][2 series][2 components] */
+  REAL *rm_ac_dx, *rm_ac_dz, *rm_pot; /* [nspec_pml*25][2] */
+  REAL *displ_old, *pot_old;  /* [nglob][2], [nglob] */
+  double pml_alpha_max;       /* ALPHA_MAX_PML = pi f0max */
 } orc_sim;
 
 /* ------------------------------------------------------------------ */
@@ -571,6 +589,361 @@ static void side_point(int code, int k, int *i, int *j) {
 
 /* edge geometry at GLL point (i,j) of an absorbing element: outward normal
    and 1D jacobian (compute_stacey_elastic.f90 / prepare_timerun.F90:1316-1460) */
+/* ------------------------------------------------------------------ */
+/* CPML set-up for the internal mesher (PML_BOUNDARY_CONDITIONS with   */
+/* PML_PARAMETER_ADJUSTMENT and ROTATE_PML_ACTIVATE false)             */
+/* ------------------------------------------------------------------ */
+static void setup_pml(orc_sim *s) {
+  const int nspec = s->nspec, nglob = s->nglob;
+  s->is_pml = (int *)calloc((size_t)nspec + 1, sizeof(int));
+  s->region = (int *)calloc((size_t)nspec + 1, sizeof(int));
+  s->spec_to_pml = (int *)calloc((size_t)nspec + 1, sizeof(int));
+  s->nspec_pml = 0;
+  s->pml_on = 0;
+  if (!s->p.pml || s->nabs == 0) return; /* prepare_pml.f90:49: nothing without absorbing edges */
+  /* element detection, pml_init.F90:67-160: per side, the elements of its absorbing edges (an element listed
+     again for another side takes that entry's flag), then NELEM_PML_THICKNESS-1 times every element sharing a
+     corner with the side's elements so far; corner lists are kept as a point mask (same membership test) */
+  char *which[5];
+  for (int b = 1; b <= 4; b++) which[b] = (char *)calloc((size_t)nspec + 1, 1);
+  char *corner = (char *)calloc((size_t)nglob + 1, 1);
+  const int cl[4] = {0, NGLL - 1, (NGLL - 1) * NGLL, NGLL2 - 1}; /* (1,1) (5,1) (1,5) (5,5) */
+  for (int b = 1; b <= 4; b++) {
+    memset(corner, 0, (size_t)nglob);
+    for (int a = 0; a < s->nabs; a++) {
+      const int e = s->abs_ispec[a] - 1;
+      which[b][e] = (s->abs_code[a] == b);
+      if (s->abs_code[a] == b)
+        for (int c = 0; c < 4; c++) corner[s->ibool[(size_t)e * NGLL2 + cl[c]] - 1] = 1;
+    }
+    for (int icoef = 2; icoef <= s->p.nelem_pml; icoef++) {
+      for (int e = 0; e < nspec; e++) {
+        if (which[b][e]) continue;
+        for (int c = 0; c < 4; c++)
+          if (corner[s->ibool[(size_t)e * NGLL2 + cl[c]] - 1]) which[b][e] = 1;
+      }
+      memset(corner, 0, (size_t)nglob);
+      for (int e = 0; e < nspec; e++) {
+        if (!which[b][e]) continue;
+        s->is_pml[e] = 1;
+        for (int c = 0; c < 4; c++) corner[s->ibool[(size_t)e * NGLL2 + cl[c]] - 1] = 1;
+      }
+    }
+  }
+  /* regions, pml_init.F90:201-248 */
+  for (int e = 0; e < nspec; e++) {
+    if (!s->is_pml[e]) continue;
+    const int L = which[4][e], R = which[2][e], T = which[3][e], B = which[1][e];
+    int r = 0;
+    if (L && !R && !T && !B) r = 1;
+    else if (!L && R && !T && !B) r = 1;
+    else if (!L && !R && T && !B) r = 2;
+    else if (!L && !R && !T && B) r = 2;
+    else if (L && !R && T && !B) r = 3;
+    else if (!L && R && T && !B) r = 3;
+    else if (L && !R && !T && B) r = 3;
+    else if (!L && R && !T && B) r = 3;
+    s->region[e] = r;
+  }
+  for (int e = 0; e < nspec; e++)
+    if (s->is_pml[e]) s->spec_to_pml[e] = ++s->nspec_pml;
+  for (int b = 1; b <= 4; b++) free(which[b]);
+  free(corner);
+  if (s->nspec_pml == 0) return;
+  s->pml_on = 1;
+
+  /* damping profiles, define_PML_coefficients (pml_init.F90:592-1366) */
+  const double NPOWER = 2.0, Rcoef = 0.001, PI = 3.141592653589793;
+  double xmin = 1e300, zmin = 1e300, xmax = -1e300, zmax = -1e300;
+  for (int g = 0; g < nglob; g++) {
+    const double x = s->coord[2 * g], z = s->coord[2 * g + 1];
+    if (x < xmin) xmin = x;
+    if (z < zmin) zmin = z;
+    if (x > xmax) xmax = x;
+    if (z > zmax) zmax = z;
+  }
+  const double xorigin = xmin + (xmax - xmin) / 2.0, zorigin = zmin + (zmax - zmin) / 2.0;
+  double zminb = 1e30, zmaxb = -1e30, xminr = 1e30, xmaxr = -1e30, zmint = 1e30, zmaxt = -1e30, xminl = 1e30,
+         xmaxl = -1e30;
+  for (int e = 0; e < nspec; e++) {
+    if (!s->is_pml[e]) continue;
+    const int r = s->region[e];
+    for (int j = 0; j < NGLL; j++)
+      for (int i = 0; i < NGLL; i++) {
+        const int g = s->ibool[(size_t)e * NGLL2 + j * NGLL + i] - 1;
+        const double x = s->coord[2 * g], z = s->coord[2 * g + 1];
+        if (z < zorigin && (r == 2 || r == 3)) { zmaxb = fmax(z, zmaxb); zminb = fmin(z, zminb); }
+        if (x > xorigin && (r == 1 || r == 3)) { xmaxr = fmax(x, xmaxr); xminr = fmin(x, xminr); }
+        if (z > zorigin && (r == 2 || r == 3)) { zmaxt = fmax(z, zmaxt); zmint = fmin(z, zmint); }
+        if (x < xorigin && (r == 1 || r == 3)) { xmaxl = fmax(x, xmaxl); xminl = fmin(x, xminl); }
+      }
+  }
+  double thl = xmaxl - xminl, thr = xmaxr - xminr, thb = zmaxb - zminb, tht = zmaxt - zmint;
+  if (fabs(thl) > 1e30) thl = 0.0;
+  if (fabs(thr) > 1e30) thr = 0.0;
+  if (fabs(thb) > 1e30) thb = 0.0;
+  if (fabs(tht) > 1e30) tht = 0.0;
+  const double xoleft = thl + xmin, xoright = xmax - thr, zobottom = thb + zmin, zotop = zmax - tht;
+  double vpmax_ac = 0.0, vpmax_el = 0.0;
+  for (int e = 0; e < nspec; e++) {
+    if (!s->is_pml[e]) continue;
+    for (int l = 0; l < NGLL2; l++) {
+      const size_t q = (size_t)e * NGLL2 + l;
+      const double vp = (double)(s->rho_vp[q] / s->rhostore[q]);
+      if (s->is_acoustic[e]) vpmax_ac = fmax(vpmax_ac, vp);
+      else vpmax_el = fmax(vpmax_el, vp);
+    }
+  }
+  double f0max = -1e300;
+  for (int k = 0; k < s->nsrc; k++) f0max = fmax(f0max, s->src[k].f0);
+  const double alpha_max = PI * f0max;
+  s->pml_alpha_max = alpha_max;
+  double d0[4][2]; /* [left, right, bottom, top][acoustic, elastic] */
+  const double th[4] = {thl, thr, thb, tht};
+  for (int k = 0; k < 4; k++) {
+    if (th[k] > 0.0) {
+      d0[k][0] = -(NPOWER + 1) * vpmax_ac * log(Rcoef) / (2.0 * th[k]);
+      d0[k][1] = -(NPOWER + 1) * vpmax_el * log(Rcoef) / (2.0 * th[k]);
+    } else {
+      d0[k][0] = d0[k][1] = 0.0;
+    }
+  }
+  const size_t np = (size_t)s->nspec_pml * NGLL2;
+  s->pml_kx = (double *)calloc(np, sizeof(double));
+  s->pml_kz = (double *)calloc(np, sizeof(double));
+  s->pml_dx = (double *)calloc(np, sizeof(double));
+  s->pml_dz = (double *)calloc(np, sizeof(double));
+  s->pml_ax = (double *)calloc(np, sizeof(double));
+  s->pml_az = (double *)calloc(np, sizeof(double));
+  const double kmin = s->p.k_min_pml, kmax = s->p.k_max_pml;
+  for (int e = 0; e < nspec; e++) {
+    if (!s->is_pml[e]) continue;
+    const int r = s->region[e], ac = s->is_acoustic[e];
+    const double damp = ac ? s->p.damping_ac : s->p.damping_el;
+    for (int j = 0; j < NGLL; j++)
+      for (int i = 0; i < NGLL; i++) {
+        double dx = 0.0, dz = 0.0, kx = 1.0, kz = 1.0, ax = 0.0, az = 0.0;
+        const int g = s->ibool[(size_t)e * NGLL2 + j * NGLL + i] - 1;
+        const double x = s->coord[2 * g], z = s->coord[2 * g + 1];
+        /* bottom, top, right, left edges in this order (pml_init.F90:908-1012) */
+        for (int side = 0; side < 4; side++) {
+          double a;
+          int zside = side < 2, apply;
+          if (side == 0) { apply = z < zorigin; a = zobottom - z; }
+          else if (side == 1) { apply = z > zorigin; a = z - zotop; }
+          else if (side == 2) { apply = x > xorigin; a = x - xoright; }
+          else { apply = x < xorigin; a = xoleft - x; }
+          if (!apply) continue;
+          if (zside ? !(r == 2 || r == 3) : !(r == 1 || r == 3)) continue;
+          const int kd = side == 0 ? 2 : side == 1 ? 3 : side == 2 ? 1 : 0;
+          double dd = 0.0, kk = 1.0, aa = 0.0;
+          if (a >= 0.0) {
+            const double an = a / th[kd];
+            dd = d0[kd][ac ? 0 : 1] / damp * (an * an);
+            kk = kmin + (kmax - 1.0) * (an * an);
+            aa = alpha_max * (1.0 - an);
+          }
+          if (zside) {
+            dz = dd; kz = kk; az = aa;
+            if (r == 2) { dx = 0.0; kx = 1.0; ax = 0.0; }
+          } else {
+            dx = dd; kx = kk; ax = aa;
+            if (r == 1) { dz = 0.0; kz = 1.0; az = 0.0; }
+          }
+        }
+        const size_t q = (size_t)(s->spec_to_pml[e] - 1) * NGLL2 + j * NGLL + i;
+        s->pml_dx[q] = dx; s->pml_dz[q] = dz; s->pml_kx[q] = kx; s->pml_kz[q] = kz;
+        s->pml_ax[q] = ax; s->pml_az[q] = az;
+      }
+  }
+  /* separation of the corner parameters (pml_init.F90:1269-1356) */
+  double dmin = 1.0e30;
+  for (int e = 0; e < nspec; e++) {
+    if (!s->is_pml[e]) continue;
+    for (int j = 0; j < NGLL - 1; j++)
+      for (int i = 0; i < NGLL - 1; i++) {
+        const int *ib = s->ibool + (size_t)e * NGLL2;
+        const int g1 = ib[j * NGLL + i] - 1, g2 = ib[j * NGLL + i + 1] - 1, g3 = ib[(j + 1) * NGLL + i] - 1;
+        const double x1 = s->coord[2 * g1], z1 = s->coord[2 * g1 + 1];
+        double x2 = s->coord[2 * g2], z2 = s->coord[2 * g2 + 1];
+        double dist = (x1 - x2) * (x1 - x2) + (z1 - z2) * (z1 - z2);
+        if (dist < dmin) dmin = dist;
+        x2 = s->coord[2 * g3]; z2 = s->coord[2 * g3 + 1];
+        dist = (x1 - x2) * (x1 - x2) + (z1 - z2) * (z1 - z2);
+        if (dist < dmin) dmin = dist;
+      }
+  }
+  dmin = sqrt(dmin);
+  const double thzmax = fmax(thb, tht), thxmax = fmax(thl, thr);
+  const double mind = alpha_max / 8.0 * dmin / fmax(thxmax, thzmax);
+  const double two = mind * 2.0;
+  for (int e = 0; e < nspec; e++) {
+    if (!s->is_pml[e] || s->region[e] != 3) continue;
+    for (int l = 0; l < NGLL2; l++) {
+      const size_t q = (size_t)(s->spec_to_pml[e] - 1) * NGLL2 + l;
+      const double kx = s->pml_kx[q], kz = s->pml_kz[q];
+      double dx = s->pml_dx[q], dz = s->pml_dz[q], ax = s->pml_ax[q], az = s->pml_az[q];
+      if (fabs(ax - az) < mind) {
+        if (ax > az) ax = az + two;
+        else az = ax + two;
+      }
+      double bx = ax + dx / kx;
+      if (fabs(bx - az) < mind) bx = az + two;
+      double bz = az + dz / kz;
+      if (fabs(bz - ax) < mind) bz = ax + two;
+      dx = (bx - ax) * kx;
+      dz = (bz - az) * kz;
+      s->pml_dx[q] = dx; s->pml_ax[q] = ax; s->pml_dz[q] = dz; s->pml_az[q] = az;
+    }
+  }
+  /* Dirichlet points on the outer boundary of the layer, determine_boundary_abs_points_PML
+     (pml_init.F90:306-421): every point of an absorbing edge of a PML element, in increasing iglob */
+  char *mel = (char *)calloc((size_t)nglob + 1, 1), *mac = (char *)calloc((size_t)nglob + 1, 1);
+  for (int a = 0; a < s->nabs; a++) {
+    const int e = s->abs_ispec[a] - 1, code = s->abs_code[a];
+    if (!s->is_pml[e]) continue;
+    for (int k = 0; k < NGLL; k++) {
+      int i, j;
+      side_point(code, k, &i, &j);
+      const int g = s->ibool[(size_t)e * NGLL2 + j * NGLL + i] - 1;
+      if (s->is_acoustic[e]) mac[g] = 1;
+      else mel[g] = 1;
+    }
+  }
+  s->npml_el = s->npml_ac = 0;
+  for (int g = 0; g < nglob; g++) { s->npml_el += mel[g]; s->npml_ac += mac[g]; }
+  s->pml_pts_el = (int *)calloc((size_t)s->npml_el + 1, sizeof(int));
+  s->pml_pts_ac = (int *)calloc((size_t)s->npml_ac + 1, sizeof(int));
+  int ne = 0, na = 0;
+  for (int g = 0; g < nglob; g++) {
+    if (mel[g]) s->pml_pts_el[ne++] = g + 1;
+    if (mac[g]) s->pml_pts_ac[na++] = g + 1;
+  }
+  free(mel);
+  free(mac);
+  s->rm_dux_dx = (REAL *)calloc(np * 2, sizeof(REAL));
+  s->rm_duz_dx = (REAL *)calloc(np * 2, sizeof(REAL));
+  s->rm_dux_dz = (REAL *)calloc(np * 2, sizeof(REAL));
+  s->rm_duz_dz = (REAL *)calloc(np * 2, sizeof(REAL));
+  s->rm_displ = (REAL *)calloc(np * 4, sizeof(REAL));
+  s->rm_ac_dx = (REAL *)calloc(np * 2, sizeof(REAL));
+  s->rm_ac_dz = (REAL *)calloc(np * 2, sizeof(REAL));
+  s->rm_pot = (REAL *)calloc(np * 2, sizeof(REAL));
+}
+
+/* A deliberate mistake planted in the CPML terms, for the tests that show each term is pinned by the reference's
+   traces (0 = none): 1 previous-step fields without the factor 1/2 on the acceleration term, 2 no Newmark damping
+   term in the mass matrix, 3 no zero condition on the outer boundary, 4 symmetric stress in PML elements, 5 no
+   damping terms in the acceleration. */
+static int g_pml_mistake = 0;
+void orc_set_pml_mistake(int which) { g_pml_mistake = which; }
+
+/* recursive convolution coefficients exp(-b dt), (1 - exp(-b dt/2))/b, (1 - exp(-b dt/2)) exp(-b dt/2)/b
+   (pml_compute.f90:34-89, compute_coef_convolution) */
+static void pml_coef(double bb, double DT, double *c0, double *c1, double *c2) {
+  const double temp = exp(-0.5 * bb * DT);
+  *c0 = temp * temp;
+  if (fabs(bb) > 1e-5) {
+    *c1 = (1.0 - temp) / bb;
+    *c2 = *c1 * temp;
+  } else {
+    *c1 = 0.5 * DT;
+    *c2 = *c1;
+  }
+}
+
+/* coefficients of the stretched gradient (pml_compute.f90:93-172, lik_parameter_computation); index_ik 13 or 31
+   swaps the roles of the x-only and z-only regions */
+static void pml_lik(double DT, double kx, double bx, double ax, double kz, double bz, double az, int region,
+                    int index_ik, double A[3], double c1[3], double c2[3]) {
+  const int xonly = index_ik == 13 ? 1 : 2, zonly = index_ik == 13 ? 2 : 1;
+  if (region == 3) {
+    const double A0 = kx / kz;
+    const double gx = (ax * bz + ax * ax + 2.0 * bx * az - 2.0 * ax * (bx + az)) / (bz - ax);
+    const double gz = (ax * bz + bz * bz + 2.0 * bx * az - 2.0 * bz * (bx + az)) / (ax - bz);
+    A[0] = A0;
+    A[1] = A0 / 2.0 * (gx - ax);
+    A[2] = A0 / 2.0 * (gz - bz);
+  } else if (region == xonly) {
+    A[0] = kx;
+    A[1] = -(kx * (ax - bx));
+    A[2] = 0.0;
+  } else if (region == zonly) {
+    const double A0 = 1.0 / kz;
+    A[0] = A0;
+    A[1] = 0.0;
+    A[2] = -(A0 * (bz - az));
+  } else {
+    A[0] = A[1] = A[2] = 0.0; /* the reference leaves them undefined outside the three regions */
+  }
+  pml_coef(ax, DT, &c1[0], &c1[1], &c1[2]);
+  pml_coef(bz, DT, &c2[0], &c2[1], &c2[2]);
+}
+
+/* coefficients of the damping terms of the acceleration (pml_compute.f90:244-318, l_parameter_computation) */
+static void pml_l(double DT, double kx, double bx, double ax, double kz, double bz, double az, int region,
+                  double A[5], double c1[3], double c2[3]) {
+  if (region == 3) {
+    const double A0 = kx * kz;
+    const double gx = (ax * az + ax * ax + 2.0 * bx * bz - 2.0 * ax * (bx + bz)) / (az - ax);
+    const double gz = (ax * az + az * az + 2.0 * bx * bz - 2.0 * az * (bx + bz)) / (ax - az);
+    A[0] = A0;
+    A[1] = A0 / 2.0 * (gx - ax + gz - az);
+    A[2] = A0 / 2.0 * (ax * ax - gx * ax + az * az - gz * az);
+    A[3] = A0 / 2.0 * (ax * ax) * (gx - ax);
+    A[4] = A0 / 2.0 * (az * az) * (gz - az);
+  } else if (region == 1) {
+    A[0] = kx;
+    A[1] = kx * (bx - ax);
+    A[2] = -(kx * ax * (bx - ax));
+    A[3] = kx * (ax * ax) * (bx - ax);
+    A[4] = 0.0;
+  } else if (region == 2) {
+    A[0] = kz;
+    A[1] = kz * (bz - az);
+    A[2] = -(kz * az * (bz - az));
+    A[3] = 0.0;
+    A[4] = kz * (az * az) * (bz - az);
+  } else {
+    A[0] = A[1] = A[2] = A[3] = A[4] = 0.0;
+  }
+  pml_coef(ax, DT, &c1[0], &c1[1], &c1[2]);
+  pml_coef(az, DT, &c2[0], &c2[1], &c2[2]);
+}
+
+/* the per-point inputs of the coefficient routines: K, beta = alpha + d/K and alpha along x and z */
+static void pml_point(const orc_sim *s, size_t q, double *kx, double *bx, double *ax, double *kz, double *bz,
+                      double *az) {
+  *kx = s->pml_kx[q]; *kz = s->pml_kz[q];
+  *ax = s->pml_ax[q]; *az = s->pml_az[q];
+  *bx = *ax + s->pml_dx[q] / *kx;
+  *bz = *az + s->pml_dz[q] / *kz;
+}
+
+/* zero fields on the outer boundary of the layer (pml_compute.f90:520-605, pml_boundary_acoustic and
+   pml_boundary_elastic) */
+static void pml_boundary_elastic(orc_sim *s) {
+  for (int k = 0; k < s->npml_el; k++) {
+    const int g = s->pml_pts_el[k] - 1;
+    for (int c = 0; c < 2; c++) {
+      s->displ_old[2 * g + c] = (REAL)0;
+      s->displ[2 * g + c] = (REAL)0;
+      s->veloc[2 * g + c] = (REAL)0;
+      s->accel[2 * g + c] = (REAL)0;
+    }
+  }
+}
+
+static void pml_boundary_acoustic(orc_sim *s) {
+  for (int k = 0; k < s->npml_ac; k++) {
+    const int g = s->pml_pts_ac[k] - 1;
+    s->pot_old[g] = (REAL)0;
+    s->pot[g] = (REAL)0;
+    s->pot_dot[g] = (REAL)0;
+    s->pot_dd[g] = (REAL)0;
+  }
+}
+
 static void edge_normal(const orc_sim *s, int code, size_t l, REAL *nx, REAL *nz, REAL *jac1d) {
   REAL xxi, zxi, xgamma, zgamma;
   if (code == 4 || code == 2) {
@@ -601,14 +974,39 @@ static void setup_mass(orc_sim *s) {
         size_t l = (size_t)e * NGLL2 + j * NGLL + i;
         int g = s->ibool[l] - 1;
         REAL wxz = s->wxgll[i] * s->wxgll[j];
+        const int r = (s->pml_on && s->is_pml[e]) ? s->region[e] : -1;
         if (s->is_acoustic[e]) {
-          /* w_i w_j J (single precision) divided by the double kappa (invert_mass_matrix.F90:424-427) */
           double kappal = (double)s->kappastore[l];
-          rma[g] = (REAL)((double)rma[g] + (double)(wxz * s->jacobian[l]) / kappal);
+          if (r < 0) {
+            /* w_i w_j J (single precision) divided by the double kappa (invert_mass_matrix.F90:424-427) */
+            rma[g] = (REAL)((double)rma[g] + (double)(wxz * s->jacobian[l]) / kappal);
+          } else if (r > 0) {
+            /* CPML: w_i w_j / kappa J times K, then the Newmark term dt/2 d (invert_mass_matrix.F90:262-345) */
+            const size_t q = (size_t)(s->spec_to_pml[e] - 1) * NGLL2 + j * NGLL + i;
+            const double base = (double)wxz / kappal * (double)s->jacobian[l];
+            const double kx = s->pml_kx[q], kz = s->pml_kz[q], dx = s->pml_dx[q], dz = s->pml_dz[q];
+            const double dto2 = (double)(REAL)(0.5 * (double)(REAL)s->p.dt);
+            const double kterm = r == 1 ? kx : r == 3 ? kx * kz : kz;
+            const double dterm = r == 1 ? dx * dto2 : r == 3 ? (dx * kz + dz * kx) * dto2 : dz * dto2;
+            rma[g] = (REAL)((double)rma[g] + base * kterm);
+            rma[g] = (REAL)((double)rma[g] + base * dterm);
+          }
           continue;
         }
         double rhol = (double)s->rhostore[l];
-        rm[2 * g] = (REAL)((double)rm[2 * g] + (double)wxz * rhol * (double)s->jacobian[l]);
+        if (r < 0) {
+          rm[2 * g] = (REAL)((double)rm[2 * g] + (double)wxz * rhol * (double)s->jacobian[l]);
+        } else if (r > 0) {
+          /* CPML: w_i w_j rho J times K, then the Newmark term dt/2 d (invert_mass_matrix.F90:133-233) */
+          const size_t q = (size_t)(s->spec_to_pml[e] - 1) * NGLL2 + j * NGLL + i;
+          const double base = (double)wxz * rhol * (double)s->jacobian[l];
+          const double kx = s->pml_kx[q], kz = s->pml_kz[q], dx = s->pml_dx[q], dz = s->pml_dz[q];
+          const double dto2 = (double)(REAL)(0.5 * (double)(REAL)s->p.dt);
+          const double kterm = r == 1 ? kx : r == 3 ? kx * kz : kz;
+          const double dterm = r == 1 ? dx * dto2 : r == 3 ? (dx * kz + dz * kx) * dto2 : dz * dto2;
+          rm[2 * g] = (REAL)((double)rm[2 * g] + base * kterm);
+          if (g_pml_mistake != 2) rm[2 * g] = (REAL)((double)rm[2 * g] + base * dterm);
+        }
         rm[2 * g + 1] = rm[2 * g];
       }
   if (s->p.stacey) {
@@ -857,9 +1255,12 @@ static void element_forces(orc_sim *s, const REAL *displ, REAL *accel) {
   const REAL (*HW)[NGLL] = (const REAL(*)[NGLL])s->hprimewgll;
   REAL u[2][NGLL][NGLL]; /* u[c][j][i] */
   REAL t1x[NGLL][NGLL], t1z[NGLL][NGLL], t2x[NGLL][NGLL], t2z[NGLL][NGLL];
+  REAL pml_acc[NGLL2][2];
+  const double DT = s->p.dt;
   for (int e = 0; e < s->nspec; e++) {
     if (s->is_acoustic[e]) continue;
     const int *ib = s->ibool + (size_t)e * NGLL2;
+    const int pml = s->pml_on && s->is_pml[e];
     for (int j = 0; j < NGLL; j++)
       for (int i = 0; i < NGLL; i++) {
         int g = ib[j * NGLL + i] - 1;
@@ -883,6 +1284,40 @@ static void element_forces(orc_sim *s, const REAL *displ, REAL *accel) {
         REAL dux_dzl = dux_dxi * xizl + dux_dga * gazl;
         REAL duz_dxl = duz_dxi * xixl + duz_dga * gaxl;
         REAL duz_dzl = duz_dxi * xizl + duz_dga * gazl;
+        /* CPML: the unmodified gradient (PML_du*) and the stretched one through the memory variables of the
+           current and previous-step gradients (pml_compute_memory_variables.f90:230-540, Newmark, not rotated) */
+        REAL p_dux_dxl = dux_dxl, p_dux_dzl = dux_dzl, p_duz_dxl = duz_dxl, p_duz_dzl = duz_dzl;
+        if (pml) {
+          const REAL *dold = s->displ_old;
+          REAL oxi = 0, ozi = 0, oxg = 0, ozg = 0;
+          for (int k = 0; k < NGLL; k++) {
+            oxi = oxi + dold[2 * (ib[j * NGLL + k] - 1)] * H[i][k];
+            ozi = ozi + dold[2 * (ib[j * NGLL + k] - 1) + 1] * H[i][k];
+            oxg = oxg + dold[2 * (ib[k * NGLL + i] - 1)] * H[j][k];
+            ozg = ozg + dold[2 * (ib[k * NGLL + i] - 1) + 1] * H[j][k];
+          }
+          const REAL o_dux_dxl = oxi * xixl + oxg * gaxl, o_dux_dzl = oxi * xizl + oxg * gazl;
+          const REAL o_duz_dxl = ozi * xixl + ozg * gaxl, o_duz_dzl = ozi * xizl + ozg * gazl;
+          const size_t q = (size_t)(s->spec_to_pml[e] - 1) * NGLL2 + j * NGLL + i;
+          double kx, bx, ax, kz, bz, az, Azx[3], Axz[3], czx1[3], czx2[3], cxz1[3], cxz2[3];
+          pml_point(s, q, &kx, &bx, &ax, &kz, &bz, &az);
+          pml_lik(DT, kz, bz, az, kx, bx, ax, s->region[e], 31, Azx, czx1, czx2);
+          pml_lik(DT, kx, bx, ax, kz, bz, az, s->region[e], 13, Axz, cxz1, cxz2);
+          REAL *rxx = s->rm_dux_dx + 2 * q, *rzx = s->rm_duz_dx + 2 * q;
+          REAL *rxz = s->rm_dux_dz + 2 * q, *rzz = s->rm_duz_dz + 2 * q;
+          rxx[0] = (REAL)(czx1[0] * rxx[0] + czx1[1] * p_dux_dxl + czx1[2] * o_dux_dxl);
+          rzx[0] = (REAL)(czx1[0] * rzx[0] + czx1[1] * p_duz_dxl + czx1[2] * o_duz_dxl);
+          rxx[1] = (REAL)(czx2[0] * rxx[1] + czx2[1] * p_dux_dxl + czx2[2] * o_dux_dxl);
+          rzx[1] = (REAL)(czx2[0] * rzx[1] + czx2[1] * p_duz_dxl + czx2[2] * o_duz_dxl);
+          rxz[0] = (REAL)(cxz1[0] * rxz[0] + cxz1[1] * p_dux_dzl + cxz1[2] * o_dux_dzl);
+          rzz[0] = (REAL)(cxz1[0] * rzz[0] + cxz1[1] * p_duz_dzl + cxz1[2] * o_duz_dzl);
+          rxz[1] = (REAL)(cxz2[0] * rxz[1] + cxz2[1] * p_dux_dzl + cxz2[2] * o_dux_dzl);
+          rzz[1] = (REAL)(cxz2[0] * rzz[1] + cxz2[1] * p_duz_dzl + cxz2[2] * o_duz_dzl);
+          dux_dxl = (REAL)(Azx[0] * p_dux_dxl + Azx[1] * rxx[0] + Azx[2] * rxx[1]);
+          duz_dxl = (REAL)(Azx[0] * p_duz_dxl + Azx[1] * rzx[0] + Azx[2] * rzx[1]);
+          dux_dzl = (REAL)(Axz[0] * p_dux_dzl + Axz[1] * rxz[0] + Axz[2] * rxz[1]);
+          duz_dzl = (REAL)(Axz[0] * p_duz_dzl + Axz[1] * rzz[0] + Axz[2] * rzz[1]);
+        }
         REAL mul = s->mustore[l], rhol = s->rhostore[l];
         REAL cpl = s->rho_vp[l] / rhol;
         REAL lambdal = rhol * cpl * cpl - (REAL)2 * mul;
@@ -919,11 +1354,42 @@ static void element_forces(orc_sim *s, const REAL *displ, REAL *accel) {
           szz = szz + lpm * e1s - (REAL)2 * mul * e11s;
         }
         REAL szx = sxz;
+        if (pml) {
+          /* non-symmetric stress of the stretched equations (compute_forces_viscoelastic.F90:596-611) */
+          sxx = lp2m * dux_dxl + lambdal * p_duz_dzl;
+          szz = lp2m * duz_dzl + lambdal * p_dux_dxl;
+          szx = mul * (p_duz_dxl + dux_dzl);
+          sxz = mul * (p_dux_dzl + duz_dxl);
+          if (g_pml_mistake == 4) szx = sxz = mul * (duz_dxl + dux_dzl);
+        }
         t1x[j][i] = jacl * (sxx * xixl + szx * xizl);
         t1z[j][i] = jacl * (sxz * xixl + szz * xizl);
         t2x[j][i] = jacl * (sxx * gaxl + szx * gazl);
         t2z[j][i] = jacl * (sxz * gaxl + szz * gazl);
       }
+    if (pml) {
+      /* damping terms of the acceleration with the memory variables of the displacement
+         (pml_compute_accel_contribution.f90:187-333, Newmark) */
+      for (int j = 0; j < NGLL; j++)
+        for (int i = 0; i < NGLL; i++) {
+          const size_t l = (size_t)e * NGLL2 + j * NGLL + i;
+          const int g = ib[j * NGLL + i] - 1;
+          const REAL fac = s->rhostore[l] * s->jacobian[l];
+          const size_t q = (size_t)(s->spec_to_pml[e] - 1) * NGLL2 + j * NGLL + i;
+          double kx, bx, ax, kz, bz, az, A[5], c1[3], c2[3];
+          pml_point(s, q, &kx, &bx, &ax, &kz, &bz, &az);
+          pml_l(DT, kx, bx, ax, kz, bz, az, s->region[e], A, c1, c2);
+          REAL *rd = s->rm_displ + 4 * q; /* [series][component] */
+          for (int c = 0; c < 2; c++) {
+            rd[c] = (REAL)(c1[0] * rd[c] + c1[1] * u[c][j][i] + c1[2] * s->displ_old[2 * g + c]);
+            rd[2 + c] = (REAL)(c2[0] * rd[2 + c] + c2[1] * u[c][j][i] + c2[2] * s->displ_old[2 * g + c]);
+          }
+          const REAL w = s->wxgll[i] * s->wxgll[j] * fac;
+          for (int c = 0; c < 2; c++)
+            pml_acc[j * NGLL + i][c] = (REAL)(w * (A[1] * s->veloc[2 * g + c] + A[2] * u[c][j][i] + A[3] * rd[c] +
+                                                   A[4] * rd[2 + c]));
+        }
+    }
     for (int j = 0; j < NGLL; j++)
       for (int i = 0; i < NGLL; i++) {
         REAL a1 = 0, a2 = 0, a3 = 0, a4 = 0;
@@ -936,6 +1402,12 @@ static void element_forces(orc_sim *s, const REAL *displ, REAL *accel) {
         int g = ib[j * NGLL + i] - 1;
         accel[2 * g] = accel[2 * g] - (s->wxgll[j] * a1 + s->wxgll[i] * a2);
         accel[2 * g + 1] = accel[2 * g + 1] - (s->wxgll[j] * a3 + s->wxgll[i] * a4);
+      }
+    if (pml && g_pml_mistake != 5)
+      for (int l = 0; l < NGLL2; l++) {
+        const int g = ib[l] - 1;
+        accel[2 * g] = accel[2 * g] - pml_acc[l][0];
+        accel[2 * g + 1] = accel[2 * g + 1] - pml_acc[l][1];
       }
   }
 }
@@ -998,9 +1470,12 @@ static void acoustic_forces(orc_sim *s, const REAL *pot, REAL *pot_dd) {
   const REAL (*H)[NGLL] = (const REAL(*)[NGLL])s->hprime;
   const REAL (*HW)[NGLL] = (const REAL(*)[NGLL])s->hprimewgll;
   REAL u[NGLL][NGLL], t1[NGLL][NGLL], t2[NGLL][NGLL]; /* [j][i] */
+  REAL pml_acc[NGLL2];
+  const double DT = s->p.dt;
   for (int e = 0; e < s->nspec; e++) {
     if (!s->is_acoustic[e]) continue;
     const int *ib = s->ibool + (size_t)e * NGLL2;
+    const int pml = s->pml_on && s->is_pml[e];
     for (int j = 0; j < NGLL; j++)
       for (int i = 0; i < NGLL; i++) u[j][i] = pot[ib[j * NGLL + i] - 1];
     for (int j = 0; j < NGLL; j++)
@@ -1013,9 +1488,49 @@ static void acoustic_forces(orc_sim *s, const REAL *pot, REAL *pot_dd) {
         REAL fac = s->jacobian[l] / s->rhostore[l];
         REAL dxl = dxi * xixl + dga * gaxl;
         REAL dzl = dxi * xizl + dga * gazl;
+        if (pml) {
+          /* stretched gradient of the potential (pml_compute_memory_variables.f90:42-224, Newmark) */
+          const REAL *pold = s->pot_old;
+          REAL oxi = 0, oga = 0;
+          for (int k = 0; k < NGLL; k++) {
+            oxi = oxi + pold[ib[j * NGLL + k] - 1] * H[i][k];
+            oga = oga + pold[ib[k * NGLL + i] - 1] * H[j][k];
+          }
+          const REAL odxl = oxi * xixl + oga * gaxl, odzl = oxi * xizl + oga * gazl;
+          const REAL pdxl = dxl, pdzl = dzl;
+          const size_t q = (size_t)(s->spec_to_pml[e] - 1) * NGLL2 + j * NGLL + i;
+          double kx, bx, ax, kz, bz, az, Azx[3], Axz[3], czx1[3], czx2[3], cxz1[3], cxz2[3];
+          pml_point(s, q, &kx, &bx, &ax, &kz, &bz, &az);
+          pml_lik(DT, kz, bz, az, kx, bx, ax, s->region[e], 31, Azx, czx1, czx2);
+          pml_lik(DT, kx, bx, ax, kz, bz, az, s->region[e], 13, Axz, cxz1, cxz2);
+          REAL *rx = s->rm_ac_dx + 2 * q, *rz = s->rm_ac_dz + 2 * q;
+          rx[0] = (REAL)(czx1[0] * rx[0] + czx1[1] * pdxl + czx1[2] * odxl);
+          rx[1] = (REAL)(czx2[0] * rx[1] + czx2[1] * pdxl + czx2[2] * odxl);
+          rz[0] = (REAL)(cxz1[0] * rz[0] + cxz1[1] * pdzl + cxz1[2] * odzl);
+          rz[1] = (REAL)(cxz2[0] * rz[1] + cxz2[1] * pdzl + cxz2[2] * odzl);
+          dxl = (REAL)(Azx[0] * pdxl + Azx[1] * rx[0] + Azx[2] * rx[1]);
+          dzl = (REAL)(Axz[0] * pdzl + Axz[1] * rz[0] + Axz[2] * rz[1]);
+        }
         t1[j][i] = fac * (xixl * dxl + xizl * dzl);
         t2[j][i] = fac * (gaxl * dxl + gazl * dzl);
       }
+    if (pml)
+      /* damping terms with the memory variables of the potential (pml_compute_accel_contribution.f90:42-181) */
+      for (int j = 0; j < NGLL; j++)
+        for (int i = 0; i < NGLL; i++) {
+          const size_t l = (size_t)e * NGLL2 + j * NGLL + i;
+          const int g = ib[j * NGLL + i] - 1;
+          const REAL fac = s->jacobian[l] / s->kappastore[l];
+          const size_t q = (size_t)(s->spec_to_pml[e] - 1) * NGLL2 + j * NGLL + i;
+          double kx, bx, ax, kz, bz, az, A[5], c1[3], c2[3];
+          pml_point(s, q, &kx, &bx, &ax, &kz, &bz, &az);
+          pml_l(DT, kx, bx, ax, kz, bz, az, s->region[e], A, c1, c2);
+          REAL *rp = s->rm_pot + 2 * q;
+          rp[0] = (REAL)(c1[0] * rp[0] + c1[1] * pot[g] + c1[2] * s->pot_old[g]);
+          rp[1] = (REAL)(c2[0] * rp[1] + c2[1] * pot[g] + c2[2] * s->pot_old[g]);
+          const REAL w = s->wxgll[i] * s->wxgll[j] * fac;
+          pml_acc[j * NGLL + i] = (REAL)(w * (A[1] * s->pot_dot[g] + A[2] * pot[g] + A[3] * rp[0] + A[4] * rp[1]));
+        }
     for (int j = 0; j < NGLL; j++)
       for (int i = 0; i < NGLL; i++) {
         REAL a1 = 0, a2 = 0;
@@ -1026,6 +1541,8 @@ static void acoustic_forces(orc_sim *s, const REAL *pot, REAL *pot_dd) {
         int g = ib[j * NGLL + i] - 1;
         pot_dd[g] = pot_dd[g] - (s->wxgll[j] * a1 + s->wxgll[i] * a2);
       }
+    if (pml)
+      for (int l = 0; l < NGLL2; l++) pot_dd[ib[l] - 1] = pot_dd[ib[l] - 1] - pml_acc[l];
   }
 }
 
@@ -1209,12 +1726,13 @@ void *orc_new(const orc_params *p, const double *rho, const double *vp, const do
   mesh_internal(s);
   setup_numbering(s);
   setup_geometry_materials(s, rho, vp, vs);
-  if (p->stacey) setup_absorbing(s);
+  if (p->stacey || p->pml) setup_absorbing(s);
   setup_domains(s);
-  setup_mass(s);
   s->nsrc = nsrc;
   s->src = (orc_source *)calloc((size_t)nsrc + 1, sizeof(orc_source));
   memcpy(s->src, src, sizeof(orc_source) * (size_t)nsrc);
+  setup_pml(s);
+  setup_mass(s);
   setup_sources(s);
   s->nrec = nrec;
   s->rec_xz = (double *)calloc((size_t)2 * nrec + 2, sizeof(double));
@@ -1226,6 +1744,8 @@ void *orc_new(const orc_params *p, const double *rho, const double *vp, const do
   s->pot = (REAL *)calloc((size_t)s->nglob, sizeof(REAL));
   s->pot_dot = (REAL *)calloc((size_t)s->nglob, sizeof(REAL));
   s->pot_dd = (REAL *)calloc((size_t)s->nglob, sizeof(REAL));
+  s->displ_old = (REAL *)calloc((size_t)2 * s->nglob, sizeof(REAL));
+  s->pot_old = (REAL *)calloc((size_t)s->nglob, sizeof(REAL));
   return s;
 }
 
@@ -1246,6 +1766,11 @@ void orc_free(void *h) {
   free(s->pot); free(s->pot_dot); free(s->pot_dd);
   free(s->att_on); free(s->att_A1); free(s->att_B1); free(s->att_A2); free(s->att_B2);
   free(s->e1); free(s->e11); free(s->e13); free(s->old_xx); free(s->old_zz); free(s->old_xz);
+  free(s->is_pml); free(s->region); free(s->spec_to_pml);
+  free(s->pml_kx); free(s->pml_kz); free(s->pml_dx); free(s->pml_dz); free(s->pml_ax); free(s->pml_az);
+  free(s->pml_pts_el); free(s->pml_pts_ac);
+  free(s->rm_dux_dx); free(s->rm_duz_dx); free(s->rm_dux_dz); free(s->rm_duz_dz); free(s->rm_displ);
+  free(s->rm_ac_dx); free(s->rm_ac_dz); free(s->rm_pot); free(s->displ_old); free(s->pot_old);
   free(s);
 }
 
@@ -1448,6 +1973,15 @@ void orc_forward(void *h, int nstep_run, double *seis, double *stacey_store_d) {
   for (int step = 0; step < nstep_run; step++) {
     const int it = s->it_done + step + 1;
     /* update_displacement_newmark_acoustic / _elastic (update_displacement_Newmark.F90:190-454) */
+    /* CPML keeps the previous step's fields, advanced by half the acceleration term
+       (update_displacement_Newmark.F90:373-380, 428-433) */
+    if (s->pml_on && s->any_acoustic)
+      for (size_t k = 0; k < n1; k++)
+        s->pot_old[k] = (REAL)((double)pa[k] + (double)deltatsqover2 / 2.0 * (double)pdd[k]);
+    if (s->pml_on && s->any_elastic)
+      for (size_t k = 0; k < n2; k++)
+        s->displ_old[k] = (REAL)((double)d[k] + (double)deltatsqover2 / (g_pml_mistake == 1 ? 1.0 : 2.0) *
+                                 (double)a[k]);
     if (s->any_acoustic)
       for (size_t k = 0; k < n1; k++) {
         pa[k] = pa[k] + deltat * pv[k] + deltatsqover2 * pdd[k];
@@ -1465,10 +1999,12 @@ void orc_forward(void *h, int nstep_run, double *seis, double *stacey_store_d) {
        (elements); mass; corrector; free surface */
     if (s->any_acoustic) {
       enforce_free_surface(s);
+      if (s->pml_on) pml_boundary_acoustic(s);
       if (s->p.stacey && s->nabs > 0) acoustic_stacey(s, pv, pdd);
       if (s->ncoup > 0) coupling_acoustic_el(s, d, pdd);
       add_sources_acoustic(s, pdd, it);
       acoustic_forces(s, pa, pdd);
+      if (s->pml_on) pml_boundary_acoustic(s);
       for (size_t k = 0; k < n1; k++) pdd[k] = pdd[k] * s->rmass_ac_inv[k];
       for (size_t k = 0; k < n1; k++) pv[k] = pv[k] + deltatover2 * pdd[k];
       enforce_free_surface(s);
@@ -1477,6 +2013,7 @@ void orc_forward(void *h, int nstep_run, double *seis, double *stacey_store_d) {
        iphase 2 (elements) */
     if (s->any_elastic) {
       if (s->p.stacey && s->nabs > 0) stacey_forces(s, v, a, store, step, nstep_run);
+      if (s->pml_on && g_pml_mistake != 3) pml_boundary_elastic(s);
       if (s->ncoup > 0) coupling_elastic_ac(s, pdd, a);
       add_sources(s, a, it);
       element_forces(s, d, a);
@@ -1497,6 +2034,38 @@ void orc_forward(void *h, int nstep_run, double *seis, double *stacey_store_d) {
 }
 
 void orc_set_ftz(void *h, int on) { ((orc_sim *)h)->ftz = on; }
+
+/* The CPML coefficient routines for one point: out[0:9] lik 31 (A, coef of bb_1, coef of bb_2), out[9:18] lik 13,
+   out[18:28] l (A0..A4, coef of alpha_x, coef of alpha_z); inputs K, d, alpha along x and z. */
+void orc_pml_coefficients(double DT, double kx, double dx, double ax, double kz, double dz, double az, int region,
+                          double *out) {
+  const double bx = ax + dx / kx, bz = az + dz / kz;
+  pml_lik(DT, kz, bz, az, kx, bx, ax, region, 31, out, out + 3, out + 6);
+  pml_lik(DT, kx, bx, ax, kz, bz, az, region, 13, out + 9, out + 12, out + 15);
+  pml_l(DT, kx, bx, ax, kz, bz, az, region, out + 18, out + 23, out + 26);
+}
+
+/* CPML set-up products: PML flag and region per element; K, d, alpha along x and z per point of each PML element
+   ([nspec_pml][25], PML elements in element order); the outer-boundary points (1-based iglob) */
+int orc_nspec_pml(void *h) { return ((orc_sim *)h)->nspec_pml; }
+int orc_npml_points(void *h, int elastic) {
+  orc_sim *s = (orc_sim *)h;
+  return elastic ? s->npml_el : s->npml_ac;
+}
+void orc_get_pml(void *h, int *is_pml, int *region, double *prof /* [6][nspec_pml*25]: kx kz dx dz ax az */,
+                 int *pts_el, int *pts_ac) {
+  orc_sim *s = (orc_sim *)h;
+  for (int e = 0; e < s->nspec; e++) {
+    is_pml[e] = s->is_pml[e];
+    region[e] = s->region[e];
+  }
+  const size_t np = (size_t)s->nspec_pml * NGLL2;
+  const double *src[6] = {s->pml_kx, s->pml_kz, s->pml_dx, s->pml_dz, s->pml_ax, s->pml_az};
+  for (int k = 0; k < 6; k++)
+    for (size_t q = 0; q < np; q++) prof[k * np + q] = src[k][q];
+  for (int k = 0; k < s->npml_el; k++) pts_el[k] = s->pml_pts_el[k];
+  for (int k = 0; k < s->npml_ac; k++) pts_ac[k] = s->pml_pts_ac[k];
+}
 
 /* acoustic domain (test accessors) */
 int orc_nspec_acoustic(void *h) { return ((orc_sim *)h)->nspec_acoustic; }

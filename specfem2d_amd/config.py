@@ -72,6 +72,14 @@ class SimulationConfig:
     qkappa: object = 9999.0
     qmu: object = 9999.0
     n_sls: int = 3
+    # PML_BOUNDARY_CONDITIONS: a convolutional PML on the absorbing sides (instead of Stacey edges), NELEM_PML_THICKNESS
+    # elements thick, with the Par_file's K_MIN_PML, K_MAX_PML and damping_change_factor_{acoustic,elastic}
+    pml: bool = False
+    nelem_pml: int = 3
+    pml_k_min: float = 1.0
+    pml_k_max: float = 1.0
+    pml_damping_acoustic: float = 0.5
+    pml_damping_elastic: float = 1.0
     name: str = "custom"
 
     # ------------------------------------------------------------------

@@ -215,11 +215,12 @@ def read_stations(path: str) -> List[Station]:
     return out
 
 
-def config_from_par_file(data_dir: str, cpml: str = "error", nstep: Optional[int] = None) -> Tuple[SimulationConfig, List[Station]]:
+def config_from_par_file(data_dir: str, cpml: str = "cpml", nstep: Optional[int] = None) -> Tuple[SimulationConfig, List[Station]]:
     """Build the configuration of ``data_dir/Par_file`` (+ its SOURCE, interfaces file and STATIONS).
 
-    ``cpml``: this path has no CPML; ``"error"`` refuses a Par_file that asks for it, ``"stacey"`` puts Stacey
-    edges on the absorbing sides instead (same results until the waves reach the absorbing layer)."""
+    ``cpml``: what to do with ``PML_BOUNDARY_CONDITIONS = .true.``: ``"cpml"`` (default) sets up the CPML layer
+    as the Par_file describes it (read_parameter_file.F90:843-900); ``"stacey"`` puts Stacey edges on the absorbing
+    sides instead (same results until the waves reach the absorbing layer); ``"error"`` refuses it."""
     par = read_par_file(os.path.join(data_dir, "Par_file"))
     g = par.get
 
@@ -245,11 +246,21 @@ def config_from_par_file(data_dir: str, cpml: str = "error", nstep: Optional[int
     refuse(int(g("NTSTEP_BETWEEN_OUTPUT_SAMPLE", "1")) != 1, "NTSTEP_BETWEEN_OUTPUT_SAMPLE > 1")
     pml = fortran_bool(g("PML_BOUNDARY_CONDITIONS", ".false."))
     stacey = fortran_bool(g("STACEY_ABSORBING_CONDITIONS", ".false."))
+    pml_kw = {}
     if pml:
-        if cpml != "stacey":
-            raise NotImplementedError("Par_file: PML_BOUNDARY_CONDITIONS (CPML) is not on this path; pass "
-                                      "cpml='stacey' to use Stacey edges on the absorbing sides instead")
-        stacey = True
+        if cpml == "stacey":
+            stacey, pml = True, False
+        elif cpml == "cpml":
+            refuse(stacey, "PML_BOUNDARY_CONDITIONS together with STACEY_ABSORBING_CONDITIONS")
+            refuse(fortran_bool(g("ROTATE_PML_ACTIVATE", ".false.")), "ROTATE_PML_ACTIVATE")
+            refuse(fortran_bool(g("PML_PARAMETER_ADJUSTMENT", ".false.")), "PML_PARAMETER_ADJUSTMENT")
+            pml_kw = dict(nelem_pml=int(g("NELEM_PML_THICKNESS", "3")),
+                          pml_k_min=fortran_float(g("K_MIN_PML", "1.d0")),
+                          pml_k_max=fortran_float(g("K_MAX_PML", "1.d0")),
+                          pml_damping_acoustic=fortran_float(g("damping_change_factor_acoustic", "0.5d0")),
+                          pml_damping_elastic=fortran_float(g("damping_change_factor_elastic", "1.d0")))
+        else:
+            raise NotImplementedError("Par_file: PML_BOUNDARY_CONDITIONS refused (cpml='error')")
     seismotypes = [int(t) for t in re.split(r"[ ,]+", g("seismotype").strip()) if t]
     refuse(len(seismotypes) != 1 or seismotypes[0] not in (1, 2, 3, 4, 6), f"seismotype = {g('seismotype')}")
     heights, nz_layers, _ = read_interfaces(os.path.join(data_dir, os.path.basename(g("interfacesfile"))))
@@ -293,7 +304,7 @@ def config_from_par_file(data_dir: str, cpml: str = "error", nstep: Optional[int
         receivers=np.array([[s.x, s.z] for s in stations]) if stations else None,
         user_t0=fortran_float(g("USER_T0", "0")), anglerec=fortran_float(g("anglerec", "0")),
         seismotype=seismotypes[0], use_trick=fortran_bool(g("USE_TRICK_FOR_BETTER_PRESSURE", ".false.")),
-        attenuation=attenuation, qkappa=rep(qk), qmu=rep(qm), n_sls=int(g("N_SLS", "3")))
+        attenuation=attenuation, qkappa=rep(qk), qmu=rep(qm), n_sls=int(g("N_SLS", "3")), pml=pml, **pml_kw)
     return cfg, stations
 
 

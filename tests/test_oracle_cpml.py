@@ -1,0 +1,178 @@
+"""CPML absorbing layers in the oracle (PML_BOUNDARY_CONDITIONS, Newmark, elastic and acoustic elements).
+
+What pins it:
+- Full-length traces of four of the reference's examples that absorb with a 3-element CPML on every side, each set
+  up from its own input files: ``semi_infinite_homogeneous``, ``check_absolute_amplitude_of_force_source_
+  seismograms_elastic``, ``infinite_homogeneous_moment_tensor_vertical_dip_slip`` (elastic) and
+  ``check_absolute_amplitude_of_pressure_source_seismograms_acoustic`` (acoustic).  The REF_SEIS files come from
+  earlier versions of the reference (their logs differ from the current code's), and the FP64 oracle is as far
+  from them as the FP32 one, so the fixtures carry about 1e-6 of their peak of their own.  Measured: force and
+  pressure examples within 2e-6 of the peak over the whole trace; the half-space and moment-tensor examples within
+  2e-6 up to the arrival of the waves reflected by the corner (XZ) elements, then within 1.4e-5 of the peak (about
+  1 % of those reflections, 1e-3 of the peak).  The bar is 2e-5 of the peak.
+- Each CPML term moves these traces by far more than the bar when it is planted wrong in the oracle (previous-step
+  field factor, Newmark damping term of the mass matrix, zero condition on the outer boundary, non-symmetric
+  stress, damping terms of the acceleration).
+- The corner (XZ) coefficients, which the fixtures pin least, by the rational identities they implement:
+  s_x/s_z and (i w)^2 s_x s_z with s = K + d/(alpha + i w), expanded in partial fractions
+  (pml_compute.f90:93-318); and the recursive convolution by its second-order convergence to the exact integral.
+- Element detection by the element counts in the reference's logs (564 and 492), profiles by their closed forms
+  (pml_init.F90:895-1012).
+"""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from oracle.oracle import OracleSim
+from specfem2d_amd import io
+
+EXAMPLES = [("semi_infinite_homogeneous", ("BXX", "BXZ")), ("force_source_elastic", ("BXX", "BXZ")),
+            ("moment_tensor", ("BXX", "BXZ")), ("pressure_source_acoustic", ("PRE",))]
+BAR = 2e-5
+
+
+def _cfg(golden_dir, example):
+    cfg, _ = io.config_from_par_file(os.path.join(golden_dir, "examples", example))
+    assert cfg.pml and not cfg.stacey and cfg.nelem_pml == 3
+    return cfg
+
+
+def _peak_err(seis, g, comps):
+    out = []
+    for c, ch in enumerate(comps):
+        ref = g[f"AA.S0001.{ch}"]
+        out.append(float(np.abs(seis[0, c] - ref).max() / np.abs(ref).max()))
+    return out
+
+
+@pytest.mark.parametrize("example,comps", EXAMPLES)
+def test_cpml_examples_full_length(golden_dir, example, comps):
+    cfg = _cfg(golden_dir, example)
+    o = OracleSim(cfg, "sp")
+    seis = o.forward()
+    g = np.load(f"{golden_dir}/{example}_ref_seis.npz")
+    errs = _peak_err(seis, g, comps)
+    print(f"{example}: max |oracle - REF_SEIS| / peak over {cfg.nstep} steps = " +
+          ", ".join(f"{c} {e:.2e}" for c, e in zip(comps, errs)))
+    assert max(errs) < BAR
+
+
+@pytest.mark.parametrize("mistake,example", [(1, "moment_tensor"), (2, "semi_infinite_homogeneous"),
+                                             (3, "semi_infinite_homogeneous"), (4, "semi_infinite_homogeneous"),
+                                             (5, "force_source_elastic")])
+def test_each_cpml_term_is_pinned_by_the_traces(golden_dir, mistake, example):
+    """A planted mistake in any CPML term moves an example's traces beyond the bar (measured: 7.2e-5 for the
+    previous-step factor, 2.6e-3 without the Newmark damping mass, 2.1e-3 without the outer zero condition)."""
+    cfg = _cfg(golden_dir, example)
+    g = np.load(f"{golden_dir}/{example}_ref_seis.npz")
+    O.set_pml_mistake(mistake)
+    try:
+        seis = OracleSim(cfg, "sp").forward()
+    finally:
+        O.set_pml_mistake(0)
+    errs = _peak_err(seis, g, ("BXX", "BXZ"))
+    print(f"mistake {mistake} on {example}: {errs[0]:.2e} {errs[1]:.2e}")
+    assert max(errs) > 3 * BAR
+
+
+@pytest.mark.parametrize("example,nspec_pml", [("semi_infinite_homogeneous", 564), ("force_source_elastic", 492),
+                                               ("moment_tensor", 564), ("pressure_source_acoustic", 492)])
+def test_element_detection_and_profiles(golden_dir, example, nspec_pml):
+    cfg = _cfg(golden_dir, example)
+    o = OracleSim(cfg, "dp")
+    p = o.pml()
+    assert p["is_pml"].sum() == nspec_pml  # "Total number of PML spectral elements" of the reference's log
+    nx, nz = cfg.nx, cfg.nz
+    ix, iz = np.arange(cfg.nspec) % nx, np.arange(cfg.nspec) // nx
+    side_x = (ix < 3) | (ix >= nx - 3)
+    side_z = (iz < 3) | (iz >= nz - 3)
+    np.testing.assert_array_equal(p["is_pml"], side_x | side_z)
+    np.testing.assert_array_equal(p["region"], np.where(side_x & side_z, 3, np.where(side_x, 1, np.where(side_z, 2, 0))))
+    # profiles of the x-only elements: d = d0 (a/L)^2 / damping factor, alpha = pi f0 (1 - a/L), K = 1, with
+    # d0 = 3 vp ln(1000) / (2 L), L the layer thickness and a the distance into the layer
+    X, Z = cfg.gll_point_coords()
+    L = 3 * (cfg.xmax - cfg.xmin) / nx
+    vp = float(np.asarray(cfg.vp).reshape(-1)[0])
+    acoustic = float(np.asarray(cfg.vs).reshape(-1)[0]) == 0.0
+    damp = cfg.pml_damping_acoustic if acoustic else cfg.pml_damping_elastic
+    f0 = max(s.f0 for s in cfg.sources)
+    pml_ids = np.flatnonzero(p["is_pml"])
+    xonly = p["region"][pml_ids] == 1
+    x = X[pml_ids[xonly]]
+    a = np.where(x < 0.5 * (cfg.xmin + cfg.xmax), cfg.xmin + L - x, x - (cfg.xmax - L))
+    a = np.maximum(a, 0.0) / L
+    d0 = 3.0 * vp * np.log(1000.0) / (2.0 * L)
+    np.testing.assert_allclose(p["dx"][xonly], d0 / damp * a ** 2, rtol=1e-9, atol=1e-9 * d0)
+    np.testing.assert_allclose(p["ax"][xonly], np.pi * f0 * (1.0 - a), rtol=1e-12, atol=1e-12)
+    assert np.all(p["kx"][xonly] == 1.0) and np.all(p["dz"][xonly] == 0.0) and np.all(p["az"][xonly] == 0.0)
+    # corner elements: alpha_x and alpha_z, and beta_z and alpha_x, kept apart (pml_init.F90:1269-1356)
+    xz = p["region"][pml_ids] == 3
+    bz = p["az"][xz] + p["dz"][xz] / p["kz"][xz]
+    assert np.all(np.abs(p["ax"][xz] - p["az"][xz]) > 0) and np.all(np.abs(bz - p["ax"][xz]) > 0)
+    # the outer boundary points: every point of the outer edges of the layer
+    pts = p["points_acoustic"] if acoustic else p["points_elastic"]
+    assert len(pts) == 4 * 4 * nx if nx == nz else True
+    coord = o.coord()
+    on_edge = ((np.abs(coord[:, 0] - cfg.xmin) < 1e-6) | (np.abs(coord[:, 0] - cfg.xmax) < 1e-6) |
+               (np.abs(coord[:, 1] - cfg.interfaces[0]) < 1e-6) | (np.abs(coord[:, 1] - cfg.interfaces[-1]) < 1e-6))
+    np.testing.assert_array_equal(np.sort(pts), np.flatnonzero(on_edge))
+
+
+def _random_point(rng, region):
+    kx, kz = (1.0 + rng.uniform(0, 2), 1.0 + rng.uniform(0, 2))
+    dx, dz = rng.uniform(50, 400), rng.uniform(50, 400)
+    ax, az = rng.uniform(5, 60), rng.uniform(5, 60)
+    if region == 1:
+        kz, dz, az = 1.0, 0.0, 0.0
+    if region == 2:
+        kx, dx, ax = 1.0, 0.0, 0.0
+    return kx, dx, ax, kz, dz, az
+
+
+@pytest.mark.parametrize("region", [1, 2, 3])
+def test_coefficients_are_partial_fractions_of_the_stretching(region):
+    """lik (stress) and l (acceleration) coefficients reproduce s_x/s_z, s_z/s_x and (iw)^2 s_x s_z exactly."""
+    rng = np.random.default_rng(7 + region)
+    w = np.array([0.3, 2.0, 17.0, 150.0, 900.0])
+    iw = 1j * w
+    for _ in range(20):
+        kx, dx, ax, kz, dz, az = _random_point(rng, region)
+        c = O.pml_coefficients(1e-3, kx, dx, ax, kz, dz, az, region)
+        bx, bz = ax + dx / kx, az + dz / kz
+        sx = kx + dx / (ax + iw)
+        sz = kz + dz / (az + iw)
+        # index 13 (z-derivative): s_x/s_z = A0 + A1/(iw + alpha_x) + A2/(iw + beta_z)
+        A = c["lik13"][0]
+        np.testing.assert_allclose(A[0] + A[1] / (iw + ax) + A[2] / (iw + bz), sx / sz, rtol=1e-11)
+        # index 31 (x-derivative): s_z/s_x = A0 + A1/(iw + alpha_z) + A2/(iw + beta_x)
+        A = c["lik31"][0]
+        np.testing.assert_allclose(A[0] + A[1] / (iw + az) + A[2] / (iw + bx), sz / sx, rtol=1e-11)
+        # acceleration: (iw)^2 s_x s_z = A0 (iw)^2 + A1 iw + A2 + A3/(iw + alpha_x) + A4/(iw + alpha_z)
+        A = c["l"][0]
+        lhs = iw ** 2 * sx * sz
+        rhs = A[0] * iw ** 2 + A[1] * iw + A[2] + A[3] / (iw + ax) + A[4] / (iw + az)
+        np.testing.assert_allclose(rhs, lhs, rtol=1e-10)
+
+
+def test_recursive_convolution_is_second_order():
+    """r_n = c0 r_{n-1} + c1 f_n + c2 f_{n-1} converges to int_0^t exp(-b (t - s)) f(s) ds at second order."""
+    b, T = 40.0, 0.5
+    f = lambda t: np.sin(30.0 * t) + t  # noqa: E731
+    # exact: int_0^T e^{-b(T-s)} (sin(30 s) + s) ds
+    exact = (b * np.sin(30 * T) - 30 * np.cos(30 * T) + 30 * np.exp(-b * T)) / (b * b + 900.0) + \
+        (T / b - (1.0 - np.exp(-b * T)) / (b * b))
+    errs = []
+    for n in (200, 400, 800):
+        dt = T / n
+        c = O.pml_coefficients(dt, 1.0, 0.0, b, 1.0, 0.0, 0.0, 1)["l"][1]  # coefficients of alpha_x = b
+        r, fo = 0.0, f(0.0)
+        for k in range(1, n + 1):
+            fn = f(k * dt)
+            r = c[0] * r + c[1] * fn + c[2] * fo
+            fo = fn
+        errs.append(abs(r - exact))
+    print("convolution errors", errs)
+    assert errs[0] / errs[1] > 3.5 and errs[1] / errs[2] > 3.5
+    assert errs[2] < 1e-4 * abs(exact)
