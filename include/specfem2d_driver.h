@@ -40,6 +40,10 @@ typedef struct {
                                           potential (receivers in fluid elements); shot batches, adjoint runs and
                                           gradients record displacement only */
   int use_trick;                       /* USE_TRICK_FOR_BETTER_PRESSURE */
+  int pml;                             /* PML_BOUNDARY_CONDITIONS: CPML layer on the absorbing sides (stacey = 0) */
+  int nelem_pml;                       /* NELEM_PML_THICKNESS */
+  double k_min_pml, k_max_pml;         /* K_MIN_PML, K_MAX_PML */
+  double damping_ac, damping_el;       /* damping_change_factor_acoustic, damping_change_factor_elastic */
 } sf2d_params;
 
 /* DATA/SOURCE subset: collocated force in a solid element, pressure source in a fluid element (vs = 0), or a
@@ -87,6 +91,11 @@ int sf2d_sim_nspec(void* sim);
 int sf2d_sim_nglob(void* sim);
 int sf2d_sim_nabs(void* sim);
 double sf2d_sim_t0(void* sim);
+/* CPML layer of the setup (pml_init.F90): counts[3] = {layer elements, outer-boundary points of elastic layer
+   elements, of acoustic ones}; region[nspec] (0, 1 X only, 2 Z only, 3 XZ), prof[6][NSPEC_PML][25] = K_x, K_z, d_x, d_z,
+   alpha_x, alpha_z (layer elements in element order), the outer-boundary points (1-based, increasing) */
+void sf2d_sim_pml_counts(void* sim, int* counts);
+void sf2d_sim_get_pml(void* sim, int* region, double* prof, int* pts_el, int* pts_ac);
 /* the opaque Fortran-style handle (value of Mesh_pointer) */
 long sf2d_sim_mesh_pointer(void* sim);
 

@@ -52,6 +52,33 @@ int sf2d_mesh_rect(long* Mesh_pointer);
    SF2D_SHOTS_PER_WAVE at set-up) */
 int sf2d_mesh_chain_elems(long* Mesh_pointer);
 void sf2d_mesh_set_shots_per_wave(long* Mesh_pointer, int shots);
+/* CPML layer in the terms of the reference's CPU path (pml_init.F90:34-1366, pml_compute*.f90), for elastic and
+   acoustic elements.  The reference's GPU set-up prepare_pml_device_ carries acoustic layers only (K = 1, one d0),
+   and its Fortran stops on elastic ones (prepare_gpu.f90:74).  Call after prepare_fields_elastic_device_ and
+   prepare_fields_acoustic_device_, before the first time step, on a handle prepared with PML_BOUNDARY_CONDITIONS
+   (forward runs; no attenuation, shot batches or gradients).
+     region_CPML[nspec]: 0 outside the layer, 1 CPML_X_ONLY, 2 CPML_Z_ONLY, 3 CPML_XZ;
+     spec_to_PML[nspec]: 1-based row of the element in the profile arrays (0 outside);
+     K_x_store .. alpha_z_store: double [NSPEC_PML][5][5] (i fastest), as pml_init.F90 leaves them (after the
+       separation of the corner parameters); d = 0, K = 1, alpha = 0 along an axis the region does not damp;
+     rhostore: [nspec][5][5] density (the damping terms of solid elements need it);
+     DT: the time step in double precision, as the reference's coefficient routines take it;
+     abs_points_elastic / _acoustic: 1-based points of the layer's outer boundary that belong to elastic / acoustic
+       layer elements (determine_boundary_abs_points_PML, pml_init.F90:306-421).
+   The mass matrices passed to prepare_fields_*_device_ carry the layer's terms (invert_mass_matrix.F90:133-345). */
+void prepare_cpml_device_(long* Mesh_pointer, int* NSPEC_PML, int* region_CPML, int* spec_to_PML, double* K_x_store,
+                          double* K_z_store, double* d_x_store, double* d_z_store, double* alpha_x_store,
+                          double* alpha_z_store, float* rhostore, double* DT, int* nglob_abs_elastic,
+                          int* abs_points_elastic, int* nglob_abs_acoustic, int* abs_points_acoustic);
+/* the elastic twin of pml_boundary_acoustic_cuda_ (pml_boundary_elastic, pml_compute.f90:565-605): displ, veloc
+   and accel are zero on the layer's outer boundary; the CPU path calls it after the Stacey step of iphase 1
+   (compute_forces_viscoelastic_calling_routine.F90:74-77) */
+void pml_boundary_elastic_cuda_(long* Mesh_pointer, int* compute_wavefield_1, int* compute_wavefield_2);
+/* zeroes the CPML memory variables and previous-step fields (a run restarted from rest) */
+void sf2d_pml_reset(long* Mesh_pointer);
+/* stats[5] = {solid layer elements, fluid layer elements, distinct coefficient sets, solid outer-boundary points,
+   fluid outer-boundary points} (zeros without a layer) */
+void sf2d_pml_stats(long* Mesh_pointer, int* stats);
 /* device pointer of a field: 0 displ 1 veloc 2 accel 3 b_displ 4 b_veloc 5 b_accel
    6 rho_kl 7 mu_kl 8 kappa_kl (NULL when not allocated).  On the device, point arrays are in the
    library's internal point numbering and kernels in its element order; the transfer entry points

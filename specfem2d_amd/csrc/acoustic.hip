@@ -428,7 +428,8 @@ AcousticState* ac_ready(Mesh* mp, const char* who) {
         geom[l] = make_float4(xix[l], xiz[l], gax[l], gaz[l]);
         // the jacobian from the inverse map, as the device solver forms it (no jacobian crosses the C-ABI)
         const float jac = 1.f / (xix[l] * gaz[l] - gax[l] * xiz[l]);
-        fac[l] = jac / rho[p];
+        // elements of a CPML layer add nothing here: their whole term comes from the layer's own kernel
+        fac[l] = pml_element(mp, e) ? 0.f : jac / rho[p];
         ids[l] = iv.id(e, p);
       }
     }
@@ -594,6 +595,23 @@ void launch_acoustic_vector_seismograms(Mesh* mp, float* dst, int cur, int seism
 // number of receivers in fluid elements
 int acoustic_receivers(Mesh* mp) { return mp->ac ? ac_ready(mp, "compute_seismograms_cuda")->nrec_fluid : 0; }
 
+int acoustic_phase_of(const Mesh* mp, int ispec) {
+  const AcousticState* ac = mp->ac;
+  if (!ac) fail("fluid element %d without prepare_fields_acoustic_device", ispec + 1);
+  if (ac->ready) fail("prepare_pml_device: the CPML layer must be prepared before the first acoustic step");
+  for (int ph = 0; ph < 2; ph++)
+    if (std::find(ac->phase_elems[ph].begin(), ac->phase_elems[ph].end(), ispec) != ac->phase_elems[ph].end())
+      return ph;
+  return -1;
+}
+
+void acoustic_rho_row(const Mesh* mp, int ispec, float* rho) {
+  const AcousticState* ac = mp->ac;
+  const int r = ac ? ac->row_of_ispec[ispec] : -1;
+  if (r < 0) fail("element %d is not a fluid element", ispec + 1);
+  std::memcpy(rho, &ac->rho_rows[(size_t)r * NGLL2], NGLL2 * sizeof(float));
+}
+
 }  // namespace sf2d
 
 using namespace sf2d;
@@ -696,6 +714,8 @@ void update_displacement_ac_cuda_(long* Mesh_pointer, realw* deltat_F, realw* de
   Mesh* mp = handle(Mesh_pointer);
   if (*compute_b_wavefield) fail("update_displacement_ac_cuda: the backward fluid wavefield is not supported");
   AcousticState* ac = ac_ready(mp, "update_displacement_ac_cuda");
+  // CPML: the previous-step potential of the layer's points, before the predictor
+  pml_acoustic_before_predictor(mp, ac->d_pot, ac->d_pdd, *deltatsqover2_F);
   const long m = (long)ac->nrange + ac->nlist;
   if (m == 0) return;
   hipLaunchKernelGGL(k_ac_predictor, dim3(grid_for(m, 256, 65536)), dim3(256), 0, mp->stream, ac->d_pot, ac->d_pd,
@@ -723,6 +743,17 @@ void compute_forces_acoustic_cuda_(long* Mesh_pointer, int* iphase, int* nspec_o
                        ac->d_ids, ac->d_geom, ac->d_geom_rect, ac->d_fac, mp->basis, ac->d_pot, ac->d_pdd);
     check_hip(hipGetLastError(), "k_acoustic_forces launch");
   }
+  pml_acoustic_forces(mp, ph, ac->d_pot, ac->d_pd, ac->d_pdd);  // the CPML layer's elements of this phase
+}
+
+// pml_compute_cuda.cu:47: potential, its time derivatives and the previous-step potential are zero on the outer
+// boundary of the layer's fluid elements
+void pml_boundary_acoustic_cuda_(long* Mesh_pointer, int* compute_wavefield_1, int* compute_wavefield_2) {
+  Mesh* mp = handle(Mesh_pointer);
+  require_forward(*compute_wavefield_2, 1, "pml_boundary_acoustic_cuda");
+  if (!*compute_wavefield_1) return;
+  AcousticState* ac = ac_ready(mp, "pml_boundary_acoustic_cuda");
+  pml_acoustic_boundary(mp, ac->d_pot, ac->d_pd, ac->d_pdd);
 }
 
 void compute_stacey_acoustic_cuda_(long* Mesh_pointer, int* iphasef, realw* h_b_absorb_potential_left,

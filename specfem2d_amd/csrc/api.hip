@@ -317,11 +317,12 @@ Plan build_plan(const std::vector<int>& ibool, int nspec, int nglob, const int* 
   return P;
 }
 
-// Applies the plan: schedule into the handle, per-element device arrays moved into the plan's order, ibool
-// rewritten in the plan's point numbering (patch mode).
+}  // namespace
+
 // Chain-kernel point ids with the first-touch flags, and the special points of first-touch assembly (points that
-// the Stacey edges and the sources also update: the predictor clears them and every force update adds to them).
-// Called when the plan is built and again when the Stacey edges arrive.
+// the Stacey edges, the sources, the fluid and the CPML layer's zero condition also update: the predictor clears
+// them and every force update adds to them).  Called when the plan is built and again when the Stacey edges or the
+// CPML layer arrive.
 void upload_chain_ids(Mesh* mp) {
   if (!mp->chain) return;
   const size_t ne = (size_t)mp->nspec * NGLL2;
@@ -333,6 +334,7 @@ void upload_chain_ids(Mesh* mp) {
   for (int g : mp->h_stacey_points) special[g] = 1;
   for (int g : mp->h_uncovered_points) special[g] = 1;
   for (int g : mp->h_coupling_points) special[mp->point_id.empty() ? g : mp->point_id[g]] = 1;
+  for (int g : mp->h_pml_points) special[g] = 1;
   std::vector<int> ids(mp->h_chain_ids);
   std::vector<int> list;
   if (mp->first_touch) {
@@ -348,6 +350,10 @@ void upload_chain_ids(Mesh* mp) {
   if (!list.empty()) mp->d_special = device_copy(list.data(), list.size());
 }
 
+namespace {
+
+// Applies the plan: schedule into the handle, per-element device arrays moved into the plan's order, ibool
+// rewritten in the plan's point numbering (patch mode).
 void sort_elements_by_colour(Mesh* mp, const int* phase_lists, int np) {
   const int nspec = mp->nspec;
   if (mp->sorted) fail("prepare_fields_elastic_device called twice");
@@ -570,7 +576,6 @@ void prepare_constants_device_(long* Mesh_pointer, int* h_NGLLX, int* NSPEC_AB, 
   if (*num_interfaces_ext_mesh > 0)
     fail("MPI interfaces (%d) passed: this library drives one GPU with one mesh partition", *num_interfaces_ext_mesh);
   if (!*P_SV) fail("SH (membrane) waves are not supported by this library; use P_SV = .true.");
-  if (*PML_BOUNDARY_CONDITIONS) fail("PML boundary conditions are not supported by this library yet");
 
   Mesh* mp = new Mesh();
   *Mesh_pointer = (long)mp;
@@ -848,6 +853,7 @@ void prepare_cleanup_device_(long* Mesh_pointer, int* ACOUSTIC_SIMULATION, int* 
   Mesh* mp = handle(Mesh_pointer);
   (void)hipStreamSynchronize(mp->stream);
   acoustic_release(mp);
+  pml_release(mp);
   device_free(mp->d_ibool); device_free(mp->d_xix); device_free(mp->d_xiz); device_free(mp->d_gammax);
   device_free(mp->d_gammaz); device_free(mp->d_kappav); device_free(mp->d_muv); device_free(mp->d_att_idx); device_free(mp->d_att_table); device_free(mp->d_att_e); device_free(mp->d_att_old); device_free(mp->d_ispec_is_elastic);
   device_free(mp->d_displ); device_free(mp->d_veloc); device_free(mp->d_accel);
@@ -1011,6 +1017,8 @@ void update_displacement_cuda_(long* Mesh_pointer, realw* deltat_F, realw* delta
                                realw* b_deltatover2_F) {  // update_displacement_cuda.cu:52
   Mesh* mp = handle(Mesh_pointer);
   const int keep = mp->first_touch;  // first-touch assembly: the force kernels define accel
+  // CPML: the previous-step field of the layer's points, from the fields before the predictor
+  pml_elastic_before_predictor(mp, *deltatsqover2_F);
   if (mp->pending_corrector[0])
     launch_newmark_corrector_predictor(mp->d_displ, mp->d_veloc, mp->d_accel, mp->d_rmassx, mp->d_rmassz, mp->nglob,
                                        mp->pending_dto2[0], *deltat_F, *deltatsqover2_F, *deltatover2_F, mp->stream,
@@ -1056,7 +1064,11 @@ void compute_forces_viscoelastic_cuda_(long* Mesh_pointer, int* iphase, realw* d
     fail("compute_forces: iphase %d has %d elements, the prepared phase list has %d", *iphase, nelem, mp->phase_count[ph]);
   const bool w1 = *compute_wavefield_1 != 0, w2 = *compute_wavefield_2 != 0;
   if (w2 && !mp->d_b_displ) fail("compute_forces: backward wavefield not allocated");
-  if (w1) apply_element_forces(mp, ph, mp->d_displ, mp->d_accel, 1, 0, first_touch);
+  if (w2 && mp->pml_state) fail("compute_forces: backward wavefields with CPML are not supported");
+  if (w1) {
+    apply_element_forces(mp, ph, mp->d_displ, mp->d_accel, 1, 0, first_touch);
+    pml_elastic_forces(mp, ph);  // the CPML layer's elements, after the main launches of the phase
+  }
   if (w2) apply_element_forces(mp, ph, mp->d_b_displ, mp->d_b_accel);
 }
 
@@ -1407,6 +1419,8 @@ extern "C" void sf2d_attenuation_reset(long* Mesh_pointer) {
   check_hip(hipMemsetAsync(mp->d_att_e, 0, 9 * ne * sizeof(float), mp->stream), "attenuation reset");
   check_hip(hipMemsetAsync(mp->d_att_old, 0, 3 * ne * sizeof(float), mp->stream), "attenuation reset");
 }
+
+extern "C" void sf2d_pml_reset(long* Mesh_pointer) { pml_reset(handle(Mesh_pointer)); }
 
 extern "C" int sf2d_mesh_affine_elements(long* Mesh_pointer) {
   Mesh* mp = handle(Mesh_pointer);

@@ -136,6 +136,13 @@ struct Sim {
   int nspec_side[5] = {0, 0, 0, 0, 0};
   std::vector<int> abs_ispec, abs_ij, edge_abs, ib_bottom, ib_left, ib_right, ib_top, ib0;
   std::vector<float> abs_normal, abs_jacw;
+  // CPML layer (pml_init.F90): region per element (0, 1 X only, 2 Z only, 3 XZ), row of each layer element in the
+  // profile arrays (1-based, 0 outside), profiles K, d, alpha along x and z [nspec_pml][25], the outer-boundary
+  // points of elastic and acoustic layer elements (1-based, increasing)
+  int nspec_pml = 0;
+  std::vector<int> region_cpml, spec_to_pml;
+  std::vector<double> pml_kx, pml_kz, pml_dx, pml_dz, pml_ax, pml_az;
+  std::vector<int> pml_pts_el, pml_pts_ac;
   // sources
   int nsrc = 0;
   std::vector<sf2d_source> src;
@@ -371,7 +378,7 @@ void build_domains(Sim& s) {
 void build_stacey(Sim& s) {
   const sf2d_params& p = s.p;
   s.nabs = 0;
-  if (!p.stacey) return;
+  if (!p.stacey && !p.pml) return;  // the CPML layer grows from the absorbing edges too
   std::vector<int> el, code;
   for (int iz = 1; iz <= p.nz; iz++)
     for (int ix = 1; ix <= p.nx; ix++) {
@@ -446,6 +453,207 @@ void build_stacey(Sim& s) {
   }
 }
 
+// CPML layer of the internal mesher (pml_init.F90:34-1366 with PML_PARAMETER_ADJUSTMENT and ROTATE_PML_ACTIVATE false)
+void build_pml(Sim& s) {
+  const int nspec = s.nspec, nglob = s.nglob;
+  s.nspec_pml = 0;
+  s.region_cpml.assign(nspec, 0);
+  s.spec_to_pml.assign(nspec, 0);
+  if (!s.p.pml || s.nabs == 0) return;  // nothing without absorbing edges (prepare_pml.f90:49)
+  // element detection (pml_init.F90:67-160), side by side (1 bottom, 2 right, 3 top, 4 left): the side's flag of
+  // every absorbing-edge element is that of its edge entry (a corner element listed again for another side keeps
+  // the flag of its later entry); then, NELEM_PML_THICKNESS - 1 times, every element with a corner point among the
+  // corners of the side's elements so far joins the side
+  std::vector<char> member((size_t)nspec * 5, 0), in_pml(nspec, 0), corner((size_t)nglob, 0);
+  const int cl[4] = {0, NGLL - 1, (NGLL - 1) * NGLL, NGLL2 - 1};
+  auto ib = [&](int e, int c) { return s.ibool[(size_t)e * NGLL2 + cl[c]] - 1; };
+  for (int side = 1; side <= 4; side++) {
+    std::fill(corner.begin(), corner.end(), 0);
+    for (int f = 0; f < s.nabs; f++) {
+      const int e = s.abs_ispec[f] - 1;
+      const bool on = s.edge_abs[f] == side;
+      member[(size_t)e * 5 + side] = on;
+      if (on)
+        for (int c = 0; c < 4; c++) corner[ib(e, c)] = 1;
+    }
+    for (int layer = 2; layer <= s.p.nelem_pml; layer++) {
+      for (int e = 0; e < nspec; e++) {
+        if (member[(size_t)e * 5 + side]) continue;
+        for (int c = 0; c < 4; c++)
+          if (corner[ib(e, c)]) member[(size_t)e * 5 + side] = 1;
+      }
+      std::fill(corner.begin(), corner.end(), 0);
+      for (int e = 0; e < nspec; e++) {
+        if (!member[(size_t)e * 5 + side]) continue;
+        in_pml[e] = 1;
+        for (int c = 0; c < 4; c++) corner[ib(e, c)] = 1;
+      }
+    }
+  }
+  // regions (pml_init.F90:201-248): one side alone, or two adjacent sides; anything else stays 0
+  for (int e = 0; e < nspec; e++) {
+    if (!in_pml[e]) continue;
+    const bool B = member[(size_t)e * 5 + 1], R = member[(size_t)e * 5 + 2], T = member[(size_t)e * 5 + 3],
+               L = member[(size_t)e * 5 + 4];
+    const int nx_sides = (int)L + (int)R, nz_sides = (int)T + (int)B;
+    int r = 0;
+    if (nx_sides == 1 && nz_sides == 0) r = 1;
+    else if (nx_sides == 0 && nz_sides == 1) r = 2;
+    else if (nx_sides == 1 && nz_sides == 1) r = 3;
+    s.region_cpml[e] = r;
+  }
+  for (int e = 0; e < nspec; e++)
+    if (in_pml[e]) s.spec_to_pml[e] = ++s.nspec_pml;
+  if (s.nspec_pml == 0) return;
+
+  // damping profiles (define_PML_coefficients, pml_init.F90:592-1012): thickness per side from the extent of the
+  // layer's points on that side of the mesh centre, d0 = -(N + 1) vp_max ln(R) / (2 L) per side and medium,
+  // d = d0 / damping_change_factor a^N, K = K_min + (K_max - 1) a^N, alpha = pi f0_max (1 - a), a the normalised
+  // distance into the layer (N = 2, R = 0.001)
+  const double NPOWER = 2.0, RCOEF = 0.001, PI = 3.141592653589793;
+  double xmin = 1e300, xmax = -1e300, zmin = 1e300, zmax = -1e300;
+  for (int g = 0; g < nglob; g++) {
+    xmin = std::min(xmin, s.coord[2 * g]);
+    xmax = std::max(xmax, s.coord[2 * g]);
+    zmin = std::min(zmin, s.coord[2 * g + 1]);
+    zmax = std::max(zmax, s.coord[2 * g + 1]);
+  }
+  const double xo = xmin + (xmax - xmin) / 2.0, zo = zmin + (zmax - zmin) / 2.0;
+  // [left, right, bottom, top] extents of the layer's points
+  double lo[4] = {1e30, 1e30, 1e30, 1e30}, hi[4] = {-1e30, -1e30, -1e30, -1e30};
+  for (int e = 0; e < nspec; e++) {
+    if (!in_pml[e]) continue;
+    const int r = s.region_cpml[e];
+    const bool zr = r == 2 || r == 3, xr = r == 1 || r == 3;
+    for (int p = 0; p < NGLL2; p++) {
+      const int g = s.ibool[(size_t)e * NGLL2 + p] - 1;
+      const double x = s.coord[2 * g], z = s.coord[2 * g + 1];
+      auto ext = [&](int k, double v) { lo[k] = std::min(lo[k], v); hi[k] = std::max(hi[k], v); };
+      if (z < zo && zr) ext(2, z);
+      if (x > xo && xr) ext(1, x);
+      if (z > zo && zr) ext(3, z);
+      if (x < xo && xr) ext(0, x);
+    }
+  }
+  double th[4];
+  for (int k = 0; k < 4; k++) {
+    th[k] = hi[k] - lo[k];
+    if (std::fabs(th[k]) > 1e30) th[k] = 0.0;
+  }
+  const double x_left = th[0] + xmin, x_right = xmax - th[1], z_bottom = th[2] + zmin, z_top = zmax - th[3];
+  double vpmax_ac = 0.0, vpmax_el = 0.0;
+  for (int e = 0; e < nspec; e++) {
+    if (!in_pml[e]) continue;
+    const bool fluid = s.nspec_ac > 0 && s.is_acoustic[e];
+    for (int p = 0; p < NGLL2; p++) {
+      const size_t l = (size_t)e * NGLL2 + p;
+      const double vp = (double)(s.rho_vp[l] / s.rho[l]);
+      (fluid ? vpmax_ac : vpmax_el) = std::max(fluid ? vpmax_ac : vpmax_el, vp);
+    }
+  }
+  double f0max = -1e300;
+  for (const sf2d_source& src : s.src) f0max = std::max(f0max, src.f0);
+  const double alpha_max = PI * f0max;
+  double d0[4][2];  // [side][fluid]
+  for (int k = 0; k < 4; k++)
+    for (int m = 0; m < 2; m++)
+      d0[k][m] = th[k] > 0.0 ? -(NPOWER + 1) * (m ? vpmax_ac : vpmax_el) * std::log(RCOEF) / (2.0 * th[k]) : 0.0;
+  const size_t np = (size_t)s.nspec_pml * NGLL2;
+  s.pml_kx.assign(np, 1.0); s.pml_kz.assign(np, 1.0);
+  s.pml_dx.assign(np, 0.0); s.pml_dz.assign(np, 0.0);
+  s.pml_ax.assign(np, 0.0); s.pml_az.assign(np, 0.0);
+  for (int e = 0; e < nspec; e++) {
+    if (!in_pml[e]) continue;
+    const int r = s.region_cpml[e];
+    const bool fluid = s.nspec_ac > 0 && s.is_acoustic[e];
+    const double damp = fluid ? s.p.damping_ac : s.p.damping_el;
+    for (int p = 0; p < NGLL2; p++) {
+      const int g = s.ibool[(size_t)e * NGLL2 + p] - 1;
+      const double x = s.coord[2 * g], z = s.coord[2 * g + 1];
+      double dx = 0.0, dz = 0.0, kx = 1.0, kz = 1.0, ax = 0.0, az = 0.0;
+      // the sides in the reference's order: bottom, top, right, left
+      const int order[4] = {2, 3, 1, 0};
+      for (int side : order) {
+        const bool zside = side >= 2;
+        const bool applies = side == 2 ? z < zo : side == 3 ? z > zo : side == 1 ? x > xo : x < xo;
+        if (!applies || (zside ? !(r == 2 || r == 3) : !(r == 1 || r == 3))) continue;
+        const double a = side == 2 ? z_bottom - z : side == 3 ? z - z_top : side == 1 ? x - x_right : x_left - x;
+        double dd = 0.0, kk = 1.0, aa = 0.0;
+        if (a >= 0.0) {
+          const double an = a / th[side];
+          dd = d0[side][fluid ? 1 : 0] / damp * (an * an);  // a^NPOWER with NPOWER = 2
+          kk = s.p.k_min_pml + (s.p.k_max_pml - 1.0) * (an * an);
+          aa = alpha_max * (1.0 - an);
+        }
+        if (zside) {
+          dz = dd; kz = kk; az = aa;
+          if (r == 2) { dx = 0.0; kx = 1.0; ax = 0.0; }
+        } else {
+          dx = dd; kx = kk; ax = aa;
+          if (r == 1) { dz = 0.0; kz = 1.0; az = 0.0; }
+        }
+      }
+      const size_t q = (size_t)(s.spec_to_pml[e] - 1) * NGLL2 + p;
+      s.pml_kx[q] = kx; s.pml_kz[q] = kz; s.pml_dx[q] = dx; s.pml_dz[q] = dz; s.pml_ax[q] = ax; s.pml_az[q] = az;
+    }
+  }
+  // corner elements keep alpha_x, alpha_z, beta_x, beta_z apart (pml_init.F90:1269-1356): the minimum distance
+  // between GLL points along an element's i and j directions sets the separation alpha_max / 8 dmin / max L
+  double dmin2 = 1e30;
+  for (int e = 0; e < nspec; e++) {
+    if (!in_pml[e]) continue;
+    for (int j = 0; j < NGLL - 1; j++)
+      for (int i = 0; i < NGLL - 1; i++) {
+        const int* b = s.ibool.data() + (size_t)e * NGLL2;
+        const int g1 = b[j * NGLL + i] - 1;
+        for (int g2 : {b[j * NGLL + i + 1] - 1, b[(j + 1) * NGLL + i] - 1}) {
+          const double ddx = s.coord[2 * g1] - s.coord[2 * g2], ddz = s.coord[2 * g1 + 1] - s.coord[2 * g2 + 1];
+          dmin2 = std::min(dmin2, ddx * ddx + ddz * ddz);
+        }
+      }
+  }
+  const double sep = alpha_max / 8.0 * std::sqrt(dmin2) / std::max(std::max(th[2], th[3]), std::max(th[0], th[1]));
+  const double sep2 = sep * 2.0;
+  for (int e = 0; e < nspec; e++) {
+    if (!in_pml[e] || s.region_cpml[e] != 3) continue;
+    for (int p = 0; p < NGLL2; p++) {
+      const size_t q = (size_t)(s.spec_to_pml[e] - 1) * NGLL2 + p;
+      const double kx = s.pml_kx[q], kz = s.pml_kz[q];
+      double ax = s.pml_ax[q], az = s.pml_az[q];
+      if (std::fabs(ax - az) < sep) {
+        if (ax > az) ax = az + sep2;
+        else az = ax + sep2;
+      }
+      double bx = ax + s.pml_dx[q] / kx;
+      if (std::fabs(bx - az) < sep) bx = az + sep2;
+      double bz = az + s.pml_dz[q] / kz;
+      if (std::fabs(bz - ax) < sep) bz = ax + sep2;
+      s.pml_dx[q] = (bx - ax) * kx;
+      s.pml_dz[q] = (bz - az) * kz;
+      s.pml_ax[q] = ax;
+      s.pml_az[q] = az;
+    }
+  }
+  // points of the absorbing edges of layer elements, by medium (determine_boundary_abs_points_PML,
+  // pml_init.F90:306-421)
+  std::vector<char> mel((size_t)nglob, 0), mac((size_t)nglob, 0);
+  for (int f = 0; f < s.nabs; f++) {
+    const int e = s.abs_ispec[f] - 1, c = s.edge_abs[f];
+    if (!in_pml[e]) continue;
+    const bool fluid = s.nspec_ac > 0 && s.is_acoustic[e];
+    for (int k = 0; k < NGLL; k++) {
+      const int i = c == 4 ? 0 : c == 2 ? NGLL - 1 : k, j = c == 1 ? 0 : c == 3 ? NGLL - 1 : k;
+      (fluid ? mac : mel)[s.ibool[(size_t)e * NGLL2 + j * NGLL + i] - 1] = 1;
+    }
+  }
+  s.pml_pts_el.clear();
+  s.pml_pts_ac.clear();
+  for (int g = 0; g < nglob; g++) {
+    if (mel[g]) s.pml_pts_el.push_back(g + 1);
+    if (mac[g]) s.pml_pts_ac.push_back(g + 1);
+  }
+}
+
 // mass matrix with the Newmark Stacey term (invert_mass_matrix.F90:34-464, 470-792, 798-848)
 void build_mass(Sim& s) {
   std::vector<float>& rm = s.rmass_inverse;
@@ -458,12 +666,36 @@ void build_mass(Sim& s) {
         const size_t l = (size_t)e * NGLL2 + j * NGLL + i;
         const int g = s.ibool[l] - 1;
         const float wxz = s.basis.w[i] * s.basis.w[j];
+        // CPML layer elements: w_i w_j J (rho or 1/kappa) times K_x, K_x K_z or K_z, then the Newmark damping term
+        // (d_x, d_x K_z + d_z K_x or d_z) dt/2 (invert_mass_matrix.F90:133-233, 262-345); a layer element of no
+        // region adds nothing
+        const int r = s.nspec_pml > 0 && s.spec_to_pml[e] > 0 ? s.region_cpml[e] : -1;
+        double kterm = 0.0, dterm = 0.0;
+        if (r > 0) {
+          const size_t q = (size_t)(s.spec_to_pml[e] - 1) * NGLL2 + j * NGLL + i;
+          const double kx = s.pml_kx[q], kz = s.pml_kz[q], dx = s.pml_dx[q], dz = s.pml_dz[q];
+          const double dto2 = (double)(float)(0.5 * (double)(float)s.p.dt);
+          kterm = r == 1 ? kx : r == 3 ? kx * kz : kz;
+          dterm = (r == 1 ? dx : r == 3 ? dx * kz + dz * kx : dz) * dto2;
+        }
         if (s.nspec_ac > 0 && s.is_acoustic[e]) {
-          // w_i w_j J in single precision over the double kappa (invert_mass_matrix.F90:424-427)
-          rma[g] = (float)((double)rma[g] + (double)(wxz * s.jacobian[l]) / (double)s.kappa[l]);
+          if (r < 0) {
+            // w_i w_j J in single precision over the double kappa (invert_mass_matrix.F90:424-427)
+            rma[g] = (float)((double)rma[g] + (double)(wxz * s.jacobian[l]) / (double)s.kappa[l]);
+          } else if (r > 0) {
+            const double base = (double)wxz / (double)s.kappa[l] * (double)s.jacobian[l];
+            rma[g] = (float)((double)rma[g] + base * kterm);
+            rma[g] = (float)((double)rma[g] + base * dterm);
+          }
           continue;
         }
-        rm[2 * g] = (float)((double)rm[2 * g] + (double)wxz * (double)s.rho[l] * (double)s.jacobian[l]);
+        if (r < 0) {
+          rm[2 * g] = (float)((double)rm[2 * g] + (double)wxz * (double)s.rho[l] * (double)s.jacobian[l]);
+        } else if (r > 0) {
+          const double base = (double)wxz * (double)s.rho[l] * (double)s.jacobian[l];
+          rm[2 * g] = (float)((double)rm[2 * g] + base * kterm);
+          rm[2 * g] = (float)((double)rm[2 * g] + base * dterm);
+        }
         rm[2 * g + 1] = rm[2 * g];
       }
   if (s.p.stacey) {
@@ -743,7 +975,7 @@ void prepare_device(Sim& s) {
   int myrank = 0, ndev = 0;
   initialize_cuda_device_(&myrank, &ndev);
   int ngllx = NGLL, nspec = s.nspec, nglob = s.nglob, zero = 0;
-  int stacey = s.p.stacey ? 1 : 0, pml = 0;
+  int stacey = s.p.stacey ? 1 : 0, pml = s.nspec_pml > 0 ? 1 : 0;
   std::vector<int> inner(s.nspec, 1), is_elastic(s.nspec, 1), is_acoustic(s.nspec, 0);
   for (int e = 0; e < s.nspec && s.nspec_ac > 0; e++) {
     is_acoustic[e] = s.is_acoustic[e];
@@ -808,6 +1040,17 @@ void prepare_device(Sim& s) {
                            &s.nspec_side[2], &s.nspec_side[3], s.abs_ispec.data(), s.abs_ij.data(),
                            s.abs_normal.data(), s.abs_jacw.data(), &nf, s.edge_abs.data(), s.ib_bottom.data(),
                            s.ib_left.data(), s.ib_right.data(), s.ib_top.data());
+  }
+  if (s.nspec_pml > 0) {
+    // the layer in the CPU path's terms (prepare_gpu.f90:215-226 hands the GPU the acoustic subset only)
+    int npml = s.nspec_pml, nel = (int)s.pml_pts_el.size(), nac_pts = (int)s.pml_pts_ac.size();
+    double dt = s.p.dt;
+    std::vector<int> pe = s.pml_pts_el, pa = s.pml_pts_ac;
+    if (pe.empty()) pe.push_back(1);
+    if (pa.empty()) pa.push_back(1);
+    prepare_cpml_device_(&s.mesh, &npml, s.region_cpml.data(), s.spec_to_pml.data(), s.pml_kx.data(), s.pml_kz.data(),
+                         s.pml_dx.data(), s.pml_dz.data(), s.pml_ax.data(), s.pml_az.data(), s.rho.data(), &dt, &nel,
+                         pe.data(), &nac_pts, pa.data());
   }
   std::vector<float> zeros((size_t)2 * s.nglob, 0.f);
   int size = 2 * s.nglob;
@@ -902,6 +1145,12 @@ void* create(const sf2d_params* p, const double* rho, const double* vp, const do
     fprintf(stderr, "sf2d_sim_create: shot batches and adjoint runs record displacement (seismotype 1)\n");
     return nullptr;
   }
+  if (p->pml && (p->stacey || nshot > 0 || simulation_type != 1 || save_forward || p->nelem_pml < 1 ||
+                 !(p->k_min_pml > 0.0) || !(p->damping_ac > 0.0) || !(p->damping_el > 0.0))) {
+    fprintf(stderr, "sf2d_sim_create: a CPML layer replaces Stacey edges, in forward runs without shot batches or "
+                    "SAVE_FORWARD, with NELEM_PML_THICKNESS >= 1, K_MIN_PML > 0 and positive damping factors\n");
+    return nullptr;
+  }
   Sim* s = new Sim();
   s->p = *p;
   s->simulation_type = simulation_type;
@@ -918,10 +1167,11 @@ void* create(const sf2d_params* p, const double* rho, const double* vp, const do
   s->model_rho.assign(rho, rho + ne);
   s->model_vp.assign(vp, vp + ne);
   s->model_vs.assign(vs, vs + ne);
-  build_stacey(*s);
-  build_mass(*s);
   s->nsrc = nsrc;
-  s->src.assign(src, src + nsrc);
+  s->src.assign(src, src + nsrc);  // the CPML profiles use the sources' highest f0
+  build_stacey(*s);
+  build_pml(*s);
+  build_mass(*s);
   s->nshot = nshot;
   if (nshot > 0) s->shot_nsrc.assign(shot_nsrc, shot_nsrc + nshot);
   build_sources(*s);
@@ -996,6 +1246,23 @@ void sf2d_sim_destroy(void* sim) {
   delete s;
 }
 
+void sf2d_sim_pml_counts(void* sim, int* counts) {
+  const Sim* s = (const Sim*)sim;
+  counts[0] = s->nspec_pml;
+  counts[1] = (int)s->pml_pts_el.size();
+  counts[2] = (int)s->pml_pts_ac.size();
+}
+
+void sf2d_sim_get_pml(void* sim, int* region, double* prof, int* pts_el, int* pts_ac) {
+  const Sim* s = (const Sim*)sim;
+  std::copy(s->region_cpml.begin(), s->region_cpml.end(), region);
+  const size_t np = (size_t)s->nspec_pml * NGLL2;
+  const std::vector<double>* src[6] = {&s->pml_kx, &s->pml_kz, &s->pml_dx, &s->pml_dz, &s->pml_ax, &s->pml_az};
+  for (int k = 0; k < 6 && np > 0; k++) std::copy(src[k]->begin(), src[k]->end(), prof + k * np);
+  std::copy(s->pml_pts_el.begin(), s->pml_pts_el.end(), pts_el);
+  std::copy(s->pml_pts_ac.begin(), s->pml_pts_ac.end(), pts_ac);
+}
+
 int sf2d_sim_nspec(void* sim) { return ((Sim*)sim)->nspec; }
 int sf2d_sim_nglob(void* sim) { return ((Sim*)sim)->nglob; }
 int sf2d_sim_nabs(void* sim) { return ((Sim*)sim)->nabs; }
@@ -1061,6 +1328,7 @@ int sf2d_sim_forward(void* sim, int nstep_run, double* seis, float* absorb, doub
   const int it_begin = s->it_done + 1;
   int i_sig = 1, nlength = s->p.nstep, it_end = s->it_done + nstep_run;
   const bool any_ac = s->nspec_ac > 0, any_el = s->nspec_ac < s->nspec;
+  const bool pml = s->nspec_pml > 0;
   int elastic_sim = any_el ? 1 : 0, acoustic_sim = any_ac ? 1 : 0, trick = s->p.use_trick ? 1 : 0;
   int nspec_inner_ac = s->nspec_ac, nfluid_solid = (int)s->coup_ac_ispec.size();
   nspec_inner = s->nspec - s->nspec_ac;
@@ -1105,6 +1373,7 @@ int sf2d_sim_forward(void* sim, int nstep_run, double* seis, float* absorb, doub
       acoustic_enforce_free_surf_cuda_(&s->mesh, &one, &zero);
       for (int iphase = 1; iphase <= 2; iphase++) {
         compute_forces_acoustic_cuda_(&s->mesh, &iphase, &zero, &nspec_inner_ac, &zero, &one, &zero);
+        if (pml) pml_boundary_acoustic_cuda_(&s->mesh, &one, &zero);
         if (iphase == 1) {
           if (s->p.stacey)
             compute_stacey_acoustic_cuda_(&s->mesh, &iphase, slice[4].data(), slice[2].data(), slice[3].data(),
@@ -1132,6 +1401,8 @@ int sf2d_sim_forward(void* sim, int nstep_run, double* seis, float* absorb, doub
         if (s->p.stacey)
           compute_stacey_viscoelastic_cuda_(&s->mesh, &iphase, slice[4].data(), slice[2].data(), slice[3].data(),
                                             slice[1].data(), &one, &zero, &zero);
+        // the CPML layer's zero condition (compute_forces_viscoelastic_calling_routine.F90:74-77)
+        if (pml) pml_boundary_elastic_cuda_(&s->mesh, &one, &zero);
         if (any_ac && nfluid_solid > 0) compute_coupling_el_ac_cuda_(&s->mesh, &iphase, &nfluid_solid, &one);
         compute_add_sources_el_cuda_(&s->mesh, &iphase, &it);
         mark(step, 5);
@@ -1333,7 +1604,7 @@ int sf2d_sim_adjoint(void* sim, const float* adj, const float* absorb, const flo
 
 int sf2d_sim_set_model(void* sim, const double* rho, const double* vp, const double* vs) {
   Sim* s = (Sim*)sim;
-  if (!s) return 1;
+  if (!s || s->nspec_pml > 0) return 1;  // the CPML layer's profiles and moduli are fixed at set-up
   const size_t ne = (size_t)s->nspec * NGLL2;
   for (size_t l = 0; l < ne; l++)
     if (!(rho[l] > 0.0) || !(vp[l] > 0.0) || !(vs[l] >= 0.0)) return 2;
@@ -1351,7 +1622,7 @@ int sf2d_sim_set_model(void* sim, const double* rho, const double* vp, const dou
 int sf2d_sim_gradient_forward(void* sim, const double* obs, int chunk, double* chi, double* seis) {
   Sim* s = (Sim*)sim;
   if (!s || !s->mesh || s->simulation_type != 1 || chunk < 1 || s->nspec_ac > 0 || s->p.seismotype != 1 ||
-      s->attenuation)
+      s->attenuation || s->nspec_pml > 0)
     return 1;
   const int N = s->p.nstep, R = s->nrec, S = std::max(1, s->nshot);
   s->grad_chunk = std::min(chunk, N);
@@ -1468,8 +1739,10 @@ int sf2d_sim_gradient_backward(void* sim, double* grad, double* parts) {
 int sf2d_sim_set_attenuation(void* sim, const double* qkappa, const double* qmu, int n_sls) {
   Sim* s = (Sim*)sim;
   if (!s || s->mesh || !qkappa || !qmu) return 1;
-  if (n_sls != 3 || s->nspec_ac > 0 || s->simulation_type != 1 || s->nshot > 0 || s->src.empty()) {
-    fprintf(stderr, "sf2d_sim_set_attenuation: 3 solids, forward runs of solid meshes with a source only\n");
+  if (n_sls != 3 || s->nspec_ac > 0 || s->simulation_type != 1 || s->nshot > 0 || s->src.empty() ||
+      s->nspec_pml > 0) {
+    fprintf(stderr, "sf2d_sim_set_attenuation: 3 solids, forward runs of solid meshes with a source and no CPML "
+                    "layer only\n");
     return 1;
   }
   const double f0 = s->src[0].f0;  // ATTENUATION_f0_REFERENCE (prepare_attenuation.f90:319-322)
@@ -1514,6 +1787,7 @@ void sf2d_sim_reset(void* sim) {
   Sim* s = (Sim*)sim;
   if (!s || !s->mesh) return;
   sf2d_attenuation_reset(&s->mesh);
+  sf2d_pml_reset(&s->mesh);
   std::vector<float> zeros((size_t)2 * s->nglob, 0.f);
   int size = 2 * s->nglob, nac = s->nglob;
   if (s->nspec_ac > 0) transfer_fields_ac_to_device_(&nac, zeros.data(), zeros.data(), zeros.data(), &s->mesh);

@@ -28,6 +28,7 @@ struct Basis {
 };
 
 struct AcousticState;  // fluid elements (acoustic.hip)
+struct PmlState;       // CPML layer (pml.hip)
 
 struct Mesh {
   int myrank = 0;
@@ -109,6 +110,7 @@ struct Mesh {
   std::vector<int> h_stacey_points;  // internal ids of the points of the absorbing edges
   std::vector<int> h_uncovered_points;  // internal ids of the points of no solid element (fluid-only points)
   std::vector<int> h_coupling_points;   // caller ids of the fluid-solid edge points (the fluid adds to accel there)
+  std::vector<int> h_pml_points;        // internal ids of the outer-boundary points of the solid's CPML layer
   int* d_special = nullptr;
   int nspecial = 0;
   int accel_partial = 0;
@@ -160,6 +162,9 @@ struct Mesh {
   // accel (or writes the fields) applies it first, so callers observe the reference's sequence.
   int pending_corrector[2] = {0, 0};  // forward/adjoint wavefield, backward wavefield
   float pending_dto2[2] = {0.f, 0.f};
+
+  // CPML layer (prepare_pml_device_ / prepare_cpml_device_)
+  PmlState* pml_state = nullptr;
 
   // scratch
   unsigned int* d_scratch_u32 = nullptr;  // reductions
@@ -219,6 +224,22 @@ void acoustic_note_stacey(Mesh* mp, int nf, const int* ispec, const int* ij, con
 void launch_acoustic_pressure_seismograms(Mesh* mp, float* dst, int cur, int use_trick, int potential);
 void launch_acoustic_vector_seismograms(Mesh* mp, float* dst, int cur, int seismotype);
 int acoustic_receivers(Mesh* mp);
+// phase (0 outer, 1 inner, -1 none) of fluid element ispec, and its density row ([25], caller's rhostore)
+int acoustic_phase_of(const Mesh* mp, int ispec);
+void acoustic_rho_row(const Mesh* mp, int ispec, float* rho);
+// CPML layer (pml.hip): whether element ispec (caller order, 0-based) is in it; release; zero its memory variables;
+// the previous-step fields (before the Newmark predictors); the layer's element forces of phase ph (after the main
+// force launches of that phase); the fluid's zero condition on the outer boundary
+bool pml_element(const Mesh* mp, int ispec);
+void pml_release(Mesh* mp);
+void pml_reset(Mesh* mp);
+void pml_elastic_before_predictor(Mesh* mp, float deltatsqover2);
+void pml_elastic_forces(Mesh* mp, int ph);
+void pml_acoustic_before_predictor(Mesh* mp, const float* pot, const float* pdd, float deltatsqover2);
+void pml_acoustic_forces(Mesh* mp, int ph, const float* pot, const float* pd, float* pdd);
+void pml_acoustic_boundary(Mesh* mp, float* pot, float* pd, float* pdd);
+// chain-kernel point ids with first-touch flags and the special points, uploaded again when the special points change
+void upload_chain_ids(Mesh* mp);
 // point arrays in the caller's numbering <-> the device numbering (n2 values per point)
 void points_to_device(const Mesh* mp, float* dev, const float* host, int n2, const char* what);
 void points_from_device(const Mesh* mp, float* host, const float* dev, int n2, const char* what);

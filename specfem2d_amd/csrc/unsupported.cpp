@@ -2,8 +2,8 @@
 //
 // They exist so that a SPECFEM2D binary built with GPU_MODE links against this library unchanged; each
 // terminates the run with a clear message (the reference's error convention) instead of silently doing
-// nothing.  Backward and adjoint fluid wavefields, acoustic kernels, PML, attenuation transfers, MPI halo
-// exchange, kernel smoothing, Hessians and moving sources fall in this group.
+// nothing.  Backward and adjoint fluid wavefields, acoustic kernels, transfers of attenuation memory variables,
+// MPI halo exchange, kernel smoothing, Hessians and moving sources fall in this group.
 #include "specfem2d_gpu.h"
 
 namespace sf2d {
@@ -27,8 +27,6 @@ void compute_add_moving_sources_ac_cuda_(long*, int*, int*, int*, int*, int*) { 
 void add_sources_ac_sim_2_or_3_cuda_(long*, int*, int*, int*, int*) { SF2D_UNSUPPORTED(add_sources_ac_sim_2_or_3_cuda, "acoustic adjoint sources"); }
 void compute_kernels_acoustic_cuda_(long*, realw*) { SF2D_UNSUPPORTED(compute_kernels_acoustic_cuda, "acoustic kernels"); }
 void compute_kernels_hess_cuda_(long*, int*, int*) { SF2D_UNSUPPORTED(compute_kernels_hess_cuda, "approximate Hessian kernels"); }
-void pml_boundary_acoustic_cuda_(long*, int*, int*) { SF2D_UNSUPPORTED(pml_boundary_acoustic_cuda, "PML"); }
-void prepare_pml_device_(long*, int*, int*, int*, int*, int*, realw*, realw*, realw*, realw*, realw*, realw*, realw*, realw*, int*, int*) { SF2D_UNSUPPORTED(prepare_pml_device, "PML"); }
 void prepare_moving_sources_cuda_(long*, int*, int*, realw*, int*, int*, realw*) { SF2D_UNSUPPORTED(prepare_moving_sources_cuda, "moving sources"); }
 void prepare_arrays_gpu_(long*, realw*, realw*, realw*, realw*, realw*, realw*, int*, int*, realw*) { SF2D_UNSUPPORTED(prepare_arrays_gpu, "kernel smoothing"); }
 void compute_smooth_(long*, realw*, realw*, realw*, realw*, const int*) { SF2D_UNSUPPORTED(compute_smooth, "kernel smoothing"); }

@@ -27,6 +27,8 @@ class Params(C.Structure):
         ("absorbbottom", C.c_int), ("absorbright", C.c_int), ("absorbtop", C.c_int), ("absorbleft", C.c_int),
         ("stacey", C.c_int), ("nstep", C.c_int), ("dt", C.c_double), ("user_t0", C.c_double),
         ("anglerec", C.c_double), ("material_mode", C.c_int), ("seismotype", C.c_int), ("use_trick", C.c_int),
+        ("pml", C.c_int), ("nelem_pml", C.c_int), ("k_min_pml", C.c_double), ("k_max_pml", C.c_double),
+        ("damping_ac", C.c_double), ("damping_el", C.c_double),
     ]
 
 
@@ -102,6 +104,9 @@ def load_libraries():
               "sf2d_sim_get_acoustic_fields", "sf2d_sim_set_acoustic_fields"):
         getattr(host, f).argtypes = [P, P, P, P]
     host.sf2d_sim_get_rmass_acoustic.argtypes = [P, P]
+    host.sf2d_sim_pml_counts.argtypes = [P, P]
+    host.sf2d_sim_get_pml.argtypes = [P, P, P, P, P]
+    gpu.sf2d_pml_stats.argtypes = [C.POINTER(C.c_long), P]
     host.sf2d_sim_get_coupling.argtypes = [P, P, P, P, P]
     gpu.sf2d_mesh_stream.argtypes = [C.POINTER(C.c_long)]
     gpu.sf2d_mesh_stream.restype = P
@@ -148,6 +153,9 @@ def params_from_config(cfg: SimulationConfig) -> Params:
     p.user_t0, p.anglerec = float(cfg.user_t0), float(cfg.anglerec)
     p.material_mode = int(cfg.material_mode)
     p.seismotype, p.use_trick = int(cfg.seismotype), int(cfg.use_trick)
+    p.pml, p.nelem_pml = int(cfg.pml), int(cfg.nelem_pml)
+    p.k_min_pml, p.k_max_pml = float(cfg.pml_k_min), float(cfg.pml_k_max)
+    p.damping_ac, p.damping_el = float(cfg.pml_damping_acoustic), float(cfg.pml_damping_elastic)
     return p
 
 
@@ -368,6 +376,32 @@ class Simulation:
             raise RuntimeError("sf2d_sim_adjoint failed")
         ker = ker.reshape(3, self.nspec, NGLL2)
         return seis[: self.cfg.nrec], {"rho": ker[0], "mu": ker[1], "kappa": ker[2]}
+
+    # CPML layer ----------------------------------------------------------
+    def pml(self):
+        """The driver's CPML set-up: dict with is_pml, region [nspec]; kx, kz, dx, dz, ax, az [nspec_pml, 25]
+        (double, layer elements in element order); the outer-boundary points of elastic and acoustic layer elements
+        (0-based, increasing)."""
+        counts = np.zeros(3, np.int32)
+        self.host.sf2d_sim_pml_counts(self.h, _ptr(counts))
+        npml, nel, nac = (int(c) for c in counts)
+        reg = np.zeros(self.nspec, np.int32)
+        prof = np.zeros(6 * max(1, npml) * NGLL2)
+        pe, pa = np.zeros(max(1, nel), np.int32), np.zeros(max(1, nac), np.int32)
+        self.host.sf2d_sim_get_pml(self.h, _ptr(reg), _ptr(prof), _ptr(pe), _ptr(pa))
+        prof = prof[: 6 * npml * NGLL2].reshape(6, npml, NGLL2)
+        out = {"region": reg, "points_elastic": pe[:nel] - 1, "points_acoustic": pa[:nac] - 1, "nspec_pml": npml}
+        out.update(zip(("kx", "kz", "dx", "dz", "ax", "az"), prof))
+        return out
+
+    def pml_stats(self):
+        """The GPU library's CPML layer: (solid layer elements, fluid layer elements, distinct coefficient sets,
+        solid outer-boundary points, fluid outer-boundary points)."""
+        self._need_device()
+        out = np.zeros(5, np.int32)
+        mp = C.c_long(self.mesh_pointer)
+        self.gpu.sf2d_pml_stats(C.byref(mp), _ptr(out))
+        return tuple(int(v) for v in out)
 
     # fluid elements ------------------------------------------------------
     def rmass_acoustic(self):
