@@ -77,6 +77,8 @@ def _lib(precision: str):
         lib.orc_apply_stiffness.argtypes = [P, dp, dp]
         lib.orc_real_size.restype = C.c_int
         lib.orc_set_ftz.argtypes = [P, C.c_int]
+        lib.orc_set_threads.restype = C.c_int
+        lib.orc_set_threads.argtypes = [C.c_int]
         lib.orc_adjoint.argtypes = [P, dp, dp, dp, dp, dp, C.c_void_p, dp, C.c_int, C.c_void_p, C.c_void_p]
         lib.orc_kernel_increment.argtypes = [P, dp, dp, dp, dp]
         lib.orc_gradient.argtypes = [P, dp, dp, dp]
@@ -104,6 +106,11 @@ def _lib(precision: str):
         lib.orc_pml_coefficients.argtypes = [C.c_double] * 7 + [C.c_int, dp]
         _LIBS[precision] = lib
     return _LIBS[precision]
+
+
+def set_threads(n: int, precision: str = "sp_omp") -> int:
+    """Threads of the OpenMP build (the CPU baseline); returns the count in effect (1 for serial builds)."""
+    return int(_lib(precision).orc_set_threads(int(n)))
 
 
 def pml_coefficients(dt, kx, dx, ax, kz, dz, az, region, precision="dp"):

@@ -19,6 +19,9 @@
  * each part (what pins it) is listed in the test suite's header
  * (tests/test_oracle_golden.py, tests/test_oracle_adjoint.py).
  */
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
@@ -140,6 +143,11 @@ typedef struct {
   REAL *rm_ac_dx, *rm_ac_dz, *rm_pot; /* [nspec_pml*25][2] */
   REAL *displ_old, *pot_old;  /* [nglob][2], [nglob] */
   double pml_alpha_max;       /* ALPHA_MAX_PML = pi f0max */
+  /* OpenMP build only (the CPU baseline): elements ordered by colour of a greedy colouring in which no two
+     elements of one colour share a point, so each colour's elements assemble in parallel without atomics */
+  int ncolour;
+  int *colour_start;          /* [ncolour + 1] */
+  int *colour_elems;          /* [nspec] */
 } orc_sim;
 
 /* ------------------------------------------------------------------ */
@@ -1248,17 +1256,17 @@ static void setup_receivers(orc_sim *s) {
 /* ------------------------------------------------------------------ */
 /* time step pieces                                                     */
 /* ------------------------------------------------------------------ */
-static void element_forces(orc_sim *s, const REAL *displ, REAL *accel) {
-  /* compute_forces_viscoelastic.F90:34-885, isotropic P-SV, no PML/AXISYM; viscoelastic attenuation in forward
-     runs (att_live) */
+static void element_force(orc_sim *s, int e, const REAL *displ, REAL *accel) {
+  /* compute_forces_viscoelastic.F90:34-885, isotropic P-SV, no AXISYM; CPML; viscoelastic attenuation in forward
+     runs (att_live) -- one element */
   const REAL (*H)[NGLL] = (const REAL(*)[NGLL])s->hprime;
   const REAL (*HW)[NGLL] = (const REAL(*)[NGLL])s->hprimewgll;
   REAL u[2][NGLL][NGLL]; /* u[c][j][i] */
   REAL t1x[NGLL][NGLL], t1z[NGLL][NGLL], t2x[NGLL][NGLL], t2z[NGLL][NGLL];
   REAL pml_acc[NGLL2][2];
   const double DT = s->p.dt;
-  for (int e = 0; e < s->nspec; e++) {
-    if (s->is_acoustic[e]) continue;
+  {
+    if (s->is_acoustic[e]) return;
     const int *ib = s->ibool + (size_t)e * NGLL2;
     const int pml = s->pml_on && s->is_pml[e];
     for (int j = 0; j < NGLL; j++)
@@ -1412,6 +1420,51 @@ static void element_forces(orc_sim *s, const REAL *displ, REAL *accel) {
   }
 }
 
+#ifdef _OPENMP
+/* greedy element colouring (OpenMP build): each element takes the lowest colour that no element sharing one of its
+   points has; a structured quad mesh gets the 2 x 2 checkerboard */
+static void setup_colours(orc_sim *s) {
+  if (s->colour_elems) return;
+  unsigned int *used = (unsigned int *)calloc((size_t)s->nglob, sizeof(unsigned int));
+  int *col = (int *)malloc((size_t)s->nspec * sizeof(int));
+  int ncol = 0;
+  for (int e = 0; e < s->nspec; e++) {
+    unsigned int m = 0;
+    for (int l = 0; l < NGLL2; l++) m |= used[s->ibool[(size_t)e * NGLL2 + l] - 1];
+    int c = 0;
+    while (c < 31 && (m >> c & 1u)) c++;
+    col[e] = c;
+    if (c + 1 > ncol) ncol = c + 1;
+    for (int l = 0; l < NGLL2; l++) used[s->ibool[(size_t)e * NGLL2 + l] - 1] |= 1u << c;
+  }
+  s->ncolour = ncol;
+  s->colour_start = (int *)calloc((size_t)ncol + 1, sizeof(int));
+  s->colour_elems = (int *)malloc((size_t)s->nspec * sizeof(int));
+  for (int e = 0; e < s->nspec; e++) s->colour_start[col[e] + 1]++;
+  for (int c = 0; c < ncol; c++) s->colour_start[c + 1] += s->colour_start[c];
+  int *fill = (int *)malloc((size_t)ncol * sizeof(int));
+  for (int c = 0; c < ncol; c++) fill[c] = s->colour_start[c];
+  for (int e = 0; e < s->nspec; e++) s->colour_elems[fill[col[e]]++] = e;
+  free(fill);
+  free(col);
+  free(used);
+}
+#endif
+
+/* the serial build visits the elements in order, as the reference's CPU loop does (parity); the OpenMP build
+   (CPU baseline) runs the elements of each colour in parallel */
+static void element_forces(orc_sim *s, const REAL *displ, REAL *accel) {
+#ifdef _OPENMP
+  setup_colours(s);
+  for (int c = 0; c < s->ncolour; c++) {
+#pragma omp parallel for schedule(static)
+    for (int k = s->colour_start[c]; k < s->colour_start[c + 1]; k++) element_force(s, s->colour_elems[k], displ, accel);
+  }
+#else
+  for (int e = 0; e < s->nspec; e++) element_force(s, e, displ, accel);
+#endif
+}
+
 static void stacey_forces(orc_sim *s, const REAL *veloc, REAL *accel, REAL *store /* may be NULL */, int it,
                           int nstep_store) {
   /* compute_stacey_elastic.f90:34-1001 (P-SV, no Bielak), SAVE_FORWARD storage */
@@ -1466,14 +1519,14 @@ static void add_sources(orc_sim *s, REAL *accel, int it /* 1-based */) {
 }
 
 /* acoustic element forces: compute_forces_viscoacoustic.f90:34-330 (no PML, attenuation or AXISYM) */
-static void acoustic_forces(orc_sim *s, const REAL *pot, REAL *pot_dd) {
+static void acoustic_force(orc_sim *s, int e, const REAL *pot, REAL *pot_dd) {
   const REAL (*H)[NGLL] = (const REAL(*)[NGLL])s->hprime;
   const REAL (*HW)[NGLL] = (const REAL(*)[NGLL])s->hprimewgll;
   REAL u[NGLL][NGLL], t1[NGLL][NGLL], t2[NGLL][NGLL]; /* [j][i] */
   REAL pml_acc[NGLL2];
   const double DT = s->p.dt;
-  for (int e = 0; e < s->nspec; e++) {
-    if (!s->is_acoustic[e]) continue;
+  {
+    if (!s->is_acoustic[e]) return;
     const int *ib = s->ibool + (size_t)e * NGLL2;
     const int pml = s->pml_on && s->is_pml[e];
     for (int j = 0; j < NGLL; j++)
@@ -1544,6 +1597,18 @@ static void acoustic_forces(orc_sim *s, const REAL *pot, REAL *pot_dd) {
     if (pml)
       for (int l = 0; l < NGLL2; l++) pot_dd[ib[l] - 1] = pot_dd[ib[l] - 1] - pml_acc[l];
   }
+}
+
+static void acoustic_forces(orc_sim *s, const REAL *pot, REAL *pot_dd) {
+#ifdef _OPENMP
+  setup_colours(s);
+  for (int c = 0; c < s->ncolour; c++) {
+#pragma omp parallel for schedule(static)
+    for (int k = s->colour_start[c]; k < s->colour_start[c + 1]; k++) acoustic_force(s, s->colour_elems[k], pot, pot_dd);
+  }
+#else
+  for (int e = 0; e < s->nspec; e++) acoustic_force(s, e, pot, pot_dd);
+#endif
 }
 
 /* compute_stacey_acoustic.f90:34-160 */
@@ -1771,6 +1836,7 @@ void orc_free(void *h) {
   free(s->pml_pts_el); free(s->pml_pts_ac);
   free(s->rm_dux_dx); free(s->rm_duz_dx); free(s->rm_dux_dz); free(s->rm_duz_dz); free(s->rm_displ);
   free(s->rm_ac_dx); free(s->rm_ac_dz); free(s->rm_pot); free(s->displ_old); free(s->pot_old);
+  free(s->colour_start); free(s->colour_elems);
   free(s);
 }
 
@@ -1976,19 +2042,23 @@ void orc_forward(void *h, int nstep_run, double *seis, double *stacey_store_d) {
     /* CPML keeps the previous step's fields, advanced by half the acceleration term
        (update_displacement_Newmark.F90:373-380, 428-433) */
     if (s->pml_on && s->any_acoustic)
+#pragma omp parallel for
       for (size_t k = 0; k < n1; k++)
         s->pot_old[k] = (REAL)((double)pa[k] + (double)deltatsqover2 / 2.0 * (double)pdd[k]);
     if (s->pml_on && s->any_elastic)
+#pragma omp parallel for
       for (size_t k = 0; k < n2; k++)
         s->displ_old[k] = (REAL)((double)d[k] + (double)deltatsqover2 / (g_pml_mistake == 1 ? 1.0 : 2.0) *
                                  (double)a[k]);
     if (s->any_acoustic)
+#pragma omp parallel for
       for (size_t k = 0; k < n1; k++) {
         pa[k] = pa[k] + deltat * pv[k] + deltatsqover2 * pdd[k];
         pv[k] = pv[k] + deltatover2 * pdd[k];
         pdd[k] = (REAL)0;
       }
     if (s->any_elastic)
+#pragma omp parallel for
       for (size_t k = 0; k < n2; k++) {
         d[k] = d[k] + deltat * v[k] + deltatsqover2 * a[k];
         v[k] = v[k] + deltatover2 * a[k];
@@ -2005,8 +2075,11 @@ void orc_forward(void *h, int nstep_run, double *seis, double *stacey_store_d) {
       add_sources_acoustic(s, pdd, it);
       acoustic_forces(s, pa, pdd);
       if (s->pml_on) pml_boundary_acoustic(s);
-      for (size_t k = 0; k < n1; k++) pdd[k] = pdd[k] * s->rmass_ac_inv[k];
-      for (size_t k = 0; k < n1; k++) pv[k] = pv[k] + deltatover2 * pdd[k];
+#pragma omp parallel for
+      for (size_t k = 0; k < n1; k++) {
+        pdd[k] = pdd[k] * s->rmass_ac_inv[k];
+        pv[k] = pv[k] + deltatover2 * pdd[k];
+      }
       enforce_free_surface(s);
     }
     /* compute_forces_viscoelastic_main: iphase 1 (Stacey, coupling with the new fluid pressure, sources),
@@ -2017,9 +2090,12 @@ void orc_forward(void *h, int nstep_run, double *seis, double *stacey_store_d) {
       if (s->ncoup > 0) coupling_elastic_ac(s, pdd, a);
       add_sources(s, a, it);
       element_forces(s, d, a);
-      for (size_t k = 0; k < n2; k++) a[k] = a[k] * s->rmass_inv[k];
-      /* update_veloc_elastic_Newmark */
-      for (size_t k = 0; k < n2; k++) v[k] = v[k] + deltatover2 * a[k];
+      /* mass, then update_veloc_elastic_Newmark */
+#pragma omp parallel for
+      for (size_t k = 0; k < n2; k++) {
+        a[k] = a[k] * s->rmass_inv[k];
+        v[k] = v[k] + deltatover2 * a[k];
+      }
     }
     if (seis && s->nrec > 0) record_seismograms(s, d, v, a, seis, step, nstep_run);
   }
@@ -2034,6 +2110,16 @@ void orc_forward(void *h, int nstep_run, double *seis, double *stacey_store_d) {
 }
 
 void orc_set_ftz(void *h, int on) { ((orc_sim *)h)->ftz = on; }
+/* threads of the OpenMP build (CPU baseline); returns the count in effect, 1 for the serial builds */
+int orc_set_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+  return omp_get_max_threads();
+#else
+  (void)n;
+  return 1;
+#endif
+}
 
 /* The CPML coefficient routines for one point: out[0:9] lik 31 (A, coef of bb_1, coef of bb_2), out[9:18] lik 13,
    out[18:28] l (A0..A4, coef of alpha_x, coef of alpha_z); inputs K, d, alpha along x and z. */

@@ -366,12 +366,13 @@ def fluid_solid_plane_wave(nstep: int = 1700, solid: bool = True, absorbtop: boo
 
 
 def c2_mesh_coupled(nstep: int = 3000, nx: int = 2000, nz: int = 1000, elem_size: float = 10.0,
-                    nrec: int = 200, seed: int = 2) -> SimulationConfig:
+                    nrec: int = 200, seed: int = 2, nelem_pml: int = 0) -> SimulationConfig:
     """The large coupled forward workload on one GPU: 2000 x 1000 elements of 10 m (20 km x 10 km, about 32 M
     GLL points), the top 1 km an acoustic water layer (vp 1500, rho 1020) under a free surface, the solid below a
     seeded Gaussian random field (correlation length 200 m, +-10 %) around vp 3000, vs 1732, rho 2500; a Ricker
-    force of 8 Hz at a third of the depth, ``nrec`` displacement receivers 20 m below the sea floor; Stacey
-    edges on the other three sides (the reference configuration's CPML is not part of this path), dt 2e-4."""
+    force of 8 Hz at a third of the depth, ``nrec`` displacement receivers 20 m below the sea floor, dt 2e-4.
+    The other three sides absorb: with a CPML layer ``nelem_pml`` elements thick (K 1, the Par_file's default
+    damping factors) when ``nelem_pml`` > 0, else with Stacey edges."""
     xmax, depth = nx * elem_size, nz * elem_size
     nzf = nz // 10
     sea_floor = -nzf * elem_size
@@ -380,8 +381,11 @@ def c2_mesh_coupled(nstep: int = 3000, nx: int = 2000, nz: int = 1000, elem_size
     base = SimulationConfig(name="c2", nx=nx, nz=nz, xmin=0.0, xmax=xmax, interfaces=[-depth, 0.0],
                             nz_layers=[nz], nstep=nstep, dt=2.0e-4)
     pert = gaussian_random_field(base, 200.0, 0.10, seed)
-    return layered_fluid_solid(nx, nz, nzf, elem_size, nstep, 2.0e-4, pert=pert,
-                               name=f"c2_mesh_coupled_{nx}x{nz}", sources=src, receivers=rec)
+    cfg = layered_fluid_solid(nx, nz, nzf, elem_size, nstep, 2.0e-4, pert=pert,
+                              name=f"c2_mesh_coupled_{nx}x{nz}", sources=src, receivers=rec)
+    if nelem_pml > 0:
+        cfg = cfg.replace(stacey=False, pml=True, nelem_pml=nelem_pml, name=f"c2_mesh_coupled_cpml_{nx}x{nz}")
+    return cfg
 
 
 def luoyang_fluid_solid(nstep: int = 1600) -> SimulationConfig:

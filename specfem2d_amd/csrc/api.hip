@@ -720,10 +720,21 @@ void prepare_fields_elastic_device_(long* Mesh_pointer, realw* rmassx, realw* rm
         std::array<float, 12> c;
         for (int k = 0; k < 3; k++)
           for (int w = 0; w < 4; w++) c[4 * k + w] = src[w][((size_t)e * NGLL2 + p) * 3 + k];
-        if (!table.empty() && c == last_c) {  // neighbouring points mostly share their set
+        if (!table.empty() && c == last_c) {  // neighbouring points mostly share their set (no NaN: NaN != NaN)
           idx[row + p] = last_id;
           continue;
         }
+        // A = exp(-dt / (2 tau_sigma)) lies in [0, 1] and B is finite (prepare_attenuation.f90:377-389); the
+        // reference leaves the arrays unset at points without attenuation (Qkappa, Qmu >= 9999, :347), so a
+        // caller must zero them there -- anything else would make the memory variables grow without bound
+        for (int k = 0; k < 3; k++)
+          for (int w = 0; w < 4; w++) {
+            const float v = c[4 * k + w];
+            if (!std::isfinite(v) || ((w == 0 || w == 2) && (v < 0.f || v > 1.f)))
+              fail("viscoelastic attenuation: invalid %s_newmark_%s coefficient %g (solid %d, point %d of element %d);"
+                   " A must lie in [0, 1] and B be finite (zero both where Qkappa and Qmu are 9999)",
+                   (w & 1) ? "B" : "A", w < 2 ? "kappa" : "mu", (double)v, k + 1, p + 1, e + 1);
+          }
         auto it = ids.find(c);
         if (it == ids.end()) {
           it = ids.emplace(c, (unsigned int)(table.size() / 3)).first;

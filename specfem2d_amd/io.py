@@ -286,7 +286,10 @@ def config_from_par_file(data_dir: str, cpml: str = "cpml", nstep: Optional[int]
         refuse(icode != 1, f"material {m} with icodemat {icode} (only isotropic tables)")
         sel = mat == m
         rho[sel], vp[sel], vs[sel] = vals[1], vals[2], vals[3]
-        if attenuation and len(vals) > 7:  # read_material_table.f90: QKappa, Qmu are the 7th and 8th values
+        if attenuation:  # read_material_table.f90: QKappa, Qmu are the 7th and 8th values
+            if len(vals) < 8:
+                raise ValueError(f"Par_file: material {m} has {len(vals)} values; ATTENUATION_VISCOELASTIC needs "
+                                 "QKappa and Qmu (7th and 8th)")
             qk[sel], qm[sel] = vals[6], vals[7]
     rep = lambda a: np.repeat(a, NGLL2)  # noqa: E731
     nsrc = int(g("NSOURCES"))

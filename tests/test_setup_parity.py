@@ -135,6 +135,15 @@ def test_attenuation_coefficients_match_oracle():
     qk = np.repeat(rng.uniform(10.0, 200.0, cfg.nspec), 25)
     qm = np.repeat(rng.uniform(8.0, 150.0, cfg.nspec), 25)
     qk[:50], qm[:50] = 9999.0, 9999.0
+    # the last elements: quality factors that vary from point to point inside the element, and mixed pairs (one
+    # of Qkappa, Qmu at 9999, the other finite: the reference still fits both, prepare_attenuation.f90:347)
+    tail = slice(n - 250, n)
+    qk[tail] = rng.uniform(12.0, 90.0, 250)
+    qm[tail] = rng.uniform(9.0, 70.0, 250)
+    mixed_k = n - 250 + np.arange(0, 250, 7)
+    mixed_m = n - 250 + np.arange(3, 250, 7)
+    qk[mixed_k] = 9999.0
+    qm[mixed_m] = 9999.0
     cfg = cfg.replace(attenuation=True, qkappa=qk, qmu=qm)
     sim = Simulation(cfg, device=False)
     mine = sim.attenuation_coefficients()
@@ -144,3 +153,8 @@ def test_attenuation_coefficients_match_oracle():
         b = b.astype(np.float32)
         assert np.all(np.abs(a - b) <= 2 * np.spacing(np.abs(b).astype(np.float32)))
     assert np.all(mine[0].reshape(n, 3)[:50] == 0)  # Q = 9999: no attenuation
+    # a mixed pair is fitted: non-zero coefficients for the finite factor (A_mu, B_mu where only Qkappa is 9999)
+    a_mu = mine[2].reshape(n, 3)
+    assert np.all(a_mu[mixed_k] != 0)
+    # per-point variation inside an element reaches the coefficients
+    assert len(np.unique(mine[3].reshape(n, 3)[tail][:25, 0])) == 22  # 4 of the 25 points have Qmu = 9999
