@@ -1,27 +1,26 @@
-"""Benchmark of the SPECFEM2D elastic time-stepping hot path on one MI355X.
+"""Benchmark of the SPECFEM2D time-stepping hot path on one MI355X.
 
-A step is one Newmark step of the elastic wavefield, driven through the reference's
-Fortran-to-C GPU entry points (predictor, element forces, Stacey, sources, corrector,
-seismograms).  The metric is elastic GLL-point updates per second: nglob x steps / time.
+A step is one Newmark step of the wavefields, driven through the reference's Fortran-to-C GPU entry points
+(predictors, element forces, CPML or Stacey edges, fluid-solid coupling, sources, correctors, seismograms).  The
+metric is GLL-point updates per second: nglob x steps / time-loop wall time.
 
-Default workload: the 2000 x 1000-element mesh of the large forward configuration
-(20 km x 10 km, 10 m elements, 32,012,001 GLL points) with a seeded Gaussian random-field
-elastic model (correlation length 200 m, +-10 % around vp 3000, vs 1732, rho 2500), Stacey
-on three sides and a free surface, Ricker 8 Hz source, 200 surface receivers, dt 2e-4 s
-(Courant number 0.38 over the closest GLL points at the model's highest vp).  The water
-layer, fluid-solid coupling and CPML of that configuration are not part of this path yet,
-so the whole mesh is elastic with Stacey edges.
+Default workload: the large fluid-solid forward configuration -- 2000 x 1000 elements of 10 m (20 km x 10 km,
+32,012,001 GLL points), the top 1 km an acoustic water layer (vp 1500, rho 1020) under a free surface, below it a
+seeded Gaussian random-field solid (correlation length 200 m, +-10 % around vp 3000, vs 1732, rho 2500), a CPML
+layer 20 elements thick on the left, right and bottom sides, a Ricker 8 Hz force source, 200 receivers below the
+sea floor, dt 2e-4 s.
 
-The same line carries the second headline figure, the FWI gradient time per shot: the exact
-discrete gradient of an L2 waveform misfit (forward run with a checkpoint every 100 steps,
-segment recomputation and reverse sweep, all on the GPU) on the single-shot 800 x 400-element,
-2000-step adjoint-gradient workload, observed data from a +5 % vs anomaly; and batched against
-plain on the multi-shot 600 x 300-element, 1500-step workload (shots along the free surface).
-
-Prints one JSON line with the throughput, the roofline of the element-force kernel (HIP
-events on the library stream around the force launches of every step inside the timed region;
-one launch per element colour, so a launch covers 1/ncolour of the elements) and a CPU
-baseline (the C oracle, one thread, on a bounded piece of the same workload).
+The same line carries:
+  * c2_elastic: the same mesh all elastic with Stacey edges, whose element-force launches give the roofline of the
+    dominant kernel (HIP events on the library stream around the force launches of every step inside the timed
+    region; one launch per colour, so a launch covers 1/ncolour of the elements);
+  * c2_attenuation: the elastic mesh with viscoelastic attenuation everywhere;
+  * the FWI gradient time per shot: the exact discrete gradient of an L2 waveform misfit (forward run with a
+    checkpoint every 100 steps, segment recomputation and reverse sweep, all on the GPU) on the single-shot
+    800 x 400-element, 2000-step adjoint-gradient workload; and batched against plain on the multi-shot
+    600 x 300-element, 1500-step workload (shots along the free surface);
+  * cpu_baseline: the C oracle's OpenMP build on the host cores, and on one core, over a bounded piece of the
+    default workload, with its calibration against the reference solver.
 """
 from __future__ import annotations
 
@@ -71,31 +70,65 @@ def make_config(name: str, nstep: int):
                                  vp0=3000.0, vs0=1732.0, rho0=2500.0, name=name)
 
 
-def cpu_baseline(name: str, target_seconds: float = 15.0):
-    """Oracle (C restatement of the reference CPU path, FP32, 1 thread) on a piece of the workload:
-    a 250 x 125-element mesh with the same element size, model statistics and boundaries."""
-    from oracle.oracle import OracleSim
+def cpu_piece_config(nstep: int):
+    """A 400 x 200-element piece of the default workload for the CPU baseline: the same element size, time step,
+    water-layer share (the top 20 rows), random-field solid and source frequency, with a CPML frame 4 elements thick
+    so that the layer holds the same share of the elements as the full mesh's 20-element frame (4 %)."""
     from specfem2d_amd import config
-    _, _, h, dt, _, f0 = WORKLOADS[name]
-    nx, nz = 250, 125
-    nstep = 2000
-    cfg = config.random_elastic(nx, nz, h, nstep, dt, 20, seed=7, f0=f0, name="cpu_sample")
-    orc = OracleSim(cfg, "sp")
-    orc.forward(nstep=2)  # warm caches
-    steps = 0
-    t0 = time.perf_counter()
-    chunk = 20
-    while time.perf_counter() - t0 < target_seconds and steps + chunk <= nstep - 2:
-        orc.forward(nstep=chunk)
-        steps += chunk
-    dt_s = time.perf_counter() - t0
+    return config.c2_mesh_coupled(nstep=nstep, nx=400, nz=200, nrec=20, seed=7, nelem_pml=4)
+
+
+def read_calibration():
+    """The committed calibration of the oracle's OpenMP build against the reference solver (oracle/calibrate.py)."""
+    path = os.path.join(ROOT, "profiles", "cpu_calibration_c1.md")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        text = f.read()
+    out = {}
+    for key in ("calibration_1core", "calibration_allcores_vs_mpi8"):
+        m = re.search(rf"^{key}:\s*([0-9.eE+-]+)", text, re.M)
+        if m:
+            out[key] = float(m.group(1))
+    return out or None
+
+
+def cpu_baseline(target_seconds: float = 10.0):
+    """The oracle (C restatement of the reference CPU path, FP32, gcc -O3 with OpenMP: the elements of each colour
+    in parallel) on a piece of the default workload, on all host cores the process may use and on one core."""
+    from oracle import oracle
+    from oracle.oracle import OracleSim
+    nstep = 4000
+    cfg = cpu_piece_config(nstep)
+    want = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+
+    def run(threads):
+        got = oracle.set_threads(threads)
+        orc = OracleSim(cfg, "sp_omp")
+        orc.forward(nstep=2)  # warm caches
+        steps, chunk = 0, 10
+        t0 = time.perf_counter()
+        while time.perf_counter() - t0 < target_seconds and steps + chunk <= nstep - 2:
+            orc.forward(nstep=chunk)
+            steps += chunk
+        dt_s = time.perf_counter() - t0
+        return got, steps, dt_s, orc.nglob * steps / dt_s, orc.nglob
+
+    threads, steps, dt_s, value, nglob = run(want)
+    _, steps1, dt1, value1, _ = run(1)
+    oracle.set_threads(want)
     return {
-        "value": orc.nglob * steps / dt_s,
+        "value": value,
         "unit": "GLL-pt updates/s",
-        "cores": 1,
+        "cores": threads,
+        "threads": threads,
+        "value_1core": value1,
         "kind": "port",
-        "sample": f"{nx}x{nz}-element piece of the workload ({orc.nglob} GLL points), {steps} Newmark steps, "
-                  f"FP32 oracle (oracle/sem2d_oracle.c, gcc -O2), {dt_s:.1f} s",
+        "sample": f"400x200-element piece of the default workload ({nglob} GLL points: water layer of 20 rows, "
+                  f"random-field solid, CPML 4 elements thick), FP32 oracle OpenMP build (oracle/sem2d_oracle.c, "
+                  f"gcc -O3 -fopenmp, elements of each colour in parallel): {steps} steps in {dt_s:.1f} s on "
+                  f"{threads} threads, {steps1} steps in {dt1:.1f} s on 1",
+        "calibration_vs_reference": read_calibration(),
     }
 
 
@@ -259,26 +292,81 @@ def c2_attenuation(steps: int = 50, warmup: int = 5):
     return out
 
 
+def c2_elastic(steps: int, warmup: int):
+    """The 2000 x 1000 mesh all elastic (the random-field model, Stacey bottom/left/right, free surface on top):
+    GLL-point updates per second, and the roofline of the element-force kernel from HIP events around its launches
+    (on the library's stream) in every timed step."""
+    import torch
+    from specfem2d_amd.driver import Simulation
+    name = "c2_mesh_elastic_2000x1000"
+    cfg = make_config(name, warmup + steps)
+    t_setup = time.perf_counter()
+    sim = Simulation(cfg)
+    t_setup = time.perf_counter() - t_setup
+    sim.forward(nstep=warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    seis, timing = sim.forward(nstep=steps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    nglob, nspec = sim.nglob, sim.nspec
+    # one force launch per colour; the events bracket all launches of a step
+    launches = sim.colours(2)
+    patch_mode = os.environ.get("SF2D_ASSEMBLY", "patch") == "patch"
+    n_affine = sim.affine_elements() if patch_mode else 0
+    chain = patch_mode and sim.chain_kernel()
+    fk_ms = timing["forces_ms"] / steps / launches
+    fk_bytes = force_kernel_bytes(nspec, nglob) / launches
+    achieved = fk_bytes / (fk_ms * 1e-3) / 1e9
+    roofline = {
+        "bound": "hbm",
+        "kernel": ("k_elastic_forces_chain" if chain else "k_elastic_forces_patch") if patch_mode
+                  else "k_elastic_forces",
+        "measured_on": name,
+        "launches_per_step": launches,
+        "achieved": achieved,
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": achieved / HBM_PEAK_GBS,
+        "traffic": read_traffic(name),
+        "algorithmic_bytes_per_launch": fk_bytes,
+        "avg_launch_ms": fk_ms,
+        # the bytes this kernel's own representation needs at minimum: affine elements carry one 16-byte
+        # geometry record instead of 25 x 16 bytes (what it reads, not the formula above)
+        "kernel_min_bytes_per_launch": kernel_min_bytes(nspec, nglob, n_affine, chain) / launches,
+    }
+    out = {"workload": name, "nspec": nspec, "nglob": nglob, "receivers": cfg.nrec, "dt": cfg.dt,
+           "boundaries": "Stacey bottom/left/right, free surface top", "steps": steps, "warmup": warmup,
+           "ms_per_step": wall * 1e3 / steps, "value": nglob * steps / wall, "unit": "GLL-pt updates/s",
+           "gpu_loop_ms_events": timing["loop_ms"], "setup_s": t_setup,
+           "assembly": os.environ.get("SF2D_ASSEMBLY", "patch"), "affine_elements": n_affine,
+           "finite": bool(np.all(np.isfinite(seis)))}
+    sim.close()
+    return out, roofline
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="c2_mesh_elastic_2000x1000", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="seconds of each CPU baseline leg")
     ap.add_argument("--no-gradient", action="store_true", help="skip the FWI gradient time per shot")
     ap.add_argument("--fwi-only", action="store_true",
                     help="only the batched-vs-plain FWI gradient (profiling); prints its JSON object")
     ap.add_argument("--fwi-steps", type=int, default=1500, help="time steps of the batched FWI workload")
     ap.add_argument("--batch-shots", type=int, default=32,
                     help="shots per batch of the multi-shot FWI gradient (32 fill about 170 GB of checkpoints)")
-    ap.add_argument("--no-coupled", action="store_true", help="skip the coupled fluid-solid C2 forward")
     ap.add_argument("--no-attenuation", action="store_true", help="skip the C2 forward with attenuation")
     ap.add_argument("--attenuation-only", action="store_true",
                     help="only the C2 forward with viscoelastic attenuation (profiling); prints its JSON object")
     ap.add_argument("--coupled-only", action="store_true",
-                    help="only the coupled fluid-solid C2 forward (profiling); prints its JSON object")
+                    help="only the coupled fluid-solid C2 forward with Stacey edges (profiling); prints its JSON")
+    ap.add_argument("--elastic-only", action="store_true",
+                    help="only the all-elastic C2 forward and its force-kernel roofline (profiling)")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="only the default workload (profiling); prints its JSON line without the other legs")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="after the timed region, run a plain copy of this size (counter calibration)")
     args = ap.parse_args()
@@ -292,6 +380,7 @@ def main():
     if not torch.cuda.is_available():
         print(json.dumps({"error": "no HIP device"}))
         sys.exit(1)
+    from specfem2d_amd import config
     from specfem2d_amd.driver import Simulation
 
     if args.fwi_only:
@@ -303,14 +392,20 @@ def main():
     if args.coupled_only:
         print(json.dumps(c2_coupled(min(args.steps, 100), min(args.warmup, 10))))
         return
-    cfg = make_config(args.workload, args.warmup + args.steps)
+    if args.elastic_only:
+        el, roof = c2_elastic(args.steps, args.warmup)
+        el["roofline"] = roof
+        print(json.dumps(el))
+        if args.copy_calibration_mib:
+            copy_calibration(args.copy_calibration_mib)
+        return
+
+    # the default workload: water layer, random-field solid, CPML 20 elements thick
+    cfg = config.c2_mesh_coupled(nstep=args.warmup + args.steps, nelem_pml=20)
     t_setup = time.perf_counter()
     sim = Simulation(cfg)
     t_setup = time.perf_counter() - t_setup
-    # warmup
     sim.forward(nstep=args.warmup)
-    torch.cuda.synchronize()
-    # timed region
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     seis, timing = sim.forward(nstep=args.steps)
@@ -319,20 +414,10 @@ def main():
     if not np.all(np.isfinite(seis)):
         print(json.dumps({"error": "non-finite seismograms"}))
         sys.exit(1)
-    nglob, nspec = sim.nglob, sim.nspec
-    value = nglob * args.steps / wall
-    # one force launch per colour; the events bracket all launches of a step
-    launches = sim.colours(2)
-    patch_mode = os.environ.get("SF2D_ASSEMBLY", "patch") == "patch"
-    n_affine = sim.affine_elements() if patch_mode else 0
-    chain = patch_mode and sim.chain_kernel()
-    fk_ms = timing["forces_ms"] / args.steps / launches
-    fk_bytes = force_kernel_bytes(nspec, nglob) / launches
-    achieved = fk_bytes / (fk_ms * 1e-3) / 1e9
-    traffic = read_traffic(args.workload)
+    pml = sim.pml_stats()
     out = {
         "metric": "elastic GLL-pt updates/s",
-        "value": value,
+        "value": sim.nglob * args.steps / wall,
         "unit": "GLL-pt updates/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -342,49 +427,38 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded Gaussian random-field elastic model)",
+        "data": "synthetic (seeded Gaussian random-field solid under a water layer)",
         "config": {
-            "workload": args.workload,
-            "nspec": nspec,
-            "nglob": nglob,
+            "workload": "c2_fluid_solid_cpml_2000x1000",
+            "nspec": sim.nspec,
+            "nspec_fluid": sim.nspec_acoustic,
+            "nglob": sim.nglob,
             "receivers": cfg.nrec,
             "dt": cfg.dt,
-            "boundaries": "Stacey bottom/left/right, free surface top",
+            "boundaries": "free surface on the water; CPML 20 elements thick on the left, right and bottom",
+            "cpml_elements": {"solid": pml[0], "fluid": pml[1]},
+            "fluid_solid_edges": sim.ncoupling,
             "parallelism": "1 GPU",
-            "assembly": os.environ.get("SF2D_ASSEMBLY", "patch"),
-            "affine_elements": n_affine,
-        },
-        "roofline": {
-            "bound": "hbm",
-            "kernel": ("k_elastic_forces_chain" if chain else "k_elastic_forces_patch") if patch_mode
-                      else "k_elastic_forces",
-            "launches_per_step": launches,
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic,
-            "algorithmic_bytes_per_launch": fk_bytes,
-            "avg_launch_ms": fk_ms,
-            # the bytes this kernel's own representation needs at minimum: affine elements carry one 16-byte
-            # geometry record instead of 25 x 16 bytes (what it reads, not the formula above)
-            "kernel_min_bytes_per_launch": kernel_min_bytes(nspec, nglob, n_affine, chain) / launches,
         },
         "gpu_loop_ms_events": timing["loop_ms"],
         "setup_s": t_setup,
     }
     sim.close()
+    if args.headline_only:
+        print(json.dumps(out))
+        return
+    el, roof = c2_elastic(args.steps, args.warmup)
+    out["roofline"] = roof
+    out["c2_elastic"] = el
     if args.copy_calibration_mib:
         copy_calibration(args.copy_calibration_mib)
-    if not args.no_coupled:
-        out["c2_coupled"] = c2_coupled()
     if not args.no_attenuation:
         out["c2_attenuation"] = c2_attenuation()
     if not args.no_gradient:
         out["fwi_gradient"] = fwi_gradient()
         out["fwi_gradient_batched"] = fwi_gradient_batched(args.batch_shots, nstep=args.fwi_steps)
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(args.workload, args.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
     print(json.dumps(out))
 
 

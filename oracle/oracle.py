@@ -57,6 +57,9 @@ def _lib(precision: str):
         ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
         lib.orc_new.restype = P
         lib.orc_new.argtypes = [C.POINTER(OrcParams), dp, dp, dp, C.c_int, C.POINTER(OrcSource), C.c_int, dp]
+        lib.orc_new_external.restype = P
+        lib.orc_new_external.argtypes = [C.POINTER(OrcParams), C.c_int, C.c_int, ip, dp, C.c_int, ip, ip, C.c_int,
+                                         ip, ip, dp, dp, dp, C.c_int, C.POINTER(OrcSource), C.c_int, dp]
         lib.orc_free.argtypes = [P]
         for f in ("orc_nspec", "orc_nglob", "orc_nabs"):
             getattr(lib, f).argtypes = [P]
@@ -165,7 +168,19 @@ class OracleSim:
             srcs[k].source_type, srcs[k].Mxx, srcs[k].Mzz, srcs[k].Mxz = s.source_type, s.Mxx, s.Mzz, s.Mxz
         rec = np.ascontiguousarray(np.asarray(cfg.receivers if cfg.nrec else np.zeros((1, 2)), dtype=np.float64))
         self._keep = (p, rho, vp, vs, srcs, rec)
-        self.h = self.lib.orc_new(C.byref(p), rho, vp, vs, len(cfg.sources), srcs, cfg.nrec, rec)
+        if cfg.mesh is None:
+            self.h = self.lib.orc_new(C.byref(p), rho, vp, vs, len(cfg.sources), srcs, cfg.nrec, rec)
+        else:
+            m = cfg.mesh
+            i32 = lambda a: np.ascontiguousarray(np.asarray(a).reshape(-1), dtype=np.int32)  # noqa: E731
+            arrs = (i32(m.knods), np.ascontiguousarray(m.coorg, np.float64).reshape(-1), i32(m.free_elem),
+                    i32(m.free_code), i32(m.abs_elem), i32(m.abs_code))
+            self._keep_mesh = arrs
+            self.h = self.lib.orc_new_external(C.byref(p), m.nspec, int(m.coorg.shape[0]), arrs[0], arrs[1],
+                                               len(m.free_elem), arrs[2], arrs[3], len(m.abs_elem), arrs[4], arrs[5],
+                                               rho, vp, vs, len(cfg.sources), srcs, cfg.nrec, rec)
+            if not self.h:
+                raise NotImplementedError("oracle: external meshes with a CPML layer")
         self.nspec = self.lib.orc_nspec(self.h)
         self.nglob = self.lib.orc_nglob(self.h)
         self.nabs = self.lib.orc_nabs(self.h)

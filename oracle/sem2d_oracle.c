@@ -76,6 +76,11 @@ typedef struct {
   orc_params p;
   int nspec, nglob, npgeo;
   int *knods;                 /* [nspec][4] 0-based control nodes */
+  /* external meshes (read_external_mesh): the sides of the files' free surface and absorbing boundary as
+     (0-based element, side code 1 bottom 2 right 3 top 4 left); internal meshes derive them from nx, nz */
+  int external;
+  int ext_nfree, ext_nabs;
+  int *ext_free_elem, *ext_free_code, *ext_abs_elem, *ext_abs_code;
   double *coorg;              /* [npgeo][2] */
   int *ibool;                 /* [nspec][25] 1-based (i fastest, then j) */
   double *coord;              /* [nglob][2] */
@@ -462,17 +467,26 @@ static void setup_geometry_materials(orc_sim *s, const double *rho, const double
 static void setup_absorbing(orc_sim *s) {
   const orc_params *p = &s->p;
   int n = 0;
-  if (p->absorb_bottom) n += p->nx;
-  if (p->absorb_top) n += p->nx;
-  if (p->absorb_left) n += p->nz;
-  if (p->absorb_right) n += p->nz;
+  if (s->external) {
+    n = s->ext_nabs;
+  } else {
+    if (p->absorb_bottom) n += p->nx;
+    if (p->absorb_top) n += p->nx;
+    if (p->absorb_left) n += p->nz;
+    if (p->absorb_right) n += p->nz;
+  }
   s->nabs = n;
   s->abs_ispec = (int *)calloc((size_t)(n + 1), sizeof(int));
   s->abs_code = (int *)calloc((size_t)(n + 1), sizeof(int));
   s->abs_corner = (int *)calloc((size_t)4 * (n + 1), sizeof(int));
   s->ib = (int *)calloc((size_t)(n + 1), sizeof(int));
   n = 0;
-  for (int iz = 1; iz <= p->nz; iz++)
+  /* an external mesh lists its absorbing sides in the file's order */
+  for (int a = 0; s->external && a < s->ext_nabs; a++) {
+    s->abs_ispec[a] = s->ext_abs_elem[a] + 1;
+    s->abs_code[a] = s->ext_abs_code[a];
+  }
+  for (int iz = 1; iz <= p->nz && !s->external; iz++)
     for (int ix = 1; ix <= p->nx; ix++) {
       int el = (iz - 1) * p->nx + ix;
       if (p->absorb_bottom && iz == 1) { s->abs_ispec[n] = el; s->abs_code[n] = 1; n++; }
@@ -498,6 +512,13 @@ static void setup_absorbing(orc_sim *s) {
 /* (src/meshfem2D/determine_acoustic_surface.f90:34-160,              */
 /*  repartition_coupling.f90:38-106, src/specfem2D/get_coupling_edges.f90) */
 /* ------------------------------------------------------------------ */
+static const long *g_side_key;
+static int cmp_side_key(const void *a, const void *b) {
+  const long ka = g_side_key[*(const int *)a], kb = g_side_key[*(const int *)b];
+  if (ka != kb) return (ka > kb) - (ka < kb);
+  return (*(const int *)a > *(const int *)b) - (*(const int *)a < *(const int *)b);
+}
+
 static void setup_domains(orc_sim *s) {
   const orc_params *p = &s->p;
   const int nx = p->nx, nz = p->nz;
@@ -513,25 +534,31 @@ static void setup_domains(orc_sim *s) {
   s->any_elastic = s->nspec_acoustic < s->nspec;
   /* free surface: the non-absorbing outer sides of acoustic elements, in the mesher's order (top, bottom, left,
      right) */
-  s->free_ispec = (int *)calloc((size_t)2 * (nx + nz) + 1, sizeof(int));
-  s->free_code = (int *)calloc((size_t)2 * (nx + nz) + 1, sizeof(int));
+  const size_t nfree_max = s->external ? (size_t)s->ext_nfree : (size_t)2 * (nx + nz);
+  s->free_ispec = (int *)calloc(nfree_max + 1, sizeof(int));
+  s->free_code = (int *)calloc(nfree_max + 1, sizeof(int));
   s->nfree = 0;
-  if (!p->absorb_top)
+  /* an external mesh's free-surface file: the sides of fluid elements count (read_external_mesh_files.F90:398-416) */
+  for (int k = 0; s->external && k < s->ext_nfree; k++) {
+    const int e = s->ext_free_elem[k];
+    if (s->is_acoustic[e]) { s->free_ispec[s->nfree] = e; s->free_code[s->nfree++] = s->ext_free_code[k]; }
+  }
+  if (!s->external && !p->absorb_top)
     for (int i = 0; i < nx; i++) {
       int e = (nz - 1) * nx + i;
       if (s->is_acoustic[e]) { s->free_ispec[s->nfree] = e; s->free_code[s->nfree++] = 3; }
     }
-  if (!p->absorb_bottom)
+  if (!s->external && !p->absorb_bottom)
     for (int i = 0; i < nx; i++) {
       int e = i;
       if (s->is_acoustic[e]) { s->free_ispec[s->nfree] = e; s->free_code[s->nfree++] = 1; }
     }
-  if (!p->absorb_left)
+  if (!s->external && !p->absorb_left)
     for (int j = 0; j < nz; j++) {
       int e = j * nx;
       if (s->is_acoustic[e]) { s->free_ispec[s->nfree] = e; s->free_code[s->nfree++] = 4; }
     }
-  if (!p->absorb_right)
+  if (!s->external && !p->absorb_right)
     for (int j = 0; j < nz; j++) {
       int e = j * nx + nx - 1;
       if (s->is_acoustic[e]) { s->free_ispec[s->nfree] = e; s->free_code[s->nfree++] = 2; }
@@ -543,7 +570,43 @@ static void setup_domains(orc_sim *s) {
   s->coup_el_ispec = (int *)calloc((size_t)4 * s->nspec + 1, sizeof(int));
   s->coup_el_code = (int *)calloc((size_t)4 * s->nspec + 1, sizeof(int));
   s->ncoup = 0;
-  if (s->any_acoustic && s->any_elastic)
+  if (s->external && s->any_acoustic && s->any_elastic) {
+    /* the elastic element sharing each side of an acoustic element: sides keyed by their sorted node pair */
+    const int ns = 4 * s->nspec;
+    long *key = (long *)malloc(sizeof(long) * (size_t)ns);
+    int *ord = (int *)malloc(sizeof(int) * (size_t)ns);
+    for (int e = 0; e < s->nspec; e++)
+      for (int c = 1; c <= 4; c++) {
+        const int a = s->knods[4 * e + c - 1], b = s->knods[4 * e + c % 4];
+        key[4 * e + c - 1] = (long)(a < b ? a : b) * (long)s->npgeo + (a < b ? b : a);
+        ord[4 * e + c - 1] = 4 * e + c - 1;
+      }
+    g_side_key = key;
+    qsort(ord, (size_t)ns, sizeof(int), cmp_side_key);
+    for (int e = 0; e < s->nspec; e++) {
+      if (!s->is_acoustic[e]) continue;
+      for (int c = 1; c <= 4; c++) {
+        const long k = key[4 * e + c - 1];
+        int lo = 0, hi = ns;  /* first sorted side with this key */
+        while (lo < hi) {
+          const int mid = (lo + hi) / 2;
+          if (key[ord[mid]] < k) lo = mid + 1; else hi = mid;
+        }
+        for (int q = lo; q < ns && key[ord[q]] == k; q++) {
+          const int f = ord[q] / 4;
+          if (f == e || s->is_acoustic[f]) continue;
+          s->coup_ac_ispec[s->ncoup] = e;
+          s->coup_ac_code[s->ncoup] = c;
+          s->coup_el_ispec[s->ncoup] = f;
+          s->coup_el_code[s->ncoup] = ord[q] % 4 + 1;
+          s->ncoup++;
+        }
+      }
+    }
+    free(ord);
+    free(key);
+  }
+  if (!s->external && s->any_acoustic && s->any_elastic)
     for (int e = 0; e < s->nspec; e++) {
       if (!s->is_acoustic[e]) continue;
       const int ix = e % nx, iz = e / nx;
@@ -1783,12 +1846,51 @@ static void record_seismograms(orc_sim *s, const REAL *d, const REAL *v, const R
 /* ------------------------------------------------------------------ */
 /* public API                                                           */
 /* ------------------------------------------------------------------ */
+static orc_sim *orc_build(orc_sim *s, const orc_params *p, const double *rho, const double *vp, const double *vs,
+                          int nsrc, const orc_source *src, int nrec, const double *rec_xz);
+
 void *orc_new(const orc_params *p, const double *rho, const double *vp, const double *vs, int nsrc,
               const orc_source *src, int nrec, const double *rec_xz) {
   orc_sim *s = (orc_sim *)calloc(1, sizeof(orc_sim));
   s->p = *p;
   setup_gll(s);
   mesh_internal(s);
+  return orc_build(s, p, rho, vp, vs, nsrc, src, nrec, rec_xz);
+}
+
+/* an external mesh (read_external_mesh): nspec 4-node elements (knods 0-based, counter-clockwise), npgeo control
+   nodes, the free-surface and absorbing sides as (0-based element, side code); nx, nz and the absorb flags of the
+   parameters are unused, and a CPML layer is not set up */
+void *orc_new_external(const orc_params *p, int nspec, int npgeo, const int *knods, const double *coorg, int nfree,
+                       const int *free_elem, const int *free_code, int nabs, const int *abs_elem, const int *abs_code,
+                       const double *rho, const double *vp, const double *vs, int nsrc, const orc_source *src,
+                       int nrec, const double *rec_xz) {
+  if (p->pml) return NULL;
+  orc_sim *s = (orc_sim *)calloc(1, sizeof(orc_sim));
+  s->p = *p;
+  setup_gll(s);
+  s->external = 1;
+  s->nspec = nspec;
+  s->npgeo = npgeo;
+  s->knods = (int *)malloc(sizeof(int) * 4 * (size_t)nspec);
+  memcpy(s->knods, knods, sizeof(int) * 4 * (size_t)nspec);
+  s->coorg = (double *)malloc(sizeof(double) * 2 * (size_t)npgeo);
+  memcpy(s->coorg, coorg, sizeof(double) * 2 * (size_t)npgeo);
+  s->ext_nfree = nfree;
+  s->ext_nabs = nabs;
+  s->ext_free_elem = (int *)calloc((size_t)nfree + 1, sizeof(int));
+  s->ext_free_code = (int *)calloc((size_t)nfree + 1, sizeof(int));
+  s->ext_abs_elem = (int *)calloc((size_t)nabs + 1, sizeof(int));
+  s->ext_abs_code = (int *)calloc((size_t)nabs + 1, sizeof(int));
+  if (nfree) memcpy(s->ext_free_elem, free_elem, sizeof(int) * (size_t)nfree);
+  if (nfree) memcpy(s->ext_free_code, free_code, sizeof(int) * (size_t)nfree);
+  if (nabs) memcpy(s->ext_abs_elem, abs_elem, sizeof(int) * (size_t)nabs);
+  if (nabs) memcpy(s->ext_abs_code, abs_code, sizeof(int) * (size_t)nabs);
+  return orc_build(s, p, rho, vp, vs, nsrc, src, nrec, rec_xz);
+}
+
+static orc_sim *orc_build(orc_sim *s, const orc_params *p, const double *rho, const double *vp, const double *vs,
+                          int nsrc, const orc_source *src, int nrec, const double *rec_xz) {
   setup_numbering(s);
   setup_geometry_materials(s, rho, vp, vs);
   if (p->stacey || p->pml) setup_absorbing(s);
@@ -1818,6 +1920,7 @@ void orc_free(void *h) {
   orc_sim *s = (orc_sim *)h;
   if (!s) return;
   free(s->knods); free(s->coorg); free(s->ibool); free(s->coord);
+  free(s->ext_free_elem); free(s->ext_free_code); free(s->ext_abs_elem); free(s->ext_abs_code);
   free(s->xix); free(s->xiz); free(s->gammax); free(s->gammaz); free(s->jacobian);
   free(s->rhostore); free(s->mustore); free(s->kappastore); free(s->rho_vp); free(s->rho_vs);
   free(s->rmass_inv);

@@ -41,6 +41,46 @@ class Source:
     Mxz: float = 0.0
 
 
+# corner nodes of each side of a 4-node element (1 bottom, 2 right, 3 top, 4 left), counter-clockwise nodes
+SIDE_NODES = {1: (0, 1), 2: (1, 2), 3: (2, 3), 4: (3, 0)}
+
+
+@dataclass
+class ExternalMesh:
+    """A mesh read from the files of ``read_external_mesh`` (src/meshfem2D/read_external_mesh_files.F90): 4-node
+    elements of any shape, their material numbers, and the element sides on the free surface and on the absorbing
+    boundary (each as element and side code, 1 bottom, 2 right, 3 top, 4 left, found from the side's two nodes)."""
+
+    knods: np.ndarray         # [nspec, 4] 0-based control nodes, counter-clockwise from the first
+    coorg: np.ndarray         # [npgeo, 2] control-node coordinates (x, z)
+    material: np.ndarray      # [nspec] 1-based material numbers
+    free_elem: np.ndarray     # [nfree] 0-based elements of the free-surface sides
+    free_code: np.ndarray     # [nfree] their side codes
+    abs_elem: np.ndarray      # [nabs] 0-based elements of the absorbing sides
+    abs_code: np.ndarray      # [nabs] their side codes
+
+    @property
+    def nspec(self) -> int:
+        return int(self.knods.shape[0])
+
+    def side_code(self, e: int, n1: int, n2: int) -> int:
+        nodes = [int(x) for x in self.knods[e]]
+        for code, (a, b) in SIDE_NODES.items():
+            if {nodes[a], nodes[b]} == {int(n1), int(n2)}:
+                return code
+        raise ValueError(f"nodes {n1 + 1}, {n2 + 1} are not a side of element {e + 1}")
+
+    def gll_point_coords(self):
+        """(x, z) of every element GLL point under the bilinear map, [nspec, 25] each."""
+        xi = np.tile(XIGLL, NGLL)
+        ga = np.repeat(XIGLL, NGLL)
+        sh = np.stack([0.25 * (xi - 1) * (ga - 1), -0.25 * (xi + 1) * (ga - 1), 0.25 * (xi + 1) * (ga + 1),
+                       -0.25 * (xi - 1) * (ga + 1)])  # [4, 25]
+        cx = self.coorg[self.knods, 0]  # [nspec, 4]
+        cz = self.coorg[self.knods, 1]
+        return cx @ sh, cz @ sh
+
+
 @dataclass
 class SimulationConfig:
     nx: int
@@ -81,14 +121,19 @@ class SimulationConfig:
     pml_damping_acoustic: float = 0.5
     pml_damping_elastic: float = 1.0
     name: str = "custom"
+    # read_external_mesh: the mesh comes from files (nx, nz, xmin, xmax, interfaces and the absorb* flags of the
+    # internal mesher are then unused; Stacey edges are the mesh's absorbing sides)
+    mesh: Optional[ExternalMesh] = None
 
     # ------------------------------------------------------------------
     @property
     def nspec(self) -> int:
-        return self.nx * self.nz
+        return self.mesh.nspec if self.mesh is not None else self.nx * self.nz
 
     @property
     def nglob(self) -> int:
+        if self.mesh is not None:
+            raise ValueError("an external mesh's point count comes from its numbering (Simulation.nglob)")
         return (4 * self.nx + 1) * (4 * self.nz + 1)
 
     @property
@@ -124,9 +169,9 @@ class SimulationConfig:
         return float(self.sources[0].f0)
 
     def gll_point_coords(self):
-        """Coordinates (x, z) of every element GLL point, shape [nspec, 25] each.
-
-        Valid for the flat-interface structured mesh this config describes."""
+        """Coordinates (x, z) of every element GLL point, shape [nspec, 25] each."""
+        if self.mesh is not None:
+            return self.mesh.gll_point_coords()
         xs = self.xmin + (self.xmax - self.xmin) * np.arange(self.nx + 1) / self.nx
         zrows = []
         for il, nzl in enumerate(self.nz_layers):

@@ -204,6 +204,8 @@ class Simulation:
             self.prepare_device()
 
     def _create(self, p, rho, vp, vs, rec, simulation_type, save_forward):
+        if self.cfg.mesh is not None:
+            raise NotImplementedError("external meshes are not set up by the host driver")
         srcs = _sources_c(self.cfg.sources)
         self._keep = (p, rho, vp, vs, srcs, rec)
         return self.host.sf2d_sim_create(C.byref(p), rho, vp, vs, len(self.cfg.sources), srcs, self.cfg.nrec, rec,

@@ -10,8 +10,9 @@ Stacey or free boundaries, collocated force / pressure sources with time functio
 Anything else in the files is refused with a message naming the parameter.
 
 Writers follow ``write_seismograms.F90:252-584``: ASCII traces ``NET.STA.CMP.semX`` with the gfortran
-list-directed layout of ``write(11,*) time, ' ', value`` (byte-compatible), and the binary
-``U{x,z,p}_file_{single,double}_X.bin`` streams ([receiver][sample], native endianness).
+list-directed layout of ``write(11,*) time, ' ', value`` (byte-compatible), the binary
+``U{x,z,p}_file_{single,double}_X.bin`` streams ([receiver][sample], native endianness), and the Seismic Unix
+``U{x,z,p}_file_single_X.su`` files of SU_FORMAT (write_output_SU.f90: a 240-byte header per trace).
 """
 from __future__ import annotations
 
@@ -23,7 +24,7 @@ from typing import Dict, List, Optional, Sequence, Tuple
 
 import numpy as np
 
-from .config import NGLL2, SimulationConfig, Source
+from .config import NGLL2, ExternalMesh, SimulationConfig, Source
 
 
 # ----------------------------------------------------------------------------------------------------------
@@ -234,7 +235,7 @@ def config_from_par_file(data_dir: str, cpml: str = "cpml", nstep: Optional[int]
     refuse(fortran_bool(g("AXISYM", ".false.")), "AXISYM")
     refuse(int(g("time_stepping_scheme", "1")) != 1, "time_stepping_scheme other than Newmark")
     refuse(g("MODEL", "default").strip().lower() != "default", f"MODEL = {g('MODEL')}")
-    refuse(fortran_bool(g("read_external_mesh", ".false.")), "read_external_mesh (external meshes)")
+    external = fortran_bool(g("read_external_mesh", ".false."))
     attenuation = fortran_bool(g("ATTENUATION_VISCOELASTIC", ".false."))
     if attenuation:
         refuse(fortran_bool(g("USE_SOLVOPT", ".false.")), "USE_SOLVOPT (nonlinear fit of the relaxation times)")
@@ -263,24 +264,34 @@ def config_from_par_file(data_dir: str, cpml: str = "cpml", nstep: Optional[int]
             raise NotImplementedError("Par_file: PML_BOUNDARY_CONDITIONS refused (cpml='error')")
     seismotypes = [int(t) for t in re.split(r"[ ,]+", g("seismotype").strip()) if t]
     refuse(len(seismotypes) != 1 or seismotypes[0] not in (1, 2, 3, 4, 6), f"seismotype = {g('seismotype')}")
-    heights, nz_layers, _ = read_interfaces(os.path.join(data_dir, os.path.basename(g("interfacesfile"))))
-    nx = int(g("nx"))
-    nz = sum(nz_layers)
-    xmin, xmax = fortran_float(g("xmin")), fortran_float(g("xmax"))
-    # materials of the elements (read_regions.f90: later regions overwrite earlier ones), per GLL point
-    mat = np.zeros(nx * nz, np.int64)
-    for ix0, ix1, iz0, iz1, m in par.regions:
-        if m not in par.materials:
-            raise ValueError(f"Par_file: region uses undefined material {m}")
-        for iz in range(iz0, iz1 + 1):
-            mat[(iz - 1) * nx + ix0 - 1:(iz - 1) * nx + ix1] = m
-    if np.any(mat == 0):
-        raise ValueError("Par_file: regions leave elements without a material")
-    rho = np.zeros(nx * nz)
-    vp = np.zeros(nx * nz)
-    vs = np.zeros(nx * nz)
-    qk = np.full(nx * nz, 9999.0)
-    qm = np.full(nx * nz, 9999.0)
+    mesh = None
+    if external:
+        refuse(int(g("NGNOD")) != 4, "read_external_mesh with NGNOD = 9")
+        refuse(pml, "read_external_mesh with PML_BOUNDARY_CONDITIONS")
+        mesh = read_external_mesh(par, data_dir)
+        nx, nz, heights, nz_layers = mesh.nspec, 1, [float(mesh.coorg[:, 1].min()), float(mesh.coorg[:, 1].max())], [1]
+        xmin, xmax = float(mesh.coorg[:, 0].min()), float(mesh.coorg[:, 0].max())
+        mat = mesh.material.astype(np.int64)
+    else:
+        heights, nz_layers, _ = read_interfaces(os.path.join(data_dir, os.path.basename(g("interfacesfile"))))
+        nx = int(g("nx"))
+        nz = sum(nz_layers)
+        xmin, xmax = fortran_float(g("xmin")), fortran_float(g("xmax"))
+        # materials of the elements (read_regions.f90: later regions overwrite earlier ones), per GLL point
+        mat = np.zeros(nx * nz, np.int64)
+        for ix0, ix1, iz0, iz1, m in par.regions:
+            if m not in par.materials:
+                raise ValueError(f"Par_file: region uses undefined material {m}")
+            for iz in range(iz0, iz1 + 1):
+                mat[(iz - 1) * nx + ix0 - 1:(iz - 1) * nx + ix1] = m
+    if np.any(mat == 0) or np.any([m not in par.materials for m in np.unique(mat)]):
+        raise ValueError("Par_file: elements without a material, or with an undefined one")
+    ne = len(mat)
+    rho = np.zeros(ne)
+    vp = np.zeros(ne)
+    vs = np.zeros(ne)
+    qk = np.full(ne, 9999.0)
+    qm = np.full(ne, 9999.0)
     for m, vals in par.materials.items():
         icode = int(vals[0])
         refuse(icode != 1, f"material {m} with icodemat {icode} (only isotropic tables)")
@@ -307,8 +318,73 @@ def config_from_par_file(data_dir: str, cpml: str = "cpml", nstep: Optional[int]
         receivers=np.array([[s.x, s.z] for s in stations]) if stations else None,
         user_t0=fortran_float(g("USER_T0", "0")), anglerec=fortran_float(g("anglerec", "0")),
         seismotype=seismotypes[0], use_trick=fortran_bool(g("USE_TRICK_FOR_BETTER_PRESSURE", ".false.")),
-        attenuation=attenuation, qkappa=rep(qk), qmu=rep(qm), n_sls=int(g("N_SLS", "3")), pml=pml, **pml_kw)
+        attenuation=attenuation, qkappa=rep(qk), qmu=rep(qm), n_sls=int(g("N_SLS", "3")), pml=pml, mesh=mesh,
+        **pml_kw)
     return cfg, stations
+
+
+def _mesh_file(par: ParFile, data_dir: str, key: str) -> str:
+    """A file named by the Par_file, relative to the run directory (the parent of DATA/) or to the Par_file's."""
+    rel = par.get(key).strip()
+    for base in (os.path.dirname(os.path.abspath(data_dir)), data_dir):
+        p = os.path.normpath(os.path.join(base, rel))
+        if os.path.exists(p):
+            return p
+    raise FileNotFoundError(f"Par_file: {key} = {rel} not found next to {data_dir}")
+
+
+def _ints(path: str) -> List[int]:
+    with open(path) as f:
+        return [int(t) for t in f.read().split()]
+
+
+def read_external_mesh(par: ParFile, data_dir: str) -> ExternalMesh:
+    """The files of ``read_external_mesh = .true.`` (src/meshfem2D/read_external_mesh_files.F90): ``mesh_file``
+    (element count, then 4 control nodes per element, 1-based), ``nodes_coords_file`` (node count, then x z per node),
+    ``materials_file`` (a material number per element), ``free_surface_file`` and ``absorbing_surface_file`` (a side
+    count, then per side: element, node count 2, the two nodes).  Free-surface sides count on fluid elements only
+    (read_external_acoustic_surface, :398-416, keeps the sides of fluid materials); absorbing sides on all."""
+    mesh_path = _mesh_file(par, data_dir, "mesh_file")
+    with open(mesh_path) as f:
+        head = f.readline().split()
+        nspec = int(head[0])
+        knods = np.array([int(t) for t in f.read().split()], np.int64)
+    if knods.size != 4 * nspec:
+        raise ValueError(f"{mesh_path}: {knods.size} node numbers for {nspec} 4-node elements")
+    with open(_mesh_file(par, data_dir, "nodes_coords_file")) as f:
+        npgeo = int(f.readline().split()[0])
+        coorg = np.array([fortran_float(t) for t in f.read().split()], np.float64).reshape(npgeo, 2)
+    mat = np.array(_ints(_mesh_file(par, data_dir, "materials_file")), np.int64)
+    if mat.size != nspec:
+        raise ValueError(f"materials_file: {mat.size} materials for {nspec} elements")
+    knods = knods.reshape(nspec, 4) - 1
+    if knods.min() < 0 or knods.max() >= npgeo:
+        raise ValueError("mesh_file: node numbers outside nodes_coords_file")
+    mesh = ExternalMesh(knods=knods, coorg=coorg, material=mat, free_elem=np.zeros(0, np.int64),
+                        free_code=np.zeros(0, np.int64), abs_elem=np.zeros(0, np.int64),
+                        abs_code=np.zeros(0, np.int64))
+
+    def sides(key):
+        try:
+            vals = _ints(_mesh_file(par, data_dir, key))
+        except (KeyError, FileNotFoundError):
+            return np.zeros(0, np.int64), np.zeros(0, np.int64)
+        n, rows = vals[0], vals[1:]
+        width = len(rows) // n if n else 4
+        if n and (len(rows) % n or width not in (4, 5)):
+            raise ValueError(f"{key}: {len(rows)} values for {n} sides")
+        el, code = [], []
+        for k in range(n):
+            r = rows[k * width:(k + 1) * width]
+            if r[1] != 2:
+                raise NotImplementedError(f"{key}: a side with {r[1]} nodes (only 2-node sides)")
+            el.append(r[0] - 1)
+            code.append(mesh.side_code(r[0] - 1, r[2] - 1, r[3] - 1))
+        return np.array(el, np.int64), np.array(code, np.int64)
+
+    mesh.free_elem, mesh.free_code = sides("free_surface_file")
+    mesh.abs_elem, mesh.abs_code = sides("absorbing_surface_file")
+    return mesh
 
 
 # ----------------------------------------------------------------------------------------------------------
@@ -378,6 +454,83 @@ def write_binary_seismograms(out_dir: str, seis: np.ndarray, seismotype: int = 1
         np.ascontiguousarray(seis[:, c, :], dtype=dtype).tofile(p)
         paths.append(p)
     return paths
+
+
+def _nint(x: float) -> int:
+    """Fortran NINT: nearest integer, halves away from zero."""
+    return int(np.sign(x) * np.floor(abs(x) + 0.5))
+
+
+SU_HEADER_WORDS = 60  # 240 bytes: 28 int32, 2 + 2 int16, 30 float32
+
+
+def su_header(irec: int, xr: float, zr: float, xs: float, zs: float, nsamples: int, sampling_dt: float,
+              receiver_interval: float) -> bytes:
+    """The 240-byte trace header of write_output_SU.f90:55-96 (little-endian, as the x86-64 build writes it):
+    int32 words 1 receiver number, 10 offset NINT(xr - xs), 19-22 NINT(xs, zs, xr, zr); int16 pair (0, samples,
+    -9999 past 32767); int16 pair (sampling interval in us, else ms, else s; 0); float32 word 18 the receiver
+    interval x2 - x1 (0 for one receiver)."""
+    h1 = np.zeros(28, "<i4")
+    h1[0] = irec
+    h1[9] = _nint(xr - xs)
+    h1[18], h1[19], h1[20], h1[21] = _nint(xs), _nint(zs), _nint(xr), _nint(zr)
+    h2 = np.array([0, nsamples if nsamples < 32768 else -9999], "<i2")
+    us, ms = _nint(sampling_dt * 1.0e6), _nint(sampling_dt * 1.0e3)
+    h3 = np.array([us if us < 32768 else (ms if ms < 32768 else _nint(sampling_dt)), 0], "<i2")
+    h4 = np.zeros(30, "<f4")
+    h4[17] = np.float32(receiver_interval)
+    return h1.tobytes() + h2.tobytes() + h3.tobytes() + h4.tobytes()
+
+
+def write_su_seismograms(out_dir: str, seis: np.ndarray, receivers, source_xz, dt: float, seismotype: int = 1,
+                         subsamp: int = 1) -> List[str]:
+    """Seismic Unix files as SU_FORMAT writes them (write_seismograms.F90:337-392, write_output_SU.f90:34-125):
+    ``U{x,z}_file_single_X.su`` (``Up_`` for pressure and potential), per receiver a 240-byte header and its
+    samples in float32.  ``seis`` [nrec, 2, nsamples], ``receivers`` [nrec, 2] (x, z), ``source_xz`` the first
+    source's position, ``subsamp`` NTSTEP_BETWEEN_OUTPUT_SAMPLE."""
+    os.makedirs(out_dir, exist_ok=True)
+    seis = np.asarray(seis, np.float64)
+    rec = np.asarray(receivers, np.float64).reshape(-1, 2)
+    nrec, nsamp = seis.shape[0], seis.shape[-1]
+    if rec.shape[0] != nrec:
+        raise ValueError("one (x, z) per receiver")
+    xs, zs = float(source_xz[0]), float(source_xz[1])
+    interval = float(np.float32(rec[1, 0] - rec[0, 0])) if nrec > 1 else 0.0
+    comp = _CHANNEL_SUFFIX[seismotype]
+    names = [("Up", 0)] if seismotype in (4, 6) else [("Ux", 0), ("Uz", 1)]
+    paths = []
+    for pre, c in names:
+        p = os.path.join(out_dir, f"{pre}_file_single_{comp}.su")
+        with open(p, "wb") as f:
+            for r in range(nrec):
+                f.write(su_header(r + 1, rec[r, 0], rec[r, 1], xs, zs, nsamp, dt * subsamp, interval))
+                f.write(np.ascontiguousarray(seis[r, c], "<f4").tobytes())
+        paths.append(p)
+    return paths
+
+
+def read_su(path: str):
+    """Traces of an SU file written as above: (headers [ntrace] as dicts, samples [ntrace, nsamples] float32),
+    read the way the reference's own helper does (EXAMPLES/BENCHMARK_CLAERBOUT_ADJOINT/helper_functions.py:367)."""
+    raw = open(path, "rb").read()
+    if len(raw) < 240:
+        raise ValueError(f"{path}: no SU trace header")
+    nsamp = int(np.frombuffer(raw[114:116], "<i2")[0])
+    size = 240 + 4 * nsamp
+    if nsamp <= 0 or len(raw) % size:
+        raise ValueError(f"{path}: not a sequence of SU traces of {nsamp} samples")
+    heads, data = [], []
+    for k in range(len(raw) // size):
+        b = raw[k * size:(k + 1) * size]
+        h1 = np.frombuffer(b[:112], "<i4")
+        h2 = np.frombuffer(b[112:116], "<i2")
+        h3 = np.frombuffer(b[116:120], "<i2")
+        h4 = np.frombuffer(b[120:240], "<f4")
+        heads.append({"irec": int(h1[0]), "offset": int(h1[9]), "xs": int(h1[18]), "zs": int(h1[19]),
+                      "xr": int(h1[20]), "zr": int(h1[21]), "nsamples": int(h2[1]), "dt_code": int(h3[0]),
+                      "receiver_interval": float(h4[17])})
+        data.append(np.frombuffer(b[240:], "<f4"))
+    return heads, np.array(data)
 
 
 def read_ascii_trace(path: str) -> Tuple[np.ndarray, np.ndarray]:

@@ -30,6 +30,18 @@ def pack(example, sub, stations, out):
     print("wrote", out, {k: a.shape for k, a in arrays.items()})
 
 
+def pack_float32(example, sub, stations, out):
+    """As pack, the traces kept in float32 (long traces; the reference computed them in single precision)."""
+    arrays = {}
+    for st in stations:
+        for comp in ("BXX", "BXZ"):
+            t, v = read_trace(os.path.join(REF, "EXAMPLES", example, sub, f"{st}.{comp}.semd"))
+            arrays[f"{st}.{comp}"] = v.astype(np.float32)
+            arrays["time"] = t
+    np.savez_compressed(os.path.join(HERE, out), **arrays)
+    print("wrote", out, {k: a.shape for k, a in arrays.items()})
+
+
 def pack_pressure(example, sub, station, out):
     t, v = read_trace(os.path.join(REF, "EXAMPLES", example, sub, f"{station}.PRE.semp"))
     np.savez_compressed(os.path.join(HERE, out), time=t, **{f"{station}.PRE": v})
@@ -45,10 +57,11 @@ def pack_forward_potential(out):
     print("wrote", out, t.shape)
 
 
-def copy_inputs(example, files, out):
-    """The example's input files (Par_file, SOURCE, interfaces: plain-text data) for the on-disk format tests."""
+def copy_inputs(example, files, out, subdir=""):
+    """The example's input files (Par_file, SOURCE, interfaces, external mesh files: plain-text data) for the
+    on-disk format tests; ``subdir`` keeps them in a subdirectory of the copy (the mesh files' own directory)."""
     import shutil
-    dst = os.path.join(HERE, "examples", out)
+    dst = os.path.join(HERE, "examples", out, subdir)
     os.makedirs(dst, exist_ok=True)
     for rel in files:
         shutil.copyfile(os.path.join(REF, "EXAMPLES", example, rel), os.path.join(dst, os.path.basename(rel)))
@@ -112,3 +125,8 @@ if __name__ == "__main__":
     copy_inputs("check_absolute_amplitude_of_force_source_seismograms_viscoelastic",
                 ["REF_SEIS/Par_file", "REF_SEIS/SOURCE", "DATA/interfaces_attenuation_analytic.dat"],
                 "force_source_viscoelastic")
+    ext = os.path.join("fluid_solid", "fluid_solid_external_mesh")
+    copy_inputs(ext, ["DATA/Par_file", "DATA/SOURCE"], "fluid_solid_external_mesh")
+    copy_inputs(ext, ["mesh/mesh", "mesh/nodes_coords", "mesh/mat", "mesh/surface_free", "mesh/surface_abs"],
+                "fluid_solid_external_mesh", subdir="mesh")
+    pack_float32(ext, "REF_SEIS", [f"AA.S{k:04d}" for k in range(1, 13)], "fluid_solid_external_mesh_ref_seis.npz")

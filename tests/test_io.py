@@ -115,3 +115,38 @@ def test_luoyang_inputs_give_the_configuration(golden_dir):
     np.testing.assert_array_equal(cfg.receivers, ref.receivers)
     for a, b in zip(cfg.material_arrays(), ref.material_arrays()):
         np.testing.assert_array_equal(a, b)
+
+
+def test_su_writer_layout(tmp_path):
+    """SU files as write_output_SU.f90 lays them out: per receiver 60 header words then the float32 samples; header
+    fields at the words the reference's own reader takes them from (irec 1, offset 10, xs 19, zs 20, xr 21, zr 22,
+    samples in the second int16 of word 29, the sampling interval in us in the first int16 of word 30, the receiver
+    interval as float word 18 of the last 30), Fortran NINT rounding (halves away from zero)."""
+    rng = np.random.default_rng(4)
+    nrec, nstep = 5, 333
+    seis = rng.standard_normal((nrec, 2, nstep))
+    rec = np.array([[100.5 + 200.0 * k, -2.5] for k in range(nrec)])
+    src = (1000.0, -500.5)
+    paths = io.write_su_seismograms(str(tmp_path), seis, rec, src, dt=1.1e-3, seismotype=1)
+    assert [os.path.basename(p) for p in paths] == ["Ux_file_single_d.su", "Uz_file_single_d.su"]
+    for c, p in enumerate(paths):
+        raw = open(p, "rb").read()
+        assert len(raw) == nrec * (240 + 4 * nstep)
+        words = np.frombuffer(raw, "<i4").reshape(nrec, 60 + nstep)
+        for r in range(nrec):
+            assert words[r, 0] == r + 1
+            assert words[r, 9] == io._nint(rec[r, 0] - src[0])
+            assert tuple(words[r, 18:22]) == (1000, -501, io._nint(rec[r, 0]), -3)
+        heads, data = io.read_su(p)
+        assert [h["nsamples"] for h in heads] == [nstep] * nrec
+        assert all(h["dt_code"] == 1100 for h in heads)
+        assert all(h["receiver_interval"] == 200.0 for h in heads)
+        np.testing.assert_array_equal(data, seis[:, c, :].astype(np.float32))
+    # the samples are the single-precision binary streams with headers interleaved
+    b = io.write_binary_seismograms(str(tmp_path), seis, seismotype=1)
+    np.testing.assert_array_equal(np.fromfile(b[0], np.float32).reshape(nrec, nstep), io.read_su(paths[0])[1])
+    # pressure: one file; intervals of 40 ms and longer are written in ms
+    p = io.write_su_seismograms(str(tmp_path), seis[:, :1], rec, src, dt=0.04, seismotype=4)
+    assert [os.path.basename(x) for x in p] == ["Up_file_single_p.su"]
+    assert io.read_su(p[0])[0][0]["dt_code"] == 40
+    assert io._nint(2.5) == 3 and io._nint(-2.5) == -3 and io._nint(-0.4) == 0
