@@ -79,6 +79,13 @@ void sf2d_pml_reset(long* Mesh_pointer);
 /* stats[5] = {solid layer elements, fluid layer elements, distinct coefficient sets, solid outer-boundary points,
    fluid outer-boundary points} (zeros without a layer) */
 void sf2d_pml_stats(long* Mesh_pointer, int* stats);
+/* get_norm_elastic_from_device_ (same arguments and result: max over the points of |displ|, or of |b_displ| when
+   *FORWARD_OR_ADJOINT == 3) that allocates nothing: no counterpart in the reference.  It is for callers that cannot
+   allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+   Its reduction scratch is a file-scope device array of 2049 words (8 KiB per GPU, held until the process ends)
+   instead of the handle's buffer; any field size fits it (grid-stride first pass, at most 2048 partial maxima).
+   Calls that overlap on one GPU (several host threads or streams) race on that array. */
+void get_norm_elastic_from_device_noalloc_(float* norm, long* Mesh_pointer, const int* FORWARD_OR_ADJOINT);
 /* device pointer of a field: 0 displ 1 veloc 2 accel 3 b_displ 4 b_veloc 5 b_accel
    6 rho_kl 7 mu_kl 8 kappa_kl (NULL when not allocated).  On the device, point arrays are in the
    library's internal point numbering and kernels in its element order; the transfer entry points

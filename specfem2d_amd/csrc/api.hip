@@ -549,6 +549,7 @@ void initialize_cuda_device_(int* myrank_f, int* ncuda_devices) {  // initialize
   int got = -1;
   check_hip(hipGetDevice(&got), "hipGetDevice");
   if (got != device) fail("device id conflict: asked %d got %d", device, got);
+  noalloc_scratch();  // the device address of get_norm_elastic_from_device_noalloc_'s scratch on this device
 }
 
 void initialize_cuda_aware_mpi_(void) {}  // initialize_cuda.cu:264 (nothing to set up on one GPU)

@@ -247,6 +247,10 @@ void points_from_device(const Mesh* mp, float* host, const float* dev, int n2, c
 std::vector<float> to_device_order(const Mesh* mp, const float* a);
 void to_ispec_order(const Mesh* mp, const float* dev_order, float* out);
 
+// device address of the no-allocation entry point's scratch array on the current device (noalloc.hip), looked up
+// once per device
+unsigned int* noalloc_scratch();
+
 // Errors follow the reference's convention: message, error file, exit(EXIT_FAILURE).
 [[noreturn]] void fail(const char* fmt, ...);
 void check_hip(hipError_t err, const char* what);
