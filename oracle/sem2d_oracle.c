@@ -134,6 +134,15 @@ typedef struct {
   REAL *att_A1, *att_B1, *att_A2, *att_B2; /* [nspec*25][nsls] */
   REAL *e1, *e11, *e13;       /* [nspec*25][nsls] */
   REAL *old_xx, *old_zz, *old_xz; /* [nspec*25] */
+  /* the Zener model's per-point parameters as attenuation_model.f90:145-161 stores them (single precision):
+     1/tau_sigma, tau_epsilon, phi [nspec*25][nsls] and Mu [nspec*25] for the bulk (1) and shear (2) modes; the
+     viscoelastic CPML uses them (pml_compute_memory_variables.f90:546-894) in elements where both quality factors
+     are finite at every point (visco_pml), with memory variables of the bulk and shear convolutions
+     [nspec_pml*25][nsls]: ka_xx, ka_zz (dux_dx, duz_dz) and mu_xx, mu_zz, mu_xz, mu_zx (dux_dx, duz_dz, dux_dz,
+     duz_dx) */
+  REAL *att_itau1, *att_itau2, *att_te1, *att_te2, *att_phi1, *att_phi2, *att_mu1, *att_mu2;
+  char *visco_pml;            /* [nspec] */
+  REAL *vp_ka_xx, *vp_ka_zz, *vp_mu_xx, *vp_mu_zz, *vp_mu_xz, *vp_mu_zx;
   int it_done;                /* forward steps already run */
   /* CPML (pml_init.F90, prepare_pml.f90): PML elements, their region (1 X only, 2 Z only, 3 XZ), per-point
      damping profiles in double precision (K, d, alpha along x and z), the outer-boundary points that are held
@@ -905,7 +914,8 @@ static void setup_pml(orc_sim *s) {
 /* A deliberate mistake planted in the CPML terms, for the tests that show each term is pinned by the reference's
    traces (0 = none): 1 previous-step fields without the factor 1/2 on the acceleration term, 2 no Newmark damping
    term in the mass matrix, 3 no zero condition on the outer boundary, 4 symmetric stress in PML elements, 5 no
-   damping terms in the acceleration. */
+   damping terms in the acceleration, 6 the elastic layer formulation in attenuating layer elements, 7 no Zener
+   convolution terms in the viscoelastic layer (A_zener and the phi sums dropped). */
 static int g_pml_mistake = 0;
 void orc_set_pml_mistake(int which) { g_pml_mistake = which; }
 
@@ -1319,6 +1329,9 @@ static void setup_receivers(orc_sim *s) {
 /* ------------------------------------------------------------------ */
 /* time step pieces                                                     */
 /* ------------------------------------------------------------------ */
+static void pml_lik_visco(double kx, double bx, double ax, double kz, double bz, double az, int region, int index_ik,
+                          double itau, double te, double A[4]);
+
 static void element_force(orc_sim *s, int e, const REAL *displ, REAL *accel) {
   /* compute_forces_viscoelastic.F90:34-885, isotropic P-SV, no AXISYM; CPML; viscoelastic attenuation in forward
      runs (att_live) -- one element */
@@ -1332,6 +1345,7 @@ static void element_force(orc_sim *s, int e, const REAL *displ, REAL *accel) {
     if (s->is_acoustic[e]) return;
     const int *ib = s->ibool + (size_t)e * NGLL2;
     const int pml = s->pml_on && s->is_pml[e];
+    const int vpml = pml && s->att_live && s->visco_pml && s->visco_pml[e] && g_pml_mistake != 6;
     for (int j = 0; j < NGLL; j++)
       for (int i = 0; i < NGLL; i++) {
         int g = ib[j * NGLL + i] - 1;
@@ -1358,6 +1372,10 @@ static void element_force(orc_sim *s, int e, const REAL *displ, REAL *accel) {
         /* CPML: the unmodified gradient (PML_du*) and the stretched one through the memory variables of the
            current and previous-step gradients (pml_compute_memory_variables.f90:230-540, Newmark, not rotated) */
         REAL p_dux_dxl = dux_dxl, p_dux_dzl = dux_dzl, p_duz_dxl = duz_dxl, p_duz_dzl = duz_dzl;
+        /* viscoelastic CPML: the bulk (k) and shear (m) stretched gradients with their Zener convolutions, and the
+           attenuated unstretched ones (pml_compute_memory_variables.f90:546-894) */
+        REAL vk_xx = 0, vk_zz = 0, vm_xx = 0, vm_zz = 0, vm_xz = 0, vm_zx = 0;
+        REAL kk_xx = 0, kk_zz = 0, mm_xx = 0, mm_zz = 0, mm_xz = 0, mm_zx = 0;
         if (pml) {
           const REAL *dold = s->displ_old;
           REAL oxi = 0, ozi = 0, oxg = 0, ozg = 0;
@@ -1384,10 +1402,70 @@ static void element_force(orc_sim *s, int e, const REAL *displ, REAL *accel) {
           rzz[0] = (REAL)(cxz1[0] * rzz[0] + cxz1[1] * p_duz_dzl + cxz1[2] * o_duz_dzl);
           rxz[1] = (REAL)(cxz2[0] * rxz[1] + cxz2[1] * p_dux_dzl + cxz2[2] * o_dux_dzl);
           rzz[1] = (REAL)(cxz2[0] * rzz[1] + cxz2[1] * p_duz_dzl + cxz2[2] * o_duz_dzl);
-          dux_dxl = (REAL)(Azx[0] * p_dux_dxl + Azx[1] * rxx[0] + Azx[2] * rxx[1]);
-          duz_dxl = (REAL)(Azx[0] * p_duz_dxl + Azx[1] * rzx[0] + Azx[2] * rzx[1]);
-          dux_dzl = (REAL)(Axz[0] * p_dux_dzl + Axz[1] * rxz[0] + Axz[2] * rxz[1]);
-          duz_dzl = (REAL)(Axz[0] * p_duz_dzl + Axz[1] * rzz[0] + Axz[2] * rzz[1]);
+          if (vpml) {
+            const int N = s->nsls;
+            const double mk = (double)s->att_mu1[l] * (double)N, mm = (double)s->att_mu2[l] * (double)N;
+            REAL ks_xx = 0, ks_zz = 0, ms_xx = 0, ms_zz = 0, ms_xz = 0, ms_zx = 0;  /* kappa_sum_*, mu_sum_* */
+            REAL sk_xx = 0, sk_zz = 0, sm_xx = 0, sm_zz = 0, sm_xz = 0, sm_zx = 0;  /* sum_kappa_*, sum_mu_* */
+            double a2k = 0, a2m = 0, a3k = 0, a3m = 0, a4k = 0, a4m = 0, a5k = 0, a5m = 0;
+            for (int k = 0; k < N; k++) {
+              const size_t la = l * N + k, qa = q * N + k;
+              const double itk = s->att_itau1[la], itm = s->att_itau2[la];
+              const double tek = s->att_te1[la], tem = s->att_te2[la];
+              const double phk = s->att_phi1[la], phm = s->att_phi2[la];
+              double K31[4], M31[4], K13[4], M13[4], clk[3], clm[3];
+              pml_lik_visco(kz, bz, az, kx, bx, ax, s->region[e], 31, itk, tek, K31);
+              pml_lik_visco(kz, bz, az, kx, bx, ax, s->region[e], 31, itm, tem, M31);
+              pml_lik_visco(kx, bx, ax, kz, bz, az, s->region[e], 13, itk, tek, K13);
+              pml_lik_visco(kx, bx, ax, kz, bz, az, s->region[e], 13, itm, tem, M13);
+              pml_coef(itk, DT, &clk[0], &clk[1], &clk[2]);
+              pml_coef(itm, DT, &clm[0], &clm[1], &clm[2]);
+              REAL *kxx = s->vp_ka_xx + qa, *kzz = s->vp_ka_zz + qa;
+              REAL *mxx = s->vp_mu_xx + qa, *mzz = s->vp_mu_zz + qa, *mxz = s->vp_mu_xz + qa, *mzx = s->vp_mu_zx + qa;
+              *kxx = (REAL)(clk[0] * *kxx + clk[1] * p_dux_dxl + clk[2] * o_dux_dxl);
+              *kzz = (REAL)(clk[0] * *kzz + clk[1] * p_duz_dzl + clk[2] * o_duz_dzl);
+              *mxx = (REAL)(clm[0] * *mxx + clm[1] * p_dux_dxl + clm[2] * o_dux_dxl);
+              *mzz = (REAL)(clm[0] * *mzz + clm[1] * p_duz_dzl + clm[2] * o_duz_dzl);
+              *mxz = (REAL)(clm[0] * *mxz + clm[1] * p_dux_dzl + clm[2] * o_dux_dzl);
+              *mzx = (REAL)(clm[0] * *mzx + clm[1] * p_duz_dxl + clm[2] * o_duz_dxl);
+              a2k = a2k + K31[1]; a2m = a2m + M31[1];
+              a3k = a3k + K31[2]; a3m = a3m + M31[2];
+              a4k = a4k + K13[1]; a4m = a4m + M13[1];
+              a5k = a5k + K13[2]; a5m = a5m + M13[2];
+              if (g_pml_mistake == 7) K31[3] = K13[3] = M31[3] = M13[3] = 0.0;
+              const double phk_ = g_pml_mistake == 7 ? 0.0 : phk, phm_ = g_pml_mistake == 7 ? 0.0 : phm;
+              ks_xx = (REAL)((double)ks_xx + K31[3] * (double)*kxx / mk);
+              ks_zz = (REAL)((double)ks_zz + K13[3] * (double)*kzz / mk);
+              ms_xx = (REAL)((double)ms_xx + M31[3] * (double)*mxx / mm);
+              ms_zz = (REAL)((double)ms_zz + M13[3] * (double)*mzz / mm);
+              ms_xz = (REAL)((double)ms_xz + M13[3] * (double)*mxz / mm);
+              ms_zx = (REAL)((double)ms_zx + M31[3] * (double)*mzx / mm);
+              sk_xx = (REAL)((double)sk_xx + phk_ * (double)*kxx);
+              sk_zz = (REAL)((double)sk_zz + phk_ * (double)*kzz);
+              sm_xx = (REAL)((double)sm_xx + phm_ * (double)*mxx);
+              sm_zz = (REAL)((double)sm_zz + phm_ * (double)*mzz);
+              sm_xz = (REAL)((double)sm_xz + phm_ * (double)*mxz);
+              sm_zx = (REAL)((double)sm_zx + phm_ * (double)*mzx);
+            }
+            const double A01 = Azx[0], A10 = Axz[0];
+            vk_xx = (REAL)(A01 * p_dux_dxl + ks_xx + a2k * rxx[0] / mk + a3k * rxx[1] / mk);
+            vk_zz = (REAL)(A10 * p_duz_dzl + ks_zz + a4k * rzz[0] / mk + a5k * rzz[1] / mk);
+            vm_xx = (REAL)(A01 * p_dux_dxl + ms_xx + a2m * rxx[0] / mm + a3m * rxx[1] / mm);
+            vm_zz = (REAL)(A10 * p_duz_dzl + ms_zz + a4m * rzz[0] / mm + a5m * rzz[1] / mm);
+            vm_xz = (REAL)(A10 * p_dux_dzl + ms_xz + a4m * rxz[0] / mm + a5m * rxz[1] / mm);
+            vm_zx = (REAL)(A01 * p_duz_dxl + ms_zx + a2m * rzx[0] / mm + a3m * rzx[1] / mm);
+            kk_xx = p_dux_dxl + sk_xx;
+            kk_zz = p_duz_dzl + sk_zz;
+            mm_xx = p_dux_dxl + sm_xx;
+            mm_zz = p_duz_dzl + sm_zz;
+            mm_xz = p_dux_dzl + sm_xz;
+            mm_zx = p_duz_dxl + sm_zx;
+          } else {
+            dux_dxl = (REAL)(Azx[0] * p_dux_dxl + Azx[1] * rxx[0] + Azx[2] * rxx[1]);
+            duz_dxl = (REAL)(Azx[0] * p_duz_dxl + Azx[1] * rzx[0] + Azx[2] * rzx[1]);
+            dux_dzl = (REAL)(Axz[0] * p_dux_dzl + Axz[1] * rxz[0] + Axz[2] * rxz[1]);
+            duz_dzl = (REAL)(Axz[0] * p_duz_dzl + Axz[1] * rzz[0] + Axz[2] * rzz[1]);
+          }
         }
         REAL mul = s->mustore[l], rhol = s->rhostore[l];
         REAL cpl = s->rho_vp[l] / rhol;
@@ -1396,7 +1474,7 @@ static void element_force(orc_sim *s, int e, const REAL *displ, REAL *accel) {
         REAL sxx = lp2m * dux_dxl + lambdal * duz_dzl;
         REAL szz = lp2m * duz_dzl + lambdal * dux_dxl;
         REAL sxz = mul * (duz_dxl + dux_dzl);
-        if (s->att_live && s->att_on[l]) {
+        if (s->att_live && s->att_on[l] && !pml) {  /* nothing in CPML elements (compute_attenuation_viscoelastic.f90:80) */
           /* compute_attenuation_viscoelastic.f90:93-121 (Newmark, convolution memory variables), then the
              unrelaxed-moduli stress correction of compute_forces_viscoelastic.F90:481-496 */
           const int N = s->nsls;
@@ -1425,7 +1503,14 @@ static void element_force(orc_sim *s, int e, const REAL *displ, REAL *accel) {
           szz = szz + lpm * e1s - (REAL)2 * mul * e11s;
         }
         REAL szx = sxz;
-        if (pml) {
+        if (vpml) {
+          /* compute_forces_viscoelastic.F90:545-556 with the unrelaxed moduli, lambda + mu = kappa */
+          const REAL lpm = lambdal + mul;
+          sxx = (lpm * vk_xx + mul * vm_xx) + (lpm * kk_zz - mul * mm_zz);
+          sxz = (mul * vm_zx + mul * mm_xz);
+          szx = (mul * mm_zx + mul * vm_xz);
+          szz = (lpm * kk_xx - mul * mm_xx) + (lpm * vk_zz + mul * vm_zz);
+        } else if (pml) {
           /* non-symmetric stress of the stretched equations (compute_forces_viscoelastic.F90:596-611) */
           sxx = lp2m * dux_dxl + lambdal * p_duz_dzl;
           szz = lp2m * duz_dzl + lambdal * p_dux_dxl;
@@ -1934,6 +2019,9 @@ void orc_free(void *h) {
   free(s->pot); free(s->pot_dot); free(s->pot_dd);
   free(s->att_on); free(s->att_A1); free(s->att_B1); free(s->att_A2); free(s->att_B2);
   free(s->e1); free(s->e11); free(s->e13); free(s->old_xx); free(s->old_zz); free(s->old_xz);
+  free(s->att_itau1); free(s->att_itau2); free(s->att_te1); free(s->att_te2); free(s->att_phi1); free(s->att_phi2);
+  free(s->att_mu1); free(s->att_mu2); free(s->visco_pml);
+  free(s->vp_ka_xx); free(s->vp_ka_zz); free(s->vp_mu_xx); free(s->vp_mu_zz); free(s->vp_mu_xz); free(s->vp_mu_zx);
   free(s->is_pml); free(s->region); free(s->spec_to_pml);
   free(s->pml_kx); free(s->pml_kz); free(s->pml_dx); free(s->pml_dz); free(s->pml_ax); free(s->pml_az);
   free(s->pml_pts_el); free(s->pml_pts_ac);
@@ -2021,6 +2109,11 @@ int orc_set_attenuation(void *h, const double *qkappa, const double *qmu, int ns
   s->old_xx = (REAL *)calloc(ne, sizeof(REAL));
   s->old_zz = (REAL *)calloc(ne, sizeof(REAL));
   s->old_xz = (REAL *)calloc(ne, sizeof(REAL));
+  REAL **par[] = {&s->att_itau1, &s->att_itau2, &s->att_te1, &s->att_te2, &s->att_phi1, &s->att_phi2};
+  for (int k = 0; k < 6; k++) *par[k] = (REAL *)calloc(ne * nsls, sizeof(REAL));
+  s->att_mu1 = (REAL *)calloc(ne, sizeof(REAL));
+  s->att_mu2 = (REAL *)calloc(ne, sizeof(REAL));
+  s->visco_pml = (char *)calloc((size_t)s->nspec + 1, 1);
   const double fmin = f0 / 10.0, fmax = f0 * 10.0;
   double te[2][16], ts[2][16];
   for (size_t l = 0; l < ne; l++) {
@@ -2044,11 +2137,67 @@ int orc_set_attenuation(void *h, const double *qkappa, const double *qmu, int ns
         REAL coef = (REAL)((1.0 - (double)temp) / (double)inv_tau);
         A[l * nsls + k] = temp;
         B[l * nsls + k] = phi * coef;
+        (mode ? s->att_itau2 : s->att_itau1)[l * nsls + k] = inv_tau;
+        (mode ? s->att_te2 : s->att_te1)[l * nsls + k] = (REAL)te[mode][k];
+        (mode ? s->att_phi2 : s->att_phi1)[l * nsls + k] = phi;
       }
+      (mode ? s->att_mu2 : s->att_mu1)[l] = (REAL)(ratio / (double)nsls);
+    }
+  }
+  /* CPML elements whose quality factors are all finite take the viscoelastic layer formulation; the reference
+     decides per element from the largest Qkappa and Qmu (compute_forces_viscoelastic.F90:178-181, 249-252) */
+  if (s->pml_on) {
+    const size_t np = (size_t)s->nspec_pml * NGLL2 * nsls;
+    REAL **mv[] = {&s->vp_ka_xx, &s->vp_ka_zz, &s->vp_mu_xx, &s->vp_mu_zz, &s->vp_mu_xz, &s->vp_mu_zx};
+    for (int k = 0; k < 6; k++) *mv[k] = (REAL *)calloc(np + 1, sizeof(REAL));
+    for (int e = 0; e < s->nspec; e++) {
+      if (!s->is_pml[e] || s->is_acoustic[e]) continue;
+      double qk = 0.0, qm = 0.0;
+      for (int p = 0; p < NGLL2; p++) {
+        qk = (qkappa[(size_t)e * NGLL2 + p] > qk ? qkappa[(size_t)e * NGLL2 + p] : qk);
+        qm = (qmu[(size_t)e * NGLL2 + p] > qm ? qmu[(size_t)e * NGLL2 + p] : qm);
+      }
+      s->visco_pml[e] = qk < 9998.999 && qm < 9998.999;
     }
   }
   s->att = 1;
   return 0;
+}
+
+/* coefficients of the stretched gradient with a standard linear solid (pml_compute.f90:173-280,
+   lik_parameter_computation_viscoelastic): A_0, A_1, A_2 and the Zener term for relaxation rate itau and
+   tau_epsilon te; index_ik 13 or 31 as in pml_lik */
+static void pml_lik_visco(double kx, double bx, double ax, double kz, double bz, double az, int region, int index_ik,
+                          double itau, double te, double A[4]) {
+  const int xonly = index_ik == 13 ? 1 : 2, zonly = index_ik == 13 ? 2 : 1;
+  if (region == 3) {
+    const double A0 = kx / kz;
+    /* the partial fractions of (s + beta_x)(s + alpha_z) / ((s + alpha_x)(s + beta_z)): poles alpha_x, beta_z,
+       residue terms from decompose_rational_fraction_PML (pml_compute.f90:578-610) */
+    const double al[2] = {ax, bz};
+    const double be[2] = {(ax * bz + ax * ax + 2.0 * bx * az - 2.0 * ax * (bx + az)) / (bz - ax),
+                          (ax * bz + bz * bz + 2.0 * bx * az - 2.0 * bz * (bx + az)) / (ax - bz)};
+    double ga[2], et[2];
+    for (int i = 0; i < 2; i++) {
+      ga[i] = (be[i] * (itau - al[i] * te * itau) + al[i] * al[i] * (te * itau - 1.0)) / (itau - al[i]);
+      et[i] = itau * (be[i] - al[i]) / (itau - al[i]);
+    }
+    A[0] = A0;
+    A[1] = A0 / 2.0 * (ga[0] - al[0]);
+    A[2] = A0 / 2.0 * (ga[1] - al[1]);
+    A[3] = A0 * (te * itau - 1.0) * ((et[0] + et[1]) / 2.0 - itau);
+  } else if (region == xonly || region == zonly) {
+    const int x = region == xonly;
+    const double A0 = x ? kx : 1.0 / kz, al = x ? ax : bz, be = x ? bx : az;
+    const double ga = (be * (itau - al * te * itau) + al * al * (te * itau - 1.0)) / (itau - al);
+    const double et = itau * (be - al) / (itau - al);
+    A[0] = A0;
+    A[1] = x ? A0 * (ga - al) : 0.0;
+    A[2] = x ? 0.0 : A0 * (ga - al);
+    A[3] = A0 * (te * itau - 1.0) * (et - itau);
+  } else {
+    A[0] = A[1] = A[2] = A[3] = 0.0;
+  }
 }
 
 int orc_real_size(void) { return (int)sizeof(REAL); }

@@ -176,3 +176,26 @@ def test_recursive_convolution_is_second_order():
     print("convolution errors", errs)
     assert errs[0] / errs[1] > 3.5 and errs[1] / errs[2] > 3.5
     assert errs[2] < 1e-4 * abs(exact)
+
+
+def test_viscoelastic_cpml_example_full_length(golden_dir):
+    """The viscoelastic CPML (pml_compute_memory_variables.f90:546-894, the Zener convolutions of the stretched
+    gradients, compute_forces_viscoelastic.F90:545-556): ``check_absolute_amplitude_of_force_source_seismograms_
+    viscoelastic`` (Qkappa 22.3, Qmu 20 everywhere, a 3-element CPML on every side) over its whole trace, set up
+    from its own input files; then two planted mistakes: the elastic layer formulation in the attenuating layer
+    elements, and the Zener terms of the layer dropped -- each moves the trace beyond the bar."""
+    cfg, _ = io.config_from_par_file(os.path.join(golden_dir, "examples", "force_source_viscoelastic"))
+    assert cfg.pml and cfg.attenuation and cfg.nelem_pml == 3
+    g = np.load(f"{golden_dir}/force_source_viscoelastic_ref_seis.npz")
+    errs = _peak_err(OracleSim(cfg, "sp").forward(), g, ("BXX", "BXZ"))
+    print(f"viscoelastic CPML example: max |oracle - REF_SEIS| / peak over {cfg.nstep} steps = "
+          f"{errs[0]:.2e}, {errs[1]:.2e}")
+    assert max(errs) < BAR
+    for mistake in (6, 7):
+        O.set_pml_mistake(mistake)
+        try:
+            wrong = _peak_err(OracleSim(cfg, "sp").forward(), g, ("BXX", "BXZ"))
+        finally:
+            O.set_pml_mistake(0)
+        print(f"mistake {mistake}: {wrong[0]:.2e} {wrong[1]:.2e}")
+        assert max(wrong) > 3 * BAR
