@@ -212,6 +212,24 @@ def fwi_gradient_batched(nshot: int = 32, repeats: int = 2, nstep: int = 1500):
             "note": "64 shots = 2 batches of 32 (checkpoint memory)"}
 
 
+def batch_forward(nshot: int, nstep: int, shots_per_wave: int = 0):
+    """A batched forward of the multi-shot FWI workload's mesh (600 x 300 elements, random-field model): nshot
+    shots along the surface, nstep steps from rest, device time (profiling of the batched force kernel)."""
+    from specfem2d_amd import config
+    from specfem2d_amd.driver import ShotBatch
+    cfg = config.batched_fwi_600x300(nstep=nstep, true_model=True)
+    b = ShotBatch(cfg, config.surface_shots(cfg, nshot))
+    if shots_per_wave:
+        b.set_shots_per_wave(shots_per_wave)
+    b.forward()  # warm-up
+    seis, ms = b.forward(timing=True)
+    out = {"workload": "batched_forward_600x300", "shots": nshot, "nstep": nstep, "nglob": b.nglob,
+           "shots_per_wave": shots_per_wave, "ms_per_step": ms / nstep, "us_per_shot_step": 1e3 * ms / nstep / nshot,
+           "finite": bool(np.all(np.isfinite(seis)))}
+    b.close()
+    return out
+
+
 def read_traffic(name: str):
     """HBM bytes per force launch (FETCH_SIZE + WRITE_SIZE, calibrated against a plain copy) from the
     committed rocprofv3 PMC summary of this workload, if present."""
@@ -365,6 +383,9 @@ def main():
                     help="only the coupled fluid-solid C2 forward with Stacey edges (profiling); prints its JSON")
     ap.add_argument("--elastic-only", action="store_true",
                     help="only the all-elastic C2 forward and its force-kernel roofline (profiling)")
+    ap.add_argument("--batch-forward-only", type=int, default=0, metavar="SHOTS",
+                    help="only a batched forward of the 600x300 mesh with this many shots (profiling)")
+    ap.add_argument("--shots-per-wave", type=int, default=0, help="shots per wave of --batch-forward-only (0: default)")
     ap.add_argument("--headline-only", action="store_true",
                     help="only the default workload (profiling); prints its JSON line without the other legs")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -385,6 +406,9 @@ def main():
 
     if args.fwi_only:
         print(json.dumps(fwi_gradient_batched(args.batch_shots, nstep=args.fwi_steps)))
+        return
+    if args.batch_forward_only:
+        print(json.dumps(batch_forward(args.batch_forward_only, args.steps, args.shots_per_wave)))
         return
     if args.attenuation_only:
         print(json.dumps(c2_attenuation(min(args.steps, 50), min(args.warmup, 5))))

@@ -67,6 +67,17 @@ void* sf2d_sim_create(const sf2d_params* p, const double* rho, const double* vp,
    time shifts are those of a run with only its sources).  On the device all shots advance together
    (sf2d_batch_* of specfem2d_gpu_ext.h); the gradient calls below then take obs[nshot][nrec][2][NSTEP],
    return the misfit summed over the shots and the gradient of that sum. */
+/* the same for a mesh read from files (read_external_mesh, src/meshfem2D/read_external_mesh_files.F90): nspec 4-node
+   elements over npgeo control nodes (knods [nspec][4] 0-based, counter-clockwise; coorg [npgeo][2]), the sides of the
+   free-surface file (only those of fluid elements count) and of the absorbing file (Stacey edges when p->stacey) as
+   0-based element and side code (1 bottom, 2 right, 3 top, 4 left); p's internal-mesher fields (nx, nz, layers,
+   absorb flags) are unused and CPML layers are refused.  The GLL points are numbered by coordinates, then by first
+   touch in element order (createnum_fast.f90, get_global.f90). */
+void* sf2d_sim_create_external(const sf2d_params* p, int nspec, int npgeo, const int* knods, const double* coorg,
+                               int nfree, const int* free_elem, const int* free_code, int nabs, const int* abs_elem,
+                               const int* abs_code, const double* rho, const double* vp, const double* vs, int nsrc,
+                               const sf2d_source* src, int nrec, const double* rec_xz, int simulation_type,
+                               int save_forward);
 void* sf2d_sim_create_shots(const sf2d_params* p, const double* rho, const double* vp, const double* vs, int nshot,
                             const int* shot_nsrc, const sf2d_source* src, int nrec, const double* rec_xz);
 /* all NSTEP steps of every shot from rest; seis[nshot][nrec][2][NSTEP] displacement; timing[0] = device ms */

@@ -56,7 +56,7 @@ void sf2d_mesh_set_shots_per_wave(long* Mesh_pointer, int shots);
    acoustic elements.  The reference's GPU set-up prepare_pml_device_ carries acoustic layers only (K = 1, one d0),
    and its Fortran stops on elastic ones (prepare_gpu.f90:74).  Call after prepare_fields_elastic_device_ and
    prepare_fields_acoustic_device_, before the first time step, on a handle prepared with PML_BOUNDARY_CONDITIONS
-   (forward runs; no attenuation, shot batches or gradients).
+   (forward runs; no shot batches or gradients; with viscoelastic attenuation see prepare_cpml_attenuation_device_).
      region_CPML[nspec]: 0 outside the layer, 1 CPML_X_ONLY, 2 CPML_Z_ONLY, 3 CPML_XZ;
      spec_to_PML[nspec]: 1-based row of the element in the profile arrays (0 outside);
      K_x_store .. alpha_z_store: double [NSPEC_PML][5][5] (i fastest), as pml_init.F90 leaves them (after the
@@ -70,6 +70,17 @@ void prepare_cpml_device_(long* Mesh_pointer, int* NSPEC_PML, int* region_CPML, 
                           double* K_z_store, double* d_x_store, double* d_z_store, double* alpha_x_store,
                           double* alpha_z_store, float* rhostore, double* DT, int* nglob_abs_elastic,
                           int* abs_points_elastic, int* nglob_abs_acoustic, int* abs_points_acoustic);
+/* the viscoelastic layer (pml_compute_memory_variables.f90:546-894, compute_forces_viscoelastic.F90:545-556): on a
+   handle with ATTENUATION_VISCOELASTIC and a CPML layer, call after prepare_cpml_device_ and before the first time
+   step with the Zener parameters the CPU path keeps (attenuation_model.f90:145-161), in its layout:
+   inv_tau_sigma_nu1/2, phi_nu1/2, tau_epsilon_nu1/2 (NGLLX, NGLLZ, nspec, N_SLS), Mu_nu1/2 and the quality factors
+   qkappa/qmu_attenuation_store (NGLLX, NGLLZ, nspec); N_SLS = 3.  Solid layer elements whose Qkappa and Qmu are below
+   9999 at every point take the viscoelastic formulation, the others the elastic one (the reference's element rule).
+   The time loop refuses to run a layer with attenuation without this call. */
+void prepare_cpml_attenuation_device_(long* Mesh_pointer, int* N_SLS, float* inv_tau_sigma_nu1, float* phi_nu1,
+                                      float* tau_epsilon_nu1, float* Mu_nu1, float* inv_tau_sigma_nu2, float* phi_nu2,
+                                      float* tau_epsilon_nu2, float* Mu_nu2, float* qkappa_attenuation_store,
+                                      float* qmu_attenuation_store);
 /* the elastic twin of pml_boundary_acoustic_cuda_ (pml_boundary_elastic, pml_compute.f90:565-605): displ, veloc
    and accel are zero on the layer's outer boundary; the CPU path calls it after the Stacey step of iphase 1
    (compute_forces_viscoelastic_calling_routine.F90:74-77) */

@@ -113,10 +113,20 @@ struct Sim {
   // viscoelastic attenuation: convolution coefficients per point and solid, [nspec][25][3] (solid fastest)
   int attenuation = 0;
   std::vector<float> att_A_kappa, att_B_kappa, att_A_mu, att_B_mu;
+  // the Zener parameters in the CPU path's layout (the viscoelastic CPML layer): 1/tau_sigma, tau_epsilon, phi
+  // [3][nspec][25] (solid slowest) and Mu [nspec][25] per mode (1 bulk, 2 shear); the quality factors per point
+  std::vector<float> zen_itau1, zen_te1, zen_phi1, zen_mu1, zen_itau2, zen_te2, zen_phi2, zen_mu2, zen_qk, zen_qm;
   int simulation_type = 1, save_forward = 0;
   Basis basis{};
   // control nodes of the structured grid
   std::vector<double> gx, gz;  // [(nz+1)][(nx+1)]
+  // an external mesh (read_external_mesh): 4-node elements (0-based, counter-clockwise) over control nodes, and the
+  // free-surface and absorbing sides of its files as (0-based element, side code 1 bottom 2 right 3 top 4 left)
+  bool external = false;
+  int npgeo = 0;
+  std::vector<int> knods;
+  std::vector<double> coorg;
+  std::vector<int> ext_free_elem, ext_free_code, ext_abs_elem, ext_abs_code;
   std::vector<int> ibool;      // 1-based [nspec][25]
   std::vector<double> coord;   // [nglob][2]
   std::vector<float> xix, xiz, gammax, gammaz, jacobian, kappa, mu, rho_vp, rho_vs, rho;
@@ -179,17 +189,26 @@ struct Sim {
 // bilinear map of element e (src/shared/define_shape_functions.f90 NGNOD=4, recompute_jacobian.f90)
 void map_point(const Sim& s, int e, double xi, double ga, double* x, double* z, double* dxdxi, double* dzdxi,
                double* dxdga, double* dzdga) {
-  const int nx = s.p.nx;
-  const int ex = e % nx, ez = e / nx;
-  const int n0 = ez * (nx + 1) + ex, n1 = n0 + 1, n2 = n0 + (nx + 1) + 1, n3 = n0 + (nx + 1);
-  const int nodes[4] = {n0, n1, n2, n3};
+  int nodes[4];
+  if (s.external) {
+    for (int k = 0; k < 4; k++) nodes[k] = s.knods[(size_t)e * 4 + k];
+  } else {
+    const int nx = s.p.nx;
+    const int ex = e % nx, ez = e / nx;
+    nodes[0] = ez * (nx + 1) + ex;
+    nodes[1] = nodes[0] + 1;
+    nodes[2] = nodes[0] + (nx + 1) + 1;
+    nodes[3] = nodes[0] + (nx + 1);
+  }
+  const std::vector<double>& cx = s.external ? s.coorg : s.gx;
   const double sp = xi + 1.0, sm = xi - 1.0, tp = ga + 1.0, tm = ga - 1.0;
   const double sh[4] = {0.25 * sm * tm, -0.25 * sp * tm, 0.25 * sp * tp, -0.25 * sm * tp};
   const double dxi[4] = {0.25 * tm, -0.25 * tm, 0.25 * tp, -0.25 * tp};
   const double dga[4] = {0.25 * sm, -0.25 * sp, 0.25 * sp, -0.25 * sm};
   double X = 0, Z = 0, a = 0, b = 0, c = 0, d = 0;
   for (int k = 0; k < 4; k++) {
-    const double xe = s.gx[nodes[k]], ze = s.gz[nodes[k]];
+    const double xe = s.external ? cx[2 * (size_t)nodes[k]] : s.gx[nodes[k]];
+    const double ze = s.external ? cx[2 * (size_t)nodes[k] + 1] : s.gz[nodes[k]];
     X += sh[k] * xe;
     Z += sh[k] * ze;
     a += dxi[k] * xe;
@@ -205,8 +224,87 @@ void map_point(const Sim& s, int e, double xi, double ga, double* x, double* z, 
   *dzdga = d;
 }
 
+// numbering of an external mesh: the GLL points fall into classes of equal coordinates (to 1e-10 of the mesh extent:
+// sorted by x, runs of close x sorted by z), then numbered by first touch in element order, i fastest
+// (createnum_fast.f90, get_global.f90:34-72)
+void number_by_coordinates(Sim& s) {
+  const size_t ntot = (size_t)s.nspec * NGLL2;
+  std::vector<double> xp(2 * ntot);
+  double xmin = 1e300, xmax = -1e300, zmin = 1e300, zmax = -1e300;
+  for (int e = 0; e < s.nspec; e++)
+    for (int j = 0; j < NGLL; j++)
+      for (int i = 0; i < NGLL; i++) {
+        double x, z, a, b, c, d;
+        map_point(s, e, s.basis.xi[i], s.basis.xi[j], &x, &z, &a, &b, &c, &d);
+        const size_t l = (size_t)e * NGLL2 + j * NGLL + i;
+        xp[2 * l] = x;
+        xp[2 * l + 1] = z;
+        xmin = std::min(xmin, x); xmax = std::max(xmax, x);
+        zmin = std::min(zmin, z); zmax = std::max(zmax, z);
+      }
+  const double tol = 1e-10 * std::max(xmax - xmin, zmax - zmin);
+  std::vector<int> idx(ntot), cls(ntot);
+  for (size_t l = 0; l < ntot; l++) idx[l] = (int)l;
+  std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return xp[2 * (size_t)a] < xp[2 * (size_t)b]; });
+  int ncls = 0;
+  for (size_t a = 0; a < ntot;) {
+    size_t b = a + 1;
+    while (b < ntot && xp[2 * (size_t)idx[b]] - xp[2 * (size_t)idx[b - 1]] <= tol) b++;
+    std::stable_sort(idx.begin() + a, idx.begin() + b,
+                     [&](int u, int v) { return xp[2 * (size_t)u + 1] < xp[2 * (size_t)v + 1]; });
+    for (size_t c = a; c < b;) {
+      size_t d = c + 1;
+      while (d < b && xp[2 * (size_t)idx[d] + 1] - xp[2 * (size_t)idx[d - 1] + 1] <= tol) d++;
+      for (size_t k = c; k < d; k++) cls[idx[k]] = ncls;
+      ncls++;
+      c = d;
+    }
+    a = b;
+  }
+  std::vector<int> mask(ncls, 0);
+  s.ibool.resize(ntot);
+  int next = 0;
+  for (size_t l = 0; l < ntot; l++) {
+    if (mask[cls[l]] == 0) mask[cls[l]] = ++next;
+    s.ibool[l] = mask[cls[l]];
+  }
+  s.nglob = next;
+}
+
+// geometry at the GLL points (setup_mesh.F90:195-236)
+bool build_geometry(Sim& s) {
+  const size_t ne = (size_t)s.nspec * NGLL2;
+  s.coord.assign((size_t)2 * s.nglob, 0.0);
+  s.xix.resize(ne); s.xiz.resize(ne); s.gammax.resize(ne); s.gammaz.resize(ne); s.jacobian.resize(ne);
+  for (int e = 0; e < s.nspec; e++)
+    for (int j = 0; j < NGLL; j++)
+      for (int i = 0; i < NGLL; i++) {
+        double x, z, xxi, zxi, xga, zga;
+        map_point(s, e, s.basis.xi[i], s.basis.xi[j], &x, &z, &xxi, &zxi, &xga, &zga);
+        const double jac = xxi * zga - xga * zxi;
+        if (!(jac > 0.0)) return false;
+        const size_t l = (size_t)e * NGLL2 + j * NGLL + i;
+        s.xix[l] = (float)(zga / jac);
+        s.gammax[l] = (float)(-zxi / jac);
+        s.xiz[l] = (float)(-xga / jac);
+        s.gammaz[l] = (float)(xxi / jac);
+        s.jacobian[l] = (float)jac;
+        const int g = s.ibool[l] - 1;
+        s.coord[2 * g] = x;
+        s.coord[2 * g + 1] = z;
+      }
+  return true;
+}
+
 bool build_mesh(Sim& s) {
   const sf2d_params& p = s.p;
+  if (s.external) {
+    if (s.nspec < 1 || s.npgeo < 4) return false;
+    for (int n : s.knods)
+      if (n < 0 || n >= s.npgeo) return false;
+    number_by_coordinates(s);
+    return build_geometry(s);
+  }
   if (p.nx < 1 || p.nz < 1 || p.nlayers < 1 || p.nlayers > SF2D_MAXLAYERS) return false;
   int nzsum = 0;
   for (int l = 0; l < p.nlayers; l++) nzsum += p.nz_layer[l];
@@ -243,28 +341,7 @@ bool build_mesh(Sim& s) {
       }
   }
   s.nglob = next;
-  // geometry at the GLL points (setup_mesh.F90:195-236)
-  const size_t ne = (size_t)s.nspec * NGLL2;
-  s.coord.assign((size_t)2 * s.nglob, 0.0);
-  s.xix.resize(ne); s.xiz.resize(ne); s.gammax.resize(ne); s.gammaz.resize(ne); s.jacobian.resize(ne);
-  for (int e = 0; e < s.nspec; e++)
-    for (int j = 0; j < NGLL; j++)
-      for (int i = 0; i < NGLL; i++) {
-        double x, z, xxi, zxi, xga, zga;
-        map_point(s, e, s.basis.xi[i], s.basis.xi[j], &x, &z, &xxi, &zxi, &xga, &zga);
-        const double jac = xxi * zga - xga * zxi;
-        if (!(jac > 0.0)) return false;
-        const size_t l = (size_t)e * NGLL2 + j * NGLL + i;
-        s.xix[l] = (float)(zga / jac);
-        s.gammax[l] = (float)(-zxi / jac);
-        s.xiz[l] = (float)(-xga / jac);
-        s.gammaz[l] = (float)(xxi / jac);
-        s.jacobian[l] = (float)jac;
-        const int g = s.ibool[l] - 1;
-        s.coord[2 * g] = x;
-        s.coord[2 * g + 1] = z;
-      }
-  return true;
+  return build_geometry(s);
 }
 
 // materials (read_materials.f90:138-141, setup_mesh.F90:680-792)
@@ -328,6 +405,66 @@ void build_domains(Sim& s) {
       s.free_ij.push_back(j + 1);
     }
   };
+  auto add_coupling = [&](int e, int code, int f, int fcode) {
+    s.coup_ac_ispec.push_back(e + 1);
+    s.coup_ac_code.push_back(code);
+    s.coup_el_ispec.push_back(f + 1);
+    s.coup_el_code.push_back(fcode);
+    for (int k = 0; k < NGLL; k++) {
+      int i, j;
+      ivalue(code, k, &i, &j);
+      const size_t l = (size_t)e * NGLL2 + j * NGLL + i;
+      float nxv, nzv, j1, w;
+      if (code == 1 || code == 3) {
+        const float xx = +s.gammaz[l] * s.jacobian[l];
+        const float zx = -s.gammax[l] * s.jacobian[l];
+        j1 = std::sqrt(xx * xx + zx * zx);
+        nxv = (code == 3 ? -zx : zx) / j1;
+        nzv = (code == 3 ? xx : -xx) / j1;
+        w = j1 * s.basis.w[i];
+      } else {
+        const float xg = -s.xiz[l] * s.jacobian[l];
+        const float zg = +s.xix[l] * s.jacobian[l];
+        j1 = std::sqrt(xg * xg + zg * zg);
+        nxv = (code == 4 ? -zg : zg) / j1;
+        nzv = (code == 4 ? xg : -xg) / j1;
+        w = j1 * s.basis.w[j];
+      }
+      s.coup_ij.push_back(i + 1);
+      s.coup_ij.push_back(j + 1);
+      s.coup_normal.push_back(nxv);
+      s.coup_normal.push_back(nzv);
+      s.coup_jacw.push_back(w);
+    }
+  };
+  if (s.external) {
+    // the free-surface file's sides of fluid elements (read_external_mesh_files.F90:398-416); the fluid-solid
+    // edges: each side of a fluid element (in element order, sides 1..4) shared with a solid element, found by
+    // the side's sorted pair of control nodes
+    for (size_t k = 0; k < s.ext_free_elem.size(); k++) add_free(s.ext_free_elem[k], s.ext_free_code[k]);
+    if (s.nspec_ac == s.nspec) return;
+    std::vector<std::pair<long, int>> sides((size_t)4 * s.nspec);
+    for (int e = 0; e < s.nspec; e++)
+      for (int c = 1; c <= 4; c++) {
+        const long a = s.knods[(size_t)e * 4 + c - 1], b = s.knods[(size_t)e * 4 + c % 4];
+        sides[(size_t)e * 4 + c - 1] = {std::min(a, b) * (long)s.npgeo + std::max(a, b), e * 4 + c - 1};
+      }
+    std::sort(sides.begin(), sides.end());
+    for (int e = 0; e < s.nspec; e++) {
+      if (!s.is_acoustic[e]) continue;
+      for (int c = 1; c <= 4; c++) {
+        const long a = s.knods[(size_t)e * 4 + c - 1], b = s.knods[(size_t)e * 4 + c % 4];
+        const long key = std::min(a, b) * (long)s.npgeo + std::max(a, b);
+        for (auto it = std::lower_bound(sides.begin(), sides.end(), std::make_pair(key, -1));
+             it != sides.end() && it->first == key; ++it) {
+          const int f = it->second / 4;
+          if (f == e || s.is_acoustic[f]) continue;
+          add_coupling(e, c, f, it->second % 4 + 1);
+        }
+      }
+    }
+    return;
+  }
   if (!s.p.absorbtop) for (int i = 0; i < nx; i++) add_free((nz - 1) * nx + i, 3);
   if (!s.p.absorbbottom) for (int i = 0; i < nx; i++) add_free(i, 1);
   if (!s.p.absorbleft) for (int j = 0; j < nz; j++) add_free(j * nx, 4);
@@ -339,37 +476,7 @@ void build_domains(Sim& s) {
     const int ac_code[4] = {1, 4, 2, 3}, el_code[4] = {3, 2, 4, 1};
     for (int q = 0; q < 4; q++) {
       if (nb[q] < 0 || s.is_acoustic[nb[q]]) continue;
-      const int code = ac_code[q];
-      s.coup_ac_ispec.push_back(e + 1);
-      s.coup_ac_code.push_back(code);
-      s.coup_el_ispec.push_back(nb[q] + 1);
-      s.coup_el_code.push_back(el_code[q]);
-      for (int k = 0; k < NGLL; k++) {
-        int i, j;
-        ivalue(code, k, &i, &j);
-        const size_t l = (size_t)e * NGLL2 + j * NGLL + i;
-        float nxv, nzv, j1, w;
-        if (code == 1 || code == 3) {
-          const float xx = +s.gammaz[l] * s.jacobian[l];
-          const float zx = -s.gammax[l] * s.jacobian[l];
-          j1 = std::sqrt(xx * xx + zx * zx);
-          nxv = (code == 3 ? -zx : zx) / j1;
-          nzv = (code == 3 ? xx : -xx) / j1;
-          w = j1 * s.basis.w[i];
-        } else {
-          const float xg = -s.xiz[l] * s.jacobian[l];
-          const float zg = +s.xix[l] * s.jacobian[l];
-          j1 = std::sqrt(xg * xg + zg * zg);
-          nxv = (code == 4 ? -zg : zg) / j1;
-          nzv = (code == 4 ? xg : -xg) / j1;
-          w = j1 * s.basis.w[j];
-        }
-        s.coup_ij.push_back(i + 1);
-        s.coup_ij.push_back(j + 1);
-        s.coup_normal.push_back(nxv);
-        s.coup_normal.push_back(nzv);
-        s.coup_jacw.push_back(w);
-      }
+      add_coupling(e, ac_code[q], nb[q], el_code[q]);
     }
   }
 }
@@ -380,7 +487,11 @@ void build_stacey(Sim& s) {
   s.nabs = 0;
   if (!p.stacey && !p.pml) return;  // the CPML layer grows from the absorbing edges too
   std::vector<int> el, code;
-  for (int iz = 1; iz <= p.nz; iz++)
+  for (size_t k = 0; s.external && k < s.ext_abs_elem.size(); k++) {  // the absorbing file's sides, in its order
+    el.push_back(s.ext_abs_elem[k] + 1);
+    code.push_back(s.ext_abs_code[k]);
+  }
+  for (int iz = 1; iz <= p.nz && !s.external; iz++)
     for (int ix = 1; ix <= p.nx; ix++) {
       const int e = (iz - 1) * p.nx + ix;
       if (p.absorbbottom && iz == 1) { el.push_back(e); code.push_back(1); }
@@ -740,7 +851,42 @@ void build_mass(Sim& s) {
 
 // point location: closest GLL point (strict <, scan order ispec,j,i), then Newton from the selected element and
 // every element sharing one of its corners (locate_receivers.F90:148-340, locate_source.F90)
+void locate_newton(const Sim& s, double px, double pz, int isel, int ixg, int izg, std::vector<int>& cand,
+                   int* ispec_out, double* xi_out, double* ga_out);
+
+// an external mesh: the nearest GLL point over all elements (first in element order), then the elements sharing a
+// corner with its element, in element order (locate_receivers.F90:197-329)
+void locate_external(const Sim& s, double px, double pz, int* ispec_out, double* xi_out, double* ga_out) {
+  double dmin = std::numeric_limits<double>::infinity();
+  int isel = 0, ixg = 0, izg = 0;
+  for (int e = 0; e < s.nspec; e++)
+    for (int j = 0; j < NGLL; j++)
+      for (int i = 0; i < NGLL; i++) {
+        const int g = s.ibool[(size_t)e * NGLL2 + j * NGLL + i] - 1;
+        const double dx = px - s.coord[2 * (size_t)g], dz = pz - s.coord[2 * (size_t)g + 1];
+        const double d2 = dx * dx + dz * dz;
+        if (d2 < dmin) { dmin = d2; isel = e; ixg = i; izg = j; }
+      }
+  const int* ib = &s.ibool[(size_t)isel * NGLL2];
+  const int corners[4] = {ib[0], ib[NGLL - 1], ib[NGLL2 - 1], ib[(NGLL - 1) * NGLL]};
+  std::vector<int> cand{isel};
+  for (int e = 0; e < s.nspec; e++) {
+    if (e == isel) continue;
+    const int* ie = &s.ibool[(size_t)e * NGLL2];
+    const int ec[4] = {ie[0], ie[NGLL - 1], ie[NGLL2 - 1], ie[(NGLL - 1) * NGLL]};
+    bool hit = false;
+    for (int a = 0; a < 4 && !hit; a++)
+      for (int b = 0; b < 4; b++) hit = hit || ec[a] == corners[b];
+    if (hit) cand.push_back(e);
+  }
+  locate_newton(s, px, pz, isel, ixg, izg, cand, ispec_out, xi_out, ga_out);
+}
+
 void locate(const Sim& s, double px, double pz, int* ispec_out, double* xi_out, double* ga_out) {
+  if (s.external) {
+    locate_external(s, px, pz, ispec_out, xi_out, ga_out);
+    return;
+  }
   const sf2d_params& p = s.p;
   // lattice lines of the GLL points: x along columns, z along rows
   const long lx = 4L * p.nx + 1, lz = 4L * p.nz + 1;
@@ -801,6 +947,14 @@ void locate(const Sim& s, double px, double pz, int* ispec_out, double* xi_out, 
       }
     std::sort(cand.begin() + 1, cand.end());
   }
+  locate_newton(s, px, pz, isel, ixg, izg, cand, ispec_out, xi_out, ga_out);
+}
+
+// Newton iterations from the nearest GLL point in the first candidate and from the element centre in the others;
+// the closest result wins
+void locate_newton(const Sim& s, double px, double pz, int isel, int ixg, int izg, std::vector<int>& cand,
+                   int* ispec_out, double* xi_out, double* ga_out) {
+  (void)isel;
   double best = std::numeric_limits<double>::infinity();
   for (size_t c = 0; c < cand.size(); c++) {
     if (c > 0) { ixg = 1; izg = 1; }  // int(NGLLX/2.0) = 2 in the reference's 1-based indexing
@@ -1051,6 +1205,12 @@ void prepare_device(Sim& s) {
     prepare_cpml_device_(&s.mesh, &npml, s.region_cpml.data(), s.spec_to_pml.data(), s.pml_kx.data(), s.pml_kz.data(),
                          s.pml_dx.data(), s.pml_dz.data(), s.pml_ax.data(), s.pml_az.data(), s.rho.data(), &dt, &nel,
                          pe.data(), &nac_pts, pa.data());
+    if (s.attenuation) {
+      int nsls = 3;
+      prepare_cpml_attenuation_device_(&s.mesh, &nsls, s.zen_itau1.data(), s.zen_phi1.data(), s.zen_te1.data(),
+                                       s.zen_mu1.data(), s.zen_itau2.data(), s.zen_phi2.data(), s.zen_te2.data(),
+                                       s.zen_mu2.data(), s.zen_qk.data(), s.zen_qm.data());
+    }
   }
   std::vector<float> zeros((size_t)2 * s.nglob, 0.f);
   int size = 2 * s.nglob;
@@ -1116,8 +1276,10 @@ bool zener_relaxation(int n, double q, double fmin, double fmax, double* tau_eps
 }
 
 // the single-precision recursive-convolution coefficients of one mode (prepare_attenuation.f90:368-397 after
-// attenuation_model.f90:152-159): A = exp(-dt / (2 tau_sigma)), B = phi (1 - A) tau_sigma
-bool newmark_coefficients(double q, double f0, double dt, float* A, float* B) {
+// attenuation_model.f90:152-159): A = exp(-dt / (2 tau_sigma)), B = phi (1 - A) tau_sigma; and the model's
+// single-precision parameters 1/tau_sigma, tau_epsilon, phi and Mu = sum(tau_epsilon / tau_sigma) / 3
+// (attenuation_model.f90:145-161), in Z[10] = {itau[3], te[3], phi[3], Mu}
+bool newmark_coefficients(double q, double f0, double dt, float* A, float* B, float* Z) {
   double te[3], ts[3];
   if (!zener_relaxation(3, q, f0 / 10.0, f0 * 10.0, te, ts)) return false;
   double ratio = 0.0;
@@ -1129,13 +1291,27 @@ bool newmark_coefficients(double q, double f0, double dt, float* A, float* B) {
     const float c = (float)((1.0 - (double)a) / (double)inv_tau);
     A[k] = a;
     B[k] = phi * c;
+    Z[k] = inv_tau;
+    Z[3 + k] = (float)te[k];
+    Z[6 + k] = phi;
   }
+  Z[9] = (float)(ratio / 3.0);
   return true;
 }
 
+struct ExternalMeshIn {
+  int nspec, npgeo;
+  const int* knods;
+  const double* coorg;
+  int nfree;
+  const int *free_elem, *free_code;
+  int nabs;
+  const int *abs_elem, *abs_code;
+};
+
 void* create(const sf2d_params* p, const double* rho, const double* vp, const double* vs, int nsrc,
              const sf2d_source* src, int nrec, const double* rec_xz, int simulation_type, int save_forward, int nshot,
-             const int* shot_nsrc) {
+             const int* shot_nsrc, const ExternalMeshIn* ext = nullptr) {
   const int st = p->seismotype;
   if (st != 1 && st != 2 && st != 3 && st != 4 && st != 6) {
     fprintf(stderr, "sf2d_sim_create: seismotype %d is not supported (1, 2, 3, 4 or 6)\n", st);
@@ -1151,8 +1327,29 @@ void* create(const sf2d_params* p, const double* rho, const double* vp, const do
                     "SAVE_FORWARD, with NELEM_PML_THICKNESS >= 1, K_MIN_PML > 0 and positive damping factors\n");
     return nullptr;
   }
+  if (ext && p->pml) {
+    fprintf(stderr, "sf2d_sim_create_external: CPML layers of external meshes are not supported\n");
+    return nullptr;
+  }
   Sim* s = new Sim();
   s->p = *p;
+  if (ext) {
+    s->external = true;
+    s->nspec = ext->nspec;
+    s->npgeo = ext->npgeo;
+    s->knods.assign(ext->knods, ext->knods + (size_t)4 * ext->nspec);
+    s->coorg.assign(ext->coorg, ext->coorg + (size_t)2 * ext->npgeo);
+    s->ext_free_elem.assign(ext->free_elem, ext->free_elem + ext->nfree);
+    s->ext_free_code.assign(ext->free_code, ext->free_code + ext->nfree);
+    s->ext_abs_elem.assign(ext->abs_elem, ext->abs_elem + ext->nabs);
+    s->ext_abs_code.assign(ext->abs_code, ext->abs_code + ext->nabs);
+    for (int k = 0; k < ext->nfree; k++)
+      if (ext->free_elem[k] < 0 || ext->free_elem[k] >= ext->nspec || ext->free_code[k] < 1 || ext->free_code[k] > 4)
+        s->npgeo = -1;
+    for (int k = 0; k < ext->nabs; k++)
+      if (ext->abs_elem[k] < 0 || ext->abs_elem[k] >= ext->nspec || ext->abs_code[k] < 1 || ext->abs_code[k] > 4)
+        s->npgeo = -1;
+  }
   s->simulation_type = simulation_type;
   s->save_forward = save_forward;
   s->basis = make_basis();
@@ -1214,6 +1411,16 @@ void* sf2d_sim_create(const sf2d_params* p, const double* rho, const double* vp,
                       const sf2d_source* src, int nrec, const double* rec_xz, int simulation_type,
                       int save_forward) {
   return create(p, rho, vp, vs, nsrc, src, nrec, rec_xz, simulation_type, save_forward, 0, nullptr);
+}
+
+void* sf2d_sim_create_external(const sf2d_params* p, int nspec, int npgeo, const int* knods, const double* coorg,
+                               int nfree, const int* free_elem, const int* free_code, int nabs, const int* abs_elem,
+                               const int* abs_code, const double* rho, const double* vp, const double* vs, int nsrc,
+                               const sf2d_source* src, int nrec, const double* rec_xz, int simulation_type,
+                               int save_forward) {
+  if (!knods || !coorg || nspec < 1 || nfree < 0 || nabs < 0) return nullptr;
+  const ExternalMeshIn m{nspec, npgeo, knods, coorg, nfree, free_elem, free_code, nabs, abs_elem, abs_code};
+  return create(p, rho, vp, vs, nsrc, src, nrec, rec_xz, simulation_type, save_forward, 0, nullptr, &m);
 }
 
 void* sf2d_sim_create_shots(const sf2d_params* p, const double* rho, const double* vp, const double* vs, int nshot,
@@ -1739,33 +1946,48 @@ int sf2d_sim_gradient_backward(void* sim, double* grad, double* parts) {
 int sf2d_sim_set_attenuation(void* sim, const double* qkappa, const double* qmu, int n_sls) {
   Sim* s = (Sim*)sim;
   if (!s || s->mesh || !qkappa || !qmu) return 1;
-  if (n_sls != 3 || s->nspec_ac > 0 || s->simulation_type != 1 || s->nshot > 0 || s->src.empty() ||
-      s->nspec_pml > 0) {
-    fprintf(stderr, "sf2d_sim_set_attenuation: 3 solids, forward runs of solid meshes with a source and no CPML "
-                    "layer only\n");
+  if (n_sls != 3 || s->nspec_ac > 0 || s->simulation_type != 1 || s->nshot > 0 || s->src.empty()) {
+    fprintf(stderr, "sf2d_sim_set_attenuation: 3 solids, forward runs of solid meshes with a source only\n");
     return 1;
   }
   const double f0 = s->src[0].f0;  // ATTENUATION_f0_REFERENCE (prepare_attenuation.f90:319-322)
   const size_t ne = (size_t)s->nspec * NGLL2;
   for (auto* v : {&s->att_A_kappa, &s->att_B_kappa, &s->att_A_mu, &s->att_B_mu}) v->assign(3 * ne, 0.f);
-  std::map<double, std::array<float, 6>> cache;  // Q -> A[3], B[3]
+  // the CPU path's defaults at points without attenuation (prepare_attenuation.f90:285-289)
+  for (auto* v : {&s->zen_itau1, &s->zen_te1, &s->zen_itau2, &s->zen_te2}) v->assign(3 * ne, -1.f);
+  for (auto* v : {&s->zen_phi1, &s->zen_phi2}) v->assign(3 * ne, 0.f);
+  for (auto* v : {&s->zen_mu1, &s->zen_mu2}) v->assign(ne, 1.f);
+  s->zen_qk.assign(qkappa, qkappa + ne);
+  s->zen_qm.assign(qmu, qmu + ne);
+  std::map<double, std::array<float, 16>> cache;  // Q -> A[3], B[3], Z[10]
   auto last = cache.end();
-  auto coefficients = [&](double q, float* A, float* B) {
+  auto coefficients = [&](double q, float* A, float* B, size_t l, int mode) {
     auto it = (last != cache.end() && last->first == q) ? last : cache.find(q);
     if (it == cache.end()) {
-      std::array<float, 6> c{};
-      if (!newmark_coefficients(q, f0, s->p.dt, c.data(), c.data() + 3)) return false;
+      std::array<float, 16> c{};
+      if (!newmark_coefficients(q, f0, s->p.dt, c.data(), c.data() + 3, c.data() + 6)) return false;
       it = cache.emplace(q, c).first;
     }
     last = it;
-    std::copy(it->second.begin(), it->second.begin() + 3, A);
-    std::copy(it->second.begin() + 3, it->second.end(), B);
+    const float* c = it->second.data();
+    std::copy(c, c + 3, A);
+    std::copy(c + 3, c + 6, B);
+    std::vector<float>& itau = mode ? s->zen_itau2 : s->zen_itau1;
+    std::vector<float>& te = mode ? s->zen_te2 : s->zen_te1;
+    std::vector<float>& phi = mode ? s->zen_phi2 : s->zen_phi1;
+    for (int k = 0; k < 3; k++) {
+      itau[l + ne * k] = c[6 + k];
+      te[l + ne * k] = c[9 + k];
+      phi[l + ne * k] = c[12 + k];
+    }
+    (mode ? s->zen_mu2 : s->zen_mu1)[l] = c[15];
     return true;
   };
   for (size_t l = 0; l < ne; l++) {
     if (qkappa[l] > 9998.999 && qmu[l] > 9998.999) continue;  // no attenuation at this point: A = B = 0
-    if (!(qkappa[l] > 0.0 && qmu[l] > 0.0) || !coefficients(qkappa[l], &s->att_A_kappa[3 * l], &s->att_B_kappa[3 * l]) ||
-        !coefficients(qmu[l], &s->att_A_mu[3 * l], &s->att_B_mu[3 * l])) {
+    if (!(qkappa[l] > 0.0 && qmu[l] > 0.0) ||
+        !coefficients(qkappa[l], &s->att_A_kappa[3 * l], &s->att_B_kappa[3 * l], l, 0) ||
+        !coefficients(qmu[l], &s->att_A_mu[3 * l], &s->att_B_mu[3 * l], l, 1)) {
       fprintf(stderr, "sf2d_sim_set_attenuation: invalid quality factors at point %zu\n", l);
       return 1;
     }

@@ -38,6 +38,15 @@ constexpr int PML_EPB = 10;                // layer elements per 256-thread work
 constexpr unsigned int ON_EDGE = 1u << 31; // coefficient index flag: the point lies on the layer's outer boundary
 constexpr int SET_F4 = 6;                  // float4 words per coefficient set
 constexpr int EL_PLANES = 12, AC_PLANES = 6;
+// viscoelastic layer (3 standard linear solids): per point an index into a table of Zener coefficient sets (NO_ZENER:
+// the elastic layer formulation), 18 more memory-variable planes; a set is 11 float4:
+//   per solid k (3 float4): {A31k/Mk, A13k/Mk, A31m/Mm, A13m/Mm}, {ck_0, ck_1, ck_2, cm_0}, {cm_1, cm_2, phi_k, phi_m}
+//   then {A2k/Mk, A3k/Mk, A4k/Mk, A5k/Mk}, {A2m/Mm, A3m/Mm, A4m/Mm, A5m/Mm}
+// (A..k/m: the Zener terms of lik_parameter_computation_viscoelastic for the bulk and shear relaxation, divided by
+// Mu = N_SLS Mu_nu; ck, cm: the recursive-convolution coefficients of 1/tau_sigma; A2..A5: the sums over the solids
+// of the stretched-gradient coefficients)
+constexpr int ZSLS = 3, ZSET_F4 = 3 * ZSLS + 2, ZPLANES = 6 * ZSLS;
+constexpr unsigned int NO_ZENER = 0xffffffffu;
 
 // One coefficient set (6 float4): the stretched-gradient coefficients A0..A2 of the x derivatives (index 31) and of
 // the z derivatives (index 13), A1..A4 of the acceleration's damping terms, and the recursive-convolution
@@ -70,11 +79,16 @@ __device__ __forceinline__ float conv(const float c[3], float r, float f, float 
 // Solid layer elements of one colour, first .. first+n-1 of the packed layer order: accel -= the element's CPML
 // force (compute_forces_viscoelastic.F90:247-270, 596-611 with pml_compute_memory_variables.f90:230-540 and
 // pml_compute_accel_contribution.f90:187-333; Newmark, not rotated).
+// VISCO: points with a Zener set take the viscoelastic layer formulation (pml_compute_memory_variables.f90:546-894,
+// compute_forces_viscoelastic.F90:545-556: the bulk and shear parts of the stretched gradients convolved with the
+// standard linear solids, stress with the unrelaxed moduli, kappa = lambda + mu).
+template <bool VISCO>
 __global__ __launch_bounds__(256) void k_pml_elastic_forces(
     int first, int n, long nq, const int* __restrict__ ids, const float4* __restrict__ geom,
     const float2* __restrict__ moduli, const float* __restrict__ dampw, const unsigned int* __restrict__ cidx,
     const float4* __restrict__ table, const float2* __restrict__ old, float* __restrict__ rm,
-    const float2* __restrict__ displ, const float2* __restrict__ veloc, float2* __restrict__ accel, const Basis basis) {
+    const float2* __restrict__ displ, const float2* __restrict__ veloc, float2* __restrict__ accel, const Basis basis,
+    const unsigned int* __restrict__ zidx, const float4* __restrict__ ztable, float* __restrict__ zrm) {
   __shared__ float s_hp[NGLL2], s_hpw[NGLL2], s_w[NGLL];
   __shared__ float s_ux[256], s_uz[256], s_ox[256], s_oz[256];
   __shared__ float s_t1x[256], s_t1z[256], s_t2x[256], s_t2z[256];
@@ -144,16 +158,68 @@ __global__ __launch_bounds__(256) void k_pml_elastic_forces(
     r[5] = conv(c.cbz, r[5], p_ux_z, o_ux_z);
     r[6] = conv(c.cax, r[6], p_uz_z, o_uz_z);
     r[7] = conv(c.cbz, r[7], p_uz_z, o_uz_z);
-    const float ux_x = c.a31[0] * p_ux_x + c.a31[1] * r[0] + c.a31[2] * r[1];
-    const float uz_x = c.a31[0] * p_uz_x + c.a31[1] * r[2] + c.a31[2] * r[3];
-    const float ux_z = c.a13[0] * p_ux_z + c.a13[1] * r[4] + c.a13[2] * r[5];
-    const float uz_z = c.a13[0] * p_uz_z + c.a13[1] * r[6] + c.a13[2] * r[7];
-    // non-symmetric stress of the stretched equations: lambda = kappa - mu, lambda + 2 mu = kappa + mu
-    const float lam = md.x - md.y, lp2m = md.x + md.y;
-    const float sxx = lp2m * ux_x + lam * p_uz_z;
-    const float szz = lp2m * uz_z + lam * p_ux_x;
-    const float szx = md.y * (p_uz_x + ux_z);
-    const float sxz = md.y * (p_ux_z + uz_x);
+    float sxx, szz, szx, sxz;
+    const unsigned int zi = VISCO ? zidx[q] : NO_ZENER;
+    if (VISCO && zi != NO_ZENER) {
+      const float4* z = ztable + (size_t)zi * ZSET_F4;
+      float ks_xx = 0.f, ks_zz = 0.f, ms_xx = 0.f, ms_zz = 0.f, ms_xz = 0.f, ms_zx = 0.f;
+      float sk_xx = 0.f, sk_zz = 0.f, sm_xx = 0.f, sm_zz = 0.f, sm_xz = 0.f, sm_zx = 0.f;
+#pragma unroll
+      for (int k = 0; k < ZSLS; k++) {
+        const float4 za = z[3 * k], zb = z[3 * k + 1], zc = z[3 * k + 2];
+        const float ck[3] = {zb.x, zb.y, zb.z}, cm[3] = {zb.w, zc.x, zc.y};
+        float* v = zrm + (size_t)(6 * k) * nq + q;
+        const float kxx = conv(ck, v[0], p_ux_x, o_ux_x);
+        const float kzz = conv(ck, v[nq], p_uz_z, o_uz_z);
+        const float mxx = conv(cm, v[2 * nq], p_ux_x, o_ux_x);
+        const float mzz = conv(cm, v[3 * nq], p_uz_z, o_uz_z);
+        const float mxz = conv(cm, v[4 * nq], p_ux_z, o_ux_z);
+        const float mzx = conv(cm, v[5 * nq], p_uz_x, o_uz_x);
+        v[0] = kxx;
+        v[nq] = kzz;
+        v[2 * nq] = mxx;
+        v[3 * nq] = mzz;
+        v[4 * nq] = mxz;
+        v[5 * nq] = mzx;
+        ks_xx += za.x * kxx;
+        ks_zz += za.y * kzz;
+        ms_xx += za.z * mxx;
+        ms_zz += za.w * mzz;
+        ms_xz += za.w * mxz;
+        ms_zx += za.z * mzx;
+        sk_xx += zc.z * kxx;
+        sk_zz += zc.z * kzz;
+        sm_xx += zc.w * mxx;
+        sm_zz += zc.w * mzz;
+        sm_xz += zc.w * mxz;
+        sm_zx += zc.w * mzx;
+      }
+      const float4 sa = z[3 * ZSLS], sb = z[3 * ZSLS + 1];
+      const float vk_xx = c.a31[0] * p_ux_x + ks_xx + sa.x * r[0] + sa.y * r[1];
+      const float vk_zz = c.a13[0] * p_uz_z + ks_zz + sa.z * r[6] + sa.w * r[7];
+      const float vm_xx = c.a31[0] * p_ux_x + ms_xx + sb.x * r[0] + sb.y * r[1];
+      const float vm_zz = c.a13[0] * p_uz_z + ms_zz + sb.z * r[6] + sb.w * r[7];
+      const float vm_xz = c.a13[0] * p_ux_z + ms_xz + sb.z * r[4] + sb.w * r[5];
+      const float vm_zx = c.a31[0] * p_uz_x + ms_zx + sb.x * r[2] + sb.y * r[3];
+      const float kk_xx = p_ux_x + sk_xx, kk_zz = p_uz_z + sk_zz;
+      const float mm_xx = p_ux_x + sm_xx, mm_zz = p_uz_z + sm_zz, mm_xz = p_ux_z + sm_xz, mm_zx = p_uz_x + sm_zx;
+      const float kap = md.x, mu = md.y;
+      sxx = (kap * vk_xx + mu * vm_xx) + (kap * kk_zz - mu * mm_zz);
+      sxz = mu * vm_zx + mu * mm_xz;
+      szx = mu * mm_zx + mu * vm_xz;
+      szz = (kap * kk_xx - mu * mm_xx) + (kap * vk_zz + mu * vm_zz);
+    } else {
+      const float ux_x = c.a31[0] * p_ux_x + c.a31[1] * r[0] + c.a31[2] * r[1];
+      const float uz_x = c.a31[0] * p_uz_x + c.a31[1] * r[2] + c.a31[2] * r[3];
+      const float ux_z = c.a13[0] * p_ux_z + c.a13[1] * r[4] + c.a13[2] * r[5];
+      const float uz_z = c.a13[0] * p_uz_z + c.a13[1] * r[6] + c.a13[2] * r[7];
+      // non-symmetric stress of the stretched equations: lambda = kappa - mu, lambda + 2 mu = kappa + mu
+      const float lam = md.x - md.y, lp2m = md.x + md.y;
+      sxx = lp2m * ux_x + lam * p_uz_z;
+      szz = lp2m * uz_z + lam * p_ux_x;
+      szx = md.y * (p_uz_x + ux_z);
+      sxz = md.y * (p_ux_z + uz_x);
+    }
     const float jac = 1.f / (gm.x * gm.w - gm.z * gm.y);
     s_t1x[tid] = jac * (sxx * gm.x + szx * gm.y);
     s_t1z[tid] = jac * (sxz * gm.x + szz * gm.y);
@@ -436,6 +502,69 @@ void l_coefficients(double kx, double bx, double ax, double kz, double bz, doubl
   }
 }
 
+// lik_parameter_computation_viscoelastic (pml_compute.f90:173-280): A_0, A_1, A_2 and the Zener term of one standard
+// linear solid (1/tau_sigma itau, tau_epsilon te); the corner elements' partial fractions as in lik_coefficients
+void lik_visco_coefficients(double kx, double bx, double ax, double kz, double bz, double az, int region,
+                            int index_ik, double itau, double te, double A[4]) {
+  const int xonly = index_ik == 13 ? 1 : 2, zonly = index_ik == 13 ? 2 : 1;
+  auto gamma_eta = [&](double al, double be, double* ga, double* et) {
+    *ga = (be * (itau - al * te * itau) + al * al * (te * itau - 1.0)) / (itau - al);
+    *et = itau * (be - al) / (itau - al);
+  };
+  if (region == 3) {
+    const double a0 = kx / kz;
+    const double al[2] = {ax, bz};
+    const double be[2] = {(ax * bz + ax * ax + 2.0 * bx * az - 2.0 * ax * (bx + az)) / (bz - ax),
+                          (ax * bz + bz * bz + 2.0 * bx * az - 2.0 * bz * (bx + az)) / (ax - bz)};
+    double ga[2], et[2];
+    for (int i = 0; i < 2; i++) gamma_eta(al[i], be[i], &ga[i], &et[i]);
+    A[0] = a0;
+    A[1] = a0 / 2.0 * (ga[0] - al[0]);
+    A[2] = a0 / 2.0 * (ga[1] - al[1]);
+    A[3] = a0 * (te * itau - 1.0) * ((et[0] + et[1]) / 2.0 - itau);
+  } else if (region == xonly || region == zonly) {
+    const bool x = region == xonly;
+    const double a0 = x ? kx : 1.0 / kz;
+    double ga, et;
+    gamma_eta(x ? ax : bz, x ? bx : az, &ga, &et);
+    A[0] = a0;
+    A[1] = x ? a0 * (ga - ax) : 0.0;
+    A[2] = x ? 0.0 : a0 * (ga - bz);
+    A[3] = a0 * (te * itau - 1.0) * (et - itau);
+  } else {
+    A[0] = A[1] = A[2] = A[3] = 0.0;
+  }
+}
+
+using ZenerKey = std::array<float, 4 * ZSET_F4>;
+
+// the Zener coefficient set of one layer point (layout above), double precision as the reference forms it
+ZenerKey zener_coefficients(double dt, const double prof[6], int region, const double itk[ZSLS],
+                            const double tek[ZSLS], const double phk[ZSLS], double mk, const double itm[ZSLS],
+                            const double tem[ZSLS], const double phm[ZSLS], double mm) {
+  const double kx = prof[0], kz = prof[1], dx = prof[2], dz = prof[3], ax = prof[4], az = prof[5];
+  const double bx = ax + dx / kx, bz = az + dz / kz;
+  ZenerKey z{};
+  double a2k = 0, a3k = 0, a4k = 0, a5k = 0, a2m = 0, a3m = 0, a4m = 0, a5m = 0;
+  for (int k = 0; k < ZSLS; k++) {
+    double K31[4], M31[4], K13[4], M13[4], ck[3], cm[3];
+    lik_visco_coefficients(kz, bz, az, kx, bx, ax, region, 31, itk[k], tek[k], K31);
+    lik_visco_coefficients(kz, bz, az, kx, bx, ax, region, 31, itm[k], tem[k], M31);
+    lik_visco_coefficients(kx, bx, ax, kz, bz, az, region, 13, itk[k], tek[k], K13);
+    lik_visco_coefficients(kx, bx, ax, kz, bz, az, region, 13, itm[k], tem[k], M13);
+    coef_convolution(itk[k], dt, ck);
+    coef_convolution(itm[k], dt, cm);
+    const double v[12] = {K31[3] / mk, K13[3] / mk, M31[3] / mm, M13[3] / mm, ck[0], ck[1], ck[2], cm[0], cm[1], cm[2],
+                          phk[k], phm[k]};
+    for (int m = 0; m < 12; m++) z[12 * k + m] = (float)v[m];
+    a2k += K31[1]; a3k += K31[2]; a4k += K13[1]; a5k += K13[2];
+    a2m += M31[1]; a3m += M31[2]; a4m += M13[1]; a5m += M13[2];
+  }
+  const double sums[8] = {a2k / mk, a3k / mk, a4k / mk, a5k / mk, a2m / mm, a3m / mm, a4m / mm, a5m / mm};
+  for (int m = 0; m < 8; m++) z[12 * ZSLS + m] = (float)sums[m];
+  return z;
+}
+
 using CoefKey = std::array<float, 22>;
 
 CoefKey point_coefficients(double dt, double kx, double dx, double ax, double kz, double dz, double az, int region) {
@@ -472,6 +601,8 @@ struct PmlLayer {  // the solid's or the fluid's layer elements, packed in (phas
   unsigned int* d_cidx = nullptr;
   float* d_old = nullptr;   // previous-step field per element point (float2 solid, float fluid)
   float* d_rm = nullptr;    // memory variables [planes][nelem * 25]
+  unsigned int* d_zidx = nullptr;  // solid, viscoelastic layer: Zener set per element point (NO_ZENER: none)
+  float* d_zrm = nullptr;          // its memory variables [ZPLANES][nelem * 25]
   int nbnd = 0;
   int* d_bnd = nullptr;     // outer-boundary point ids (device numbering)
   long nq() const { return (long)nelem * NGLL2; }
@@ -482,6 +613,14 @@ struct PmlState {
   int ntable = 0;
   float4* d_table = nullptr;
   PmlLayer el, ac;
+  // viscoelastic attenuation in the solid layer: its Zener coefficient sets arrive by prepare_cpml_attenuation_device_
+  // (needed when the handle has attenuation; the time loop refuses to run without them); the solid layer's caller
+  // elements in packed order and their profiles (K_x, K_z, d_x, d_z, alpha_x, alpha_z per point) are kept until then
+  int visco_needed = 0, visco_ready = 0, nztable = 0;
+  float4* d_ztable = nullptr;
+  double dt = 0.0;
+  std::vector<int> el_order;
+  std::vector<double> el_prof;  // [nq][6]
 };
 
 namespace {
@@ -489,6 +628,7 @@ namespace {
 void release_layer(PmlLayer& L) {
   device_free(L.d_ids); device_free(L.d_geom); device_free(L.d_mod); device_free(L.d_dampw);
   device_free(L.d_cidx); device_free(L.d_old); device_free(L.d_rm); device_free(L.d_bnd);
+  device_free(L.d_zidx); device_free(L.d_zrm);
 }
 
 // greedy colouring of a layer element list (device point ids), then the packed order per phase
@@ -531,11 +671,11 @@ void pml_setup(Mesh* mp, const std::vector<int>& region, const std::vector<int>&
   if (mp->pml_state) fail("prepare_pml_device: the CPML layer was already prepared");
   if (mp->nspec_elastic > 0 && !mp->sorted)
     fail("prepare_pml_device: call after prepare_fields_elastic_device (the layer follows the solid's element order)");
-  if (mp->attenuation) fail("CPML layers with viscoelastic attenuation are not supported by this library");
   if (mp->simulation_type != 1) fail("CPML layers: forward simulations (SIMULATION_TYPE=1) only");
   if (mp->batch) fail("CPML layers: shot batches are not supported");
   PmlState* S = new PmlState();
   mp->pml_state = S;
+  S->dt = dt;
   const int nspec = mp->nspec;
   S->region.assign(nspec, 0);
   // coefficient sets: a table of the distinct ones, an index per layer element point (caller order)
@@ -632,6 +772,15 @@ void pml_setup(Mesh* mp, const std::vector<int>& region, const std::vector<int>&
         cq[q] = idx[(size_t)e * NGLL2 + p] | (on_edge[ibool_dev[(size_t)pos[k] * NGLL2 + p]] ? ON_EDGE : 0u);
       }
     }
+    if (side == 0 && mp->attenuation) {  // kept for the Zener coefficient sets of the viscoelastic layer
+      S->visco_needed = 1;
+      S->el_order = order;
+      S->el_prof.resize((size_t)nq * 6);
+      for (int k = 0; k < L.nelem; k++)
+        for (int p = 0; p < NGLL2; p++)
+          for (int m = 0; m < 6; m++)
+            S->el_prof[((size_t)k * NGLL2 + p) * 6 + m] = prof[m][(size_t)row[order[k]] * NGLL2 + p];
+    }
     int* d_pos = device_copy(pos.data(), pos.size());
     float* d_rho = device_copy(rq.data(), rq.size());
     L.d_ids = device_alloc<int>((size_t)nq);
@@ -676,6 +825,7 @@ void pml_release(Mesh* mp) {
   release_layer(S->el);
   release_layer(S->ac);
   device_free(S->d_table);
+  device_free(S->d_ztable);
   delete S;
   mp->pml_state = nullptr;
 }
@@ -689,6 +839,8 @@ void pml_reset(Mesh* mp) {
     check_hip(hipMemsetAsync(L->d_old, 0, (size_t)L->nq() * (fluid ? 1 : 2) * sizeof(float), mp->stream), "memset");
     check_hip(hipMemsetAsync(L->d_rm, 0, (size_t)L->nq() * (fluid ? AC_PLANES : EL_PLANES) * sizeof(float),
                              mp->stream), "memset");
+    if (L->d_zrm)
+      check_hip(hipMemsetAsync(L->d_zrm, 0, (size_t)L->nq() * ZPLANES * sizeof(float), mp->stream), "memset");
   }
 }
 
@@ -706,13 +858,21 @@ void pml_elastic_forces(Mesh* mp, int ph) {
   PmlState* S = mp->pml_state;
   if (!S || S->el.nelem == 0) return;
   PmlLayer& L = S->el;
+  if (S->visco_needed && !S->visco_ready)
+    fail("CPML with viscoelastic attenuation: call prepare_cpml_attenuation_device before the time loop");
   for (int c = 0; c < L.ncolour[ph]; c++) {
     const int n = L.colour_count[ph][c];
     if (n <= 0) continue;
-    hipLaunchKernelGGL(k_pml_elastic_forces, dim3((n + PML_EPB - 1) / PML_EPB), dim3(256), 0, mp->stream,
-                       L.colour_start[ph][c], n, L.nq(), L.d_ids, L.d_geom, L.d_mod, L.d_dampw, L.d_cidx, S->d_table,
-                       (const float2*)L.d_old, L.d_rm, (const float2*)mp->d_displ, (const float2*)mp->d_veloc,
-                       (float2*)mp->d_accel, mp->basis);
+    if (L.d_zidx)
+      hipLaunchKernelGGL(k_pml_elastic_forces<true>, dim3((n + PML_EPB - 1) / PML_EPB), dim3(256), 0, mp->stream,
+                         L.colour_start[ph][c], n, L.nq(), L.d_ids, L.d_geom, L.d_mod, L.d_dampw, L.d_cidx, S->d_table,
+                         (const float2*)L.d_old, L.d_rm, (const float2*)mp->d_displ, (const float2*)mp->d_veloc,
+                         (float2*)mp->d_accel, mp->basis, L.d_zidx, S->d_ztable, L.d_zrm);
+    else
+      hipLaunchKernelGGL(k_pml_elastic_forces<false>, dim3((n + PML_EPB - 1) / PML_EPB), dim3(256), 0, mp->stream,
+                         L.colour_start[ph][c], n, L.nq(), L.d_ids, L.d_geom, L.d_mod, L.d_dampw, L.d_cidx, S->d_table,
+                         (const float2*)L.d_old, L.d_rm, (const float2*)mp->d_displ, (const float2*)mp->d_veloc,
+                         (float2*)mp->d_accel, mp->basis, nullptr, nullptr, nullptr);
     check_hip(hipGetLastError(), "k_pml_elastic_forces launch");
   }
 }
@@ -846,6 +1006,76 @@ void pml_boundary_elastic_cuda_(long* Mesh_pointer, int* compute_wavefield_1, in
   hipLaunchKernelGGL(k_pml_zero_elastic, dim3((S->el.nbnd + 255) / 256), dim3(256), 0, mp->stream, S->el.nbnd,
                      S->el.d_bnd, (float2*)mp->d_displ, (float2*)mp->d_veloc, (float2*)mp->d_accel);
   check_hip(hipGetLastError(), "k_pml_zero_elastic launch");
+}
+
+// The viscoelastic layer: the Zener parameters of the points (attenuation_model.f90:145-161 as the CPU path stores
+// them) for the attenuating solid layer elements.  Arrays in the reference's layout: inv_tau_sigma_nu*, phi_nu*,
+// tau_epsilon_nu* (NGLLX, NGLLZ, nspec, N_SLS), Mu_nu*, qkappa/qmu_attenuation_store (NGLLX, NGLLZ, nspec).
+void prepare_cpml_attenuation_device_(long* Mesh_pointer, int* N_SLS, realw* inv_tau_sigma_nu1, realw* phi_nu1,
+                                      realw* tau_epsilon_nu1, realw* Mu_nu1, realw* inv_tau_sigma_nu2, realw* phi_nu2,
+                                      realw* tau_epsilon_nu2, realw* Mu_nu2, realw* qkappa_attenuation_store,
+                                      realw* qmu_attenuation_store) {
+  Mesh* mp = handle(Mesh_pointer);
+  PmlState* S = mp->pml_state;
+  if (!S) fail("prepare_cpml_attenuation_device: call after prepare_cpml_device");
+  if (!mp->attenuation) fail("prepare_cpml_attenuation_device: the handle has no viscoelastic attenuation");
+  if (*N_SLS != ZSLS) fail("prepare_cpml_attenuation_device: N_SLS = %d (this library: %d)", *N_SLS, ZSLS);
+  if (S->visco_ready) fail("prepare_cpml_attenuation_device: already prepared");
+  PmlLayer& L = S->el;
+  S->visco_ready = 1;
+  if (L.nelem == 0) return;
+  const size_t ne = (size_t)mp->nspec * NGLL2;
+  const long nq = L.nq();
+  std::map<ZenerKey, unsigned int> ids;
+  std::vector<float4> table;
+  std::vector<unsigned int> zidx((size_t)nq, NO_ZENER);
+  for (int k = 0; k < L.nelem; k++) {
+    const int e = S->el_order[k];
+    // the element rule of compute_forces_viscoelastic.F90:178-181, 249-252: both quality factors below 9999 at every
+    // point, else the elastic layer formulation
+    float qk = 0.f, qm = 0.f;
+    for (int p = 0; p < NGLL2; p++) {
+      qk = std::max(qk, qkappa_attenuation_store[(size_t)e * NGLL2 + p]);
+      qm = std::max(qm, qmu_attenuation_store[(size_t)e * NGLL2 + p]);
+    }
+    if (!(qk < 9998.999f && qm < 9998.999f)) continue;
+    for (int p = 0; p < NGLL2; p++) {
+      const size_t l = (size_t)e * NGLL2 + p;
+      double itk[ZSLS], tek[ZSLS], phk[ZSLS], itm[ZSLS], tem[ZSLS], phm[ZSLS];
+      for (int s2 = 0; s2 < ZSLS; s2++) {
+        itk[s2] = inv_tau_sigma_nu1[l + ne * s2];
+        tek[s2] = tau_epsilon_nu1[l + ne * s2];
+        phk[s2] = phi_nu1[l + ne * s2];
+        itm[s2] = inv_tau_sigma_nu2[l + ne * s2];
+        tem[s2] = tau_epsilon_nu2[l + ne * s2];
+        phm[s2] = phi_nu2[l + ne * s2];
+        if (!(itk[s2] > 0.0) || !(itm[s2] > 0.0) || !std::isfinite(tek[s2]) || !std::isfinite(tem[s2]) ||
+            !std::isfinite(phk[s2]) || !std::isfinite(phm[s2]))
+          fail("prepare_cpml_attenuation_device: invalid Zener parameters at element %d point %d", e + 1, p + 1);
+      }
+      const double mk = (double)Mu_nu1[l] * (double)ZSLS, mm = (double)Mu_nu2[l] * (double)ZSLS;
+      if (!(mk > 0.0) || !(mm > 0.0)) fail("prepare_cpml_attenuation_device: Mu_nu <= 0 at element %d", e + 1);
+      const ZenerKey key = zener_coefficients(S->dt, &S->el_prof[((size_t)k * NGLL2 + p) * 6], S->region[e], itk,
+                                              tek, phk, mk, itm, tem, phm, mm);
+      for (float v : key)
+        if (!std::isfinite(v)) fail("prepare_cpml_attenuation_device: non-finite coefficient at element %d", e + 1);
+      auto it = ids.find(key);
+      if (it == ids.end()) {
+        it = ids.emplace(key, (unsigned int)(table.size() / ZSET_F4)).first;
+        for (int m = 0; m < ZSET_F4; m++)
+          table.push_back(make_float4(key[4 * m], key[4 * m + 1], key[4 * m + 2], key[4 * m + 3]));
+      }
+      zidx[(size_t)k * NGLL2 + p] = it->second;
+    }
+  }
+  S->el_prof.clear();
+  S->el_prof.shrink_to_fit();
+  S->nztable = (int)(table.size() / ZSET_F4);
+  if (S->nztable == 0) return;  // no attenuating layer element: the elastic layer kernel serves
+  S->d_ztable = device_copy(table.data(), table.size());
+  L.d_zidx = device_copy(zidx.data(), zidx.size());
+  L.d_zrm = device_zeros<float>((size_t)nq * ZPLANES);
+  check_hip(hipDeviceSynchronize(), "prepare_cpml_attenuation_device");
 }
 
 // layer statistics: elements (solid, fluid), coefficient sets, outer-boundary points (solid, fluid)

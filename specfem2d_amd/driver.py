@@ -87,6 +87,9 @@ def load_libraries():
     host.sf2d_sim_gradient_forward.restype = C.c_int
     host.sf2d_sim_gradient_backward.argtypes = [P, P, P]
     host.sf2d_sim_gradient_backward.restype = C.c_int
+    host.sf2d_sim_create_external.restype = P
+    host.sf2d_sim_create_external.argtypes = [C.POINTER(Params), C.c_int, C.c_int, P, P, C.c_int, P, P, C.c_int, P, P,
+                                              dp, dp, dp, C.c_int, C.POINTER(SourceC), C.c_int, dp, C.c_int, C.c_int]
     host.sf2d_sim_create_shots.restype = P
     host.sf2d_sim_create_shots.argtypes = [C.POINTER(Params), dp, dp, dp, C.c_int, P, C.POINTER(SourceC), C.c_int,
                                            dp]
@@ -204,10 +207,18 @@ class Simulation:
             self.prepare_device()
 
     def _create(self, p, rho, vp, vs, rec, simulation_type, save_forward):
-        if self.cfg.mesh is not None:
-            raise NotImplementedError("external meshes are not set up by the host driver")
         srcs = _sources_c(self.cfg.sources)
         self._keep = (p, rho, vp, vs, srcs, rec)
+        m = self.cfg.mesh
+        if m is not None:
+            i32 = lambda a: np.ascontiguousarray(np.asarray(a).reshape(-1), dtype=np.int32)  # noqa: E731
+            arrs = (i32(m.knods), np.ascontiguousarray(m.coorg, np.float64).reshape(-1), i32(m.free_elem),
+                    i32(m.free_code), i32(m.abs_elem), i32(m.abs_code))
+            self._keep_mesh = arrs
+            return self.host.sf2d_sim_create_external(
+                C.byref(p), m.nspec, int(m.coorg.shape[0]), _ptr(arrs[0]), _ptr(arrs[1]), len(m.free_elem),
+                _ptr(arrs[2]), _ptr(arrs[3]), len(m.abs_elem), _ptr(arrs[4]), _ptr(arrs[5]), rho, vp, vs,
+                len(self.cfg.sources), srcs, self.cfg.nrec, rec, simulation_type, save_forward)
         return self.host.sf2d_sim_create(C.byref(p), rho, vp, vs, len(self.cfg.sources), srcs, self.cfg.nrec, rec,
                                          simulation_type, save_forward)
 
@@ -514,6 +525,8 @@ class ShotBatch(Simulation):
             raise ValueError("every shot needs at least one source")
         if np.any(np.asarray(cfg.material_arrays()[2]) == 0.0):
             raise NotImplementedError("shot batches of meshes with fluid elements (vs = 0) are not supported")
+        if cfg.mesh is not None:
+            raise NotImplementedError("shot batches of external meshes are not supported")
         super().__init__(cfg, simulation_type=1, save_forward=False, device=device)
         self.nshot = len(self.shots)
 

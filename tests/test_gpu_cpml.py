@@ -1,7 +1,8 @@
 """CPML absorbing layers on the GPU (PML_BOUNDARY_CONDITIONS, Newmark; elastic and acoustic elements), driven through
 the Fortran-style C-ABI by the host driver (prepare_cpml_device_, pml_boundary_elastic_cuda_,
 pml_boundary_acoustic_cuda_), against the FP32 oracle -- which four of the reference's CPML examples pin over their
-whole traces -- and against those examples' committed traces.
+whole traces -- and against those examples' committed traces.  The viscoelastic example (Qkappa 22.3, Qmu 20) runs
+the viscoelastic layer (prepare_cpml_attenuation_device_: the Zener convolutions of the stretched gradients).
 
 Tolerances: FP32 seismograms within rel-L2 1e-5 of the FP32 oracle, as for the other paths; against REF_SEIS the
 oracle's own bar, 2e-5 of the peak over the whole trace (the fixtures come from earlier versions of the reference and
@@ -21,7 +22,8 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-5
 REF_BAR = 2e-5
 EXAMPLES = [("semi_infinite_homogeneous", ("BXX", "BXZ"), 564), ("force_source_elastic", ("BXX", "BXZ"), 492),
-            ("moment_tensor", ("BXX", "BXZ"), 564), ("pressure_source_acoustic", ("PRE",), 492)]
+            ("moment_tensor", ("BXX", "BXZ"), 564), ("pressure_source_acoustic", ("PRE",), 492),
+            ("force_source_viscoelastic", ("BXX", "BXZ"), 492)]
 
 
 def rel(a, b):
