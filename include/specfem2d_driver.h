@@ -78,6 +78,13 @@ void* sf2d_sim_create_external(const sf2d_params* p, int nspec, int npgeo, const
                                const int* abs_code, const double* rho, const double* vp, const double* vs, int nsrc,
                                const sf2d_source* src, int nrec, const double* rec_xz, int simulation_type,
                                int save_forward);
+/* An uneven batch (no counterpart in the reference): the disjoint union of n host-only simulations made by
+   sf2d_sim_create / sf2d_sim_create_external (+ sf2d_sim_set_attenuation) -- different meshes, sizes and physics
+   (fluid, elastic, viscoelastic, CPML or Stacey), one DT, NSTEP and seismotype -- as one simulation whose kernels
+   cover every instance in each launch.  Elements, points, sources and receivers follow in instance order; receiver
+   r of instance k is union receiver sum_{j<k} nrec_j + r.  The instances stay valid and are released by the caller;
+   returns NULL when they do not fit together. */
+void* sf2d_sim_create_union(int n, void* const* sims);
 void* sf2d_sim_create_shots(const sf2d_params* p, const double* rho, const double* vp, const double* vs, int nshot,
                             const int* shot_nsrc, const sf2d_source* src, int nrec, const double* rec_xz);
 /* all NSTEP steps of every shot from rest; seis[nshot][nrec][2][NSTEP] displacement; timing[0] = device ms */

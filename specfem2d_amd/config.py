@@ -433,6 +433,43 @@ def c2_mesh_coupled(nstep: int = 3000, nx: int = 2000, nz: int = 1000, elem_size
     return cfg
 
 
+def uneven_batch_instances(n: int = 48, seed: int = 5, elem_size: float = 20.0, dt: float = 4.0e-4,
+                           nelem_pml: int = 10, size_range=((100, 1000), (100, 500)), nstep_range=(1000, 4000),
+                           q_range=(30.0, 100.0), nrec: int = 20) -> List[SimulationConfig]:
+    """The uneven-batch workload: ``n`` instances, mesh sizes drawn log-uniformly (nx in 100..1000, nz in 100..500
+    elements of ``elem_size``), half acoustic (water, vp 1500, rho 1020) and half viscoelastic (a seeded random-field
+    solid, +-10 % around vp 3000, vs 1732, rho 2500, with Qkappa and Qmu drawn uniformly in 30..100, 3 solids), NSTEP
+    uniform in 1000..4000, a CPML frame ``nelem_pml`` elements thick on the left, right and bottom and a free surface on
+    top, one Ricker source (a pressure source in the water, a force in the solid; f0 8-12 Hz) and ``nrec`` displacement
+    receivers along a line 40 m down; FP32, Newmark, one time step for all.  Seeded."""
+    rng = np.random.default_rng(seed)
+    (nx0, nx1), (nz0, nz1) = size_range
+    out = []
+    for k in range(n):
+        nx = int(round(np.exp(rng.uniform(np.log(nx0), np.log(nx1)))))
+        nz = int(round(np.exp(rng.uniform(np.log(nz0), np.log(nz1)))))
+        nstep = int(rng.integers(nstep_range[0], nstep_range[1] + 1))
+        f0 = float(rng.uniform(8.0, 12.0))
+        xmax, depth = nx * elem_size, nz * elem_size
+        fluid = k % 2 == 0
+        src = [Source(x=xmax * rng.uniform(0.3, 0.7), z=-depth * rng.uniform(0.2, 0.5), f0=f0,
+                      factor=1.0e4 if fluid else 1.0e10)]
+        rec = np.stack([np.linspace(xmax * 0.1, xmax * 0.9, nrec), np.full(nrec, -40.0)], axis=1)
+        cfg = SimulationConfig(name=f"uneven_{k}_{'acoustic' if fluid else 'viscoelastic'}_{nx}x{nz}", nx=nx, nz=nz,
+                               xmin=0.0, xmax=xmax, interfaces=[-depth, 0.0], nz_layers=[nz], nstep=nstep, dt=dt,
+                               material_mode=1, stacey=False, pml=True, nelem_pml=nelem_pml, absorbtop=False,
+                               sources=src, receivers=rec)
+        n25 = cfg.nspec * NGLL2
+        if fluid:
+            cfg = cfg.replace(rho=np.full(n25, 1020.0), vp=np.full(n25, 1500.0), vs=np.zeros(n25))
+        else:
+            pert = gaussian_random_field(cfg, 200.0, 0.10, seed * 1000 + k)
+            cfg = cfg.replace(rho=2500.0 * (1 + pert), vp=3000.0 * (1 + pert), vs=1732.0 * (1 + pert),
+                              attenuation=True, qkappa=float(rng.uniform(*q_range)), qmu=float(rng.uniform(*q_range)))
+        out.append(cfg)
+    return out
+
+
 def luoyang_fluid_solid(nstep: int = 1600) -> SimulationConfig:
     """``EXAMPLES/LuoYang_fluid_solid_kernel/DATA/Par_file`` + ``SOURCE`` (forward run): 100 x 60 elements of 50 m
     (5 km x 3 km, three layers of 20 rows), fluid (rho 2500, vp 2700) in columns 1-40 and 61-100 around a solid

@@ -686,9 +686,9 @@ void prepare_fields_elastic_device_(long* Mesh_pointer, realw* rmassx, realw* rm
   if (*ANISOTROPY) fail("anisotropic elements are not supported by this library yet");
   if (*h_ninterface_elastic > 0) fail("MPI elastic interfaces passed: one GPU, one partition only");
   if (*ATTENUATION_VISCOELASTIC) {
-    // as the reference's GPU code (prepare_gpu.f90:69-72): forward runs of solid meshes, 3 solids
+    // forward runs, 3 solids, in the solid elements (the reference's GPU code refuses attenuation with fluid elements,
+    // prepare_gpu.f90:69-72; its CPU path runs them, and so does this library: the fluid is not attenuated)
     if (mp->simulation_type != 1) fail("viscoelastic attenuation: forward simulations (SIMULATION_TYPE=1) only");
-    if (mp->nspec_acoustic > 0) fail("viscoelastic attenuation: coupled fluid-solid meshes are not supported");
     if (!h_A_newmark_mu || !h_B_newmark_mu || !h_A_newmark_kappa || !h_B_newmark_kappa)
       fail("viscoelastic attenuation: coefficient arrays missing");
     mp->attenuation = 1;

@@ -13,10 +13,12 @@
 #include <array>
 #include <cmath>
 #include <cstdio>
+#include <cstdint>
 #include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <map>
+#include <type_traits>
 #include <vector>
 
 #include "specfem2d_driver.h"
@@ -1423,6 +1425,185 @@ void* sf2d_sim_create_external(const sf2d_params* p, int nspec, int npgeo, const
   return create(p, rho, vp, vs, nsrc, src, nrec, rec_xz, simulation_type, save_forward, 0, nullptr, &m);
 }
 
+// An uneven batch: the disjoint union of n host-only simulations (different meshes, sizes and physics, one time step
+// and one recording kind) as one simulation -- their elements, points, boundaries, layers, sources and receivers
+// concatenated in instance order with the element, point, layer-row and side offsets applied -- so that every kernel
+// launch of a time step covers all instances.  No counterpart in the reference (one mesh per run; its multi-run mode
+// is MPI sub-communicators, src/shared/parallel.F90:95-116).  The union runs the longest instance's NSTEP; the
+// instances are set up with that NSTEP (their source time functions then cover it) and record their own first steps.
+void* sf2d_sim_create_union(int n, void* const* sims) {
+  if (n < 1 || !sims) return nullptr;
+  const Sim* f = (const Sim*)sims[0];
+  for (int k = 0; k < n; k++) {
+    const Sim* a = (const Sim*)sims[k];
+    if (!a || a->mesh || a->simulation_type != 1 || a->nshot > 0 || a->save_forward || a->p.dt != f->p.dt ||
+        a->p.nstep != f->p.nstep || a->p.seismotype != f->p.seismotype || a->p.use_trick != f->p.use_trick) {
+      fprintf(stderr, "sf2d_sim_create_union: instances must be host-only forward simulations with one time step, "
+                      "NSTEP and recording kind\n");
+      return nullptr;
+    }
+  }
+  Sim* u = new Sim();
+  u->p = f->p;
+  u->basis = f->basis;
+  u->simulation_type = 1;
+  long ne_tot = 0, ng_tot = 0;
+  for (int k = 0; k < n; k++) {
+    const Sim* a = (const Sim*)sims[k];
+    ne_tot += a->nspec;
+    ng_tot += a->nglob;
+    u->p.stacey = u->p.stacey || a->p.stacey;
+    u->p.pml = u->p.pml || a->p.pml;
+    u->attenuation = u->attenuation || a->attenuation;
+  }
+  if (ne_tot * NGLL2 > (long)INT32_MAX || ng_tot * 2 > (long)INT32_MAX) {
+    fprintf(stderr, "sf2d_sim_create_union: %ld elements, %ld points: too many for 32-bit indices\n", ne_tot, ng_tot);
+    delete u;
+    return nullptr;
+  }
+  u->nspec = (int)ne_tot;
+  u->nglob = (int)ng_tot;
+  const size_t ne = (size_t)u->nspec * NGLL2;
+  // per-side counters of the absorbing faces
+  int side_count[5] = {0, 0, 0, 0, 0};
+  int eoff = 0, goff = 0, poff = 0, nrec = 0;
+  for (int k = 0; k < n; k++) nrec += ((const Sim*)sims[k])->nrec;
+  u->nrec = nrec;
+  u->xir.assign((size_t)NGLL * std::max(nrec, 1), 0.f);
+  u->gammar.assign(u->xir.size(), 0.f);
+  // Zener parameters: only the viscoelastic CPML layer uses them
+  const bool zener = u->attenuation && u->p.pml;
+  if (zener) {
+    u->zen_itau1.assign(3 * ne, -1.f); u->zen_te1.assign(3 * ne, -1.f); u->zen_phi1.assign(3 * ne, 0.f);
+    u->zen_itau2.assign(3 * ne, -1.f); u->zen_te2.assign(3 * ne, -1.f); u->zen_phi2.assign(3 * ne, 0.f);
+    u->zen_mu1.assign(ne, 1.f); u->zen_mu2.assign(ne, 1.f);
+    u->zen_qk.assign(ne, 9999.f); u->zen_qm.assign(ne, 9999.f);
+  }
+  // reserve the concatenated arrays once: they hold tens of GB for large batches
+  auto reserve_sum = [&](auto member) {
+    size_t len = 0;
+    for (int k = 0; k < n; k++) len += (((const Sim*)sims[k])->*member).size();
+    (u->*member).reserve(len);
+  };
+  reserve_sum(&Sim::ibool); reserve_sum(&Sim::coord); reserve_sum(&Sim::xix); reserve_sum(&Sim::xiz);
+  reserve_sum(&Sim::gammax); reserve_sum(&Sim::gammaz); reserve_sum(&Sim::jacobian); reserve_sum(&Sim::kappa);
+  reserve_sum(&Sim::mu); reserve_sum(&Sim::rho_vp); reserve_sum(&Sim::rho_vs); reserve_sum(&Sim::rho);
+  reserve_sum(&Sim::model_rho); reserve_sum(&Sim::model_vp); reserve_sum(&Sim::model_vs);
+  if (u->attenuation)
+    for (auto m : {&Sim::att_A_kappa, &Sim::att_B_kappa, &Sim::att_A_mu, &Sim::att_B_mu}) (u->*m).reserve(3 * ne);
+  int roff = 0;
+  for (int k = 0; k < n; k++) {
+    const Sim& a = *(const Sim*)sims[k];
+    const size_t ae = (size_t)a.nspec * NGLL2;
+    auto cat = [](auto& dst, const auto& src) { dst.insert(dst.end(), src.begin(), src.end()); };
+    auto cat_or_zero = [](auto& dst, const auto& src, size_t len) {
+      if (src.size() == len) dst.insert(dst.end(), src.begin(), src.end());
+      else dst.insert(dst.end(), len, typename std::decay_t<decltype(dst)>::value_type{});
+    };
+    for (int v : a.ibool) u->ibool.push_back(v + goff);
+    cat(u->coord, a.coord);
+    cat(u->xix, a.xix); cat(u->xiz, a.xiz); cat(u->gammax, a.gammax); cat(u->gammaz, a.gammaz);
+    cat(u->jacobian, a.jacobian); cat(u->kappa, a.kappa); cat(u->mu, a.mu); cat(u->rho_vp, a.rho_vp);
+    cat(u->rho_vs, a.rho_vs); cat(u->rho, a.rho);
+    cat_or_zero(u->rmass_inverse, a.rmass_inverse, (size_t)2 * a.nglob);
+    cat_or_zero(u->rmassx, a.rmassx, (size_t)a.nglob);
+    cat_or_zero(u->rmassz, a.rmassz, (size_t)a.nglob);
+    cat_or_zero(u->rmass_ac, a.rmass_ac, (size_t)a.nglob);
+    cat_or_zero(u->is_acoustic, a.is_acoustic, (size_t)a.nspec);
+    u->nspec_ac += a.nspec_ac;
+    for (int v : a.free_ispec) u->free_ispec.push_back(v + eoff);
+    cat(u->free_ij, a.free_ij);
+    for (int v : a.coup_ac_ispec) u->coup_ac_ispec.push_back(v + eoff);
+    for (int v : a.coup_el_ispec) u->coup_el_ispec.push_back(v + eoff);
+    cat(u->coup_ac_code, a.coup_ac_code); cat(u->coup_el_code, a.coup_el_code); cat(u->coup_ij, a.coup_ij);
+    cat(u->coup_normal, a.coup_normal); cat(u->coup_jacw, a.coup_jacw);
+    // Stacey faces: element offset, indices within each side continued across the instances (a CPML instance's
+    // absorbing sides only seeded its layer, which comes merged below)
+    const int nabs_a = a.p.stacey ? a.nabs : 0;
+    for (int q = 0; q < nabs_a; q++) {
+      const int c = a.edge_abs[q];
+      const int kk = ++side_count[c];
+      u->abs_ispec.push_back(a.abs_ispec[q] + eoff);
+      u->edge_abs.push_back(c);
+      u->ib0.push_back(kk);
+      u->ib_bottom.push_back(c == 1 ? kk : 0);
+      u->ib_right.push_back(c == 2 ? kk : 0);
+      u->ib_top.push_back(c == 3 ? kk : 0);
+      u->ib_left.push_back(c == 4 ? kk : 0);
+    }
+    u->nabs += nabs_a;
+    if (nabs_a) {
+      cat(u->abs_ij, a.abs_ij); cat(u->abs_normal, a.abs_normal); cat(u->abs_jacw, a.abs_jacw);
+    }
+    // CPML layer: rows of the profile arrays continued, outer-boundary points offset
+    cat_or_zero(u->region_cpml, a.region_cpml, (size_t)a.nspec);
+    for (int e = 0; e < a.nspec; e++)
+      u->spec_to_pml.push_back(a.spec_to_pml.empty() || a.spec_to_pml[e] == 0 ? 0 : a.spec_to_pml[e] + poff);
+    cat(u->pml_kx, a.pml_kx); cat(u->pml_kz, a.pml_kz); cat(u->pml_dx, a.pml_dx); cat(u->pml_dz, a.pml_dz);
+    cat(u->pml_ax, a.pml_ax); cat(u->pml_az, a.pml_az);
+    for (int g : a.pml_pts_el) u->pml_pts_el.push_back(g + goff);
+    for (int g : a.pml_pts_ac) u->pml_pts_ac.push_back(g + goff);
+    u->nspec_pml += a.nspec_pml;
+    // attenuation: coefficients per point ([nspec][25][3]), Zener parameters ([3][nspec][25]), quality factors
+    if (u->attenuation) {
+      cat_or_zero(u->att_A_kappa, a.att_A_kappa, 3 * ae); cat_or_zero(u->att_B_kappa, a.att_B_kappa, 3 * ae);
+      cat_or_zero(u->att_A_mu, a.att_A_mu, 3 * ae); cat_or_zero(u->att_B_mu, a.att_B_mu, 3 * ae);
+    }
+    if (zener && a.attenuation && !a.zen_itau1.empty()) {
+      const size_t o = (size_t)eoff * NGLL2;
+      for (int sl = 0; sl < 3; sl++)
+        for (size_t l = 0; l < ae; l++) {
+          u->zen_itau1[sl * ne + o + l] = a.zen_itau1[sl * ae + l];
+          u->zen_te1[sl * ne + o + l] = a.zen_te1[sl * ae + l];
+          u->zen_phi1[sl * ne + o + l] = a.zen_phi1[sl * ae + l];
+          u->zen_itau2[sl * ne + o + l] = a.zen_itau2[sl * ae + l];
+          u->zen_te2[sl * ne + o + l] = a.zen_te2[sl * ae + l];
+          u->zen_phi2[sl * ne + o + l] = a.zen_phi2[sl * ae + l];
+        }
+      std::copy(a.zen_mu1.begin(), a.zen_mu1.end(), u->zen_mu1.begin() + o);
+      std::copy(a.zen_mu2.begin(), a.zen_mu2.end(), u->zen_mu2.begin() + o);
+      std::copy(a.zen_qk.begin(), a.zen_qk.end(), u->zen_qk.begin() + o);
+      std::copy(a.zen_qm.begin(), a.zen_qm.end(), u->zen_qm.begin() + o);
+    }
+    cat(u->model_rho, a.model_rho); cat(u->model_vp, a.model_vp); cat(u->model_vs, a.model_vs);
+    // sources: element offset; their arrays and time functions (each instance timed as a run of its own)
+    cat(u->src, a.src);
+    for (int v : a.src_ispec) u->src_ispec.push_back(v + eoff);
+    cat(u->src_xi, a.src_xi); cat(u->src_gamma, a.src_gamma); cat(u->sourcearrays, a.sourcearrays);
+    u->nsrc += a.nsrc;
+    // receivers: element offset, interpolation weights [5][nrec] re-laid for the union
+    for (int r = 0; r < a.nrec; r++) {
+      u->rec_ispec.push_back(a.rec_ispec[r] + eoff);
+      u->rec_xi.push_back(a.rec_xi[r]);
+      u->rec_gamma.push_back(a.rec_gamma[r]);
+      u->cosrot.push_back(a.cosrot[r]);
+      u->sinrot.push_back(a.sinrot[r]);
+      for (int i = 0; i < NGLL; i++) {
+        u->xir[(size_t)i * nrec + roff + r] = a.xir[(size_t)i * a.nrec + r];
+        u->gammar[(size_t)i * nrec + roff + r] = a.gammar[(size_t)i * a.nrec + r];
+      }
+    }
+    roff += a.nrec;
+    eoff += a.nspec;
+    goff += a.nglob;
+    poff += a.nspec_pml;
+  }
+  for (int c = 1; c <= 4; c++) u->nspec_side[c] = side_count[c];
+  if (u->is_acoustic.empty()) u->is_acoustic.assign(u->nspec, 0);
+  if (u->cosrot.empty()) { u->cosrot.assign(1, 1.f); u->sinrot.assign(1, 0.f); }
+  // time functions [nstep][nsrc], source fastest
+  const int nstep = u->p.nstep;
+  u->stf.assign((size_t)nstep * std::max(u->nsrc, 1), 0.f);
+  for (int k = 0, soff = 0; k < n; k++) {
+    const Sim& a = *(const Sim*)sims[k];
+    for (int t = 0; t < nstep; t++)
+      for (int q = 0; q < a.nsrc; q++) u->stf[(size_t)t * u->nsrc + soff + q] = a.stf[(size_t)t * a.nsrc + q];
+    soff += a.nsrc;
+  }
+  u->t0 = f->t0;
+  return u;
+}
+
 void* sf2d_sim_create_shots(const sf2d_params* p, const double* rho, const double* vp, const double* vs, int nshot,
                             const int* shot_nsrc, const sf2d_source* src, int nrec, const double* rec_xz) {
   if (nshot < 1 || !shot_nsrc) return nullptr;
@@ -1946,8 +2127,8 @@ int sf2d_sim_gradient_backward(void* sim, double* grad, double* parts) {
 int sf2d_sim_set_attenuation(void* sim, const double* qkappa, const double* qmu, int n_sls) {
   Sim* s = (Sim*)sim;
   if (!s || s->mesh || !qkappa || !qmu) return 1;
-  if (n_sls != 3 || s->nspec_ac > 0 || s->simulation_type != 1 || s->nshot > 0 || s->src.empty()) {
-    fprintf(stderr, "sf2d_sim_set_attenuation: 3 solids, forward runs of solid meshes with a source only\n");
+  if (n_sls != 3 || s->simulation_type != 1 || s->nshot > 0 || s->src.empty()) {
+    fprintf(stderr, "sf2d_sim_set_attenuation: 3 solids, forward runs with a source only\n");
     return 1;
   }
   const double f0 = s->src[0].f0;  // ATTENUATION_f0_REFERENCE (prepare_attenuation.f90:319-322)
@@ -1985,6 +2166,10 @@ int sf2d_sim_set_attenuation(void* sim, const double* qkappa, const double* qmu,
   };
   for (size_t l = 0; l < ne; l++) {
     if (qkappa[l] > 9998.999 && qmu[l] > 9998.999) continue;  // no attenuation at this point: A = B = 0
+    if (s->nspec_ac > 0 && s->is_acoustic[l / NGLL2]) {  // viscoacoustic attenuation is another model
+      fprintf(stderr, "sf2d_sim_set_attenuation: finite quality factors at point %zu of a fluid element\n", l);
+      return 1;
+    }
     if (!(qkappa[l] > 0.0 && qmu[l] > 0.0) ||
         !coefficients(qkappa[l], &s->att_A_kappa[3 * l], &s->att_B_kappa[3 * l], l, 0) ||
         !coefficients(qmu[l], &s->att_A_mu[3 * l], &s->att_B_mu[3 * l], l, 1)) {

@@ -90,6 +90,8 @@ def load_libraries():
     host.sf2d_sim_create_external.restype = P
     host.sf2d_sim_create_external.argtypes = [C.POINTER(Params), C.c_int, C.c_int, P, P, C.c_int, P, P, C.c_int, P, P,
                                               dp, dp, dp, C.c_int, C.POINTER(SourceC), C.c_int, dp, C.c_int, C.c_int]
+    host.sf2d_sim_create_union.restype = P
+    host.sf2d_sim_create_union.argtypes = [C.c_int, P]
     host.sf2d_sim_create_shots.restype = P
     host.sf2d_sim_create_shots.argtypes = [C.POINTER(Params), dp, dp, dp, C.c_int, P, C.POINTER(SourceC), C.c_int,
                                            dp]
@@ -201,7 +203,8 @@ class Simulation:
         if cfg.attenuation:
             qk, qm = cfg.attenuation_arrays()
             if self.host.sf2d_sim_set_attenuation(self.h, qk, qm, int(cfg.n_sls)) != 0:
-                raise NotImplementedError("viscoelastic attenuation: 3 solids, forward runs of solid meshes only")
+                raise NotImplementedError("viscoelastic attenuation: 3 solids, forward runs, quality factors 9999 "
+                                          "in fluid elements")
         self.on_device = False
         if device:
             self.prepare_device()
@@ -581,3 +584,68 @@ class ShotBatch(Simulation):
 
     def adjoint(self, *a, **k):
         raise RuntimeError("SIMULATION_TYPE=3 runs take one simulation; use Simulation(simulation_type=3)")
+
+
+class UnevenBatch(Simulation):
+    """An uneven batch: ``len(cfgs)`` simulations of different meshes, sizes and physics (fluid or viscoelastic
+    solids, CPML or Stacey edges, different NSTEP) advanced together on the GPU, one launch per kernel covering every
+    instance.  Their time step and recording kind must agree.  The union runs the longest NSTEP; ``forward()``
+    returns every instance's seismograms over its own NSTEP (a list of [nrec_k, 2, nstep_k] arrays).  No counterpart
+    in the reference, which runs one mesh per simulation."""
+
+    def __init__(self, cfgs, device: bool = True):
+        cfgs = list(cfgs)
+        if not cfgs:
+            raise ValueError("an uneven batch needs at least one instance")
+        nstep = max(int(c.nstep) for c in cfgs)
+        if len({float(c.dt) for c in cfgs}) != 1 or len({int(c.seismotype) for c in cfgs}) != 1:
+            raise ValueError("the instances of an uneven batch share DT and seismotype")
+        self.instance_cfgs = cfgs
+        self.instance_nstep = [int(c.nstep) for c in cfgs]
+        # the instances are set up with the union's NSTEP (their time functions cover it), on the host only
+        self.parts = [Simulation(c.replace(nstep=nstep), device=False) for c in cfgs]
+        self.instance_nrec = [c.nrec for c in cfgs]
+        self.instance_nglob = [p.nglob for p in self.parts]
+        self.instance_nspec = [p.nspec for p in self.parts]
+        # the union's own configuration carries the sources and receivers of all instances; its materials and
+        # quality factors come from the instances' set-ups, which the union handle copies
+        self._union_cfg = cfgs[0].replace(
+            nstep=nstep, sources=[s for c in cfgs for s in c.sources], rho=1.0, vp=1.0, vs=0.0, attenuation=False,
+            mesh=None, receivers=np.concatenate(
+                [np.asarray(c.receivers, np.float64).reshape(-1, 2) for c in cfgs if c.nrec] or [np.zeros((0, 2))]))
+        super().__init__(self._union_cfg, simulation_type=1, save_forward=False, device=False)
+        for q in self.parts:  # the union holds copies of everything it needs
+            q.close()
+        self.parts = []
+        if device:
+            self.prepare_device()
+
+    def _create(self, p, rho, vp, vs, rec, simulation_type, save_forward):
+        arr = (C.c_void_p * len(self.parts))(*[q.h for q in self.parts])
+        self._keep = (arr,)
+        return self.host.sf2d_sim_create_union(len(self.parts), arr)
+
+    def forward(self, nstep: Optional[int] = None, save_stacey: bool = False, phase_timing: bool = False):
+        """All NSTEP steps of the union from rest: (per-instance seismograms, timing)."""
+        if nstep is not None or save_stacey:
+            raise ValueError("an uneven batch runs whole: forward() from rest")
+        self.reset()
+        seis, t = super().forward(phase_timing=phase_timing)
+        out, r0 = [], 0
+        for nr, ns in zip(self.instance_nrec, self.instance_nstep):
+            out.append(seis[r0:r0 + nr, :, :ns])
+            r0 += nr
+        return out, t
+
+    def useful_updates(self) -> int:
+        """Sum over the instances of GLL points x their own NSTEP."""
+        return int(sum(g * n for g, n in zip(self.instance_nglob, self.instance_nstep)))
+
+    def close(self):
+        super().close()
+        for q in getattr(self, "parts", []):
+            q.close()
+        self.parts = []
+
+    def attenuation_coefficients(self):
+        raise NotImplementedError("per instance: use the instances' own simulations")

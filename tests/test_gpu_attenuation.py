@@ -92,7 +92,40 @@ def test_heterogeneous_quality_factors_against_oracle(assembly, per_point, monke
     assert r < TOL and rel(d, od) < 5 * TOL and rel(el, ref) > 0.05
 
 
-def test_attenuation_refuses_fluids():
+def test_attenuation_refuses_finite_quality_factors_in_fluids():
     cfg = config.layered_fluid_solid(8, 6, 2, 100.0, 10, 1e-3, sources=[Source(x=420.0, z=-450.0, f0=10.0)])
     with pytest.raises(NotImplementedError):
         Simulation(cfg.replace(attenuation=True, qkappa=30.0, qmu=20.0))
+
+
+@pytest.mark.parametrize("pml", [False, True])
+def test_attenuation_in_a_coupled_mesh_against_oracle(pml):
+    """A water layer over an attenuating random solid (the fluid's quality factors 9999: the fluid is not attenuated),
+    fluid-solid coupling, a force in the solid and a pressure source in the water; Stacey edges, or a CPML frame
+    whose solid elements take the viscoelastic layer formulation."""
+    nx, nz, nf = 30, 22, 6
+    rng = np.random.default_rng(8)
+    pert = np.repeat(0.05 * rng.standard_normal(nx * nz), 25)
+    src = [Source(x=700.0, z=-800.0, f0=12.0, factor=1e9, anglesource=20.0),
+           Source(x=1100.0, z=-90.0, f0=12.0, factor=1e4)]
+    rec = np.array([[300.0, -60.0], [900.0, -250.0], [1300.0, -700.0], [500.0, -1000.0]])
+    cfg = config.layered_fluid_solid(nx, nz, nf, 50.0, 1200, 1.0e-3, pert=pert, sources=src, receivers=rec)
+    fluid = np.asarray(cfg.vs).reshape(-1) == 0.0
+    qk = np.where(fluid, 9999.0, rng.uniform(20.0, 60.0, fluid.size))
+    qm = np.where(fluid, 9999.0, rng.uniform(15.0, 50.0, fluid.size))
+    cfg = cfg.replace(attenuation=True, qkappa=qk, qmu=qm)
+    if pml:
+        cfg = cfg.replace(stacey=False, pml=True, nelem_pml=3)
+    print(f"case: coupled {nx}x{nz} with attenuation, pml {pml}", flush=True)
+    sim = Simulation(cfg)
+    seis, t = sim.forward()
+    sim.close()
+    ref = OracleSim(cfg, "sp").forward()
+    exact = OracleSim(cfg, "dp").forward()
+    gap = rel(ref, exact)
+    r, r64 = rel(seis, ref), rel(seis, exact)
+    el = OracleSim(cfg.replace(attenuation=False), "sp").forward()
+    print(f"coupled with attenuation (pml {pml}): rel-L2 vs FP32 oracle {r:.3e}, vs FP64 {r64:.3e} (FP32 oracle vs "
+          f"FP64 {gap:.3e}; the elastic run is {rel(el, ref):.2f} away); loop {t['loop_ms']:.1f} ms")
+    bar = max(TOL, 2.0 * gap)
+    assert r < bar and r64 < bar and rel(el, ref) > 0.05
