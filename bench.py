@@ -363,38 +363,43 @@ def c2_elastic(steps: int, warmup: int):
     return out, roofline
 
 
-def c5_uneven(n: int = 48, seed: int = 5):
+def c5_uneven(n: int = 48, seed: int = 5, max_groups: int = 6):
     """The uneven batch: ``n`` instances of log-uniform sizes (100..1000 x 100..500 elements of 20 m), half water
     with a pressure source, half viscoelastic random-field solids (3 solids, Q 30..100) with a force, CPML 10
-    elements thick, NSTEP uniform in 1000..4000, advanced as one union mesh (one launch per kernel for all of them)
-    from rest through the longest NSTEP.  Useful throughput: the sum over the instances of GLL points x their own
-    NSTEP, per second of the whole run; the union also steps finished instances until the longest one ends
-    (``padded_fraction`` of the updates it makes)."""
+    elements thick, NSTEP uniform in 1000..4000.  They run as unions of instances of similar NSTEP (at most
+    ``max_groups``; each union one launch per kernel for all its instances), from rest through each union's longest
+    NSTEP, the unions one after the other.  Useful throughput: the sum over the instances of GLL points x their own
+    NSTEP, per second of all runs; ``padded_fraction``: the share of the unions' updates spent on instances that
+    have already ended."""
     import torch
     from specfem2d_amd import config
-    from specfem2d_amd.driver import Simulation, UnevenBatch
+    from specfem2d_amd.driver import Simulation, UnevenRuns
     cfgs = config.uneven_batch_instances(n=n, seed=seed)
     t = time.perf_counter()
-    u = UnevenBatch(cfgs)
+    runs = UnevenRuns(cfgs, max_groups=max_groups)
     setup = time.perf_counter() - t
-    print(f"uneven batch: {n} instances, {u.nspec} elements, {u.nglob} points set up in {setup:.0f} s",
-          file=sys.stderr, flush=True)
-    Simulation.forward(u, nstep=5)  # first launches of every kernel, outside the timed run
+    print(f"uneven batch: {n} instances in {len(runs.groups)} unions set up in {setup:.0f} s", file=sys.stderr,
+          flush=True)
+    for b in runs.batches:
+        Simulation.forward(b, nstep=5)  # first launches of every kernel, outside the timed run
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    seis, timing = u.forward()
+    seis, timing = runs.forward()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    nstep = max(c.nstep for c in cfgs)
-    useful = u.useful_updates()
+    useful, union = runs.useful_updates(), runs.union_updates()
+    groups = [{"instances": len(g), "nglob": b.nglob, "nstep": b.cfg.nstep,
+               "ms_per_step": t["loop_ms"] / b.cfg.nstep} for g, b, t in zip(runs.groups, runs.batches,
+                                                                          timing["groups"])]
     out = {"workload": f"c5_uneven_batch_{n}", "instances": n, "acoustic": sum(1 for c in cfgs if not c.attenuation),
-           "viscoelastic": sum(1 for c in cfgs if c.attenuation), "nspec": u.nspec, "nspec_fluid": u.nspec_acoustic,
-           "nglob": u.nglob, "nstep_union": nstep, "nstep_range": [min(c.nstep for c in cfgs), nstep],
+           "viscoelastic": sum(1 for c in cfgs if c.attenuation), "nspec": sum(b.nspec for b in runs.batches),
+           "nglob": sum(b.nglob for b in runs.batches), "nstep_range": [min(c.nstep for c in cfgs),
+                                                                       max(c.nstep for c in cfgs)],
            "value": useful / wall, "unit": "GLL-pt updates/s (useful)", "wall_s": wall,
-           "ms_per_step": wall * 1e3 / nstep, "union_updates_per_s": u.nglob * nstep / wall,
-           "padded_fraction": 1.0 - useful / (u.nglob * nstep), "gpu_loop_ms_events": timing["loop_ms"],
-           "setup_s": setup, "finite": bool(all(np.all(np.isfinite(s)) for s in seis))}
-    u.close()
+           "union_updates_per_s": union / wall, "padded_fraction": 1.0 - useful / union, "unions": groups,
+           "gpu_loop_ms_events": timing["loop_ms"], "setup_s": setup,
+           "finite": bool(all(np.all(np.isfinite(s)) for s in seis))}
+    runs.close()
     return out
 
 
@@ -426,6 +431,7 @@ def main():
     ap.add_argument("--no-uneven", action="store_true", help="skip the uneven batch of 48 instances")
     ap.add_argument("--uneven-only", type=int, default=0, metavar="N",
                     help="only the uneven batch, of N instances (48: the full workload)")
+    ap.add_argument("--uneven-groups", type=int, default=6, help="most unions of the uneven batch (1: one union)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="after the timed region, run a plain copy of this size (counter calibration)")
     args = ap.parse_args()
@@ -446,7 +452,7 @@ def main():
         print(json.dumps(fwi_gradient_batched(args.batch_shots, nstep=args.fwi_steps)))
         return
     if args.uneven_only:
-        print(json.dumps(c5_uneven(args.uneven_only)))
+        print(json.dumps(c5_uneven(args.uneven_only, max_groups=args.uneven_groups)))
         return
     if args.batch_forward_only:
         print(json.dumps(batch_forward(args.batch_forward_only, args.steps, args.shots_per_wave)))
@@ -523,7 +529,7 @@ def main():
         out["fwi_gradient"] = fwi_gradient()
         out["fwi_gradient_batched"] = fwi_gradient_batched(args.batch_shots, nstep=args.fwi_steps)
     if not args.no_uneven:
-        out["c5_uneven"] = c5_uneven()
+        out["c5_uneven"] = c5_uneven(max_groups=args.uneven_groups)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
     print(json.dumps(out))

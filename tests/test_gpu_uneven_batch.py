@@ -42,4 +42,25 @@ def test_uneven_batch_against_oracle_and_single_runs():
     # a second run from rest repeats the first up to the order of FP32 atomic additions
     worst = max(_rel(a, b) for a, b in zip(again, seis))
     print(f"second run vs first: {worst:.2e}; union loop {t['loop_ms']:.1f} ms")
-    assert worst < 1e-6
+    assert worst < 5e-6
+
+
+def test_uneven_runs_in_nstep_groups():
+    """The batch split into unions of similar NSTEP returns each instance's traces in input order, equal to the one
+    union's up to the order of FP32 atomic additions."""
+    from specfem2d_amd.driver import UnevenRuns
+    cfgs = config.uneven_batch_instances(n=5, seed=4, size_range=((10, 30), (8, 20)), nstep_range=(300, 900),
+                                         nelem_pml=3, nrec=3)
+    print("case: 5 instances in NSTEP groups, NSTEP " + ", ".join(str(c.nstep) for c in cfgs), flush=True)
+    runs = UnevenRuns(cfgs, max_groups=3, step_cost=0.0)  # these meshes are small: split them all the same
+    split, _ = runs.forward()
+    padded = 1 - runs.useful_updates() / runs.union_updates()
+    groups = runs.groups
+    runs.close()
+    one = UnevenBatch(cfgs)
+    whole, _ = one.forward()
+    one.close()
+    worst = max(_rel(a, b) for a, b in zip(split, whole))
+    print(f"groups {groups}, padded fraction {padded:.3f}; grouped vs one union: {worst:.2e}")
+    assert len(groups) > 1 and all(s.shape == (c.nrec, 2, c.nstep) for s, c in zip(split, cfgs))
+    assert worst < 5e-6

@@ -2,6 +2,7 @@
 fluid and viscoelastic solids, CPML and Stacey edges -- carries each instance's numbering, mass, source and receiver
 set-up at its offsets.  CPU only."""
 import numpy as np
+import pytest
 
 from specfem2d_amd import config
 from specfem2d_amd.driver import Simulation, UnevenBatch
@@ -43,3 +44,37 @@ def test_union_concatenates_the_instances():
         r0 += len(qi)
     # only the Stacey instance's absorbing sides are Stacey faces of the union
     assert u.nabs == singles[3].nabs > 0
+
+
+def _brute_force_cost(cfgs, max_groups, step_cost):
+    """Least modelled cost over every split of the NSTEP-sorted instances into at most max_groups runs."""
+    import itertools
+    from specfem2d_amd.driver import VISCO_POINT_COST
+    order = sorted(range(len(cfgs)), key=lambda k: cfgs[k].nstep)
+    n, best = len(order), float("inf")
+    for g in range(1, max_groups + 1):
+        for cuts in itertools.combinations(range(1, n), g - 1):
+            bounds = (0,) + cuts + (n,)
+            cost = 0.0
+            for a, b in zip(bounds[:-1], bounds[1:]):
+                idx = order[a:b]
+                w = sum(cfgs[k].nglob * (VISCO_POINT_COST if cfgs[k].attenuation else 1.0) for k in idx)
+                cost += max(cfgs[k].nstep for k in idx) * (step_cost + w)
+            best = min(best, cost)
+    return best
+
+
+def test_nstep_groups_minimise_the_modelled_work():
+    from specfem2d_amd.driver import VISCO_POINT_COST, plan_nstep_groups
+    cfgs = config.uneven_batch_instances(n=9, seed=11)
+    for max_groups, step_cost in ((1, 2e6), (3, 2e6), (4, 0.0), (9, 5e7)):
+        groups = plan_nstep_groups(cfgs, max_groups, step_cost)
+        assert sorted(k for g in groups for k in g) == list(range(len(cfgs))) and len(groups) <= max_groups
+        # each group a contiguous run in NSTEP order
+        ns = [[cfgs[k].nstep for k in g] for g in groups]
+        assert all(max(a) <= min(b) for a, b in zip(ns[:-1], ns[1:]))
+        cost = sum(max(cfgs[k].nstep for k in g) * (step_cost + sum(
+            cfgs[k].nglob * (VISCO_POINT_COST if cfgs[k].attenuation else 1.0) for k in g)) for g in groups)
+        assert cost == pytest.approx(_brute_force_cost(cfgs, max_groups, step_cost), rel=1e-12)
+    # no fixed cost and enough groups: every distinct NSTEP its own group, no padding
+    assert len(plan_nstep_groups(cfgs, 9, 0.0)) == len({c.nstep for c in cfgs})
