@@ -62,11 +62,6 @@ typedef struct {
 void* sf2d_sim_create(const sf2d_params* p, const double* rho, const double* vp, const double* vs, int nsrc,
                       const sf2d_source* src, int nrec, const double* rec_xz, int simulation_type,
                       int save_forward);
-/* A shot batch (no counterpart in the reference): nshot forward simulations of one mesh, model and receiver
-   set, shot k driven by the next shot_nsrc[k] sources of src (timed as that shot's own simulation: its t0 and
-   time shifts are those of a run with only its sources).  On the device all shots advance together
-   (sf2d_batch_* of specfem2d_gpu_ext.h); the gradient calls below then take obs[nshot][nrec][2][NSTEP],
-   return the misfit summed over the shots and the gradient of that sum. */
 /* the same for a mesh read from files (read_external_mesh, src/meshfem2D/read_external_mesh_files.F90): nspec 4-node
    elements over npgeo control nodes (knods [nspec][4] 0-based, counter-clockwise; coorg [npgeo][2]), the sides of the
    free-surface file (only those of fluid elements count) and of the absorbing file (Stacey edges when p->stacey) as
@@ -85,6 +80,11 @@ void* sf2d_sim_create_external(const sf2d_params* p, int nspec, int npgeo, const
    r of instance k is union receiver sum_{j<k} nrec_j + r.  The instances stay valid and are released by the caller;
    returns NULL when they do not fit together. */
 void* sf2d_sim_create_union(int n, void* const* sims);
+/* A shot batch (no counterpart in the reference): nshot forward simulations of one mesh, model and receiver
+   set, shot k driven by the next shot_nsrc[k] sources of src (timed as that shot's own simulation: its t0 and
+   time shifts are those of a run with only its sources).  On the device all shots advance together
+   (sf2d_batch_* of specfem2d_gpu_ext.h); the gradient calls below then take obs[nshot][nrec][2][NSTEP],
+   return the misfit summed over the shots and the gradient of that sum. */
 void* sf2d_sim_create_shots(const sf2d_params* p, const double* rho, const double* vp, const double* vs, int nshot,
                             const int* shot_nsrc, const sf2d_source* src, int nrec, const double* rec_xz);
 /* all NSTEP steps of every shot from rest; seis[nshot][nrec][2][NSTEP] displacement; timing[0] = device ms */
