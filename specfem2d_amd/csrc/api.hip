@@ -345,6 +345,11 @@ void upload_chain_ids(Mesh* mp) {
   }
   if (!mp->d_chain_ids) check_hip(hipMalloc(&mp->d_chain_ids, ne * sizeof(int)), "hipMalloc");
   check_hip(hipMemcpy(mp->d_chain_ids, ids.data(), ne * sizeof(int), hipMemcpyHostToDevice), "chain ids H2D");
+  if (!getenv("SF2D_ROW_CHAIN") && !mp->d_col_ids) {
+    mp->d_col_ids = device_zeros<int>(ne);
+    mp->d_col_mod = device_zeros<float2>(ne);
+  }
+  launch_col_pack(mp);
   device_free(mp->d_special);
   mp->nspecial = (int)list.size();
   if (!list.empty()) mp->d_special = device_copy(list.data(), list.size());
@@ -894,6 +899,8 @@ void prepare_cleanup_device_(long* Mesh_pointer, int* ACOUSTIC_SIMULATION, int* 
   device_free(mp->d_moduli);
   device_free(mp->d_geom_elem);
   device_free(mp->d_chain_ids);
+  device_free(mp->d_col_ids);
+  device_free(mp->d_col_mod);
   device_free(mp->d_special);
   if (mp->grad) sf2d_grad_release(Mesh_pointer);
   if (mp->batch) sf2d_batch_release(Mesh_pointer);

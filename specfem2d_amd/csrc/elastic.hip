@@ -706,6 +706,200 @@ __global__ __launch_bounds__(CHAIN_LANES) void k_elastic_forces_chain(
   }  // shots
 }
 
+// The chain kernel with lanes along columns: lane 5 k + i holds column i of element k (its five points (i, j),
+// j = 0..4), so that the wave's access to GLL row j of the patch touches consecutive points (the patch's points are
+// numbered row by row) and its element data, stored in lane order (Mesh::d_col_ids, d_col_mod), consecutive words.
+// The roles of the directions swap against the row-lane kernel: gamma-derivatives and the gamma-flux contraction run
+// along the lane's own column in registers, the xi direction goes through the two LDS transposes; the force of the
+// shared column (4, j) of element k moves to lane 5 (k + 1) by a one-lane shuffle.  Every sum has the row-lane
+// kernel's terms in its order, and the same (element, point) pairs write accel, so the first-touch flags carry over.
+template <bool FIRST, bool MULTI, bool RECT>
+__global__ __launch_bounds__(CHAIN_LANES) void k_elastic_forces_col(
+    int first_patch, int npatch, int nshot, int group, long shot_stride, const int4* __restrict__ desc,
+    const int* __restrict__ col_ids, const float4* __restrict__ geom, const float4* __restrict__ geom_elem,
+    const float2* __restrict__ col_mod, const float2* __restrict__ displ, float2* __restrict__ accel,
+    const Basis basis) {
+  __shared__ float2 s_t[Mesh::CHAIN_ELEMS + 1][NGLL][NGLL + 1];
+  __shared__ float2 s_v[Mesh::CHAIN_ELEMS + 1][NGLL][NGLL + 1];
+  __shared__ float s_hp[NGLL2], s_hpw[NGLL2];
+  const int lane = threadIdx.x;
+  if (lane < NGLL2) {
+    s_hp[lane] = basis.hp[lane];
+    s_hpw[lane] = basis.hpw[lane];
+  }
+  const int nw = gridDim.x;
+  const int item = (blockIdx.x % NXCD) * (nw / NXCD) + blockIdx.x / NXCD;
+  if (!MULTI) group = 1;
+  const int ngroup = (nshot + group - 1) / group;
+  const int nitem = npatch * ngroup;
+  if (item >= nitem) return;  // uniform over the wave
+  const int4 d = desc[first_patch + item / ngroup];
+  const int s0 = (item % ngroup) * group, s1 = min(nshot, s0 + group);
+  const int ne = patch_ne(d.w);
+  const int k = min(lane / NGLL, Mesh::CHAIN_ELEMS);  // element of the patch (CHAIN_ELEMS: lanes past it)
+  const int i = lane % NGLL;                          // this lane's column
+  const bool active = lane / NGLL < ne;
+  const int e = d.x + min(k, ne - 1);
+  const int row = NGLL * ne;                                   // words per GLL row of the patch in lane order
+  const unsigned c0 = (unsigned)d.x * NGLL2 + (unsigned)min(lane, row - 1);  // this lane's word of row 0
+  int g[NGLL];
+  bool first[NGLL];
+  float2 u[NGLL], mod[NGLL];
+  float4 geo[NGLL];
+#pragma unroll
+  for (int j = 0; j < NGLL; j++) {
+    const int id = ld_off(col_ids, 4u * (c0 + (unsigned)(j * row)));
+    g[j] = id & ~Mesh::FIRST_TOUCH;
+    first[j] = FIRST && (id & Mesh::FIRST_TOUCH);
+    mod[j] = ld_off(col_mod, 8u * (c0 + (unsigned)(j * row)));
+  }
+  if (RECT || patch_affine(d.w)) {
+    const float4 ge = ld_off(geom_elem, 16u * (unsigned)e);
+#pragma unroll
+    for (int j = 0; j < NGLL; j++) geo[j] = ge;
+  } else {
+#pragma unroll
+    for (int j = 0; j < NGLL; j++) geo[j] = ld_off(geom, 16u * ((unsigned)e * NGLL2 + j * NGLL + i));
+  }
+  const bool last = k == ne - 1;
+  const bool writer = active && (i < NGLL - 1 || last);  // the shared column is written by the next element's lane
+  float2 un[NGLL];
+#pragma unroll
+  for (int j = 0; j < NGLL; j++) un[j] = ld_off(displ + (long)s0 * shot_stride, 8u * (unsigned)g[j]);
+  for (int shot = s0; shot < s1; shot++) {
+  const long base = (long)shot * shot_stride;
+#pragma unroll
+  for (int j = 0; j < NGLL; j++) u[j] = un[j];
+  if (MULTI && shot + 1 < s1) {
+    const float2* dn = displ + base + shot_stride;
+#pragma unroll
+    for (int j = 0; j < NGLL; j++) un[j] = ld_off(dn, 8u * (unsigned)g[j]);
+  }
+  float2* as = accel + base;
+  float2 a[NGLL];
+#pragma unroll
+  for (int j = 0; j < NGLL; j++)
+    a[j] = (writer && !first[j]) ? ld_off((const float2*)as, 8u * (unsigned)g[j]) : make_float2(0.f, 0.f);
+
+  // column i of u into the tile; xi-derivatives by rows: this lane takes row r = i
+#pragma unroll
+  for (int j = 0; j < NGLL; j++) s_t[k][j][i] = u[j];
+  __syncthreads();
+  {
+    const int r = i;
+    float2 rw[NGLL];
+#pragma unroll
+    for (int m = 0; m < NGLL; m++) rw[m] = s_t[k][r][m];
+#pragma unroll
+    for (int c = 0; c < NGLL; c++) {
+      float2 dx = make_float2(0.f, 0.f);
+#pragma unroll
+      for (int m = 0; m < NGLL; m++) {
+        const float h = s_hp[c * NGLL + m];
+        dx.x += rw[m].x * h;
+        dx.y += rw[m].y * h;
+      }
+      s_v[k][r][c] = dx;  // d u / d xi at point (c, r)
+    }
+  }
+  __syncthreads();
+  // stress and fluxes at the column's points
+  float2 t1[NGLL], t2[NGLL];
+#pragma unroll
+  for (int j = 0; j < NGLL; j++) {
+    float dux_dga = 0.f, duz_dga = 0.f;
+#pragma unroll
+    for (int m = 0; m < NGLL; m++) {
+      const float h = s_hp[j * NGLL + m];
+      dux_dga += u[m].x * h;
+      duz_dga += u[m].y * h;
+    }
+    const float2 dxi = s_v[k][j][i];
+    const float kap = mod[j].x, mu = mod[j].y;
+    const float lambdal = kap - mu;
+    const float lp2m = kap + mu;
+    if (RECT) {  // xi_z = gamma_x = 0: J xi_x = 1 / gamma_z, J gamma_z = 1 / xi_x
+      const float xixl = geo[j].x, gazl = geo[j].w;
+      const float dux_dxl = dxi.x * xixl, dux_dzl = dux_dga * gazl;
+      const float duz_dxl = dxi.y * xixl, duz_dzl = duz_dga * gazl;
+      const float sxx = lp2m * dux_dxl + lambdal * duz_dzl;
+      const float szz = lp2m * duz_dzl + lambdal * dux_dxl;
+      const float sxz = mu * (duz_dxl + dux_dzl);
+      const float cx = 1.f / gazl, cz = 1.f / xixl;
+      t1[j] = make_float2(cx * sxx, cx * sxz);
+      t2[j] = make_float2(cz * sxz, cz * szz);
+    } else {
+      const float xixl = geo[j].x, xizl = geo[j].y, gaxl = geo[j].z, gazl = geo[j].w;
+      const float dux_dxl = dxi.x * xixl + dux_dga * gaxl;
+      const float dux_dzl = dxi.x * xizl + dux_dga * gazl;
+      const float duz_dxl = dxi.y * xixl + duz_dga * gaxl;
+      const float duz_dzl = dxi.y * xizl + duz_dga * gazl;
+      const float jac = 1.f / (xixl * gazl - gaxl * xizl);
+      const float sxx = lp2m * dux_dxl + lambdal * duz_dzl;
+      const float szz = lp2m * duz_dzl + lambdal * dux_dxl;
+      const float sxz = mu * (duz_dxl + dux_dzl);
+      t1[j] = make_float2(jac * (sxx * xixl + sxz * xizl), jac * (sxz * xixl + szz * xizl));
+      t2[j] = make_float2(jac * (sxx * gaxl + sxz * gazl), jac * (sxz * gaxl + szz * gazl));
+    }
+  }
+  __syncthreads();  // every lane has read its derivatives and the tile's u
+#pragma unroll
+  for (int j = 0; j < NGLL; j++) s_t[k][j][i] = t1[j];
+  __syncthreads();
+  {
+    const int r = i;
+    float2 rw[NGLL];
+#pragma unroll
+    for (int m = 0; m < NGLL; m++) rw[m] = s_t[k][r][m];
+#pragma unroll
+    for (int c = 0; c < NGLL; c++) {
+      float2 sx = make_float2(0.f, 0.f);
+#pragma unroll
+      for (int m = 0; m < NGLL; m++) {
+        const float h = s_hpw[m * NGLL + c];
+        sx.x += rw[m].x * h;
+        sx.y += rw[m].y * h;
+      }
+      s_v[k][r][c] = sx;  // xi-flux contraction at point (c, r)
+    }
+  }
+  __syncthreads();
+  float2 f[NGLL];
+  const float wi = basis.w[i];
+#pragma unroll
+  for (int j = 0; j < NGLL; j++) {
+    float sgx = 0.f, sgz = 0.f;
+#pragma unroll
+    for (int m = 0; m < NGLL; m++) {
+      const float h = s_hpw[m * NGLL + j];
+      sgx += t2[m].x * h;
+      sgz += t2[m].y * h;
+    }
+    const float2 sx = s_v[k][j][i];
+    const float wj = basis.w[j];
+    f[j] = make_float2(-(wj * sx.x + wi * sgx), -(wj * sx.y + wi * sgz));
+  }
+  // the shared column's force moves from element k - 1 (lane 5 k - 1) to element k (lane 5 k)
+#pragma unroll
+  for (int j = 0; j < NGLL; j++) {
+    const float fx = __shfl_up(f[j].x, 1, CHAIN_LANES);
+    const float fz = __shfl_up(f[j].y, 1, CHAIN_LANES);
+    if (i == 0 && k > 0) {
+      f[j].x += fx;
+      f[j].y += fz;
+    }
+  }
+  if (writer) {
+#pragma unroll
+    for (int j = 0; j < NGLL; j++) {
+      a[j].x += f[j].x;
+      a[j].y += f[j].y;
+      *reinterpret_cast<float2*>(reinterpret_cast<char*>(as) + 8u * (unsigned)g[j]) = a[j];
+    }
+  }
+  }  // shots
+}
+
 // Shot-minor batches (fields [point][shot]: ps = nshot shots per point): the lanes of a wave are shots.  A wave holds
 // 64 / L (patch, shot group) items, L = nshot rounded up to a power of two and at most 64 (lanes past the batch
 // idle), and each lane walks its item's chain element by element with the element's 25 points of its shot in
@@ -1276,6 +1470,19 @@ void launch_elastic_forces_patch(const Mesh* mp, int first_patch, int npatch, co
          {k_elastic_forces_chain<false, true, false>, k_elastic_forces_chain<false, true, true>}},
         {{k_elastic_forces_chain<true, false, false>, k_elastic_forces_chain<true, false, true>},
          {k_elastic_forces_chain<true, true, false>, k_elastic_forces_chain<true, true, true>}}};
+    if (mp->d_col_ids) {  // lanes along columns, element data in lane order
+      const K ckerns[2][2][2] = {
+          {{k_elastic_forces_col<false, false, false>, k_elastic_forces_col<false, false, true>},
+           {k_elastic_forces_col<false, true, false>, k_elastic_forces_col<false, true, true>}},
+          {{k_elastic_forces_col<true, false, false>, k_elastic_forces_col<true, false, true>},
+           {k_elastic_forces_col<true, true, false>, k_elastic_forces_col<true, true, true>}}};
+      hipLaunchKernelGGL(ckerns[first_touch ? 1 : 0][group > 1 ? 1 : 0][mp->rect ? 1 : 0], dim3(grid),
+                         dim3(CHAIN_LANES), 0, mp->stream, first_patch, npatch, nshot, group, shot_stride,
+                         (const int4*)mp->d_patch_desc, mp->d_col_ids, mp->d_geom, mp->d_geom_elem, mp->d_col_mod,
+                         (const float2*)displ, (float2*)accel, mp->basis);
+      check_hip(hipGetLastError(), "k_elastic_forces_col launch");
+      return;
+    }
     const K kern = kerns[first_touch ? 1 : 0][group > 1 ? 1 : 0][mp->rect ? 1 : 0];
     hipLaunchKernelGGL(kern, dim3(grid), dim3(CHAIN_LANES), 0, mp->stream, first_patch, npatch, nshot, group,
                        shot_stride, (const int4*)mp->d_patch_desc, mp->d_chain_ids, mp->d_geom, mp->d_geom_elem,
@@ -1324,11 +1531,36 @@ void launch_next_step(int* step, hipStream_t s) {
   check_hip(hipGetLastError(), "k_next_step launch");
 }
 
+// the column-lane kernel's lane-order copies of a patch's point ids and moduli (one workgroup per patch)
+__global__ __launch_bounds__(256) void k_col_pack(int npatch, const int4* __restrict__ desc, const int* __restrict__ ids,
+                                                  const float2* __restrict__ mod, int* __restrict__ col_ids,
+                                                  float2* __restrict__ col_mod) {
+  const int q = blockIdx.x;
+  if (q >= npatch) return;
+  const int4 d = desc[q];
+  const int ne = patch_ne(d.w);
+  const size_t e0 = (size_t)d.x * NGLL2;
+  for (int t = threadIdx.x; t < ne * NGLL2; t += blockDim.x) {
+    const int k = t / NGLL2, p = t % NGLL2, j = p / NGLL, i = p % NGLL;
+    const size_t dst = e0 + (size_t)j * NGLL * ne + NGLL * k + i;
+    col_ids[dst] = ids[e0 + t];
+    col_mod[dst] = mod[e0 + t];
+  }
+}
+
+void launch_col_pack(const Mesh* mp) {
+  if (!mp->chain || !mp->d_col_ids || !mp->d_chain_ids || !mp->d_moduli || mp->npatch <= 0) return;
+  hipLaunchKernelGGL(k_col_pack, dim3(mp->npatch), dim3(256), 0, mp->stream, mp->npatch,
+                     (const int4*)mp->d_patch_desc, mp->d_chain_ids, mp->d_moduli, mp->d_col_ids, mp->d_col_mod);
+  check_hip(hipGetLastError(), "k_col_pack launch");
+}
+
 void launch_pack_elements(const Mesh* mp) {
   const size_t n = (size_t)mp->nspec * NGLL2;
   hipLaunchKernelGGL(k_pack_elements, dim3(blocks_for((long)n, 256, 65536)), dim3(256), 0, mp->stream, n, mp->d_xix,
                      mp->d_xiz, mp->d_gammax, mp->d_gammaz, mp->d_kappav, mp->d_muv, mp->d_geom, mp->d_moduli);
   check_hip(hipGetLastError(), "k_pack_elements launch");
+  launch_col_pack(mp);
 }
 
 int launch_flag_affine(Mesh* mp) {

@@ -54,31 +54,65 @@ __device__ __forceinline__ T ld_off(const T* base, unsigned byte_off) {  // 32-b
 }
 constexpr int GEPB = 10;  // elements per workgroup in k_grad_elastic
 
-// fields of shot s hold point g, component c at float 2 (g ps + s ss) + c
-__global__ __launch_bounds__(GT) void k_adj_pointwise(int first, int n, int nshot, int ps, long ss, float dt, float dto2,
-                                                      float h, const float* __restrict__ D, float* __restrict__ Vb,
-                                                      float* __restrict__ Rb, const float* __restrict__ rmx,
-                                                      const float* __restrict__ rmz, const float* __restrict__ r,
-                                                      float* __restrict__ pm) {
-  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
-    const int g = k >> 1;
-    const float w = (k & 1) ? rmz[g] : rmx[g];
-    float dm = 0.f;
-    const size_t q0 = (size_t)2 * ((size_t)g * ps) + (k & 1);
-    for (int s = 0; s < nshot; s++) {
-      const size_t q = q0 + (size_t)2 * s * ss;
-      float v = 0.f, a = 0.f;
-      if (!first) {  // state cotangents of the step after this one
-        const float vb = Vb[q], db = D[q];
-        v = vb + dt * db;
-        a = dto2 * vb + h * db;
+// fields of shot s hold point g at float2 g + s ss (shot-major batches and single shots); one thread per point,
+// both components at once, the shots four at a time so that twelve loads are in flight per thread
+__device__ __forceinline__ void adj_point(int first, float dt, float dto2, float h, float2 w, float2 vb, float2 db,
+                                          float2 rr, float2& v_out, float2& r_out, float2& dm) {
+  float2 v = make_float2(0.f, 0.f), a = make_float2(0.f, 0.f);
+  if (!first) {  // state cotangents of the step after this one
+    v.x = vb.x + dt * db.x;
+    v.y = vb.y + dt * db.y;
+    a.x = dto2 * vb.x + h * db.x;
+    a.y = dto2 * vb.y + h * db.y;
+  }
+  a.x = a.x + dto2 * v.x;
+  a.y = a.y + dto2 * v.y;
+  v_out = v;
+  r_out = make_float2(w.x * a.x, w.y * a.y);
+  dm.x += a.x * rr.x * w.x * w.x;
+  dm.y += a.y * rr.y * w.y * w.y;
+}
+
+__global__ __launch_bounds__(GT) void k_adj_pointwise(int first, int npts, int nshot, long ss, float dt, float dto2,
+                                                      float h, const float2* __restrict__ D, float2* __restrict__ Vb,
+                                                      float2* __restrict__ Rb, const float* __restrict__ rmx,
+                                                      const float* __restrict__ rmz, const float2* __restrict__ r,
+                                                      float2* __restrict__ pm) {
+  constexpr int U = 4;
+  for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < npts; g += gridDim.x * blockDim.x) {
+    const float2 w = make_float2(rmx[g], rmz[g]);
+    float2 dm = make_float2(0.f, 0.f);
+    int s = 0;
+    for (; s + U <= nshot; s += U) {
+      float2 vb[U], db[U], rr[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const size_t q = (size_t)g + (size_t)(s + u) * ss;
+        vb[u] = first ? make_float2(0.f, 0.f) : Vb[q];
+        db[u] = first ? make_float2(0.f, 0.f) : D[q];
+        rr[u] = r[q];
       }
-      a = a + dto2 * v;
-      Vb[q] = v;
-      Rb[q] = w * a;
-      dm += a * r[q] * w * w;
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const size_t q = (size_t)g + (size_t)(s + u) * ss;
+        float2 vo, ro;
+        adj_point(first, dt, dto2, h, w, vb[u], db[u], rr[u], vo, ro, dm);
+        Vb[q] = vo;
+        Rb[q] = ro;
+      }
     }
-    pm[k] -= dm;
+    for (; s < nshot; s++) {
+      const size_t q = (size_t)g + (size_t)s * ss;
+      float2 vo, ro;
+      adj_point(first, dt, dto2, h, w, first ? make_float2(0.f, 0.f) : Vb[q], first ? make_float2(0.f, 0.f) : D[q],
+                r[q], vo, ro, dm);
+      Vb[q] = vo;
+      Rb[q] = ro;
+    }
+    float2 p = pm[g];
+    p.x -= dm.x;
+    p.y -= dm.y;
+    pm[g] = p;
   }
 }
 
@@ -129,14 +163,24 @@ __global__ __launch_bounds__(GT) void k_grad_elastic(int nelem, int nshot, int p
   }
   const int base = le * NGLL2;
   float sk = 0.f, sm = 0.f;
+  // the next shot's two values are loaded while the current one is computed
+  float2 dnext = make_float2(0.f, 0.f), bnext = dnext;
+  if (active) {
+    dnext = dn[(size_t)g * ps];
+    bnext = rb[(size_t)g * ps];
+  }
   for (int s = 0; s < nshot; s++) {
     if (s > 0) __syncthreads();  // the previous shot's LDS reads are done
     if (active) {
-      const size_t q = (size_t)g * ps + (size_t)s * ss;
-      s_d[tid] = dn[q];
-      s_b[tid] = rb[q];
+      s_d[tid] = dnext;
+      s_b[tid] = bnext;
     }
     __syncthreads();
+    if (active && s + 1 < nshot) {
+      const size_t q = (size_t)g * ps + (size_t)(s + 1) * ss;
+      dnext = dn[q];
+      bnext = rb[q];
+    }
     if (active) {
       float a[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
 #pragma unroll
@@ -457,8 +501,9 @@ void sf2d_grad_reverse_step(long* Mesh_pointer, int n, int slot, int first, floa
                        (long)G->nfield, lshift, dt, dto2, h, G->D, G->Vb, G->Rb, mp->d_rmassx, mp->d_rmassz, rn,
                        G->pm32);
   else
-    hipLaunchKernelGGL(k_adj_pointwise, dim3(grid_for(G->n2)), dim3(GT), 0, mp->stream, first, (int)G->n2, G->nshot,
-                       G->ps, G->stride, dt, dto2, h, G->D, G->Vb, G->Rb, mp->d_rmassx, mp->d_rmassz, rn, G->pm32);
+    hipLaunchKernelGGL(k_adj_pointwise, dim3(grid_for(G->n2 / 2)), dim3(GT), 0, mp->stream, first, (int)(G->n2 / 2),
+                       G->nshot, G->stride, dt, dto2, h, (const float2*)G->D, (float2*)G->Vb, (float2*)G->Rb,
+                       mp->d_rmassx, mp->d_rmassz, (const float2*)rn, (float2*)G->pm32);
   if (G->nrec > 0)
     hipLaunchKernelGGL(k_adj_receivers, dim3(G->nrec, G->nshot), dim3(64), 0, mp->stream, G->nrec,
                        G->res + (size_t)(n - 1) * 2 * G->nrec * G->nshot, mp->d_ispec_selected_rec, mp->d_ibool,

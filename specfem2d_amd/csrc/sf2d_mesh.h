@@ -100,6 +100,11 @@ struct Mesh {
                                      // keeps the general patch kernel)
   int* d_chain_ids = nullptr;        // chain kernel: [nspec][25] device order, internal point id | FIRST_TOUCH
   std::vector<int> h_chain_ids;      // the same ids without flags, and the first-touch flags of the plan
+  // the column-lane chain kernel's copies of the point ids and moduli in lane order: the block of a patch of ne
+  // elements starting at position e0 holds element point (k, i, j) at e0 * 25 + j * 5 ne + 5 k + i, so that the
+  // wave's load of GLL row j reads consecutive words (SF2D_ROW_CHAIN keeps the row-lane kernel)
+  int* d_col_ids = nullptr;
+  float2* d_col_mod = nullptr;
   std::vector<uint8_t> h_first;
   // First-touch assembly (forward wavefield of SIMULATION_TYPE=1 and shot batches; SF2D_NO_FIRST_TOUCH turns it
   // off): the predictor leaves accel as it is except at the special points (Stacey edges, source elements),
@@ -240,6 +245,8 @@ void pml_acoustic_forces(Mesh* mp, int ph, const float* pot, const float* pd, fl
 void pml_acoustic_boundary(Mesh* mp, float* pot, float* pd, float* pdd);
 // chain-kernel point ids with first-touch flags and the special points, uploaded again when the special points change
 void upload_chain_ids(Mesh* mp);
+// fills the column-lane copies of the chain ids and moduli from d_chain_ids and d_moduli (when both are allocated)
+void launch_col_pack(const Mesh* mp);
 // point arrays in the caller's numbering <-> the device numbering (n2 values per point)
 void points_to_device(const Mesh* mp, float* dev, const float* host, int n2, const char* what);
 void points_from_device(const Mesh* mp, float* host, const float* dev, int n2, const char* what);
