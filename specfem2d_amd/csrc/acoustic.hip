@@ -55,10 +55,11 @@ __global__ __launch_bounds__(256) void k_acoustic_forces(int first, int n, const
   const int i = p % NGLL, j = p / NGLL;
   int g = 0;
   float4 gm = make_float4(0.f, 0.f, 0.f, 0.f);
-  float f = 0.f;
+  float f = 0.f, pdd_old = 0.f;
   if (active) {
     const size_t l = (size_t)e * NGLL2 + p;
     g = ids[l];
+    pdd_old = pdd[g];  // no other element of this colour has the point: read with the potential, in one round trip
     if (RECT) {
       const float2 r = geom_rect[e];
       gm = make_float4(r.x, 0.f, 0.f, r.y);
@@ -95,7 +96,7 @@ __global__ __launch_bounds__(256) void k_acoustic_forces(int first, int n, const
       a1 = a1 + s_t1[le][j * NGLL + k] * b.hpw[k * NGLL + i];
       a2 = a2 + s_t2[le][k * NGLL + i] * b.hpw[k * NGLL + j];
     }
-    pdd[g] = pdd[g] - (b.w[j] * a1 + b.w[i] * a2);
+    pdd[g] = pdd_old - (b.w[j] * a1 + b.w[i] * a2);
   }
 }
 

@@ -189,7 +189,9 @@ __global__ __launch_bounds__(FORCE_THREADS) void k_elastic_forces(
   // attenuation state of this point, loaded with the element's other inputs so the loads overlap
   float4 cf[3];
   float ev[9], ov[3];
+  float2 a_old = make_float2(0.f, 0.f);  // coloured launches: no other element of the launch has the point
   if (active) {
+    if (!ATOMIC) a_old = reinterpret_cast<const float2*>(accel)[ibool[(size_t)(first + e) * NGLL2 + p]];
     const size_t l = (size_t)(first + e) * NGLL2 + p;
     if (ATT) {
       const float4* coef = att_table + 3 * (size_t)att_idx[l];
@@ -285,11 +287,10 @@ __global__ __launch_bounds__(FORCE_THREADS) void k_elastic_forces(
       atomic_add_f32(&accel[2 * (size_t)g], fx);
       atomic_add_f32(&accel[2 * (size_t)g + 1], fz);
     } else {
-      float2* a2 = reinterpret_cast<float2*>(accel) + g;
-      float2 a = *a2;
+      float2 a = a_old;
       a.x += fx;
       a.y += fz;
-      *a2 = a;
+      reinterpret_cast<float2*>(accel)[g] = a;
     }
   }
 }

@@ -106,13 +106,14 @@ __global__ __launch_bounds__(256) void k_pml_elastic_forces(
   const size_t q = (size_t)(first + (active ? e : 0)) * NGLL2 + p;
   int g = 0;
   float4 gm = make_float4(0.f, 0.f, 0.f, 0.f);
-  float2 md = make_float2(0.f, 0.f), u = md, o = md, v = md;
+  float2 md = make_float2(0.f, 0.f), u = md, o = md, v = md, a_old = md;
   float w = 0.f;
   float r[EL_PLANES];
   Coefs c;
   if (active) {
     g = ids[q];
     u = displ[g];
+    a_old = accel[g];  // no other element of this colour has the point: read with the fields, in one round trip
     o = old[q];
     v = veloc[g];
     gm = geom[q];
@@ -246,7 +247,7 @@ __global__ __launch_bounds__(256) void k_pml_elastic_forces(
       t2x += s_t2x[base + k * NGLL + i] * wj;
       t2z += s_t2z[base + k * NGLL + i] * wj;
     }
-    float2 a = accel[g];
+    float2 a = a_old;
     a.x = (a.x - (s_w[j] * t1x + s_w[i] * t2x)) - pacc.x;
     a.y = (a.y - (s_w[j] * t1z + s_w[i] * t2z)) - pacc.y;
     accel[g] = a;
@@ -276,13 +277,14 @@ __global__ __launch_bounds__(256) void k_pml_acoustic_forces(
   const size_t q = (size_t)(first + (active ? e : 0)) * NGLL2 + p;
   int g = 0;
   float4 gm = make_float4(0.f, 0.f, 0.f, 0.f);
-  float u = 0.f, o = 0.f, v = 0.f, f = 0.f, w = 0.f;
+  float u = 0.f, o = 0.f, v = 0.f, f = 0.f, w = 0.f, pdd_old = 0.f;
   float r[AC_PLANES];
   Coefs c;
   if (active) {
     g = ids[q];
     u = pot[g];
     v = pd[g];
+    pdd_old = pdd[g];  // no other element of this colour has the point
     o = old[q];
     gm = geom[q];
     f = fac[q].x;
@@ -330,7 +332,7 @@ __global__ __launch_bounds__(256) void k_pml_acoustic_forces(
       a1 += s_t1[base + j * NGLL + k] * s_hpw[k * NGLL + i];
       a2 += s_t2[base + k * NGLL + i] * s_hpw[k * NGLL + j];
     }
-    pdd[g] = (pdd[g] - (s_w[j] * a1 + s_w[i] * a2)) - pacc;
+    pdd[g] = (pdd_old - (s_w[j] * a1 + s_w[i] * a2)) - pacc;
   }
 }
 
