@@ -338,7 +338,8 @@ def c2_elastic(steps: int, warmup: int):
     achieved = fk_bytes / (fk_ms * 1e-3) / 1e9
     roofline = {
         "bound": "hbm",
-        "kernel": ("k_elastic_forces_chain" if chain else "k_elastic_forces_patch") if patch_mode
+        "kernel": (("k_elastic_forces_chain" if os.environ.get("SF2D_ROW_CHAIN") else "k_elastic_forces_col")
+                   if chain else "k_elastic_forces_patch") if patch_mode
                   else "k_elastic_forces",
         "measured_on": name,
         "launches_per_step": launches,
