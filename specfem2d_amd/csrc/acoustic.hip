@@ -38,6 +38,10 @@ __device__ __forceinline__ void atomic_add(float* p, float v) { unsafeAtomicAdd(
 // (compute_forces_viscoacoustic.f90:107-330; Kernel_2_acoustic_impl in src/gpu/kernels).  RECT: every element is
 // an axis-aligned rectangle (xi_z = gamma_x = 0, xi_x and gamma_z constant over the element), whose geometry is one
 // float2 {xi_x, gamma_z} per element instead of a float4 per point.
+// Threads go GLL row by GLL row across the workgroup's nb elements (thread t: row j = t / 5 nb, element
+// (t % 5 nb) / 5, column t % 5), and the packed per-point arrays follow the same order (row_block_pos), so that a
+// wave's gather of the potential reads GLL row j of ten neighbouring elements: a few cache lines instead of one or
+// two per element.
 template <bool RECT>
 __global__ __launch_bounds__(256) void k_acoustic_forces(int first, int n, const int* __restrict__ ids,
                                                          const float4* __restrict__ geom,
@@ -47,17 +51,27 @@ __global__ __launch_bounds__(256) void k_acoustic_forces(int first, int n, const
   __shared__ float s_u[AC_EPB][NGLL2];
   __shared__ float s_t1[AC_EPB][NGLL2];
   __shared__ float s_t2[AC_EPB][NGLL2];
+  __shared__ float s_hp[NGLL2], s_hpw[NGLL2], s_w[NGLL];  // the basis, read per thread from LDS (not kernarg memory)
   const int t = threadIdx.x;
-  const int le = t / NGLL2;
-  const int p = t - le * NGLL2;
-  const int e = first + blockIdx.x * AC_EPB + le;
-  const bool active = le < AC_EPB && e < first + n;
-  const int i = p % NGLL, j = p / NGLL;
+  if (t < NGLL2) {
+    s_hp[t] = b.hp[t];
+    s_hpw[t] = b.hpw[t];
+  }
+  if (t < NGLL) s_w[t] = b.w[t];
+  const int e0 = first + blockIdx.x * AC_EPB;  // the workgroup's first element
+  const int nb = min(AC_EPB, first + n - e0);  // its elements
+  const int rw = NGLL * nb;                    // threads per GLL row
+  const bool active = t < NGLL2 * nb;
+  const int j = active ? t / rw : 0;
+  const int le = active ? (t - j * rw) / NGLL : 0;
+  const int i = active ? t % NGLL : 0;
+  const int p = j * NGLL + i;
+  const int e = e0 + le;
   int g = 0;
   float4 gm = make_float4(0.f, 0.f, 0.f, 0.f);
   float f = 0.f, pdd_old = 0.f;
   if (active) {
-    const size_t l = (size_t)e * NGLL2 + p;
+    const size_t l = (size_t)e0 * NGLL2 + t;  // row-block order
     g = ids[l];
     pdd_old = pdd[g];  // no other element of this colour has the point: read with the potential, in one round trip
     if (RECT) {
@@ -72,12 +86,12 @@ __global__ __launch_bounds__(256) void k_acoustic_forces(int first, int n, const
   __syncthreads();
   if (active) {
     const float* u = s_u[le];
-    const float dxi = u[j * NGLL + 0] * b.hp[i * NGLL + 0] + u[j * NGLL + 1] * b.hp[i * NGLL + 1] +
-                      u[j * NGLL + 2] * b.hp[i * NGLL + 2] + u[j * NGLL + 3] * b.hp[i * NGLL + 3] +
-                      u[j * NGLL + 4] * b.hp[i * NGLL + 4];
-    const float dga = u[0 * NGLL + i] * b.hp[j * NGLL + 0] + u[1 * NGLL + i] * b.hp[j * NGLL + 1] +
-                      u[2 * NGLL + i] * b.hp[j * NGLL + 2] + u[3 * NGLL + i] * b.hp[j * NGLL + 3] +
-                      u[4 * NGLL + i] * b.hp[j * NGLL + 4];
+    const float dxi = u[j * NGLL + 0] * s_hp[i * NGLL + 0] + u[j * NGLL + 1] * s_hp[i * NGLL + 1] +
+                      u[j * NGLL + 2] * s_hp[i * NGLL + 2] + u[j * NGLL + 3] * s_hp[i * NGLL + 3] +
+                      u[j * NGLL + 4] * s_hp[i * NGLL + 4];
+    const float dga = u[0 * NGLL + i] * s_hp[j * NGLL + 0] + u[1 * NGLL + i] * s_hp[j * NGLL + 1] +
+                      u[2 * NGLL + i] * s_hp[j * NGLL + 2] + u[3 * NGLL + i] * s_hp[j * NGLL + 3] +
+                      u[4 * NGLL + i] * s_hp[j * NGLL + 4];
     if (RECT) {
       s_t1[le][p] = f * (gm.x * (dxi * gm.x));
       s_t2[le][p] = f * (gm.w * (dga * gm.w));
@@ -93,10 +107,10 @@ __global__ __launch_bounds__(256) void k_acoustic_forces(int first, int n, const
     float a1 = 0.f, a2 = 0.f;
 #pragma unroll
     for (int k = 0; k < NGLL; k++) {
-      a1 = a1 + s_t1[le][j * NGLL + k] * b.hpw[k * NGLL + i];
-      a2 = a2 + s_t2[le][k * NGLL + i] * b.hpw[k * NGLL + j];
+      a1 = a1 + s_t1[le][j * NGLL + k] * s_hpw[k * NGLL + i];
+      a2 = a2 + s_t2[le][k * NGLL + i] * s_hpw[k * NGLL + j];
     }
-    pdd[g] = pdd_old - (b.w[j] * a1 + b.w[i] * a2);
+    pdd[g] = pdd_old - (s_w[j] * a1 + s_w[i] * a2);
   }
 }
 
@@ -434,9 +448,31 @@ AcousticState* ac_ready(Mesh* mp, const char* who) {
         ids[l] = iv.id(e, p);
       }
     }
-    ac->d_ids = device_copy(ids.data(), n);
-    ac->d_geom = device_copy(geom.data(), n);
-    ac->d_fac = device_copy(fac.data(), n);
+    {
+      // per-point arrays in the force kernel's row-block order: workgroup blocks of AC_EPB elements of one colour,
+      // inside a block of nb elements point (j, i) of element le at j * 5 nb + 5 le + i
+      std::vector<int> ids_b(n);
+      std::vector<float4> geom_b(n);
+      std::vector<float> fac_b(n);
+      for (int ph = 0; ph < 2; ph++)
+        for (int c = 0; c < ac->ncolour[ph]; c++) {
+          const int cs = ac->colour_start[ph][c], cn = ac->colour_count[ph][c];
+          for (int b0 = 0; b0 < cn; b0 += AC_EPB) {
+            const int nb = std::min(AC_EPB, cn - b0);
+            for (int le = 0; le < nb; le++)
+              for (int p = 0; p < NGLL2; p++) {
+                const size_t src = (size_t)(cs + b0 + le) * NGLL2 + p;
+                const size_t dst = (size_t)(cs + b0) * NGLL2 + (size_t)(p / NGLL) * NGLL * nb + NGLL * le + p % NGLL;
+                ids_b[dst] = ids[src];
+                geom_b[dst] = geom[src];
+                fac_b[dst] = fac[src];
+              }
+          }
+        }
+      ac->d_ids = device_copy(ids_b.data(), n);
+      ac->d_geom = device_copy(geom_b.data(), n);
+      ac->d_fac = device_copy(fac_b.data(), n);
+    }
     // rectangles: xi_z, gamma_x zero and xi_x, gamma_z constant over each element, to 2^-22 of the largest entry
     // (meshers leave cancellation residues) -- SF2D_NO_RECT keeps the per-point geometry
     bool rect = !getenv("SF2D_NO_RECT") && !order.empty();

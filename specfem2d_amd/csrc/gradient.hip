@@ -162,6 +162,7 @@ __global__ __launch_bounds__(GT) void k_grad_elastic(int nelem, int nshot, int p
     H = gammaz[l];
   }
   const int base = le * NGLL2;
+  const float wij = basis.w[i] * basis.w[j];  // read once, not per shot
   float sk = 0.f, sm = 0.f;
   // the next shot's two values are loaded while the current one is computed
   float2 dnext = make_float2(0.f, 0.f), bnext = dnext;
@@ -193,7 +194,7 @@ __global__ __launch_bounds__(GT) void k_grad_elastic(int nelem, int nshot, int p
       }
       const float ux = a[0] * X + a[2] * G, uxz = a[0] * Z + a[2] * H, uzx = a[1] * X + a[3] * G, uz = a[1] * Z + a[3] * H;
       const float wx = a[4] * X + a[6] * G, wxz = a[4] * Z + a[6] * H, wzx = a[5] * X + a[7] * G, wz = a[5] * Z + a[7] * H;
-      const float wj = basis.w[i] * basis.w[j] / (X * H - G * Z);
+      const float wj = wij / (X * H - G * Z);
       sk += wj * (ux + uz) * (wx + wz);
       sm += wj * (ux * wx + uz * wz - ux * wz - uz * wx + (uxz + uzx) * (wxz + wzx));
     }
