@@ -416,6 +416,29 @@ void sort_elements_by_colour(Mesh* mp, const int* phase_lists, int np) {
     launch_pack_elements(mp);
     mp->no_affine = getenv("SF2D_NO_AFFINE") != nullptr;
     mp->n_affine = launch_flag_affine(mp);
+  } else if (!getenv("SF2D_NO_RECT")) {
+    // the per-point kernel's rectangle form: xi_z, gamma_x zero and xi_x, gamma_z constant over each element, to
+    // 2^-22 of the element's largest entry (meshers leave cancellation residues)
+    std::vector<float> gx(ne), gz(ne), ga(ne), gc(ne);
+    check_hip(hipMemcpy(gx.data(), mp->d_xix, ne * sizeof(float), hipMemcpyDeviceToHost), "xix D2H");
+    check_hip(hipMemcpy(gz.data(), mp->d_xiz, ne * sizeof(float), hipMemcpyDeviceToHost), "xiz D2H");
+    check_hip(hipMemcpy(ga.data(), mp->d_gammax, ne * sizeof(float), hipMemcpyDeviceToHost), "gammax D2H");
+    check_hip(hipMemcpy(gc.data(), mp->d_gammaz, ne * sizeof(float), hipMemcpyDeviceToHost), "gammaz D2H");
+    std::vector<float> rect((size_t)2 * nspec);
+    bool all = nspec > 0;
+    for (int e = 0; e < nspec && all; e++) {
+      const size_t l0 = (size_t)e * NGLL2;
+      float m = 0.f;
+      for (int p = 0; p < NGLL2; p++)
+        m = std::max({m, std::fabs(gx[l0 + p]), std::fabs(gz[l0 + p]), std::fabs(ga[l0 + p]), std::fabs(gc[l0 + p])});
+      const float tol = m * 0x1p-22f;
+      for (int p = 0; p < NGLL2 && all; p++)
+        all = std::fabs(gz[l0 + p]) <= tol && std::fabs(ga[l0 + p]) <= tol && std::fabs(gx[l0 + p] - gx[l0]) <= tol &&
+              std::fabs(gc[l0 + p] - gc[l0]) <= tol;
+      rect[2 * e] = gx[l0];
+      rect[2 * e + 1] = gc[l0];
+    }
+    if (all) mp->d_geom_rect = device_copy(rect.data(), rect.size());
   }
   {
     // ibool in device element order and, in patch mode, internal point numbering
@@ -898,6 +921,7 @@ void prepare_cleanup_device_(long* Mesh_pointer, int* ACOUSTIC_SIMULATION, int* 
   device_free(mp->d_geom);
   device_free(mp->d_moduli);
   device_free(mp->d_geom_elem);
+  device_free(mp->d_geom_rect);
   device_free(mp->d_chain_ids);
   device_free(mp->d_col_ids);
   device_free(mp->d_col_mod);

@@ -153,14 +153,16 @@ __global__ __launch_bounds__(256) void k_zero_points(float2* __restrict__ a, con
 // {A_kappa, B_kappa, A_mu, B_mu} x 3 solids from a small table through a per-point index; e and old are
 // [plane][ne] arrays in device element order (planes e1, e11, e13 per solid; previous dux_dxl, duz_dzl,
 // dux_dzl + duz_dxl)
-template <bool ATOMIC, bool ATT>
+// RECT: every element an axis-aligned rectangle (xi_z = gamma_x = 0, xi_x and gamma_z constant over the element):
+// one float2 {xi_x, gamma_z} per element (geom_rect, device element order) instead of four per-point arrays
+template <bool ATOMIC, bool ATT, bool RECT>
 __global__ __launch_bounds__(FORCE_THREADS) void k_elastic_forces(
     int first, int nelem, int nbatch, int per_xcd, const int* __restrict__ ibool,
     const float* __restrict__ xix, const float* __restrict__ xiz, const float* __restrict__ gammax,
     const float* __restrict__ gammaz, const float* __restrict__ kappav, const float* __restrict__ muv,
     const float2* __restrict__ displ, float* __restrict__ accel, const Basis basis,
     const unsigned int* __restrict__ att_idx, const float4* __restrict__ att_table, float* __restrict__ att_e,
-    float* __restrict__ att_old, long ne) {
+    float* __restrict__ att_old, long ne, const float2* __restrict__ geom_rect) {
   // XCD-aware remap: blocks b, b+8, b+16, ... (one XCD) take consecutive batches
   const int bid = blockIdx.x;
   const int batch = (bid % NXCD) * per_xcd + bid / NXCD;
@@ -206,10 +208,16 @@ __global__ __launch_bounds__(FORCE_THREADS) void k_elastic_forces(
     const float2 u = displ[g];
     s_ux[tid] = u.x;
     s_uz[tid] = u.y;
-    xixl = xix[l];
-    xizl = xiz[l];
-    gaxl = gammax[l];
-    gazl = gammaz[l];
+    if (RECT) {
+      const float2 r = geom_rect[first + e];
+      xixl = r.x;
+      gazl = r.y;
+    } else {
+      xixl = xix[l];
+      xizl = xiz[l];
+      gaxl = gammax[l];
+      gazl = gammaz[l];
+    }
     kap = kappav[l];
     mu = muv[l];
   }
@@ -1429,13 +1437,17 @@ void launch_elastic_forces(const Mesh* mp, int first, int nelem, int atomic, con
   const int nbatch = (nelem + FORCE_EPB - 1) / FORCE_EPB;
   const int per_xcd = (nbatch + NXCD - 1) / NXCD;
   const int nblocks = per_xcd * NXCD;
-  using K = decltype(&k_elastic_forces<true, true>);
-  const K kernels[2][2] = {{k_elastic_forces<false, false>, k_elastic_forces<false, true>},
-                           {k_elastic_forces<true, false>, k_elastic_forces<true, true>}};
-  hipLaunchKernelGGL(kernels[atomic ? 1 : 0][att ? 1 : 0], dim3(nblocks), dim3(FORCE_THREADS), 0, mp->stream, first,
-                     nelem, nbatch, per_xcd, mp->d_ibool, mp->d_xix, mp->d_xiz, mp->d_gammax, mp->d_gammaz,
-                     mp->d_kappav, mp->d_muv, (const float2*)displ, accel, mp->basis, mp->d_att_idx,
-                     mp->d_att_table, mp->d_att_e, mp->d_att_old, (long)mp->nspec * NGLL2);
+  using K = decltype(&k_elastic_forces<true, true, true>);
+  const K kernels[2][2][2] = {
+      {{k_elastic_forces<false, false, false>, k_elastic_forces<false, false, true>},
+       {k_elastic_forces<false, true, false>, k_elastic_forces<false, true, true>}},
+      {{k_elastic_forces<true, false, false>, k_elastic_forces<true, false, true>},
+       {k_elastic_forces<true, true, false>, k_elastic_forces<true, true, true>}}};
+  hipLaunchKernelGGL(kernels[atomic ? 1 : 0][att ? 1 : 0][mp->d_geom_rect ? 1 : 0], dim3(nblocks), dim3(FORCE_THREADS),
+                     0, mp->stream, first, nelem, nbatch, per_xcd, mp->d_ibool, mp->d_xix, mp->d_xiz, mp->d_gammax,
+                     mp->d_gammaz, mp->d_kappav, mp->d_muv, (const float2*)displ, accel, mp->basis, mp->d_att_idx,
+                     mp->d_att_table, mp->d_att_e, mp->d_att_old, (long)mp->nspec * NGLL2,
+                     (const float2*)mp->d_geom_rect);
   check_hip(hipGetLastError(), "k_elastic_forces launch");
 }
 

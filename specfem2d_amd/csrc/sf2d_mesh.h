@@ -89,6 +89,9 @@ struct Mesh {
   float4* d_geom = nullptr;          // [nspec][25] {xix, xiz, gammax, gammaz}, device element order (patch mode)
   float2* d_moduli = nullptr;        // [nspec][25] {kappa, mu}, device element order (patch mode)
   float4* d_geom_elem = nullptr;     // [nspec] geometry of point 0; the whole element's when it is affine
+  // coloured and atomic assembly (the per-point kernel): {xi_x, gamma_z} per element, device element order, when
+  // every element is an axis-aligned rectangle (SF2D_NO_RECT keeps the per-point geometry)
+  float* d_geom_rect = nullptr;
   int n_affine = 0;                  // elements whose 25 points carry bitwise the same geometry
   int no_affine = 0;                 // SF2D_NO_AFFINE: always read the per-point geometry
   int rect = 0;                      // every element affine and axis-aligned (chain kernel's rectangle form)
