@@ -39,12 +39,13 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
 
-def kernel_min_bytes(nspec: int, nglob: int, n_affine: int, chain: bool) -> int:
+def kernel_min_bytes(nspec: int, nglob: int, n_affine: int, chain: bool, rows: bool = False) -> int:
     """Minimal bytes of the force kernel's representation per step: per element 25 x (kappa, mu) = 200 B, the
-    25 point indices (4 B each for the row-lane chain kernel, 2 B slot/rank words for the general patch kernel),
-    the geometry (16 B for an affine element, 400 B otherwise); per unique point displ read and accel
-    read-modify-write (24 B)."""
-    per_elem = 300 if chain else 250
+    point indices (25 x 4 B for the chain kernels with per-point ids; one 16-byte record per GLL row of a chain of
+    up to 12 elements, 80 B per chain, counted as 7 B per element, with row records; 25 x 2 B slot/rank words for
+    the general patch kernel), the geometry (16 B for an affine element, 400 B otherwise); per unique point displ
+    read and accel read-modify-write (24 B)."""
+    per_elem = (207 if rows else 300) if chain else 250
     return per_elem * nspec + 16 * n_affine + 400 * (nspec - n_affine) + 24 * nglob
 
 
@@ -356,7 +357,9 @@ def c2_elastic(steps: int, warmup: int):
         "avg_launch_ms": fk_ms,
         # the bytes this kernel's own representation needs at minimum: affine elements carry one 16-byte
         # geometry record instead of 25 x 16 bytes (what it reads, not the formula above)
-        "kernel_min_bytes_per_launch": kernel_min_bytes(nspec, nglob, n_affine, chain) / launches,
+        "kernel_min_bytes_per_launch": kernel_min_bytes(nspec, nglob, n_affine, chain,
+                                                        chain and sim.col_rows()) / launches,
+        "chain_row_records": bool(chain and sim.col_rows()),
     }
     out = {"workload": name, "nspec": nspec, "nglob": nglob, "receivers": cfg.nrec, "dt": cfg.dt,
            "boundaries": "Stacey bottom/left/right, free surface top", "steps": steps, "warmup": warmup,

@@ -41,9 +41,12 @@ int sf2d_plan_validate(int nspec, int nglob, const int* ibool_1based, int np, co
    exactly one, 0 when the plan cannot use it). */
 int sf2d_plan_chains(int nspec, int nglob, const int* ibool_1based, int np, const int* phase_lists,
                      int* first_writes);
-/* 1 when the handle's element forces run on the row-lane kernel for chain patches (0 with SF2D_NO_CHAIN set
-   or for other meshes) */
+/* 1 when the handle's element forces run on a chain kernel for chain patches (0 with SF2D_NO_CHAIN set or for
+   other meshes): the column-lane kernel by default, the row-lane kernel with SF2D_ROW_CHAIN */
 int sf2d_mesh_chain(long* Mesh_pointer);
+/* 1 when the column-lane kernel reads its point ids as one {base, id at x = 0, first-touch mask} record per chain
+   row (every chain row numbered consecutively; SF2D_COL_IDS keeps the per-point ids) */
+int sf2d_mesh_col_rows(long* Mesh_pointer);
 /* 1 when every element is affine and axis-aligned (xi_z = gamma_x = 0 to 2^-22 of the element's largest geometry
    entry): the chain kernel then uses its rectangle form (SF2D_NO_RECT keeps the general form) */
 int sf2d_mesh_rect(long* Mesh_pointer);

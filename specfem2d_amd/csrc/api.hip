@@ -350,6 +350,36 @@ void upload_chain_ids(Mesh* mp) {
     mp->d_col_mod = device_zeros<float2>(ne);
   }
   launch_col_pack(mp);
+  // the ids of each chain row as {base, id at x = 0, first-touch mask}, verified point by point
+  device_free(mp->d_col_rows);
+  mp->d_col_rows = nullptr;
+  if (mp->d_col_ids && !getenv("SF2D_COL_IDS")) {
+    const std::vector<int>& ib = mp->h_chain_ids;
+    std::vector<int4> rows((size_t)mp->npatch * NGLL);
+    bool ok = (int)mp->h_patch_desc.size() == 4 * mp->npatch;
+    for (int q = 0; q < mp->npatch && ok; q++) {
+      const int e0 = mp->h_patch_desc[4 * q], ne = mp->h_patch_desc[4 * q + 3] & 0xf;
+      for (int j = 0; j < NGLL && ok; j++) {
+        // element point of the lane that writes column x of row j: (x / 4, x % 4), and (ne - 1, 4) for x = 4 ne
+        auto at = [&](int x) {
+          const int k = x == 4 * ne ? ne - 1 : x / 4;
+          return (size_t)(e0 + k) * NGLL2 + j * NGLL + (x - 4 * k);
+        };
+        const int base = ib[at(1)] - 1;
+        unsigned long long mask = 0ull;
+        for (int x = 0; x <= 4 * ne && ok; x++) {
+          if (x > 0 && ib[at(x)] != base + x) ok = false;
+          if (ids[at(x)] & Mesh::FIRST_TOUCH) mask |= 1ull << x;
+        }
+        // the other element point of each shared column holds the same point (the chain property)
+        for (int k = 1; k < ne && ok; k++)
+          if (ib[(size_t)(e0 + k - 1) * NGLL2 + j * NGLL + NGLL - 1] != ib[(size_t)(e0 + k) * NGLL2 + j * NGLL]) ok = false;
+        rows[(size_t)q * NGLL + j] = make_int4(base, ib[at(0)], (int)(unsigned)(mask & 0xffffffffull),
+                                               (int)(unsigned)(mask >> 32));
+      }
+    }
+    if (ok && mp->npatch > 0) mp->d_col_rows = device_copy(rows.data(), rows.size());
+  }
   device_free(mp->d_special);
   mp->nspecial = (int)list.size();
   if (!list.empty()) mp->d_special = device_copy(list.data(), list.size());
@@ -389,6 +419,7 @@ void sort_elements_by_colour(Mesh* mp, const int* phase_lists, int np) {
   for (int pos = 0; pos < nspec; pos++) mp->pos_of_ispec[order[pos]] = pos;
   if (mp->assembly == 0) {
     mp->npatch = (int)P.desc.size() / 4;
+    mp->h_patch_desc = P.desc;
     mp->d_patch_desc = reinterpret_cast<int4*>(device_copy(P.desc.data(), P.desc.size()));
     mp->d_halo_nodes = device_copy(P.halo.data(), P.halo.size());
     mp->d_elem_slot = device_copy(P.slot.data(), P.slot.size());
@@ -925,6 +956,7 @@ void prepare_cleanup_device_(long* Mesh_pointer, int* ACOUSTIC_SIMULATION, int* 
   device_free(mp->d_chain_ids);
   device_free(mp->d_col_ids);
   device_free(mp->d_col_mod);
+  device_free(mp->d_col_rows);
   device_free(mp->d_special);
   if (mp->grad) sf2d_grad_release(Mesh_pointer);
   if (mp->batch) sf2d_batch_release(Mesh_pointer);
@@ -1486,6 +1518,8 @@ extern "C" int sf2d_plan_chains(int nspec, int nglob, const int* ibool_1based, i
 extern "C" int sf2d_mesh_chain(long* Mesh_pointer) { return handle(Mesh_pointer)->chain; }
 
 extern "C" int sf2d_mesh_rect(long* Mesh_pointer) { return handle(Mesh_pointer)->rect; }
+
+extern "C" int sf2d_mesh_col_rows(long* Mesh_pointer) { return handle(Mesh_pointer)->d_col_rows != nullptr; }
 
 // tuning knobs of a handle: elements per chain patch (fixed at set-up) and shots per wave of batched launches
 extern "C" int sf2d_mesh_chain_elems(long* Mesh_pointer) { return handle(Mesh_pointer)->chain_elems; }

@@ -722,12 +722,15 @@ __global__ __launch_bounds__(CHAIN_LANES) void k_elastic_forces_chain(
 // along the lane's own column in registers, the xi direction goes through the two LDS transposes; the force of the
 // shared column (4, j) of element k moves to lane 5 (k + 1) by a one-lane shuffle.  Every sum has the row-lane
 // kernel's terms in its order, and the same (element, point) pairs write accel, so the first-touch flags carry over.
+// rows: when not null, the point ids and first-touch flags come from one int4 per GLL row of the patch
+// (Mesh::d_col_rows: id = base + x for column x > 0 of the chain, the id of x = 0, the flags as a 64-bit mask)
+// instead of col_ids
 template <bool FIRST, bool MULTI, bool RECT>
 __global__ __launch_bounds__(CHAIN_LANES) void k_elastic_forces_col(
     int first_patch, int npatch, int nshot, int group, long shot_stride, const int4* __restrict__ desc,
     const int* __restrict__ col_ids, const float4* __restrict__ geom, const float4* __restrict__ geom_elem,
     const float2* __restrict__ col_mod, const float2* __restrict__ displ, float2* __restrict__ accel,
-    const Basis basis) {
+    const Basis basis, const int4* __restrict__ rows) {
   __shared__ float2 s_t[Mesh::CHAIN_ELEMS + 1][NGLL][NGLL + 1];
   __shared__ float2 s_v[Mesh::CHAIN_ELEMS + 1][NGLL][NGLL + 1];
   __shared__ float s_hp[NGLL2], s_hpw[NGLL2];
@@ -755,12 +758,25 @@ __global__ __launch_bounds__(CHAIN_LANES) void k_elastic_forces_col(
   bool first[NGLL];
   float2 u[NGLL], mod[NGLL];
   float4 geo[NGLL];
+  if (rows) {
+    const int x = 4 * min(k, ne - 1) + i;  // column of the chain (lanes past it: the last element's)
+    const int4* rp = rows + (size_t)(first_patch + item / ngroup) * NGLL;
 #pragma unroll
-  for (int j = 0; j < NGLL; j++) {
-    const int id = ld_off(col_ids, 4u * (c0 + (unsigned)(j * row)));
-    g[j] = id & ~Mesh::FIRST_TOUCH;
-    first[j] = FIRST && (id & Mesh::FIRST_TOUCH);
-    mod[j] = ld_off(col_mod, 8u * (c0 + (unsigned)(j * row)));
+    for (int j = 0; j < NGLL; j++) {
+      const int4 r = rp[j];  // the same word for every lane
+      g[j] = x == 0 ? r.y : r.x + x;
+      const unsigned bits = x < 32 ? ((unsigned)r.z >> x) : ((unsigned)r.w >> (x - 32));
+      first[j] = FIRST && (bits & 1u);
+      mod[j] = ld_off(col_mod, 8u * (c0 + (unsigned)(j * row)));
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < NGLL; j++) {
+      const int id = ld_off(col_ids, 4u * (c0 + (unsigned)(j * row)));
+      g[j] = id & ~Mesh::FIRST_TOUCH;
+      first[j] = FIRST && (id & Mesh::FIRST_TOUCH);
+      mod[j] = ld_off(col_mod, 8u * (c0 + (unsigned)(j * row)));
+    }
   }
   if (RECT || patch_affine(d.w)) {
     const float4 ge = ld_off(geom_elem, 16u * (unsigned)e);
@@ -1484,7 +1500,8 @@ void launch_elastic_forces_patch(const Mesh* mp, int first_patch, int npatch, co
         {{k_elastic_forces_chain<true, false, false>, k_elastic_forces_chain<true, false, true>},
          {k_elastic_forces_chain<true, true, false>, k_elastic_forces_chain<true, true, true>}}};
     if (mp->d_col_ids) {  // lanes along columns, element data in lane order
-      const K ckerns[2][2][2] = {
+      using KC = decltype(&k_elastic_forces_col<true, true, true>);
+      const KC ckerns[2][2][2] = {
           {{k_elastic_forces_col<false, false, false>, k_elastic_forces_col<false, false, true>},
            {k_elastic_forces_col<false, true, false>, k_elastic_forces_col<false, true, true>}},
           {{k_elastic_forces_col<true, false, false>, k_elastic_forces_col<true, false, true>},
@@ -1492,7 +1509,7 @@ void launch_elastic_forces_patch(const Mesh* mp, int first_patch, int npatch, co
       hipLaunchKernelGGL(ckerns[first_touch ? 1 : 0][group > 1 ? 1 : 0][mp->rect ? 1 : 0], dim3(grid),
                          dim3(CHAIN_LANES), 0, mp->stream, first_patch, npatch, nshot, group, shot_stride,
                          (const int4*)mp->d_patch_desc, mp->d_col_ids, mp->d_geom, mp->d_geom_elem, mp->d_col_mod,
-                         (const float2*)displ, (float2*)accel, mp->basis);
+                         (const float2*)displ, (float2*)accel, mp->basis, (const int4*)mp->d_col_rows);
       check_hip(hipGetLastError(), "k_elastic_forces_col launch");
       return;
     }

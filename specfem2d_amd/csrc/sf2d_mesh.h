@@ -108,6 +108,12 @@ struct Mesh {
   // wave's load of GLL row j reads consecutive words (SF2D_ROW_CHAIN keeps the row-lane kernel)
   int* d_col_ids = nullptr;
   float2* d_col_mod = nullptr;
+  // the column-lane kernel's point ids as rows: when GLL row j of every chain numbers its points x = 1 .. 4 ne
+  // consecutively (id = base_j + x; the numbering of the plan makes it so), a patch needs per row only base_j, the
+  // id of its point x = 0 and a 64-bit mask of the first-touch flags: int4 {base_j, id(0, j), mask low, mask high}
+  // per (patch, row), 80 B per chain instead of 1200 B of ids (SF2D_COL_IDS keeps the per-point ids)
+  int4* d_col_rows = nullptr;
+  std::vector<int> h_patch_desc;
   std::vector<uint8_t> h_first;
   // First-touch assembly (forward wavefield of SIMULATION_TYPE=1 and shot batches; SF2D_NO_FIRST_TOUCH turns it
   // off): the predictor leaves accel as it is except at the special points (Stacey edges, source elements),

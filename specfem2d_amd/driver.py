@@ -124,6 +124,8 @@ def load_libraries():
     gpu.sf2d_mesh_set_shots_per_wave.argtypes = [C.POINTER(C.c_long), C.c_int]
     gpu.sf2d_mesh_rect.argtypes = [C.POINTER(C.c_long)]
     gpu.sf2d_mesh_rect.restype = C.c_int
+    gpu.sf2d_mesh_col_rows.argtypes = [C.POINTER(C.c_long)]
+    gpu.sf2d_mesh_col_rows.restype = C.c_int
     gpu.sf2d_mesh_chain.argtypes = [C.POINTER(C.c_long)]
     gpu.sf2d_mesh_chain.restype = C.c_int
     gpu.sf2d_mesh_colours.argtypes = [C.POINTER(C.c_long), C.c_int]
@@ -279,6 +281,12 @@ class Simulation:
         self._need_device()
         mp = C.c_long(self.mesh_pointer)
         return bool(self.gpu.sf2d_mesh_rect(C.byref(mp)))
+
+    def col_rows(self) -> bool:
+        """Whether the column-lane kernel reads its point ids as one record per chain row."""
+        self._need_device()
+        mp = C.c_long(self.mesh_pointer)
+        return bool(self.gpu.sf2d_mesh_col_rows(C.byref(mp)))
 
     def colours(self, iphase: int = 2) -> int:
         mp = C.c_long(self.mesh_pointer)
