@@ -46,10 +46,10 @@ void check_hip(hipError_t err, const char* what) {
 
 void flush_correctors(Mesh* mp) {
   if (mp->pending_corrector[0])
-    launch_newmark_corrector(mp->d_veloc, mp->d_accel, mp->d_rmassx, mp->d_rmassz, mp->nglob, mp->pending_dto2[0], 1,
+    launch_newmark_corrector(mp->d_veloc, mp->d_accel, mp->d_rmassx, mp->d_rmassz, mp->nglob_el, mp->pending_dto2[0], 1,
                              mp->stream);
   if (mp->pending_corrector[1])
-    launch_newmark_corrector(mp->d_b_veloc, mp->d_b_accel, mp->d_rmassx, mp->d_rmassz, mp->nglob,
+    launch_newmark_corrector(mp->d_b_veloc, mp->d_b_accel, mp->d_rmassx, mp->d_rmassz, mp->nglob_el,
                              mp->pending_dto2[1], 1, mp->stream);
   mp->pending_corrector[0] = mp->pending_corrector[1] = 0;
 }
@@ -332,7 +332,8 @@ void upload_chain_ids(Mesh* mp) {
     for (int p = 0; p < NGLL2; p++) special[mp->h_chain_ids[(size_t)pos * NGLL2 + p]] = 1;
   }
   for (int g : mp->h_stacey_points) special[g] = 1;
-  for (int g : mp->h_uncovered_points) special[g] = 1;
+  for (int g : mp->h_uncovered_points)
+    if (g < mp->nglob_el) special[g] = 1;  // beyond nglob_el nothing touches the solid's fields
   for (int g : mp->h_coupling_points) special[mp->point_id.empty() ? g : mp->point_id[g]] = 1;
   for (int g : mp->h_pml_points) special[g] = 1;
   std::vector<int> ids(mp->h_chain_ids);
@@ -420,6 +421,7 @@ void sort_elements_by_colour(Mesh* mp, const int* phase_lists, int np) {
   if (mp->assembly == 0) {
     mp->npatch = (int)P.desc.size() / 4;
     mp->h_patch_desc = P.desc;
+
     mp->d_patch_desc = reinterpret_cast<int4*>(device_copy(P.desc.data(), P.desc.size()));
     mp->d_halo_nodes = device_copy(P.halo.data(), P.halo.size());
     mp->d_elem_slot = device_copy(P.slot.data(), P.slot.size());
@@ -427,6 +429,16 @@ void sort_elements_by_colour(Mesh* mp, const int* phase_lists, int np) {
     mp->chain = P.chains && try_chain;
   }
 
+  // the solid's Newmark updates visit ids 0 .. nglob_el-1: up to the largest point id of an elastic element (in
+  // patch mode the plan numbers the points of elastic elements first, so the others form the tail)
+  if (!getenv("SF2D_ALL_POINTS")) {
+    int top = -1;
+    for (int pos = 0; pos < P.phase_count[0] + P.phase_count[1]; pos++) {
+      const int* src = mp->h_ibool.data() + (size_t)order[pos] * NGLL2;
+      for (int p = 0; p < NGLL2; p++) top = std::max(top, mp->point_id.empty() ? src[p] : mp->point_id[src[p]]);
+    }
+    mp->nglob_el = top + 1;
+  }
   // move the per-element device arrays
   int* d_map = device_copy(order.data(), order.size());
   const size_t ne = (size_t)nspec * NGLL2;
@@ -644,6 +656,7 @@ void prepare_constants_device_(long* Mesh_pointer, int* h_NGLLX, int* NSPEC_AB, 
   mp->myrank = *h_myrank;
   mp->nspec = *NSPEC_AB;
   mp->nglob = *NGLOB_AB;
+  mp->nglob_el = mp->nglob;
   mp->simulation_type = *SIMULATION_TYPE;
   mp->save_forward = *SAVE_FORWARD;
   mp->p_sv = *P_SV;
@@ -1095,11 +1108,11 @@ void update_displacement_cuda_(long* Mesh_pointer, realw* deltat_F, realw* delta
   // CPML: the previous-step field of the layer's points, from the fields before the predictor
   pml_elastic_before_predictor(mp, *deltatsqover2_F);
   if (mp->pending_corrector[0])
-    launch_newmark_corrector_predictor(mp->d_displ, mp->d_veloc, mp->d_accel, mp->d_rmassx, mp->d_rmassz, mp->nglob,
+    launch_newmark_corrector_predictor(mp->d_displ, mp->d_veloc, mp->d_accel, mp->d_rmassx, mp->d_rmassz, mp->nglob_el,
                                        mp->pending_dto2[0], *deltat_F, *deltatsqover2_F, *deltatover2_F, mp->stream,
                                        1, 0, !keep);
   else
-    launch_newmark_predictor(mp->d_displ, mp->d_veloc, mp->d_accel, mp->nglob, *deltat_F, *deltatsqover2_F,
+    launch_newmark_predictor(mp->d_displ, mp->d_veloc, mp->d_accel, mp->nglob_el, *deltat_F, *deltatsqover2_F,
                              *deltatover2_F, mp->stream, 1, 0, !keep);
   if (keep) {
     launch_zero_points(mp, mp->d_accel);
@@ -1108,10 +1121,10 @@ void update_displacement_cuda_(long* Mesh_pointer, realw* deltat_F, realw* delta
   if (mp->simulation_type == 3) {
     if (mp->pending_corrector[1])
       launch_newmark_corrector_predictor(mp->d_b_displ, mp->d_b_veloc, mp->d_b_accel, mp->d_rmassx, mp->d_rmassz,
-                                         mp->nglob, mp->pending_dto2[1], *b_deltat_F, *b_deltatsqover2_F,
+                                         mp->nglob_el, mp->pending_dto2[1], *b_deltat_F, *b_deltatsqover2_F,
                                          *b_deltatover2_F, mp->stream);
     else
-      launch_newmark_predictor(mp->d_b_displ, mp->d_b_veloc, mp->d_b_accel, mp->nglob, *b_deltat_F, *b_deltatsqover2_F,
+      launch_newmark_predictor(mp->d_b_displ, mp->d_b_veloc, mp->d_b_accel, mp->nglob_el, *b_deltat_F, *b_deltatsqover2_F,
                                *b_deltatover2_F, mp->stream);
   } else if (mp->pending_corrector[1]) {
     flush_correctors(mp);
@@ -1230,9 +1243,9 @@ void kernel_3_b_cuda_(long* Mesh_pointer, realw* deltatover2_F,
                       realw* b_deltatover2_F) {  // update_displacement_cuda.cu:273 (velocity-only corrector)
   Mesh* mp = handle(Mesh_pointer);
   flush_correctors(mp);
-  launch_newmark_corrector(mp->d_veloc, mp->d_accel, nullptr, nullptr, mp->nglob, *deltatover2_F, 0, mp->stream);
+  launch_newmark_corrector(mp->d_veloc, mp->d_accel, nullptr, nullptr, mp->nglob_el, *deltatover2_F, 0, mp->stream);
   if (mp->simulation_type == 3)
-    launch_newmark_corrector(mp->d_b_veloc, mp->d_b_accel, nullptr, nullptr, mp->nglob, *b_deltatover2_F, 0,
+    launch_newmark_corrector(mp->d_b_veloc, mp->d_b_accel, nullptr, nullptr, mp->nglob_el, *b_deltatover2_F, 0,
                              mp->stream);
 }
 

@@ -34,6 +34,9 @@ struct Mesh {
   int myrank = 0;
   int device = 0;
   int nspec = 0, nglob = 0;
+  // the solid's Newmark updates run over internal ids 0 .. nglob_el-1: in patch mode the plan numbers the points of
+  // elastic elements first, and nothing changes the solid's fields at the others (SF2D_ALL_POINTS: all points)
+  int nglob_el = 0;
   int simulation_type = 1, save_forward = 0, p_sv = 1;
   int stacey = 0, pml = 0;
   int nstep = 0;

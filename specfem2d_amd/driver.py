@@ -598,8 +598,10 @@ class UnevenBatch(Simulation):
     """An uneven batch: ``len(cfgs)`` simulations of different meshes, sizes and physics (fluid or viscoelastic
     solids, CPML or Stacey edges, different NSTEP) advanced together on the GPU, one launch per kernel covering every
     instance.  Their time step and recording kind must agree.  The union runs the longest NSTEP; ``forward()``
-    returns every instance's seismograms over its own NSTEP (a list of [nrec_k, 2, nstep_k] arrays).  No counterpart
-    in the reference, which runs one mesh per simulation."""
+    returns every instance's seismograms over its own NSTEP (a list of [nrec_k, 2, nstep_k] arrays).  The union
+    numbers the points in instance order and the solid's Newmark updates stop at the last point of a solid element,
+    so instances with solid elements placed first spare those updates over the points of all-fluid instances after
+    them (``UnevenRuns`` orders them so).  No counterpart in the reference, which runs one mesh per simulation."""
 
     def __init__(self, cfgs, device: bool = True):
         cfgs = list(cfgs)
@@ -666,6 +668,11 @@ VISCO_POINT_COST = 1.8
 UNION_STEP_COST = 2.0e6
 
 
+def _all_fluid(cfg) -> bool:
+    """Whether every GLL point of an internal-mesher configuration has vs = 0 (a mesh of fluid elements only)."""
+    return bool(np.all(np.asarray(cfg.vs, dtype=np.float64) == 0.0))
+
+
 def plan_nstep_groups(cfgs, max_groups: int = 6, step_cost: float = UNION_STEP_COST):
     """Split uneven-batch instances into at most ``max_groups`` unions of similar NSTEP: contiguous runs of the
     instances in NSTEP order, chosen by dynamic programming to minimise the modelled work
@@ -704,7 +711,9 @@ class UnevenRuns:
 
     def __init__(self, cfgs, max_groups: int = 6, device: bool = True, step_cost: float = UNION_STEP_COST):
         self.cfgs = list(cfgs)
-        self.groups = plan_nstep_groups(self.cfgs, max_groups, step_cost)
+        # inside each union the instances with solid elements first (see UnevenBatch)
+        self.groups = [sorted(g, key=lambda k: _all_fluid(self.cfgs[k]))
+                       for g in plan_nstep_groups(self.cfgs, max_groups, step_cost)]
         self.batches = []
         try:
             for idx in self.groups:
