@@ -44,12 +44,15 @@ void check_hip(hipError_t err, const char* what) {
   if (err != hipSuccess) fail("%s: %s", what, hipGetErrorString(err));
 }
 
+// the z inverse mass for the Newmark kernels: null when it equals the x one at every point (one array read)
+const float* rmass_z(const Mesh* mp) { return mp->same_mass ? nullptr : mp->d_rmassz; }
+
 void flush_correctors(Mesh* mp) {
   if (mp->pending_corrector[0])
-    launch_newmark_corrector(mp->d_veloc, mp->d_accel, mp->d_rmassx, mp->d_rmassz, mp->nglob_el, mp->pending_dto2[0], 1,
+    launch_newmark_corrector(mp->d_veloc, mp->d_accel, mp->d_rmassx, rmass_z(mp), mp->nglob_el, mp->pending_dto2[0], 1,
                              mp->stream);
   if (mp->pending_corrector[1])
-    launch_newmark_corrector(mp->d_b_veloc, mp->d_b_accel, mp->d_rmassx, mp->d_rmassz, mp->nglob_el,
+    launch_newmark_corrector(mp->d_b_veloc, mp->d_b_accel, mp->d_rmassx, rmass_z(mp), mp->nglob_el,
                              mp->pending_dto2[1], 1, mp->stream);
   mp->pending_corrector[0] = mp->pending_corrector[1] = 0;
 }
@@ -828,6 +831,10 @@ void prepare_fields_elastic_device_(long* Mesh_pointer, realw* rmassx, realw* rm
   mp->d_rmassz = device_zeros<float>((size_t)mp->nglob);
   points_to_device(mp, mp->d_rmassx, rmassx, 1, "rmassx H2D");
   points_to_device(mp, mp->d_rmassz, rmassz, 1, "rmassz H2D");
+  // the two components' inverse masses differ only where the Stacey terms of the mass differ (absorbing edges);
+  // without them the Newmark kernels read one array
+  mp->same_mass = !getenv("SF2D_TWO_MASSES") &&
+                  std::equal(rmassx, rmassx + mp->nglob, rmassz, [](float a, float b) { return a == b; });
   check_hip(hipDeviceSynchronize(), "prepare_fields_elastic_device");
 }
 
@@ -1108,7 +1115,7 @@ void update_displacement_cuda_(long* Mesh_pointer, realw* deltat_F, realw* delta
   // CPML: the previous-step field of the layer's points, from the fields before the predictor
   pml_elastic_before_predictor(mp, *deltatsqover2_F);
   if (mp->pending_corrector[0])
-    launch_newmark_corrector_predictor(mp->d_displ, mp->d_veloc, mp->d_accel, mp->d_rmassx, mp->d_rmassz, mp->nglob_el,
+    launch_newmark_corrector_predictor(mp->d_displ, mp->d_veloc, mp->d_accel, mp->d_rmassx, rmass_z(mp), mp->nglob_el,
                                        mp->pending_dto2[0], *deltat_F, *deltatsqover2_F, *deltatover2_F, mp->stream,
                                        1, 0, !keep);
   else
@@ -1120,7 +1127,7 @@ void update_displacement_cuda_(long* Mesh_pointer, realw* deltat_F, realw* delta
   }
   if (mp->simulation_type == 3) {
     if (mp->pending_corrector[1])
-      launch_newmark_corrector_predictor(mp->d_b_displ, mp->d_b_veloc, mp->d_b_accel, mp->d_rmassx, mp->d_rmassz,
+      launch_newmark_corrector_predictor(mp->d_b_displ, mp->d_b_veloc, mp->d_b_accel, mp->d_rmassx, rmass_z(mp),
                                          mp->nglob_el, mp->pending_dto2[1], *b_deltat_F, *b_deltatsqover2_F,
                                          *b_deltatover2_F, mp->stream);
     else

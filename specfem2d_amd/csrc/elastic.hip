@@ -64,8 +64,9 @@ __global__ __launch_bounds__(256) void k_corrector(float2* __restrict__ v, float
     float2 aa = a[k], vv = v[k];
     if (apply_mass) {
       const int g = ps == 1 ? k : k / ps;
-      aa.x = aa.x * rmx[g];
-      aa.y = aa.y * rmz[g];
+      const float wx = rmx[g];
+      aa.x = aa.x * wx;
+      aa.y = aa.y * (rmz ? rmz[g] : wx);  // rmz null: one inverse mass for both components
       a[k] = aa;
     }
     vv.x = vv.x + dto2 * aa.x;
@@ -88,8 +89,9 @@ __global__ __launch_bounds__(256) void k_corrector_predictor(float2* __restrict_
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
     float2 dd = d[k], vv = v[k], aa = a[k];
     const int g = ps == 1 ? k : k / ps;
-    aa.x = aa.x * rmx[g];
-    aa.y = aa.y * rmz[g];
+    const float wx = rmx[g];
+    aa.x = aa.x * wx;
+    aa.y = aa.y * (rmz ? rmz[g] : wx);  // rmz null: one inverse mass for both components
     vv.x = vv.x + dto2_prev * aa.x;
     vv.y = vv.y + dto2_prev * aa.y;
     dd.x = dd.x + dt * vv.x + dt2o2 * aa.x;
@@ -120,8 +122,9 @@ __global__ __launch_bounds__(256) void k_corrector_predictor_to(
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
     float2 dd = d[k], vv = v[k], aa = a[k];
     const int g = ps == 1 ? k : k / ps;
-    aa.x = aa.x * rmx[g];
-    aa.y = aa.y * rmz[g];
+    const float wx = rmx[g];
+    aa.x = aa.x * wx;
+    aa.y = aa.y * (rmz ? rmz[g] : wx);  // rmz null: one inverse mass for both components
     vv.x = vv.x + dto2_prev * aa.x;
     vv.y = vv.y + dto2_prev * aa.y;
     dd.x = dd.x + dt * vv.x + dt2o2 * aa.x;

@@ -614,6 +614,7 @@ void sf2d_mesh_set_materials(long* Mesh_pointer, const float* kappav, const floa
   put(mp->d_rho_vs, rho_vs);
   if (rmassx) points_to_device(mp, mp->d_rmassx, rmassx, 1, "rmassx H2D");
   if (rmassz) points_to_device(mp, mp->d_rmassz, rmassz, 1, "rmassz H2D");
+  if (rmassx || rmassz) mp->same_mass = 0;  // the Newmark kernels read both arrays from here on
 }
 
 }  // extern "C"

@@ -37,6 +37,7 @@ struct Mesh {
   // the solid's Newmark updates run over internal ids 0 .. nglob_el-1: in patch mode the plan numbers the points of
   // elastic elements first, and nothing changes the solid's fields at the others (SF2D_ALL_POINTS: all points)
   int nglob_el = 0;
+  int same_mass = 0;  // rmassx == rmassz at every point: the Newmark kernels read rmassx for both components
   int simulation_type = 1, save_forward = 0, p_sv = 1;
   int stacey = 0, pml = 0;
   int nstep = 0;
