@@ -408,7 +408,14 @@ void sort_elements_by_colour(Mesh* mp, const int* phase_lists, int np) {
   if (const char* env = getenv("SF2D_SHOTS_PER_WAVE")) mp->shots_per_wave = std::max(1, atoi(env));
   Plan P = build_plan(mp->h_ibool, nspec, mp->nglob, phase_lists, np, mp->assembly,
                       try_chain ? mp->chain_elems : Mesh::PATCH_ELEMS);
-  if (try_chain && !P.chains) P = build_plan(mp->h_ibool, nspec, mp->nglob, phase_lists, np, mp->assembly);
+  if (try_chain && !P.chains) {
+    if (mp->attenuation) {  // the memory variables ride in the chain kernel only: atomic assembly otherwise
+      mp->assembly = 2;
+      P = build_plan(mp->h_ibool, nspec, mp->nglob, phase_lists, np, 2);
+    } else {
+      P = build_plan(mp->h_ibool, nspec, mp->nglob, phase_lists, np, mp->assembly);
+    }
+  }
   std::vector<int>& order = P.order;
   for (int ph = 0; ph < 2; ph++) {
     mp->phase_count[ph] = P.phase_count[ph];
@@ -767,10 +774,12 @@ void prepare_fields_elastic_device_(long* Mesh_pointer, realw* rmassx, realw* rm
     if (!h_A_newmark_mu || !h_B_newmark_mu || !h_A_newmark_kappa || !h_B_newmark_kappa)
       fail("viscoelastic attenuation: coefficient arrays missing");
     mp->attenuation = 1;
-    // the per-element kernel carries the memory variables; one launch over all elements with FP32 atomic
-    // assembly (2.17 ms per step on the 2000 x 1000 mesh) beats four coloured launches (2.60 ms) and the chain
-    // kernel (4.12 ms); SF2D_ASSEMBLY=colour keeps the deterministic coloured order
-    if (mp->assembly == 0) mp->assembly = 2;
+    // chain meshes: the column-lane chain kernel carries the memory variables in lane order; other meshes (and
+    // SF2D_ATT_ATOMIC, SF2D_ROW_CHAIN, SF2D_NO_CHAIN) the per-point kernel in one launch over all elements with FP32
+    // atomic assembly, which beats four coloured launches; SF2D_ASSEMBLY=colour keeps the deterministic coloured
+    // order
+    if (mp->assembly == 0 && (getenv("SF2D_ATT_ATOMIC") || getenv("SF2D_ROW_CHAIN") || getenv("SF2D_NO_CHAIN")))
+      mp->assembly = 2;
   }
   const size_t n2 = (size_t)2 * mp->nglob;
   mp->d_displ = device_zeros<float>(n2);
@@ -823,6 +832,19 @@ void prepare_fields_elastic_device_(long* Mesh_pointer, realw* rmassx, realw* rm
     }
     mp->att_ntable = (int)(table.size() / 3);
     mp->d_att_table = device_copy(table.data(), table.size());
+    if (mp->chain) {
+      // the column-lane kernel's lane order: the block of a chain of ne elements at position e0 holds point
+      // (k, i, j) at e0 * 25 + j * 5 ne + 5 k + i
+      std::vector<unsigned int> lane(idx);
+      for (int q = 0; q < mp->npatch; q++) {
+        const int e0 = mp->h_patch_desc[4 * q], nel = mp->h_patch_desc[4 * q + 3] & 0xf;
+        for (int k = 0; k < nel; k++)
+          for (int p = 0; p < NGLL2; p++)
+            lane[(size_t)e0 * NGLL2 + (size_t)(p / NGLL) * NGLL * nel + NGLL * k + p % NGLL] =
+                idx[(size_t)(e0 + k) * NGLL2 + p];
+      }
+      idx.swap(lane);
+    }
     mp->d_att_idx = device_copy(idx.data(), ne);
     mp->d_att_e = device_zeros<float>(9 * ne);
     mp->d_att_old = device_zeros<float>(3 * ne);

@@ -156,6 +156,38 @@ __global__ __launch_bounds__(256) void k_zero_points(float2* __restrict__ a, con
 // {A_kappa, B_kappa, A_mu, B_mu} x 3 solids from a small table through a per-point index; e and old are
 // [plane][ne] arrays in device element order (planes e1, e11, e13 per solid; previous dux_dxl, duz_dzl,
 // dux_dzl + duz_dxl)
+// One point's viscoelastic update (3 standard linear solids): the memory variables ev (e1, e11, e13 per solid)
+// advance by the recursive convolution of compute_attenuation_viscoelastic.f90:93-121 with the coefficient set
+// cf[k] = {A_kappa, B_kappa, A_mu, B_mu}, the previous strains ov (dux_dxl, duz_dzl, dux_dzl + duz_dxl) become
+// this step's, and the stress takes the unrelaxed-moduli correction.
+__device__ __forceinline__ void attenuate_point(const float4 (&cf)[3], float (&ev)[9], float (&ov)[3], float kap,
+                                                float mu, float dux_dxl, float dux_dzl, float duz_dxl, float duz_dzl,
+                                                float& sxx, float& sxz, float& szz) {
+  const float th = dux_dxl + duz_dzl;
+  const float oxx = ov[0], ozz = ov[1], oxz = ov[2];
+  const float th_old = oxx + ozz;
+  float e1s = 0.f, e11s = 0.f, e13s = 0.f;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const float a1 = cf[k].x, b1 = cf[k].y, a2 = cf[k].z, b2 = cf[k].w;
+    const float e1 = a1 * a1 * ev[3 * k] + b1 * (th + a1 * th_old);
+    const float e11 = a2 * a2 * ev[3 * k + 1] + b2 * (dux_dxl - 0.5f * th + a2 * (oxx - 0.5f * th_old));
+    const float e13 = a2 * a2 * ev[3 * k + 2] + b2 * (dux_dzl + duz_dxl + a2 * oxz);
+    ev[3 * k] = e1;
+    ev[3 * k + 1] = e11;
+    ev[3 * k + 2] = e13;
+    e1s += e1;
+    e11s += e11;
+    e13s += e13;
+  }
+  ov[0] = dux_dxl;
+  ov[1] = duz_dzl;
+  ov[2] = dux_dzl + duz_dxl;
+  sxx += kap * e1s + 2.f * mu * e11s;
+  sxz += mu * e13s;
+  szz += kap * e1s - 2.f * mu * e11s;
+}
+
 // RECT: every element an axis-aligned rectangle (xi_z = gamma_x = 0, xi_x and gamma_z constant over the element):
 // one float2 {xi_x, gamma_z} per element (geom_rect, device element order) instead of four per-point arrays
 template <bool ATOMIC, bool ATT, bool RECT>
@@ -250,29 +282,11 @@ __global__ __launch_bounds__(FORCE_THREADS) void k_elastic_forces(
     float sxz = mu * (duz_dxl + dux_dzl);
     if (ATT) {
       const size_t l = (size_t)(first + e) * NGLL2 + p;
-      const float th = dux_dxl + duz_dzl;
-      const float oxx = ov[0], ozz = ov[1], oxz = ov[2];
-      const float th_old = oxx + ozz;
-      float e1s = 0.f, e11s = 0.f, e13s = 0.f;
+      attenuate_point(cf, ev, ov, kap, mu, dux_dxl, dux_dzl, duz_dxl, duz_dzl, sxx, sxz, szz);
 #pragma unroll
-      for (int k = 0; k < 3; k++) {
-        const float a1 = cf[k].x, b1 = cf[k].y, a2 = cf[k].z, b2 = cf[k].w;
-        const float e1 = a1 * a1 * ev[3 * k] + b1 * (th + a1 * th_old);
-        const float e11 = a2 * a2 * ev[3 * k + 1] + b2 * (dux_dxl - 0.5f * th + a2 * (oxx - 0.5f * th_old));
-        const float e13 = a2 * a2 * ev[3 * k + 2] + b2 * (dux_dzl + duz_dxl + a2 * oxz);
-        att_e[(size_t)(3 * k) * ne + l] = e1;
-        att_e[(size_t)(3 * k + 1) * ne + l] = e11;
-        att_e[(size_t)(3 * k + 2) * ne + l] = e13;
-        e1s += e1;
-        e11s += e11;
-        e13s += e13;
-      }
-      att_old[l] = dux_dxl;
-      att_old[ne + l] = duz_dzl;
-      att_old[2 * ne + l] = dux_dzl + duz_dxl;
-      sxx += kap * e1s + 2.f * mu * e11s;
-      sxz += mu * e13s;
-      szz += kap * e1s - 2.f * mu * e11s;
+      for (int q = 0; q < 9; q++) att_e[(size_t)q * ne + l] = ev[q];
+#pragma unroll
+      for (int q = 0; q < 3; q++) att_old[(size_t)q * ne + l] = ov[q];
     }
     s_t1x[tid] = jac * (sxx * xixl + sxz * xizl);
     s_t1z[tid] = jac * (sxz * xixl + szz * xizl);
@@ -728,12 +742,43 @@ __global__ __launch_bounds__(CHAIN_LANES) void k_elastic_forces_chain(
 // rows: when not null, the point ids and first-touch flags come from one int4 per GLL row of the patch
 // (Mesh::d_col_rows: id = base + x for column x > 0 of the chain, the id of x = 0, the flags as a 64-bit mask)
 // instead of col_ids
-template <bool FIRST, bool MULTI, bool RECT>
+// ATT (one wavefield): viscoelastic attenuation, the memory variables and previous strains as [plane][ne] arrays in
+// lane order (att_e 9 planes, att_old 3) and a coefficient-set index per point in lane order (att_idx)
+struct ColAtt {
+  const unsigned int* idx;
+  const float4* table;
+  float* e;
+  float* old;
+  long ne;
+};
+
+// the attenuation update of the point at lane-order position la (lanes past the patch compute and store nothing)
+__device__ __forceinline__ void col_attenuate(const ColAtt& att, bool active, unsigned la, float kap, float mu,
+                                              float dux_dxl, float dux_dzl, float duz_dxl, float duz_dzl, float& sxx,
+                                              float& sxz, float& szz) {
+  if (!active) return;
+  float4 cf[3];
+  float ev[9], ov[3];
+  const float4* coef = att.table + 3 * (size_t)att.idx[la];
+#pragma unroll
+  for (int k = 0; k < 3; k++) cf[k] = coef[k];
+#pragma unroll
+  for (int q = 0; q < 9; q++) ev[q] = att.e[(size_t)q * att.ne + la];
+#pragma unroll
+  for (int q = 0; q < 3; q++) ov[q] = att.old[(size_t)q * att.ne + la];
+  attenuate_point(cf, ev, ov, kap, mu, dux_dxl, dux_dzl, duz_dxl, duz_dzl, sxx, sxz, szz);
+#pragma unroll
+  for (int q = 0; q < 9; q++) att.e[(size_t)q * att.ne + la] = ev[q];
+#pragma unroll
+  for (int q = 0; q < 3; q++) att.old[(size_t)q * att.ne + la] = ov[q];
+}
+
+template <bool FIRST, bool MULTI, bool RECT, bool ATT = false>
 __global__ __launch_bounds__(CHAIN_LANES) void k_elastic_forces_col(
     int first_patch, int npatch, int nshot, int group, long shot_stride, const int4* __restrict__ desc,
     const int* __restrict__ col_ids, const float4* __restrict__ geom, const float4* __restrict__ geom_elem,
     const float2* __restrict__ col_mod, const float2* __restrict__ displ, float2* __restrict__ accel,
-    const Basis basis, const int4* __restrict__ rows) {
+    const Basis basis, const int4* __restrict__ rows, const ColAtt att) {
   __shared__ float2 s_t[Mesh::CHAIN_ELEMS + 1][NGLL][NGLL + 1];
   __shared__ float2 s_v[Mesh::CHAIN_ELEMS + 1][NGLL][NGLL + 1];
   __shared__ float s_hp[NGLL2], s_hpw[NGLL2];
@@ -850,9 +895,11 @@ __global__ __launch_bounds__(CHAIN_LANES) void k_elastic_forces_col(
       const float xixl = geo[j].x, gazl = geo[j].w;
       const float dux_dxl = dxi.x * xixl, dux_dzl = dux_dga * gazl;
       const float duz_dxl = dxi.y * xixl, duz_dzl = duz_dga * gazl;
-      const float sxx = lp2m * dux_dxl + lambdal * duz_dzl;
-      const float szz = lp2m * duz_dzl + lambdal * dux_dxl;
-      const float sxz = mu * (duz_dxl + dux_dzl);
+      float sxx = lp2m * dux_dxl + lambdal * duz_dzl;
+      float szz = lp2m * duz_dzl + lambdal * dux_dxl;
+      float sxz = mu * (duz_dxl + dux_dzl);
+      if (ATT) col_attenuate(att, active, c0 + (unsigned)(j * row), kap, mu, dux_dxl, dux_dzl, duz_dxl, duz_dzl, sxx,
+                             sxz, szz);
       const float cx = 1.f / gazl, cz = 1.f / xixl;
       t1[j] = make_float2(cx * sxx, cx * sxz);
       t2[j] = make_float2(cz * sxz, cz * szz);
@@ -863,9 +910,11 @@ __global__ __launch_bounds__(CHAIN_LANES) void k_elastic_forces_col(
       const float duz_dxl = dxi.y * xixl + duz_dga * gaxl;
       const float duz_dzl = dxi.y * xizl + duz_dga * gazl;
       const float jac = 1.f / (xixl * gazl - gaxl * xizl);
-      const float sxx = lp2m * dux_dxl + lambdal * duz_dzl;
-      const float szz = lp2m * duz_dzl + lambdal * dux_dxl;
-      const float sxz = mu * (duz_dxl + dux_dzl);
+      float sxx = lp2m * dux_dxl + lambdal * duz_dzl;
+      float szz = lp2m * duz_dzl + lambdal * dux_dxl;
+      float sxz = mu * (duz_dxl + dux_dzl);
+      if (ATT) col_attenuate(att, active, c0 + (unsigned)(j * row), kap, mu, dux_dxl, dux_dzl, duz_dxl, duz_dzl, sxx,
+                             sxz, szz);
       t1[j] = make_float2(jac * (sxx * xixl + sxz * xizl), jac * (sxz * xixl + szz * xizl));
       t2[j] = make_float2(jac * (sxx * gaxl + sxz * gazl), jac * (sxz * gaxl + szz * gazl));
     }
@@ -1509,10 +1558,17 @@ void launch_elastic_forces_patch(const Mesh* mp, int first_patch, int npatch, co
            {k_elastic_forces_col<false, true, false>, k_elastic_forces_col<false, true, true>}},
           {{k_elastic_forces_col<true, false, false>, k_elastic_forces_col<true, false, true>},
            {k_elastic_forces_col<true, true, false>, k_elastic_forces_col<true, true, true>}}};
-      hipLaunchKernelGGL(ckerns[first_touch ? 1 : 0][group > 1 ? 1 : 0][mp->rect ? 1 : 0], dim3(grid),
-                         dim3(CHAIN_LANES), 0, mp->stream, first_patch, npatch, nshot, group, shot_stride,
+      // viscoelastic attenuation of the forward wavefield: one shot, memory variables in lane order
+      const bool att = mp->attenuation && accel == mp->d_accel;
+      if (att && nshot != 1) fail("viscoelastic attenuation: one wavefield per call");
+      const KC akerns[2][2] = {{k_elastic_forces_col<false, false, false, true>, k_elastic_forces_col<false, false, true, true>},
+                               {k_elastic_forces_col<true, false, false, true>, k_elastic_forces_col<true, false, true, true>}};
+      const ColAtt ca{mp->d_att_idx, mp->d_att_table, mp->d_att_e, mp->d_att_old, (long)mp->nspec * NGLL2};
+      hipLaunchKernelGGL(att ? akerns[first_touch ? 1 : 0][mp->rect ? 1 : 0]
+                             : ckerns[first_touch ? 1 : 0][group > 1 ? 1 : 0][mp->rect ? 1 : 0],
+                         dim3(grid), dim3(CHAIN_LANES), 0, mp->stream, first_patch, npatch, nshot, group, shot_stride,
                          (const int4*)mp->d_patch_desc, mp->d_col_ids, mp->d_geom, mp->d_geom_elem, mp->d_col_mod,
-                         (const float2*)displ, (float2*)accel, mp->basis, (const int4*)mp->d_col_rows);
+                         (const float2*)displ, (float2*)accel, mp->basis, (const int4*)mp->d_col_rows, ca);
       check_hip(hipGetLastError(), "k_elastic_forces_col launch");
       return;
     }

@@ -44,15 +44,17 @@ def test_viscoelastic_example_from_its_input_files(golden_dir):
     assert rel(seis2, seis) < TOL  # atomic assembly may reorder FP32 sums between runs
 
 
-@pytest.mark.parametrize("assembly,per_point", [("atomic", False), ("colour", False), ("atomic", True)])
+@pytest.mark.parametrize("assembly,per_point", [("chain", False), ("colour", False), ("chain", True),
+                                               ("atomic", True)])
 def test_heterogeneous_quality_factors_against_oracle(assembly, per_point, monkeypatch):
     """Qkappa, Qmu over a random model, some elements without attenuation (Q = 9999), Stacey edges, a tilted force
-    and a moment tensor, receivers rotated by anglerec; elements assembled with FP32 atomics in one launch (the
-    default with attenuation) or by colours.  per_point: the factors vary from GLL point to GLL point inside the
+    and a moment tensor, receivers rotated by anglerec; the memory variables carried by the column-lane chain kernel
+    (the default on these meshes), or by the per-point kernel assembling by colours or with FP32 atomics in one
+    launch (the default on other meshes).  per_point: the factors vary from GLL point to GLL point inside the
     elements, and a seventh of the points pair a finite factor with a 9999 one (the reference fits such points), so
     the coefficient table holds thousands of sets reached through the per-point index."""
-    if assembly == "colour":
-        monkeypatch.setenv("SF2D_ASSEMBLY", "colour")
+    if assembly in ("colour", "atomic"):
+        monkeypatch.setenv("SF2D_ASSEMBLY", assembly)
     nx, nz = 40, 30
     rng = np.random.default_rng(21)
     n = nx * nz * 25
@@ -75,9 +77,10 @@ def test_heterogeneous_quality_factors_against_oracle(assembly, per_point, monke
         stacey=True, absorbtop=False, sources=src, receivers=rec, anglerec=20.0, attenuation=True, qkappa=qk, qmu=qm)
     print(f"case: heterogeneous Q 40x30, {assembly} assembly, per-point {per_point}", flush=True)
     sim = Simulation(cfg)
+    assert sim.chain_kernel() == (assembly == "chain")
     seis, t = sim.forward()
     d, _, _ = sim.fields()
-    if assembly == "colour":  # coloured assembly has a fixed summation order: two runs agree bit for bit
+    if assembly != "atomic":  # chains and colours have a fixed summation order: two runs agree bit for bit
         sim.reset()
         again, _ = sim.forward()
         assert np.array_equal(again, seis)
