@@ -114,6 +114,132 @@ __global__ __launch_bounds__(256) void k_acoustic_forces(int first, int n, const
   }
 }
 
+// Chains of fluid elements, lanes along GLL columns (the scalar twin of the solid's column-lane kernel): one wave per
+// chain of up to 12 elements, lane 5 k + i holding column i of element k; gamma contractions in registers, xi
+// contractions through two LDS transposes; the shared column's term moves to the next element's lane by a one-lane
+// shuffle, and each point of the chain is updated by one lane with a read-modify-write of pdd (no point is shared
+// between the chains of one colour).  Same element operator as k_acoustic_forces.
+constexpr int AC_CHAIN = 12;
+constexpr int AC_NXCD = 8;  // XCD-contiguous blocks of chains, as the solid's kernels
+template <bool RECT>
+__global__ __launch_bounds__(64) void k_acoustic_forces_col(int first_patch, int npatch,
+                                                            const int4* __restrict__ desc,
+                                                            const int* __restrict__ ids,
+                                                            const float* __restrict__ fac,
+                                                            const float4* __restrict__ geom,
+                                                            const float2* __restrict__ rect, Basis b,
+                                                            const float* __restrict__ pot,
+                                                            float* __restrict__ pdd) {
+  __shared__ float s_t[AC_CHAIN + 1][NGLL][NGLL + 1];
+  __shared__ float s_v[AC_CHAIN + 1][NGLL][NGLL + 1];
+  __shared__ float s_hp[NGLL2], s_hpw[NGLL2], s_w[NGLL];
+  const int lane = threadIdx.x;
+  if (lane < NGLL2) {
+    s_hp[lane] = b.hp[lane];
+    s_hpw[lane] = b.hpw[lane];
+  }
+  if (lane < NGLL) s_w[lane] = b.w[lane];
+  const int nw = gridDim.x;
+  const int item = (blockIdx.x % AC_NXCD) * (nw / AC_NXCD) + blockIdx.x / AC_NXCD;
+  if (item >= npatch) return;  // uniform over the wave
+  const int4 d = desc[first_patch + item];
+  const int ne = d.w & 0xf;
+  const int k = min(lane / NGLL, AC_CHAIN);
+  const int i = lane % NGLL;
+  const bool active = lane / NGLL < ne;
+  const int e = d.x + min(k, ne - 1);
+  const int row = NGLL * ne;
+  const size_t c0 = (size_t)d.x * NGLL2 + (size_t)min(lane, row - 1);
+  const bool writer = active && (i < NGLL - 1 || k == ne - 1);
+  int g[NGLL];
+  float u[NGLL], f[NGLL], old[NGLL];
+  float4 gm[NGLL];
+#pragma unroll
+  for (int j = 0; j < NGLL; j++) {
+    g[j] = ids[c0 + (size_t)j * row];
+    f[j] = fac[c0 + (size_t)j * row];
+    u[j] = pot[g[j]];
+    old[j] = writer ? pdd[g[j]] : 0.f;  // no other chain of this colour has the point
+    if (RECT) {
+      const float2 r = rect[e];
+      gm[j] = make_float4(r.x, 0.f, 0.f, r.y);
+    } else {
+      gm[j] = geom[c0 + (size_t)j * row];
+    }
+  }
+  // xi-derivatives through the tile: this lane takes row r = i
+#pragma unroll
+  for (int j = 0; j < NGLL; j++) s_t[k][j][i] = u[j];
+  __syncthreads();
+  {
+    const int r = i;
+    float rw[NGLL];
+#pragma unroll
+    for (int m = 0; m < NGLL; m++) rw[m] = s_t[k][r][m];
+#pragma unroll
+    for (int c = 0; c < NGLL; c++) {
+      float dx = 0.f;
+#pragma unroll
+      for (int m = 0; m < NGLL; m++) dx += rw[m] * s_hp[c * NGLL + m];
+      s_v[k][r][c] = dx;
+    }
+  }
+  __syncthreads();
+  float t1[NGLL], t2[NGLL];
+#pragma unroll
+  for (int j = 0; j < NGLL; j++) {
+    float dga = 0.f;
+#pragma unroll
+    for (int m = 0; m < NGLL; m++) dga += u[m] * s_hp[j * NGLL + m];
+    const float dxi = s_v[k][j][i];
+    if (RECT) {
+      t1[j] = f[j] * (gm[j].x * (dxi * gm[j].x));
+      t2[j] = f[j] * (gm[j].w * (dga * gm[j].w));
+    } else {
+      const float dxl = dxi * gm[j].x + dga * gm[j].z;
+      const float dzl = dxi * gm[j].y + dga * gm[j].w;
+      t1[j] = f[j] * (gm[j].x * dxl + gm[j].y * dzl);
+      t2[j] = f[j] * (gm[j].z * dxl + gm[j].w * dzl);
+    }
+  }
+  __syncthreads();  // every lane has read its derivatives
+#pragma unroll
+  for (int j = 0; j < NGLL; j++) s_t[k][j][i] = t1[j];
+  __syncthreads();
+  {
+    const int r = i;
+    float rw[NGLL];
+#pragma unroll
+    for (int m = 0; m < NGLL; m++) rw[m] = s_t[k][r][m];
+#pragma unroll
+    for (int c = 0; c < NGLL; c++) {
+      float sx = 0.f;
+#pragma unroll
+      for (int m = 0; m < NGLL; m++) sx += rw[m] * s_hpw[m * NGLL + c];
+      s_v[k][r][c] = sx;
+    }
+  }
+  __syncthreads();
+  float fo[NGLL];
+  const float wi = s_w[i];
+#pragma unroll
+  for (int j = 0; j < NGLL; j++) {
+    float a2 = 0.f;
+#pragma unroll
+    for (int m = 0; m < NGLL; m++) a2 += t2[m] * s_hpw[m * NGLL + j];
+    fo[j] = s_w[j] * s_v[k][j][i] + wi * a2;
+  }
+#pragma unroll
+  for (int j = 0; j < NGLL; j++) {
+    const float fl = __shfl_up(fo[j], 1, 64);
+    if (i == 0 && k > 0) fo[j] += fl;
+  }
+  if (writer) {
+#pragma unroll
+    for (int j = 0; j < NGLL; j++) pdd[g[j]] = old[j] - fo[j];
+  }
+}
+
 // Newmark predictor on the potential (update_displacement_Newmark.F90:190-240) over the fluid's points: the id
 // range [r0, r0 + n) and the listed ids below it (list[0 .. nlist))
 __global__ __launch_bounds__(256) void k_ac_predictor(float* __restrict__ pot, float* __restrict__ pd,
@@ -319,6 +445,18 @@ struct AcousticState {
   float2* d_geom_rect = nullptr;  // [nelem] {xi_x, gamma_z} when every fluid element is an axis-aligned rectangle
   int rect = 0;
   float* d_fac = nullptr;    // [nelem][25] J / rho
+  // chains of fluid elements (the column-lane fluid kernel): patches of the fluid phase lists, coloured; per patch
+  // an int4 {first position, -, -, ne}; ids, J / rho and per-point geometry in lane order (the block of a chain of
+  // ne elements at position e0 holds point (k, i, j) at e0 * 25 + j * 5 ne + 5 k + i); rectangles per element
+  int chain = 0;
+  int cn_colour[2] = {0, 0};
+  int cc_start[2][Mesh::MAX_COLOURS] = {};
+  int cc_count[2][Mesh::MAX_COLOURS] = {};
+  int4* d_cdesc = nullptr;
+  int* d_cids = nullptr;
+  float* d_cfac = nullptr;
+  float4* d_cgeom = nullptr;
+  float2* d_crect = nullptr;
   int nfree = 0;
   int* d_free = nullptr;
   int ncoup = 0;
@@ -342,6 +480,7 @@ void acoustic_release(Mesh* mp) {
   if (!ac) return;
   device_free(ac->d_pot); device_free(ac->d_pd); device_free(ac->d_pdd); device_free(ac->d_rm);
   device_free(ac->d_list); device_free(ac->d_ids); device_free(ac->d_geom); device_free(ac->d_geom_rect); device_free(ac->d_fac); device_free(ac->d_free);
+  device_free(ac->d_cdesc); device_free(ac->d_cids); device_free(ac->d_cfac); device_free(ac->d_cgeom); device_free(ac->d_crect);
   device_free(ac->d_coup); device_free(ac->d_coup_n); device_free(ac->d_coup_w);
   device_free(ac->d_st); device_free(ac->d_st_w); device_free(ac->d_st_rvp);
   device_free(ac->d_src); device_free(ac->d_src_pts); device_free(ac->d_src_sa); device_free(ac->d_src_kap);
@@ -490,6 +629,53 @@ AcousticState* ac_ready(Mesh* mp, const char* who) {
     }
     ac->rect = rect;
     if (rect) ac->d_geom_rect = device_copy(gr.data(), gr.size());
+    // chains of fluid elements for the column-lane kernel (SF2D_NO_FLUID_CHAIN keeps the per-point kernel)
+    if (!getenv("SF2D_NO_FLUID_CHAIN") && !order.empty()) {
+      const int np = (int)std::max(ac->phase_elems[0].size(), ac->phase_elems[1].size());
+      std::vector<int> lists((size_t)2 * np, 0);
+      for (int ph = 0; ph < 2; ph++)
+        for (size_t q = 0; q < ac->phase_elems[ph].size(); q++) lists[(size_t)ph * np + q] = ac->phase_elems[ph][q] + 1;
+      std::vector<int> ibd((size_t)mp->nspec * NGLL2);
+      for (int e = 0; e < mp->nspec; e++)
+        for (int p = 0; p < NGLL2; p++) ibd[(size_t)e * NGLL2 + p] = iv.id(e, p);
+      std::vector<int> corder, cdesc;
+      if (chain_patches(ibd, mp->nspec, mp->nglob, lists.data(), np, corder, cdesc, ac->cn_colour, ac->cc_start,
+                        ac->cc_count)) {
+        // the packed (colour-order) index of each fluid element, for its operands computed above
+        std::vector<int> packed((size_t)mp->nspec, -1);
+        for (size_t q = 0; q < order.size(); q++) packed[order[q]] = (int)q;
+        const size_t nf = order.size();
+        std::vector<int> cids(nf * NGLL2);
+        std::vector<float> cfac(nf * NGLL2);
+        std::vector<float4> cgeom(nf * NGLL2);
+        std::vector<float2> crect(nf);
+        const int npatch = (int)cdesc.size() / 4;
+        bool ok = true;
+        for (int q = 0; q < npatch && ok; q++) {
+          const int e0 = cdesc[4 * q], nel = cdesc[4 * q + 3] & 0xf;
+          for (int kk = 0; kk < nel && ok; kk++) {
+            const int src = packed[corder[e0 + kk]];
+            if (src < 0 || e0 + kk >= (int)nf) { ok = false; break; }
+            crect[e0 + kk] = gr[src];
+            for (int p = 0; p < NGLL2; p++) {
+              const size_t dst = (size_t)e0 * NGLL2 + (size_t)(p / NGLL) * NGLL * nel + NGLL * kk + p % NGLL;
+              cids[dst] = ids[(size_t)src * NGLL2 + p];
+              cfac[dst] = fac[(size_t)src * NGLL2 + p];
+              cgeom[dst] = geom[(size_t)src * NGLL2 + p];
+            }
+          }
+        }
+        if (ok) {
+          for (int q = 0; q < npatch; q++) cdesc[4 * q + 1] = cdesc[4 * q + 2] = 0;
+          ac->d_cdesc = reinterpret_cast<int4*>(device_copy(cdesc.data(), cdesc.size()));
+          ac->d_cids = device_copy(cids.data(), cids.size());
+          ac->d_cfac = device_copy(cfac.data(), cfac.size());
+          if (rect) ac->d_crect = device_copy(crect.data(), crect.size());
+          else ac->d_cgeom = device_copy(cgeom.data(), cgeom.size());
+          ac->chain = 1;
+        }
+      }
+    }
     // the Newmark updates visit the fluid's points only (the potential is zero elsewhere): the solid's numbering
     // puts the points of fluid elements alone after all others, so one id range holds them and a short list the
     // ones on fluid-solid edges; the split minimises range length + 3 x list length
@@ -772,6 +958,19 @@ void compute_forces_acoustic_cuda_(long* Mesh_pointer, int* iphase, int* nspec_o
   AcousticState* ac = ac_ready(mp, "compute_forces_acoustic_cuda");
   const int ph = *iphase - 1;
   if (ph < 0 || ph > 1) fail("compute_forces_acoustic_cuda: iphase %d", *iphase);
+  if (ac->chain) {  // chains of fluid elements, one wave per chain
+    for (int c = 0; c < ac->cn_colour[ph]; c++) {
+      const int n = ac->cc_count[ph][c];
+      if (n == 0) continue;
+      hipLaunchKernelGGL(ac->rect ? k_acoustic_forces_col<true> : k_acoustic_forces_col<false>,
+                         dim3((n + AC_NXCD - 1) / AC_NXCD * AC_NXCD), dim3(64), 0, mp->stream, ac->cc_start[ph][c], n,
+                         ac->d_cdesc, ac->d_cids, ac->d_cfac, ac->d_cgeom, ac->d_crect, mp->basis, ac->d_pot,
+                         ac->d_pdd);
+      check_hip(hipGetLastError(), "k_acoustic_forces_col launch");
+    }
+    pml_acoustic_forces(mp, ph, ac->d_pot, ac->d_pd, ac->d_pdd);
+    return;
+  }
   for (int c = 0; c < ac->ncolour[ph]; c++) {
     const int n = ac->colour_count[ph][c];
     if (n == 0) continue;

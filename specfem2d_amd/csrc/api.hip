@@ -322,6 +322,26 @@ Plan build_plan(const std::vector<int>& ibool, int nspec, int nglob, const int* 
 
 }  // namespace
 
+// Patches of up to CHAIN_ELEMS elements over the given phase lists (1-based, zero-padded rows of np), coloured so
+// that no two patches of a colour share a point; true when every patch is a chain.  For the fluid's chain kernel:
+// only the element order, the patch descriptors and the colour ranges are used (not the plan's point numbering).
+bool chain_patches(const std::vector<int>& ibool, int nspec, int nglob, const int* phase_lists, int np,
+                   std::vector<int>& order, std::vector<int>& desc, int ncolour[2],
+                   int colour_start[2][Mesh::MAX_COLOURS], int colour_count[2][Mesh::MAX_COLOURS]) {
+  Plan P = build_plan(ibool, nspec, nglob, phase_lists, np, 0, Mesh::CHAIN_ELEMS);
+  if (!P.chains) return false;
+  order = std::move(P.order);
+  desc = std::move(P.desc);
+  for (int ph = 0; ph < 2; ph++) {
+    ncolour[ph] = P.ncolour[ph];
+    for (int c = 0; c < Mesh::MAX_COLOURS; c++) {
+      colour_start[ph][c] = P.colour_start[ph][c];
+      colour_count[ph][c] = P.colour_count[ph][c];
+    }
+  }
+  return true;
+}
+
 // Chain-kernel point ids with the first-touch flags, and the special points of first-touch assembly (points that
 // the Stacey edges, the sources, the fluid and the CPML layer's zero condition also update: the predictor clears
 // them and every force update adds to them).  Called when the plan is built and again when the Stacey edges or the

@@ -258,6 +258,9 @@ void pml_acoustic_forces(Mesh* mp, int ph, const float* pot, const float* pd, fl
 void pml_acoustic_boundary(Mesh* mp, float* pot, float* pd, float* pdd);
 // chain-kernel point ids with first-touch flags and the special points, uploaded again when the special points change
 void upload_chain_ids(Mesh* mp);
+bool chain_patches(const std::vector<int>& ibool, int nspec, int nglob, const int* phase_lists, int np,
+                   std::vector<int>& order, std::vector<int>& desc, int ncolour[2],
+                   int colour_start[2][Mesh::MAX_COLOURS], int colour_count[2][Mesh::MAX_COLOURS]);
 // fills the column-lane copies of the chain ids and moduli from d_chain_ids and d_moduli (when both are allocated)
 void launch_col_pack(const Mesh* mp);
 // point arrays in the caller's numbering <-> the device numbering (n2 values per point)
