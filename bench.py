@@ -286,10 +286,10 @@ def c2_attenuation(steps: int = 50, warmup: int = 5):
     """Forward of the C2 elastic mesh (2000 x 1000 elements of 10 m, the random-field model) with viscoelastic
     attenuation everywhere (Qkappa 60, Qmu 30, 3 standard linear solids): GLL-point updates per second, and the
     element-force launches' share of the step with the survey's algorithmic bytes for this kernel, the elastic
-    700 B per element plus 3600 B per element of attenuation, plus 24 B per point.  This build's representation
-    reads and writes 2808 B per element (25 points x (ibool 4 B, moduli 8 B, coefficient-set index 4 B, 9 memory
-    variables and 3 previous strains read and written 96 B), one 8-byte rectangle record per element) plus 24 B per
-    point."""
+    700 B per element plus 3600 B per element of attenuation, plus 24 B per point.  This build's chain kernel
+    reads and writes 2723 B per element (25 points x (moduli 8 B, coefficient-set index 4 B, 9 memory variables and
+    3 previous strains read and written 96 B), a 16-byte geometry record and about 7 B of point-id row records per
+    element) plus 24 B per point."""
     import torch
     from specfem2d_amd.driver import Simulation
     cfg = make_config("c2_mesh_elastic_2000x1000", warmup + steps).replace(attenuation=True, qkappa=60.0, qmu=30.0)
@@ -301,7 +301,7 @@ def c2_attenuation(steps: int = 50, warmup: int = 5):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     traffic = (700 + 3600) * sim.nspec + 24 * sim.nglob
-    own = 2808 * sim.nspec + 24 * sim.nglob
+    own = 2723 * sim.nspec + 24 * sim.nglob
     forces_s = timing["forces_ms"] * 1e-3 / steps
     out = {"workload": "c2_mesh_elastic_2000x1000 + attenuation (Qkappa 60, Qmu 30, N_SLS 3)", "nspec": sim.nspec,
            "nglob": sim.nglob, "steps": steps, "ms_per_step": wall * 1e3 / steps, "value": sim.nglob * steps / wall,

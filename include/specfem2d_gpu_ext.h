@@ -47,6 +47,9 @@ int sf2d_mesh_chain(long* Mesh_pointer);
 /* 1 when the column-lane kernel reads its point ids as one {base, id at x = 0, first-touch mask} record per chain
    row (every chain row numbered consecutively; SF2D_COL_IDS keeps the per-point ids) */
 int sf2d_mesh_col_rows(long* Mesh_pointer);
+/* 1 when the fluid's element forces run on the fluid chain kernel (every patch of the fluid elements a chain;
+   SF2D_NO_FLUID_CHAIN keeps the per-point kernel); known after the first acoustic step, 0 before */
+int sf2d_mesh_fluid_chain(long* Mesh_pointer);
 /* 1 when every element is affine and axis-aligned (xi_z = gamma_x = 0 to 2^-22 of the element's largest geometry
    entry): the chain kernel then uses its rectangle form (SF2D_NO_RECT keeps the general form) */
 int sf2d_mesh_rect(long* Mesh_pointer);

@@ -841,6 +841,12 @@ using namespace sf2d;
 
 extern "C" {
 
+// 1 when the fluid's element forces run on the chain kernel (resolved at the first acoustic step)
+int sf2d_mesh_fluid_chain(long* Mesh_pointer) {
+  const Mesh* mp = handle(Mesh_pointer);
+  return mp->ac && mp->ac->ready && mp->ac->chain;
+}
+
 void prepare_fields_acoustic_device_(long* Mesh_pointer, realw* rmass_acoustic, realw* rhostore, realw* kappastore,
                                      int* num_phase_ispec_acoustic, int* phase_ispec_inner_acoustic,
                                      int* num_free_surface_faces, int* free_surface_ispec, int* free_surface_ijk,

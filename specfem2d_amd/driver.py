@@ -126,6 +126,8 @@ def load_libraries():
     gpu.sf2d_mesh_rect.restype = C.c_int
     gpu.sf2d_mesh_col_rows.argtypes = [C.POINTER(C.c_long)]
     gpu.sf2d_mesh_col_rows.restype = C.c_int
+    gpu.sf2d_mesh_fluid_chain.argtypes = [C.POINTER(C.c_long)]
+    gpu.sf2d_mesh_fluid_chain.restype = C.c_int
     gpu.sf2d_mesh_chain.argtypes = [C.POINTER(C.c_long)]
     gpu.sf2d_mesh_chain.restype = C.c_int
     gpu.sf2d_mesh_colours.argtypes = [C.POINTER(C.c_long), C.c_int]
@@ -259,7 +261,7 @@ class Simulation:
         return self.gpu.sf2d_mesh_affine_elements(C.byref(mp))
 
     def chain_kernel(self) -> bool:
-        """Whether the element forces run on the row-lane kernel for chain patches."""
+        """Whether the element forces run on a chain kernel (column lanes by default) for chain patches."""
         self._need_device()
         mp = C.c_long(self.mesh_pointer)
         return bool(self.gpu.sf2d_mesh_chain(C.byref(mp)))
@@ -281,6 +283,12 @@ class Simulation:
         self._need_device()
         mp = C.c_long(self.mesh_pointer)
         return bool(self.gpu.sf2d_mesh_rect(C.byref(mp)))
+
+    def fluid_chain(self) -> bool:
+        """Whether the fluid's element forces run on the fluid chain kernel (known after the first step)."""
+        self._need_device()
+        mp = C.c_long(self.mesh_pointer)
+        return bool(self.gpu.sf2d_mesh_fluid_chain(C.byref(mp)))
 
     def col_rows(self) -> bool:
         """Whether the column-lane kernel reads its point ids as one record per chain row."""

@@ -172,6 +172,30 @@ def test_fluid_rectangle_form_matches_per_point_geometry(monkeypatch):
     assert r < TOL
 
 
+@pytest.mark.parametrize("rect", [True, False])
+def test_fluid_chain_kernel_matches_per_point_kernel(rect, monkeypatch):
+    """The fluid chain kernel (chains of fluid elements, lanes along GLL columns) gives the per-point fluid kernel's
+    traces (SF2D_NO_FLUID_CHAIN) to FP32 rounding -- only the order of the sums at a chain's shared columns differs --
+    in the rectangle form and with per-point geometry (SF2D_NO_RECT)."""
+    if not rect:
+        monkeypatch.setenv("SF2D_NO_RECT", "1")
+    cfg = _coupled_case(4).replace(nstep=500)
+    print(f"case: coupled mesh, fluid chain kernel vs per-point kernel, rect {rect}", flush=True)
+    sim = Simulation(cfg)
+    seis, _ = sim.forward()
+    chain = sim.fluid_chain()
+    sim.close()
+    monkeypatch.setenv("SF2D_NO_FLUID_CHAIN", "1")
+    sim2 = Simulation(cfg)
+    seis2, _ = sim2.forward()
+    chain2 = sim2.fluid_chain()
+    sim2.close()
+    r = rel(seis, seis2)
+    print(f"fluid chain kernel vs per-point kernel (rect {rect}): rel-L2 {r:.2e}")
+    assert chain and not chain2
+    assert r < TOL
+
+
 def test_luoyang_fluid_solid_against_reference_trace(golden_dir, tmp_path):
     """The LuoYang fluid-solid example (a solid strip between two fluids, fluid-solid edges on vertical sides,
     fluid Stacey edges next to them), run from its own input files, against the reference's forward potential
