@@ -866,8 +866,7 @@ void prepare_fields_elastic_device_(long* Mesh_pointer, realw* rmassx, realw* rm
       idx.swap(lane);
     }
     mp->d_att_idx = device_copy(idx.data(), ne);
-    mp->d_att_e = device_zeros<float>(9 * ne);
-    mp->d_att_old = device_zeros<float>(3 * ne);
+    mp->d_att_e = device_zeros<float>(12 * ne);  // [3][ne] float4: 9 memory variables, 3 previous strains
   }
   mp->d_rmassx = device_zeros<float>((size_t)mp->nglob);
   mp->d_rmassz = device_zeros<float>((size_t)mp->nglob);
@@ -1553,8 +1552,7 @@ extern "C" void sf2d_attenuation_reset(long* Mesh_pointer) {
   Mesh* mp = handle(Mesh_pointer);
   if (!mp->attenuation) return;
   const size_t ne = (size_t)mp->nspec * NGLL2;
-  check_hip(hipMemsetAsync(mp->d_att_e, 0, 9 * ne * sizeof(float), mp->stream), "attenuation reset");
-  check_hip(hipMemsetAsync(mp->d_att_old, 0, 3 * ne * sizeof(float), mp->stream), "attenuation reset");
+  check_hip(hipMemsetAsync(mp->d_att_e, 0, 12 * ne * sizeof(float), mp->stream), "attenuation reset");
 }
 
 extern "C" void sf2d_pml_reset(long* Mesh_pointer) { pml_reset(handle(Mesh_pointer)); }

@@ -153,9 +153,9 @@ __global__ __launch_bounds__(256) void k_zero_points(float2* __restrict__ a, con
 // ATT: viscoelastic attenuation with 3 standard linear solids -- the memory variables advance with the
 // recursive convolution of compute_attenuation_viscoelastic.f90:93-121 (Kernel_2_viscoelastic_impl.cu:1019-1065
 // on the reference's GPU) and correct the stress with the unrelaxed moduli; each point reads its coefficient set
-// {A_kappa, B_kappa, A_mu, B_mu} x 3 solids from a small table through a per-point index; e and old are
-// [plane][ne] arrays in device element order (planes e1, e11, e13 per solid; previous dux_dxl, duz_dzl,
-// dux_dzl + duz_dxl)
+// {A_kappa, B_kappa, A_mu, B_mu} x 3 solids from a small table through a per-point index; the state (memory
+// variables e1, e11, e13 per solid, previous dux_dxl, duz_dzl, dux_dzl + duz_dxl) is three float4 planes of
+// [3][ne] in device element order (att_load)
 // One point's viscoelastic update (3 standard linear solids): the memory variables ev (e1, e11, e13 per solid)
 // advance by the recursive convolution of compute_attenuation_viscoelastic.f90:93-121 with the coefficient set
 // cf[k] = {A_kappa, B_kappa, A_mu, B_mu}, the previous strains ov (dux_dxl, duz_dzl, dux_dzl + duz_dxl) become
@@ -186,6 +186,22 @@ __device__ __forceinline__ void attenuate_point(const float4 (&cf)[3], float (&e
   sxx += kap * e1s + 2.f * mu * e11s;
   sxz += mu * e13s;
   szz += kap * e1s - 2.f * mu * e11s;
+}
+
+// A point's attenuation state as three float4 planes of [3][ne] (ne = element points): the 9 memory variables then
+// the 3 previous strains, {e1, e11, e13} of solid 1, 2, 3 and {dux_dxl, duz_dzl, dux_dzl + duz_dxl}
+__device__ __forceinline__ void att_load(const float* st, long ne, size_t l, float (&ev)[9], float (&ov)[3]) {
+  const float4* s4 = reinterpret_cast<const float4*>(st);
+  const float4 a = s4[l], b = s4[(size_t)ne + l], c = s4[2 * (size_t)ne + l];
+  ev[0] = a.x; ev[1] = a.y; ev[2] = a.z; ev[3] = a.w;
+  ev[4] = b.x; ev[5] = b.y; ev[6] = b.z; ev[7] = b.w;
+  ev[8] = c.x; ov[0] = c.y; ov[1] = c.z; ov[2] = c.w;
+}
+__device__ __forceinline__ void att_store(float* st, long ne, size_t l, const float (&ev)[9], const float (&ov)[3]) {
+  float4* s4 = reinterpret_cast<float4*>(st);
+  s4[l] = make_float4(ev[0], ev[1], ev[2], ev[3]);
+  s4[(size_t)ne + l] = make_float4(ev[4], ev[5], ev[6], ev[7]);
+  s4[2 * (size_t)ne + l] = make_float4(ev[8], ov[0], ov[1], ov[2]);
 }
 
 // RECT: every element an axis-aligned rectangle (xi_z = gamma_x = 0, xi_x and gamma_z constant over the element):
@@ -234,10 +250,7 @@ __global__ __launch_bounds__(FORCE_THREADS) void k_elastic_forces(
       const float4* coef = att_table + 3 * (size_t)att_idx[l];
 #pragma unroll
       for (int k = 0; k < 3; k++) cf[k] = coef[k];
-#pragma unroll
-      for (int q = 0; q < 9; q++) ev[q] = att_e[(size_t)q * ne + l];
-#pragma unroll
-      for (int q = 0; q < 3; q++) ov[q] = att_old[(size_t)q * ne + l];
+      att_load(att_e, ne, l, ev, ov);
     }
     g = ibool[l];
     const float2 u = displ[g];
@@ -283,10 +296,7 @@ __global__ __launch_bounds__(FORCE_THREADS) void k_elastic_forces(
     if (ATT) {
       const size_t l = (size_t)(first + e) * NGLL2 + p;
       attenuate_point(cf, ev, ov, kap, mu, dux_dxl, dux_dzl, duz_dxl, duz_dzl, sxx, sxz, szz);
-#pragma unroll
-      for (int q = 0; q < 9; q++) att_e[(size_t)q * ne + l] = ev[q];
-#pragma unroll
-      for (int q = 0; q < 3; q++) att_old[(size_t)q * ne + l] = ov[q];
+      att_store(att_e, ne, l, ev, ov);
     }
     s_t1x[tid] = jac * (sxx * xixl + sxz * xizl);
     s_t1z[tid] = jac * (sxz * xixl + szz * xizl);
@@ -742,8 +752,8 @@ __global__ __launch_bounds__(CHAIN_LANES) void k_elastic_forces_chain(
 // rows: when not null, the point ids and first-touch flags come from one int4 per GLL row of the patch
 // (Mesh::d_col_rows: id = base + x for column x > 0 of the chain, the id of x = 0, the flags as a 64-bit mask)
 // instead of col_ids
-// ATT (one wavefield): viscoelastic attenuation, the memory variables and previous strains as [plane][ne] arrays in
-// lane order (att_e 9 planes, att_old 3) and a coefficient-set index per point in lane order (att_idx)
+// ATT (one wavefield): viscoelastic attenuation, the memory variables and previous strains as three float4 planes
+// of [3][ne] in lane order (att_e; see att_load) and a coefficient-set index per point in lane order (att_idx)
 struct ColAtt {
   const unsigned int* idx;
   const float4* table;
@@ -762,15 +772,9 @@ __device__ __forceinline__ void col_attenuate(const ColAtt& att, bool active, un
   const float4* coef = att.table + 3 * (size_t)att.idx[la];
 #pragma unroll
   for (int k = 0; k < 3; k++) cf[k] = coef[k];
-#pragma unroll
-  for (int q = 0; q < 9; q++) ev[q] = att.e[(size_t)q * att.ne + la];
-#pragma unroll
-  for (int q = 0; q < 3; q++) ov[q] = att.old[(size_t)q * att.ne + la];
+  att_load(att.e, att.ne, la, ev, ov);
   attenuate_point(cf, ev, ov, kap, mu, dux_dxl, dux_dzl, duz_dxl, duz_dzl, sxx, sxz, szz);
-#pragma unroll
-  for (int q = 0; q < 9; q++) att.e[(size_t)q * att.ne + la] = ev[q];
-#pragma unroll
-  for (int q = 0; q < 3; q++) att.old[(size_t)q * att.ne + la] = ov[q];
+  att_store(att.e, att.ne, la, ev, ov);
 }
 
 template <bool FIRST, bool MULTI, bool RECT, bool ATT = false>
@@ -1500,7 +1504,7 @@ void launch_elastic_forces(const Mesh* mp, int first, int nelem, int atomic, con
                            int att) {
   if (nelem <= 0) return;
   if (first < 0 || first + nelem > mp->nspec) fail("element range [%d, %d) outside the mesh", first, first + nelem);
-  if (att && (!mp->d_att_idx || !mp->d_att_table || !mp->d_att_e || !mp->d_att_old))
+  if (att && (!mp->d_att_idx || !mp->d_att_table || !mp->d_att_e))
     fail("attenuation arrays not prepared");
   const int nbatch = (nelem + FORCE_EPB - 1) / FORCE_EPB;
   const int per_xcd = (nbatch + NXCD - 1) / NXCD;

@@ -120,8 +120,17 @@ __global__ __launch_bounds__(256) void k_pml_elastic_forces(
     md = moduli[q];
     w = dampw[q];
     c = load_coefs(table, cidx[q] & ~ON_EDGE);
+    {  // the 12 memory variables of the point as three float4 planes
+      const float4* rm4 = reinterpret_cast<const float4*>(rm);
 #pragma unroll
-    for (int k = 0; k < EL_PLANES; k++) r[k] = rm[(size_t)k * nq + q];
+      for (int k = 0; k < EL_PLANES / 4; k++) {
+        const float4 v4 = rm4[(size_t)k * nq + q];
+        r[4 * k] = v4.x;
+        r[4 * k + 1] = v4.y;
+        r[4 * k + 2] = v4.z;
+        r[4 * k + 3] = v4.w;
+      }
+    }
     s_ux[tid] = u.x;
     s_uz[tid] = u.y;
     s_ox[tid] = o.x;
@@ -233,8 +242,12 @@ __global__ __launch_bounds__(256) void k_pml_elastic_forces(
     r[11] = conv(c.caz, r[11], u.y, o.y);
     pacc.x = w * (c.al[0] * v.x + c.al[1] * u.x + c.al[2] * r[8] + c.al[3] * r[10]);
     pacc.y = w * (c.al[0] * v.y + c.al[1] * u.y + c.al[2] * r[9] + c.al[3] * r[11]);
+    {
+      float4* rm4 = reinterpret_cast<float4*>(rm);
 #pragma unroll
-    for (int k = 0; k < EL_PLANES; k++) rm[(size_t)k * nq + q] = r[k];
+      for (int k = 0; k < EL_PLANES / 4; k++)
+        rm4[(size_t)k * nq + q] = make_float4(r[4 * k], r[4 * k + 1], r[4 * k + 2], r[4 * k + 3]);
+    }
   }
   __syncthreads();
   if (active) {
@@ -290,8 +303,15 @@ __global__ __launch_bounds__(256) void k_pml_acoustic_forces(
     f = fac[q].x;
     w = dampw[q];
     c = load_coefs(table, cidx[q] & ~ON_EDGE);
+    {  // the 6 memory variables of the point as three float2 planes
+      const float2* rm2 = reinterpret_cast<const float2*>(rm);
 #pragma unroll
-    for (int k = 0; k < AC_PLANES; k++) r[k] = rm[(size_t)k * nq + q];
+      for (int k = 0; k < AC_PLANES / 2; k++) {
+        const float2 v2 = rm2[(size_t)k * nq + q];
+        r[2 * k] = v2.x;
+        r[2 * k + 1] = v2.y;
+      }
+    }
     s_u[tid] = u;
     s_o[tid] = o;
   }
@@ -321,8 +341,11 @@ __global__ __launch_bounds__(256) void k_pml_acoustic_forces(
     r[4] = conv(c.cax, r[4], u, o);
     r[5] = conv(c.caz, r[5], u, o);
     pacc = w * (c.al[0] * v + c.al[1] * u + c.al[2] * r[4] + c.al[3] * r[5]);
+    {
+      float2* rm2 = reinterpret_cast<float2*>(rm);
 #pragma unroll
-    for (int k = 0; k < AC_PLANES; k++) rm[(size_t)k * nq + q] = r[k];
+      for (int k = 0; k < AC_PLANES / 2; k++) rm2[(size_t)k * nq + q] = make_float2(r[2 * k], r[2 * k + 1]);
+    }
   }
   __syncthreads();
   if (active) {
@@ -602,7 +625,7 @@ struct PmlLayer {  // the solid's or the fluid's layer elements, packed in (phas
   float* d_dampw = nullptr;
   unsigned int* d_cidx = nullptr;
   float* d_old = nullptr;   // previous-step field per element point (float2 solid, float fluid)
-  float* d_rm = nullptr;    // memory variables [planes][nelem * 25]
+  float* d_rm = nullptr;    // memory variables: solid [3][nelem * 25] float4, fluid [3][nelem * 25] float2
   unsigned int* d_zidx = nullptr;  // solid, viscoelastic layer: Zener set per element point (NO_ZENER: none)
   float* d_zrm = nullptr;          // its memory variables [ZPLANES][nelem * 25]
   int nbnd = 0;

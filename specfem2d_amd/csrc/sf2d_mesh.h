@@ -54,6 +54,8 @@ struct Mesh {
   int attenuation = 0, att_ntable = 0;
   unsigned int* d_att_idx = nullptr;
   float4* d_att_table = nullptr;
+  // attenuation state: [3][nspec * 25] float4 (e1, e11, e13 of the 3 solids, then the 3 previous strains), in the
+  // force kernel's point order; d_att_old is unused (null)
   float *d_att_e = nullptr, *d_att_old = nullptr;
   int* d_ispec_is_elastic = nullptr;
   int nspec_elastic = 0, nspec_acoustic = 0;
