@@ -178,19 +178,20 @@ __global__ __launch_bounds__(256) void k_pml_elastic_forces(
       for (int k = 0; k < ZSLS; k++) {
         const float4 za = z[3 * k], zb = z[3 * k + 1], zc = z[3 * k + 2];
         const float ck[3] = {zb.x, zb.y, zb.z}, cm[3] = {zb.w, zc.x, zc.y};
-        float* v = zrm + (size_t)(6 * k) * nq + q;
-        const float kxx = conv(ck, v[0], p_ux_x, o_ux_x);
-        const float kzz = conv(ck, v[nq], p_uz_z, o_uz_z);
-        const float mxx = conv(cm, v[2 * nq], p_ux_x, o_ux_x);
-        const float mzz = conv(cm, v[3 * nq], p_uz_z, o_uz_z);
-        const float mxz = conv(cm, v[4 * nq], p_ux_z, o_ux_z);
-        const float mzx = conv(cm, v[5 * nq], p_uz_x, o_uz_x);
-        v[0] = kxx;
-        v[nq] = kzz;
-        v[2 * nq] = mxx;
-        v[3 * nq] = mzz;
-        v[4 * nq] = mxz;
-        v[5 * nq] = mzx;
+        // solid k's 6 memory variables: a float4 plane {kxx, kzz, mxx, mzz} of [3][nq] and a float2 plane {mxz, mzx}
+        // of [3][nq] after the float4 planes
+        float4* v4 = reinterpret_cast<float4*>(zrm) + (size_t)k * nq + q;
+        float2* v2 = reinterpret_cast<float2*>(zrm + (size_t)12 * nq) + (size_t)k * nq + q;
+        const float4 a4 = *v4;
+        const float2 a2 = *v2;
+        const float kxx = conv(ck, a4.x, p_ux_x, o_ux_x);
+        const float kzz = conv(ck, a4.y, p_uz_z, o_uz_z);
+        const float mxx = conv(cm, a4.z, p_ux_x, o_ux_x);
+        const float mzz = conv(cm, a4.w, p_uz_z, o_uz_z);
+        const float mxz = conv(cm, a2.x, p_ux_z, o_ux_z);
+        const float mzx = conv(cm, a2.y, p_uz_x, o_uz_x);
+        *v4 = make_float4(kxx, kzz, mxx, mzz);
+        *v2 = make_float2(mxz, mzx);
         ks_xx += za.x * kxx;
         ks_zz += za.y * kzz;
         ms_xx += za.z * mxx;
@@ -627,7 +628,7 @@ struct PmlLayer {  // the solid's or the fluid's layer elements, packed in (phas
   float* d_old = nullptr;   // previous-step field per element point (float2 solid, float fluid)
   float* d_rm = nullptr;    // memory variables: solid [3][nelem * 25] float4, fluid [3][nelem * 25] float2
   unsigned int* d_zidx = nullptr;  // solid, viscoelastic layer: Zener set per element point (NO_ZENER: none)
-  float* d_zrm = nullptr;          // its memory variables [ZPLANES][nelem * 25]
+  float* d_zrm = nullptr;          // its memory variables: [3][nelem * 25] float4, then [3][nelem * 25] float2
   int nbnd = 0;
   int* d_bnd = nullptr;     // outer-boundary point ids (device numbering)
   long nq() const { return (long)nelem * NGLL2; }
