@@ -47,6 +47,11 @@ int sf2d_mesh_chain(long* Mesh_pointer);
 /* 1 when the column-lane kernel reads its point ids as one {base, id at x = 0, first-touch mask} record per chain
    row (every chain row numbered consecutively; SF2D_COL_IDS keeps the per-point ids) */
 int sf2d_mesh_col_rows(long* Mesh_pointer);
+/* host only: 1 when the row records hold for the chain plan of a mesh (ibool 1-based [nspec][25], phase lists as
+   for sf2d_plan_validate) and reproduce every writer's point id and first-touch flag, 0 when they do not, -1 when
+   the plan is not all chains; nrecords: the records (5 per chain) */
+int sf2d_plan_row_records(int nspec, int nglob, const int* ibool_1based, int np, const int* phase_lists,
+                          int* nrecords);
 /* 1 when the fluid's element forces run on the fluid chain kernel (every patch of the fluid elements a chain;
    SF2D_NO_FLUID_CHAIN keeps the per-point kernel); known after the first acoustic step, 0 before */
 int sf2d_mesh_fluid_chain(long* Mesh_pointer);

@@ -342,6 +342,40 @@ bool chain_patches(const std::vector<int>& ibool, int nspec, int nglob, const in
   return true;
 }
 
+// Row records of chain point ids: per patch (desc: {first position, -, -, ne}) and GLL row j, {base, id of column
+// 0, first-touch mask low, high} such that column x > 0 of the row is point base + x.  ids: per element point (device
+// order) the point id; flagged: the same with the FIRST_TOUCH flags.  False when some row is not numbered
+// consecutively (or a shared column is not shared): the per-point ids are kept then.
+bool chain_row_records(const std::vector<int>& desc, const std::vector<int>& ids, const std::vector<int>& flagged,
+                       std::vector<int4>& rows) {
+  const int npatch = (int)desc.size() / 4;
+  rows.assign((size_t)npatch * NGLL, make_int4(0, 0, 0, 0));
+  for (int q = 0; q < npatch; q++) {
+    const int e0 = desc[4 * q], ne = desc[4 * q + 3] & 0xf;
+    if (ne < 1 || 4 * ne + 1 > 64 || (size_t)(e0 + ne) * NGLL2 > ids.size()) return false;
+    for (int j = 0; j < NGLL; j++) {
+      // element point of the lane that writes column x of row j: (x / 4, x % 4), and (ne - 1, 4) for x = 4 ne
+      auto at = [&](int x) {
+        const int k = x == 4 * ne ? ne - 1 : x / 4;
+        return (size_t)(e0 + k) * NGLL2 + j * NGLL + (x - 4 * k);
+      };
+      const int base = ids[at(1)] - 1;
+      unsigned long long mask = 0ull;
+      for (int x = 0; x <= 4 * ne; x++) {
+        if (x > 0 && ids[at(x)] != base + x) return false;
+        if (flagged[at(x)] & Mesh::FIRST_TOUCH) mask |= 1ull << x;
+      }
+      // the other element point of each shared column holds the same point (the chain property)
+      for (int k = 1; k < ne; k++)
+        if (ids[(size_t)(e0 + k - 1) * NGLL2 + j * NGLL + NGLL - 1] != ids[(size_t)(e0 + k) * NGLL2 + j * NGLL])
+          return false;
+      rows[(size_t)q * NGLL + j] =
+          make_int4(base, ids[at(0)], (int)(unsigned)(mask & 0xffffffffull), (int)(unsigned)(mask >> 32));
+    }
+  }
+  return true;
+}
+
 // Chain-kernel point ids with the first-touch flags, and the special points of first-touch assembly (points that
 // the Stacey edges, the sources, the fluid and the CPML layer's zero condition also update: the predictor clears
 // them and every force update adds to them).  Called when the plan is built and again when the Stacey edges or the
@@ -378,31 +412,9 @@ void upload_chain_ids(Mesh* mp) {
   device_free(mp->d_col_rows);
   mp->d_col_rows = nullptr;
   if (mp->d_col_ids && !getenv("SF2D_COL_IDS")) {
-    const std::vector<int>& ib = mp->h_chain_ids;
-    std::vector<int4> rows((size_t)mp->npatch * NGLL);
-    bool ok = (int)mp->h_patch_desc.size() == 4 * mp->npatch;
-    for (int q = 0; q < mp->npatch && ok; q++) {
-      const int e0 = mp->h_patch_desc[4 * q], ne = mp->h_patch_desc[4 * q + 3] & 0xf;
-      for (int j = 0; j < NGLL && ok; j++) {
-        // element point of the lane that writes column x of row j: (x / 4, x % 4), and (ne - 1, 4) for x = 4 ne
-        auto at = [&](int x) {
-          const int k = x == 4 * ne ? ne - 1 : x / 4;
-          return (size_t)(e0 + k) * NGLL2 + j * NGLL + (x - 4 * k);
-        };
-        const int base = ib[at(1)] - 1;
-        unsigned long long mask = 0ull;
-        for (int x = 0; x <= 4 * ne && ok; x++) {
-          if (x > 0 && ib[at(x)] != base + x) ok = false;
-          if (ids[at(x)] & Mesh::FIRST_TOUCH) mask |= 1ull << x;
-        }
-        // the other element point of each shared column holds the same point (the chain property)
-        for (int k = 1; k < ne && ok; k++)
-          if (ib[(size_t)(e0 + k - 1) * NGLL2 + j * NGLL + NGLL - 1] != ib[(size_t)(e0 + k) * NGLL2 + j * NGLL]) ok = false;
-        rows[(size_t)q * NGLL + j] = make_int4(base, ib[at(0)], (int)(unsigned)(mask & 0xffffffffull),
-                                               (int)(unsigned)(mask >> 32));
-      }
-    }
-    if (ok && mp->npatch > 0) mp->d_col_rows = device_copy(rows.data(), rows.size());
+    std::vector<int4> rows;
+    if (chain_row_records(mp->h_patch_desc, mp->h_chain_ids, ids, rows) && !rows.empty())
+      mp->d_col_rows = device_copy(rows.data(), rows.size());
   }
   device_free(mp->d_special);
   mp->nspecial = (int)list.size();
@@ -1573,6 +1585,43 @@ extern "C" int sf2d_plan_chains(int nspec, int nglob, const int* ibool_1based, i
     for (uint8_t f : P.first) *first_writes += f;
   }
   return P.chains ? 1 : 0;
+}
+
+// host-only: whether the column-lane kernel's row records hold for the chain plan of this mesh (every chain row
+// numbered consecutively in the plan's numbering), and the number of records; -1 when the plan is not all chains
+extern "C" int sf2d_plan_row_records(int nspec, int nglob, const int* ibool_1based, int np, const int* phase_lists,
+                                     int* nrecords) {
+  std::vector<int> ib = to_zero_based(ibool_1based, (size_t)nspec * NGLL2, 1, nglob, "ibool");
+  Plan P = build_plan(ib, nspec, nglob, phase_lists, np, 0, Mesh::CHAIN_ELEMS);  // as the device plan is chosen
+  if (!P.chains) P = build_plan(ib, nspec, nglob, phase_lists, np, 0);
+  if (!P.chains) return -1;
+  std::vector<int> ids((size_t)nspec * NGLL2);
+  for (int pos = 0; pos < nspec; pos++)
+    for (int p = 0; p < NGLL2; p++) ids[(size_t)pos * NGLL2 + p] = P.newid[ib[(size_t)P.order[pos] * NGLL2 + p]];
+  std::vector<int> flagged(ids);
+  for (size_t l = 0; l < flagged.size() && l < P.first.size(); l++)
+    if (P.first[l]) flagged[l] |= Mesh::FIRST_TOUCH;
+  std::vector<int4> rows;
+  const bool ok = chain_row_records(P.desc, ids, flagged, rows);
+  if (nrecords) *nrecords = ok ? (int)rows.size() : 0;
+  if (!ok) return 0;
+  // the records reproduce every writer's id and flag
+  for (int q = 0; q < (int)P.desc.size() / 4; q++) {
+    const int e0 = P.desc[4 * q], ne = P.desc[4 * q + 3] & 0xf;
+    for (int j = 0; j < NGLL; j++)
+      for (int k = 0; k < ne; k++)
+        for (int i = 0; i < NGLL; i++) {
+          const int x = 4 * k + i;
+          const int4 r = rows[(size_t)q * NGLL + j];
+          const int id = x == 0 ? r.y : r.x + x;
+          const size_t l = (size_t)(e0 + k) * NGLL2 + j * NGLL + i;
+          if (id != ids[l]) return 0;
+          const bool writer = i < NGLL - 1 || k == ne - 1;
+          const unsigned bits = x < 32 ? ((unsigned)r.z >> x) : ((unsigned)r.w >> (x - 32));
+          if (writer && ((bits & 1u) != 0) != ((flagged[l] & Mesh::FIRST_TOUCH) != 0)) return 0;
+        }
+  }
+  return 1;
 }
 
 extern "C" int sf2d_mesh_chain(long* Mesh_pointer) { return handle(Mesh_pointer)->chain; }

@@ -260,6 +260,8 @@ void pml_acoustic_forces(Mesh* mp, int ph, const float* pot, const float* pd, fl
 void pml_acoustic_boundary(Mesh* mp, float* pot, float* pd, float* pdd);
 // chain-kernel point ids with first-touch flags and the special points, uploaded again when the special points change
 void upload_chain_ids(Mesh* mp);
+bool chain_row_records(const std::vector<int>& desc, const std::vector<int>& ids, const std::vector<int>& flagged,
+                       std::vector<int4>& rows);
 bool chain_patches(const std::vector<int>& ibool, int nspec, int nglob, const int* phase_lists, int np,
                    std::vector<int>& order, std::vector<int>& desc, int ncolour[2],
                    int colour_start[2][Mesh::MAX_COLOURS], int colour_count[2][Mesh::MAX_COLOURS]);

@@ -123,3 +123,30 @@ def test_chain_patches_on_structured_meshes(nx, expect):
     lists = inner_only(o.nspec)
     lists[1] = rng.permutation(lists[1])
     assert chains(ib, o.nglob, lists) == 0
+
+
+def row_records(ibool, nglob, lists):
+    gpu, _ = driver.load_libraries()
+    ib = np.ascontiguousarray(ibool.reshape(-1), dtype=np.int32)
+    ph = np.ascontiguousarray(lists.reshape(-1), dtype=np.int32)
+    f = gpu.sf2d_plan_row_records
+    f.restype = C.c_int
+    n = C.c_int(-1)
+    rc = f(C.c_int(ibool.shape[0]), C.c_int(nglob), ib.ctypes.data_as(C.c_void_p), C.c_int(lists.shape[1]),
+           ph.ctypes.data_as(C.c_void_p), C.byref(n))
+    return rc, n.value
+
+
+@pytest.mark.parametrize("nx,expect", [(80, 1), (12, 1), (13, 1), (10, 1), (8, -1)])
+def test_chain_row_records(nx, expect):
+    """The column-lane kernel's point ids as one record per chain row: in the plan's numbering every GLL row of a
+    chain is numbered consecutively (column x > 0 at base + x), so {base, id of column 0, first-touch mask} per row
+    reproduces the id and first-touch flag of every writing lane (checked point by point on the host); meshes whose
+    patches are not chains have no records."""
+    cfg = config.small_test_mesh(nx=nx, nz=4, nstep=2)
+    o = OracleSim(cfg, "sp")
+    rc, n = row_records(o.ibool(), o.nglob, inner_only(o.nspec))
+    assert rc == expect
+    if expect == 1:
+        chains_per_row = -(-nx // 12) if nx >= 12 else 1  # rows narrower than 12 elements: one chain of 10
+        assert n == 5 * chains_per_row * cfg.nz
