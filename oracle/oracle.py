@@ -83,6 +83,8 @@ def _lib(precision: str):
         lib.orc_set_threads.restype = C.c_int
         lib.orc_set_threads.argtypes = [C.c_int]
         lib.orc_adjoint.argtypes = [P, dp, dp, dp, dp, dp, C.c_void_p, dp, C.c_int, C.c_void_p, C.c_void_p]
+        lib.orc_adjoint_full.argtypes = [P, dp, dp, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        lib.orc_adjoint_full.restype = C.c_int
         lib.orc_kernel_increment.argtypes = [P, dp, dp, dp, dp]
         lib.orc_gradient.argtypes = [P, dp, dp, dp]
         lib.orc_gradient.restype = C.c_double
@@ -376,6 +378,35 @@ class OracleSim:
         if len(steps):
             return out + (dumps[: len(steps)].reshape(len(steps), -1, 2),)
         return out
+
+    def adjoint_full(self, adj, stacey_store, lastframe_el=None, lastframe_ac=None):
+        """SIMULATION_TYPE=3 run over fluid, solid or coupled meshes (no CPML, no attenuation).  ``adj``
+        [nrec,2,nstep] adjoint source traces (a receiver in a fluid element uses component 0),
+        ``stacey_store`` from a SAVE_FORWARD run (``forward(save_stacey=True)``), ``lastframe_el`` =
+        (displ, veloc, accel) and ``lastframe_ac`` = (potential, its first and second time derivative) at the
+        end of that run.  Returns (adjoint seismograms [nrec,2,nstep], solid kernels dict rho/mu/kappa scaled
+        as save_adjoint_kernels.f90 writes them, fluid kernels dict rho/kappa/rhop/alpha), kernels [nspec,25]."""
+        n = self.cfg.nstep
+        adj = np.ascontiguousarray(np.asarray(adj, np.float64).reshape(-1))
+        if adj.size == 0:
+            adj = np.zeros(1)
+        st = np.ascontiguousarray(np.asarray(stacey_store, np.float64).reshape(-1))
+        if st.size == 0:
+            st = np.zeros(1)
+        lel = None if lastframe_el is None else np.ascontiguousarray(
+            np.concatenate([np.asarray(x, np.float64).reshape(-1) for x in lastframe_el]))
+        lac = None if lastframe_ac is None else np.ascontiguousarray(
+            np.concatenate([np.asarray(x, np.float64).reshape(-1) for x in lastframe_ac]))
+        seis = np.zeros((max(1, self.cfg.nrec), 2, n))
+        kel = np.zeros(3 * self.nspec * NGLL2)
+        kac = np.zeros(4 * self.nspec * NGLL2)
+        rc = self.lib.orc_adjoint_full(self.h, adj, st, _ptr(lel), _ptr(lac), _ptr(seis), _ptr(kel), _ptr(kac))
+        if rc != 0:
+            raise NotImplementedError("oracle: adjoint runs with CPML or attenuation, or without the last frames")
+        kel = kel.reshape(3, self.nspec, NGLL2)
+        kac = kac.reshape(4, self.nspec, NGLL2)
+        return (seis[: self.cfg.nrec], dict(zip(("rho", "mu", "kappa"), kel)),
+                dict(zip(("rho", "kappa", "rhop", "alpha"), kac)))
 
     def gradient(self, obs):
         """Exact gradient of chi = 1/2 dt sum |s - obs|^2 over a full forward run from rest (seismotype

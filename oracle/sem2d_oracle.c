@@ -1759,8 +1759,9 @@ static void acoustic_forces(orc_sim *s, const REAL *pot, REAL *pot_dd) {
 #endif
 }
 
-/* compute_stacey_acoustic.f90:34-160 */
-static void acoustic_stacey(orc_sim *s, const REAL *pot_dot, REAL *pot_dd) {
+/* compute_stacey_acoustic.f90:34-160; with store, the SAVE_FORWARD copy of each subtracted term
+   (b_absorb_acoustic_*, :47-108) in component 0 of store[it][nabs][5][2] */
+static void acoustic_stacey(orc_sim *s, const REAL *pot_dot, REAL *pot_dd, REAL *store, int it) {
   for (int a = 0; a < s->nabs; a++) {
     int e = s->abs_ispec[a] - 1, code = s->abs_code[a];
     if (!s->is_acoustic[e]) continue;
@@ -1772,7 +1773,9 @@ static void acoustic_stacey(orc_sim *s, const REAL *pot_dot, REAL *pot_dd) {
       REAL nx, nz, j1d;
       edge_normal(s, code, l, &nx, &nz, &j1d);
       REAL weight = j1d * ((code == 4 || code == 2) ? s->wxgll[j] : s->wxgll[i]);
-      pot_dd[g] = pot_dd[g] - pot_dot[g] * weight / s->rho_vp[l];
+      REAL term = pot_dot[g] * weight / s->rho_vp[l];
+      pot_dd[g] = pot_dd[g] - term;
+      if (store) store[(((size_t)it * s->nabs + a) * NGLL + k) * 2] = term;
     }
   }
 }
@@ -1853,17 +1856,18 @@ static void coupling_elastic_ac(orc_sim *s, const REAL *pot_dd, REAL *accel) {
 }
 
 /* enforce_acoustic_free_surface.f90:34-80 */
-static void enforce_free_surface(orc_sim *s) {
+static void free_surface_fields(orc_sim *s, REAL *pot, REAL *pot_dot, REAL *pot_dd) {
   for (int f = 0; f < s->nfree; f++)
     for (int k = 0; k < NGLL; k++) {
       int i, j;
       side_point(s->free_code[f], k, &i, &j);
       int g = s->ibool[(size_t)s->free_ispec[f] * NGLL2 + j * NGLL + i] - 1;
-      s->pot[g] = (REAL)0;
-      s->pot_dot[g] = (REAL)0;
-      s->pot_dd[g] = (REAL)0;
+      pot[g] = (REAL)0;
+      pot_dot[g] = (REAL)0;
+      pot_dd[g] = (REAL)0;
     }
 }
+static void enforce_free_surface(orc_sim *s) { free_surface_fields(s, s->pot, s->pot_dot, s->pot_dd); }
 
 static void record_seismograms(orc_sim *s, const REAL *d, const REAL *v, const REAL *a, double *seis, int it,
                                int nstep) {
@@ -2322,7 +2326,7 @@ void orc_forward(void *h, int nstep_run, double *seis, double *stacey_store_d) {
     if (s->any_acoustic) {
       enforce_free_surface(s);
       if (s->pml_on) pml_boundary_acoustic(s);
-      if (s->p.stacey && s->nabs > 0) acoustic_stacey(s, pv, pdd);
+      if (s->p.stacey && s->nabs > 0) acoustic_stacey(s, pv, pdd, store, step);
       if (s->ncoup > 0) coupling_acoustic_el(s, d, pdd);
       add_sources_acoustic(s, pdd, it);
       acoustic_forces(s, pa, pdd);
@@ -2508,6 +2512,7 @@ static void kernels_accumulate(orc_sim *s, const REAL *displ, const REAL *accel,
   /* compute_kernels.f90 compute_kernels_el (P-SV, isotropic); the jacobian terms are double there */
   const REAL (*H)[NGLL] = (const REAL(*)[NGLL])s->hprime;
   for (int e = 0; e < s->nspec; e++) {
+    if (s->is_acoustic[e]) continue;  /* elastic elements only (compute_kernels.f90:119) */
     const int *ib = s->ibool + (size_t)e * NGLL2;
     for (int j = 0; j < NGLL; j++)
       for (int i = 0; i < NGLL; i++) {
@@ -2643,6 +2648,234 @@ void orc_adjoint(void *h, const double *adj, const double *stacey_store, const d
   }
   free(bd); free(bv); free(ba); free(rho_kl); free(mu_kl); free(kappa_kl); free(sadj);
   _mm_setcsr(csr);
+}
+
+/* compute_kernels.f90:293-449 compute_kernels_ac (CPU path, NTSTEP_BETWEEN_COMPUTE_KERNELS = 1): gradients of
+   the adjoint potential and of the reconstructed potential (the xi/gamma derivatives in single precision, the
+   jacobian products in double, rounded to single precision), then
+   rho_ac_kl += 1/rho (grad pot . grad b_pot) deltat and kappa_ac_kl += 1/kappa pot_dd b_pot deltat */
+static void kernels_ac_accumulate(orc_sim *s, const REAL *pot, const REAL *pot_dd, const REAL *b_pot, REAL deltat,
+                                  REAL *rho_ac_kl, REAL *kappa_ac_kl) {
+  const REAL (*H)[NGLL] = (const REAL(*)[NGLL])s->hprime;
+  for (int e = 0; e < s->nspec; e++) {
+    if (!s->is_acoustic[e]) continue;
+    const int *ib = s->ibool + (size_t)e * NGLL2;
+    for (int j = 0; j < NGLL; j++)
+      for (int i = 0; i < NGLL; i++) {
+        const size_t l = (size_t)e * NGLL2 + j * NGLL + i;
+        const int g = ib[j * NGLL + i] - 1;
+        REAL t1 = 0, t2 = 0, bt1 = 0, bt2 = 0;
+        for (int k = 0; k < NGLL; k++) {
+          const int g1 = ib[j * NGLL + k] - 1, g2 = ib[k * NGLL + i] - 1;
+          t1 = t1 + pot[g1] * H[i][k];
+          bt1 = bt1 + b_pot[g1] * H[i][k];
+          t2 = t2 + pot[g2] * H[j][k];
+          bt2 = bt2 + b_pot[g2] * H[j][k];
+        }
+        const double xixl = s->xix[l], xizl = s->xiz[l], gaxl = s->gammax[l], gazl = s->gammaz[l];
+        const REAL a1 = (REAL)((double)t1 * xixl + (double)t2 * gaxl);
+        const REAL a2 = (REAL)((double)t1 * xizl + (double)t2 * gazl);
+        const REAL b1 = (REAL)((double)bt1 * xixl + (double)bt2 * gaxl);
+        const REAL b2 = (REAL)((double)bt1 * xizl + (double)bt2 * gazl);
+        const REAL dot = a1 * b1 + a2 * b2;
+        rho_ac_kl[l] = rho_ac_kl[l] + (REAL)1 / s->rhostore[l] * dot * deltat;
+        kappa_ac_kl[l] = kappa_ac_kl[l] + (REAL)1 / s->kappastore[l] * pot_dd[g] * b_pot[g] * deltat;
+      }
+  }
+}
+
+/* SIMULATION_TYPE=3 run with backward reconstruction over fluid, solid and coupled meshes (iterate_time.F90:155-390,
+   Newmark, no CPML, no attenuation).  Per step: the predictors of the adjoint and of the reconstructed fields;
+   the adjoint solid (compute_forces_viscoelastic_calling_routine.F90:34-205: Stacey, coupling with the adjoint
+   potential as the pressure, compute_coupling_viscoelastic_ac.f90:98-102, adjoint sources, elements, mass,
+   corrector), then the adjoint fluid (compute_forces_viscoacoustic_calling_routine.F90:34-228: Stacey, coupling
+   with minus the new adjoint solid acceleration, compute_coupling_acoustic_el.f90:104-109, adjoint sources
+   without 1/kappa, compute_add_sources_acoustic.f90:247-290, elements, mass, corrector); then the reconstructed
+   fluid and solid (:234-374 and compute_forces_viscoelastic_calling_routine.F90:214-352: stored Stacey terms of step
+   NSTEP-it+1, forward coupling, sources at NSTEP-it+1); after step 1 the last forward frames replace the
+   reconstructed fields (read_forward_arrays.f90:52-98); seismograms; kernels.
+   adj[nrec][2][nstep]: a receiver in a fluid element takes component 0 (source_adjoint(:,:,1)).
+   stacey_store[nstep][nabs][5][2] from the SAVE_FORWARD run (fluid faces in component 0).
+   last_el: displ, veloc, accel [3][2*nglob] (may be NULL without solids); last_ac: potential and its two time
+   derivatives [3][nglob] (NULL without fluids).  kernels_el[3][ne]: rho, mu, kappa as save_adjoint_kernels.f90
+   scales them; kernels_ac[4][ne]: rho_ac_kl, kappa_ac_kl, rhop_ac_kl, alpha_ac_kl (compute_kernels.f90:426-439).
+   Returns 0, or -1 for a set-up it does not cover. */
+int orc_adjoint_full(void *h, const double *adj, const double *stacey_store, const double *last_el,
+                     const double *last_ac, double *seis, double *kernels_el, double *kernels_ac) {
+  orc_sim *s = (orc_sim *)h;
+  if (s->pml_on || s->att) return -1;
+  const int any_ac = s->nspec_acoustic > 0, any_el = s->nspec_acoustic < s->nspec;
+  if ((any_el && !last_el) || (any_ac && !last_ac)) return -1;
+  unsigned int csr = _mm_getcsr();
+  if (s->ftz) _mm_setcsr(csr | 0x8040);
+  const int nstep = s->p.nstep;
+  REAL deltat = (REAL)s->p.dt;
+  REAL deltatover2 = (REAL)(0.5 * (double)deltat);
+  REAL deltatsqover2 = (REAL)(0.5 * (double)deltat * (double)deltat);
+  REAL b_deltat = -deltat;
+  REAL b_deltatover2 = (REAL)(0.5 * (double)b_deltat);
+  REAL b_deltatsqover2 = (REAL)(0.5 * (double)b_deltat * (double)b_deltat);
+  const size_t n1 = (size_t)s->nglob, n2 = 2 * n1, ne = (size_t)s->nspec * NGLL2;
+  REAL *d = s->displ, *v = s->veloc, *a = s->accel;
+  REAL *pa = s->pot, *pv = s->pot_dot, *pdd = s->pot_dd;
+  memset(d, 0, sizeof(REAL) * n2); memset(v, 0, sizeof(REAL) * n2); memset(a, 0, sizeof(REAL) * n2);
+  memset(pa, 0, sizeof(REAL) * n1); memset(pv, 0, sizeof(REAL) * n1); memset(pdd, 0, sizeof(REAL) * n1);
+  REAL *bd = (REAL *)calloc(n2, sizeof(REAL)), *bv = (REAL *)calloc(n2, sizeof(REAL)), *ba = (REAL *)calloc(n2, sizeof(REAL));
+  REAL *bp = (REAL *)calloc(n1, sizeof(REAL)), *bpv = (REAL *)calloc(n1, sizeof(REAL)), *bpdd = (REAL *)calloc(n1, sizeof(REAL));
+  REAL *tmp1 = (REAL *)calloc(n1, sizeof(REAL)), *tmp2 = (REAL *)calloc(n2, sizeof(REAL));
+  REAL *rho_kl = (REAL *)calloc(ne, sizeof(REAL)), *mu_kl = (REAL *)calloc(ne, sizeof(REAL)), *kappa_kl = (REAL *)calloc(ne, sizeof(REAL));
+  REAL *rho_ac = (REAL *)calloc(ne, sizeof(REAL)), *kappa_ac = (REAL *)calloc(ne, sizeof(REAL));
+  REAL *sadj = (REAL *)calloc((size_t)(s->nrec + 1) * 2 * nstep, sizeof(REAL));
+  for (size_t k = 0; k < (size_t)s->nrec * 2 * nstep; k++) sadj[k] = (REAL)adj[k];
+  for (int it = 1; it <= nstep; it++) {
+    const int it_tmp = nstep - it + 1;
+    /* update_displ_Newmark, then update_displ_Newmark_backward (update_displacement_Newmark.F90:58-120) */
+    for (size_t k = 0; k < n1 && any_ac; k++) {
+      pa[k] = pa[k] + deltat * pv[k] + deltatsqover2 * pdd[k];
+      pv[k] = pv[k] + deltatover2 * pdd[k];
+      pdd[k] = (REAL)0;
+    }
+    for (size_t k = 0; k < n2 && any_el; k++) {
+      d[k] = d[k] + deltat * v[k] + deltatsqover2 * a[k];
+      v[k] = v[k] + deltatover2 * a[k];
+      a[k] = (REAL)0;
+    }
+    for (size_t k = 0; k < n1 && any_ac; k++) {
+      bp[k] = bp[k] + b_deltat * bpv[k] + b_deltatsqover2 * bpdd[k];
+      bpv[k] = bpv[k] + b_deltatover2 * bpdd[k];
+      bpdd[k] = (REAL)0;
+    }
+    for (size_t k = 0; k < n2 && any_el; k++) {
+      bd[k] = bd[k] + b_deltat * bv[k] + b_deltatsqover2 * ba[k];
+      bv[k] = bv[k] + b_deltatover2 * ba[k];
+      ba[k] = (REAL)0;
+    }
+    /* adjoint solid first (iterate_time.F90:227-246) */
+    if (any_el) {
+      if (s->p.stacey && s->nabs > 0) stacey_forces(s, v, a, NULL, 0, 0);
+      if (s->ncoup > 0) {
+        for (size_t k = 0; k < n1; k++) tmp1[k] = -pa[k];  /* pressure = potential (coupling negates its input) */
+        coupling_elastic_ac(s, tmp1, a);
+      }
+      for (int r = 0; r < s->nrec; r++) {
+        const int e = s->rec_ispec[r] - 1;
+        if (s->is_acoustic[e]) continue;
+        for (int j = 0; j < NGLL; j++)
+          for (int i = 0; i < NGLL; i++) {
+            const int g = s->ibool[e * NGLL2 + j * NGLL + i] - 1;
+            const REAL hl = s->xir[r * NGLL + i] * s->gammar[r * NGLL + j];
+            a[2 * g] = a[2 * g] + hl * sadj[((size_t)r * 2 + 0) * nstep + (it_tmp - 1)];
+            a[2 * g + 1] = a[2 * g + 1] + hl * sadj[((size_t)r * 2 + 1) * nstep + (it_tmp - 1)];
+          }
+      }
+      element_forces(s, d, a);
+      for (size_t k = 0; k < n2; k++) a[k] = a[k] * s->rmass_inv[k];
+      for (size_t k = 0; k < n2; k++) v[k] = v[k] + deltatover2 * a[k];
+    }
+    /* then the adjoint fluid */
+    if (any_ac) {
+      free_surface_fields(s, pa, pv, pdd);
+      if (s->p.stacey && s->nabs > 0) acoustic_stacey(s, pv, pdd, NULL, 0);
+      if (s->ncoup > 0) {
+        for (size_t k = 0; k < n2; k++) tmp2[k] = -a[k];  /* displacement = minus the adjoint acceleration */
+        coupling_acoustic_el(s, tmp2, pdd);
+      }
+      for (int r = 0; r < s->nrec; r++) {
+        const int e = s->rec_ispec[r] - 1;
+        if (!s->is_acoustic[e]) continue;
+        for (int j = 0; j < NGLL; j++)
+          for (int i = 0; i < NGLL; i++) {
+            const int g = s->ibool[e * NGLL2 + j * NGLL + i] - 1;
+            const REAL stf = s->xir[r * NGLL + i] * s->gammar[r * NGLL + j] * sadj[((size_t)r * 2) * nstep + (it_tmp - 1)];
+            pdd[g] = pdd[g] + stf;
+          }
+      }
+      acoustic_forces(s, pa, pdd);
+      for (size_t k = 0; k < n1; k++) {
+        pdd[k] = pdd[k] * s->rmass_ac_inv[k];
+        pv[k] = pv[k] + deltatover2 * pdd[k];
+      }
+      free_surface_fields(s, pa, pv, pdd);
+    }
+    /* reconstructed fluid, then solid, in the forward order */
+    if (any_ac) {
+      free_surface_fields(s, bp, bpv, bpdd);
+      if (s->p.stacey && s->nabs > 0)
+        for (int f = 0; f < s->nabs; f++) {
+          const int e = s->abs_ispec[f] - 1, code = s->abs_code[f];
+          if (!s->is_acoustic[e]) continue;
+          for (int k = s->abs_jbegin[f]; k <= s->abs_jend[f]; k++) {
+            int i, j;
+            side_point(code, k, &i, &j);
+            const int g = s->ibool[(size_t)e * NGLL2 + j * NGLL + i] - 1;
+            bpdd[g] = bpdd[g] - (REAL)stacey_store[(((size_t)(it_tmp - 1) * s->nabs + f) * NGLL + k) * 2];
+          }
+        }
+      if (s->ncoup > 0) coupling_acoustic_el(s, bd, bpdd);
+      add_sources_acoustic(s, bpdd, it_tmp);
+      acoustic_forces(s, bp, bpdd);
+      for (size_t k = 0; k < n1; k++) {
+        bpdd[k] = bpdd[k] * s->rmass_ac_inv[k];
+        bpv[k] = bpv[k] + b_deltatover2 * bpdd[k];
+      }
+      free_surface_fields(s, bp, bpv, bpdd);
+    }
+    if (any_el) {
+      if (s->p.stacey && s->nabs > 0)
+        for (int f = 0; f < s->nabs; f++) {
+          const int e = s->abs_ispec[f] - 1, code = s->abs_code[f];
+          if (s->is_acoustic[e]) continue;
+          for (int k = 0; k < NGLL; k++) {
+            int i, j;
+            if (code == 4) { i = 0; j = k; }
+            else if (code == 2) { i = NGLL - 1; j = k; }
+            else if (code == 1) { i = k; j = 0; }
+            else { i = k; j = NGLL - 1; }
+            const int g = s->ibool[e * NGLL2 + j * NGLL + i] - 1;
+            const double *st = stacey_store + (((size_t)(it_tmp - 1) * s->nabs + f) * NGLL + k) * 2;
+            ba[2 * g] = ba[2 * g] - (REAL)st[0];
+            ba[2 * g + 1] = ba[2 * g + 1] - (REAL)st[1];
+          }
+        }
+      if (s->ncoup > 0) coupling_elastic_ac(s, bpdd, ba);
+      add_sources(s, ba, it_tmp);
+      element_forces(s, bd, ba);
+      for (size_t k = 0; k < n2; k++) ba[k] = ba[k] * s->rmass_inv[k];
+      for (size_t k = 0; k < n2; k++) bv[k] = bv[k] + b_deltatover2 * ba[k];
+    }
+    if (it == 1) {
+      for (size_t k = 0; k < n2 && any_el; k++) {
+        bd[k] = (REAL)last_el[k];
+        bv[k] = (REAL)last_el[n2 + k];
+        ba[k] = (REAL)last_el[2 * n2 + k];
+      }
+      for (size_t k = 0; k < n1 && any_ac; k++) {
+        bp[k] = (REAL)last_ac[k];
+        bpv[k] = (REAL)last_ac[n1 + k];
+        bpdd[k] = (REAL)last_ac[2 * n1 + k];
+      }
+      if (any_ac) free_surface_fields(s, bp, bpv, bpdd);
+    }
+    if (seis && s->nrec > 0) record_seismograms(s, d, v, a, seis, it - 1, nstep);
+    if (any_el) kernels_accumulate(s, d, a, bd, rho_kl, mu_kl, kappa_kl);
+    if (any_ac) kernels_ac_accumulate(s, pa, pdd, bp, deltat * (REAL)1, rho_ac, kappa_ac);
+  }
+  for (size_t l = 0; l < ne && kernels_el; l++) {
+    REAL rhol = s->rhostore[l], mul = s->mustore[l], kappal = s->kappastore[l];
+    kernels_el[l] = (double)(-rhol * (deltat * (REAL)1) * rho_kl[l]);
+    kernels_el[ne + l] = (double)(REAL)(-2.0 * (double)mul * (double)(deltat * (REAL)1) * (double)mu_kl[l]);
+    kernels_el[2 * ne + l] = (double)(-kappal * (deltat * (REAL)1) * kappa_kl[l]);
+  }
+  for (size_t l = 0; l < ne && kernels_ac; l++) {
+    kernels_ac[l] = rho_ac[l];
+    kernels_ac[ne + l] = kappa_ac[l];
+    kernels_ac[2 * ne + l] = (double)(rho_ac[l] + kappa_ac[l]);
+    kernels_ac[3 * ne + l] = (double)((REAL)2 * kappa_ac[l]);
+  }
+  free(bd); free(bv); free(ba); free(bp); free(bpv); free(bpdd); free(tmp1); free(tmp2);
+  free(rho_kl); free(mu_kl); free(kappa_kl); free(rho_ac); free(kappa_ac); free(sadj);
+  _mm_setcsr(csr);
+  return 0;
 }
 
 /* One accumulation step of the kernels (unscaled) for given fields; out[3][nspec*25]. */

@@ -68,6 +68,53 @@ def copy_inputs(example, files, out, subdir=""):
     print("copied", files, "->", dst)
 
 
+def claerbout_log_numbers(example, out):
+    """The figures that EXAMPLES/BENCHMARK_CLAERBOUT_ADJOINT/<example>/REF_KERNEL/output_run.log prints for the
+    reference's kernel benchmark (its run_this_example_kernel.sh): misfit of the synthetics against the data, the
+    adjoint source's peak, the kernels' min/max and sums of squares, the model-space and data-space inner
+    products; written as JSON, numbers as printed."""
+    import json
+    import re
+    with open(os.path.join(REF, "EXAMPLES", "BENCHMARK_CLAERBOUT_ADJOINT", example, "REF_KERNEL",
+                           "output_run.log")) as f:
+        text = f.read()
+    num = r"([-+0-9.eE]+)"
+
+    def first(pattern, n=1):
+        m = re.search(pattern, text)
+        if not m:
+            raise ValueError(f"{example}: '{pattern}' not in the log")
+        return [float(m.group(k + 1)) for k in range(n)]
+
+    vals = {
+        "max_waveform_difference": first(r"maximum waveform difference \(syn - dat\) =\s+" + num)[0],
+        "total_misfit": first(r"total misfit: sum\(s - d\)\^2 =\s+" + num)[0],
+        "max_adjoint_source": first(r"maximum amplitude \|f\^adj\| =\s+" + num)[0],
+        "kernel_rhop_minmax": first(r"kernel rhop : min/max =\s+" + num + r"\s+" + num, 2),
+        "kernel_alpha_minmax": first(r"kernel alpha: min/max =\s+" + num + r"\s+" + num, 2),
+        "kernel_beta_minmax": first(r"kernel beta : min/max =\s+" + num + r"\s+" + num, 2),
+        "norm_rhop": first(r"norm of rhop kernel  =\s+" + num)[0],
+        "norm_alpha": first(r"norm of alpha kernel =\s+" + num)[0],
+        "norm_beta": first(r"norm of beta kernel  =\s+" + num)[0],
+        "modelspace": first(r"MODELSPACE\s+=\s+" + num)[0],
+        "dataspace": first(r"DATASPACE\s+=\s+" + num)[0],
+        "dataspace_new": first(r"DATASPACE_NEW\s+=\s+" + num)[0],
+    }
+    vals["source"] = f"EXAMPLES/BENCHMARK_CLAERBOUT_ADJOINT/{example}/REF_KERNEL/output_run.log"
+    with open(os.path.join(HERE, out), "w") as f:
+        json.dump(vals, f, indent=1)
+    print("wrote", out)
+
+
+def claerbout_inputs(example, out):
+    """Par_file, SOURCE and interfaces of a Claerbout benchmark, plus the STATIONS file its
+    create_STATIONS_file.py writes for one station line in x and z (one station at x0 = 700, z0 = 500)."""
+    copy_inputs(os.path.join("BENCHMARK_CLAERBOUT_ADJOINT", example),
+                ["DATA/Par_file", "DATA/SOURCE", "DATA/interfaces.dat"], out)
+    with open(os.path.join(HERE, "examples", out, "STATIONS"), "w") as f:
+        f.write("S1\tAA\t700.0\t500.0\t0.0\t0.0\n")
+
+
 def trace_digests(out):
     """sha256 of one ASCII trace per example and recording kind the reference keeps under EXAMPLES/*/REF_SEIS,
     with its parsed values, so the writer's byte compatibility is checked without copying the files."""
@@ -130,3 +177,6 @@ if __name__ == "__main__":
     copy_inputs(ext, ["mesh/mesh", "mesh/nodes_coords", "mesh/mat", "mesh/surface_free", "mesh/surface_abs"],
                 "fluid_solid_external_mesh", subdir="mesh")
     pack_float32(ext, "REF_SEIS", [f"AA.S{k:04d}" for k in range(1, 13)], "fluid_solid_external_mesh_ref_seis.npz")
+    for example, name in (("ACOUSTIC", "claerbout_acoustic"), ("ACOUSTIC_ELASTIC", "claerbout_acoustic_elastic")):
+        claerbout_inputs(example, name)
+        claerbout_log_numbers(example, f"{name}_kernel_log.json")
