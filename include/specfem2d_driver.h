@@ -128,8 +128,8 @@ void sf2d_sim_get_stacey(void* sim, int* ispec, int* ij, float* normal, float* j
 
 /* Forward time loop (SIMULATION_TYPE=1): runs the next nstep_run steps (continuing after the steps
    already run, up to NSTEP), displacement seismograms of those steps into seis[nrec][2][nstep_run].
-   With save_forward the Stacey traction per step is copied to absorb[nstep_run][nabs][5][2] (may be
-   NULL).  If timing is not NULL, HIP events on the library stream give {loop ms, predictor ms,
+   With save_forward the Stacey term per step is copied to absorb[nstep_run][nabs][5][2] (may be NULL;
+   the traction of a solid face, the potential term of a fluid face in component 0).  If timing is not NULL, HIP events on the library stream give {loop ms, predictor ms,
    forces ms, boundary+sources ms, corrector ms, seismogram ms}: the loop and the element-force
    launches always, the other phases only when timing[0] < 0 on entry.  Returns 0 on success. */
 int sf2d_sim_forward(void* sim, int nstep_run, double* seis, float* absorb, double* timing);
@@ -141,6 +141,18 @@ int sf2d_sim_forward(void* sim, int nstep_run, double* seis, float* absorb, doub
    kernels[3][nspec][25] in ispec order.  Returns 0 on success. */
 int sf2d_sim_adjoint(void* sim, const float* adj, const float* absorb, const float* last_displ,
                      const float* last_veloc, const float* last_accel, double* seis, float* kernels);
+/* Adjoint run over fluid, solid and coupled meshes (no CPML, no attenuation): as sf2d_sim_adjoint, plus the
+   last fluid frame (potential and its first and second time derivatives, [nglob] each; NULL without fluid
+   elements; the solid frame is NULL without solid elements).  A receiver in a fluid element takes component x of
+   adj as its source (source_adjoint(:,:,1)) and records the adjoint field's seismotype.  kernels[3][nspec][25]:
+   solid K_rho, K_mu, K_kappa as sf2d_sim_adjoint (zero in fluid elements); kernels_ac[4][nspec][25] (may be NULL):
+   the fluid's rho_ac_kl, kappa_ac_kl, rho' = rho + kappa and c = 2 kappa (compute_kernels.f90:426-439, zero in
+   solid elements).  Coupled meshes run the adjoint solid, the adjoint fluid, then the reconstructed fluid and
+   solid (iterate_time.F90:227-358).  Returns 0 on success. */
+int sf2d_sim_adjoint_full(void* sim, const float* adj, const float* absorb, const float* last_displ,
+                          const float* last_veloc, const float* last_accel, const float* last_pot,
+                          const float* last_pot_dot, const float* last_pot_dd, double* seis, float* kernels,
+                          float* kernels_ac);
 /* replaces the model (MODEL=gll parameters rho, vp, vs per GLL point, [nspec][25]) of a simulation: moduli,
    Stacey impedances and mass matrix are recomputed and uploaded.  Returns 0 on success. */
 int sf2d_sim_set_model(void* sim, const double* rho, const double* vp, const double* vs);

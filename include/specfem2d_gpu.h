@@ -13,17 +13,17 @@
  * Errors: like the reference (src/gpu/helper_functions.cu:95-121), a failing
  * call prints a message, appends it to OUTPUT_FILES/error_message_*.txt when
  * that directory exists, and terminates the process with EXIT_FAILURE.
- * Implemented: elastic forward and adjoint (SIMULATION_TYPE 1 and 3) runs
- * with Stacey edges, sources, seismograms and kernels; viscoelastic
- * attenuation (forward, solid meshes); forward fluid (acoustic) elements
- * with their free surface, Stacey edges, pressure sources and seismograms
- * and fluid-solid coupling; CPML layers (forward, elastic and acoustic
+ * Implemented: forward and adjoint (SIMULATION_TYPE 1 and 3) runs of solid,
+ * fluid (acoustic) and coupled fluid-solid meshes with Stacey edges, free
+ * surfaces, sources, seismograms, the stored Stacey terms of SAVE_FORWARD,
+ * the reconstructed fields and the solid and fluid kernels; viscoelastic
+ * attenuation (forward runs); CPML layers (forward, elastic and acoustic
  * elements: prepare_pml_device_, pml_boundary_acoustic_cuda_, and the
  * extensions of specfem2d_gpu_ext.h).  Entry points of physics this library
- * does not implement (backward/adjoint fluid wavefields and acoustic
- * kernels, transfers of attenuation memory variables, MPI halo exchange,
- * kernel smoothing, Hessians, moving sources) are exported so that the
- * reference binary links, and fail loudly when called.
+ * does not implement (viscoacoustic attenuation, transfers of attenuation
+ * memory variables, MPI halo exchange, kernel smoothing, Hessians, moving
+ * sources) are exported so that the reference binary links, and fail loudly
+ * when called.
  *
  * Each declaration cites the reference definition it replaces.
  */

@@ -86,6 +86,7 @@ def _lib(precision: str):
         lib.orc_adjoint_full.argtypes = [P, dp, dp, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         lib.orc_adjoint_full.restype = C.c_int
         lib.orc_kernel_increment.argtypes = [P, dp, dp, dp, dp]
+        lib.orc_kernel_ac_increment.argtypes = [P, dp, dp, dp, C.c_double, dp]
         lib.orc_gradient.argtypes = [P, dp, dp, dp]
         lib.orc_gradient.restype = C.c_double
         lib.orc_set_attenuation.argtypes = [P, dp, dp, C.c_int, C.c_double]
@@ -423,6 +424,14 @@ class OracleSim:
         grad = grad.reshape(3, self.nspec, NGLL2)
         return chi, {"rho": grad[0], "vp": grad[1], "vs": grad[2]}, dict(
             zip(("kappa", "mu", "rho_mass", "rho_vp", "rho_vs"), parts))
+
+    def kernel_ac_increment(self, pot, pot_dd, b_pot, dt):
+        """One fluid kernel accumulation (compute_kernels_ac) for given fields: dict rho/kappa of [nspec,25]."""
+        args = [np.ascontiguousarray(np.asarray(x, np.float64).reshape(-1)) for x in (pot, pot_dd, b_pot)]
+        out = np.zeros(2 * self.nspec * NGLL2)
+        self.lib.orc_kernel_ac_increment(self.h, *args, float(dt), out)
+        out = out.reshape(2, self.nspec, NGLL2)
+        return {"rho": out[0], "kappa": out[1]}
 
     def kernel_increment(self, displ, accel, b_displ):
         """One unscaled kernel accumulation (compute_kernels_el) for given fields."""

@@ -2878,6 +2878,20 @@ int orc_adjoint_full(void *h, const double *adj, const double *stacey_store, con
   return 0;
 }
 
+/* One accumulation step of the fluid kernels for given fields (adjoint potential and its second time derivative,
+   reconstructed potential) and time step dt; out[2][nspec*25]: rho_ac_kl, kappa_ac_kl increments. */
+void orc_kernel_ac_increment(void *h, const double *pot, const double *pot_dd, const double *b_pot, double dt,
+                             double *out) {
+  orc_sim *s = (orc_sim *)h;
+  const size_t n1 = (size_t)s->nglob, ne = (size_t)s->nspec * NGLL2;
+  REAL *p = (REAL *)malloc(sizeof(REAL) * n1), *a = (REAL *)malloc(sizeof(REAL) * n1), *b = (REAL *)malloc(sizeof(REAL) * n1);
+  for (size_t k = 0; k < n1; k++) { p[k] = (REAL)pot[k]; a[k] = (REAL)pot_dd[k]; b[k] = (REAL)b_pot[k]; }
+  REAL *r = (REAL *)calloc(ne, sizeof(REAL)), *kk = (REAL *)calloc(ne, sizeof(REAL));
+  kernels_ac_accumulate(s, p, a, b, (REAL)dt, r, kk);
+  for (size_t l = 0; l < ne; l++) { out[l] = r[l]; out[ne + l] = kk[l]; }
+  free(p); free(a); free(b); free(r); free(kk);
+}
+
 /* One accumulation step of the kernels (unscaled) for given fields; out[3][nspec*25]. */
 void orc_kernel_increment(void *h, const double *displ, const double *accel, const double *b_displ, double *out) {
   orc_sim *s = (orc_sim *)h;

@@ -281,36 +281,53 @@ __global__ __launch_bounds__(256) void k_ac_free_surface(const int* __restrict__
   pdd[g] = 0.f;
 }
 
-// compute_stacey_acoustic.f90:34-160: pdd -= pd w / (rho vp) along the fluid's absorbing edges
+// compute_stacey_acoustic.f90:34-160: pdd -= pd w / (rho vp) along the fluid's absorbing edges; with store, each
+// term also goes to its edge slot (the SAVE_FORWARD copy b_absorb_acoustic_*, :46-48, 65-67, 86-88, 107-109)
 __global__ __launch_bounds__(256) void k_ac_stacey(int n, const int* __restrict__ pts, const float* __restrict__ w,
                                                    const float* __restrict__ rvp, const float* __restrict__ pd,
-                                                   float* __restrict__ pdd) {
+                                                   float* __restrict__ pdd, const int* __restrict__ slot,
+                                                   float* __restrict__ store) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
   const int g = pts[k];
-  atomic_add(&pdd[g], -(pd[g] * w[k] / rvp[k]));
+  const float t = pd[g] * w[k] / rvp[k];
+  atomic_add(&pdd[g], -t);
+  if (store) store[slot[k]] = t;
 }
 
-// fluid side of a fluid-solid edge: pdd += w (u . n) (compute_coupling_acoustic_el.f90:34-150)
+// reconstructed field: b_pdd -= the stored term of step NSTEP - it + 1 (compute_stacey_acoustic.f90:162-240)
+__global__ __launch_bounds__(256) void k_ac_stacey_backward(int n, const int* __restrict__ pts,
+                                                            const int* __restrict__ slot,
+                                                            const float* __restrict__ store, float* __restrict__ bpdd) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  atomic_add(&bpdd[pts[k]], -store[slot[k]]);
+}
+
+// fluid side of a fluid-solid edge: pdd += w (u . n) (compute_coupling_acoustic_el.f90:34-150); sign -1 gives the
+// adjoint run's term, u = minus the adjoint acceleration (:104-109)
 __global__ __launch_bounds__(256) void k_coupling_ac_el(int n, const int* __restrict__ pts,
                                                         const float2* __restrict__ nrm, const float* __restrict__ w,
-                                                        const float2* __restrict__ displ, float* __restrict__ pdd) {
+                                                        const float2* __restrict__ displ, float sign,
+                                                        float* __restrict__ pdd) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
   const int g = pts[k];
   const float2 u = displ[g], nv = nrm[k];
-  const float dn = u.x * nv.x + u.y * nv.y;
+  const float dn = (sign * u.x) * nv.x + (sign * u.y) * nv.y;
   atomic_add(&pdd[g], w[k] * dn);
 }
 
-// solid side: accel += w n p with the fluid pressure p = -pdd (compute_coupling_viscoelastic_ac.f90:34-130)
+// solid side: accel += w n p with the fluid pressure p = -pdd (compute_coupling_viscoelastic_ac.f90:34-130); with
+// sign -1 and the adjoint potential in place of pdd, the adjoint run's pressure p = potential (:98-102)
 __global__ __launch_bounds__(256) void k_coupling_el_ac(int n, const int* __restrict__ pts,
                                                         const float2* __restrict__ nrm, const float* __restrict__ w,
-                                                        const float* __restrict__ pdd, float* __restrict__ accel) {
+                                                        const float* __restrict__ pdd, float sign,
+                                                        float* __restrict__ accel) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
   const int g = pts[k];
-  const float pr = -pdd[g];
+  const float pr = sign * -pdd[g];
   const float2 nv = nrm[k];
   const float wk = w[k];
   atomic_add(&accel[2 * (size_t)g], wk * nv.x * pr);
@@ -327,6 +344,68 @@ __global__ __launch_bounds__(64) void k_ac_sources(int nas, int it, int nsrc, co
   const size_t l = (size_t)s * NGLL2 + p;
   const float f = stf[(size_t)it * nsrc + src[s]];
   atomic_add(&pdd[pts[l]], sa[l] * f / kap[l]);
+}
+
+// adjoint sources of receivers in fluid elements: pdd += h_i h_j adj(NSTEP - it + 1), without a 1/kappa factor
+// (compute_add_sources_acoustic.f90:247-290); adj is source_adjoint(nrec, NSTEP, 2), component 1; rows of
+// receivers in solid elements (pts = -1) are skipped
+__global__ __launch_bounds__(64) void k_ac_adjoint_sources(int nrec, int it, int nstep, const int* __restrict__ pts,
+                                                           const float* __restrict__ xir,
+                                                           const float* __restrict__ gammar,
+                                                           const float* __restrict__ adj, float* __restrict__ pdd) {
+  const int r = blockIdx.x, p = threadIdx.x;
+  if (r >= nrec || p >= NGLL2) return;
+  const int g = pts[(size_t)r * NGLL2 + p];
+  if (g < 0) return;
+  const int i = p % NGLL, j = p / NGLL;
+  const float h = xir[i * nrec + r] * gammar[j * nrec + r];
+  atomic_add(&pdd[g], h * adj[(size_t)it * nrec + r]);
+}
+
+// fluid kernels (compute_kernels.f90:293-449, NTSTEP_BETWEEN_COMPUTE_KERNELS = 1): per point of a fluid element the
+// gradients of the adjoint and of the reconstructed potential (xi/gamma derivatives in single precision, the
+// jacobian products in double), rho_kl += 1/rho (grad pot . grad b_pot) dt, kappa_kl += 1/kappa pdd b_pot dt.
+// One block of 256 threads takes 10 elements (rows of the fluid elements, 25 points each); products and sums are
+// rounded one at a time as the CPU path forms them.
+constexpr int KL_EPB = 10;
+__global__ __launch_bounds__(256) void k_ac_kernels(int nrows, const int* __restrict__ ids,
+                                                    const float4* __restrict__ geom, const float2* __restrict__ rk,
+                                                    const Basis basis, const float* __restrict__ pot,
+                                                    const float* __restrict__ pdd, const float* __restrict__ bpot,
+                                                    float dt, float* __restrict__ rho_kl,
+                                                    float* __restrict__ kappa_kl) {
+  __shared__ float s_p[KL_EPB][NGLL2], s_b[KL_EPB][NGLL2], s_h[NGLL2];
+  const int t = threadIdx.x, le = t / NGLL2, p = t - le * NGLL2;
+  const long r = (long)blockIdx.x * KL_EPB + le;
+  const bool act = le < KL_EPB && r < nrows;
+  if (t < NGLL2) s_h[t] = basis.hp[t];
+  int g = 0;
+  if (act) {
+    g = ids[r * NGLL2 + p];
+    s_p[le][p] = pot[g];
+    s_b[le][p] = bpot[g];
+  }
+  __syncthreads();
+  if (!act) return;
+  const int i = p % NGLL, j = p / NGLL;
+  float t1 = 0.f, t2 = 0.f, u1 = 0.f, u2 = 0.f;
+#pragma unroll
+  for (int k = 0; k < NGLL; k++) {
+    t1 = __fadd_rn(t1, __fmul_rn(s_p[le][j * NGLL + k], s_h[i * NGLL + k]));
+    u1 = __fadd_rn(u1, __fmul_rn(s_b[le][j * NGLL + k], s_h[i * NGLL + k]));
+    t2 = __fadd_rn(t2, __fmul_rn(s_p[le][k * NGLL + i], s_h[j * NGLL + k]));
+    u2 = __fadd_rn(u2, __fmul_rn(s_b[le][k * NGLL + i], s_h[j * NGLL + k]));
+  }
+  const float4 G = geom[r * NGLL2 + p];  // {xi_x, xi_z, gamma_x, gamma_z}
+  const float a1 = (float)__dadd_rn(__dmul_rn(t1, G.x), __dmul_rn(t2, G.z));
+  const float a2 = (float)__dadd_rn(__dmul_rn(t1, G.y), __dmul_rn(t2, G.w));
+  const float b1 = (float)__dadd_rn(__dmul_rn(u1, G.x), __dmul_rn(u2, G.z));
+  const float b2 = (float)__dadd_rn(__dmul_rn(u1, G.y), __dmul_rn(u2, G.w));
+  const float dot = __fadd_rn(__fmul_rn(a1, b1), __fmul_rn(a2, b2));
+  const float2 m = rk[r * NGLL2 + p];  // {rho, kappa}
+  const long l = r * NGLL2 + p;
+  rho_kl[l] = __fadd_rn(rho_kl[l], __fmul_rn(__fmul_rn(1.f / m.x, dot), dt));
+  kappa_kl[l] = __fadd_rn(kappa_kl[l], __fmul_rn(__fmul_rn(__fmul_rn(1.f / m.y, pdd[g]), s_b[le][p]), dt));
 }
 
 // pressure (sign = -1: -potential or -potential_dot_dot) or potential (sign = +1) seismograms of the receivers in
@@ -431,6 +510,12 @@ struct AcousticState {
   std::vector<float> coup_n, coup_w;      // [2] normal out of the fluid, weight
   std::vector<int> st_e, st_ij;           // fluid Stacey points
   std::vector<float> st_w, st_rvp;
+  std::vector<int> st_slot;               // slot of each Stacey point in the edge storage (below)
+  // SAVE_FORWARD storage of the Stacey terms, the four sides' slices one after the other (bottom, right, top, left;
+  // a side's slice is Fortran (NGLLX, nspec_side): slot ib * 5 + k)
+  int abs_count[5] = {0, 0, 0, 0, 0}, abs_base[5] = {0, 0, 0, 0, 0}, abs_total = 0;
+  int* d_st_slot = nullptr;
+  float* d_absorb = nullptr;
   // device
   float *d_pot = nullptr, *d_pd = nullptr, *d_pdd = nullptr, *d_rm = nullptr;
   int ncolour[2] = {0, 0};
@@ -473,6 +558,13 @@ struct AcousticState {
   int* d_rec_pts = nullptr;  // [nrec][25], -1 rows for receivers in solid elements
   float4* d_rec_geom = nullptr;  // [nrec][25] {xi_x, xi_z, gamma_x, gamma_z} of the receiver's element (fluid rows)
   float* d_rec_rho = nullptr;    // [nrec][25] density (fluid rows)
+  // SIMULATION_TYPE = 3: the reconstructed potential and the fluid kernels, per fluid element row [row][25]
+  // (rows in element order): point ids, geometry, {rho, kappa}, rho_ac_kl, kappa_ac_kl
+  float *d_bpot = nullptr, *d_bpd = nullptr, *d_bpdd = nullptr;
+  int* d_kl_ids = nullptr;
+  float4* d_kl_geom = nullptr;
+  float2* d_kl_rk = nullptr;
+  float *d_rho_kl = nullptr, *d_kappa_kl = nullptr;
 };
 
 void acoustic_release(Mesh* mp) {
@@ -485,15 +577,26 @@ void acoustic_release(Mesh* mp) {
   device_free(ac->d_st); device_free(ac->d_st_w); device_free(ac->d_st_rvp);
   device_free(ac->d_src); device_free(ac->d_src_pts); device_free(ac->d_src_sa); device_free(ac->d_src_kap);
   device_free(ac->d_rec_pts); device_free(ac->d_rec_geom); device_free(ac->d_rec_rho);
+  device_free(ac->d_st_slot); device_free(ac->d_absorb);
+  device_free(ac->d_bpot); device_free(ac->d_bpd); device_free(ac->d_bpdd);
+  device_free(ac->d_kl_ids); device_free(ac->d_kl_geom); device_free(ac->d_kl_rk);
+  device_free(ac->d_rho_kl); device_free(ac->d_kappa_kl);
   delete ac;
   mp->ac = nullptr;
 }
 
 // the fluid's Stacey points, recorded by prepare_stacey_device_ (ispec 0-based caller order; ij 0-based with -1 for
 // points the edge does not own; rho_vp caller order [nspec][25])
-void acoustic_note_stacey(Mesh* mp, int nf, const int* ispec, const int* ij, const float* jacw, const float* rho_vp) {
+void acoustic_note_stacey(Mesh* mp, int nf, const int* ispec, const int* ij, const float* jacw, const float* rho_vp,
+                          const int* edge, const int* ib, const int* counts) {
   AcousticState* ac = mp->ac;
   if (!ac) fail("prepare_stacey_device: fluid elements without prepare_fields_acoustic_device");
+  ac->abs_total = 0;
+  for (int side = 1; side <= 4; side++) {
+    ac->abs_count[side] = counts[side];
+    ac->abs_base[side] = ac->abs_total;
+    ac->abs_total += NGLL * counts[side];
+  }
   for (int f = 0; f < nf; f++) {
     const int e = ispec[f];
     if (ac->row_of_ispec[e] < 0) continue;
@@ -504,6 +607,7 @@ void acoustic_note_stacey(Mesh* mp, int nf, const int* ispec, const int* ij, con
       ac->st_ij.push_back(i + NGLL * j);
       ac->st_w.push_back(jacw[(size_t)f * NGLL + k]);
       ac->st_rvp.push_back(rho_vp[(size_t)e * NGLL2 + j * NGLL + i]);
+      ac->st_slot.push_back(ac->abs_base[edge[f]] + ib[f] * NGLL + k);
     }
   }
 }
@@ -726,7 +830,32 @@ AcousticState* ac_ready(Mesh* mp, const char* who) {
       ac->d_st = device_copy(pts.data(), pts.size());
       ac->d_st_w = device_copy(ac->st_w.data(), ac->st_w.size());
       ac->d_st_rvp = device_copy(ac->st_rvp.data(), ac->st_rvp.size());
+      ac->d_st_slot = device_copy(ac->st_slot.data(), ac->st_slot.size());
     }
+    if (ac->abs_total && (mp->simulation_type == 3 || mp->save_forward))
+      ac->d_absorb = device_zeros<float>((size_t)ac->abs_total);
+  }
+  if (mp->simulation_type == 3) {  // fluid kernels: per fluid element row, element order
+    std::vector<int> rows;
+    for (int e = 0; e < mp->nspec; e++)
+      if (ac->row_of_ispec[e] >= 0) rows.push_back(e);
+    std::vector<float> xix = download_rows(mp, mp->d_xix, rows), xiz = download_rows(mp, mp->d_xiz, rows);
+    std::vector<float> gax = download_rows(mp, mp->d_gammax, rows), gaz = download_rows(mp, mp->d_gammaz, rows);
+    std::vector<int> ids(rows.size() * NGLL2);
+    std::vector<float4> geom(ids.size());
+    std::vector<float2> rk(ids.size());
+    for (size_t q = 0; q < rows.size(); q++)
+      for (int p = 0; p < NGLL2; p++) {
+        const size_t l = q * NGLL2 + p;
+        ids[l] = iv.id(rows[q], p);
+        geom[l] = make_float4(xix[l], xiz[l], gax[l], gaz[l]);
+        rk[l] = make_float2(ac->rho_rows[l], ac->kappa_rows[l]);
+      }
+    ac->d_kl_ids = device_copy(ids.data(), ids.size());
+    ac->d_kl_geom = device_copy(geom.data(), geom.size());
+    ac->d_kl_rk = device_copy(rk.data(), rk.size());
+    ac->d_rho_kl = device_zeros<float>(ids.size());
+    ac->d_kappa_kl = device_zeros<float>(ids.size());
   }
   if (mp->nsources_local > 0) {
     std::vector<float> sa((size_t)mp->nsources_local * NGLL2 * 2);
@@ -784,9 +913,12 @@ AcousticState* ac_ready(Mesh* mp, const char* who) {
   return ac;
 }
 
-void require_forward(int cw2, int fwd_or_adj, const char* who) {
-  if (cw2) fail("%s: the backward fluid wavefield is not supported by this library", who);
-  if (fwd_or_adj != 1) fail("%s: adjoint fluid wavefields are not supported by this library", who);
+// the reconstructed (backward) potential exists only in SIMULATION_TYPE = 3 runs
+void require_backward(const Mesh* mp, const AcousticState* ac, const char* who) {
+  if (!ac->d_bpot)
+    fail("%s: backward fluid wavefield requested, but the run is not SIMULATION_TYPE = 3 or "
+         "prepare_fields_acoustic_adj_dev was not called", who);
+  if (mp->pml) fail("%s: backward fluid wavefields with CPML are not supported by this library", who);
 }
 
 }  // namespace
@@ -931,18 +1063,28 @@ void prepare_fields_acoustic_device_(long* Mesh_pointer, realw* rmass_acoustic, 
 
 void prepare_fields_acoustic_adj_dev_(long* Mesh_pointer, int* APPROXIMATE_HESS_KL, int* ATTENUATION_VISCOACOUSTIC,
                                       int* NO_BACKWARD_RECONSTRUCTION) {  // prepare_mesh_constants_cuda.cu:442
-  (void)APPROXIMATE_HESS_KL; (void)ATTENUATION_VISCOACOUSTIC; (void)NO_BACKWARD_RECONSTRUCTION;
   Mesh* mp = handle(Mesh_pointer);
-  if (mp->simulation_type == 3) fail("adjoint simulations of fluid elements are not supported by this library");
+  AcousticState* ac = mp->ac;
+  if (!ac) fail("prepare_fields_acoustic_adj_dev: prepare_fields_acoustic_device was not called");
+  if (*APPROXIMATE_HESS_KL) fail("approximate Hessian kernels are not supported by this library");
+  if (*ATTENUATION_VISCOACOUSTIC) fail("viscoacoustic attenuation is not supported by this library yet");
+  if (*NO_BACKWARD_RECONSTRUCTION) fail("NO_BACKWARD_RECONSTRUCTION is not supported by this library");
+  if (mp->simulation_type != 3) return;
+  if (mp->pml) fail("adjoint runs of fluid elements with CPML are not supported by this library");
+  if (ac->d_bpot) return;
+  ac->d_bpot = device_zeros<float>((size_t)mp->nglob);
+  ac->d_bpd = device_zeros<float>((size_t)mp->nglob);
+  ac->d_bpdd = device_zeros<float>((size_t)mp->nglob);
 }
 
 void update_displacement_ac_cuda_(long* Mesh_pointer, realw* deltat_F, realw* deltatsqover2_F, realw* deltatover2_F,
                                   realw* b_deltat_F, realw* b_deltatsqover2_F, realw* b_deltatover2_F,
                                   int* compute_b_wavefield, int* UNDO_ATTENUATION_AND_OR_PML) {  // update_displacement_cuda.cu:118
-  (void)b_deltat_F; (void)b_deltatsqover2_F; (void)b_deltatover2_F; (void)UNDO_ATTENUATION_AND_OR_PML;
+  // the Fortran passes compute_b_wavefield = .true. unless NO_BACKWARD_RECONSTRUCTION, also in forward runs: the
+  // reconstructed potential moves only in SIMULATION_TYPE = 3 runs (update_displacement_cuda.cu:172-188)
   Mesh* mp = handle(Mesh_pointer);
-  if (*compute_b_wavefield) fail("update_displacement_ac_cuda: the backward fluid wavefield is not supported");
   AcousticState* ac = ac_ready(mp, "update_displacement_ac_cuda");
+  if (*UNDO_ATTENUATION_AND_OR_PML) fail("UNDO_ATTENUATION_AND_OR_PML is not supported by this library");
   // CPML: the previous-step potential of the layer's points, before the predictor
   pml_acoustic_before_predictor(mp, ac->d_pot, ac->d_pdd, *deltatsqover2_F);
   const long m = (long)ac->nrange + ac->nlist;
@@ -951,6 +1093,13 @@ void update_displacement_ac_cuda_(long* Mesh_pointer, realw* deltat_F, realw* de
                      ac->d_pdd, ac->r0, ac->nrange, ac->d_list, ac->nlist, *deltat_F, *deltatsqover2_F,
                      *deltatover2_F);
   check_hip(hipGetLastError(), "k_ac_predictor launch");
+  if (mp->simulation_type == 3 && *compute_b_wavefield) {
+    require_backward(mp, ac, "update_displacement_ac_cuda");
+    hipLaunchKernelGGL(k_ac_predictor, dim3(grid_for(m, 256, 65536)), dim3(256), 0, mp->stream, ac->d_bpot,
+                       ac->d_bpd, ac->d_bpdd, ac->r0, ac->nrange, ac->d_list, ac->nlist, *b_deltat_F,
+                       *b_deltatsqover2_F, *b_deltatover2_F);
+    check_hip(hipGetLastError(), "k_ac_predictor launch");
+  }
 }
 
 void compute_forces_acoustic_cuda_(long* Mesh_pointer, int* iphase, int* nspec_outer_acoustic,
@@ -959,40 +1108,43 @@ void compute_forces_acoustic_cuda_(long* Mesh_pointer, int* iphase, int* nspec_o
   (void)nspec_outer_acoustic; (void)nspec_inner_acoustic;
   Mesh* mp = handle(Mesh_pointer);
   if (*ATTENUATION_VISCOACOUSTIC) fail("viscoacoustic attenuation is not supported by this library yet");
-  require_forward(*compute_wavefield_2, 1, "compute_forces_acoustic_cuda");
-  if (!*compute_wavefield_1) return;
   AcousticState* ac = ac_ready(mp, "compute_forces_acoustic_cuda");
   const int ph = *iphase - 1;
   if (ph < 0 || ph > 1) fail("compute_forces_acoustic_cuda: iphase %d", *iphase);
-  if (ac->chain) {  // chains of fluid elements, one wave per chain
-    for (int c = 0; c < ac->cn_colour[ph]; c++) {
-      const int n = ac->cc_count[ph][c];
-      if (n == 0) continue;
-      hipLaunchKernelGGL(ac->rect ? k_acoustic_forces_col<true> : k_acoustic_forces_col<false>,
-                         dim3((n + AC_NXCD - 1) / AC_NXCD * AC_NXCD), dim3(64), 0, mp->stream, ac->cc_start[ph][c], n,
-                         ac->d_cdesc, ac->d_cids, ac->d_cfac, ac->d_cgeom, ac->d_crect, mp->basis, ac->d_pot,
-                         ac->d_pdd);
-      check_hip(hipGetLastError(), "k_acoustic_forces_col launch");
+  if (*compute_wavefield_2) require_backward(mp, ac, "compute_forces_acoustic_cuda");
+  for (int w = 0; w < 2; w++) {
+    if (!(w == 0 ? *compute_wavefield_1 : *compute_wavefield_2)) continue;
+    const float* pot = w == 0 ? ac->d_pot : ac->d_bpot;
+    float* pdd = w == 0 ? ac->d_pdd : ac->d_bpdd;
+    if (ac->chain) {  // chains of fluid elements, one wave per chain
+      for (int c = 0; c < ac->cn_colour[ph]; c++) {
+        const int n = ac->cc_count[ph][c];
+        if (n == 0) continue;
+        hipLaunchKernelGGL(ac->rect ? k_acoustic_forces_col<true> : k_acoustic_forces_col<false>,
+                           dim3((n + AC_NXCD - 1) / AC_NXCD * AC_NXCD), dim3(64), 0, mp->stream, ac->cc_start[ph][c],
+                           n, ac->d_cdesc, ac->d_cids, ac->d_cfac, ac->d_cgeom, ac->d_crect, mp->basis, pot, pdd);
+        check_hip(hipGetLastError(), "k_acoustic_forces_col launch");
+      }
+    } else {
+      for (int c = 0; c < ac->ncolour[ph]; c++) {
+        const int n = ac->colour_count[ph][c];
+        if (n == 0) continue;
+        hipLaunchKernelGGL(ac->rect ? k_acoustic_forces<true> : k_acoustic_forces<false>,
+                           dim3((n + AC_EPB - 1) / AC_EPB), dim3(256), 0, mp->stream, ac->colour_start[ph][c], n,
+                           ac->d_ids, ac->d_geom, ac->d_geom_rect, ac->d_fac, mp->basis, pot, pdd);
+        check_hip(hipGetLastError(), "k_acoustic_forces launch");
+      }
     }
-    pml_acoustic_forces(mp, ph, ac->d_pot, ac->d_pd, ac->d_pdd);
-    return;
+    // the CPML layer's elements of this phase (forward field only: backward fields refuse CPML)
+    if (w == 0) pml_acoustic_forces(mp, ph, ac->d_pot, ac->d_pd, ac->d_pdd);
   }
-  for (int c = 0; c < ac->ncolour[ph]; c++) {
-    const int n = ac->colour_count[ph][c];
-    if (n == 0) continue;
-    hipLaunchKernelGGL(ac->rect ? k_acoustic_forces<true> : k_acoustic_forces<false>,
-                       dim3((n + AC_EPB - 1) / AC_EPB), dim3(256), 0, mp->stream, ac->colour_start[ph][c], n,
-                       ac->d_ids, ac->d_geom, ac->d_geom_rect, ac->d_fac, mp->basis, ac->d_pot, ac->d_pdd);
-    check_hip(hipGetLastError(), "k_acoustic_forces launch");
-  }
-  pml_acoustic_forces(mp, ph, ac->d_pot, ac->d_pd, ac->d_pdd);  // the CPML layer's elements of this phase
 }
 
 // pml_compute_cuda.cu:47: potential, its time derivatives and the previous-step potential are zero on the outer
 // boundary of the layer's fluid elements
 void pml_boundary_acoustic_cuda_(long* Mesh_pointer, int* compute_wavefield_1, int* compute_wavefield_2) {
   Mesh* mp = handle(Mesh_pointer);
-  require_forward(*compute_wavefield_2, 1, "pml_boundary_acoustic_cuda");
+  if (*compute_wavefield_2) fail("pml_boundary_acoustic_cuda: backward fluid wavefields with CPML are not supported");
   if (!*compute_wavefield_1) return;
   AcousticState* ac = ac_ready(mp, "pml_boundary_acoustic_cuda");
   pml_acoustic_boundary(mp, ac->d_pot, ac->d_pd, ac->d_pdd);
@@ -1002,31 +1154,59 @@ void compute_stacey_acoustic_cuda_(long* Mesh_pointer, int* iphasef, realw* h_b_
                                    realw* h_b_absorb_potential_right, realw* h_b_absorb_potential_top,
                                    realw* h_b_absorb_potential_bottom, int* compute_wavefield_1,
                                    int* compute_wavefield_2, int* UNDO_ATTENUATION_AND_OR_PML) {  // compute_stacey_acoustic_cuda.cu:40
-  (void)h_b_absorb_potential_left; (void)h_b_absorb_potential_right; (void)h_b_absorb_potential_top;
-  (void)h_b_absorb_potential_bottom; (void)UNDO_ATTENUATION_AND_OR_PML;
   Mesh* mp = handle(Mesh_pointer);
   if (*iphasef != 1) return;
-  require_forward(*compute_wavefield_2, 1, "compute_stacey_acoustic_cuda");
-  if (mp->save_forward || mp->simulation_type != 1)
-    fail("compute_stacey_acoustic_cuda: storage of the fluid's Stacey terms (SAVE_FORWARD) is not supported");
-  if (!*compute_wavefield_1) return;
+  if (*UNDO_ATTENUATION_AND_OR_PML) fail("UNDO_ATTENUATION_AND_OR_PML is not supported by this library");
   AcousticState* ac = ac_ready(mp, "compute_stacey_acoustic_cuda");
-  if (ac->nst == 0) return;
-  hipLaunchKernelGGL(k_ac_stacey, dim3((ac->nst + 255) / 256), dim3(256), 0, mp->stream, ac->nst, ac->d_st,
-                     ac->d_st_w, ac->d_st_rvp, ac->d_pd, ac->d_pdd);
-  check_hip(hipGetLastError(), "k_ac_stacey launch");
+  // the caller's slices of one step, Fortran (NGLLX, nspec_side) each
+  realw* host[5] = {nullptr, h_b_absorb_potential_bottom, h_b_absorb_potential_right, h_b_absorb_potential_top,
+                    h_b_absorb_potential_left};
+  const int write_abs = mp->simulation_type == 1 && mp->save_forward && ac->d_absorb;
+  if (*compute_wavefield_1 && ac->nst > 0) {
+    hipLaunchKernelGGL(k_ac_stacey, dim3((ac->nst + 255) / 256), dim3(256), 0, mp->stream, ac->nst, ac->d_st,
+                       ac->d_st_w, ac->d_st_rvp, ac->d_pd, ac->d_pdd, ac->d_st_slot,
+                       write_abs ? ac->d_absorb : nullptr);
+    check_hip(hipGetLastError(), "k_ac_stacey launch");
+  }
+  if (write_abs) {
+    for (int side = 1; side <= 4; side++)
+      if (ac->abs_count[side] > 0)
+        check_hip(hipMemcpyAsync(host[side], ac->d_absorb + ac->abs_base[side],
+                                 sizeof(float) * NGLL * ac->abs_count[side], hipMemcpyDeviceToHost, mp->stream),
+                  "b_absorb_potential D2H");
+    check_hip(hipStreamSynchronize(mp->stream), "compute_stacey_acoustic_cuda");
+  }
+  if (*compute_wavefield_2) {
+    require_backward(mp, ac, "compute_stacey_acoustic_cuda");
+    if (ac->nst == 0) return;
+    for (int side = 1; side <= 4; side++)
+      if (ac->abs_count[side] > 0)
+        check_hip(hipMemcpyAsync(ac->d_absorb + ac->abs_base[side], host[side],
+                                 sizeof(float) * NGLL * ac->abs_count[side], hipMemcpyHostToDevice, mp->stream),
+                  "b_absorb_potential H2D");
+    hipLaunchKernelGGL(k_ac_stacey_backward, dim3((ac->nst + 255) / 256), dim3(256), 0, mp->stream, ac->nst,
+                       ac->d_st, ac->d_st_slot, ac->d_absorb, ac->d_bpdd);
+    check_hip(hipGetLastError(), "k_ac_stacey_backward launch");
+    check_hip(hipStreamSynchronize(mp->stream), "compute_stacey_acoustic_cuda");  // the host slice is reused
+  }
 }
 
 void acoustic_enforce_free_surf_cuda_(long* Mesh_pointer, int* compute_wavefield_1,
                                       int* compute_wavefield_2) {  // enforce_acoustic_free_surface_cuda.cu:46
   Mesh* mp = handle(Mesh_pointer);
-  require_forward(*compute_wavefield_2, 1, "acoustic_enforce_free_surf_cuda");
-  if (!*compute_wavefield_1) return;
   AcousticState* ac = ac_ready(mp, "acoustic_enforce_free_surf_cuda");
+  if (*compute_wavefield_2) require_backward(mp, ac, "acoustic_enforce_free_surf_cuda");
   if (ac->nfree == 0) return;
-  hipLaunchKernelGGL(k_ac_free_surface, dim3((ac->nfree + 255) / 256), dim3(256), 0, mp->stream, ac->d_free,
-                     ac->nfree, ac->d_pot, ac->d_pd, ac->d_pdd);
-  check_hip(hipGetLastError(), "k_ac_free_surface launch");
+  if (*compute_wavefield_1) {
+    hipLaunchKernelGGL(k_ac_free_surface, dim3((ac->nfree + 255) / 256), dim3(256), 0, mp->stream, ac->d_free,
+                       ac->nfree, ac->d_pot, ac->d_pd, ac->d_pdd);
+    check_hip(hipGetLastError(), "k_ac_free_surface launch");
+  }
+  if (*compute_wavefield_2) {
+    hipLaunchKernelGGL(k_ac_free_surface, dim3((ac->nfree + 255) / 256), dim3(256), 0, mp->stream, ac->d_free,
+                       ac->nfree, ac->d_bpot, ac->d_bpd, ac->d_bpdd);
+    check_hip(hipGetLastError(), "k_ac_free_surface launch");
+  }
 }
 
 void compute_add_sources_ac_cuda_(long* Mesh_pointer, int* iphasef, int* itf) {  // compute_add_sources_viscoacoustic_cuda.cu:46
@@ -1041,47 +1221,144 @@ void compute_add_sources_ac_cuda_(long* Mesh_pointer, int* iphasef, int* itf) { 
   check_hip(hipGetLastError(), "k_ac_sources launch");
 }
 
+// FORWARD_OR_ADJOINT 1: the forward field (SIMULATION_TYPE 1) or the adjoint field (SIMULATION_TYPE 3, coupled
+// through minus the adjoint acceleration); 3: the reconstructed field, coupled through the reconstructed displacement
+// the reconstructed field's pressure sources at step it (compute_add_sources_acoustic(b_pdd, NSTEP - it + 1))
+void compute_add_sources_ac_s3_cuda_(long* Mesh_pointer, int* iphasef, int* itf) {  // compute_add_sources_viscoacoustic_cuda.cu:91
+  Mesh* mp = handle(Mesh_pointer);
+  if (*iphasef != 1) return;
+  AcousticState* ac = ac_ready(mp, "compute_add_sources_ac_s3_cuda");
+  require_backward(mp, ac, "compute_add_sources_ac_s3_cuda");
+  if (ac->nas == 0) return;
+  const int it = *itf - 1;
+  if (it < 0 || it >= mp->nstep) fail("compute_add_sources_ac_s3_cuda: it %d outside 1..%d", *itf, mp->nstep);
+  hipLaunchKernelGGL(k_ac_sources, dim3(ac->nas), dim3(64), 0, mp->stream, ac->nas, it, mp->nsources_local,
+                     ac->d_src, ac->d_src_pts, ac->d_src_sa, ac->d_src_kap, mp->d_stf, ac->d_bpdd);
+  check_hip(hipGetLastError(), "k_ac_sources launch");
+}
+
+// adjoint sources of the receivers in fluid elements at step it (= NSTEP - it_forward + 1), into the adjoint field
+void add_sources_ac_sim_2_or_3_cuda_(long* Mesh_pointer, int* iphasef, int* itf, int* nadj_rec_local,
+                                     int* NSTEP) {  // compute_add_sources_viscoacoustic_cuda.cu:208
+  Mesh* mp = handle(Mesh_pointer);
+  if (*iphasef != 1) return;
+  AcousticState* ac = ac_ready(mp, "add_sources_ac_sim_2_or_3_cuda");
+  if (*nadj_rec_local != mp->nadj_rec_local) fail("add_sources_ac_sim_2_or_3_cuda: nadj_rec_local not equal");
+  if (mp->nadj_rec_local == 0 || ac->nrec_fluid == 0) return;
+  if (*NSTEP != mp->nstep) fail("add_sources_ac_sim_2_or_3_cuda: NSTEP mismatch");
+  if (*itf < 1 || *itf > mp->nstep) fail("add_sources_ac_sim_2_or_3_cuda: it=%d outside 1..%d", *itf, mp->nstep);
+  hipLaunchKernelGGL(k_ac_adjoint_sources, dim3(mp->nadj_rec_local), dim3(64), 0, mp->stream, mp->nadj_rec_local,
+                     *itf - 1, mp->nstep, ac->d_rec_pts, mp->d_xir, mp->d_gammar, mp->d_source_adjoint, ac->d_pdd);
+  check_hip(hipGetLastError(), "k_ac_adjoint_sources launch");
+}
+
+// rho_ac_kl, kappa_ac_kl += this step's increments (with deltat; the caller multiplies by
+// NTSTEP_BETWEEN_COMPUTE_KERNELS, iterate_time.F90:489-497)
+void compute_kernels_acoustic_cuda_(long* Mesh_pointer, realw* deltat) {  // compute_kernels_cuda.cu:92
+  Mesh* mp = handle(Mesh_pointer);
+  AcousticState* ac = ac_ready(mp, "compute_kernels_acoustic_cuda");
+  require_backward(mp, ac, "compute_kernels_acoustic_cuda");
+  const int nrows = (int)ac->rho_rows.size() / NGLL2;
+  if (nrows == 0) return;
+  hipLaunchKernelGGL(k_ac_kernels, dim3((nrows + KL_EPB - 1) / KL_EPB), dim3(256), 0, mp->stream, nrows,
+                     ac->d_kl_ids, ac->d_kl_geom, ac->d_kl_rk, mp->basis, ac->d_pot, ac->d_pdd, ac->d_bpot, *deltat,
+                     ac->d_rho_kl, ac->d_kappa_kl);
+  check_hip(hipGetLastError(), "k_ac_kernels launch");
+}
+
+void transfer_kernels_ac_to_host_(long* Mesh_pointer, realw* h_rho_ac_kl, realw* h_kappa_ac_kl,
+                                  int* NSPEC_AB) {  // transfer_fields_cuda.cu:391
+  Mesh* mp = handle(Mesh_pointer);
+  if (*NSPEC_AB != mp->nspec) fail("transfer_kernels_ac_to_host: NSPEC_AB %d != nspec %d", *NSPEC_AB, mp->nspec);
+  AcousticState* ac = ac_ready(mp, "transfer_kernels_ac_to_host");
+  require_backward(mp, ac, "transfer_kernels_ac_to_host");
+  std::vector<float> r(ac->rho_rows.size()), k(ac->rho_rows.size());
+  check_hip(hipStreamSynchronize(mp->stream), "transfer_kernels_ac_to_host");
+  check_hip(hipMemcpy(r.data(), ac->d_rho_kl, r.size() * sizeof(float), hipMemcpyDeviceToHost), "rho_ac_kl D2H");
+  check_hip(hipMemcpy(k.data(), ac->d_kappa_kl, k.size() * sizeof(float), hipMemcpyDeviceToHost), "kappa_ac_kl D2H");
+  for (int e = 0; e < mp->nspec; e++) {
+    const int row = ac->row_of_ispec[e];
+    for (int p = 0; p < NGLL2; p++) {
+      h_rho_ac_kl[(size_t)e * NGLL2 + p] = row < 0 ? 0.f : r[(size_t)row * NGLL2 + p];
+      h_kappa_ac_kl[(size_t)e * NGLL2 + p] = row < 0 ? 0.f : k[(size_t)row * NGLL2 + p];
+    }
+  }
+}
+
 void compute_coupling_ac_el_cuda_(long* Mesh_pointer, int* iphasef, int* num_coupling_ac_el_facesf,
                                   int* FORWARD_OR_ADJOINT) {  // compute_coupling_cuda.cu:47
   Mesh* mp = handle(Mesh_pointer);
   if (*iphasef != 1) return;
-  require_forward(0, *FORWARD_OR_ADJOINT, "compute_coupling_ac_el_cuda");
+  const int which = *FORWARD_OR_ADJOINT;
+  if (which != 1 && which != 3) fail("compute_coupling_ac_el_cuda: FORWARD_OR_ADJOINT = %d", which);
   AcousticState* ac = ac_ready(mp, "compute_coupling_ac_el_cuda");
   if (*num_coupling_ac_el_facesf * NGLL != ac->ncoup)
     fail("compute_coupling_ac_el_cuda: %d faces, %d prepared points", *num_coupling_ac_el_facesf, ac->ncoup);
   if (ac->ncoup == 0) return;
   if (!mp->d_displ) fail("compute_coupling_ac_el_cuda: the solid fields are not prepared");
+  const float* u = mp->d_displ;
+  float sign = 1.f;
+  float* pdd = ac->d_pdd;
+  if (which == 3) {
+    require_backward(mp, ac, "compute_coupling_ac_el_cuda");
+    u = mp->d_b_displ;
+    pdd = ac->d_bpdd;
+  } else if (mp->simulation_type == 3) {
+    flush_correctors(mp);  // the adjoint acceleration after its mass and corrector
+    u = mp->d_accel;
+    sign = -1.f;
+  }
   hipLaunchKernelGGL(k_coupling_ac_el, dim3((ac->ncoup + 255) / 256), dim3(256), 0, mp->stream, ac->ncoup,
-                     ac->d_coup, ac->d_coup_n, ac->d_coup_w, reinterpret_cast<const float2*>(mp->d_displ), ac->d_pdd);
+                     ac->d_coup, ac->d_coup_n, ac->d_coup_w, reinterpret_cast<const float2*>(u), sign, pdd);
   check_hip(hipGetLastError(), "k_coupling_ac_el launch");
 }
 
+// FORWARD_OR_ADJOINT 1: forward field (pressure -pdd) or, in SIMULATION_TYPE 3, the adjoint field (pressure = the
+// adjoint potential); 3: the reconstructed field
 void compute_coupling_el_ac_cuda_(long* Mesh_pointer, int* iphasef, int* num_coupling_ac_el_facesf,
                                   int* FORWARD_OR_ADJOINT) {  // compute_coupling_cuda.cu:125
   Mesh* mp = handle(Mesh_pointer);
   if (*iphasef != 1) return;
-  require_forward(0, *FORWARD_OR_ADJOINT, "compute_coupling_el_ac_cuda");
+  const int which = *FORWARD_OR_ADJOINT;
+  if (which != 1 && which != 3) fail("compute_coupling_el_ac_cuda: FORWARD_OR_ADJOINT = %d", which);
   AcousticState* ac = ac_ready(mp, "compute_coupling_el_ac_cuda");
   if (*num_coupling_ac_el_facesf * NGLL != ac->ncoup)
     fail("compute_coupling_el_ac_cuda: %d faces, %d prepared points", *num_coupling_ac_el_facesf, ac->ncoup);
   if (ac->ncoup == 0) return;
+  const float* p = ac->d_pdd;
+  float sign = 1.f;
+  float* acc = mp->d_accel;
+  if (which == 3) {
+    require_backward(mp, ac, "compute_coupling_el_ac_cuda");
+    if (!mp->d_b_accel) fail("compute_coupling_el_ac_cuda: the backward solid field is not prepared");
+    p = ac->d_bpdd;
+    acc = mp->d_b_accel;
+  } else if (mp->simulation_type == 3) {
+    p = ac->d_pot;
+    sign = -1.f;
+  }
   hipLaunchKernelGGL(k_coupling_el_ac, dim3((ac->ncoup + 255) / 256), dim3(256), 0, mp->stream, ac->ncoup,
-                     ac->d_coup, ac->d_coup_n, ac->d_coup_w, ac->d_pdd, mp->d_accel);
+                     ac->d_coup, ac->d_coup_n, ac->d_coup_w, p, sign, acc);
   check_hip(hipGetLastError(), "k_coupling_el_ac launch");
 }
 
 void kernel_3_acoustic_cuda_(long* Mesh_pointer, realw* deltatover2, realw* b_deltatover2, int* compute_wavefield_1,
                              int* compute_wavefield_2) {  // update_displacement_cuda.cu:243
-  (void)b_deltatover2;
   Mesh* mp = handle(Mesh_pointer);
-  require_forward(*compute_wavefield_2, 1, "kernel_3_acoustic_cuda");
-  if (!*compute_wavefield_1) return;
   AcousticState* ac = ac_ready(mp, "kernel_3_acoustic_cuda");
+  if (*compute_wavefield_2) require_backward(mp, ac, "kernel_3_acoustic_cuda");
   const long m = (long)ac->nrange + ac->nlist;
   if (m == 0) return;
-  hipLaunchKernelGGL(k_ac_corrector, dim3(grid_for(m, 256, 65536)), dim3(256), 0, mp->stream, ac->d_pd, ac->d_pdd,
-                     ac->d_rm, ac->r0, ac->nrange, ac->d_list, ac->nlist, *deltatover2);
-  check_hip(hipGetLastError(), "k_ac_corrector launch");
+  if (*compute_wavefield_1) {
+    hipLaunchKernelGGL(k_ac_corrector, dim3(grid_for(m, 256, 65536)), dim3(256), 0, mp->stream, ac->d_pd, ac->d_pdd,
+                       ac->d_rm, ac->r0, ac->nrange, ac->d_list, ac->nlist, *deltatover2);
+    check_hip(hipGetLastError(), "k_ac_corrector launch");
+  }
+  if (*compute_wavefield_2) {
+    hipLaunchKernelGGL(k_ac_corrector, dim3(grid_for(m, 256, 65536)), dim3(256), 0, mp->stream, ac->d_bpd,
+                       ac->d_bpdd, ac->d_rm, ac->r0, ac->nrange, ac->d_list, ac->nlist, *b_deltatover2);
+    check_hip(hipGetLastError(), "k_ac_corrector launch");
+  }
 }
 
 void transfer_fields_ac_to_device_(int* size, realw* potential_acoustic, realw* potential_dot_acoustic,
@@ -1111,12 +1388,59 @@ void transfer_dot_dot_from_device_(int* size, realw* potential_dot_dot_acoustic,
   points_from_device(mp, potential_dot_dot_acoustic, ac->d_pdd, 1, "potential_dot_dot D2H");
 }
 
+void transfer_b_fields_ac_to_device_(int* size, realw* b_potential_acoustic, realw* b_potential_dot_acoustic,
+                                     realw* b_potential_dot_dot_acoustic, long* Mesh_pointer) {  // transfer_fields_cuda.cu:248
+  Mesh* mp = handle(Mesh_pointer);
+  if (*size != mp->nglob) fail("transfer_b_fields_ac_to_device: size %d != nglob %d", *size, mp->nglob);
+  AcousticState* ac = ac_ready(mp, "transfer_b_fields_ac_to_device");
+  require_backward(mp, ac, "transfer_b_fields_ac_to_device");
+  points_to_device(mp, ac->d_bpot, b_potential_acoustic, 1, "b_potential H2D");
+  points_to_device(mp, ac->d_bpd, b_potential_dot_acoustic, 1, "b_potential_dot H2D");
+  points_to_device(mp, ac->d_bpdd, b_potential_dot_dot_acoustic, 1, "b_potential_dot_dot H2D");
+}
+
+void transfer_b_fields_ac_from_device_(int* size, realw* b_potential_acoustic, realw* b_potential_dot_acoustic,
+                                       realw* b_potential_dot_dot_acoustic, long* Mesh_pointer) {  // transfer_fields_cuda.cu:300
+  Mesh* mp = handle(Mesh_pointer);
+  if (*size != mp->nglob) fail("transfer_b_fields_ac_from_device: size %d != nglob %d", *size, mp->nglob);
+  AcousticState* ac = ac_ready(mp, "transfer_b_fields_ac_from_device");
+  require_backward(mp, ac, "transfer_b_fields_ac_from_device");
+  points_from_device(mp, b_potential_acoustic, ac->d_bpot, 1, "b_potential D2H");
+  points_from_device(mp, b_potential_dot_acoustic, ac->d_bpd, 1, "b_potential_dot D2H");
+  points_from_device(mp, b_potential_dot_dot_acoustic, ac->d_bpdd, 1, "b_potential_dot_dot D2H");
+}
+
+void transfer_b_potential_ac_from_device_(int* size, realw* b_potential_acoustic, long* Mesh_pointer) {  // transfer_fields_cuda.cu:324
+  Mesh* mp = handle(Mesh_pointer);
+  if (*size != mp->nglob) fail("transfer_b_potential_ac_from_device: size %d != nglob %d", *size, mp->nglob);
+  AcousticState* ac = ac_ready(mp, "transfer_b_potential_ac_from_device");
+  require_backward(mp, ac, "transfer_b_potential_ac_from_device");
+  points_from_device(mp, b_potential_acoustic, ac->d_bpot, 1, "b_potential D2H");
+}
+
+void transfer_b_potential_ac_to_device_(int* size, realw* b_potential_acoustic, long* Mesh_pointer) {  // transfer_fields_cuda.cu:341
+  Mesh* mp = handle(Mesh_pointer);
+  if (*size != mp->nglob) fail("transfer_b_potential_ac_to_device: size %d != nglob %d", *size, mp->nglob);
+  AcousticState* ac = ac_ready(mp, "transfer_b_potential_ac_to_device");
+  require_backward(mp, ac, "transfer_b_potential_ac_to_device");
+  points_to_device(mp, ac->d_bpot, b_potential_acoustic, 1, "b_potential H2D");
+}
+
+void transfer_b_dot_dot_from_device_(int* size, realw* b_potential_dot_dot_acoustic, long* Mesh_pointer) {  // transfer_fields_cuda.cu:374
+  Mesh* mp = handle(Mesh_pointer);
+  if (*size != mp->nglob) fail("transfer_b_dot_dot_from_device: size %d != nglob %d", *size, mp->nglob);
+  AcousticState* ac = ac_ready(mp, "transfer_b_dot_dot_from_device");
+  require_backward(mp, ac, "transfer_b_dot_dot_from_device");
+  points_from_device(mp, b_potential_dot_dot_acoustic, ac->d_bpdd, 1, "b_potential_dot_dot D2H");
+}
+
 void get_norm_acoustic_from_device_(realw* norm, long* Mesh_pointer, const int* FORWARD_OR_ADJOINT) {  // check_fields_cuda.cu:240
   Mesh* mp = handle(Mesh_pointer);
-  require_forward(0, *FORWARD_OR_ADJOINT, "get_norm_acoustic_from_device");
   AcousticState* ac = ac_ready(mp, "get_norm_acoustic_from_device");
+  if (*FORWARD_OR_ADJOINT == 3) require_backward(mp, ac, "get_norm_acoustic_from_device");
+  const float* f = *FORWARD_OR_ADJOINT == 3 ? ac->d_bpot : ac->d_pot;
   check_hip(hipMemsetAsync(mp->d_scratch_u32, 0, sizeof(unsigned int), mp->stream), "memset");
-  hipLaunchKernelGGL(k_ac_max_abs, dim3(grid_for(mp->nglob, 256, 2048)), dim3(256), 0, mp->stream, ac->d_pot,
+  hipLaunchKernelGGL(k_ac_max_abs, dim3(grid_for(mp->nglob, 256, 2048)), dim3(256), 0, mp->stream, f,
                      mp->nglob, mp->d_scratch_u32);
   check_hip(hipGetLastError(), "k_ac_max_abs launch");
   unsigned int bits = 0;

@@ -937,11 +937,6 @@ void prepare_stacey_device_(long* Mesh_pointer, int* ACOUSTIC_SIMULATION, int* E
   // after marking, a skipped point has -1 in one coordinate: make both negative
   for (int k = 0; k < nf * NGLL; k++)
     if (ij[2 * k] < 0 || ij[2 * k + 1] < 0) ij[2 * k] = ij[2 * k + 1] = -1;
-  if (*ACOUSTIC_SIMULATION && mp->nspec_acoustic > 0) {
-    // the fluid's points keep the caller's element numbering until the first acoustic step resolves them
-    std::vector<int> isp0 = to_zero_based(h_abs_boundary_ispec, nf, 1, mp->nspec, "abs_boundary_ispec");
-    acoustic_note_stacey(mp, nf, isp0.data(), ij.data(), h_abs_boundary_jacobian1Dw, rho_vp);
-  }
   std::vector<int> ib(nf);
   const int counts[5] = {0, mp->nspec_bottom, mp->nspec_right, mp->nspec_top, mp->nspec_left};
   for (int f = 0; f < nf; f++) {
@@ -950,6 +945,12 @@ void prepare_stacey_device_(long* Mesh_pointer, int* ACOUSTIC_SIMULATION, int* E
     if (side < 1 || side > 4) fail("edge_abs(%d) = %d is not 1..4", f + 1, side);
     if (v < 1 || v > counts[side]) fail("ib index %d of face %d outside 1..%d", v, f + 1, counts[side]);
     ib[f] = v - 1;
+  }
+  if (*ACOUSTIC_SIMULATION && mp->nspec_acoustic > 0) {
+    // the fluid's points keep the caller's element numbering until the first acoustic step resolves them
+    std::vector<int> isp0 = to_zero_based(h_abs_boundary_ispec, nf, 1, mp->nspec, "abs_boundary_ispec");
+    acoustic_note_stacey(mp, nf, isp0.data(), ij.data(), h_abs_boundary_jacobian1Dw, rho_vp, h_edge_abs, ib.data(),
+                         counts);
   }
   if (mp->chain) {  // points of the absorbing edges, for first-touch assembly
     for (int f = 0; f < nf; f++)

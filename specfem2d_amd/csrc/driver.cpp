@@ -1172,6 +1172,7 @@ void prepare_device(Sim& s) {
                                     phase.data(), &nfree, fi.data(), fij.data(), &any_el, &ncoup, ci.data(),
                                     cij.data(), cn.data(), cw.data(), &zero, &dummy_i, &zero, &dummy_f, &dummy_f,
                                     &zero, &dummy_f);
+    if (s.simulation_type == 3) prepare_fields_acoustic_adj_dev_(&s.mesh, &zero, &zero, &zero);
   }
   if (nspec_el > 0) {
     int num_phase = nspec_el;
@@ -1186,7 +1187,7 @@ void prepare_device(Sim& s) {
                                    att ? s.att_A_mu.data() : &dummy_f, att ? s.att_B_mu.data() : &dummy_f,
                                    att ? s.att_A_kappa.data() : &dummy_f, att ? s.att_B_kappa.data() : &dummy_f);
   }
-  if (s.simulation_type == 3) {
+  if (s.simulation_type == 3 && nspec_el > 0) {
     int size = 2 * s.nglob;
     prepare_fields_elastic_adj_dev_(&s.mesh, &size, &zero, &zero, &zero);
   }
@@ -1219,6 +1220,7 @@ void prepare_device(Sim& s) {
   if (s.nspec_ac > 0) {
     int nac = s.nglob;
     transfer_fields_ac_to_device_(&nac, zeros.data(), zeros.data(), zeros.data(), &s.mesh);
+    if (s.simulation_type == 3) transfer_b_fields_ac_to_device_(&nac, zeros.data(), zeros.data(), zeros.data(), &s.mesh);
   }
   if (nspec_el > 0) {
     transfer_fields_el_to_device_(&size, zeros.data(), zeros.data(), zeros.data(), &s.mesh);
@@ -1319,8 +1321,8 @@ void* create(const sf2d_params* p, const double* rho, const double* vp, const do
     fprintf(stderr, "sf2d_sim_create: seismotype %d is not supported (1, 2, 3, 4 or 6)\n", st);
     return nullptr;
   }
-  if (st != 1 && (nshot > 0 || simulation_type != 1)) {
-    fprintf(stderr, "sf2d_sim_create: shot batches and adjoint runs record displacement (seismotype 1)\n");
+  if (st != 1 && nshot > 0) {
+    fprintf(stderr, "sf2d_sim_create: shot batches record displacement (seismotype 1)\n");
     return nullptr;
   }
   if (p->pml && (p->stacey || nshot > 0 || simulation_type != 1 || save_forward || p->nelem_pml < 1 ||
@@ -1720,14 +1722,15 @@ int sf2d_sim_forward(void* sim, int nstep_run, double* seis, float* absorb, doub
   int elastic_sim = any_el ? 1 : 0, acoustic_sim = any_ac ? 1 : 0, trick = s->p.use_trick ? 1 : 0;
   int nspec_inner_ac = s->nspec_ac, nfluid_solid = (int)s->coup_ac_ispec.size();
   nspec_inner = s->nspec - s->nspec_ac;
-  if (any_ac && (s->save_forward || s->grad_save)) return 1;  // no fluid Stacey storage or fluid gradients
+  if (any_ac && s->grad_save) return 1;  // no fluid gradients
   if (s->sisux.empty()) {
     s->sisux.assign((size_t)std::max(1, s->nrec) * nlength, 0.0);
     s->sisuz.assign(s->sisux.size(), 0.0);
   }
   const int counts[5] = {0, s->nspec_side[1], s->nspec_side[2], s->nspec_side[3], s->nspec_side[4]};
-  std::vector<float> slice[5];
+  std::vector<float> slice[5], slice_ac[5];
   for (int c = 1; c <= 4; c++) slice[c].assign((size_t)2 * NGLL * std::max(1, counts[c]), 0.f);
+  for (int c = 1; c <= 4; c++) slice_ac[c].assign((size_t)NGLL * std::max(1, counts[c]), 0.f);
   // timing: events are recorded on the library's stream and read once after the loop
   const bool timed = timing != nullptr;
   const bool phases = timed && timing[0] < 0.0;
@@ -1748,9 +1751,10 @@ int sf2d_sim_forward(void* sim, int nstep_run, double* seis, float* absorb, doub
     const int step = it - it_begin;
     mark(step, 2);
     // update_displ_Newmark (update_displacement_Newmark.F90:130-300): fluid, then solid
+    // compute_b_wavefield = .true. as update_displ_acoustic_forward passes it (update_displacement_Newmark.F90:150-155)
     if (any_ac)
       update_displacement_ac_cuda_(&s->mesh, &deltat, &deltatsqover2, &deltatover2, &b_deltat, &b_deltatsqover2,
-                                   &b_deltatover2, &zero, &zero);
+                                   &b_deltatover2, &one, &zero);
     if (any_el)
       update_displacement_cuda_(&s->mesh, &deltat, &deltatsqover2, &deltatover2, &b_deltat, &b_deltatsqover2,
                                 &b_deltatover2);
@@ -1764,8 +1768,8 @@ int sf2d_sim_forward(void* sim, int nstep_run, double* seis, float* absorb, doub
         if (pml) pml_boundary_acoustic_cuda_(&s->mesh, &one, &zero);
         if (iphase == 1) {
           if (s->p.stacey)
-            compute_stacey_acoustic_cuda_(&s->mesh, &iphase, slice[4].data(), slice[2].data(), slice[3].data(),
-                                          slice[1].data(), &one, &zero, &zero);
+            compute_stacey_acoustic_cuda_(&s->mesh, &iphase, slice_ac[4].data(), slice_ac[2].data(),
+                                          slice_ac[3].data(), slice_ac[1].data(), &one, &zero, &zero);
           if (any_el && nfluid_solid > 0) compute_coupling_ac_el_cuda_(&s->mesh, &iphase, &nfluid_solid, &one);
           compute_add_sources_ac_cuda_(&s->mesh, &iphase, &it);
         }
@@ -1797,13 +1801,16 @@ int sf2d_sim_forward(void* sim, int nstep_run, double* seis, float* absorb, doub
       }
     }
     if (absorb && s->save_forward && s->nabs > 0) {
-      // the Fortran caller keeps the per-step slices (compute_gpu_elastic.f90:267-273)
+      // the Fortran caller keeps the per-step slices (compute_gpu_elastic.f90:267-273, compute_gpu_acoustic.f90:
+      // 162-167); a fluid face's term goes to component 0
       for (int f = 0; f < s->nabs; f++) {
         const int c = s->edge_abs[f];
         const int ib = s->ib0[f] - 1;
+        const bool fluid = any_ac && s->is_acoustic[s->abs_ispec[f] - 1];
         for (int k = 0; k < NGLL; k++)
           for (int d = 0; d < 2; d++)
-            absorb[(((size_t)step * s->nabs + f) * NGLL + k) * 2 + d] = slice[c][((size_t)ib * NGLL + k) * 2 + d];
+            absorb[(((size_t)step * s->nabs + f) * NGLL + k) * 2 + d] =
+                fluid ? (d == 0 ? slice_ac[c][(size_t)ib * NGLL + k] : 0.f) : slice[c][((size_t)ib * NGLL + k) * 2 + d];
       }
     }
     mark(step, 6);
@@ -1901,9 +1908,24 @@ void sf2d_sim_batch_get_fields(void* sim, int shot, float* displ, float* veloc, 
 int sf2d_sim_adjoint(void* sim, const float* adj, const float* absorb, const float* last_displ,
                      const float* last_veloc, const float* last_accel, double* seis, float* kernels) {
   Sim* s = (Sim*)sim;
-  if (!s || !s->mesh || s->simulation_type != 3 || s->nspec_ac > 0) return 1;
+  if (!s || s->nspec_ac > 0) return 1;
+  return sf2d_sim_adjoint_full(sim, adj, absorb, last_displ, last_veloc, last_accel, nullptr, nullptr, nullptr, seis,
+                               kernels, nullptr);
+}
+
+int sf2d_sim_adjoint_full(void* sim, const float* adj, const float* absorb, const float* last_displ,
+                          const float* last_veloc, const float* last_accel, const float* last_pot,
+                          const float* last_pot_dot, const float* last_pot_dd, double* seis, float* kernels,
+                          float* kernels_ac) {
+  Sim* s = (Sim*)sim;
+  if (!s || !s->mesh || s->simulation_type != 3 || s->nspec_pml > 0) return 1;
+  const bool any_ac = s->nspec_ac > 0, any_el = s->nspec_ac < s->nspec;
+  if ((any_el && (!last_displ || !last_veloc || !last_accel)) || (any_ac && (!last_pot || !last_pot_dot || !last_pot_dd)))
+    return 1;
+  const bool coupled = any_ac && any_el && !s->coup_ac_ispec.empty();
   const int nstep = s->p.nstep;
-  // adjoint sources: Fortran source_adjoint(nrecloc, NSTEP, 2) (setup_sources_receivers.F90:578)
+  // adjoint sources: Fortran source_adjoint(nrecloc, NSTEP, 2) (setup_sources_receivers.F90:578); a receiver in a
+  // fluid element reads component 1 (compute_arrays_source.f90:415-424)
   {
     std::vector<float> sa((size_t)2 * nstep * std::max(1, s->nrec), 0.f);
     for (int r = 0; r < s->nrec; r++)
@@ -1919,53 +1941,115 @@ int sf2d_sim_adjoint(void* sim, const float* adj, const float* absorb, const flo
   float b_deltat = -deltat;
   float b_deltatover2 = (float)(0.5 * (double)b_deltat);
   float b_deltatsqover2 = (float)(0.5 * (double)b_deltat * (double)b_deltat);
-  int nspec_outer = 0, nspec_inner = s->nspec, zero = 0, one = 1, att = s->attenuation;
+  int nspec_outer = 0, nspec_inner = s->nspec - s->nspec_ac, nspec_inner_ac = s->nspec_ac, zero = 0, one = 1;
+  int att = s->attenuation, nfluid_solid = (int)s->coup_ac_ispec.size(), three = 3;
   int i_sig = 1, nlength = nstep, it_end = nstep, nadj = s->nrec, ns = nstep;
-  int elastic_sim = 1, acoustic_sim = 0;
+  int elastic_sim = any_el ? 1 : 0, acoustic_sim = any_ac ? 1 : 0, trick = s->p.use_trick ? 1 : 0;
   s->sisux.assign((size_t)std::max(1, s->nrec) * nlength, 0.0);
   s->sisuz.assign(s->sisux.size(), 0.0);
   const int counts[5] = {0, s->nspec_side[1], s->nspec_side[2], s->nspec_side[3], s->nspec_side[4]};
-  std::vector<float> slice[5];
+  std::vector<float> slice[5], slice_ac[5];
   for (int c = 1; c <= 4; c++) slice[c].assign((size_t)2 * NGLL * std::max(1, counts[c]), 0.f);
-  int size = 2 * s->nglob;
-  for (int it = 1; it <= nstep; it++) {
-    update_displacement_cuda_(&s->mesh, &deltat, &deltatsqover2, &deltatover2, &b_deltat, &b_deltatsqover2,
-                              &b_deltatover2);
-    const int it_tmp = nstep - it + 1;
-    for (int iphase = 1; iphase <= 2; iphase++) {
-      compute_forces_viscoelastic_cuda_(&s->mesh, &iphase, &deltat, &nspec_outer, &nspec_inner, &zero, &att, &one,
-                                        &one);
-      if (iphase == 1) {
-        if (s->p.stacey && s->nabs > 0) {
-          // the Fortran caller hands the slices of step NSTEP-it+1 (compute_gpu_elastic.f90:252-258)
-          for (int f = 0; f < s->nabs; f++) {
-            const int c = s->edge_abs[f];
-            const int ib = s->ib0[f] - 1;
-            for (int k = 0; k < NGLL; k++)
-              for (int d = 0; d < 2; d++)
-                slice[c][((size_t)ib * NGLL + k) * 2 + d] = absorb[(((size_t)(it_tmp - 1) * s->nabs + f) * NGLL + k) * 2 + d];
-          }
-          compute_stacey_viscoelastic_cuda_(&s->mesh, &iphase, slice[4].data(), slice[2].data(), slice[3].data(),
-                                            slice[1].data(), &one, &one, &zero);
+  for (int c = 1; c <= 4; c++) slice_ac[c].assign((size_t)NGLL * std::max(1, counts[c]), 0.f);
+  int size = 2 * s->nglob, nac = s->nglob;
+  // the caller's slices of the stored Stacey terms of step it_tmp (compute_gpu_elastic.f90:252-258,
+  // compute_gpu_acoustic.f90:148-153)
+  auto load_slices = [&](int it_tmp) {
+    for (int f = 0; f < s->nabs; f++) {
+      const int c = s->edge_abs[f];
+      const int ib = s->ib0[f] - 1;
+      const bool fluid = any_ac && s->is_acoustic[s->abs_ispec[f] - 1];
+      for (int k = 0; k < NGLL; k++) {
+        const float* st = absorb + (((size_t)(it_tmp - 1) * s->nabs + f) * NGLL + k) * 2;
+        if (fluid) {
+          slice_ac[c][(size_t)ib * NGLL + k] = st[0];
+        } else {
+          slice[c][((size_t)ib * NGLL + k) * 2] = st[0];
+          slice[c][((size_t)ib * NGLL + k) * 2 + 1] = st[1];
         }
-        int itt = it_tmp;
-        if (s->nrec > 0) add_sources_el_sim_type_2_or_3_(&s->mesh, &iphase, &itt, &nadj, &ns);
-        compute_add_sources_el_s3_cuda_(&s->mesh, &iphase, &itt);
       }
     }
-    kernel_3_a_cuda_(&s->mesh, &deltatover2, &b_deltatover2, &one, &one);
+  };
+  // compute_forces_viscoacoustic_GPU (compute_gpu_acoustic.f90:34-220) for the wavefields w1 (adjoint), w2
+  // (reconstructed)
+  auto fluid_step = [&](int w1, int w2, int it_tmp) {
+    acoustic_enforce_free_surf_cuda_(&s->mesh, &w1, &w2);
+    for (int iphase = 1; iphase <= 2; iphase++) {
+      compute_forces_acoustic_cuda_(&s->mesh, &iphase, &zero, &nspec_inner_ac, &zero, &w1, &w2);
+      if (iphase == 1) {
+        if (s->p.stacey)
+          compute_stacey_acoustic_cuda_(&s->mesh, &iphase, slice_ac[4].data(), slice_ac[2].data(), slice_ac[3].data(),
+                                        slice_ac[1].data(), &w1, &w2, &zero);
+        if (coupled && w1) compute_coupling_ac_el_cuda_(&s->mesh, &iphase, &nfluid_solid, &one);
+        if (coupled && w2) compute_coupling_ac_el_cuda_(&s->mesh, &iphase, &nfluid_solid, &three);
+        int itt = it_tmp;
+        if (s->nrec > 0 && w1) add_sources_ac_sim_2_or_3_cuda_(&s->mesh, &iphase, &itt, &nadj, &ns);
+        if (w2) compute_add_sources_ac_s3_cuda_(&s->mesh, &iphase, &itt);
+      }
+    }
+    kernel_3_acoustic_cuda_(&s->mesh, &deltatover2, &b_deltatover2, &w1, &w2);
+    acoustic_enforce_free_surf_cuda_(&s->mesh, &w1, &w2);
+  };
+  // compute_forces_viscoelastic_GPU (compute_gpu_elastic.f90:34-215)
+  auto solid_step = [&](int w1, int w2, int it_tmp) {
+    for (int iphase = 1; iphase <= 2; iphase++) {
+      compute_forces_viscoelastic_cuda_(&s->mesh, &iphase, &deltat, &nspec_outer, &nspec_inner, &zero, &att, &w1,
+                                        &w2);
+      if (iphase == 1) {
+        if (s->p.stacey && s->nabs > 0)
+          compute_stacey_viscoelastic_cuda_(&s->mesh, &iphase, slice[4].data(), slice[2].data(), slice[3].data(),
+                                            slice[1].data(), &w1, &w2, &zero);
+        if (coupled && w1) compute_coupling_el_ac_cuda_(&s->mesh, &iphase, &nfluid_solid, &one);
+        if (coupled && w2) compute_coupling_el_ac_cuda_(&s->mesh, &iphase, &nfluid_solid, &three);
+        int itt = it_tmp;
+        if (s->nrec > 0 && w1) add_sources_el_sim_type_2_or_3_(&s->mesh, &iphase, &itt, &nadj, &ns);
+        if (w2) compute_add_sources_el_s3_cuda_(&s->mesh, &iphase, &itt);
+      }
+    }
+    kernel_3_a_cuda_(&s->mesh, &deltatover2, &b_deltatover2, &w1, &w2);
+  };
+  for (int it = 1; it <= nstep; it++) {
+    const int it_tmp = nstep - it + 1;
+    // update_displ_Newmark: both fields of each domain (update_displacement_Newmark.F90:126-160, 500-548)
+    if (any_ac)
+      update_displacement_ac_cuda_(&s->mesh, &deltat, &deltatsqover2, &deltatover2, &b_deltat, &b_deltatsqover2,
+                                   &b_deltatover2, &one, &zero);
+    if (any_el)
+      update_displacement_cuda_(&s->mesh, &deltat, &deltatsqover2, &deltatover2, &b_deltat, &b_deltatsqover2,
+                                &b_deltatover2);
+    if (s->p.stacey && s->nabs > 0) load_slices(it_tmp);
+    // iterate_time.F90:227-246, 328-358: the adjoint solid, then the adjoint fluid; on coupled meshes the
+    // reconstructed fields follow in the forward order, fluid then solid
+    if (coupled) {
+      solid_step(1, 0, it_tmp);
+      fluid_step(1, 0, it_tmp);
+      fluid_step(0, 1, it_tmp);
+      solid_step(0, 1, it_tmp);
+    } else {
+      if (any_el) solid_step(1, 1, it_tmp);
+      if (any_ac) fluid_step(1, 1, it_tmp);
+    }
     if (it == 1) {
-      // read_forward_arrays.f90:77-98: the last forward frame replaces the reconstructed field
-      std::vector<float> d(last_displ, last_displ + size), v(last_veloc, last_veloc + size),
-          a(last_accel, last_accel + size);
-      transfer_b_fields_to_device_(&size, d.data(), v.data(), a.data(), &s->mesh);
+      // read_forward_arrays.f90:52-98: the last forward frames replace the reconstructed fields
+      if (any_ac) {
+        std::vector<float> p(last_pot, last_pot + nac), v(last_pot_dot, last_pot_dot + nac),
+            a(last_pot_dd, last_pot_dd + nac);
+        transfer_b_fields_ac_to_device_(&nac, p.data(), v.data(), a.data(), &s->mesh);
+      }
+      if (any_el) {
+        std::vector<float> d(last_displ, last_displ + size), v(last_veloc, last_veloc + size),
+            a(last_accel, last_accel + size);
+        transfer_b_fields_to_device_(&size, d.data(), v.data(), a.data(), &s->mesh);
+      }
     }
     if (s->nrec > 0) {
       int cur = it;
       compute_seismograms_cuda_(&s->mesh, &i_sig, s->sisux.data(), s->sisuz.data(), &cur, &nlength, &elastic_sim,
-                                &acoustic_sim, &zero, &zero, &it, &it_end);
+                                &acoustic_sim, &trick, &zero, &it, &it_end);
     }
-    compute_kernels_elastic_cuda_(&s->mesh, &deltat);
+    // compute_kernels.f90:34-80: solid, then fluid
+    if (any_el) compute_kernels_elastic_cuda_(&s->mesh, &deltat);
+    if (any_ac) compute_kernels_acoustic_cuda_(&s->mesh, &deltat);
   }
   s->it_done = nstep;
   sf2d_mesh_synchronize(&s->mesh);
@@ -1975,16 +2059,27 @@ int sf2d_sim_adjoint(void* sim, const float* adj, const float* absorb, const flo
         seis[((size_t)r * 2 + 0) * nstep + k] = s->sisux[(size_t)r * nlength + k];
         seis[((size_t)r * 2 + 1) * nstep + k] = s->sisuz[(size_t)r * nlength + k];
       }
+  const size_t ne = (size_t)s->nspec * NGLL2;
+  int nsp = s->nspec;
   if (kernels) {
-    // transfer_kernels_el_to_host + save_adjoint_kernels.f90:100-140 scaling
-    const size_t ne = (size_t)s->nspec * NGLL2;
-    std::vector<float> rk(ne), mk(ne), kk(ne);
-    int nsp = s->nspec;
-    transfer_kernels_el_to_host_(&s->mesh, rk.data(), mk.data(), kk.data(), &nsp);
+    // transfer_kernels_el_to_host + save_adjoint_kernels.f90:100-140 scaling (zero in fluid elements)
+    std::vector<float> rk(ne, 0.f), mk(ne, 0.f), kk(ne, 0.f);
+    if (any_el) transfer_kernels_el_to_host_(&s->mesh, rk.data(), mk.data(), kk.data(), &nsp);
     for (size_t l = 0; l < ne; l++) {
       kernels[l] = -s->rho[l] * deltat * rk[l];
       kernels[ne + l] = (float)(-2.0 * (double)s->mu[l] * (double)deltat * (double)mk[l]);
       kernels[2 * ne + l] = -s->kappa[l] * deltat * kk[l];
+    }
+  }
+  if (kernels_ac) {
+    // transfer_kernels_ac_to_host and it_transfer_from_GPU (iterate_time.F90:485-497): rho, kappa, rho' and c
+    std::vector<float> rk(ne, 0.f), kk(ne, 0.f);
+    if (any_ac) transfer_kernels_ac_to_host_(&s->mesh, rk.data(), kk.data(), &nsp);
+    for (size_t l = 0; l < ne; l++) {
+      kernels_ac[l] = rk[l];
+      kernels_ac[ne + l] = kk[l];
+      kernels_ac[2 * ne + l] = rk[l] + kk[l];
+      kernels_ac[3 * ne + l] = (float)(2.0 * (double)kk[l]);
     }
   }
   return 0;

@@ -240,7 +240,10 @@ void apply_element_forces(const Mesh* mp, int ph, const float* u, float* acc, in
 // pressure seismograms (-potential with USE_TRICK_FOR_BETTER_PRESSURE, else -potential_dot_dot) or potential
 // seismograms (potential != 0) and the number of receivers in fluid elements
 void acoustic_release(Mesh* mp);
-void acoustic_note_stacey(Mesh* mp, int nf, const int* ispec, const int* ij, const float* jacw, const float* rho_vp);
+// edge: side code of each face (1 bottom, 2 right, 3 top, 4 left), ib: 0-based slot of the face in its side's
+// storage, counts[side]: faces per side (the SAVE_FORWARD slices)
+void acoustic_note_stacey(Mesh* mp, int nf, const int* ispec, const int* ij, const float* jacw, const float* rho_vp,
+                          const int* edge, const int* ib, const int* counts);
 void launch_acoustic_pressure_seismograms(Mesh* mp, float* dst, int cur, int use_trick, int potential);
 void launch_acoustic_vector_seismograms(Mesh* mp, float* dst, int cur, int seismotype);
 int acoustic_receivers(Mesh* mp);

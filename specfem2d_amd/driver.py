@@ -81,6 +81,8 @@ def load_libraries():
     host.sf2d_sim_forward.restype = C.c_int
     host.sf2d_sim_adjoint.argtypes = [P, P, P, P, P, P, P, P]
     host.sf2d_sim_adjoint.restype = C.c_int
+    host.sf2d_sim_adjoint_full.argtypes = [P] * 12
+    host.sf2d_sim_adjoint_full.restype = C.c_int
     host.sf2d_sim_set_model.argtypes = [P, P, P, P]
     host.sf2d_sim_set_model.restype = C.c_int
     host.sf2d_sim_gradient_forward.argtypes = [P, P, C.c_int, P, P]
@@ -408,6 +410,33 @@ class Simulation:
             raise RuntimeError("sf2d_sim_adjoint failed")
         ker = ker.reshape(3, self.nspec, NGLL2)
         return seis[: self.cfg.nrec], {"rho": ker[0], "mu": ker[1], "kappa": ker[2]}
+
+    def adjoint_full(self, adj, absorb, lastframe_el=None, lastframe_ac=None):
+        """SIMULATION_TYPE=3 run over fluid, solid or coupled meshes (``sf2d_sim_adjoint_full``).  ``adj``
+        [nrec,2,nstep] adjoint source traces (a receiver in a fluid element uses component 0), ``absorb`` the
+        Stacey storage of a save_forward run, ``lastframe_el`` (displ, veloc, accel) and ``lastframe_ac``
+        (potential and its two time derivatives) at its end.  Returns (adjoint seismograms [nrec,2,nstep], solid
+        kernels dict rho/mu/kappa, fluid kernels dict rho/kappa/rhop/alpha), kernels [nspec,25]."""
+        self._need_device()
+        if self.simulation_type != 3:
+            raise RuntimeError("create the simulation with simulation_type=3 for an adjoint run")
+        n = self.cfg.nstep
+        f32 = lambda x: np.ascontiguousarray(np.asarray(x, np.float32).reshape(-1))  # noqa: E731
+        a = f32(adj)
+        st = f32(absorb) if self.nabs else np.zeros(1, np.float32)
+        el = [None] * 3 if lastframe_el is None else [f32(x) for x in lastframe_el]
+        ac = [None] * 3 if lastframe_ac is None else [f32(x) for x in lastframe_ac]
+        seis = np.zeros((max(1, self.cfg.nrec), 2, n))
+        kel = np.zeros(3 * self.nspec * NGLL2, np.float32)
+        kac = np.zeros(4 * self.nspec * NGLL2, np.float32)
+        rc = self.host.sf2d_sim_adjoint_full(self.h, _ptr(a), _ptr(st), *[_ptr(x) for x in el + ac], _ptr(seis),
+                                             _ptr(kel), _ptr(kac))
+        if rc != 0:
+            raise RuntimeError("sf2d_sim_adjoint_full failed (CPML, or a missing last frame)")
+        kel = kel.reshape(3, self.nspec, NGLL2)
+        kac = kac.reshape(4, self.nspec, NGLL2)
+        return (seis[: self.cfg.nrec], dict(zip(("rho", "mu", "kappa"), kel)),
+                dict(zip(("rho", "kappa", "rhop", "alpha"), kac)))
 
     # CPML layer ----------------------------------------------------------
     def pml(self):

@@ -104,3 +104,41 @@ def test_kernel_increment_vs_numpy():
     ref = _numpy_kernel_increment(cfg, ib, *(orc.elem_array(n) for n in ("xix", "xiz", "gammax", "gammaz")), d, a, b)
     for k in ("rho", "mu", "kappa"):
         np.testing.assert_allclose(got[k], ref[k], rtol=1e-10, atol=1e-10 * np.abs(ref[k]).max())
+
+
+def test_fluid_kernel_increment_vs_numpy():
+    """compute_kernels.f90:426-433: rho_ac_kl += 1/rho grad(pot).grad(b_pot) dt, kappa_ac_kl += 1/kappa pdd b_pot dt,
+    against an independent numpy evaluation (derivative matrix from polynomial algebra) on a fluid mesh with
+    uneven elements and per-point densities and velocities."""
+    cfg = config.small_test_mesh(nx=5, nz=4, nstep=2).replace(
+        interfaces=[-400.0, -150.0, 0.0], nz_layers=[2, 2], xmax=630.0, material_mode=1)
+    rng = np.random.default_rng(5)
+    n = cfg.nspec * 25
+    cfg = cfg.replace(rho=1000.0 + 200.0 * rng.random(n), vp=1500.0 + 300.0 * rng.random(n), vs=np.zeros(n))
+    orc = OracleSim(cfg, "dp")
+    assert orc.nspec_acoustic == orc.nspec
+    pot, pdd, bpot = (rng.standard_normal(orc.nglob) for _ in range(3))
+    dt = 1.7e-3
+    got = orc.kernel_ac_increment(pot, pdd, bpot, dt)
+    ib = orc.ibool()
+    xi = config.XIGLL
+    D = np.zeros((5, 5))
+    for k in range(5):
+        yk = np.zeros(5)
+        yk[k] = 1.0
+        c = np.polynomial.polynomial.polyfit(xi, yk, 4)
+        D[:, k] = np.polynomial.polynomial.polyval(xi, np.polynomial.polynomial.polyder(c))
+    X, Z, G, H = (orc.elem_array(q).reshape(-1, 5, 5) for q in ("xix", "xiz", "gammax", "gammaz"))
+
+    def grad(u):
+        ue = u[ib - 1].reshape(-1, 5, 5)  # [e, j, i]
+        dxi = np.einsum("ik,ejk->eji", D, ue)
+        dga = np.einsum("jk,eki->eji", D, ue)
+        return dxi * X + dga * G, dxi * Z + dga * H
+
+    (ax, az), (bx, bz) = grad(pot), grad(bpot)
+    rho, kappa = orc.elem_array("rho"), orc.elem_array("kappa")
+    ref_rho = ((ax * bx + az * bz).reshape(-1, 25) / rho) * dt
+    ref_kappa = (pdd[ib - 1] * bpot[ib - 1] / kappa) * dt
+    for k, ref in (("rho", ref_rho), ("kappa", ref_kappa)):
+        np.testing.assert_allclose(got[k], ref, rtol=1e-10, atol=1e-10 * np.abs(ref).max())
