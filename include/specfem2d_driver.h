@@ -153,6 +153,11 @@ int sf2d_sim_adjoint_full(void* sim, const float* adj, const float* absorb, cons
                           const float* last_veloc, const float* last_accel, const float* last_pot,
                           const float* last_pot_dot, const float* last_pot_dd, double* seis, float* kernels,
                           float* kernels_ac);
+/* Anisotropic elements (material table icodemat 2, read_materials.f90:160-200), before the device is prepared:
+   is_aniso[nspec] flags, c[6][nspec][25] = c11, c13, c15, c33, c35, c55 per GLL point (the element's rho, vp =
+   sqrt(c33/rho), vs = sqrt(c55/rho) of the creation call give its mass and Stacey impedances).  Forward runs without
+   attenuation, CPML or shot batches.  Returns 0 on success. */
+int sf2d_sim_set_anisotropy(void* sim, const int* is_aniso, const double* c);
 /* replaces the model (MODEL=gll parameters rho, vp, vs per GLL point, [nspec][25]) of a simulation: moduli,
    Stacey impedances and mass matrix are recomputed and uploaded.  Returns 0 on success. */
 int sf2d_sim_set_model(void* sim, const double* rho, const double* vp, const double* vs);

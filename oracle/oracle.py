@@ -83,6 +83,8 @@ def _lib(precision: str):
         lib.orc_set_threads.restype = C.c_int
         lib.orc_set_threads.argtypes = [C.c_int]
         lib.orc_adjoint.argtypes = [P, dp, dp, dp, dp, dp, C.c_void_p, dp, C.c_int, C.c_void_p, C.c_void_p]
+        lib.orc_set_anisotropy.argtypes = [P, ip, dp]
+        lib.orc_set_anisotropy.restype = C.c_int
         lib.orc_adjoint_full.argtypes = [P, dp, dp, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         lib.orc_adjoint_full.restype = C.c_int
         lib.orc_kernel_increment.argtypes = [P, dp, dp, dp, dp]
@@ -191,6 +193,12 @@ class OracleSim:
         self.nspec_acoustic = self.lib.orc_nspec_acoustic(self.h)
         self.ncoupling = self.lib.orc_ncoupling(self.h)
         self.lib.orc_set_ftz(self.h, int(ftz))
+        if cfg.aniso_elem is not None:
+            ae = np.ascontiguousarray(np.asarray(cfg.aniso_elem, np.int32).reshape(-1))
+            ac = np.ascontiguousarray(np.asarray(cfg.aniso_c, np.float64).T.reshape(-1))  # [6][nspec*25]
+            self._keep_aniso = (ae, ac)
+            if self.lib.orc_set_anisotropy(self.h, ae, ac) != 0:
+                raise NotImplementedError("oracle: anisotropic elements with attenuation or CPML")
         if cfg.attenuation:
             qk, qm = cfg.attenuation_arrays()
             self._keep_q = (qk, qm)

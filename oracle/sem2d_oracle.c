@@ -115,6 +115,8 @@ typedef struct {
   double cosrot, sinrot;
   /* domains: an element is acoustic when vs = 0 at all its GLL points */
   int *is_acoustic;           /* [nspec] */
+  int *is_aniso;              /* [nspec] anisotropic elements (NULL: none) */
+  REAL *aniso_c;              /* [nspec*25][6] c11 c13 c15 c33 c35 c55 */
   int nspec_acoustic, any_acoustic, any_elastic;
   REAL *rmass_ac_inv;         /* [nglob] acoustic inverse mass */
   int nfree;                  /* acoustic free-surface edges: [nfree] element, side code */
@@ -1474,6 +1476,14 @@ static void element_force(orc_sim *s, int e, const REAL *displ, REAL *accel) {
         REAL sxx = lp2m * dux_dxl + lambdal * duz_dzl;
         REAL szz = lp2m * duz_dzl + lambdal * dux_dxl;
         REAL sxz = mul * (duz_dxl + dux_dzl);
+        if (s->is_aniso && s->is_aniso[e]) {
+          /* compute_forces_viscoelastic.F90:372-447 (P-SV): the stress of an anisotropic element */
+          const REAL *c = s->aniso_c + l * 6;
+          const REAL sh = duz_dxl + dux_dzl;
+          sxx = c[0] * dux_dxl + c[1] * duz_dzl + c[2] * sh;
+          szz = c[1] * dux_dxl + c[3] * duz_dzl + c[4] * sh;
+          sxz = c[2] * dux_dxl + c[4] * duz_dzl + c[5] * sh;
+        }
         if (s->att_live && s->att_on[l] && !pml) {  /* nothing in CPML elements (compute_attenuation_viscoelastic.f90:80) */
           /* compute_attenuation_viscoelastic.f90:93-121 (Newmark, convolution memory variables), then the
              unrelaxed-moduli stress correction of compute_forces_viscoelastic.F90:481-496 */
@@ -2009,6 +2019,7 @@ void orc_free(void *h) {
   orc_sim *s = (orc_sim *)h;
   if (!s) return;
   free(s->knods); free(s->coorg); free(s->ibool); free(s->coord);
+  free(s->is_aniso); free(s->aniso_c);
   free(s->ext_free_elem); free(s->ext_free_code); free(s->ext_abs_elem); free(s->ext_abs_code);
   free(s->xix); free(s->xiz); free(s->gammax); free(s->gammaz); free(s->jacobian);
   free(s->rhostore); free(s->mustore); free(s->kappastore); free(s->rho_vp); free(s->rho_vs);
@@ -2202,6 +2213,23 @@ static void pml_lik_visco(double kx, double bx, double ax, double kz, double bz,
   } else {
     A[0] = A[1] = A[2] = A[3] = 0.0;
   }
+}
+
+/* anisotropic elements (setup_mesh.F90:824-870: c_ij of the material table per GLL point, single precision):
+   is_aniso[nspec], c[6][nspec*25] = c11, c13, c15, c33, c35, c55.  Returns -1 with attenuation or CPML (the
+   anisotropic branch of the reference has neither here). */
+int orc_set_anisotropy(void *h, const int *is_aniso, const double *c) {
+  orc_sim *s = (orc_sim *)h;
+  if (s->att || s->pml_on) return -1;
+  const size_t ne = (size_t)s->nspec * NGLL2;
+  free(s->is_aniso);
+  free(s->aniso_c);
+  s->is_aniso = (int *)calloc((size_t)s->nspec, sizeof(int));
+  s->aniso_c = (REAL *)calloc(ne * 6, sizeof(REAL));
+  for (int e = 0; e < s->nspec; e++) s->is_aniso[e] = is_aniso[e] != 0;
+  for (size_t l = 0; l < ne; l++)
+    for (int k = 0; k < 6; k++) s->aniso_c[l * 6 + k] = (REAL)c[k * ne + l];
+  return 0;
 }
 
 int orc_real_size(void) { return (int)sizeof(REAL); }

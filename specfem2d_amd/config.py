@@ -124,6 +124,11 @@ class SimulationConfig:
     # read_external_mesh: the mesh comes from files (nx, nz, xmin, xmax, interfaces and the absorb* flags of the
     # internal mesher are then unused; Stacey edges are the mesh's absorbing sides)
     mesh: Optional[ExternalMesh] = None
+    # anisotropic elements (material table icodemat 2): a flag per element [nspec] and, per GLL point, the
+    # stiffness (c11, c13, c15, c33, c35, c55) [nspec*25, 6]; rho, vp = sqrt(c33/rho) and vs = sqrt(c55/rho) stay in
+    # rho/vp/vs for the mass matrix and the Stacey impedances (read_materials.f90:160-200)
+    aniso_elem: Optional[np.ndarray] = None
+    aniso_c: Optional[np.ndarray] = None
 
     # ------------------------------------------------------------------
     @property

@@ -794,10 +794,17 @@ void prepare_fields_elastic_device_(long* Mesh_pointer, realw* rmassx, realw* rm
                                     int* ATTENUATION_VISCOELASTIC, realw* h_A_newmark_mu, realw* h_B_newmark_mu,
                                     realw* h_A_newmark_kappa,
                                     realw* h_B_newmark_kappa) {  // prepare_mesh_constants_cuda.cu:527
-  (void)ispec_is_anisotropic; (void)c11store; (void)c12store; (void)c13store; (void)c15store; (void)c23store;
-  (void)c25store; (void)c33store; (void)c35store; (void)c55store; (void)h_inum_interfaces_elastic;
+  (void)c12store; (void)c23store; (void)c25store; (void)h_inum_interfaces_elastic;  // AXISYM and SH terms
   Mesh* mp = handle(Mesh_pointer);
-  if (*ANISOTROPY) fail("anisotropic elements are not supported by this library yet");
+  if (*ANISOTROPY) {
+    // anisotropic elements (Kernel_2_viscoelastic_impl.cu:616-853): the per-point kernel with coloured assembly
+    if (*ATTENUATION_VISCOELASTIC || mp->simulation_type != 1 || mp->pml)
+      fail("anisotropic elements: forward runs without attenuation or CPML only");
+    if (!ispec_is_anisotropic || !c11store || !c13store || !c15store || !c33store || !c35store || !c55store)
+      fail("anisotropic elements: stiffness arrays missing");
+    mp->anisotropy = 1;
+    mp->assembly = 1;
+  }
   if (*h_ninterface_elastic > 0) fail("MPI elastic interfaces passed: one GPU, one partition only");
   if (*ATTENUATION_VISCOELASTIC) {
     // forward runs, 3 solids, in the solid elements (the reference's GPU code refuses attenuation with fluid elements,
@@ -820,6 +827,29 @@ void prepare_fields_elastic_device_(long* Mesh_pointer, realw* rmassx, realw* rm
   const int np = *num_phase_ispec_elastic;
   mp->num_phase_ispec_elastic = np;
   sort_elements_by_colour(mp, phase_ispec_inner_elastic, np);
+  if (mp->anisotropy) {
+    // stiffness per point in device element order: the caller's c_ij (Fortran (NGLLX, NGLLZ, nspec)) for the
+    // anisotropic elements, the isotropic equivalent of kappa and mu for the others
+    const size_t ne = (size_t)mp->nspec * NGLL2;
+    std::vector<float> kap(ne), mu(ne);
+    check_hip(hipMemcpy(kap.data(), mp->d_kappav, ne * sizeof(float), hipMemcpyDeviceToHost), "kappa D2H");
+    check_hip(hipMemcpy(mu.data(), mp->d_muv, ne * sizeof(float), hipMemcpyDeviceToHost), "mu D2H");
+    std::vector<float4> c(2 * ne);
+    for (int pos = 0; pos < mp->nspec; pos++) {
+      const int e = mp->sorted ? mp->ispec_of_pos[pos] : pos;
+      for (int q = 0; q < NGLL2; q++) {
+        const size_t l = (size_t)pos * NGLL2 + q, h = (size_t)e * NGLL2 + q;
+        if (ispec_is_anisotropic[e]) {
+          c[2 * l] = make_float4(c11store[h], c13store[h], c15store[h], c33store[h]);
+          c[2 * l + 1] = make_float4(c35store[h], c55store[h], 0.f, 0.f);
+        } else {
+          c[2 * l] = make_float4(kap[l] + mu[l], kap[l] - mu[l], 0.f, kap[l] + mu[l]);
+          c[2 * l + 1] = make_float4(0.f, mu[l], 0.f, 0.f);
+        }
+      }
+    }
+    mp->d_aniso = device_copy(c.data(), c.size());
+  }
   if (mp->attenuation) {
     // host arrays A(N_SLS, NGLLX, NGLLZ, nspec) (solid fastest, ispec order): the distinct coefficient sets of
     // the points go to a table (a few per model: one per distinct pair of quality factors), each point keeps a
@@ -1027,6 +1057,7 @@ void prepare_cleanup_device_(long* Mesh_pointer, int* ACOUSTIC_SIMULATION, int* 
   device_free(mp->d_moduli);
   device_free(mp->d_geom_elem);
   device_free(mp->d_geom_rect);
+  device_free(mp->d_aniso);
   device_free(mp->d_chain_ids);
   device_free(mp->d_col_ids);
   device_free(mp->d_col_mod);
@@ -1200,7 +1231,8 @@ void compute_forces_viscoelastic_cuda_(long* Mesh_pointer, int* iphase, realw* d
   (void)deltat;
   Mesh* mp = handle(Mesh_pointer);
   flush_correctors(mp);
-  if (*ANISOTROPY) fail("anisotropic elements are not supported by this library yet");
+  if ((*ANISOTROPY != 0) != (mp->anisotropy != 0))
+    fail("compute_forces_viscoelastic_cuda: ANISOTROPY differs from prepare_fields_elastic_device");
   if ((*ATTENUATION_VISCOELASTIC != 0) != (mp->attenuation != 0))
     fail("compute_forces_viscoelastic_cuda: ATTENUATION_VISCOELASTIC differs from prepare_fields_elastic_device");
   const int ph = (*iphase == 1) ? 0 : 1;

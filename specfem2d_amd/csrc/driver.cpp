@@ -136,6 +136,8 @@ struct Sim {
   std::vector<float> rmassx, rmassz;
   // fluid elements (vs = 0 at all 25 points) and their boundaries
   std::vector<int> is_acoustic;      // [nspec]
+  std::vector<int> aniso_elem;       // [nspec] anisotropic elements (empty: none)
+  std::vector<float> aniso_c[6];     // c11, c13, c15, c33, c35, c55 [nspec][25]
   int nspec_ac = 0;
   std::vector<float> rmass_ac;       // [nglob] inverse fluid mass
   std::vector<int> free_ispec, free_ij;            // acoustic free-surface edges: 1-based element, [nf][5][2] (i, j)
@@ -1179,11 +1181,16 @@ void prepare_device(Sim& s) {
     s.phase_ispec.assign((size_t)2 * num_phase, 0);
     for (int e = 0, q = 0; e < s.nspec; e++)
       if (is_elastic[e]) s.phase_ispec[(size_t)num_phase + q++] = e + 1;
-    std::vector<int> aniso(s.nspec, 0);
-    int att = s.attenuation;
+    std::vector<int> aniso = s.aniso_elem.empty() ? std::vector<int>(s.nspec, 0) : s.aniso_elem;
+    int att = s.attenuation, any_aniso = s.aniso_elem.empty() ? 0 : 1;
+    std::vector<float> c0((size_t)s.nspec * NGLL2, 0.f);  // c12, c23, c25: AXISYM and SH only
+    float* c[6];
+    for (int k = 0; k < 6; k++) c[k] = any_aniso ? s.aniso_c[k].data() : &dummy_f;
+    // Fortran argument order c11, c12, c13, c15, c23, c25, c33, c35, c55 (prepare_gpu.f90:189-199)
     prepare_fields_elastic_device_(&s.mesh, s.rmassx.data(), s.rmassz.data(), &num_phase, s.phase_ispec.data(),
-                                   aniso.data(), &zero, &dummy_f, &dummy_f, &dummy_f, &dummy_f, &dummy_f, &dummy_f,
-                                   &dummy_f, &dummy_f, &dummy_f, &zero, &dummy_i, &att,
+                                   aniso.data(), &any_aniso, c[0], any_aniso ? c0.data() : &dummy_f, c[1], c[2],
+                                   any_aniso ? c0.data() : &dummy_f, any_aniso ? c0.data() : &dummy_f, c[3], c[4],
+                                   c[5], &zero, &dummy_i, &att,
                                    att ? s.att_A_mu.data() : &dummy_f, att ? s.att_B_mu.data() : &dummy_f,
                                    att ? s.att_A_kappa.data() : &dummy_f, att ? s.att_B_kappa.data() : &dummy_f);
   }
@@ -1721,6 +1728,7 @@ int sf2d_sim_forward(void* sim, int nstep_run, double* seis, float* absorb, doub
   const bool pml = s->nspec_pml > 0;
   int elastic_sim = any_el ? 1 : 0, acoustic_sim = any_ac ? 1 : 0, trick = s->p.use_trick ? 1 : 0;
   int nspec_inner_ac = s->nspec_ac, nfluid_solid = (int)s->coup_ac_ispec.size();
+  int aniso = s->aniso_elem.empty() ? 0 : 1;
   nspec_inner = s->nspec - s->nspec_ac;
   if (any_ac && s->grad_save) return 1;  // no fluid gradients
   if (s->sisux.empty()) {
@@ -1785,7 +1793,7 @@ int sf2d_sim_forward(void* sim, int nstep_run, double* seis, float* absorb, doub
     // then the solid, coupled to the new fluid pressure (compute_gpu_elastic.f90:34-215)
     for (int iphase = 1; iphase <= 2 && any_el; iphase++) {
       if (iphase == 2) mark(step, 0);
-      compute_forces_viscoelastic_cuda_(&s->mesh, &iphase, &deltat, &nspec_outer, &nspec_inner, &zero, &att, &one,
+      compute_forces_viscoelastic_cuda_(&s->mesh, &iphase, &deltat, &nspec_outer, &nspec_inner, &aniso, &att, &one,
                                         &zero);
       if (iphase == 2) mark(step, 1);
       if (iphase == 1) {
@@ -2085,6 +2093,20 @@ int sf2d_sim_adjoint_full(void* sim, const float* adj, const float* absorb, cons
   return 0;
 }
 
+int sf2d_sim_set_anisotropy(void* sim, const int* is_aniso, const double* c) {
+  Sim* s = (Sim*)sim;
+  if (!s || s->mesh || s->simulation_type != 1 || s->nshot > 0 || s->attenuation || s->nspec_pml > 0) return 1;
+  const size_t ne = (size_t)s->nspec * NGLL2;
+  for (int e = 0; e < s->nspec; e++)
+    if (is_aniso[e] && s->nspec_ac > 0 && s->is_acoustic[e]) return 2;
+  s->aniso_elem.assign(is_aniso, is_aniso + s->nspec);
+  for (int k = 0; k < 6; k++) {
+    s->aniso_c[k].resize(ne);
+    for (size_t l = 0; l < ne; l++) s->aniso_c[k][l] = (float)c[k * ne + l];
+  }
+  return 0;
+}
+
 int sf2d_sim_set_model(void* sim, const double* rho, const double* vp, const double* vs) {
   Sim* s = (Sim*)sim;
   if (!s || s->nspec_pml > 0) return 1;  // the CPML layer's profiles and moduli are fixed at set-up
@@ -2104,7 +2126,8 @@ int sf2d_sim_set_model(void* sim, const double* rho, const double* vp, const dou
 
 int sf2d_sim_gradient_forward(void* sim, const double* obs, int chunk, double* chi, double* seis) {
   Sim* s = (Sim*)sim;
-  if (!s || !s->mesh || s->simulation_type != 1 || chunk < 1 || s->nspec_ac > 0 || s->p.seismotype != 1 ||
+  if (!s || !s->mesh || s->simulation_type != 1 || chunk < 1 || s->nspec_ac > 0 || !s->aniso_elem.empty() ||
+      s->p.seismotype != 1 ||
       s->attenuation || s->nspec_pml > 0)
     return 1;
   const int N = s->p.nstep, R = s->nrec, S = std::max(1, s->nshot);

@@ -213,7 +213,7 @@ __global__ __launch_bounds__(FORCE_THREADS) void k_elastic_forces(
     const float* __restrict__ gammaz, const float* __restrict__ kappav, const float* __restrict__ muv,
     const float2* __restrict__ displ, float* __restrict__ accel, const Basis basis,
     const unsigned int* __restrict__ att_idx, const float4* __restrict__ att_table, float* __restrict__ att_e,
-    float* __restrict__ att_old, long ne, const float2* __restrict__ geom_rect) {
+    float* __restrict__ att_old, long ne, const float2* __restrict__ geom_rect, const float4* __restrict__ aniso) {
   // XCD-aware remap: blocks b, b+8, b+16, ... (one XCD) take consecutive batches
   const int bid = blockIdx.x;
   const int batch = (bid % NXCD) * per_xcd + bid / NXCD;
@@ -243,7 +243,12 @@ __global__ __launch_bounds__(FORCE_THREADS) void k_elastic_forces(
   float4 cf[3];
   float ev[9], ov[3];
   float2 a_old = make_float2(0.f, 0.f);  // coloured launches: no other element of the launch has the point
+  float4 ca = make_float4(0.f, 0.f, 0.f, 0.f), cb = ca;  // anisotropic stiffness of the point
   if (active) {
+    if (aniso) {
+      ca = aniso[2 * ((size_t)(first + e) * NGLL2 + p)];
+      cb = aniso[2 * ((size_t)(first + e) * NGLL2 + p) + 1];
+    }
     if (!ATOMIC) a_old = reinterpret_cast<const float2*>(accel)[ibool[(size_t)(first + e) * NGLL2 + p]];
     const size_t l = (size_t)(first + e) * NGLL2 + p;
     if (ATT) {
@@ -293,6 +298,12 @@ __global__ __launch_bounds__(FORCE_THREADS) void k_elastic_forces(
     float sxx = lp2m * dux_dxl + lambdal * duz_dzl;
     float szz = lp2m * duz_dzl + lambdal * dux_dxl;
     float sxz = mu * (duz_dxl + dux_dzl);
+    if (aniso) {  // compute_forces_viscoelastic.F90:372-447 (P-SV)
+      const float sh = duz_dxl + dux_dzl;
+      sxx = ca.x * dux_dxl + ca.y * duz_dzl + ca.z * sh;
+      szz = ca.y * dux_dxl + ca.w * duz_dzl + cb.x * sh;
+      sxz = ca.z * dux_dxl + cb.x * duz_dzl + cb.y * sh;
+    }
     if (ATT) {
       const size_t l = (size_t)(first + e) * NGLL2 + p;
       attenuate_point(cf, ev, ov, kap, mu, dux_dxl, dux_dzl, duz_dxl, duz_dzl, sxx, sxz, szz);
@@ -1519,7 +1530,7 @@ void launch_elastic_forces(const Mesh* mp, int first, int nelem, int atomic, con
                      0, mp->stream, first, nelem, nbatch, per_xcd, mp->d_ibool, mp->d_xix, mp->d_xiz, mp->d_gammax,
                      mp->d_gammaz, mp->d_kappav, mp->d_muv, (const float2*)displ, accel, mp->basis, mp->d_att_idx,
                      mp->d_att_table, mp->d_att_e, mp->d_att_old, (long)mp->nspec * NGLL2,
-                     (const float2*)mp->d_geom_rect);
+                     (const float2*)mp->d_geom_rect, mp->d_aniso);
   check_hip(hipGetLastError(), "k_elastic_forces launch");
 }
 

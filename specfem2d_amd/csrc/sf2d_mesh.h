@@ -58,6 +58,10 @@ struct Mesh {
   // force kernel's point order; d_att_old is unused (null)
   float *d_att_e = nullptr, *d_att_old = nullptr;
   int* d_ispec_is_elastic = nullptr;
+  // anisotropic elements: per point of every element (device element order) {c11, c13, c15, c33}, {c35, c55, 0, 0};
+  // isotropic elements carry their equivalent constants (c11 = c33 = kappa + mu, c13 = kappa - mu, c55 = mu)
+  int anisotropy = 0;
+  float4* d_aniso = nullptr;
   int nspec_elastic = 0, nspec_acoustic = 0;
   int all_elastic = 1;
   std::vector<char> h_is_acoustic;  // [nspec] caller order (empty: no fluid element)

@@ -81,6 +81,8 @@ def load_libraries():
     host.sf2d_sim_forward.restype = C.c_int
     host.sf2d_sim_adjoint.argtypes = [P, P, P, P, P, P, P, P]
     host.sf2d_sim_adjoint.restype = C.c_int
+    host.sf2d_sim_set_anisotropy.argtypes = [P, P, P]
+    host.sf2d_sim_set_anisotropy.restype = C.c_int
     host.sf2d_sim_adjoint_full.argtypes = [P] * 12
     host.sf2d_sim_adjoint_full.restype = C.c_int
     host.sf2d_sim_set_model.argtypes = [P, P, P, P]
@@ -211,6 +213,12 @@ class Simulation:
             if self.host.sf2d_sim_set_attenuation(self.h, qk, qm, int(cfg.n_sls)) != 0:
                 raise NotImplementedError("viscoelastic attenuation: 3 solids, forward runs, quality factors 9999 "
                                           "in fluid elements")
+        if cfg.aniso_elem is not None:
+            ae = np.ascontiguousarray(np.asarray(cfg.aniso_elem, np.int32).reshape(-1))
+            ac = np.ascontiguousarray(np.asarray(cfg.aniso_c, np.float64).T.reshape(-1))  # [6][nspec*25]
+            if self.host.sf2d_sim_set_anisotropy(self.h, _ptr(ae), _ptr(ac)) != 0:
+                raise NotImplementedError("anisotropic elements: forward runs of solids without attenuation, CPML "
+                                          "or shot batches")
         self.on_device = False
         if device:
             self.prepare_device()

@@ -292,10 +292,24 @@ def config_from_par_file(data_dir: str, cpml: str = "cpml", nstep: Optional[int]
     vs = np.zeros(ne)
     qk = np.full(ne, 9999.0)
     qm = np.full(ne, 9999.0)
+    aniso_elem, aniso_c = None, None
     for m, vals in par.materials.items():
         icode = int(vals[0])
-        refuse(icode != 1, f"material {m} with icodemat {icode} (only isotropic tables)")
+        refuse(icode not in (1, 2), f"material {m} with icodemat {icode} (isotropic and anisotropic tables only)")
         sel = mat == m
+        if icode == 2:
+            # read_materials.f90:160-200: rho c11 c13 c15 c33 c35 c55 c12 c23 c25 c22 QKappa Qmu; cp = sqrt(c33/rho),
+            # cs = sqrt(c55/rho) for the mass and the Stacey impedances
+            refuse(attenuation, "anisotropic materials with ATTENUATION_VISCOELASTIC")
+            refuse(pml, "anisotropic materials with PML_BOUNDARY_CONDITIONS")
+            r, c11, c13, c15, c33, c35, c55 = (float(v) for v in vals[1:8])
+            if aniso_elem is None:
+                aniso_elem = np.zeros(len(mat), bool)
+                aniso_c = np.zeros((len(mat) * NGLL2, 6))
+            aniso_elem[sel] = True
+            aniso_c[np.repeat(sel, NGLL2)] = (c11, c13, c15, c33, c35, c55)
+            rho[sel], vp[sel], vs[sel] = r, np.sqrt(c33 / r), np.sqrt(c55 / r)
+            continue
         rho[sel], vp[sel], vs[sel] = vals[1], vals[2], vals[3]
         if attenuation:  # read_material_table.f90: QKappa, Qmu are the 7th and 8th values
             if len(vals) < 8:
@@ -319,7 +333,7 @@ def config_from_par_file(data_dir: str, cpml: str = "cpml", nstep: Optional[int]
         user_t0=fortran_float(g("USER_T0", "0")), anglerec=fortran_float(g("anglerec", "0")),
         seismotype=seismotypes[0], use_trick=fortran_bool(g("USE_TRICK_FOR_BETTER_PRESSURE", ".false.")),
         attenuation=attenuation, qkappa=rep(qk), qmu=rep(qm), n_sls=int(g("N_SLS", "3")), pml=pml, mesh=mesh,
-        **pml_kw)
+        aniso_elem=aniso_elem, aniso_c=aniso_c, **pml_kw)
     return cfg, stations
 
 

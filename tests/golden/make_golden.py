@@ -177,6 +177,9 @@ if __name__ == "__main__":
     copy_inputs(ext, ["mesh/mesh", "mesh/nodes_coords", "mesh/mat", "mesh/surface_free", "mesh/surface_abs"],
                 "fluid_solid_external_mesh", subdir="mesh")
     pack_float32(ext, "REF_SEIS", [f"AA.S{k:04d}" for k in range(1, 13)], "fluid_solid_external_mesh_ref_seis.npz")
+    pack("anisotropic_isotropic_model", "REF_SEIS", ["AA.S0001"], "anisotropic_isotropic_model_ref_seis.npz")
+    copy_inputs("anisotropic_isotropic_model", ["DATA/Par_file", "DATA/SOURCE", "DATA/interfaces.dat"],
+                "anisotropic_isotropic_model")
     for example, name in (("ACOUSTIC", "claerbout_acoustic"), ("ACOUSTIC_ELASTIC", "claerbout_acoustic_elastic")):
         claerbout_inputs(example, name)
         claerbout_log_numbers(example, f"{name}_kernel_log.json")
