@@ -16,8 +16,8 @@
  * Implemented: forward and adjoint (SIMULATION_TYPE 1 and 3) runs of solid,
  * fluid (acoustic) and coupled fluid-solid meshes with Stacey edges, free
  * surfaces, sources, seismograms, the stored Stacey terms of SAVE_FORWARD,
- * the reconstructed fields and the solid and fluid kernels; viscoelastic
- * attenuation (forward runs); CPML layers (forward, elastic and acoustic
+ * the reconstructed fields and the solid and fluid kernels; anisotropic
+ * elements (forward runs); viscoelastic attenuation (forward runs); CPML layers (forward, elastic and acoustic
  * elements: prepare_pml_device_, pml_boundary_acoustic_cuda_, and the
  * extensions of specfem2d_gpu_ext.h).  Entry points of physics this library
  * does not implement (viscoacoustic attenuation, transfers of attenuation
