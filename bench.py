@@ -174,12 +174,10 @@ def fwi_gradient_batched(nshot: int = 32, repeats: int = 2, nstep: int = 1500):
     b = ShotBatch(true_cfg, shots)
     obs = b.forward()
     b.close()
-    saved = {k: os.environ.get(k) for k in ("SF2D_CHAIN_ELEMS", "SF2D_SHOTS_PER_WAVE")}
-    # launch shapes tuned on this workload's mesh for each path (short forwards from rest)
+    # launch shapes tuned on this workload's mesh for each path (short forwards from rest), set per handle
     tuned_plain = autotune.tune(cfg.replace(sources=shots[0]), None, steps=20)
     tuned_batch = autotune.tune(cfg, shots, steps=20)
-    autotune.apply(tuned_plain)
-    sim = Simulation(cfg.replace(sources=shots[0]))
+    sim = Simulation(cfg.replace(sources=shots[0]), **autotune.settings(tuned_plain))
     plain = []
     for _ in range(repeats):  # the first run includes the allocation of the checkpoint buffers
         t0 = time.perf_counter()
@@ -187,11 +185,8 @@ def fwi_gradient_batched(nshot: int = 32, repeats: int = 2, nstep: int = 1500):
         sim.gradient_backward()
         plain.append(time.perf_counter() - t0)
     sim.close()
-    for k, v in saved.items():
-        os.environ.pop(k, None) if v is None else os.environ.__setitem__(k, v)
-    autotune.apply(tuned_batch)
     chunk = int(round(np.sqrt(cfg.nstep)))
-    b = ShotBatch(cfg, shots)
+    b = ShotBatch(cfg, shots, **autotune.settings(tuned_batch))
     batched = []
     for _ in range(repeats):
         t0 = time.perf_counter()
@@ -199,8 +194,6 @@ def fwi_gradient_batched(nshot: int = 32, repeats: int = 2, nstep: int = 1500):
         g = b.gradient_backward()
         batched.append(time.perf_counter() - t0)
     b.close()
-    for k, v in saved.items():
-        os.environ.pop(k, None) if v is None else os.environ.__setitem__(k, v)
     gnorm = float(np.linalg.norm(g["vs"]))
     if not (np.isfinite(chi) and chi > 0 and np.isfinite(gnorm) and gnorm > 0):
         raise RuntimeError("degenerate batched gradient")

@@ -158,6 +158,9 @@ int sf2d_sim_adjoint_full(void* sim, const float* adj, const float* absorb, cons
    sqrt(c33/rho), vs = sqrt(c55/rho) of the creation call give its mass and Stacey impedances).  Forward runs without
    attenuation, CPML or shot batches.  Returns 0 on success. */
 int sf2d_sim_set_anisotropy(void* sim, const int* is_aniso, const double* c);
+/* Elements per chain patch (1..12, 0 = default) of the device plan, before the device is prepared (per handle;
+   the autotuner's knob).  Returns 0 on success. */
+int sf2d_sim_set_chain_elems(void* sim, int n);
 /* replaces the model (MODEL=gll parameters rho, vp, vs per GLL point, [nspec][25]) of a simulation: moduli,
    Stacey impedances and mass matrix are recomputed and uploaded.  Returns 0 on success. */
 int sf2d_sim_set_model(void* sim, const double* rho, const double* vp, const double* vs);

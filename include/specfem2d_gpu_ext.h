@@ -63,6 +63,10 @@ int sf2d_mesh_rect(long* Mesh_pointer);
    SF2D_SHOTS_PER_WAVE at set-up) */
 int sf2d_mesh_chain_elems(long* Mesh_pointer);
 void sf2d_mesh_set_shots_per_wave(long* Mesh_pointer, int shots);
+/* Elements per chain patch (1..12; 0 = the default 12) of the handle's element plan: call between
+   prepare_constants_device_ and prepare_fields_elastic_device_; takes precedence over SF2D_CHAIN_ELEMS.  No
+   counterpart in the reference (tuning of this library's assembly plan). */
+void sf2d_mesh_set_chain_elems(long* Mesh_pointer, int n);
 /* CPML layer in the terms of the reference's CPU path (pml_init.F90:34-1366, pml_compute*.f90), for elastic and
    acoustic elements.  The reference's GPU set-up prepare_pml_device_ carries acoustic layers only (K = 1, one d0),
    and its Fortran stops on elastic ones (prepare_gpu.f90:74).  Call after prepare_fields_elastic_device_ and

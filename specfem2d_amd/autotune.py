@@ -1,6 +1,6 @@
 """Autotuner for the force kernel's launch shape: elements per chain patch and shots per wave.
 
-The patch plan of a mesh (``SF2D_CHAIN_ELEMS``, elements per chain patch, 1..12) fixes how many element rows
+The patch plan of a mesh (``sf2d_mesh_set_chain_elems``, elements per chain patch, 1..12) fixes how many element rows
 one wave of the chain kernel holds and how many waves a colour launch has; the shots per wave of a batched
 launch (``sf2d_mesh_set_shots_per_wave``) trade element-data reuse against waves in flight.  Which pair is
 fastest depends on the mesh size, the number of shots and the device, so it is measured: every candidate runs
@@ -8,12 +8,12 @@ a short forward (``steps`` time steps from rest) on the workload's own mesh, mod
 device time wins.  The colour schedule itself is the plan's greedy colouring (four colours on the structured
 meshes), in launch order.
 
-``tune(cfg, shots)`` returns the choice and the table; ``apply(choice)`` sets the environment that the library
-reads when a handle is set up, so every simulation created afterwards in the process uses it.
+``tune(cfg, shots)`` returns the choice and the table; ``settings(choice)`` gives it as the ``chain_elems`` and
+``shots_per_wave`` arguments of the handle it is for (``Simulation`` / ``ShotBatch``): per handle, nothing
+process-wide.
 """
 from __future__ import annotations
 
-import os
 from typing import Dict, Iterable, List, Optional, Sequence
 
 from .config import SimulationConfig
@@ -41,37 +41,26 @@ def tune(cfg: SimulationConfig, shots: Optional[Sequence[list]] = None, steps: i
     batch = shots is not None
     groups: List[int] = sorted({min(g, len(shots)) for g in shots_per_wave}) if batch else [0]
     table = []
-    saved = os.environ.get("SF2D_CHAIN_ELEMS")
-    try:
-        for n in chain_elems:
-            os.environ["SF2D_CHAIN_ELEMS"] = str(n)
-            sim = ShotBatch(small, shots) if batch else Simulation(small)
-            try:
-                if not sim.chain_kernel():
-                    continue  # the mesh's patches are not chains: nothing to tune
-                for g in groups:
-                    if batch:
-                        sim.set_shots_per_wave(g)
-                    _forward_ms(sim, steps, batch)  # warm-up
-                    ms = _forward_ms(sim, steps, batch)
-                    table.append({"chain_elems": sim.chain_elems(), "shots_per_wave": g,
-                                  "ms_per_step": ms / steps})
-            finally:
-                sim.close()
-    finally:
-        if saved is None:
-            os.environ.pop("SF2D_CHAIN_ELEMS", None)
-        else:
-            os.environ["SF2D_CHAIN_ELEMS"] = saved
+    for n in chain_elems:
+        sim = ShotBatch(small, shots, chain_elems=n) if batch else Simulation(small, chain_elems=n)
+        try:
+            if not sim.chain_kernel():
+                continue  # the mesh's patches are not chains: nothing to tune
+            for g in groups:
+                if batch:
+                    sim.set_shots_per_wave(g)
+                _forward_ms(sim, steps, batch)  # warm-up
+                ms = _forward_ms(sim, steps, batch)
+                table.append({"chain_elems": sim.chain_elems(), "shots_per_wave": g, "ms_per_step": ms / steps})
+        finally:
+            sim.close()
     if not table:
         return {"chain_elems": None, "shots_per_wave": None, "ms_per_step": None, "table": []}
     best = min(table, key=lambda r: r["ms_per_step"])
     return dict(best, table=table)
 
 
-def apply(choice: Dict) -> None:
-    """Make simulations created from now on use a tuned choice (environment read at set-up)."""
-    if choice.get("chain_elems"):
-        os.environ["SF2D_CHAIN_ELEMS"] = str(choice["chain_elems"])
-    if choice.get("shots_per_wave"):
-        os.environ["SF2D_SHOTS_PER_WAVE"] = str(choice["shots_per_wave"])
+def settings(choice: Dict) -> Dict:
+    """The tuned choice as constructor arguments of ``Simulation`` / ``ShotBatch`` (per handle: the library reads
+    them when it builds that handle's plan and launch shape)."""
+    return {"chain_elems": int(choice.get("chain_elems") or 0), "shots_per_wave": int(choice.get("shots_per_wave") or 0)}

@@ -431,8 +431,8 @@ void sort_elements_by_colour(Mesh* mp, const int* phase_lists, int np) {
   // patch mode: chains of up to CHAIN_ELEMS elements when every patch is one (the chain kernel), otherwise
   // patches of up to PATCH_ELEMS for the general patch kernel
   const bool try_chain = mp->assembly == 0 && !getenv("SF2D_NO_CHAIN");
-  mp->chain_elems = Mesh::CHAIN_ELEMS;
-  if (const char* env = getenv("SF2D_CHAIN_ELEMS")) {
+  mp->chain_elems = mp->chain_request > 0 ? mp->chain_request : Mesh::CHAIN_ELEMS;
+  if (const char* env = mp->chain_request > 0 ? nullptr : getenv("SF2D_CHAIN_ELEMS")) {
     const int n = atoi(env);
     if (n < 1 || n > Mesh::CHAIN_ELEMS) fail("SF2D_CHAIN_ELEMS=%s outside 1..%d", env, Mesh::CHAIN_ELEMS);
     mp->chain_elems = n;
@@ -1665,6 +1665,15 @@ extern "C" int sf2d_mesh_col_rows(long* Mesh_pointer) { return handle(Mesh_point
 
 // tuning knobs of a handle: elements per chain patch (fixed at set-up) and shots per wave of batched launches
 extern "C" int sf2d_mesh_chain_elems(long* Mesh_pointer) { return handle(Mesh_pointer)->chain_elems; }
+
+// elements per chain patch of this handle's plan (1..12), before prepare_fields_elastic_device_ builds it; takes
+// precedence over SF2D_CHAIN_ELEMS
+extern "C" void sf2d_mesh_set_chain_elems(long* Mesh_pointer, int n) {
+  Mesh* mp = handle(Mesh_pointer);
+  if (mp->sorted) fail("sf2d_mesh_set_chain_elems: the element plan is already built");
+  if (n < 0 || n > Mesh::CHAIN_ELEMS) fail("sf2d_mesh_set_chain_elems: %d outside 0..%d", n, Mesh::CHAIN_ELEMS);
+  mp->chain_request = n;
+}
 
 extern "C" void sf2d_mesh_set_shots_per_wave(long* Mesh_pointer, int shots) {
   Mesh* mp = handle(Mesh_pointer);

@@ -137,6 +137,7 @@ struct Sim {
   // fluid elements (vs = 0 at all 25 points) and their boundaries
   std::vector<int> is_acoustic;      // [nspec]
   std::vector<int> aniso_elem;       // [nspec] anisotropic elements (empty: none)
+  int chain_request = 0;             // elements per chain patch for the device plan (0: the library's default)
   std::vector<float> aniso_c[6];     // c11, c13, c15, c33, c35, c55 [nspec][25]
   int nspec_ac = 0;
   std::vector<float> rmass_ac;       // [nglob] inverse fluid mass
@@ -1158,6 +1159,7 @@ void prepare_device(Sim& s) {
                             s.cosrot.data(), s.sinrot.data(), &simtype, &p_sv, &nspec_ac, &nspec_el,
                             is_acoustic.data(), is_elastic.data(), &myrank, &save_fwd, s.xir.data(), s.gammar.data(),
                             &nsig, s.seismotype, &nlength);
+  if (s.chain_request > 0) sf2d_mesh_set_chain_elems(&s.mesh, s.chain_request);
   // single partition: every element is "inner" (iphase 2); the outer lists are empty
   float dummy_f = 0.f;
   if (s.nspec_ac > 0) {
@@ -2090,6 +2092,13 @@ int sf2d_sim_adjoint_full(void* sim, const float* adj, const float* absorb, cons
       kernels_ac[3 * ne + l] = (float)(2.0 * (double)kk[l]);
     }
   }
+  return 0;
+}
+
+int sf2d_sim_set_chain_elems(void* sim, int n) {
+  Sim* s = (Sim*)sim;
+  if (!s || s->mesh || n < 0 || n > 12) return 1;
+  s->chain_request = n;
   return 0;
 }
 

@@ -108,6 +108,7 @@ struct Mesh {
   // tuning: elements per chain patch (SF2D_CHAIN_ELEMS at set-up, 1..CHAIN_ELEMS), shots per wave of the
   // batched chain kernel (SF2D_SHOTS_PER_WAVE or sf2d_mesh_set_shots_per_wave; 0 = built-in default)
   int chain_elems = 0;
+  int chain_request = 0;  // elements per chain patch asked for by sf2d_mesh_set_chain_elems (0: default)
   int shots_per_wave = 0;
   int chain = 0;                     // every patch is a chain of elements: the row-lane force kernel (SF2D_NO_CHAIN
                                      // keeps the general patch kernel)

@@ -81,6 +81,8 @@ def load_libraries():
     host.sf2d_sim_forward.restype = C.c_int
     host.sf2d_sim_adjoint.argtypes = [P, P, P, P, P, P, P, P]
     host.sf2d_sim_adjoint.restype = C.c_int
+    host.sf2d_sim_set_chain_elems.argtypes = [P, C.c_int]
+    host.sf2d_sim_set_chain_elems.restype = C.c_int
     host.sf2d_sim_set_anisotropy.argtypes = [P, P, P]
     host.sf2d_sim_set_anisotropy.restype = C.c_int
     host.sf2d_sim_adjoint_full.argtypes = [P] * 12
@@ -191,7 +193,7 @@ class Simulation:
             "rho_vp": 7, "rho_vs": 8, "rho": 9}
 
     def __init__(self, cfg: SimulationConfig, simulation_type: int = 1, save_forward: bool = False,
-                 device: bool = True):
+                 device: bool = True, chain_elems: int = 0, shots_per_wave: int = 0):
         self.cfg = cfg
         self.gpu, self.host = load_libraries()
         rho, vp, vs = cfg.material_arrays()
@@ -219,9 +221,13 @@ class Simulation:
             if self.host.sf2d_sim_set_anisotropy(self.h, _ptr(ae), _ptr(ac)) != 0:
                 raise NotImplementedError("anisotropic elements: forward runs of solids without attenuation, CPML "
                                           "or shot batches")
+        if chain_elems and self.host.sf2d_sim_set_chain_elems(self.h, int(chain_elems)) != 0:
+            raise ValueError(f"chain_elems {chain_elems} outside 0..12")
         self.on_device = False
         if device:
             self.prepare_device()
+            if shots_per_wave:
+                self.set_shots_per_wave(shots_per_wave)
 
     def _create(self, p, rho, vp, vs, rec, simulation_type, save_forward):
         srcs = _sources_c(self.cfg.sources)
@@ -575,7 +581,8 @@ class ShotBatch(Simulation):
     (``cfg.sources`` is ignored).  ``forward()`` returns seismograms [nshot, nrec, 2, nstep]; the gradient
     calls take observations of that shape and return the misfit summed over the shots and its gradient."""
 
-    def __init__(self, cfg: SimulationConfig, shots, device: bool = True):
+    def __init__(self, cfg: SimulationConfig, shots, device: bool = True, chain_elems: int = 0,
+                 shots_per_wave: int = 0):
         self.shots = [list(s) for s in shots]
         if not self.shots or any(len(s) == 0 for s in self.shots):
             raise ValueError("every shot needs at least one source")
@@ -583,8 +590,9 @@ class ShotBatch(Simulation):
             raise NotImplementedError("shot batches of meshes with fluid elements (vs = 0) are not supported")
         if cfg.mesh is not None:
             raise NotImplementedError("shot batches of external meshes are not supported")
-        super().__init__(cfg, simulation_type=1, save_forward=False, device=device)
         self.nshot = len(self.shots)
+        super().__init__(cfg, simulation_type=1, save_forward=False, device=device, chain_elems=chain_elems,
+                         shots_per_wave=shots_per_wave)
 
     def _create(self, p, rho, vp, vs, rec, simulation_type, save_forward):
         flat = [s for shot in self.shots for s in shot]

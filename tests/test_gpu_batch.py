@@ -216,10 +216,8 @@ def test_autotuner_picks_a_candidate_and_keeps_the_results(case, monkeypatch):
     assert choice["ms_per_step"] == min(r["ms_per_step"] for r in choice["table"])
     out = []
     for n in (4, 12):
-        monkeypatch.setenv("SF2D_CHAIN_ELEMS", str(n))
-        b = ShotBatch(cfg, shots)
+        b = ShotBatch(cfg, shots, **autotune.settings({"chain_elems": n, "shots_per_wave": 1 if n == 4 else 4}))
         assert b.chain_elems() == n
-        b.set_shots_per_wave(1 if n == 4 else 4)
         out.append(b.forward())
         b.close()
     assert _rel(out[0], out[1]) < 2e-6
